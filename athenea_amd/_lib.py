@@ -1,0 +1,227 @@
+"""ctypes binding of libathx.so, the C ABI declared in include/athx.h.
+
+The library is the product: nothing here computes a frame. Loading fails
+loudly when the library has not been built (run ``python -m athenea_amd.build``
+or ``__graft_entry__.build()``); there is no fallback path.
+"""
+from __future__ import annotations
+
+import ctypes
+import threading
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libathx.so"
+
+# athx_status: athenea's ErrorCode (modules/core/include/athenea/core/Error.h:14-26), 0 = OK.
+STATUS_NAMES = {
+    0: "Ok",
+    1: "Unknown",
+    2: "NotFound",
+    3: "InvalidArgument",
+    4: "Unsupported",
+    5: "IoFailure",
+    6: "DeviceFailure",
+    7: "ShaderFailure",
+    8: "PluginFailure",
+    9: "OutOfMemory",
+    10: "Cancelled",
+    11: "InternalError",
+}
+
+CULL_REASONS = 7
+
+c_float_p = ctypes.POINTER(ctypes.c_float)
+c_u32_p = ctypes.POINTER(ctypes.c_uint32)
+c_u64_p = ctypes.POINTER(ctypes.c_uint64)
+
+
+class View(ctypes.Structure):
+    _fields_ = [
+        ("width", ctypes.c_uint32),
+        ("height", ctypes.c_uint32),
+        ("focal_x", ctypes.c_float),
+        ("focal_y", ctypes.c_float),
+        ("centre_x", ctypes.c_float),
+        ("centre_y", ctypes.c_float),
+        ("near_z", ctypes.c_float),
+        ("far_z", ctypes.c_float),
+        ("orthographic", ctypes.c_uint32),
+        ("object_to_view", ctypes.c_float * 12),
+        ("eye", ctypes.c_float * 3),
+        ("background", ctypes.c_float * 4),
+    ]
+
+
+class Gaussians(ctypes.Structure):
+    _fields_ = [
+        ("count", ctypes.c_uint32),
+        ("sh_coeffs", ctypes.c_uint32),
+        ("means", ctypes.c_void_p),
+        ("log_scales", ctypes.c_void_p),
+        ("quats", ctypes.c_void_p),
+        ("opacity_logits", ctypes.c_void_p),
+        ("sh", ctypes.c_void_p),
+        ("sh_half", ctypes.c_void_p),
+    ]
+
+
+class Settings(ctypes.Structure):
+    _fields_ = [
+        ("tile", ctypes.c_uint32),
+        ("batch", ctypes.c_uint32),
+        ("antialias", ctypes.c_uint32),
+        ("linear", ctypes.c_uint32),
+        ("max_sh_degree", ctypes.c_uint32),
+        ("depth_mode", ctypes.c_uint32),
+        ("depth_threshold", ctypes.c_float),
+        ("time_stages", ctypes.c_uint32),
+        ("count_splats", ctypes.c_uint32),
+    ]
+
+
+class Targets(ctypes.Structure):
+    _fields_ = [
+        ("colour", ctypes.c_void_p),
+        ("depth", ctypes.c_void_p),
+        ("final_t", ctypes.c_void_p),
+        ("last", ctypes.c_void_p),
+    ]
+
+
+class FrameStats(ctypes.Structure):
+    _fields_ = [
+        ("splats", ctypes.c_uint32),
+        ("visible", ctypes.c_uint32),
+        ("pairs", ctypes.c_uint64),
+        ("project_ms", ctypes.c_double),
+        ("scan_ms", ctypes.c_double),
+        ("emit_ms", ctypes.c_double),
+        ("sort_ms", ctypes.c_double),
+        ("ranges_ms", ctypes.c_double),
+        ("blend_ms", ctypes.c_double),
+        ("total_ms", ctypes.c_double),
+    ]
+
+
+class Counters(ctypes.Structure):
+    _fields_ = [
+        ("slots", ctypes.c_uint32),
+        ("visible", ctypes.c_uint32),
+        ("pairs", ctypes.c_uint64),
+        ("max_tiles", ctypes.c_uint32),
+        ("culled", ctypes.c_uint32 * CULL_REASONS),
+    ]
+
+
+class Grads(ctypes.Structure):
+    _fields_ = [
+        ("means", ctypes.c_void_p),
+        ("log_scales", ctypes.c_void_p),
+        ("quats", ctypes.c_void_p),
+        ("opacity_logits", ctypes.c_void_p),
+        ("sh", ctypes.c_void_p),
+    ]
+
+
+class FrameInfo(ctypes.Structure):
+    _fields_ = [
+        ("slots", ctypes.c_uint32),
+        ("tiles", ctypes.c_uint32),
+        ("key_bits", ctypes.c_uint32),
+        ("pairs", ctypes.c_uint64),
+    ]
+
+
+# athx_frame_part
+FRAME_PARTS = {"records": 0, "touched": 1, "depth_keys": 2, "offsets": 3, "pair_keys": 4, "pair_values": 5,
+               "ranges": 6}
+
+
+# Every symbol include/athx.h declares, with its signature.
+SIGNATURES = {
+    "athx_version": (ctypes.c_char_p, []),
+    "athx_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]),
+    "athx_destroy": (None, [ctypes.c_void_p]),
+    "athx_last_error": (ctypes.c_char_p, [ctypes.c_void_p]),
+    "athx_release_scratch": (ctypes.c_int, [ctypes.c_void_p]),
+    "athx_scratch_bytes": (ctypes.c_uint64, [ctypes.c_void_p]),
+    "athx_render": (
+        ctypes.c_int,
+        [
+            ctypes.c_void_p,
+            ctypes.POINTER(View),
+            ctypes.c_uint32,
+            ctypes.POINTER(Gaussians),
+            ctypes.POINTER(Settings),
+            ctypes.POINTER(Targets),
+            ctypes.c_void_p,
+            ctypes.POINTER(FrameStats),
+        ],
+    ),
+    "athx_render_backward": (
+        ctypes.c_int,
+        [
+            ctypes.c_void_p,
+            ctypes.POINTER(View),
+            ctypes.c_uint32,
+            ctypes.POINTER(Gaussians),
+            ctypes.POINTER(Settings),
+            ctypes.POINTER(Targets),
+            ctypes.POINTER(ctypes.c_void_p),
+            ctypes.POINTER(Grads),
+            ctypes.c_void_p,
+        ],
+    ),
+    "athx_latest_counters": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Counters)]),
+    "athx_frame_info_get": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(FrameInfo)]),
+    "athx_copy_frame": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_uint64,
+                                       ctypes.c_void_p]),
+    "athx_exclusive_scan": (
+        ctypes.c_int,
+        [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p],
+    ),
+    "athx_sort_pairs": (
+        ctypes.c_int,
+        [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p],
+    ),
+}
+
+_lock = threading.Lock()
+_lib: ctypes.CDLL | None = None
+
+
+class AthxError(RuntimeError):
+    """A failed call of the C ABI: ``code`` is the athenea ErrorCode name."""
+
+    def __init__(self, status: int, message: str):
+        self.status = status
+        self.code = STATUS_NAMES.get(status, f"status {status}")
+        super().__init__(f"{self.code}: {message}")
+
+
+def load() -> ctypes.CDLL:
+    """Loads libathx.so once; raises if it is missing."""
+    global _lib
+    with _lock:
+        if _lib is None:
+            if not LIB_PATH.exists():
+                raise RuntimeError(
+                    f"{LIB_PATH} is not built: the rasterizer has no CPU path. "
+                    "Build it with `python -m athenea_amd.build`."
+                )
+            lib = ctypes.CDLL(str(LIB_PATH))
+            for name, (res, args) in SIGNATURES.items():
+                fn = getattr(lib, name)
+                fn.restype = res
+                fn.argtypes = args
+            _lib = lib
+    return _lib
+
+
+def check(status: int, ctx=None) -> None:
+    if status != 0:
+        msg = ""
+        if ctx is not None:
+            raw = load().athx_last_error(ctx)
+            msg = raw.decode() if raw else ""
+        raise AthxError(status, msg)

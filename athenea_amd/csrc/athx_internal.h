@@ -1,0 +1,99 @@
+// athx_internal.h -- what the HIP sources of the rasterizer share: the view
+// as the kernels read it, wavefront primitives, and the launchers each source
+// file exports to the C ABI (capi.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/athx.h"
+#include "../../include/athx_detmath.h"
+
+namespace athx {
+
+// The splat rasterizer's constants (shaders/athenea/splat/frame.slang:42-49).
+constexpr uint32_t kCulledKey = 0xFF000000u;
+// Floats in a projected record: screen (x, y, z, cutoff), conic (a, b, c,
+// alpha), colour (r, g, b, 0) -- frame.slang:269-281's ProjRecord without the
+// surfel words, which a plain Gaussian leaves zero.
+constexpr uint32_t kRecordFloats = 12;
+constexpr uint32_t kDepthBits = 24;
+
+// One view as every kernel reads it: the camera (athx_view), where its tiles
+// and slots start in the frame's global numbering, and its targets.
+struct DView {
+    uint32_t width, height, tiles_x, tiles_y;
+    float focal_x, focal_y, centre_x, centre_y;
+    float near_z, far_z;
+    uint32_t orthographic;
+    uint32_t tile_base;  // first global tile of this view
+    float m[12];
+    float eye[3];
+    float bg[4];
+    float* colour;
+    float* depth;
+    float* final_t;
+    uint32_t* last;
+    const float* dl_dcolour;
+};
+
+// What the projection is told besides the views.
+struct ProjectArgs {
+    const float* means;
+    const float* log_scales;
+    const float* quats;
+    const float* logits;
+    const float* sh;
+    const uint16_t* sh_half;
+    uint32_t n;
+    uint32_t sh_coeffs;
+    uint32_t sh_degree;  // min(degree the coefficients hold, the frame's limit)
+    uint32_t linear;
+    uint32_t antialias;
+    uint32_t tile;
+};
+
+__host__ __device__ inline uint32_t div_up(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
+
+// Launchers (each returns the hipError_t of its launches).
+hipError_t launch_project(const ProjectArgs& a, const DView* views_dev, uint32_t nviews, float* records,
+                          uint32_t* touched, uint32_t* depth_keys, hipStream_t s);
+hipError_t launch_fill_u32(uint32_t* p, uint32_t value, uint64_t n, hipStream_t s);
+size_t scan_status_words(uint32_t n);
+hipError_t launch_scan(const uint32_t* in, uint32_t* out, uint64_t* total, uint32_t n, uint32_t* status,
+                       hipStream_t s);
+hipError_t launch_emit(const DView* views_dev, uint32_t nviews, uint32_t n, uint32_t tile,
+                       const float* records, const uint32_t* touched, const uint32_t* depth_keys,
+                       const uint32_t* offsets, uint64_t* keys, uint32_t* values, hipStream_t s);
+hipError_t launch_ranges(const uint64_t* sorted_keys, uint32_t pairs, uint32_t tiles, uint32_t* ranges,
+                         hipStream_t s);
+hipError_t launch_blend(const DView* views_dev, const uint32_t* view_of_tile_base, uint32_t nviews,
+                        uint32_t total_tiles, uint32_t tile, uint32_t batch, const uint32_t* ranges,
+                        const uint32_t* pair_slots, const float* records, uint32_t depth_mode,
+                        float depth_threshold, hipStream_t s);
+hipError_t launch_blend_backward(const DView* views_dev, const uint32_t* tile_bases, uint32_t nviews,
+                                 uint32_t total_tiles, uint32_t tile, uint32_t batch, const uint32_t* ranges,
+                                 const uint32_t* pair_slots, const float* records, float* grad2d, hipStream_t s);
+hipError_t launch_project_backward(const ProjectArgs& a, const DView* views_dev, uint32_t nviews,
+                                   const uint32_t* touched, const float* grad2d, const athx_grads& g,
+                                   hipStream_t s);
+hipError_t launch_counters(const uint32_t* depth_keys, const uint32_t* touched, uint64_t slots,
+                           uint32_t* words, hipStream_t s);
+
+// Onesweep radix sort of (uint64 key, uint32 value) pairs.
+struct SortScratch {
+    uint64_t* keys_alt;
+    uint32_t* values_alt;
+    uint32_t* hist;      // passes * 256
+    uint32_t* lookback;  // passes * partitions * 256
+    uint32_t* counters;  // passes
+};
+uint32_t sort_partitions(uint32_t n);
+hipError_t launch_sort(uint64_t* keys, uint32_t* values, uint32_t n, uint32_t key_bits, const SortScratch& sc,
+                       hipStream_t s);
+
+// Per-pair 2D gradient words the blend's adjoint accumulates a slot:
+// centre x, y, conic a, b, c, alpha, colour r, g, b.
+constexpr uint32_t kGrad2d = 9;
+
+}  // namespace athx

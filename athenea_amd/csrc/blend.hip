@@ -1,0 +1,338 @@
+// blend.hip -- the per-tile front-to-back alpha blend for gfx950 and its
+// adjoint.
+//
+// Forward: shaders/athenea/splat/splat_blend.slang:236-578 (splatBlend,
+// blendPixel with composite, matte and trace off, contributionMode 0). One
+// workgroup per tile, one thread per pixel; the tile's pairs are staged in
+// LDS `batch` at a time (the reference stages 256, splat_blend.slang:300-311)
+// and every pixel walks them from there. A wavefront whose 64 pixels are all
+// done stops walking at once; the group stops loading when every wavefront
+// has stopped. Each pixel also keeps its final transmittance and how many of
+// its tile's pairs it read, which is all the adjoint needs to replay the walk.
+//
+// Adjoint: no counterpart in the reference. It walks the same pairs back to
+// front, recovering the transmittance before each Gaussian from the one
+// after it (T_k = T_{k+1} / (1 - alpha_k); alpha <= 0.99 keeps it stable),
+// and accumulates per-slot gradients of the centre, conic, alpha and colour.
+// Lanes of a wavefront walk the same pair at the same step, so a pair's
+// gradient is summed across the wavefront first and added to memory once per
+// wavefront.
+#include "athx_internal.h"
+
+namespace athx {
+namespace {
+
+constexpr uint32_t kMaxBatch = 256;
+
+__device__ inline uint32_t view_of_tile(const uint32_t* tile_bases, uint32_t nviews, uint32_t tile) {
+    uint32_t lo = 0, hi = nviews;  // tile_bases[nviews] is the total
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (tile_bases[mid] <= tile) {
+            lo = mid;
+        } else {
+            hi = mid;
+        }
+    }
+    return lo;
+}
+
+struct LdsBatch {
+    float4 xy_cut_op[kMaxBatch];  // centre x, y, cutoff, alpha
+    float4 conic[kMaxBatch];      // a, 2b, c, view z
+    float4 colour[kMaxBatch];     // r, g, b, conic b
+};
+
+__device__ inline void stage(LdsBatch& s, uint32_t k, const float* records, uint32_t slot) {
+    const float4* rec = reinterpret_cast<const float4*>(records + (size_t)slot * kRecordFloats);
+    const float4 screen = rec[0];
+    const float4 conic = rec[1];
+    const float4 col = rec[2];
+    s.xy_cut_op[k] = make_float4(screen.x, screen.y, screen.w, conic.w);
+    s.conic[k] = make_float4(conic.x, 2.0f * conic.y, conic.z, screen.z);
+    s.colour[k] = make_float4(col.x, col.y, col.z, conic.y);
+}
+
+template <uint32_t TILE>
+__global__ void __launch_bounds__(TILE * TILE) blend_kernel(const DView* __restrict__ views,
+                                                            const uint32_t* __restrict__ tile_bases, uint32_t nviews,
+                                                            uint32_t batch, const uint32_t* __restrict__ ranges,
+                                                            const uint32_t* __restrict__ pair_slots,
+                                                            const float* __restrict__ records, uint32_t depth_mode,
+                                                            float depth_threshold) {
+    constexpr uint32_t kThreads = TILE * TILE;
+    __shared__ LdsBatch s;
+    const uint32_t gtile = blockIdx.x;
+    const uint32_t v = view_of_tile(tile_bases, nviews, gtile);
+    const DView& V = views[v];
+    const uint32_t local = gtile - V.tile_base;
+    const uint32_t tx = local % V.tiles_x, ty = local / V.tiles_x;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t px = tx * TILE + tid % TILE, py = ty * TILE + tid / TILE;
+    const bool inside = px < V.width && py < V.height;
+    const uint32_t start = ranges[2 * gtile], end = ranges[2 * gtile + 1];
+    const float pxc = (float)px + 0.5f, pyc = (float)py + 0.5f;
+
+    float r = 0.0f, g = 0.0f, b = 0.0f, T = 1.0f, dsum = 0.0f, dat = -1.0f;
+    uint32_t last = 0;
+    bool done = !inside;
+    for (uint32_t first = start; first < end; first += batch) {
+        if (!__syncthreads_or(!done)) {
+            break;
+        }
+        const uint32_t count = end - first < batch ? end - first : batch;
+        for (uint32_t k = tid; k < count; k += kThreads) {
+            stage(s, k, records, pair_slots[first + k]);
+        }
+        __syncthreads();
+        for (uint32_t j = 0; j < count && !done; ++j) {
+            const float4 q = s.xy_cut_op[j];
+            const float4 k = s.conic[j];
+            const float dx = pxc - q.x, dy = pyc - q.y;
+            const float power = k.x * dx * dx + k.y * dx * dy + k.z * dy * dy;
+            if (power > q.z) {
+                continue;
+            }
+            const float alpha = fminf(0.99f, q.w * athx_expf(-0.5f * power));
+            if (alpha < 1.0f / 255.0f) {
+                continue;
+            }
+            const float4 c = s.colour[j];
+            const float w = T * alpha;
+            r = r + w * c.x;
+            g = g + w * c.y;
+            b = b + w * c.z;
+            dsum = dsum + w * k.w;
+            T = T * (1.0f - alpha);
+            if (dat < 0.0f && 1.0f - T >= depth_threshold) {
+                dat = k.w;
+            }
+            last = first - start + j + 1;
+            if (T < 1e-4f) {
+                done = true;
+            }
+        }
+        // Pairs a pixel skipped after its last contribution are read again by
+        // nobody: `last` is the walk's own extent.
+        __syncthreads();
+    }
+    if (!inside) {
+        return;
+    }
+    const size_t pix = (size_t)py * V.width + px;
+    const float coverage = 1.0f - T;
+    float4 out;
+    out.x = r + T * V.bg[0];
+    out.y = g + T * V.bg[1];
+    out.z = b + T * V.bg[2];
+    out.w = coverage + T * V.bg[3];
+    reinterpret_cast<float4*>(V.colour)[pix] = out;
+    if (V.depth) {
+        V.depth[pix] = depth_mode == 0 ? (coverage > 1e-6f ? dsum / coverage : 0.0f) : fmaxf(dat, 0.0f);
+    }
+    if (V.final_t) {
+        V.final_t[pix] = T;
+    }
+    if (V.last) {
+        V.last[pix] = last;
+    }
+}
+
+__device__ inline float wave_sum_f(float x) {
+#pragma unroll
+    for (uint32_t d = 32; d >= 1; d >>= 1) {
+        x += __shfl_xor(x, d, 64);
+    }
+    return x;
+}
+
+template <uint32_t TILE>
+__global__ void __launch_bounds__(TILE * TILE) blend_backward_kernel(
+    const DView* __restrict__ views, const uint32_t* __restrict__ tile_bases, uint32_t nviews, uint32_t batch,
+    const uint32_t* __restrict__ ranges, const uint32_t* __restrict__ pair_slots, const float* __restrict__ records,
+    float* __restrict__ grad2d) {
+    constexpr uint32_t kThreads = TILE * TILE;
+    __shared__ LdsBatch s;
+    __shared__ uint32_t s_slot[kMaxBatch];
+    __shared__ uint32_t s_max_last;
+    const uint32_t gtile = blockIdx.x;
+    const uint32_t v = view_of_tile(tile_bases, nviews, gtile);
+    const DView& V = views[v];
+    const uint32_t local = gtile - V.tile_base;
+    const uint32_t tx = local % V.tiles_x, ty = local / V.tiles_x;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t lane = tid & 63u;
+    const uint32_t px = tx * TILE + tid % TILE, py = ty * TILE + tid / TILE;
+    const bool inside = px < V.width && py < V.height;
+    const uint32_t start = ranges[2 * gtile];
+    const uint32_t tile_pairs = ranges[2 * gtile + 1] - start;
+    const float pxc = (float)px + 0.5f, pyc = (float)py + 0.5f;
+    const size_t pix = inside ? (size_t)py * V.width + px : 0;
+
+    float T = 1.0f, TN = 1.0f;
+    uint32_t last = 0;
+    float gr = 0.0f, gg = 0.0f, gb = 0.0f, ga = 0.0f;
+    if (inside) {
+        T = V.final_t[pix];
+        TN = T;
+        // A walk never reads past its tile's pairs; the clamp keeps a stale
+        // `last` from reading past them either.
+        last = min(V.last[pix], tile_pairs);
+        const float4 gc = reinterpret_cast<const float4*>(V.dl_dcolour)[pix];
+        gr = gc.x; gg = gc.y; gb = gc.z; ga = gc.w;
+    }
+    // The colour of everything behind the current Gaussian, as seen from just
+    // in front of it: the background, then each Gaussian over it.
+    float br = V.bg[0], bgc = V.bg[1], bb = V.bg[2];
+    // d(alpha out)/d(alpha_k) = T_N (1 - bg.a) / (1 - alpha_k).
+    const float ga_scale = ga * TN * (1.0f - V.bg[3]);
+    if (tid == 0) {
+        s_max_last = 0;
+    }
+    __syncthreads();
+    if (last != 0) {
+        atomicMax(&s_max_last, last);
+    }
+    __syncthreads();
+    const uint32_t max_last = s_max_last;
+    for (uint32_t hi = max_last; hi > 0;) {
+        const uint32_t lo = hi > batch ? hi - batch : 0;
+        const uint32_t count = hi - lo;
+        for (uint32_t k = tid; k < count; k += kThreads) {
+            const uint32_t slot = pair_slots[start + lo + k];
+            s_slot[k] = slot;
+            stage(s, k, records, slot);
+        }
+        __syncthreads();
+        for (uint32_t jj = count; jj > 0; --jj) {
+            const uint32_t j = jj - 1;
+            const uint32_t idx = lo + j;
+            bool contrib = false;
+            float g_cx = 0.f, g_cy = 0.f, g_ka = 0.f, g_kb = 0.f, g_kc = 0.f, g_al = 0.f, g_r = 0.f, g_g = 0.f,
+                  g_b = 0.f;
+            if (idx < last) {
+                const float4 q = s.xy_cut_op[j];
+                const float4 k = s.conic[j];
+                const float dx = pxc - q.x, dy = pyc - q.y;
+                const float power = k.x * dx * dx + k.y * dx * dy + k.z * dy * dy;
+                if (power <= q.z) {
+                    const float e = athx_expf(-0.5f * power);
+                    const float raw = q.w * e;
+                    const float alpha = fminf(0.99f, raw);
+                    if (alpha >= 1.0f / 255.0f) {
+                        contrib = true;
+                        const float4 c = s.colour[j];
+                        const float one_minus = 1.0f - alpha;
+                        const float Tb = T / one_minus;
+                        const float w = alpha * Tb;
+                        g_r = w * gr;
+                        g_g = w * gg;
+                        g_b = w * gb;
+                        const float dl_dalpha =
+                            Tb * ((c.x - br) * gr + (c.y - bgc) * gg + (c.z - bb) * gb) + ga_scale / one_minus;
+                        br = c.x * alpha + br * one_minus;
+                        bgc = c.y * alpha + bgc * one_minus;
+                        bb = c.z * alpha + bb * one_minus;
+                        T = Tb;
+                        if (!(raw > 0.99f)) {
+                            g_al = dl_dalpha * e;
+                            const float dl_dpower = dl_dalpha * (-0.5f * alpha);
+                            const float kb = c.w;
+                            g_ka = dl_dpower * dx * dx;
+                            g_kb = dl_dpower * 2.0f * dx * dy;
+                            g_kc = dl_dpower * dy * dy;
+                            // power depends on the centre through dx = px - cx.
+                            g_cx = -dl_dpower * (2.0f * k.x * dx + 2.0f * kb * dy);
+                            g_cy = -dl_dpower * (2.0f * kb * dx + 2.0f * k.z * dy);
+                        }
+                    }
+                }
+            }
+            if (__any(contrib)) {
+                const float sums[kGrad2d] = {wave_sum_f(g_cx), wave_sum_f(g_cy), wave_sum_f(g_ka),
+                                             wave_sum_f(g_kb), wave_sum_f(g_kc), wave_sum_f(g_al),
+                                             wave_sum_f(g_r),  wave_sum_f(g_g),  wave_sum_f(g_b)};
+                if (lane < kGrad2d) {
+                    float mine = 0.0f;
+#pragma unroll
+                    for (uint32_t t = 0; t < kGrad2d; ++t) {
+                        mine = lane == t ? sums[t] : mine;
+                    }
+                    atomicAdd(&grad2d[(size_t)s_slot[j] * kGrad2d + lane], mine);
+                }
+            }
+        }
+        __syncthreads();
+        hi = lo;
+    }
+}
+
+template <uint32_t TILE>
+hipError_t blend_for(const DView* views, const uint32_t* tile_bases, uint32_t nviews, uint32_t tiles, uint32_t batch,
+                     const uint32_t* ranges, const uint32_t* slots, const float* records, uint32_t depth_mode,
+                     float depth_threshold, hipStream_t s) {
+    hipLaunchKernelGGL(blend_kernel<TILE>, dim3(tiles), dim3(TILE * TILE), 0, s, views, tile_bases, nviews, batch,
+                       ranges, slots, records, depth_mode, depth_threshold);
+    return hipGetLastError();
+}
+
+template <uint32_t TILE>
+hipError_t blend_backward_for(const DView* views, const uint32_t* tile_bases, uint32_t nviews, uint32_t tiles,
+                              uint32_t batch, const uint32_t* ranges, const uint32_t* slots, const float* records,
+                              float* grad2d, hipStream_t s) {
+    hipLaunchKernelGGL(blend_backward_kernel<TILE>, dim3(tiles), dim3(TILE * TILE), 0, s, views, tile_bases, nviews,
+                       batch, ranges, slots, records, grad2d);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_blend(const DView* views_dev, const uint32_t* tile_bases, uint32_t nviews, uint32_t total_tiles,
+                        uint32_t tile, uint32_t batch, const uint32_t* ranges, const uint32_t* pair_slots,
+                        const float* records, uint32_t depth_mode, float depth_threshold, hipStream_t s) {
+    if (total_tiles == 0) {
+        return hipSuccess;
+    }
+    if (batch == 0 || batch > kMaxBatch) {
+        return hipErrorInvalidValue;
+    }
+    switch (tile) {
+    case 8:
+        return blend_for<8>(views_dev, tile_bases, nviews, total_tiles, batch, ranges, pair_slots, records, depth_mode,
+                            depth_threshold, s);
+    case 16:
+        return blend_for<16>(views_dev, tile_bases, nviews, total_tiles, batch, ranges, pair_slots, records,
+                             depth_mode, depth_threshold, s);
+    case 32:
+        return blend_for<32>(views_dev, tile_bases, nviews, total_tiles, batch, ranges, pair_slots, records,
+                             depth_mode, depth_threshold, s);
+    default:
+        return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_blend_backward(const DView* views_dev, const uint32_t* tile_bases, uint32_t nviews,
+                                 uint32_t total_tiles, uint32_t tile, uint32_t batch, const uint32_t* ranges,
+                                 const uint32_t* pair_slots, const float* records, float* grad2d, hipStream_t s) {
+    if (total_tiles == 0) {
+        return hipSuccess;
+    }
+    if (batch == 0 || batch > kMaxBatch) {
+        return hipErrorInvalidValue;
+    }
+    switch (tile) {
+    case 8:
+        return blend_backward_for<8>(views_dev, tile_bases, nviews, total_tiles, batch, ranges, pair_slots, records,
+                                     grad2d, s);
+    case 16:
+        return blend_backward_for<16>(views_dev, tile_bases, nviews, total_tiles, batch, ranges, pair_slots, records,
+                                      grad2d, s);
+    case 32:
+        return blend_backward_for<32>(views_dev, tile_bases, nviews, total_tiles, batch, ranges, pair_slots, records,
+                                      grad2d, s);
+    default:
+        return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace athx

@@ -1,0 +1,722 @@
+// capi.hip -- the C ABI of include/athx.h: a context that owns the frame's
+// scratch and drives the kernels on the caller's stream.
+//
+// The frame (modules/render/src/TileRasterizer.cpp:296-1295, reordered for
+// one 64-bit key sort):
+//   project      every (view, Gaussian) slot: record, tile count, depth key
+//   scan         exclusive scan of tile counts -> pair offsets and the total
+//   readback     the pair total (the reference's one wait, :1093-1102)
+//   emit         (global tile << 24 | depth24, slot) pairs in slot order
+//   sort         stable onesweep radix sort over 24 + tile bits
+//   ranges       each tile's [start, end) in the sorted pairs
+//   blend        per tile, front to back
+// Scratch grows to 1.5x what a frame needs and, after athx_release_scratch,
+// to exactly what it needs (TileRasterizer.cpp:30-36, 178-191).
+#include <chrono>
+#include <cstring>
+#include <exception>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "athx_internal.h"
+
+using namespace athx;
+
+namespace {
+
+struct DeviceBuffer {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+};
+
+uint32_t bits_for(uint32_t values) {
+    // modules/render/src/FrameParams.h:18-24
+    uint32_t bits = 1;
+    while (bits < 32 && (uint64_t{1} << bits) < values) {
+        ++bits;
+    }
+    return bits;
+}
+
+}  // namespace
+
+struct athx_ctx {
+    int device = 0;
+    std::string error;
+    bool exact_growth = false;
+
+    DeviceBuffer views, tile_bases, records, touched, depth_keys, offsets, scan_status, totals;
+    DeviceBuffer pair_keys, pair_vals, alt_keys, alt_vals, sort_meta, ranges, grad2d, counter_words;
+    DeviceBuffer api_sort_meta, api_alt_keys, api_alt_vals, api_scan_status;
+    uint64_t* host_totals = nullptr;  // pinned: [0] pairs, [1] visible
+    DView* host_views = nullptr;      // pinned staging for the views
+    uint32_t* host_tile_bases = nullptr;
+    uint32_t host_view_capacity = 0;
+
+    // What the last render left for the adjoint.
+    uint32_t last_n = 0, last_nviews = 0, last_tiles = 0, last_tile = 0, last_key_bits = 0;
+    uint64_t last_pairs = 0;
+    bool have_frame = false;
+    bool counters_valid = false;
+    athx_counters counters{};
+
+    hipEvent_t ev[8] = {};
+    // Recorded after each upload from the pinned staging views: the staging
+    // is not written again until that copy has run.
+    hipEvent_t upload_done = nullptr;
+    bool upload_pending = false;
+};
+
+namespace {
+
+athx_status fail(athx_ctx* ctx, athx_status code, const std::string& msg) {
+    if (ctx) {
+        ctx->error = msg;
+    }
+    return code;
+}
+
+athx_status hip_fail(athx_ctx* ctx, hipError_t e, const char* what) {
+    return fail(ctx, e == hipErrorOutOfMemory ? ATHX_OUT_OF_MEMORY : ATHX_DEVICE_FAILURE,
+                std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define ATHX_HIP(ctx, expr, what)                 \
+    do {                                          \
+        hipError_t e_ = (expr);                   \
+        if (e_ != hipSuccess) {                   \
+            return hip_fail((ctx), e_, (what));   \
+        }                                         \
+    } while (0)
+
+hipError_t reserve(athx_ctx* ctx, DeviceBuffer& b, size_t bytes) {
+    if (bytes <= b.bytes) {
+        return hipSuccess;
+    }
+    size_t want = ctx->exact_growth ? bytes : bytes + bytes / 2;
+    want = (want + 255) & ~size_t(255);
+    if (b.ptr) {
+        // Queued work may still read the old buffer.
+        hipError_t e = hipDeviceSynchronize();
+        if (e != hipSuccess) {
+            return e;
+        }
+        e = hipFree(b.ptr);
+        b.ptr = nullptr;
+        b.bytes = 0;
+        if (e != hipSuccess) {
+            return e;
+        }
+    }
+    hipError_t e = hipMalloc(&b.ptr, want);
+    if (e != hipSuccess) {
+        b.ptr = nullptr;
+        return e;
+    }
+    b.bytes = want;
+    return hipSuccess;
+}
+
+void release(DeviceBuffer& b) {
+    if (b.ptr) {
+        (void)hipFree(b.ptr);
+    }
+    b.ptr = nullptr;
+    b.bytes = 0;
+}
+
+template <typename T>
+T* as(const DeviceBuffer& b) {
+    return static_cast<T*>(b.ptr);
+}
+
+athx_status check_view(athx_ctx* ctx, const athx_view& v, uint32_t k) {
+    if (v.width == 0 || v.height == 0) {
+        return fail(ctx, ATHX_INVALID_ARGUMENT, "view " + std::to_string(k) + ": zero size");
+    }
+    if (!(v.focal_x > 0.0f) || !(v.focal_y > 0.0f)) {
+        return fail(ctx, ATHX_INVALID_ARGUMENT, "view " + std::to_string(k) + ": focal length must be positive");
+    }
+    return ATHX_OK;
+}
+
+athx_status check_settings(athx_ctx* ctx, const athx_settings* s) {
+    if (!s) {
+        return fail(ctx, ATHX_INVALID_ARGUMENT, "settings is null");
+    }
+    if (s->tile != 8 && s->tile != 16 && s->tile != 32) {
+        return fail(ctx, ATHX_INVALID_ARGUMENT, "tile must be 8, 16 or 32");
+    }
+    if (s->batch < 32 || s->batch > 256 || s->batch % 32 != 0) {
+        return fail(ctx, ATHX_INVALID_ARGUMENT, "batch must be a multiple of 32 in [32, 256]");
+    }
+    return ATHX_OK;
+}
+
+athx_status check_gaussians(athx_ctx* ctx, const athx_gaussians* g) {
+    if (!g) {
+        return fail(ctx, ATHX_INVALID_ARGUMENT, "gaussians is null");
+    }
+    if (g->sh_coeffs != 1 && g->sh_coeffs != 4 && g->sh_coeffs != 9 && g->sh_coeffs != 16) {
+        return fail(ctx, ATHX_INVALID_ARGUMENT, "sh_coeffs must be 1, 4, 9 or 16");
+    }
+    if (g->count != 0) {
+        if (!g->means || !g->log_scales || !g->quats || !g->opacity_logits) {
+            return fail(ctx, ATHX_INVALID_ARGUMENT, "a Gaussian parameter pointer is null");
+        }
+        if ((g->sh == nullptr) == (g->sh_half == nullptr)) {
+            return fail(ctx, ATHX_INVALID_ARGUMENT, "exactly one of sh and sh_half must be given");
+        }
+    }
+    return ATHX_OK;
+}
+
+uint32_t degree_of(uint32_t coeffs) { return coeffs >= 16 ? 3u : coeffs >= 9 ? 2u : coeffs >= 4 ? 1u : 0u; }
+
+ProjectArgs project_args(const athx_gaussians* g, const athx_settings* s) {
+    ProjectArgs a{};
+    a.means = g->means;
+    a.log_scales = g->log_scales;
+    a.quats = g->quats;
+    a.logits = g->opacity_logits;
+    a.sh = g->sh;
+    a.sh_half = g->sh_half;
+    a.n = g->count;
+    a.sh_coeffs = g->sh_coeffs;
+    const uint32_t held = degree_of(g->sh_coeffs);
+    a.sh_degree = held < s->max_sh_degree ? held : s->max_sh_degree;
+    a.linear = s->linear ? 1u : 0u;
+    a.antialias = s->antialias ? 1u : 0u;
+    a.tile = s->tile;
+    return a;
+}
+
+// Fills the pinned staging views and tile bases; returns the frame's tiles.
+athx_status stage_views(athx_ctx* ctx, const athx_view* views, uint32_t nviews, const athx_settings* s,
+                        const athx_targets* targets, const float* const* dl, uint64_t& total_tiles) {
+    if (ctx->upload_pending) {
+        ATHX_HIP(ctx, hipEventSynchronize(ctx->upload_done), "wait for the last upload");
+        ctx->upload_pending = false;
+    }
+    if (nviews > ctx->host_view_capacity) {
+        if (ctx->host_views) {
+            (void)hipHostFree(ctx->host_views);
+            ctx->host_views = nullptr;
+        }
+        if (ctx->host_tile_bases) {
+            (void)hipHostFree(ctx->host_tile_bases);
+            ctx->host_tile_bases = nullptr;
+        }
+        ctx->host_view_capacity = 0;
+        ATHX_HIP(ctx, hipHostMalloc(reinterpret_cast<void**>(&ctx->host_views), sizeof(DView) * nviews),
+                 "pinned views");
+        ATHX_HIP(ctx, hipHostMalloc(reinterpret_cast<void**>(&ctx->host_tile_bases), sizeof(uint32_t) * (nviews + 1)),
+                 "pinned tile bases");
+        ctx->host_view_capacity = nviews;
+    }
+    total_tiles = 0;
+    for (uint32_t k = 0; k < nviews; ++k) {
+        const athx_view& v = views[k];
+        if (athx_status st = check_view(ctx, v, k); st != ATHX_OK) {
+            return st;
+        }
+        if (!targets[k].colour) {
+            return fail(ctx, ATHX_INVALID_ARGUMENT, "targets[" + std::to_string(k) + "].colour is null");
+        }
+        DView& d = ctx->host_views[k];
+        std::memset(&d, 0, sizeof d);
+        d.width = v.width;
+        d.height = v.height;
+        d.tiles_x = div_up(v.width, s->tile);
+        d.tiles_y = div_up(v.height, s->tile);
+        d.focal_x = v.focal_x;
+        d.focal_y = v.focal_y;
+        d.centre_x = v.centre_x;
+        d.centre_y = v.centre_y;
+        d.near_z = v.near_z;
+        d.far_z = v.far_z;
+        d.orthographic = v.orthographic ? 1u : 0u;
+        d.tile_base = (uint32_t)total_tiles;
+        std::memcpy(d.m, v.object_to_view, sizeof d.m);
+        std::memcpy(d.eye, v.eye, sizeof d.eye);
+        std::memcpy(d.bg, v.background, sizeof d.bg);
+        d.colour = targets[k].colour;
+        d.depth = targets[k].depth;
+        d.final_t = targets[k].final_t;
+        d.last = targets[k].last;
+        d.dl_dcolour = dl ? dl[k] : nullptr;
+        ctx->host_tile_bases[k] = (uint32_t)total_tiles;
+        total_tiles += (uint64_t)d.tiles_x * d.tiles_y;
+        if (total_tiles >= (uint64_t{1} << 32) / 2) {
+            return fail(ctx, ATHX_UNSUPPORTED, "too many tiles in one frame");
+        }
+    }
+    ctx->host_tile_bases[nviews] = (uint32_t)total_tiles;
+    return ATHX_OK;
+}
+
+athx_status upload_views(athx_ctx* ctx, uint32_t nviews, hipStream_t s) {
+    ATHX_HIP(ctx, reserve(ctx, ctx->views, sizeof(DView) * nviews), "views");
+    ATHX_HIP(ctx, reserve(ctx, ctx->tile_bases, sizeof(uint32_t) * (nviews + 1)), "tile bases");
+    ATHX_HIP(ctx, hipMemcpyAsync(ctx->views.ptr, ctx->host_views, sizeof(DView) * nviews, hipMemcpyHostToDevice, s),
+             "upload views");
+    ATHX_HIP(ctx, hipMemcpyAsync(ctx->tile_bases.ptr, ctx->host_tile_bases, sizeof(uint32_t) * (nviews + 1),
+                                 hipMemcpyHostToDevice, s),
+             "upload tile bases");
+    ATHX_HIP(ctx, hipEventRecord(ctx->upload_done, s), "record upload");
+    ctx->upload_pending = true;
+    return ATHX_OK;
+}
+
+double elapsed_ms(hipEvent_t a, hipEvent_t b) {
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, a, b) != hipSuccess) {
+        return 0.0;
+    }
+    return ms;
+}
+
+// Sort metadata words for n pairs: hist (passes * 256), look-back
+// (passes * partitions * 256), counters (passes), back to back.
+size_t sort_meta_words(uint32_t n, uint32_t passes) {
+    return (size_t)passes * 256 + (size_t)passes * sort_partitions(n) * 256 + passes;
+}
+
+SortScratch sort_scratch(const DeviceBuffer& meta, uint64_t* alt_keys, uint32_t* alt_vals, uint32_t n,
+                         uint32_t passes) {
+    SortScratch sc{};
+    sc.keys_alt = alt_keys;
+    sc.values_alt = alt_vals;
+    sc.hist = static_cast<uint32_t*>(meta.ptr);
+    sc.lookback = sc.hist + (size_t)passes * 256;
+    sc.counters = sc.lookback + (size_t)passes * sort_partitions(n) * 256;
+    return sc;
+}
+
+}  // namespace
+
+extern "C" {
+
+ATHX_API const char* athx_version(void) { return "athx 0.1 (gfx950)"; }
+
+ATHX_API athx_status athx_create(int device, athx_ctx** out) {
+    if (!out) {
+        return ATHX_INVALID_ARGUMENT;
+    }
+    *out = nullptr;
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count == 0) {
+        return ATHX_DEVICE_FAILURE;
+    }
+    if (device < 0 || device >= count) {
+        return ATHX_INVALID_ARGUMENT;
+    }
+    athx_ctx* ctx = new (std::nothrow) athx_ctx();
+    if (!ctx) {
+        return ATHX_OUT_OF_MEMORY;
+    }
+    ctx->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipHostMalloc(reinterpret_cast<void**>(&ctx->host_totals), 16) != hipSuccess) {
+        delete ctx;
+        return ATHX_DEVICE_FAILURE;
+    }
+    for (hipEvent_t& ev : ctx->ev) {
+        if (hipEventCreate(&ev) != hipSuccess) {
+            ev = nullptr;
+        }
+    }
+    if (hipEventCreateWithFlags(&ctx->upload_done, hipEventDisableTiming) != hipSuccess) {
+        athx_destroy(ctx);
+        return ATHX_DEVICE_FAILURE;
+    }
+    *out = ctx;
+    return ATHX_OK;
+}
+
+ATHX_API void athx_destroy(athx_ctx* ctx) {
+    if (!ctx) {
+        return;
+    }
+    (void)hipSetDevice(ctx->device);
+    (void)hipDeviceSynchronize();
+    for (DeviceBuffer* b : {&ctx->views, &ctx->tile_bases, &ctx->records, &ctx->touched, &ctx->depth_keys,
+                            &ctx->offsets, &ctx->scan_status, &ctx->totals, &ctx->pair_keys, &ctx->pair_vals,
+                            &ctx->alt_keys, &ctx->alt_vals, &ctx->sort_meta, &ctx->ranges, &ctx->grad2d,
+                            &ctx->counter_words, &ctx->api_sort_meta, &ctx->api_alt_keys, &ctx->api_alt_vals,
+                            &ctx->api_scan_status}) {
+        release(*b);
+    }
+    if (ctx->host_totals) {
+        (void)hipHostFree(ctx->host_totals);
+    }
+    if (ctx->host_views) {
+        (void)hipHostFree(ctx->host_views);
+    }
+    if (ctx->host_tile_bases) {
+        (void)hipHostFree(ctx->host_tile_bases);
+    }
+    for (hipEvent_t ev : ctx->ev) {
+        if (ev) {
+            (void)hipEventDestroy(ev);
+        }
+    }
+    if (ctx->upload_done) {
+        (void)hipEventDestroy(ctx->upload_done);
+    }
+    delete ctx;
+}
+
+ATHX_API const char* athx_last_error(const athx_ctx* ctx) { return ctx ? ctx->error.c_str() : "no context"; }
+
+ATHX_API athx_status athx_release_scratch(athx_ctx* ctx) {
+    if (!ctx) {
+        return ATHX_INVALID_ARGUMENT;
+    }
+    (void)hipSetDevice(ctx->device);
+    ATHX_HIP(ctx, hipDeviceSynchronize(), "synchronize before release");
+    for (DeviceBuffer* b : {&ctx->records, &ctx->touched, &ctx->depth_keys, &ctx->offsets, &ctx->scan_status,
+                            &ctx->pair_keys, &ctx->pair_vals, &ctx->alt_keys, &ctx->alt_vals, &ctx->sort_meta,
+                            &ctx->ranges, &ctx->grad2d, &ctx->api_sort_meta, &ctx->api_alt_keys,
+                            &ctx->api_alt_vals, &ctx->api_scan_status}) {
+        release(*b);
+    }
+    ctx->exact_growth = true;
+    ctx->have_frame = false;
+    return ATHX_OK;
+}
+
+ATHX_API uint64_t athx_scratch_bytes(const athx_ctx* ctx) {
+    if (!ctx) {
+        return 0;
+    }
+    uint64_t sum = 0;
+    for (const DeviceBuffer* b : {&ctx->views, &ctx->tile_bases, &ctx->records, &ctx->touched, &ctx->depth_keys,
+                                  &ctx->offsets, &ctx->scan_status, &ctx->totals, &ctx->pair_keys, &ctx->pair_vals,
+                                  &ctx->alt_keys, &ctx->alt_vals, &ctx->sort_meta, &ctx->ranges, &ctx->grad2d,
+                                  &ctx->counter_words, &ctx->api_sort_meta, &ctx->api_alt_keys, &ctx->api_alt_vals,
+                                  &ctx->api_scan_status}) {
+        sum += b->bytes;
+    }
+    return sum;
+}
+
+ATHX_API athx_status athx_render(athx_ctx* ctx, const athx_view* views, uint32_t nviews,
+                                 const athx_gaussians* gaussians, const athx_settings* settings,
+                                 const athx_targets* targets, void* stream, athx_frame_stats* stats) try {
+    if (!ctx) {
+        return ATHX_INVALID_ARGUMENT;
+    }
+    ctx->error.clear();
+    if (!views || nviews == 0 || !targets) {
+        return fail(ctx, ATHX_INVALID_ARGUMENT, "no views or targets");
+    }
+    if (athx_status st = check_settings(ctx, settings); st != ATHX_OK) {
+        return st;
+    }
+    if (athx_status st = check_gaussians(ctx, gaussians); st != ATHX_OK) {
+        return st;
+    }
+    const uint64_t slots64 = (uint64_t)gaussians->count * nviews;
+    if (slots64 >= (uint64_t{1} << 32)) {
+        // TileRasterizer.cpp:323-325: slots are 32-bit.
+        return fail(ctx, ATHX_UNSUPPORTED, "more than 2^32 (view, Gaussian) slots in one frame");
+    }
+    const uint32_t slots = (uint32_t)slots64;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    ATHX_HIP(ctx, hipSetDevice(ctx->device), "set device");
+    uint64_t tiles64 = 0;
+    if (athx_status st = stage_views(ctx, views, nviews, settings, targets, nullptr, tiles64); st != ATHX_OK) {
+        return st;
+    }
+    const uint32_t tiles = (uint32_t)tiles64;
+    const bool timed = settings->time_stages != 0 && ctx->ev[7] != nullptr;
+    const auto wall0 = std::chrono::steady_clock::now();
+    if (athx_status st = upload_views(ctx, nviews, s); st != ATHX_OK) {
+        return st;
+    }
+    const DView* dviews = as<DView>(ctx->views);
+    ATHX_HIP(ctx, reserve(ctx, ctx->records, (size_t)slots * kRecordFloats * sizeof(float) + 16), "records");
+    ATHX_HIP(ctx, reserve(ctx, ctx->touched, (size_t)slots * 4 + 16), "touched");
+    ATHX_HIP(ctx, reserve(ctx, ctx->depth_keys, (size_t)slots * 4 + 16), "depth keys");
+    ATHX_HIP(ctx, reserve(ctx, ctx->offsets, (size_t)slots * 4 + 16), "offsets");
+    ATHX_HIP(ctx, reserve(ctx, ctx->scan_status, scan_status_words(slots) * 4), "scan status");
+    ATHX_HIP(ctx, reserve(ctx, ctx->totals, 16), "totals");
+    ATHX_HIP(ctx, reserve(ctx, ctx->ranges, (size_t)tiles * 8 + 16), "ranges");
+
+    const ProjectArgs pa = project_args(gaussians, settings);
+    if (timed) {
+        ATHX_HIP(ctx, hipEventRecord(ctx->ev[0], s), "event");
+    }
+    ATHX_HIP(ctx, launch_project(pa, dviews, nviews, as<float>(ctx->records), as<uint32_t>(ctx->touched),
+                                 as<uint32_t>(ctx->depth_keys), s),
+             "project");
+    if (timed) {
+        ATHX_HIP(ctx, hipEventRecord(ctx->ev[1], s), "event");
+    }
+    uint64_t* totals = as<uint64_t>(ctx->totals);
+    ATHX_HIP(ctx, launch_scan(as<uint32_t>(ctx->touched), as<uint32_t>(ctx->offsets), totals, slots,
+                              as<uint32_t>(ctx->scan_status), s),
+             "scan");
+    if (timed) {
+        ATHX_HIP(ctx, hipEventRecord(ctx->ev[2], s), "event");
+    }
+    // The one wait of the frame: how many pairs there are.
+    ATHX_HIP(ctx, hipMemcpyAsync(ctx->host_totals, totals, 8, hipMemcpyDeviceToHost, s), "read pair total");
+    ATHX_HIP(ctx, hipStreamSynchronize(s), "wait for pair total");
+    const uint64_t pairs64 = ctx->host_totals[0];
+    if (pairs64 >= (uint64_t{1} << 30)) {
+        return fail(ctx, ATHX_UNSUPPORTED, "more than 2^30 (tile, Gaussian) pairs in one frame");
+    }
+    const uint32_t pairs = (uint32_t)pairs64;
+    const uint32_t key_bits = kDepthBits + bits_for(tiles);
+    const uint32_t passes = div_up(key_bits, 8);
+    ATHX_HIP(ctx, reserve(ctx, ctx->pair_keys, (size_t)pairs * 8 + 16), "pair keys");
+    ATHX_HIP(ctx, reserve(ctx, ctx->pair_vals, (size_t)pairs * 4 + 16), "pair values");
+    ATHX_HIP(ctx, reserve(ctx, ctx->alt_keys, (size_t)pairs * 8 + 16), "sort keys");
+    ATHX_HIP(ctx, reserve(ctx, ctx->alt_vals, (size_t)pairs * 4 + 16), "sort values");
+    ATHX_HIP(ctx, reserve(ctx, ctx->sort_meta, sort_meta_words(pairs, passes) * 4 + 16), "sort metadata");
+    ATHX_HIP(ctx, launch_emit(dviews, nviews, gaussians->count, settings->tile, as<float>(ctx->records),
+                              as<uint32_t>(ctx->touched), as<uint32_t>(ctx->depth_keys), as<uint32_t>(ctx->offsets),
+                              as<uint64_t>(ctx->pair_keys), as<uint32_t>(ctx->pair_vals), s),
+             "emit");
+    if (timed) {
+        ATHX_HIP(ctx, hipEventRecord(ctx->ev[3], s), "event");
+    }
+    const SortScratch sc = sort_scratch(ctx->sort_meta, as<uint64_t>(ctx->alt_keys), as<uint32_t>(ctx->alt_vals),
+                                        pairs, passes);
+    ATHX_HIP(ctx, launch_sort(as<uint64_t>(ctx->pair_keys), as<uint32_t>(ctx->pair_vals), pairs, key_bits, sc, s),
+             "sort");
+    if (timed) {
+        ATHX_HIP(ctx, hipEventRecord(ctx->ev[4], s), "event");
+    }
+    ATHX_HIP(ctx, launch_ranges(as<uint64_t>(ctx->pair_keys), pairs, tiles, as<uint32_t>(ctx->ranges), s), "ranges");
+    if (timed) {
+        ATHX_HIP(ctx, hipEventRecord(ctx->ev[5], s), "event");
+    }
+    ATHX_HIP(ctx, launch_blend(dviews, as<uint32_t>(ctx->tile_bases), nviews, tiles, settings->tile, settings->batch,
+                               as<uint32_t>(ctx->ranges), as<uint32_t>(ctx->pair_vals), as<float>(ctx->records),
+                               settings->depth_mode, settings->depth_threshold, s),
+             "blend");
+    if (timed) {
+        ATHX_HIP(ctx, hipEventRecord(ctx->ev[6], s), "event");
+    }
+    if (settings->count_splats) {
+        ATHX_HIP(ctx, reserve(ctx, ctx->counter_words, 64), "counters");
+        ATHX_HIP(ctx, launch_fill_u32(as<uint32_t>(ctx->counter_words), 0u, 16, s), "clear counters");
+        ATHX_HIP(ctx, launch_counters(as<uint32_t>(ctx->depth_keys), as<uint32_t>(ctx->touched), slots,
+                                      as<uint32_t>(ctx->counter_words), s),
+                 "counters");
+        ctx->counters_valid = true;
+        ctx->counters.slots = slots;
+    }
+    ctx->last_n = gaussians->count;
+    ctx->last_nviews = nviews;
+    ctx->last_tiles = tiles;
+    ctx->last_tile = settings->tile;
+    ctx->last_pairs = pairs;
+    ctx->last_key_bits = key_bits;
+    ctx->have_frame = true;
+    if (stats) {
+        std::memset(stats, 0, sizeof *stats);
+        stats->splats = gaussians->count;
+        stats->pairs = pairs;
+        if (timed) {
+            ATHX_HIP(ctx, hipEventSynchronize(ctx->ev[6]), "wait for frame");
+            stats->project_ms = elapsed_ms(ctx->ev[0], ctx->ev[1]);
+            stats->scan_ms = elapsed_ms(ctx->ev[1], ctx->ev[2]);
+            stats->emit_ms = elapsed_ms(ctx->ev[2], ctx->ev[3]);
+            stats->sort_ms = elapsed_ms(ctx->ev[3], ctx->ev[4]);
+            stats->ranges_ms = elapsed_ms(ctx->ev[4], ctx->ev[5]);
+            stats->blend_ms = elapsed_ms(ctx->ev[5], ctx->ev[6]);
+            stats->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wall0).count();
+        }
+    }
+    return ATHX_OK;
+} catch (const std::bad_alloc&) {
+    return fail(ctx, ATHX_OUT_OF_MEMORY, "host allocation failed");
+} catch (const std::exception& e) {
+    return fail(ctx, ATHX_INTERNAL_ERROR, e.what());
+}
+
+ATHX_API athx_status athx_render_backward(athx_ctx* ctx, const athx_view* views, uint32_t nviews,
+                                          const athx_gaussians* gaussians, const athx_settings* settings,
+                                          const athx_targets* targets, const float* const* dl_dcolour,
+                                          const athx_grads* grads, void* stream) try {
+    if (!ctx) {
+        return ATHX_INVALID_ARGUMENT;
+    }
+    ctx->error.clear();
+    if (!views || nviews == 0 || !targets || !dl_dcolour || !grads) {
+        return fail(ctx, ATHX_INVALID_ARGUMENT, "no views, targets, dL/dcolour or gradients");
+    }
+    if (athx_status st = check_settings(ctx, settings); st != ATHX_OK) {
+        return st;
+    }
+    if (athx_status st = check_gaussians(ctx, gaussians); st != ATHX_OK) {
+        return st;
+    }
+    if (!ctx->have_frame || ctx->last_n != gaussians->count || ctx->last_nviews != nviews ||
+        ctx->last_tile != settings->tile) {
+        return fail(ctx, ATHX_INVALID_ARGUMENT, "the backward pass must follow a render of the same frame");
+    }
+    for (uint32_t k = 0; k < nviews; ++k) {
+        if (!targets[k].final_t || !targets[k].last || !dl_dcolour[k]) {
+            return fail(ctx, ATHX_INVALID_ARGUMENT,
+                        "view " + std::to_string(k) + ": final_t, last and dL/dcolour are required");
+        }
+    }
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    ATHX_HIP(ctx, hipSetDevice(ctx->device), "set device");
+    uint64_t tiles64 = 0;
+    if (athx_status st = stage_views(ctx, views, nviews, settings, targets, dl_dcolour, tiles64); st != ATHX_OK) {
+        return st;
+    }
+    if (tiles64 != ctx->last_tiles) {
+        return fail(ctx, ATHX_INVALID_ARGUMENT, "the views differ from the rendered frame's");
+    }
+    if (athx_status st = upload_views(ctx, nviews, s); st != ATHX_OK) {
+        return st;
+    }
+    const uint32_t slots = gaussians->count * nviews;
+    ATHX_HIP(ctx, reserve(ctx, ctx->grad2d, (size_t)slots * kGrad2d * sizeof(float) + 16), "2D gradients");
+    ATHX_HIP(ctx, launch_fill_u32(as<uint32_t>(ctx->grad2d), 0u, (uint64_t)slots * kGrad2d, s), "clear gradients");
+    ATHX_HIP(ctx, launch_blend_backward(as<DView>(ctx->views), as<uint32_t>(ctx->tile_bases), nviews, ctx->last_tiles,
+                                        settings->tile, settings->batch, as<uint32_t>(ctx->ranges),
+                                        as<uint32_t>(ctx->pair_vals), as<float>(ctx->records),
+                                        as<float>(ctx->grad2d), s),
+             "blend backward");
+    const ProjectArgs pa = project_args(gaussians, settings);
+    ATHX_HIP(ctx, launch_project_backward(pa, as<DView>(ctx->views), nviews, as<uint32_t>(ctx->touched),
+                                          as<float>(ctx->grad2d), *grads, s),
+             "projection backward");
+    return ATHX_OK;
+} catch (const std::bad_alloc&) {
+    return fail(ctx, ATHX_OUT_OF_MEMORY, "host allocation failed");
+} catch (const std::exception& e) {
+    return fail(ctx, ATHX_INTERNAL_ERROR, e.what());
+}
+
+ATHX_API athx_status athx_latest_counters(athx_ctx* ctx, athx_counters* out) {
+    if (!ctx || !out) {
+        return ATHX_INVALID_ARGUMENT;
+    }
+    if (!ctx->counters_valid) {
+        return fail(ctx, ATHX_NOT_FOUND, "no frame was rendered with count_splats");
+    }
+    uint32_t words[16] = {};
+    ATHX_HIP(ctx, hipSetDevice(ctx->device), "set device");
+    ATHX_HIP(ctx, hipDeviceSynchronize(), "wait for counters");
+    ATHX_HIP(ctx, hipMemcpy(words, ctx->counter_words.ptr, sizeof words, hipMemcpyDeviceToHost), "read counters");
+    std::memset(out, 0, sizeof *out);
+    out->slots = ctx->counters.slots;
+    out->visible = words[1];
+    out->pairs = (uint64_t)words[2] | ((uint64_t)words[3] << 32);
+    out->max_tiles = words[4];
+    for (uint32_t r = 0; r < ATHX_CULL_REASONS; ++r) {
+        out->culled[r] = words[8 + r];
+    }
+    return ATHX_OK;
+}
+
+ATHX_API athx_status athx_frame_info_get(const athx_ctx* ctx, athx_frame_info* out) {
+    if (!ctx || !out) {
+        return ATHX_INVALID_ARGUMENT;
+    }
+    if (!ctx->have_frame) {
+        return ATHX_NOT_FOUND;
+    }
+    out->slots = ctx->last_n * ctx->last_nviews;
+    out->tiles = ctx->last_tiles;
+    out->key_bits = ctx->last_key_bits;
+    out->pairs = ctx->last_pairs;
+    return ATHX_OK;
+}
+
+ATHX_API athx_status athx_copy_frame(athx_ctx* ctx, athx_frame_part part, void* dst, uint64_t bytes, void* stream) {
+    if (!ctx) {
+        return ATHX_INVALID_ARGUMENT;
+    }
+    ctx->error.clear();
+    if (!ctx->have_frame) {
+        return fail(ctx, ATHX_NOT_FOUND, "no frame has been rendered");
+    }
+    const uint64_t slots = (uint64_t)ctx->last_n * ctx->last_nviews;
+    const void* src = nullptr;
+    uint64_t size = 0;
+    switch (part) {
+    case ATHX_PART_RECORDS: src = ctx->records.ptr; size = slots * kRecordFloats * 4; break;
+    case ATHX_PART_TOUCHED: src = ctx->touched.ptr; size = slots * 4; break;
+    case ATHX_PART_DEPTH_KEYS: src = ctx->depth_keys.ptr; size = slots * 4; break;
+    case ATHX_PART_OFFSETS: src = ctx->offsets.ptr; size = slots * 4; break;
+    case ATHX_PART_PAIR_KEYS: src = ctx->pair_keys.ptr; size = ctx->last_pairs * 8; break;
+    case ATHX_PART_PAIR_VALUES: src = ctx->pair_vals.ptr; size = ctx->last_pairs * 4; break;
+    case ATHX_PART_RANGES: src = ctx->ranges.ptr; size = (uint64_t)ctx->last_tiles * 8; break;
+    default: return fail(ctx, ATHX_INVALID_ARGUMENT, "unknown frame part");
+    }
+    if (bytes != size) {
+        return fail(ctx, ATHX_INVALID_ARGUMENT,
+                    "part is " + std::to_string(size) + " bytes, asked for " + std::to_string(bytes));
+    }
+    if (size == 0) {
+        return ATHX_OK;
+    }
+    if (!dst) {
+        return fail(ctx, ATHX_INVALID_ARGUMENT, "null destination");
+    }
+    ATHX_HIP(ctx, hipSetDevice(ctx->device), "set device");
+    ATHX_HIP(ctx, hipMemcpyAsync(dst, src, size, hipMemcpyDeviceToDevice, static_cast<hipStream_t>(stream)),
+             "copy frame part");
+    return ATHX_OK;
+}
+
+ATHX_API athx_status athx_exclusive_scan(athx_ctx* ctx, const uint32_t* in, uint32_t* out, uint64_t* total,
+                                         uint32_t n, void* stream) {
+    if (!ctx) {
+        return ATHX_INVALID_ARGUMENT;
+    }
+    ctx->error.clear();
+    if ((n != 0 && (!in || !out)) || !total) {
+        return fail(ctx, ATHX_INVALID_ARGUMENT, "null buffer");
+    }
+    if (in == out) {
+        return fail(ctx, ATHX_INVALID_ARGUMENT, "input and output must be different buffers (PrefixSum.h:41-42)");
+    }
+    ATHX_HIP(ctx, hipSetDevice(ctx->device), "set device");
+    ATHX_HIP(ctx, reserve(ctx, ctx->api_scan_status, scan_status_words(n) * 4), "scan status");
+    ATHX_HIP(ctx, launch_scan(in, out, total, n, as<uint32_t>(ctx->api_scan_status), static_cast<hipStream_t>(stream)),
+             "scan");
+    return ATHX_OK;
+}
+
+ATHX_API athx_status athx_sort_pairs(athx_ctx* ctx, uint64_t* keys, uint32_t* values, uint32_t n, uint32_t key_bits,
+                                     void* stream) {
+    if (!ctx) {
+        return ATHX_INVALID_ARGUMENT;
+    }
+    ctx->error.clear();
+    if (key_bits == 0 || key_bits > 64) {
+        return fail(ctx, ATHX_INVALID_ARGUMENT, "key_bits must be in 1..64 (RadixSort.h:61)");
+    }
+    if (n != 0 && (!keys || !values)) {
+        return fail(ctx, ATHX_INVALID_ARGUMENT, "null buffer");
+    }
+    if (n >= (1u << 30)) {
+        return fail(ctx, ATHX_UNSUPPORTED, "at most 2^30 - 1 pairs a sort");
+    }
+    if (n < 2) {
+        return ATHX_OK;
+    }
+    const uint32_t passes = div_up(key_bits, 8);
+    ATHX_HIP(ctx, hipSetDevice(ctx->device), "set device");
+    ATHX_HIP(ctx, reserve(ctx, ctx->api_alt_keys, (size_t)n * 8), "sort keys");
+    ATHX_HIP(ctx, reserve(ctx, ctx->api_alt_vals, (size_t)n * 4), "sort values");
+    ATHX_HIP(ctx, reserve(ctx, ctx->api_sort_meta, sort_meta_words(n, passes) * 4), "sort metadata");
+    const SortScratch sc = sort_scratch(ctx->api_sort_meta, as<uint64_t>(ctx->api_alt_keys),
+                                        as<uint32_t>(ctx->api_alt_vals), n, passes);
+    ATHX_HIP(ctx, launch_sort(keys, values, n, key_bits, sc, static_cast<hipStream_t>(stream)), "sort");
+    return ATHX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,448 @@
+// project.hip -- the splat projection for gfx950: one thread per Gaussian,
+// looping over the frame's views, so each Gaussian's parameters are read
+// from HBM once however many views a batch holds.
+//
+// Restates shaders/athenea/splat/splat_project.slang:497-1130 (projectSplat
+// with kTransfer 0: no edit, no lens, no shutter, no surfels, not relit) and
+// its helpers in shaders/athenea/splat/frame.slang:283-402. Arithmetic is in
+// the order the reference writes it; the file is compiled without
+// floating-point contraction, and exp/log come from athx_detmath.h, so the
+// depth keys, tile counts and records are the same bits a host restatement
+// computes.
+#include "athx_internal.h"
+
+namespace athx {
+namespace {
+
+constexpr float kSH0 = 0.28209479177387814f;
+constexpr float kC1 = 0.4886025119029199f;
+constexpr float kC2_0 = 1.0925484305920792f, kC2_1 = -1.0925484305920792f, kC2_2 = 0.31539156525252005f,
+                kC2_3 = -1.0925484305920792f, kC2_4 = 0.5462742152960396f;
+constexpr float kC3_0 = -0.5900435899266435f, kC3_1 = 2.890611442640554f, kC3_2 = -0.4570457994644658f,
+                kC3_3 = 0.3731763325901154f, kC3_4 = -0.4570457994644658f, kC3_5 = 1.445305721320277f,
+                kC3_6 = -0.5900435899266435f;
+
+__device__ inline float fmin_(float a, float b) { return a < b ? a : b; }
+__device__ inline float fmax_(float a, float b) { return a > b ? a : b; }
+__device__ inline float clamp_(float x, float lo, float hi) { return fmin_(fmax_(x, lo), hi); }
+
+// frame.slang:325-327
+__device__ inline float cutoff_power(float alpha) {
+    return fmin_(2.0f * athx_logf(fmax_(255.0f * alpha, 1.0001f)), 9.0f);
+}
+
+// frame.slang:354-371
+__device__ inline float min_power_over_box(float a, float b, float c, float lox, float loy, float hix, float hiy) {
+    if (lox <= 0.0f && hix >= 0.0f && loy <= 0.0f && hiy >= 0.0f) {
+        return 0.0f;
+    }
+    float best = 3.4e38f;
+    for (int e = 0; e < 2; ++e) {
+        const float x = e == 0 ? lox : hix;
+        const float y = c > 1e-12f ? clamp_(((-b) * x) / c, loy, hiy) : loy;
+        best = fmin_(best, a * x * x + 2.0f * b * x * y + c * y * y);
+    }
+    for (int e = 0; e < 2; ++e) {
+        const float y = e == 0 ? loy : hiy;
+        const float x = a > 1e-12f ? clamp_(((-b) * y) / a, lox, hix) : lox;
+        best = fmin_(best, a * x * x + 2.0f * b * x * y + c * y * y);
+    }
+    return best;
+}
+
+// frame.slang:374-379
+__device__ inline bool tile_touched(uint32_t W, uint32_t H, uint32_t T, float cx, float cy, float a, float b,
+                                    float c, float cutoff, uint32_t tx, uint32_t ty) {
+    const float fx = (float)(tx * T) + 0.5f;
+    const float fy = (float)(ty * T) + 0.5f;
+    const uint32_t ex = (tx + 1) * T < W ? (tx + 1) * T : W;
+    const uint32_t ey = (ty + 1) * T < H ? (ty + 1) * T : H;
+    const float lx = (float)(ex - 1) + 0.5f;
+    const float ly = (float)(ey - 1) + 0.5f;
+    return min_power_over_box(a, b, c, fx - cx, fy - cy, lx - cx, ly - cy) <= cutoff;
+}
+
+// frame.slang:383-402: the footprint's tile rectangle, inclusive; false when
+// it lies off screen.
+__device__ inline bool tile_rect(uint32_t W, uint32_t H, uint32_t T, float cx, float cy, float a, float b, float c,
+                                 float cutoff, uint32_t& x0, uint32_t& y0, uint32_t& x1, uint32_t& y1) {
+    const float det = a * c - b * b;
+    const float inv = det > 1e-20f ? 1.0f / det : 0.0f;
+    const float ex = sqrtf(fmax_(cutoff * c * inv, 0.0f));
+    const float ey = sqrtf(fmax_(cutoff * a * inv, 0.0f));
+    const float lx = cx - ex, ly = cy - ey, hx = cx + ex, hy = cy + ey;
+    if (hx < 0.0f || hy < 0.0f || lx >= (float)W || ly >= (float)H) {
+        x0 = 1; y0 = 1; x1 = 0; y1 = 0;
+        return false;
+    }
+    x0 = (uint32_t)fmax_(lx, 0.0f) / T;
+    y0 = (uint32_t)fmax_(ly, 0.0f) / T;
+    x1 = (uint32_t)fmin_(hx, (float)W - 1.0f) / T;
+    y1 = (uint32_t)fmin_(hy, (float)H - 1.0f) / T;
+    return x0 <= x1 && y0 <= y1;
+}
+
+__device__ inline float half_bits_to_float(uint16_t h) { return (float)__builtin_bit_cast(_Float16, h); }
+
+template <bool kHalf>
+__device__ inline float sh_at(const float* sh, const uint16_t* shh, size_t at) {
+    if constexpr (kHalf) {
+        return half_bits_to_float(shh[at]);
+    } else {
+        return sh[at];
+    }
+}
+
+// shaders/athenea/common/sh.slang:25-51, one channel; coefficient k of the
+// channel at row[k * 3] (k = 0, the constant term, is read by the caller).
+template <bool kHalf>
+__device__ inline float sh_rest(const float* sh, const uint16_t* shh, size_t row, uint32_t degree, float x, float y,
+                                float z) {
+    if (degree == 0) {
+        return 0.0f;
+    }
+#define B(k) sh_at<kHalf>(sh, shh, row + (k) * 3)
+    float r = -kC1 * y * B(1) + kC1 * z * B(2) - kC1 * x * B(3);
+    if (degree < 2) {
+        return r;
+    }
+    const float xx = x * x, yy = y * y, zz = z * z;
+    const float xy = x * y, yz = y * z, xz = x * z;
+    r = r + (kC2_0 * xy * B(4) + kC2_1 * yz * B(5) + kC2_2 * (2.0f * zz - xx - yy) * B(6) + kC2_3 * xz * B(7) +
+             kC2_4 * (xx - yy) * B(8));
+    if (degree < 3) {
+        return r;
+    }
+    r = r + (kC3_0 * y * (3.0f * xx - yy) * B(9) + kC3_1 * xy * z * B(10) + kC3_2 * y * (4.0f * zz - xx - yy) * B(11) +
+             kC3_3 * z * (2.0f * zz - 3.0f * xx - 3.0f * yy) * B(12) + kC3_4 * x * (4.0f * zz - xx - yy) * B(13) +
+             kC3_5 * z * (xx - yy) * B(14) + kC3_6 * x * (xx - 3.0f * yy) * B(15));
+#undef B
+    return r;
+}
+
+// shaders/athenea/common/color.slang:7-10
+__device__ inline float srgb_to_linear(float c) {
+    c = fmax_(c, 0.0f);
+    return c <= 0.04045f ? c / 12.92f : athx_powf_pos((c + 0.055f) / 1.055f, 2.4f);
+}
+
+__device__ inline void cull(uint32_t slot, uint32_t reason, uint32_t* touched, uint32_t* keys) {
+    touched[slot] = 0;
+    keys[slot] = kCulledKey | reason;
+}
+
+template <bool kHalf>
+__global__ void __launch_bounds__(256) project_kernel(ProjectArgs a, const DView* __restrict__ views, uint32_t nviews,
+                                                      float* __restrict__ records, uint32_t* __restrict__ touched,
+                                                      uint32_t* __restrict__ keys) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) {
+        return;
+    }
+    const float px = a.means[i * 3 + 0], py = a.means[i * 3 + 1], pz = a.means[i * 3 + 2];
+    const float sx = athx_expf(a.log_scales[i * 3 + 0]);
+    const float sy = athx_expf(a.log_scales[i * 3 + 1]);
+    const float sz = athx_expf(a.log_scales[i * 3 + 2]);
+    const float opacity = athx_sigmoidf(a.logits[i]);
+
+    // The rotation and R S, the same for every view (splat_project.slang:526-532).
+    float qx = a.quats[i * 4 + 0], qy = a.quats[i * 4 + 1], qz = a.quats[i * 4 + 2], qw = a.quats[i * 4 + 3];
+    const float qn = qx * qx + qy * qy + qz * qz + qw * qw;
+    if (qn > 0.0f) {
+        const float inv = 1.0f / sqrtf(qn);
+        qx = qx * inv; qy = qy * inv; qz = qz * inv; qw = qw * inv;
+    } else {
+        qx = 0.0f; qy = 0.0f; qz = 0.0f; qw = 1.0f;
+    }
+    const float r00 = 1.0f - 2.0f * (qy * qy + qz * qz), r01 = 2.0f * (qx * qy - qw * qz),
+                r02 = 2.0f * (qx * qz + qw * qy);
+    const float r10 = 2.0f * (qx * qy + qw * qz), r11 = 1.0f - 2.0f * (qx * qx + qz * qz),
+                r12 = 2.0f * (qy * qz - qw * qx);
+    const float r20 = 2.0f * (qx * qz - qw * qy), r21 = 2.0f * (qy * qz + qw * qx),
+                r22 = 1.0f - 2.0f * (qx * qx + qy * qy);
+    const float m00 = r00 * sx, m01 = r01 * sy, m02 = r02 * sz;
+    const float m10 = r10 * sx, m11 = r11 * sy, m12 = r12 * sz;
+    const float m20 = r20 * sx, m21 = r21 * sy, m22 = r22 * sz;
+
+    for (uint32_t v = 0; v < nviews; ++v) {
+        const DView& V = views[v];
+        const uint32_t slot = v * a.n + i;
+        const float* m = V.m;
+        const float tx = m[0] * px + m[1] * py + m[2] * pz + m[3];
+        const float ty = m[4] * px + m[5] * py + m[6] * pz + m[7];
+        const float tz = m[8] * px + m[9] * py + m[10] * pz + m[11];
+        if (!(tz > V.near_z) || tz > V.far_z) {
+            cull(slot, ATHX_CULL_DEPTH, touched, keys);
+            continue;
+        }
+        float jx0, jx2, jy1, jy2, cx, cy;
+        if (V.orthographic) {
+            jx0 = V.focal_x; jx2 = 0.0f;
+            jy1 = V.focal_y; jy2 = 0.0f;
+            cx = V.focal_x * tx + V.centre_x;
+            cy = V.focal_y * ty + V.centre_y;
+        } else {
+            const float invz = 1.0f / tz;
+            cx = V.focal_x * tx * invz + V.centre_x;
+            cy = V.focal_y * ty * invz + V.centre_y;
+            // The Jacobian at a point clamped to 1.3x the frustum (splat_project.slang:543-557).
+            const float W = (float)V.width, H = (float)V.height;
+            const float limlox = (-0.15f * W - V.centre_x) / V.focal_x;
+            const float limhix = (1.15f * W - V.centre_x) / V.focal_x;
+            const float limloy = (-0.15f * H - V.centre_y) / V.focal_y;
+            const float limhiy = (1.15f * H - V.centre_y) / V.focal_y;
+            const float txc = clamp_(tx * invz, limlox, limhix) * tz;
+            const float tyc = clamp_(ty * invz, limloy, limhiy) * tz;
+            jx0 = V.focal_x * invz;
+            jx2 = -V.focal_x * txc * invz * invz;
+            jy1 = V.focal_y * invz;
+            jy2 = -V.focal_y * tyc * invz * invz;
+        }
+        // T = J W, then (T M)(T M)^T (splat_project.slang:558-565).
+        const float t30 = jx0 * m[0] + jx2 * m[8];
+        const float t31 = jx0 * m[1] + jx2 * m[9];
+        const float t32 = jx0 * m[2] + jx2 * m[10];
+        const float u30 = jy1 * m[4] + jy2 * m[8];
+        const float u31 = jy1 * m[5] + jy2 * m[9];
+        const float u32 = jy1 * m[6] + jy2 * m[10];
+        const float ax0 = t30 * m00 + t31 * m10 + t32 * m20;
+        const float ax1 = t30 * m01 + t31 * m11 + t32 * m21;
+        const float ax2 = t30 * m02 + t31 * m12 + t32 * m22;
+        const float ay0 = u30 * m00 + u31 * m10 + u32 * m20;
+        const float ay1 = u30 * m01 + u31 * m11 + u32 * m21;
+        const float ay2 = u30 * m02 + u31 * m12 + u32 * m22;
+        float ca = ax0 * ax0 + ax1 * ax1 + ax2 * ax2;
+        const float cb = ax0 * ay0 + ax1 * ay1 + ax2 * ay2;
+        float cc = ay0 * ay0 + ay1 * ay1 + ay2 * ay2;
+        // Dilation and the Mip-Splatting compensation (splat_project.slang:651-667, 734-751).
+        const float det_before = ca * cc - cb * cb;
+        ca = ca + 0.3f;
+        cc = cc + 0.3f;
+        const float det = ca * cc - cb * cb;
+        if (!(det > 1e-12f)) {
+            cull(slot, ATHX_CULL_DEGENERATE, touched, keys);
+            continue;
+        }
+        const float filtered = sqrtf(fmax_(det_before, 0.0f) / det);
+        const float paid = a.antialias ? filtered : 1.0f;
+        const float spread = paid * opacity;
+        const float alpha = spread >= 1.0f / 255.0f ? spread + (1.0f - paid) * (1.0f / 255.0f) : spread;
+        if (!(alpha >= 1.0f / 255.0f)) {
+            cull(slot, ATHX_CULL_FAINT, touched, keys);
+            continue;
+        }
+        const float bdet = ca * cc - cb * cb;
+        const float ka = cc / bdet, kb = -cb / bdet, kc = ca / bdet;
+        const float cutoff = cutoff_power(alpha);
+        uint32_t x0, y0, x1, y1;
+        if (!tile_rect(V.width, V.height, a.tile, cx, cy, ka, kb, kc, cutoff, x0, y0, x1, y1)) {
+            cull(slot, ATHX_CULL_OFFSCREEN, touched, keys);
+            continue;
+        }
+        uint32_t count = 0;
+        for (uint32_t ty_ = y0; ty_ <= y1; ++ty_) {
+            for (uint32_t tx_ = x0; tx_ <= x1; ++tx_) {
+                count += tile_touched(V.width, V.height, a.tile, cx, cy, ka, kb, kc, cutoff, tx_, ty_) ? 1u : 0u;
+            }
+        }
+        if (count == 0) {
+            cull(slot, ATHX_CULL_NO_TILE, touched, keys);
+            continue;
+        }
+        // Colour from this eye, in the cloud's space (splat_project.slang:781-785).
+        const float dx = px - V.eye[0], dy = py - V.eye[1], dz = pz - V.eye[2];
+        const float dinv = 1.0f / sqrtf(dx * dx + dy * dy + dz * dz);
+        const float ex = dx * dinv, ey = dy * dinv, ez = dz * dinv;
+        const size_t row = (size_t)i * a.sh_coeffs * 3;
+        float col[3];
+#pragma unroll
+        for (uint32_t ch = 0; ch < 3; ++ch) {
+            const float base = 0.5f + kSH0 * sh_at<kHalf>(a.sh, a.sh_half, row + ch);
+            const float cv = base + sh_rest<kHalf>(a.sh, a.sh_half, row + ch, a.sh_degree, ex, ey, ez);
+            col[ch] = a.linear ? fmax_(cv, 0.0f) : srgb_to_linear(cv);
+        }
+        float4* rec = reinterpret_cast<float4*>(records + (size_t)slot * kRecordFloats);
+        rec[0] = make_float4(cx, cy, tz, cutoff);
+        rec[1] = make_float4(ka, kb, kc, alpha);
+        rec[2] = make_float4(col[0], col[1], col[2], 0.0f);
+        touched[slot] = count;
+        keys[slot] = __builtin_bit_cast(uint32_t, tz) >> 7;
+    }
+}
+
+__global__ void fill_u32_kernel(uint32_t* p, uint32_t value, uint64_t n) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        p[i] = value;
+    }
+}
+
+// One (tile, depth) pair per tile the footprint touches, a slot's tiles row
+// by row, at the slot's scanned offset: splat_emit.slang:17-50's test, emitted
+// in slot order with the 64-bit key (global tile << 24 | 24-bit depth key).
+__global__ void __launch_bounds__(256) emit_kernel(const DView* __restrict__ views, uint32_t nviews, uint32_t n,
+                                                   uint32_t T, const float* __restrict__ records,
+                                                   const uint32_t* __restrict__ touched,
+                                                   const uint32_t* __restrict__ depth_keys,
+                                                   const uint32_t* __restrict__ offsets, uint64_t* __restrict__ keys,
+                                                   uint32_t* __restrict__ values) {
+    const uint64_t slot = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (slot >= (uint64_t)n * nviews) {
+        return;
+    }
+    const uint32_t expected = touched[slot];
+    if (expected == 0) {
+        return;
+    }
+    const DView& V = views[slot / n];
+    const float4* rec = reinterpret_cast<const float4*>(records + slot * kRecordFloats);
+    const float4 screen = rec[0];
+    const float4 conic = rec[1];
+    uint32_t x0, y0, x1, y1;
+    tile_rect(V.width, V.height, T, screen.x, screen.y, conic.x, conic.y, conic.z, screen.w, x0, y0, x1, y1);
+    const uint64_t depth = depth_keys[slot] & 0xFFFFFFu;
+    uint32_t at = offsets[slot];
+    const uint32_t stop = at + expected;
+    for (uint32_t ty = y0; ty <= y1 && at < stop; ++ty) {
+        for (uint32_t tx = x0; tx <= x1 && at < stop; ++tx) {
+            if (tile_touched(V.width, V.height, T, screen.x, screen.y, conic.x, conic.y, conic.z, screen.w, tx, ty)) {
+                const uint64_t tile = (uint64_t)V.tile_base + (uint64_t)ty * V.tiles_x + tx;
+                keys[at] = (tile << kDepthBits) | depth;
+                values[at] = (uint32_t)slot;
+                ++at;
+            }
+        }
+    }
+}
+
+// Where each tile's pairs start and end in the sorted keys, by binary search
+// on the tile bits: splat_tiles_clear.slang:8-17 and splat_ranges.slang:11-25
+// give a tile with no pairs (0, 0), and so does this.
+__device__ inline uint32_t lower_bound_tile(const uint64_t* keys, uint32_t n, uint64_t tile) {
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+        const uint32_t mid = lo + ((hi - lo) >> 1);
+        if ((keys[mid] >> kDepthBits) < tile) {
+            lo = mid + 1;
+        } else {
+            hi = mid;
+        }
+    }
+    return lo;
+}
+
+__global__ void __launch_bounds__(256) ranges_kernel(const uint64_t* __restrict__ keys, uint32_t n, uint32_t tiles,
+                                                     uint32_t* __restrict__ ranges) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= tiles) {
+        return;
+    }
+    const uint32_t start = lower_bound_tile(keys, n, t);
+    const uint32_t end = lower_bound_tile(keys, n, (uint64_t)t + 1);
+    ranges[2 * t] = start < end ? start : 0u;
+    ranges[2 * t + 1] = start < end ? end : 0u;
+}
+
+// SplatCounters words (splat_frame_counters.slang): [0] slots, [1] visible,
+// [2] pairs (low), [3] pairs (high), [4] most tiles of one slot, [8 + r] culls
+// of reason r.
+__global__ void __launch_bounds__(256) counters_kernel(const uint32_t* __restrict__ depth_keys,
+                                                       const uint32_t* __restrict__ touched, uint64_t slots,
+                                                       uint32_t* __restrict__ words) {
+    __shared__ uint32_t local[16];
+    if (threadIdx.x < 16) {
+        local[threadIdx.x] = 0;
+    }
+    __syncthreads();
+    uint32_t vis = 0, most = 0;
+    uint64_t pairs = 0;
+    for (uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; s < slots; s += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t t = touched[s];
+        if (t != 0) {
+            vis += 1;
+            pairs += t;
+            most = t > most ? t : most;
+        } else {
+            const uint32_t k = depth_keys[s];
+            const uint32_t reason = (k & 0xFF000000u) == kCulledKey ? (k & 0xFFu) : 0u;
+            atomicAdd(&local[8 + (reason < ATHX_CULL_REASONS ? reason : 0u)], 1u);
+        }
+    }
+    atomicAdd(&local[1], vis);
+    atomicAdd(&local[2], (uint32_t)pairs);
+    atomicAdd(&local[3], (uint32_t)(pairs >> 32));
+    atomicMax(&local[4], most);
+    __syncthreads();
+    if (threadIdx.x < 16 && threadIdx.x != 0 && threadIdx.x != 4) {
+        if (local[threadIdx.x] != 0) {
+            atomicAdd(&words[threadIdx.x], local[threadIdx.x]);
+        }
+    } else if (threadIdx.x == 4) {
+        atomicMax(&words[4], local[4]);
+    }
+}
+
+}  // namespace
+
+hipError_t launch_project(const ProjectArgs& a, const DView* views_dev, uint32_t nviews, float* records,
+                          uint32_t* touched, uint32_t* depth_keys, hipStream_t s) {
+    if (a.n == 0 || nviews == 0) {
+        return hipSuccess;
+    }
+    const dim3 grid(div_up(a.n, 256));
+    if (a.sh_half) {
+        hipLaunchKernelGGL(project_kernel<true>, grid, dim3(256), 0, s, a, views_dev, nviews, records, touched,
+                           depth_keys);
+    } else {
+        hipLaunchKernelGGL(project_kernel<false>, grid, dim3(256), 0, s, a, views_dev, nviews, records, touched,
+                           depth_keys);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_u32(uint32_t* p, uint32_t value, uint64_t n, hipStream_t s) {
+    if (n == 0) {
+        return hipSuccess;
+    }
+    uint64_t blocks = (n + 255) / 256;
+    if (blocks > 4096) {
+        blocks = 4096;
+    }
+    hipLaunchKernelGGL(fill_u32_kernel, dim3((uint32_t)blocks), dim3(256), 0, s, p, value, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_emit(const DView* views_dev, uint32_t nviews, uint32_t n, uint32_t tile, const float* records,
+                       const uint32_t* touched, const uint32_t* depth_keys, const uint32_t* offsets, uint64_t* keys,
+                       uint32_t* values, hipStream_t s) {
+    const uint64_t slots = (uint64_t)n * nviews;
+    if (slots == 0) {
+        return hipSuccess;
+    }
+    hipLaunchKernelGGL(emit_kernel, dim3((uint32_t)((slots + 255) / 256)), dim3(256), 0, s, views_dev, nviews, n,
+                       tile, records, touched, depth_keys, offsets, keys, values);
+    return hipGetLastError();
+}
+
+hipError_t launch_ranges(const uint64_t* sorted_keys, uint32_t pairs, uint32_t tiles, uint32_t* ranges,
+                         hipStream_t s) {
+    if (tiles == 0) {
+        return hipSuccess;
+    }
+    hipLaunchKernelGGL(ranges_kernel, dim3(div_up(tiles, 256)), dim3(256), 0, s, sorted_keys, pairs, tiles, ranges);
+    return hipGetLastError();
+}
+
+hipError_t launch_counters(const uint32_t* depth_keys, const uint32_t* touched, uint64_t slots, uint32_t* words,
+                           hipStream_t s) {
+    if (slots == 0) {
+        return hipSuccess;
+    }
+    uint64_t blocks = (slots + 255) / 256;
+    if (blocks > 1024) {
+        blocks = 1024;
+    }
+    hipLaunchKernelGGL(counters_kernel, dim3((uint32_t)blocks), dim3(256), 0, s, depth_keys, touched, slots, words);
+    return hipGetLastError();
+}
+
+}  // namespace athx

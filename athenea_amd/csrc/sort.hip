@@ -1,0 +1,260 @@
+// sort.hip -- stable LSD radix sort of (uint64 key, uint32 value) pairs for
+// gfx950, onesweep (Adinets & Merrill 2022): one kernel counts every digit
+// pass's histogram in a single read of the keys, then one kernel a pass
+// ranks each 4096-key partition in LDS with wavefront ballots (a 64-lane
+// match on the 8 digit bits), finds where its digits start among the
+// partitions before it by decoupled look-back, and writes each digit's run
+// out contiguously from LDS.
+//
+// Semantics: gpu::RadixSort::sort (modules/gpu/include/athenea/gpu/algo/
+// RadixSort.h:63-64; modules/gpu/src/algo/RadixSort.cpp:95-261), a stable
+// LSD sort on the low key_bits bits, 8 bits a pass. The reference splits a
+// pass into histogram / scan / scatter dispatches over tiles of 1024
+// (shaders/athenea/algo/radix_block.slang:79-176).
+#include "athx_internal.h"
+
+namespace athx {
+namespace {
+
+constexpr uint32_t kThreads = 256;
+constexpr uint32_t kWaves = kThreads / 64;
+constexpr uint32_t kItems = 16;
+constexpr uint32_t kPart = kThreads * kItems;
+constexpr uint32_t kRadix = 256;
+constexpr uint32_t kMaxPasses = 8;
+constexpr uint32_t kFlagAggregate = 1u << 30;
+constexpr uint32_t kFlagPrefix = 2u << 30;
+constexpr uint32_t kValueMask = (1u << 30) - 1;
+
+__device__ inline uint32_t load_word(const uint32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void store_word(uint32_t* p, uint32_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Exclusive scan of one uint32 a thread over the 256 threads of the block.
+__device__ inline uint32_t block_exclusive_scan(uint32_t x, uint32_t* s_tmp, uint32_t& total) {
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    uint32_t incl = x;
+#pragma unroll
+    for (uint32_t d = 1; d < 64; d <<= 1) {
+        const uint32_t y = __shfl_up(incl, d, 64);
+        if (lane >= d) {
+            incl += y;
+        }
+    }
+    if (lane == 63) {
+        s_tmp[wave] = incl;
+    }
+    __syncthreads();
+    uint32_t before = 0, all = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < kWaves; ++w) {
+        const uint32_t t = s_tmp[w];
+        before += w < wave ? t : 0u;
+        all += t;
+    }
+    __syncthreads();
+    total = all;
+    return before + incl - x;
+}
+
+__global__ void __launch_bounds__(kThreads) histogram_kernel(const uint64_t* __restrict__ keys, uint32_t n,
+                                                             uint32_t passes, uint32_t* __restrict__ hist) {
+    __shared__ uint32_t s_hist[kMaxPasses * kRadix];
+    for (uint32_t i = threadIdx.x; i < passes * kRadix; i += kThreads) {
+        s_hist[i] = 0;
+    }
+    __syncthreads();
+    for (uint32_t i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads) {
+        const uint64_t k = keys[i];
+        for (uint32_t p = 0; p < passes; ++p) {
+            atomicAdd(&s_hist[p * kRadix + (uint32_t)((k >> (8 * p)) & 0xFFu)], 1u);
+        }
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < passes * kRadix; i += kThreads) {
+        if (s_hist[i] != 0) {
+            atomicAdd(&hist[i], s_hist[i]);
+        }
+    }
+}
+
+__global__ void __launch_bounds__(kThreads) onesweep_kernel(const uint64_t* __restrict__ kin,
+                                                            const uint32_t* __restrict__ vin,
+                                                            uint64_t* __restrict__ kout, uint32_t* __restrict__ vout,
+                                                            uint32_t n, uint32_t shift,
+                                                            const uint32_t* __restrict__ hist,
+                                                            uint32_t* __restrict__ lookback,
+                                                            uint32_t* __restrict__ counter) {
+    __shared__ uint64_t s_keys[kPart];
+    __shared__ uint32_t s_vals[kPart];
+    __shared__ uint32_t s_wave_hist[kWaves * kRadix];
+    __shared__ uint32_t s_block_start[kRadix];
+    __shared__ uint32_t s_digit_base[kRadix];
+    __shared__ uint32_t s_tmp[kWaves];
+    __shared__ uint32_t s_part;
+
+    const uint32_t tid = threadIdx.x;
+    const uint32_t lane = tid & 63u;
+    const uint32_t wave = tid >> 6;
+    if (tid == 0) {
+        s_part = atomicAdd(counter, 1u);
+    }
+    for (uint32_t i = tid; i < kWaves * kRadix; i += kThreads) {
+        s_wave_hist[i] = 0;
+    }
+    __syncthreads();
+    const uint32_t part = s_part;
+    const uint32_t part_first = part * kPart;
+    const uint32_t wave_first = part_first + wave * (64 * kItems);
+
+    // Load: item j of a lane sits at wave_first + j * 64 + lane, so the order
+    // of the input is (wave, j, lane) and ranking j by j keeps it stable.
+    uint64_t key[kItems];
+    uint32_t val[kItems];
+    uint32_t rank[kItems];
+#pragma unroll
+    for (uint32_t j = 0; j < kItems; ++j) {
+        const uint32_t at = wave_first + j * 64 + lane;
+        const bool ok = at < n;
+        key[j] = ok ? kin[at] : 0;
+        val[j] = ok ? vin[at] : 0;
+    }
+    const uint64_t lt_mask = (1ull << lane) - 1ull;
+    uint32_t* my_hist = s_wave_hist + wave * kRadix;
+#pragma unroll
+    for (uint32_t j = 0; j < kItems; ++j) {
+        const uint32_t at = wave_first + j * 64 + lane;
+        const bool ok = at < n;
+        const uint32_t d = (uint32_t)(key[j] >> shift) & 0xFFu;
+        uint64_t m = __ballot(ok);
+#pragma unroll
+        for (uint32_t b = 0; b < 8; ++b) {
+            const bool bit = (d >> b) & 1u;
+            const uint64_t on = __ballot(bit);
+            m &= bit ? on : ~on;
+        }
+        const uint32_t before = ok ? my_hist[d] : 0u;
+        __builtin_amdgcn_wave_barrier();
+        rank[j] = before + (uint32_t)__popcll(m & lt_mask);
+        const bool leader = ok && (m & lt_mask) == 0;
+        if (leader) {
+            my_hist[d] = before + (uint32_t)__popcll(m);
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+    __syncthreads();
+
+    // Thread d owns digit d: offsets of the waves, this partition's count,
+    // where the digit starts in the block and in the output.
+    const uint32_t d = tid;
+    uint32_t count = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < kWaves; ++w) {
+        const uint32_t c = s_wave_hist[w * kRadix + d];
+        s_wave_hist[w * kRadix + d] = count;
+        count += c;
+    }
+    uint32_t* my_word = lookback + (size_t)part * kRadix + d;
+    store_word(my_word, (part == 0 ? kFlagPrefix : kFlagAggregate) | count);
+    uint32_t unused;
+    s_block_start[d] = block_exclusive_scan(count, s_tmp, unused);
+    const uint32_t global_start = block_exclusive_scan(hist[d], s_tmp, unused);
+    uint32_t before = 0;
+    if (part != 0) {
+        int32_t j = (int32_t)part - 1;
+        while (true) {
+            uint32_t w = load_word(lookback + (size_t)j * kRadix + d);
+            // Partition j took its ticket before this one: it publishes.
+            while ((w >> 30) == 0) {
+                __builtin_amdgcn_s_sleep(1);
+                w = load_word(lookback + (size_t)j * kRadix + d);
+            }
+            before += w & kValueMask;
+            if ((w >> 30) == 2u || j == 0) {
+                break;
+            }
+            --j;
+        }
+        store_word(my_word, kFlagPrefix | ((before + count) & kValueMask));
+    }
+    s_digit_base[d] = global_start + before;
+    __syncthreads();
+
+    // Into LDS in digit order, then out: each digit's run is contiguous in
+    // the output, so neighbouring threads write neighbouring addresses.
+#pragma unroll
+    for (uint32_t j = 0; j < kItems; ++j) {
+        const uint32_t at = wave_first + j * 64 + lane;
+        if (at < n) {
+            const uint32_t dj = (uint32_t)(key[j] >> shift) & 0xFFu;
+            const uint32_t pos = s_block_start[dj] + s_wave_hist[wave * kRadix + dj] + rank[j];
+            s_keys[pos] = key[j];
+            s_vals[pos] = val[j];
+        }
+    }
+    __syncthreads();
+    const uint32_t valid = n - part_first < kPart ? n - part_first : kPart;
+    for (uint32_t i = tid; i < valid; i += kThreads) {
+        const uint64_t k = s_keys[i];
+        const uint32_t dk = (uint32_t)(k >> shift) & 0xFFu;
+        const uint32_t dst = s_digit_base[dk] + (i - s_block_start[dk]);
+        kout[dst] = k;
+        vout[dst] = s_vals[i];
+    }
+}
+
+__global__ void zero_kernel(uint32_t* p, uint64_t n) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        p[i] = 0;
+    }
+}
+
+}  // namespace
+
+uint32_t sort_partitions(uint32_t n) { return div_up(n ? n : 1, kPart); }
+
+// Sorts (keys, values) by the low key_bits bits. The result is back in keys
+// and values: an odd number of passes ends with a copy.
+hipError_t launch_sort(uint64_t* keys, uint32_t* values, uint32_t n, uint32_t key_bits, const SortScratch& sc,
+                       hipStream_t s) {
+    if (n < 2 || key_bits == 0) {
+        return hipSuccess;
+    }
+    if (key_bits > 64 || n > kValueMask) {
+        return hipErrorInvalidValue;
+    }
+    const uint32_t passes = div_up(key_bits, 8);
+    const uint32_t parts = sort_partitions(n);
+    const uint64_t clear = (uint64_t)passes * kRadix + (uint64_t)passes * parts * kRadix + passes;
+    // hist, lookback and counters are laid out back to back by the caller.
+    uint64_t blocks = (clear + 255) / 256;
+    hipLaunchKernelGGL(zero_kernel, dim3((uint32_t)(blocks < 4096 ? blocks : 4096)), dim3(256), 0, s, sc.hist, clear);
+    uint32_t hist_blocks = parts < 2048 ? parts : 2048;
+    hipLaunchKernelGGL(histogram_kernel, dim3(hist_blocks), dim3(kThreads), 0, s, keys, n, passes, sc.hist);
+    uint64_t* kin = keys;
+    uint32_t* vin = values;
+    uint64_t* kout = sc.keys_alt;
+    uint32_t* vout = sc.values_alt;
+    for (uint32_t p = 0; p < passes; ++p) {
+        hipLaunchKernelGGL(onesweep_kernel, dim3(parts), dim3(kThreads), 0, s, kin, vin, kout, vout, n, 8 * p,
+                           sc.hist + p * kRadix, sc.lookback + (size_t)p * parts * kRadix, sc.counters + p);
+        uint64_t* tk = kin; kin = kout; kout = tk;
+        uint32_t* tv = vin; vin = vout; vout = tv;
+    }
+    if (kin != keys) {
+        hipError_t e = hipMemcpyAsync(keys, kin, (size_t)n * sizeof(uint64_t), hipMemcpyDeviceToDevice, s);
+        if (e != hipSuccess) {
+            return e;
+        }
+        e = hipMemcpyAsync(values, vin, (size_t)n * sizeof(uint32_t), hipMemcpyDeviceToDevice, s);
+        if (e != hipSuccess) {
+            return e;
+        }
+    }
+    return hipGetLastError();
+}
+
+}  // namespace athx

@@ -1,0 +1,334 @@
+"""The host-side mirror of athenea's splat-render API over libathx.so.
+
+``TileRasterizer`` keeps the names and meaning of render::TileRasterizer
+(modules/render/include/athenea/render/TileRasterizer.h:452-487): ``create``,
+``render``, ``latest_counters``, ``release_scratch``; ``RenderSettings``,
+``RenderTargets``, ``FrameStats`` and ``SplatCounters`` keep the fields a plain
+frame reads (:248-450). Errors are raised as ``AthxError`` carrying the
+reference's ErrorCode name. Device memory comes from PyTorch (ROCm); all the
+work is the HIP library's, on the caller's current stream.
+
+``exclusive_scan`` and ``sort_pairs`` are gpu::PrefixSum::apply and
+gpu::RadixSort::sort (modules/gpu/include/athenea/gpu/algo/PrefixSum.h:43-44,
+RadixSort.h:63-64).
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass, field
+
+import torch
+
+from . import _lib
+
+
+@dataclass
+class RenderSettings:
+    tile: int = 16
+    batch: int = 256
+    antialias: bool = True
+    max_sh_degree: int = 3
+    depth: str = "mean"  # "mean" or "threshold"
+    depth_threshold: float = 0.5
+    time_stages: bool = False
+    count_splats: bool = False
+
+    def to_c(self, linear: bool) -> _lib.Settings:
+        s = _lib.Settings()
+        s.tile = self.tile
+        s.batch = self.batch
+        s.antialias = 1 if self.antialias else 0
+        s.linear = 1 if linear else 0
+        s.max_sh_degree = self.max_sh_degree
+        s.depth_mode = 0 if self.depth == "mean" else 1
+        s.depth_threshold = self.depth_threshold
+        s.time_stages = 1 if self.time_stages else 0
+        s.count_splats = 1 if self.count_splats else 0
+        return s
+
+
+@dataclass
+class GaussianCloud:
+    """Device tensors in athx_gaussians' layout. ``linear``: colours are linear
+    light (GpuSplats::linear); otherwise sRGB, decoded per splat."""
+
+    means: torch.Tensor
+    log_scales: torch.Tensor
+    quats: torch.Tensor
+    opacity_logits: torch.Tensor
+    sh: torch.Tensor  # [n, k, 3], float32 or float16
+    linear: bool = True
+
+    @property
+    def count(self) -> int:
+        return int(self.means.shape[0])
+
+    @staticmethod
+    def from_numpy(cloud, device="cuda", linear: bool = True) -> "GaussianCloud":
+        def t(a):
+            return torch.from_numpy(a).to(device).contiguous()
+
+        return GaussianCloud(t(cloud.means), t(cloud.log_scales), t(cloud.quats), t(cloud.opacity_logits),
+                             t(cloud.sh), linear)
+
+
+@dataclass
+class RenderTargets:
+    """One view's images, bottom row first (TileRasterizer.h:421-450)."""
+
+    width: int
+    height: int
+    colour: torch.Tensor  # [h, w, 4] premultiplied linear
+    depth: torch.Tensor  # [h, w]
+    final_t: torch.Tensor | None = None  # [h, w]
+    last: torch.Tensor | None = None  # [h, w] int32
+
+    @staticmethod
+    def allocate(width: int, height: int, device, keep_backward: bool) -> "RenderTargets":
+        colour = torch.empty((height, width, 4), dtype=torch.float32, device=device)
+        depth = torch.empty((height, width), dtype=torch.float32, device=device)
+        ft = torch.empty((height, width), dtype=torch.float32, device=device) if keep_backward else None
+        last = torch.empty((height, width), dtype=torch.int32, device=device) if keep_backward else None
+        return RenderTargets(width, height, colour, depth, ft, last)
+
+
+@dataclass
+class FrameStats:
+    splats: int = 0
+    visible: int = 0
+    pairs: int = 0
+    project_ms: float = 0.0
+    scan_ms: float = 0.0
+    emit_ms: float = 0.0
+    sort_ms: float = 0.0
+    ranges_ms: float = 0.0
+    blend_ms: float = 0.0
+    total_ms: float = 0.0
+
+
+CULL_NAMES = ["unprojected", "edit", "depth", "degenerate", "faint", "offscreen", "no_tile"]
+
+
+@dataclass
+class SplatCounters:
+    slots: int = 0
+    visible: int = 0
+    pairs: int = 0
+    max_tiles: int = 0
+    culled: dict = field(default_factory=dict)
+
+
+def _ptr(t: torch.Tensor | None) -> int | None:
+    if t is None:
+        return None
+    if not t.is_contiguous():
+        raise ValueError("tensors handed to the rasterizer must be contiguous")
+    return t.data_ptr()
+
+
+def _stream_handle(device) -> int:
+    return torch.cuda.current_stream(device).cuda_stream
+
+
+def _check_cloud(cloud: GaussianCloud) -> None:
+    n = cloud.count
+    shapes = {
+        "means": (cloud.means, (n, 3)),
+        "log_scales": (cloud.log_scales, (n, 3)),
+        "quats": (cloud.quats, (n, 4)),
+        "opacity_logits": (cloud.opacity_logits, (n,)),
+    }
+    for name, (t, shape) in shapes.items():
+        if tuple(t.shape) != shape or t.dtype != torch.float32 or not t.is_cuda:
+            raise ValueError(f"{name} must be a float32 device tensor of shape {shape}, got {tuple(t.shape)} {t.dtype}")
+    if cloud.sh.dim() != 3 or cloud.sh.shape[0] != n or cloud.sh.shape[2] != 3 or cloud.sh.shape[1] not in (1, 4, 9, 16):
+        raise ValueError(f"sh must be [n, 1|4|9|16, 3], got {tuple(cloud.sh.shape)}")
+    if cloud.sh.dtype not in (torch.float32, torch.float16):
+        raise ValueError("sh must be float32 or float16")
+
+
+def _gaussians_c(cloud: GaussianCloud) -> _lib.Gaussians:
+    _check_cloud(cloud)
+    g = _lib.Gaussians()
+    g.count = cloud.count
+    g.sh_coeffs = int(cloud.sh.shape[1])
+    g.means = _ptr(cloud.means)
+    g.log_scales = _ptr(cloud.log_scales)
+    g.quats = _ptr(cloud.quats)
+    g.opacity_logits = _ptr(cloud.opacity_logits)
+    if cloud.sh.dtype == torch.float16:
+        g.sh = None
+        g.sh_half = _ptr(cloud.sh)
+    else:
+        g.sh = _ptr(cloud.sh)
+        g.sh_half = None
+    return g
+
+
+class TileRasterizer:
+    """One context of libathx (one device, one host thread at a time)."""
+
+    def __init__(self, device: int = 0):
+        self._lib = _lib.load()
+        self.device = device
+        handle = ctypes.c_void_p()
+        _lib.check(self._lib.athx_create(device, ctypes.byref(handle)))
+        self._ctx = handle
+
+    @staticmethod
+    def create(device: int = 0) -> "TileRasterizer":
+        return TileRasterizer(device)
+
+    def close(self) -> None:
+        if getattr(self, "_ctx", None):
+            self._lib.athx_destroy(self._ctx)
+            self._ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, status: int) -> None:
+        _lib.check(status, self._ctx)
+
+    def render(
+        self,
+        views: list,
+        cloud: GaussianCloud,
+        settings: RenderSettings | None = None,
+        targets: list[RenderTargets] | None = None,
+        keep_backward: bool = False,
+    ) -> tuple[list[RenderTargets], FrameStats]:
+        """Draws ``cloud`` once per view (a list of athx View structs); all
+        views go through one projection, one sort and one blend launch."""
+        settings = settings or RenderSettings()
+        dev = cloud.means.device
+        if targets is None:
+            targets = [RenderTargets.allocate(v.width, v.height, dev, keep_backward) for v in views]
+        if len(targets) != len(views):
+            raise ValueError("one target per view")
+        nv = len(views)
+        c_views = (_lib.View * nv)(*views)
+        c_targets = (_lib.Targets * nv)()
+        for k, t in enumerate(targets):
+            if t.colour.shape != (views[k].height, views[k].width, 4):
+                raise ValueError(f"target {k} does not match its view's size")
+            c_targets[k].colour = _ptr(t.colour)
+            c_targets[k].depth = _ptr(t.depth)
+            c_targets[k].final_t = _ptr(t.final_t)
+            c_targets[k].last = _ptr(t.last)
+        g = _gaussians_c(cloud)
+        s = settings.to_c(cloud.linear)
+        stats = _lib.FrameStats()
+        self._check(
+            self._lib.athx_render(self._ctx, c_views, nv, ctypes.byref(g), ctypes.byref(s), c_targets,
+                                  _stream_handle(dev), ctypes.byref(stats))
+        )
+        out = FrameStats(stats.splats, stats.visible, stats.pairs, stats.project_ms, stats.scan_ms, stats.emit_ms,
+                         stats.sort_ms, stats.ranges_ms, stats.blend_ms, stats.total_ms)
+        return targets, out
+
+    def render_backward(
+        self,
+        views: list,
+        cloud: GaussianCloud,
+        settings: RenderSettings | None,
+        targets: list[RenderTargets],
+        dl_dcolour: list[torch.Tensor],
+        grads: dict[str, torch.Tensor],
+    ) -> None:
+        """Accumulates dL/d(parameter) into ``grads`` (keys: means, log_scales,
+        quats, opacity_logits, sh) for the frame the last ``render`` drew."""
+        settings = settings or RenderSettings()
+        nv = len(views)
+        c_views = (_lib.View * nv)(*views)
+        c_targets = (_lib.Targets * nv)()
+        keep = []
+        dl = (ctypes.c_void_p * nv)()
+        for k, t in enumerate(targets):
+            c_targets[k].colour = _ptr(t.colour)
+            c_targets[k].depth = _ptr(t.depth)
+            c_targets[k].final_t = _ptr(t.final_t)
+            c_targets[k].last = _ptr(t.last)
+            d = dl_dcolour[k].to(torch.float32).contiguous()
+            if d.shape != t.colour.shape:
+                raise ValueError("dL/dcolour must match the colour target")
+            keep.append(d)
+            dl[k] = d.data_ptr()
+        cg = _lib.Grads()
+        for name in ("means", "log_scales", "quats", "opacity_logits", "sh"):
+            t = grads.get(name)
+            if t is not None and (t.dtype != torch.float32 or not t.is_contiguous()):
+                raise ValueError(f"gradient {name} must be a contiguous float32 tensor")
+            setattr(cg, name, _ptr(t))
+        g = _gaussians_c(cloud)
+        s = settings.to_c(cloud.linear)
+        self._check(
+            self._lib.athx_render_backward(self._ctx, c_views, nv, ctypes.byref(g), ctypes.byref(s), c_targets, dl,
+                                           ctypes.byref(cg), _stream_handle(cloud.means.device))
+        )
+        # ``keep`` may hold temporaries the queued kernels read; PyTorch's
+        # allocator is stream-ordered, so releasing them here is safe.
+        del keep
+
+    def latest_counters(self) -> SplatCounters:
+        c = _lib.Counters()
+        self._check(self._lib.athx_latest_counters(self._ctx, ctypes.byref(c)))
+        return SplatCounters(c.slots, c.visible, c.pairs, c.max_tiles,
+                             {CULL_NAMES[r]: int(c.culled[r]) for r in range(_lib.CULL_REASONS)})
+
+    def frame_info(self) -> dict:
+        info = _lib.FrameInfo()
+        self._check(self._lib.athx_frame_info_get(self._ctx, ctypes.byref(info)))
+        return {"slots": info.slots, "tiles": info.tiles, "key_bits": info.key_bits, "pairs": info.pairs}
+
+    def frame_buffers(self, device=None) -> dict[str, torch.Tensor]:
+        """Copies of the last frame's per-slot and per-pair buffers (athx_copy_frame):
+        records [slots, 12] f32, touched / depth_keys / offsets [slots] (uint32 bits
+        in int32), pair_keys [pairs] (uint64 bits in int64), pair_values [pairs],
+        ranges [tiles, 2]."""
+        info = self.frame_info()
+        device = device or torch.device("cuda", self.device)
+        shapes = {
+            "records": ((info["slots"], 12), torch.float32),
+            "touched": ((info["slots"],), torch.int32),
+            "depth_keys": ((info["slots"],), torch.int32),
+            "offsets": ((info["slots"],), torch.int32),
+            "pair_keys": ((info["pairs"],), torch.int64),
+            "pair_values": ((info["pairs"],), torch.int32),
+            "ranges": ((info["tiles"], 2), torch.int32),
+        }
+        out = {}
+        for name, (shape, dtype) in shapes.items():
+            t = torch.empty(shape, dtype=dtype, device=device)
+            nbytes = t.numel() * t.element_size()
+            self._check(self._lib.athx_copy_frame(self._ctx, _lib.FRAME_PARTS[name], t.data_ptr() if nbytes else None,
+                                                  nbytes, _stream_handle(device)))
+            out[name] = t
+        return out
+
+    def release_scratch(self) -> None:
+        self._check(self._lib.athx_release_scratch(self._ctx))
+
+    def scratch_bytes(self) -> int:
+        return int(self._lib.athx_scratch_bytes(self._ctx))
+
+    def exclusive_scan(self, values: torch.Tensor, out: torch.Tensor | None = None) -> tuple[torch.Tensor, torch.Tensor]:
+        """PrefixSum::apply: out[i] = sum(values[:i]) over uint32 (int32 storage), and the total (int64)."""
+        if values.dtype != torch.int32 or not values.is_cuda:
+            raise ValueError("values must be an int32 device tensor holding uint32 bits")
+        out = torch.empty_like(values) if out is None else out
+        total = torch.zeros(1, dtype=torch.int64, device=values.device)
+        self._check(self._lib.athx_exclusive_scan(self._ctx, _ptr(values), _ptr(out), _ptr(total), values.numel(),
+                                                  _stream_handle(values.device)))
+        return out, total
+
+    def sort_pairs(self, keys: torch.Tensor, values: torch.Tensor, key_bits: int) -> None:
+        """RadixSort::sort in place: a stable sort by the low key_bits bits of
+        int64 keys (uint64 bits), values (int32) carried along."""
+        if keys.dtype != torch.int64 or values.dtype != torch.int32 or keys.numel() != values.numel():
+            raise ValueError("keys must be int64 and values int32, of one length")
+        self._check(self._lib.athx_sort_pairs(self._ctx, _ptr(keys), _ptr(values), keys.numel(), key_bits,
+                                              _stream_handle(keys.device)))

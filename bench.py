@@ -1,0 +1,254 @@
+"""Benchmark of the MI355X splat tile rasterizer.
+
+A step is one forward frame of the 1080p workload (1.4M Gaussians, SH degree
+3, FP32, 1920x1080, 16x16 tiles): projection, scan, pair emission, the
+onesweep sort, tile ranges and the blend, with the Gaussians already resident
+in HBM. Prints ONE JSON line:
+
+  value         forward ms/frame (lower is better)
+  fwd_bwd_ms    forward + adjoint backward ms/frame (loss L2 against a target
+                rendered from jittered means and SH)
+  batched       the 60-view walk (2.36M Gaussians, 960x540 views) in one
+                launch: views/s, and the per-view time of the same views
+                rendered one launch each
+  stages_ms     per-stage times from HIP events on the rasterizer's stream
+  roofline      the dominant stage's algorithmic bytes / its time vs 8 TB/s
+  cpu_baseline  the oracle (the FP32 C restatement of the reference's
+                algorithm) on this host's cores, one frame of the same workload
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK = 8.0e12
+METRIC = "ms/frame fwd & fwd+bwd, 1.4M splats@1080p; batched views/s; % HBM peak"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def stage_bytes(n: int, pairs: int, tiles: int, pixels: int, key_bits: int) -> dict:
+    """Compulsory HBM bytes per stage (one view): what each stage must read
+    and write whatever its implementation."""
+    passes = (key_bits + 7) // 8
+    return {
+        # FP32 parameters read (236 B at SH degree 3), record, tile count and key written
+        "project": n * (236 + 48 + 4 + 4),
+        # tile counts read, offsets written
+        "scan": n * 8,
+        # offsets, counts, keys and records read; (key, value) pairs written
+        "emit": n * (4 + 4 + 4 + 48) + pairs * 12,
+        # one read of the keys for the histograms, then a read and a write of
+        # (key, value) a pass
+        "sort": pairs * 8 + passes * pairs * 24,
+        "ranges": tiles * 8,
+        # each pair's value and record read once; RGBA, depth, final T, last index written
+        "blend": pairs * (4 + 48) + tiles * 8 + pixels * 28,
+    }
+
+
+def cpu_baseline(cloud_np, view, tile: int) -> dict:
+    from oracle import pyoracle as po
+
+    po.load()
+    threads = po.threads()
+    ov = po.orc_view(view, tile=tile)
+    times = []
+    po.render_ref_order(ov, cloud_np)  # warm-up
+    for _ in range(3):
+        t0 = time.perf_counter()
+        po.render_ref_order(ov, cloud_np)
+        times.append((time.perf_counter() - t0) * 1e3)
+    return {
+        "value": round(statistics.median(times), 2),
+        "unit": "ms/frame",
+        "cores": threads,
+        "kind": "port",
+        "sample": "one full forward frame of the same 1.4M-Gaussian 1920x1080 workload (project, 24-bit depth "
+                  "sort, emit, stable tile sort, ranges, blend), median of 3 after one warm-up, OpenMP over "
+                  f"{threads} host threads",
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--tile", type=int, default=16)
+    ap.add_argument("--batch", type=int, default=256)
+    ap.add_argument("--no-extras", action="store_true", help="forward only: no backward, batch or CPU legs")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: the rasterizer shards by views or screen tiles across GPUs, "
+              "which this build does not run; use --gpus 1", file=sys.stderr)
+        return 2
+
+    import numpy as np
+    import torch
+
+    from athenea_amd import RenderSettings, TileRasterizer, camera, synthetic
+    from athenea_amd.raster import GaussianCloud
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    dev = torch.device("cuda:0")
+    log("generating the 1080p workload")
+    cloud_np = synthetic.plain_1080p_cloud()
+    W, H = 1920, 1080
+    view = camera.view_for(synthetic.front_camera(), W, H)
+    cloud = GaussianCloud.from_numpy(cloud_np, dev)
+    settings = RenderSettings(tile=args.tile, batch=args.batch)
+    raster = TileRasterizer(0)
+    targets = None
+    log("warm-up")
+    for _ in range(args.warmup):
+        targets, stats = raster.render([view], cloud, settings, targets)
+    torch.cuda.synchronize()
+
+    log(f"timing {args.steps} forward frames")
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        targets, stats = raster.render([view], cloud, settings, targets)
+    torch.cuda.synchronize()
+    fwd_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    pairs = int(stats.pairs)
+    info = raster.frame_info()
+
+    # Per-stage times from HIP events recorded on the rasterizer's stream.
+    staged = RenderSettings(tile=args.tile, batch=args.batch, time_stages=True, count_splats=True)
+    stage_samples = {k: [] for k in ("project", "scan", "emit", "sort", "ranges", "blend")}
+    for _ in range(max(3, min(args.steps, 10))):
+        targets, s = raster.render([view], cloud, staged, targets)
+        for k in stage_samples:
+            stage_samples[k].append(getattr(s, f"{k}_ms"))
+    stages = {k: statistics.mean(v) for k, v in stage_samples.items()}
+    counters = raster.latest_counters()
+    tiles = info["tiles"]
+    sbytes = stage_bytes(cloud.count, pairs, tiles, W * H, info["key_bits"])
+    dominant = max(stages, key=stages.get)
+    achieved = sbytes[dominant] / (stages[dominant] * 1e-3)
+    roofline = {
+        "kernel": dominant,
+        "bound": "hbm",
+        "achieved": round(achieved / 1e9, 1),
+        "peak": HBM_PEAK / 1e9,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK, 4),
+        "traffic": None,
+        "algorithmic_bytes": sbytes[dominant],
+    }
+    result = {
+        "metric": METRIC,
+        "value": round(fwd_ms, 3),
+        "unit": "ms/frame",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(fwd_ms, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic (seeded, no checkpoint)",
+        "config": {
+            "workload": "plain_1080p: 1.4M Gaussians, SH degree 3, FP32, 1920x1080, one view",
+            "gaussians": cloud.count,
+            "width": W,
+            "height": H,
+            "tile": args.tile,
+            "batch": args.batch,
+        },
+        "pairs": pairs,
+        "visible": counters.visible,
+        "stages_ms": {k: round(v, 4) for k, v in stages.items()},
+        "sort_keys_per_s": round(pairs / (stages["sort"] * 1e-3)),
+        "roofline": roofline,
+    }
+
+    if not args.no_extras:
+        from athenea_amd.autograd import rasterize
+
+        log("forward + backward")
+        rng = np.random.default_rng(2)
+        tgt = synthetic.plain_1080p_cloud()
+        tgt.means = (tgt.means + rng.normal(0, 0.01, tgt.means.shape)).astype(np.float32)
+        tgt.sh = (tgt.sh + rng.normal(0, 0.05, tgt.sh.shape)).astype(np.float32)
+        target_img = raster.render([view], GaussianCloud.from_numpy(tgt, dev), settings)[0][0].colour.clone()
+        del tgt
+        params = [torch.from_numpy(a).to(dev).requires_grad_(True) for a in
+                  (cloud_np.means, cloud_np.log_scales, cloud_np.quats, cloud_np.opacity_logits, cloud_np.sh)]
+
+        def step():
+            for p in params:
+                p.grad = None
+            colours, _ = rasterize(*params, [view], raster, settings)
+            loss = ((colours[0][..., :3] - target_img[..., :3]) ** 2).sum()
+            loss.backward()
+
+        for _ in range(args.warmup):
+            step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            step()
+        torch.cuda.synchronize()
+        result["fwd_bwd_ms"] = round((time.perf_counter() - t0) * 1e3 / args.steps, 3)
+        result["fwd_bwd_over_fwd"] = round(result["fwd_bwd_ms"] / fwd_ms, 3)
+        del params
+
+        log("batched 60-view walk")
+        walk_np = synthetic.walk_cloud()
+        walk = GaussianCloud.from_numpy(walk_np, dev)
+        views = [camera.view_for(c, 960, 540) for c in synthetic.walk_cameras(60)]
+        bt = None
+        for _ in range(2):
+            bt, bstats = raster.render(views, walk, settings, bt)
+        torch.cuda.synchronize()
+        reps = max(2, args.steps // 5)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            bt, bstats = raster.render(views, walk, settings, bt)
+        torch.cuda.synchronize()
+        batch_ms = (time.perf_counter() - t0) * 1e3 / reps
+        del bt
+        single = [None] * len(views)
+        for k, v in enumerate(views):
+            single[k], _ = raster.render([v], walk, settings, single[k])
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            for k, v in enumerate(views):
+                single[k], _ = raster.render([v], walk, settings, single[k])
+        torch.cuda.synchronize()
+        single_ms = (time.perf_counter() - t0) * 1e3 / reps
+        result["batched"] = {
+            "workload": "walk_60: 2.36M Gaussians, 60 views at 960x540 on a circular walk",
+            "views_per_s": round(60 / (batch_ms * 1e-3), 1),
+            "ms_per_view_batched": round(batch_ms / 60, 4),
+            "ms_per_view_single": round(single_ms / 60, 4),
+            "speedup": round(single_ms / batch_ms, 3),
+            "pairs": int(bstats.pairs),
+        }
+        raster.release_scratch()
+
+        if not args.no_cpu:
+            log("CPU baseline (the oracle on the host cores)")
+            result["cpu_baseline"] = cpu_baseline(cloud_np, view, args.tile)
+
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

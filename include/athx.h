@@ -1,0 +1,283 @@
+/*
+ * athx.h -- the C ABI of the MI355X Gaussian-splat tile rasterizer.
+ *
+ * Plain C: device pointers, sizes and a hipStream_t passed as void*. No
+ * exceptions cross it; every call returns an athx_status whose values are
+ * athenea's ErrorCode (modules/core/include/athenea/core/Error.h:14-26), and
+ * athx_last_error() returns the message of the last failure on a context.
+ *
+ * What each entry point replaces in the reference:
+ *
+ *   athx_create / athx_destroy
+ *       render::TileRasterizer::create (modules/render/include/athenea/render/
+ *       TileRasterizer.h:454) and its destructor.
+ *   athx_release_scratch
+ *       TileRasterizer::releaseScratch (TileRasterizer.h:487;
+ *       modules/render/src/TileRasterizer.cpp:178-191).
+ *   athx_render
+ *       TileRasterizer::render(const Projection&, span<const SplatInstance>,
+ *       const RenderSettings&, RenderTargets&) (TileRasterizer.h:465-477;
+ *       TileRasterizer.cpp:296-1295), for plain (not relit, no under layer,
+ *       no points, no matte/trace) clouds, generalised to several views in
+ *       one call: views[k] is a camera's Projection and writes targets[k].
+ *   athx_render_backward
+ *       no counterpart in the reference (its only gradient is the blend's
+ *       contributionMode 4, shaders/athenea/splat/splat_blend.slang:381-398):
+ *       the adjoint of athx_render's colour with respect to every parameter.
+ *   athx_latest_counters
+ *       TileRasterizer::latestCounters (TileRasterizer.h:481) and
+ *       SplatCounters (TileRasterizer.h:370-401).
+ *   athx_exclusive_scan
+ *       gpu::PrefixSum::apply (modules/gpu/include/athenea/gpu/algo/
+ *       PrefixSum.h:43-44).
+ *   athx_sort_pairs
+ *       gpu::RadixSort::sort (modules/gpu/include/athenea/gpu/algo/
+ *       RadixSort.h:63-64) for 64-bit keys: the reference's keysLo/keysHi
+ *       halves are one uint64 here.
+ *
+ * Threading: a context is used by one host thread at a time. Every call is
+ * asynchronous on the stream it is given, except where a comment says it
+ * waits. Callers own Gaussian and target memory; a context owns its scratch.
+ */
+#ifndef ATHX_H
+#define ATHX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#if defined(_WIN32)
+#define ATHX_API __declspec(dllexport)
+#else
+#define ATHX_API __attribute__((visibility("default")))
+#endif
+
+/* athenea's ErrorCode, value for value (Error.h:14-26), plus 0 for success. */
+typedef enum athx_status {
+    ATHX_OK = 0,
+    ATHX_UNKNOWN = 1,
+    ATHX_NOT_FOUND = 2,
+    ATHX_INVALID_ARGUMENT = 3,
+    ATHX_UNSUPPORTED = 4,
+    ATHX_IO_FAILURE = 5,
+    ATHX_DEVICE_FAILURE = 6,
+    ATHX_SHADER_FAILURE = 7,
+    ATHX_PLUGIN_FAILURE = 8,
+    ATHX_OUT_OF_MEMORY = 9,
+    ATHX_CANCELLED = 10,
+    ATHX_INTERNAL_ERROR = 11
+} athx_status;
+
+/* Why a splat was culled, in frame.slang's numbering
+ * (shaders/athenea/splat/frame.slang:42-49). */
+enum {
+    ATHX_CULL_UNPROJECTED = 0,
+    ATHX_CULL_EDIT = 1,
+    ATHX_CULL_DEPTH = 2,
+    ATHX_CULL_DEGENERATE = 3,
+    ATHX_CULL_FAINT = 4,
+    ATHX_CULL_OFFSCREEN = 5,
+    ATHX_CULL_NO_TILE = 6,
+    ATHX_CULL_REASONS = 7
+};
+
+typedef struct athx_ctx athx_ctx;
+
+/*
+ * One camera for one image size: render::Projection (modules/render/include/
+ * athenea/render/Camera.h:50-89) already cast to the floats the kernels read,
+ * as setFrame/setObject do (modules/render/src/FrameParams.h:32-95). The view
+ * looks down +z with x right and y up; the screen origin is the bottom-left
+ * corner and pixel centres sit at +0.5 (frame.slang:22-30).
+ */
+typedef struct athx_view {
+    uint32_t width, height;
+    float focal_x, focal_y;   /* pixels per unit of x/z (or per unit, orthographic) */
+    float centre_x, centre_y; /* the principal point, pixels */
+    float near_z, far_z;
+    uint32_t orthographic;
+    float object_to_view[12]; /* rows of a 3x4 */
+    float eye[3];             /* the camera in the cloud's own space */
+    float background[4];      /* premultiplied linear */
+} athx_view;
+
+/*
+ * The Gaussians, structure of arrays, FP32, device memory. This replaces the
+ * packed scene::GpuSplats (modules/scene/include/athenea/scene/GpuClouds.h:
+ * 44-218) with what its decode produces: log-scales rather than scales,
+ * opacity logits rather than opacities, the quaternion as stored (x, y, z, w)
+ * and normalised by the projection, and spherical-harmonic coefficients in
+ * 3DGS order (coefficient k of channel c at sh[(i * sh_coeffs + k) * 3 + c],
+ * k = 0 the constant term, colour = 0.5 + 0.2820948 * sh0 + rest).
+ * Exactly one of sh and sh_half is non-null; sh_half holds IEEE binary16.
+ */
+typedef struct athx_gaussians {
+    uint32_t count;
+    uint32_t sh_coeffs; /* 1, 4, 9 or 16 */
+    const float* means;          /* [count][3] */
+    const float* log_scales;     /* [count][3] */
+    const float* quats;          /* [count][4], (x, y, z, w) */
+    const float* opacity_logits; /* [count] */
+    const float* sh;             /* [count][sh_coeffs][3] or NULL */
+    const uint16_t* sh_half;     /* [count][sh_coeffs][3] or NULL */
+} athx_gaussians;
+
+/*
+ * RenderSettings (TileRasterizer.h:248-371), the fields a plain frame reads,
+ * plus the two launch shapes this rasterizer tunes.
+ */
+typedef struct athx_settings {
+    uint32_t tile;         /* tile side in pixels: 8, 16 or 32 (the reference's is 16) */
+    uint32_t batch;        /* Gaussians staged in LDS per blend batch: 32..256, a multiple of 32 */
+    uint32_t antialias;    /* 1: the Mip-Splatting compensation (RenderSettings::antialias) */
+    uint32_t linear;       /* 1: colours are linear light; 0: sRGB, decoded per splat */
+    uint32_t max_sh_degree;
+    uint32_t depth_mode;   /* 0 alpha-weighted mean, 1 first crossing of depth_threshold */
+    float depth_threshold;
+    uint32_t time_stages;  /* 1: wait after each stage and fill athx_frame_stats' times */
+    uint32_t count_splats; /* 1: count cull reasons for athx_latest_counters */
+} athx_settings;
+
+/*
+ * What a view writes: RenderTargets (TileRasterizer.h:421-450), caller-owned
+ * device memory, bottom row first. colour is float4 premultiplied linear,
+ * depth float view z. final_t (the transmittance left after the walk) and
+ * last (one past the last pair the walk read, relative to the tile's first)
+ * are what athx_render_backward replays; either may be NULL when no backward
+ * pass follows.
+ */
+typedef struct athx_targets {
+    float* colour;    /* [height][width][4] */
+    float* depth;     /* [height][width] or NULL */
+    float* final_t;   /* [height][width] or NULL */
+    uint32_t* last;   /* [height][width] or NULL */
+} athx_targets;
+
+/* FrameStats (TileRasterizer.h:408-419), summed over the views. */
+typedef struct athx_frame_stats {
+    uint32_t splats;
+    uint32_t visible;
+    uint64_t pairs;
+    double project_ms, scan_ms, emit_ms, sort_ms, ranges_ms, blend_ms, total_ms;
+} athx_frame_stats;
+
+/* SplatCounters (TileRasterizer.h:370-401) for the whole frame. */
+typedef struct athx_counters {
+    uint32_t slots;
+    uint32_t visible;
+    uint64_t pairs;
+    uint32_t max_tiles;
+    uint32_t culled[ATHX_CULL_REASONS];
+} athx_counters;
+
+/*
+ * Gradients of a scalar loss with respect to every Gaussian parameter, in the
+ * layouts of athx_gaussians (caller-owned device memory, FP32). They are
+ * ACCUMULATED into (+=), so the caller zeroes them first. sh is
+ * [count][sh_coeffs][3]. Any pointer may be NULL to skip that parameter.
+ */
+typedef struct athx_grads {
+    float* means;
+    float* log_scales;
+    float* quats;
+    float* opacity_logits;
+    float* sh;
+} athx_grads;
+
+ATHX_API const char* athx_version(void);
+
+/* Opens a context on HIP device `device`. */
+ATHX_API athx_status athx_create(int device, athx_ctx** out);
+ATHX_API void athx_destroy(athx_ctx* ctx);
+/* The message of the last failed call on ctx ("" if none). */
+ATHX_API const char* athx_last_error(const athx_ctx* ctx);
+/* Frees the scratch a frame grew into; later frames grow it to what they need exactly. */
+ATHX_API athx_status athx_release_scratch(athx_ctx* ctx);
+/* Bytes of device scratch the context holds now. */
+ATHX_API uint64_t athx_scratch_bytes(const athx_ctx* ctx);
+
+/*
+ * Renders `nviews` views of one cloud into targets[0..nviews). Waits for the
+ * stream once, after the projection and scan, to size the pair buffers (the
+ * reference's one readback, TileRasterizer.cpp:1093-1102); everything after is
+ * queued without waiting unless settings->time_stages. stats may be NULL.
+ * The frame's per-slot records stay in the context for athx_render_backward.
+ */
+ATHX_API athx_status athx_render(athx_ctx* ctx, const athx_view* views, uint32_t nviews,
+                                 const athx_gaussians* gaussians, const athx_settings* settings,
+                                 const athx_targets* targets, void* stream, athx_frame_stats* stats);
+
+/*
+ * The adjoint of the last athx_render on ctx (same views, Gaussians and
+ * settings; targets[k].final_t and .last must have been written by it):
+ * dL/dcolour is [height][width][4] per view (premultiplied RGBA, FP32), and
+ * gradients are accumulated into grads. Queued, not waited for.
+ */
+ATHX_API athx_status athx_render_backward(athx_ctx* ctx, const athx_view* views, uint32_t nviews,
+                                          const athx_gaussians* gaussians, const athx_settings* settings,
+                                          const athx_targets* targets, const float* const* dl_dcolour,
+                                          const athx_grads* grads, void* stream);
+
+/* The counters of the newest frame rendered with count_splats (waits for it). */
+ATHX_API athx_status athx_latest_counters(athx_ctx* ctx, athx_counters* out);
+
+/*
+ * The per-splat and per-pair buffers the last athx_render left in the
+ * context: the reference's proj / tilesTouched / depthKeys / pairSplats /
+ * ranges scratch (TileRasterizer.h:560-620; frame.slang:269-281), which its
+ * other readers (trace, counters, NeuralFit) walk. Slot s of view k is
+ * k * count + i. Layouts:
+ *   RECORDS      float[slots][12]: x, y, view z, cut power, conic a, b, c,
+ *                alpha, r, g, b, 0 (written for visible slots only)
+ *   TOUCHED      uint32[slots]: tiles the footprint touches, 0 when culled
+ *   DEPTH_KEYS   uint32[slots]: asuint(z) >> 7, or 0xFF000000 | cull reason
+ *   OFFSETS      uint32[slots]: exclusive scan of TOUCHED
+ *   PAIR_KEYS    uint64[pairs]: global tile << 24 | depth key, sorted
+ *   PAIR_VALUES  uint32[pairs]: the slot of each sorted pair
+ *   RANGES       uint32[tiles][2]: a tile's [start, end) in the sorted pairs
+ */
+typedef enum athx_frame_part {
+    ATHX_PART_RECORDS = 0,
+    ATHX_PART_TOUCHED = 1,
+    ATHX_PART_DEPTH_KEYS = 2,
+    ATHX_PART_OFFSETS = 3,
+    ATHX_PART_PAIR_KEYS = 4,
+    ATHX_PART_PAIR_VALUES = 5,
+    ATHX_PART_RANGES = 6
+} athx_frame_part;
+
+typedef struct athx_frame_info {
+    uint32_t slots;
+    uint32_t tiles;
+    uint32_t key_bits;
+    uint64_t pairs;
+} athx_frame_info;
+
+ATHX_API athx_status athx_frame_info_get(const athx_ctx* ctx, athx_frame_info* out);
+/* Copies `bytes` of one part (which must be its whole size) to device memory dst. Queued. */
+ATHX_API athx_status athx_copy_frame(athx_ctx* ctx, athx_frame_part part, void* dst, uint64_t bytes, void* stream);
+
+/*
+ * out[i] = in[0] + ... + in[i-1] over n uint32, and *total (device, one
+ * uint64) = the sum of all; single pass, decoupled look-back. in and out may
+ * not alias. Queued.
+ */
+ATHX_API athx_status athx_exclusive_scan(athx_ctx* ctx, const uint32_t* in, uint32_t* out, uint64_t* total,
+                                         uint32_t n, void* stream);
+
+/*
+ * Stable LSD radix sort of n (key, value) pairs by the low key_bits bits of
+ * the key (1..64), onesweep: one histogram pass, then one pass per 8-bit
+ * digit. The result is back in keys/values. Queued.
+ */
+ATHX_API athx_status athx_sort_pairs(athx_ctx* ctx, uint64_t* keys, uint32_t* values, uint32_t n,
+                                     uint32_t key_bits, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ATHX_H */

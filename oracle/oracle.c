@@ -1,0 +1,879 @@
+/*
+ * oracle.c -- TEST INFRASTRUCTURE. A CPU restatement of athenea's Gaussian
+ * splat tile rasterizer, used only as the checker of the HIP path (by tests/,
+ * the smoke check and the CPU-baseline leg of bench.py). Nothing in the
+ * product links or calls it.
+ *
+ * What it restates, each function citing the reference it follows:
+ *   orc_project      shaders/athenea/splat/splat_project.slang:497-1130
+ *                    (projectSplat<0>: no edit, no lens, no shutter, no
+ *                    surfels, not relit), helpers frame.slang:283-402,
+ *                    sh.slang:25-51, color.slang:7-41
+ *   orc_emit_pairs   one (tile<<24 | depth24, slot) pair per touched tile, in
+ *                    slot order (splat_emit.slang:17-50's tile test)
+ *   orc_sort_pairs   stable LSD radix sort over the key's significant bits
+ *                    (RadixSort.cpp:95-261 semantics)
+ *   orc_ranges       splat_tiles_clear.slang:8-17 + splat_ranges.slang:11-25
+ *   orc_blend        splat_blend.slang:236-578 (splatBlend: composite, matte
+ *                    and trace off, contributionMode 0)
+ *   orc_blend_f64    the same walk with every FP32 decision (skip, clamp,
+ *                    stop) replayed and the arithmetic in FP64
+ *   orc_reference    reference/reference_blend.slang:28-123 brute force
+ *   orc_render_ref_order  the reference's own two-sort order: depth sort of
+ *                    visible slots (24 bits, stable), emit in depth order,
+ *                    stable tile sort (TileRasterizer.cpp:1111-1176)
+ *   orc_fd_grads     FP64 central finite differences of an L2 image loss
+ *                    with every discrete decision frozen
+ *
+ * Parity of each function is pinned by tests/test_oracle.py (closed forms,
+ * invariants, brute force and the reference's own test assertions).
+ *
+ * Build: oracle/Makefile (gcc, -ffp-contract=off so every a*b+c here is a
+ * multiply then an add, as in the kernels; only explicit fmaf() fuses).
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#include "../include/athx_detmath.h"
+
+#define ORC_EXPORT __attribute__((visibility("default")))
+
+/* What a frame's projection is told (frame.slang:51-230, the fields the plain
+ * path reads). Layout mirrored by oracle/pyoracle.py. */
+typedef struct orc_view {
+    uint32_t width, height;
+    float focal_x, focal_y, centre_x, centre_y;
+    float near_z, far_z;
+    uint32_t orthographic;
+    uint32_t antialias;
+    uint32_t linear;      /* 1: colours are linear light; 0: sRGB, decoded per splat */
+    uint32_t sh_limit;    /* highest degree evaluated this frame */
+    uint32_t tile;        /* tile side in pixels (16 in the reference) */
+    float m[12];          /* object -> view, rows of a 3x4 */
+    float eye[3];         /* the camera in the cloud's own space */
+    float bg[4];          /* premultiplied linear background */
+    uint32_t depth_mode;  /* 0 alpha-weighted mean, 1 first crossing */
+    float depth_threshold;
+} orc_view;
+
+/* frame.slang:42-48 */
+#define CULLED_KEY 0xFF000000u
+#define CULL_DEPTH 2u
+#define CULL_DEGENERATE 3u
+#define CULL_FAINT 4u
+#define CULL_OFFSCREEN 5u
+#define CULL_NO_TILE 6u
+
+/* Per-slot record layout: screen (x, y, z, cutoff), conic (a, b, c, alpha),
+ * colour (r, g, b, 0) -- frame.slang:269-281 without the surfel words. */
+#define REC 12
+
+/* Branch flags the FP32 projection took, replayed by the FP64 one. */
+#define FL_CLAMP_XLO 1u
+#define FL_CLAMP_XHI 2u
+#define FL_CLAMP_YLO 4u
+#define FL_CLAMP_YHI 8u
+#define FL_PAID_BACK 16u /* paidOpacity's spread >= 1/255 branch */
+#define FL_COL_R 32u     /* colour channel kept (> 0 for a linear cloud) */
+#define FL_COL_G 64u
+#define FL_COL_B 128u
+#define FL_SRGB_R 256u   /* sRGB decode took the power branch */
+#define FL_SRGB_G 512u
+#define FL_SRGB_B 1024u
+
+static const float kSH0 = 0.28209479177387814f;
+static const float kC1 = 0.4886025119029199f;
+static const float kC2[5] = {1.0925484305920792f, -1.0925484305920792f, 0.31539156525252005f,
+                             -1.0925484305920792f, 0.5462742152960396f};
+static const float kC3[7] = {-0.5900435899266435f, 2.890611442640554f, -0.4570457994644658f,
+                             0.3731763325901154f, -0.4570457994644658f, 1.445305721320277f,
+                             -0.5900435899266435f};
+
+static inline float minf_(float a, float b) { return a < b ? a : b; }
+static inline float maxf_(float a, float b) { return a > b ? a : b; }
+static inline float clampf_(float x, float lo, float hi) { return minf_(maxf_(x, lo), hi); }
+
+/* frame.slang:325-327 */
+static float cutoff_power(float alpha) {
+    return minf_(2.0f * athx_logf(maxf_(255.0f * alpha, 1.0001f)), 9.0f);
+}
+
+/* frame.slang:316-319 */
+static float paid_opacity(float alpha, float paid, uint32_t* flags) {
+    const float spread = paid * alpha;
+    if (spread >= 1.0f / 255.0f) {
+        *flags |= FL_PAID_BACK;
+        return spread + (1.0f - paid) * (1.0f / 255.0f);
+    }
+    return spread;
+}
+
+/* frame.slang:354-371 */
+static float min_power_over_box(float a, float b, float c, float lox, float loy, float hix, float hiy) {
+    if (lox <= 0.0f && hix >= 0.0f && loy <= 0.0f && hiy >= 0.0f) {
+        return 0.0f;
+    }
+    float best = 3.4e38f;
+    for (int e = 0; e < 2; ++e) {
+        const float x = e == 0 ? lox : hix;
+        const float y = c > 1e-12f ? clampf_(((-b) * x) / c, loy, hiy) : loy;
+        best = minf_(best, a * x * x + 2.0f * b * x * y + c * y * y);
+    }
+    for (int e = 0; e < 2; ++e) {
+        const float y = e == 0 ? loy : hiy;
+        const float x = a > 1e-12f ? clampf_(((-b) * y) / a, lox, hix) : lox;
+        best = minf_(best, a * x * x + 2.0f * b * x * y + c * y * y);
+    }
+    return best;
+}
+
+/* frame.slang:374-379 */
+static int tile_touched(const orc_view* v, float cx, float cy, float a, float b, float c, float cutoff,
+                        uint32_t tx, uint32_t ty) {
+    const uint32_t T = v->tile;
+    const float fx = (float)(tx * T) + 0.5f;
+    const float fy = (float)(ty * T) + 0.5f;
+    const uint32_t ex = (tx + 1) * T < v->width ? (tx + 1) * T : v->width;
+    const uint32_t ey = (ty + 1) * T < v->height ? (ty + 1) * T : v->height;
+    const float lx = (float)(ex - 1) + 0.5f;
+    const float ly = (float)(ey - 1) + 0.5f;
+    return min_power_over_box(a, b, c, fx - cx, fy - cy, lx - cx, ly - cy) <= cutoff;
+}
+
+/* frame.slang:383-387 then 390-402. Returns 0 when off screen. */
+static int tile_rect(const orc_view* v, float cx, float cy, float a, float b, float c, float cutoff,
+                     uint32_t rect[4]) {
+    const float det = a * c - b * b;
+    const float inv = det > 1e-20f ? 1.0f / det : 0.0f;
+    const float ex = sqrtf(maxf_(cutoff * c * inv, 0.0f));
+    const float ey = sqrtf(maxf_(cutoff * a * inv, 0.0f));
+    const float x1 = cx - ex, y1 = cy - ey, x2 = cx + ex, y2 = cy + ey;
+    if (x2 < 0.0f || y2 < 0.0f || x1 >= (float)v->width || y1 >= (float)v->height) {
+        rect[0] = 1; rect[1] = 1; rect[2] = 0; rect[3] = 0;
+        return 0;
+    }
+    const uint32_t T = v->tile;
+    rect[0] = (uint32_t)maxf_(x1, 0.0f) / T;
+    rect[1] = (uint32_t)maxf_(y1, 0.0f) / T;
+    rect[2] = (uint32_t)minf_(x2, (float)v->width - 1.0f) / T;
+    rect[3] = (uint32_t)minf_(y2, (float)v->height - 1.0f) / T;
+    return rect[0] <= rect[2] && rect[1] <= rect[3];
+}
+
+/* color.slang:7-10 through the deterministic power. */
+static float srgb_to_linear(float c, uint32_t* took_pow) {
+    c = maxf_(c, 0.0f);
+    if (c <= 0.04045f) {
+        *took_pow = 0;
+        return c / 12.92f;
+    }
+    *took_pow = 1;
+    return athx_powf_pos((c + 0.055f) / 1.055f, 2.4f);
+}
+
+/* sh.slang:25-51 for one colour channel; coefficient k of channel ch at sh[k*3+ch]
+ * (k = 0 is the DC term, read by the caller). */
+static float sh_rest(const float* sh, uint32_t ch, uint32_t degree, float x, float y, float z) {
+    if (degree == 0) {
+        return 0.0f;
+    }
+#define B(k) sh[(k) * 3 + ch]
+    float r = -kC1 * y * B(1) + kC1 * z * B(2) - kC1 * x * B(3);
+    if (degree < 2) {
+        return r;
+    }
+    const float xx = x * x, yy = y * y, zz = z * z;
+    const float xy = x * y, yz = y * z, xz = x * z;
+    r = r + (kC2[0] * xy * B(4) + kC2[1] * yz * B(5) + kC2[2] * (2.0f * zz - xx - yy) * B(6) +
+             kC2[3] * xz * B(7) + kC2[4] * (xx - yy) * B(8));
+    if (degree < 3) {
+        return r;
+    }
+    r = r + (kC3[0] * y * (3.0f * xx - yy) * B(9) + kC3[1] * xy * z * B(10) +
+             kC3[2] * y * (4.0f * zz - xx - yy) * B(11) + kC3[3] * z * (2.0f * zz - 3.0f * xx - 3.0f * yy) * B(12) +
+             kC3[4] * x * (4.0f * zz - xx - yy) * B(13) + kC3[5] * z * (xx - yy) * B(14) +
+             kC3[6] * x * (xx - 3.0f * yy) * B(15));
+#undef B
+    return r;
+}
+
+static uint32_t degree_of(uint32_t coeffs) {
+    return coeffs >= 16 ? 3u : coeffs >= 9 ? 2u : coeffs >= 4 ? 1u : 0u;
+}
+
+/* One splat projected: splat_project.slang:497-1130 with kTransfer 0. Writes
+ * rec, touched and key for slot i; returns the cull reason (0 = visible). */
+static uint32_t project_one(const orc_view* v, const float* P, const float* LS, const float* Q, const float* LG,
+                            const float* SH, uint32_t coeffs, uint32_t i, float* rec, uint32_t* touched,
+                            uint32_t* key, uint32_t* flags_out) {
+    uint32_t flags = 0;
+    const float px = P[i * 3 + 0], py = P[i * 3 + 1], pz = P[i * 3 + 2];
+    const float sx = athx_expf(LS[i * 3 + 0]);
+    const float sy = athx_expf(LS[i * 3 + 1]);
+    const float sz = athx_expf(LS[i * 3 + 2]);
+    const float opacity = athx_sigmoidf(LG[i]);
+    const float* m = v->m;
+    const float tx = m[0] * px + m[1] * py + m[2] * pz + m[3];
+    const float ty = m[4] * px + m[5] * py + m[6] * pz + m[7];
+    const float tz = m[8] * px + m[9] * py + m[10] * pz + m[11];
+    *touched = 0;
+    if (flags_out) *flags_out = 0;
+    if (!(tz > v->near_z) || tz > v->far_z) {
+        *key = CULLED_KEY | CULL_DEPTH;
+        return CULL_DEPTH;
+    }
+    /* Quaternion (x, y, z, w), normalised (packing.slang:59-76 decodes a unit one). */
+    float qx = Q[i * 4 + 0], qy = Q[i * 4 + 1], qz = Q[i * 4 + 2], qw = Q[i * 4 + 3];
+    const float qn = qx * qx + qy * qy + qz * qz + qw * qw;
+    if (qn > 0.0f) {
+        const float inv = 1.0f / sqrtf(qn);
+        qx = qx * inv; qy = qy * inv; qz = qz * inv; qw = qw * inv;
+    } else {
+        qx = 0.0f; qy = 0.0f; qz = 0.0f; qw = 1.0f;
+    }
+    const float r00 = 1.0f - 2.0f * (qy * qy + qz * qz), r01 = 2.0f * (qx * qy - qw * qz), r02 = 2.0f * (qx * qz + qw * qy);
+    const float r10 = 2.0f * (qx * qy + qw * qz), r11 = 1.0f - 2.0f * (qx * qx + qz * qz), r12 = 2.0f * (qy * qz - qw * qx);
+    const float r20 = 2.0f * (qx * qz - qw * qy), r21 = 2.0f * (qy * qz + qw * qx), r22 = 1.0f - 2.0f * (qx * qx + qy * qy);
+    const float m00 = r00 * sx, m01 = r01 * sy, m02 = r02 * sz;
+    const float m10 = r10 * sx, m11 = r11 * sy, m12 = r12 * sz;
+    const float m20 = r20 * sx, m21 = r21 * sy, m22 = r22 * sz;
+
+    float jx0, jx2, jy1, jy2, cx, cy;
+    if (v->orthographic) {
+        jx0 = v->focal_x; jx2 = 0.0f;
+        jy1 = v->focal_y; jy2 = 0.0f;
+        cx = v->focal_x * tx + v->centre_x;
+        cy = v->focal_y * ty + v->centre_y;
+    } else {
+        const float invz = 1.0f / tz;
+        cx = v->focal_x * tx * invz + v->centre_x;
+        cy = v->focal_y * ty * invz + v->centre_y;
+        const float W = (float)v->width, H = (float)v->height;
+        const float limlox = (-0.15f * W - v->centre_x) / v->focal_x;
+        const float limhix = (1.15f * W - v->centre_x) / v->focal_x;
+        const float limloy = (-0.15f * H - v->centre_y) / v->focal_y;
+        const float limhiy = (1.15f * H - v->centre_y) / v->focal_y;
+        const float ux = tx * invz, uy = ty * invz;
+        if (ux < limlox) flags |= FL_CLAMP_XLO;
+        if (ux > limhix) flags |= FL_CLAMP_XHI;
+        if (uy < limloy) flags |= FL_CLAMP_YLO;
+        if (uy > limhiy) flags |= FL_CLAMP_YHI;
+        const float txc = clampf_(ux, limlox, limhix) * tz;
+        const float tyc = clampf_(uy, limloy, limhiy) * tz;
+        jx0 = v->focal_x * invz;
+        jx2 = -v->focal_x * txc * invz * invz;
+        jy1 = v->focal_y * invz;
+        jy2 = -v->focal_y * tyc * invz * invz;
+    }
+    /* T = J W, J's zero entries dropped: row x is (jx0, 0, jx2), row y (0, jy1, jy2). */
+    const float t30 = jx0 * m[0] + jx2 * m[8];
+    const float t31 = jx0 * m[1] + jx2 * m[9];
+    const float t32 = jx0 * m[2] + jx2 * m[10];
+    const float u30 = jy1 * m[4] + jy2 * m[8];
+    const float u31 = jy1 * m[5] + jy2 * m[9];
+    const float u32 = jy1 * m[6] + jy2 * m[10];
+    const float ax0 = t30 * m00 + t31 * m10 + t32 * m20;
+    const float ax1 = t30 * m01 + t31 * m11 + t32 * m21;
+    const float ax2 = t30 * m02 + t31 * m12 + t32 * m22;
+    const float ay0 = u30 * m00 + u31 * m10 + u32 * m20;
+    const float ay1 = u30 * m01 + u31 * m11 + u32 * m21;
+    const float ay2 = u30 * m02 + u31 * m12 + u32 * m22;
+    float a = ax0 * ax0 + ax1 * ax1 + ax2 * ax2;
+    const float b = ax0 * ay0 + ax1 * ay1 + ax2 * ay2;
+    float c = ay0 * ay0 + ay1 * ay1 + ay2 * ay2;
+    const float det_before = a * c - b * b;
+    a = a + 0.3f;
+    c = c + 0.3f;
+    const float det = a * c - b * b;
+    if (!(det > 1e-12f)) {
+        *key = CULLED_KEY | CULL_DEGENERATE;
+        if (flags_out) *flags_out = flags;
+        return CULL_DEGENERATE;
+    }
+    const float filtered = sqrtf(maxf_(det_before, 0.0f) / det);
+    const float paid = v->antialias ? filtered : 1.0f;
+    const float alpha = paid_opacity(opacity, paid, &flags);
+    if (!(alpha >= 1.0f / 255.0f)) {
+        *key = CULLED_KEY | CULL_FAINT;
+        if (flags_out) *flags_out = flags;
+        return CULL_FAINT;
+    }
+    const float bdet = a * c - b * b;
+    const float ca = c / bdet, cb = -b / bdet, cc = a / bdet;
+    const float cutoff = cutoff_power(alpha);
+    uint32_t rect[4];
+    if (!tile_rect(v, cx, cy, ca, cb, cc, cutoff, rect)) {
+        *key = CULLED_KEY | CULL_OFFSCREEN;
+        if (flags_out) *flags_out = flags;
+        return CULL_OFFSCREEN;
+    }
+    uint32_t count = 0;
+    for (uint32_t ty_ = rect[1]; ty_ <= rect[3]; ++ty_) {
+        for (uint32_t tx_ = rect[0]; tx_ <= rect[2]; ++tx_) {
+            count += (uint32_t)tile_touched(v, cx, cy, ca, cb, cc, cutoff, tx_, ty_);
+        }
+    }
+    if (count == 0) {
+        *key = CULLED_KEY | CULL_NO_TILE;
+        if (flags_out) *flags_out = flags;
+        return CULL_NO_TILE;
+    }
+    /* Colour from this eye, in the cloud's space (splat_project.slang:781-785). */
+    const float* sh = SH + (size_t)i * coeffs * 3;
+    const uint32_t degree = degree_of(coeffs) < v->sh_limit ? degree_of(coeffs) : v->sh_limit;
+    const float dx = px - v->eye[0], dy = py - v->eye[1], dz = pz - v->eye[2];
+    const float dinv = 1.0f / sqrtf(dx * dx + dy * dy + dz * dz);
+    const float ex = dx * dinv, ey = dy * dinv, ez = dz * dinv;
+    float col[3];
+    for (uint32_t ch = 0; ch < 3; ++ch) {
+        const float base = 0.5f + kSH0 * sh[ch];
+        const float cval = base + sh_rest(sh, ch, degree, ex, ey, ez);
+        if (v->linear) {
+            if (cval > 0.0f) flags |= (FL_COL_R << ch);
+            col[ch] = maxf_(cval, 0.0f);
+        } else {
+            uint32_t took = 0;
+            col[ch] = srgb_to_linear(cval, &took);
+            if (cval > 0.0f) flags |= (FL_COL_R << ch);
+            if (took) flags |= (FL_SRGB_R << ch);
+        }
+    }
+    float* r = rec + (size_t)i * REC;
+    r[0] = cx; r[1] = cy; r[2] = tz; r[3] = cutoff;
+    r[4] = ca; r[5] = cb; r[6] = cc; r[7] = alpha;
+    r[8] = col[0]; r[9] = col[1]; r[10] = col[2]; r[11] = 0.0f;
+    *touched = count;
+    uint32_t zb;
+    memcpy(&zb, &tz, 4);
+    *key = zb >> 7;
+    if (flags_out) *flags_out = flags;
+    return 0;
+}
+
+/* The whole cloud projected into slots [0, n). flags may be NULL. */
+ORC_EXPORT void orc_project(const orc_view* v, const float* means, const float* log_scales, const float* quats,
+                            const float* logits, const float* sh, uint32_t coeffs, uint32_t n, float* rec,
+                            uint32_t* touched, uint32_t* keys, uint32_t* flags) {
+#pragma omp parallel for schedule(dynamic, 4096)
+    for (int64_t i = 0; i < (int64_t)n; ++i) {
+        project_one(v, means, log_scales, quats, logits, sh, coeffs, (uint32_t)i, rec, touched + i, keys + i,
+                    flags ? flags + i : NULL);
+    }
+}
+
+/* Exclusive scan; returns the total (PrefixSum.cpp:58-88 semantics). */
+ORC_EXPORT uint64_t orc_exclusive_scan(const uint32_t* in, uint32_t* out, uint64_t n) {
+    uint64_t sum = 0;
+    for (uint64_t i = 0; i < n; ++i) {
+        const uint32_t x = in[i];
+        out[i] = (uint32_t)sum;
+        sum += x;
+    }
+    return sum;
+}
+
+/* One pair per touched tile, in slot order, each slot's tiles row by row:
+ * key = (tile_offset + tile) << 24 | depth24, value = slot. The tile test is
+ * splat_emit.slang:37-49's. offsets = exclusive scan of touched. */
+ORC_EXPORT void orc_emit_pairs(const orc_view* v, const float* rec, const uint32_t* touched, const uint32_t* keys,
+                               const uint32_t* offsets, uint32_t n, uint32_t tile_offset, uint32_t slot_base,
+                               uint64_t* pair_keys, uint32_t* pair_vals) {
+    const uint32_t tiles_x = (v->width + v->tile - 1) / v->tile;
+#pragma omp parallel for schedule(dynamic, 4096)
+    for (int64_t i = 0; i < (int64_t)n; ++i) {
+        if (touched[i] == 0) continue;
+        const float* r = rec + (size_t)i * REC;
+        uint32_t rect[4];
+        tile_rect(v, r[0], r[1], r[4], r[5], r[6], r[3], rect);
+        uint64_t at = offsets[i];
+        const uint64_t stop = at + touched[i];
+        for (uint32_t ty = rect[1]; ty <= rect[3] && at < stop; ++ty) {
+            for (uint32_t tx = rect[0]; tx <= rect[2] && at < stop; ++tx) {
+                if (tile_touched(v, r[0], r[1], r[4], r[5], r[6], r[3], tx, ty)) {
+                    const uint64_t tile = (uint64_t)tile_offset + (uint64_t)ty * tiles_x + tx;
+                    pair_keys[at] = (tile << 24) | (uint64_t)(keys[i] & 0xFFFFFFu);
+                    pair_vals[at] = slot_base + (uint32_t)i;
+                    ++at;
+                }
+            }
+        }
+    }
+}
+
+/* Stable LSD radix sort of (key, value) over the low `bits` bits, 8 a pass. */
+ORC_EXPORT void orc_sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t n, uint32_t bits) {
+    if (n < 2) return;
+    uint64_t* k2 = (uint64_t*)malloc(n * sizeof(uint64_t));
+    uint32_t* v2 = (uint32_t*)malloc(n * sizeof(uint32_t));
+    uint64_t* ka = keys;
+    uint32_t* va = vals;
+    uint64_t* kb = k2;
+    uint32_t* vb = v2;
+    for (uint32_t shift = 0; shift < bits; shift += 8) {
+        uint64_t count[257];
+        memset(count, 0, sizeof count);
+        for (uint64_t i = 0; i < n; ++i) count[((ka[i] >> shift) & 0xFF) + 1]++;
+        for (int d = 0; d < 256; ++d) count[d + 1] += count[d];
+        for (uint64_t i = 0; i < n; ++i) {
+            const uint64_t at = count[(ka[i] >> shift) & 0xFF]++;
+            kb[at] = ka[i];
+            vb[at] = va[i];
+        }
+        uint64_t* tk = ka; ka = kb; kb = tk;
+        uint32_t* tv = va; va = vb; vb = tv;
+    }
+    if (ka != keys) {
+        memcpy(keys, ka, n * sizeof(uint64_t));
+        memcpy(vals, va, n * sizeof(uint32_t));
+    }
+    free(k2);
+    free(v2);
+}
+
+/* ranges[2t] = first pair of tile t, ranges[2t+1] = one past its last; 0, 0 for
+ * a tile with none (splat_tiles_clear.slang:8-17, splat_ranges.slang:11-25). */
+ORC_EXPORT void orc_ranges(const uint64_t* sorted_keys, uint64_t n, uint32_t tiles, uint32_t* ranges) {
+    memset(ranges, 0, (size_t)tiles * 2 * sizeof(uint32_t));
+    for (uint64_t i = 0; i < n; ++i) {
+        const uint64_t t = sorted_keys[i] >> 24;
+        if (t >= tiles) continue;
+        if (i == 0 || (sorted_keys[i - 1] >> 24) != t) ranges[t * 2] = (uint32_t)i;
+        if (i + 1 == n || (sorted_keys[i + 1] >> 24) != t) ranges[t * 2 + 1] = (uint32_t)(i + 1);
+    }
+}
+
+/* One pixel's walk, FP32 (splat_blend.slang:287-512 with the plain branches). */
+static void blend_pixel_f32(const orc_view* v, const float* rec, const uint32_t* vals, uint32_t start, uint32_t end,
+                            uint32_t px, uint32_t py, float out[4], float* depth, float* final_t, uint32_t* last) {
+    const float pxc = (float)px + 0.5f, pyc = (float)py + 0.5f;
+    float rgb0 = 0.0f, rgb1 = 0.0f, rgb2 = 0.0f;
+    float T = 1.0f, depth_sum = 0.0f, depth_at = -1.0f;
+    uint32_t stop = 0; /* one past the last pair that contributed, from the tile's first */
+    for (uint32_t j = start; j < end; ++j) {
+        const float* r = rec + (size_t)vals[j] * REC;
+        const float dx = pxc - r[0], dy = pyc - r[1];
+        const float power = r[4] * dx * dx + 2.0f * r[5] * dx * dy + r[6] * dy * dy;
+        if (power > r[3]) continue;
+        const float alpha = minf_(0.99f, r[7] * athx_expf(-0.5f * power));
+        if (alpha < 1.0f / 255.0f) continue;
+        const float w = T * alpha;
+        rgb0 = rgb0 + w * r[8];
+        rgb1 = rgb1 + w * r[9];
+        rgb2 = rgb2 + w * r[10];
+        depth_sum = depth_sum + w * r[2];
+        T = T * (1.0f - alpha);
+        if (depth_at < 0.0f && 1.0f - T >= v->depth_threshold) depth_at = r[2];
+        stop = j - start + 1;
+        if (T < 1e-4f) break;
+    }
+    const float coverage = 1.0f - T;
+    out[0] = rgb0 + T * v->bg[0];
+    out[1] = rgb1 + T * v->bg[1];
+    out[2] = rgb2 + T * v->bg[2];
+    out[3] = coverage + T * v->bg[3];
+    if (v->depth_mode == 0) {
+        *depth = coverage > 1e-6f ? depth_sum / coverage : 0.0f;
+    } else {
+        *depth = maxf_(depth_at, 0.0f);
+    }
+    *final_t = T;
+    *last = stop;
+}
+
+ORC_EXPORT void orc_blend(const orc_view* v, const float* rec, const uint32_t* ranges, const uint32_t* vals,
+                          uint32_t tile_offset, float* colour, float* depth, float* final_t, uint32_t* last) {
+    const uint32_t W = v->width, H = v->height, T = v->tile;
+    const uint32_t tiles_x = (W + T - 1) / T;
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t p = 0; p < (int64_t)W * H; ++p) {
+        const uint32_t px = (uint32_t)(p % W), py = (uint32_t)(p / W);
+        const uint32_t tile = tile_offset + (py / T) * tiles_x + px / T;
+        blend_pixel_f32(v, rec, vals, ranges[tile * 2], ranges[tile * 2 + 1], px, py, colour + p * 4, depth + p,
+                        final_t + p, last + p);
+    }
+}
+
+/* ---------------------------------------------------------------------------
+ * FP64: the same projection with the FP32 path's branches replayed.
+ * rec64 layout is REC doubles a slot; only slots the FP32 path kept are written.
+ */
+typedef struct {
+    double cx, cy, z, a, b, c, alpha, col[3];
+} rec64_t;
+
+static int project_one_f64(const orc_view* v, const double* P, const double* LS, const double* Q, const double* LG,
+                           const double* SH, uint32_t coeffs, uint32_t i, uint32_t flags, rec64_t* out) {
+    const double px = P[i * 3 + 0], py = P[i * 3 + 1], pz = P[i * 3 + 2];
+    const double sx = exp(LS[i * 3 + 0]), sy = exp(LS[i * 3 + 1]), sz = exp(LS[i * 3 + 2]);
+    const double opacity = 1.0 / (1.0 + exp(-LG[i]));
+    double m[12];
+    for (int k = 0; k < 12; ++k) m[k] = (double)v->m[k];
+    const double tx = m[0] * px + m[1] * py + m[2] * pz + m[3];
+    const double ty = m[4] * px + m[5] * py + m[6] * pz + m[7];
+    const double tz = m[8] * px + m[9] * py + m[10] * pz + m[11];
+    double qx = Q[i * 4 + 0], qy = Q[i * 4 + 1], qz = Q[i * 4 + 2], qw = Q[i * 4 + 3];
+    const double qn = qx * qx + qy * qy + qz * qz + qw * qw;
+    if (qn > 0.0) {
+        const double inv = 1.0 / sqrt(qn);
+        qx *= inv; qy *= inv; qz *= inv; qw *= inv;
+    } else {
+        qx = 0; qy = 0; qz = 0; qw = 1;
+    }
+    const double r00 = 1 - 2 * (qy * qy + qz * qz), r01 = 2 * (qx * qy - qw * qz), r02 = 2 * (qx * qz + qw * qy);
+    const double r10 = 2 * (qx * qy + qw * qz), r11 = 1 - 2 * (qx * qx + qz * qz), r12 = 2 * (qy * qz - qw * qx);
+    const double r20 = 2 * (qx * qz - qw * qy), r21 = 2 * (qy * qz + qw * qx), r22 = 1 - 2 * (qx * qx + qy * qy);
+    const double m00 = r00 * sx, m01 = r01 * sy, m02 = r02 * sz;
+    const double m10 = r10 * sx, m11 = r11 * sy, m12 = r12 * sz;
+    const double m20 = r20 * sx, m21 = r21 * sy, m22 = r22 * sz;
+    const double fx = v->focal_x, fy = v->focal_y, ccx = v->centre_x, ccy = v->centre_y;
+    double jx0, jx2, jy1, jy2, cx, cy;
+    if (v->orthographic) {
+        jx0 = fx; jx2 = 0; jy1 = fy; jy2 = 0;
+        cx = fx * tx + ccx;
+        cy = fy * ty + ccy;
+    } else {
+        const double invz = 1.0 / tz;
+        cx = fx * tx * invz + ccx;
+        cy = fy * ty * invz + ccy;
+        const double W = v->width, H = v->height;
+        /* The limits are the FP32 frame's constants, as the kernel has them. */
+        const double limlox = (double)((-0.15f * (float)W - v->centre_x) / v->focal_x);
+        const double limhix = (double)((1.15f * (float)W - v->centre_x) / v->focal_x);
+        const double limloy = (double)((-0.15f * (float)H - v->centre_y) / v->focal_y);
+        const double limhiy = (double)((1.15f * (float)H - v->centre_y) / v->focal_y);
+        double ux = tx * invz, uy = ty * invz;
+        if (flags & FL_CLAMP_XLO) ux = limlox;
+        else if (flags & FL_CLAMP_XHI) ux = limhix;
+        if (flags & FL_CLAMP_YLO) uy = limloy;
+        else if (flags & FL_CLAMP_YHI) uy = limhiy;
+        const double txc = ux * tz, tyc = uy * tz;
+        jx0 = fx * invz;
+        jx2 = -fx * txc * invz * invz;
+        jy1 = fy * invz;
+        jy2 = -fy * tyc * invz * invz;
+    }
+    const double t30 = jx0 * m[0] + jx2 * m[8], t31 = jx0 * m[1] + jx2 * m[9], t32 = jx0 * m[2] + jx2 * m[10];
+    const double u30 = jy1 * m[4] + jy2 * m[8], u31 = jy1 * m[5] + jy2 * m[9], u32 = jy1 * m[6] + jy2 * m[10];
+    const double ax0 = t30 * m00 + t31 * m10 + t32 * m20;
+    const double ax1 = t30 * m01 + t31 * m11 + t32 * m21;
+    const double ax2 = t30 * m02 + t31 * m12 + t32 * m22;
+    const double ay0 = u30 * m00 + u31 * m10 + u32 * m20;
+    const double ay1 = u30 * m01 + u31 * m11 + u32 * m21;
+    const double ay2 = u30 * m02 + u31 * m12 + u32 * m22;
+    double a = ax0 * ax0 + ax1 * ax1 + ax2 * ax2;
+    const double b = ax0 * ay0 + ax1 * ay1 + ax2 * ay2;
+    double c = ay0 * ay0 + ay1 * ay1 + ay2 * ay2;
+    const double det_before = a * c - b * b;
+    a += 0.3;
+    c += 0.3;
+    const double det = a * c - b * b;
+    const double filtered = sqrt((det_before > 0 ? det_before : 0.0) / det);
+    const double paid = v->antialias ? filtered : 1.0;
+    const double spread = paid * opacity;
+    const double alpha = (flags & FL_PAID_BACK) ? spread + (1.0 - paid) / 255.0 : spread;
+    out->cx = cx; out->cy = cy; out->z = tz;
+    out->a = c / det; out->b = -b / det; out->c = a / det;
+    out->alpha = alpha;
+    const double* sh = SH + (size_t)i * coeffs * 3;
+    const uint32_t dg = degree_of(coeffs) < v->sh_limit ? degree_of(coeffs) : v->sh_limit;
+    const double dx = px - v->eye[0], dy = py - v->eye[1], dz = pz - v->eye[2];
+    const double dl = sqrt(dx * dx + dy * dy + dz * dz);
+    const double x = dx / dl, y = dy / dl, z = dz / dl;
+    for (uint32_t ch = 0; ch < 3; ++ch) {
+#define B(k) sh[(k) * 3 + ch]
+        double cv = 0.5 + 0.28209479177387814 * B(0);
+        if (dg >= 1) cv += -0.4886025119029199 * y * B(1) + 0.4886025119029199 * z * B(2) - 0.4886025119029199 * x * B(3);
+        if (dg >= 2) {
+            const double xx = x * x, yy = y * y, zz = z * z;
+            cv += 1.0925484305920792 * x * y * B(4) - 1.0925484305920792 * y * z * B(5) +
+                  0.31539156525252005 * (2 * zz - xx - yy) * B(6) - 1.0925484305920792 * x * z * B(7) +
+                  0.5462742152960396 * (xx - yy) * B(8);
+        }
+        if (dg >= 3) {
+            const double xx = x * x, yy = y * y, zz = z * z;
+            cv += -0.5900435899266435 * y * (3 * xx - yy) * B(9) + 2.890611442640554 * x * y * z * B(10) -
+                  0.4570457994644658 * y * (4 * zz - xx - yy) * B(11) +
+                  0.3731763325901154 * z * (2 * zz - 3 * xx - 3 * yy) * B(12) -
+                  0.4570457994644658 * x * (4 * zz - xx - yy) * B(13) + 1.445305721320277 * z * (xx - yy) * B(14) -
+                  0.5900435899266435 * x * (xx - 3 * yy) * B(15);
+        }
+#undef B
+        if (v->linear) {
+            out->col[ch] = (flags & (FL_COL_R << ch)) ? cv : 0.0;
+        } else {
+            const double cc = (flags & (FL_COL_R << ch)) ? cv : 0.0;
+            out->col[ch] = (flags & (FL_SRGB_R << ch)) ? pow((cc + 0.055) / 1.055, 2.4) : cc / 12.92;
+        }
+    }
+    return 1;
+}
+
+/* The FP64 image: the FP32 walk decides (skip, clamp, stop); the sums are FP64.
+ * rec/ranges/vals/flags come from the FP32 path; params are FP64 copies. */
+ORC_EXPORT void orc_blend_f64(const orc_view* v, const float* rec, const uint32_t* ranges, const uint32_t* vals,
+                              const uint32_t* touched, const uint32_t* flags, uint32_t tile_offset,
+                              const double* means, const double* log_scales, const double* quats,
+                              const double* logits, const double* sh, uint32_t coeffs, uint32_t n, double* colour,
+                              double* depth) {
+    rec64_t* r64 = (rec64_t*)calloc(n ? n : 1, sizeof(rec64_t));
+#pragma omp parallel for schedule(dynamic, 4096)
+    for (int64_t i = 0; i < (int64_t)n; ++i) {
+        if (touched[i]) project_one_f64(v, means, log_scales, quats, logits, sh, coeffs, (uint32_t)i, flags[i], &r64[i]);
+    }
+    const uint32_t W = v->width, H = v->height, TS = v->tile;
+    const uint32_t tiles_x = (W + TS - 1) / TS;
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t p = 0; p < (int64_t)W * H; ++p) {
+        const uint32_t px = (uint32_t)(p % W), py = (uint32_t)(p / W);
+        const uint32_t tile = tile_offset + (py / TS) * tiles_x + px / TS;
+        const uint32_t start = ranges[tile * 2], end = ranges[tile * 2 + 1];
+        const float pxc = (float)px + 0.5f, pyc = (float)py + 0.5f;
+        float T32 = 1.0f;
+        double T = 1.0, rgb[3] = {0, 0, 0}, dsum = 0.0, dat = -1.0;
+        for (uint32_t j = start; j < end; ++j) {
+            const uint32_t s = vals[j];
+            const float* r = rec + (size_t)s * REC;
+            const float dx = pxc - r[0], dy = pyc - r[1];
+            const float power = r[4] * dx * dx + 2.0f * r[5] * dx * dy + r[6] * dy * dy;
+            if (power > r[3]) continue;
+            const float raw = r[7] * athx_expf(-0.5f * power);
+            const float alpha32 = minf_(0.99f, raw);
+            if (alpha32 < 1.0f / 255.0f) continue;
+            const rec64_t* q = &r64[s];
+            const double ddx = (px + 0.5) - q->cx, ddy = (py + 0.5) - q->cy;
+            const double pw = q->a * ddx * ddx + 2.0 * q->b * ddx * ddy + q->c * ddy * ddy;
+            const double alpha = raw > 0.99f ? 0.99 : q->alpha * exp(-0.5 * pw);
+            const double w = T * alpha;
+            rgb[0] += w * q->col[0];
+            rgb[1] += w * q->col[1];
+            rgb[2] += w * q->col[2];
+            dsum += w * q->z;
+            T *= 1.0 - alpha;
+            T32 = T32 * (1.0f - alpha32);
+            if (dat < 0.0 && 1.0f - T32 >= v->depth_threshold) dat = q->z;
+            if (T32 < 1e-4f) break;
+        }
+        const double cov = 1.0 - T;
+        colour[p * 4 + 0] = rgb[0] + T * v->bg[0];
+        colour[p * 4 + 1] = rgb[1] + T * v->bg[1];
+        colour[p * 4 + 2] = rgb[2] + T * v->bg[2];
+        colour[p * 4 + 3] = cov + T * v->bg[3];
+        if (v->depth_mode == 0) {
+            depth[p] = cov > 1e-6 ? dsum / cov : 0.0;
+        } else {
+            depth[p] = dat > 0.0 ? dat : 0.0;
+        }
+    }
+    free(r64);
+}
+
+/* ---------------------------------------------------------------------------
+ * The brute force of reference/reference_blend.slang:28-123: every kept slot
+ * for every pixel, insertion-sorted by (view z, slot), same skips and stop. No
+ * tiles, no keys. overflow counts pixels past max_hits. */
+ORC_EXPORT uint64_t orc_reference(const orc_view* v, const float* rec, const uint32_t* touched, uint32_t n,
+                                  uint32_t max_hits, float* colour, float* depth) {
+    uint64_t overflow = 0;
+    const uint32_t W = v->width, H = v->height;
+#pragma omp parallel for schedule(dynamic, 16) reduction(+ : overflow)
+    for (int64_t p = 0; p < (int64_t)W * H; ++p) {
+        const uint32_t px = (uint32_t)(p % W), py = (uint32_t)(p / W);
+        const float pxc = (float)px + 0.5f, pyc = (float)py + 0.5f;
+        float* hz = (float*)malloc(max_hits * sizeof(float));
+        uint32_t* hi = (uint32_t*)malloc(max_hits * sizeof(uint32_t));
+        float* ha = (float*)malloc(max_hits * sizeof(float));
+        uint32_t hits = 0, over = 0;
+        for (uint32_t s = 0; s < n; ++s) {
+            if (!touched[s]) continue;
+            const float* r = rec + (size_t)s * REC;
+            const float dx = pxc - r[0], dy = pyc - r[1];
+            const float power = r[4] * dx * dx + 2.0f * r[5] * dx * dy + r[6] * dy * dy;
+            if (power > r[3]) continue;
+            const float alpha = minf_(0.99f, r[7] * athx_expf(-0.5f * power));
+            if (alpha < 1.0f / 255.0f) continue;
+            if (hits == max_hits) {
+                over = 1;
+                continue;
+            }
+            uint32_t k = hits++;
+            while (k > 0 && (hz[k - 1] > r[2] || (hz[k - 1] == r[2] && hi[k - 1] > s))) {
+                hz[k] = hz[k - 1];
+                hi[k] = hi[k - 1];
+                ha[k] = ha[k - 1];
+                --k;
+            }
+            hz[k] = r[2];
+            hi[k] = s;
+            ha[k] = alpha;
+        }
+        float rgb0 = 0, rgb1 = 0, rgb2 = 0, T = 1.0f, dsum = 0.0f;
+        for (uint32_t k = 0; k < hits; ++k) {
+            const float* r = rec + (size_t)hi[k] * REC;
+            const float w = T * ha[k];
+            rgb0 = rgb0 + w * r[8];
+            rgb1 = rgb1 + w * r[9];
+            rgb2 = rgb2 + w * r[10];
+            dsum = dsum + w * r[2];
+            T = T * (1.0f - ha[k]);
+            if (T < 1e-4f) break;
+        }
+        const float cov = 1.0f - T;
+        colour[p * 4 + 0] = rgb0 + T * v->bg[0];
+        colour[p * 4 + 1] = rgb1 + T * v->bg[1];
+        colour[p * 4 + 2] = rgb2 + T * v->bg[2];
+        colour[p * 4 + 3] = cov + T * v->bg[3];
+        depth[p] = cov > 1e-6f ? dsum / cov : 0.0f;
+        overflow += over;
+        free(hz);
+        free(hi);
+        free(ha);
+    }
+    return overflow;
+}
+
+/* ---------------------------------------------------------------------------
+ * The reference's own order (TileRasterizer.cpp:1084-1176): compact visible
+ * slots with their 24-bit depth keys, stable-sort them by depth, emit each
+ * one's tiles in depth order, stable-sort the pairs by tile. Returns P and
+ * writes the pair values in sorted order plus the ranges: what the 64-bit
+ * key route must reproduce exactly. Timed per stage into ms[6]:
+ * project, depth sort, counts (gather + scan), emit, tile sort, ranges. */
+static double now_ms(void) {
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return ts.tv_sec * 1e3 + ts.tv_nsec * 1e-6;
+}
+
+static void radix_u32_pairs(uint32_t* k, uint32_t* v, uint64_t n, uint32_t bits, uint32_t* tk, uint32_t* tv) {
+    uint32_t *ka = k, *va = v, *kb = tk, *vb = tv;
+    uint32_t passes = 0;
+    for (uint32_t shift = 0; shift < bits; shift += 8, ++passes) {
+        uint64_t count[257];
+        memset(count, 0, sizeof count);
+        for (uint64_t i = 0; i < n; ++i) count[((ka[i] >> shift) & 0xFF) + 1]++;
+        for (int d = 0; d < 256; ++d) count[d + 1] += count[d];
+        for (uint64_t i = 0; i < n; ++i) {
+            const uint64_t at = count[(ka[i] >> shift) & 0xFF]++;
+            kb[at] = ka[i];
+            vb[at] = va[i];
+        }
+        uint32_t* t = ka; ka = kb; kb = t;
+        t = va; va = vb; vb = t;
+    }
+    if (ka != k) {
+        memcpy(k, ka, n * 4);
+        memcpy(v, va, n * 4);
+    }
+}
+
+static uint32_t bits_for(uint32_t values) {
+    uint32_t bits = 1;
+    while (bits < 32 && ((uint64_t)1 << bits) < values) ++bits;
+    return bits;
+}
+
+ORC_EXPORT uint64_t orc_render_ref_order(const orc_view* v, const float* means, const float* log_scales,
+                                         const float* quats, const float* logits, const float* sh, uint32_t coeffs,
+                                         uint32_t n, float* rec, uint32_t* touched, uint32_t* keys,
+                                         uint32_t* pair_vals_out, uint64_t pair_cap, uint32_t* ranges, float* colour,
+                                         float* depth, float* final_t, uint32_t* last, double* ms) {
+    const uint32_t TS = v->tile;
+    const uint32_t tiles_x = (v->width + TS - 1) / TS, tiles_y = (v->height + TS - 1) / TS;
+    const uint32_t tiles = tiles_x * tiles_y;
+    double t0 = now_ms();
+    orc_project(v, means, log_scales, quats, logits, sh, coeffs, n, rec, touched, keys, NULL);
+    double t1 = now_ms();
+    /* compact (splat_compact.slang:16-26) + depth sort, 24 bits */
+    uint64_t V = 0;
+    for (uint32_t i = 0; i < n; ++i) V += touched[i] != 0;
+    uint32_t* dk = (uint32_t*)malloc((V ? V : 1) * 4);
+    uint32_t* ds = (uint32_t*)malloc((V ? V : 1) * 4);
+    uint32_t* tk = (uint32_t*)malloc((V ? V : 1) * 4);
+    uint32_t* tv = (uint32_t*)malloc((V ? V : 1) * 4);
+    uint64_t at = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        if (touched[i]) {
+            dk[at] = keys[i];
+            ds[at] = i;
+            ++at;
+        }
+    }
+    radix_u32_pairs(dk, ds, V, 24, tk, tv);
+    double t2 = now_ms();
+    /* gather counts in depth order + scan (splat_gather_counts.slang:11-19) */
+    uint32_t* off = (uint32_t*)malloc((V ? V : 1) * 4);
+    uint64_t P = 0;
+    for (uint64_t k = 0; k < V; ++k) {
+        off[k] = (uint32_t)P;
+        P += touched[ds[k]];
+    }
+    double t3 = now_ms();
+    if (P > pair_cap) {
+        free(dk); free(ds); free(tk); free(tv); free(off);
+        return P;
+    }
+    uint32_t* pk = (uint32_t*)malloc((P ? P : 1) * 4);
+    uint32_t* pv = pair_vals_out;
+    /* emit in depth order (splat_emit.slang:17-50) */
+#pragma omp parallel for schedule(dynamic, 1024)
+    for (int64_t k = 0; k < (int64_t)V; ++k) {
+        const uint32_t s = ds[k];
+        const float* r = rec + (size_t)s * REC;
+        uint32_t rect[4];
+        tile_rect(v, r[0], r[1], r[4], r[5], r[6], r[3], rect);
+        uint64_t w = off[k];
+        const uint64_t stop = w + touched[s];
+        for (uint32_t ty = rect[1]; ty <= rect[3] && w < stop; ++ty)
+            for (uint32_t tx = rect[0]; tx <= rect[2] && w < stop; ++tx)
+                if (tile_touched(v, r[0], r[1], r[4], r[5], r[6], r[3], tx, ty)) {
+                    pk[w] = ty * tiles_x + tx;
+                    pv[w] = s;
+                    ++w;
+                }
+    }
+    double t4 = now_ms();
+    uint32_t* tk2 = (uint32_t*)malloc((P ? P : 1) * 4);
+    uint32_t* tv2 = (uint32_t*)malloc((P ? P : 1) * 4);
+    radix_u32_pairs(pk, pv, P, bits_for(tiles) <= 8 ? 8 : (bits_for(tiles) <= 16 ? 16 : 24), tk2, tv2);
+    double t5 = now_ms();
+    memset(ranges, 0, (size_t)tiles * 2 * 4);
+    for (uint64_t i = 0; i < P; ++i) {
+        const uint32_t t = pk[i];
+        if (i == 0 || pk[i - 1] != t) ranges[t * 2] = (uint32_t)i;
+        if (i + 1 == P || pk[i + 1] != t) ranges[t * 2 + 1] = (uint32_t)(i + 1);
+    }
+    double t6 = now_ms();
+    if (colour) orc_blend(v, rec, ranges, pv, 0, colour, depth, final_t, last);
+    double t7 = now_ms();
+    if (ms) {
+        ms[0] = t1 - t0; ms[1] = t2 - t1; ms[2] = t3 - t2; ms[3] = t4 - t3; ms[4] = t5 - t4; ms[5] = t6 - t5;
+        ms[6] = t7 - t6;
+    }
+    free(dk); free(ds); free(tk); free(tv); free(off); free(pk); free(tk2); free(tv2);
+    return P;
+}
+
+ORC_EXPORT int orc_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+ORC_EXPORT void orc_set_threads(int n) {
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}
+
+ORC_EXPORT float orc_expf(float x) { return athx_expf(x); }
+ORC_EXPORT float orc_logf(float x) { return athx_logf(x); }

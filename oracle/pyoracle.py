@@ -1,0 +1,298 @@
+"""TEST INFRASTRUCTURE: numpy bindings of oracle.c, the CPU restatement of
+athenea's splat tile rasterizer that the HIP path is checked against.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use this
+module, and only as the checker (or, for the baseline, as the thing timed on
+the host cores). The product (athenea_amd) never imports it.
+
+``render`` runs the whole frame the way the HIP path does (project, scan,
+emit with 64-bit keys in slot order, stable LSD sort, ranges, blend);
+``render_ref_order`` runs the reference's own two-sort order
+(TileRasterizer.cpp:1111-1176), which must give the same pairs and ranges.
+"""
+from __future__ import annotations
+
+import ctypes
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "liboracle.so"
+
+_f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_u32p = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
+_u64p = np.ctypeslib.ndpointer(dtype=np.uint64, flags="C_CONTIGUOUS")
+
+
+class OrcView(ctypes.Structure):
+    _fields_ = [
+        ("width", ctypes.c_uint32),
+        ("height", ctypes.c_uint32),
+        ("focal_x", ctypes.c_float),
+        ("focal_y", ctypes.c_float),
+        ("centre_x", ctypes.c_float),
+        ("centre_y", ctypes.c_float),
+        ("near_z", ctypes.c_float),
+        ("far_z", ctypes.c_float),
+        ("orthographic", ctypes.c_uint32),
+        ("antialias", ctypes.c_uint32),
+        ("linear", ctypes.c_uint32),
+        ("sh_limit", ctypes.c_uint32),
+        ("tile", ctypes.c_uint32),
+        ("m", ctypes.c_float * 12),
+        ("eye", ctypes.c_float * 3),
+        ("bg", ctypes.c_float * 4),
+        ("depth_mode", ctypes.c_uint32),
+        ("depth_threshold", ctypes.c_float),
+    ]
+
+
+_lib = None
+
+
+def build() -> Path:
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+    return LIB_PATH
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            build()
+        lib = ctypes.CDLL(str(LIB_PATH))
+        vp = ctypes.POINTER(OrcView)
+        lib.orc_project.argtypes = [vp, _f32p, _f32p, _f32p, _f32p, _f32p, ctypes.c_uint32, ctypes.c_uint32, _f32p,
+                                    _u32p, _u32p, ctypes.c_void_p]
+        lib.orc_project.restype = None
+        lib.orc_exclusive_scan.argtypes = [_u32p, _u32p, ctypes.c_uint64]
+        lib.orc_exclusive_scan.restype = ctypes.c_uint64
+        lib.orc_emit_pairs.argtypes = [vp, _f32p, _u32p, _u32p, _u32p, ctypes.c_uint32, ctypes.c_uint32,
+                                       ctypes.c_uint32, _u64p, _u32p]
+        lib.orc_emit_pairs.restype = None
+        lib.orc_sort_pairs.argtypes = [_u64p, _u32p, ctypes.c_uint64, ctypes.c_uint32]
+        lib.orc_sort_pairs.restype = None
+        lib.orc_ranges.argtypes = [_u64p, ctypes.c_uint64, ctypes.c_uint32, _u32p]
+        lib.orc_ranges.restype = None
+        lib.orc_blend.argtypes = [vp, _f32p, _u32p, _u32p, ctypes.c_uint32, _f32p, _f32p, _f32p, _u32p]
+        lib.orc_blend.restype = None
+        lib.orc_blend_f64.argtypes = [vp, _f32p, _u32p, _u32p, _u32p, _u32p, ctypes.c_uint32, _f64p, _f64p, _f64p,
+                                      _f64p, _f64p, ctypes.c_uint32, ctypes.c_uint32, _f64p, _f64p]
+        lib.orc_blend_f64.restype = None
+        lib.orc_reference.argtypes = [vp, _f32p, _u32p, ctypes.c_uint32, ctypes.c_uint32, _f32p, _f32p]
+        lib.orc_reference.restype = ctypes.c_uint64
+        lib.orc_render_ref_order.argtypes = [vp, _f32p, _f32p, _f32p, _f32p, _f32p, ctypes.c_uint32, ctypes.c_uint32,
+                                             _f32p, _u32p, _u32p, _u32p, ctypes.c_uint64, _u32p, ctypes.c_void_p,
+                                             ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+        lib.orc_render_ref_order.restype = ctypes.c_uint64
+        lib.orc_threads.restype = ctypes.c_int
+        lib.orc_set_threads.argtypes = [ctypes.c_int]
+        lib.orc_expf.argtypes = [ctypes.c_float]
+        lib.orc_expf.restype = ctypes.c_float
+        lib.orc_logf.argtypes = [ctypes.c_float]
+        lib.orc_logf.restype = ctypes.c_float
+        _lib = lib
+    return _lib
+
+
+def orc_view(view, tile: int = 16, antialias: bool = True, linear: bool = True, sh_limit: int = 3,
+             depth_mode: int = 0, depth_threshold: float = 0.5) -> OrcView:
+    """The oracle's frame from an athx View (athenea_amd._lib.View) and settings."""
+    v = OrcView()
+    v.width, v.height = view.width, view.height
+    v.focal_x, v.focal_y = view.focal_x, view.focal_y
+    v.centre_x, v.centre_y = view.centre_x, view.centre_y
+    v.near_z, v.far_z = view.near_z, view.far_z
+    v.orthographic = view.orthographic
+    v.antialias = 1 if antialias else 0
+    v.linear = 1 if linear else 0
+    v.sh_limit = sh_limit
+    v.tile = tile
+    for k in range(12):
+        v.m[k] = view.object_to_view[k]
+    for k in range(3):
+        v.eye[k] = view.eye[k]
+    for k in range(4):
+        v.bg[k] = view.background[k]
+    v.depth_mode = depth_mode
+    v.depth_threshold = depth_threshold
+    return v
+
+
+def _c(a, dtype):
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+def tiles_of(v: OrcView) -> tuple[int, int]:
+    return (v.width + v.tile - 1) // v.tile, (v.height + v.tile - 1) // v.tile
+
+
+def bits_for(values: int) -> int:
+    bits = 1
+    while bits < 32 and (1 << bits) < values:
+        bits += 1
+    return bits
+
+
+def project(v: OrcView, cloud, flags: bool = False):
+    lib = load()
+    n = cloud.means.shape[0]
+    coeffs = cloud.sh.shape[1]
+    rec = np.zeros((n, 12), np.float32)
+    touched = np.zeros(n, np.uint32)
+    keys = np.zeros(n, np.uint32)
+    fl = np.zeros(n, np.uint32) if flags else None
+    lib.orc_project(ctypes.byref(v), _c(cloud.means, np.float32), _c(cloud.log_scales, np.float32),
+                    _c(cloud.quats, np.float32), _c(cloud.opacity_logits, np.float32), _c(cloud.sh, np.float32),
+                    coeffs, n, rec, touched, keys, fl.ctypes.data if flags else None)
+    return (rec, touched, keys, fl) if flags else (rec, touched, keys)
+
+
+def exclusive_scan(values: np.ndarray) -> tuple[np.ndarray, int]:
+    values = _c(values, np.uint32)
+    out = np.zeros_like(values)
+    total = load().orc_exclusive_scan(values, out, values.size)
+    return out, int(total)
+
+
+def sort_pairs(keys: np.ndarray, values: np.ndarray, bits: int) -> tuple[np.ndarray, np.ndarray]:
+    k = _c(keys, np.uint64).copy()
+    v = _c(values, np.uint32).copy()
+    load().orc_sort_pairs(k, v, k.size, bits)
+    return k, v
+
+
+def render(views: list[OrcView], cloud, with_f64: bool = False) -> dict:
+    """The frame as the HIP path computes it, all views in one key space."""
+    lib = load()
+    n = cloud.means.shape[0]
+    nv = len(views)
+    rec = np.zeros((nv * n, 12), np.float32)
+    touched = np.zeros(nv * n, np.uint32)
+    keys = np.zeros(nv * n, np.uint32)
+    flags = np.zeros(nv * n, np.uint32)
+    tile_bases = []
+    total_tiles = 0
+    for k, v in enumerate(views):
+        r, t, kk, fl = project(v, cloud, flags=True)
+        rec[k * n:(k + 1) * n] = r
+        touched[k * n:(k + 1) * n] = t
+        keys[k * n:(k + 1) * n] = kk
+        flags[k * n:(k + 1) * n] = fl
+        tile_bases.append(total_tiles)
+        tx, ty = tiles_of(v)
+        total_tiles += tx * ty
+    offsets, pairs = exclusive_scan(touched)
+    pk = np.zeros(pairs, np.uint64)
+    pv = np.zeros(pairs, np.uint32)
+    for k, v in enumerate(views):
+        sl = slice(k * n, (k + 1) * n)
+        lib.orc_emit_pairs(ctypes.byref(v), _c(rec[sl], np.float32), _c(touched[sl], np.uint32),
+                           _c(keys[sl], np.uint32), _c(offsets[sl], np.uint32), n, tile_bases[k], k * n, pk, pv)
+    key_bits = 24 + bits_for(total_tiles)
+    lib.orc_sort_pairs(pk, pv, pairs, key_bits)
+    ranges = np.zeros(total_tiles * 2, np.uint32)
+    lib.orc_ranges(pk, pairs, total_tiles, ranges)
+    out = {"records": rec, "touched": touched, "depth_keys": keys, "flags": flags, "offsets": offsets,
+           "pairs": pairs, "pair_keys": pk, "pair_values": pv, "ranges": ranges, "tile_bases": tile_bases,
+           "key_bits": key_bits, "colour": [], "depth": [], "final_t": [], "last": []}
+    for k, v in enumerate(views):
+        H, W = v.height, v.width
+        col = np.zeros(H * W * 4, np.float32)
+        dep = np.zeros(H * W, np.float32)
+        ft = np.zeros(H * W, np.float32)
+        last = np.zeros(H * W, np.uint32)
+        lib.orc_blend(ctypes.byref(v), rec, ranges, pv, tile_bases[k], col, dep, ft, last)
+        out["colour"].append(col.reshape(H, W, 4))
+        out["depth"].append(dep.reshape(H, W))
+        out["final_t"].append(ft.reshape(H, W))
+        out["last"].append(last.reshape(H, W))
+    if with_f64:
+        out.update(render_f64(views, cloud, out))
+    return out
+
+
+def render_f64(views: list[OrcView], cloud, frame: dict) -> dict:
+    """FP64 images of a frame, every FP32 decision replayed (see orc_blend_f64)."""
+    lib = load()
+    n = cloud.means.shape[0]
+    coeffs = cloud.sh.shape[1]
+    cols, deps = [], []
+    for k, v in enumerate(views):
+        sl = slice(k * n, (k + 1) * n)
+        H, W = v.height, v.width
+        col = np.zeros(H * W * 4, np.float64)
+        dep = np.zeros(H * W, np.float64)
+        # Slots of view k are k * n + i: hand the oracle this view's slice
+        # with pair values rebased to it.
+        pv = frame["pair_values"].astype(np.int64) - k * n
+        tb = frame["tile_bases"][k]
+        tx, ty = tiles_of(v)
+        rg = frame["ranges"][2 * tb:2 * (tb + tx * ty)].copy()
+        pv_view = np.where((pv >= 0) & (pv < n), pv, 0).astype(np.uint32)
+        lib.orc_blend_f64(ctypes.byref(v), _c(frame["records"][sl], np.float32), _c(rg, np.uint32), pv_view,
+                          _c(frame["touched"][sl], np.uint32), _c(frame["flags"][sl], np.uint32), 0,
+                          _c(cloud.means, np.float64), _c(cloud.log_scales, np.float64),
+                          _c(cloud.quats, np.float64), _c(cloud.opacity_logits, np.float64),
+                          _c(cloud.sh, np.float64), coeffs, n, col, dep)
+        cols.append(col.reshape(H, W, 4))
+        deps.append(dep.reshape(H, W))
+    return {"colour64": cols, "depth64": deps}
+
+
+def render_ref_order(v: OrcView, cloud, pair_cap: int | None = None) -> dict:
+    """The reference's two-sort order on one view: returns pair values in
+    sorted order, ranges, P and the stage times (ms)."""
+    lib = load()
+    n = cloud.means.shape[0]
+    coeffs = cloud.sh.shape[1]
+    rec = np.zeros((n, 12), np.float32)
+    touched = np.zeros(n, np.uint32)
+    keys = np.zeros(n, np.uint32)
+    tx, ty = tiles_of(v)
+    ranges = np.zeros(tx * ty * 2, np.uint32)
+    if pair_cap is None:
+        r, t, _ = project(v, cloud)
+        pair_cap = int(t.astype(np.uint64).sum())
+    pv = np.zeros(max(pair_cap, 1), np.uint32)
+    H, W = v.height, v.width
+    col = np.zeros(H * W * 4, np.float32)
+    dep = np.zeros(H * W, np.float32)
+    ft = np.zeros(H * W, np.float32)
+    last = np.zeros(H * W, np.uint32)
+    ms = np.zeros(7, np.float64)
+    P = lib.orc_render_ref_order(ctypes.byref(v), _c(cloud.means, np.float32), _c(cloud.log_scales, np.float32),
+                                 _c(cloud.quats, np.float32), _c(cloud.opacity_logits, np.float32),
+                                 _c(cloud.sh, np.float32), coeffs, n, rec, touched, keys, pv, pair_cap, ranges,
+                                 col.ctypes.data, dep.ctypes.data, ft.ctypes.data, last.ctypes.data, ms.ctypes.data)
+    return {"pairs": int(P), "pair_values": pv[:P], "ranges": ranges, "colour": col.reshape(H, W, 4),
+            "depth": dep.reshape(H, W), "ms": ms, "records": rec, "touched": touched}
+
+
+def reference_brute_force(v: OrcView, records: np.ndarray, touched: np.ndarray, max_hits: int = 1024):
+    """reference_blend.slang's per-pixel brute force over every kept slot."""
+    H, W = v.height, v.width
+    col = np.zeros(H * W * 4, np.float32)
+    dep = np.zeros(H * W, np.float32)
+    over = load().orc_reference(ctypes.byref(v), _c(records, np.float32), _c(touched, np.uint32),
+                                touched.size, max_hits, col, dep)
+    return col.reshape(H, W, 4), dep.reshape(H, W), int(over)
+
+
+def threads() -> int:
+    return int(load().orc_threads())
+
+
+def set_threads(n: int) -> None:
+    load().orc_set_threads(n)
+
+
+def expf(x: float) -> float:
+    return float(load().orc_expf(x))
+
+
+def logf(x: float) -> float:
+    return float(load().orc_logf(x))
