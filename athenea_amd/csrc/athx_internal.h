@@ -14,10 +14,19 @@ namespace athx {
 // The splat rasterizer's constants (shaders/athenea/splat/frame.slang:42-49).
 constexpr uint32_t kCulledKey = 0xFF000000u;
 // Floats in a projected record: screen (x, y, z, cutoff), conic (a, b, c,
-// alpha), colour (r, g, b, 0) -- frame.slang:269-281's ProjRecord without the
-// surfel words, which a plain Gaussian leaves zero.
+// alpha), colour (r, g, b, branch bits) -- frame.slang:269-281's ProjRecord
+// without the surfel words, which a plain Gaussian leaves zero. The last word
+// holds, as uint bits, the branches the projection took (kFlag*), which the
+// adjoint replays.
 constexpr uint32_t kRecordFloats = 12;
 constexpr uint32_t kDepthBits = 24;
+constexpr uint32_t kFlagClampXLo = 1u;   // Jacobian point clamped to the frustum's left
+constexpr uint32_t kFlagClampXHi = 2u;   // ... right
+constexpr uint32_t kFlagClampYLo = 4u;   // ... bottom
+constexpr uint32_t kFlagClampYHi = 8u;   // ... top
+constexpr uint32_t kFlagPaidBack = 16u;  // paidOpacity's pay-back branch (frame.slang:316-319)
+constexpr uint32_t kFlagColourR = 32u;   // channel's harmonics sum > 0 (R; G and B the next bits)
+constexpr uint32_t kFlagSrgbR = 256u;    // sRGB decode took the power branch (R; G, B next)
 
 // One view as every kernel reads it: the camera (athx_view), where its tiles
 // and slots start in the frame's global numbering, and its targets.
@@ -56,15 +65,19 @@ struct ProjectArgs {
 __host__ __device__ inline uint32_t div_up(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
 
 // Launchers (each returns the hipError_t of its launches).
+// Splats whose footprint spans many tiles are listed in big_list (capacity:
+// one word a slot; big_count one word) and counted a wavefront each.
 hipError_t launch_project(const ProjectArgs& a, const DView* views_dev, uint32_t nviews, float* records,
-                          uint32_t* touched, uint32_t* depth_keys, hipStream_t s);
+                          uint32_t* touched, uint32_t* depth_keys, uint32_t* big_list, uint32_t* big_count,
+                          hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t value, uint64_t n, hipStream_t s);
 size_t scan_status_words(uint32_t n);
 hipError_t launch_scan(const uint32_t* in, uint32_t* out, uint64_t* total, uint32_t n, uint32_t* status,
                        hipStream_t s);
-hipError_t launch_emit(const DView* views_dev, uint32_t nviews, uint32_t n, uint32_t tile,
-                       const float* records, const uint32_t* touched, const uint32_t* depth_keys,
-                       const uint32_t* offsets, uint64_t* keys, uint32_t* values, hipStream_t s);
+hipError_t launch_emit(const DView* views_dev, uint32_t nviews, uint32_t n, uint32_t tile, const float* records,
+                       const uint32_t* touched, const uint32_t* depth_keys, const uint32_t* offsets,
+                       const uint32_t* big_list, const uint32_t* big_count, uint64_t* keys, uint32_t* values,
+                       hipStream_t s);
 hipError_t launch_ranges(const uint64_t* sorted_keys, uint32_t pairs, uint32_t tiles, uint32_t* ranges,
                          hipStream_t s);
 hipError_t launch_blend(const DView* views_dev, const uint32_t* view_of_tile_base, uint32_t nviews,
@@ -75,8 +88,8 @@ hipError_t launch_blend_backward(const DView* views_dev, const uint32_t* tile_ba
                                  uint32_t total_tiles, uint32_t tile, uint32_t batch, const uint32_t* ranges,
                                  const uint32_t* pair_slots, const float* records, float* grad2d, hipStream_t s);
 hipError_t launch_project_backward(const ProjectArgs& a, const DView* views_dev, uint32_t nviews,
-                                   const uint32_t* touched, const float* grad2d, const athx_grads& g,
-                                   hipStream_t s);
+                                   const uint32_t* touched, const float* records, const float* grad2d,
+                                   const athx_grads& g, hipStream_t s);
 hipError_t launch_counters(const uint32_t* depth_keys, const uint32_t* touched, uint64_t slots,
                            uint32_t* words, hipStream_t s);
 

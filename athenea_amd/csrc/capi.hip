@@ -47,7 +47,7 @@ struct athx_ctx {
     bool exact_growth = false;
 
     DeviceBuffer views, tile_bases, records, touched, depth_keys, offsets, scan_status, totals;
-    DeviceBuffer pair_keys, pair_vals, alt_keys, alt_vals, sort_meta, ranges, grad2d, counter_words;
+    DeviceBuffer pair_keys, pair_vals, alt_keys, alt_vals, sort_meta, ranges, grad2d, counter_words, big_list;
     DeviceBuffer api_sort_meta, api_alt_keys, api_alt_vals, api_scan_status;
     uint64_t* host_totals = nullptr;  // pinned: [0] pairs, [1] visible
     DView* host_views = nullptr;      // pinned staging for the views
@@ -345,7 +345,7 @@ ATHX_API void athx_destroy(athx_ctx* ctx) {
                             &ctx->offsets, &ctx->scan_status, &ctx->totals, &ctx->pair_keys, &ctx->pair_vals,
                             &ctx->alt_keys, &ctx->alt_vals, &ctx->sort_meta, &ctx->ranges, &ctx->grad2d,
                             &ctx->counter_words, &ctx->api_sort_meta, &ctx->api_alt_keys, &ctx->api_alt_vals,
-                            &ctx->api_scan_status}) {
+                            &ctx->api_scan_status, &ctx->big_list}) {
         release(*b);
     }
     if (ctx->host_totals) {
@@ -377,7 +377,7 @@ ATHX_API athx_status athx_release_scratch(athx_ctx* ctx) {
     (void)hipSetDevice(ctx->device);
     ATHX_HIP(ctx, hipDeviceSynchronize(), "synchronize before release");
     for (DeviceBuffer* b : {&ctx->records, &ctx->touched, &ctx->depth_keys, &ctx->offsets, &ctx->scan_status,
-                            &ctx->pair_keys, &ctx->pair_vals, &ctx->alt_keys, &ctx->alt_vals, &ctx->sort_meta,
+                            &ctx->big_list, &ctx->pair_keys, &ctx->pair_vals, &ctx->alt_keys, &ctx->alt_vals, &ctx->sort_meta,
                             &ctx->ranges, &ctx->grad2d, &ctx->api_sort_meta, &ctx->api_alt_keys,
                             &ctx->api_alt_vals, &ctx->api_scan_status}) {
         release(*b);
@@ -396,7 +396,7 @@ ATHX_API uint64_t athx_scratch_bytes(const athx_ctx* ctx) {
                                   &ctx->offsets, &ctx->scan_status, &ctx->totals, &ctx->pair_keys, &ctx->pair_vals,
                                   &ctx->alt_keys, &ctx->alt_vals, &ctx->sort_meta, &ctx->ranges, &ctx->grad2d,
                                   &ctx->counter_words, &ctx->api_sort_meta, &ctx->api_alt_keys, &ctx->api_alt_vals,
-                                  &ctx->api_scan_status}) {
+                                  &ctx->api_scan_status, &ctx->big_list}) {
         sum += b->bytes;
     }
     return sum;
@@ -442,15 +442,17 @@ ATHX_API athx_status athx_render(athx_ctx* ctx, const athx_view* views, uint32_t
     ATHX_HIP(ctx, reserve(ctx, ctx->depth_keys, (size_t)slots * 4 + 16), "depth keys");
     ATHX_HIP(ctx, reserve(ctx, ctx->offsets, (size_t)slots * 4 + 16), "offsets");
     ATHX_HIP(ctx, reserve(ctx, ctx->scan_status, scan_status_words(slots) * 4), "scan status");
-    ATHX_HIP(ctx, reserve(ctx, ctx->totals, 16), "totals");
+    ATHX_HIP(ctx, reserve(ctx, ctx->totals, 32), "totals");
+    ATHX_HIP(ctx, reserve(ctx, ctx->big_list, (size_t)slots * 4 + 16), "big footprints");
     ATHX_HIP(ctx, reserve(ctx, ctx->ranges, (size_t)tiles * 8 + 16), "ranges");
 
     const ProjectArgs pa = project_args(gaussians, settings);
     if (timed) {
         ATHX_HIP(ctx, hipEventRecord(ctx->ev[0], s), "event");
     }
+    uint32_t* big_count = as<uint32_t>(ctx->totals) + 4;  // totals: [0..1] pair total, [4] big footprints
     ATHX_HIP(ctx, launch_project(pa, dviews, nviews, as<float>(ctx->records), as<uint32_t>(ctx->touched),
-                                 as<uint32_t>(ctx->depth_keys), s),
+                                 as<uint32_t>(ctx->depth_keys), as<uint32_t>(ctx->big_list), big_count, s),
              "project");
     if (timed) {
         ATHX_HIP(ctx, hipEventRecord(ctx->ev[1], s), "event");
@@ -479,7 +481,8 @@ ATHX_API athx_status athx_render(athx_ctx* ctx, const athx_view* views, uint32_t
     ATHX_HIP(ctx, reserve(ctx, ctx->sort_meta, sort_meta_words(pairs, passes) * 4 + 16), "sort metadata");
     ATHX_HIP(ctx, launch_emit(dviews, nviews, gaussians->count, settings->tile, as<float>(ctx->records),
                               as<uint32_t>(ctx->touched), as<uint32_t>(ctx->depth_keys), as<uint32_t>(ctx->offsets),
-                              as<uint64_t>(ctx->pair_keys), as<uint32_t>(ctx->pair_vals), s),
+                              as<uint32_t>(ctx->big_list), big_count, as<uint64_t>(ctx->pair_keys),
+                              as<uint32_t>(ctx->pair_vals), s),
              "emit");
     if (timed) {
         ATHX_HIP(ctx, hipEventRecord(ctx->ev[3], s), "event");
@@ -589,7 +592,7 @@ ATHX_API athx_status athx_render_backward(athx_ctx* ctx, const athx_view* views,
              "blend backward");
     const ProjectArgs pa = project_args(gaussians, settings);
     ATHX_HIP(ctx, launch_project_backward(pa, as<DView>(ctx->views), nviews, as<uint32_t>(ctx->touched),
-                                          as<float>(ctx->grad2d), *grads, s),
+                                          as<float>(ctx->records), as<float>(ctx->grad2d), *grads, s),
              "projection backward");
     return ATHX_OK;
 } catch (const std::bad_alloc&) {

@@ -22,6 +22,10 @@ constexpr float kC3_0 = -0.5900435899266435f, kC3_1 = 2.890611442640554f, kC3_2 
                 kC3_3 = 0.3731763325901154f, kC3_4 = -0.4570457994644658f, kC3_5 = 1.445305721320277f,
                 kC3_6 = -0.5900435899266435f;
 
+// Footprints whose tile rectangle holds more tiles than this are counted and
+// emitted a wavefront per splat rather than a thread per splat.
+constexpr uint32_t kBigRect = 64;
+
 __device__ inline float fmin_(float a, float b) { return a < b ? a : b; }
 __device__ inline float fmax_(float a, float b) { return a > b ? a : b; }
 __device__ inline float clamp_(float x, float lo, float hi) { return fmin_(fmax_(x, lo), hi); }
@@ -134,7 +138,8 @@ __device__ inline void cull(uint32_t slot, uint32_t reason, uint32_t* touched, u
 template <bool kHalf>
 __global__ void __launch_bounds__(256) project_kernel(ProjectArgs a, const DView* __restrict__ views, uint32_t nviews,
                                                       float* __restrict__ records, uint32_t* __restrict__ touched,
-                                                      uint32_t* __restrict__ keys) {
+                                                      uint32_t* __restrict__ keys, uint32_t* __restrict__ big_list,
+                                                      uint32_t* __restrict__ big_count) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.n) {
         return;
@@ -176,6 +181,7 @@ __global__ void __launch_bounds__(256) project_kernel(ProjectArgs a, const DView
             continue;
         }
         float jx0, jx2, jy1, jy2, cx, cy;
+        uint32_t flags = 0;  // the branches taken, kept for the adjoint (kFlag*)
         if (V.orthographic) {
             jx0 = V.focal_x; jx2 = 0.0f;
             jy1 = V.focal_y; jy2 = 0.0f;
@@ -191,8 +197,11 @@ __global__ void __launch_bounds__(256) project_kernel(ProjectArgs a, const DView
             const float limhix = (1.15f * W - V.centre_x) / V.focal_x;
             const float limloy = (-0.15f * H - V.centre_y) / V.focal_y;
             const float limhiy = (1.15f * H - V.centre_y) / V.focal_y;
-            const float txc = clamp_(tx * invz, limlox, limhix) * tz;
-            const float tyc = clamp_(ty * invz, limloy, limhiy) * tz;
+            const float ux = tx * invz, uy = ty * invz;
+            flags |= (ux < limlox ? kFlagClampXLo : 0u) | (ux > limhix ? kFlagClampXHi : 0u) |
+                     (uy < limloy ? kFlagClampYLo : 0u) | (uy > limhiy ? kFlagClampYHi : 0u);
+            const float txc = clamp_(ux, limlox, limhix) * tz;
+            const float tyc = clamp_(uy, limloy, limhiy) * tz;
             jx0 = V.focal_x * invz;
             jx2 = -V.focal_x * txc * invz * invz;
             jy1 = V.focal_y * invz;
@@ -227,6 +236,7 @@ __global__ void __launch_bounds__(256) project_kernel(ProjectArgs a, const DView
         const float paid = a.antialias ? filtered : 1.0f;
         const float spread = paid * opacity;
         const float alpha = spread >= 1.0f / 255.0f ? spread + (1.0f - paid) * (1.0f / 255.0f) : spread;
+        flags |= spread >= 1.0f / 255.0f ? kFlagPaidBack : 0u;
         if (!(alpha >= 1.0f / 255.0f)) {
             cull(slot, ATHX_CULL_FAINT, touched, keys);
             continue;
@@ -239,15 +249,24 @@ __global__ void __launch_bounds__(256) project_kernel(ProjectArgs a, const DView
             cull(slot, ATHX_CULL_OFFSCREEN, touched, keys);
             continue;
         }
+        // The exact count, with the test the emission repeats
+        // (splat_project.slang:764-776). A footprint over kBigRect tiles is
+        // counted by a whole wavefront afterwards (count_big_kernel): one
+        // thread walking thousands of tiles would hold its wave back.
         uint32_t count = 0;
-        for (uint32_t ty_ = y0; ty_ <= y1; ++ty_) {
-            for (uint32_t tx_ = x0; tx_ <= x1; ++tx_) {
-                count += tile_touched(V.width, V.height, a.tile, cx, cy, ka, kb, kc, cutoff, tx_, ty_) ? 1u : 0u;
+        const bool big = (x1 - x0 + 1) * (y1 - y0 + 1) > kBigRect;
+        if (big) {
+            big_list[atomicAdd(big_count, 1u)] = slot;
+        } else {
+            for (uint32_t ty_ = y0; ty_ <= y1; ++ty_) {
+                for (uint32_t tx_ = x0; tx_ <= x1; ++tx_) {
+                    count += tile_touched(V.width, V.height, a.tile, cx, cy, ka, kb, kc, cutoff, tx_, ty_) ? 1u : 0u;
+                }
             }
-        }
-        if (count == 0) {
-            cull(slot, ATHX_CULL_NO_TILE, touched, keys);
-            continue;
+            if (count == 0) {
+                cull(slot, ATHX_CULL_NO_TILE, touched, keys);
+                continue;
+            }
         }
         // Colour from this eye, in the cloud's space (splat_project.slang:781-785).
         const float dx = px - V.eye[0], dy = py - V.eye[1], dz = pz - V.eye[2];
@@ -260,11 +279,13 @@ __global__ void __launch_bounds__(256) project_kernel(ProjectArgs a, const DView
             const float base = 0.5f + kSH0 * sh_at<kHalf>(a.sh, a.sh_half, row + ch);
             const float cv = base + sh_rest<kHalf>(a.sh, a.sh_half, row + ch, a.sh_degree, ex, ey, ez);
             col[ch] = a.linear ? fmax_(cv, 0.0f) : srgb_to_linear(cv);
+            flags |= (cv > 0.0f ? kFlagColourR : 0u) << ch;
+            flags |= (!a.linear && fmax_(cv, 0.0f) > 0.04045f ? kFlagSrgbR : 0u) << ch;
         }
         float4* rec = reinterpret_cast<float4*>(records + (size_t)slot * kRecordFloats);
         rec[0] = make_float4(cx, cy, tz, cutoff);
         rec[1] = make_float4(ka, kb, kc, alpha);
-        rec[2] = make_float4(col[0], col[1], col[2], 0.0f);
+        rec[2] = make_float4(col[0], col[1], col[2], __builtin_bit_cast(float, flags));
         touched[slot] = count;
         keys[slot] = __builtin_bit_cast(uint32_t, tz) >> 7;
     }
@@ -299,6 +320,9 @@ __global__ void __launch_bounds__(256) emit_kernel(const DView* __restrict__ vie
     const float4 conic = rec[1];
     uint32_t x0, y0, x1, y1;
     tile_rect(V.width, V.height, T, screen.x, screen.y, conic.x, conic.y, conic.z, screen.w, x0, y0, x1, y1);
+    if ((x1 - x0 + 1) * (y1 - y0 + 1) > kBigRect) {
+        return;  // emit_big_kernel's
+    }
     const uint64_t depth = depth_keys[slot] & 0xFFFFFFu;
     uint32_t at = offsets[slot];
     const uint32_t stop = at + expected;
@@ -310,6 +334,102 @@ __global__ void __launch_bounds__(256) emit_kernel(const DView* __restrict__ vie
                 values[at] = (uint32_t)slot;
                 ++at;
             }
+        }
+    }
+}
+
+// A wavefront per big footprint: lane k tests tile k of each run of 64 in
+// the rectangle's row-major order, and a ballot gives each hit its place,
+// so the count is the serial loop's and the pairs come out in its order.
+constexpr uint32_t kBigThreads = 256;
+constexpr uint32_t kBigBlocks = 2048;
+
+struct BigFootprint {
+    const DView* V;
+    float4 screen, conic;
+    uint32_t x0, y0, w, area;
+};
+
+__device__ inline BigFootprint big_footprint(const DView* views, uint32_t n, uint32_t T, const float* records,
+                                             uint32_t slot) {
+    BigFootprint f;
+    f.V = &views[slot / n];
+    const float4* rec = reinterpret_cast<const float4*>(records + (size_t)slot * kRecordFloats);
+    f.screen = rec[0];
+    f.conic = rec[1];
+    uint32_t x1, y1;
+    tile_rect(f.V->width, f.V->height, T, f.screen.x, f.screen.y, f.conic.x, f.conic.y, f.conic.z, f.screen.w, f.x0,
+              f.y0, x1, y1);
+    f.w = x1 - f.x0 + 1;
+    f.area = f.w * (y1 - f.y0 + 1);
+    return f;
+}
+
+__device__ inline bool big_hit(const BigFootprint& f, uint32_t T, uint32_t k) {
+    return k < f.area && tile_touched(f.V->width, f.V->height, T, f.screen.x, f.screen.y, f.conic.x, f.conic.y,
+                                      f.conic.z, f.screen.w, f.x0 + k % f.w, f.y0 + k / f.w);
+}
+
+__global__ void __launch_bounds__(kBigThreads) count_big_kernel(const DView* __restrict__ views, uint32_t n,
+                                                                uint32_t T, const float* __restrict__ records,
+                                                                const uint32_t* __restrict__ big_list,
+                                                                const uint32_t* __restrict__ big_count,
+                                                                uint32_t* __restrict__ touched,
+                                                                uint32_t* __restrict__ keys) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t waves = gridDim.x * (kBigThreads / 64);
+    const uint32_t total = *big_count;
+    for (uint32_t b = blockIdx.x * (kBigThreads / 64) + (threadIdx.x >> 6); b < total; b += waves) {
+        const uint32_t slot = big_list[b];
+        const BigFootprint f = big_footprint(views, n, T, records, slot);
+        uint32_t count = 0;
+        for (uint32_t k0 = 0; k0 < f.area; k0 += 64) {
+            count += (uint32_t)__popcll(__ballot(big_hit(f, T, k0 + lane)));
+        }
+        if (lane == 0) {
+            touched[slot] = count;
+            if (count == 0) {
+                keys[slot] = kCulledKey | ATHX_CULL_NO_TILE;
+            }
+        }
+    }
+}
+
+__global__ void __launch_bounds__(kBigThreads) emit_big_kernel(const DView* __restrict__ views, uint32_t n,
+                                                               uint32_t T, const float* __restrict__ records,
+                                                               const uint32_t* __restrict__ big_list,
+                                                               const uint32_t* __restrict__ big_count,
+                                                               const uint32_t* __restrict__ touched,
+                                                               const uint32_t* __restrict__ depth_keys,
+                                                               const uint32_t* __restrict__ offsets,
+                                                               uint64_t* __restrict__ keys,
+                                                               uint32_t* __restrict__ values) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t lt_mask = (1ull << lane) - 1ull;
+    const uint32_t waves = gridDim.x * (kBigThreads / 64);
+    const uint32_t total = *big_count;
+    for (uint32_t b = blockIdx.x * (kBigThreads / 64) + (threadIdx.x >> 6); b < total; b += waves) {
+        const uint32_t slot = big_list[b];
+        const uint32_t expected = touched[slot];
+        if (expected == 0) {
+            continue;
+        }
+        const BigFootprint f = big_footprint(views, n, T, records, slot);
+        const uint64_t depth = depth_keys[slot] & 0xFFFFFFu;
+        uint32_t at = offsets[slot];
+        const uint32_t stop = at + expected;
+        for (uint32_t k0 = 0; k0 < f.area; k0 += 64) {
+            const uint32_t k = k0 + lane;
+            const bool hit = big_hit(f, T, k);
+            const uint64_t mask = __ballot(hit);
+            const uint32_t mine = at + (uint32_t)__popcll(mask & lt_mask);
+            if (hit && mine < stop) {
+                const uint64_t tile = (uint64_t)f.V->tile_base + (uint64_t)(f.y0 + k / f.w) * f.V->tiles_x +
+                                      (f.x0 + k % f.w);
+                keys[mine] = (tile << kDepthBits) | depth;
+                values[mine] = slot;
+            }
+            at += (uint32_t)__popcll(mask);
         }
     }
 }
@@ -384,18 +504,22 @@ __global__ void __launch_bounds__(256) counters_kernel(const uint32_t* __restric
 }  // namespace
 
 hipError_t launch_project(const ProjectArgs& a, const DView* views_dev, uint32_t nviews, float* records,
-                          uint32_t* touched, uint32_t* depth_keys, hipStream_t s) {
+                          uint32_t* touched, uint32_t* depth_keys, uint32_t* big_list, uint32_t* big_count,
+                          hipStream_t s) {
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, s, big_count, 0u, (uint64_t)1);
     if (a.n == 0 || nviews == 0) {
-        return hipSuccess;
+        return hipGetLastError();
     }
     const dim3 grid(div_up(a.n, 256));
     if (a.sh_half) {
         hipLaunchKernelGGL(project_kernel<true>, grid, dim3(256), 0, s, a, views_dev, nviews, records, touched,
-                           depth_keys);
+                           depth_keys, big_list, big_count);
     } else {
         hipLaunchKernelGGL(project_kernel<false>, grid, dim3(256), 0, s, a, views_dev, nviews, records, touched,
-                           depth_keys);
+                           depth_keys, big_list, big_count);
     }
+    hipLaunchKernelGGL(count_big_kernel, dim3(kBigBlocks), dim3(kBigThreads), 0, s, views_dev, a.n, a.tile, records,
+                       big_list, big_count, touched, depth_keys);
     return hipGetLastError();
 }
 
@@ -412,14 +536,17 @@ hipError_t launch_fill_u32(uint32_t* p, uint32_t value, uint64_t n, hipStream_t 
 }
 
 hipError_t launch_emit(const DView* views_dev, uint32_t nviews, uint32_t n, uint32_t tile, const float* records,
-                       const uint32_t* touched, const uint32_t* depth_keys, const uint32_t* offsets, uint64_t* keys,
-                       uint32_t* values, hipStream_t s) {
+                       const uint32_t* touched, const uint32_t* depth_keys, const uint32_t* offsets,
+                       const uint32_t* big_list, const uint32_t* big_count, uint64_t* keys, uint32_t* values,
+                       hipStream_t s) {
     const uint64_t slots = (uint64_t)n * nviews;
     if (slots == 0) {
         return hipSuccess;
     }
     hipLaunchKernelGGL(emit_kernel, dim3((uint32_t)((slots + 255) / 256)), dim3(256), 0, s, views_dev, nviews, n,
                        tile, records, touched, depth_keys, offsets, keys, values);
+    hipLaunchKernelGGL(emit_big_kernel, dim3(kBigBlocks), dim3(kBigThreads), 0, s, views_dev, n, tile, records,
+                       big_list, big_count, touched, depth_keys, offsets, keys, values);
     return hipGetLastError();
 }
 

@@ -231,7 +231,10 @@ ATHX_API athx_status athx_latest_counters(athx_ctx* ctx, athx_counters* out);
  * other readers (trace, counters, NeuralFit) walk. Slot s of view k is
  * k * count + i. Layouts:
  *   RECORDS      float[slots][12]: x, y, view z, cut power, conic a, b, c,
- *                alpha, r, g, b, 0 (written for visible slots only)
+ *                alpha, r, g, b, and the projection's branch bits as a uint
+ *                (frustum clamp x lo/hi, y lo/hi = 1/2/4/8, opacity pay-back
+ *                16, colour channel > 0 = 32 << c, sRGB power branch = 256 << c),
+ *                written for visible slots only
  *   TOUCHED      uint32[slots]: tiles the footprint touches, 0 when culled
  *   DEPTH_KEYS   uint32[slots]: asuint(z) >> 7, or 0xFF000000 | cull reason
  *   OFFSETS      uint32[slots]: exclusive scan of TOUCHED

@@ -71,7 +71,8 @@ typedef struct orc_view {
 #define CULL_NO_TILE 6u
 
 /* Per-slot record layout: screen (x, y, z, cutoff), conic (a, b, c, alpha),
- * colour (r, g, b, 0) -- frame.slang:269-281 without the surfel words. */
+ * colour (r, g, b, branch bits FL_*) -- frame.slang:269-281 without the
+ * surfel words. */
 #define REC 12
 
 /* Branch flags the FP32 projection took, replayed by the FP64 one. */
@@ -347,7 +348,10 @@ static uint32_t project_one(const orc_view* v, const float* P, const float* LS, 
     float* r = rec + (size_t)i * REC;
     r[0] = cx; r[1] = cy; r[2] = tz; r[3] = cutoff;
     r[4] = ca; r[5] = cb; r[6] = cc; r[7] = alpha;
-    r[8] = col[0]; r[9] = col[1]; r[10] = col[2]; r[11] = 0.0f;
+    r[8] = col[0]; r[9] = col[1]; r[10] = col[2];
+    /* The branches taken, as uint bits in the record's last word (the kernels
+     * keep them there for the adjoint; same FL_* layout). */
+    memcpy(&r[11], &flags, sizeof flags);
     *touched = count;
     uint32_t zb;
     memcpy(&zb, &tz, 4);
@@ -877,3 +881,141 @@ ORC_EXPORT void orc_set_threads(int n) {
 
 ORC_EXPORT float orc_expf(float x) { return athx_expf(x); }
 ORC_EXPORT float orc_logf(float x) { return athx_logf(x); }
+
+/* ---------------------------------------------------------------------------
+ * FP64 central finite differences of the adjoint's product, for its check.
+ *
+ * L(theta) = sum over pixels and channels of G[px][c] * C[px][c](theta): the
+ * image paired with a fixed upstream gradient G (dL/dcolour), which is the
+ * vector-Jacobian product an adjoint computes. C is the FP64 image of
+ * orc_blend_f64 with every discrete decision (culls, the clamp of the
+ * Jacobian, the pay-back branch, max(c, 0), the cut, the per-pair skips, the
+ * 0.99 clamp and the early stop) taken from the FP32 frame and frozen, so L is
+ * a smooth function of the parameters and its central difference is what the
+ * adjoint must reproduce. `magnitudes` gets the sum over pixels of each
+ * pixel's share's magnitude, |G . dC/dtheta|: the size of the terms an adjoint
+ * sums, which bounds the rounding a float32 sum of them can carry. One view; rec, ranges, vals, touched and flags are
+ * that view's FP32 frame (slots 0..n-1, tiles from 0).
+ */
+static void walk_f64(const orc_view* v, const float* rec, const rec64_t* r64, const uint32_t* vals, uint32_t start,
+                     uint32_t end, uint32_t px, uint32_t py, uint32_t sub, const rec64_t* sub_rec, double out[4]) {
+    const float pxc = (float)px + 0.5f, pyc = (float)py + 0.5f;
+    float T32 = 1.0f;
+    double T = 1.0, rgb[3] = {0, 0, 0};
+    for (uint32_t j = start; j < end; ++j) {
+        const uint32_t s = vals[j];
+        const float* r = rec + (size_t)s * REC;
+        const float dx = pxc - r[0], dy = pyc - r[1];
+        const float power = r[4] * dx * dx + 2.0f * r[5] * dx * dy + r[6] * dy * dy;
+        if (power > r[3]) continue;
+        const float raw = r[7] * athx_expf(-0.5f * power);
+        const float alpha32 = minf_(0.99f, raw);
+        if (alpha32 < 1.0f / 255.0f) continue;
+        const rec64_t* q = s == sub ? sub_rec : &r64[s];
+        const double ddx = (px + 0.5) - q->cx, ddy = (py + 0.5) - q->cy;
+        const double pw = q->a * ddx * ddx + 2.0 * q->b * ddx * ddy + q->c * ddy * ddy;
+        const double alpha = raw > 0.99f ? 0.99 : q->alpha * exp(-0.5 * pw);
+        const double w = T * alpha;
+        rgb[0] += w * q->col[0];
+        rgb[1] += w * q->col[1];
+        rgb[2] += w * q->col[2];
+        T *= 1.0 - alpha;
+        T32 = T32 * (1.0f - alpha32);
+        if (T32 < 1e-4f) break;
+    }
+    out[0] = rgb[0] + T * v->bg[0];
+    out[1] = rgb[1] + T * v->bg[1];
+    out[2] = rgb[2] + T * v->bg[2];
+    out[3] = (1.0 - T) + T * v->bg[3];
+}
+
+/* Over the tiles in `tiles`: sum of G . (C+ - C-) with slot `sub` drawn from
+ * rp and then rm, and the sum of its per-pixel magnitudes |G . (C+ - C-)|. */
+static void tiles_diff(const orc_view* v, const float* rec, const rec64_t* r64, const uint32_t* ranges,
+                       const uint32_t* vals, const uint32_t* tiles, uint32_t ntiles, uint32_t sub, const rec64_t* rp,
+                       const rec64_t* rm, const double* upstream, double* sum, double* mag) {
+    const uint32_t TS = v->tile, W = v->width, H = v->height;
+    const uint32_t tiles_x = (W + TS - 1) / TS;
+    double acc = 0.0, acc_mag = 0.0;
+    for (uint32_t k = 0; k < ntiles; ++k) {
+        const uint32_t t = tiles[k];
+        const uint32_t tx = t % tiles_x, ty = t / tiles_x;
+        for (uint32_t py = ty * TS; py < (ty + 1) * TS && py < H; ++py) {
+            for (uint32_t px = tx * TS; px < (tx + 1) * TS && px < W; ++px) {
+                double cp[4], cm[4];
+                walk_f64(v, rec, r64, vals, ranges[2 * t], ranges[2 * t + 1], px, py, sub, rp, cp);
+                walk_f64(v, rec, r64, vals, ranges[2 * t], ranges[2 * t + 1], px, py, sub, rm, cm);
+                const double* g = upstream + ((size_t)py * W + px) * 4;
+                const double d = g[0] * (cp[0] - cm[0]) + g[1] * (cp[1] - cm[1]) + g[2] * (cp[2] - cm[2]) +
+                                 g[3] * (cp[3] - cm[3]);
+                acc += d;
+                acc_mag += fabs(d);
+            }
+        }
+    }
+    *sum = acc;
+    *mag = acc_mag;
+}
+
+ORC_EXPORT void orc_fd_grads(const orc_view* v, const float* rec, const uint32_t* ranges, const uint32_t* vals,
+                             uint32_t pairs, const uint32_t* touched, const uint32_t* flags, const double* means,
+                             const double* log_scales, const double* quats, const double* logits, const double* sh,
+                             uint32_t coeffs, uint32_t n, const double* upstream, const uint32_t* ids, uint32_t nids,
+                             double h_rel, double* grads, double* magnitudes) {
+    const uint32_t TS = v->tile;
+    const uint32_t tiles = ((v->width + TS - 1) / TS) * ((v->height + TS - 1) / TS);
+    const uint32_t per = 11 + coeffs * 3;
+    rec64_t* r64 = (rec64_t*)calloc(n ? n : 1, sizeof(rec64_t));
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int64_t i = 0; i < (int64_t)n; ++i) {
+        if (touched[i]) project_one_f64(v, means, log_scales, quats, logits, sh, coeffs, (uint32_t)i, flags[i], &r64[i]);
+    }
+    /* Which tile each pair belongs to: one pass over the ranges. */
+    uint32_t* tile_of_pair = (uint32_t*)malloc((pairs ? pairs : 1) * sizeof(uint32_t));
+    for (uint32_t t = 0; t < tiles; ++t) {
+        for (uint32_t j = ranges[2 * t]; j < ranges[2 * t + 1]; ++j) tile_of_pair[j] = t;
+    }
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int64_t q = 0; q < (int64_t)nids; ++q) {
+        const uint32_t g = ids[q];
+        double* out = grads + (size_t)q * per;
+        double* mag = magnitudes + (size_t)q * per;
+        for (uint32_t k = 0; k < per; ++k) out[k] = mag[k] = 0.0;
+        if (g >= n || !touched[g]) continue;
+        uint32_t* mine = (uint32_t*)malloc(touched[g] * sizeof(uint32_t));
+        uint32_t nt = 0;
+        for (uint32_t j = 0; j < pairs && nt < touched[g]; ++j) {
+            if (vals[j] == g) mine[nt++] = tile_of_pair[j];
+        }
+        double P[3], LS[3], Q[4], LG[1];
+        double* SH = (double*)malloc(coeffs * 3 * sizeof(double));
+        for (uint32_t k = 0; k < per; ++k) {
+            double* slot;
+            memcpy(P, means + (size_t)g * 3, sizeof P);
+            memcpy(LS, log_scales + (size_t)g * 3, sizeof LS);
+            memcpy(Q, quats + (size_t)g * 4, sizeof Q);
+            LG[0] = logits[g];
+            memcpy(SH, sh + (size_t)g * coeffs * 3, coeffs * 3 * sizeof(double));
+            if (k < 3) slot = &P[k];
+            else if (k < 6) slot = &LS[k - 3];
+            else if (k < 10) slot = &Q[k - 6];
+            else if (k < 11) slot = &LG[0];
+            else slot = &SH[k - 11];
+            const double x0 = *slot;
+            const double h = h_rel * (fabs(x0) > 1.0 ? fabs(x0) : 1.0);
+            rec64_t rp, rm;
+            *slot = x0 + h;
+            project_one_f64(v, P, LS, Q, LG, SH, coeffs, 0, flags[g], &rp);
+            *slot = x0 - h;
+            project_one_f64(v, P, LS, Q, LG, SH, coeffs, 0, flags[g], &rm);
+            double diff, diff_mag;
+            tiles_diff(v, rec, r64, ranges, vals, mine, nt, g, &rp, &rm, upstream, &diff, &diff_mag);
+            out[k] = diff / (2.0 * h);
+            mag[k] = diff_mag / (2.0 * h);
+        }
+        free(SH);
+        free(mine);
+    }
+    free(tile_of_pair);
+    free(r64);
+}

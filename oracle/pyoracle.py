@@ -296,3 +296,33 @@ def expf(x: float) -> float:
 
 def logf(x: float) -> float:
     return float(load().orc_logf(x))
+
+
+def fd_grads(v: OrcView, cloud, frame: dict, upstream: np.ndarray, ids, h_rel: float = 1e-4) -> np.ndarray:
+    """FP64 central differences of L = sum(upstream * C(theta)) -- the image
+    paired with a fixed dL/dcolour, the product an adjoint computes -- with
+    every FP32 decision of ``frame`` (one view, from ``render``) frozen.
+    Returns [len(ids), 11 + 3 * coeffs]: means (3), log-scales (3),
+    quaternion (4), opacity logit (1), SH (coeffs * 3, coefficient-major);
+    and, of the same shape, the sum over pixels of each pixel's share's
+    magnitude |upstream . dC/dtheta| (what a float32 sum of the shares rounds
+    against)."""
+    lib = load()
+    if not hasattr(lib, "_fd_ready"):
+        lib.orc_fd_grads.argtypes = [ctypes.POINTER(OrcView), _f32p, _u32p, _u32p, ctypes.c_uint32, _u32p, _u32p,
+                                     _f64p, _f64p, _f64p, _f64p, _f64p, ctypes.c_uint32, ctypes.c_uint32, _f64p,
+                                     _u32p, ctypes.c_uint32, ctypes.c_double, _f64p, _f64p]
+        lib.orc_fd_grads.restype = None
+        lib._fd_ready = True
+    n = cloud.means.shape[0]
+    coeffs = cloud.sh.shape[1]
+    ids = _c(ids, np.uint32)
+    out = np.zeros((ids.size, 11 + 3 * coeffs), np.float64)
+    mag = np.zeros_like(out)
+    lib.orc_fd_grads(ctypes.byref(v), _c(frame["records"][:n], np.float32), _c(frame["ranges"], np.uint32),
+                     _c(frame["pair_values"], np.uint32), int(frame["pairs"]), _c(frame["touched"][:n], np.uint32),
+                     _c(frame["flags"][:n], np.uint32), _c(cloud.means, np.float64),
+                     _c(cloud.log_scales, np.float64), _c(cloud.quats, np.float64),
+                     _c(cloud.opacity_logits, np.float64), _c(cloud.sh, np.float64), coeffs, n,
+                     _c(upstream, np.float64), ids, ids.size, h_rel, out, mag)
+    return out, mag

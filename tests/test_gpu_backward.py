@@ -1,0 +1,109 @@
+"""The adjoint against FP64 central finite differences, through the C ABI.
+
+The workload is the forward+backward one at a size the oracle differentiates
+in seconds: the first 2 000 Gaussians of the 1080p generator drawn at 64x64,
+loss L2 against a target rendered from the same Gaussians with means jittered
+N(0, 0.01) and SH N(0, 0.05). The adjoint computes the vector-Jacobian
+product of the image with the loss's dL/dcolour; the check differentiates
+that same product, sum(dL/dC * C(theta)) with dL/dC held fixed, by FP64
+central differences that freeze every FP32 decision of the frame (culls,
+clamps, cuts, skips, early stops) -- the decisions the GPU's walk took bit for
+bit -- so the two differentiate the same function.
+
+Tolerance, per parameter of every Gaussian:
+    |g - g_fd| <= 1e-3 * max(|g_fd|, 1e-6 * max|g_fd| over its group, 1e-2 * S)
+where S is the sum over pixels of the magnitudes of the shares the gradient
+adds up (|dL/dC . dC/dtheta| per pixel). A gradient that cancels to under a
+hundredth of its terms is judged against a hundredth of them: a float32 sum of
+n terms is only good to about n * 6e-8 of their magnitude, which is what the
+1e-6-of-the-maximum floor alone cannot see.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+from athenea_amd import RenderSettings, TileRasterizer, camera, synthetic
+from athenea_amd.autograd import rasterize
+from oracle import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+GRAD_TOL = 1e-3
+CANCEL_FLOOR = 1e-2
+GROUPS = {"means": slice(0, 3), "log_scales": slice(3, 6), "quats": slice(6, 10), "opacity_logits": slice(10, 11),
+          "sh": slice(11, None)}
+
+
+def rel_errors(g: np.ndarray, fd: np.ndarray, mag: np.ndarray) -> np.ndarray:
+    scale = np.maximum(np.maximum(np.abs(fd), 1e-6 * np.abs(fd).max()), CANCEL_FLOOR * mag)
+    return np.abs(g - fd) / np.where(scale > 0, scale, 1.0)
+
+
+def run_case(weights, linear=True, antialias=True, n=2000, size=64, seed_jitter=3):
+    print("backward case", weights, "linear", linear, "antialias", antialias, flush=True)
+    cloud = synthetic.plain_1080p_cloud(n)
+    view = camera.view_for(synthetic.front_camera(), size, size, background=(0.05, 0.1, 0.15, 0.3))
+    settings = RenderSettings(antialias=antialias)
+    ov = po.orc_view(view, antialias=antialias, linear=linear)
+    rng = np.random.default_rng(seed_jitter)
+    jit = synthetic.plain_1080p_cloud(n)
+    jit.means = (jit.means + rng.normal(0, 0.01, jit.means.shape)).astype(np.float32)
+    jit.sh = (jit.sh + rng.normal(0, 0.05, jit.sh.shape)).astype(np.float32)
+    target = po.render([ov], jit)["colour"][0].astype(np.float64)
+
+    raster = TileRasterizer(0)
+    params = [torch.from_numpy(a.copy()).cuda().requires_grad_(True)
+              for a in (cloud.means, cloud.log_scales, cloud.quats, cloud.opacity_logits, cloud.sh)]
+    colours, _ = rasterize(*params, [view], raster, settings, linear=linear)
+    w = torch.tensor(weights, dtype=torch.float32, device="cuda")
+    tgt = torch.from_numpy(target.astype(np.float32)).cuda()
+    loss = (w * (colours[0] - tgt) ** 2).sum()
+    loss.backward()
+    torch.cuda.synchronize()
+    grads = [p.grad.cpu().numpy().astype(np.float64).reshape(n, -1) for p in params]
+    g = np.concatenate(grads, axis=1)
+
+    frame = po.render([ov], cloud)
+    image = colours[0].detach().cpu().numpy()
+    assert np.array_equal(image, frame["colour"][0]), "forward differs"
+    # The upstream gradient the adjoint was handed, dL/dC at the FP32 image:
+    # the finite differences differentiate the image paired with it.
+    upstream = 2.0 * np.asarray(weights, np.float64) * (image.astype(np.float64) - target)
+    ids = np.nonzero(frame["touched"])[0]
+    fd, mag = po.fd_grads(ov, cloud, frame, upstream, ids)
+    raster.close()
+    return g[ids], fd, mag, g, ids
+
+
+def check(g, fd, mag):
+    report = {}
+    for name, sl in GROUPS.items():
+        err = rel_errors(g[:, sl], fd[:, sl], mag[:, sl])
+        report[name] = float(err.max())
+    print("max relative error by group:", report, flush=True)
+    for name, e in report.items():
+        assert e <= GRAD_TOL, (name, e, report)
+
+
+def test_rgb_loss_gradients_match_finite_differences():
+    g, fd, mag, full, ids = run_case([1.0, 1.0, 1.0, 0.0])
+    check(g, fd, mag)
+
+
+def test_alpha_channel_and_srgb_gradients():
+    g, fd, mag, full, ids = run_case([0.7, 1.0, 1.3, 0.5], linear=False, seed_jitter=5)
+    check(g, fd, mag)
+
+
+def test_no_antialias_gradients():
+    g, fd, mag, full, ids = run_case([1.0, 1.0, 1.0, 1.0], antialias=False, n=1500, seed_jitter=9)
+    check(g, fd, mag)
+
+
+def test_culled_gaussians_get_no_gradient():
+    g, fd, mag, full, ids = run_case([1.0, 1.0, 1.0, 0.0], n=600, size=48)
+    culled = np.setdiff1d(np.arange(full.shape[0]), ids)
+    assert culled.size > 0
+    assert np.all(full[culled] == 0.0)
