@@ -94,9 +94,10 @@ class FrameStats(ctypes.Structure):
         ("visible", ctypes.c_uint32),
         ("pairs", ctypes.c_uint64),
         ("project_ms", ctypes.c_double),
-        ("scan_ms", ctypes.c_double),
+        ("depth_sort_ms", ctypes.c_double),
+        ("counts_ms", ctypes.c_double),
         ("emit_ms", ctypes.c_double),
-        ("sort_ms", ctypes.c_double),
+        ("tile_sort_ms", ctypes.c_double),
         ("ranges_ms", ctypes.c_double),
         ("blend_ms", ctypes.c_double),
         ("total_ms", ctypes.c_double),
@@ -172,6 +173,43 @@ SIGNATURES = {
             ctypes.c_void_p,
         ],
     ),
+    "athx_render_async": (
+        ctypes.c_int,
+        [
+            ctypes.c_void_p,
+            ctypes.POINTER(View),
+            ctypes.c_uint32,
+            ctypes.POINTER(Gaussians),
+            ctypes.POINTER(Settings),
+            ctypes.POINTER(Targets),
+            ctypes.c_uint64,
+            ctypes.c_void_p,
+        ],
+    ),
+    "athx_frame_pairs": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint64),
+                                        ctypes.POINTER(ctypes.c_uint64)]),
+    "athx_graph_create": (
+        ctypes.c_int,
+        [
+            ctypes.c_int,
+            ctypes.POINTER(View),
+            ctypes.c_uint32,
+            ctypes.POINTER(Gaussians),
+            ctypes.POINTER(Settings),
+            ctypes.POINTER(Targets),
+            ctypes.c_uint64,
+            ctypes.POINTER(ctypes.c_void_p),
+            ctypes.POINTER(Grads),
+            ctypes.POINTER(ctypes.c_void_p),
+            ctypes.c_char_p,
+            ctypes.c_uint32,
+        ],
+    ),
+    "athx_graph_launch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "athx_graph_launch_backward": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "athx_graph_pairs": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint64),
+                                        ctypes.POINTER(ctypes.c_uint64)]),
+    "athx_graph_destroy": (None, [ctypes.c_void_p]),
     "athx_latest_counters": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Counters)]),
     "athx_frame_info_get": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(FrameInfo)]),
     "athx_copy_frame": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_uint64,

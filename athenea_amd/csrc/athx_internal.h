@@ -74,12 +74,15 @@ hipError_t launch_fill_u32(uint32_t* p, uint32_t value, uint64_t n, hipStream_t 
 size_t scan_status_words(uint32_t n);
 hipError_t launch_scan(const uint32_t* in, uint32_t* out, uint64_t* total, uint32_t n, uint32_t* status,
                        hipStream_t s);
+// Pairs land at their scanned offsets; those at or past cap are not written
+// (the frame is then incomplete, as the reference's emit past its pair
+// capacity is, splat_emit.slang:43-46).
 hipError_t launch_emit(const DView* views_dev, uint32_t nviews, uint32_t n, uint32_t tile, const float* records,
                        const uint32_t* touched, const uint32_t* depth_keys, const uint32_t* offsets,
-                       const uint32_t* big_list, const uint32_t* big_count, uint64_t* keys, uint32_t* values,
-                       hipStream_t s);
-hipError_t launch_ranges(const uint64_t* sorted_keys, uint32_t pairs, uint32_t tiles, uint32_t* ranges,
-                         hipStream_t s);
+                       const uint32_t* big_list, const uint32_t* big_count, uint32_t cap, uint64_t* keys,
+                       uint32_t* values, hipStream_t s);
+hipError_t launch_ranges(const uint64_t* sorted_keys, const uint64_t* n_dev, uint32_t cap, uint32_t tiles,
+                         uint32_t* ranges, hipStream_t s);
 hipError_t launch_blend(const DView* views_dev, const uint32_t* view_of_tile_base, uint32_t nviews,
                         uint32_t total_tiles, uint32_t tile, uint32_t batch, const uint32_t* ranges,
                         const uint32_t* pair_slots, const float* records, uint32_t depth_mode,
@@ -102,8 +105,19 @@ struct SortScratch {
     uint32_t* counters;  // passes
 };
 uint32_t sort_partitions(uint32_t n);
-hipError_t launch_sort(uint64_t* keys, uint32_t* values, uint32_t n, uint32_t key_bits, const SortScratch& sc,
-                       hipStream_t s);
+// Sorts the first min(*n_dev, cap) pairs stably by key bits [lo_bit,
+// key_bits); scratch is sized for cap and (key_bits - lo_bit) / 8 passes.
+hipError_t launch_sort(uint64_t* keys, uint32_t* values, const uint64_t* n_dev, uint32_t cap, uint32_t lo_bit,
+                       uint32_t key_bits, const SortScratch& sc, hipStream_t s);
+// The visible slots in slot order with their depth keys; the count lands in
+// *visible_total (device).
+hipError_t launch_compact(const uint32_t* touched, const uint32_t* depth_keys, uint32_t slots, uint64_t* keys_out,
+                          uint32_t* slots_out, uint64_t* visible_total, uint32_t* status_words, hipStream_t s);
+// Tile counts of the depth-sorted slots, scanned: slot_offsets[slot] is its
+// first pair; the pair total lands in *pair_total (device).
+hipError_t launch_gather_scan(const uint32_t* touched, const uint32_t* sorted_slots, const uint64_t* visible_dev,
+                              uint32_t max_visible, uint32_t* slot_offsets, uint64_t* pair_total,
+                              uint32_t* status_words, hipStream_t s);
 
 // Per-pair 2D gradient words the blend's adjoint accumulates a slot:
 // centre x, y, conic a, b, c, alpha, colour r, g, b.

@@ -12,7 +12,9 @@
 //   blend        per tile, front to back
 // Scratch grows to 1.5x what a frame needs and, after athx_release_scratch,
 // to exactly what it needs (TileRasterizer.cpp:30-36, 178-191).
+#include <algorithm>
 #include <chrono>
+#include <cstdio>
 #include <cstring>
 #include <exception>
 #include <new>
@@ -29,6 +31,8 @@ struct DeviceBuffer {
     void* ptr = nullptr;
     size_t bytes = 0;
 };
+
+constexpr uint32_t kDepthPasses = 3;  // 24-bit depth keys (splat_project.slang:1129; TileRasterizer.cpp:1122)
 
 uint32_t bits_for(uint32_t values) {
     // modules/render/src/FrameParams.h:18-24
@@ -48,15 +52,19 @@ struct athx_ctx {
 
     DeviceBuffer views, tile_bases, records, touched, depth_keys, offsets, scan_status, totals;
     DeviceBuffer pair_keys, pair_vals, alt_keys, alt_vals, sort_meta, ranges, grad2d, counter_words, big_list;
-    DeviceBuffer api_sort_meta, api_alt_keys, api_alt_vals, api_scan_status;
+    DeviceBuffer depth_keys64, depth_slots, depth_alt_keys, depth_alt_slots, depth_sort_meta;
+    DeviceBuffer api_sort_meta, api_alt_keys, api_alt_vals, api_scan_status, api_count;
     uint64_t* host_totals = nullptr;  // pinned: [0] pairs, [1] visible
     DView* host_views = nullptr;      // pinned staging for the views
     uint32_t* host_tile_bases = nullptr;
     uint32_t host_view_capacity = 0;
 
     // What the last render left for the adjoint.
-    uint32_t last_n = 0, last_nviews = 0, last_tiles = 0, last_tile = 0, last_key_bits = 0;
-    uint64_t last_pairs = 0;
+    uint32_t last_n = 0, last_nviews = 0, last_tiles = 0, last_tile = 0, last_key_bits = 0, last_capacity = 0;
+    uint64_t last_true_pairs = 0;  // the last frame's pair total (an async frame's once athx_frame_pairs ran)
+    uint64_t pair_hint = 0;        // the capacity an async frame takes when given none: 1.5x the largest seen
+    hipEvent_t async_done = nullptr;
+    bool async_pending = false;
     bool have_frame = false;
     bool counters_valid = false;
     athx_counters counters{};
@@ -294,6 +302,198 @@ SortScratch sort_scratch(const DeviceBuffer& meta, uint64_t* alt_keys, uint32_t*
     return sc;
 }
 
+// The sizes a frame runs at.
+struct FrameShape {
+    uint32_t nviews = 0;
+    uint32_t slots = 0;     // views * Gaussians
+    uint32_t tiles = 0;     // over all views
+    uint32_t key_bits = 0;  // 24 depth bits + the tile bits
+    uint32_t passes = 0;    // tile sort passes
+    uint32_t capacity = 0;  // pairs the pair buffers hold
+};
+
+// Validates the call and stages the views (pinned), filling the shape.
+athx_status begin_frame(athx_ctx* ctx, const athx_view* views, uint32_t nviews, const athx_gaussians* gaussians,
+                        const athx_settings* settings, const athx_targets* targets, const float* const* dl,
+                        FrameShape& fs) {
+    if (!views || nviews == 0 || !targets) {
+        return fail(ctx, ATHX_INVALID_ARGUMENT, "no views or targets");
+    }
+    if (athx_status st = check_settings(ctx, settings); st != ATHX_OK) {
+        return st;
+    }
+    if (athx_status st = check_gaussians(ctx, gaussians); st != ATHX_OK) {
+        return st;
+    }
+    const uint64_t slots64 = (uint64_t)gaussians->count * nviews;
+    if (slots64 >= (uint64_t{1} << 32)) {
+        // TileRasterizer.cpp:323-325: slots are 32-bit.
+        return fail(ctx, ATHX_UNSUPPORTED, "more than 2^32 (view, Gaussian) slots in one frame");
+    }
+    ATHX_HIP(ctx, hipSetDevice(ctx->device), "set device");
+    uint64_t tiles64 = 0;
+    if (athx_status st = stage_views(ctx, views, nviews, settings, targets, dl, tiles64); st != ATHX_OK) {
+        return st;
+    }
+    fs.nviews = nviews;
+    fs.slots = (uint32_t)slots64;
+    fs.tiles = (uint32_t)tiles64;
+    fs.key_bits = kDepthBits + bits_for(fs.tiles);
+    fs.passes = div_up(fs.key_bits - kDepthBits, 8);  // the tile digits: the depth ones come presorted
+    return ATHX_OK;
+}
+
+athx_status reserve_front(athx_ctx* ctx, const FrameShape& fs) {
+    const size_t slots = fs.slots;
+    ATHX_HIP(ctx, reserve(ctx, ctx->records, slots * kRecordFloats * sizeof(float) + 16), "records");
+    ATHX_HIP(ctx, reserve(ctx, ctx->touched, slots * 4 + 16), "touched");
+    ATHX_HIP(ctx, reserve(ctx, ctx->depth_keys, slots * 4 + 16), "depth keys");
+    ATHX_HIP(ctx, reserve(ctx, ctx->offsets, slots * 4 + 16), "offsets");
+    ATHX_HIP(ctx, reserve(ctx, ctx->scan_status, scan_status_words(fs.slots) * 4), "scan status");
+    ATHX_HIP(ctx, reserve(ctx, ctx->totals, 32), "totals");
+    ATHX_HIP(ctx, reserve(ctx, ctx->big_list, slots * 4 + 16), "big footprints");
+    ATHX_HIP(ctx, reserve(ctx, ctx->ranges, (size_t)fs.tiles * 8 + 16), "ranges");
+    ATHX_HIP(ctx, reserve(ctx, ctx->counter_words, 64), "counters");
+    ATHX_HIP(ctx, reserve(ctx, ctx->depth_keys64, slots * 8 + 16), "depth sort keys");
+    ATHX_HIP(ctx, reserve(ctx, ctx->depth_slots, slots * 4 + 16), "depth sort slots");
+    ATHX_HIP(ctx, reserve(ctx, ctx->depth_alt_keys, slots * 8 + 16), "depth sort keys");
+    ATHX_HIP(ctx, reserve(ctx, ctx->depth_alt_slots, slots * 4 + 16), "depth sort slots");
+    ATHX_HIP(ctx, reserve(ctx, ctx->depth_sort_meta, sort_meta_words(fs.slots, kDepthPasses) * 4 + 16),
+             "depth sort metadata");
+    return ATHX_OK;
+}
+
+athx_status reserve_back(athx_ctx* ctx, const FrameShape& fs) {
+    const size_t cap = fs.capacity;
+    ATHX_HIP(ctx, reserve(ctx, ctx->pair_keys, cap * 8 + 16), "pair keys");
+    ATHX_HIP(ctx, reserve(ctx, ctx->pair_vals, cap * 4 + 16), "pair values");
+    ATHX_HIP(ctx, reserve(ctx, ctx->alt_keys, cap * 8 + 16), "sort keys");
+    ATHX_HIP(ctx, reserve(ctx, ctx->alt_vals, cap * 4 + 16), "sort values");
+    ATHX_HIP(ctx, reserve(ctx, ctx->sort_meta, sort_meta_words(fs.capacity, fs.passes) * 4 + 16), "sort metadata");
+    return ATHX_OK;
+}
+
+uint32_t* big_count_of(athx_ctx* ctx) { return as<uint32_t>(ctx->totals) + 4; }  // [0..1] pair total, [4] big list
+
+// Projection, then the reference's depth order (TileRasterizer.cpp:1111-1137):
+// the visible slots compacted in slot order, stably sorted by their 24-bit
+// depth keys, and their tile counts scanned in that order into each slot's
+// first pair. Emitting in depth order leaves only the tile digits of the
+// 64-bit keys to sort. The visible total lands in totals[1], the pair total
+// in totals[0], both on the device.
+athx_status enqueue_front(athx_ctx* ctx, const ProjectArgs& pa, const FrameShape& fs, hipStream_t s, bool timed) {
+    uint64_t* totals = as<uint64_t>(ctx->totals);
+    if (timed) {
+        ATHX_HIP(ctx, hipEventRecord(ctx->ev[0], s), "event");
+    }
+    ATHX_HIP(ctx, launch_project(pa, as<DView>(ctx->views), fs.nviews, as<float>(ctx->records),
+                                 as<uint32_t>(ctx->touched), as<uint32_t>(ctx->depth_keys), as<uint32_t>(ctx->big_list),
+                                 big_count_of(ctx), s),
+             "project");
+    if (timed) {
+        ATHX_HIP(ctx, hipEventRecord(ctx->ev[1], s), "event");
+    }
+    ATHX_HIP(ctx, launch_compact(as<uint32_t>(ctx->touched), as<uint32_t>(ctx->depth_keys), fs.slots,
+                                 as<uint64_t>(ctx->depth_keys64), as<uint32_t>(ctx->depth_slots), totals + 1,
+                                 as<uint32_t>(ctx->scan_status), s),
+             "compact");
+    const SortScratch dsc = sort_scratch(ctx->depth_sort_meta, as<uint64_t>(ctx->depth_alt_keys),
+                                         as<uint32_t>(ctx->depth_alt_slots), fs.slots, kDepthPasses);
+    ATHX_HIP(ctx, launch_sort(as<uint64_t>(ctx->depth_keys64), as<uint32_t>(ctx->depth_slots), totals + 1, fs.slots, 0,
+                              kDepthBits, dsc, s),
+             "depth sort");
+    if (timed) {
+        ATHX_HIP(ctx, hipEventRecord(ctx->ev[2], s), "event");
+    }
+    ATHX_HIP(ctx, launch_gather_scan(as<uint32_t>(ctx->touched), as<uint32_t>(ctx->depth_slots), totals + 1, fs.slots,
+                                     as<uint32_t>(ctx->offsets), totals, as<uint32_t>(ctx->scan_status), s),
+             "counts");
+    if (timed) {
+        ATHX_HIP(ctx, hipEventRecord(ctx->ev[3], s), "event");
+    }
+    return ATHX_OK;
+}
+
+// Emission, sort, ranges, blend and counters, every count read on the device
+// and bounded by the pair capacity: no host round trip, so the launches can
+// be captured in a graph.
+athx_status enqueue_back(athx_ctx* ctx, const ProjectArgs& pa, const FrameShape& fs, const athx_settings* settings,
+                         hipStream_t s, bool timed) {
+    const uint64_t* pairs_dev = as<uint64_t>(ctx->totals);
+    ATHX_HIP(ctx, launch_emit(as<DView>(ctx->views), fs.nviews, pa.n, settings->tile, as<float>(ctx->records),
+                              as<uint32_t>(ctx->touched), as<uint32_t>(ctx->depth_keys), as<uint32_t>(ctx->offsets),
+                              as<uint32_t>(ctx->big_list), big_count_of(ctx), fs.capacity,
+                              as<uint64_t>(ctx->pair_keys), as<uint32_t>(ctx->pair_vals), s),
+             "emit");
+    if (timed) {
+        ATHX_HIP(ctx, hipEventRecord(ctx->ev[4], s), "event");
+    }
+    const SortScratch sc = sort_scratch(ctx->sort_meta, as<uint64_t>(ctx->alt_keys), as<uint32_t>(ctx->alt_vals),
+                                        fs.capacity, fs.passes);
+    ATHX_HIP(ctx, launch_sort(as<uint64_t>(ctx->pair_keys), as<uint32_t>(ctx->pair_vals), pairs_dev, fs.capacity,
+                              kDepthBits, fs.key_bits, sc, s),
+             "tile sort");
+    if (timed) {
+        ATHX_HIP(ctx, hipEventRecord(ctx->ev[5], s), "event");
+    }
+    ATHX_HIP(ctx, launch_ranges(as<uint64_t>(ctx->pair_keys), pairs_dev, fs.capacity, fs.tiles,
+                                as<uint32_t>(ctx->ranges), s),
+             "ranges");
+    if (timed) {
+        ATHX_HIP(ctx, hipEventRecord(ctx->ev[6], s), "event");
+    }
+    ATHX_HIP(ctx, launch_blend(as<DView>(ctx->views), as<uint32_t>(ctx->tile_bases), fs.nviews, fs.tiles,
+                               settings->tile, settings->batch, as<uint32_t>(ctx->ranges), as<uint32_t>(ctx->pair_vals),
+                               as<float>(ctx->records), settings->depth_mode, settings->depth_threshold, s),
+             "blend");
+    if (timed) {
+        ATHX_HIP(ctx, hipEventRecord(ctx->ev[7], s), "event");
+    }
+    if (settings->count_splats) {
+        ATHX_HIP(ctx, launch_fill_u32(as<uint32_t>(ctx->counter_words), 0u, 16, s), "clear counters");
+        ATHX_HIP(ctx, launch_counters(as<uint32_t>(ctx->depth_keys), as<uint32_t>(ctx->touched), fs.slots,
+                                      as<uint32_t>(ctx->counter_words), s),
+                 "counters");
+    }
+    return ATHX_OK;
+}
+
+// The adjoint: clear the per-slot 2D gradients, walk every tile back to
+// front, then differentiate the projection. grad2d must be reserved.
+athx_status enqueue_backward(athx_ctx* ctx, const ProjectArgs& pa, uint32_t nviews, uint32_t slots,
+                             const athx_settings* settings, const athx_grads& grads, hipStream_t s) {
+    ATHX_HIP(ctx, launch_fill_u32(as<uint32_t>(ctx->grad2d), 0u, (uint64_t)slots * kGrad2d, s), "clear gradients");
+    ATHX_HIP(ctx, launch_blend_backward(as<DView>(ctx->views), as<uint32_t>(ctx->tile_bases), nviews, ctx->last_tiles,
+                                        settings->tile, settings->batch, as<uint32_t>(ctx->ranges),
+                                        as<uint32_t>(ctx->pair_vals), as<float>(ctx->records),
+                                        as<float>(ctx->grad2d), s),
+             "blend backward");
+    ATHX_HIP(ctx, launch_project_backward(pa, as<DView>(ctx->views), nviews, as<uint32_t>(ctx->touched),
+                                          as<float>(ctx->records), as<float>(ctx->grad2d), grads, s),
+             "projection backward");
+    return ATHX_OK;
+}
+
+// What the adjoint and the readers of the frame's buffers need to know.
+void finish_frame(athx_ctx* ctx, const athx_gaussians* g, const athx_settings* settings, const FrameShape& fs,
+                  uint64_t pairs_known) {
+    if (settings->count_splats) {
+        ctx->counters_valid = true;
+        ctx->counters.slots = fs.slots;
+    }
+    ctx->last_n = g->count;
+    ctx->last_nviews = fs.nviews;
+    ctx->last_tiles = fs.tiles;
+    ctx->last_tile = settings->tile;
+    ctx->last_capacity = fs.capacity;
+    ctx->last_key_bits = fs.key_bits;
+    ctx->last_true_pairs = pairs_known;
+    if (!ctx->async_pending) {
+        ctx->pair_hint = std::max<uint64_t>(ctx->pair_hint, pairs_known + pairs_known / 2);
+    }
+    ctx->have_frame = true;
+}
+
 }  // namespace
 
 extern "C" {
@@ -327,7 +527,8 @@ ATHX_API athx_status athx_create(int device, athx_ctx** out) {
             ev = nullptr;
         }
     }
-    if (hipEventCreateWithFlags(&ctx->upload_done, hipEventDisableTiming) != hipSuccess) {
+    if (hipEventCreateWithFlags(&ctx->upload_done, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->async_done, hipEventDisableTiming) != hipSuccess) {
         athx_destroy(ctx);
         return ATHX_DEVICE_FAILURE;
     }
@@ -345,7 +546,8 @@ ATHX_API void athx_destroy(athx_ctx* ctx) {
                             &ctx->offsets, &ctx->scan_status, &ctx->totals, &ctx->pair_keys, &ctx->pair_vals,
                             &ctx->alt_keys, &ctx->alt_vals, &ctx->sort_meta, &ctx->ranges, &ctx->grad2d,
                             &ctx->counter_words, &ctx->api_sort_meta, &ctx->api_alt_keys, &ctx->api_alt_vals,
-                            &ctx->api_scan_status, &ctx->big_list}) {
+                            &ctx->api_scan_status, &ctx->big_list, &ctx->api_count, &ctx->depth_keys64,
+                            &ctx->depth_slots, &ctx->depth_alt_keys, &ctx->depth_alt_slots, &ctx->depth_sort_meta}) {
         release(*b);
     }
     if (ctx->host_totals) {
@@ -365,6 +567,9 @@ ATHX_API void athx_destroy(athx_ctx* ctx) {
     if (ctx->upload_done) {
         (void)hipEventDestroy(ctx->upload_done);
     }
+    if (ctx->async_done) {
+        (void)hipEventDestroy(ctx->async_done);
+    }
     delete ctx;
 }
 
@@ -377,7 +582,9 @@ ATHX_API athx_status athx_release_scratch(athx_ctx* ctx) {
     (void)hipSetDevice(ctx->device);
     ATHX_HIP(ctx, hipDeviceSynchronize(), "synchronize before release");
     for (DeviceBuffer* b : {&ctx->records, &ctx->touched, &ctx->depth_keys, &ctx->offsets, &ctx->scan_status,
-                            &ctx->big_list, &ctx->pair_keys, &ctx->pair_vals, &ctx->alt_keys, &ctx->alt_vals, &ctx->sort_meta,
+                            &ctx->big_list, &ctx->depth_keys64, &ctx->depth_slots, &ctx->depth_alt_keys,
+                            &ctx->depth_alt_slots, &ctx->depth_sort_meta, &ctx->pair_keys, &ctx->pair_vals,
+                            &ctx->alt_keys, &ctx->alt_vals, &ctx->sort_meta,
                             &ctx->ranges, &ctx->grad2d, &ctx->api_sort_meta, &ctx->api_alt_keys,
                             &ctx->api_alt_vals, &ctx->api_scan_status}) {
         release(*b);
@@ -396,7 +603,8 @@ ATHX_API uint64_t athx_scratch_bytes(const athx_ctx* ctx) {
                                   &ctx->offsets, &ctx->scan_status, &ctx->totals, &ctx->pair_keys, &ctx->pair_vals,
                                   &ctx->alt_keys, &ctx->alt_vals, &ctx->sort_meta, &ctx->ranges, &ctx->grad2d,
                                   &ctx->counter_words, &ctx->api_sort_meta, &ctx->api_alt_keys, &ctx->api_alt_vals,
-                                  &ctx->api_scan_status, &ctx->big_list}) {
+                                  &ctx->api_scan_status, &ctx->big_list, &ctx->depth_keys64, &ctx->depth_slots,
+                                  &ctx->depth_alt_keys, &ctx->depth_alt_slots, &ctx->depth_sort_meta}) {
         sum += b->bytes;
     }
     return sum;
@@ -409,130 +617,52 @@ ATHX_API athx_status athx_render(athx_ctx* ctx, const athx_view* views, uint32_t
         return ATHX_INVALID_ARGUMENT;
     }
     ctx->error.clear();
-    if (!views || nviews == 0 || !targets) {
-        return fail(ctx, ATHX_INVALID_ARGUMENT, "no views or targets");
-    }
-    if (athx_status st = check_settings(ctx, settings); st != ATHX_OK) {
+    FrameShape fs{};
+    if (athx_status st = begin_frame(ctx, views, nviews, gaussians, settings, targets, nullptr, fs); st != ATHX_OK) {
         return st;
     }
-    if (athx_status st = check_gaussians(ctx, gaussians); st != ATHX_OK) {
-        return st;
-    }
-    const uint64_t slots64 = (uint64_t)gaussians->count * nviews;
-    if (slots64 >= (uint64_t{1} << 32)) {
-        // TileRasterizer.cpp:323-325: slots are 32-bit.
-        return fail(ctx, ATHX_UNSUPPORTED, "more than 2^32 (view, Gaussian) slots in one frame");
-    }
-    const uint32_t slots = (uint32_t)slots64;
     hipStream_t s = static_cast<hipStream_t>(stream);
-    ATHX_HIP(ctx, hipSetDevice(ctx->device), "set device");
-    uint64_t tiles64 = 0;
-    if (athx_status st = stage_views(ctx, views, nviews, settings, targets, nullptr, tiles64); st != ATHX_OK) {
-        return st;
-    }
-    const uint32_t tiles = (uint32_t)tiles64;
     const bool timed = settings->time_stages != 0 && ctx->ev[7] != nullptr;
     const auto wall0 = std::chrono::steady_clock::now();
     if (athx_status st = upload_views(ctx, nviews, s); st != ATHX_OK) {
         return st;
     }
-    const DView* dviews = as<DView>(ctx->views);
-    ATHX_HIP(ctx, reserve(ctx, ctx->records, (size_t)slots * kRecordFloats * sizeof(float) + 16), "records");
-    ATHX_HIP(ctx, reserve(ctx, ctx->touched, (size_t)slots * 4 + 16), "touched");
-    ATHX_HIP(ctx, reserve(ctx, ctx->depth_keys, (size_t)slots * 4 + 16), "depth keys");
-    ATHX_HIP(ctx, reserve(ctx, ctx->offsets, (size_t)slots * 4 + 16), "offsets");
-    ATHX_HIP(ctx, reserve(ctx, ctx->scan_status, scan_status_words(slots) * 4), "scan status");
-    ATHX_HIP(ctx, reserve(ctx, ctx->totals, 32), "totals");
-    ATHX_HIP(ctx, reserve(ctx, ctx->big_list, (size_t)slots * 4 + 16), "big footprints");
-    ATHX_HIP(ctx, reserve(ctx, ctx->ranges, (size_t)tiles * 8 + 16), "ranges");
-
+    if (athx_status st = reserve_front(ctx, fs); st != ATHX_OK) {
+        return st;
+    }
     const ProjectArgs pa = project_args(gaussians, settings);
-    if (timed) {
-        ATHX_HIP(ctx, hipEventRecord(ctx->ev[0], s), "event");
+    if (athx_status st = enqueue_front(ctx, pa, fs, s, timed); st != ATHX_OK) {
+        return st;
     }
-    uint32_t* big_count = as<uint32_t>(ctx->totals) + 4;  // totals: [0..1] pair total, [4] big footprints
-    ATHX_HIP(ctx, launch_project(pa, dviews, nviews, as<float>(ctx->records), as<uint32_t>(ctx->touched),
-                                 as<uint32_t>(ctx->depth_keys), as<uint32_t>(ctx->big_list), big_count, s),
-             "project");
-    if (timed) {
-        ATHX_HIP(ctx, hipEventRecord(ctx->ev[1], s), "event");
-    }
-    uint64_t* totals = as<uint64_t>(ctx->totals);
-    ATHX_HIP(ctx, launch_scan(as<uint32_t>(ctx->touched), as<uint32_t>(ctx->offsets), totals, slots,
-                              as<uint32_t>(ctx->scan_status), s),
-             "scan");
-    if (timed) {
-        ATHX_HIP(ctx, hipEventRecord(ctx->ev[2], s), "event");
-    }
-    // The one wait of the frame: how many pairs there are.
-    ATHX_HIP(ctx, hipMemcpyAsync(ctx->host_totals, totals, 8, hipMemcpyDeviceToHost, s), "read pair total");
+    // The one wait of the frame: how many pairs there are (and visible splats).
+    ATHX_HIP(ctx, hipMemcpyAsync(ctx->host_totals, ctx->totals.ptr, 16, hipMemcpyDeviceToHost, s), "read pair total");
     ATHX_HIP(ctx, hipStreamSynchronize(s), "wait for pair total");
     const uint64_t pairs64 = ctx->host_totals[0];
     if (pairs64 >= (uint64_t{1} << 30)) {
         return fail(ctx, ATHX_UNSUPPORTED, "more than 2^30 (tile, Gaussian) pairs in one frame");
     }
-    const uint32_t pairs = (uint32_t)pairs64;
-    const uint32_t key_bits = kDepthBits + bits_for(tiles);
-    const uint32_t passes = div_up(key_bits, 8);
-    ATHX_HIP(ctx, reserve(ctx, ctx->pair_keys, (size_t)pairs * 8 + 16), "pair keys");
-    ATHX_HIP(ctx, reserve(ctx, ctx->pair_vals, (size_t)pairs * 4 + 16), "pair values");
-    ATHX_HIP(ctx, reserve(ctx, ctx->alt_keys, (size_t)pairs * 8 + 16), "sort keys");
-    ATHX_HIP(ctx, reserve(ctx, ctx->alt_vals, (size_t)pairs * 4 + 16), "sort values");
-    ATHX_HIP(ctx, reserve(ctx, ctx->sort_meta, sort_meta_words(pairs, passes) * 4 + 16), "sort metadata");
-    ATHX_HIP(ctx, launch_emit(dviews, nviews, gaussians->count, settings->tile, as<float>(ctx->records),
-                              as<uint32_t>(ctx->touched), as<uint32_t>(ctx->depth_keys), as<uint32_t>(ctx->offsets),
-                              as<uint32_t>(ctx->big_list), big_count, as<uint64_t>(ctx->pair_keys),
-                              as<uint32_t>(ctx->pair_vals), s),
-             "emit");
-    if (timed) {
-        ATHX_HIP(ctx, hipEventRecord(ctx->ev[3], s), "event");
+    fs.capacity = (uint32_t)pairs64;
+    if (athx_status st = reserve_back(ctx, fs); st != ATHX_OK) {
+        return st;
     }
-    const SortScratch sc = sort_scratch(ctx->sort_meta, as<uint64_t>(ctx->alt_keys), as<uint32_t>(ctx->alt_vals),
-                                        pairs, passes);
-    ATHX_HIP(ctx, launch_sort(as<uint64_t>(ctx->pair_keys), as<uint32_t>(ctx->pair_vals), pairs, key_bits, sc, s),
-             "sort");
-    if (timed) {
-        ATHX_HIP(ctx, hipEventRecord(ctx->ev[4], s), "event");
+    if (athx_status st = enqueue_back(ctx, pa, fs, settings, s, timed); st != ATHX_OK) {
+        return st;
     }
-    ATHX_HIP(ctx, launch_ranges(as<uint64_t>(ctx->pair_keys), pairs, tiles, as<uint32_t>(ctx->ranges), s), "ranges");
-    if (timed) {
-        ATHX_HIP(ctx, hipEventRecord(ctx->ev[5], s), "event");
-    }
-    ATHX_HIP(ctx, launch_blend(dviews, as<uint32_t>(ctx->tile_bases), nviews, tiles, settings->tile, settings->batch,
-                               as<uint32_t>(ctx->ranges), as<uint32_t>(ctx->pair_vals), as<float>(ctx->records),
-                               settings->depth_mode, settings->depth_threshold, s),
-             "blend");
-    if (timed) {
-        ATHX_HIP(ctx, hipEventRecord(ctx->ev[6], s), "event");
-    }
-    if (settings->count_splats) {
-        ATHX_HIP(ctx, reserve(ctx, ctx->counter_words, 64), "counters");
-        ATHX_HIP(ctx, launch_fill_u32(as<uint32_t>(ctx->counter_words), 0u, 16, s), "clear counters");
-        ATHX_HIP(ctx, launch_counters(as<uint32_t>(ctx->depth_keys), as<uint32_t>(ctx->touched), slots,
-                                      as<uint32_t>(ctx->counter_words), s),
-                 "counters");
-        ctx->counters_valid = true;
-        ctx->counters.slots = slots;
-    }
-    ctx->last_n = gaussians->count;
-    ctx->last_nviews = nviews;
-    ctx->last_tiles = tiles;
-    ctx->last_tile = settings->tile;
-    ctx->last_pairs = pairs;
-    ctx->last_key_bits = key_bits;
-    ctx->have_frame = true;
+    finish_frame(ctx, gaussians, settings, fs, pairs64);
     if (stats) {
         std::memset(stats, 0, sizeof *stats);
         stats->splats = gaussians->count;
-        stats->pairs = pairs;
+        stats->visible = (uint32_t)ctx->host_totals[1];
+        stats->pairs = pairs64;
         if (timed) {
-            ATHX_HIP(ctx, hipEventSynchronize(ctx->ev[6]), "wait for frame");
+            ATHX_HIP(ctx, hipEventSynchronize(ctx->ev[7]), "wait for frame");
             stats->project_ms = elapsed_ms(ctx->ev[0], ctx->ev[1]);
-            stats->scan_ms = elapsed_ms(ctx->ev[1], ctx->ev[2]);
-            stats->emit_ms = elapsed_ms(ctx->ev[2], ctx->ev[3]);
-            stats->sort_ms = elapsed_ms(ctx->ev[3], ctx->ev[4]);
-            stats->ranges_ms = elapsed_ms(ctx->ev[4], ctx->ev[5]);
-            stats->blend_ms = elapsed_ms(ctx->ev[5], ctx->ev[6]);
+            stats->depth_sort_ms = elapsed_ms(ctx->ev[1], ctx->ev[2]);
+            stats->counts_ms = elapsed_ms(ctx->ev[2], ctx->ev[3]);
+            stats->emit_ms = elapsed_ms(ctx->ev[3], ctx->ev[4]);
+            stats->tile_sort_ms = elapsed_ms(ctx->ev[4], ctx->ev[5]);
+            stats->ranges_ms = elapsed_ms(ctx->ev[5], ctx->ev[6]);
+            stats->blend_ms = elapsed_ms(ctx->ev[6], ctx->ev[7]);
             stats->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wall0).count();
         }
     }
@@ -541,6 +671,74 @@ ATHX_API athx_status athx_render(athx_ctx* ctx, const athx_view* views, uint32_t
     return fail(ctx, ATHX_OUT_OF_MEMORY, "host allocation failed");
 } catch (const std::exception& e) {
     return fail(ctx, ATHX_INTERNAL_ERROR, e.what());
+}
+
+ATHX_API athx_status athx_render_async(athx_ctx* ctx, const athx_view* views, uint32_t nviews,
+                                       const athx_gaussians* gaussians, const athx_settings* settings,
+                                       const athx_targets* targets, uint64_t pair_capacity, void* stream) try {
+    if (!ctx) {
+        return ATHX_INVALID_ARGUMENT;
+    }
+    ctx->error.clear();
+    FrameShape fs{};
+    if (athx_status st = begin_frame(ctx, views, nviews, gaussians, settings, targets, nullptr, fs); st != ATHX_OK) {
+        return st;
+    }
+    const uint64_t cap = pair_capacity ? pair_capacity : ctx->pair_hint;
+    if (cap == 0) {
+        return fail(ctx, ATHX_INVALID_ARGUMENT, "no pair capacity given and no frame seen to size one from");
+    }
+    if (cap >= (uint64_t{1} << 30)) {
+        return fail(ctx, ATHX_UNSUPPORTED, "a pair capacity of 2^30 or more");
+    }
+    fs.capacity = (uint32_t)cap;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (athx_status st = upload_views(ctx, nviews, s); st != ATHX_OK) {
+        return st;
+    }
+    if (athx_status st = reserve_front(ctx, fs); st != ATHX_OK) {
+        return st;
+    }
+    if (athx_status st = reserve_back(ctx, fs); st != ATHX_OK) {
+        return st;
+    }
+    const ProjectArgs pa = project_args(gaussians, settings);
+    if (athx_status st = enqueue_front(ctx, pa, fs, s, false); st != ATHX_OK) {
+        return st;
+    }
+    if (athx_status st = enqueue_back(ctx, pa, fs, settings, s, false); st != ATHX_OK) {
+        return st;
+    }
+    ATHX_HIP(ctx, hipMemcpyAsync(ctx->host_totals, ctx->totals.ptr, 8, hipMemcpyDeviceToHost, s), "read pair total");
+    ATHX_HIP(ctx, hipEventRecord(ctx->async_done, s), "record frame");
+    ctx->async_pending = true;
+    finish_frame(ctx, gaussians, settings, fs, cap);
+    return ATHX_OK;
+} catch (const std::bad_alloc&) {
+    return fail(ctx, ATHX_OUT_OF_MEMORY, "host allocation failed");
+} catch (const std::exception& e) {
+    return fail(ctx, ATHX_INTERNAL_ERROR, e.what());
+}
+
+ATHX_API athx_status athx_frame_pairs(athx_ctx* ctx, uint64_t* pairs, uint64_t* capacity) {
+    if (!ctx || !pairs) {
+        return ATHX_INVALID_ARGUMENT;
+    }
+    if (!ctx->have_frame) {
+        return fail(ctx, ATHX_NOT_FOUND, "no frame has been rendered");
+    }
+    if (ctx->async_pending) {
+        ATHX_HIP(ctx, hipEventSynchronize(ctx->async_done), "wait for the frame");
+        ctx->async_pending = false;
+        const uint64_t p = ctx->host_totals[0];
+        ctx->last_true_pairs = p;
+        ctx->pair_hint = std::max<uint64_t>(ctx->pair_hint, p + p / 2);
+    }
+    *pairs = ctx->last_true_pairs;
+    if (capacity) {
+        *capacity = ctx->last_capacity;
+    }
+    return ATHX_OK;
 }
 
 ATHX_API athx_status athx_render_backward(athx_ctx* ctx, const athx_view* views, uint32_t nviews,
@@ -584,17 +782,7 @@ ATHX_API athx_status athx_render_backward(athx_ctx* ctx, const athx_view* views,
     }
     const uint32_t slots = gaussians->count * nviews;
     ATHX_HIP(ctx, reserve(ctx, ctx->grad2d, (size_t)slots * kGrad2d * sizeof(float) + 16), "2D gradients");
-    ATHX_HIP(ctx, launch_fill_u32(as<uint32_t>(ctx->grad2d), 0u, (uint64_t)slots * kGrad2d, s), "clear gradients");
-    ATHX_HIP(ctx, launch_blend_backward(as<DView>(ctx->views), as<uint32_t>(ctx->tile_bases), nviews, ctx->last_tiles,
-                                        settings->tile, settings->batch, as<uint32_t>(ctx->ranges),
-                                        as<uint32_t>(ctx->pair_vals), as<float>(ctx->records),
-                                        as<float>(ctx->grad2d), s),
-             "blend backward");
-    const ProjectArgs pa = project_args(gaussians, settings);
-    ATHX_HIP(ctx, launch_project_backward(pa, as<DView>(ctx->views), nviews, as<uint32_t>(ctx->touched),
-                                          as<float>(ctx->records), as<float>(ctx->grad2d), *grads, s),
-             "projection backward");
-    return ATHX_OK;
+    return enqueue_backward(ctx, project_args(gaussians, settings), nviews, slots, settings, *grads, s);
 } catch (const std::bad_alloc&) {
     return fail(ctx, ATHX_OUT_OF_MEMORY, "host allocation failed");
 } catch (const std::exception& e) {
@@ -623,17 +811,18 @@ ATHX_API athx_status athx_latest_counters(athx_ctx* ctx, athx_counters* out) {
     return ATHX_OK;
 }
 
-ATHX_API athx_status athx_frame_info_get(const athx_ctx* ctx, athx_frame_info* out) {
+ATHX_API athx_status athx_frame_info_get(athx_ctx* ctx, athx_frame_info* out) {
     if (!ctx || !out) {
         return ATHX_INVALID_ARGUMENT;
     }
-    if (!ctx->have_frame) {
-        return ATHX_NOT_FOUND;
+    uint64_t pairs = 0, cap = 0;
+    if (athx_status st = athx_frame_pairs(ctx, &pairs, &cap); st != ATHX_OK) {
+        return st;
     }
     out->slots = ctx->last_n * ctx->last_nviews;
     out->tiles = ctx->last_tiles;
     out->key_bits = ctx->last_key_bits;
-    out->pairs = ctx->last_pairs;
+    out->pairs = std::min(pairs, cap);
     return ATHX_OK;
 }
 
@@ -645,6 +834,11 @@ ATHX_API athx_status athx_copy_frame(athx_ctx* ctx, athx_frame_part part, void* 
     if (!ctx->have_frame) {
         return fail(ctx, ATHX_NOT_FOUND, "no frame has been rendered");
     }
+    uint64_t true_pairs = 0, cap = 0;
+    if (athx_status st = athx_frame_pairs(ctx, &true_pairs, &cap); st != ATHX_OK) {
+        return st;
+    }
+    const uint64_t pairs = std::min(true_pairs, cap);
     const uint64_t slots = (uint64_t)ctx->last_n * ctx->last_nviews;
     const void* src = nullptr;
     uint64_t size = 0;
@@ -653,8 +847,8 @@ ATHX_API athx_status athx_copy_frame(athx_ctx* ctx, athx_frame_part part, void* 
     case ATHX_PART_TOUCHED: src = ctx->touched.ptr; size = slots * 4; break;
     case ATHX_PART_DEPTH_KEYS: src = ctx->depth_keys.ptr; size = slots * 4; break;
     case ATHX_PART_OFFSETS: src = ctx->offsets.ptr; size = slots * 4; break;
-    case ATHX_PART_PAIR_KEYS: src = ctx->pair_keys.ptr; size = ctx->last_pairs * 8; break;
-    case ATHX_PART_PAIR_VALUES: src = ctx->pair_vals.ptr; size = ctx->last_pairs * 4; break;
+    case ATHX_PART_PAIR_KEYS: src = ctx->pair_keys.ptr; size = pairs * 8; break;
+    case ATHX_PART_PAIR_VALUES: src = ctx->pair_vals.ptr; size = pairs * 4; break;
     case ATHX_PART_RANGES: src = ctx->ranges.ptr; size = (uint64_t)ctx->last_tiles * 8; break;
     default: return fail(ctx, ATHX_INVALID_ARGUMENT, "unknown frame part");
     }
@@ -716,10 +910,220 @@ ATHX_API athx_status athx_sort_pairs(athx_ctx* ctx, uint64_t* keys, uint32_t* va
     ATHX_HIP(ctx, reserve(ctx, ctx->api_alt_keys, (size_t)n * 8), "sort keys");
     ATHX_HIP(ctx, reserve(ctx, ctx->api_alt_vals, (size_t)n * 4), "sort values");
     ATHX_HIP(ctx, reserve(ctx, ctx->api_sort_meta, sort_meta_words(n, passes) * 4), "sort metadata");
+    ATHX_HIP(ctx, reserve(ctx, ctx->api_count, 16), "sort count");
     const SortScratch sc = sort_scratch(ctx->api_sort_meta, as<uint64_t>(ctx->api_alt_keys),
                                         as<uint32_t>(ctx->api_alt_vals), n, passes);
-    ATHX_HIP(ctx, launch_sort(keys, values, n, key_bits, sc, static_cast<hipStream_t>(stream)), "sort");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    // The sort reads its count on the device: put n there (a kernel, not a
+    // memset, so a captured graph replays it faithfully).
+    ATHX_HIP(ctx, launch_fill_u32(as<uint32_t>(ctx->api_count), n, 1, s), "sort count");
+    ATHX_HIP(ctx, launch_fill_u32(as<uint32_t>(ctx->api_count) + 1, 0u, 1, s), "sort count");
+    ATHX_HIP(ctx, launch_sort(keys, values, as<uint64_t>(ctx->api_count), n, 0, key_bits, sc, s), "sort");
     return ATHX_OK;
+}
+
+
+struct athx_graph {
+    athx_ctx* ctx = nullptr;
+    hipStream_t capture = nullptr;
+    hipGraph_t forward = nullptr, backward = nullptr;
+    hipGraphExec_t forward_exec = nullptr, backward_exec = nullptr;
+};
+
+ATHX_API void athx_graph_destroy(athx_graph* g) {
+    if (!g) {
+        return;
+    }
+    if (g->ctx) {
+        (void)hipSetDevice(g->ctx->device);
+        (void)hipDeviceSynchronize();
+    }
+    if (g->forward_exec) {
+        (void)hipGraphExecDestroy(g->forward_exec);
+    }
+    if (g->backward_exec) {
+        (void)hipGraphExecDestroy(g->backward_exec);
+    }
+    if (g->forward) {
+        (void)hipGraphDestroy(g->forward);
+    }
+    if (g->backward) {
+        (void)hipGraphDestroy(g->backward);
+    }
+    if (g->capture) {
+        (void)hipStreamDestroy(g->capture);
+    }
+    athx_destroy(g->ctx);
+    delete g;
+}
+
+namespace {
+
+athx_status graph_fail(athx_graph* g, athx_status st, char* error, uint32_t error_bytes) {
+    if (error && error_bytes) {
+        const std::string msg = g && g->ctx ? g->ctx->error : std::string("graph creation failed");
+        std::snprintf(error, error_bytes, "%s", msg.c_str());
+    }
+    athx_graph_destroy(g);
+    return st;
+}
+
+athx_status capture_forward(athx_graph* g, const ProjectArgs& pa, const FrameShape& fs, const athx_settings* settings) {
+    athx_ctx* ctx = g->ctx;
+    ATHX_HIP(ctx, hipStreamBeginCapture(g->capture, hipStreamCaptureModeThreadLocal), "begin capture");
+    athx_status st = enqueue_front(ctx, pa, fs, g->capture, false);
+    if (st == ATHX_OK) {
+        st = enqueue_back(ctx, pa, fs, settings, g->capture, false);
+    }
+    hipError_t copy = hipSuccess;
+    if (st == ATHX_OK) {
+        copy = hipMemcpyAsync(ctx->host_totals, ctx->totals.ptr, 8, hipMemcpyDeviceToHost, g->capture);
+    }
+    hipError_t end = hipStreamEndCapture(g->capture, &g->forward);
+    if (st != ATHX_OK) {
+        return st;
+    }
+    ATHX_HIP(ctx, copy, "capture the pair readback");
+    ATHX_HIP(ctx, end, "end capture");
+    ATHX_HIP(ctx, hipGraphInstantiate(&g->forward_exec, g->forward, nullptr, nullptr, 0), "instantiate the frame");
+    return ATHX_OK;
+}
+
+athx_status capture_backward(athx_graph* g, const ProjectArgs& pa, const FrameShape& fs,
+                             const athx_settings* settings, const athx_grads& grads) {
+    athx_ctx* ctx = g->ctx;
+    ATHX_HIP(ctx, hipStreamBeginCapture(g->capture, hipStreamCaptureModeThreadLocal), "begin capture");
+    const athx_status st = enqueue_backward(ctx, pa, fs.nviews, fs.slots, settings, grads, g->capture);
+    hipError_t end = hipStreamEndCapture(g->capture, &g->backward);
+    if (st != ATHX_OK) {
+        return st;
+    }
+    ATHX_HIP(ctx, end, "end capture");
+    ATHX_HIP(ctx, hipGraphInstantiate(&g->backward_exec, g->backward, nullptr, nullptr, 0),
+             "instantiate the adjoint");
+    return ATHX_OK;
+}
+
+}  // namespace
+
+ATHX_API athx_status athx_graph_create(int device, const athx_view* views, uint32_t nviews,
+                                       const athx_gaussians* gaussians, const athx_settings* settings,
+                                       const athx_targets* targets, uint64_t pair_capacity,
+                                       const float* const* dl_dcolour, const athx_grads* grads, athx_graph** out,
+                                       char* error, uint32_t error_bytes) try {
+    if (!out) {
+        return ATHX_INVALID_ARGUMENT;
+    }
+    *out = nullptr;
+    athx_graph* g = new athx_graph();
+    if (athx_status st = athx_create(device, &g->ctx); st != ATHX_OK) {
+        return graph_fail(g, st, error, error_bytes);
+    }
+    athx_ctx* ctx = g->ctx;
+    ctx->exact_growth = true;
+    const bool with_backward = dl_dcolour != nullptr && grads != nullptr;
+    if (with_backward) {
+        for (uint32_t k = 0; views && targets && k < nviews; ++k) {
+            if (!targets[k].final_t || !targets[k].last || !dl_dcolour[k]) {
+                return graph_fail(g, fail(ctx, ATHX_INVALID_ARGUMENT, "the adjoint needs final_t, last and dL/dcolour"),
+                                  error, error_bytes);
+            }
+        }
+    }
+    if (pair_capacity == 0) {
+        // Size from one eager frame of the same inputs.
+        athx_frame_stats stats{};
+        if (athx_status st = athx_render(ctx, views, nviews, gaussians, settings, targets, nullptr, &stats);
+            st != ATHX_OK) {
+            return graph_fail(g, st, error, error_bytes);
+        }
+        pair_capacity = std::max<uint64_t>(stats.pairs + stats.pairs / 2, 1024);
+    }
+    if (pair_capacity >= (uint64_t{1} << 30)) {
+        return graph_fail(g, fail(ctx, ATHX_UNSUPPORTED, "a pair capacity of 2^30 or more"), error, error_bytes);
+    }
+    FrameShape fs{};
+    if (athx_status st = begin_frame(ctx, views, nviews, gaussians, settings, targets,
+                                     with_backward ? dl_dcolour : nullptr, fs);
+        st != ATHX_OK) {
+        return graph_fail(g, st, error, error_bytes);
+    }
+    fs.capacity = (uint32_t)pair_capacity;
+    if (athx_status st = reserve_front(ctx, fs); st != ATHX_OK) {
+        return graph_fail(g, st, error, error_bytes);
+    }
+    if (athx_status st = reserve_back(ctx, fs); st != ATHX_OK) {
+        return graph_fail(g, st, error, error_bytes);
+    }
+    if (with_backward) {
+        if (hipError_t e = reserve(ctx, ctx->grad2d, (size_t)fs.slots * kGrad2d * sizeof(float) + 16);
+            e != hipSuccess) {
+            return graph_fail(g, hip_fail(ctx, e, "2D gradients"), error, error_bytes);
+        }
+    }
+    // The views once, before capture: the graph reads them from the device.
+    if (hipError_t e = reserve(ctx, ctx->views, sizeof(DView) * nviews); e != hipSuccess) {
+        return graph_fail(g, hip_fail(ctx, e, "views"), error, error_bytes);
+    }
+    if (hipError_t e = reserve(ctx, ctx->tile_bases, sizeof(uint32_t) * (nviews + 1)); e != hipSuccess) {
+        return graph_fail(g, hip_fail(ctx, e, "tile bases"), error, error_bytes);
+    }
+    if (hipMemcpy(ctx->views.ptr, ctx->host_views, sizeof(DView) * nviews, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(ctx->tile_bases.ptr, ctx->host_tile_bases, sizeof(uint32_t) * (nviews + 1),
+                  hipMemcpyHostToDevice) != hipSuccess) {
+        return graph_fail(g, fail(ctx, ATHX_DEVICE_FAILURE, "upload views"), error, error_bytes);
+    }
+    if (hipStreamCreateWithFlags(&g->capture, hipStreamNonBlocking) != hipSuccess) {
+        return graph_fail(g, fail(ctx, ATHX_DEVICE_FAILURE, "capture stream"), error, error_bytes);
+    }
+    const ProjectArgs pa = project_args(gaussians, settings);
+    finish_frame(ctx, gaussians, settings, fs, 0);
+    if (athx_status st = capture_forward(g, pa, fs, settings); st != ATHX_OK) {
+        return graph_fail(g, st, error, error_bytes);
+    }
+    if (with_backward) {
+        if (athx_status st = capture_backward(g, pa, fs, settings, *grads); st != ATHX_OK) {
+            return graph_fail(g, st, error, error_bytes);
+        }
+    }
+    ctx->have_frame = false;  // nothing has been drawn until the first replay
+    *out = g;
+    return ATHX_OK;
+} catch (const std::bad_alloc&) {
+    return ATHX_OUT_OF_MEMORY;
+}
+
+ATHX_API athx_status athx_graph_launch(athx_graph* g, void* stream) {
+    if (!g || !g->forward_exec) {
+        return ATHX_INVALID_ARGUMENT;
+    }
+    athx_ctx* ctx = g->ctx;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    ATHX_HIP(ctx, hipSetDevice(ctx->device), "set device");
+    ATHX_HIP(ctx, hipGraphLaunch(g->forward_exec, s), "launch the frame");
+    ATHX_HIP(ctx, hipEventRecord(ctx->async_done, s), "record frame");
+    ctx->async_pending = true;
+    ctx->have_frame = true;
+    return ATHX_OK;
+}
+
+ATHX_API athx_status athx_graph_launch_backward(athx_graph* g, void* stream) {
+    if (!g || !g->backward_exec) {
+        return ATHX_INVALID_ARGUMENT;
+    }
+    if (!g->ctx->have_frame) {
+        return fail(g->ctx, ATHX_INVALID_ARGUMENT, "replay the frame before its adjoint");
+    }
+    ATHX_HIP(g->ctx, hipSetDevice(g->ctx->device), "set device");
+    ATHX_HIP(g->ctx, hipGraphLaunch(g->backward_exec, static_cast<hipStream_t>(stream)), "launch the adjoint");
+    return ATHX_OK;
+}
+
+ATHX_API athx_status athx_graph_pairs(athx_graph* g, uint64_t* pairs, uint64_t* capacity) {
+    if (!g) {
+        return ATHX_INVALID_ARGUMENT;
+    }
+    return athx_frame_pairs(g->ctx, pairs, capacity);
 }
 
 }  // extern "C"

@@ -304,8 +304,8 @@ __global__ void __launch_bounds__(256) emit_kernel(const DView* __restrict__ vie
                                                    uint32_t T, const float* __restrict__ records,
                                                    const uint32_t* __restrict__ touched,
                                                    const uint32_t* __restrict__ depth_keys,
-                                                   const uint32_t* __restrict__ offsets, uint64_t* __restrict__ keys,
-                                                   uint32_t* __restrict__ values) {
+                                                   const uint32_t* __restrict__ offsets, uint32_t cap,
+                                                   uint64_t* __restrict__ keys, uint32_t* __restrict__ values) {
     const uint64_t slot = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (slot >= (uint64_t)n * nviews) {
         return;
@@ -325,7 +325,7 @@ __global__ void __launch_bounds__(256) emit_kernel(const DView* __restrict__ vie
     }
     const uint64_t depth = depth_keys[slot] & 0xFFFFFFu;
     uint32_t at = offsets[slot];
-    const uint32_t stop = at + expected;
+    const uint32_t stop = (uint64_t)at + expected < cap ? at + expected : cap;
     for (uint32_t ty = y0; ty <= y1 && at < stop; ++ty) {
         for (uint32_t tx = x0; tx <= x1 && at < stop; ++tx) {
             if (tile_touched(V.width, V.height, T, screen.x, screen.y, conic.x, conic.y, conic.z, screen.w, tx, ty)) {
@@ -401,7 +401,7 @@ __global__ void __launch_bounds__(kBigThreads) emit_big_kernel(const DView* __re
                                                                const uint32_t* __restrict__ big_count,
                                                                const uint32_t* __restrict__ touched,
                                                                const uint32_t* __restrict__ depth_keys,
-                                                               const uint32_t* __restrict__ offsets,
+                                                               const uint32_t* __restrict__ offsets, uint32_t cap,
                                                                uint64_t* __restrict__ keys,
                                                                uint32_t* __restrict__ values) {
     const uint32_t lane = threadIdx.x & 63u;
@@ -417,7 +417,7 @@ __global__ void __launch_bounds__(kBigThreads) emit_big_kernel(const DView* __re
         const BigFootprint f = big_footprint(views, n, T, records, slot);
         const uint64_t depth = depth_keys[slot] & 0xFFFFFFu;
         uint32_t at = offsets[slot];
-        const uint32_t stop = at + expected;
+        const uint32_t stop = (uint64_t)at + expected < cap ? at + expected : cap;
         for (uint32_t k0 = 0; k0 < f.area; k0 += 64) {
             const uint32_t k = k0 + lane;
             const bool hit = big_hit(f, T, k);
@@ -450,12 +450,14 @@ __device__ inline uint32_t lower_bound_tile(const uint64_t* keys, uint32_t n, ui
     return lo;
 }
 
-__global__ void __launch_bounds__(256) ranges_kernel(const uint64_t* __restrict__ keys, uint32_t n, uint32_t tiles,
+__global__ void __launch_bounds__(256) ranges_kernel(const uint64_t* __restrict__ keys,
+                                                     const uint64_t* __restrict__ n_dev, uint32_t cap, uint32_t tiles,
                                                      uint32_t* __restrict__ ranges) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= tiles) {
         return;
     }
+    const uint32_t n = *n_dev < cap ? (uint32_t)*n_dev : cap;
     const uint32_t start = lower_bound_tile(keys, n, t);
     const uint32_t end = lower_bound_tile(keys, n, (uint64_t)t + 1);
     ranges[2 * t] = start < end ? start : 0u;
@@ -537,25 +539,26 @@ hipError_t launch_fill_u32(uint32_t* p, uint32_t value, uint64_t n, hipStream_t 
 
 hipError_t launch_emit(const DView* views_dev, uint32_t nviews, uint32_t n, uint32_t tile, const float* records,
                        const uint32_t* touched, const uint32_t* depth_keys, const uint32_t* offsets,
-                       const uint32_t* big_list, const uint32_t* big_count, uint64_t* keys, uint32_t* values,
-                       hipStream_t s) {
+                       const uint32_t* big_list, const uint32_t* big_count, uint32_t cap, uint64_t* keys,
+                       uint32_t* values, hipStream_t s) {
     const uint64_t slots = (uint64_t)n * nviews;
-    if (slots == 0) {
+    if (slots == 0 || cap == 0) {
         return hipSuccess;
     }
     hipLaunchKernelGGL(emit_kernel, dim3((uint32_t)((slots + 255) / 256)), dim3(256), 0, s, views_dev, nviews, n,
-                       tile, records, touched, depth_keys, offsets, keys, values);
+                       tile, records, touched, depth_keys, offsets, cap, keys, values);
     hipLaunchKernelGGL(emit_big_kernel, dim3(kBigBlocks), dim3(kBigThreads), 0, s, views_dev, n, tile, records,
-                       big_list, big_count, touched, depth_keys, offsets, keys, values);
+                       big_list, big_count, touched, depth_keys, offsets, cap, keys, values);
     return hipGetLastError();
 }
 
-hipError_t launch_ranges(const uint64_t* sorted_keys, uint32_t pairs, uint32_t tiles, uint32_t* ranges,
-                         hipStream_t s) {
+hipError_t launch_ranges(const uint64_t* sorted_keys, const uint64_t* n_dev, uint32_t cap, uint32_t tiles,
+                         uint32_t* ranges, hipStream_t s) {
     if (tiles == 0) {
         return hipSuccess;
     }
-    hipLaunchKernelGGL(ranges_kernel, dim3(div_up(tiles, 256)), dim3(256), 0, s, sorted_keys, pairs, tiles, ranges);
+    hipLaunchKernelGGL(ranges_kernel, dim3(div_up(tiles, 256)), dim3(256), 0, s, sorted_keys, n_dev, cap, tiles,
+                       ranges);
     return hipGetLastError();
 }
 

@@ -1,13 +1,20 @@
-// scan.hip -- exclusive prefix sum of uint32 in one pass: each workgroup
-// takes a ticket (so every partition it waits on has already started and
-// will finish), reduces its 4096 elements, publishes the aggregate, and one
+// scan.hip -- exclusive prefix sums in one pass: each workgroup takes a
+// ticket (so every partition it waits on has already started and will
+// finish), reduces its 4096 elements, publishes the aggregate, and one
 // wavefront looks back 64 partitions at a time for the inclusive prefix of
 // its predecessors (decoupled look-back, Merrill & Garland 2016).
 //
-// Semantics: gpu::PrefixSum::apply (modules/gpu/include/athenea/gpu/algo/
-// PrefixSum.h:43-44; modules/gpu/src/algo/PrefixSum.cpp:58-88): out[i] =
-// in[0] + ... + in[i-1], and the total of all. The reference does it in
-// three dispatches (prefix_chunk_totals, prefix_chunk_starts, prefix_local).
+// Three uses share the kernel, each a small policy type:
+//   PlainScan    out[i] = in[0] + ... + in[i-1] and the total: gpu::PrefixSum::
+//                apply (modules/gpu/include/athenea/gpu/algo/PrefixSum.h:43-44;
+//                modules/gpu/src/algo/PrefixSum.cpp:58-88), which the reference
+//                runs as three dispatches (prefix_chunk_totals/starts/local).
+//   CompactScan  the visible slots, in slot order, with their depth keys:
+//                splat_compact.slang:16-26 over the scan of `visible`.
+//   GatherScan   the tile counts in depth order, scanned into each slot's first
+//                pair: splat_gather_counts.slang:11-19 and the scan after it
+//                (TileRasterizer.cpp:1129-1137). Its length is read on the
+//                device (the compaction's total), so no host round trip.
 #include "athx_internal.h"
 
 namespace athx {
@@ -46,10 +53,48 @@ __device__ inline uint64_t wave_sum(uint64_t x) {
     return x;
 }
 
-// status[0] is the ticket counter; status[1 + p] partition p's word.
-__global__ void __launch_bounds__(kThreads) scan_kernel(const uint32_t* __restrict__ in, uint32_t* __restrict__ out,
-                                                        uint64_t* __restrict__ total, uint32_t n,
-                                                        uint64_t* __restrict__ status, bool aligned) {
+struct PlainScan {
+    const uint32_t* in;
+    uint32_t* out;
+    uint32_t n;
+    __device__ uint32_t size() const { return n; }
+    __device__ uint32_t load(uint32_t i) const { return in[i]; }
+    __device__ void store(uint32_t i, uint32_t, uint64_t excl) const { out[i] = (uint32_t)excl; }
+};
+
+struct CompactScan {
+    const uint32_t* touched;
+    const uint32_t* depth_keys;
+    uint64_t* keys_out;
+    uint32_t* slots_out;
+    uint32_t n;
+    __device__ uint32_t size() const { return n; }
+    __device__ uint32_t load(uint32_t i) const { return touched[i] != 0 ? 1u : 0u; }
+    __device__ void store(uint32_t i, uint32_t v, uint64_t excl) const {
+        if (v) {
+            keys_out[excl] = depth_keys[i];
+            slots_out[excl] = i;
+        }
+    }
+};
+
+struct GatherScan {
+    const uint32_t* touched;
+    const uint32_t* sorted_slots;
+    uint32_t* slot_offsets;
+    const uint64_t* n_dev;
+    uint32_t cap;
+    __device__ uint32_t size() const { return *n_dev < cap ? (uint32_t)*n_dev : cap; }
+    __device__ uint32_t load(uint32_t k) const { return touched[sorted_slots[k]]; }
+    __device__ void store(uint32_t k, uint32_t, uint64_t excl) const { slot_offsets[sorted_slots[k]] = (uint32_t)excl; }
+};
+
+// status[0] is the ticket counter; status[1 + p] partition p's word. The grid
+// may hold more partitions than the scan's length needs (a length read on the
+// device): tickets past it leave at once, and nobody waits on them.
+template <typename Policy>
+__global__ void __launch_bounds__(kThreads) scan_kernel(Policy pol, uint64_t* __restrict__ total,
+                                                        uint64_t* __restrict__ status) {
     __shared__ uint32_t s_part;
     __shared__ uint64_t s_wave[kThreads / 64];
     __shared__ uint64_t s_prefix;
@@ -60,23 +105,18 @@ __global__ void __launch_bounds__(kThreads) scan_kernel(const uint32_t* __restri
         s_part = atomicAdd(reinterpret_cast<uint32_t*>(status), 1u);
     }
     __syncthreads();
+    const uint32_t n = pol.size();
     const uint32_t part = s_part;
-    const uint32_t parts = div_up(n, kTile);
+    const uint32_t parts = div_up(n ? n : 1, kTile);
+    if (part >= parts) {
+        return;
+    }
     const uint64_t first = (uint64_t)part * kTile + (uint64_t)tid * kItems;
 
     uint32_t v[kItems];
-    if (aligned && first + kItems <= n) {
-        const uint4* src = reinterpret_cast<const uint4*>(in + first);
 #pragma unroll
-        for (uint32_t k = 0; k < kItems / 4; ++k) {
-            const uint4 q = src[k];
-            v[4 * k + 0] = q.x; v[4 * k + 1] = q.y; v[4 * k + 2] = q.z; v[4 * k + 3] = q.w;
-        }
-    } else {
-#pragma unroll
-        for (uint32_t k = 0; k < kItems; ++k) {
-            v[k] = first + k < n ? in[first + k] : 0u;
-        }
+    for (uint32_t k = 0; k < kItems; ++k) {
+        v[k] = first + k < n ? pol.load((uint32_t)(first + k)) : 0u;
     }
     uint64_t mine = 0;
 #pragma unroll
@@ -139,53 +179,60 @@ __global__ void __launch_bounds__(kThreads) scan_kernel(const uint32_t* __restri
     }
     __syncthreads();
     uint64_t run = s_prefix + wave_before + (incl - mine);
-    uint32_t o[kItems];
 #pragma unroll
     for (uint32_t k = 0; k < kItems; ++k) {
-        o[k] = (uint32_t)run;
+        if (first + k < n) {
+            pol.store((uint32_t)(first + k), v[k], run);
+        }
         run += v[k];
-    }
-    if (aligned && first + kItems <= n) {
-        uint4* dst = reinterpret_cast<uint4*>(out + first);
-#pragma unroll
-        for (uint32_t k = 0; k < kItems / 4; ++k) {
-            dst[k] = make_uint4(o[4 * k + 0], o[4 * k + 1], o[4 * k + 2], o[4 * k + 3]);
-        }
-    } else {
-#pragma unroll
-        for (uint32_t k = 0; k < kItems; ++k) {
-            if (first + k < n) {
-                out[first + k] = o[k];
-            }
-        }
     }
 }
 
-__global__ void zero_u64_kernel(uint64_t* p, uint64_t n) {
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
-        p[i] = 0;
+// Zeroes the tickets and the words of the partitions the scan will use
+// (ticket counter included), and the total for an empty scan.
+template <typename Policy>
+__global__ void clear_status_kernel(Policy pol, uint64_t* __restrict__ status, uint64_t* __restrict__ total) {
+    const uint32_t n = pol.size();
+    const uint64_t words = 1 + (uint64_t)div_up(n ? n : 1, kTile);
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < words; i += (uint64_t)gridDim.x * blockDim.x) {
+        status[i] = 0;
     }
+    if (n == 0 && blockIdx.x == 0 && threadIdx.x == 0) {
+        *total = 0;
+    }
+}
+
+template <typename Policy>
+hipError_t run_scan(const Policy& pol, uint32_t max_n, uint64_t* total, uint32_t* status_words, hipStream_t s) {
+    uint64_t* status = reinterpret_cast<uint64_t*>(status_words);
+    const uint32_t parts = div_up(max_n ? max_n : 1, kTile);
+    const uint32_t clear_blocks = div_up(parts + 1, 256) < 1024 ? div_up(parts + 1, 256) : 1024;
+    hipLaunchKernelGGL(clear_status_kernel<Policy>, dim3(clear_blocks), dim3(256), 0, s, pol, status, total);
+    hipLaunchKernelGGL(scan_kernel<Policy>, dim3(parts), dim3(kThreads), 0, s, pol, total, status);
+    return hipGetLastError();
 }
 
 }  // namespace
 
-// u32 words of status a scan of n elements needs (u64 words, counted in u32).
+// u32 words of status a scan of up to n elements needs (u64 words, counted in u32).
 size_t scan_status_words(uint32_t n) { return 2 * (1 + (size_t)div_up(n ? n : 1, kTile)); }
 
 hipError_t launch_scan(const uint32_t* in, uint32_t* out, uint64_t* total, uint32_t n, uint32_t* status_words,
                        hipStream_t s) {
-    uint64_t* status = reinterpret_cast<uint64_t*>(status_words);
-    if (n == 0) {
-        hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, s, total, 1);
-        return hipGetLastError();
-    }
-    const uint32_t parts = div_up(n, kTile);
-    const uint64_t words = 1 + (uint64_t)parts;
-    hipLaunchKernelGGL(zero_u64_kernel, dim3((uint32_t)((words + 255) / 256 < 1024 ? (words + 255) / 256 : 1024)),
-                       dim3(256), 0, s, status, words);
-    const bool aligned = ((reinterpret_cast<uintptr_t>(in) | reinterpret_cast<uintptr_t>(out)) & 15u) == 0;
-    hipLaunchKernelGGL(scan_kernel, dim3(parts), dim3(kThreads), 0, s, in, out, total, n, status, aligned);
-    return hipGetLastError();
+    return run_scan(PlainScan{in, out, n}, n, total, status_words, s);
+}
+
+hipError_t launch_compact(const uint32_t* touched, const uint32_t* depth_keys, uint32_t slots, uint64_t* keys_out,
+                          uint32_t* slots_out, uint64_t* visible_total, uint32_t* status_words, hipStream_t s) {
+    return run_scan(CompactScan{touched, depth_keys, keys_out, slots_out, slots}, slots, visible_total, status_words,
+                    s);
+}
+
+hipError_t launch_gather_scan(const uint32_t* touched, const uint32_t* sorted_slots, const uint64_t* visible_dev,
+                              uint32_t max_visible, uint32_t* slot_offsets, uint64_t* pair_total,
+                              uint32_t* status_words, hipStream_t s) {
+    return run_scan(GatherScan{touched, sorted_slots, slot_offsets, visible_dev, max_visible}, max_visible, pair_total,
+                    status_words, s);
 }
 
 }  // namespace athx

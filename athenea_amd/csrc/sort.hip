@@ -60,9 +60,19 @@ __device__ inline uint32_t block_exclusive_scan(uint32_t x, uint32_t* s_tmp, uin
     return before + incl - x;
 }
 
-__global__ void __launch_bounds__(kThreads) histogram_kernel(const uint64_t* __restrict__ keys, uint32_t n,
-                                                             uint32_t passes, uint32_t* __restrict__ hist) {
+// The pair count lives in device memory (the scan's total), bounded by the
+// buffers' capacity, so a frame's launches need no host round trip.
+__device__ inline uint32_t count_of(const uint64_t* n_dev, uint32_t cap) {
+    const uint64_t n = *n_dev;
+    return n < cap ? (uint32_t)n : cap;
+}
+
+__global__ void __launch_bounds__(kThreads) histogram_kernel(const uint64_t* __restrict__ keys,
+                                                             const uint64_t* __restrict__ n_dev, uint32_t cap,
+                                                             uint32_t lo_bit, uint32_t passes,
+                                                             uint32_t* __restrict__ hist) {
     __shared__ uint32_t s_hist[kMaxPasses * kRadix];
+    const uint32_t n = count_of(n_dev, cap);
     for (uint32_t i = threadIdx.x; i < passes * kRadix; i += kThreads) {
         s_hist[i] = 0;
     }
@@ -70,7 +80,7 @@ __global__ void __launch_bounds__(kThreads) histogram_kernel(const uint64_t* __r
     for (uint32_t i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads) {
         const uint64_t k = keys[i];
         for (uint32_t p = 0; p < passes; ++p) {
-            atomicAdd(&s_hist[p * kRadix + (uint32_t)((k >> (8 * p)) & 0xFFu)], 1u);
+            atomicAdd(&s_hist[p * kRadix + (uint32_t)((k >> (lo_bit + 8 * p)) & 0xFFu)], 1u);
         }
     }
     __syncthreads();
@@ -84,8 +94,8 @@ __global__ void __launch_bounds__(kThreads) histogram_kernel(const uint64_t* __r
 __global__ void __launch_bounds__(kThreads) onesweep_kernel(const uint64_t* __restrict__ kin,
                                                             const uint32_t* __restrict__ vin,
                                                             uint64_t* __restrict__ kout, uint32_t* __restrict__ vout,
-                                                            uint32_t n, uint32_t shift,
-                                                            const uint32_t* __restrict__ hist,
+                                                            const uint64_t* __restrict__ n_dev, uint32_t cap,
+                                                            uint32_t shift, const uint32_t* __restrict__ hist,
                                                             uint32_t* __restrict__ lookback,
                                                             uint32_t* __restrict__ counter) {
     __shared__ uint64_t s_keys[kPart];
@@ -106,7 +116,14 @@ __global__ void __launch_bounds__(kThreads) onesweep_kernel(const uint64_t* __re
         s_wave_hist[i] = 0;
     }
     __syncthreads();
+    const uint32_t n = count_of(n_dev, cap);
     const uint32_t part = s_part;
+    // The grid is sized for the capacity: tickets past this frame's
+    // partitions leave at once. Nobody waits on them -- a partition waits
+    // only on lower tickets.
+    if (part >= div_up(n, kPart)) {
+        return;
+    }
     const uint32_t part_first = part * kPart;
     const uint32_t wave_first = part_first + wave * (64 * kItems);
 
@@ -206,9 +223,38 @@ __global__ void __launch_bounds__(kThreads) onesweep_kernel(const uint64_t* __re
     }
 }
 
-__global__ void zero_kernel(uint32_t* p, uint64_t n) {
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
-        p[i] = 0;
+// Clears what this frame's passes use: the histograms and tickets whole, and
+// each pass's look-back words for this frame's partitions only. Layout, as
+// SortScratch: hist[passes][256], lookback[passes][partitions(cap)][256],
+// counters[passes].
+__global__ void clear_kernel(uint32_t* __restrict__ hist, uint32_t* __restrict__ lookback,
+                             uint32_t* __restrict__ counters, uint32_t passes, const uint64_t* __restrict__ n_dev,
+                             uint32_t cap) {
+    const uint32_t n = count_of(n_dev, cap);
+    const uint64_t parts_cap = div_up(cap ? cap : 1, kPart);
+    const uint64_t per_pass = (uint64_t)div_up(n ? n : 1, kPart) * kRadix;
+    const uint64_t look = (uint64_t)passes * per_pass;
+    const uint64_t total = look + (uint64_t)passes * kRadix + passes;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (uint64_t)gridDim.x * blockDim.x) {
+        if (i < look) {
+            const uint64_t p = i / per_pass;
+            lookback[p * parts_cap * kRadix + (i - p * per_pass)] = 0;
+        } else if (i < look + (uint64_t)passes * kRadix) {
+            hist[i - look] = 0;
+        } else {
+            counters[i - look - (uint64_t)passes * kRadix] = 0;
+        }
+    }
+}
+
+// An odd number of passes leaves the result in the alternate buffers.
+__global__ void copy_back_kernel(uint64_t* __restrict__ keys, uint32_t* __restrict__ values,
+                                 const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin,
+                                 const uint64_t* __restrict__ n_dev, uint32_t cap) {
+    const uint32_t n = count_of(n_dev, cap);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        keys[i] = kin[i];
+        values[i] = vin[i];
     }
 }
 
@@ -216,43 +262,39 @@ __global__ void zero_kernel(uint32_t* p, uint64_t n) {
 
 uint32_t sort_partitions(uint32_t n) { return div_up(n ? n : 1, kPart); }
 
-// Sorts (keys, values) by the low key_bits bits. The result is back in keys
-// and values: an odd number of passes ends with a copy.
-hipError_t launch_sort(uint64_t* keys, uint32_t* values, uint32_t n, uint32_t key_bits, const SortScratch& sc,
-                       hipStream_t s) {
-    if (n < 2 || key_bits == 0) {
+// Sorts the first *n_dev (at most cap) of (keys, values) stably by bits
+// [lo_bit, key_bits) of the key, the count read on the device. Digits below
+// lo_bit are left as the input orders them: a caller whose input is already
+// in order of those bits (the pairs, emitted in depth order) sorts only the
+// rest. The result is back in keys and values.
+hipError_t launch_sort(uint64_t* keys, uint32_t* values, const uint64_t* n_dev, uint32_t cap, uint32_t lo_bit,
+                       uint32_t key_bits, const SortScratch& sc, hipStream_t s) {
+    if (cap < 2 || key_bits <= lo_bit) {
         return hipSuccess;
     }
-    if (key_bits > 64 || n > kValueMask) {
+    if (key_bits > 64 || cap > kValueMask) {
         return hipErrorInvalidValue;
     }
-    const uint32_t passes = div_up(key_bits, 8);
-    const uint32_t parts = sort_partitions(n);
-    const uint64_t clear = (uint64_t)passes * kRadix + (uint64_t)passes * parts * kRadix + passes;
-    // hist, lookback and counters are laid out back to back by the caller.
-    uint64_t blocks = (clear + 255) / 256;
-    hipLaunchKernelGGL(zero_kernel, dim3((uint32_t)(blocks < 4096 ? blocks : 4096)), dim3(256), 0, s, sc.hist, clear);
-    uint32_t hist_blocks = parts < 2048 ? parts : 2048;
-    hipLaunchKernelGGL(histogram_kernel, dim3(hist_blocks), dim3(kThreads), 0, s, keys, n, passes, sc.hist);
+    const uint32_t passes = div_up(key_bits - lo_bit, 8);
+    const uint32_t parts = sort_partitions(cap);
+    const uint32_t spread = parts < 2048 ? parts : 2048;
+    hipLaunchKernelGGL(clear_kernel, dim3(spread), dim3(256), 0, s, sc.hist, sc.lookback, sc.counters, passes, n_dev,
+                       cap);
+    hipLaunchKernelGGL(histogram_kernel, dim3(spread), dim3(kThreads), 0, s, keys, n_dev, cap, lo_bit, passes,
+                       sc.hist);
     uint64_t* kin = keys;
     uint32_t* vin = values;
     uint64_t* kout = sc.keys_alt;
     uint32_t* vout = sc.values_alt;
     for (uint32_t p = 0; p < passes; ++p) {
-        hipLaunchKernelGGL(onesweep_kernel, dim3(parts), dim3(kThreads), 0, s, kin, vin, kout, vout, n, 8 * p,
+        hipLaunchKernelGGL(onesweep_kernel, dim3(parts), dim3(kThreads), 0, s, kin, vin, kout, vout, n_dev, cap,
+                           lo_bit + 8 * p,
                            sc.hist + p * kRadix, sc.lookback + (size_t)p * parts * kRadix, sc.counters + p);
         uint64_t* tk = kin; kin = kout; kout = tk;
         uint32_t* tv = vin; vin = vout; vout = tv;
     }
     if (kin != keys) {
-        hipError_t e = hipMemcpyAsync(keys, kin, (size_t)n * sizeof(uint64_t), hipMemcpyDeviceToDevice, s);
-        if (e != hipSuccess) {
-            return e;
-        }
-        e = hipMemcpyAsync(values, vin, (size_t)n * sizeof(uint32_t), hipMemcpyDeviceToDevice, s);
-        if (e != hipSuccess) {
-            return e;
-        }
+        hipLaunchKernelGGL(copy_back_kernel, dim3(spread), dim3(256), 0, s, keys, values, kin, vin, n_dev, cap);
     }
     return hipGetLastError();
 }

@@ -98,9 +98,10 @@ class FrameStats:
     visible: int = 0
     pairs: int = 0
     project_ms: float = 0.0
-    scan_ms: float = 0.0
+    depth_sort_ms: float = 0.0
+    counts_ms: float = 0.0
     emit_ms: float = 0.0
-    sort_ms: float = 0.0
+    tile_sort_ms: float = 0.0
     ranges_ms: float = 0.0
     blend_ms: float = 0.0
     total_ms: float = 0.0
@@ -165,6 +166,92 @@ def _gaussians_c(cloud: GaussianCloud) -> _lib.Gaussians:
     return g
 
 
+def _views_and_targets(views: list, targets: list[RenderTargets]):
+    if len(targets) != len(views):
+        raise ValueError("one target per view")
+    nv = len(views)
+    c_views = (_lib.View * nv)(*views)
+    c_targets = (_lib.Targets * nv)()
+    for k, t in enumerate(targets):
+        if t.colour.shape != (views[k].height, views[k].width, 4):
+            raise ValueError(f"target {k} does not match its view's size")
+        c_targets[k].colour = _ptr(t.colour)
+        c_targets[k].depth = _ptr(t.depth)
+        c_targets[k].final_t = _ptr(t.final_t)
+        c_targets[k].last = _ptr(t.last)
+    return c_views, c_targets
+
+
+def _grads_c(grads: dict[str, torch.Tensor]) -> _lib.Grads:
+    cg = _lib.Grads()
+    for name in ("means", "log_scales", "quats", "opacity_logits", "sh"):
+        t = grads.get(name)
+        if t is not None and (t.dtype != torch.float32 or not t.is_contiguous()):
+            raise ValueError(f"gradient {name} must be a contiguous float32 tensor")
+        setattr(cg, name, _ptr(t))
+    return cg
+
+
+class FrameGraph:
+    """A frame captured once into a hipGraph (athx_graph_create) and replayed:
+    the batched, wait-free route. It owns a context of its own sized for these
+    views and this pair capacity (0: sized from one eager frame, 1.5x its
+    pairs); every replay reads whatever the Gaussian, target and gradient
+    tensors hold then. With ``dl_dcolour`` and ``grads`` the adjoint is
+    captured too (``launch_backward``; the caller zeroes ``grads``)."""
+
+    def __init__(self, views: list, cloud: GaussianCloud, settings: RenderSettings | None = None,
+                 targets: list[RenderTargets] | None = None, pair_capacity: int = 0,
+                 dl_dcolour: list[torch.Tensor] | None = None, grads: dict[str, torch.Tensor] | None = None,
+                 device: int = 0):
+        self._lib = _lib.load()
+        self.settings = settings or RenderSettings()
+        dev = cloud.means.device
+        backward = dl_dcolour is not None and grads is not None
+        self.targets = targets or [RenderTargets.allocate(v.width, v.height, dev, backward) for v in views]
+        self.cloud = cloud
+        self.dl_dcolour = [d.contiguous() for d in dl_dcolour] if backward else None
+        self.grads = grads
+        c_views, c_targets = _views_and_targets(views, self.targets)
+        g = _gaussians_c(cloud)
+        s = self.settings.to_c(cloud.linear)
+        dl = None
+        cg = None
+        if backward:
+            dl = (ctypes.c_void_p * len(views))(*[d.data_ptr() for d in self.dl_dcolour])
+            cg = ctypes.byref(_grads_c(grads))
+        handle = ctypes.c_void_p()
+        err = ctypes.create_string_buffer(512)
+        status = self._lib.athx_graph_create(device, c_views, len(views), ctypes.byref(g), ctypes.byref(s),
+                                             c_targets, int(pair_capacity), dl, cg, ctypes.byref(handle), err, 512)
+        if status != 0:
+            raise _lib.AthxError(status, err.value.decode())
+        self._graph = handle
+
+    def launch(self) -> None:
+        _lib.check(self._lib.athx_graph_launch(self._graph, _stream_handle(self.cloud.means.device)))
+
+    def launch_backward(self) -> None:
+        _lib.check(self._lib.athx_graph_launch_backward(self._graph, _stream_handle(self.cloud.means.device)))
+
+    def pairs(self) -> tuple[int, int]:
+        """(pair total, capacity) of the last replay; waits for it."""
+        pairs, cap = ctypes.c_uint64(), ctypes.c_uint64()
+        _lib.check(self._lib.athx_graph_pairs(self._graph, ctypes.byref(pairs), ctypes.byref(cap)))
+        return int(pairs.value), int(cap.value)
+
+    def close(self) -> None:
+        if getattr(self, "_graph", None):
+            self._lib.athx_graph_destroy(self._graph)
+            self._graph = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class TileRasterizer:
     """One context of libathx (one device, one host thread at a time)."""
 
@@ -226,9 +313,39 @@ class TileRasterizer:
             self._lib.athx_render(self._ctx, c_views, nv, ctypes.byref(g), ctypes.byref(s), c_targets,
                                   _stream_handle(dev), ctypes.byref(stats))
         )
-        out = FrameStats(stats.splats, stats.visible, stats.pairs, stats.project_ms, stats.scan_ms, stats.emit_ms,
-                         stats.sort_ms, stats.ranges_ms, stats.blend_ms, stats.total_ms)
+        out = FrameStats(stats.splats, stats.visible, stats.pairs, stats.project_ms, stats.depth_sort_ms,
+                         stats.counts_ms, stats.emit_ms, stats.tile_sort_ms, stats.ranges_ms, stats.blend_ms,
+                         stats.total_ms)
         return targets, out
+
+    def render_async(
+        self,
+        views: list,
+        cloud: GaussianCloud,
+        settings: RenderSettings | None = None,
+        targets: list[RenderTargets] | None = None,
+        pair_capacity: int = 0,
+        keep_backward: bool = False,
+    ) -> list[RenderTargets]:
+        """athx_render_async: the frame with no host wait, its pair buffers
+        sized for ``pair_capacity`` (0: 1.5x the largest frame seen). Check
+        ``frame_pairs()`` for a frame that outgrew its capacity."""
+        settings = settings or RenderSettings()
+        dev = cloud.means.device
+        if targets is None:
+            targets = [RenderTargets.allocate(v.width, v.height, dev, keep_backward) for v in views]
+        c_views, c_targets = _views_and_targets(views, targets)
+        g = _gaussians_c(cloud)
+        s = settings.to_c(cloud.linear)
+        self._check(self._lib.athx_render_async(self._ctx, c_views, len(views), ctypes.byref(g), ctypes.byref(s),
+                                                c_targets, int(pair_capacity), _stream_handle(dev)))
+        return targets
+
+    def frame_pairs(self) -> tuple[int, int]:
+        """(pair total, pair capacity) of the last frame; waits for it."""
+        pairs, cap = ctypes.c_uint64(), ctypes.c_uint64()
+        self._check(self._lib.athx_frame_pairs(self._ctx, ctypes.byref(pairs), ctypes.byref(cap)))
+        return int(pairs.value), int(cap.value)
 
     def render_backward(
         self,

@@ -37,20 +37,24 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def stage_bytes(n: int, pairs: int, tiles: int, pixels: int, key_bits: int) -> dict:
+def stage_bytes(n: int, visible: int, pairs: int, tiles: int, pixels: int, key_bits: int) -> dict:
     """Compulsory HBM bytes per stage (one view): what each stage must read
     and write whatever its implementation."""
-    passes = (key_bits + 7) // 8
+    depth_passes = 3
+    tile_passes = (key_bits - 24 + 7) // 8
     return {
-        # FP32 parameters read (236 B at SH degree 3), record, tile count and key written
+        # FP32 parameters read (236 B at SH degree 3); record, tile count and key written
         "project": n * (236 + 48 + 4 + 4),
-        # tile counts read, offsets written
-        "scan": n * 8,
-        # offsets, counts, keys and records read; (key, value) pairs written
-        "emit": n * (4 + 4 + 4 + 48) + pairs * 12,
-        # one read of the keys for the histograms, then a read and a write of
-        # (key, value) a pass
-        "sort": pairs * 8 + passes * pairs * 24,
+        # compaction (counts and keys read, visible (key, slot) written), then
+        # a stable 24-bit sort of the visible: a histogram read and a read and
+        # write of (key, slot) a pass
+        "depth_sort": n * 8 + visible * 12 + visible * 8 + depth_passes * visible * 24,
+        # counts gathered in depth order, scanned, offsets scattered to slots
+        "counts": visible * 16,
+        # records, offsets and keys read; (key, value) pairs written
+        "emit": visible * (48 + 4 + 4 + 4) + pairs * 12,
+        # a histogram read of the keys, then a read and a write of (key, value) a pass
+        "tile_sort": pairs * 8 + tile_passes * pairs * 24,
         "ranges": tiles * 8,
         # each pair's value and record read once; RGBA, depth, final T, last index written
         "blend": pairs * (4 + 48) + tiles * 8 + pixels * 28,
@@ -127,7 +131,7 @@ def main() -> int:
 
     # Per-stage times from HIP events recorded on the rasterizer's stream.
     staged = RenderSettings(tile=args.tile, batch=args.batch, time_stages=True, count_splats=True)
-    stage_samples = {k: [] for k in ("project", "scan", "emit", "sort", "ranges", "blend")}
+    stage_samples = {k: [] for k in ("project", "depth_sort", "counts", "emit", "tile_sort", "ranges", "blend")}
     for _ in range(max(3, min(args.steps, 10))):
         targets, s = raster.render([view], cloud, staged, targets)
         for k in stage_samples:
@@ -135,7 +139,7 @@ def main() -> int:
     stages = {k: statistics.mean(v) for k, v in stage_samples.items()}
     counters = raster.latest_counters()
     tiles = info["tiles"]
-    sbytes = stage_bytes(cloud.count, pairs, tiles, W * H, info["key_bits"])
+    sbytes = stage_bytes(cloud.count, counters.visible, pairs, tiles, W * H, info["key_bits"])
     dominant = max(stages, key=stages.get)
     achieved = sbytes[dominant] / (stages[dominant] * 1e-3)
     roofline = {
@@ -172,7 +176,7 @@ def main() -> int:
         "pairs": pairs,
         "visible": counters.visible,
         "stages_ms": {k: round(v, 4) for k, v in stages.items()},
-        "sort_keys_per_s": round(pairs / (stages["sort"] * 1e-3)),
+        "sort_keys_per_s": round(pairs / (stages["tile_sort"] * 1e-3)),
         "roofline": roofline,
     }
 
@@ -222,6 +226,20 @@ def main() -> int:
         torch.cuda.synchronize()
         batch_ms = (time.perf_counter() - t0) * 1e3 / reps
         del bt
+        # The same batch as one captured graph: no host wait, one launch.
+        from athenea_amd.raster import FrameGraph
+
+        graph = FrameGraph(views, walk, settings)
+        graph.launch()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            graph.launch()
+        torch.cuda.synchronize()
+        graph_ms = (time.perf_counter() - t0) * 1e3 / reps
+        gpairs, gcap = graph.pairs()
+        graph.close()
+        del graph
         single = [None] * len(views)
         for k, v in enumerate(views):
             single[k], _ = raster.render([v], walk, settings, single[k])
@@ -234,11 +252,13 @@ def main() -> int:
         single_ms = (time.perf_counter() - t0) * 1e3 / reps
         result["batched"] = {
             "workload": "walk_60: 2.36M Gaussians, 60 views at 960x540 on a circular walk",
-            "views_per_s": round(60 / (batch_ms * 1e-3), 1),
-            "ms_per_view_batched": round(batch_ms / 60, 4),
+            "views_per_s": round(60 / (graph_ms * 1e-3), 1),
+            "ms_per_view_graph": round(graph_ms / 60, 4),
+            "ms_per_view_batched_eager": round(batch_ms / 60, 4),
             "ms_per_view_single": round(single_ms / 60, 4),
-            "speedup": round(single_ms / batch_ms, 3),
+            "speedup_graph_vs_single": round(single_ms / graph_ms, 3),
             "pairs": int(bstats.pairs),
+            "graph_pairs_within_capacity": bool(gpairs <= gcap),
         }
         raster.release_scratch()
 

@@ -156,12 +156,15 @@ typedef struct athx_targets {
     uint32_t* last;   /* [height][width] or NULL */
 } athx_targets;
 
-/* FrameStats (TileRasterizer.h:408-419), summed over the views. */
+/* FrameStats (TileRasterizer.h:408-419), summed over the views. The stage
+ * times (filled with time_stages) are HIP-event times on the frame's stream:
+ * projection; compaction + depth sort; tile counts gathered and scanned in
+ * depth order; emission; tile sort; ranges; blend. */
 typedef struct athx_frame_stats {
     uint32_t splats;
     uint32_t visible;
     uint64_t pairs;
-    double project_ms, scan_ms, emit_ms, sort_ms, ranges_ms, blend_ms, total_ms;
+    double project_ms, depth_sort_ms, counts_ms, emit_ms, tile_sort_ms, ranges_ms, blend_ms, total_ms;
 } athx_frame_stats;
 
 /* SplatCounters (TileRasterizer.h:370-401) for the whole frame. */
@@ -211,6 +214,47 @@ ATHX_API athx_status athx_render(athx_ctx* ctx, const athx_view* views, uint32_t
                                  const athx_targets* targets, void* stream, athx_frame_stats* stats);
 
 /*
+ * athx_render without its wait: the pair buffers are sized for
+ * pair_capacity pairs (0: 1.5x the largest frame this context has seen) and
+ * every count after the scan is read on the device, so nothing waits for the
+ * host. A frame with more pairs than the capacity draws incompletely (the
+ * reference's emit drops pairs past its capacity the same way,
+ * shaders/athenea/splat/splat_emit.slang:43-46): athx_frame_pairs says so.
+ */
+ATHX_API athx_status athx_render_async(athx_ctx* ctx, const athx_view* views, uint32_t nviews,
+                                       const athx_gaussians* gaussians, const athx_settings* settings,
+                                       const athx_targets* targets, uint64_t pair_capacity, void* stream);
+
+/* The last frame's pair total and its pair capacity; waits for an async
+ * frame. pairs > capacity: that frame was incomplete, draw it again with
+ * a capacity of at least pairs. */
+ATHX_API athx_status athx_frame_pairs(athx_ctx* ctx, uint64_t* pairs, uint64_t* capacity);
+
+/*
+ * A frame captured once into a hipGraph and replayed: the batched, wait-free
+ * route. athx_graph_create renders nothing; it sizes a context of its own for
+ * these views, Gaussians, targets and pair capacity, uploads the views, and
+ * captures the async frame's launches (and, when dl_dcolour and grads are
+ * given, the adjoint's: dL/dcolour read from dl_dcolour[k], gradients added
+ * into grads, which the caller zeroes). Every replay reads the buffers the
+ * pointers name at that time: change the Gaussians in place, replay, and the
+ * frame is of the new values. Replays on one graph must not overlap.
+ */
+typedef struct athx_graph athx_graph;
+ATHX_API athx_status athx_graph_create(int device, const athx_view* views, uint32_t nviews,
+                                       const athx_gaussians* gaussians, const athx_settings* settings,
+                                       const athx_targets* targets, uint64_t pair_capacity,
+                                       const float* const* dl_dcolour, const athx_grads* grads, athx_graph** out,
+                                       char* error, uint32_t error_bytes);
+/* Replays the forward frame on stream. */
+ATHX_API athx_status athx_graph_launch(athx_graph* graph, void* stream);
+/* Replays the adjoint (the forward must have been replayed before it). */
+ATHX_API athx_status athx_graph_launch_backward(athx_graph* graph, void* stream);
+/* The last replay's pair total (waits for it) and the graph's capacity. */
+ATHX_API athx_status athx_graph_pairs(athx_graph* graph, uint64_t* pairs, uint64_t* capacity);
+ATHX_API void athx_graph_destroy(athx_graph* graph);
+
+/*
  * The adjoint of the last athx_render on ctx (same views, Gaussians and
  * settings; targets[k].final_t and .last must have been written by it):
  * dL/dcolour is [height][width][4] per view (premultiplied RGBA, FP32), and
@@ -237,7 +281,9 @@ ATHX_API athx_status athx_latest_counters(athx_ctx* ctx, athx_counters* out);
  *                written for visible slots only
  *   TOUCHED      uint32[slots]: tiles the footprint touches, 0 when culled
  *   DEPTH_KEYS   uint32[slots]: asuint(z) >> 7, or 0xFF000000 | cull reason
- *   OFFSETS      uint32[slots]: exclusive scan of TOUCHED
+ *   OFFSETS      uint32[slots]: a visible slot's first pair -- the exclusive
+ *                scan of TOUCHED taken in depth order (visible slots stably
+ *                sorted by depth key); undefined for culled slots
  *   PAIR_KEYS    uint64[pairs]: global tile << 24 | depth key, sorted
  *   PAIR_VALUES  uint32[pairs]: the slot of each sorted pair
  *   RANGES       uint32[tiles][2]: a tile's [start, end) in the sorted pairs
@@ -259,7 +305,8 @@ typedef struct athx_frame_info {
     uint64_t pairs;
 } athx_frame_info;
 
-ATHX_API athx_status athx_frame_info_get(const athx_ctx* ctx, athx_frame_info* out);
+/* The last frame's sizes; pairs is what its buffers hold (waits for an async frame). */
+ATHX_API athx_status athx_frame_info_get(athx_ctx* ctx, athx_frame_info* out);
 /* Copies `bytes` of one part (which must be its whole size) to device memory dst. Queued. */
 ATHX_API athx_status athx_copy_frame(athx_ctx* ctx, athx_frame_part part, void* dst, uint64_t bytes, void* stream);
 

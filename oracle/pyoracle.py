@@ -63,6 +63,9 @@ def load():
     if _lib is None:
         if not LIB_PATH.exists():
             build()
+        sources = [HERE / "oracle.c", HERE.parent / "include" / "athx_detmath.h"]
+        if any(src.stat().st_mtime > LIB_PATH.stat().st_mtime + 2.0 for src in sources):
+            raise RuntimeError(f"{LIB_PATH} is older than its sources: rebuild it (make -C oracle)")
         lib = ctypes.CDLL(str(LIB_PATH))
         vp = ctypes.POINTER(OrcView)
         lib.orc_project.argtypes = [vp, _f32p, _f32p, _f32p, _f32p, _f32p, ctypes.c_uint32, ctypes.c_uint32, _f32p,
@@ -185,7 +188,15 @@ def render(views: list[OrcView], cloud, with_f64: bool = False) -> dict:
         tile_bases.append(total_tiles)
         tx, ty = tiles_of(v)
         total_tiles += tx * ty
-    offsets, pairs = exclusive_scan(touched)
+    # Each slot's first pair, its tiles counted in the reference's depth order
+    # (TileRasterizer.cpp:1111-1137): visible slots stably sorted by the 24-bit
+    # depth key, counts scanned in that order. The full key sort below does not
+    # depend on this order; the HIP path's per-slot offsets do.
+    visible = np.nonzero(touched)[0]
+    order = visible[np.argsort(keys[visible] & np.uint32(0xFFFFFF), kind="stable")]
+    sorted_offsets, pairs = exclusive_scan(touched[order])
+    offsets = np.zeros(nv * n, np.uint32)
+    offsets[order] = sorted_offsets
     pk = np.zeros(pairs, np.uint64)
     pv = np.zeros(pairs, np.uint32)
     for k, v in enumerate(views):
