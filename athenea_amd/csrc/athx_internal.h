@@ -64,12 +64,25 @@ struct ProjectArgs {
 
 __host__ __device__ inline uint32_t div_up(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
 
+// Which tiles each footprint touches, kept from the count for the emission:
+// the reference tests every tile twice (splat_project.slang:764-776 and
+// splat_emit.slang:37-49); here the second pass reads the first one's answer.
+struct Footprints {
+    uint2* rects;             // [slots] tile rect: x0 | y0 << 16, w | h << 16
+    uint64_t* masks;          // [slots] a small footprint's hits, bit k = tile k of its rect, row-major
+    uint32_t* big_list;       // [slots] slots whose rect holds over kBigRect tiles
+    uint32_t* big_mask_base;  // [slots] where a big footprint's hit words start, or ~0u
+    uint64_t* big_masks;      // [big_mask_cap] 64 tiles a word, row-major
+    uint32_t big_mask_cap;
+    uint32_t* counters;       // [0] big footprints, [1] big mask words taken
+};
+constexpr uint32_t kBigRect = 64;
+
 // Launchers (each returns the hipError_t of its launches).
 // Splats whose footprint spans many tiles are listed in big_list (capacity:
 // one word a slot; big_count one word) and counted a wavefront each.
 hipError_t launch_project(const ProjectArgs& a, const DView* views_dev, uint32_t nviews, float* records,
-                          uint32_t* touched, uint32_t* depth_keys, uint32_t* big_list, uint32_t* big_count,
-                          hipStream_t s);
+                          uint32_t* touched, uint32_t* depth_keys, const Footprints& fp, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t value, uint64_t n, hipStream_t s);
 size_t scan_status_words(uint32_t n);
 hipError_t launch_scan(const uint32_t* in, uint32_t* out, uint64_t* total, uint32_t n, uint32_t* status,
@@ -79,8 +92,7 @@ hipError_t launch_scan(const uint32_t* in, uint32_t* out, uint64_t* total, uint3
 // capacity is, splat_emit.slang:43-46).
 hipError_t launch_emit(const DView* views_dev, uint32_t nviews, uint32_t n, uint32_t tile, const float* records,
                        const uint32_t* touched, const uint32_t* depth_keys, const uint32_t* offsets,
-                       const uint32_t* big_list, const uint32_t* big_count, uint32_t cap, uint64_t* keys,
-                       uint32_t* values, hipStream_t s);
+                       const Footprints& fp, uint32_t cap, uint64_t* keys, uint32_t* values, hipStream_t s);
 hipError_t launch_ranges(const uint64_t* sorted_keys, const uint64_t* n_dev, uint32_t cap, uint32_t tiles,
                          uint32_t* ranges, hipStream_t s);
 hipError_t launch_blend(const DView* views_dev, const uint32_t* view_of_tile_base, uint32_t nviews,

@@ -138,12 +138,31 @@ __global__ void __launch_bounds__(TILE * TILE) blend_kernel(const DView* __restr
     }
 }
 
-__device__ inline float wave_sum_f(float x) {
-#pragma unroll
-    for (uint32_t d = 32; d >= 1; d >>= 1) {
-        x += __shfl_xor(x, d, 64);
+// The sum over a wavefront's 64 lanes, every lane active: DPP moves within
+// each row of 16 lanes (quad swaps, then rotations by 4 and 8) leave each row's
+// sum in all of its lanes, and four lane reads add the rows. All VALU, no LDS.
+__device__ inline float dpp_add(float x, int ctrl_sel) {
+    int moved = 0;
+    switch (ctrl_sel) {
+    case 0: moved = __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, x), 0xB1, 0xF, 0xF, false); break;   // quad [1,0,3,2]
+    case 1: moved = __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, x), 0x4E, 0xF, 0xF, false); break;   // quad [2,3,0,1]
+    case 2: moved = __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, x), 0x124, 0xF, 0xF, false); break;  // row_ror 4
+    default: moved = __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, x), 0x128, 0xF, 0xF, false); break; // row_ror 8
     }
-    return x;
+    return x + __builtin_bit_cast(float, moved);
+}
+
+__device__ inline float wave_sum_f(float x) {
+    x = dpp_add(x, 0);
+    x = dpp_add(x, 1);
+    x = dpp_add(x, 2);
+    x = dpp_add(x, 3);
+    const int bits = __builtin_bit_cast(int, x);
+    const float r0 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(bits, 0));
+    const float r1 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(bits, 16));
+    const float r2 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(bits, 32));
+    const float r3 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(bits, 48));
+    return (r0 + r1) + (r2 + r3);
 }
 
 template <uint32_t TILE>

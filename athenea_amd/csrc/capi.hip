@@ -32,6 +32,10 @@ struct DeviceBuffer {
     size_t bytes = 0;
 };
 
+// Hit words kept for big footprints: one a slot is room for 64 tiles a slot
+// on average; a footprint past the room has its tiles tested again.
+uint32_t big_mask_words(uint32_t slots) { return slots + 1024; }
+
 constexpr uint32_t kDepthPasses = 3;  // 24-bit depth keys (splat_project.slang:1129; TileRasterizer.cpp:1122)
 
 uint32_t bits_for(uint32_t values) {
@@ -53,6 +57,7 @@ struct athx_ctx {
     DeviceBuffer views, tile_bases, records, touched, depth_keys, offsets, scan_status, totals;
     DeviceBuffer pair_keys, pair_vals, alt_keys, alt_vals, sort_meta, ranges, grad2d, counter_words, big_list;
     DeviceBuffer depth_keys64, depth_slots, depth_alt_keys, depth_alt_slots, depth_sort_meta;
+    DeviceBuffer fp_rects, fp_masks, fp_big_base, fp_big_masks;
     DeviceBuffer api_sort_meta, api_alt_keys, api_alt_vals, api_scan_status, api_count;
     uint64_t* host_totals = nullptr;  // pinned: [0] pairs, [1] visible
     DView* host_views = nullptr;      // pinned staging for the views
@@ -360,7 +365,23 @@ athx_status reserve_front(athx_ctx* ctx, const FrameShape& fs) {
     ATHX_HIP(ctx, reserve(ctx, ctx->depth_alt_slots, slots * 4 + 16), "depth sort slots");
     ATHX_HIP(ctx, reserve(ctx, ctx->depth_sort_meta, sort_meta_words(fs.slots, kDepthPasses) * 4 + 16),
              "depth sort metadata");
+    ATHX_HIP(ctx, reserve(ctx, ctx->fp_rects, slots * 8 + 16), "footprint rects");
+    ATHX_HIP(ctx, reserve(ctx, ctx->fp_masks, slots * 8 + 16), "footprint masks");
+    ATHX_HIP(ctx, reserve(ctx, ctx->fp_big_base, slots * 4 + 16), "big footprint bases");
+    ATHX_HIP(ctx, reserve(ctx, ctx->fp_big_masks, big_mask_words(fs.slots) * 8 + 16), "big footprint masks");
     return ATHX_OK;
+}
+
+Footprints footprints_of(athx_ctx* ctx, uint32_t slots) {
+    Footprints fp{};
+    fp.rects = as<uint2>(ctx->fp_rects);
+    fp.masks = as<uint64_t>(ctx->fp_masks);
+    fp.big_list = as<uint32_t>(ctx->big_list);
+    fp.big_mask_base = as<uint32_t>(ctx->fp_big_base);
+    fp.big_masks = as<uint64_t>(ctx->fp_big_masks);
+    fp.big_mask_cap = big_mask_words(slots);
+    fp.counters = as<uint32_t>(ctx->totals) + 4;  // totals: [0..1] pairs, [2..3] visible, [4] big, [5] big words
+    return fp;
 }
 
 athx_status reserve_back(athx_ctx* ctx, const FrameShape& fs) {
@@ -373,7 +394,6 @@ athx_status reserve_back(athx_ctx* ctx, const FrameShape& fs) {
     return ATHX_OK;
 }
 
-uint32_t* big_count_of(athx_ctx* ctx) { return as<uint32_t>(ctx->totals) + 4; }  // [0..1] pair total, [4] big list
 
 // Projection, then the reference's depth order (TileRasterizer.cpp:1111-1137):
 // the visible slots compacted in slot order, stably sorted by their 24-bit
@@ -387,8 +407,8 @@ athx_status enqueue_front(athx_ctx* ctx, const ProjectArgs& pa, const FrameShape
         ATHX_HIP(ctx, hipEventRecord(ctx->ev[0], s), "event");
     }
     ATHX_HIP(ctx, launch_project(pa, as<DView>(ctx->views), fs.nviews, as<float>(ctx->records),
-                                 as<uint32_t>(ctx->touched), as<uint32_t>(ctx->depth_keys), as<uint32_t>(ctx->big_list),
-                                 big_count_of(ctx), s),
+                                 as<uint32_t>(ctx->touched), as<uint32_t>(ctx->depth_keys),
+                                 footprints_of(ctx, fs.slots), s),
              "project");
     if (timed) {
         ATHX_HIP(ctx, hipEventRecord(ctx->ev[1], s), "event");
@@ -422,8 +442,8 @@ athx_status enqueue_back(athx_ctx* ctx, const ProjectArgs& pa, const FrameShape&
     const uint64_t* pairs_dev = as<uint64_t>(ctx->totals);
     ATHX_HIP(ctx, launch_emit(as<DView>(ctx->views), fs.nviews, pa.n, settings->tile, as<float>(ctx->records),
                               as<uint32_t>(ctx->touched), as<uint32_t>(ctx->depth_keys), as<uint32_t>(ctx->offsets),
-                              as<uint32_t>(ctx->big_list), big_count_of(ctx), fs.capacity,
-                              as<uint64_t>(ctx->pair_keys), as<uint32_t>(ctx->pair_vals), s),
+                              footprints_of(ctx, fs.slots), fs.capacity, as<uint64_t>(ctx->pair_keys),
+                              as<uint32_t>(ctx->pair_vals), s),
              "emit");
     if (timed) {
         ATHX_HIP(ctx, hipEventRecord(ctx->ev[4], s), "event");
@@ -547,7 +567,8 @@ ATHX_API void athx_destroy(athx_ctx* ctx) {
                             &ctx->alt_keys, &ctx->alt_vals, &ctx->sort_meta, &ctx->ranges, &ctx->grad2d,
                             &ctx->counter_words, &ctx->api_sort_meta, &ctx->api_alt_keys, &ctx->api_alt_vals,
                             &ctx->api_scan_status, &ctx->big_list, &ctx->api_count, &ctx->depth_keys64,
-                            &ctx->depth_slots, &ctx->depth_alt_keys, &ctx->depth_alt_slots, &ctx->depth_sort_meta}) {
+                            &ctx->depth_slots, &ctx->depth_alt_keys, &ctx->depth_alt_slots, &ctx->depth_sort_meta,
+                            &ctx->fp_rects, &ctx->fp_masks, &ctx->fp_big_base, &ctx->fp_big_masks}) {
         release(*b);
     }
     if (ctx->host_totals) {
@@ -583,7 +604,8 @@ ATHX_API athx_status athx_release_scratch(athx_ctx* ctx) {
     ATHX_HIP(ctx, hipDeviceSynchronize(), "synchronize before release");
     for (DeviceBuffer* b : {&ctx->records, &ctx->touched, &ctx->depth_keys, &ctx->offsets, &ctx->scan_status,
                             &ctx->big_list, &ctx->depth_keys64, &ctx->depth_slots, &ctx->depth_alt_keys,
-                            &ctx->depth_alt_slots, &ctx->depth_sort_meta, &ctx->pair_keys, &ctx->pair_vals,
+                            &ctx->depth_alt_slots, &ctx->depth_sort_meta, &ctx->fp_rects, &ctx->fp_masks,
+                            &ctx->fp_big_base, &ctx->fp_big_masks, &ctx->pair_keys, &ctx->pair_vals,
                             &ctx->alt_keys, &ctx->alt_vals, &ctx->sort_meta,
                             &ctx->ranges, &ctx->grad2d, &ctx->api_sort_meta, &ctx->api_alt_keys,
                             &ctx->api_alt_vals, &ctx->api_scan_status}) {
@@ -604,7 +626,8 @@ ATHX_API uint64_t athx_scratch_bytes(const athx_ctx* ctx) {
                                   &ctx->alt_keys, &ctx->alt_vals, &ctx->sort_meta, &ctx->ranges, &ctx->grad2d,
                                   &ctx->counter_words, &ctx->api_sort_meta, &ctx->api_alt_keys, &ctx->api_alt_vals,
                                   &ctx->api_scan_status, &ctx->big_list, &ctx->depth_keys64, &ctx->depth_slots,
-                                  &ctx->depth_alt_keys, &ctx->depth_alt_slots, &ctx->depth_sort_meta}) {
+                                  &ctx->depth_alt_keys, &ctx->depth_alt_slots, &ctx->depth_sort_meta,
+                                  &ctx->fp_rects, &ctx->fp_masks, &ctx->fp_big_base, &ctx->fp_big_masks}) {
         sum += b->bytes;
     }
     return sum;

@@ -22,10 +22,6 @@ constexpr float kC3_0 = -0.5900435899266435f, kC3_1 = 2.890611442640554f, kC3_2 
                 kC3_3 = 0.3731763325901154f, kC3_4 = -0.4570457994644658f, kC3_5 = 1.445305721320277f,
                 kC3_6 = -0.5900435899266435f;
 
-// Footprints whose tile rectangle holds more tiles than this are counted and
-// emitted a wavefront per splat rather than a thread per splat.
-constexpr uint32_t kBigRect = 64;
-
 __device__ inline float fmin_(float a, float b) { return a < b ? a : b; }
 __device__ inline float fmax_(float a, float b) { return a > b ? a : b; }
 __device__ inline float clamp_(float x, float lo, float hi) { return fmin_(fmax_(x, lo), hi); }
@@ -138,8 +134,7 @@ __device__ inline void cull(uint32_t slot, uint32_t reason, uint32_t* touched, u
 template <bool kHalf>
 __global__ void __launch_bounds__(256) project_kernel(ProjectArgs a, const DView* __restrict__ views, uint32_t nviews,
                                                       float* __restrict__ records, uint32_t* __restrict__ touched,
-                                                      uint32_t* __restrict__ keys, uint32_t* __restrict__ big_list,
-                                                      uint32_t* __restrict__ big_count) {
+                                                      uint32_t* __restrict__ keys, Footprints fp) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.n) {
         return;
@@ -253,21 +248,30 @@ __global__ void __launch_bounds__(256) project_kernel(ProjectArgs a, const DView
         // (splat_project.slang:764-776). A footprint over kBigRect tiles is
         // counted by a whole wavefront afterwards (count_big_kernel): one
         // thread walking thousands of tiles would hold its wave back.
+        // The hits are kept as a bitmask, so the emission need not test again.
         uint32_t count = 0;
-        const bool big = (x1 - x0 + 1) * (y1 - y0 + 1) > kBigRect;
+        const uint32_t w = x1 - x0 + 1, h = y1 - y0 + 1;
+        const bool big = w * h > kBigRect;
         if (big) {
-            big_list[atomicAdd(big_count, 1u)] = slot;
+            fp.big_list[atomicAdd(&fp.counters[0], 1u)] = slot;
         } else {
+            uint64_t mask = 0;
+            uint32_t k = 0;
             for (uint32_t ty_ = y0; ty_ <= y1; ++ty_) {
-                for (uint32_t tx_ = x0; tx_ <= x1; ++tx_) {
-                    count += tile_touched(V.width, V.height, a.tile, cx, cy, ka, kb, kc, cutoff, tx_, ty_) ? 1u : 0u;
+                for (uint32_t tx_ = x0; tx_ <= x1; ++tx_, ++k) {
+                    if (tile_touched(V.width, V.height, a.tile, cx, cy, ka, kb, kc, cutoff, tx_, ty_)) {
+                        mask |= 1ull << k;
+                    }
                 }
             }
+            count = (uint32_t)__popcll(mask);
             if (count == 0) {
                 cull(slot, ATHX_CULL_NO_TILE, touched, keys);
                 continue;
             }
+            fp.masks[slot] = mask;
         }
+        fp.rects[slot] = make_uint2(x0 | (y0 << 16), w | (h << 16));
         // Colour from this eye, in the cloud's space (splat_project.slang:781-785).
         const float dx = px - V.eye[0], dy = py - V.eye[1], dz = pz - V.eye[2];
         const float dinv = 1.0f / sqrtf(dx * dx + dy * dy + dz * dz);
@@ -298,13 +302,13 @@ __global__ void fill_u32_kernel(uint32_t* p, uint32_t value, uint64_t n) {
 }
 
 // One (tile, depth) pair per tile the footprint touches, a slot's tiles row
-// by row, at the slot's scanned offset: splat_emit.slang:17-50's test, emitted
-// in slot order with the 64-bit key (global tile << 24 | 24-bit depth key).
+// by row, at the slot's scanned offset: splat_emit.slang:17-50, emitted with
+// the 64-bit key (global tile << 24 | 24-bit depth key). The tiles come from
+// the projection's hit mask; a big footprint is emit_big_kernel's.
 __global__ void __launch_bounds__(256) emit_kernel(const DView* __restrict__ views, uint32_t nviews, uint32_t n,
-                                                   uint32_t T, const float* __restrict__ records,
                                                    const uint32_t* __restrict__ touched,
                                                    const uint32_t* __restrict__ depth_keys,
-                                                   const uint32_t* __restrict__ offsets, uint32_t cap,
+                                                   const uint32_t* __restrict__ offsets, Footprints fp, uint32_t cap,
                                                    uint64_t* __restrict__ keys, uint32_t* __restrict__ values) {
     const uint64_t slot = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (slot >= (uint64_t)n * nviews) {
@@ -314,27 +318,21 @@ __global__ void __launch_bounds__(256) emit_kernel(const DView* __restrict__ vie
     if (expected == 0) {
         return;
     }
-    const DView& V = views[slot / n];
-    const float4* rec = reinterpret_cast<const float4*>(records + slot * kRecordFloats);
-    const float4 screen = rec[0];
-    const float4 conic = rec[1];
-    uint32_t x0, y0, x1, y1;
-    tile_rect(V.width, V.height, T, screen.x, screen.y, conic.x, conic.y, conic.z, screen.w, x0, y0, x1, y1);
-    if ((x1 - x0 + 1) * (y1 - y0 + 1) > kBigRect) {
-        return;  // emit_big_kernel's
+    const uint2 rect = fp.rects[slot];
+    const uint32_t x0 = rect.x & 0xFFFFu, y0 = rect.x >> 16, w = rect.y & 0xFFFFu, h = rect.y >> 16;
+    if (w * h > kBigRect) {
+        return;
     }
+    const DView& V = views[slot / n];
     const uint64_t depth = depth_keys[slot] & 0xFFFFFFu;
+    const uint64_t row_base = (uint64_t)V.tile_base + (uint64_t)y0 * V.tiles_x + x0;
     uint32_t at = offsets[slot];
     const uint32_t stop = (uint64_t)at + expected < cap ? at + expected : cap;
-    for (uint32_t ty = y0; ty <= y1 && at < stop; ++ty) {
-        for (uint32_t tx = x0; tx <= x1 && at < stop; ++tx) {
-            if (tile_touched(V.width, V.height, T, screen.x, screen.y, conic.x, conic.y, conic.z, screen.w, tx, ty)) {
-                const uint64_t tile = (uint64_t)V.tile_base + (uint64_t)ty * V.tiles_x + tx;
-                keys[at] = (tile << kDepthBits) | depth;
-                values[at] = (uint32_t)slot;
-                ++at;
-            }
-        }
+    for (uint64_t m = fp.masks[slot]; m != 0 && at < stop; m &= m - 1, ++at) {
+        const uint32_t k = (uint32_t)__builtin_ctzll(m);
+        const uint64_t tile = row_base + (uint64_t)(k / w) * V.tiles_x + (k % w);
+        keys[at] = (tile << kDepthBits) | depth;
+        values[at] = (uint32_t)slot;
     }
 }
 
@@ -372,19 +370,31 @@ __device__ inline bool big_hit(const BigFootprint& f, uint32_t T, uint32_t k) {
 
 __global__ void __launch_bounds__(kBigThreads) count_big_kernel(const DView* __restrict__ views, uint32_t n,
                                                                 uint32_t T, const float* __restrict__ records,
-                                                                const uint32_t* __restrict__ big_list,
-                                                                const uint32_t* __restrict__ big_count,
-                                                                uint32_t* __restrict__ touched,
+                                                                Footprints fp, uint32_t* __restrict__ touched,
                                                                 uint32_t* __restrict__ keys) {
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t waves = gridDim.x * (kBigThreads / 64);
-    const uint32_t total = *big_count;
+    const uint32_t total = fp.counters[0];
     for (uint32_t b = blockIdx.x * (kBigThreads / 64) + (threadIdx.x >> 6); b < total; b += waves) {
-        const uint32_t slot = big_list[b];
+        const uint32_t slot = fp.big_list[b];
         const BigFootprint f = big_footprint(views, n, T, records, slot);
+        const uint32_t words = div_up(f.area, 64);
+        // Room for this footprint's hit words, if the buffer has it; if not,
+        // the emission tests its tiles again.
+        uint32_t base = 0;
+        if (lane == 0) {
+            base = atomicAdd(&fp.counters[1], words);
+            fp.big_mask_base[b] = (uint64_t)base + words <= fp.big_mask_cap ? base : ~0u;
+        }
+        base = __shfl(base, 0, 64);
+        const bool keep = (uint64_t)base + words <= fp.big_mask_cap;
         uint32_t count = 0;
         for (uint32_t k0 = 0; k0 < f.area; k0 += 64) {
-            count += (uint32_t)__popcll(__ballot(big_hit(f, T, k0 + lane)));
+            const uint64_t hits = __ballot(big_hit(f, T, k0 + lane));
+            count += (uint32_t)__popcll(hits);
+            if (keep && lane == 0) {
+                fp.big_masks[base + k0 / 64] = hits;
+            }
         }
         if (lane == 0) {
             touched[slot] = count;
@@ -397,9 +407,7 @@ __global__ void __launch_bounds__(kBigThreads) count_big_kernel(const DView* __r
 
 __global__ void __launch_bounds__(kBigThreads) emit_big_kernel(const DView* __restrict__ views, uint32_t n,
                                                                uint32_t T, const float* __restrict__ records,
-                                                               const uint32_t* __restrict__ big_list,
-                                                               const uint32_t* __restrict__ big_count,
-                                                               const uint32_t* __restrict__ touched,
+                                                               Footprints fp, const uint32_t* __restrict__ touched,
                                                                const uint32_t* __restrict__ depth_keys,
                                                                const uint32_t* __restrict__ offsets, uint32_t cap,
                                                                uint64_t* __restrict__ keys,
@@ -407,21 +415,22 @@ __global__ void __launch_bounds__(kBigThreads) emit_big_kernel(const DView* __re
     const uint32_t lane = threadIdx.x & 63u;
     const uint64_t lt_mask = (1ull << lane) - 1ull;
     const uint32_t waves = gridDim.x * (kBigThreads / 64);
-    const uint32_t total = *big_count;
+    const uint32_t total = fp.counters[0];
     for (uint32_t b = blockIdx.x * (kBigThreads / 64) + (threadIdx.x >> 6); b < total; b += waves) {
-        const uint32_t slot = big_list[b];
+        const uint32_t slot = fp.big_list[b];
         const uint32_t expected = touched[slot];
         if (expected == 0) {
             continue;
         }
         const BigFootprint f = big_footprint(views, n, T, records, slot);
+        const uint32_t base = fp.big_mask_base[b];
         const uint64_t depth = depth_keys[slot] & 0xFFFFFFu;
         uint32_t at = offsets[slot];
         const uint32_t stop = (uint64_t)at + expected < cap ? at + expected : cap;
         for (uint32_t k0 = 0; k0 < f.area; k0 += 64) {
             const uint32_t k = k0 + lane;
-            const bool hit = big_hit(f, T, k);
-            const uint64_t mask = __ballot(hit);
+            const uint64_t mask = base != ~0u ? fp.big_masks[base + k0 / 64] : __ballot(big_hit(f, T, k));
+            const bool hit = (mask >> lane) & 1u;
             const uint32_t mine = at + (uint32_t)__popcll(mask & lt_mask);
             if (hit && mine < stop) {
                 const uint64_t tile = (uint64_t)f.V->tile_base + (uint64_t)(f.y0 + k / f.w) * f.V->tiles_x +
@@ -506,22 +515,21 @@ __global__ void __launch_bounds__(256) counters_kernel(const uint32_t* __restric
 }  // namespace
 
 hipError_t launch_project(const ProjectArgs& a, const DView* views_dev, uint32_t nviews, float* records,
-                          uint32_t* touched, uint32_t* depth_keys, uint32_t* big_list, uint32_t* big_count,
-                          hipStream_t s) {
-    hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, s, big_count, 0u, (uint64_t)1);
+                          uint32_t* touched, uint32_t* depth_keys, const Footprints& fp, hipStream_t s) {
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, s, fp.counters, 0u, (uint64_t)2);
     if (a.n == 0 || nviews == 0) {
         return hipGetLastError();
     }
     const dim3 grid(div_up(a.n, 256));
     if (a.sh_half) {
         hipLaunchKernelGGL(project_kernel<true>, grid, dim3(256), 0, s, a, views_dev, nviews, records, touched,
-                           depth_keys, big_list, big_count);
+                           depth_keys, fp);
     } else {
         hipLaunchKernelGGL(project_kernel<false>, grid, dim3(256), 0, s, a, views_dev, nviews, records, touched,
-                           depth_keys, big_list, big_count);
+                           depth_keys, fp);
     }
     hipLaunchKernelGGL(count_big_kernel, dim3(kBigBlocks), dim3(kBigThreads), 0, s, views_dev, a.n, a.tile, records,
-                       big_list, big_count, touched, depth_keys);
+                       fp, touched, depth_keys);
     return hipGetLastError();
 }
 
@@ -539,16 +547,15 @@ hipError_t launch_fill_u32(uint32_t* p, uint32_t value, uint64_t n, hipStream_t 
 
 hipError_t launch_emit(const DView* views_dev, uint32_t nviews, uint32_t n, uint32_t tile, const float* records,
                        const uint32_t* touched, const uint32_t* depth_keys, const uint32_t* offsets,
-                       const uint32_t* big_list, const uint32_t* big_count, uint32_t cap, uint64_t* keys,
-                       uint32_t* values, hipStream_t s) {
+                       const Footprints& fp, uint32_t cap, uint64_t* keys, uint32_t* values, hipStream_t s) {
     const uint64_t slots = (uint64_t)n * nviews;
     if (slots == 0 || cap == 0) {
         return hipSuccess;
     }
     hipLaunchKernelGGL(emit_kernel, dim3((uint32_t)((slots + 255) / 256)), dim3(256), 0, s, views_dev, nviews, n,
-                       tile, records, touched, depth_keys, offsets, cap, keys, values);
-    hipLaunchKernelGGL(emit_big_kernel, dim3(kBigBlocks), dim3(kBigThreads), 0, s, views_dev, n, tile, records,
-                       big_list, big_count, touched, depth_keys, offsets, cap, keys, values);
+                       touched, depth_keys, offsets, fp, cap, keys, values);
+    hipLaunchKernelGGL(emit_big_kernel, dim3(kBigBlocks), dim3(kBigThreads), 0, s, views_dev, n, tile, records, fp,
+                       touched, depth_keys, offsets, cap, keys, values);
     return hipGetLastError();
 }
 
