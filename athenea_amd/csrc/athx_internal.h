@@ -89,12 +89,19 @@ hipError_t launch_scan(const uint32_t* in, uint32_t* out, uint64_t* total, uint3
                        hipStream_t s);
 // Pairs land at their scanned offsets; those at or past cap are not written
 // (the frame is then incomplete, as the reference's emit past its pair
-// capacity is, splat_emit.slang:43-46).
+// capacity is, splat_emit.slang:43-46). A pair's key is its global tile:
+// the pairs come out in depth order, so the depth half of the reference's
+// 64-bit key (global tile << 24 | depth key) is already sorted and is not
+// stored.
 hipError_t launch_emit(const DView* views_dev, uint32_t nviews, uint32_t n, uint32_t tile, const float* records,
-                       const uint32_t* touched, const uint32_t* depth_keys, const uint32_t* offsets,
-                       const Footprints& fp, uint32_t cap, uint64_t* keys, uint32_t* values, hipStream_t s);
-hipError_t launch_ranges(const uint64_t* sorted_keys, const uint64_t* n_dev, uint32_t cap, uint32_t tiles,
+                       const uint32_t* touched, const uint32_t* offsets, const Footprints& fp, uint32_t cap,
+                       uint32_t* tiles_out, uint32_t* values, hipStream_t s);
+hipError_t launch_ranges(const uint32_t* sorted_tiles, const uint64_t* n_dev, uint32_t cap, uint32_t tiles,
                          uint32_t* ranges, hipStream_t s);
+// The reference's 64-bit pair keys of the sorted pairs: tile << 24 | the
+// depth key of the pair's slot.
+hipError_t launch_widen_keys(const uint32_t* sorted_tiles, const uint32_t* values, const uint32_t* depth_keys,
+                             uint64_t pairs, uint64_t* keys, hipStream_t s);
 hipError_t launch_blend(const DView* views_dev, const uint32_t* view_of_tile_base, uint32_t nviews,
                         uint32_t total_tiles, uint32_t tile, uint32_t batch, const uint32_t* ranges,
                         const uint32_t* pair_slots, const float* records, uint32_t depth_mode,
@@ -108,9 +115,9 @@ hipError_t launch_project_backward(const ProjectArgs& a, const DView* views_dev,
 hipError_t launch_counters(const uint32_t* depth_keys, const uint32_t* touched, uint64_t slots,
                            uint32_t* words, hipStream_t s);
 
-// Onesweep radix sort of (uint64 key, uint32 value) pairs.
+// Onesweep radix sort of (uint32 or uint64 key, uint32 value) pairs.
 struct SortScratch {
-    uint64_t* keys_alt;
+    void* keys_alt;  // keys of the sort's width
     uint32_t* values_alt;
     uint32_t* hist;      // passes * 256
     uint32_t* lookback;  // passes * partitions * 256
@@ -121,9 +128,11 @@ uint32_t sort_partitions(uint32_t n);
 // key_bits); scratch is sized for cap and (key_bits - lo_bit) / 8 passes.
 hipError_t launch_sort(uint64_t* keys, uint32_t* values, const uint64_t* n_dev, uint32_t cap, uint32_t lo_bit,
                        uint32_t key_bits, const SortScratch& sc, hipStream_t s);
+hipError_t launch_sort(uint32_t* keys, uint32_t* values, const uint64_t* n_dev, uint32_t cap, uint32_t lo_bit,
+                       uint32_t key_bits, const SortScratch& sc, hipStream_t s);
 // The visible slots in slot order with their depth keys; the count lands in
 // *visible_total (device).
-hipError_t launch_compact(const uint32_t* touched, const uint32_t* depth_keys, uint32_t slots, uint64_t* keys_out,
+hipError_t launch_compact(const uint32_t* touched, const uint32_t* depth_keys, uint32_t slots, uint32_t* keys_out,
                           uint32_t* slots_out, uint64_t* visible_total, uint32_t* status_words, hipStream_t s);
 // Tile counts of the depth-sorted slots, scanned: slot_offsets[slot] is its
 // first pair; the pair total lands in *pair_total (device).

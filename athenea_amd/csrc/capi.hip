@@ -56,7 +56,7 @@ struct athx_ctx {
 
     DeviceBuffer views, tile_bases, records, touched, depth_keys, offsets, scan_status, totals;
     DeviceBuffer pair_keys, pair_vals, alt_keys, alt_vals, sort_meta, ranges, grad2d, counter_words, big_list;
-    DeviceBuffer depth_keys64, depth_slots, depth_alt_keys, depth_alt_slots, depth_sort_meta;
+    DeviceBuffer depth_keys_sorted, depth_slots, depth_alt_keys, depth_alt_slots, depth_sort_meta;
     DeviceBuffer fp_rects, fp_masks, fp_big_base, fp_big_masks;
     DeviceBuffer api_sort_meta, api_alt_keys, api_alt_vals, api_scan_status, api_count;
     uint64_t* host_totals = nullptr;  // pinned: [0] pairs, [1] visible
@@ -296,7 +296,7 @@ size_t sort_meta_words(uint32_t n, uint32_t passes) {
     return (size_t)passes * 256 + (size_t)passes * sort_partitions(n) * 256 + passes;
 }
 
-SortScratch sort_scratch(const DeviceBuffer& meta, uint64_t* alt_keys, uint32_t* alt_vals, uint32_t n,
+SortScratch sort_scratch(const DeviceBuffer& meta, void* alt_keys, uint32_t* alt_vals, uint32_t n,
                          uint32_t passes) {
     SortScratch sc{};
     sc.keys_alt = alt_keys;
@@ -359,9 +359,9 @@ athx_status reserve_front(athx_ctx* ctx, const FrameShape& fs) {
     ATHX_HIP(ctx, reserve(ctx, ctx->big_list, slots * 4 + 16), "big footprints");
     ATHX_HIP(ctx, reserve(ctx, ctx->ranges, (size_t)fs.tiles * 8 + 16), "ranges");
     ATHX_HIP(ctx, reserve(ctx, ctx->counter_words, 64), "counters");
-    ATHX_HIP(ctx, reserve(ctx, ctx->depth_keys64, slots * 8 + 16), "depth sort keys");
+    ATHX_HIP(ctx, reserve(ctx, ctx->depth_keys_sorted, slots * 4 + 16), "depth sort keys");
     ATHX_HIP(ctx, reserve(ctx, ctx->depth_slots, slots * 4 + 16), "depth sort slots");
-    ATHX_HIP(ctx, reserve(ctx, ctx->depth_alt_keys, slots * 8 + 16), "depth sort keys");
+    ATHX_HIP(ctx, reserve(ctx, ctx->depth_alt_keys, slots * 4 + 16), "depth sort keys");
     ATHX_HIP(ctx, reserve(ctx, ctx->depth_alt_slots, slots * 4 + 16), "depth sort slots");
     ATHX_HIP(ctx, reserve(ctx, ctx->depth_sort_meta, sort_meta_words(fs.slots, kDepthPasses) * 4 + 16),
              "depth sort metadata");
@@ -386,9 +386,9 @@ Footprints footprints_of(athx_ctx* ctx, uint32_t slots) {
 
 athx_status reserve_back(athx_ctx* ctx, const FrameShape& fs) {
     const size_t cap = fs.capacity;
-    ATHX_HIP(ctx, reserve(ctx, ctx->pair_keys, cap * 8 + 16), "pair keys");
+    ATHX_HIP(ctx, reserve(ctx, ctx->pair_keys, cap * 4 + 16), "pair tiles");
     ATHX_HIP(ctx, reserve(ctx, ctx->pair_vals, cap * 4 + 16), "pair values");
-    ATHX_HIP(ctx, reserve(ctx, ctx->alt_keys, cap * 8 + 16), "sort keys");
+    ATHX_HIP(ctx, reserve(ctx, ctx->alt_keys, cap * 4 + 16), "sort keys");
     ATHX_HIP(ctx, reserve(ctx, ctx->alt_vals, cap * 4 + 16), "sort values");
     ATHX_HIP(ctx, reserve(ctx, ctx->sort_meta, sort_meta_words(fs.capacity, fs.passes) * 4 + 16), "sort metadata");
     return ATHX_OK;
@@ -414,12 +414,12 @@ athx_status enqueue_front(athx_ctx* ctx, const ProjectArgs& pa, const FrameShape
         ATHX_HIP(ctx, hipEventRecord(ctx->ev[1], s), "event");
     }
     ATHX_HIP(ctx, launch_compact(as<uint32_t>(ctx->touched), as<uint32_t>(ctx->depth_keys), fs.slots,
-                                 as<uint64_t>(ctx->depth_keys64), as<uint32_t>(ctx->depth_slots), totals + 1,
+                                 as<uint32_t>(ctx->depth_keys_sorted), as<uint32_t>(ctx->depth_slots), totals + 1,
                                  as<uint32_t>(ctx->scan_status), s),
              "compact");
-    const SortScratch dsc = sort_scratch(ctx->depth_sort_meta, as<uint64_t>(ctx->depth_alt_keys),
+    const SortScratch dsc = sort_scratch(ctx->depth_sort_meta, ctx->depth_alt_keys.ptr,
                                          as<uint32_t>(ctx->depth_alt_slots), fs.slots, kDepthPasses);
-    ATHX_HIP(ctx, launch_sort(as<uint64_t>(ctx->depth_keys64), as<uint32_t>(ctx->depth_slots), totals + 1, fs.slots, 0,
+    ATHX_HIP(ctx, launch_sort(as<uint32_t>(ctx->depth_keys_sorted), as<uint32_t>(ctx->depth_slots), totals + 1, fs.slots, 0,
                               kDepthBits, dsc, s),
              "depth sort");
     if (timed) {
@@ -441,22 +441,21 @@ athx_status enqueue_back(athx_ctx* ctx, const ProjectArgs& pa, const FrameShape&
                          hipStream_t s, bool timed) {
     const uint64_t* pairs_dev = as<uint64_t>(ctx->totals);
     ATHX_HIP(ctx, launch_emit(as<DView>(ctx->views), fs.nviews, pa.n, settings->tile, as<float>(ctx->records),
-                              as<uint32_t>(ctx->touched), as<uint32_t>(ctx->depth_keys), as<uint32_t>(ctx->offsets),
-                              footprints_of(ctx, fs.slots), fs.capacity, as<uint64_t>(ctx->pair_keys),
-                              as<uint32_t>(ctx->pair_vals), s),
+                              as<uint32_t>(ctx->touched), as<uint32_t>(ctx->offsets), footprints_of(ctx, fs.slots),
+                              fs.capacity, as<uint32_t>(ctx->pair_keys), as<uint32_t>(ctx->pair_vals), s),
              "emit");
     if (timed) {
         ATHX_HIP(ctx, hipEventRecord(ctx->ev[4], s), "event");
     }
-    const SortScratch sc = sort_scratch(ctx->sort_meta, as<uint64_t>(ctx->alt_keys), as<uint32_t>(ctx->alt_vals),
-                                        fs.capacity, fs.passes);
-    ATHX_HIP(ctx, launch_sort(as<uint64_t>(ctx->pair_keys), as<uint32_t>(ctx->pair_vals), pairs_dev, fs.capacity,
-                              kDepthBits, fs.key_bits, sc, s),
+    const SortScratch sc = sort_scratch(ctx->sort_meta, ctx->alt_keys.ptr, as<uint32_t>(ctx->alt_vals), fs.capacity,
+                                        fs.passes);
+    ATHX_HIP(ctx, launch_sort(as<uint32_t>(ctx->pair_keys), as<uint32_t>(ctx->pair_vals), pairs_dev, fs.capacity, 0,
+                              fs.key_bits - kDepthBits, sc, s),
              "tile sort");
     if (timed) {
         ATHX_HIP(ctx, hipEventRecord(ctx->ev[5], s), "event");
     }
-    ATHX_HIP(ctx, launch_ranges(as<uint64_t>(ctx->pair_keys), pairs_dev, fs.capacity, fs.tiles,
+    ATHX_HIP(ctx, launch_ranges(as<uint32_t>(ctx->pair_keys), pairs_dev, fs.capacity, fs.tiles,
                                 as<uint32_t>(ctx->ranges), s),
              "ranges");
     if (timed) {
@@ -566,7 +565,7 @@ ATHX_API void athx_destroy(athx_ctx* ctx) {
                             &ctx->offsets, &ctx->scan_status, &ctx->totals, &ctx->pair_keys, &ctx->pair_vals,
                             &ctx->alt_keys, &ctx->alt_vals, &ctx->sort_meta, &ctx->ranges, &ctx->grad2d,
                             &ctx->counter_words, &ctx->api_sort_meta, &ctx->api_alt_keys, &ctx->api_alt_vals,
-                            &ctx->api_scan_status, &ctx->big_list, &ctx->api_count, &ctx->depth_keys64,
+                            &ctx->api_scan_status, &ctx->big_list, &ctx->api_count, &ctx->depth_keys_sorted,
                             &ctx->depth_slots, &ctx->depth_alt_keys, &ctx->depth_alt_slots, &ctx->depth_sort_meta,
                             &ctx->fp_rects, &ctx->fp_masks, &ctx->fp_big_base, &ctx->fp_big_masks}) {
         release(*b);
@@ -603,7 +602,7 @@ ATHX_API athx_status athx_release_scratch(athx_ctx* ctx) {
     (void)hipSetDevice(ctx->device);
     ATHX_HIP(ctx, hipDeviceSynchronize(), "synchronize before release");
     for (DeviceBuffer* b : {&ctx->records, &ctx->touched, &ctx->depth_keys, &ctx->offsets, &ctx->scan_status,
-                            &ctx->big_list, &ctx->depth_keys64, &ctx->depth_slots, &ctx->depth_alt_keys,
+                            &ctx->big_list, &ctx->depth_keys_sorted, &ctx->depth_slots, &ctx->depth_alt_keys,
                             &ctx->depth_alt_slots, &ctx->depth_sort_meta, &ctx->fp_rects, &ctx->fp_masks,
                             &ctx->fp_big_base, &ctx->fp_big_masks, &ctx->pair_keys, &ctx->pair_vals,
                             &ctx->alt_keys, &ctx->alt_vals, &ctx->sort_meta,
@@ -625,7 +624,7 @@ ATHX_API uint64_t athx_scratch_bytes(const athx_ctx* ctx) {
                                   &ctx->offsets, &ctx->scan_status, &ctx->totals, &ctx->pair_keys, &ctx->pair_vals,
                                   &ctx->alt_keys, &ctx->alt_vals, &ctx->sort_meta, &ctx->ranges, &ctx->grad2d,
                                   &ctx->counter_words, &ctx->api_sort_meta, &ctx->api_alt_keys, &ctx->api_alt_vals,
-                                  &ctx->api_scan_status, &ctx->big_list, &ctx->depth_keys64, &ctx->depth_slots,
+                                  &ctx->api_scan_status, &ctx->big_list, &ctx->depth_keys_sorted, &ctx->depth_slots,
                                   &ctx->depth_alt_keys, &ctx->depth_alt_slots, &ctx->depth_sort_meta,
                                   &ctx->fp_rects, &ctx->fp_masks, &ctx->fp_big_base, &ctx->fp_big_masks}) {
         sum += b->bytes;
@@ -870,7 +869,7 @@ ATHX_API athx_status athx_copy_frame(athx_ctx* ctx, athx_frame_part part, void* 
     case ATHX_PART_TOUCHED: src = ctx->touched.ptr; size = slots * 4; break;
     case ATHX_PART_DEPTH_KEYS: src = ctx->depth_keys.ptr; size = slots * 4; break;
     case ATHX_PART_OFFSETS: src = ctx->offsets.ptr; size = slots * 4; break;
-    case ATHX_PART_PAIR_KEYS: src = ctx->pair_keys.ptr; size = pairs * 8; break;
+    case ATHX_PART_PAIR_KEYS: size = pairs * 8; break;  // rebuilt from the tiles and depth keys
     case ATHX_PART_PAIR_VALUES: src = ctx->pair_vals.ptr; size = pairs * 4; break;
     case ATHX_PART_RANGES: src = ctx->ranges.ptr; size = (uint64_t)ctx->last_tiles * 8; break;
     default: return fail(ctx, ATHX_INVALID_ARGUMENT, "unknown frame part");
@@ -886,6 +885,13 @@ ATHX_API athx_status athx_copy_frame(athx_ctx* ctx, athx_frame_part part, void* 
         return fail(ctx, ATHX_INVALID_ARGUMENT, "null destination");
     }
     ATHX_HIP(ctx, hipSetDevice(ctx->device), "set device");
+    if (part == ATHX_PART_PAIR_KEYS) {
+        ATHX_HIP(ctx, launch_widen_keys(as<uint32_t>(ctx->pair_keys), as<uint32_t>(ctx->pair_vals),
+                                        as<uint32_t>(ctx->depth_keys), pairs, static_cast<uint64_t*>(dst),
+                                        static_cast<hipStream_t>(stream)),
+                 "pair keys");
+        return ATHX_OK;
+    }
     ATHX_HIP(ctx, hipMemcpyAsync(dst, src, size, hipMemcpyDeviceToDevice, static_cast<hipStream_t>(stream)),
              "copy frame part");
     return ATHX_OK;
@@ -934,7 +940,7 @@ ATHX_API athx_status athx_sort_pairs(athx_ctx* ctx, uint64_t* keys, uint32_t* va
     ATHX_HIP(ctx, reserve(ctx, ctx->api_alt_vals, (size_t)n * 4), "sort values");
     ATHX_HIP(ctx, reserve(ctx, ctx->api_sort_meta, sort_meta_words(n, passes) * 4), "sort metadata");
     ATHX_HIP(ctx, reserve(ctx, ctx->api_count, 16), "sort count");
-    const SortScratch sc = sort_scratch(ctx->api_sort_meta, as<uint64_t>(ctx->api_alt_keys),
+    const SortScratch sc = sort_scratch(ctx->api_sort_meta, ctx->api_alt_keys.ptr,
                                         as<uint32_t>(ctx->api_alt_vals), n, passes);
     hipStream_t s = static_cast<hipStream_t>(stream);
     // The sort reads its count on the device: put n there (a kernel, not a

@@ -307,9 +307,8 @@ __global__ void fill_u32_kernel(uint32_t* p, uint32_t value, uint64_t n) {
 // the projection's hit mask; a big footprint is emit_big_kernel's.
 __global__ void __launch_bounds__(256) emit_kernel(const DView* __restrict__ views, uint32_t nviews, uint32_t n,
                                                    const uint32_t* __restrict__ touched,
-                                                   const uint32_t* __restrict__ depth_keys,
                                                    const uint32_t* __restrict__ offsets, Footprints fp, uint32_t cap,
-                                                   uint64_t* __restrict__ keys, uint32_t* __restrict__ values) {
+                                                   uint32_t* __restrict__ keys, uint32_t* __restrict__ values) {
     const uint64_t slot = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (slot >= (uint64_t)n * nviews) {
         return;
@@ -324,48 +323,81 @@ __global__ void __launch_bounds__(256) emit_kernel(const DView* __restrict__ vie
         return;
     }
     const DView& V = views[slot / n];
-    const uint64_t depth = depth_keys[slot] & 0xFFFFFFu;
-    const uint64_t row_base = (uint64_t)V.tile_base + (uint64_t)y0 * V.tiles_x + x0;
+    const uint32_t row_base = V.tile_base + y0 * V.tiles_x + x0;
     uint32_t at = offsets[slot];
     const uint32_t stop = (uint64_t)at + expected < cap ? at + expected : cap;
     for (uint64_t m = fp.masks[slot]; m != 0 && at < stop; m &= m - 1, ++at) {
         const uint32_t k = (uint32_t)__builtin_ctzll(m);
-        const uint64_t tile = row_base + (uint64_t)(k / w) * V.tiles_x + (k % w);
-        keys[at] = (tile << kDepthBits) | depth;
+        keys[at] = row_base + (k / w) * V.tiles_x + (k % w);
         values[at] = (uint32_t)slot;
     }
 }
 
-// A wavefront per big footprint: lane k tests tile k of each run of 64 in
-// the rectangle's row-major order, and a ballot gives each hit its place,
-// so the count is the serial loop's and the pairs come out in its order.
+// Big footprints go 64 to a wavefront: lane j reads footprint j's record
+// and rectangle, the wavefront claims the hit-mask words of all 64 with one
+// atomic (one per footprint, on one address, serialises in the memory
+// system), then walks them one at a time with the footprint's fields read
+// from its lane. Lane k of a step tests tile k of the step's run of 64 in
+// the rectangle's row-major order, and a ballot gives each hit its place, so
+// the count is the serial loop's and the pairs come out in its order.
 constexpr uint32_t kBigThreads = 256;
 constexpr uint32_t kBigBlocks = 2048;
 
 struct BigFootprint {
-    const DView* V;
     float4 screen, conic;
-    uint32_t x0, y0, w, area;
+    uint32_t view, x0, y0, w, area;
 };
 
 __device__ inline BigFootprint big_footprint(const DView* views, uint32_t n, uint32_t T, const float* records,
                                              uint32_t slot) {
     BigFootprint f;
-    f.V = &views[slot / n];
+    f.view = slot / n;
+    const DView& V = views[f.view];
     const float4* rec = reinterpret_cast<const float4*>(records + (size_t)slot * kRecordFloats);
     f.screen = rec[0];
     f.conic = rec[1];
     uint32_t x1, y1;
-    tile_rect(f.V->width, f.V->height, T, f.screen.x, f.screen.y, f.conic.x, f.conic.y, f.conic.z, f.screen.w, f.x0,
-              f.y0, x1, y1);
+    tile_rect(V.width, V.height, T, f.screen.x, f.screen.y, f.conic.x, f.conic.y, f.conic.z, f.screen.w, f.x0, f.y0,
+              x1, y1);
     f.w = x1 - f.x0 + 1;
     f.area = f.w * (y1 - f.y0 + 1);
     return f;
 }
 
-__device__ inline bool big_hit(const BigFootprint& f, uint32_t T, uint32_t k) {
-    return k < f.area && tile_touched(f.V->width, f.V->height, T, f.screen.x, f.screen.y, f.conic.x, f.conic.y,
-                                      f.conic.z, f.screen.w, f.x0 + k % f.w, f.y0 + k / f.w);
+__device__ inline uint32_t lane_u(uint32_t v, uint32_t j) { return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)j); }
+__device__ inline float lane_f(float v, uint32_t j) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), (int)j));
+}
+__device__ inline uint64_t lane_u64(uint64_t v, uint32_t j) {
+    return (uint64_t)lane_u((uint32_t)v, j) | (uint64_t)lane_u((uint32_t)(v >> 32), j) << 32;
+}
+
+// Footprint j of the wavefront's 64, read from lane j (j is wave-uniform, so
+// the fields land in scalar registers).
+__device__ inline BigFootprint big_of_lane(const BigFootprint& f, uint32_t j) {
+    BigFootprint g;
+    g.screen = make_float4(lane_f(f.screen.x, j), lane_f(f.screen.y, j), 0.0f, lane_f(f.screen.w, j));
+    g.conic = make_float4(lane_f(f.conic.x, j), lane_f(f.conic.y, j), lane_f(f.conic.z, j), 0.0f);
+    g.view = lane_u(f.view, j);
+    g.x0 = lane_u(f.x0, j);
+    g.y0 = lane_u(f.y0, j);
+    g.w = lane_u(f.w, j);
+    g.area = lane_u(f.area, j);
+    return g;
+}
+
+__device__ inline bool big_hit(const DView& V, const BigFootprint& f, uint32_t T, uint32_t k) {
+    return k < f.area && tile_touched(V.width, V.height, T, f.screen.x, f.screen.y, f.conic.x, f.conic.y, f.conic.z,
+                                      f.screen.w, f.x0 + k % f.w, f.y0 + k / f.w);
+}
+
+__device__ inline uint32_t wave_inclusive_sum(uint32_t v, uint32_t lane) {
+#pragma unroll
+    for (uint32_t d = 1; d < 64; d <<= 1) {
+        const uint32_t t = __shfl_up(v, d, 64);
+        v += lane >= d ? t : 0u;
+    }
+    return v;
 }
 
 __global__ void __launch_bounds__(kBigThreads) count_big_kernel(const DView* __restrict__ views, uint32_t n,
@@ -375,30 +407,52 @@ __global__ void __launch_bounds__(kBigThreads) count_big_kernel(const DView* __r
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t waves = gridDim.x * (kBigThreads / 64);
     const uint32_t total = fp.counters[0];
-    for (uint32_t b = blockIdx.x * (kBigThreads / 64) + (threadIdx.x >> 6); b < total; b += waves) {
-        const uint32_t slot = fp.big_list[b];
-        const BigFootprint f = big_footprint(views, n, T, records, slot);
+    for (uint32_t c0 = (blockIdx.x * (kBigThreads / 64) + (threadIdx.x >> 6)) * 64u; c0 < total; c0 += waves * 64u) {
+        const uint32_t b = c0 + lane;
+        const bool mine = b < total;
+        const uint32_t slot = mine ? fp.big_list[b] : 0u;
+        BigFootprint f = {};
+        if (mine) {
+            f = big_footprint(views, n, T, records, slot);
+        }
+        // Room for the 64 footprints' hit words, if the buffer has it; a
+        // footprint that does not fit is tested again by the emission.
         const uint32_t words = div_up(f.area, 64);
-        // Room for this footprint's hit words, if the buffer has it; if not,
-        // the emission tests its tiles again.
-        uint32_t base = 0;
-        if (lane == 0) {
-            base = atomicAdd(&fp.counters[1], words);
-            fp.big_mask_base[b] = (uint64_t)base + words <= fp.big_mask_cap ? base : ~0u;
+        const uint32_t incl = wave_inclusive_sum(words, lane);
+        uint32_t claimed = 0;
+        if (lane == 63) {
+            claimed = atomicAdd(&fp.counters[1], incl);
         }
-        base = __shfl(base, 0, 64);
-        const bool keep = (uint64_t)base + words <= fp.big_mask_cap;
-        uint32_t count = 0;
-        for (uint32_t k0 = 0; k0 < f.area; k0 += 64) {
-            const uint64_t hits = __ballot(big_hit(f, T, k0 + lane));
-            count += (uint32_t)__popcll(hits);
-            if (keep && lane == 0) {
-                fp.big_masks[base + k0 / 64] = hits;
+        const uint64_t at = (uint64_t)lane_u(claimed, 63) + incl - words;
+        const bool keep = at + words <= fp.big_mask_cap;
+        if (mine) {
+            fp.big_mask_base[b] = keep ? (uint32_t)at : ~0u;
+        }
+        const uint32_t here = total - c0 < 64u ? total - c0 : 64u;
+        uint32_t my_count = 0;
+        for (uint32_t j = 0; j < here; ++j) {
+            const BigFootprint g = big_of_lane(f, j);
+            const DView& V = views[g.view];
+            const bool keep_j = lane_u(keep ? 1u : 0u, j) != 0;
+            const uint64_t at_j = lane_u64(at, j);
+            uint32_t count = 0;
+            uint64_t run = 0;  // lane r holds the hit word of step r of the current run of 64 steps
+            for (uint32_t k0 = 0; k0 < g.area; k0 += 64) {
+                const uint64_t hits = __ballot(big_hit(V, g, T, k0 + lane));
+                count += (uint32_t)__popcll(hits);
+                const uint32_t step = (k0 >> 6) & 63u;
+                run = lane == step ? hits : run;
+                if (keep_j && (step == 63u || k0 + 64 >= g.area)) {
+                    if (lane <= step) {
+                        fp.big_masks[at_j + ((k0 >> 6) & ~63u) + lane] = run;
+                    }
+                }
             }
+            my_count = lane == j ? count : my_count;
         }
-        if (lane == 0) {
-            touched[slot] = count;
-            if (count == 0) {
+        if (mine) {
+            touched[slot] = my_count;
+            if (my_count == 0) {
                 keys[slot] = kCulledKey | ATHX_CULL_NO_TILE;
             }
         }
@@ -408,37 +462,60 @@ __global__ void __launch_bounds__(kBigThreads) count_big_kernel(const DView* __r
 __global__ void __launch_bounds__(kBigThreads) emit_big_kernel(const DView* __restrict__ views, uint32_t n,
                                                                uint32_t T, const float* __restrict__ records,
                                                                Footprints fp, const uint32_t* __restrict__ touched,
-                                                               const uint32_t* __restrict__ depth_keys,
                                                                const uint32_t* __restrict__ offsets, uint32_t cap,
-                                                               uint64_t* __restrict__ keys,
+                                                               uint32_t* __restrict__ keys,
                                                                uint32_t* __restrict__ values) {
     const uint32_t lane = threadIdx.x & 63u;
     const uint64_t lt_mask = (1ull << lane) - 1ull;
     const uint32_t waves = gridDim.x * (kBigThreads / 64);
     const uint32_t total = fp.counters[0];
-    for (uint32_t b = blockIdx.x * (kBigThreads / 64) + (threadIdx.x >> 6); b < total; b += waves) {
-        const uint32_t slot = fp.big_list[b];
-        const uint32_t expected = touched[slot];
-        if (expected == 0) {
-            continue;
+    for (uint32_t c0 = (blockIdx.x * (kBigThreads / 64) + (threadIdx.x >> 6)) * 64u; c0 < total; c0 += waves * 64u) {
+        const uint32_t b = c0 + lane;
+        const bool mine = b < total;
+        const uint32_t slot = mine ? fp.big_list[b] : 0u;
+        const uint32_t expected = mine ? touched[slot] : 0u;
+        BigFootprint f = {};
+        uint32_t base = ~0u, first = 0;
+        if (expected != 0) {
+            f = big_footprint(views, n, T, records, slot);
+            base = fp.big_mask_base[b];
+            first = offsets[slot];
         }
-        const BigFootprint f = big_footprint(views, n, T, records, slot);
-        const uint32_t base = fp.big_mask_base[b];
-        const uint64_t depth = depth_keys[slot] & 0xFFFFFFu;
-        uint32_t at = offsets[slot];
-        const uint32_t stop = (uint64_t)at + expected < cap ? at + expected : cap;
-        for (uint32_t k0 = 0; k0 < f.area; k0 += 64) {
-            const uint32_t k = k0 + lane;
-            const uint64_t mask = base != ~0u ? fp.big_masks[base + k0 / 64] : __ballot(big_hit(f, T, k));
-            const bool hit = (mask >> lane) & 1u;
-            const uint32_t mine = at + (uint32_t)__popcll(mask & lt_mask);
-            if (hit && mine < stop) {
-                const uint64_t tile = (uint64_t)f.V->tile_base + (uint64_t)(f.y0 + k / f.w) * f.V->tiles_x +
-                                      (f.x0 + k % f.w);
-                keys[mine] = (tile << kDepthBits) | depth;
-                values[mine] = slot;
+        const uint32_t here = total - c0 < 64u ? total - c0 : 64u;
+        for (uint32_t j = 0; j < here; ++j) {
+            const uint32_t expected_j = lane_u(expected, j);
+            if (expected_j == 0) {
+                continue;
             }
-            at += (uint32_t)__popcll(mask);
+            const BigFootprint g = big_of_lane(f, j);
+            const DView& V = views[g.view];
+            const uint32_t base_j = lane_u(base, j);
+            const uint32_t slot_j = lane_u(slot, j);
+            const uint32_t row0 = V.tile_base + g.y0 * V.tiles_x + g.x0;
+            uint32_t at = lane_u(first, j);
+            const uint32_t stop = (uint64_t)at + expected_j < cap ? at + expected_j : cap;
+            uint64_t run = 0;  // lane r: the hit word of step r of the current run of 64 steps
+            for (uint32_t k0 = 0; k0 < g.area && at < stop; k0 += 64) {
+                const uint32_t k = k0 + lane;
+                const uint32_t step = (k0 >> 6) & 63u;
+                uint64_t mask;
+                if (base_j != ~0u) {
+                    if (step == 0) {
+                        const uint32_t word = (k0 >> 6) + lane;
+                        run = word < div_up(g.area, 64) ? fp.big_masks[(uint64_t)base_j + word] : 0ull;
+                    }
+                    mask = lane_u64(run, step);
+                } else {
+                    mask = __ballot(big_hit(V, g, T, k));
+                }
+                const bool hit = (mask >> lane) & 1u;
+                const uint32_t mine_at = at + (uint32_t)__popcll(mask & lt_mask);
+                if (hit && mine_at < stop) {
+                    keys[mine_at] = row0 + (k / g.w) * V.tiles_x + k % g.w;
+                    values[mine_at] = slot_j;
+                }
+                at += (uint32_t)__popcll(mask);
+            }
         }
     }
 }
@@ -446,11 +523,11 @@ __global__ void __launch_bounds__(kBigThreads) emit_big_kernel(const DView* __re
 // Where each tile's pairs start and end in the sorted keys, by binary search
 // on the tile bits: splat_tiles_clear.slang:8-17 and splat_ranges.slang:11-25
 // give a tile with no pairs (0, 0), and so does this.
-__device__ inline uint32_t lower_bound_tile(const uint64_t* keys, uint32_t n, uint64_t tile) {
+__device__ inline uint32_t lower_bound_tile(const uint32_t* tiles, uint32_t n, uint64_t tile) {
     uint32_t lo = 0, hi = n;
     while (lo < hi) {
         const uint32_t mid = lo + ((hi - lo) >> 1);
-        if ((keys[mid] >> kDepthBits) < tile) {
+        if ((uint64_t)tiles[mid] < tile) {
             lo = mid + 1;
         } else {
             hi = mid;
@@ -459,7 +536,7 @@ __device__ inline uint32_t lower_bound_tile(const uint64_t* keys, uint32_t n, ui
     return lo;
 }
 
-__global__ void __launch_bounds__(256) ranges_kernel(const uint64_t* __restrict__ keys,
+__global__ void __launch_bounds__(256) ranges_kernel(const uint32_t* __restrict__ keys,
                                                      const uint64_t* __restrict__ n_dev, uint32_t cap, uint32_t tiles,
                                                      uint32_t* __restrict__ ranges) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -471,6 +548,14 @@ __global__ void __launch_bounds__(256) ranges_kernel(const uint64_t* __restrict_
     const uint32_t end = lower_bound_tile(keys, n, (uint64_t)t + 1);
     ranges[2 * t] = start < end ? start : 0u;
     ranges[2 * t + 1] = start < end ? end : 0u;
+}
+
+__global__ void widen_keys_kernel(const uint32_t* __restrict__ tiles, const uint32_t* __restrict__ values,
+                                  const uint32_t* __restrict__ depth_keys, uint64_t pairs,
+                                  uint64_t* __restrict__ keys) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < pairs; i += (uint64_t)gridDim.x * blockDim.x) {
+        keys[i] = (uint64_t)tiles[i] << kDepthBits | (depth_keys[values[i]] & 0xFFFFFFu);
+    }
 }
 
 // SplatCounters words (splat_frame_counters.slang): [0] slots, [1] visible,
@@ -546,26 +631,37 @@ hipError_t launch_fill_u32(uint32_t* p, uint32_t value, uint64_t n, hipStream_t 
 }
 
 hipError_t launch_emit(const DView* views_dev, uint32_t nviews, uint32_t n, uint32_t tile, const float* records,
-                       const uint32_t* touched, const uint32_t* depth_keys, const uint32_t* offsets,
-                       const Footprints& fp, uint32_t cap, uint64_t* keys, uint32_t* values, hipStream_t s) {
+                       const uint32_t* touched, const uint32_t* offsets, const Footprints& fp, uint32_t cap,
+                       uint32_t* keys, uint32_t* values, hipStream_t s) {
     const uint64_t slots = (uint64_t)n * nviews;
     if (slots == 0 || cap == 0) {
         return hipSuccess;
     }
     hipLaunchKernelGGL(emit_kernel, dim3((uint32_t)((slots + 255) / 256)), dim3(256), 0, s, views_dev, nviews, n,
-                       touched, depth_keys, offsets, fp, cap, keys, values);
+                       touched, offsets, fp, cap, keys, values);
     hipLaunchKernelGGL(emit_big_kernel, dim3(kBigBlocks), dim3(kBigThreads), 0, s, views_dev, n, tile, records, fp,
-                       touched, depth_keys, offsets, cap, keys, values);
+                       touched, offsets, cap, keys, values);
     return hipGetLastError();
 }
 
-hipError_t launch_ranges(const uint64_t* sorted_keys, const uint64_t* n_dev, uint32_t cap, uint32_t tiles,
+hipError_t launch_ranges(const uint32_t* sorted_keys, const uint64_t* n_dev, uint32_t cap, uint32_t tiles,
                          uint32_t* ranges, hipStream_t s) {
     if (tiles == 0) {
         return hipSuccess;
     }
     hipLaunchKernelGGL(ranges_kernel, dim3(div_up(tiles, 256)), dim3(256), 0, s, sorted_keys, n_dev, cap, tiles,
                        ranges);
+    return hipGetLastError();
+}
+
+hipError_t launch_widen_keys(const uint32_t* sorted_tiles, const uint32_t* values, const uint32_t* depth_keys,
+                             uint64_t pairs, uint64_t* keys, hipStream_t s) {
+    if (pairs == 0) {
+        return hipSuccess;
+    }
+    const uint64_t blocks = (pairs + 255) / 256 < 4096 ? (pairs + 255) / 256 : 4096;
+    hipLaunchKernelGGL(widen_keys_kernel, dim3((uint32_t)blocks), dim3(256), 0, s, sorted_tiles, values, depth_keys,
+                       pairs, keys);
     return hipGetLastError();
 }
 

@@ -65,7 +65,7 @@ struct PlainScan {
 struct CompactScan {
     const uint32_t* touched;
     const uint32_t* depth_keys;
-    uint64_t* keys_out;
+    uint32_t* keys_out;
     uint32_t* slots_out;
     uint32_t n;
     __device__ uint32_t size() const { return n; }
@@ -222,7 +222,7 @@ hipError_t launch_scan(const uint32_t* in, uint32_t* out, uint64_t* total, uint3
     return run_scan(PlainScan{in, out, n}, n, total, status_words, s);
 }
 
-hipError_t launch_compact(const uint32_t* touched, const uint32_t* depth_keys, uint32_t slots, uint64_t* keys_out,
+hipError_t launch_compact(const uint32_t* touched, const uint32_t* depth_keys, uint32_t slots, uint32_t* keys_out,
                           uint32_t* slots_out, uint64_t* visible_total, uint32_t* status_words, hipStream_t s) {
     return run_scan(CompactScan{touched, depth_keys, keys_out, slots_out, slots}, slots, visible_total, status_words,
                     s);

@@ -1,5 +1,5 @@
-// sort.hip -- stable LSD radix sort of (uint64 key, uint32 value) pairs for
-// gfx950, onesweep (Adinets & Merrill 2022): one kernel counts every digit
+// sort.hip -- stable LSD radix sort of (uint32 or uint64 key, uint32 value)
+// pairs for gfx950, onesweep (Adinets & Merrill 2022): one kernel counts every digit
 // pass's histogram in a single read of the keys, then one kernel a pass
 // ranks each 4096-key partition in LDS with wavefront ballots (a 64-lane
 // match on the 8 digit bits), finds where its digits start among the
@@ -67,7 +67,8 @@ __device__ inline uint32_t count_of(const uint64_t* n_dev, uint32_t cap) {
     return n < cap ? (uint32_t)n : cap;
 }
 
-__global__ void __launch_bounds__(kThreads) histogram_kernel(const uint64_t* __restrict__ keys,
+template <typename K>
+__global__ void __launch_bounds__(kThreads) histogram_kernel(const K* __restrict__ keys,
                                                              const uint64_t* __restrict__ n_dev, uint32_t cap,
                                                              uint32_t lo_bit, uint32_t passes,
                                                              uint32_t* __restrict__ hist) {
@@ -78,7 +79,7 @@ __global__ void __launch_bounds__(kThreads) histogram_kernel(const uint64_t* __r
     }
     __syncthreads();
     for (uint32_t i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads) {
-        const uint64_t k = keys[i];
+        const K k = keys[i];
         for (uint32_t p = 0; p < passes; ++p) {
             atomicAdd(&s_hist[p * kRadix + (uint32_t)((k >> (lo_bit + 8 * p)) & 0xFFu)], 1u);
         }
@@ -91,14 +92,15 @@ __global__ void __launch_bounds__(kThreads) histogram_kernel(const uint64_t* __r
     }
 }
 
-__global__ void __launch_bounds__(kThreads) onesweep_kernel(const uint64_t* __restrict__ kin,
+template <typename K>
+__global__ void __launch_bounds__(kThreads) onesweep_kernel(const K* __restrict__ kin,
                                                             const uint32_t* __restrict__ vin,
-                                                            uint64_t* __restrict__ kout, uint32_t* __restrict__ vout,
+                                                            K* __restrict__ kout, uint32_t* __restrict__ vout,
                                                             const uint64_t* __restrict__ n_dev, uint32_t cap,
                                                             uint32_t shift, const uint32_t* __restrict__ hist,
                                                             uint32_t* __restrict__ lookback,
                                                             uint32_t* __restrict__ counter) {
-    __shared__ uint64_t s_keys[kPart];
+    __shared__ K s_keys[kPart];
     __shared__ uint32_t s_vals[kPart];
     __shared__ uint32_t s_wave_hist[kWaves * kRadix];
     __shared__ uint32_t s_block_start[kRadix];
@@ -129,7 +131,7 @@ __global__ void __launch_bounds__(kThreads) onesweep_kernel(const uint64_t* __re
 
     // Load: item j of a lane sits at wave_first + j * 64 + lane, so the order
     // of the input is (wave, j, lane) and ranking j by j keeps it stable.
-    uint64_t key[kItems];
+    K key[kItems];
     uint32_t val[kItems];
     uint32_t rank[kItems];
 #pragma unroll
@@ -215,7 +217,7 @@ __global__ void __launch_bounds__(kThreads) onesweep_kernel(const uint64_t* __re
     __syncthreads();
     const uint32_t valid = n - part_first < kPart ? n - part_first : kPart;
     for (uint32_t i = tid; i < valid; i += kThreads) {
-        const uint64_t k = s_keys[i];
+        const K k = s_keys[i];
         const uint32_t dk = (uint32_t)(k >> shift) & 0xFFu;
         const uint32_t dst = s_digit_base[dk] + (i - s_block_start[dk]);
         kout[dst] = k;
@@ -248,14 +250,48 @@ __global__ void clear_kernel(uint32_t* __restrict__ hist, uint32_t* __restrict__
 }
 
 // An odd number of passes leaves the result in the alternate buffers.
-__global__ void copy_back_kernel(uint64_t* __restrict__ keys, uint32_t* __restrict__ values,
-                                 const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin,
+template <typename K>
+__global__ void copy_back_kernel(K* __restrict__ keys, uint32_t* __restrict__ values,
+                                 const K* __restrict__ kin, const uint32_t* __restrict__ vin,
                                  const uint64_t* __restrict__ n_dev, uint32_t cap) {
     const uint32_t n = count_of(n_dev, cap);
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         keys[i] = kin[i];
         values[i] = vin[i];
     }
+}
+
+template <typename K>
+hipError_t sort_impl(K* keys, uint32_t* values, const uint64_t* n_dev, uint32_t cap, uint32_t lo_bit,
+                     uint32_t key_bits, const SortScratch& sc, hipStream_t s) {
+    if (cap < 2 || key_bits <= lo_bit) {
+        return hipSuccess;
+    }
+    if (key_bits > 8 * sizeof(K) || cap > kValueMask) {
+        return hipErrorInvalidValue;
+    }
+    const uint32_t passes = div_up(key_bits - lo_bit, 8);
+    const uint32_t parts = sort_partitions(cap);
+    const uint32_t spread = parts < 2048 ? parts : 2048;
+    hipLaunchKernelGGL(clear_kernel, dim3(spread), dim3(256), 0, s, sc.hist, sc.lookback, sc.counters, passes, n_dev,
+                       cap);
+    hipLaunchKernelGGL(histogram_kernel<K>, dim3(spread), dim3(kThreads), 0, s, keys, n_dev, cap, lo_bit, passes,
+                       sc.hist);
+    K* kin = keys;
+    uint32_t* vin = values;
+    K* kout = static_cast<K*>(sc.keys_alt);
+    uint32_t* vout = sc.values_alt;
+    for (uint32_t p = 0; p < passes; ++p) {
+        hipLaunchKernelGGL(onesweep_kernel<K>, dim3(parts), dim3(kThreads), 0, s, kin, vin, kout, vout, n_dev, cap,
+                           lo_bit + 8 * p, sc.hist + p * kRadix, sc.lookback + (size_t)p * parts * kRadix,
+                           sc.counters + p);
+        K* tk = kin; kin = kout; kout = tk;
+        uint32_t* tv = vin; vin = vout; vout = tv;
+    }
+    if (kin != keys) {
+        hipLaunchKernelGGL(copy_back_kernel<K>, dim3(spread), dim3(256), 0, s, keys, values, kin, vin, n_dev, cap);
+    }
+    return hipGetLastError();
 }
 
 }  // namespace
@@ -265,38 +301,16 @@ uint32_t sort_partitions(uint32_t n) { return div_up(n ? n : 1, kPart); }
 // Sorts the first *n_dev (at most cap) of (keys, values) stably by bits
 // [lo_bit, key_bits) of the key, the count read on the device. Digits below
 // lo_bit are left as the input orders them: a caller whose input is already
-// in order of those bits (the pairs, emitted in depth order) sorts only the
-// rest. The result is back in keys and values.
+// in order of those bits sorts only the rest. The result is back in keys and
+// values; sc.keys_alt holds keys of the same width.
 hipError_t launch_sort(uint64_t* keys, uint32_t* values, const uint64_t* n_dev, uint32_t cap, uint32_t lo_bit,
                        uint32_t key_bits, const SortScratch& sc, hipStream_t s) {
-    if (cap < 2 || key_bits <= lo_bit) {
-        return hipSuccess;
-    }
-    if (key_bits > 64 || cap > kValueMask) {
-        return hipErrorInvalidValue;
-    }
-    const uint32_t passes = div_up(key_bits - lo_bit, 8);
-    const uint32_t parts = sort_partitions(cap);
-    const uint32_t spread = parts < 2048 ? parts : 2048;
-    hipLaunchKernelGGL(clear_kernel, dim3(spread), dim3(256), 0, s, sc.hist, sc.lookback, sc.counters, passes, n_dev,
-                       cap);
-    hipLaunchKernelGGL(histogram_kernel, dim3(spread), dim3(kThreads), 0, s, keys, n_dev, cap, lo_bit, passes,
-                       sc.hist);
-    uint64_t* kin = keys;
-    uint32_t* vin = values;
-    uint64_t* kout = sc.keys_alt;
-    uint32_t* vout = sc.values_alt;
-    for (uint32_t p = 0; p < passes; ++p) {
-        hipLaunchKernelGGL(onesweep_kernel, dim3(parts), dim3(kThreads), 0, s, kin, vin, kout, vout, n_dev, cap,
-                           lo_bit + 8 * p,
-                           sc.hist + p * kRadix, sc.lookback + (size_t)p * parts * kRadix, sc.counters + p);
-        uint64_t* tk = kin; kin = kout; kout = tk;
-        uint32_t* tv = vin; vin = vout; vout = tv;
-    }
-    if (kin != keys) {
-        hipLaunchKernelGGL(copy_back_kernel, dim3(spread), dim3(256), 0, s, keys, values, kin, vin, n_dev, cap);
-    }
-    return hipGetLastError();
+    return sort_impl(keys, values, n_dev, cap, lo_bit, key_bits, sc, s);
+}
+
+hipError_t launch_sort(uint32_t* keys, uint32_t* values, const uint64_t* n_dev, uint32_t cap, uint32_t lo_bit,
+                       uint32_t key_bits, const SortScratch& sc, hipStream_t s) {
+    return sort_impl(keys, values, n_dev, cap, lo_bit, key_bits, sc, s);
 }
 
 }  // namespace athx

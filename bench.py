@@ -46,15 +46,18 @@ def stage_bytes(n: int, visible: int, pairs: int, tiles: int, pixels: int, key_b
         # FP32 parameters read (236 B at SH degree 3); record, tile count and key written
         "project": n * (236 + 48 + 4 + 4),
         # compaction (counts and keys read, visible (key, slot) written), then
-        # a stable 24-bit sort of the visible: a histogram read and a read and
-        # write of (key, slot) a pass
-        "depth_sort": n * 8 + visible * 12 + visible * 8 + depth_passes * visible * 24,
-        # counts gathered in depth order, scanned, offsets scattered to slots
-        "counts": visible * 16,
-        # records, offsets and keys read; (key, value) pairs written
-        "emit": visible * (48 + 4 + 4 + 4) + pairs * 12,
-        # a histogram read of the keys, then a read and a write of (key, value) a pass
-        "tile_sort": pairs * 8 + tile_passes * pairs * 24,
+        # a stable 24-bit sort of the visible u32 keys: a histogram read and a
+        # read and write of (key, slot) a pass
+        "depth_sort": n * 8 + visible * 8 + visible * 4 + depth_passes * visible * 16,
+        # sorted slots read, counts gathered, offsets scattered to slots
+        "counts": visible * 12,
+        # count, offset and the footprint's tile hits (rect + mask) read a
+        # slot; (tile, slot) pairs written. The depth half of the reference's
+        # 64-bit key is the emission order itself, so it is not stored.
+        "emit": visible * (4 + 4 + 16) + pairs * 8,
+        # a histogram read of the u32 tile keys, then a read and a write of
+        # (tile, slot) a pass
+        "tile_sort": pairs * 4 + tile_passes * pairs * 16,
         "ranges": tiles * 8,
         # each pair's value and record read once; RGBA, depth, final T, last index written
         "blend": pairs * (4 + 48) + tiles * 8 + pixels * 28,
@@ -93,6 +96,7 @@ def main() -> int:
     ap.add_argument("--batch", type=int, default=256)
     ap.add_argument("--no-extras", action="store_true", help="forward only: no backward, batch or CPU legs")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 60-view walk")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the rasterizer shards by views or screen tiles across GPUs, "
@@ -211,6 +215,7 @@ def main() -> int:
         result["fwd_bwd_over_fwd"] = round(result["fwd_bwd_ms"] / fwd_ms, 3)
         del params
 
+    if not args.no_extras and not args.no_batch:
         log("batched 60-view walk")
         walk_np = synthetic.walk_cloud()
         walk = GaussianCloud.from_numpy(walk_np, dev)
@@ -262,9 +267,9 @@ def main() -> int:
         }
         raster.release_scratch()
 
-        if not args.no_cpu:
-            log("CPU baseline (the oracle on the host cores)")
-            result["cpu_baseline"] = cpu_baseline(cloud_np, view, args.tile)
+    if not args.no_extras and not args.no_cpu:
+        log("CPU baseline (the oracle on the host cores)")
+        result["cpu_baseline"] = cpu_baseline(cloud_np, view, args.tile)
 
     print(json.dumps(result), flush=True)
     return 0

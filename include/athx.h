@@ -284,7 +284,9 @@ ATHX_API athx_status athx_latest_counters(athx_ctx* ctx, athx_counters* out);
  *   OFFSETS      uint32[slots]: a visible slot's first pair -- the exclusive
  *                scan of TOUCHED taken in depth order (visible slots stably
  *                sorted by depth key); undefined for culled slots
- *   PAIR_KEYS    uint64[pairs]: global tile << 24 | depth key, sorted
+ *   PAIR_KEYS    uint64[pairs]: global tile << 24 | depth key, sorted (the
+ *                frame stores each pair's tile; the key is rebuilt from it
+ *                and the depth key of the pair's slot, on the stream)
  *   PAIR_VALUES  uint32[pairs]: the slot of each sorted pair
  *   RANGES       uint32[tiles][2]: a tile's [start, end) in the sorted pairs
  */
