@@ -52,6 +52,26 @@ class View(ctypes.Structure):
     ]
 
 
+class Sky(ctypes.Structure):
+    _fields_ = [
+        ("texels", ctypes.c_void_p),
+        ("sh", ctypes.c_void_p),
+        ("sun", ctypes.c_void_p),
+        ("base_side", ctypes.c_uint32),
+    ]
+
+
+class Relight(ctypes.Structure):
+    _fields_ = [
+        ("normals", ctypes.c_void_p),
+        ("albedo", ctypes.c_void_p),
+        ("metallic", ctypes.c_void_p),
+        ("roughness", ctypes.c_void_p),
+        ("object_to_world", ctypes.c_float * 12),
+        ("sky", Sky),
+    ]
+
+
 class Gaussians(ctypes.Structure):
     _fields_ = [
         ("count", ctypes.c_uint32),
@@ -62,6 +82,7 @@ class Gaussians(ctypes.Structure):
         ("opacity_logits", ctypes.c_void_p),
         ("sh", ctypes.c_void_p),
         ("sh_half", ctypes.c_void_p),
+        ("relight", ctypes.POINTER(Relight)),
     ]
 
 

@@ -27,6 +27,7 @@ constexpr uint32_t kFlagClampYHi = 8u;   // ... top
 constexpr uint32_t kFlagPaidBack = 16u;  // paidOpacity's pay-back branch (frame.slang:316-319)
 constexpr uint32_t kFlagColourR = 32u;   // channel's harmonics sum > 0 (R; G and B the next bits)
 constexpr uint32_t kFlagSrgbR = 256u;    // sRGB decode took the power branch (R; G, B next)
+constexpr uint32_t kFlagRelit = 2048u;   // the colour is the relit one
 
 // One view as every kernel reads it: the camera (athx_view), where its tiles
 // and slots start in the frame's global numbering, and its targets.
@@ -46,6 +47,20 @@ struct DView {
     const float* dl_dcolour;
 };
 
+// What the relit colour reads besides the Gaussian's own geometry
+// (athx_relight; the sky is one prepared dome).
+struct RelightArgs {
+    const float* normals;    // [n][3] stored shading normals (cloud space), or null
+    const float* albedo;     // [n][3], or null: the evaluated harmonics
+    const float* metallic;   // [n]
+    const float* roughness;  // [n]
+    float o2w[12];           // object -> world rows
+    const uint32_t* texels;  // prepared dome: 8 octahedral levels, 2 words a texel
+    const float* sky_sh;     // 16 float4
+    const float* sky_sun;    // 2 float4
+    uint32_t base_side;
+};
+
 // What the projection is told besides the views.
 struct ProjectArgs {
     const float* means;
@@ -60,6 +75,8 @@ struct ProjectArgs {
     uint32_t linear;
     uint32_t antialias;
     uint32_t tile;
+    uint32_t relit;  // 1: the colour is the relit one (rl), not the harmonics'
+    RelightArgs rl;
 };
 
 __host__ __device__ inline uint32_t div_up(uint32_t a, uint32_t b) { return (a + b - 1) / b; }

@@ -181,6 +181,17 @@ athx_status check_gaussians(athx_ctx* ctx, const athx_gaussians* g) {
         if ((g->sh == nullptr) == (g->sh_half == nullptr)) {
             return fail(ctx, ATHX_INVALID_ARGUMENT, "exactly one of sh and sh_half must be given");
         }
+        if (const athx_relight* r = g->relight) {
+            if (!r->metallic || !r->roughness) {
+                return fail(ctx, ATHX_INVALID_ARGUMENT, "relight needs metallic and roughness");
+            }
+            if (!r->sky.texels || !r->sky.sh || !r->sky.sun || r->sky.base_side == 0) {
+                return fail(ctx, ATHX_INVALID_ARGUMENT, "relight needs a prepared sky (texels, sh, sun, base side)");
+            }
+            if (r->sky.base_side > 16384) {
+                return fail(ctx, ATHX_INVALID_ARGUMENT, "sky base side over 16384");
+            }
+        }
     }
     return ATHX_OK;
 }
@@ -202,6 +213,20 @@ ProjectArgs project_args(const athx_gaussians* g, const athx_settings* s) {
     a.linear = s->linear ? 1u : 0u;
     a.antialias = s->antialias ? 1u : 0u;
     a.tile = s->tile;
+    if (const athx_relight* r = g->relight) {
+        a.relit = 1;
+        a.rl.normals = r->normals;
+        a.rl.albedo = r->albedo;
+        a.rl.metallic = r->metallic;
+        a.rl.roughness = r->roughness;
+        for (int k = 0; k < 12; ++k) {
+            a.rl.o2w[k] = r->object_to_world[k];
+        }
+        a.rl.texels = r->sky.texels;
+        a.rl.sky_sh = r->sky.sh;
+        a.rl.sky_sun = r->sky.sun;
+        a.rl.base_side = r->sky.base_side;
+    }
     return a;
 }
 
@@ -784,6 +809,9 @@ ATHX_API athx_status athx_render_backward(athx_ctx* ctx, const athx_view* views,
         ctx->last_tile != settings->tile) {
         return fail(ctx, ATHX_INVALID_ARGUMENT, "the backward pass must follow a render of the same frame");
     }
+    if (gaussians->relight) {
+        return fail(ctx, ATHX_UNSUPPORTED, "the adjoint of the relit colour is not built: backward is SH-coloured only");
+    }
     for (uint32_t k = 0; k < nviews; ++k) {
         if (!targets[k].final_t || !targets[k].last || !dl_dcolour[k]) {
             return fail(ctx, ATHX_INVALID_ARGUMENT,
@@ -1051,6 +1079,10 @@ ATHX_API athx_status athx_graph_create(int device, const athx_view* views, uint3
     athx_ctx* ctx = g->ctx;
     ctx->exact_growth = true;
     const bool with_backward = dl_dcolour != nullptr && grads != nullptr;
+    if (with_backward && gaussians && gaussians->relight) {
+        return graph_fail(g, fail(ctx, ATHX_UNSUPPORTED, "the adjoint of the relit colour is not built"), error,
+                          error_bytes);
+    }
     if (with_backward) {
         for (uint32_t k = 0; views && targets && k < nviews; ++k) {
             if (!targets[k].final_t || !targets[k].last || !dl_dcolour[k]) {

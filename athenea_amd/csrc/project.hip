@@ -10,6 +10,7 @@
 // depth keys, tile counts and records are the same bits a host restatement
 // computes.
 #include "athx_internal.h"
+#include "relight.h"
 
 namespace athx {
 namespace {
@@ -131,7 +132,13 @@ __device__ inline void cull(uint32_t slot, uint32_t reason, uint32_t* touched, u
     keys[slot] = kCulledKey | reason;
 }
 
-template <bool kHalf>
+// A slot culled before its tiles are counted: no rectangle to count over.
+__device__ inline void cull_early(uint32_t slot, uint32_t reason, uint32_t* touched, uint32_t* keys, uint2* rects) {
+    cull(slot, reason, touched, keys);
+    rects[slot] = make_uint2(0u, 0u);
+}
+
+template <bool kHalf, bool kRelit>
 __global__ void __launch_bounds__(256) project_kernel(ProjectArgs a, const DView* __restrict__ views, uint32_t nviews,
                                                       float* __restrict__ records, uint32_t* __restrict__ touched,
                                                       uint32_t* __restrict__ keys, Footprints fp) {
@@ -172,7 +179,7 @@ __global__ void __launch_bounds__(256) project_kernel(ProjectArgs a, const DView
         const float ty = m[4] * px + m[5] * py + m[6] * pz + m[7];
         const float tz = m[8] * px + m[9] * py + m[10] * pz + m[11];
         if (!(tz > V.near_z) || tz > V.far_z) {
-            cull(slot, ATHX_CULL_DEPTH, touched, keys);
+            cull_early(slot, ATHX_CULL_DEPTH, touched, keys, fp.rects);
             continue;
         }
         float jx0, jx2, jy1, jy2, cx, cy;
@@ -224,7 +231,7 @@ __global__ void __launch_bounds__(256) project_kernel(ProjectArgs a, const DView
         cc = cc + 0.3f;
         const float det = ca * cc - cb * cb;
         if (!(det > 1e-12f)) {
-            cull(slot, ATHX_CULL_DEGENERATE, touched, keys);
+            cull_early(slot, ATHX_CULL_DEGENERATE, touched, keys, fp.rects);
             continue;
         }
         const float filtered = sqrtf(fmax_(det_before, 0.0f) / det);
@@ -233,7 +240,7 @@ __global__ void __launch_bounds__(256) project_kernel(ProjectArgs a, const DView
         const float alpha = spread >= 1.0f / 255.0f ? spread + (1.0f - paid) * (1.0f / 255.0f) : spread;
         flags |= spread >= 1.0f / 255.0f ? kFlagPaidBack : 0u;
         if (!(alpha >= 1.0f / 255.0f)) {
-            cull(slot, ATHX_CULL_FAINT, touched, keys);
+            cull_early(slot, ATHX_CULL_FAINT, touched, keys, fp.rects);
             continue;
         }
         const float bdet = ca * cc - cb * cb;
@@ -241,35 +248,19 @@ __global__ void __launch_bounds__(256) project_kernel(ProjectArgs a, const DView
         const float cutoff = cutoff_power(alpha);
         uint32_t x0, y0, x1, y1;
         if (!tile_rect(V.width, V.height, a.tile, cx, cy, ka, kb, kc, cutoff, x0, y0, x1, y1)) {
-            cull(slot, ATHX_CULL_OFFSCREEN, touched, keys);
+            cull_early(slot, ATHX_CULL_OFFSCREEN, touched, keys, fp.rects);
             continue;
         }
-        // The exact count, with the test the emission repeats
-        // (splat_project.slang:764-776). A footprint over kBigRect tiles is
-        // counted by a whole wavefront afterwards (count_big_kernel): one
-        // thread walking thousands of tiles would hold its wave back.
-        // The hits are kept as a bitmask, so the emission need not test again.
-        uint32_t count = 0;
+        // The exact count (splat_project.slang:764-776) is taken after this
+        // kernel, with the tiles of many footprints spread over the lanes of
+        // a wavefront: count_small_kernel for rectangles of up to kBigRect
+        // tiles, count_big_kernel a wavefront per larger one. A thread
+        // walking its own rectangle would hold its wavefront to the largest
+        // of 64. The count kernels also write the slot's tile count and,
+        // when no tile is touched, its cull.
         const uint32_t w = x1 - x0 + 1, h = y1 - y0 + 1;
-        const bool big = w * h > kBigRect;
-        if (big) {
+        if (w * h > kBigRect) {
             fp.big_list[atomicAdd(&fp.counters[0], 1u)] = slot;
-        } else {
-            uint64_t mask = 0;
-            uint32_t k = 0;
-            for (uint32_t ty_ = y0; ty_ <= y1; ++ty_) {
-                for (uint32_t tx_ = x0; tx_ <= x1; ++tx_, ++k) {
-                    if (tile_touched(V.width, V.height, a.tile, cx, cy, ka, kb, kc, cutoff, tx_, ty_)) {
-                        mask |= 1ull << k;
-                    }
-                }
-            }
-            count = (uint32_t)__popcll(mask);
-            if (count == 0) {
-                cull(slot, ATHX_CULL_NO_TILE, touched, keys);
-                continue;
-            }
-            fp.masks[slot] = mask;
         }
         fp.rects[slot] = make_uint2(x0 | (y0 << 16), w | (h << 16));
         // Colour from this eye, in the cloud's space (splat_project.slang:781-785).
@@ -286,11 +277,21 @@ __global__ void __launch_bounds__(256) project_kernel(ProjectArgs a, const DView
             flags |= (cv > 0.0f ? kFlagColourR : 0u) << ch;
             flags |= (!a.linear && fmax_(cv, 0.0f) > 0.04045f ? kFlagSrgbR : 0u) << ch;
         }
+        if constexpr (kRelit) {
+            // The relit branch (splat_project.slang:799-976): the evaluated
+            // colour is the surface's albedo unless one is given.
+            const float R[3][3] = {{r00, r01, r02}, {r10, r11, r12}, {r20, r21, r22}};
+            const float3 lit = relit_colour(a.rl, i, R, sx, sy, sz, px, py, pz, V.eye,
+                                            make_float3(col[0], col[1], col[2]));
+            col[0] = lit.x;
+            col[1] = lit.y;
+            col[2] = lit.z;
+            flags |= kFlagRelit;
+        }
         float4* rec = reinterpret_cast<float4*>(records + (size_t)slot * kRecordFloats);
         rec[0] = make_float4(cx, cy, tz, cutoff);
         rec[1] = make_float4(ka, kb, kc, alpha);
         rec[2] = make_float4(col[0], col[1], col[2], __builtin_bit_cast(float, flags));
-        touched[slot] = count;
         keys[slot] = __builtin_bit_cast(uint32_t, tz) >> 7;
     }
 }
@@ -298,38 +299,6 @@ __global__ void __launch_bounds__(256) project_kernel(ProjectArgs a, const DView
 __global__ void fill_u32_kernel(uint32_t* p, uint32_t value, uint64_t n) {
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
         p[i] = value;
-    }
-}
-
-// One (tile, depth) pair per tile the footprint touches, a slot's tiles row
-// by row, at the slot's scanned offset: splat_emit.slang:17-50, emitted with
-// the 64-bit key (global tile << 24 | 24-bit depth key). The tiles come from
-// the projection's hit mask; a big footprint is emit_big_kernel's.
-__global__ void __launch_bounds__(256) emit_kernel(const DView* __restrict__ views, uint32_t nviews, uint32_t n,
-                                                   const uint32_t* __restrict__ touched,
-                                                   const uint32_t* __restrict__ offsets, Footprints fp, uint32_t cap,
-                                                   uint32_t* __restrict__ keys, uint32_t* __restrict__ values) {
-    const uint64_t slot = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (slot >= (uint64_t)n * nviews) {
-        return;
-    }
-    const uint32_t expected = touched[slot];
-    if (expected == 0) {
-        return;
-    }
-    const uint2 rect = fp.rects[slot];
-    const uint32_t x0 = rect.x & 0xFFFFu, y0 = rect.x >> 16, w = rect.y & 0xFFFFu, h = rect.y >> 16;
-    if (w * h > kBigRect) {
-        return;
-    }
-    const DView& V = views[slot / n];
-    const uint32_t row_base = V.tile_base + y0 * V.tiles_x + x0;
-    uint32_t at = offsets[slot];
-    const uint32_t stop = (uint64_t)at + expected < cap ? at + expected : cap;
-    for (uint64_t m = fp.masks[slot]; m != 0 && at < stop; m &= m - 1, ++at) {
-        const uint32_t k = (uint32_t)__builtin_ctzll(m);
-        keys[at] = row_base + (k / w) * V.tiles_x + (k % w);
-        values[at] = (uint32_t)slot;
     }
 }
 
@@ -520,6 +489,160 @@ __global__ void __launch_bounds__(kBigThreads) emit_big_kernel(const DView* __re
     }
 }
 
+// Small footprints (rectangles of up to kBigRect tiles), 64 slots to a
+// wavefront, their tiles spread over the lanes: lane j holds slot c0 + j
+// and its rectangle's area A_j (0 for a culled or a big slot), and the
+// wavefront walks the A_0 + ... + A_63 tiles 64 at a time, lane L taking
+// tile g = g0 + L of the concatenation -- tile k = g - start_j of the
+// footprint j whose run holds g (a binary search over the runs' ends in
+// LDS). Every lane tests a tile on every step but the last, whatever the
+// footprints' sizes, where a thread walking its own rectangle would hold
+// its wavefront to the largest of 64.
+constexpr uint32_t kSmallThreads = 256;
+
+// k / w and k % w for k < 64, 1 <= w <= 64: floor(k rcp(w) + 1e-3) is exact
+// there (rcp's relative error, under 2^-22, moves k/w by at most 1.6e-5,
+// and a k/w that is not whole is at least 1/64 below the next integer).
+__device__ inline uint32_t small_row(uint32_t k, uint32_t w) {
+    return (uint32_t)((float)k * __builtin_amdgcn_rcpf((float)w) + 1.0e-3f);
+}
+
+// The footprint whose run holds g: the first j with end[j] > g.
+__device__ inline uint32_t run_of(const uint32_t* ends, uint32_t g) {
+    uint32_t lo = 0, hi = 63;
+#pragma unroll
+    for (uint32_t step = 0; step < 6; ++step) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (ends[mid] > g) {
+            hi = mid;
+        } else {
+            lo = mid + 1;
+        }
+    }
+    return lo;
+}
+
+__global__ void __launch_bounds__(kSmallThreads) count_small_kernel(const DView* __restrict__ views, uint32_t n,
+                                                                    uint32_t slots, uint32_t T,
+                                                                    const float* __restrict__ records, Footprints fp,
+                                                                    uint32_t* __restrict__ touched,
+                                                                    uint32_t* __restrict__ keys) {
+    __shared__ uint32_t s_end[kSmallThreads];
+    __shared__ float4 s_geo[kSmallThreads];     // centre x, y, cut power, conic a
+    __shared__ float4 s_shape[kSmallThreads];   // conic b, c, x0 | y0 << 16, w | view << 16 (bits)
+    __shared__ uint64_t s_mask[kSmallThreads];
+    const uint32_t lane = threadIdx.x & 63u, wbase = threadIdx.x & ~63u;
+    const uint32_t slot = blockIdx.x * kSmallThreads + threadIdx.x;
+    const uint2 rect = slot < slots ? fp.rects[slot] : make_uint2(0u, 0u);
+    const uint32_t w = rect.y & 0xFFFFu, area = w * (rect.y >> 16);
+    const uint32_t mine = area <= kBigRect ? area : 0u;
+    if (mine != 0) {
+        const float4* rec = reinterpret_cast<const float4*>(records + (size_t)slot * kRecordFloats);
+        const float4 r0 = rec[0], r1 = rec[1];
+        s_geo[threadIdx.x] = make_float4(r0.x, r0.y, r0.w, r1.x);
+        s_shape[threadIdx.x] = make_float4(r1.y, r1.z, __builtin_bit_cast(float, rect.x),
+                                           __builtin_bit_cast(float, w | (slot / n) << 16));
+    }
+    const uint32_t end = wave_inclusive_sum(mine, lane);
+    s_end[threadIdx.x] = end;
+    s_mask[threadIdx.x] = 0;
+    const uint32_t total = lane_u(end, 63);
+    __syncthreads();
+    const uint32_t* ends = s_end + wbase;
+    for (uint32_t g0 = 0; g0 < total; g0 += 64) {
+        const uint32_t g = g0 + lane;
+        const bool on = g < total;
+        const uint32_t j = on ? run_of(ends, g) : 0u;
+        const uint32_t k = on ? g - (j != 0 ? ends[j - 1] : 0u) : 0u;
+        bool hit = false;
+        if (on) {
+            const float4 geo = s_geo[wbase + j], shape = s_shape[wbase + j];
+            const uint32_t corner = __builtin_bit_cast(uint32_t, shape.z);
+            const uint32_t wv = __builtin_bit_cast(uint32_t, shape.w);
+            const uint32_t wj = wv & 0xFFFFu;
+            const DView& V = views[wv >> 16];
+            const uint32_t q = small_row(k, wj);
+            hit = tile_touched(V.width, V.height, T, geo.x, geo.y, geo.w, shape.x, shape.y, geo.z,
+                               (corner & 0xFFFFu) + (k - q * wj), (corner >> 16) + q);
+        }
+        // The first lane of each footprint's run in this step folds the
+        // run's hits into the footprint's mask.
+        const uint64_t hits = __ballot(hit);
+        const bool leader = on && (k == 0 || lane == 0);
+        const uint64_t leaders = __ballot(leader);
+        if (leader) {
+            const uint64_t after = leaders & ~((2ull << lane) - 1ull);
+            const uint32_t len = (after != 0 ? (uint32_t)__builtin_ctzll(after) : 64u) - lane;
+            const uint64_t run = (hits >> lane) & (len >= 64 ? ~0ull : (1ull << len) - 1ull);
+            s_mask[wbase + j] |= run << k;
+        }
+    }
+    __syncthreads();
+    if (mine != 0) {
+        const uint64_t mask = s_mask[threadIdx.x];
+        const uint32_t count = (uint32_t)__popcll(mask);
+        fp.masks[slot] = mask;
+        touched[slot] = count;
+        if (count == 0) {
+            keys[slot] = kCulledKey | ATHX_CULL_NO_TILE;
+        }
+    }
+}
+
+// One (tile, slot) pair per tile a small footprint touches, its tiles row by
+// row from its first pair on (splat_emit.slang:17-50), read from the count's
+// hit mask: the same spread of rectangles over lanes, a tile's place among
+// its footprint's pairs the hits before it in the mask. The pairs of one
+// footprint land side by side, so the stores of a step are contiguous runs.
+__global__ void __launch_bounds__(kSmallThreads) emit_small_kernel(const DView* __restrict__ views, uint32_t n,
+                                                                   uint32_t slots,
+                                                                   const uint32_t* __restrict__ touched,
+                                                                   const uint32_t* __restrict__ offsets,
+                                                                   Footprints fp, uint32_t cap,
+                                                                   uint32_t* __restrict__ keys,
+                                                                   uint32_t* __restrict__ values) {
+    __shared__ uint32_t s_end[kSmallThreads];
+    __shared__ uint64_t s_mask[kSmallThreads];
+    __shared__ uint4 s_at[kSmallThreads];  // first pair, first tile (the rectangle's corner), w, tiles_x
+    const uint32_t lane = threadIdx.x & 63u, wbase = threadIdx.x & ~63u;
+    const uint32_t slot = blockIdx.x * kSmallThreads + threadIdx.x;
+    const uint32_t count = slot < slots ? touched[slot] : 0u;
+    uint32_t mine = 0;
+    if (count != 0) {
+        const uint2 rect = fp.rects[slot];
+        const uint32_t w = rect.y & 0xFFFFu, area = w * (rect.y >> 16);
+        if (area <= kBigRect) {
+            mine = area;
+            const DView& V = views[slot / n];
+            s_mask[threadIdx.x] = fp.masks[slot];
+            s_at[threadIdx.x] = make_uint4(offsets[slot], V.tile_base + (rect.x >> 16) * V.tiles_x + (rect.x & 0xFFFFu),
+                                           w, V.tiles_x);
+        }
+    }
+    const uint32_t end = wave_inclusive_sum(mine, lane);
+    s_end[threadIdx.x] = end;
+    const uint32_t total = lane_u(end, 63);
+    __syncthreads();
+    const uint32_t* ends = s_end + wbase;
+    for (uint32_t g0 = 0; g0 < total; g0 += 64) {
+        const uint32_t g = g0 + lane;
+        if (g < total) {
+            const uint32_t j = run_of(ends, g);
+            const uint32_t k = g - (j != 0 ? ends[j - 1] : 0u);
+            const uint64_t mask = s_mask[wbase + j];
+            if ((mask >> k) & 1u) {
+                const uint4 at = s_at[wbase + j];
+                const uint32_t dst = at.x + (uint32_t)__popcll(mask & ((1ull << k) - 1ull));
+                if (dst < cap) {
+                    const uint32_t q = small_row(k, at.z);
+                    keys[dst] = at.y + q * at.w + (k - q * at.z);
+                    values[dst] = blockIdx.x * kSmallThreads + wbase + j;
+                }
+            }
+        }
+    }
+}
+
 // Where each tile's pairs start and end in the sorted keys, by binary search
 // on the tile bits: splat_tiles_clear.slang:8-17 and splat_ranges.slang:11-25
 // give a tile with no pairs (0, 0), and so does this.
@@ -606,13 +729,12 @@ hipError_t launch_project(const ProjectArgs& a, const DView* views_dev, uint32_t
         return hipGetLastError();
     }
     const dim3 grid(div_up(a.n, 256));
-    if (a.sh_half) {
-        hipLaunchKernelGGL(project_kernel<true>, grid, dim3(256), 0, s, a, views_dev, nviews, records, touched,
-                           depth_keys, fp);
-    } else {
-        hipLaunchKernelGGL(project_kernel<false>, grid, dim3(256), 0, s, a, views_dev, nviews, records, touched,
-                           depth_keys, fp);
-    }
+    auto kernel = a.sh_half ? (a.relit ? project_kernel<true, true> : project_kernel<true, false>)
+                            : (a.relit ? project_kernel<false, true> : project_kernel<false, false>);
+    hipLaunchKernelGGL(kernel, grid, dim3(256), 0, s, a, views_dev, nviews, records, touched, depth_keys, fp);
+    const uint32_t slots = a.n * nviews;
+    hipLaunchKernelGGL(count_small_kernel, dim3(div_up(slots, kSmallThreads)), dim3(kSmallThreads), 0, s, views_dev,
+                       a.n, slots, a.tile, records, fp, touched, depth_keys);
     hipLaunchKernelGGL(count_big_kernel, dim3(kBigBlocks), dim3(kBigThreads), 0, s, views_dev, a.n, a.tile, records,
                        fp, touched, depth_keys);
     return hipGetLastError();
@@ -637,8 +759,8 @@ hipError_t launch_emit(const DView* views_dev, uint32_t nviews, uint32_t n, uint
     if (slots == 0 || cap == 0) {
         return hipSuccess;
     }
-    hipLaunchKernelGGL(emit_kernel, dim3((uint32_t)((slots + 255) / 256)), dim3(256), 0, s, views_dev, nviews, n,
-                       touched, offsets, fp, cap, keys, values);
+    hipLaunchKernelGGL(emit_small_kernel, dim3(div_up((uint32_t)slots, kSmallThreads)), dim3(kSmallThreads), 0, s,
+                       views_dev, n, (uint32_t)slots, touched, offsets, fp, cap, keys, values);
     hipLaunchKernelGGL(emit_big_kernel, dim3(kBigBlocks), dim3(kBigThreads), 0, s, views_dev, n, tile, records, fp,
                        touched, offsets, cap, keys, values);
     return hipGetLastError();

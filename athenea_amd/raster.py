@@ -48,9 +48,26 @@ class RenderSettings:
 
 
 @dataclass
+class Relight:
+    """athx_relight: relitSplat's dome term with the plain lobes in place of
+    the SH colour (SplatInstance::relight with one prepared dome). Tensors
+    are float32 device tensors; ``sky`` is a DeviceSky (athenea_amd.sky).
+    ``albedo`` None: the evaluated SH colour is the albedo; ``normals`` None:
+    the disc's short axis is the shading normal."""
+
+    metallic: torch.Tensor  # [n]
+    roughness: torch.Tensor  # [n]
+    sky: object
+    normals: torch.Tensor | None = None  # [n, 3]
+    albedo: torch.Tensor | None = None  # [n, 3]
+    object_to_world: tuple = (1.0, 0.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0, 0.0, 1.0, 0.0)
+
+
+@dataclass
 class GaussianCloud:
     """Device tensors in athx_gaussians' layout. ``linear``: colours are linear
-    light (GpuSplats::linear); otherwise sRGB, decoded per splat."""
+    light (GpuSplats::linear); otherwise sRGB, decoded per splat. ``relight``:
+    the relit colour instead of the harmonics' (forward only)."""
 
     means: torch.Tensor
     log_scales: torch.Tensor
@@ -58,18 +75,21 @@ class GaussianCloud:
     opacity_logits: torch.Tensor
     sh: torch.Tensor  # [n, k, 3], float32 or float16
     linear: bool = True
+    relight: Relight | None = None
 
     @property
     def count(self) -> int:
         return int(self.means.shape[0])
 
     @staticmethod
-    def from_numpy(cloud, device="cuda", linear: bool = True) -> "GaussianCloud":
+    def from_numpy(cloud, device="cuda", linear: bool = True, relight=None) -> "GaussianCloud":
+        """``relight``: an athenea_amd.sky.RelightAttributes, or None."""
+
         def t(a):
             return torch.from_numpy(a).to(device).contiguous()
 
         return GaussianCloud(t(cloud.means), t(cloud.log_scales), t(cloud.quats), t(cloud.opacity_logits),
-                             t(cloud.sh), linear)
+                             t(cloud.sh), linear, relight.to_device(device) if relight is not None else None)
 
 
 @dataclass
@@ -146,6 +166,20 @@ def _check_cloud(cloud: GaussianCloud) -> None:
         raise ValueError(f"sh must be [n, 1|4|9|16, 3], got {tuple(cloud.sh.shape)}")
     if cloud.sh.dtype not in (torch.float32, torch.float16):
         raise ValueError("sh must be float32 or float16")
+    r = cloud.relight
+    if r is not None:
+        per = {"metallic": (r.metallic, (n,)), "roughness": (r.roughness, (n,)), "normals": (r.normals, (n, 3)),
+               "albedo": (r.albedo, (n, 3))}
+        for name, (t, shape) in per.items():
+            if t is None and name in ("normals", "albedo"):
+                continue
+            if t is None or tuple(t.shape) != shape or t.dtype != torch.float32 or not t.is_cuda:
+                raise ValueError(f"relight {name} must be a float32 device tensor of shape {shape}")
+        if len(r.object_to_world) != 12:
+            raise ValueError("relight object_to_world is 12 floats (rows of a 3x4)")
+        sky = r.sky
+        if tuple(sky.sh.shape) != (16, 4) or tuple(sky.sun.shape) != (2, 4):
+            raise ValueError("sky sh is [16, 4] and sun [2, 4]")
 
 
 def _gaussians_c(cloud: GaussianCloud) -> _lib.Gaussians:
@@ -163,6 +197,21 @@ def _gaussians_c(cloud: GaussianCloud) -> _lib.Gaussians:
     else:
         g.sh = _ptr(cloud.sh)
         g.sh_half = None
+    r = cloud.relight
+    if r is not None:
+        c = _lib.Relight()
+        c.normals = _ptr(r.normals)
+        c.albedo = _ptr(r.albedo)
+        c.metallic = _ptr(r.metallic)
+        c.roughness = _ptr(r.roughness)
+        for k in range(12):
+            c.object_to_world[k] = float(r.object_to_world[k])
+        c.sky.texels = _ptr(r.sky.texels)
+        c.sky.sh = _ptr(r.sky.sh)
+        c.sky.sun = _ptr(r.sky.sun)
+        c.sky.base_side = int(r.sky.base_side)
+        g.relight = ctypes.pointer(c)
+        g._relight = c  # the pointee lives as long as the struct that points to it
     return g
 
 

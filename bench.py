@@ -1,13 +1,16 @@
 """Benchmark of the MI355X splat tile rasterizer.
 
 A step is one forward frame of the 1080p workload (1.4M Gaussians, SH degree
-3, FP32, 1920x1080, 16x16 tiles): projection, scan, pair emission, the
-onesweep sort, tile ranges and the blend, with the Gaussians already resident
-in HBM. Prints ONE JSON line:
+3, FP32, 1920x1080, 16x16 tiles, each Gaussian with relight attributes --
+albedo, roughness, metallic, a stored normal -- under a prepared 64x32
+log-normal HDR sky): projection with the relit colour, scan, pair emission,
+the onesweep sort, tile ranges and the blend, with the Gaussians and the sky
+already resident in HBM. Prints ONE JSON line:
 
   value         forward ms/frame (lower is better)
-  fwd_bwd_ms    forward + adjoint backward ms/frame (loss L2 against a target
-                rendered from jittered means and SH)
+  fwd_sh_ms     the same frame coloured by the harmonics instead
+  fwd_bwd_ms    forward + adjoint backward ms/frame of the SH-coloured frame
+                (loss L2 against a target rendered from jittered means and SH)
   batched       the 60-view walk (2.36M Gaussians, 960x540 views) in one
                 launch: views/s, and the per-view time of the same views
                 rendered one launch each
@@ -37,14 +40,17 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def stage_bytes(n: int, visible: int, pairs: int, tiles: int, pixels: int, key_bits: int) -> dict:
+def stage_bytes(n: int, visible: int, pairs: int, tiles: int, pixels: int, key_bits: int,
+                relit: bool = False) -> dict:
     """Compulsory HBM bytes per stage (one view): what each stage must read
     and write whatever its implementation."""
     depth_passes = 3
     tile_passes = (key_bits - 24 + 7) // 8
     return {
-        # FP32 parameters read (236 B at SH degree 3); record, tile count and key written
-        "project": n * (236 + 48 + 4 + 4),
+        # FP32 parameters read (236 B at SH degree 3, + 32 B of relight
+        # attributes: albedo, normal, metallic, roughness); record, tile count
+        # and key written
+        "project": n * (236 + (32 if relit else 0) + 48 + 4 + 4),
         # compaction (counts and keys read, visible (key, slot) written), then
         # a stable 24-bit sort of the visible u32 keys: a histogram read and a
         # read and write of (key, slot) a pass
@@ -64,26 +70,26 @@ def stage_bytes(n: int, visible: int, pairs: int, tiles: int, pixels: int, key_b
     }
 
 
-def cpu_baseline(cloud_np, view, tile: int) -> dict:
+def cpu_baseline(cloud_np, view, tile: int, relight) -> dict:
     from oracle import pyoracle as po
 
     po.load()
     threads = po.threads()
     ov = po.orc_view(view, tile=tile)
     times = []
-    po.render_ref_order(ov, cloud_np)  # warm-up
+    po.render_ref_order(ov, cloud_np, relight=relight)  # warm-up
     for _ in range(3):
         t0 = time.perf_counter()
-        po.render_ref_order(ov, cloud_np)
+        po.render_ref_order(ov, cloud_np, relight=relight)
         times.append((time.perf_counter() - t0) * 1e3)
     return {
         "value": round(statistics.median(times), 2),
         "unit": "ms/frame",
         "cores": threads,
         "kind": "port",
-        "sample": "one full forward frame of the same 1.4M-Gaussian 1920x1080 workload (project, 24-bit depth "
-                  "sort, emit, stable tile sort, ranges, blend), median of 3 after one warm-up, OpenMP over "
-                  f"{threads} host threads",
+        "sample": "one full forward frame of the same relit 1.4M-Gaussian 1920x1080 workload (project with the "
+                  "relit colour, 24-bit depth sort, emit, stable tile sort, ranges, blend), median of 3 after one "
+                  f"warm-up, OpenMP over {threads} host threads",
     }
 
 
@@ -106,30 +112,37 @@ def main() -> int:
     import numpy as np
     import torch
 
-    from athenea_amd import RenderSettings, TileRasterizer, camera, synthetic
+    from athenea_amd import RenderSettings, TileRasterizer, camera, sky as skymod, synthetic
     from athenea_amd.raster import GaussianCloud
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     dev = torch.device("cuda:0")
-    log("generating the 1080p workload")
+    log("generating the 1080p workload and preparing its sky")
     cloud_np = synthetic.plain_1080p_cloud()
+    sky = skymod.prepare_sky(skymod.lognormal_equirect())
+    attrs = skymod.relight_attributes(cloud_np.means.shape[0], sky)
     W, H = 1920, 1080
     view = camera.view_for(synthetic.front_camera(), W, H)
-    cloud = GaussianCloud.from_numpy(cloud_np, dev)
+    cloud_sh = GaussianCloud.from_numpy(cloud_np, dev)
+    cloud = GaussianCloud.from_numpy(cloud_np, dev, relight=attrs)
     settings = RenderSettings(tile=args.tile, batch=args.batch)
     raster = TileRasterizer(0)
-    targets = None
-    log("warm-up")
-    for _ in range(args.warmup):
-        targets, stats = raster.render([view], cloud, settings, targets)
-    torch.cuda.synchronize()
 
-    log(f"timing {args.steps} forward frames")
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        targets, stats = raster.render([view], cloud, settings, targets)
-    torch.cuda.synchronize()
-    fwd_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    def time_forward(c, steps, warmup):
+        targets = None
+        for _ in range(warmup):
+            targets, stats = raster.render([view], c, settings, targets)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            targets, stats = raster.render([view], c, settings, targets)
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) * 1e3 / steps, stats, targets
+
+    log(f"timing {args.steps} SH-coloured forward frames")
+    fwd_sh_ms, _, _ = time_forward(cloud_sh, args.steps, args.warmup)
+    log(f"timing {args.steps} relit forward frames")
+    fwd_ms, stats, targets = time_forward(cloud, args.steps, args.warmup)
     pairs = int(stats.pairs)
     info = raster.frame_info()
 
@@ -143,7 +156,7 @@ def main() -> int:
     stages = {k: statistics.mean(v) for k, v in stage_samples.items()}
     counters = raster.latest_counters()
     tiles = info["tiles"]
-    sbytes = stage_bytes(cloud.count, counters.visible, pairs, tiles, W * H, info["key_bits"])
+    sbytes = stage_bytes(cloud.count, counters.visible, pairs, tiles, W * H, info["key_bits"], relit=True)
     dominant = max(stages, key=stages.get)
     achieved = sbytes[dominant] / (stages[dominant] * 1e-3)
     roofline = {
@@ -170,13 +183,15 @@ def main() -> int:
         "dtype": "fp32",
         "data": "synthetic (seeded, no checkpoint)",
         "config": {
-            "workload": "plain_1080p: 1.4M Gaussians, SH degree 3, FP32, 1920x1080, one view",
+            "workload": "1080p relit: 1.4M Gaussians, SH degree 3, FP32, 1920x1080, one view, relight "
+                        "attributes (albedo, roughness, metallic, normal) under a prepared 64x32 HDR sky",
             "gaussians": cloud.count,
             "width": W,
             "height": H,
             "tile": args.tile,
             "batch": args.batch,
         },
+        "fwd_sh_ms": round(fwd_sh_ms, 3),
         "pairs": pairs,
         "visible": counters.visible,
         "stages_ms": {k: round(v, 4) for k, v in stages.items()},
@@ -212,7 +227,7 @@ def main() -> int:
             step()
         torch.cuda.synchronize()
         result["fwd_bwd_ms"] = round((time.perf_counter() - t0) * 1e3 / args.steps, 3)
-        result["fwd_bwd_over_fwd"] = round(result["fwd_bwd_ms"] / fwd_ms, 3)
+        result["fwd_bwd_over_fwd"] = round(result["fwd_bwd_ms"] / fwd_sh_ms, 3)
         del params
 
     if not args.no_extras and not args.no_batch:
@@ -269,7 +284,7 @@ def main() -> int:
 
     if not args.no_extras and not args.no_cpu:
         log("CPU baseline (the oracle on the host cores)")
-        result["cpu_baseline"] = cpu_baseline(cloud_np, view, args.tile)
+        result["cpu_baseline"] = cpu_baseline(cloud_np, view, args.tile, attrs)
 
     print(json.dumps(result), flush=True)
     return 0

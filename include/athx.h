@@ -114,6 +114,44 @@ typedef struct athx_view {
  * k = 0 the constant term, colour = 0.5 + 0.2820948 * sh0 + rest).
  * Exactly one of sh and sh_half is non-null; sh_half holds IEEE binary16.
  */
+/*
+ * One prepared dome (SplatLights' envTexels / envSh / envSun with envBaseSide,
+ * TileRasterizer.h:178-246; layout light/environment.slang:50-110), device
+ * memory:
+ *   texels  8 octahedral levels of the sky convolved with GGX lobes of
+ *           roughness (k/7)^2, level k of side max(max(base_side, 256) >> k,
+ *           16), levels back to back, texels row by row; 2 words a texel:
+ *           word 0 = red | green << 16, word 1 = blue, IEEE binary16
+ *   sh      16 float4: the sky's radiance in real spherical harmonics (the
+ *           order of light/environment.slang:202-224), rgb; the first 9 read
+ *   sun     2 float4: (direction, disc size), w > 0 when there is a sun;
+ *           (irradiance rgb, unused)
+ */
+typedef struct athx_sky {
+    const uint32_t* texels;
+    const float* sh;
+    const float* sun;
+    uint32_t base_side;
+} athx_sky;
+
+/*
+ * Relighting (SplatInstance::relight, TileRasterizer.h:85-172, with a
+ * SplatLights of one prepared dome): the colour becomes relitSplat's dome
+ * term for a surface with the plain lobes (splat_relight.slang:1379-1692,
+ * 1775-1824) -- albedo (1 - metallic) (irradiance / pi + sun) + the GGX
+ * environment BRDF times the prefiltered sky along the mirror direction.
+ * Per-Gaussian arrays are FP32 device memory; the struct is read at call
+ * time. The eye in the world is object_to_world applied to the view's eye.
+ */
+typedef struct athx_relight {
+    const float* normals;    /* [count][3] stored shading normals (cloud space), or NULL: the disc's short axis */
+    const float* albedo;     /* [count][3], or NULL: the evaluated SH colour is the albedo (splat_project.slang:819) */
+    const float* metallic;   /* [count] */
+    const float* roughness;  /* [count] */
+    float object_to_world[12]; /* rows of a 3x4 */
+    athx_sky sky;
+} athx_relight;
+
 typedef struct athx_gaussians {
     uint32_t count;
     uint32_t sh_coeffs; /* 1, 4, 9 or 16 */
@@ -123,6 +161,7 @@ typedef struct athx_gaussians {
     const float* opacity_logits; /* [count] */
     const float* sh;             /* [count][sh_coeffs][3] or NULL */
     const uint16_t* sh_half;     /* [count][sh_coeffs][3] or NULL */
+    const athx_relight* relight; /* NULL: the SH colour; else the relit colour (forward only) */
 } athx_gaussians;
 
 /*

@@ -87,6 +87,7 @@ typedef struct orc_view {
 #define FL_SRGB_R 256u   /* sRGB decode took the power branch */
 #define FL_SRGB_G 512u
 #define FL_SRGB_B 1024u
+#define FL_RELIT 2048u  /* the colour is the relit one (no SH adjoint through it) */
 
 static const float kSH0 = 0.28209479177387814f;
 static const float kC1 = 0.4886025119029199f;
@@ -208,11 +209,319 @@ static uint32_t degree_of(uint32_t coeffs) {
     return coeffs >= 16 ? 3u : coeffs >= 9 ? 2u : coeffs >= 4 ? 1u : 0u;
 }
 
+/* ---------------------------------------------------------------------------
+ * Relit colour, the non-transfer dome branch: relitSplat
+ * (shaders/athenea/splat/splat_relight.slang:1775-1824) with one dome whose
+ * sky was prepared (relitByDome :1640-1692), a surface with the plain lobes
+ * (no coat, sheen, transmission or emission), and the prepared-sky reader of
+ * shaders/athenea/light/environment.slang:50-225.
+ */
+typedef struct orc_relight {
+    const float* normals;   /* [n][3] stored shading normals (cloud space), or NULL: the disc's axis */
+    const float* albedo;    /* [n][3] or NULL: the evaluated harmonics are the albedo (splat_project.slang:819) */
+    const float* metallic;  /* [n] */
+    const float* roughness; /* [n] */
+    float o2w[12];          /* object -> world rows */
+    const uint32_t* texels; /* 2 words a texel: rgb halves */
+    const float* sky_sh;    /* 16 float4: radiance coefficients */
+    const float* sky_sun;   /* 2 float4: direction + disc size (w > 0: a sun), irradiance */
+    uint32_t base_side;
+} orc_relight;
+
+#define ENV_LEVELS 8u
+#define ENV_COARSE 256u
+#define ENV_FLOOR 16u
+
+static inline float lerpf_(float a, float b, float t) { return a + (b - a) * t; }
+static inline float satf_(float x) { return clampf_(x, 0.0f, 1.0f); }
+
+/* IEEE binary16 to binary32, exactly. */
+static float half_to_float(uint32_t h) {
+    const uint32_t sign = (h & 0x8000u) << 16, e = (h >> 10) & 0x1Fu, m = h & 0x3FFu;
+    uint32_t bits;
+    if (e == 0) {
+        if (m == 0) {
+            bits = sign;
+        } else {
+            int ee = -1;
+            uint32_t mm = m;
+            do { ++ee; mm <<= 1; } while (!(mm & 0x400u));
+            bits = sign | ((uint32_t)(127 - 15 - ee) << 23) | ((mm & 0x3FFu) << 13);
+        }
+    } else if (e == 31) {
+        bits = sign | 0x7F800000u | (m << 13);
+    } else {
+        bits = sign | ((e - 15 + 127) << 23) | (m << 13);
+    }
+    float f;
+    memcpy(&f, &bits, 4);
+    return f;
+}
+
+/* environment.slang:66-69 */
+static uint32_t env_side(uint32_t level, uint32_t base_side) {
+    const uint32_t base = base_side > ENV_COARSE ? base_side : ENV_COARSE;
+    const uint32_t side = base >> (level < ENV_LEVELS - 1 ? level : ENV_LEVELS - 1);
+    return side > ENV_FLOOR ? side : ENV_FLOOR;
+}
+
+/* environment.slang:77-92, the walk (the closed form's same sum) */
+static uint32_t env_level_offset(uint32_t level, uint32_t base_side) {
+    uint32_t at = 0;
+    for (uint32_t k = 0; k < (level < ENV_LEVELS ? level : ENV_LEVELS); ++k) {
+        const uint32_t side = env_side(k, base_side);
+        at += side * side;
+    }
+    return at;
+}
+
+/* common/octahedral.slang:12-23 */
+static void oct_encode(float nx, float ny, float nz, float* u, float* v) {
+    const float s = maxf_(fabsf(nx) + fabsf(ny) + fabsf(nz), 1.0e-20f);
+    nx = nx / s;
+    ny = ny / s;
+    nz = nz / s;
+    float px = nx, py = ny;
+    if (!(nz >= 0.0f)) {
+        px = (1.0f - fabsf(ny)) * (nx >= 0.0f ? 1.0f : -1.0f);
+        py = (1.0f - fabsf(nx)) * (ny >= 0.0f ? 1.0f : -1.0f);
+    }
+    *u = px * 0.5f + 0.5f;
+    *v = py * 0.5f + 0.5f;
+}
+
+static void env_words(const uint32_t* texels, uint32_t texel, float out[3]) {
+    const uint32_t lo = texels[texel * 2], hi = texels[texel * 2 + 1];
+    out[0] = half_to_float(lo & 0xFFFFu);
+    out[1] = half_to_float(lo >> 16);
+    out[2] = half_to_float(hi & 0xFFFFu);
+}
+
+/* environment.slang:133-147: bilinear over one level, clamped at the seam */
+static void env_level_at(const orc_relight* rl, uint32_t level, const float dir[3], float out[3]) {
+    const uint32_t side_u = env_side(level, rl->base_side);
+    const float side = (float)side_u;
+    float u, v;
+    oct_encode(dir[0], dir[1], dir[2], &u, &v);
+    const float atx = clampf_(u * side - 0.5f, 0.0f, side - 1.0f);
+    const float aty = clampf_(v * side - 0.5f, 0.0f, side - 1.0f);
+    const uint32_t lox = (uint32_t)floorf(atx), loy = (uint32_t)floorf(aty);
+    const uint32_t hix = lox + 1 < side_u - 1 ? lox + 1 : side_u - 1;
+    const uint32_t hiy = loy + 1 < side_u - 1 ? loy + 1 : side_u - 1;
+    const float fx = atx - (float)lox, fy = aty - (float)loy;
+    const uint32_t first = env_level_offset(level, rl->base_side);
+    float v00[3], v10[3], v01[3], v11[3];
+    env_words(rl->texels, first + loy * side_u + lox, v00);
+    env_words(rl->texels, first + loy * side_u + hix, v10);
+    env_words(rl->texels, first + hiy * side_u + lox, v01);
+    env_words(rl->texels, first + hiy * side_u + hix, v11);
+    for (int c = 0; c < 3; ++c) {
+        out[c] = lerpf_(lerpf_(v00[c], v10[c], fx), lerpf_(v01[c], v11[c], fx), fy);
+    }
+}
+
+/* environment.slang:162-170: the two levels that bracket sqrt(roughness) */
+static void env_radiance(const orc_relight* rl, const float dir[3], float roughness, float out[3]) {
+    const float level = sqrtf(satf_(roughness)) * (float)(ENV_LEVELS - 1);
+    const uint32_t lo = (uint32_t)floorf(level);
+    const uint32_t hi = lo + 1 < ENV_LEVELS - 1 ? lo + 1 : ENV_LEVELS - 1;
+    const float f = level - (float)lo;
+    float a[3], b[3];
+    env_level_at(rl, lo, dir, a);
+    env_level_at(rl, hi, dir, b);
+    for (int c = 0; c < 3; ++c) out[c] = lerpf_(a[c], b[c], f);
+}
+
+/* environment.slang:182-223: irradiance at n from nine radiance coefficients */
+static float env_basis(uint32_t k, float x, float y, float z) {
+    switch (k) {
+    case 0: return 0.28209479177387814f;
+    case 1: return -0.4886025119029199f * y;
+    case 2: return 0.4886025119029199f * z;
+    case 3: return -0.4886025119029199f * x;
+    case 4: return 1.0925484305920792f * x * y;
+    case 5: return -1.0925484305920792f * y * z;
+    case 6: return 0.31539156525252005f * (2.0f * z * z - x * x - y * y);
+    case 7: return -1.0925484305920792f * x * z;
+    default: return 0.5462742152960396f * (x * x - y * y);
+    }
+}
+
+static void env_irradiance(const orc_relight* rl, const float n[3], float out[3]) {
+    out[0] = out[1] = out[2] = 0.0f;
+    for (uint32_t k = 0; k < 9; ++k) {
+        const float band = k == 0 ? 3.14159265358979f : (k < 4 ? 2.0943951023932f : 0.78539816339745f);
+        const float w = band * env_basis(k, n[0], n[1], n[2]);
+        for (int c = 0; c < 3; ++c) out[c] = out[c] + rl->sky_sh[k * 4 + c] * w;
+    }
+}
+
+/* material/microfacet.slang:66-82 (MaterialX's fit); returns (a, b) */
+static void ggx_albedo_ab(float x, float y, float* ra, float* rb) {
+    const float x2 = x * x, y2 = y * y;
+    static const float c[9][4] = {{0.1003f, 0.9345f, 1.0f, 1.0f},     {-0.6303f, -2.323f, -1.765f, 0.2281f},
+                                  {9.748f, 2.229f, 8.263f, 15.94f},    {-2.038f, -3.748f, 11.53f, -55.83f},
+                                  {29.34f, 1.424f, 28.96f, 13.08f},    {-8.245f, -0.7684f, -7.507f, 41.26f},
+                                  {-26.44f, 1.436f, -36.11f, 54.9f},   {19.99f, 0.2913f, 15.86f, 300.2f},
+                                  {-5.448f, 0.6286f, 33.37f, -285.1f}};
+    float r[4];
+    for (int i = 0; i < 4; ++i) {
+        r[i] = c[0][i] + c[1][i] * x + c[2][i] * y + c[3][i] * x * y + c[4][i] * x2 + c[5][i] * y2 +
+               c[6][i] * x2 * y + c[7][i] * x * y2 + c[8][i] * x2 * y2;
+    }
+    *ra = satf_(r[0] / r[2]);
+    *rb = satf_(r[1] / r[3]);
+}
+
+/* microfacet.slang:85-88 (Turquin 2019) */
+static float ggx_compensation(float nv, float alpha, float fss) {
+    float a, b;
+    ggx_albedo_ab(nv, alpha, &a, &b);
+    const float ess = a * 1.0f + 1.0f * b;
+    return 1.0f + fss * (1.0f - ess) / ess;
+}
+
+/* microfacet.slang:99-107 */
+static float fresnel_dielectric(float c, float ior) {
+    const float g2 = ior * ior + c * c - 1.0f;
+    if (g2 < 0.0f) return 1.0f;
+    const float g = sqrtf(g2);
+    const float q = (g - c) / (g + c);
+    const float t = ((g + c) * c - 1.0f) / ((g - c) * c + 1.0f);
+    return 0.5f * (q * q) * (1.0f + t * t);
+}
+
+/* splat_relight.slang:455-460 at the plain lobes' index 1.5 (dielectricF0 = 0.04, tint 1) */
+static float ggx_env_dielectric(float roughness, float nv) {
+    const float alpha = maxf_(roughness * roughness, 1.0e-4f);
+    const float F = fresnel_dielectric(nv, 1.5f);
+    float a, b;
+    ggx_albedo_ab(nv, alpha, &a, &b);
+    return (0.04f * a + 1.0f * b) * ggx_compensation(nv, alpha, F);
+}
+
+/* splat_relight.slang:472-488 with a white edge: mxArtisticIor (microfacet.slang:117-126),
+ * fresnelConductor (:128-143), the fit and its compensation, one channel */
+static float ggx_env_conductor(float albedo, float roughness, float nv) {
+    const float alpha = maxf_(roughness * roughness, 1.0e-4f);
+    const float r = clampf_(satf_(albedo), 0.0f, 0.99f);
+    const float root = sqrtf(r);
+    const float low = (1.0f - r) / (1.0f + r);
+    const float high = (1.0f + root) / (1.0f - root);
+    const float n = lerpf_(high, low, 1.0f);
+    const float up = n + 1.0f, down = n - 1.0f;
+    const float k = sqrtf(maxf_((up * up * r - down * down) / maxf_(1.0f - r, 1.0e-6f), 0.0f));
+    const float ct = satf_(nv);
+    const float ct2 = ct * ct, st2 = 1.0f - ct2;
+    const float n2 = n * n, k2 = k * k;
+    const float t0 = n2 - k2 - st2;
+    const float a2b2 = sqrtf(t0 * t0 + 4.0f * n2 * k2);
+    const float t1 = a2b2 + ct2;
+    const float aa = sqrtf(maxf_(0.5f * (a2b2 + t0), 0.0f));
+    const float t2 = 2.0f * aa * nv;
+    const float rs = (t1 - t2) / (t1 + t2);
+    const float t3 = ct2 * a2b2 + st2 * st2;
+    const float t4 = t2 * st2;
+    const float rp = rs * (t3 - t4) / (t3 + t4);
+    const float F = 0.5f * (rp + rs);
+    float a, b;
+    ggx_albedo_ab(nv, alpha, &a, &b);
+    return (F * a + F * b) * ggx_compensation(nv, alpha, F);
+}
+
+/* splat_relight.slang:1860-1880 (splatShadingNormal with splatNormal :1843-1850) */
+static void shading_normal(const float R[3][3], float sx, float sy, float sz, const float p[3], const float eye[3],
+                           const float* stored, float out[3]) {
+    const uint32_t s = sx <= sy ? (sx <= sz ? 0u : 2u) : (sy <= sz ? 1u : 2u);
+    float d[3] = {R[0][s], R[1][s], R[2][s]};
+    const float dl = 1.0f / sqrtf(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    d[0] = d[0] * dl; d[1] = d[1] * dl; d[2] = d[2] * dl;
+    float e[3] = {eye[0] - p[0], eye[1] - p[1], eye[2] - p[2]};
+    const float el = 1.0f / sqrtf(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]);
+    const float toward = d[0] * (e[0] * el) + d[1] * (e[1] * el) + d[2] * (e[2] * el);
+    if (toward < 0.0f) {
+        d[0] = -d[0]; d[1] = -d[1]; d[2] = -d[2];
+    }
+    if (!stored) {
+        out[0] = d[0]; out[1] = d[1]; out[2] = d[2];
+        return;
+    }
+    const float along = stored[0] * d[0] + stored[1] * d[1] + stored[2] * d[2];
+    float flip;
+    if (fabsf(along) >= 0.5f) {
+        flip = along < 0.0f ? -1.0f : 1.0f;
+    } else {
+        flip = stored[0] * e[0] + stored[1] * e[1] + stored[2] * e[2] < 0.0f ? -1.0f : 1.0f;
+    }
+    out[0] = flip * stored[0]; out[1] = flip * stored[1]; out[2] = flip * stored[2];
+}
+
+/* The relit colour of slot i: relitSplat's dome term, body * irradiance / pi
+ * (+ the sun) + the environment BRDF * the prepared sky along the mirror. */
+static void relit_colour(const orc_relight* rl, uint32_t i, const float R[3][3], float sx, float sy, float sz,
+                         const float p[3], const float eye_obj[3], const float albedo_sh[3], float out[3]) {
+    float albedo[3] = {albedo_sh[0], albedo_sh[1], albedo_sh[2]};
+    if (rl->albedo) {
+        albedo[0] = rl->albedo[i * 3 + 0]; albedo[1] = rl->albedo[i * 3 + 1]; albedo[2] = rl->albedo[i * 3 + 2];
+    }
+    const float metallic = rl->metallic[i], roughness = rl->roughness[i];
+    float nc[3];
+    shading_normal(R, sx, sy, sz, p, eye_obj, rl->normals ? rl->normals + (size_t)i * 3 : NULL, nc);
+    /* To the world (relightToWorld, relightNormalToWorld :1882-1915). */
+    const float* w = rl->o2w;
+    const float pw[3] = {w[0] * p[0] + w[1] * p[1] + w[2] * p[2] + w[3] * 1.0f,
+                         w[4] * p[0] + w[5] * p[1] + w[6] * p[2] + w[7] * 1.0f,
+                         w[8] * p[0] + w[9] * p[1] + w[10] * p[2] + w[11] * 1.0f};
+    const float a[3] = {w[0], w[4], w[8]}, b[3] = {w[1], w[5], w[9]}, c[3] = {w[2], w[6], w[10]};
+    const float bc[3] = {b[1] * c[2] - b[2] * c[1], b[2] * c[0] - b[0] * c[2], b[0] * c[1] - b[1] * c[0]};
+    const float ca[3] = {c[1] * a[2] - c[2] * a[1], c[2] * a[0] - c[0] * a[2], c[0] * a[1] - c[1] * a[0]};
+    const float ab[3] = {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
+    const float side = a[0] * bc[0] + a[1] * bc[1] + a[2] * bc[2] < 0.0f ? -1.0f : 1.0f;
+    float n[3];
+    for (int k = 0; k < 3; ++k) n[k] = side * (nc[0] * bc[k] + nc[1] * ca[k] + nc[2] * ab[k]);
+    const float nn = n[0] * n[0] + n[1] * n[1] + n[2] * n[2];
+    if (nn > 1.0e-20f) {
+        const float inv = 1.0f / sqrtf(nn);
+        n[0] = n[0] * inv; n[1] = n[1] * inv; n[2] = n[2] * inv;
+    } else {
+        n[0] = 0.0f; n[1] = 0.0f; n[2] = 1.0f;
+    }
+    /* The eye in the world: the object-space eye taken through the same rows. */
+    const float* e = eye_obj;
+    const float ew[3] = {w[0] * e[0] + w[1] * e[1] + w[2] * e[2] + w[3] * 1.0f,
+                         w[4] * e[0] + w[5] * e[1] + w[6] * e[2] + w[7] * 1.0f,
+                         w[8] * e[0] + w[9] * e[1] + w[10] * e[2] + w[11] * 1.0f};
+    float wo[3] = {ew[0] - pw[0], ew[1] - pw[1], ew[2] - pw[2]};
+    const float wl = 1.0f / sqrtf(wo[0] * wo[0] + wo[1] * wo[1] + wo[2] * wo[2]);
+    wo[0] = wo[0] * wl; wo[1] = wo[1] * wl; wo[2] = wo[2] * wl;
+    /* relitByDome, the prepared dome (:1379-1384, 1640-1692) */
+    const float ndv = n[0] * wo[0] + n[1] * wo[1] + n[2] * wo[2];
+    const float nv = maxf_(ndv, 1.0e-4f);
+    const float i_dot = n[0] * -wo[0] + n[1] * -wo[1] + n[2] * -wo[2];
+    const float mirror[3] = {-wo[0] - 2.0f * i_dot * n[0], -wo[1] - 2.0f * i_dot * n[1], -wo[2] - 2.0f * i_dot * n[2]};
+    const float over_pi = 1.0f / 3.14159265358979f;
+    float above[3];
+    env_irradiance(rl, n, above);
+    for (int k = 0; k < 3; ++k) above[k] = above[k] * over_pi;
+    if (rl->sky_sun[3] > 0.0f) {
+        const float cosine = n[0] * rl->sky_sun[0] + n[1] * rl->sky_sun[1] + n[2] * rl->sky_sun[2];
+        for (int k = 0; k < 3; ++k) above[k] = above[k] + rl->sky_sun[4 + k] * (maxf_(cosine, 0.0f) * over_pi);
+    }
+    float reflected[3];
+    env_radiance(rl, mirror, roughness, reflected);
+    const float diel = ggx_env_dielectric(roughness, nv);
+    for (int k = 0; k < 3; ++k) {
+        const float body = albedo[k] * (1.0f - metallic);
+        const float polish = lerpf_(diel, ggx_env_conductor(albedo[k], roughness, nv), satf_(metallic));
+        out[k] = body * above[k] + polish * reflected[k];
+    }
+}
+
 /* One splat projected: splat_project.slang:497-1130 with kTransfer 0. Writes
  * rec, touched and key for slot i; returns the cull reason (0 = visible). */
 static uint32_t project_one(const orc_view* v, const float* P, const float* LS, const float* Q, const float* LG,
                             const float* SH, uint32_t coeffs, uint32_t i, float* rec, uint32_t* touched,
-                            uint32_t* key, uint32_t* flags_out) {
+                            uint32_t* key, uint32_t* flags_out, const orc_relight* rl) {
     uint32_t flags = 0;
     const float px = P[i * 3 + 0], py = P[i * 3 + 1], pz = P[i * 3 + 2];
     const float sx = athx_expf(LS[i * 3 + 0]);
@@ -345,6 +654,16 @@ static uint32_t project_one(const orc_view* v, const float* P, const float* LS, 
             if (took) flags |= (FL_SRGB_R << ch);
         }
     }
+    if (rl) {
+        /* The relit branch (splat_project.slang:799-976): the evaluated
+         * colour is the surface's albedo unless one is given. */
+        const float R[3][3] = {{r00, r01, r02}, {r10, r11, r12}, {r20, r21, r22}};
+        const float p[3] = {px, py, pz};
+        float lit[3];
+        relit_colour(rl, i, R, sx, sy, sz, p, v->eye, col, lit);
+        col[0] = lit[0]; col[1] = lit[1]; col[2] = lit[2];
+        flags |= FL_RELIT;
+    }
     float* r = rec + (size_t)i * REC;
     r[0] = cx; r[1] = cy; r[2] = tz; r[3] = cutoff;
     r[4] = ca; r[5] = cb; r[6] = cc; r[7] = alpha;
@@ -360,15 +679,90 @@ static uint32_t project_one(const orc_view* v, const float* P, const float* LS, 
     return 0;
 }
 
-/* The whole cloud projected into slots [0, n). flags may be NULL. */
-ORC_EXPORT void orc_project(const orc_view* v, const float* means, const float* log_scales, const float* quats,
-                            const float* logits, const float* sh, uint32_t coeffs, uint32_t n, float* rec,
-                            uint32_t* touched, uint32_t* keys, uint32_t* flags) {
+/* The whole cloud projected into slots [0, n). flags may be NULL; rl NULL:
+ * the harmonics' colour, otherwise the relit one. */
+ORC_EXPORT void orc_project_lit(const orc_view* v, const float* means, const float* log_scales, const float* quats,
+                                const float* logits, const float* sh, uint32_t coeffs, uint32_t n, float* rec,
+                                uint32_t* touched, uint32_t* keys, uint32_t* flags, const orc_relight* rl) {
 #pragma omp parallel for schedule(dynamic, 4096)
     for (int64_t i = 0; i < (int64_t)n; ++i) {
         project_one(v, means, log_scales, quats, logits, sh, coeffs, (uint32_t)i, rec, touched + i, keys + i,
-                    flags ? flags + i : NULL);
+                    flags ? flags + i : NULL, rl);
     }
+}
+
+ORC_EXPORT void orc_project(const orc_view* v, const float* means, const float* log_scales, const float* quats,
+                            const float* logits, const float* sh, uint32_t coeffs, uint32_t n, float* rec,
+                            uint32_t* touched, uint32_t* keys, uint32_t* flags) {
+    orc_project_lit(v, means, log_scales, quats, logits, sh, coeffs, n, rec, touched, keys, flags, NULL);
+}
+
+/* The relit colour of one surface point alone, for the tests: n the
+ * world-space shading normal, wo the unit direction to the eye. */
+ORC_EXPORT void orc_relit_point(const orc_relight* rl, const float n[3], const float wo[3], const float albedo[3],
+                                float metallic, float roughness, float out[3]) {
+    const float ndv = n[0] * wo[0] + n[1] * wo[1] + n[2] * wo[2];
+    const float nv = maxf_(ndv, 1.0e-4f);
+    const float i_dot = n[0] * -wo[0] + n[1] * -wo[1] + n[2] * -wo[2];
+    const float mirror[3] = {-wo[0] - 2.0f * i_dot * n[0], -wo[1] - 2.0f * i_dot * n[1], -wo[2] - 2.0f * i_dot * n[2]};
+    const float over_pi = 1.0f / 3.14159265358979f;
+    float above[3];
+    env_irradiance(rl, n, above);
+    for (int k = 0; k < 3; ++k) above[k] = above[k] * over_pi;
+    if (rl->sky_sun[3] > 0.0f) {
+        const float cosine = n[0] * rl->sky_sun[0] + n[1] * rl->sky_sun[1] + n[2] * rl->sky_sun[2];
+        for (int k = 0; k < 3; ++k) above[k] = above[k] + rl->sky_sun[4 + k] * (maxf_(cosine, 0.0f) * over_pi);
+    }
+    float reflected[3];
+    env_radiance(rl, mirror, roughness, reflected);
+    const float diel = ggx_env_dielectric(roughness, nv);
+    for (int k = 0; k < 3; ++k) {
+        const float body = albedo[k] * (1.0f - metallic);
+        const float polish = lerpf_(diel, ggx_env_conductor(albedo[k], roughness, nv), satf_(metallic));
+        out[k] = body * above[k] + polish * reflected[k];
+    }
+}
+
+/* Pieces of the prepared-sky reader and the BRDF fit, for the tests. */
+ORC_EXPORT void orc_env_radiance(const orc_relight* rl, const float dir[3], float roughness, float out[3]) {
+    env_radiance(rl, dir, roughness, out);
+}
+ORC_EXPORT void orc_env_irradiance(const orc_relight* rl, const float n[3], float out[3]) {
+    env_irradiance(rl, n, out);
+}
+ORC_EXPORT uint32_t orc_env_level_offset(uint32_t level, uint32_t base_side) {
+    return env_level_offset(level, base_side);
+}
+ORC_EXPORT float orc_half_to_float(uint32_t h) { return half_to_float(h); }
+ORC_EXPORT void orc_ggx_albedo(float nv, float alpha, float* a, float* b) { ggx_albedo_ab(nv, alpha, a, b); }
+ORC_EXPORT float orc_ggx_dielectric(float roughness, float nv) { return ggx_env_dielectric(roughness, nv); }
+ORC_EXPORT float orc_ggx_conductor(float albedo, float roughness, float nv) {
+    return ggx_env_conductor(albedo, roughness, nv);
+}
+ORC_EXPORT float orc_fresnel_dielectric(float c, float ior) { return fresnel_dielectric(c, ior); }
+/* The conductor's Fresnel alone: mxArtisticIor with a white edge, then
+ * fresnelConductor (microfacet.slang:117-143), as ggx_env_conductor takes it. */
+ORC_EXPORT float orc_fresnel_conductor(float albedo, float nv) {
+    const float r = clampf_(satf_(albedo), 0.0f, 0.99f);
+    const float root = sqrtf(r);
+    const float low = (1.0f - r) / (1.0f + r);
+    const float high = (1.0f + root) / (1.0f - root);
+    const float n = lerpf_(high, low, 1.0f);
+    const float up = n + 1.0f, down = n - 1.0f;
+    const float k = sqrtf(maxf_((up * up * r - down * down) / maxf_(1.0f - r, 1.0e-6f), 0.0f));
+    const float ct = satf_(nv);
+    const float ct2 = ct * ct, st2 = 1.0f - ct2;
+    const float n2 = n * n, k2 = k * k;
+    const float t0 = n2 - k2 - st2;
+    const float a2b2 = sqrtf(t0 * t0 + 4.0f * n2 * k2);
+    const float t1 = a2b2 + ct2;
+    const float aa = sqrtf(maxf_(0.5f * (a2b2 + t0), 0.0f));
+    const float t2 = 2.0f * aa * nv;
+    const float rs = (t1 - t2) / (t1 + t2);
+    const float t3 = ct2 * a2b2 + st2 * st2;
+    const float t4 = t2 * st2;
+    const float rp = rs * (t3 - t4) / (t3 + t4);
+    return 0.5f * (rp + rs);
 }
 
 /* Exclusive scan; returns the total (PrefixSum.cpp:58-88 semantics). */
@@ -628,7 +1022,14 @@ ORC_EXPORT void orc_blend_f64(const orc_view* v, const float* rec, const uint32_
     rec64_t* r64 = (rec64_t*)calloc(n ? n : 1, sizeof(rec64_t));
 #pragma omp parallel for schedule(dynamic, 4096)
     for (int64_t i = 0; i < (int64_t)n; ++i) {
-        if (touched[i]) project_one_f64(v, means, log_scales, quats, logits, sh, coeffs, (uint32_t)i, flags[i], &r64[i]);
+        if (touched[i]) {
+            project_one_f64(v, means, log_scales, quats, logits, sh, coeffs, (uint32_t)i, flags[i], &r64[i]);
+            /* A relit colour is the blend's input as the FP32 projection made
+             * it: the FP64 replay covers the geometry and the blend. */
+            if (flags[i] & FL_RELIT) {
+                for (int ch = 0; ch < 3; ++ch) r64[i].col[ch] = (double)rec[(size_t)i * REC + 8 + ch];
+            }
+        }
     }
     const uint32_t W = v->width, H = v->height, TS = v->tile;
     const uint32_t tiles_x = (W + TS - 1) / TS;
@@ -786,12 +1187,13 @@ ORC_EXPORT uint64_t orc_render_ref_order(const orc_view* v, const float* means, 
                                          const float* quats, const float* logits, const float* sh, uint32_t coeffs,
                                          uint32_t n, float* rec, uint32_t* touched, uint32_t* keys,
                                          uint32_t* pair_vals_out, uint64_t pair_cap, uint32_t* ranges, float* colour,
-                                         float* depth, float* final_t, uint32_t* last, double* ms) {
+                                         float* depth, float* final_t, uint32_t* last, double* ms,
+                                         const orc_relight* rl) {
     const uint32_t TS = v->tile;
     const uint32_t tiles_x = (v->width + TS - 1) / TS, tiles_y = (v->height + TS - 1) / TS;
     const uint32_t tiles = tiles_x * tiles_y;
     double t0 = now_ms();
-    orc_project(v, means, log_scales, quats, logits, sh, coeffs, n, rec, touched, keys, NULL);
+    orc_project_lit(v, means, log_scales, quats, logits, sh, coeffs, n, rec, touched, keys, NULL, rl);
     double t1 = now_ms();
     /* compact (splat_compact.slang:16-26) + depth sort, 24 bits */
     uint64_t V = 0;

@@ -50,6 +50,49 @@ class OrcView(ctypes.Structure):
     ]
 
 
+class OrcRelight(ctypes.Structure):
+    _fields_ = [
+        ("normals", ctypes.c_void_p),
+        ("albedo", ctypes.c_void_p),
+        ("metallic", ctypes.c_void_p),
+        ("roughness", ctypes.c_void_p),
+        ("o2w", ctypes.c_float * 12),
+        ("texels", ctypes.c_void_p),
+        ("sky_sh", ctypes.c_void_p),
+        ("sky_sun", ctypes.c_void_p),
+        ("base_side", ctypes.c_uint32),
+    ]
+
+
+def orc_relight(relight) -> tuple[OrcRelight, list]:
+    """The oracle's relight inputs from a numpy-side description with
+    attributes normals (n, 3) or None, albedo (n, 3) or None, metallic (n,),
+    roughness (n,), object_to_world (12,), and sky with texels (uint32),
+    sh (16, 4), sun (2, 4), base_side. Returns the struct and the arrays it
+    points into (keep them alive while it is used)."""
+    keep = []
+
+    def ptr(a, dtype):
+        if a is None:
+            return None
+        a = np.ascontiguousarray(a, dtype=dtype)
+        keep.append(a)
+        return a.ctypes.data
+
+    r = OrcRelight()
+    r.normals = ptr(relight.normals, np.float32)
+    r.albedo = ptr(relight.albedo, np.float32)
+    r.metallic = ptr(relight.metallic, np.float32)
+    r.roughness = ptr(relight.roughness, np.float32)
+    for k in range(12):
+        r.o2w[k] = float(relight.object_to_world[k])
+    r.texels = ptr(relight.sky.texels, np.uint32)
+    r.sky_sh = ptr(relight.sky.sh, np.float32)
+    r.sky_sun = ptr(relight.sky.sun, np.float32)
+    r.base_side = int(relight.sky.base_side)
+    return r, keep
+
+
 _lib = None
 
 
@@ -68,6 +111,32 @@ def load():
             raise RuntimeError(f"{LIB_PATH} is older than its sources: rebuild it (make -C oracle)")
         lib = ctypes.CDLL(str(LIB_PATH))
         vp = ctypes.POINTER(OrcView)
+        rp = ctypes.POINTER(OrcRelight)
+        f3 = ctypes.c_float * 3
+        lib.orc_project_lit.argtypes = [vp, _f32p, _f32p, _f32p, _f32p, _f32p, ctypes.c_uint32, ctypes.c_uint32,
+                                        _f32p, _u32p, _u32p, ctypes.c_void_p, rp]
+        lib.orc_project_lit.restype = None
+        lib.orc_relit_point.argtypes = [rp, f3, f3, f3, ctypes.c_float, ctypes.c_float, f3]
+        lib.orc_relit_point.restype = None
+        lib.orc_env_radiance.argtypes = [rp, f3, ctypes.c_float, f3]
+        lib.orc_env_radiance.restype = None
+        lib.orc_env_irradiance.argtypes = [rp, f3, f3]
+        lib.orc_env_irradiance.restype = None
+        lib.orc_env_level_offset.argtypes = [ctypes.c_uint32, ctypes.c_uint32]
+        lib.orc_env_level_offset.restype = ctypes.c_uint32
+        lib.orc_half_to_float.argtypes = [ctypes.c_uint32]
+        lib.orc_half_to_float.restype = ctypes.c_float
+        lib.orc_ggx_albedo.argtypes = [ctypes.c_float, ctypes.c_float, ctypes.POINTER(ctypes.c_float),
+                                       ctypes.POINTER(ctypes.c_float)]
+        lib.orc_ggx_albedo.restype = None
+        lib.orc_ggx_dielectric.argtypes = [ctypes.c_float, ctypes.c_float]
+        lib.orc_ggx_dielectric.restype = ctypes.c_float
+        lib.orc_ggx_conductor.argtypes = [ctypes.c_float, ctypes.c_float, ctypes.c_float]
+        lib.orc_ggx_conductor.restype = ctypes.c_float
+        lib.orc_fresnel_dielectric.argtypes = [ctypes.c_float, ctypes.c_float]
+        lib.orc_fresnel_dielectric.restype = ctypes.c_float
+        lib.orc_fresnel_conductor.argtypes = [ctypes.c_float, ctypes.c_float]
+        lib.orc_fresnel_conductor.restype = ctypes.c_float
         lib.orc_project.argtypes = [vp, _f32p, _f32p, _f32p, _f32p, _f32p, ctypes.c_uint32, ctypes.c_uint32, _f32p,
                                     _u32p, _u32p, ctypes.c_void_p]
         lib.orc_project.restype = None
@@ -89,7 +158,7 @@ def load():
         lib.orc_reference.restype = ctypes.c_uint64
         lib.orc_render_ref_order.argtypes = [vp, _f32p, _f32p, _f32p, _f32p, _f32p, ctypes.c_uint32, ctypes.c_uint32,
                                              _f32p, _u32p, _u32p, _u32p, ctypes.c_uint64, _u32p, ctypes.c_void_p,
-                                             ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+                                             ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, rp]
         lib.orc_render_ref_order.restype = ctypes.c_uint64
         lib.orc_threads.restype = ctypes.c_int
         lib.orc_set_threads.argtypes = [ctypes.c_int]
@@ -140,7 +209,9 @@ def bits_for(values: int) -> int:
     return bits
 
 
-def project(v: OrcView, cloud, flags: bool = False):
+def project(v: OrcView, cloud, flags: bool = False, relight=None):
+    """Records, tile counts and depth keys of every Gaussian; ``relight``
+    (see orc_relight) swaps the harmonics' colour for the relit one."""
     lib = load()
     n = cloud.means.shape[0]
     coeffs = cloud.sh.shape[1]
@@ -148,9 +219,12 @@ def project(v: OrcView, cloud, flags: bool = False):
     touched = np.zeros(n, np.uint32)
     keys = np.zeros(n, np.uint32)
     fl = np.zeros(n, np.uint32) if flags else None
-    lib.orc_project(ctypes.byref(v), _c(cloud.means, np.float32), _c(cloud.log_scales, np.float32),
-                    _c(cloud.quats, np.float32), _c(cloud.opacity_logits, np.float32), _c(cloud.sh, np.float32),
-                    coeffs, n, rec, touched, keys, fl.ctypes.data if flags else None)
+    rl, keep = orc_relight(relight) if relight is not None else (None, None)
+    lib.orc_project_lit(ctypes.byref(v), _c(cloud.means, np.float32), _c(cloud.log_scales, np.float32),
+                        _c(cloud.quats, np.float32), _c(cloud.opacity_logits, np.float32), _c(cloud.sh, np.float32),
+                        coeffs, n, rec, touched, keys, fl.ctypes.data if flags else None,
+                        ctypes.byref(rl) if rl is not None else None)
+    del keep
     return (rec, touched, keys, fl) if flags else (rec, touched, keys)
 
 
@@ -168,7 +242,7 @@ def sort_pairs(keys: np.ndarray, values: np.ndarray, bits: int) -> tuple[np.ndar
     return k, v
 
 
-def render(views: list[OrcView], cloud, with_f64: bool = False) -> dict:
+def render(views: list[OrcView], cloud, with_f64: bool = False, relight=None) -> dict:
     """The frame as the HIP path computes it, all views in one key space."""
     lib = load()
     n = cloud.means.shape[0]
@@ -180,7 +254,7 @@ def render(views: list[OrcView], cloud, with_f64: bool = False) -> dict:
     tile_bases = []
     total_tiles = 0
     for k, v in enumerate(views):
-        r, t, kk, fl = project(v, cloud, flags=True)
+        r, t, kk, fl = project(v, cloud, flags=True, relight=relight)
         rec[k * n:(k + 1) * n] = r
         touched[k * n:(k + 1) * n] = t
         keys[k * n:(k + 1) * n] = kk
@@ -254,10 +328,11 @@ def render_f64(views: list[OrcView], cloud, frame: dict) -> dict:
     return {"colour64": cols, "depth64": deps}
 
 
-def render_ref_order(v: OrcView, cloud, pair_cap: int | None = None) -> dict:
+def render_ref_order(v: OrcView, cloud, pair_cap: int | None = None, relight=None) -> dict:
     """The reference's two-sort order on one view: returns pair values in
     sorted order, ranges, P and the stage times (ms)."""
     lib = load()
+    rl, keep = orc_relight(relight) if relight is not None else (None, None)
     n = cloud.means.shape[0]
     coeffs = cloud.sh.shape[1]
     rec = np.zeros((n, 12), np.float32)
@@ -266,7 +341,7 @@ def render_ref_order(v: OrcView, cloud, pair_cap: int | None = None) -> dict:
     tx, ty = tiles_of(v)
     ranges = np.zeros(tx * ty * 2, np.uint32)
     if pair_cap is None:
-        r, t, _ = project(v, cloud)
+        r, t, _ = project(v, cloud, relight=relight)
         pair_cap = int(t.astype(np.uint64).sum())
     pv = np.zeros(max(pair_cap, 1), np.uint32)
     H, W = v.height, v.width
@@ -278,7 +353,9 @@ def render_ref_order(v: OrcView, cloud, pair_cap: int | None = None) -> dict:
     P = lib.orc_render_ref_order(ctypes.byref(v), _c(cloud.means, np.float32), _c(cloud.log_scales, np.float32),
                                  _c(cloud.quats, np.float32), _c(cloud.opacity_logits, np.float32),
                                  _c(cloud.sh, np.float32), coeffs, n, rec, touched, keys, pv, pair_cap, ranges,
-                                 col.ctypes.data, dep.ctypes.data, ft.ctypes.data, last.ctypes.data, ms.ctypes.data)
+                                 col.ctypes.data, dep.ctypes.data, ft.ctypes.data, last.ctypes.data, ms.ctypes.data,
+                                 ctypes.byref(rl) if rl is not None else None)
+    del keep
     return {"pairs": int(P), "pair_values": pv[:P], "ranges": ranges, "colour": col.reshape(H, W, 4),
             "depth": dep.reshape(H, W), "ms": ms, "records": rec, "touched": touched}
 
@@ -337,3 +414,60 @@ def fd_grads(v: OrcView, cloud, frame: dict, upstream: np.ndarray, ids, h_rel: f
                      _c(cloud.opacity_logits, np.float64), _c(cloud.sh, np.float64), coeffs, n,
                      _c(upstream, np.float64), ids, ids.size, h_rel, out, mag)
     return out, mag
+
+
+def _f3(x):
+    return (ctypes.c_float * 3)(*[float(t) for t in x])
+
+
+def relit_point(relight, n, wo, albedo, metallic: float, roughness: float) -> np.ndarray:
+    """The relit colour of one surface point: n the world shading normal, wo
+    the unit direction to the eye."""
+    rl, keep = orc_relight(relight)
+    out = _f3((0, 0, 0))
+    load().orc_relit_point(ctypes.byref(rl), _f3(n), _f3(wo), _f3(albedo), metallic, roughness, out)
+    return np.array(out[:], np.float32)
+
+
+def env_radiance(relight, direction, roughness: float) -> np.ndarray:
+    rl, keep = orc_relight(relight)
+    out = _f3((0, 0, 0))
+    load().orc_env_radiance(ctypes.byref(rl), _f3(direction), roughness, out)
+    return np.array(out[:], np.float32)
+
+
+def env_irradiance(relight, normal) -> np.ndarray:
+    rl, keep = orc_relight(relight)
+    out = _f3((0, 0, 0))
+    load().orc_env_irradiance(ctypes.byref(rl), _f3(normal), out)
+    return np.array(out[:], np.float32)
+
+
+def env_level_offset(level: int, base_side: int) -> int:
+    return int(load().orc_env_level_offset(level, base_side))
+
+
+def half_to_float(h: int) -> float:
+    return float(load().orc_half_to_float(h))
+
+
+def ggx_albedo(nv: float, alpha: float) -> tuple[float, float]:
+    a, b = ctypes.c_float(), ctypes.c_float()
+    load().orc_ggx_albedo(nv, alpha, ctypes.byref(a), ctypes.byref(b))
+    return a.value, b.value
+
+
+def ggx_dielectric(roughness: float, nv: float) -> float:
+    return float(load().orc_ggx_dielectric(roughness, nv))
+
+
+def ggx_conductor(albedo: float, roughness: float, nv: float) -> float:
+    return float(load().orc_ggx_conductor(albedo, roughness, nv))
+
+
+def fresnel_dielectric(c: float, ior: float) -> float:
+    return float(load().orc_fresnel_dielectric(c, ior))
+
+
+def fresnel_conductor(albedo: float, nv: float) -> float:
+    return float(load().orc_fresnel_conductor(albedo, nv))

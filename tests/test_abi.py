@@ -48,6 +48,8 @@ def test_library_exports_nothing_else():
 STRUCTS = {
     "athx_view": _lib.View,
     "athx_gaussians": _lib.Gaussians,
+    "athx_sky": _lib.Sky,
+    "athx_relight": _lib.Relight,
     "athx_settings": _lib.Settings,
     "athx_targets": _lib.Targets,
     "athx_frame_stats": _lib.FrameStats,
