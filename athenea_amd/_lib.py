@@ -7,10 +7,12 @@ or ``__graft_entry__.build()``); there is no fallback path.
 from __future__ import annotations
 
 import ctypes
+import os
 import threading
 from pathlib import Path
 
-LIB_PATH = Path(__file__).resolve().parent / "libathx.so"
+# ATHX_LIBRARY names another build of the same library (to compare builds).
+LIB_PATH = Path(os.environ.get("ATHX_LIBRARY") or Path(__file__).resolve().parent / "libathx.so")
 
 # athx_status: athenea's ErrorCode (modules/core/include/athenea/core/Error.h:14-26), 0 = OK.
 STATUS_NAMES = {
@@ -67,8 +69,19 @@ class Relight(ctypes.Structure):
         ("albedo", ctypes.c_void_p),
         ("metallic", ctypes.c_void_p),
         ("roughness", ctypes.c_void_p),
+        ("normals_packed", ctypes.c_void_p),
+        ("pbr", ctypes.c_void_p),
         ("object_to_world", ctypes.c_float * 12),
         ("sky", Sky),
+    ]
+
+
+class PackedSplats(ctypes.Structure):
+    _fields_ = [
+        ("positions", ctypes.c_void_p),
+        ("shape", ctypes.c_void_p),
+        ("sh", ctypes.c_void_p),
+        ("rest_per_colour", ctypes.c_uint32),
     ]
 
 
@@ -83,6 +96,7 @@ class Gaussians(ctypes.Structure):
         ("sh", ctypes.c_void_p),
         ("sh_half", ctypes.c_void_p),
         ("relight", ctypes.POINTER(Relight)),
+        ("packed", ctypes.POINTER(PackedSplats)),
     ]
 
 

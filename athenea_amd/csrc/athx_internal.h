@@ -54,6 +54,8 @@ struct RelightArgs {
     const float* albedo;     // [n][3], or null: the evaluated harmonics
     const float* metallic;   // [n]
     const float* roughness;  // [n]
+    const uint32_t* normals_packed;  // [n] octahedral 2 x 16 bits, or null (then normals)
+    const uint32_t* pbr;     // [n] metallic byte 0, roughness byte 1, or null (then metallic, roughness)
     float o2w[12];           // object -> world rows
     const uint32_t* texels;  // prepared dome: 8 octahedral levels, 2 words a texel
     const float* sky_sh;     // 16 float4
@@ -77,6 +79,12 @@ struct ProjectArgs {
     uint32_t tile;
     uint32_t relit;  // 1: the colour is the relit one (rl), not the harmonics'
     RelightArgs rl;
+    // athenea's packed layout (athx_packed_splats) instead of the arrays above.
+    uint32_t packed;
+    const float* positions;     // float4 a Gaussian: xyz, opacity
+    const uint32_t* shape;      // 4 words a Gaussian
+    const uint32_t* packed_sh;  // packed_words a Gaussian: rest coefficients as halves
+    uint32_t packed_words;
 };
 
 __host__ __device__ inline uint32_t div_up(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
@@ -84,22 +92,38 @@ __host__ __device__ inline uint32_t div_up(uint32_t a, uint32_t b) { return (a +
 // Which tiles each footprint touches, kept from the count for the emission:
 // the reference tests every tile twice (splat_project.slang:764-776 and
 // splat_emit.slang:37-49); here the second pass reads the first one's answer.
+//
+// A footprint over kBigRect tiles is walked in steps of one hit word each
+// (big_steps); the big footprints are listed, and their words placed, by a
+// scan over the slots (launch_big_list), so no kernel takes a place with an
+// atomic on one shared address -- those serialise in the memory system.
 struct Footprints {
-    uint2* rects;             // [slots] tile rect: x0 | y0 << 16, w | h << 16
+    uint2* rects;             // [slots] tile rect: x0 | y0 << 16, w | h << 16 (0, 0 when culled before)
     uint64_t* masks;          // [slots] a small footprint's hits, bit k = tile k of its rect, row-major
-    uint32_t* big_list;       // [slots] slots whose rect holds over kBigRect tiles
-    uint32_t* big_mask_base;  // [slots] where a big footprint's hit words start, or ~0u
-    uint64_t* big_masks;      // [big_mask_cap] 64 tiles a word, row-major
+    uint32_t* big_list;       // [slots] slots whose rect holds over kBigRect tiles, in slot order
+    uint32_t* big_mask_base;  // [slots] a big footprint's first hit word (its first step)
+    uint64_t* big_masks;      // [big_mask_cap] a word a step
     uint32_t big_mask_cap;
-    uint32_t* counters;       // [0] big footprints, [1] big mask words taken
+    uint64_t* big_total;      // big footprints << 32 | their steps, written by the scan
 };
 constexpr uint32_t kBigRect = 64;
 
+// The steps a big footprint's w x h rectangle is walked in, a hit word each:
+// whole rows, 64 / w a step, when it is at most 64 tiles wide, else 64
+// columns of one row a step.
+__host__ __device__ inline uint32_t big_steps(uint32_t w, uint32_t h) {
+    return w <= 64 ? div_up(h, 64 / w) : h * div_up(w, 64);
+}
+
 // Launchers (each returns the hipError_t of its launches).
-// Splats whose footprint spans many tiles are listed in big_list (capacity:
-// one word a slot; big_count one word) and counted a wavefront each.
+// The projection, then the exact tile counts: small rectangles spread over
+// the lanes of a wavefront, big ones listed by a scan and walked in steps.
 hipError_t launch_project(const ProjectArgs& a, const DView* views_dev, uint32_t nviews, float* records,
-                          uint32_t* touched, uint32_t* depth_keys, const Footprints& fp, hipStream_t s);
+                          uint32_t* touched, uint32_t* depth_keys, const Footprints& fp, uint32_t* status_words,
+                          hipStream_t s);
+// Lists the big footprints (rectangles over kBigRect tiles) in slot order
+// with each one's first step; the count and the steps land in *fp.big_total.
+hipError_t launch_big_list(const Footprints& fp, uint32_t slots, uint32_t* status_words, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t value, uint64_t n, hipStream_t s);
 size_t scan_status_words(uint32_t n);
 hipError_t launch_scan(const uint32_t* in, uint32_t* out, uint64_t* total, uint32_t n, uint32_t* status,
@@ -139,8 +163,10 @@ struct SortScratch {
     uint32_t* hist;      // passes * 256
     uint32_t* lookback;  // passes * partitions * 256
     uint32_t* counters;  // passes
+    uint32_t* partials;  // passes * 256 * sort_hist_blocks: each histogram block's counts
 };
 uint32_t sort_partitions(uint32_t n);
+uint32_t sort_hist_blocks(uint32_t n);
 // Sorts the first min(*n_dev, cap) pairs stably by key bits [lo_bit,
 // key_bits); scratch is sized for cap and (key_bits - lo_bit) / 8 passes.
 hipError_t launch_sort(uint64_t* keys, uint32_t* values, const uint64_t* n_dev, uint32_t cap, uint32_t lo_bit,

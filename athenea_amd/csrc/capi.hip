@@ -171,19 +171,34 @@ athx_status check_gaussians(athx_ctx* ctx, const athx_gaussians* g) {
     if (!g) {
         return fail(ctx, ATHX_INVALID_ARGUMENT, "gaussians is null");
     }
-    if (g->sh_coeffs != 1 && g->sh_coeffs != 4 && g->sh_coeffs != 9 && g->sh_coeffs != 16) {
-        return fail(ctx, ATHX_INVALID_ARGUMENT, "sh_coeffs must be 1, 4, 9 or 16");
+    if (const athx_packed_splats* p = g->packed) {
+        const uint32_t rest = p->rest_per_colour;
+        if (rest != 0 && rest != 3 && rest != 8 && rest != 15) {
+            return fail(ctx, ATHX_INVALID_ARGUMENT, "rest_per_colour must be 0, 3, 8 or 15 (packing.slang:454-456)");
+        }
+        if (g->count != 0 && (!p->positions || !p->shape || (rest != 0 && !p->sh))) {
+            return fail(ctx, ATHX_INVALID_ARGUMENT, "a packed buffer is null");
+        }
+        if ((reinterpret_cast<uintptr_t>(p->positions) | reinterpret_cast<uintptr_t>(p->shape)) & 15u) {
+            return fail(ctx, ATHX_INVALID_ARGUMENT, "packed positions and shape must be 16-byte aligned");
+        }
+    } else {
+        if (g->sh_coeffs != 1 && g->sh_coeffs != 4 && g->sh_coeffs != 9 && g->sh_coeffs != 16) {
+            return fail(ctx, ATHX_INVALID_ARGUMENT, "sh_coeffs must be 1, 4, 9 or 16");
+        }
+        if (g->count != 0) {
+            if (!g->means || !g->log_scales || !g->quats || !g->opacity_logits) {
+                return fail(ctx, ATHX_INVALID_ARGUMENT, "a Gaussian parameter pointer is null");
+            }
+            if ((g->sh == nullptr) == (g->sh_half == nullptr)) {
+                return fail(ctx, ATHX_INVALID_ARGUMENT, "exactly one of sh and sh_half must be given");
+            }
+        }
     }
     if (g->count != 0) {
-        if (!g->means || !g->log_scales || !g->quats || !g->opacity_logits) {
-            return fail(ctx, ATHX_INVALID_ARGUMENT, "a Gaussian parameter pointer is null");
-        }
-        if ((g->sh == nullptr) == (g->sh_half == nullptr)) {
-            return fail(ctx, ATHX_INVALID_ARGUMENT, "exactly one of sh and sh_half must be given");
-        }
         if (const athx_relight* r = g->relight) {
-            if (!r->metallic || !r->roughness) {
-                return fail(ctx, ATHX_INVALID_ARGUMENT, "relight needs metallic and roughness");
+            if (!r->pbr && (!r->metallic || !r->roughness)) {
+                return fail(ctx, ATHX_INVALID_ARGUMENT, "relight needs metallic and roughness, or pbr words");
             }
             if (!r->sky.texels || !r->sky.sh || !r->sky.sun || r->sky.base_side == 0) {
                 return fail(ctx, ATHX_INVALID_ARGUMENT, "relight needs a prepared sky (texels, sh, sun, base side)");
@@ -208,7 +223,15 @@ ProjectArgs project_args(const athx_gaussians* g, const athx_settings* s) {
     a.sh_half = g->sh_half;
     a.n = g->count;
     a.sh_coeffs = g->sh_coeffs;
-    const uint32_t held = degree_of(g->sh_coeffs);
+    if (const athx_packed_splats* p = g->packed) {
+        a.packed = 1;
+        a.positions = p->positions;
+        a.shape = p->shape;
+        a.packed_sh = p->sh;
+        a.packed_words = (p->rest_per_colour * 3 + 1) / 2;  // packing.slang:456 (shWordsFor)
+        a.sh_coeffs = p->rest_per_colour + 1;
+    }
+    const uint32_t held = degree_of(a.sh_coeffs);
     a.sh_degree = held < s->max_sh_degree ? held : s->max_sh_degree;
     a.linear = s->linear ? 1u : 0u;
     a.antialias = s->antialias ? 1u : 0u;
@@ -219,6 +242,8 @@ ProjectArgs project_args(const athx_gaussians* g, const athx_settings* s) {
         a.rl.albedo = r->albedo;
         a.rl.metallic = r->metallic;
         a.rl.roughness = r->roughness;
+        a.rl.normals_packed = r->normals_packed;
+        a.rl.pbr = r->pbr;
         for (int k = 0; k < 12; ++k) {
             a.rl.o2w[k] = r->object_to_world[k];
         }
@@ -316,9 +341,11 @@ double elapsed_ms(hipEvent_t a, hipEvent_t b) {
 }
 
 // Sort metadata words for n pairs: hist (passes * 256), look-back
-// (passes * partitions * 256), counters (passes), back to back.
+// (passes * partitions * 256), counters (passes), the histogram blocks'
+// partial counts (passes * 256 * blocks), back to back.
 size_t sort_meta_words(uint32_t n, uint32_t passes) {
-    return (size_t)passes * 256 + (size_t)passes * sort_partitions(n) * 256 + passes;
+    return (size_t)passes * 256 + (size_t)passes * sort_partitions(n) * 256 + passes +
+           (size_t)passes * 256 * sort_hist_blocks(n);
 }
 
 SortScratch sort_scratch(const DeviceBuffer& meta, void* alt_keys, uint32_t* alt_vals, uint32_t n,
@@ -329,6 +356,7 @@ SortScratch sort_scratch(const DeviceBuffer& meta, void* alt_keys, uint32_t* alt
     sc.hist = static_cast<uint32_t*>(meta.ptr);
     sc.lookback = sc.hist + (size_t)passes * 256;
     sc.counters = sc.lookback + (size_t)passes * sort_partitions(n) * 256;
+    sc.partials = sc.counters + passes;
     return sc;
 }
 
@@ -405,7 +433,7 @@ Footprints footprints_of(athx_ctx* ctx, uint32_t slots) {
     fp.big_mask_base = as<uint32_t>(ctx->fp_big_base);
     fp.big_masks = as<uint64_t>(ctx->fp_big_masks);
     fp.big_mask_cap = big_mask_words(slots);
-    fp.counters = as<uint32_t>(ctx->totals) + 4;  // totals: [0..1] pairs, [2..3] visible, [4] big, [5] big words
+    fp.big_total = as<uint64_t>(ctx->totals) + 2;  // totals (u64): [0] pairs, [1] visible, [2] big << 32 | steps
     return fp;
 }
 
@@ -433,7 +461,7 @@ athx_status enqueue_front(athx_ctx* ctx, const ProjectArgs& pa, const FrameShape
     }
     ATHX_HIP(ctx, launch_project(pa, as<DView>(ctx->views), fs.nviews, as<float>(ctx->records),
                                  as<uint32_t>(ctx->touched), as<uint32_t>(ctx->depth_keys),
-                                 footprints_of(ctx, fs.slots), s),
+                                 footprints_of(ctx, fs.slots), as<uint32_t>(ctx->scan_status), s),
              "project");
     if (timed) {
         ATHX_HIP(ctx, hipEventRecord(ctx->ev[1], s), "event");
@@ -812,6 +840,9 @@ ATHX_API athx_status athx_render_backward(athx_ctx* ctx, const athx_view* views,
     if (gaussians->relight) {
         return fail(ctx, ATHX_UNSUPPORTED, "the adjoint of the relit colour is not built: backward is SH-coloured only");
     }
+    if (gaussians->packed) {
+        return fail(ctx, ATHX_UNSUPPORTED, "the adjoint is taken with respect to the FP32 arrays, not a packed cloud");
+    }
     for (uint32_t k = 0; k < nviews; ++k) {
         if (!targets[k].final_t || !targets[k].last || !dl_dcolour[k]) {
             return fail(ctx, ATHX_INVALID_ARGUMENT,
@@ -1079,7 +1110,7 @@ ATHX_API athx_status athx_graph_create(int device, const athx_view* views, uint3
     athx_ctx* ctx = g->ctx;
     ctx->exact_growth = true;
     const bool with_backward = dl_dcolour != nullptr && grads != nullptr;
-    if (with_backward && gaussians && gaussians->relight) {
+    if (with_backward && gaussians && (gaussians->relight || gaussians->packed)) {
         return graph_fail(g, fail(ctx, ATHX_UNSUPPORTED, "the adjoint of the relit colour is not built"), error,
                           error_bytes);
     }

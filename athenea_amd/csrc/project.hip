@@ -32,35 +32,46 @@ __device__ inline float cutoff_power(float alpha) {
     return fmin_(2.0f * athx_logf(fmax_(255.0f * alpha, 1.0001f)), 9.0f);
 }
 
-// frame.slang:354-371
-__device__ inline float min_power_over_box(float a, float b, float c, float lox, float loy, float hix, float hiy) {
+// frame.slang:354-371: the least power a * x^2 + 2 b x y + c y^2 takes
+// over the box [lox, hix] x [loy, hiy]: zero when the box holds the centre,
+// else the least over its four edges, each at the edge's clamped 1D
+// minimiser. The minimiser ((-b) x) / c is taken as ((-b) x) * (1 / c),
+// with the reciprocals (ia = 1 / a, ic = 1 / c) formed once a footprint:
+// the reference's divide has no defined rounding across its backends, and
+// this form is the one every test of a tile here and in the host
+// restatement computes, bit for bit.
+__device__ inline float min_power_over_box(float a, float b, float c, float ia, float ic, float lox, float loy,
+                                           float hix, float hiy) {
     if (lox <= 0.0f && hix >= 0.0f && loy <= 0.0f && hiy >= 0.0f) {
         return 0.0f;
     }
     float best = 3.4e38f;
     for (int e = 0; e < 2; ++e) {
         const float x = e == 0 ? lox : hix;
-        const float y = c > 1e-12f ? clamp_(((-b) * x) / c, loy, hiy) : loy;
+        const float y = c > 1e-12f ? clamp_(((-b) * x) * ic, loy, hiy) : loy;
         best = fmin_(best, a * x * x + 2.0f * b * x * y + c * y * y);
     }
     for (int e = 0; e < 2; ++e) {
         const float y = e == 0 ? loy : hiy;
-        const float x = a > 1e-12f ? clamp_(((-b) * y) / a, lox, hix) : lox;
+        const float x = a > 1e-12f ? clamp_(((-b) * y) * ia, lox, hix) : lox;
         best = fmin_(best, a * x * x + 2.0f * b * x * y + c * y * y);
     }
     return best;
 }
 
+// The reciprocals min_power_over_box takes, formed once a footprint.
+__device__ inline float recip_of(float v) { return v > 1e-12f ? 1.0f / v : 0.0f; }
+
 // frame.slang:374-379
 __device__ inline bool tile_touched(uint32_t W, uint32_t H, uint32_t T, float cx, float cy, float a, float b,
-                                    float c, float cutoff, uint32_t tx, uint32_t ty) {
+                                    float c, float ia, float ic, float cutoff, uint32_t tx, uint32_t ty) {
     const float fx = (float)(tx * T) + 0.5f;
     const float fy = (float)(ty * T) + 0.5f;
     const uint32_t ex = (tx + 1) * T < W ? (tx + 1) * T : W;
     const uint32_t ey = (ty + 1) * T < H ? (ty + 1) * T : H;
     const float lx = (float)(ex - 1) + 0.5f;
     const float ly = (float)(ey - 1) + 0.5f;
-    return min_power_over_box(a, b, c, fx - cx, fy - cy, lx - cx, ly - cy) <= cutoff;
+    return min_power_over_box(a, b, c, ia, ic, fx - cx, fy - cy, lx - cx, ly - cy) <= cutoff;
 }
 
 // frame.slang:383-402: the footprint's tile rectangle, inclusive; false when
@@ -94,15 +105,13 @@ __device__ inline float sh_at(const float* sh, const uint16_t* shh, size_t at) {
     }
 }
 
-// shaders/athenea/common/sh.slang:25-51, one channel; coefficient k of the
-// channel at row[k * 3] (k = 0, the constant term, is read by the caller).
-template <bool kHalf>
-__device__ inline float sh_rest(const float* sh, const uint16_t* shh, size_t row, uint32_t degree, float x, float y,
-                                float z) {
+// shaders/athenea/common/sh.slang:25-51, one channel: B(k) is coefficient k
+// of the channel (k = 0, the constant term, is read by the caller).
+template <typename Coeff>
+__device__ inline float sh_rest(Coeff B, uint32_t degree, float x, float y, float z) {
     if (degree == 0) {
         return 0.0f;
     }
-#define B(k) sh_at<kHalf>(sh, shh, row + (k) * 3)
     float r = -kC1 * y * B(1) + kC1 * z * B(2) - kC1 * x * B(3);
     if (degree < 2) {
         return r;
@@ -117,8 +126,38 @@ __device__ inline float sh_rest(const float* sh, const uint16_t* shh, size_t row
     r = r + (kC3_0 * y * (3.0f * xx - yy) * B(9) + kC3_1 * xy * z * B(10) + kC3_2 * y * (4.0f * zz - xx - yy) * B(11) +
              kC3_3 * z * (2.0f * zz - 3.0f * xx - 3.0f * yy) * B(12) + kC3_4 * x * (4.0f * zz - xx - yy) * B(13) +
              kC3_5 * z * (xx - yy) * B(14) + kC3_6 * x * (xx - 3.0f * yy) * B(15));
-#undef B
     return r;
+}
+
+// All 16 x 3 coefficients of Gaussian i in 16-byte loads (12 of FP32, 6 of
+// FP16): a row is 192 (96) bytes, so a lane's loads are whole 16-byte words
+// and a wavefront touches 12 (6) lines a lane in all, where one coefficient
+// a load touched 48.
+template <bool kHalf>
+__device__ inline void load_sh16(const float* sh, const uint16_t* shh, uint32_t i, float c[48]) {
+    if constexpr (kHalf) {
+        const uint4* p = reinterpret_cast<const uint4*>(shh + (size_t)i * 48);
+#pragma unroll
+        for (uint32_t q = 0; q < 6; ++q) {
+            const uint4 v = p[q];
+            const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+            for (uint32_t e = 0; e < 4; ++e) {
+                c[q * 8 + 2 * e] = half_bits_to_float((uint16_t)(w[e] & 0xFFFFu));
+                c[q * 8 + 2 * e + 1] = half_bits_to_float((uint16_t)(w[e] >> 16));
+            }
+        }
+    } else {
+        const float4* p = reinterpret_cast<const float4*>(sh + (size_t)i * 48);
+#pragma unroll
+        for (uint32_t q = 0; q < 12; ++q) {
+            const float4 v = p[q];
+            c[q * 4 + 0] = v.x;
+            c[q * 4 + 1] = v.y;
+            c[q * 4 + 2] = v.z;
+            c[q * 4 + 3] = v.w;
+        }
+    }
 }
 
 // shaders/athenea/common/color.slang:7-10
@@ -138,29 +177,81 @@ __device__ inline void cull_early(uint32_t slot, uint32_t reason, uint32_t* touc
     rects[slot] = make_uint2(0u, 0u);
 }
 
-template <bool kHalf, bool kRelit>
+// packing.slang:59-76 (decodeQuaternion): three 10-bit components of the
+// smallest-three encoding and the index of the dropped (largest) one, which
+// is rebuilt from the unit length. Not normalised again, as there.
+__device__ inline void decode_quaternion(uint32_t word, float& x, float& y, float& z, float& w) {
+    constexpr float kSqrtHalf = 0.70710678f;
+    const uint32_t largest = word & 3u;
+    const float s0 = ((float)((word >> 22) & 1023u) / 1023.0f * 2.0f - 1.0f) * kSqrtHalf;
+    const float s1 = ((float)((word >> 12) & 1023u) / 1023.0f * 2.0f - 1.0f) * kSqrtHalf;
+    const float s2 = ((float)((word >> 2) & 1023u) / 1023.0f * 2.0f - 1.0f) * kSqrtHalf;
+    const float rest = sqrtf(fmax_(0.0f, 1.0f - (s0 * s0 + s1 * s1 + s2 * s2)));
+    float q[4];
+    const float small[3] = {s0, s1, s2};
+    uint32_t next = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < 4; ++k) {
+        if (k == largest) {
+            q[k] = rest;
+        } else {
+            q[k] = small[next < 3 ? next : 2];
+            next += 1;
+        }
+    }
+    x = q[0]; y = q[1]; z = q[2]; w = q[3];
+}
+
+// The Gaussians' input, a template argument of the projection: the FP32 or
+// FP16-SH structure of arrays (Vec: 16 coefficients a channel in aligned
+// rows, read whole once for every view), or athenea's packed layout.
+constexpr uint32_t kInSoa32 = 0, kInSoa32Vec = 1, kInSoa16 = 2, kInSoa16Vec = 3, kInPacked = 4;
+
+template <uint32_t kIn, bool kRelit>
 __global__ void __launch_bounds__(256) project_kernel(ProjectArgs a, const DView* __restrict__ views, uint32_t nviews,
                                                       float* __restrict__ records, uint32_t* __restrict__ touched,
                                                       uint32_t* __restrict__ keys, Footprints fp) {
+    constexpr bool kHalf = kIn == kInSoa16 || kIn == kInSoa16Vec;
+    constexpr bool kVec = kIn == kInSoa32Vec || kIn == kInSoa16Vec;
+    constexpr bool kPacked = kIn == kInPacked;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.n) {
         return;
     }
-    const float px = a.means[i * 3 + 0], py = a.means[i * 3 + 1], pz = a.means[i * 3 + 2];
-    const float sx = athx_expf(a.log_scales[i * 3 + 0]);
-    const float sy = athx_expf(a.log_scales[i * 3 + 1]);
-    const float sz = athx_expf(a.log_scales[i * 3 + 2]);
-    const float opacity = athx_sigmoidf(a.logits[i]);
+    float px, py, pz, sx, sy, sz, opacity, qx, qy, qz, qw;
+    float stored[3] = {0.0f, 0.0f, 0.0f};  // the packed layout's base colour
+    if constexpr (kPacked) {
+        // splat_project.slang:507-510, 526, 781-782: position and linear
+        // opacity, f16 log-scales, the quaternion and the base colour.
+        const float4 po = reinterpret_cast<const float4*>(a.positions)[i];
+        const uint4 shape = reinterpret_cast<const uint4*>(a.shape)[i];
+        px = po.x; py = po.y; pz = po.z;
+        opacity = po.w;
+        sx = athx_expf(half_bits_to_float((uint16_t)(shape.y & 0xFFFFu)));
+        sy = athx_expf(half_bits_to_float((uint16_t)(shape.y >> 16)));
+        sz = athx_expf(half_bits_to_float((uint16_t)(shape.z & 0xFFFFu)));
+        decode_quaternion(shape.x, qx, qy, qz, qw);
+        stored[0] = half_bits_to_float((uint16_t)(shape.z >> 16));
+        stored[1] = half_bits_to_float((uint16_t)(shape.w & 0xFFFFu));
+        stored[2] = half_bits_to_float((uint16_t)(shape.w >> 16));
+    } else {
+        px = a.means[i * 3 + 0]; py = a.means[i * 3 + 1]; pz = a.means[i * 3 + 2];
+        sx = athx_expf(a.log_scales[i * 3 + 0]);
+        sy = athx_expf(a.log_scales[i * 3 + 1]);
+        sz = athx_expf(a.log_scales[i * 3 + 2]);
+        opacity = athx_sigmoidf(a.logits[i]);
+        // Quaternion (x, y, z, w), normalised (packing.slang:59-76 decodes a unit one).
+        qx = a.quats[i * 4 + 0]; qy = a.quats[i * 4 + 1]; qz = a.quats[i * 4 + 2]; qw = a.quats[i * 4 + 3];
+        const float qn = qx * qx + qy * qy + qz * qz + qw * qw;
+        if (qn > 0.0f) {
+            const float inv = 1.0f / sqrtf(qn);
+            qx = qx * inv; qy = qy * inv; qz = qz * inv; qw = qw * inv;
+        } else {
+            qx = 0.0f; qy = 0.0f; qz = 0.0f; qw = 1.0f;
+        }
+    }
 
     // The rotation and R S, the same for every view (splat_project.slang:526-532).
-    float qx = a.quats[i * 4 + 0], qy = a.quats[i * 4 + 1], qz = a.quats[i * 4 + 2], qw = a.quats[i * 4 + 3];
-    const float qn = qx * qx + qy * qy + qz * qz + qw * qw;
-    if (qn > 0.0f) {
-        const float inv = 1.0f / sqrtf(qn);
-        qx = qx * inv; qy = qy * inv; qz = qz * inv; qw = qw * inv;
-    } else {
-        qx = 0.0f; qy = 0.0f; qz = 0.0f; qw = 1.0f;
-    }
     const float r00 = 1.0f - 2.0f * (qy * qy + qz * qz), r01 = 2.0f * (qx * qy - qw * qz),
                 r02 = 2.0f * (qx * qz + qw * qy);
     const float r10 = 2.0f * (qx * qy + qw * qz), r11 = 1.0f - 2.0f * (qx * qx + qz * qz),
@@ -170,6 +261,12 @@ __global__ void __launch_bounds__(256) project_kernel(ProjectArgs a, const DView
     const float m00 = r00 * sx, m01 = r01 * sy, m02 = r02 * sz;
     const float m10 = r10 * sx, m11 = r11 * sy, m12 = r12 * sz;
     const float m20 = r20 * sx, m21 = r21 * sy, m22 = r22 * sz;
+    // kVec: 16 coefficients a channel in 16-byte aligned rows, read once for
+    // every view.
+    float shc[kVec ? 48 : 1];
+    if constexpr (kVec) {
+        load_sh16<kHalf>(a.sh, a.sh_half, i, shc);
+    }
 
     for (uint32_t v = 0; v < nviews; ++v) {
         const DView& V = views[v];
@@ -254,14 +351,11 @@ __global__ void __launch_bounds__(256) project_kernel(ProjectArgs a, const DView
         // The exact count (splat_project.slang:764-776) is taken after this
         // kernel, with the tiles of many footprints spread over the lanes of
         // a wavefront: count_small_kernel for rectangles of up to kBigRect
-        // tiles, count_big_kernel a wavefront per larger one. A thread
-        // walking its own rectangle would hold its wavefront to the largest
-        // of 64. The count kernels also write the slot's tile count and,
-        // when no tile is touched, its cull.
+        // tiles, count_big_kernel, a wavefront at a time, for the larger
+        // ones (listed by a scan). A thread walking its own rectangle would
+        // hold its wavefront to the largest of 64. The count kernels also
+        // write the slot's tile count and, when no tile is touched, its cull.
         const uint32_t w = x1 - x0 + 1, h = y1 - y0 + 1;
-        if (w * h > kBigRect) {
-            fp.big_list[atomicAdd(&fp.counters[0], 1u)] = slot;
-        }
         fp.rects[slot] = make_uint2(x0 | (y0 << 16), w | (h << 16));
         // Colour from this eye, in the cloud's space (splat_project.slang:781-785).
         const float dx = px - V.eye[0], dy = py - V.eye[1], dz = pz - V.eye[2];
@@ -271,8 +365,28 @@ __global__ void __launch_bounds__(256) project_kernel(ProjectArgs a, const DView
         float col[3];
 #pragma unroll
         for (uint32_t ch = 0; ch < 3; ++ch) {
-            const float base = 0.5f + kSH0 * sh_at<kHalf>(a.sh, a.sh_half, row + ch);
-            const float cv = base + sh_rest<kHalf>(a.sh, a.sh_half, row + ch, a.sh_degree, ex, ey, ez);
+            float base, rest;
+            if constexpr (kPacked) {
+                // sh.slang:15-22: rest coefficient k - 1 of this channel is
+                // half (k - 1) * 3 + ch of the Gaussian's words.
+                const uint32_t* words = a.packed_sh + (size_t)i * a.packed_words;
+                base = stored[ch];
+                rest = sh_rest(
+                    [&](uint32_t k) {
+                        const uint32_t h = (k - 1) * 3 + ch;
+                        const uint32_t word = words[h >> 1];
+                        return half_bits_to_float((uint16_t)((h & 1u) ? word >> 16 : word & 0xFFFFu));
+                    },
+                    a.sh_degree, ex, ey, ez);
+            } else if constexpr (kVec) {
+                base = 0.5f + kSH0 * shc[ch];
+                rest = sh_rest([&](uint32_t k) { return shc[k * 3 + ch]; }, a.sh_degree, ex, ey, ez);
+            } else {
+                base = 0.5f + kSH0 * sh_at<kHalf>(a.sh, a.sh_half, row + ch);
+                rest = sh_rest([&](uint32_t k) { return sh_at<kHalf>(a.sh, a.sh_half, row + ch + k * 3); },
+                               a.sh_degree, ex, ey, ez);
+            }
+            const float cv = base + rest;
             col[ch] = a.linear ? fmax_(cv, 0.0f) : srgb_to_linear(cv);
             flags |= (cv > 0.0f ? kFlagColourR : 0u) << ch;
             flags |= (!a.linear && fmax_(cv, 0.0f) > 0.04045f ? kFlagSrgbR : 0u) << ch;
@@ -302,19 +416,37 @@ __global__ void fill_u32_kernel(uint32_t* p, uint32_t value, uint64_t n) {
     }
 }
 
-// Big footprints go 64 to a wavefront: lane j reads footprint j's record
-// and rectangle, the wavefront claims the hit-mask words of all 64 with one
-// atomic (one per footprint, on one address, serialises in the memory
-// system), then walks them one at a time with the footprint's fields read
-// from its lane. Lane k of a step tests tile k of the step's run of 64 in
-// the rectangle's row-major order, and a ballot gives each hit its place, so
-// the count is the serial loop's and the pairs come out in its order.
+// k / w and k % w for k < 64, 1 <= w <= 64: floor(k rcp(w) + 1e-3) is exact
+// there (rcp's relative error, under 2^-22, moves k/w by at most 1.6e-5,
+// and a k/w that is not whole is at least 1/64 below the next integer).
+__device__ inline uint32_t small_row(uint32_t k, uint32_t w) {
+    return (uint32_t)((float)k * __builtin_amdgcn_rcpf((float)w) + 1.0e-3f);
+}
+
+// Big footprints (rectangles of more than kBigRect tiles) go kBigPerWave to
+// a wavefront, in the order the listing scan put them (slot order): lane j
+// reads footprint j's record and rectangle, then the wavefront walks them
+// one at a time with the footprint's fields read from its lane. The scan
+// gave each footprint its first hit word, so nothing is claimed here.
+//
+// A step covers a run of the rectangle's tiles in row-major order, one tile
+// a lane and the ballot of the hits one word: whole rows, 64 / w of them,
+// when the rectangle is at most 64 tiles wide (lane = row in the step * w +
+// column, so a lane keeps its column for the whole footprint), or 64
+// columns of one row when it is wider. The hit words are the emission's.
+//
+// The tile test is min_power_over_box's, taken apart: what it computes
+// from a tile's column alone (the box's x edges, the y that minimises the
+// power on each, the x parts of the power there) and from its row alone
+// are each computed once for the lane's column or row, and a tile combines
+// them in the order the whole function does, so the hits are its bits.
 constexpr uint32_t kBigThreads = 256;
-constexpr uint32_t kBigBlocks = 2048;
+constexpr uint32_t kBigBlocks = 4096;
+constexpr uint32_t kBigPerWave = 16;
 
 struct BigFootprint {
-    float4 screen, conic;
-    uint32_t view, x0, y0, w, area;
+    float cx, cy, a, b, c, cutoff, ia, ic;
+    uint32_t view, x0, y0, w, h;
 };
 
 __device__ inline BigFootprint big_footprint(const DView* views, uint32_t n, uint32_t T, const float* records,
@@ -323,13 +455,19 @@ __device__ inline BigFootprint big_footprint(const DView* views, uint32_t n, uin
     f.view = slot / n;
     const DView& V = views[f.view];
     const float4* rec = reinterpret_cast<const float4*>(records + (size_t)slot * kRecordFloats);
-    f.screen = rec[0];
-    f.conic = rec[1];
+    const float4 screen = rec[0], conic = rec[1];
+    f.cx = screen.x;
+    f.cy = screen.y;
+    f.cutoff = screen.w;
+    f.a = conic.x;
+    f.b = conic.y;
+    f.c = conic.z;
+    f.ia = recip_of(f.a);
+    f.ic = recip_of(f.c);
     uint32_t x1, y1;
-    tile_rect(V.width, V.height, T, f.screen.x, f.screen.y, f.conic.x, f.conic.y, f.conic.z, f.screen.w, f.x0, f.y0,
-              x1, y1);
+    tile_rect(V.width, V.height, T, f.cx, f.cy, f.a, f.b, f.c, f.cutoff, f.x0, f.y0, x1, y1);
     f.w = x1 - f.x0 + 1;
-    f.area = f.w * (y1 - f.y0 + 1);
+    f.h = y1 - f.y0 + 1;
     return f;
 }
 
@@ -345,19 +483,111 @@ __device__ inline uint64_t lane_u64(uint64_t v, uint32_t j) {
 // the fields land in scalar registers).
 __device__ inline BigFootprint big_of_lane(const BigFootprint& f, uint32_t j) {
     BigFootprint g;
-    g.screen = make_float4(lane_f(f.screen.x, j), lane_f(f.screen.y, j), 0.0f, lane_f(f.screen.w, j));
-    g.conic = make_float4(lane_f(f.conic.x, j), lane_f(f.conic.y, j), lane_f(f.conic.z, j), 0.0f);
+    g.cx = lane_f(f.cx, j);
+    g.cy = lane_f(f.cy, j);
+    g.a = lane_f(f.a, j);
+    g.b = lane_f(f.b, j);
+    g.c = lane_f(f.c, j);
+    g.cutoff = lane_f(f.cutoff, j);
+    g.ia = lane_f(f.ia, j);
+    g.ic = lane_f(f.ic, j);
     g.view = lane_u(f.view, j);
     g.x0 = lane_u(f.x0, j);
     g.y0 = lane_u(f.y0, j);
     g.w = lane_u(f.w, j);
-    g.area = lane_u(f.area, j);
+    g.h = lane_u(f.h, j);
     return g;
 }
 
-__device__ inline bool big_hit(const DView& V, const BigFootprint& f, uint32_t T, uint32_t k) {
-    return k < f.area && tile_touched(V.width, V.height, T, f.screen.x, f.screen.y, f.conic.x, f.conic.y, f.conic.z,
-                                      f.screen.w, f.x0 + k % f.w, f.y0 + k / f.w);
+// One axis of a tile's box (frame.slang:374-379) and what min_power_over_box
+// (:354-371) computes from it alone. Along x: the box's edges lo, hi (pixel
+// centres relative to the splat's), the y minimising the power on each edge
+// before its clamp, a x^2 and 2b x. Along y: the edges, the x minimising
+// the power on each, and c y^2.
+struct BoxAxis {
+    float lo, hi, q_lo, q_hi, s_lo, s_hi, t_lo, t_hi;
+};
+
+__device__ inline BoxAxis box_axis(uint32_t t, uint32_t T, uint32_t extent, float centre, float inv_across, float b,
+                                   float along, float b2) {
+    BoxAxis r;
+    const uint32_t e = (t + 1) * T < extent ? (t + 1) * T : extent;
+    r.lo = ((float)(t * T) + 0.5f) - centre;
+    r.hi = ((float)(e - 1) + 0.5f) - centre;
+    // The other coordinate's minimiser on each edge: ((-b) * edge) * (1 / across).
+    r.q_lo = ((-b) * r.lo) * inv_across;
+    r.q_hi = ((-b) * r.hi) * inv_across;
+    r.s_lo = along * r.lo * r.lo;
+    r.s_hi = along * r.hi * r.hi;
+    r.t_lo = b2 * r.lo;
+    r.t_hi = b2 * r.hi;
+    return r;
+}
+
+// min_power_over_box(a, b, c, box) <= cutoff, from the tile's column values
+// X = box_axis(tx, ..., cx, c, b, a, 2b) and row values Y = box_axis(ty,
+// ..., cy, a, b, c, 2b): the same operations in the same order.
+__device__ inline bool box_hit(const BigFootprint& g, float b2, const BoxAxis& X, const BoxAxis& Y) {
+    if (X.lo <= 0.0f && X.hi >= 0.0f && Y.lo <= 0.0f && Y.hi >= 0.0f) {
+        return true;  // a power of 0; the cut power is above 0
+    }
+    float best = 3.4e38f;
+    {
+        const float y = g.c > 1e-12f ? clamp_(X.q_lo, Y.lo, Y.hi) : Y.lo;
+        best = fmin_(best, X.s_lo + X.t_lo * y + g.c * y * y);
+    }
+    {
+        const float y = g.c > 1e-12f ? clamp_(X.q_hi, Y.lo, Y.hi) : Y.lo;
+        best = fmin_(best, X.s_hi + X.t_hi * y + g.c * y * y);
+    }
+    {
+        const float x = g.a > 1e-12f ? clamp_(Y.q_lo, X.lo, X.hi) : X.lo;
+        best = fmin_(best, g.a * x * x + b2 * x * Y.lo + Y.s_lo);
+    }
+    {
+        const float x = g.a > 1e-12f ? clamp_(Y.q_hi, X.lo, X.hi) : X.lo;
+        best = fmin_(best, g.a * x * x + b2 * x * Y.hi + Y.s_hi);
+    }
+    return best <= g.cutoff;
+}
+
+__device__ inline BoxAxis column_of(const BigFootprint& g, const DView& V, uint32_t T, float b2, uint32_t tx) {
+    return box_axis(tx, T, V.width, g.cx, g.ic, g.b, g.a, b2);
+}
+__device__ inline BoxAxis row_of(const BigFootprint& g, const DView& V, uint32_t T, float b2, uint32_t ty) {
+    return box_axis(ty, T, V.height, g.cy, g.ia, g.b, g.c, b2);
+}
+
+// Walks footprint g's steps, calling step(s, hit, tile) with this lane's tile
+// (column, row within the rectangle) and whether the lane has one.
+template <typename Step>
+__device__ inline void walk_big(const BigFootprint& g, const DView& V, uint32_t T, uint32_t lane, Step&& step) {
+    const float b2 = 2.0f * g.b;
+    if (g.w <= 64) {
+        const uint32_t rows = 64 / g.w;
+        const uint32_t dr = small_row(lane, g.w), col = lane - dr * g.w;
+        const bool lane_on = dr < rows;
+        const BoxAxis X = column_of(g, V, T, b2, g.x0 + col);
+        const uint32_t steps = div_up(g.h, rows);
+        for (uint32_t s = 0; s < steps; ++s) {
+            const uint32_t r = s * rows + dr;
+            const bool on = lane_on && r < g.h;
+            const bool hit = on && box_hit(g, b2, X, row_of(g, V, T, b2, g.y0 + r));
+            step(s, on, hit, col, r);
+        }
+    } else {
+        const uint32_t chunks = div_up(g.w, 64);
+        uint32_t s = 0;
+        for (uint32_t r = 0; r < g.h; ++r) {
+            const BoxAxis Y = row_of(g, V, T, b2, g.y0 + r);
+            for (uint32_t ch = 0; ch < chunks; ++ch, ++s) {
+                const uint32_t col = ch * 64 + lane;
+                const bool on = col < g.w;
+                const bool hit = on && box_hit(g, b2, column_of(g, V, T, b2, g.x0 + col), Y);
+                step(s, on, hit, col, r);
+            }
+        }
+    }
 }
 
 __device__ inline uint32_t wave_inclusive_sum(uint32_t v, uint32_t lane) {
@@ -375,48 +605,40 @@ __global__ void __launch_bounds__(kBigThreads) count_big_kernel(const DView* __r
                                                                 uint32_t* __restrict__ keys) {
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t waves = gridDim.x * (kBigThreads / 64);
-    const uint32_t total = fp.counters[0];
-    for (uint32_t c0 = (blockIdx.x * (kBigThreads / 64) + (threadIdx.x >> 6)) * 64u; c0 < total; c0 += waves * 64u) {
+    const uint32_t total = (uint32_t)(*fp.big_total >> 32);
+    for (uint32_t c0 = (blockIdx.x * (kBigThreads / 64) + (threadIdx.x >> 6)) * kBigPerWave; c0 < total;
+         c0 += waves * kBigPerWave) {
         const uint32_t b = c0 + lane;
-        const bool mine = b < total;
+        const bool mine = lane < kBigPerWave && b < total;
         const uint32_t slot = mine ? fp.big_list[b] : 0u;
         BigFootprint f = {};
+        uint64_t at = 0;
         if (mine) {
             f = big_footprint(views, n, T, records, slot);
+            at = fp.big_mask_base[b];
         }
-        // Room for the 64 footprints' hit words, if the buffer has it; a
-        // footprint that does not fit is tested again by the emission.
-        const uint32_t words = div_up(f.area, 64);
-        const uint32_t incl = wave_inclusive_sum(words, lane);
-        uint32_t claimed = 0;
-        if (lane == 63) {
-            claimed = atomicAdd(&fp.counters[1], incl);
-        }
-        const uint64_t at = (uint64_t)lane_u(claimed, 63) + incl - words;
-        const bool keep = at + words <= fp.big_mask_cap;
-        if (mine) {
-            fp.big_mask_base[b] = keep ? (uint32_t)at : ~0u;
-        }
-        const uint32_t here = total - c0 < 64u ? total - c0 : 64u;
+        // Words past the buffer are not kept: the emission tests those
+        // footprints' tiles again.
+        const bool keep = mine && at + big_steps(f.w, f.h) <= fp.big_mask_cap;
+        const uint32_t here = total - c0 < kBigPerWave ? total - c0 : kBigPerWave;
         uint32_t my_count = 0;
         for (uint32_t j = 0; j < here; ++j) {
             const BigFootprint g = big_of_lane(f, j);
             const DView& V = views[g.view];
             const bool keep_j = lane_u(keep ? 1u : 0u, j) != 0;
             const uint64_t at_j = lane_u64(at, j);
+            const uint32_t steps = big_steps(g.w, g.h);
             uint32_t count = 0;
             uint64_t run = 0;  // lane r holds the hit word of step r of the current run of 64 steps
-            for (uint32_t k0 = 0; k0 < g.area; k0 += 64) {
-                const uint64_t hits = __ballot(big_hit(V, g, T, k0 + lane));
+            walk_big(g, V, T, lane, [&](uint32_t s, bool, bool hit, uint32_t, uint32_t) {
+                const uint64_t hits = __ballot(hit);
                 count += (uint32_t)__popcll(hits);
-                const uint32_t step = (k0 >> 6) & 63u;
-                run = lane == step ? hits : run;
-                if (keep_j && (step == 63u || k0 + 64 >= g.area)) {
-                    if (lane <= step) {
-                        fp.big_masks[at_j + ((k0 >> 6) & ~63u) + lane] = run;
-                    }
+                const uint32_t r = s & 63u;
+                run = lane == r ? hits : run;
+                if (keep_j && (r == 63u || s + 1 == steps) && lane <= r) {
+                    fp.big_masks[at_j + (s & ~63u) + lane] = run;
                 }
-            }
+            });
             my_count = lane == j ? count : my_count;
         }
         if (mine) {
@@ -437,20 +659,22 @@ __global__ void __launch_bounds__(kBigThreads) emit_big_kernel(const DView* __re
     const uint32_t lane = threadIdx.x & 63u;
     const uint64_t lt_mask = (1ull << lane) - 1ull;
     const uint32_t waves = gridDim.x * (kBigThreads / 64);
-    const uint32_t total = fp.counters[0];
-    for (uint32_t c0 = (blockIdx.x * (kBigThreads / 64) + (threadIdx.x >> 6)) * 64u; c0 < total; c0 += waves * 64u) {
+    const uint32_t total = (uint32_t)(*fp.big_total >> 32);
+    for (uint32_t c0 = (blockIdx.x * (kBigThreads / 64) + (threadIdx.x >> 6)) * kBigPerWave; c0 < total;
+         c0 += waves * kBigPerWave) {
         const uint32_t b = c0 + lane;
-        const bool mine = b < total;
+        const bool mine = lane < kBigPerWave && b < total;
         const uint32_t slot = mine ? fp.big_list[b] : 0u;
         const uint32_t expected = mine ? touched[slot] : 0u;
         BigFootprint f = {};
         uint32_t base = ~0u, first = 0;
         if (expected != 0) {
             f = big_footprint(views, n, T, records, slot);
-            base = fp.big_mask_base[b];
+            const uint32_t b0 = fp.big_mask_base[b];
+            base = (uint64_t)b0 + big_steps(f.w, f.h) <= fp.big_mask_cap ? b0 : ~0u;  // ~0u: not kept, test again
             first = offsets[slot];
         }
-        const uint32_t here = total - c0 < 64u ? total - c0 : 64u;
+        const uint32_t here = total - c0 < kBigPerWave ? total - c0 : kBigPerWave;
         for (uint32_t j = 0; j < here; ++j) {
             const uint32_t expected_j = lane_u(expected, j);
             if (expected_j == 0) {
@@ -460,30 +684,47 @@ __global__ void __launch_bounds__(kBigThreads) emit_big_kernel(const DView* __re
             const DView& V = views[g.view];
             const uint32_t base_j = lane_u(base, j);
             const uint32_t slot_j = lane_u(slot, j);
-            const uint32_t row0 = V.tile_base + g.y0 * V.tiles_x + g.x0;
+            const uint32_t corner = V.tile_base + g.y0 * V.tiles_x + g.x0;
+            const uint32_t steps = big_steps(g.w, g.h);
             uint32_t at = lane_u(first, j);
             const uint32_t stop = (uint64_t)at + expected_j < cap ? at + expected_j : cap;
-            uint64_t run = 0;  // lane r: the hit word of step r of the current run of 64 steps
-            for (uint32_t k0 = 0; k0 < g.area && at < stop; k0 += 64) {
-                const uint32_t k = k0 + lane;
-                const uint32_t step = (k0 >> 6) & 63u;
-                uint64_t mask;
-                if (base_j != ~0u) {
-                    if (step == 0) {
-                        const uint32_t word = (k0 >> 6) + lane;
-                        run = word < div_up(g.area, 64) ? fp.big_masks[(uint64_t)base_j + word] : 0ull;
+            if (base_j != ~0u) {
+                // The count's hit words, 64 steps' worth a load.
+                for (uint32_t s0 = 0; s0 < steps && at < stop; s0 += 64) {
+                    const uint64_t run = s0 + lane < steps ? fp.big_masks[(uint64_t)base_j + s0 + lane] : 0ull;
+                    const uint32_t last = steps - s0 < 64u ? steps - s0 : 64u;
+                    for (uint32_t r = 0; r < last; ++r) {
+                        const uint64_t word = lane_u64(run, r);
+                        const uint32_t s = s0 + r;
+                        uint32_t col, row;
+                        if (g.w <= 64) {
+                            const uint32_t rows = 64 / g.w;
+                            const uint32_t dr = small_row(lane, g.w);
+                            col = lane - dr * g.w;
+                            row = s * rows + dr;
+                        } else {
+                            const uint32_t chunks = div_up(g.w, 64);
+                            row = s / chunks;
+                            col = (s - row * chunks) * 64 + lane;
+                        }
+                        const uint32_t dst = at + (uint32_t)__popcll(word & lt_mask);
+                        if (((word >> lane) & 1u) && dst < stop) {
+                            keys[dst] = corner + row * V.tiles_x + col;
+                            values[dst] = slot_j;
+                        }
+                        at += (uint32_t)__popcll(word);
                     }
-                    mask = lane_u64(run, step);
-                } else {
-                    mask = __ballot(big_hit(V, g, T, k));
                 }
-                const bool hit = (mask >> lane) & 1u;
-                const uint32_t mine_at = at + (uint32_t)__popcll(mask & lt_mask);
-                if (hit && mine_at < stop) {
-                    keys[mine_at] = row0 + (k / g.w) * V.tiles_x + k % g.w;
-                    values[mine_at] = slot_j;
-                }
-                at += (uint32_t)__popcll(mask);
+            } else {
+                walk_big(g, V, T, lane, [&](uint32_t, bool, bool hit, uint32_t col, uint32_t row) {
+                    const uint64_t word = __ballot(hit);
+                    const uint32_t dst = at + (uint32_t)__popcll(word & lt_mask);
+                    if (hit && dst < stop) {
+                        keys[dst] = corner + row * V.tiles_x + col;
+                        values[dst] = slot_j;
+                    }
+                    at += (uint32_t)__popcll(word);
+                });
             }
         }
     }
@@ -499,13 +740,6 @@ __global__ void __launch_bounds__(kBigThreads) emit_big_kernel(const DView* __re
 // footprints' sizes, where a thread walking its own rectangle would hold
 // its wavefront to the largest of 64.
 constexpr uint32_t kSmallThreads = 256;
-
-// k / w and k % w for k < 64, 1 <= w <= 64: floor(k rcp(w) + 1e-3) is exact
-// there (rcp's relative error, under 2^-22, moves k/w by at most 1.6e-5,
-// and a k/w that is not whole is at least 1/64 below the next integer).
-__device__ inline uint32_t small_row(uint32_t k, uint32_t w) {
-    return (uint32_t)((float)k * __builtin_amdgcn_rcpf((float)w) + 1.0e-3f);
-}
 
 // The footprint whose run holds g: the first j with end[j] > g.
 __device__ inline uint32_t run_of(const uint32_t* ends, uint32_t g) {
@@ -530,6 +764,7 @@ __global__ void __launch_bounds__(kSmallThreads) count_small_kernel(const DView*
     __shared__ uint32_t s_end[kSmallThreads];
     __shared__ float4 s_geo[kSmallThreads];     // centre x, y, cut power, conic a
     __shared__ float4 s_shape[kSmallThreads];   // conic b, c, x0 | y0 << 16, w | view << 16 (bits)
+    __shared__ float2 s_inv[kSmallThreads];     // 1 / conic a, 1 / conic c
     __shared__ uint64_t s_mask[kSmallThreads];
     const uint32_t lane = threadIdx.x & 63u, wbase = threadIdx.x & ~63u;
     const uint32_t slot = blockIdx.x * kSmallThreads + threadIdx.x;
@@ -542,6 +777,7 @@ __global__ void __launch_bounds__(kSmallThreads) count_small_kernel(const DView*
         s_geo[threadIdx.x] = make_float4(r0.x, r0.y, r0.w, r1.x);
         s_shape[threadIdx.x] = make_float4(r1.y, r1.z, __builtin_bit_cast(float, rect.x),
                                            __builtin_bit_cast(float, w | (slot / n) << 16));
+        s_inv[threadIdx.x] = make_float2(recip_of(r1.x), recip_of(r1.z));
     }
     const uint32_t end = wave_inclusive_sum(mine, lane);
     s_end[threadIdx.x] = end;
@@ -557,12 +793,13 @@ __global__ void __launch_bounds__(kSmallThreads) count_small_kernel(const DView*
         bool hit = false;
         if (on) {
             const float4 geo = s_geo[wbase + j], shape = s_shape[wbase + j];
+            const float2 inv = s_inv[wbase + j];
             const uint32_t corner = __builtin_bit_cast(uint32_t, shape.z);
             const uint32_t wv = __builtin_bit_cast(uint32_t, shape.w);
             const uint32_t wj = wv & 0xFFFFu;
             const DView& V = views[wv >> 16];
             const uint32_t q = small_row(k, wj);
-            hit = tile_touched(V.width, V.height, T, geo.x, geo.y, geo.w, shape.x, shape.y, geo.z,
+            hit = tile_touched(V.width, V.height, T, geo.x, geo.y, geo.w, shape.x, shape.y, inv.x, inv.y, geo.z,
                                (corner & 0xFFFFu) + (k - q * wj), (corner >> 16) + q);
         }
         // The first lane of each footprint's run in this step folds the
@@ -723,18 +960,31 @@ __global__ void __launch_bounds__(256) counters_kernel(const uint32_t* __restric
 }  // namespace
 
 hipError_t launch_project(const ProjectArgs& a, const DView* views_dev, uint32_t nviews, float* records,
-                          uint32_t* touched, uint32_t* depth_keys, const Footprints& fp, hipStream_t s) {
-    hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, s, fp.counters, 0u, (uint64_t)2);
+                          uint32_t* touched, uint32_t* depth_keys, const Footprints& fp, uint32_t* status_words,
+                          hipStream_t s) {
     if (a.n == 0 || nviews == 0) {
-        return hipGetLastError();
+        return launch_big_list(fp, 0, status_words, s);
     }
     const dim3 grid(div_up(a.n, 256));
-    auto kernel = a.sh_half ? (a.relit ? project_kernel<true, true> : project_kernel<true, false>)
-                            : (a.relit ? project_kernel<false, true> : project_kernel<false, false>);
+    // 16-byte loads of whole coefficient rows where the rows are 16 coefficients
+    // a channel and the array is 16-byte aligned (torch's allocations are).
+    const void* shp = a.sh_half ? static_cast<const void*>(a.sh_half) : static_cast<const void*>(a.sh);
+    const bool vec = !a.packed && a.sh_coeffs == 16 && (reinterpret_cast<uintptr_t>(shp) & 15u) == 0;
+    using Kernel = void (*)(ProjectArgs, const DView*, uint32_t, float*, uint32_t*, uint32_t*, Footprints);
+    const Kernel kernels[5][2] = {{project_kernel<kInSoa32, false>, project_kernel<kInSoa32, true>},
+                                  {project_kernel<kInSoa32Vec, false>, project_kernel<kInSoa32Vec, true>},
+                                  {project_kernel<kInSoa16, false>, project_kernel<kInSoa16, true>},
+                                  {project_kernel<kInSoa16Vec, false>, project_kernel<kInSoa16Vec, true>},
+                                  {project_kernel<kInPacked, false>, project_kernel<kInPacked, true>}};
+    const uint32_t in = a.packed ? kInPacked : (a.sh_half ? 2u : 0u) + (vec ? 1u : 0u);
+    const Kernel kernel = kernels[in][a.relit ? 1 : 0];
     hipLaunchKernelGGL(kernel, grid, dim3(256), 0, s, a, views_dev, nviews, records, touched, depth_keys, fp);
     const uint32_t slots = a.n * nviews;
     hipLaunchKernelGGL(count_small_kernel, dim3(div_up(slots, kSmallThreads)), dim3(kSmallThreads), 0, s, views_dev,
                        a.n, slots, a.tile, records, fp, touched, depth_keys);
+    if (hipError_t e = launch_big_list(fp, slots, status_words, s); e != hipSuccess) {
+        return e;
+    }
     hipLaunchKernelGGL(count_big_kernel, dim3(kBigBlocks), dim3(kBigThreads), 0, s, views_dev, a.n, a.tile, records,
                        fp, touched, depth_keys);
     return hipGetLastError();

@@ -186,6 +186,20 @@ __device__ inline float ggx_env_conductor(float albedo, float roughness, float n
     return (F * ab.x + F * ab.y) * ggx_compensation(nv, alpha, F);
 }
 
+// packing.slang:129-135 (unpackNormal): a unit vector from its octahedral
+// word, x in the low 16 bits and y in the high, each a unorm.
+__device__ inline float3 unpack_normal(uint32_t word) {
+    const float fx = (float)(word & 0xFFFFu) / 65535.0f * 2.0f - 1.0f;
+    const float fy = (float)(word >> 16) / 65535.0f * 2.0f - 1.0f;
+    float nx = fx, ny = fy;
+    const float nz = 1.0f - fabsf(fx) - fabsf(fy);
+    const float t = rl_sat(-nz);
+    nx = nx + (nx >= 0.0f ? -t : t);
+    ny = ny + (ny >= 0.0f ? -t : t);
+    const float inv = 1.0f / sqrtf(nx * nx + ny * ny + nz * nz);
+    return make_float3(nx * inv, ny * inv, nz * inv);
+}
+
 // The relit colour of Gaussian i seen from eye (cloud space): the shading
 // normal (splatShadingNormal, splat_relight.slang:1843-1880) and the point
 // taken to the world (relightToWorld, relightNormalToWorld :1882-1915),
@@ -196,7 +210,10 @@ __device__ inline float3 relit_colour(const RelightArgs& rl, uint32_t i, const f
     if (rl.albedo) {
         albedo = make_float3(rl.albedo[i * 3 + 0], rl.albedo[i * 3 + 1], rl.albedo[i * 3 + 2]);
     }
-    const float metallic = rl.metallic[i], roughness = rl.roughness[i];
+    // splat_encoding.slang:212-213 (pbrMetallic, pbrRoughness) for a packed word.
+    const uint32_t pbr = rl.pbr ? rl.pbr[i] : 0u;
+    const float metallic = rl.pbr ? (float)(pbr & 0xFFu) / 255.0f : rl.metallic[i];
+    const float roughness = rl.pbr ? (float)((pbr >> 8) & 0xFFu) / 255.0f : rl.roughness[i];
     // The disc's axis: the shortest scale's column, towards the eye.
     const uint32_t s = sx <= sy ? (sx <= sz ? 0u : 2u) : (sy <= sz ? 1u : 2u);
     float d0 = R[0][s], d1 = R[1][s], d2 = R[2][s];
@@ -208,8 +225,14 @@ __device__ inline float3 relit_colour(const RelightArgs& rl, uint32_t i, const f
         d0 = -d0; d1 = -d1; d2 = -d2;
     }
     float c0 = d0, c1 = d1, c2 = d2;
-    if (rl.normals) {
-        const float s0 = rl.normals[i * 3 + 0], s1 = rl.normals[i * 3 + 1], s2 = rl.normals[i * 3 + 2];
+    if (rl.normals || rl.normals_packed) {
+        float s0, s1, s2;
+        if (rl.normals_packed) {
+            const float3 u = unpack_normal(rl.normals_packed[i]);
+            s0 = u.x; s1 = u.y; s2 = u.z;
+        } else {
+            s0 = rl.normals[i * 3 + 0]; s1 = rl.normals[i * 3 + 1]; s2 = rl.normals[i * 3 + 2];
+        }
         const float along = s0 * d0 + s1 * d1 + s2 * d2;
         float flip;
         if (fabsf(along) >= 0.5f) {

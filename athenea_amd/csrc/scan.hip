@@ -15,6 +15,9 @@
 //                pair: splat_gather_counts.slang:11-19 and the scan after it
 //                (TileRasterizer.cpp:1129-1137). Its length is read on the
 //                device (the compaction's total), so no host round trip.
+//   BigScan      the footprints over kBigRect tiles, listed in slot order with
+//                the first of their hit-word steps: one 64-bit sum, the count
+//                in the high half and the steps in the low.
 #include "athx_internal.h"
 
 namespace athx {
@@ -54,6 +57,7 @@ __device__ inline uint64_t wave_sum(uint64_t x) {
 }
 
 struct PlainScan {
+    using Value = uint32_t;
     const uint32_t* in;
     uint32_t* out;
     uint32_t n;
@@ -63,6 +67,7 @@ struct PlainScan {
 };
 
 struct CompactScan {
+    using Value = uint32_t;
     const uint32_t* touched;
     const uint32_t* depth_keys;
     uint32_t* keys_out;
@@ -78,7 +83,28 @@ struct CompactScan {
     }
 };
 
+struct BigScan {
+    using Value = uint64_t;
+    const uint2* rects;
+    uint32_t* list;
+    uint32_t* first_step;
+    uint32_t n;
+    __device__ uint32_t size() const { return n; }
+    __device__ uint64_t load(uint32_t i) const {
+        const uint2 r = rects[i];
+        const uint32_t w = r.y & 0xFFFFu, h = r.y >> 16;
+        return w * h > kBigRect ? (1ull << 32) | big_steps(w, h) : 0ull;
+    }
+    __device__ void store(uint32_t i, uint64_t v, uint64_t excl) const {
+        if (v != 0) {
+            list[excl >> 32] = i;
+            first_step[excl >> 32] = (uint32_t)excl;
+        }
+    }
+};
+
 struct GatherScan {
+    using Value = uint32_t;
     const uint32_t* touched;
     const uint32_t* sorted_slots;
     uint32_t* slot_offsets;
@@ -113,7 +139,7 @@ __global__ void __launch_bounds__(kThreads) scan_kernel(Policy pol, uint64_t* __
     }
     const uint64_t first = (uint64_t)part * kTile + (uint64_t)tid * kItems;
 
-    uint32_t v[kItems];
+    typename Policy::Value v[kItems];
 #pragma unroll
     for (uint32_t k = 0; k < kItems; ++k) {
         v[k] = first + k < n ? pol.load((uint32_t)(first + k)) : 0u;
@@ -226,6 +252,10 @@ hipError_t launch_compact(const uint32_t* touched, const uint32_t* depth_keys, u
                           uint32_t* slots_out, uint64_t* visible_total, uint32_t* status_words, hipStream_t s) {
     return run_scan(CompactScan{touched, depth_keys, keys_out, slots_out, slots}, slots, visible_total, status_words,
                     s);
+}
+
+hipError_t launch_big_list(const Footprints& fp, uint32_t slots, uint32_t* status_words, hipStream_t s) {
+    return run_scan(BigScan{fp.rects, fp.big_list, fp.big_mask_base, slots}, slots, fp.big_total, status_words, s);
 }
 
 hipError_t launch_gather_scan(const uint32_t* touched, const uint32_t* sorted_slots, const uint64_t* visible_dev,

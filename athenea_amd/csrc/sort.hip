@@ -67,28 +67,97 @@ __device__ inline uint32_t count_of(const uint64_t* n_dev, uint32_t cap) {
     return n < cap ? (uint32_t)n : cap;
 }
 
+// The digits of a sort over `bits` key bits from lo_bit: ceil(bits / 8)
+// passes of as near equal widths as can be (13 bits: 7 then 6), so no pass
+// has more digits than it must and each digit's run out of a partition is
+// as long as it can be.
+__host__ __device__ inline void pass_digit(uint32_t p, uint32_t lo_bit, uint32_t bits, uint32_t passes,
+                                           uint32_t& shift, uint32_t& width) {
+    const uint32_t base = bits / passes, extra = bits % passes;
+    width = base + (p < extra ? 1u : 0u);
+    shift = lo_bit + p * base + (p < extra ? p : extra);
+}
+
+// The lanes of the wavefront whose digit equals this lane's: a ballot a bit.
+__device__ inline uint64_t match_digit(uint32_t d, uint32_t width, bool ok) {
+    uint64_t m = __ballot(ok);
+#pragma unroll
+    for (uint32_t b = 0; b < 8; ++b) {
+        if (b < width) {
+            const bool bit = (d >> b) & 1u;
+            const uint64_t on = __ballot(bit);
+            m &= bit ? on : ~on;
+        }
+    }
+    return m;
+}
+
+// Every pass's digit counts in one read of the keys, a block's counts in LDS
+// and then written out as its own column (partials[digit row][block]); a
+// second kernel sums the rows. Adding 256 words a block into one histogram
+// with device-scope atomics serialises in the memory system.
+constexpr uint32_t kHistBlocks = 1024;
+
 template <typename K>
 __global__ void __launch_bounds__(kThreads) histogram_kernel(const K* __restrict__ keys,
                                                              const uint64_t* __restrict__ n_dev, uint32_t cap,
-                                                             uint32_t lo_bit, uint32_t passes,
-                                                             uint32_t* __restrict__ hist) {
+                                                             uint32_t lo_bit, uint32_t bits, uint32_t passes,
+                                                             uint32_t* __restrict__ partials) {
+    constexpr uint32_t kUnroll = 4;
     __shared__ uint32_t s_hist[kMaxPasses * kRadix];
     const uint32_t n = count_of(n_dev, cap);
     for (uint32_t i = threadIdx.x; i < passes * kRadix; i += kThreads) {
         s_hist[i] = 0;
     }
     __syncthreads();
-    for (uint32_t i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads) {
-        const K k = keys[i];
-        for (uint32_t p = 0; p < passes; ++p) {
-            atomicAdd(&s_hist[p * kRadix + (uint32_t)((k >> (lo_bit + 8 * p)) & 0xFFu)], 1u);
+    for (uint32_t base = blockIdx.x * kThreads * kUnroll; base < n; base += gridDim.x * kThreads * kUnroll) {
+        K k[kUnroll];
+#pragma unroll
+        for (uint32_t u = 0; u < kUnroll; ++u) {
+            const uint32_t i = base + u * kThreads + threadIdx.x;
+            k[u] = i < n ? keys[i] : K(0);
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < kUnroll; ++u) {
+            if (base + u * kThreads + threadIdx.x < n) {
+                for (uint32_t p = 0; p < passes; ++p) {
+                    uint32_t shift, width;
+                    pass_digit(p, lo_bit, bits, passes, shift, width);
+                    atomicAdd(&s_hist[p * kRadix + ((uint32_t)(k[u] >> shift) & ((1u << width) - 1u))], 1u);
+                }
+            }
         }
     }
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < passes * kRadix; i += kThreads) {
-        if (s_hist[i] != 0) {
-            atomicAdd(&hist[i], s_hist[i]);
+        partials[(size_t)i * gridDim.x + blockIdx.x] = s_hist[i];
+    }
+}
+
+// hist[row] = the sum of the blocks' counts in that row, a block a row.
+__global__ void __launch_bounds__(kThreads) hist_reduce_kernel(const uint32_t* __restrict__ partials,
+                                                               uint32_t blocks, uint32_t* __restrict__ hist) {
+    __shared__ uint32_t s_sum[kWaves];
+    const uint32_t* row = partials + (size_t)blockIdx.x * blocks;
+    uint32_t sum = 0;
+    for (uint32_t b = threadIdx.x; b < blocks; b += kThreads) {
+        sum += row[b];
+    }
+#pragma unroll
+    for (uint32_t d = 32; d >= 1; d >>= 1) {
+        sum += __shfl_xor(sum, d, 64);
+    }
+    if ((threadIdx.x & 63u) == 0) {
+        s_sum[threadIdx.x >> 6] = sum;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t total = 0;
+#pragma unroll
+        for (uint32_t w = 0; w < kWaves; ++w) {
+            total += s_sum[w];
         }
+        hist[blockIdx.x] = total;
     }
 }
 
@@ -97,7 +166,8 @@ __global__ void __launch_bounds__(kThreads) onesweep_kernel(const K* __restrict_
                                                             const uint32_t* __restrict__ vin,
                                                             K* __restrict__ kout, uint32_t* __restrict__ vout,
                                                             const uint64_t* __restrict__ n_dev, uint32_t cap,
-                                                            uint32_t shift, const uint32_t* __restrict__ hist,
+                                                            uint32_t shift, uint32_t width,
+                                                            const uint32_t* __restrict__ hist,
                                                             uint32_t* __restrict__ lookback,
                                                             uint32_t* __restrict__ counter) {
     __shared__ K s_keys[kPart];
@@ -142,19 +212,14 @@ __global__ void __launch_bounds__(kThreads) onesweep_kernel(const K* __restrict_
         val[j] = ok ? vin[at] : 0;
     }
     const uint64_t lt_mask = (1ull << lane) - 1ull;
+    const uint32_t dmask = (1u << width) - 1u;
     uint32_t* my_hist = s_wave_hist + wave * kRadix;
 #pragma unroll
     for (uint32_t j = 0; j < kItems; ++j) {
         const uint32_t at = wave_first + j * 64 + lane;
         const bool ok = at < n;
-        const uint32_t d = (uint32_t)(key[j] >> shift) & 0xFFu;
-        uint64_t m = __ballot(ok);
-#pragma unroll
-        for (uint32_t b = 0; b < 8; ++b) {
-            const bool bit = (d >> b) & 1u;
-            const uint64_t on = __ballot(bit);
-            m &= bit ? on : ~on;
-        }
+        const uint32_t d = (uint32_t)(key[j] >> shift) & dmask;
+        const uint64_t m = match_digit(d, width, ok);
         const uint32_t before = ok ? my_hist[d] : 0u;
         __builtin_amdgcn_wave_barrier();
         rank[j] = before + (uint32_t)__popcll(m & lt_mask);
@@ -176,13 +241,17 @@ __global__ void __launch_bounds__(kThreads) onesweep_kernel(const K* __restrict_
         s_wave_hist[w * kRadix + d] = count;
         count += c;
     }
+    // Digits past this pass's radix have no keys and no look-back words.
+    const bool digit = d <= dmask;
     uint32_t* my_word = lookback + (size_t)part * kRadix + d;
-    store_word(my_word, (part == 0 ? kFlagPrefix : kFlagAggregate) | count);
+    if (digit) {
+        store_word(my_word, (part == 0 ? kFlagPrefix : kFlagAggregate) | count);
+    }
     uint32_t unused;
     s_block_start[d] = block_exclusive_scan(count, s_tmp, unused);
     const uint32_t global_start = block_exclusive_scan(hist[d], s_tmp, unused);
     uint32_t before = 0;
-    if (part != 0) {
+    if (part != 0 && digit) {
         int32_t j = (int32_t)part - 1;
         while (true) {
             uint32_t w = load_word(lookback + (size_t)j * kRadix + d);
@@ -208,7 +277,7 @@ __global__ void __launch_bounds__(kThreads) onesweep_kernel(const K* __restrict_
     for (uint32_t j = 0; j < kItems; ++j) {
         const uint32_t at = wave_first + j * 64 + lane;
         if (at < n) {
-            const uint32_t dj = (uint32_t)(key[j] >> shift) & 0xFFu;
+            const uint32_t dj = (uint32_t)(key[j] >> shift) & dmask;
             const uint32_t pos = s_block_start[dj] + s_wave_hist[wave * kRadix + dj] + rank[j];
             s_keys[pos] = key[j];
             s_vals[pos] = val[j];
@@ -218,38 +287,34 @@ __global__ void __launch_bounds__(kThreads) onesweep_kernel(const K* __restrict_
     const uint32_t valid = n - part_first < kPart ? n - part_first : kPart;
     for (uint32_t i = tid; i < valid; i += kThreads) {
         const K k = s_keys[i];
-        const uint32_t dk = (uint32_t)(k >> shift) & 0xFFu;
+        const uint32_t dk = (uint32_t)(k >> shift) & dmask;
         const uint32_t dst = s_digit_base[dk] + (i - s_block_start[dk]);
         kout[dst] = k;
         vout[dst] = s_vals[i];
     }
 }
 
-// Clears what this frame's passes use: the histograms and tickets whole, and
-// each pass's look-back words for this frame's partitions only. Layout, as
-// SortScratch: hist[passes][256], lookback[passes][partitions(cap)][256],
-// counters[passes].
-__global__ void clear_kernel(uint32_t* __restrict__ hist, uint32_t* __restrict__ lookback,
-                             uint32_t* __restrict__ counters, uint32_t passes, const uint64_t* __restrict__ n_dev,
-                             uint32_t cap) {
+// Clears what this frame's passes use: the tickets whole, and each pass's
+// look-back words for this frame's partitions only (the histogram is written
+// whole by hist_reduce_kernel). Layout, as SortScratch:
+// lookback[passes][partitions(cap)][256], counters[passes].
+__global__ void clear_kernel(uint32_t* __restrict__ lookback, uint32_t* __restrict__ counters, uint32_t passes,
+                             const uint64_t* __restrict__ n_dev, uint32_t cap) {
     const uint32_t n = count_of(n_dev, cap);
     const uint64_t parts_cap = div_up(cap ? cap : 1, kPart);
     const uint64_t per_pass = (uint64_t)div_up(n ? n : 1, kPart) * kRadix;
     const uint64_t look = (uint64_t)passes * per_pass;
-    const uint64_t total = look + (uint64_t)passes * kRadix + passes;
+    const uint64_t total = look + passes;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (uint64_t)gridDim.x * blockDim.x) {
         if (i < look) {
             const uint64_t p = i / per_pass;
             lookback[p * parts_cap * kRadix + (i - p * per_pass)] = 0;
-        } else if (i < look + (uint64_t)passes * kRadix) {
-            hist[i - look] = 0;
         } else {
-            counters[i - look - (uint64_t)passes * kRadix] = 0;
+            counters[i - look] = 0;
         }
     }
 }
 
-// An odd number of passes leaves the result in the alternate buffers.
 template <typename K>
 __global__ void copy_back_kernel(K* __restrict__ keys, uint32_t* __restrict__ values,
                                  const K* __restrict__ kin, const uint32_t* __restrict__ vin,
@@ -273,17 +338,21 @@ hipError_t sort_impl(K* keys, uint32_t* values, const uint64_t* n_dev, uint32_t 
     const uint32_t passes = div_up(key_bits - lo_bit, 8);
     const uint32_t parts = sort_partitions(cap);
     const uint32_t spread = parts < 2048 ? parts : 2048;
-    hipLaunchKernelGGL(clear_kernel, dim3(spread), dim3(256), 0, s, sc.hist, sc.lookback, sc.counters, passes, n_dev,
-                       cap);
-    hipLaunchKernelGGL(histogram_kernel<K>, dim3(spread), dim3(kThreads), 0, s, keys, n_dev, cap, lo_bit, passes,
+    const uint32_t hist_blocks = sort_hist_blocks(cap);
+    hipLaunchKernelGGL(clear_kernel, dim3(spread), dim3(256), 0, s, sc.lookback, sc.counters, passes, n_dev, cap);
+    hipLaunchKernelGGL(histogram_kernel<K>, dim3(hist_blocks), dim3(kThreads), 0, s, keys, n_dev, cap, lo_bit,
+                       key_bits - lo_bit, passes, sc.partials);
+    hipLaunchKernelGGL(hist_reduce_kernel, dim3(passes * kRadix), dim3(kThreads), 0, s, sc.partials, hist_blocks,
                        sc.hist);
     K* kin = keys;
     uint32_t* vin = values;
     K* kout = static_cast<K*>(sc.keys_alt);
     uint32_t* vout = sc.values_alt;
     for (uint32_t p = 0; p < passes; ++p) {
+        uint32_t shift, width;
+        pass_digit(p, lo_bit, key_bits - lo_bit, passes, shift, width);
         hipLaunchKernelGGL(onesweep_kernel<K>, dim3(parts), dim3(kThreads), 0, s, kin, vin, kout, vout, n_dev, cap,
-                           lo_bit + 8 * p, sc.hist + p * kRadix, sc.lookback + (size_t)p * parts * kRadix,
+                           shift, width, sc.hist + p * kRadix, sc.lookback + (size_t)p * parts * kRadix,
                            sc.counters + p);
         K* tk = kin; kin = kout; kout = tk;
         uint32_t* tv = vin; vin = vout; vout = tv;
@@ -297,6 +366,11 @@ hipError_t sort_impl(K* keys, uint32_t* values, const uint64_t* n_dev, uint32_t 
 }  // namespace
 
 uint32_t sort_partitions(uint32_t n) { return div_up(n ? n : 1, kPart); }
+
+uint32_t sort_hist_blocks(uint32_t n) {
+    const uint32_t parts = sort_partitions(n);
+    return parts < kHistBlocks ? parts : kHistBlocks;
+}
 
 // Sorts the first *n_dev (at most cap) of (keys, values) stably by bits
 // [lo_bit, key_bits) of the key, the count read on the device. Digits below

@@ -61,6 +61,8 @@ class Relight:
     normals: torch.Tensor | None = None  # [n, 3]
     albedo: torch.Tensor | None = None  # [n, 3]
     object_to_world: tuple = (1.0, 0.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0, 0.0, 1.0, 0.0)
+    normals_packed: torch.Tensor | None = None  # [n] int32 octahedral words, read instead of normals
+    pbr: torch.Tensor | None = None  # [n] int32 pbr words, read instead of metallic and roughness
 
 
 @dataclass
@@ -90,6 +92,26 @@ class GaussianCloud:
 
         return GaussianCloud(t(cloud.means), t(cloud.log_scales), t(cloud.quats), t(cloud.opacity_logits),
                              t(cloud.sh), linear, relight.to_device(device) if relight is not None else None)
+
+
+@dataclass
+class PackedGaussians:
+    """athx_packed_splats: a cloud in athenea's packed device layout
+    (scene::GpuSplats) -- positions float4 [n, 4] (xyz, linear opacity), shape
+    words [n, 4] (smallest-three quaternion, f16 log-scales and base colour),
+    the rest coefficients as halves [n, (rest_per_colour * 3 + 1) // 2] (int32
+    tensors holding the uint32 words). Forward only."""
+
+    positions: torch.Tensor
+    shape: torch.Tensor
+    sh: torch.Tensor | None
+    rest_per_colour: int
+    linear: bool = True
+    relight: Relight | None = None
+
+    @property
+    def count(self) -> int:
+        return int(self.positions.shape[0])
 
 
 @dataclass
@@ -147,6 +169,10 @@ def _ptr(t: torch.Tensor | None) -> int | None:
     return t.data_ptr()
 
 
+def _device_of(cloud) -> torch.device:
+    return cloud.positions.device if isinstance(cloud, PackedGaussians) else cloud.means.device
+
+
 def _stream_handle(device) -> int:
     return torch.cuda.current_stream(device).cuda_stream
 
@@ -171,7 +197,7 @@ def _check_cloud(cloud: GaussianCloud) -> None:
         per = {"metallic": (r.metallic, (n,)), "roughness": (r.roughness, (n,)), "normals": (r.normals, (n, 3)),
                "albedo": (r.albedo, (n, 3))}
         for name, (t, shape) in per.items():
-            if t is None and name in ("normals", "albedo"):
+            if t is None and (name in ("normals", "albedo") or r.pbr is not None):
                 continue
             if t is None or tuple(t.shape) != shape or t.dtype != torch.float32 or not t.is_cuda:
                 raise ValueError(f"relight {name} must be a float32 device tensor of shape {shape}")
@@ -182,21 +208,45 @@ def _check_cloud(cloud: GaussianCloud) -> None:
             raise ValueError("sky sh is [16, 4] and sun [2, 4]")
 
 
-def _gaussians_c(cloud: GaussianCloud) -> _lib.Gaussians:
-    _check_cloud(cloud)
+def _check_packed(cloud: PackedGaussians) -> None:
+    n = cloud.count
+    if tuple(cloud.positions.shape) != (n, 4) or cloud.positions.dtype != torch.float32:
+        raise ValueError("packed positions must be float32 [n, 4]")
+    if tuple(cloud.shape.shape) != (n, 4) or cloud.shape.dtype not in (torch.int32, torch.uint32):
+        raise ValueError("packed shape must be 32-bit words [n, 4]")
+    if cloud.rest_per_colour not in (0, 3, 8, 15):
+        raise ValueError("rest_per_colour must be 0, 3, 8 or 15")
+    words = (cloud.rest_per_colour * 3 + 1) // 2
+    if cloud.rest_per_colour and (cloud.sh is None or tuple(cloud.sh.shape) != (n, words)):
+        raise ValueError(f"packed sh must be 32-bit words [n, {words}]")
+
+
+def _gaussians_c(cloud) -> _lib.Gaussians:
     g = _lib.Gaussians()
     g.count = cloud.count
-    g.sh_coeffs = int(cloud.sh.shape[1])
-    g.means = _ptr(cloud.means)
-    g.log_scales = _ptr(cloud.log_scales)
-    g.quats = _ptr(cloud.quats)
-    g.opacity_logits = _ptr(cloud.opacity_logits)
-    if cloud.sh.dtype == torch.float16:
-        g.sh = None
-        g.sh_half = _ptr(cloud.sh)
+    if isinstance(cloud, PackedGaussians):
+        _check_packed(cloud)
+        p = _lib.PackedSplats()
+        p.positions = _ptr(cloud.positions)
+        p.shape = _ptr(cloud.shape)
+        p.sh = _ptr(cloud.sh)
+        p.rest_per_colour = cloud.rest_per_colour
+        g.sh_coeffs = cloud.rest_per_colour + 1
+        g.packed = ctypes.pointer(p)
+        g._packed = p
     else:
-        g.sh = _ptr(cloud.sh)
-        g.sh_half = None
+        _check_cloud(cloud)
+        g.sh_coeffs = int(cloud.sh.shape[1])
+        g.means = _ptr(cloud.means)
+        g.log_scales = _ptr(cloud.log_scales)
+        g.quats = _ptr(cloud.quats)
+        g.opacity_logits = _ptr(cloud.opacity_logits)
+        if cloud.sh.dtype == torch.float16:
+            g.sh = None
+            g.sh_half = _ptr(cloud.sh)
+        else:
+            g.sh = _ptr(cloud.sh)
+            g.sh_half = None
     r = cloud.relight
     if r is not None:
         c = _lib.Relight()
@@ -204,6 +254,8 @@ def _gaussians_c(cloud: GaussianCloud) -> _lib.Gaussians:
         c.albedo = _ptr(r.albedo)
         c.metallic = _ptr(r.metallic)
         c.roughness = _ptr(r.roughness)
+        c.normals_packed = _ptr(r.normals_packed)
+        c.pbr = _ptr(r.pbr)
         for k in range(12):
             c.object_to_world[k] = float(r.object_to_world[k])
         c.sky.texels = _ptr(r.sky.texels)
@@ -255,7 +307,7 @@ class FrameGraph:
                  device: int = 0):
         self._lib = _lib.load()
         self.settings = settings or RenderSettings()
-        dev = cloud.means.device
+        dev = _device_of(cloud)
         backward = dl_dcolour is not None and grads is not None
         self.targets = targets or [RenderTargets.allocate(v.width, v.height, dev, backward) for v in views]
         self.cloud = cloud
@@ -278,10 +330,10 @@ class FrameGraph:
         self._graph = handle
 
     def launch(self) -> None:
-        _lib.check(self._lib.athx_graph_launch(self._graph, _stream_handle(self.cloud.means.device)))
+        _lib.check(self._lib.athx_graph_launch(self._graph, _stream_handle(_device_of(self.cloud))))
 
     def launch_backward(self) -> None:
-        _lib.check(self._lib.athx_graph_launch_backward(self._graph, _stream_handle(self.cloud.means.device)))
+        _lib.check(self._lib.athx_graph_launch_backward(self._graph, _stream_handle(_device_of(self.cloud))))
 
     def pairs(self) -> tuple[int, int]:
         """(pair total, capacity) of the last replay; waits for it."""
@@ -340,7 +392,7 @@ class TileRasterizer:
         """Draws ``cloud`` once per view (a list of athx View structs); all
         views go through one projection, one sort and one blend launch."""
         settings = settings or RenderSettings()
-        dev = cloud.means.device
+        dev = _device_of(cloud)
         if targets is None:
             targets = [RenderTargets.allocate(v.width, v.height, dev, keep_backward) for v in views]
         if len(targets) != len(views):
@@ -380,7 +432,7 @@ class TileRasterizer:
         sized for ``pair_capacity`` (0: 1.5x the largest frame seen). Check
         ``frame_pairs()`` for a frame that outgrew its capacity."""
         settings = settings or RenderSettings()
-        dev = cloud.means.device
+        dev = _device_of(cloud)
         if targets is None:
             targets = [RenderTargets.allocate(v.width, v.height, dev, keep_backward) for v in views]
         c_views, c_targets = _views_and_targets(views, targets)
@@ -433,7 +485,7 @@ class TileRasterizer:
         s = settings.to_c(cloud.linear)
         self._check(
             self._lib.athx_render_backward(self._ctx, c_views, nv, ctypes.byref(g), ctypes.byref(s), c_targets, dl,
-                                           ctypes.byref(cg), _stream_handle(cloud.means.device))
+                                           ctypes.byref(cg), _stream_handle(_device_of(cloud)))
         )
         # ``keep`` may hold temporaries the queued kernels read; PyTorch's
         # allocator is stream-ordered, so releasing them here is safe.

@@ -148,9 +148,39 @@ typedef struct athx_relight {
     const float* albedo;     /* [count][3], or NULL: the evaluated SH colour is the albedo (splat_project.slang:819) */
     const float* metallic;   /* [count] */
     const float* roughness;  /* [count] */
+    const uint32_t* normals_packed; /* [count] GpuSplats::normals (octahedral, 2 x 16-bit unorms,
+                                       packing.slang:116-135), or NULL: normals is read */
+    const uint32_t* pbr;     /* [count] GpuSplats' pbr words (metallic in byte 0, roughness in byte 1,
+                                scene/splat_encoding.slang:189-214), or NULL: metallic and roughness */
     float object_to_world[12]; /* rows of a 3x4 */
     athx_sky sky;
 } athx_relight;
+
+/*
+ * athenea's packed device layout of a cloud (scene::GpuSplats,
+ * modules/scene/include/athenea/scene/GpuClouds.h:44-52; the field layout of
+ * shaders/athenea/common/packing.slang:1-26), device memory, 124 bytes a
+ * Gaussian at degree 3 where the FP32 arrays take 236:
+ *   positions  float4 [count]: x, y, z, opacity (linear, 0..1)
+ *   shape      uint32 [count][4]:
+ *              [0] rotation, smallest three: components in bits 22-31, 12-21
+ *                  and 2-11 (unorms of [-sqrt(1/2), sqrt(1/2)]), the index
+ *                  of the dropped largest one in bits 0-1
+ *              [1] f16 log scale x | f16 log scale y << 16
+ *              [2] f16 log scale z | f16 colour red << 16
+ *              [3] f16 colour green | f16 colour blue << 16
+ *   sh         uint32 [count][(rest_per_colour * 3 + 1) / 2]: the rest
+ *              coefficients as IEEE halves, low half first, rgb per basis
+ *              function (common/sh.slang:15-22); NULL at rest_per_colour 0
+ * The colour is linear or sRGB as athx_settings.linear says. The quaternion
+ * is used as decoded (unit to 10 bits), not normalised again.
+ */
+typedef struct athx_packed_splats {
+    const float* positions;
+    const uint32_t* shape;
+    const uint32_t* sh;
+    uint32_t rest_per_colour; /* 0, 3, 8 or 15 */
+} athx_packed_splats;
 
 typedef struct athx_gaussians {
     uint32_t count;
@@ -162,6 +192,8 @@ typedef struct athx_gaussians {
     const float* sh;             /* [count][sh_coeffs][3] or NULL */
     const uint16_t* sh_half;     /* [count][sh_coeffs][3] or NULL */
     const athx_relight* relight; /* NULL: the SH colour; else the relit colour (forward only) */
+    const athx_packed_splats* packed; /* non-NULL: the packed layout is read instead of the arrays
+                                         above, which may then be NULL (forward only) */
 } athx_gaussians;
 
 /*

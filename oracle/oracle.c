@@ -116,20 +116,23 @@ static float paid_opacity(float alpha, float paid, uint32_t* flags) {
     return spread;
 }
 
-/* frame.slang:354-371 */
+/* frame.slang:354-371, the edge minimiser ((-b) x) / c taken as
+ * ((-b) x) * (1 / c): the reference's divide has no defined rounding across
+ * its backends, and this is the form the kernels compute too. */
 static float min_power_over_box(float a, float b, float c, float lox, float loy, float hix, float hiy) {
     if (lox <= 0.0f && hix >= 0.0f && loy <= 0.0f && hiy >= 0.0f) {
         return 0.0f;
     }
+    const float ia = a > 1e-12f ? 1.0f / a : 0.0f, ic = c > 1e-12f ? 1.0f / c : 0.0f;
     float best = 3.4e38f;
     for (int e = 0; e < 2; ++e) {
         const float x = e == 0 ? lox : hix;
-        const float y = c > 1e-12f ? clampf_(((-b) * x) / c, loy, hiy) : loy;
+        const float y = c > 1e-12f ? clampf_(((-b) * x) * ic, loy, hiy) : loy;
         best = minf_(best, a * x * x + 2.0f * b * x * y + c * y * y);
     }
     for (int e = 0; e < 2; ++e) {
         const float y = e == 0 ? loy : hiy;
-        const float x = a > 1e-12f ? clampf_(((-b) * y) / a, lox, hix) : lox;
+        const float x = a > 1e-12f ? clampf_(((-b) * y) * ia, lox, hix) : lox;
         best = minf_(best, a * x * x + 2.0f * b * x * y + c * y * y);
     }
     return best;
@@ -226,6 +229,8 @@ typedef struct orc_relight {
     const float* sky_sh;    /* 16 float4: radiance coefficients */
     const float* sky_sun;   /* 2 float4: direction + disc size (w > 0: a sun), irradiance */
     uint32_t base_side;
+    const uint32_t* normals_packed; /* [n] octahedral words (packing.slang:116-135), or NULL: normals */
+    const uint32_t* pbr;            /* [n] metallic byte 0, roughness byte 1 (splat_encoding.slang:212-213), or NULL */
 } orc_relight;
 
 #define ENV_LEVELS 8u
@@ -429,6 +434,8 @@ static float ggx_env_conductor(float albedo, float roughness, float nv) {
     return (F * a + F * b) * ggx_compensation(nv, alpha, F);
 }
 
+void orc_unpack_normal(uint32_t word, float out[3]);
+
 /* splat_relight.slang:1860-1880 (splatShadingNormal with splatNormal :1843-1850) */
 static void shading_normal(const float R[3][3], float sx, float sy, float sz, const float p[3], const float eye[3],
                            const float* stored, float out[3]) {
@@ -464,9 +471,17 @@ static void relit_colour(const orc_relight* rl, uint32_t i, const float R[3][3],
     if (rl->albedo) {
         albedo[0] = rl->albedo[i * 3 + 0]; albedo[1] = rl->albedo[i * 3 + 1]; albedo[2] = rl->albedo[i * 3 + 2];
     }
-    const float metallic = rl->metallic[i], roughness = rl->roughness[i];
-    float nc[3];
-    shading_normal(R, sx, sy, sz, p, eye_obj, rl->normals ? rl->normals + (size_t)i * 3 : NULL, nc);
+    const float metallic = rl->pbr ? (float)(rl->pbr[i] & 0xFFu) / 255.0f : rl->metallic[i];
+    const float roughness = rl->pbr ? (float)((rl->pbr[i] >> 8) & 0xFFu) / 255.0f : rl->roughness[i];
+    float nc[3], unpacked[3];
+    const float* stored = NULL;
+    if (rl->normals_packed) {
+        orc_unpack_normal(rl->normals_packed[i], unpacked);
+        stored = unpacked;
+    } else if (rl->normals) {
+        stored = rl->normals + (size_t)i * 3;
+    }
+    shading_normal(R, sx, sy, sz, p, eye_obj, stored, nc);
     /* To the world (relightToWorld, relightNormalToWorld :1882-1915). */
     const float* w = rl->o2w;
     const float pw[3] = {w[0] * p[0] + w[1] * p[1] + w[2] * p[2] + w[3] * 1.0f,
@@ -517,17 +532,91 @@ static void relit_colour(const orc_relight* rl, uint32_t i, const float R[3][3],
     }
 }
 
+/* A Gaussian as the projection reads it, from either input layout. */
+typedef struct orc_gaussian {
+    float p[3], s[3], q[4]; /* position, scales (exp'd), rotation (x, y, z, w) as used */
+    float opacity;          /* linear */
+    float base[3];          /* the colour before the harmonics' rest */
+    float row[48];          /* row[k * 3 + c]: coefficient k >= 1 of channel c */
+    uint32_t coeffs;        /* coefficients a channel, constant term included */
+} orc_gaussian;
+
+/* The FP32 arrays: exp of the log-scales, sigmoid of the logit, the
+ * quaternion normalised, base 0.5 + SH0 dc. */
+static void decode_soa(const float* P, const float* LS, const float* Q, const float* LG, const float* SH,
+                       uint32_t coeffs, uint32_t i, orc_gaussian* g) {
+    for (int k = 0; k < 3; ++k) {
+        g->p[k] = P[i * 3 + k];
+        g->s[k] = athx_expf(LS[i * 3 + k]);
+    }
+    g->opacity = athx_sigmoidf(LG[i]);
+    float qx = Q[i * 4 + 0], qy = Q[i * 4 + 1], qz = Q[i * 4 + 2], qw = Q[i * 4 + 3];
+    const float qn = qx * qx + qy * qy + qz * qz + qw * qw;
+    if (qn > 0.0f) {
+        const float inv = 1.0f / sqrtf(qn);
+        qx = qx * inv; qy = qy * inv; qz = qz * inv; qw = qw * inv;
+    } else {
+        qx = 0.0f; qy = 0.0f; qz = 0.0f; qw = 1.0f;
+    }
+    g->q[0] = qx; g->q[1] = qy; g->q[2] = qz; g->q[3] = qw;
+    const float* sh = SH + (size_t)i * coeffs * 3;
+    for (int c = 0; c < 3; ++c) g->base[c] = 0.5f + kSH0 * sh[c];
+    memset(g->row, 0, sizeof g->row);
+    for (uint32_t k = 3; k < coeffs * 3; ++k) g->row[k] = sh[k];
+    g->coeffs = coeffs;
+}
+
+/* packing.slang:59-76 (decodeQuaternion), used as decoded. */
+static void decode_quaternion(uint32_t word, float q[4]) {
+    const float kSqrtHalf = 0.70710678f;
+    const uint32_t largest = word & 3u;
+    const float s0 = ((float)((word >> 22) & 1023u) / 1023.0f * 2.0f - 1.0f) * kSqrtHalf;
+    const float s1 = ((float)((word >> 12) & 1023u) / 1023.0f * 2.0f - 1.0f) * kSqrtHalf;
+    const float s2 = ((float)((word >> 2) & 1023u) / 1023.0f * 2.0f - 1.0f) * kSqrtHalf;
+    const float rest = sqrtf(maxf_(0.0f, 1.0f - (s0 * s0 + s1 * s1 + s2 * s2)));
+    const float small[3] = {s0, s1, s2};
+    uint32_t next = 0;
+    for (uint32_t k = 0; k < 4; ++k) {
+        if (k == largest) {
+            q[k] = rest;
+        } else {
+            q[k] = small[next++];
+        }
+    }
+}
+
+/* athenea's packed layout (packing.slang:1-26): positions float4 (xyz,
+ * linear opacity); shape words (quaternion; f16 log-scales; f16 base
+ * colour); the rest coefficients as halves, rgb per basis (sh.slang:15-22). */
+static void decode_packed(const float* positions, const uint32_t* shape, const uint32_t* words, uint32_t rest,
+                          uint32_t i, orc_gaussian* g) {
+    for (int k = 0; k < 3; ++k) g->p[k] = positions[i * 4 + k];
+    g->opacity = positions[i * 4 + 3];
+    const uint32_t* w = shape + (size_t)i * 4;
+    g->s[0] = athx_expf(half_to_float(w[1] & 0xFFFFu));
+    g->s[1] = athx_expf(half_to_float(w[1] >> 16));
+    g->s[2] = athx_expf(half_to_float(w[2] & 0xFFFFu));
+    decode_quaternion(w[0], g->q);
+    g->base[0] = half_to_float(w[2] >> 16);
+    g->base[1] = half_to_float(w[3] & 0xFFFFu);
+    g->base[2] = half_to_float(w[3] >> 16);
+    memset(g->row, 0, sizeof g->row);
+    const uint32_t nwords = (rest * 3 + 1) / 2;
+    for (uint32_t h = 0; h < rest * 3; ++h) {
+        const uint32_t word = words[(size_t)i * nwords + h / 2];
+        g->row[3 + h] = half_to_float((h & 1u) ? word >> 16 : word & 0xFFFFu);
+    }
+    g->coeffs = rest + 1;
+}
+
 /* One splat projected: splat_project.slang:497-1130 with kTransfer 0. Writes
  * rec, touched and key for slot i; returns the cull reason (0 = visible). */
-static uint32_t project_one(const orc_view* v, const float* P, const float* LS, const float* Q, const float* LG,
-                            const float* SH, uint32_t coeffs, uint32_t i, float* rec, uint32_t* touched,
+static uint32_t project_one(const orc_view* v, const orc_gaussian* g, uint32_t i, float* rec, uint32_t* touched,
                             uint32_t* key, uint32_t* flags_out, const orc_relight* rl) {
     uint32_t flags = 0;
-    const float px = P[i * 3 + 0], py = P[i * 3 + 1], pz = P[i * 3 + 2];
-    const float sx = athx_expf(LS[i * 3 + 0]);
-    const float sy = athx_expf(LS[i * 3 + 1]);
-    const float sz = athx_expf(LS[i * 3 + 2]);
-    const float opacity = athx_sigmoidf(LG[i]);
+    const float px = g->p[0], py = g->p[1], pz = g->p[2];
+    const float sx = g->s[0], sy = g->s[1], sz = g->s[2];
+    const float opacity = g->opacity;
     const float* m = v->m;
     const float tx = m[0] * px + m[1] * py + m[2] * pz + m[3];
     const float ty = m[4] * px + m[5] * py + m[6] * pz + m[7];
@@ -538,15 +627,7 @@ static uint32_t project_one(const orc_view* v, const float* P, const float* LS, 
         *key = CULLED_KEY | CULL_DEPTH;
         return CULL_DEPTH;
     }
-    /* Quaternion (x, y, z, w), normalised (packing.slang:59-76 decodes a unit one). */
-    float qx = Q[i * 4 + 0], qy = Q[i * 4 + 1], qz = Q[i * 4 + 2], qw = Q[i * 4 + 3];
-    const float qn = qx * qx + qy * qy + qz * qz + qw * qw;
-    if (qn > 0.0f) {
-        const float inv = 1.0f / sqrtf(qn);
-        qx = qx * inv; qy = qy * inv; qz = qz * inv; qw = qw * inv;
-    } else {
-        qx = 0.0f; qy = 0.0f; qz = 0.0f; qw = 1.0f;
-    }
+    const float qx = g->q[0], qy = g->q[1], qz = g->q[2], qw = g->q[3];
     const float r00 = 1.0f - 2.0f * (qy * qy + qz * qz), r01 = 2.0f * (qx * qy - qw * qz), r02 = 2.0f * (qx * qz + qw * qy);
     const float r10 = 2.0f * (qx * qy + qw * qz), r11 = 1.0f - 2.0f * (qx * qx + qz * qz), r12 = 2.0f * (qy * qz - qw * qx);
     const float r20 = 2.0f * (qx * qz - qw * qy), r21 = 2.0f * (qy * qz + qw * qx), r22 = 1.0f - 2.0f * (qx * qx + qy * qy);
@@ -635,15 +716,14 @@ static uint32_t project_one(const orc_view* v, const float* P, const float* LS, 
         return CULL_NO_TILE;
     }
     /* Colour from this eye, in the cloud's space (splat_project.slang:781-785). */
-    const float* sh = SH + (size_t)i * coeffs * 3;
-    const uint32_t degree = degree_of(coeffs) < v->sh_limit ? degree_of(coeffs) : v->sh_limit;
+    const float* sh = g->row;
+    const uint32_t degree = degree_of(g->coeffs) < v->sh_limit ? degree_of(g->coeffs) : v->sh_limit;
     const float dx = px - v->eye[0], dy = py - v->eye[1], dz = pz - v->eye[2];
     const float dinv = 1.0f / sqrtf(dx * dx + dy * dy + dz * dz);
     const float ex = dx * dinv, ey = dy * dinv, ez = dz * dinv;
     float col[3];
     for (uint32_t ch = 0; ch < 3; ++ch) {
-        const float base = 0.5f + kSH0 * sh[ch];
-        const float cval = base + sh_rest(sh, ch, degree, ex, ey, ez);
+        const float cval = g->base[ch] + sh_rest(sh, ch, degree, ex, ey, ez);
         if (v->linear) {
             if (cval > 0.0f) flags |= (FL_COL_R << ch);
             col[ch] = maxf_(cval, 0.0f);
@@ -686,10 +766,38 @@ ORC_EXPORT void orc_project_lit(const orc_view* v, const float* means, const flo
                                 uint32_t* touched, uint32_t* keys, uint32_t* flags, const orc_relight* rl) {
 #pragma omp parallel for schedule(dynamic, 4096)
     for (int64_t i = 0; i < (int64_t)n; ++i) {
-        project_one(v, means, log_scales, quats, logits, sh, coeffs, (uint32_t)i, rec, touched + i, keys + i,
-                    flags ? flags + i : NULL, rl);
+        orc_gaussian g;
+        decode_soa(means, log_scales, quats, logits, sh, coeffs, (uint32_t)i, &g);
+        project_one(v, &g, (uint32_t)i, rec, touched + i, keys + i, flags ? flags + i : NULL, rl);
     }
 }
+
+/* The same from athenea's packed layout. */
+ORC_EXPORT void orc_project_packed(const orc_view* v, const float* positions, const uint32_t* shape,
+                                   const uint32_t* words, uint32_t rest, uint32_t n, float* rec, uint32_t* touched,
+                                   uint32_t* keys, uint32_t* flags, const orc_relight* rl) {
+#pragma omp parallel for schedule(dynamic, 4096)
+    for (int64_t i = 0; i < (int64_t)n; ++i) {
+        orc_gaussian g;
+        decode_packed(positions, shape, words, rest, (uint32_t)i, &g);
+        project_one(v, &g, (uint32_t)i, rec, touched + i, keys + i, flags ? flags + i : NULL, rl);
+    }
+}
+
+/* packing.slang:129-135 (unpackNormal), for the tests. */
+ORC_EXPORT void orc_unpack_normal(uint32_t word, float out[3]) {
+    const float fx = (float)(word & 0xFFFFu) / 65535.0f * 2.0f - 1.0f;
+    const float fy = (float)(word >> 16) / 65535.0f * 2.0f - 1.0f;
+    float nx = fx, ny = fy;
+    const float nz = 1.0f - fabsf(fx) - fabsf(fy);
+    const float t = satf_(-nz);
+    nx = nx + (nx >= 0.0f ? -t : t);
+    ny = ny + (ny >= 0.0f ? -t : t);
+    const float inv = 1.0f / sqrtf(nx * nx + ny * ny + nz * nz);
+    out[0] = nx * inv; out[1] = ny * inv; out[2] = nz * inv;
+}
+
+ORC_EXPORT void orc_decode_quaternion(uint32_t word, float q[4]) { decode_quaternion(word, q); }
 
 ORC_EXPORT void orc_project(const orc_view* v, const float* means, const float* log_scales, const float* quats,
                             const float* logits, const float* sh, uint32_t coeffs, uint32_t n, float* rec,

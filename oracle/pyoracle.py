@@ -61,6 +61,8 @@ class OrcRelight(ctypes.Structure):
         ("sky_sh", ctypes.c_void_p),
         ("sky_sun", ctypes.c_void_p),
         ("base_side", ctypes.c_uint32),
+        ("normals_packed", ctypes.c_void_p),
+        ("pbr", ctypes.c_void_p),
     ]
 
 
@@ -90,6 +92,8 @@ def orc_relight(relight) -> tuple[OrcRelight, list]:
     r.sky_sh = ptr(relight.sky.sh, np.float32)
     r.sky_sun = ptr(relight.sky.sun, np.float32)
     r.base_side = int(relight.sky.base_side)
+    r.normals_packed = ptr(getattr(relight, "normals_packed", None), np.uint32)
+    r.pbr = ptr(getattr(relight, "pbr", None), np.uint32)
     return r, keep
 
 
@@ -116,6 +120,13 @@ def load():
         lib.orc_project_lit.argtypes = [vp, _f32p, _f32p, _f32p, _f32p, _f32p, ctypes.c_uint32, ctypes.c_uint32,
                                         _f32p, _u32p, _u32p, ctypes.c_void_p, rp]
         lib.orc_project_lit.restype = None
+        lib.orc_project_packed.argtypes = [vp, _f32p, _u32p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32,
+                                           _f32p, _u32p, _u32p, ctypes.c_void_p, rp]
+        lib.orc_project_packed.restype = None
+        lib.orc_unpack_normal.argtypes = [ctypes.c_uint32, f3]
+        lib.orc_unpack_normal.restype = None
+        lib.orc_decode_quaternion.argtypes = [ctypes.c_uint32, ctypes.c_float * 4]
+        lib.orc_decode_quaternion.restype = None
         lib.orc_relit_point.argtypes = [rp, f3, f3, f3, ctypes.c_float, ctypes.c_float, f3]
         lib.orc_relit_point.restype = None
         lib.orc_env_radiance.argtypes = [rp, f3, ctypes.c_float, f3]
@@ -213,13 +224,27 @@ def project(v: OrcView, cloud, flags: bool = False, relight=None):
     """Records, tile counts and depth keys of every Gaussian; ``relight``
     (see orc_relight) swaps the harmonics' colour for the relit one."""
     lib = load()
+    rl, keep = orc_relight(relight) if relight is not None else (None, None)
+    if hasattr(cloud, "shape") and hasattr(cloud, "positions"):
+        # athenea's packed layout (athenea_amd.packing.PackedCloud)
+        n = cloud.positions.shape[0]
+        rec = np.zeros((n, 12), np.float32)
+        touched = np.zeros(n, np.uint32)
+        keys = np.zeros(n, np.uint32)
+        fl = np.zeros(n, np.uint32) if flags else None
+        words = _c(cloud.sh, np.uint32) if cloud.sh is not None else None
+        lib.orc_project_packed(ctypes.byref(v), _c(cloud.positions, np.float32), _c(cloud.shape, np.uint32),
+                               words.ctypes.data if words is not None else None, cloud.rest_per_colour, n, rec,
+                               touched, keys, fl.ctypes.data if flags else None,
+                               ctypes.byref(rl) if rl is not None else None)
+        del keep
+        return (rec, touched, keys, fl) if flags else (rec, touched, keys)
     n = cloud.means.shape[0]
     coeffs = cloud.sh.shape[1]
     rec = np.zeros((n, 12), np.float32)
     touched = np.zeros(n, np.uint32)
     keys = np.zeros(n, np.uint32)
     fl = np.zeros(n, np.uint32) if flags else None
-    rl, keep = orc_relight(relight) if relight is not None else (None, None)
     lib.orc_project_lit(ctypes.byref(v), _c(cloud.means, np.float32), _c(cloud.log_scales, np.float32),
                         _c(cloud.quats, np.float32), _c(cloud.opacity_logits, np.float32), _c(cloud.sh, np.float32),
                         coeffs, n, rec, touched, keys, fl.ctypes.data if flags else None,
@@ -242,10 +267,12 @@ def sort_pairs(keys: np.ndarray, values: np.ndarray, bits: int) -> tuple[np.ndar
     return k, v
 
 
-def render(views: list[OrcView], cloud, with_f64: bool = False, relight=None) -> dict:
-    """The frame as the HIP path computes it, all views in one key space."""
+def render(views: list[OrcView], cloud, with_f64: bool = False, relight=None, f64_cloud=None) -> dict:
+    """The frame as the HIP path computes it, all views in one key space.
+    ``f64_cloud``: the FP32 arrays the FP64 replay reads (a packed cloud's
+    decoded fields); the cloud itself by default."""
     lib = load()
-    n = cloud.means.shape[0]
+    n = cloud.positions.shape[0] if hasattr(cloud, "positions") else cloud.means.shape[0]
     nv = len(views)
     rec = np.zeros((nv * n, 12), np.float32)
     touched = np.zeros(nv * n, np.uint32)
@@ -296,7 +323,7 @@ def render(views: list[OrcView], cloud, with_f64: bool = False, relight=None) ->
         out["final_t"].append(ft.reshape(H, W))
         out["last"].append(last.reshape(H, W))
     if with_f64:
-        out.update(render_f64(views, cloud, out))
+        out.update(render_f64(views, f64_cloud if f64_cloud is not None else cloud, out))
     return out
 
 
@@ -471,3 +498,15 @@ def fresnel_dielectric(c: float, ior: float) -> float:
 
 def fresnel_conductor(albedo: float, nv: float) -> float:
     return float(load().orc_fresnel_conductor(albedo, nv))
+
+
+def unpack_normal(word: int) -> np.ndarray:
+    out = (ctypes.c_float * 3)()
+    load().orc_unpack_normal(word, out)
+    return np.array(out[:], np.float32)
+
+
+def decode_quaternion(word: int) -> np.ndarray:
+    out = (ctypes.c_float * 4)()
+    load().orc_decode_quaternion(word, out)
+    return np.array(out[:], np.float32)

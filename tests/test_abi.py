@@ -50,6 +50,7 @@ STRUCTS = {
     "athx_gaussians": _lib.Gaussians,
     "athx_sky": _lib.Sky,
     "athx_relight": _lib.Relight,
+    "athx_packed_splats": _lib.PackedSplats,
     "athx_settings": _lib.Settings,
     "athx_targets": _lib.Targets,
     "athx_frame_stats": _lib.FrameStats,
