@@ -164,15 +164,22 @@ struct SortScratch {
     uint32_t* lookback;  // passes * partitions * 256
     uint32_t* counters;  // passes
     uint32_t* partials;  // passes * 256 * sort_hist_blocks: each histogram block's counts
+    uint32_t* counts;    // 256 * partitions: a pass's digit counts, digit-major
+    uint32_t* offsets;   // 256 * partitions: their exclusive scan
+    uint32_t* scan_status;  // scan_status_words(256 * partitions)
+    uint64_t* scan_total;
 };
 uint32_t sort_partitions(uint32_t n);
 uint32_t sort_hist_blocks(uint32_t n);
 // Sorts the first min(*n_dev, cap) pairs stably by key bits [lo_bit,
 // key_bits); scratch is sized for cap and (key_bits - lo_bit) / 8 passes.
+// in_alt non-null: an odd number of passes leaves the result in the alternate
+// buffers (sc.keys_alt, sc.values_alt) and *in_alt says so, where it would
+// otherwise be copied back.
 hipError_t launch_sort(uint64_t* keys, uint32_t* values, const uint64_t* n_dev, uint32_t cap, uint32_t lo_bit,
-                       uint32_t key_bits, const SortScratch& sc, hipStream_t s);
+                       uint32_t key_bits, const SortScratch& sc, hipStream_t s, bool* in_alt = nullptr);
 hipError_t launch_sort(uint32_t* keys, uint32_t* values, const uint64_t* n_dev, uint32_t cap, uint32_t lo_bit,
-                       uint32_t key_bits, const SortScratch& sc, hipStream_t s);
+                       uint32_t key_bits, const SortScratch& sc, hipStream_t s, bool* in_alt = nullptr);
 // The visible slots in slot order with their depth keys; the count lands in
 // *visible_total (device).
 hipError_t launch_compact(const uint32_t* touched, const uint32_t* depth_keys, uint32_t slots, uint32_t* keys_out,

@@ -344,8 +344,10 @@ double elapsed_ms(hipEvent_t a, hipEvent_t b) {
 // (passes * partitions * 256), counters (passes), the histogram blocks'
 // partial counts (passes * 256 * blocks), back to back.
 size_t sort_meta_words(uint32_t n, uint32_t passes) {
+    const size_t digit_cells = (size_t)256 * sort_partitions(n);
     return (size_t)passes * 256 + (size_t)passes * sort_partitions(n) * 256 + passes +
-           (size_t)passes * 256 * sort_hist_blocks(n);
+           (size_t)passes * 256 * sort_hist_blocks(n) + 2 * digit_cells + scan_status_words((uint32_t)digit_cells) +
+           4;
 }
 
 SortScratch sort_scratch(const DeviceBuffer& meta, void* alt_keys, uint32_t* alt_vals, uint32_t n,
@@ -357,6 +359,16 @@ SortScratch sort_scratch(const DeviceBuffer& meta, void* alt_keys, uint32_t* alt
     sc.lookback = sc.hist + (size_t)passes * 256;
     sc.counters = sc.lookback + (size_t)passes * sort_partitions(n) * 256;
     sc.partials = sc.counters + passes;
+    const size_t digit_cells = (size_t)256 * sort_partitions(n);
+    sc.counts = sc.partials + (size_t)passes * 256 * sort_hist_blocks(n);
+    sc.offsets = sc.counts + digit_cells;
+    // The scan's status words are 64-bit: an even word offset from the (256-byte
+    // aligned) buffer start.
+    const size_t status_at = (size_t)((sc.offsets + digit_cells) - sc.hist);
+    sc.scan_status = sc.hist + ((status_at + 1) & ~size_t{1});
+    // The scan's total, 8-byte aligned.
+    uint32_t* after = sc.scan_status + scan_status_words((uint32_t)digit_cells);
+    sc.scan_total = reinterpret_cast<uint64_t*>(reinterpret_cast<uintptr_t>(after + 1) & ~uintptr_t{7});
     return sc;
 }
 
@@ -472,13 +484,15 @@ athx_status enqueue_front(athx_ctx* ctx, const ProjectArgs& pa, const FrameShape
              "compact");
     const SortScratch dsc = sort_scratch(ctx->depth_sort_meta, ctx->depth_alt_keys.ptr,
                                          as<uint32_t>(ctx->depth_alt_slots), fs.slots, kDepthPasses);
+    bool in_alt = false;  // three passes end in the alternate buffers: read them there
     ATHX_HIP(ctx, launch_sort(as<uint32_t>(ctx->depth_keys_sorted), as<uint32_t>(ctx->depth_slots), totals + 1, fs.slots, 0,
-                              kDepthBits, dsc, s),
+                              kDepthBits, dsc, s, &in_alt),
              "depth sort");
     if (timed) {
         ATHX_HIP(ctx, hipEventRecord(ctx->ev[2], s), "event");
     }
-    ATHX_HIP(ctx, launch_gather_scan(as<uint32_t>(ctx->touched), as<uint32_t>(ctx->depth_slots), totals + 1, fs.slots,
+    const uint32_t* sorted_slots = as<uint32_t>(in_alt ? ctx->depth_alt_slots : ctx->depth_slots);
+    ATHX_HIP(ctx, launch_gather_scan(as<uint32_t>(ctx->touched), sorted_slots, totals + 1, fs.slots,
                                      as<uint32_t>(ctx->offsets), totals, as<uint32_t>(ctx->scan_status), s),
              "counts");
     if (timed) {

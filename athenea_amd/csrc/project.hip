@@ -675,7 +675,23 @@ __global__ void __launch_bounds__(kBigThreads) emit_big_kernel(const DView* __re
             first = offsets[slot];
         }
         const uint32_t here = total - c0 < kBigPerWave ? total - c0 : kBigPerWave;
+        // The first 64 hit words of footprint jj, loaded one footprint ahead
+        // so the load's latency hides behind the previous footprint's steps.
+        auto first_words = [&](uint32_t jj) -> uint64_t {
+            if (jj >= here) {
+                return 0ull;
+            }
+            const uint32_t bj = lane_u(base, jj);
+            if (lane_u(expected, jj) == 0 || bj == ~0u) {
+                return 0ull;
+            }
+            const uint32_t st = big_steps(lane_u(f.w, jj), lane_u(f.h, jj));
+            return lane < st ? fp.big_masks[(uint64_t)bj + lane] : 0ull;
+        };
+        uint64_t ahead = first_words(0);
         for (uint32_t j = 0; j < here; ++j) {
+            const uint64_t words0 = ahead;
+            ahead = first_words(j + 1);
             const uint32_t expected_j = lane_u(expected, j);
             if (expected_j == 0) {
                 continue;
@@ -689,30 +705,34 @@ __global__ void __launch_bounds__(kBigThreads) emit_big_kernel(const DView* __re
             uint32_t at = lane_u(first, j);
             const uint32_t stop = (uint64_t)at + expected_j < cap ? at + expected_j : cap;
             if (base_j != ~0u) {
-                // The count's hit words, 64 steps' worth a load.
+                // This lane's tile in step s: (row, col) advance by a fixed
+                // stride a step (rows of 64 / w when w <= 64, else 64 columns
+                // of one row, wrapping to the next row).
+                const bool narrow = g.w <= 64;
+                const uint32_t rows = narrow ? 64 / g.w : 1u;
+                const uint32_t dr = narrow ? small_row(lane, g.w) : 0u;
+                uint32_t row = dr, col = narrow ? lane - dr * g.w : lane;
                 for (uint32_t s0 = 0; s0 < steps && at < stop; s0 += 64) {
-                    const uint64_t run = s0 + lane < steps ? fp.big_masks[(uint64_t)base_j + s0 + lane] : 0ull;
+                    const uint64_t run = s0 == 0 ? words0
+                                                 : (s0 + lane < steps ? fp.big_masks[(uint64_t)base_j + s0 + lane] : 0ull);
                     const uint32_t last = steps - s0 < 64u ? steps - s0 : 64u;
                     for (uint32_t r = 0; r < last; ++r) {
                         const uint64_t word = lane_u64(run, r);
-                        const uint32_t s = s0 + r;
-                        uint32_t col, row;
-                        if (g.w <= 64) {
-                            const uint32_t rows = 64 / g.w;
-                            const uint32_t dr = small_row(lane, g.w);
-                            col = lane - dr * g.w;
-                            row = s * rows + dr;
-                        } else {
-                            const uint32_t chunks = div_up(g.w, 64);
-                            row = s / chunks;
-                            col = (s - row * chunks) * 64 + lane;
-                        }
                         const uint32_t dst = at + (uint32_t)__popcll(word & lt_mask);
                         if (((word >> lane) & 1u) && dst < stop) {
                             keys[dst] = corner + row * V.tiles_x + col;
                             values[dst] = slot_j;
                         }
                         at += (uint32_t)__popcll(word);
+                        if (narrow) {
+                            row += rows;
+                        } else {
+                            col += 64;
+                            if (col - lane >= g.w) {
+                                col = lane;
+                                row += 1;
+                            }
+                        }
                     }
                 }
             } else {

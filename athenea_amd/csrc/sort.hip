@@ -13,6 +13,10 @@
 // (shaders/athenea/algo/radix_block.slang:79-176).
 #include "athx_internal.h"
 
+#ifndef ATHX_REDUCE_SCAN_MIN
+#define ATHX_REDUCE_SCAN_MIN (1u << 22)
+#endif
+
 namespace athx {
 namespace {
 
@@ -97,6 +101,9 @@ __device__ inline uint64_t match_digit(uint32_t d, uint32_t width, bool ok) {
 // second kernel sums the rows. Adding 256 words a block into one histogram
 // with device-scope atomics serialises in the memory system.
 constexpr uint32_t kHistBlocks = 1024;
+// Sorts of at least this capacity take the reduce-scan-scatter passes;
+// smaller ones onesweep's, whose few partitions wait little on each other.
+constexpr uint32_t kReduceScanMin = ATHX_REDUCE_SCAN_MIN;
 
 template <typename K>
 __global__ void __launch_bounds__(kThreads) histogram_kernel(const K* __restrict__ keys,
@@ -294,6 +301,132 @@ __global__ void __launch_bounds__(kThreads) onesweep_kernel(const K* __restrict_
     }
 }
 
+// Reduce, scan, scatter: the same pass without look-back. upsweep_kernel
+// counts each partition's digits into counts[digit][partition], a plain
+// scan of that array (digit-major) gives every (digit, partition) its first
+// output slot, and downsweep_kernel ranks the partition as onesweep_kernel
+// does and scatters from those slots. One more read of the keys a pass, but
+// no partition waits on another: on MI355X the look-back's coherent loads
+// and the ticket atomics cost more than that read.
+template <typename K>
+__global__ void __launch_bounds__(kThreads) upsweep_kernel(const K* __restrict__ kin,
+                                                           const uint64_t* __restrict__ n_dev, uint32_t cap,
+                                                           uint32_t shift, uint32_t width, uint32_t parts_cap,
+                                                           uint32_t* __restrict__ counts) {
+    __shared__ uint32_t s_hist[kRadix];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t part = blockIdx.x;
+    const uint32_t n = count_of(n_dev, cap);
+    const uint32_t radix = 1u << width;
+    s_hist[tid] = 0;
+    __syncthreads();
+    const uint32_t part_first = part * kPart;
+    if (part_first < n) {
+        const uint32_t dmask = radix - 1u;
+#pragma unroll
+        for (uint32_t j = 0; j < kItems; ++j) {
+            const uint32_t at = part_first + j * kThreads + tid;
+            if (at < n) {
+                atomicAdd(&s_hist[(uint32_t)(kin[at] >> shift) & dmask], 1u);
+            }
+        }
+    }
+    __syncthreads();
+    if (tid < radix) {
+        counts[(size_t)tid * parts_cap + part] = s_hist[tid];
+    }
+}
+
+template <typename K>
+__global__ void __launch_bounds__(kThreads) downsweep_kernel(const K* __restrict__ kin,
+                                                             const uint32_t* __restrict__ vin,
+                                                             K* __restrict__ kout, uint32_t* __restrict__ vout,
+                                                             const uint64_t* __restrict__ n_dev, uint32_t cap,
+                                                             uint32_t shift, uint32_t width, uint32_t parts_cap,
+                                                             const uint32_t* __restrict__ offsets) {
+    __shared__ K s_keys[kPart];
+    __shared__ uint32_t s_vals[kPart];
+    __shared__ uint32_t s_wave_hist[kWaves * kRadix];
+    __shared__ uint32_t s_block_start[kRadix];
+    __shared__ uint32_t s_digit_base[kRadix];
+    __shared__ uint32_t s_tmp[kWaves];
+
+    const uint32_t tid = threadIdx.x;
+    const uint32_t lane = tid & 63u;
+    const uint32_t wave = tid >> 6;
+    for (uint32_t i = tid; i < kWaves * kRadix; i += kThreads) {
+        s_wave_hist[i] = 0;
+    }
+    __syncthreads();
+    const uint32_t n = count_of(n_dev, cap);
+    const uint32_t part = blockIdx.x;
+    if (part >= div_up(n, kPart)) {
+        return;
+    }
+    const uint32_t part_first = part * kPart;
+    const uint32_t wave_first = part_first + wave * (64 * kItems);
+    K key[kItems];
+    uint32_t val[kItems];
+    uint32_t rank[kItems];
+#pragma unroll
+    for (uint32_t j = 0; j < kItems; ++j) {
+        const uint32_t at = wave_first + j * 64 + lane;
+        const bool ok = at < n;
+        key[j] = ok ? kin[at] : 0;
+        val[j] = ok ? vin[at] : 0;
+    }
+    const uint64_t lt_mask = (1ull << lane) - 1ull;
+    const uint32_t dmask = (1u << width) - 1u;
+    uint32_t* my_hist = s_wave_hist + wave * kRadix;
+#pragma unroll
+    for (uint32_t j = 0; j < kItems; ++j) {
+        const uint32_t at = wave_first + j * 64 + lane;
+        const bool ok = at < n;
+        const uint32_t d = (uint32_t)(key[j] >> shift) & dmask;
+        const uint64_t m = match_digit(d, width, ok);
+        const uint32_t before = ok ? my_hist[d] : 0u;
+        __builtin_amdgcn_wave_barrier();
+        rank[j] = before + (uint32_t)__popcll(m & lt_mask);
+        const bool leader = ok && (m & lt_mask) == 0;
+        if (leader) {
+            my_hist[d] = before + (uint32_t)__popcll(m);
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+    __syncthreads();
+    const uint32_t d = tid;
+    uint32_t count = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < kWaves; ++w) {
+        const uint32_t c = s_wave_hist[w * kRadix + d];
+        s_wave_hist[w * kRadix + d] = count;
+        count += c;
+    }
+    uint32_t unused;
+    s_block_start[d] = block_exclusive_scan(count, s_tmp, unused);
+    s_digit_base[d] = d <= dmask ? offsets[(size_t)d * parts_cap + part] : 0u;
+    __syncthreads();
+#pragma unroll
+    for (uint32_t j = 0; j < kItems; ++j) {
+        const uint32_t at = wave_first + j * 64 + lane;
+        if (at < n) {
+            const uint32_t dj = (uint32_t)(key[j] >> shift) & dmask;
+            const uint32_t pos = s_block_start[dj] + s_wave_hist[wave * kRadix + dj] + rank[j];
+            s_keys[pos] = key[j];
+            s_vals[pos] = val[j];
+        }
+    }
+    __syncthreads();
+    const uint32_t valid = n - part_first < kPart ? n - part_first : kPart;
+    for (uint32_t i = tid; i < valid; i += kThreads) {
+        const K k = s_keys[i];
+        const uint32_t dk = (uint32_t)(k >> shift) & dmask;
+        const uint32_t dst = s_digit_base[dk] + (i - s_block_start[dk]);
+        kout[dst] = k;
+        vout[dst] = s_vals[i];
+    }
+}
+
 // Clears what this frame's passes use: the tickets whole, and each pass's
 // look-back words for this frame's partitions only (the histogram is written
 // whole by hist_reduce_kernel). Layout, as SortScratch:
@@ -328,7 +461,10 @@ __global__ void copy_back_kernel(K* __restrict__ keys, uint32_t* __restrict__ va
 
 template <typename K>
 hipError_t sort_impl(K* keys, uint32_t* values, const uint64_t* n_dev, uint32_t cap, uint32_t lo_bit,
-                     uint32_t key_bits, const SortScratch& sc, hipStream_t s) {
+                     uint32_t key_bits, const SortScratch& sc, hipStream_t s, bool* in_alt) {
+    if (in_alt) {
+        *in_alt = false;
+    }
     if (cap < 2 || key_bits <= lo_bit) {
         return hipSuccess;
     }
@@ -338,6 +474,36 @@ hipError_t sort_impl(K* keys, uint32_t* values, const uint64_t* n_dev, uint32_t 
     const uint32_t passes = div_up(key_bits - lo_bit, 8);
     const uint32_t parts = sort_partitions(cap);
     const uint32_t spread = parts < 2048 ? parts : 2048;
+    if (cap >= kReduceScanMin) {
+        K* kin = keys;
+        uint32_t* vin = values;
+        K* kout = static_cast<K*>(sc.keys_alt);
+        uint32_t* vout = sc.values_alt;
+        for (uint32_t p = 0; p < passes; ++p) {
+            uint32_t shift, width;
+            pass_digit(p, lo_bit, key_bits - lo_bit, passes, shift, width);
+            hipLaunchKernelGGL(upsweep_kernel<K>, dim3(parts), dim3(kThreads), 0, s, kin, n_dev, cap, shift, width,
+                               parts, sc.counts);
+            if (hipError_t e = launch_scan(sc.counts, sc.offsets, sc.scan_total, (1u << width) * parts,
+                                           sc.scan_status, s);
+                e != hipSuccess) {
+                return e;
+            }
+            hipLaunchKernelGGL(downsweep_kernel<K>, dim3(parts), dim3(kThreads), 0, s, kin, vin, kout, vout, n_dev,
+                               cap, shift, width, parts, sc.offsets);
+            K* tk = kin; kin = kout; kout = tk;
+            uint32_t* tv = vin; vin = vout; vout = tv;
+        }
+        if (kin != keys) {
+            if (in_alt) {
+                *in_alt = true;
+            } else {
+                hipLaunchKernelGGL(copy_back_kernel<K>, dim3(spread), dim3(256), 0, s, keys, values, kin, vin, n_dev,
+                                   cap);
+            }
+        }
+        return hipGetLastError();
+    }
     const uint32_t hist_blocks = sort_hist_blocks(cap);
     hipLaunchKernelGGL(clear_kernel, dim3(spread), dim3(256), 0, s, sc.lookback, sc.counters, passes, n_dev, cap);
     hipLaunchKernelGGL(histogram_kernel<K>, dim3(hist_blocks), dim3(kThreads), 0, s, keys, n_dev, cap, lo_bit,
@@ -358,7 +524,11 @@ hipError_t sort_impl(K* keys, uint32_t* values, const uint64_t* n_dev, uint32_t 
         uint32_t* tv = vin; vin = vout; vout = tv;
     }
     if (kin != keys) {
-        hipLaunchKernelGGL(copy_back_kernel<K>, dim3(spread), dim3(256), 0, s, keys, values, kin, vin, n_dev, cap);
+        if (in_alt) {
+            *in_alt = true;
+        } else {
+            hipLaunchKernelGGL(copy_back_kernel<K>, dim3(spread), dim3(256), 0, s, keys, values, kin, vin, n_dev, cap);
+        }
     }
     return hipGetLastError();
 }
@@ -378,13 +548,13 @@ uint32_t sort_hist_blocks(uint32_t n) {
 // in order of those bits sorts only the rest. The result is back in keys and
 // values; sc.keys_alt holds keys of the same width.
 hipError_t launch_sort(uint64_t* keys, uint32_t* values, const uint64_t* n_dev, uint32_t cap, uint32_t lo_bit,
-                       uint32_t key_bits, const SortScratch& sc, hipStream_t s) {
-    return sort_impl(keys, values, n_dev, cap, lo_bit, key_bits, sc, s);
+                       uint32_t key_bits, const SortScratch& sc, hipStream_t s, bool* in_alt) {
+    return sort_impl(keys, values, n_dev, cap, lo_bit, key_bits, sc, s, in_alt);
 }
 
 hipError_t launch_sort(uint32_t* keys, uint32_t* values, const uint64_t* n_dev, uint32_t cap, uint32_t lo_bit,
-                       uint32_t key_bits, const SortScratch& sc, hipStream_t s) {
-    return sort_impl(keys, values, n_dev, cap, lo_bit, key_bits, sc, s);
+                       uint32_t key_bits, const SortScratch& sc, hipStream_t s, bool* in_alt) {
+    return sort_impl(keys, values, n_dev, cap, lo_bit, key_bits, sc, s, in_alt);
 }
 
 }  // namespace athx

@@ -70,6 +70,43 @@ def stage_bytes(n: int, visible: int, pairs: int, tiles: int, pixels: int, key_b
     }
 
 
+# The kernels each timed stage launches, in order.
+STAGE_KERNELS = {
+    "project": "project_kernel, count_small_kernel, the big-footprint listing scan, count_big_kernel",
+    "depth_sort": "the visible compaction scan, histogram_kernel, hist_reduce_kernel, onesweep_kernel x3",
+    "counts": "the gather scan",
+    "emit": "emit_small_kernel, emit_big_kernel",
+    "tile_sort": "histogram_kernel, hist_reduce_kernel, onesweep_kernel x2",
+    "ranges": "ranges_kernel",
+    "blend": "blend_kernel",
+}
+
+
+def measured_traffic(stage: str):
+    """HBM bytes a frame's stage moves, from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE and WRITE_SIZE summed over the stage's kernels of
+    one frame, each scaled by what the counter reads for a copy of known
+    size): profiles/traffic_1080p_relit.json. None when absent."""
+    path = ROOT / "profiles" / "traffic_1080p_relit.json"
+    if not path.exists():
+        return None
+    stages = json.loads(path.read_text()).get("stages", {})
+    entry = stages.get(stage)
+    return int(entry["fetch_bytes"] + entry["write_bytes"]) if entry else None
+
+
+def copy_probe() -> None:
+    """One 256 MiB device-to-device copy: its FETCH_SIZE and WRITE_SIZE
+    counters give the scale of the bytes those counters report."""
+    import torch
+
+    src = torch.ones(64 << 20, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+
+
 def cpu_baseline(cloud_np, view, tile: int, relight) -> dict:
     from oracle import pyoracle as po
 
@@ -103,6 +140,9 @@ def main() -> int:
     ap.add_argument("--no-extras", action="store_true", help="forward only: no backward, batch or CPU legs")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the 60-view walk")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="also run one 256 MiB device-to-device copy, the known-size kernel that scales the "
+                         "FETCH_SIZE / WRITE_SIZE counters of a rocprofv3 --pmc run of this command")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the rasterizer shards by views or screen tiles across GPUs, "
@@ -160,13 +200,13 @@ def main() -> int:
     dominant = max(stages, key=stages.get)
     achieved = sbytes[dominant] / (stages[dominant] * 1e-3)
     roofline = {
-        "kernel": dominant,
+        "kernel": f"{dominant} ({STAGE_KERNELS[dominant]})",
         "bound": "hbm",
         "achieved": round(achieved / 1e9, 1),
         "peak": HBM_PEAK / 1e9,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK, 4),
-        "traffic": None,
+        "traffic": measured_traffic(dominant),
         "algorithmic_bytes": sbytes[dominant],
     }
     result = {
@@ -198,6 +238,9 @@ def main() -> int:
         "sort_keys_per_s": round(pairs / (stages["tile_sort"] * 1e-3)),
         "roofline": roofline,
     }
+
+    if args.copy_probe:
+        copy_probe()
 
     if not args.no_extras:
         from athenea_amd.autograd import rasterize
