@@ -3,8 +3,8 @@
 A step is one forward frame of the 1080p workload (1.4M Gaussians, SH degree
 3, FP32, 1920x1080, 16x16 tiles, each Gaussian with relight attributes --
 albedo, roughness, metallic, a stored normal -- under a prepared 64x32
-log-normal HDR sky): projection with the relit colour, scan, pair emission,
-the onesweep sort, tile ranges and the blend, with the Gaussians and the sky
+log-normal HDR sky): projection with the relit colour, the depth sort, pair emission,
+the tile sort, tile ranges and the blend, with the Gaussians and the sky
 already resident in HBM. Prints ONE JSON line:
 
   value         forward ms/frame (lower is better)
@@ -76,7 +76,8 @@ STAGE_KERNELS = {
     "depth_sort": "the visible compaction scan, histogram_kernel, hist_reduce_kernel, onesweep_kernel x3",
     "counts": "the gather scan",
     "emit": "emit_small_kernel, emit_big_kernel",
-    "tile_sort": "histogram_kernel, hist_reduce_kernel, onesweep_kernel x2",
+    "tile_sort": "per pass: upsweep_kernel, the digit-count scan, downsweep_kernel (reduce-scan-scatter, "
+                 "2 passes of the 13-bit tile key)",
     "ranges": "ranges_kernel",
     "blend": "blend_kernel",
 }
