@@ -165,6 +165,8 @@ class FrameInfo(ctypes.Structure):
         ("tiles", ctypes.c_uint32),
         ("key_bits", ctypes.c_uint32),
         ("pairs", ctypes.c_uint64),
+        ("strips", ctypes.c_uint32),
+        ("entries", ctypes.c_uint64),
     ]
 
 

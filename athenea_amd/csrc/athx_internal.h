@@ -36,7 +36,9 @@ struct DView {
     float focal_x, focal_y, centre_x, centre_y;
     float near_z, far_z;
     uint32_t orthographic;
-    uint32_t tile_base;  // first global tile of this view
+    uint32_t tile_base;   // first global tile of this view
+    uint32_t strips_x;    // strips a tile row: div_up(tiles_x, kStripTiles)
+    uint32_t strip_base;  // first global strip of this view
     float m[12];
     float eye[3];
     float bg[4];
@@ -89,9 +91,9 @@ struct ProjectArgs {
 
 __host__ __device__ inline uint32_t div_up(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
 
-// Which tiles each footprint touches, kept from the count for the emission:
-// the reference tests every tile twice (splat_project.slang:764-776 and
-// splat_emit.slang:37-49); here the second pass reads the first one's answer.
+// The exact tile counts' working set (launch_exact_counts; the frame itself
+// bins its pairs instead): each footprint's rectangle, and which of its
+// tiles it touches.
 //
 // A footprint over kBigRect tiles is walked in steps of one hit word each
 // (big_steps); the big footprints are listed, and their words placed, by a
@@ -115,12 +117,85 @@ __host__ __device__ inline uint32_t big_steps(uint32_t w, uint32_t h) {
     return w <= 64 ? div_up(h, 64 / w) : h * div_up(w, 64);
 }
 
+// The frame bins its pairs in two levels. A strip is kStripTiles tiles of one
+// tile row of one view; strips are numbered view by view, row by row, left
+// to right, so a strip's tiles are consecutive global tiles and the strips'
+// order is the tiles' order. Each drawn footprint lists one strip entry for
+// every (row, strip) its tile rectangle covers, in depth order; a stable
+// sort on the strip puts each strip's entries in depth order; and each run
+// of kStripSeg entries of one strip (a segment) tests its entries against
+// the strip's tiles, counts the hits of every tile, and -- after one scan
+// of those counts, laid out strip by strip, tile by tile, segment by
+// segment -- writes each hit straight to its place in the tile-sorted pair
+// list. That list is the reference's sort of (tile << 24 | depth) keys
+// (TileRasterizer.cpp:1138-1150), without the sort over the pairs.
+constexpr uint32_t kStripTiles = 16;
+constexpr uint32_t kStripSeg = 1024;  // entries a segment: a workgroup of 256 lanes, 4 entries a lane
+// A slot's word in `touched` after the projection: its strip entries, with
+// kUnsettled set while the tile under its centre is not touched (whether
+// any tile is, is settled by launch_project's second kernel).
+constexpr uint32_t kUnsettled = 1u << 31;
+
 // Launchers (each returns the hipError_t of its launches).
-// The projection, then the exact tile counts: small rectangles spread over
-// the lanes of a wavefront, big ones listed by a scan and walked in steps.
+// The projection: records, depth keys, tile rectangles and each slot's
+// strip entries (0: not drawn, with the cull in its depth key), then the
+// footprints whose centre tile is not touched settled by a wavefront each
+// walking their rectangle until a tile is (splat_project.slang:764-776's
+// zero count is the cull no-tile).
 hipError_t launch_project(const ProjectArgs& a, const DView* views_dev, uint32_t nviews, float* records,
-                          uint32_t* touched, uint32_t* depth_keys, const Footprints& fp, uint32_t* status_words,
-                          hipStream_t s);
+                          uint32_t* touched, uint32_t* depth_keys, uint2* rects, hipStream_t s);
+// The exact tile count of every slot (splat_project.slang:764-776), for the
+// readers of the frame (copy_frame, the counters) -- not on the frame's
+// path: small rectangles spread over the lanes of a wavefront, big ones
+// listed by a scan and walked in steps.
+hipError_t launch_exact_counts(const DView* views_dev, uint32_t nviews, uint32_t n, uint32_t tile,
+                               const float* records, uint32_t* counts, uint32_t* depth_keys, const Footprints& fp,
+                               uint32_t* status_words, hipStream_t s);
+// The strip entries of the depth-sorted slots: rank k's entries start at
+// offsets[k]; entries at or past cap are not written.
+hipError_t launch_strip_emit(const DView* views_dev, uint32_t n, const uint2* rects, const uint32_t* sorted_slots,
+                             const uint32_t* offsets, const uint64_t* visible_dev, uint32_t max_visible, uint32_t cap,
+                             uint32_t* strip_keys, uint32_t* strip_slots, hipStream_t s);
+// The strip-sorted entries binned into tile-sorted pairs.
+struct StripBins {
+    const DView* views;
+    const uint32_t* strip_bases;  // [nviews + 1] first strip of each view, then the strips
+    uint32_t nviews;
+    uint32_t n;                   // Gaussians (slot = view * n + Gaussian)
+    uint32_t tile;
+    uint32_t strips;
+    const float* records;
+    const uint2* rects;
+    const uint32_t* strip_slots;   // [entries] strip-sorted entries' slots
+    const uint32_t* strip_ranges;  // [strips][2] each strip's entries (0, 0 when none)
+    const uint32_t* segbase;       // [strips] first segment of each strip
+    const uint4* seg_desc;         // [seg_cap][2] each segment's entries, table column, strip place
+    const uint64_t* seg_total;     // segments (device)
+    uint32_t seg_cap;              // segments the table holds
+    uint16_t* masks;               // [entries] tiles of its strip an entry touches
+    uint32_t* table;               // [16 * seg_cap] hit counts, then their exclusive scan
+    const uint64_t* pair_total;    // pairs (device; the table's total)
+    uint32_t pair_cap;
+    uint32_t* pair_slots;          // [pair_cap] the tile-sorted pairs' slots
+};
+// Each strip's segments, scanned: segbase[s] and the total in *seg_total.
+hipError_t launch_seg_scan(const uint32_t* strip_ranges, uint32_t strips, uint32_t* segbase, uint64_t* seg_total,
+                           uint32_t* status_words, hipStream_t s);
+// Each segment's descriptor (its entries, its first table cell, its strip's
+// view, row and first column), a wavefront a strip; the segments past the
+// total up to the capacity get empty ones, so a workgroup of the binning
+// starts from one load.
+hipError_t launch_seg_desc(const StripBins& b, uint4* desc, hipStream_t s);
+// Tests and counts every segment's entries (masks, table).
+hipError_t launch_strip_count(const StripBins& b, hipStream_t s);
+// The table's exclusive scan in place; the pair total lands in *pair_total.
+hipError_t launch_table_scan(uint32_t* table, const uint64_t* seg_total, uint32_t seg_cap, uint64_t* pair_total,
+                             uint32_t* status_words, hipStream_t s);
+// Every tile's pair range (0, 0 when empty), from the scanned table.
+hipError_t launch_tile_ranges(const StripBins& b, const uint32_t* tile_bases, uint32_t tiles, uint32_t* ranges,
+                              hipStream_t s);
+// The pairs' slots at their places.
+hipError_t launch_strip_pairs(const StripBins& b, hipStream_t s);
 // Lists the big footprints (rectangles over kBigRect tiles) in slot order
 // with each one's first step; the count and the steps land in *fp.big_total.
 hipError_t launch_big_list(const Footprints& fp, uint32_t slots, uint32_t* status_words, hipStream_t s);
@@ -128,21 +203,14 @@ hipError_t launch_fill_u32(uint32_t* p, uint32_t value, uint64_t n, hipStream_t 
 size_t scan_status_words(uint32_t n);
 hipError_t launch_scan(const uint32_t* in, uint32_t* out, uint64_t* total, uint32_t n, uint32_t* status,
                        hipStream_t s);
-// Pairs land at their scanned offsets; those at or past cap are not written
-// (the frame is then incomplete, as the reference's emit past its pair
-// capacity is, splat_emit.slang:43-46). A pair's key is its global tile:
-// the pairs come out in depth order, so the depth half of the reference's
-// 64-bit key (global tile << 24 | depth key) is already sorted and is not
-// stored.
-hipError_t launch_emit(const DView* views_dev, uint32_t nviews, uint32_t n, uint32_t tile, const float* records,
-                       const uint32_t* touched, const uint32_t* offsets, const Footprints& fp, uint32_t cap,
-                       uint32_t* tiles_out, uint32_t* values, hipStream_t s);
-hipError_t launch_ranges(const uint32_t* sorted_tiles, const uint64_t* n_dev, uint32_t cap, uint32_t tiles,
+// [start, end) of every key in 0..keys-1 among the first min(*n_dev, cap)
+// sorted keys; (0, 0) for a key that is absent.
+hipError_t launch_ranges(const uint32_t* sorted_keys, const uint64_t* n_dev, uint32_t cap, uint32_t keys,
                          uint32_t* ranges, hipStream_t s);
-// The reference's 64-bit pair keys of the sorted pairs: tile << 24 | the
-// depth key of the pair's slot.
-hipError_t launch_widen_keys(const uint32_t* sorted_tiles, const uint32_t* values, const uint32_t* depth_keys,
-                             uint64_t pairs, uint64_t* keys, hipStream_t s);
+// The reference's 64-bit pair keys of the tile-sorted pairs: tile << 24 |
+// the depth key of the pair's slot, a workgroup a tile over its range.
+hipError_t launch_pair_keys(const uint32_t* ranges, uint32_t tiles, const uint32_t* values,
+                            const uint32_t* depth_keys, uint64_t* keys, hipStream_t s);
 hipError_t launch_blend(const DView* views_dev, const uint32_t* view_of_tile_base, uint32_t nviews,
                         uint32_t total_tiles, uint32_t tile, uint32_t batch, const uint32_t* ranges,
                         const uint32_t* pair_slots, const float* records, uint32_t depth_mode,
@@ -184,6 +252,11 @@ hipError_t launch_sort(uint32_t* keys, uint32_t* values, const uint64_t* n_dev, 
 // *visible_total (device).
 hipError_t launch_compact(const uint32_t* touched, const uint32_t* depth_keys, uint32_t slots, uint32_t* keys_out,
                           uint32_t* slots_out, uint64_t* visible_total, uint32_t* status_words, hipStream_t s);
+// Strip entries of the depth-sorted slots, scanned in that order:
+// offsets[k] is rank k's first entry; the total lands in *entry_total.
+hipError_t launch_entry_scan(const uint32_t* touched, const uint32_t* sorted_slots, const uint64_t* visible_dev,
+                             uint32_t max_visible, uint32_t* offsets, uint64_t* entry_total, uint32_t* status_words,
+                             hipStream_t s);
 // Tile counts of the depth-sorted slots, scanned: slot_offsets[slot] is its
 // first pair; the pair total lands in *pair_total (device).
 hipError_t launch_gather_scan(const uint32_t* touched, const uint32_t* sorted_slots, const uint64_t* visible_dev,

@@ -38,6 +38,14 @@ uint32_t big_mask_words(uint32_t slots) { return slots + 1024; }
 
 constexpr uint32_t kDepthPasses = 3;  // 24-bit depth keys (splat_project.slang:1129; TileRasterizer.cpp:1122)
 
+// The frame's totals, u64 words on the device (and pinned on the host):
+// pairs, visible slots, the exact counts' big footprints, strip entries,
+// segments, the exact counts' sum.
+constexpr uint32_t kTotalPairs = 0, kTotalVisible = 1, kTotalBig = 2, kTotalEntries = 3, kTotalSegs = 4;
+constexpr uint32_t kTotalExact = 5;  // the exact counts' total (copy_frame's offsets)
+constexpr uint32_t kTotalWords = 6;
+constexpr uint64_t kMaxPairs = uint64_t{1} << 30;  // pairs and entries are 32-bit, with room
+
 uint32_t bits_for(uint32_t values) {
     // modules/render/src/FrameParams.h:18-24
     uint32_t bits = 1;
@@ -55,19 +63,30 @@ struct athx_ctx {
     bool exact_growth = false;
 
     DeviceBuffer views, tile_bases, records, touched, depth_keys, offsets, scan_status, totals;
-    DeviceBuffer pair_keys, pair_vals, alt_keys, alt_vals, sort_meta, ranges, grad2d, counter_words, big_list;
+    DeviceBuffer pair_vals, ranges, grad2d, counter_words, big_list;
     DeviceBuffer depth_keys_sorted, depth_slots, depth_alt_keys, depth_alt_slots, depth_sort_meta;
     DeviceBuffer fp_rects, fp_masks, fp_big_base, fp_big_masks;
+    // The binning: strip entries (keys, slots, the sort's alternates and
+    // metadata), each strip's range and first segment, the entries' hit
+    // masks and the per (tile, segment) table.
+    DeviceBuffer strip_keys, strip_slots, strip_alt_keys, strip_alt_slots, strip_sort_meta;
+    DeviceBuffer strip_ranges, segbase, seg_map, masks, seg_table;
+    // For the frame's readers: exact tile counts and their scan in depth order.
+    DeviceBuffer exact, exact_offsets, exact_status;
     DeviceBuffer api_sort_meta, api_alt_keys, api_alt_vals, api_scan_status, api_count;
-    uint64_t* host_totals = nullptr;  // pinned: [0] pairs, [1] visible
+    uint64_t* host_totals = nullptr;  // pinned: the totals words (kTotal*)
     DView* host_views = nullptr;      // pinned staging for the views
-    uint32_t* host_tile_bases = nullptr;
+    uint32_t* host_tile_bases = nullptr;  // [nviews + 1] tile bases, then [nviews + 1] strip bases
     uint32_t host_view_capacity = 0;
 
-    // What the last render left for the adjoint.
+    // What the last render left for the adjoint and the frame's readers.
     uint32_t last_n = 0, last_nviews = 0, last_tiles = 0, last_tile = 0, last_key_bits = 0, last_capacity = 0;
+    uint32_t last_strips = 0, last_entry_cap = 0;
+    bool last_sorted_in_alt = false;  // the depth-sorted slots are in depth_alt_slots
     uint64_t last_true_pairs = 0;  // the last frame's pair total (an async frame's once athx_frame_pairs ran)
-    uint64_t pair_hint = 0;        // the capacity an async frame takes when given none: 1.5x the largest seen
+    uint64_t pair_hint = 0;        // the pair capacity a frame takes before it knows: 1.25x the largest seen
+    uint64_t entry_hint = 0;       // the same for strip entries
+    uint64_t pair_last = 0, entry_last = 0;  // 1.25x the last eager frame's: what the next one is queued with
     hipEvent_t async_done = nullptr;
     bool async_pending = false;
     bool have_frame = false;
@@ -82,6 +101,17 @@ struct athx_ctx {
 };
 
 namespace {
+
+std::vector<DeviceBuffer*> all_buffers(athx_ctx* ctx) {
+    return {&ctx->views, &ctx->tile_bases, &ctx->records, &ctx->touched, &ctx->depth_keys, &ctx->offsets,
+            &ctx->scan_status, &ctx->totals, &ctx->pair_vals, &ctx->ranges, &ctx->grad2d, &ctx->counter_words,
+            &ctx->big_list, &ctx->depth_keys_sorted, &ctx->depth_slots, &ctx->depth_alt_keys, &ctx->depth_alt_slots,
+            &ctx->depth_sort_meta, &ctx->fp_rects, &ctx->fp_masks, &ctx->fp_big_base, &ctx->fp_big_masks,
+            &ctx->strip_keys, &ctx->strip_slots, &ctx->strip_alt_keys, &ctx->strip_alt_slots, &ctx->strip_sort_meta,
+            &ctx->strip_ranges, &ctx->segbase, &ctx->seg_map, &ctx->masks, &ctx->seg_table, &ctx->exact, &ctx->exact_offsets,
+            &ctx->exact_status, &ctx->api_sort_meta, &ctx->api_alt_keys, &ctx->api_alt_vals, &ctx->api_scan_status,
+            &ctx->api_count};
+}
 
 athx_status fail(athx_ctx* ctx, athx_status code, const std::string& msg) {
     if (ctx) {
@@ -257,7 +287,8 @@ ProjectArgs project_args(const athx_gaussians* g, const athx_settings* s) {
 
 // Fills the pinned staging views and tile bases; returns the frame's tiles.
 athx_status stage_views(athx_ctx* ctx, const athx_view* views, uint32_t nviews, const athx_settings* s,
-                        const athx_targets* targets, const float* const* dl, uint64_t& total_tiles) {
+                        const athx_targets* targets, const float* const* dl, uint64_t& total_tiles,
+                        uint64_t* strips_out = nullptr) {
     if (ctx->upload_pending) {
         ATHX_HIP(ctx, hipEventSynchronize(ctx->upload_done), "wait for the last upload");
         ctx->upload_pending = false;
@@ -274,11 +305,14 @@ athx_status stage_views(athx_ctx* ctx, const athx_view* views, uint32_t nviews, 
         ctx->host_view_capacity = 0;
         ATHX_HIP(ctx, hipHostMalloc(reinterpret_cast<void**>(&ctx->host_views), sizeof(DView) * nviews),
                  "pinned views");
-        ATHX_HIP(ctx, hipHostMalloc(reinterpret_cast<void**>(&ctx->host_tile_bases), sizeof(uint32_t) * (nviews + 1)),
+        ATHX_HIP(ctx, hipHostMalloc(reinterpret_cast<void**>(&ctx->host_tile_bases),
+                                    sizeof(uint32_t) * 2 * (nviews + 1)),
                  "pinned tile bases");
         ctx->host_view_capacity = nviews;
     }
     total_tiles = 0;
+    uint64_t total_strips = 0;
+    uint32_t* strip_bases = ctx->host_tile_bases + nviews + 1;
     for (uint32_t k = 0; k < nviews; ++k) {
         const athx_view& v = views[k];
         if (athx_status st = check_view(ctx, v, k); st != ATHX_OK) {
@@ -301,6 +335,10 @@ athx_status stage_views(athx_ctx* ctx, const athx_view* views, uint32_t nviews, 
         d.far_z = v.far_z;
         d.orthographic = v.orthographic ? 1u : 0u;
         d.tile_base = (uint32_t)total_tiles;
+        d.strips_x = div_up(d.tiles_x, kStripTiles);
+        d.strip_base = (uint32_t)total_strips;
+        strip_bases[k] = d.strip_base;
+        total_strips += (uint64_t)d.strips_x * d.tiles_y;
         std::memcpy(d.m, v.object_to_view, sizeof d.m);
         std::memcpy(d.eye, v.eye, sizeof d.eye);
         std::memcpy(d.bg, v.background, sizeof d.bg);
@@ -316,15 +354,19 @@ athx_status stage_views(athx_ctx* ctx, const athx_view* views, uint32_t nviews, 
         }
     }
     ctx->host_tile_bases[nviews] = (uint32_t)total_tiles;
+    strip_bases[nviews] = (uint32_t)total_strips;
+    if (strips_out) {
+        *strips_out = total_strips;
+    }
     return ATHX_OK;
 }
 
 athx_status upload_views(athx_ctx* ctx, uint32_t nviews, hipStream_t s) {
     ATHX_HIP(ctx, reserve(ctx, ctx->views, sizeof(DView) * nviews), "views");
-    ATHX_HIP(ctx, reserve(ctx, ctx->tile_bases, sizeof(uint32_t) * (nviews + 1)), "tile bases");
+    ATHX_HIP(ctx, reserve(ctx, ctx->tile_bases, sizeof(uint32_t) * 2 * (nviews + 1)), "tile bases");
     ATHX_HIP(ctx, hipMemcpyAsync(ctx->views.ptr, ctx->host_views, sizeof(DView) * nviews, hipMemcpyHostToDevice, s),
              "upload views");
-    ATHX_HIP(ctx, hipMemcpyAsync(ctx->tile_bases.ptr, ctx->host_tile_bases, sizeof(uint32_t) * (nviews + 1),
+    ATHX_HIP(ctx, hipMemcpyAsync(ctx->tile_bases.ptr, ctx->host_tile_bases, sizeof(uint32_t) * 2 * (nviews + 1),
                                  hipMemcpyHostToDevice, s),
              "upload tile bases");
     ATHX_HIP(ctx, hipEventRecord(ctx->upload_done, s), "record upload");
@@ -375,12 +417,21 @@ SortScratch sort_scratch(const DeviceBuffer& meta, void* alt_keys, uint32_t* alt
 // The sizes a frame runs at.
 struct FrameShape {
     uint32_t nviews = 0;
-    uint32_t slots = 0;     // views * Gaussians
-    uint32_t tiles = 0;     // over all views
-    uint32_t key_bits = 0;  // 24 depth bits + the tile bits
-    uint32_t passes = 0;    // tile sort passes
-    uint32_t capacity = 0;  // pairs the pair buffers hold
+    uint32_t slots = 0;        // views * Gaussians
+    uint32_t tiles = 0;        // over all views
+    uint32_t strips = 0;       // over all views
+    uint32_t key_bits = 0;     // 24 depth bits + the tile bits (the reference's pair keys)
+    uint32_t strip_bits = 0;   // bits of a strip number
+    uint32_t strip_passes = 0;
+    uint32_t entry_cap = 0;    // strip entries the binning holds
+    uint32_t seg_cap = 0;      // segments: entry_cap / kStripSeg + a partial one a strip
+    uint32_t capacity = 0;     // pairs the pair buffer holds
 };
+
+void set_entry_cap(FrameShape& fs, uint32_t entries) {
+    fs.entry_cap = entries;
+    fs.seg_cap = div_up(entries, kStripSeg) + fs.strips;
+}
 
 // Validates the call and stages the views (pinned), filling the shape.
 athx_status begin_frame(athx_ctx* ctx, const athx_view* views, uint32_t nviews, const athx_gaussians* gaussians,
@@ -401,15 +452,17 @@ athx_status begin_frame(athx_ctx* ctx, const athx_view* views, uint32_t nviews, 
         return fail(ctx, ATHX_UNSUPPORTED, "more than 2^32 (view, Gaussian) slots in one frame");
     }
     ATHX_HIP(ctx, hipSetDevice(ctx->device), "set device");
-    uint64_t tiles64 = 0;
-    if (athx_status st = stage_views(ctx, views, nviews, settings, targets, dl, tiles64); st != ATHX_OK) {
+    uint64_t tiles64 = 0, strips64 = 0;
+    if (athx_status st = stage_views(ctx, views, nviews, settings, targets, dl, tiles64, &strips64); st != ATHX_OK) {
         return st;
     }
     fs.nviews = nviews;
     fs.slots = (uint32_t)slots64;
     fs.tiles = (uint32_t)tiles64;
+    fs.strips = (uint32_t)strips64;
     fs.key_bits = kDepthBits + bits_for(fs.tiles);
-    fs.passes = div_up(fs.key_bits - kDepthBits, 8);  // the tile digits: the depth ones come presorted
+    fs.strip_bits = bits_for(fs.strips);
+    fs.strip_passes = div_up(fs.strip_bits, 8);
     return ATHX_OK;
 }
 
@@ -418,10 +471,10 @@ athx_status reserve_front(athx_ctx* ctx, const FrameShape& fs) {
     ATHX_HIP(ctx, reserve(ctx, ctx->records, slots * kRecordFloats * sizeof(float) + 16), "records");
     ATHX_HIP(ctx, reserve(ctx, ctx->touched, slots * 4 + 16), "touched");
     ATHX_HIP(ctx, reserve(ctx, ctx->depth_keys, slots * 4 + 16), "depth keys");
-    ATHX_HIP(ctx, reserve(ctx, ctx->offsets, slots * 4 + 16), "offsets");
-    ATHX_HIP(ctx, reserve(ctx, ctx->scan_status, scan_status_words(fs.slots) * 4), "scan status");
-    ATHX_HIP(ctx, reserve(ctx, ctx->totals, 32), "totals");
-    ATHX_HIP(ctx, reserve(ctx, ctx->big_list, slots * 4 + 16), "big footprints");
+    ATHX_HIP(ctx, reserve(ctx, ctx->offsets, (slots / 64 + 1) * 4 + 16), "entry offsets");
+    ATHX_HIP(ctx, reserve(ctx, ctx->scan_status, scan_status_words(std::max(fs.slots, fs.strips)) * 4),
+             "scan status");
+    ATHX_HIP(ctx, reserve(ctx, ctx->totals, 8 * kTotalWords), "totals");
     ATHX_HIP(ctx, reserve(ctx, ctx->ranges, (size_t)fs.tiles * 8 + 16), "ranges");
     ATHX_HIP(ctx, reserve(ctx, ctx->counter_words, 64), "counters");
     ATHX_HIP(ctx, reserve(ctx, ctx->depth_keys_sorted, slots * 4 + 16), "depth sort keys");
@@ -431,9 +484,43 @@ athx_status reserve_front(athx_ctx* ctx, const FrameShape& fs) {
     ATHX_HIP(ctx, reserve(ctx, ctx->depth_sort_meta, sort_meta_words(fs.slots, kDepthPasses) * 4 + 16),
              "depth sort metadata");
     ATHX_HIP(ctx, reserve(ctx, ctx->fp_rects, slots * 8 + 16), "footprint rects");
-    ATHX_HIP(ctx, reserve(ctx, ctx->fp_masks, slots * 8 + 16), "footprint masks");
-    ATHX_HIP(ctx, reserve(ctx, ctx->fp_big_base, slots * 4 + 16), "big footprint bases");
-    ATHX_HIP(ctx, reserve(ctx, ctx->fp_big_masks, big_mask_words(fs.slots) * 8 + 16), "big footprint masks");
+    ATHX_HIP(ctx, reserve(ctx, ctx->strip_ranges, (size_t)fs.strips * 8 + 16), "strip ranges");
+    ATHX_HIP(ctx, reserve(ctx, ctx->segbase, (size_t)fs.strips * 4 + 16), "strip segments");
+    return ATHX_OK;
+}
+
+// The binning's buffers for fs.entry_cap entries.
+athx_status reserve_bins(athx_ctx* ctx, const FrameShape& fs) {
+    const size_t cap = fs.entry_cap;
+    ATHX_HIP(ctx, reserve(ctx, ctx->strip_keys, cap * 4 + 16), "strip keys");
+    ATHX_HIP(ctx, reserve(ctx, ctx->strip_slots, cap * 4 + 16), "strip slots");
+    ATHX_HIP(ctx, reserve(ctx, ctx->strip_alt_keys, cap * 4 + 16), "strip sort keys");
+    ATHX_HIP(ctx, reserve(ctx, ctx->strip_alt_slots, cap * 4 + 16), "strip sort slots");
+    ATHX_HIP(ctx, reserve(ctx, ctx->strip_sort_meta, sort_meta_words(fs.entry_cap, fs.strip_passes) * 4 + 16),
+             "strip sort metadata");
+    ATHX_HIP(ctx, reserve(ctx, ctx->masks, cap * 2 + 16), "entry masks");
+    ATHX_HIP(ctx, reserve(ctx, ctx->seg_map, (size_t)fs.seg_cap * 32 + 32), "segment descriptors");
+    const size_t table = (size_t)kStripTiles * fs.seg_cap;
+    // The table's scan status follows the table.
+    ATHX_HIP(ctx, reserve(ctx, ctx->seg_table, table * 4 + scan_status_words((uint32_t)table) * 4 + 64),
+             "segment table");
+    return ATHX_OK;
+}
+
+athx_status reserve_pairs(athx_ctx* ctx, const FrameShape& fs) {
+    ATHX_HIP(ctx, reserve(ctx, ctx->pair_vals, (size_t)fs.capacity * 4 + 16), "pair values");
+    return ATHX_OK;
+}
+
+// The buffers of the exact counts (copy_frame, the counters).
+athx_status reserve_exact(athx_ctx* ctx, uint32_t slots) {
+    ATHX_HIP(ctx, reserve(ctx, ctx->exact, (size_t)slots * 4 + 16), "exact counts");
+    ATHX_HIP(ctx, reserve(ctx, ctx->exact_offsets, (size_t)slots * 4 + 16), "exact offsets");
+    ATHX_HIP(ctx, reserve(ctx, ctx->exact_status, scan_status_words(slots) * 4 + 16), "exact scan status");
+    ATHX_HIP(ctx, reserve(ctx, ctx->big_list, (size_t)slots * 4 + 16), "big footprints");
+    ATHX_HIP(ctx, reserve(ctx, ctx->fp_masks, (size_t)slots * 8 + 16), "footprint masks");
+    ATHX_HIP(ctx, reserve(ctx, ctx->fp_big_base, (size_t)slots * 4 + 16), "big footprint bases");
+    ATHX_HIP(ctx, reserve(ctx, ctx->fp_big_masks, big_mask_words(slots) * 8 + 16), "big footprint masks");
     return ATHX_OK;
 }
 
@@ -445,85 +532,136 @@ Footprints footprints_of(athx_ctx* ctx, uint32_t slots) {
     fp.big_mask_base = as<uint32_t>(ctx->fp_big_base);
     fp.big_masks = as<uint64_t>(ctx->fp_big_masks);
     fp.big_mask_cap = big_mask_words(slots);
-    fp.big_total = as<uint64_t>(ctx->totals) + 2;  // totals (u64): [0] pairs, [1] visible, [2] big << 32 | steps
+    fp.big_total = as<uint64_t>(ctx->totals) + kTotalBig;
     return fp;
 }
 
-athx_status reserve_back(athx_ctx* ctx, const FrameShape& fs) {
-    const size_t cap = fs.capacity;
-    ATHX_HIP(ctx, reserve(ctx, ctx->pair_keys, cap * 4 + 16), "pair tiles");
-    ATHX_HIP(ctx, reserve(ctx, ctx->pair_vals, cap * 4 + 16), "pair values");
-    ATHX_HIP(ctx, reserve(ctx, ctx->alt_keys, cap * 4 + 16), "sort keys");
-    ATHX_HIP(ctx, reserve(ctx, ctx->alt_vals, cap * 4 + 16), "sort values");
-    ATHX_HIP(ctx, reserve(ctx, ctx->sort_meta, sort_meta_words(fs.capacity, fs.passes) * 4 + 16), "sort metadata");
+// Enqueues the exact tile counts into ctx->exact (reserved).
+athx_status enqueue_exact(athx_ctx* ctx, uint32_t n, uint32_t nviews, uint32_t tile, hipStream_t s) {
+    ATHX_HIP(ctx, launch_exact_counts(as<DView>(ctx->views), nviews, n, tile, as<float>(ctx->records),
+                                      as<uint32_t>(ctx->exact), as<uint32_t>(ctx->depth_keys),
+                                      footprints_of(ctx, n * nviews), as<uint32_t>(ctx->exact_status), s),
+             "exact counts");
     return ATHX_OK;
 }
 
+const uint32_t* sorted_slots_of(const athx_ctx* ctx, bool in_alt) {
+    return static_cast<const uint32_t*>(in_alt ? ctx->depth_alt_slots.ptr : ctx->depth_slots.ptr);
+}
+
+StripBins bins_of(athx_ctx* ctx, const FrameShape& fs, uint32_t n, uint32_t tile, bool strips_in_alt) {
+    StripBins b{};
+    b.views = as<DView>(ctx->views);
+    b.strip_bases = as<uint32_t>(ctx->tile_bases) + fs.nviews + 1;
+    b.nviews = fs.nviews;
+    b.n = n;
+    b.tile = tile;
+    b.strips = fs.strips;
+    b.records = as<float>(ctx->records);
+    b.rects = as<uint2>(ctx->fp_rects);
+    b.strip_slots = as<uint32_t>(strips_in_alt ? ctx->strip_alt_slots : ctx->strip_slots);
+    b.strip_ranges = as<uint32_t>(ctx->strip_ranges);
+    b.segbase = as<uint32_t>(ctx->segbase);
+    b.seg_desc = as<uint4>(ctx->seg_map);
+    b.seg_total = as<uint64_t>(ctx->totals) + kTotalSegs;
+    b.seg_cap = fs.seg_cap;
+    b.masks = as<uint16_t>(ctx->masks);
+    b.table = as<uint32_t>(ctx->seg_table);
+    b.pair_total = as<uint64_t>(ctx->totals) + kTotalPairs;
+    b.pair_cap = fs.capacity;
+    b.pair_slots = as<uint32_t>(ctx->pair_vals);
+    return b;
+}
 
 // Projection, then the reference's depth order (TileRasterizer.cpp:1111-1137):
 // the visible slots compacted in slot order, stably sorted by their 24-bit
-// depth keys, and their tile counts scanned in that order into each slot's
-// first pair. Emitting in depth order leaves only the tile digits of the
-// 64-bit keys to sort. The visible total lands in totals[1], the pair total
-// in totals[0], both on the device.
+// depth keys, and their strip entries scanned in that order. The visible
+// total lands in totals[kTotalVisible], the entry total in
+// totals[kTotalEntries], both on the device.
 athx_status enqueue_front(athx_ctx* ctx, const ProjectArgs& pa, const FrameShape& fs, hipStream_t s, bool timed) {
     uint64_t* totals = as<uint64_t>(ctx->totals);
     if (timed) {
         ATHX_HIP(ctx, hipEventRecord(ctx->ev[0], s), "event");
     }
     ATHX_HIP(ctx, launch_project(pa, as<DView>(ctx->views), fs.nviews, as<float>(ctx->records),
-                                 as<uint32_t>(ctx->touched), as<uint32_t>(ctx->depth_keys),
-                                 footprints_of(ctx, fs.slots), as<uint32_t>(ctx->scan_status), s),
+                                 as<uint32_t>(ctx->touched), as<uint32_t>(ctx->depth_keys), as<uint2>(ctx->fp_rects),
+                                 s),
              "project");
     if (timed) {
         ATHX_HIP(ctx, hipEventRecord(ctx->ev[1], s), "event");
     }
     ATHX_HIP(ctx, launch_compact(as<uint32_t>(ctx->touched), as<uint32_t>(ctx->depth_keys), fs.slots,
-                                 as<uint32_t>(ctx->depth_keys_sorted), as<uint32_t>(ctx->depth_slots), totals + 1,
-                                 as<uint32_t>(ctx->scan_status), s),
+                                 as<uint32_t>(ctx->depth_keys_sorted), as<uint32_t>(ctx->depth_slots),
+                                 totals + kTotalVisible, as<uint32_t>(ctx->scan_status), s),
              "compact");
     const SortScratch dsc = sort_scratch(ctx->depth_sort_meta, ctx->depth_alt_keys.ptr,
                                          as<uint32_t>(ctx->depth_alt_slots), fs.slots, kDepthPasses);
     bool in_alt = false;  // three passes end in the alternate buffers: read them there
-    ATHX_HIP(ctx, launch_sort(as<uint32_t>(ctx->depth_keys_sorted), as<uint32_t>(ctx->depth_slots), totals + 1, fs.slots, 0,
-                              kDepthBits, dsc, s, &in_alt),
+    ATHX_HIP(ctx, launch_sort(as<uint32_t>(ctx->depth_keys_sorted), as<uint32_t>(ctx->depth_slots),
+                              totals + kTotalVisible, fs.slots, 0, kDepthBits, dsc, s, &in_alt),
              "depth sort");
+    ctx->last_sorted_in_alt = in_alt;
     if (timed) {
         ATHX_HIP(ctx, hipEventRecord(ctx->ev[2], s), "event");
     }
-    const uint32_t* sorted_slots = as<uint32_t>(in_alt ? ctx->depth_alt_slots : ctx->depth_slots);
-    ATHX_HIP(ctx, launch_gather_scan(as<uint32_t>(ctx->touched), sorted_slots, totals + 1, fs.slots,
-                                     as<uint32_t>(ctx->offsets), totals, as<uint32_t>(ctx->scan_status), s),
-             "counts");
+    ATHX_HIP(ctx, launch_entry_scan(as<uint32_t>(ctx->touched), sorted_slots_of(ctx, in_alt), totals + kTotalVisible,
+                                    fs.slots, as<uint32_t>(ctx->offsets), totals + kTotalEntries,
+                                    as<uint32_t>(ctx->scan_status), s),
+             "entries");
     if (timed) {
         ATHX_HIP(ctx, hipEventRecord(ctx->ev[3], s), "event");
     }
     return ATHX_OK;
 }
 
-// Emission, sort, ranges, blend and counters, every count read on the device
-// and bounded by the pair capacity: no host round trip, so the launches can
-// be captured in a graph.
-athx_status enqueue_back(athx_ctx* ctx, const ProjectArgs& pa, const FrameShape& fs, const athx_settings* settings,
-                         hipStream_t s, bool timed) {
-    const uint64_t* pairs_dev = as<uint64_t>(ctx->totals);
-    ATHX_HIP(ctx, launch_emit(as<DView>(ctx->views), fs.nviews, pa.n, settings->tile, as<float>(ctx->records),
-                              as<uint32_t>(ctx->touched), as<uint32_t>(ctx->offsets), footprints_of(ctx, fs.slots),
-                              fs.capacity, as<uint32_t>(ctx->pair_keys), as<uint32_t>(ctx->pair_vals), s),
-             "emit");
+// The strip entries, emitted in depth order, sorted on the strip, their
+// segments scanned, tested and counted, and the counts scanned: the pair
+// total lands in totals[kTotalPairs]. Every count is read on the device
+// and bounded by fs's capacities, so the launches can be captured.
+athx_status enqueue_bins(athx_ctx* ctx, const FrameShape& fs, uint32_t n, uint32_t tile, hipStream_t s, bool timed,
+                         bool& strips_in_alt) {
+    uint64_t* totals = as<uint64_t>(ctx->totals);
+    ATHX_HIP(ctx, launch_strip_emit(as<DView>(ctx->views), n, as<uint2>(ctx->fp_rects),
+                                    sorted_slots_of(ctx, ctx->last_sorted_in_alt), as<uint32_t>(ctx->offsets),
+                                    totals + kTotalVisible, fs.slots, fs.entry_cap, as<uint32_t>(ctx->strip_keys),
+                                    as<uint32_t>(ctx->strip_slots), s),
+             "strip entries");
+    const SortScratch sc = sort_scratch(ctx->strip_sort_meta, ctx->strip_alt_keys.ptr,
+                                        as<uint32_t>(ctx->strip_alt_slots), fs.entry_cap, fs.strip_passes);
+    strips_in_alt = false;
+    ATHX_HIP(ctx, launch_sort(as<uint32_t>(ctx->strip_keys), as<uint32_t>(ctx->strip_slots), totals + kTotalEntries,
+                              fs.entry_cap, 0, fs.strip_bits, sc, s, &strips_in_alt),
+             "strip sort");
+    const uint32_t* keys = as<uint32_t>(strips_in_alt ? ctx->strip_alt_keys : ctx->strip_keys);
+    ATHX_HIP(ctx, launch_ranges(keys, totals + kTotalEntries, fs.entry_cap, fs.strips, as<uint32_t>(ctx->strip_ranges),
+                                s),
+             "strip ranges");
+    ATHX_HIP(ctx, launch_seg_scan(as<uint32_t>(ctx->strip_ranges), fs.strips, as<uint32_t>(ctx->segbase),
+                                  totals + kTotalSegs, as<uint32_t>(ctx->scan_status), s),
+             "segments");
+    const StripBins b = bins_of(ctx, fs, n, tile, strips_in_alt);
+    ATHX_HIP(ctx, launch_seg_desc(b, as<uint4>(ctx->seg_map), s), "segment descriptors");
     if (timed) {
         ATHX_HIP(ctx, hipEventRecord(ctx->ev[4], s), "event");
     }
-    const SortScratch sc = sort_scratch(ctx->sort_meta, ctx->alt_keys.ptr, as<uint32_t>(ctx->alt_vals), fs.capacity,
-                                        fs.passes);
-    ATHX_HIP(ctx, launch_sort(as<uint32_t>(ctx->pair_keys), as<uint32_t>(ctx->pair_vals), pairs_dev, fs.capacity, 0,
-                              fs.key_bits - kDepthBits, sc, s),
-             "tile sort");
+    ATHX_HIP(ctx, launch_strip_count(b, s), "strip counts");
+    uint32_t* table = as<uint32_t>(ctx->seg_table);
+    uint32_t* table_status = table + (size_t)kStripTiles * fs.seg_cap;
+    table_status += (reinterpret_cast<uintptr_t>(table_status) & 7u) ? 1 : 0;
+    ATHX_HIP(ctx, launch_table_scan(table, totals + kTotalSegs, fs.seg_cap, totals + kTotalPairs, table_status, s),
+             "table scan");
+    return ATHX_OK;
+}
+
+// Ranges, the pairs at their places, the blend and the counters.
+athx_status enqueue_draw(athx_ctx* ctx, const ProjectArgs& pa, const FrameShape& fs, const athx_settings* settings,
+                         hipStream_t s, bool timed, bool strips_in_alt) {
+    const StripBins b = bins_of(ctx, fs, pa.n, settings->tile, strips_in_alt);
+    ATHX_HIP(ctx, launch_strip_pairs(b, s), "pairs");
     if (timed) {
         ATHX_HIP(ctx, hipEventRecord(ctx->ev[5], s), "event");
     }
-    ATHX_HIP(ctx, launch_ranges(as<uint32_t>(ctx->pair_keys), pairs_dev, fs.capacity, fs.tiles,
-                                as<uint32_t>(ctx->ranges), s),
+    ATHX_HIP(ctx, launch_tile_ranges(b, as<uint32_t>(ctx->tile_bases), fs.tiles, as<uint32_t>(ctx->ranges), s),
              "ranges");
     if (timed) {
         ATHX_HIP(ctx, hipEventRecord(ctx->ev[6], s), "event");
@@ -536,8 +674,14 @@ athx_status enqueue_back(athx_ctx* ctx, const ProjectArgs& pa, const FrameShape&
         ATHX_HIP(ctx, hipEventRecord(ctx->ev[7], s), "event");
     }
     if (settings->count_splats) {
+        if (athx_status st = reserve_exact(ctx, fs.slots); st != ATHX_OK) {
+            return st;
+        }
+        if (athx_status st = enqueue_exact(ctx, pa.n, fs.nviews, settings->tile, s); st != ATHX_OK) {
+            return st;
+        }
         ATHX_HIP(ctx, launch_fill_u32(as<uint32_t>(ctx->counter_words), 0u, 16, s), "clear counters");
-        ATHX_HIP(ctx, launch_counters(as<uint32_t>(ctx->depth_keys), as<uint32_t>(ctx->touched), fs.slots,
+        ATHX_HIP(ctx, launch_counters(as<uint32_t>(ctx->depth_keys), as<uint32_t>(ctx->exact), fs.slots,
                                       as<uint32_t>(ctx->counter_words), s),
                  "counters");
     }
@@ -560,6 +704,8 @@ athx_status enqueue_backward(athx_ctx* ctx, const ProjectArgs& pa, uint32_t nvie
     return ATHX_OK;
 }
 
+uint64_t grown(uint64_t seen) { return std::max<uint64_t>(seen + seen / 4, 1024); }
+
 // What the adjoint and the readers of the frame's buffers need to know.
 void finish_frame(athx_ctx* ctx, const athx_gaussians* g, const athx_settings* settings, const FrameShape& fs,
                   uint64_t pairs_known) {
@@ -570,14 +716,23 @@ void finish_frame(athx_ctx* ctx, const athx_gaussians* g, const athx_settings* s
     ctx->last_n = g->count;
     ctx->last_nviews = fs.nviews;
     ctx->last_tiles = fs.tiles;
+    ctx->last_strips = fs.strips;
     ctx->last_tile = settings->tile;
     ctx->last_capacity = fs.capacity;
+    ctx->last_entry_cap = fs.entry_cap;
     ctx->last_key_bits = fs.key_bits;
     ctx->last_true_pairs = pairs_known;
-    if (!ctx->async_pending) {
-        ctx->pair_hint = std::max<uint64_t>(ctx->pair_hint, pairs_known + pairs_known / 2);
-    }
     ctx->have_frame = true;
+}
+
+
+
+// Reads the frame's totals into the pinned words, waiting for them.
+athx_status read_totals(athx_ctx* ctx, hipStream_t s) {
+    ATHX_HIP(ctx, hipMemcpyAsync(ctx->host_totals, ctx->totals.ptr, 8 * kTotalWords, hipMemcpyDeviceToHost, s),
+             "read totals");
+    ATHX_HIP(ctx, hipStreamSynchronize(s), "wait for totals");
+    return ATHX_OK;
 }
 
 }  // namespace
@@ -604,7 +759,7 @@ ATHX_API athx_status athx_create(int device, athx_ctx** out) {
         return ATHX_OUT_OF_MEMORY;
     }
     ctx->device = device;
-    if (hipSetDevice(device) != hipSuccess || hipHostMalloc(reinterpret_cast<void**>(&ctx->host_totals), 16) != hipSuccess) {
+    if (hipSetDevice(device) != hipSuccess || hipHostMalloc(reinterpret_cast<void**>(&ctx->host_totals), 8 * kTotalWords) != hipSuccess) {
         delete ctx;
         return ATHX_DEVICE_FAILURE;
     }
@@ -628,13 +783,7 @@ ATHX_API void athx_destroy(athx_ctx* ctx) {
     }
     (void)hipSetDevice(ctx->device);
     (void)hipDeviceSynchronize();
-    for (DeviceBuffer* b : {&ctx->views, &ctx->tile_bases, &ctx->records, &ctx->touched, &ctx->depth_keys,
-                            &ctx->offsets, &ctx->scan_status, &ctx->totals, &ctx->pair_keys, &ctx->pair_vals,
-                            &ctx->alt_keys, &ctx->alt_vals, &ctx->sort_meta, &ctx->ranges, &ctx->grad2d,
-                            &ctx->counter_words, &ctx->api_sort_meta, &ctx->api_alt_keys, &ctx->api_alt_vals,
-                            &ctx->api_scan_status, &ctx->big_list, &ctx->api_count, &ctx->depth_keys_sorted,
-                            &ctx->depth_slots, &ctx->depth_alt_keys, &ctx->depth_alt_slots, &ctx->depth_sort_meta,
-                            &ctx->fp_rects, &ctx->fp_masks, &ctx->fp_big_base, &ctx->fp_big_masks}) {
+    for (DeviceBuffer* b : all_buffers(ctx)) {
         release(*b);
     }
     if (ctx->host_totals) {
@@ -668,16 +817,17 @@ ATHX_API athx_status athx_release_scratch(athx_ctx* ctx) {
     }
     (void)hipSetDevice(ctx->device);
     ATHX_HIP(ctx, hipDeviceSynchronize(), "synchronize before release");
-    for (DeviceBuffer* b : {&ctx->records, &ctx->touched, &ctx->depth_keys, &ctx->offsets, &ctx->scan_status,
-                            &ctx->big_list, &ctx->depth_keys_sorted, &ctx->depth_slots, &ctx->depth_alt_keys,
-                            &ctx->depth_alt_slots, &ctx->depth_sort_meta, &ctx->fp_rects, &ctx->fp_masks,
-                            &ctx->fp_big_base, &ctx->fp_big_masks, &ctx->pair_keys, &ctx->pair_vals,
-                            &ctx->alt_keys, &ctx->alt_vals, &ctx->sort_meta,
-                            &ctx->ranges, &ctx->grad2d, &ctx->api_sort_meta, &ctx->api_alt_keys,
-                            &ctx->api_alt_vals, &ctx->api_scan_status}) {
-        release(*b);
+    for (DeviceBuffer* b : all_buffers(ctx)) {
+        if (b != &ctx->views && b != &ctx->tile_bases && b != &ctx->totals && b != &ctx->counter_words &&
+            b != &ctx->api_count) {
+            release(*b);
+        }
     }
     ctx->exact_growth = true;
+    ctx->pair_hint = 0;
+    ctx->entry_hint = 0;
+    ctx->pair_last = 0;
+    ctx->entry_last = 0;
     ctx->have_frame = false;
     return ATHX_OK;
 }
@@ -687,13 +837,7 @@ ATHX_API uint64_t athx_scratch_bytes(const athx_ctx* ctx) {
         return 0;
     }
     uint64_t sum = 0;
-    for (const DeviceBuffer* b : {&ctx->views, &ctx->tile_bases, &ctx->records, &ctx->touched, &ctx->depth_keys,
-                                  &ctx->offsets, &ctx->scan_status, &ctx->totals, &ctx->pair_keys, &ctx->pair_vals,
-                                  &ctx->alt_keys, &ctx->alt_vals, &ctx->sort_meta, &ctx->ranges, &ctx->grad2d,
-                                  &ctx->counter_words, &ctx->api_sort_meta, &ctx->api_alt_keys, &ctx->api_alt_vals,
-                                  &ctx->api_scan_status, &ctx->big_list, &ctx->depth_keys_sorted, &ctx->depth_slots,
-                                  &ctx->depth_alt_keys, &ctx->depth_alt_slots, &ctx->depth_sort_meta,
-                                  &ctx->fp_rects, &ctx->fp_masks, &ctx->fp_big_base, &ctx->fp_big_masks}) {
+    for (const DeviceBuffer* b : all_buffers(const_cast<athx_ctx*>(ctx))) {
         sum += b->bytes;
     }
     return sum;
@@ -720,28 +864,85 @@ ATHX_API athx_status athx_render(athx_ctx* ctx, const athx_view* views, uint32_t
         return st;
     }
     const ProjectArgs pa = project_args(gaussians, settings);
+    const uint64_t* host = ctx->host_totals;
+    // With the sizes of earlier frames known, the whole frame is queued at
+    // once and the one wait is at its end, where the totals say whether it
+    // fitted; the first frame (and one that outgrows them) waits for its
+    // entry and pair totals before sizing what holds them, as the reference
+    // does for its pairs (TileRasterizer.cpp:1093-1102).
+    const bool ahead = ctx->entry_last != 0 && ctx->pair_last != 0;
+    if (ahead) {
+        set_entry_cap(fs, (uint32_t)ctx->entry_last);
+        fs.capacity = (uint32_t)ctx->pair_last;
+        if (athx_status st = reserve_bins(ctx, fs); st != ATHX_OK) {
+            return st;
+        }
+        if (athx_status st = reserve_pairs(ctx, fs); st != ATHX_OK) {
+            return st;
+        }
+    }
     if (athx_status st = enqueue_front(ctx, pa, fs, s, timed); st != ATHX_OK) {
         return st;
     }
-    // The one wait of the frame: how many pairs there are (and visible splats).
-    ATHX_HIP(ctx, hipMemcpyAsync(ctx->host_totals, ctx->totals.ptr, 16, hipMemcpyDeviceToHost, s), "read pair total");
-    ATHX_HIP(ctx, hipStreamSynchronize(s), "wait for pair total");
-    const uint64_t pairs64 = ctx->host_totals[0];
-    if (pairs64 >= (uint64_t{1} << 30)) {
-        return fail(ctx, ATHX_UNSUPPORTED, "more than 2^30 (tile, Gaussian) pairs in one frame");
-    }
-    fs.capacity = (uint32_t)pairs64;
-    if (athx_status st = reserve_back(ctx, fs); st != ATHX_OK) {
+    bool strips_in_alt = false;
+    bool fitted = ahead;
+    if (ahead) {
+        if (athx_status st = enqueue_bins(ctx, fs, pa.n, settings->tile, s, timed, strips_in_alt); st != ATHX_OK) {
+            return st;
+        }
+        if (athx_status st = enqueue_draw(ctx, pa, fs, settings, s, timed, strips_in_alt); st != ATHX_OK) {
+            return st;
+        }
+        if (athx_status st = read_totals(ctx, s); st != ATHX_OK) {
+            return st;
+        }
+        fitted = host[kTotalEntries] <= fs.entry_cap && host[kTotalPairs] <= fs.capacity;
+    } else if (athx_status st = read_totals(ctx, s); st != ATHX_OK) {
         return st;
     }
-    if (athx_status st = enqueue_back(ctx, pa, fs, settings, s, timed); st != ATHX_OK) {
-        return st;
+    if (!fitted) {
+        // The front half's outputs stand; bin and draw again at the exact sizes.
+        const uint64_t entries = host[kTotalEntries];
+        if (entries >= kMaxPairs) {
+            return fail(ctx, ATHX_UNSUPPORTED, "more than 2^30 strip entries in one frame");
+        }
+        if (!ahead || entries > fs.entry_cap) {
+            set_entry_cap(fs, std::max<uint32_t>((uint32_t)entries, 1u));
+            if (athx_status st = reserve_bins(ctx, fs); st != ATHX_OK) {
+                return st;
+            }
+            if (athx_status st = enqueue_bins(ctx, fs, pa.n, settings->tile, s, timed, strips_in_alt); st != ATHX_OK) {
+                return st;
+            }
+            if (athx_status st = read_totals(ctx, s); st != ATHX_OK) {
+                return st;
+            }
+        }
+        const uint64_t pairs = host[kTotalPairs];
+        if (pairs >= kMaxPairs) {
+            return fail(ctx, ATHX_UNSUPPORTED, "more than 2^30 (tile, Gaussian) pairs in one frame");
+        }
+        fs.capacity = std::max<uint32_t>(std::max<uint32_t>((uint32_t)pairs, fs.capacity), 1u);
+        if (athx_status st = reserve_pairs(ctx, fs); st != ATHX_OK) {
+            return st;
+        }
+        if (athx_status st = enqueue_draw(ctx, pa, fs, settings, s, timed, strips_in_alt); st != ATHX_OK) {
+            return st;
+        }
+        if (timed) {
+            ATHX_HIP(ctx, hipStreamSynchronize(s), "wait for frame");
+        }
     }
+    const uint64_t pairs64 = host[kTotalPairs];
+    ctx->entry_last = std::min<uint64_t>(grown(host[kTotalEntries]), kMaxPairs - 1);
+    ctx->pair_last = std::min<uint64_t>(grown(pairs64), kMaxPairs - 1);
+    ctx->entry_hint = std::max(ctx->entry_hint, ctx->entry_last);
+    ctx->pair_hint = std::max(ctx->pair_hint, ctx->pair_last);
     finish_frame(ctx, gaussians, settings, fs, pairs64);
     if (stats) {
         std::memset(stats, 0, sizeof *stats);
         stats->splats = gaussians->count;
-        stats->visible = (uint32_t)ctx->host_totals[1];
+        stats->visible = (uint32_t)host[kTotalVisible];
         stats->pairs = pairs64;
         if (timed) {
             ATHX_HIP(ctx, hipEventSynchronize(ctx->ev[7]), "wait for frame");
@@ -777,10 +978,14 @@ ATHX_API athx_status athx_render_async(athx_ctx* ctx, const athx_view* views, ui
     if (cap == 0) {
         return fail(ctx, ATHX_INVALID_ARGUMENT, "no pair capacity given and no frame seen to size one from");
     }
-    if (cap >= (uint64_t{1} << 30)) {
+    // Strip entries are fewer than pairs in any scene worth drawing; a frame
+    // seen before gives their size.
+    const uint64_t entry_cap = ctx->entry_hint ? ctx->entry_hint : cap;
+    if (cap >= kMaxPairs || entry_cap >= kMaxPairs) {
         return fail(ctx, ATHX_UNSUPPORTED, "a pair capacity of 2^30 or more");
     }
     fs.capacity = (uint32_t)cap;
+    set_entry_cap(fs, (uint32_t)entry_cap);
     hipStream_t s = static_cast<hipStream_t>(stream);
     if (athx_status st = upload_views(ctx, nviews, s); st != ATHX_OK) {
         return st;
@@ -788,17 +993,25 @@ ATHX_API athx_status athx_render_async(athx_ctx* ctx, const athx_view* views, ui
     if (athx_status st = reserve_front(ctx, fs); st != ATHX_OK) {
         return st;
     }
-    if (athx_status st = reserve_back(ctx, fs); st != ATHX_OK) {
+    if (athx_status st = reserve_bins(ctx, fs); st != ATHX_OK) {
+        return st;
+    }
+    if (athx_status st = reserve_pairs(ctx, fs); st != ATHX_OK) {
         return st;
     }
     const ProjectArgs pa = project_args(gaussians, settings);
     if (athx_status st = enqueue_front(ctx, pa, fs, s, false); st != ATHX_OK) {
         return st;
     }
-    if (athx_status st = enqueue_back(ctx, pa, fs, settings, s, false); st != ATHX_OK) {
+    bool strips_in_alt = false;
+    if (athx_status st = enqueue_bins(ctx, fs, pa.n, settings->tile, s, false, strips_in_alt); st != ATHX_OK) {
         return st;
     }
-    ATHX_HIP(ctx, hipMemcpyAsync(ctx->host_totals, ctx->totals.ptr, 8, hipMemcpyDeviceToHost, s), "read pair total");
+    if (athx_status st = enqueue_draw(ctx, pa, fs, settings, s, false, strips_in_alt); st != ATHX_OK) {
+        return st;
+    }
+    ATHX_HIP(ctx, hipMemcpyAsync(ctx->host_totals, ctx->totals.ptr, 8 * kTotalWords, hipMemcpyDeviceToHost, s),
+             "read totals");
     ATHX_HIP(ctx, hipEventRecord(ctx->async_done, s), "record frame");
     ctx->async_pending = true;
     finish_frame(ctx, gaussians, settings, fs, cap);
@@ -819,9 +1032,12 @@ ATHX_API athx_status athx_frame_pairs(athx_ctx* ctx, uint64_t* pairs, uint64_t* 
     if (ctx->async_pending) {
         ATHX_HIP(ctx, hipEventSynchronize(ctx->async_done), "wait for the frame");
         ctx->async_pending = false;
-        const uint64_t p = ctx->host_totals[0];
-        ctx->last_true_pairs = p;
-        ctx->pair_hint = std::max<uint64_t>(ctx->pair_hint, p + p / 2);
+        const uint64_t p = ctx->host_totals[kTotalPairs], e = ctx->host_totals[kTotalEntries];
+        // A frame whose strip entries passed their room is incomplete even
+        // when its (undercounted) pairs fit: it reports more than its capacity.
+        ctx->last_true_pairs = e > ctx->last_entry_cap ? std::max<uint64_t>(p, (uint64_t)ctx->last_capacity + 1) : p;
+        ctx->pair_hint = std::max<uint64_t>(ctx->pair_hint, std::min<uint64_t>(grown(p), kMaxPairs - 1));
+        ctx->entry_hint = std::max<uint64_t>(ctx->entry_hint, std::min<uint64_t>(grown(e), kMaxPairs - 1));
     }
     *pairs = ctx->last_true_pairs;
     if (capacity) {
@@ -918,6 +1134,8 @@ ATHX_API athx_status athx_frame_info_get(athx_ctx* ctx, athx_frame_info* out) {
     out->tiles = ctx->last_tiles;
     out->key_bits = ctx->last_key_bits;
     out->pairs = std::min(pairs, cap);
+    out->strips = ctx->last_strips;
+    out->entries = ctx->host_totals[kTotalEntries];
     return ATHX_OK;
 }
 
@@ -939,10 +1157,10 @@ ATHX_API athx_status athx_copy_frame(athx_ctx* ctx, athx_frame_part part, void* 
     uint64_t size = 0;
     switch (part) {
     case ATHX_PART_RECORDS: src = ctx->records.ptr; size = slots * kRecordFloats * 4; break;
-    case ATHX_PART_TOUCHED: src = ctx->touched.ptr; size = slots * 4; break;
+    case ATHX_PART_TOUCHED: size = slots * 4; break;  // the exact counts, taken now
     case ATHX_PART_DEPTH_KEYS: src = ctx->depth_keys.ptr; size = slots * 4; break;
-    case ATHX_PART_OFFSETS: src = ctx->offsets.ptr; size = slots * 4; break;
-    case ATHX_PART_PAIR_KEYS: size = pairs * 8; break;  // rebuilt from the tiles and depth keys
+    case ATHX_PART_OFFSETS: size = slots * 4; break;  // their scan in depth order, taken now
+    case ATHX_PART_PAIR_KEYS: size = pairs * 8; break;  // rebuilt from the ranges and depth keys
     case ATHX_PART_PAIR_VALUES: src = ctx->pair_vals.ptr; size = pairs * 4; break;
     case ATHX_PART_RANGES: src = ctx->ranges.ptr; size = (uint64_t)ctx->last_tiles * 8; break;
     default: return fail(ctx, ATHX_INVALID_ARGUMENT, "unknown frame part");
@@ -958,15 +1176,36 @@ ATHX_API athx_status athx_copy_frame(athx_ctx* ctx, athx_frame_part part, void* 
         return fail(ctx, ATHX_INVALID_ARGUMENT, "null destination");
     }
     ATHX_HIP(ctx, hipSetDevice(ctx->device), "set device");
+    hipStream_t s = static_cast<hipStream_t>(stream);
     if (part == ATHX_PART_PAIR_KEYS) {
-        ATHX_HIP(ctx, launch_widen_keys(as<uint32_t>(ctx->pair_keys), as<uint32_t>(ctx->pair_vals),
-                                        as<uint32_t>(ctx->depth_keys), pairs, static_cast<uint64_t*>(dst),
-                                        static_cast<hipStream_t>(stream)),
+        ATHX_HIP(ctx, launch_pair_keys(as<uint32_t>(ctx->ranges), ctx->last_tiles, as<uint32_t>(ctx->pair_vals),
+                                       as<uint32_t>(ctx->depth_keys), static_cast<uint64_t*>(dst), s),
                  "pair keys");
         return ATHX_OK;
     }
-    ATHX_HIP(ctx, hipMemcpyAsync(dst, src, size, hipMemcpyDeviceToDevice, static_cast<hipStream_t>(stream)),
-             "copy frame part");
+    if (part == ATHX_PART_TOUCHED || part == ATHX_PART_OFFSETS) {
+        // The frame never counts a slot's tiles (it bins them); the exact
+        // counts (splat_project.slang:764-776) and their scan in depth order
+        // (splat_gather_counts.slang:11-19) are taken on request. A slot not
+        // drawn has offset 0.
+        if (athx_status st = reserve_exact(ctx, (uint32_t)slots); st != ATHX_OK) {
+            return st;
+        }
+        if (athx_status st = enqueue_exact(ctx, ctx->last_n, ctx->last_nviews, ctx->last_tile, s); st != ATHX_OK) {
+            return st;
+        }
+        src = ctx->exact.ptr;
+        if (part == ATHX_PART_OFFSETS) {
+            uint64_t* totals = as<uint64_t>(ctx->totals);
+            ATHX_HIP(ctx, launch_fill_u32(as<uint32_t>(ctx->exact_offsets), 0u, slots, s), "clear offsets");
+            ATHX_HIP(ctx, launch_gather_scan(as<uint32_t>(ctx->exact), sorted_slots_of(ctx, ctx->last_sorted_in_alt),
+                                             totals + kTotalVisible, (uint32_t)slots, as<uint32_t>(ctx->exact_offsets),
+                                             totals + kTotalExact, as<uint32_t>(ctx->exact_status), s),
+                     "exact offsets");
+            src = ctx->exact_offsets.ptr;
+        }
+    }
+    ATHX_HIP(ctx, hipMemcpyAsync(dst, src, size, hipMemcpyDeviceToDevice, s), "copy frame part");
     return ATHX_OK;
 }
 
@@ -1074,12 +1313,16 @@ athx_status capture_forward(athx_graph* g, const ProjectArgs& pa, const FrameSha
     athx_ctx* ctx = g->ctx;
     ATHX_HIP(ctx, hipStreamBeginCapture(g->capture, hipStreamCaptureModeThreadLocal), "begin capture");
     athx_status st = enqueue_front(ctx, pa, fs, g->capture, false);
+    bool strips_in_alt = false;
     if (st == ATHX_OK) {
-        st = enqueue_back(ctx, pa, fs, settings, g->capture, false);
+        st = enqueue_bins(ctx, fs, pa.n, settings->tile, g->capture, false, strips_in_alt);
+    }
+    if (st == ATHX_OK) {
+        st = enqueue_draw(ctx, pa, fs, settings, g->capture, false, strips_in_alt);
     }
     hipError_t copy = hipSuccess;
     if (st == ATHX_OK) {
-        copy = hipMemcpyAsync(ctx->host_totals, ctx->totals.ptr, 8, hipMemcpyDeviceToHost, g->capture);
+        copy = hipMemcpyAsync(ctx->host_totals, ctx->totals.ptr, 8 * kTotalWords, hipMemcpyDeviceToHost, g->capture);
     }
     hipError_t end = hipStreamEndCapture(g->capture, &g->forward);
     if (st != ATHX_OK) {
@@ -1145,7 +1388,8 @@ ATHX_API athx_status athx_graph_create(int device, const athx_view* views, uint3
         }
         pair_capacity = std::max<uint64_t>(stats.pairs + stats.pairs / 2, 1024);
     }
-    if (pair_capacity >= (uint64_t{1} << 30)) {
+    const uint64_t entry_capacity = ctx->entry_hint ? ctx->entry_hint : pair_capacity;
+    if (pair_capacity >= kMaxPairs || entry_capacity >= kMaxPairs) {
         return graph_fail(g, fail(ctx, ATHX_UNSUPPORTED, "a pair capacity of 2^30 or more"), error, error_bytes);
     }
     FrameShape fs{};
@@ -1155,11 +1399,20 @@ ATHX_API athx_status athx_graph_create(int device, const athx_view* views, uint3
         return graph_fail(g, st, error, error_bytes);
     }
     fs.capacity = (uint32_t)pair_capacity;
+    set_entry_cap(fs, (uint32_t)entry_capacity);
     if (athx_status st = reserve_front(ctx, fs); st != ATHX_OK) {
         return graph_fail(g, st, error, error_bytes);
     }
-    if (athx_status st = reserve_back(ctx, fs); st != ATHX_OK) {
+    if (athx_status st = reserve_bins(ctx, fs); st != ATHX_OK) {
         return graph_fail(g, st, error, error_bytes);
+    }
+    if (athx_status st = reserve_pairs(ctx, fs); st != ATHX_OK) {
+        return graph_fail(g, st, error, error_bytes);
+    }
+    if (settings && settings->count_splats) {
+        if (athx_status st = reserve_exact(ctx, fs.slots); st != ATHX_OK) {
+            return graph_fail(g, st, error, error_bytes);
+        }
     }
     if (with_backward) {
         if (hipError_t e = reserve(ctx, ctx->grad2d, (size_t)fs.slots * kGrad2d * sizeof(float) + 16);
@@ -1171,11 +1424,11 @@ ATHX_API athx_status athx_graph_create(int device, const athx_view* views, uint3
     if (hipError_t e = reserve(ctx, ctx->views, sizeof(DView) * nviews); e != hipSuccess) {
         return graph_fail(g, hip_fail(ctx, e, "views"), error, error_bytes);
     }
-    if (hipError_t e = reserve(ctx, ctx->tile_bases, sizeof(uint32_t) * (nviews + 1)); e != hipSuccess) {
+    if (hipError_t e = reserve(ctx, ctx->tile_bases, sizeof(uint32_t) * 2 * (nviews + 1)); e != hipSuccess) {
         return graph_fail(g, hip_fail(ctx, e, "tile bases"), error, error_bytes);
     }
     if (hipMemcpy(ctx->views.ptr, ctx->host_views, sizeof(DView) * nviews, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(ctx->tile_bases.ptr, ctx->host_tile_bases, sizeof(uint32_t) * (nviews + 1),
+        hipMemcpy(ctx->tile_bases.ptr, ctx->host_tile_bases, sizeof(uint32_t) * 2 * (nviews + 1),
                   hipMemcpyHostToDevice) != hipSuccess) {
         return graph_fail(g, fail(ctx, ATHX_DEVICE_FAILURE, "upload views"), error, error_bytes);
     }

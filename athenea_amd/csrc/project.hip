@@ -210,7 +210,7 @@ constexpr uint32_t kInSoa32 = 0, kInSoa32Vec = 1, kInSoa16 = 2, kInSoa16Vec = 3,
 template <uint32_t kIn, bool kRelit>
 __global__ void __launch_bounds__(256) project_kernel(ProjectArgs a, const DView* __restrict__ views, uint32_t nviews,
                                                       float* __restrict__ records, uint32_t* __restrict__ touched,
-                                                      uint32_t* __restrict__ keys, Footprints fp) {
+                                                      uint32_t* __restrict__ keys, uint2* __restrict__ rects) {
     constexpr bool kHalf = kIn == kInSoa16 || kIn == kInSoa16Vec;
     constexpr bool kVec = kIn == kInSoa32Vec || kIn == kInSoa16Vec;
     constexpr bool kPacked = kIn == kInPacked;
@@ -276,7 +276,7 @@ __global__ void __launch_bounds__(256) project_kernel(ProjectArgs a, const DView
         const float ty = m[4] * px + m[5] * py + m[6] * pz + m[7];
         const float tz = m[8] * px + m[9] * py + m[10] * pz + m[11];
         if (!(tz > V.near_z) || tz > V.far_z) {
-            cull_early(slot, ATHX_CULL_DEPTH, touched, keys, fp.rects);
+            cull_early(slot, ATHX_CULL_DEPTH, touched, keys, rects);
             continue;
         }
         float jx0, jx2, jy1, jy2, cx, cy;
@@ -328,7 +328,7 @@ __global__ void __launch_bounds__(256) project_kernel(ProjectArgs a, const DView
         cc = cc + 0.3f;
         const float det = ca * cc - cb * cb;
         if (!(det > 1e-12f)) {
-            cull_early(slot, ATHX_CULL_DEGENERATE, touched, keys, fp.rects);
+            cull_early(slot, ATHX_CULL_DEGENERATE, touched, keys, rects);
             continue;
         }
         const float filtered = sqrtf(fmax_(det_before, 0.0f) / det);
@@ -337,7 +337,7 @@ __global__ void __launch_bounds__(256) project_kernel(ProjectArgs a, const DView
         const float alpha = spread >= 1.0f / 255.0f ? spread + (1.0f - paid) * (1.0f / 255.0f) : spread;
         flags |= spread >= 1.0f / 255.0f ? kFlagPaidBack : 0u;
         if (!(alpha >= 1.0f / 255.0f)) {
-            cull_early(slot, ATHX_CULL_FAINT, touched, keys, fp.rects);
+            cull_early(slot, ATHX_CULL_FAINT, touched, keys, rects);
             continue;
         }
         const float bdet = ca * cc - cb * cb;
@@ -345,18 +345,23 @@ __global__ void __launch_bounds__(256) project_kernel(ProjectArgs a, const DView
         const float cutoff = cutoff_power(alpha);
         uint32_t x0, y0, x1, y1;
         if (!tile_rect(V.width, V.height, a.tile, cx, cy, ka, kb, kc, cutoff, x0, y0, x1, y1)) {
-            cull_early(slot, ATHX_CULL_OFFSCREEN, touched, keys, fp.rects);
+            cull_early(slot, ATHX_CULL_OFFSCREEN, touched, keys, rects);
             continue;
         }
-        // The exact count (splat_project.slang:764-776) is taken after this
-        // kernel, with the tiles of many footprints spread over the lanes of
-        // a wavefront: count_small_kernel for rectangles of up to kBigRect
-        // tiles, count_big_kernel, a wavefront at a time, for the larger
-        // ones (listed by a scan). A thread walking its own rectangle would
-        // hold its wavefront to the largest of 64. The count kernels also
-        // write the slot's tile count and, when no tile is touched, its cull.
         const uint32_t w = x1 - x0 + 1, h = y1 - y0 + 1;
-        fp.rects[slot] = make_uint2(x0 | (y0 << 16), w | (h << 16));
+        rects[slot] = make_uint2(x0 | (y0 << 16), w | (h << 16));
+        // The reference counts the touched tiles here and culls a zero count
+        // (splat_project.slang:764-776); what the frame needs is only whether
+        // the count is zero, and the tile under the (clamped) centre nearly
+        // always settles that: it is touched whenever the centre lies in its
+        // box of pixel centres. The rest are settled by settle_kernel. The
+        // slot's word is its strip entries: one a (row, strip) of its
+        // rectangle.
+        const uint32_t ctx = min(max((uint32_t)clamp_(cx, 0.0f, (float)(V.width - 1u)) / a.tile, x0), x1);
+        const uint32_t cty = min(max((uint32_t)clamp_(cy, 0.0f, (float)(V.height - 1u)) / a.tile, y0), y1);
+        const bool settled = tile_touched(V.width, V.height, a.tile, cx, cy, ka, kb, kc, recip_of(ka), recip_of(kc),
+                                          cutoff, ctx, cty);
+        const uint32_t entries = h * (x1 / kStripTiles - x0 / kStripTiles + 1u);
         // Colour from this eye, in the cloud's space (splat_project.slang:781-785).
         const float dx = px - V.eye[0], dy = py - V.eye[1], dz = pz - V.eye[2];
         const float dinv = 1.0f / sqrtf(dx * dx + dy * dy + dz * dz);
@@ -407,6 +412,7 @@ __global__ void __launch_bounds__(256) project_kernel(ProjectArgs a, const DView
         rec[1] = make_float4(ka, kb, kc, alpha);
         rec[2] = make_float4(col[0], col[1], col[2], __builtin_bit_cast(float, flags));
         keys[slot] = __builtin_bit_cast(uint32_t, tz) >> 7;
+        touched[slot] = entries | (settled ? 0u : kUnsettled);
     }
 }
 
@@ -650,106 +656,6 @@ __global__ void __launch_bounds__(kBigThreads) count_big_kernel(const DView* __r
     }
 }
 
-__global__ void __launch_bounds__(kBigThreads) emit_big_kernel(const DView* __restrict__ views, uint32_t n,
-                                                               uint32_t T, const float* __restrict__ records,
-                                                               Footprints fp, const uint32_t* __restrict__ touched,
-                                                               const uint32_t* __restrict__ offsets, uint32_t cap,
-                                                               uint32_t* __restrict__ keys,
-                                                               uint32_t* __restrict__ values) {
-    const uint32_t lane = threadIdx.x & 63u;
-    const uint64_t lt_mask = (1ull << lane) - 1ull;
-    const uint32_t waves = gridDim.x * (kBigThreads / 64);
-    const uint32_t total = (uint32_t)(*fp.big_total >> 32);
-    for (uint32_t c0 = (blockIdx.x * (kBigThreads / 64) + (threadIdx.x >> 6)) * kBigPerWave; c0 < total;
-         c0 += waves * kBigPerWave) {
-        const uint32_t b = c0 + lane;
-        const bool mine = lane < kBigPerWave && b < total;
-        const uint32_t slot = mine ? fp.big_list[b] : 0u;
-        const uint32_t expected = mine ? touched[slot] : 0u;
-        BigFootprint f = {};
-        uint32_t base = ~0u, first = 0;
-        if (expected != 0) {
-            f = big_footprint(views, n, T, records, slot);
-            const uint32_t b0 = fp.big_mask_base[b];
-            base = (uint64_t)b0 + big_steps(f.w, f.h) <= fp.big_mask_cap ? b0 : ~0u;  // ~0u: not kept, test again
-            first = offsets[slot];
-        }
-        const uint32_t here = total - c0 < kBigPerWave ? total - c0 : kBigPerWave;
-        // The first 64 hit words of footprint jj, loaded one footprint ahead
-        // so the load's latency hides behind the previous footprint's steps.
-        auto first_words = [&](uint32_t jj) -> uint64_t {
-            if (jj >= here) {
-                return 0ull;
-            }
-            const uint32_t bj = lane_u(base, jj);
-            if (lane_u(expected, jj) == 0 || bj == ~0u) {
-                return 0ull;
-            }
-            const uint32_t st = big_steps(lane_u(f.w, jj), lane_u(f.h, jj));
-            return lane < st ? fp.big_masks[(uint64_t)bj + lane] : 0ull;
-        };
-        uint64_t ahead = first_words(0);
-        for (uint32_t j = 0; j < here; ++j) {
-            const uint64_t words0 = ahead;
-            ahead = first_words(j + 1);
-            const uint32_t expected_j = lane_u(expected, j);
-            if (expected_j == 0) {
-                continue;
-            }
-            const BigFootprint g = big_of_lane(f, j);
-            const DView& V = views[g.view];
-            const uint32_t base_j = lane_u(base, j);
-            const uint32_t slot_j = lane_u(slot, j);
-            const uint32_t corner = V.tile_base + g.y0 * V.tiles_x + g.x0;
-            const uint32_t steps = big_steps(g.w, g.h);
-            uint32_t at = lane_u(first, j);
-            const uint32_t stop = (uint64_t)at + expected_j < cap ? at + expected_j : cap;
-            if (base_j != ~0u) {
-                // This lane's tile in step s: (row, col) advance by a fixed
-                // stride a step (rows of 64 / w when w <= 64, else 64 columns
-                // of one row, wrapping to the next row).
-                const bool narrow = g.w <= 64;
-                const uint32_t rows = narrow ? 64 / g.w : 1u;
-                const uint32_t dr = narrow ? small_row(lane, g.w) : 0u;
-                uint32_t row = dr, col = narrow ? lane - dr * g.w : lane;
-                for (uint32_t s0 = 0; s0 < steps && at < stop; s0 += 64) {
-                    const uint64_t run = s0 == 0 ? words0
-                                                 : (s0 + lane < steps ? fp.big_masks[(uint64_t)base_j + s0 + lane] : 0ull);
-                    const uint32_t last = steps - s0 < 64u ? steps - s0 : 64u;
-                    for (uint32_t r = 0; r < last; ++r) {
-                        const uint64_t word = lane_u64(run, r);
-                        const uint32_t dst = at + (uint32_t)__popcll(word & lt_mask);
-                        if (((word >> lane) & 1u) && dst < stop) {
-                            keys[dst] = corner + row * V.tiles_x + col;
-                            values[dst] = slot_j;
-                        }
-                        at += (uint32_t)__popcll(word);
-                        if (narrow) {
-                            row += rows;
-                        } else {
-                            col += 64;
-                            if (col - lane >= g.w) {
-                                col = lane;
-                                row += 1;
-                            }
-                        }
-                    }
-                }
-            } else {
-                walk_big(g, V, T, lane, [&](uint32_t, bool, bool hit, uint32_t col, uint32_t row) {
-                    const uint64_t word = __ballot(hit);
-                    const uint32_t dst = at + (uint32_t)__popcll(word & lt_mask);
-                    if (hit && dst < stop) {
-                        keys[dst] = corner + row * V.tiles_x + col;
-                        values[dst] = slot_j;
-                    }
-                    at += (uint32_t)__popcll(word);
-                });
-            }
-        }
-    }
-}
-
 // Small footprints (rectangles of up to kBigRect tiles), 64 slots to a
 // wavefront, their tiles spread over the lanes: lane j holds slot c0 + j
 // and its rectangle's area A_j (0 for a culled or a big slot), and the
@@ -846,60 +752,6 @@ __global__ void __launch_bounds__(kSmallThreads) count_small_kernel(const DView*
     }
 }
 
-// One (tile, slot) pair per tile a small footprint touches, its tiles row by
-// row from its first pair on (splat_emit.slang:17-50), read from the count's
-// hit mask: the same spread of rectangles over lanes, a tile's place among
-// its footprint's pairs the hits before it in the mask. The pairs of one
-// footprint land side by side, so the stores of a step are contiguous runs.
-__global__ void __launch_bounds__(kSmallThreads) emit_small_kernel(const DView* __restrict__ views, uint32_t n,
-                                                                   uint32_t slots,
-                                                                   const uint32_t* __restrict__ touched,
-                                                                   const uint32_t* __restrict__ offsets,
-                                                                   Footprints fp, uint32_t cap,
-                                                                   uint32_t* __restrict__ keys,
-                                                                   uint32_t* __restrict__ values) {
-    __shared__ uint32_t s_end[kSmallThreads];
-    __shared__ uint64_t s_mask[kSmallThreads];
-    __shared__ uint4 s_at[kSmallThreads];  // first pair, first tile (the rectangle's corner), w, tiles_x
-    const uint32_t lane = threadIdx.x & 63u, wbase = threadIdx.x & ~63u;
-    const uint32_t slot = blockIdx.x * kSmallThreads + threadIdx.x;
-    const uint32_t count = slot < slots ? touched[slot] : 0u;
-    uint32_t mine = 0;
-    if (count != 0) {
-        const uint2 rect = fp.rects[slot];
-        const uint32_t w = rect.y & 0xFFFFu, area = w * (rect.y >> 16);
-        if (area <= kBigRect) {
-            mine = area;
-            const DView& V = views[slot / n];
-            s_mask[threadIdx.x] = fp.masks[slot];
-            s_at[threadIdx.x] = make_uint4(offsets[slot], V.tile_base + (rect.x >> 16) * V.tiles_x + (rect.x & 0xFFFFu),
-                                           w, V.tiles_x);
-        }
-    }
-    const uint32_t end = wave_inclusive_sum(mine, lane);
-    s_end[threadIdx.x] = end;
-    const uint32_t total = lane_u(end, 63);
-    __syncthreads();
-    const uint32_t* ends = s_end + wbase;
-    for (uint32_t g0 = 0; g0 < total; g0 += 64) {
-        const uint32_t g = g0 + lane;
-        if (g < total) {
-            const uint32_t j = run_of(ends, g);
-            const uint32_t k = g - (j != 0 ? ends[j - 1] : 0u);
-            const uint64_t mask = s_mask[wbase + j];
-            if ((mask >> k) & 1u) {
-                const uint4 at = s_at[wbase + j];
-                const uint32_t dst = at.x + (uint32_t)__popcll(mask & ((1ull << k) - 1ull));
-                if (dst < cap) {
-                    const uint32_t q = small_row(k, at.z);
-                    keys[dst] = at.y + q * at.w + (k - q * at.z);
-                    values[dst] = blockIdx.x * kSmallThreads + wbase + j;
-                }
-            }
-        }
-    }
-}
-
 // Where each tile's pairs start and end in the sorted keys, by binary search
 // on the tile bits: splat_tiles_clear.slang:8-17 and splat_ranges.slang:11-25
 // give a tile with no pairs (0, 0), and so does this.
@@ -930,11 +782,375 @@ __global__ void __launch_bounds__(256) ranges_kernel(const uint32_t* __restrict_
     ranges[2 * t + 1] = start < end ? end : 0u;
 }
 
-__global__ void widen_keys_kernel(const uint32_t* __restrict__ tiles, const uint32_t* __restrict__ values,
-                                  const uint32_t* __restrict__ depth_keys, uint64_t pairs,
-                                  uint64_t* __restrict__ keys) {
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < pairs; i += (uint64_t)gridDim.x * blockDim.x) {
-        keys[i] = (uint64_t)tiles[i] << kDepthBits | (depth_keys[values[i]] & 0xFFFFFFu);
+// The footprints the projection left unsettled (the tile under the centre
+// not touched): a wavefront takes each in turn and tests its rectangle's
+// tiles 64 a step, row-major, until one is touched. None touched is the
+// reference's zero count: culled, no tile (splat_project.slang:764-776).
+__global__ void __launch_bounds__(256) settle_kernel(const DView* __restrict__ views, uint32_t n, uint32_t T,
+                                                     uint32_t slots, const float* __restrict__ records,
+                                                     uint32_t* __restrict__ touched, uint32_t* __restrict__ keys) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t word = i < slots ? touched[i] : 0u;
+    uint64_t todo = __ballot((word & kUnsettled) != 0u);
+    const uint32_t first = i - lane;
+    while (todo != 0) {
+        const uint32_t j = (uint32_t)__builtin_ctzll(todo);
+        todo &= todo - 1;
+        const uint32_t slot = first + j;
+        const BigFootprint g = big_footprint(views, n, T, records, slot);
+        const DView& V = views[g.view];
+        const uint32_t area = g.w * g.h;
+        bool any = false;
+        for (uint32_t k0 = 0; k0 < area && !any; k0 += 64) {
+            const uint32_t k = k0 + lane;
+            const uint32_t r = k / g.w;
+            const bool hit = k < area && tile_touched(V.width, V.height, T, g.cx, g.cy, g.a, g.b, g.c, g.ia, g.ic,
+                                                      g.cutoff, g.x0 + (k - r * g.w), g.y0 + r);
+            any = __ballot(hit) != 0;
+        }
+        if (lane == j) {
+            touched[slot] = any ? (word & ~kUnsettled) : 0u;
+            if (!any) {
+                keys[slot] = kCulledKey | ATHX_CULL_NO_TILE;
+            }
+        }
+    }
+}
+
+// One strip entry for every (row, strip) a drawn footprint's rectangle
+// covers, rank by rank in depth order, a footprint's entries row by row and
+// left to right: key = its global strip, value = the slot. A wavefront's 64
+// ranks own one contiguous run of entries (offsets holds every 64th rank's
+// first entry); the wavefront walks it 64 entries at a time, lane L taking
+// entry g0 + L and finding its rank by a binary search over the ranks' ends
+// in LDS, so the stores of a step are contiguous.
+constexpr uint32_t kEmitThreads = 256;
+
+__global__ void __launch_bounds__(kEmitThreads) strip_emit_kernel(const DView* __restrict__ views, uint32_t n,
+                                                                  const uint2* __restrict__ rects,
+                                                                  const uint32_t* __restrict__ sorted_slots,
+                                                                  const uint32_t* __restrict__ offsets,
+                                                                  const uint64_t* __restrict__ visible_dev,
+                                                                  uint32_t max_visible, uint32_t cap,
+                                                                  uint32_t* __restrict__ keys_out,
+                                                                  uint32_t* __restrict__ slots_out) {
+    __shared__ uint32_t s_end[kEmitThreads];
+    __shared__ uint4 s_rank[kEmitThreads];  // slot, first strip, strips a row, strips between rows
+    const uint32_t lane = threadIdx.x & 63u, wbase = threadIdx.x & ~63u;
+    const uint32_t visible = *visible_dev < max_visible ? (uint32_t)*visible_dev : max_visible;
+    const uint32_t k = blockIdx.x * kEmitThreads + threadIdx.x;
+    const uint32_t k0 = k - lane;
+    uint32_t count = 0;
+    if (k < visible) {
+        const uint32_t slot = sorted_slots[k];
+        const uint2 r = rects[slot];
+        const uint32_t x0 = r.x & 0xFFFFu, y0 = r.x >> 16, w = r.y & 0xFFFFu, h = r.y >> 16;
+        const DView& V = views[slot / n];
+        const uint32_t sx0 = x0 / kStripTiles;
+        const uint32_t cols = (x0 + w - 1u) / kStripTiles - sx0 + 1u;
+        count = h * cols;
+        s_rank[threadIdx.x] = make_uint4(slot, V.strip_base + y0 * V.strips_x + sx0, cols, V.strips_x);
+    }
+    const uint32_t end = wave_inclusive_sum(count, lane);
+    s_end[threadIdx.x] = end;
+    const uint32_t total = lane_u(end, 63);
+    const uint32_t base = k0 < visible ? offsets[k0 / 64u] : 0u;
+    __syncthreads();
+    const uint32_t* ends = s_end + wbase;
+    for (uint32_t g0 = 0; g0 < total; g0 += 64) {
+        const uint32_t g = g0 + lane;
+        if (g < total) {
+            const uint32_t j = run_of(ends, g);
+            const uint32_t q = g - (j != 0 ? ends[j - 1] : 0u);
+            const uint4 rk = s_rank[wbase + j];
+            const uint32_t row = q / rk.z, col = q - row * rk.z;
+            const uint64_t at = (uint64_t)base + g;
+            if (at < cap) {
+                keys_out[at] = rk.y + row * rk.w + col;
+                slots_out[at] = rk.x;
+            }
+        }
+    }
+}
+
+// A segment's descriptor, two words of uint4: (first entry, one past its
+// last, its first table cell = 16 * the strip's first segment + its index
+// in the strip, the strip's segments) and (view, tile row, first tile
+// column, strip). An empty one (first == last) is a segment past the total.
+__global__ void __launch_bounds__(256) seg_desc_kernel(StripBins b, uint4* __restrict__ desc, uint32_t strip_blocks) {
+    const uint64_t total = *b.seg_total;
+    const uint32_t segs = total < b.seg_cap ? (uint32_t)total : b.seg_cap;
+    if (blockIdx.x < strip_blocks) {
+        const uint32_t strip = blockIdx.x * 4u + (threadIdx.x >> 6), lane = threadIdx.x & 63u;
+        if (strip >= b.strips) {
+            return;
+        }
+        const uint32_t start = b.strip_ranges[2u * strip], end = b.strip_ranges[2u * strip + 1u];
+        const uint32_t nseg = div_up(end - start, kStripSeg);
+        if (nseg == 0) {
+            return;
+        }
+        const uint32_t base = b.segbase[strip];
+        const bool fits = (uint64_t)base + nseg <= segs;
+        uint32_t lo = 0, hi = b.nviews;
+        while (hi - lo > 1u) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (b.strip_bases[mid] <= strip) {
+                lo = mid;
+            } else {
+                hi = mid;
+            }
+        }
+        const DView& V = b.views[lo];
+        const uint32_t local = strip - V.strip_base;
+        const uint32_t ty = local / V.strips_x;
+        const uint32_t tx0 = (local - ty * V.strips_x) * kStripTiles;
+        for (uint32_t k = lane; k < nseg; k += 64u) {
+            if ((uint64_t)base + k < b.seg_cap) {
+                const uint32_t first = start + k * kStripSeg;
+                const uint32_t last = end - first < kStripSeg ? end : first + kStripSeg;
+                desc[2u * (base + k)] = fits ? make_uint4(first, last, kStripTiles * base + k, nseg) : make_uint4(0, 0, 0, 0);
+                desc[2u * (base + k) + 1u] = make_uint4(lo, ty, tx0, strip);
+            }
+        }
+    } else {
+        const uint32_t stride = (gridDim.x - strip_blocks) * blockDim.x;
+        for (uint32_t i = segs + (blockIdx.x - strip_blocks) * blockDim.x + threadIdx.x; i < b.seg_cap; i += stride) {
+            desc[2u * i] = make_uint4(0, 0, 0, 0);
+        }
+    }
+}
+
+// An entry's footprint as the strip test reads it.
+struct StripFootprint {
+    float4 screen;  // centre x, y, view z, cut power
+    float4 conic;   // a, b, c, alpha
+    uint32_t x0, x1;
+};
+
+__device__ inline StripFootprint strip_footprint(const StripBins& b, uint32_t slot) {
+    StripFootprint f;
+    const float4* rec = reinterpret_cast<const float4*>(b.records + (size_t)slot * kRecordFloats);
+    f.screen = rec[0];
+    f.conic = rec[1];
+    const uint2 r = b.rects[slot];
+    f.x0 = r.x & 0xFFFFu;
+    f.x1 = f.x0 + (r.y & 0xFFFFu) - 1u;
+    return f;
+}
+
+// Which of the strip's tiles the footprint touches (bit j: tile tx0 + j):
+// frame.slang:374-379's test of each tile its rectangle holds in the strip,
+// the row's half of it taken once.
+__device__ inline uint32_t strip_hits(const StripFootprint& sf, const DView& V, uint32_t T, uint32_t ty,
+                                      uint32_t tx0) {
+    BigFootprint f{};
+    f.cx = sf.screen.x;
+    f.cy = sf.screen.y;
+    f.cutoff = sf.screen.w;
+    f.a = sf.conic.x;
+    f.b = sf.conic.y;
+    f.c = sf.conic.z;
+    f.ia = recip_of(f.a);
+    f.ic = recip_of(f.c);
+    const uint32_t c0 = sf.x0 > tx0 ? sf.x0 : tx0;
+    const uint32_t c1 = sf.x1 < tx0 + kStripTiles - 1u ? sf.x1 : tx0 + kStripTiles - 1u;
+    const float b2 = 2.0f * f.b;
+    const BoxAxis Y = row_of(f, V, T, b2, ty);
+    uint32_t mask = 0;
+    for (uint32_t tx = c0; tx <= c1; ++tx) {
+        if (box_hit(f, b2, column_of(f, V, T, b2, tx), Y)) {
+            mask |= 1u << (tx - tx0);
+        }
+    }
+    return mask;
+}
+
+// A segment's 1024 entries go 4 to a lane: entry j * 256 + thread of the
+// segment, so each step j of the workgroup is 256 consecutive entries.
+constexpr uint32_t kSegThreads = 256;
+constexpr uint32_t kSegPer = kStripSeg / kSegThreads;
+
+// Tests the segment's entries and counts each strip tile's hits:
+// table[16 base + j nseg + k] for tile j of segment k of a strip whose
+// first segment is base, so the table read in order is strip by strip,
+// tile by tile, segment by segment -- the order of the tile-sorted pairs.
+// Every entry's slot, then every footprint, is loaded before any is tested.
+__global__ void __launch_bounds__(kSegThreads) strip_count_kernel(StripBins b) {
+    __shared__ uint32_t s_hits[kSegThreads / 64][kStripTiles];
+    const uint4 d0 = b.seg_desc[2u * blockIdx.x];
+    if (d0.x >= d0.y) {
+        return;  // past the segments: the same for the whole workgroup
+    }
+    const uint4 d1 = b.seg_desc[2u * blockIdx.x + 1u];
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const DView& V = b.views[d1.x];
+    uint32_t slot[kSegPer];
+#pragma unroll
+    for (uint32_t j = 0; j < kSegPer; ++j) {
+        const uint32_t e = d0.x + j * kSegThreads + threadIdx.x;
+        slot[j] = e < d0.y ? b.strip_slots[e] : ~0u;
+    }
+    StripFootprint f[kSegPer];
+#pragma unroll
+    for (uint32_t j = 0; j < kSegPer; ++j) {
+        f[j] = strip_footprint(b, slot[j] != ~0u ? slot[j] : 0u);
+    }
+    uint32_t count[kStripTiles];
+#pragma unroll
+    for (uint32_t t = 0; t < kStripTiles; ++t) {
+        count[t] = 0;
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < kSegPer; ++j) {
+        const uint32_t e = d0.x + j * kSegThreads + threadIdx.x;
+        uint32_t mask = 0;
+        if (slot[j] != ~0u) {
+            mask = strip_hits(f[j], V, b.tile, d1.y, d1.z);
+            b.masks[e] = (uint16_t)mask;
+        }
+#pragma unroll
+        for (uint32_t t = 0; t < kStripTiles; ++t) {
+            count[t] += (uint32_t)__popcll(__ballot((mask >> t) & 1u));
+        }
+    }
+    if (lane == 0) {
+#pragma unroll
+        for (uint32_t t = 0; t < kStripTiles; ++t) {
+            s_hits[wave][t] = count[t];
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < kStripTiles) {
+        uint32_t sum = 0;
+#pragma unroll
+        for (uint32_t w = 0; w < kSegThreads / 64; ++w) {
+            sum += s_hits[w][threadIdx.x];
+        }
+        b.table[d0.z + threadIdx.x * d0.w] = sum;
+    }
+}
+
+// Writes each hit's slot at its place: the scanned table's start of (tile
+// t, this segment), plus the hits of tile t at earlier entries of the
+// segment (earlier entries are nearer: the strip's entries are in depth
+// order). A tile's hits of one step of a wavefront land side by side.
+__global__ void __launch_bounds__(kSegThreads) strip_pairs_kernel(StripBins b) {
+    __shared__ uint32_t s_hits[kSegPer][kSegThreads / 64][kStripTiles];
+    __shared__ uint32_t s_base[kStripTiles];
+    const uint4 d0 = b.seg_desc[2u * blockIdx.x];
+    if (d0.x >= d0.y) {
+        return;
+    }
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint64_t lt_mask = (1ull << lane) - 1ull;
+    uint32_t mask[kSegPer], slot[kSegPer];
+#pragma unroll
+    for (uint32_t j = 0; j < kSegPer; ++j) {
+        const uint32_t e = d0.x + j * kSegThreads + threadIdx.x;
+        mask[j] = e < d0.y ? b.masks[e] : 0u;
+        slot[j] = e < d0.y ? b.strip_slots[e] : 0u;
+    }
+    if (threadIdx.x < kStripTiles) {
+        s_base[threadIdx.x] = b.table[d0.z + threadIdx.x * d0.w];
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < kSegPer; ++j) {
+#pragma unroll
+        for (uint32_t t = 0; t < kStripTiles; ++t) {
+            const uint32_t hits = (uint32_t)__popcll(__ballot((mask[j] >> t) & 1u));
+            if (lane == 0) {
+                s_hits[j][wave][t] = hits;
+            }
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < kStripTiles) {
+        // Exclusive prefix over (step, wavefront), the entries' order.
+        uint32_t run = s_base[threadIdx.x];
+#pragma unroll
+        for (uint32_t j = 0; j < kSegPer; ++j) {
+#pragma unroll
+            for (uint32_t w = 0; w < kSegThreads / 64; ++w) {
+                const uint32_t h = s_hits[j][w][threadIdx.x];
+                s_hits[j][w][threadIdx.x] = run;
+                run += h;
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (uint32_t j = 0; j < kSegPer; ++j) {
+#pragma unroll
+        for (uint32_t t = 0; t < kStripTiles; ++t) {
+            const bool mine = (mask[j] >> t) & 1u;
+            const uint64_t bal = __ballot(mine);
+            if (mine) {
+                const uint32_t at = s_hits[j][wave][t] + (uint32_t)__popcll(bal & lt_mask);
+                if (at < b.pair_cap) {
+                    b.pair_slots[at] = slot[j];
+                }
+            }
+        }
+    }
+}
+
+// Each tile's pairs from the scanned table: [table(j, 0), table(j + 1, 0))
+// of its strip, the strip's next tile's start being the one past its last
+// segment; (0, 0) for a tile with none, as splat_tiles_clear.slang:8-17 and
+// splat_ranges.slang:11-25 leave it. Clamped to the pair capacity.
+__global__ void __launch_bounds__(256) tile_ranges_kernel(StripBins b, const uint32_t* __restrict__ tile_bases,
+                                                          uint32_t tiles, uint32_t* __restrict__ ranges) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= tiles) {
+        return;
+    }
+    uint32_t vlo = 0, vhi = b.nviews;
+    while (vhi - vlo > 1u) {
+        const uint32_t mid = (vlo + vhi) >> 1;
+        if (tile_bases[mid] <= t) {
+            vlo = mid;
+        } else {
+            vhi = mid;
+        }
+    }
+    const DView& V = b.views[vlo];
+    const uint32_t local = t - V.tile_base;
+    const uint32_t ty = local / V.tiles_x, tx = local - ty * V.tiles_x;
+    const uint32_t strip = V.strip_base + ty * V.strips_x + tx / kStripTiles, j = tx % kStripTiles;
+    const uint64_t total = *b.seg_total;
+    const uint32_t segs = total < b.seg_cap ? (uint32_t)total : b.seg_cap;
+    const uint32_t base = b.segbase[strip];
+    const uint32_t next = strip + 1u < b.strips ? b.segbase[strip + 1u] : (uint32_t)total;
+    const uint32_t nseg = next - base;
+    uint32_t start = 0, end = 0;
+    if (nseg != 0 && (uint64_t)base + nseg <= segs) {
+        const uint64_t pairs = *b.pair_total;
+        const uint32_t i0 = kStripTiles * base + j * nseg, i1 = i0 + nseg;
+        start = b.table[i0];
+        end = i1 < kStripTiles * segs ? b.table[i1] : (uint32_t)pairs;
+        start = start < b.pair_cap ? start : b.pair_cap;
+        end = end < b.pair_cap ? end : b.pair_cap;
+        if (start >= end) {
+            start = 0;
+            end = 0;
+        }
+    }
+    ranges[2u * t] = start;
+    ranges[2u * t + 1u] = end;
+}
+
+// The reference's 64-bit keys of the tile-sorted pairs (tile << 24 | depth
+// key), a workgroup a tile over its range.
+__global__ void __launch_bounds__(256) pair_keys_kernel(const uint32_t* __restrict__ ranges,
+                                                        const uint32_t* __restrict__ values,
+                                                        const uint32_t* __restrict__ depth_keys,
+                                                        uint64_t* __restrict__ keys) {
+    const uint32_t t = blockIdx.x;
+    const uint32_t start = ranges[2u * t], end = ranges[2u * t + 1u];
+    for (uint32_t i = start + threadIdx.x; i < end; i += blockDim.x) {
+        keys[i] = (uint64_t)t << kDepthBits | (depth_keys[values[i]] & 0xFFFFFFu);
     }
 }
 
@@ -980,17 +1196,16 @@ __global__ void __launch_bounds__(256) counters_kernel(const uint32_t* __restric
 }  // namespace
 
 hipError_t launch_project(const ProjectArgs& a, const DView* views_dev, uint32_t nviews, float* records,
-                          uint32_t* touched, uint32_t* depth_keys, const Footprints& fp, uint32_t* status_words,
-                          hipStream_t s) {
+                          uint32_t* touched, uint32_t* depth_keys, uint2* rects, hipStream_t s) {
     if (a.n == 0 || nviews == 0) {
-        return launch_big_list(fp, 0, status_words, s);
+        return hipSuccess;
     }
     const dim3 grid(div_up(a.n, 256));
     // 16-byte loads of whole coefficient rows where the rows are 16 coefficients
     // a channel and the array is 16-byte aligned (torch's allocations are).
     const void* shp = a.sh_half ? static_cast<const void*>(a.sh_half) : static_cast<const void*>(a.sh);
     const bool vec = !a.packed && a.sh_coeffs == 16 && (reinterpret_cast<uintptr_t>(shp) & 15u) == 0;
-    using Kernel = void (*)(ProjectArgs, const DView*, uint32_t, float*, uint32_t*, uint32_t*, Footprints);
+    using Kernel = void (*)(ProjectArgs, const DView*, uint32_t, float*, uint32_t*, uint32_t*, uint2*);
     const Kernel kernels[5][2] = {{project_kernel<kInSoa32, false>, project_kernel<kInSoa32, true>},
                                   {project_kernel<kInSoa32Vec, false>, project_kernel<kInSoa32Vec, true>},
                                   {project_kernel<kInSoa16, false>, project_kernel<kInSoa16, true>},
@@ -998,15 +1213,30 @@ hipError_t launch_project(const ProjectArgs& a, const DView* views_dev, uint32_t
                                   {project_kernel<kInPacked, false>, project_kernel<kInPacked, true>}};
     const uint32_t in = a.packed ? kInPacked : (a.sh_half ? 2u : 0u) + (vec ? 1u : 0u);
     const Kernel kernel = kernels[in][a.relit ? 1 : 0];
-    hipLaunchKernelGGL(kernel, grid, dim3(256), 0, s, a, views_dev, nviews, records, touched, depth_keys, fp);
+    hipLaunchKernelGGL(kernel, grid, dim3(256), 0, s, a, views_dev, nviews, records, touched, depth_keys, rects);
     const uint32_t slots = a.n * nviews;
+    hipLaunchKernelGGL(settle_kernel, dim3(div_up(slots, 256)), dim3(256), 0, s, views_dev, a.n, a.tile, slots,
+                       records, touched, depth_keys);
+    return hipGetLastError();
+}
+
+hipError_t launch_exact_counts(const DView* views_dev, uint32_t nviews, uint32_t n, uint32_t tile,
+                               const float* records, uint32_t* counts, uint32_t* depth_keys, const Footprints& fp,
+                               uint32_t* status_words, hipStream_t s) {
+    const uint32_t slots = n * nviews;
+    if (slots == 0) {
+        return launch_big_list(fp, 0, status_words, s);
+    }
+    if (hipError_t e = launch_fill_u32(counts, 0u, slots, s); e != hipSuccess) {
+        return e;
+    }
     hipLaunchKernelGGL(count_small_kernel, dim3(div_up(slots, kSmallThreads)), dim3(kSmallThreads), 0, s, views_dev,
-                       a.n, slots, a.tile, records, fp, touched, depth_keys);
+                       n, slots, tile, records, fp, counts, depth_keys);
     if (hipError_t e = launch_big_list(fp, slots, status_words, s); e != hipSuccess) {
         return e;
     }
-    hipLaunchKernelGGL(count_big_kernel, dim3(kBigBlocks), dim3(kBigThreads), 0, s, views_dev, a.n, a.tile, records,
-                       fp, touched, depth_keys);
+    hipLaunchKernelGGL(count_big_kernel, dim3(kBigBlocks), dim3(kBigThreads), 0, s, views_dev, n, tile, records, fp,
+                       counts, depth_keys);
     return hipGetLastError();
 }
 
@@ -1022,20 +1252,6 @@ hipError_t launch_fill_u32(uint32_t* p, uint32_t value, uint64_t n, hipStream_t 
     return hipGetLastError();
 }
 
-hipError_t launch_emit(const DView* views_dev, uint32_t nviews, uint32_t n, uint32_t tile, const float* records,
-                       const uint32_t* touched, const uint32_t* offsets, const Footprints& fp, uint32_t cap,
-                       uint32_t* keys, uint32_t* values, hipStream_t s) {
-    const uint64_t slots = (uint64_t)n * nviews;
-    if (slots == 0 || cap == 0) {
-        return hipSuccess;
-    }
-    hipLaunchKernelGGL(emit_small_kernel, dim3(div_up((uint32_t)slots, kSmallThreads)), dim3(kSmallThreads), 0, s,
-                       views_dev, n, (uint32_t)slots, touched, offsets, fp, cap, keys, values);
-    hipLaunchKernelGGL(emit_big_kernel, dim3(kBigBlocks), dim3(kBigThreads), 0, s, views_dev, n, tile, records, fp,
-                       touched, offsets, cap, keys, values);
-    return hipGetLastError();
-}
-
 hipError_t launch_ranges(const uint32_t* sorted_keys, const uint64_t* n_dev, uint32_t cap, uint32_t tiles,
                          uint32_t* ranges, hipStream_t s) {
     if (tiles == 0) {
@@ -1043,17 +1259,6 @@ hipError_t launch_ranges(const uint32_t* sorted_keys, const uint64_t* n_dev, uin
     }
     hipLaunchKernelGGL(ranges_kernel, dim3(div_up(tiles, 256)), dim3(256), 0, s, sorted_keys, n_dev, cap, tiles,
                        ranges);
-    return hipGetLastError();
-}
-
-hipError_t launch_widen_keys(const uint32_t* sorted_tiles, const uint32_t* values, const uint32_t* depth_keys,
-                             uint64_t pairs, uint64_t* keys, hipStream_t s) {
-    if (pairs == 0) {
-        return hipSuccess;
-    }
-    const uint64_t blocks = (pairs + 255) / 256 < 4096 ? (pairs + 255) / 256 : 4096;
-    hipLaunchKernelGGL(widen_keys_kernel, dim3((uint32_t)blocks), dim3(256), 0, s, sorted_tiles, values, depth_keys,
-                       pairs, keys);
     return hipGetLastError();
 }
 
@@ -1067,6 +1272,62 @@ hipError_t launch_counters(const uint32_t* depth_keys, const uint32_t* touched, 
         blocks = 1024;
     }
     hipLaunchKernelGGL(counters_kernel, dim3((uint32_t)blocks), dim3(256), 0, s, depth_keys, touched, slots, words);
+    return hipGetLastError();
+}
+
+hipError_t launch_strip_emit(const DView* views_dev, uint32_t n, const uint2* rects, const uint32_t* sorted_slots,
+                             const uint32_t* offsets, const uint64_t* visible_dev, uint32_t max_visible, uint32_t cap,
+                             uint32_t* strip_keys, uint32_t* strip_slots, hipStream_t s) {
+    if (max_visible == 0 || cap == 0) {
+        return hipSuccess;
+    }
+    hipLaunchKernelGGL(strip_emit_kernel, dim3(div_up(max_visible, kEmitThreads)), dim3(kEmitThreads), 0, s,
+                       views_dev, n, rects, sorted_slots, offsets, visible_dev, max_visible, cap, strip_keys,
+                       strip_slots);
+    return hipGetLastError();
+}
+
+hipError_t launch_seg_desc(const StripBins& b, uint4* desc, hipStream_t s) {
+    if (b.seg_cap == 0 || b.strips == 0) {
+        return hipSuccess;
+    }
+    const uint32_t strip_blocks = div_up(b.strips, 4);
+    const uint32_t tail_blocks = div_up(b.seg_cap, 256) < 1024 ? div_up(b.seg_cap, 256) : 1024;
+    hipLaunchKernelGGL(seg_desc_kernel, dim3(strip_blocks + tail_blocks), dim3(256), 0, s, b, desc, strip_blocks);
+    return hipGetLastError();
+}
+
+hipError_t launch_strip_count(const StripBins& b, hipStream_t s) {
+    if (b.seg_cap == 0) {
+        return hipSuccess;
+    }
+    hipLaunchKernelGGL(strip_count_kernel, dim3(b.seg_cap), dim3(kSegThreads), 0, s, b);
+    return hipGetLastError();
+}
+
+hipError_t launch_strip_pairs(const StripBins& b, hipStream_t s) {
+    if (b.seg_cap == 0) {
+        return hipSuccess;
+    }
+    hipLaunchKernelGGL(strip_pairs_kernel, dim3(b.seg_cap), dim3(kSegThreads), 0, s, b);
+    return hipGetLastError();
+}
+
+hipError_t launch_tile_ranges(const StripBins& b, const uint32_t* tile_bases, uint32_t tiles, uint32_t* ranges,
+                              hipStream_t s) {
+    if (tiles == 0) {
+        return hipSuccess;
+    }
+    hipLaunchKernelGGL(tile_ranges_kernel, dim3(div_up(tiles, 256)), dim3(256), 0, s, b, tile_bases, tiles, ranges);
+    return hipGetLastError();
+}
+
+hipError_t launch_pair_keys(const uint32_t* ranges, uint32_t tiles, const uint32_t* values,
+                            const uint32_t* depth_keys, uint64_t* keys, hipStream_t s) {
+    if (tiles == 0) {
+        return hipSuccess;
+    }
+    hipLaunchKernelGGL(pair_keys_kernel, dim3(tiles), dim3(256), 0, s, ranges, values, depth_keys, keys);
     return hipGetLastError();
 }
 

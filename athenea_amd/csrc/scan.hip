@@ -18,6 +18,13 @@
 //   BigScan      the footprints over kBigRect tiles, listed in slot order with
 //                the first of their hit-word steps: one 64-bit sum, the count
 //                in the high half and the steps in the low.
+//   EntryScan    the strip entries of the depth-sorted slots (the frame's
+//                binning, in place of the reference's tile counts): only
+//                every 64th rank's offset is kept, one a wavefront of the
+//                strip emission.
+//   SegScan      each strip's segments (kStripSeg entries each).
+//   TableScan    the binning's hit counts in place, their length read on the
+//                device: the pair total and every (tile, segment)'s start.
 #include "athx_internal.h"
 
 namespace athx {
@@ -113,6 +120,42 @@ struct GatherScan {
     __device__ uint32_t size() const { return *n_dev < cap ? (uint32_t)*n_dev : cap; }
     __device__ uint32_t load(uint32_t k) const { return touched[sorted_slots[k]]; }
     __device__ void store(uint32_t k, uint32_t, uint64_t excl) const { slot_offsets[sorted_slots[k]] = (uint32_t)excl; }
+};
+
+struct EntryScan {
+    using Value = uint32_t;
+    const uint32_t* touched;
+    const uint32_t* sorted_slots;
+    uint32_t* offsets;  // [ranks / 64]
+    const uint64_t* n_dev;
+    uint32_t cap;
+    __device__ uint32_t size() const { return *n_dev < cap ? (uint32_t)*n_dev : cap; }
+    __device__ uint32_t load(uint32_t k) const { return touched[sorted_slots[k]]; }
+    __device__ void store(uint32_t k, uint32_t, uint64_t excl) const {
+        if ((k & 63u) == 0) {
+            offsets[k >> 6] = (uint32_t)excl;
+        }
+    }
+};
+
+struct SegScan {
+    using Value = uint32_t;
+    const uint32_t* ranges;
+    uint32_t* segbase;
+    uint32_t n;
+    __device__ uint32_t size() const { return n; }
+    __device__ uint32_t load(uint32_t s) const { return div_up(ranges[2u * s + 1u] - ranges[2u * s], kStripSeg); }
+    __device__ void store(uint32_t s, uint32_t, uint64_t excl) const { segbase[s] = (uint32_t)excl; }
+};
+
+struct TableScan {
+    using Value = uint32_t;
+    uint32_t* table;
+    const uint64_t* segs_dev;
+    uint32_t seg_cap;
+    __device__ uint32_t size() const { return kStripTiles * (*segs_dev < seg_cap ? (uint32_t)*segs_dev : seg_cap); }
+    __device__ uint32_t load(uint32_t i) const { return table[i]; }
+    __device__ void store(uint32_t i, uint32_t, uint64_t excl) const { table[i] = (uint32_t)excl; }
 };
 
 // status[0] is the ticket counter; status[1 + p] partition p's word. The grid
@@ -256,6 +299,23 @@ hipError_t launch_compact(const uint32_t* touched, const uint32_t* depth_keys, u
 
 hipError_t launch_big_list(const Footprints& fp, uint32_t slots, uint32_t* status_words, hipStream_t s) {
     return run_scan(BigScan{fp.rects, fp.big_list, fp.big_mask_base, slots}, slots, fp.big_total, status_words, s);
+}
+
+hipError_t launch_entry_scan(const uint32_t* touched, const uint32_t* sorted_slots, const uint64_t* visible_dev,
+                             uint32_t max_visible, uint32_t* offsets, uint64_t* entry_total, uint32_t* status_words,
+                             hipStream_t s) {
+    return run_scan(EntryScan{touched, sorted_slots, offsets, visible_dev, max_visible}, max_visible, entry_total,
+                    status_words, s);
+}
+
+hipError_t launch_seg_scan(const uint32_t* strip_ranges, uint32_t strips, uint32_t* segbase, uint64_t* seg_total,
+                           uint32_t* status_words, hipStream_t s) {
+    return run_scan(SegScan{strip_ranges, segbase, strips}, strips, seg_total, status_words, s);
+}
+
+hipError_t launch_table_scan(uint32_t* table, const uint64_t* seg_total, uint32_t seg_cap, uint64_t* pair_total,
+                             uint32_t* status_words, hipStream_t s) {
+    return run_scan(TableScan{table, seg_total, seg_cap}, kStripTiles * seg_cap, pair_total, status_words, s);
 }
 
 hipError_t launch_gather_scan(const uint32_t* touched, const uint32_t* sorted_slots, const uint64_t* visible_dev,

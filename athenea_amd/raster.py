@@ -500,7 +500,8 @@ class TileRasterizer:
     def frame_info(self) -> dict:
         info = _lib.FrameInfo()
         self._check(self._lib.athx_frame_info_get(self._ctx, ctypes.byref(info)))
-        return {"slots": info.slots, "tiles": info.tiles, "key_bits": info.key_bits, "pairs": info.pairs}
+        return {"slots": info.slots, "tiles": info.tiles, "key_bits": info.key_bits, "pairs": info.pairs,
+                "strips": info.strips, "entries": info.entries}
 
     def frame_buffers(self, device=None) -> dict[str, torch.Tensor]:
         """Copies of the last frame's per-slot and per-pair buffers (athx_copy_frame):

@@ -3,8 +3,8 @@
 A step is one forward frame of the 1080p workload (1.4M Gaussians, SH degree
 3, FP32, 1920x1080, 16x16 tiles, each Gaussian with relight attributes --
 albedo, roughness, metallic, a stored normal -- under a prepared 64x32
-log-normal HDR sky): projection with the relit colour, the depth sort, pair emission,
-the tile sort, tile ranges and the blend, with the Gaussians and the sky
+log-normal HDR sky): projection with the relit colour, the depth sort, the binning of
+(tile, Gaussian) pairs through 16-tile strips, tile ranges and the blend, with the Gaussians and the sky
 already resident in HBM. Prints ONE JSON line:
 
   value         forward ms/frame (lower is better)
@@ -40,45 +40,49 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def stage_bytes(n: int, visible: int, pairs: int, tiles: int, pixels: int, key_bits: int,
+def stage_bytes(n: int, visible: int, pairs: int, entries: int, strips: int, tiles: int, pixels: int,
                 relit: bool = False) -> dict:
     """Compulsory HBM bytes per stage (one view): what each stage must read
     and write whatever its implementation."""
     depth_passes = 3
-    tile_passes = (key_bits - 24 + 7) // 8
+    strip_passes = (max(strips - 1, 1).bit_length() + 7) // 8
+    segments = entries // 256 + strips
     return {
         # FP32 parameters read (236 B at SH degree 3, + 32 B of relight
-        # attributes: albedo, normal, metallic, roughness); record, tile count
-        # and key written
-        "project": n * (236 + (32 if relit else 0) + 48 + 4 + 4),
+        # attributes: albedo, normal, metallic, roughness); record, depth key,
+        # strip-entry count and tile rectangle written
+        "project": n * (236 + (32 if relit else 0) + 48 + 4 + 4 + 8),
         # compaction (counts and keys read, visible (key, slot) written), then
         # a stable 24-bit sort of the visible u32 keys: a histogram read and a
         # read and write of (key, slot) a pass
         "depth_sort": n * 8 + visible * 8 + visible * 4 + depth_passes * visible * 16,
-        # sorted slots read, counts gathered, offsets scattered to slots
-        "counts": visible * 12,
-        # count, offset and the footprint's tile hits (rect + mask) read a
-        # slot; (tile, slot) pairs written. The depth half of the reference's
-        # 64-bit key is the emission order itself, so it is not stored.
-        "emit": visible * (4 + 4 + 16) + pairs * 8,
-        # a histogram read of the u32 tile keys, then a read and a write of
-        # (tile, slot) a pass
-        "tile_sort": pairs * 4 + tile_passes * pairs * 16,
+        # sorted slots read, entry counts gathered
+        "counts": visible * 8,
+        # sorted slot, rectangle and offset read a rank; (strip, slot) written
+        # an entry; the strip sort: a histogram read, then a read and a write
+        # of (strip, slot) a pass
+        "emit": visible * (4 + 8) + entries * 8 + entries * 4 + strip_passes * entries * 16,
+        # an entry's slot, its footprint's record half and rectangle read and
+        # its hit mask written (count); the table written, scanned (read and
+        # written) and read; the slot and mask read again and each pair's slot
+        # written at its place
+        "tile_sort": entries * (4 + 32 + 8 + 2) + segments * 16 * 4 * 4 + entries * (4 + 2) + pairs * 4,
         "ranges": tiles * 8,
-        # each pair's value and record read once; RGBA, depth, final T, last index written
-        "blend": pairs * (4 + 48) + tiles * 8 + pixels * 28,
+        # each pair's slot read, each drawn record once (the blend's re-reads
+        # of a record hit the caches); RGBA, depth, final T, last index written
+        "blend": pairs * 4 + visible * 48 + pixels * 28,
     }
 
 
 # The kernels each timed stage launches, in order.
 STAGE_KERNELS = {
-    "project": "project_kernel, count_small_kernel, the big-footprint listing scan, count_big_kernel",
+    "project": "project_kernel, settle_kernel",
     "depth_sort": "the visible compaction scan, histogram_kernel, hist_reduce_kernel, onesweep_kernel x3",
-    "counts": "the gather scan",
-    "emit": "emit_small_kernel, emit_big_kernel",
-    "tile_sort": "per pass: upsweep_kernel, the digit-count scan, downsweep_kernel (reduce-scan-scatter, "
-                 "2 passes of the 13-bit tile key)",
-    "ranges": "ranges_kernel",
+    "counts": "the strip-entry scan in depth order",
+    "emit": "strip_emit_kernel, the strip sort (histogram, onesweep or reduce-scan-scatter passes), "
+            "ranges_kernel over the strips, the segment scan",
+    "tile_sort": "strip_count_kernel, the table scan, strip_pairs_kernel",
+    "ranges": "tile_ranges_kernel",
     "blend": "blend_kernel",
 }
 
@@ -197,7 +201,8 @@ def main() -> int:
     stages = {k: statistics.mean(v) for k, v in stage_samples.items()}
     counters = raster.latest_counters()
     tiles = info["tiles"]
-    sbytes = stage_bytes(cloud.count, counters.visible, pairs, tiles, W * H, info["key_bits"], relit=True)
+    sbytes = stage_bytes(cloud.count, counters.visible, pairs, info["entries"], info["strips"], tiles, W * H,
+                         relit=True)
     dominant = max(stages, key=stages.get)
     achieved = sbytes[dominant] / (stages[dominant] * 1e-3)
     roofline = {
@@ -234,9 +239,10 @@ def main() -> int:
         },
         "fwd_sh_ms": round(fwd_sh_ms, 3),
         "pairs": pairs,
+        "strip_entries": info["entries"],
         "visible": counters.visible,
         "stages_ms": {k: round(v, 4) for k, v in stages.items()},
-        "sort_keys_per_s": round(pairs / (stages["tile_sort"] * 1e-3)),
+        "pairs_binned_per_s": round(pairs / ((stages["emit"] + stages["tile_sort"]) * 1e-3)),
         "roofline": roofline,
     }
 

@@ -229,8 +229,11 @@ typedef struct athx_targets {
 
 /* FrameStats (TileRasterizer.h:408-419), summed over the views. The stage
  * times (filled with time_stages) are HIP-event times on the frame's stream:
- * projection; compaction + depth sort; tile counts gathered and scanned in
- * depth order; emission; tile sort; ranges; blend. */
+ * projection (with the no-tile cull settled); compaction + depth sort; the
+ * strip entries scanned in depth order; their emission, sorted on the strip
+ * (project_ms .. emit_ms); the binning of the entries into tile-sorted pairs
+ * in place of the reference's sort of the pair keys (tile_sort_ms); tile
+ * ranges; blend. */
 typedef struct athx_frame_stats {
     uint32_t splats;
     uint32_t visible;
@@ -376,6 +379,8 @@ typedef struct athx_frame_info {
     uint32_t tiles;
     uint32_t key_bits;
     uint64_t pairs;
+    uint32_t strips;   /* 16-tile strips of tile rows the pairs are binned through */
+    uint64_t entries;  /* (footprint, strip) entries of the binning */
 } athx_frame_info;
 
 /* The last frame's sizes; pairs is what its buffers hold (waits for an async frame). */
