@@ -151,11 +151,14 @@ hipError_t launch_project(const ProjectArgs& a, const DView* views_dev, uint32_t
 hipError_t launch_exact_counts(const DView* views_dev, uint32_t nviews, uint32_t n, uint32_t tile,
                                const float* records, uint32_t* counts, uint32_t* depth_keys, const Footprints& fp,
                                uint32_t* status_words, hipStream_t s);
-// The strip entries of the depth-sorted slots: rank k's entries start at
-// offsets[k]; entries at or past cap are not written.
+// The strip entries of the depth-sorted slots (key: strip, value: depth
+// rank), rank k's from offsets[k / 64] on (every 64th rank's first entry),
+// entries at or past cap not written; and each rank's footprint, 32 bytes
+// (fp[2k], fp[2k + 1]) the binning reads.
 hipError_t launch_strip_emit(const DView* views_dev, uint32_t n, const uint2* rects, const uint32_t* sorted_slots,
                              const uint32_t* offsets, const uint64_t* visible_dev, uint32_t max_visible, uint32_t cap,
-                             uint32_t* strip_keys, uint32_t* strip_slots, hipStream_t s);
+                             const float* records, uint4* fp, uint32_t* strip_keys, uint32_t* strip_ranks,
+                             hipStream_t s);
 // The strip-sorted entries binned into tile-sorted pairs.
 struct StripBins {
     const DView* views;
@@ -166,7 +169,8 @@ struct StripBins {
     uint32_t strips;
     const float* records;
     const uint2* rects;
-    const uint32_t* strip_slots;   // [entries] strip-sorted entries' slots
+    uint32_t* strip_slots;         // [entries] strip-sorted entries' depth ranks, made slots by the count
+    const uint4* fp;               // [visible][2] each rank's footprint (launch_strip_emit)
     const uint32_t* strip_ranges;  // [strips][2] each strip's entries (0, 0 when none)
     const uint32_t* segbase;       // [strips] first segment of each strip
     const uint4* seg_desc;         // [seg_cap][2] each segment's entries, table column, strip place

@@ -165,6 +165,39 @@ __device__ inline float wave_sum_f(float x) {
     return (r0 + r1) + (r2 + r3);
 }
 
+__device__ inline float as_float(uint32_t u) { return __builtin_bit_cast(float, u); }
+__device__ inline uint32_t as_uint(float f) { return __builtin_bit_cast(uint32_t, f); }
+
+// Eight sums over a wavefront's 64 lanes at once, every lane active (a
+// reduce-scatter): a swap of the wave's halves (v_permlane32_swap) pairs
+// value i with value i + 4 so each lane adds one of them over lanes l and
+// l + 32; a swap of rows within each half (v_permlane16_swap) does the same
+// for the four values left, over l and l + 16; a rotation by 8 within each
+// row of 16, and three DPP steps over each group of 8 lanes, finish. Sum k
+// lands in lane 8k. 26 VALU operations, where eight whole-wave sums take 88.
+__device__ inline float wave_sum8(const float v[8], uint32_t lane) {
+    float r1[4];
+#pragma unroll
+    for (uint32_t i = 0; i < 4; ++i) {
+        const auto p = __builtin_amdgcn_permlane32_swap(as_uint(v[i]), as_uint(v[i + 4]), false, false);
+        r1[i] = as_float(p[0]) + as_float(p[1]);  // lanes 0-31: value i; 32-63: value i + 4
+    }
+    float r2[2];
+#pragma unroll
+    for (uint32_t i = 0; i < 2; ++i) {
+        const auto p = __builtin_amdgcn_permlane16_swap(as_uint(r1[i]), as_uint(r1[i + 2]), false, false);
+        r2[i] = as_float(p[0]) + as_float(p[1]);  // row R of 16 lanes: value i + 2R
+    }
+    const bool upper = (lane & 8u) != 0;
+    const float mine = upper ? r2[1] : r2[0];
+    const float other = upper ? r2[0] : r2[1];
+    float x = mine + as_float((uint32_t)__builtin_amdgcn_mov_dpp((int)as_uint(other), 0x128, 0xF, 0xF, false));
+    x = x + as_float((uint32_t)__builtin_amdgcn_mov_dpp((int)as_uint(x), 0x141, 0xF, 0xF, false));  // half mirror
+    x = dpp_add(x, 1);
+    x = dpp_add(x, 0);
+    return x;
+}
+
 template <uint32_t TILE>
 __global__ void __launch_bounds__(TILE * TILE) blend_backward_kernel(
     const DView* __restrict__ views, const uint32_t* __restrict__ tile_bases, uint32_t nviews, uint32_t batch,
@@ -173,6 +206,9 @@ __global__ void __launch_bounds__(TILE * TILE) blend_backward_kernel(
     constexpr uint32_t kThreads = TILE * TILE;
     __shared__ LdsBatch s;
     __shared__ uint32_t s_slot[kMaxBatch];
+    // The batch's gradients summed over the workgroup's wavefronts, then added
+    // to the slots' words once a batch.
+    __shared__ float s_grad[kMaxBatch * kGrad2d];
     __shared__ uint32_t s_max_last;
     const uint32_t gtile = blockIdx.x;
     const uint32_t v = view_of_tile(tile_bases, nviews, gtile);
@@ -222,6 +258,9 @@ __global__ void __launch_bounds__(TILE * TILE) blend_backward_kernel(
             s_slot[k] = slot;
             stage(s, k, records, slot);
         }
+        for (uint32_t k = tid; k < count * kGrad2d; k += kThreads) {
+            s_grad[k] = 0.0f;
+        }
         __syncthreads();
         for (uint32_t jj = count; jj > 0; --jj) {
             const uint32_t j = jj - 1;
@@ -268,17 +307,21 @@ __global__ void __launch_bounds__(TILE * TILE) blend_backward_kernel(
                 }
             }
             if (__any(contrib)) {
-                const float sums[kGrad2d] = {wave_sum_f(g_cx), wave_sum_f(g_cy), wave_sum_f(g_ka),
-                                             wave_sum_f(g_kb), wave_sum_f(g_kc), wave_sum_f(g_al),
-                                             wave_sum_f(g_r),  wave_sum_f(g_g),  wave_sum_f(g_b)};
-                if (lane < kGrad2d) {
-                    float mine = 0.0f;
-#pragma unroll
-                    for (uint32_t t = 0; t < kGrad2d; ++t) {
-                        mine = lane == t ? sums[t] : mine;
-                    }
-                    atomicAdd(&grad2d[(size_t)s_slot[j] * kGrad2d + lane], mine);
+                const float eight[8] = {g_cx, g_cy, g_ka, g_kb, g_kc, g_al, g_r, g_g};
+                const float sum8 = wave_sum8(eight, lane);
+                const float sum_b = wave_sum_f(g_b);
+                if ((lane & 7u) == 0) {
+                    atomicAdd(&s_grad[j * kGrad2d + (lane >> 3)], sum8);
+                } else if (lane == 1) {
+                    atomicAdd(&s_grad[j * kGrad2d + 8], sum_b);
                 }
+            }
+        }
+        __syncthreads();
+        for (uint32_t k = tid; k < count * kGrad2d; k += kThreads) {
+            const float g = s_grad[k];
+            if (g != 0.0f) {
+                atomicAdd(&grad2d[(size_t)s_slot[k / kGrad2d] * kGrad2d + k % kGrad2d], g);
             }
         }
         __syncthreads();

@@ -70,7 +70,7 @@ struct athx_ctx {
     // metadata), each strip's range and first segment, the entries' hit
     // masks and the per (tile, segment) table.
     DeviceBuffer strip_keys, strip_slots, strip_alt_keys, strip_alt_slots, strip_sort_meta;
-    DeviceBuffer strip_ranges, segbase, seg_map, masks, seg_table;
+    DeviceBuffer strip_ranges, segbase, seg_map, masks, seg_table, rank_fp;
     // For the frame's readers: exact tile counts and their scan in depth order.
     DeviceBuffer exact, exact_offsets, exact_status;
     DeviceBuffer api_sort_meta, api_alt_keys, api_alt_vals, api_scan_status, api_count;
@@ -108,7 +108,7 @@ std::vector<DeviceBuffer*> all_buffers(athx_ctx* ctx) {
             &ctx->big_list, &ctx->depth_keys_sorted, &ctx->depth_slots, &ctx->depth_alt_keys, &ctx->depth_alt_slots,
             &ctx->depth_sort_meta, &ctx->fp_rects, &ctx->fp_masks, &ctx->fp_big_base, &ctx->fp_big_masks,
             &ctx->strip_keys, &ctx->strip_slots, &ctx->strip_alt_keys, &ctx->strip_alt_slots, &ctx->strip_sort_meta,
-            &ctx->strip_ranges, &ctx->segbase, &ctx->seg_map, &ctx->masks, &ctx->seg_table, &ctx->exact, &ctx->exact_offsets,
+            &ctx->strip_ranges, &ctx->segbase, &ctx->seg_map, &ctx->rank_fp, &ctx->masks, &ctx->seg_table, &ctx->exact, &ctx->exact_offsets,
             &ctx->exact_status, &ctx->api_sort_meta, &ctx->api_alt_keys, &ctx->api_alt_vals, &ctx->api_scan_status,
             &ctx->api_count};
 }
@@ -486,6 +486,7 @@ athx_status reserve_front(athx_ctx* ctx, const FrameShape& fs) {
     ATHX_HIP(ctx, reserve(ctx, ctx->fp_rects, slots * 8 + 16), "footprint rects");
     ATHX_HIP(ctx, reserve(ctx, ctx->strip_ranges, (size_t)fs.strips * 8 + 16), "strip ranges");
     ATHX_HIP(ctx, reserve(ctx, ctx->segbase, (size_t)fs.strips * 4 + 16), "strip segments");
+    ATHX_HIP(ctx, reserve(ctx, ctx->rank_fp, slots * 32 + 32), "ranked footprints");
     return ATHX_OK;
 }
 
@@ -560,6 +561,7 @@ StripBins bins_of(athx_ctx* ctx, const FrameShape& fs, uint32_t n, uint32_t tile
     b.records = as<float>(ctx->records);
     b.rects = as<uint2>(ctx->fp_rects);
     b.strip_slots = as<uint32_t>(strips_in_alt ? ctx->strip_alt_slots : ctx->strip_slots);
+    b.fp = as<uint4>(ctx->rank_fp);
     b.strip_ranges = as<uint32_t>(ctx->strip_ranges);
     b.segbase = as<uint32_t>(ctx->segbase);
     b.seg_desc = as<uint4>(ctx->seg_map);
@@ -623,7 +625,8 @@ athx_status enqueue_bins(athx_ctx* ctx, const FrameShape& fs, uint32_t n, uint32
     uint64_t* totals = as<uint64_t>(ctx->totals);
     ATHX_HIP(ctx, launch_strip_emit(as<DView>(ctx->views), n, as<uint2>(ctx->fp_rects),
                                     sorted_slots_of(ctx, ctx->last_sorted_in_alt), as<uint32_t>(ctx->offsets),
-                                    totals + kTotalVisible, fs.slots, fs.entry_cap, as<uint32_t>(ctx->strip_keys),
+                                    totals + kTotalVisible, fs.slots, fs.entry_cap, as<float>(ctx->records),
+                                    as<uint4>(ctx->rank_fp), as<uint32_t>(ctx->strip_keys),
                                     as<uint32_t>(ctx->strip_slots), s),
              "strip entries");
     const SortScratch sc = sort_scratch(ctx->strip_sort_meta, ctx->strip_alt_keys.ptr,

@@ -820,7 +820,11 @@ __global__ void __launch_bounds__(256) settle_kernel(const DView* __restrict__ v
 
 // One strip entry for every (row, strip) a drawn footprint's rectangle
 // covers, rank by rank in depth order, a footprint's entries row by row and
-// left to right: key = its global strip, value = the slot. A wavefront's 64
+// left to right: key = its global strip, value = the rank. The rank's
+// footprint is written once beside them, 32 aligned bytes at the rank
+// (fp[2k]: centre x, y, conic a, b; fp[2k + 1]: conic c, cut power, first |
+// last tile column << 16, slot), so the binning reads one sector an entry
+// instead of the slot's record and rectangle. A wavefront's 64
 // ranks own one contiguous run of entries (offsets holds every 64th rank's
 // first entry); the wavefront walks it 64 entries at a time, lane L taking
 // entry g0 + L and finding its rank by a binary search over the ranks' ends
@@ -833,8 +837,10 @@ __global__ void __launch_bounds__(kEmitThreads) strip_emit_kernel(const DView* _
                                                                   const uint32_t* __restrict__ offsets,
                                                                   const uint64_t* __restrict__ visible_dev,
                                                                   uint32_t max_visible, uint32_t cap,
+                                                                  const float* __restrict__ records,
+                                                                  uint4* __restrict__ fp_out,
                                                                   uint32_t* __restrict__ keys_out,
-                                                                  uint32_t* __restrict__ slots_out) {
+                                                                  uint32_t* __restrict__ ranks_out) {
     __shared__ uint32_t s_end[kEmitThreads];
     __shared__ uint4 s_rank[kEmitThreads];  // slot, first strip, strips a row, strips between rows
     const uint32_t lane = threadIdx.x & 63u, wbase = threadIdx.x & ~63u;
@@ -850,7 +856,13 @@ __global__ void __launch_bounds__(kEmitThreads) strip_emit_kernel(const DView* _
         const uint32_t sx0 = x0 / kStripTiles;
         const uint32_t cols = (x0 + w - 1u) / kStripTiles - sx0 + 1u;
         count = h * cols;
-        s_rank[threadIdx.x] = make_uint4(slot, V.strip_base + y0 * V.strips_x + sx0, cols, V.strips_x);
+        s_rank[threadIdx.x] = make_uint4(k, V.strip_base + y0 * V.strips_x + sx0, cols, V.strips_x);
+        const float4* rec = reinterpret_cast<const float4*>(records + (size_t)slot * kRecordFloats);
+        const float4 screen = rec[0], conic = rec[1];
+        fp_out[2u * k] = make_uint4(__builtin_bit_cast(uint32_t, screen.x), __builtin_bit_cast(uint32_t, screen.y),
+                                    __builtin_bit_cast(uint32_t, conic.x), __builtin_bit_cast(uint32_t, conic.y));
+        fp_out[2u * k + 1u] = make_uint4(__builtin_bit_cast(uint32_t, conic.z), __builtin_bit_cast(uint32_t, screen.w),
+                                         x0 | ((x0 + w - 1u) << 16), slot);
     }
     const uint32_t end = wave_inclusive_sum(count, lane);
     s_end[threadIdx.x] = end;
@@ -868,7 +880,7 @@ __global__ void __launch_bounds__(kEmitThreads) strip_emit_kernel(const DView* _
             const uint64_t at = (uint64_t)base + g;
             if (at < cap) {
                 keys_out[at] = rk.y + row * rk.w + col;
-                slots_out[at] = rk.x;
+                ranks_out[at] = rk.x;
             }
         }
     }
@@ -929,14 +941,16 @@ struct StripFootprint {
     uint32_t x0, x1;
 };
 
-__device__ inline StripFootprint strip_footprint(const StripBins& b, uint32_t slot) {
+__device__ inline StripFootprint strip_footprint(const StripBins& b, uint32_t rank, uint32_t& slot) {
     StripFootprint f;
-    const float4* rec = reinterpret_cast<const float4*>(b.records + (size_t)slot * kRecordFloats);
-    f.screen = rec[0];
-    f.conic = rec[1];
-    const uint2 r = b.rects[slot];
-    f.x0 = r.x & 0xFFFFu;
-    f.x1 = f.x0 + (r.y & 0xFFFFu) - 1u;
+    const uint4 p = b.fp[2u * rank], q = b.fp[2u * rank + 1u];
+    f.screen = make_float4(__builtin_bit_cast(float, p.x), __builtin_bit_cast(float, p.y), 0.0f,
+                           __builtin_bit_cast(float, q.y));
+    f.conic = make_float4(__builtin_bit_cast(float, p.z), __builtin_bit_cast(float, p.w), __builtin_bit_cast(float, q.x),
+                          0.0f);
+    f.x0 = q.z & 0xFFFFu;
+    f.x1 = q.z >> 16;
+    slot = q.w;
     return f;
 }
 
@@ -976,16 +990,36 @@ constexpr uint32_t kSegPer = kStripSeg / kSegThreads;
 // table[16 base + j nseg + k] for tile j of segment k of a strip whose
 // first segment is base, so the table read in order is strip by strip,
 // tile by tile, segment by segment -- the order of the tile-sorted pairs.
-// Every entry's slot, then every footprint, is loaded before any is tested.
+//
+// An entry's footprint covers 1 to 16 of the strip's tiles. A lane testing
+// its own entry's tiles would hold its wavefront to the widest of 64, so the
+// tests are spread over the lanes: each lane writes its entry's (entry,
+// column) items to an LDS list (lane L's items after the ones of lanes
+// before it), and the wavefront takes the list 64 items a step, every lane
+// one test, folding a hit into its entry's mask with an LDS OR. Each entry's
+// footprint and row terms are in LDS beside the list. Every entry's slot,
+// then every footprint, is loaded before any is tested.
+struct StripEntryLds {
+    float4 geo;   // centre x, conic a, conic b, conic c
+    float4 row;   // the row's box terms: lo, hi, q_lo, q_hi
+    float4 more;  // s_lo, s_hi, 1 / c, cut power
+};
+
 __global__ void __launch_bounds__(kSegThreads) strip_count_kernel(StripBins b) {
+    __shared__ StripEntryLds s_entry[kSegThreads];
+    __shared__ uint16_t s_items[kSegThreads / 64][64 * kStripTiles];
+    __shared__ uint32_t s_mask[kSegThreads];
     __shared__ uint32_t s_hits[kSegThreads / 64][kStripTiles];
     const uint4 d0 = b.seg_desc[2u * blockIdx.x];
     if (d0.x >= d0.y) {
         return;  // past the segments: the same for the whole workgroup
     }
     const uint4 d1 = b.seg_desc[2u * blockIdx.x + 1u];
-    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6, wbase = threadIdx.x & ~63u;
     const DView& V = b.views[d1.x];
+    const uint32_t T = b.tile, ty = d1.y, tx0 = d1.z;
+    // The entries' ranks, then their footprints; each entry's rank is
+    // replaced by its slot, which the pairs carry.
     uint32_t slot[kSegPer];
 #pragma unroll
     for (uint32_t j = 0; j < kSegPer; ++j) {
@@ -995,25 +1029,87 @@ __global__ void __launch_bounds__(kSegThreads) strip_count_kernel(StripBins b) {
     StripFootprint f[kSegPer];
 #pragma unroll
     for (uint32_t j = 0; j < kSegPer; ++j) {
-        f[j] = strip_footprint(b, slot[j] != ~0u ? slot[j] : 0u);
+        uint32_t s = 0;
+        f[j] = strip_footprint(b, slot[j] != ~0u ? slot[j] : 0u, s);
+        if (slot[j] != ~0u) {
+            slot[j] = s;
+            b.strip_slots[d0.x + j * kSegThreads + threadIdx.x] = s;
+        }
     }
     uint32_t count[kStripTiles];
 #pragma unroll
     for (uint32_t t = 0; t < kStripTiles; ++t) {
         count[t] = 0;
     }
-#pragma unroll
+    uint16_t* items = s_items[wave];
     for (uint32_t j = 0; j < kSegPer; ++j) {
-        const uint32_t e = d0.x + j * kSegThreads + threadIdx.x;
-        uint32_t mask = 0;
+        // This lane's entry: its columns in the strip and its terms in LDS.
+        uint32_t c0 = 0, cols = 0;
         if (slot[j] != ~0u) {
-            mask = strip_hits(f[j], V, b.tile, d1.y, d1.z);
-            b.masks[e] = (uint16_t)mask;
+            BigFootprint g{};
+            g.cx = f[j].screen.x;
+            g.cy = f[j].screen.y;
+            g.cutoff = f[j].screen.w;
+            g.a = f[j].conic.x;
+            g.b = f[j].conic.y;
+            g.c = f[j].conic.z;
+            g.ia = recip_of(g.a);
+            g.ic = recip_of(g.c);
+            c0 = f[j].x0 > tx0 ? f[j].x0 : tx0;
+            const uint32_t c1 = f[j].x1 < tx0 + kStripTiles - 1u ? f[j].x1 : tx0 + kStripTiles - 1u;
+            cols = c1 >= c0 ? c1 - c0 + 1u : 0u;
+            const BoxAxis Y = row_of(g, V, T, 2.0f * g.b, ty);
+            s_entry[threadIdx.x].geo = make_float4(g.cx, g.a, g.b, g.c);
+            s_entry[threadIdx.x].row = make_float4(Y.lo, Y.hi, Y.q_lo, Y.q_hi);
+            s_entry[threadIdx.x].more = make_float4(Y.s_lo, Y.s_hi, g.ic, g.cutoff);
+        }
+        s_mask[threadIdx.x] = 0;
+        const uint32_t end = wave_inclusive_sum(cols, lane);
+        const uint32_t total = lane_u(end, 63);
+        for (uint32_t k = 0; k < cols; ++k) {
+            items[end - cols + k] = (uint16_t)(lane | ((c0 - tx0 + k) << 6));
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        for (uint32_t g0 = 0; g0 < total; g0 += 64u) {
+            const uint32_t gi = g0 + lane;
+            if (gi < total) {
+                const uint32_t item = items[gi];
+                const uint32_t owner = item & 63u, col = item >> 6;
+                const StripEntryLds en = s_entry[wbase + owner];
+                BigFootprint g{};
+                g.cx = en.geo.x;
+                g.a = en.geo.y;
+                g.b = en.geo.z;
+                g.c = en.geo.w;
+                g.ic = en.more.z;
+                g.cutoff = en.more.w;
+                BoxAxis Y;
+                Y.lo = en.row.x;
+                Y.hi = en.row.y;
+                Y.q_lo = en.row.z;
+                Y.q_hi = en.row.w;
+                Y.s_lo = en.more.x;
+                Y.s_hi = en.more.y;
+                const float b2 = 2.0f * g.b;
+                if (box_hit(g, b2, column_of(g, V, T, b2, tx0 + col), Y)) {
+                    atomicOr(&s_mask[wbase + owner], 1u << col);
+                }
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        const uint32_t mask = s_mask[threadIdx.x];
+        if (slot[j] != ~0u) {
+            b.masks[d0.x + j * kSegThreads + threadIdx.x] = (uint16_t)mask;
         }
 #pragma unroll
         for (uint32_t t = 0; t < kStripTiles; ++t) {
             count[t] += (uint32_t)__popcll(__ballot((mask >> t) & 1u));
         }
+        __builtin_amdgcn_wave_barrier();
     }
     if (lane == 0) {
 #pragma unroll
@@ -1277,13 +1373,14 @@ hipError_t launch_counters(const uint32_t* depth_keys, const uint32_t* touched, 
 
 hipError_t launch_strip_emit(const DView* views_dev, uint32_t n, const uint2* rects, const uint32_t* sorted_slots,
                              const uint32_t* offsets, const uint64_t* visible_dev, uint32_t max_visible, uint32_t cap,
-                             uint32_t* strip_keys, uint32_t* strip_slots, hipStream_t s) {
+                             const float* records, uint4* fp, uint32_t* strip_keys, uint32_t* strip_ranks,
+                             hipStream_t s) {
     if (max_visible == 0 || cap == 0) {
         return hipSuccess;
     }
     hipLaunchKernelGGL(strip_emit_kernel, dim3(div_up(max_visible, kEmitThreads)), dim3(kEmitThreads), 0, s,
-                       views_dev, n, rects, sorted_slots, offsets, visible_dev, max_visible, cap, strip_keys,
-                       strip_slots);
+                       views_dev, n, rects, sorted_slots, offsets, visible_dev, max_visible, cap, records, fp,
+                       strip_keys, strip_ranks);
     return hipGetLastError();
 }
 

@@ -12,32 +12,40 @@
 // their view direction, and max(c, 0) or the sRGB decode (color.slang:7-41).
 // The cut radius and the tile set are decisions, not functions: no gradient.
 //
-// The arithmetic is FP64: a thin, turned Gaussian's 2D determinant is a
-// small difference of large products, and in FP32 the chain from it back to
-// the quaternion and scales loses three digits. Every branch is the one the
-// FP32 forward took, read from the record's branch bits, so the adjoint
-// differentiates exactly the function the forward computed. MI355X runs FP64
-// vector math at half its FP32 rate, and this kernel is one thread per
-// Gaussian: its cost is small beside the blend's adjoint.
+// The geometry's arithmetic is FP64: a thin, turned Gaussian's 2D
+// determinant is a small difference of large products, and in FP32 the
+// chain from it back to the quaternion and scales loses three digits. The
+// colour's (a sum of harmonic terms and the view direction's normalisation,
+// nothing cancelling) is FP32, in a kernel of its own that runs after the
+// geometry's: its 16 harmonics and their three derivatives would otherwise
+// hold the geometry's thread to a quarter of its occupancy. Every branch is
+// the one the FP32 forward took, read from the record's branch bits, so the
+// adjoint differentiates exactly the function the forward computed.
 #include "athx_internal.h"
 
 namespace athx {
 namespace {
 
-constexpr double kSH0 = 0.28209479177387814;
-constexpr double kC1 = 0.4886025119029199;
-constexpr double kC2_0 = 1.0925484305920792, kC2_1 = -1.0925484305920792, kC2_2 = 0.31539156525252005,
-                 kC2_3 = -1.0925484305920792, kC2_4 = 0.5462742152960396;
-constexpr double kC3_0 = -0.5900435899266435, kC3_1 = 2.890611442640554, kC3_2 = -0.4570457994644658,
-                 kC3_3 = 0.3731763325901154, kC3_4 = -0.4570457994644658, kC3_5 = 1.445305721320277,
-                 kC3_6 = -0.5900435899266435;
+}  // namespace
+constexpr double kSH0d = 0.28209479177387814;
+constexpr double kC1d = 0.4886025119029199;
+namespace {
+constexpr double kC2d[5] = {1.0925484305920792, -1.0925484305920792, 0.31539156525252005, -1.0925484305920792,
+                            0.5462742152960396};
+constexpr double kC3d[7] = {-0.5900435899266435, 2.890611442640554, -0.4570457994644658, 0.3731763325901154,
+                            -0.4570457994644658, 1.445305721320277, -0.5900435899266435};
 
 // The 16 real harmonics at dir and their gradients with respect to dir
 // (sh.slang:25-51 and shBasisValue :59-80).
-__device__ inline void sh_basis(uint32_t degree, double x, double y, double z, double* Y, double* Dx, double* Dy,
-                                double* Dz) {
+template <typename Real>
+__device__ inline void sh_basis(uint32_t degree, Real x, Real y, Real z, Real* Y, Real* Dx, Real* Dy, Real* Dz) {
+    constexpr Real kSH0 = (Real)athx::kSH0d, kC1 = (Real)athx::kC1d;
+    constexpr Real kC2_0 = (Real)kC2d[0], kC2_1 = (Real)kC2d[1], kC2_2 = (Real)kC2d[2], kC2_3 = (Real)kC2d[3],
+                   kC2_4 = (Real)kC2d[4];
+    constexpr Real kC3_0 = (Real)kC3d[0], kC3_1 = (Real)kC3d[1], kC3_2 = (Real)kC3d[2], kC3_3 = (Real)kC3d[3],
+                   kC3_4 = (Real)kC3d[4], kC3_5 = (Real)kC3d[5], kC3_6 = (Real)kC3d[6];
     for (int k = 0; k < 16; ++k) {
-        Y[k] = 0.0; Dx[k] = 0.0; Dy[k] = 0.0; Dz[k] = 0.0;
+        Y[k] = 0; Dx[k] = 0; Dy[k] = 0; Dz[k] = 0;
     }
     Y[0] = kSH0;
     if (degree == 0) {
@@ -52,23 +60,26 @@ __device__ inline void sh_basis(uint32_t degree, double x, double y, double z, d
     const double xx = x * x, yy = y * y, zz = z * z;
     Y[4] = kC2_0 * x * y; Dx[4] = kC2_0 * y; Dy[4] = kC2_0 * x;
     Y[5] = kC2_1 * y * z; Dy[5] = kC2_1 * z; Dz[5] = kC2_1 * y;
-    Y[6] = kC2_2 * (2.0 * zz - xx - yy); Dx[6] = -2.0 * kC2_2 * x; Dy[6] = -2.0 * kC2_2 * y; Dz[6] = 4.0 * kC2_2 * z;
+    Y[6] = kC2_2 * ((Real)2.0 * zz - xx - yy); Dx[6] = -(Real)2.0 * kC2_2 * x; Dy[6] = -(Real)2.0 * kC2_2 * y;
+    Dz[6] = (Real)4.0 * kC2_2 * z;
     Y[7] = kC2_3 * x * z; Dx[7] = kC2_3 * z; Dz[7] = kC2_3 * x;
-    Y[8] = kC2_4 * (xx - yy); Dx[8] = 2.0 * kC2_4 * x; Dy[8] = -2.0 * kC2_4 * y;
+    Y[8] = kC2_4 * (xx - yy); Dx[8] = (Real)2.0 * kC2_4 * x; Dy[8] = -(Real)2.0 * kC2_4 * y;
     if (degree < 3) {
         return;
     }
-    Y[9] = kC3_0 * y * (3.0 * xx - yy); Dx[9] = kC3_0 * 6.0 * x * y; Dy[9] = kC3_0 * (3.0 * xx - 3.0 * yy);
+    Y[9] = kC3_0 * y * ((Real)3.0 * xx - yy); Dx[9] = kC3_0 * (Real)6.0 * x * y;
+    Dy[9] = kC3_0 * ((Real)3.0 * xx - (Real)3.0 * yy);
     Y[10] = kC3_1 * x * y * z; Dx[10] = kC3_1 * y * z; Dy[10] = kC3_1 * x * z; Dz[10] = kC3_1 * x * y;
-    Y[11] = kC3_2 * y * (4.0 * zz - xx - yy); Dx[11] = kC3_2 * (-2.0 * x * y);
-    Dy[11] = kC3_2 * (4.0 * zz - xx - 3.0 * yy); Dz[11] = kC3_2 * 8.0 * y * z;
-    Y[12] = kC3_3 * z * (2.0 * zz - 3.0 * xx - 3.0 * yy); Dx[12] = kC3_3 * (-6.0 * x * z);
-    Dy[12] = kC3_3 * (-6.0 * y * z); Dz[12] = kC3_3 * (6.0 * zz - 3.0 * xx - 3.0 * yy);
-    Y[13] = kC3_4 * x * (4.0 * zz - xx - yy); Dx[13] = kC3_4 * (4.0 * zz - 3.0 * xx - yy);
-    Dy[13] = kC3_4 * (-2.0 * x * y); Dz[13] = kC3_4 * 8.0 * x * z;
-    Y[14] = kC3_5 * z * (xx - yy); Dx[14] = kC3_5 * 2.0 * x * z; Dy[14] = kC3_5 * (-2.0 * y * z);
+    Y[11] = kC3_2 * y * ((Real)4.0 * zz - xx - yy); Dx[11] = kC3_2 * (-(Real)2.0 * x * y);
+    Dy[11] = kC3_2 * ((Real)4.0 * zz - xx - (Real)3.0 * yy); Dz[11] = kC3_2 * (Real)8.0 * y * z;
+    Y[12] = kC3_3 * z * ((Real)2.0 * zz - (Real)3.0 * xx - (Real)3.0 * yy); Dx[12] = kC3_3 * (-(Real)6.0 * x * z);
+    Dy[12] = kC3_3 * (-(Real)6.0 * y * z); Dz[12] = kC3_3 * ((Real)6.0 * zz - (Real)3.0 * xx - (Real)3.0 * yy);
+    Y[13] = kC3_4 * x * ((Real)4.0 * zz - xx - yy); Dx[13] = kC3_4 * ((Real)4.0 * zz - (Real)3.0 * xx - yy);
+    Dy[13] = kC3_4 * (-(Real)2.0 * x * y); Dz[13] = kC3_4 * (Real)8.0 * x * z;
+    Y[14] = kC3_5 * z * (xx - yy); Dx[14] = kC3_5 * (Real)2.0 * x * z; Dy[14] = kC3_5 * (-(Real)2.0 * y * z);
     Dz[14] = kC3_5 * (xx - yy);
-    Y[15] = kC3_6 * x * (xx - 3.0 * yy); Dx[15] = kC3_6 * (3.0 * xx - 3.0 * yy); Dy[15] = kC3_6 * (-6.0 * x * y);
+    Y[15] = kC3_6 * x * (xx - (Real)3.0 * yy); Dx[15] = kC3_6 * ((Real)3.0 * xx - (Real)3.0 * yy);
+    Dy[15] = kC3_6 * (-(Real)6.0 * x * y);
 }
 
 template <bool kHalf>
@@ -122,9 +133,6 @@ __global__ void __launch_bounds__(256) project_backward_kernel(ProjectArgs a, co
     double g_p[3] = {0.0, 0.0, 0.0};
     double g_M[3][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
     double g_opacity = 0.0;
-    const size_t row = (size_t)i * a.sh_coeffs * 3;
-    const uint32_t nk = (a.sh_degree + 1) * (a.sh_degree + 1);
-
     for (uint32_t v = 0; v < nviews; ++v) {
         const size_t slot = (size_t)v * a.n + i;
         if (touched[slot] == 0) {
@@ -133,7 +141,6 @@ __global__ void __launch_bounds__(256) project_backward_kernel(ProjectArgs a, co
         const uint32_t flags = __builtin_bit_cast(uint32_t, records[slot * kRecordFloats + 11]);
         const float* G = grad2d + slot * kGrad2d;
         const double g_cx = G[0], g_cy = G[1], g_ka = G[2], g_kb = G[3], g_kc = G[4], g_al = G[5];
-        const double g_col[3] = {G[6], G[7], G[8]};
         const DView& V = views[v];
         double m[12];
         for (int k = 0; k < 12; ++k) {
@@ -251,46 +258,6 @@ __global__ void __launch_bounds__(256) project_backward_kernel(ProjectArgs a, co
             g_p[c] += m[c] * g_t[0] + m[4 + c] * g_t[1] + m[8 + c] * g_t[2];
         }
 
-        // Colour: 0.5 + SH0 * sh0 + rest(dir), then max(c, 0) or sRGB, each
-        // channel's branch the forward's.
-        const double dx = px - (double)V.eye[0], dy = py - (double)V.eye[1], dz = pz - (double)V.eye[2];
-        const double dinv = 1.0 / sqrt(dx * dx + dy * dy + dz * dz);
-        const double ex = dx * dinv, ey = dy * dinv, ez = dz * dinv;
-        double Y[16], Dx[16], Dy[16], Dz[16];
-        sh_basis(a.sh_degree, ex, ey, ez, Y, Dx, Dy, Dz);
-        double g_dir[3] = {0.0, 0.0, 0.0};
-        for (uint32_t ch = 0; ch < 3; ++ch) {
-            if (!(flags & (kFlagColourR << ch))) {
-                continue;  // max(c, 0) took 0
-            }
-            double g_cv = g_col[ch];
-            if (!a.linear) {
-                if (flags & (kFlagSrgbR << ch)) {
-                    double cv = 0.5;
-                    for (uint32_t k = 0; k < nk; ++k) {
-                        cv += Y[k] * sh_load<kHalf>(a.sh, a.sh_half, row + k * 3 + ch);
-                    }
-                    g_cv *= (2.4 / 1.055) * pow((cv + 0.055) / 1.055, 1.4);
-                } else {
-                    g_cv *= 1.0 / 12.92;
-                }
-            }
-            if (out.sh) {
-                for (uint32_t k = 0; k < nk; ++k) {
-                    out.sh[row + k * 3 + ch] += (float)(Y[k] * g_cv);
-                }
-            }
-            for (uint32_t k = 1; k < nk; ++k) {
-                const double coef = sh_load<kHalf>(a.sh, a.sh_half, row + k * 3 + ch) * g_cv;
-                g_dir[0] += coef * Dx[k];
-                g_dir[1] += coef * Dy[k];
-                g_dir[2] += coef * Dz[k];
-            }
-        }
-        const double dot = ex * g_dir[0] + ey * g_dir[1] + ez * g_dir[2];
-        g_p[0] += (g_dir[0] - ex * dot) * dinv;
-        g_p[1] += (g_dir[1] - ey * dot) * dinv;
-        g_p[2] += (g_dir[2] - ez * dot) * dinv;
     }
 
     if (out.means) {
@@ -329,6 +296,152 @@ __global__ void __launch_bounds__(256) project_backward_kernel(ProjectArgs a, co
     }
 }
 
+// The colour's adjoint in FP32 (after the geometry's, whose mean gradients it
+// adds to): 0.5 + SH0 sh0 + rest(dir), then max(c, 0) or the sRGB decode, each
+// channel's branch the forward's, to the coefficients and, through the view
+// direction, to the mean. A Gaussian's coefficients are read once, in whole
+// 16-byte words where its row allows, and their gradients summed over the
+// views in registers and added to memory once; every loop over the 16
+// harmonics is unrolled, so nothing is indexed at run time.
+template <bool kHalf>
+__global__ void __launch_bounds__(256) colour_backward_kernel(ProjectArgs a, const DView* __restrict__ views,
+                                                              uint32_t nviews, const uint32_t* __restrict__ touched,
+                                                              const float* __restrict__ records,
+                                                              const float* __restrict__ grad2d, athx_grads out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) {
+        return;
+    }
+    bool any = false;
+    for (uint32_t v = 0; v < nviews; ++v) {
+        any = any || touched[(size_t)v * a.n + i] != 0;
+    }
+    if (!any) {
+        return;
+    }
+    const uint32_t coeffs = a.sh_coeffs;
+    const uint32_t nk = (a.sh_degree + 1) * (a.sh_degree + 1);
+    const size_t row = (size_t)i * coeffs * 3;
+    // Coefficient k of channel ch at c[k * 3 + ch]; zero past the row.
+    float c[48];
+    const bool whole = coeffs == 16 &&
+                       ((reinterpret_cast<uintptr_t>(kHalf ? (const void*)a.sh_half : (const void*)a.sh) & 15u) == 0);
+    if (whole) {
+        if constexpr (kHalf) {
+            const uint4* p = reinterpret_cast<const uint4*>(a.sh_half + row);
+#pragma unroll
+            for (uint32_t q = 0; q < 6; ++q) {
+                const uint4 w = p[q];
+                const uint32_t u[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+                for (uint32_t e = 0; e < 4; ++e) {
+                    c[q * 8 + 2 * e] = (float)__builtin_bit_cast(_Float16, (uint16_t)(u[e] & 0xFFFFu));
+                    c[q * 8 + 2 * e + 1] = (float)__builtin_bit_cast(_Float16, (uint16_t)(u[e] >> 16));
+                }
+            }
+        } else {
+            const float4* p = reinterpret_cast<const float4*>(a.sh + row);
+#pragma unroll
+            for (uint32_t q = 0; q < 12; ++q) {
+                const float4 w = p[q];
+                c[q * 4 + 0] = w.x;
+                c[q * 4 + 1] = w.y;
+                c[q * 4 + 2] = w.z;
+                c[q * 4 + 3] = w.w;
+            }
+        }
+    } else {
+#pragma unroll
+        for (uint32_t k = 0; k < 48; ++k) {
+            c[k] = k < coeffs * 3 ? (float)sh_load<kHalf>(a.sh, a.sh_half, row + k) : 0.0f;
+        }
+    }
+    float g_sh[48];
+#pragma unroll
+    for (uint32_t k = 0; k < 48; ++k) {
+        g_sh[k] = 0.0f;
+    }
+    const float px = a.means[i * 3 + 0], py = a.means[i * 3 + 1], pz = a.means[i * 3 + 2];
+    float g_p[3] = {0.0f, 0.0f, 0.0f};
+    for (uint32_t v = 0; v < nviews; ++v) {
+        const size_t slot = (size_t)v * a.n + i;
+        if (touched[slot] == 0) {
+            continue;
+        }
+        const uint32_t flags = __builtin_bit_cast(uint32_t, records[slot * kRecordFloats + 11]);
+        const float* G = grad2d + slot * kGrad2d;
+        const float g_col[3] = {G[6], G[7], G[8]};
+        const DView& V = views[v];
+        const float dx = px - V.eye[0], dy = py - V.eye[1], dz = pz - V.eye[2];
+        const float dinv = 1.0f / sqrtf(dx * dx + dy * dy + dz * dz);
+        const float ex = dx * dinv, ey = dy * dinv, ez = dz * dinv;
+        float Y[16], Dx[16], Dy[16], Dz[16];
+        sh_basis(a.sh_degree, ex, ey, ez, Y, Dx, Dy, Dz);
+        float g_dir[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (uint32_t ch = 0; ch < 3; ++ch) {
+            if (!(flags & (kFlagColourR << ch))) {
+                continue;  // max(c, 0) took 0
+            }
+            float g_cv = g_col[ch];
+            if (!a.linear) {
+                if (flags & (kFlagSrgbR << ch)) {
+                    float cv = 0.5f;
+#pragma unroll
+                    for (uint32_t k = 0; k < 16; ++k) {
+                        cv += k < nk ? Y[k] * c[k * 3 + ch] : 0.0f;
+                    }
+                    g_cv *= (2.4f / 1.055f) * powf((cv + 0.055f) / 1.055f, 1.4f);
+                } else {
+                    g_cv *= 1.0f / 12.92f;
+                }
+            }
+#pragma unroll
+            for (uint32_t k = 0; k < 16; ++k) {
+                if (k < nk) {
+                    g_sh[k * 3 + ch] += Y[k] * g_cv;
+                    if (k != 0) {
+                        const float coef = c[k * 3 + ch] * g_cv;
+                        g_dir[0] += coef * Dx[k];
+                        g_dir[1] += coef * Dy[k];
+                        g_dir[2] += coef * Dz[k];
+                    }
+                }
+            }
+        }
+        const float dot = ex * g_dir[0] + ey * g_dir[1] + ez * g_dir[2];
+        g_p[0] += (g_dir[0] - ex * dot) * dinv;
+        g_p[1] += (g_dir[1] - ey * dot) * dinv;
+        g_p[2] += (g_dir[2] - ez * dot) * dinv;
+    }
+    if (out.sh) {
+        if (coeffs == 16 && (reinterpret_cast<uintptr_t>(out.sh) & 15u) == 0) {
+            float4* p = reinterpret_cast<float4*>(out.sh + row);
+#pragma unroll
+            for (uint32_t q = 0; q < 12; ++q) {
+                float4 w = p[q];
+                w.x += g_sh[q * 4 + 0];
+                w.y += g_sh[q * 4 + 1];
+                w.z += g_sh[q * 4 + 2];
+                w.w += g_sh[q * 4 + 3];
+                p[q] = w;
+            }
+        } else {
+#pragma unroll
+            for (uint32_t k = 0; k < 48; ++k) {
+                if (k < coeffs * 3) {
+                    out.sh[row + k] += g_sh[k];
+                }
+            }
+        }
+    }
+    if (out.means) {
+        for (int cc = 0; cc < 3; ++cc) {
+            out.means[i * 3 + cc] += g_p[cc];
+        }
+    }
+}
+
 }  // namespace
 
 hipError_t launch_project_backward(const ProjectArgs& a, const DView* views_dev, uint32_t nviews,
@@ -341,8 +454,12 @@ hipError_t launch_project_backward(const ProjectArgs& a, const DView* views_dev,
     if (a.sh_half) {
         hipLaunchKernelGGL(project_backward_kernel<true>, grid, dim3(256), 0, s, a, views_dev, nviews, touched,
                            records, grad2d, g);
+        hipLaunchKernelGGL(colour_backward_kernel<true>, grid, dim3(256), 0, s, a, views_dev, nviews, touched,
+                           records, grad2d, g);
     } else {
         hipLaunchKernelGGL(project_backward_kernel<false>, grid, dim3(256), 0, s, a, views_dev, nviews, touched,
+                           records, grad2d, g);
+        hipLaunchKernelGGL(colour_backward_kernel<false>, grid, dim3(256), 0, s, a, views_dev, nviews, touched,
                            records, grad2d, g);
     }
     return hipGetLastError();
