@@ -91,6 +91,17 @@ struct ProjectArgs {
 
 __host__ __device__ inline uint32_t div_up(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
 
+// MI355X deals a launch's workgroups to its 8 XCDs round robin (workgroup b
+// to XCD b % 8), and each XCD has its own L2. Work item xcd_band(b, n) gives
+// XCD x the x-th eighth of n items, in order, so items that read the same
+// data (neighbouring tiles, a strip's neighbours) share an L2.
+constexpr uint32_t kXcds = 8;
+__device__ inline uint32_t xcd_band(uint32_t b, uint32_t n) {
+    const uint32_t xcd = b % kXcds, local = b / kXcds;
+    const uint32_t per = n / kXcds, extra = n % kXcds;
+    return xcd * per + (xcd < extra ? xcd : extra) + local;
+}
+
 // The exact tile counts' working set (launch_exact_counts; the frame itself
 // bins its pairs instead): each footprint's rectangle, and which of its
 // tiles it touches.

@@ -62,7 +62,7 @@ __global__ void __launch_bounds__(TILE * TILE) blend_kernel(const DView* __restr
                                                             float depth_threshold) {
     constexpr uint32_t kThreads = TILE * TILE;
     __shared__ LdsBatch s;
-    const uint32_t gtile = blockIdx.x;
+    const uint32_t gtile = xcd_band(blockIdx.x, gridDim.x);
     const uint32_t v = view_of_tile(tile_bases, nviews, gtile);
     const DView& V = views[v];
     const uint32_t local = gtile - V.tile_base;
@@ -210,7 +210,7 @@ __global__ void __launch_bounds__(TILE * TILE) blend_backward_kernel(
     // to the slots' words once a batch.
     __shared__ float s_grad[kMaxBatch * kGrad2d];
     __shared__ uint32_t s_max_last;
-    const uint32_t gtile = blockIdx.x;
+    const uint32_t gtile = xcd_band(blockIdx.x, gridDim.x);
     const uint32_t v = view_of_tile(tile_bases, nviews, gtile);
     const DView& V = views[v];
     const uint32_t local = gtile - V.tile_base;

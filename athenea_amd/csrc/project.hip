@@ -1010,11 +1010,18 @@ __global__ void __launch_bounds__(kSegThreads) strip_count_kernel(StripBins b) {
     __shared__ uint16_t s_items[kSegThreads / 64][64 * kStripTiles];
     __shared__ uint32_t s_mask[kSegThreads];
     __shared__ uint32_t s_hits[kSegThreads / 64][kStripTiles];
-    const uint4 d0 = b.seg_desc[2u * blockIdx.x];
+    // The live segments, banded over the XCDs (the grid is the capacity's).
+    const uint64_t total = *b.seg_total;
+    const uint32_t segs = total < b.seg_cap ? (uint32_t)total : b.seg_cap;
+    if (blockIdx.x >= segs) {
+        return;
+    }
+    const uint32_t seg = xcd_band(blockIdx.x, segs);
+    const uint4 d0 = b.seg_desc[2u * seg];
     if (d0.x >= d0.y) {
         return;  // past the segments: the same for the whole workgroup
     }
-    const uint4 d1 = b.seg_desc[2u * blockIdx.x + 1u];
+    const uint4 d1 = b.seg_desc[2u * seg + 1u];
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6, wbase = threadIdx.x & ~63u;
     const DView& V = b.views[d1.x];
     const uint32_t T = b.tile, ty = d1.y, tx0 = d1.z;
