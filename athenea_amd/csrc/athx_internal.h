@@ -181,6 +181,8 @@ struct StripBins {
     const float* records;
     const uint2* rects;
     uint32_t* strip_slots;         // [entries] strip-sorted entries' depth ranks, made slots by the count
+                                   // (slots already when strip_keys is set)
+    const uint32_t* strip_keys;    // [entries] sorted keys carrying the masks (strip_mask_emit), or null
     const uint4* fp;               // [visible][2] each rank's footprint (launch_strip_emit)
     const uint32_t* strip_ranges;  // [strips][2] each strip's entries (0, 0 when none)
     const uint32_t* segbase;       // [strips] first segment of each strip
@@ -193,6 +195,13 @@ struct StripBins {
     uint32_t pair_cap;
     uint32_t* pair_slots;          // [pair_cap] the tile-sorted pairs' slots
 };
+// The strip entries with their tiles decided (frames of at most 65 536
+// strips): key = strip | 16-bit mask of its touched tiles << 16, value = the
+// slot; the binning then reads only these.
+hipError_t launch_strip_mask_emit(const DView* views_dev, uint32_t n, uint32_t tile, const uint2* rects,
+                                  const uint32_t* sorted_slots, const uint32_t* offsets, const uint64_t* visible_dev,
+                                  uint32_t max_visible, uint32_t cap, const float* records, uint32_t* strip_keys,
+                                  uint32_t* strip_slots, hipStream_t s);
 // Each strip's segments, scanned: segbase[s] and the total in *seg_total.
 hipError_t launch_seg_scan(const uint32_t* strip_ranges, uint32_t strips, uint32_t* segbase, uint64_t* seg_total,
                            uint32_t* status_words, hipStream_t s);
@@ -219,9 +228,9 @@ size_t scan_status_words(uint32_t n);
 hipError_t launch_scan(const uint32_t* in, uint32_t* out, uint64_t* total, uint32_t n, uint32_t* status,
                        hipStream_t s);
 // [start, end) of every key in 0..keys-1 among the first min(*n_dev, cap)
-// sorted keys; (0, 0) for a key that is absent.
+// sorted keys (compared under key_mask); (0, 0) for a key that is absent.
 hipError_t launch_ranges(const uint32_t* sorted_keys, const uint64_t* n_dev, uint32_t cap, uint32_t keys,
-                         uint32_t* ranges, hipStream_t s);
+                         uint32_t* ranges, hipStream_t s, uint32_t key_mask = ~0u);
 // The reference's 64-bit pair keys of the tile-sorted pairs: tile << 24 |
 // the depth key of the pair's slot, a workgroup a tile over its range.
 hipError_t launch_pair_keys(const uint32_t* ranges, uint32_t tiles, const uint32_t* values,

@@ -210,6 +210,7 @@ __global__ void __launch_bounds__(TILE * TILE) blend_backward_kernel(
     // to the slots' words once a batch.
     __shared__ float s_grad[kMaxBatch * kGrad2d];
     __shared__ uint32_t s_max_last;
+    __shared__ uint32_t s_wave_last[kThreads / 64];
     const uint32_t gtile = xcd_band(blockIdx.x, gridDim.x);
     const uint32_t v = view_of_tile(tile_bases, nviews, gtile);
     const DView& V = views[v];
@@ -244,12 +245,18 @@ __global__ void __launch_bounds__(TILE * TILE) blend_backward_kernel(
     if (tid == 0) {
         s_max_last = 0;
     }
+    if (lane == 0) {
+        s_wave_last[tid >> 6] = 0;
+    }
     __syncthreads();
     if (last != 0) {
         atomicMax(&s_max_last, last);
+        atomicMax(&s_wave_last[tid >> 6], last);
     }
     __syncthreads();
     const uint32_t max_last = s_max_last;
+    // A wavefront walks only the pairs below its own pixels' furthest.
+    const uint32_t wave_last = s_wave_last[tid >> 6];
     for (uint32_t hi = max_last; hi > 0;) {
         const uint32_t lo = hi > batch ? hi - batch : 0;
         const uint32_t count = hi - lo;
@@ -262,7 +269,8 @@ __global__ void __launch_bounds__(TILE * TILE) blend_backward_kernel(
             s_grad[k] = 0.0f;
         }
         __syncthreads();
-        for (uint32_t jj = count; jj > 0; --jj) {
+        const uint32_t top = wave_last > lo ? (wave_last - lo < count ? wave_last - lo : count) : 0u;
+        for (uint32_t jj = top; jj > 0; --jj) {
             const uint32_t j = jj - 1;
             const uint32_t idx = lo + j;
             bool contrib = false;

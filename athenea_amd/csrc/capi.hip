@@ -550,6 +550,10 @@ const uint32_t* sorted_slots_of(const athx_ctx* ctx, bool in_alt) {
     return static_cast<const uint32_t*>(in_alt ? ctx->depth_alt_slots.ptr : ctx->depth_slots.ptr);
 }
 
+// Frames of at most 65 536 strips decide their entries' tiles in the
+// emission and carry the masks in the keys' upper half.
+bool masks_in_keys(const FrameShape& fs) { return fs.strip_bits <= 16; }
+
 StripBins bins_of(athx_ctx* ctx, const FrameShape& fs, uint32_t n, uint32_t tile, bool strips_in_alt) {
     StripBins b{};
     b.views = as<DView>(ctx->views);
@@ -562,6 +566,7 @@ StripBins bins_of(athx_ctx* ctx, const FrameShape& fs, uint32_t n, uint32_t tile
     b.rects = as<uint2>(ctx->fp_rects);
     b.strip_slots = as<uint32_t>(strips_in_alt ? ctx->strip_alt_slots : ctx->strip_slots);
     b.fp = as<uint4>(ctx->rank_fp);
+    b.strip_keys = masks_in_keys(fs) ? as<uint32_t>(strips_in_alt ? ctx->strip_alt_keys : ctx->strip_keys) : nullptr;
     b.strip_ranges = as<uint32_t>(ctx->strip_ranges);
     b.segbase = as<uint32_t>(ctx->segbase);
     b.seg_desc = as<uint4>(ctx->seg_map);
@@ -623,12 +628,21 @@ athx_status enqueue_front(athx_ctx* ctx, const ProjectArgs& pa, const FrameShape
 athx_status enqueue_bins(athx_ctx* ctx, const FrameShape& fs, uint32_t n, uint32_t tile, hipStream_t s, bool timed,
                          bool& strips_in_alt) {
     uint64_t* totals = as<uint64_t>(ctx->totals);
-    ATHX_HIP(ctx, launch_strip_emit(as<DView>(ctx->views), n, as<uint2>(ctx->fp_rects),
-                                    sorted_slots_of(ctx, ctx->last_sorted_in_alt), as<uint32_t>(ctx->offsets),
-                                    totals + kTotalVisible, fs.slots, fs.entry_cap, as<float>(ctx->records),
-                                    as<uint4>(ctx->rank_fp), as<uint32_t>(ctx->strip_keys),
-                                    as<uint32_t>(ctx->strip_slots), s),
-             "strip entries");
+    const bool masked = masks_in_keys(fs);
+    if (masked) {
+        ATHX_HIP(ctx, launch_strip_mask_emit(as<DView>(ctx->views), n, tile, as<uint2>(ctx->fp_rects),
+                                             sorted_slots_of(ctx, ctx->last_sorted_in_alt), as<uint32_t>(ctx->offsets),
+                                             totals + kTotalVisible, fs.slots, fs.entry_cap, as<float>(ctx->records),
+                                             as<uint32_t>(ctx->strip_keys), as<uint32_t>(ctx->strip_slots), s),
+                 "strip entries");
+    } else {
+        ATHX_HIP(ctx, launch_strip_emit(as<DView>(ctx->views), n, as<uint2>(ctx->fp_rects),
+                                        sorted_slots_of(ctx, ctx->last_sorted_in_alt), as<uint32_t>(ctx->offsets),
+                                        totals + kTotalVisible, fs.slots, fs.entry_cap, as<float>(ctx->records),
+                                        as<uint4>(ctx->rank_fp), as<uint32_t>(ctx->strip_keys),
+                                        as<uint32_t>(ctx->strip_slots), s),
+                 "strip entries");
+    }
     const SortScratch sc = sort_scratch(ctx->strip_sort_meta, ctx->strip_alt_keys.ptr,
                                         as<uint32_t>(ctx->strip_alt_slots), fs.entry_cap, fs.strip_passes);
     strips_in_alt = false;
@@ -637,7 +651,7 @@ athx_status enqueue_bins(athx_ctx* ctx, const FrameShape& fs, uint32_t n, uint32
              "strip sort");
     const uint32_t* keys = as<uint32_t>(strips_in_alt ? ctx->strip_alt_keys : ctx->strip_keys);
     ATHX_HIP(ctx, launch_ranges(keys, totals + kTotalEntries, fs.entry_cap, fs.strips, as<uint32_t>(ctx->strip_ranges),
-                                s),
+                                s, masked ? 0xFFFFu : ~0u),
              "strip ranges");
     ATHX_HIP(ctx, launch_seg_scan(as<uint32_t>(ctx->strip_ranges), fs.strips, as<uint32_t>(ctx->segbase),
                                   totals + kTotalSegs, as<uint32_t>(ctx->scan_status), s),

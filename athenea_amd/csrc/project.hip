@@ -755,11 +755,11 @@ __global__ void __launch_bounds__(kSmallThreads) count_small_kernel(const DView*
 // Where each tile's pairs start and end in the sorted keys, by binary search
 // on the tile bits: splat_tiles_clear.slang:8-17 and splat_ranges.slang:11-25
 // give a tile with no pairs (0, 0), and so does this.
-__device__ inline uint32_t lower_bound_tile(const uint32_t* tiles, uint32_t n, uint64_t tile) {
+__device__ inline uint32_t lower_bound_tile(const uint32_t* tiles, uint32_t n, uint64_t tile, uint32_t key_mask) {
     uint32_t lo = 0, hi = n;
     while (lo < hi) {
         const uint32_t mid = lo + ((hi - lo) >> 1);
-        if ((uint64_t)tiles[mid] < tile) {
+        if ((uint64_t)(tiles[mid] & key_mask) < tile) {
             lo = mid + 1;
         } else {
             hi = mid;
@@ -770,14 +770,15 @@ __device__ inline uint32_t lower_bound_tile(const uint32_t* tiles, uint32_t n, u
 
 __global__ void __launch_bounds__(256) ranges_kernel(const uint32_t* __restrict__ keys,
                                                      const uint64_t* __restrict__ n_dev, uint32_t cap, uint32_t tiles,
+                                                     uint32_t key_mask,
                                                      uint32_t* __restrict__ ranges) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= tiles) {
         return;
     }
     const uint32_t n = *n_dev < cap ? (uint32_t)*n_dev : cap;
-    const uint32_t start = lower_bound_tile(keys, n, t);
-    const uint32_t end = lower_bound_tile(keys, n, (uint64_t)t + 1);
+    const uint32_t start = lower_bound_tile(keys, n, t, key_mask);
+    const uint32_t end = lower_bound_tile(keys, n, (uint64_t)t + 1, key_mask);
     ranges[2 * t] = start < end ? start : 0u;
     ranges[2 * t + 1] = start < end ? end : 0u;
 }
@@ -816,6 +817,100 @@ __global__ void __launch_bounds__(256) settle_kernel(const DView* __restrict__ v
             }
         }
     }
+}
+
+// Deciding most of an entry's tiles without the float tile test. The test
+// (box_hit) evaluates a x^2 + 2 b x y + c y^2 at points of the tile's box
+// whose coordinates are the box edges rounded relative to the centre, in
+// about ten float operations; its result differs from the exact least power
+// over the exact box by a few units of 2^-24 M, M = a X^2 + 2 |b| X Y + c Y^2
+// with X, Y the box's largest distances from the centre. The exact least
+// power over a tile box is at most C exactly when the box's x-range meets
+// the x-extent of the ellipse q <= C sliced by the box's row band (the slice
+// is convex and spans the band's height). That extent is computed here in
+// float, once an entry, at C = cutoff -+ kSlack M: a tile meeting the inner
+// one is touched by the float test, a tile missing the outer one is not, and
+// only the tiles in between are tested. The extent's own rounding is of the
+// same order in power (where its square root is ill-conditioned, at the
+// ellipse's top and bottom, the power's x-gradient vanishes), and the
+// determinant is taken in FP64, where a thin ellipse's does not cancel;
+// kSlack = 256 * 2^-24 covers both with a wide margin.
+constexpr float kSlack = 256.0f * 0x1p-24f;
+
+// [xa, xb]: the x-extent of {(x, y) : y0 <= y <= y1, q(x, y) <= C}
+// (coordinates relative to the centre; sa = sqrt(a / det), sc = sqrt(c /
+// det)); false when it is empty.
+__device__ inline bool band_extent(float a, float b, float ia, float ic, float det, float sa, float sc, float C,
+                                   float y0, float y1, float& xa, float& xb) {
+    if (!(C > 0.0f)) {
+        return false;
+    }
+    // The ellipse's rows: |y| <= sqrt(a C / det); its widest points (x = -+xg)
+    // at y = +-b xg / c.
+    const float sC = __builtin_amdgcn_sqrtf(C);
+    const float ym = sC * sa, xg = sC * sc;
+    const float ya = fmax_(y0, -ym), yb = fmin_(y1, ym);
+    if (ya > yb) {
+        return false;
+    }
+    // Right end r(y) = (-b y + sqrt(a C - det y^2)) / a is concave, the left
+    // end convex: each is extreme at its widest point's row, clamped to the band.
+    const float yr = clamp_((-b * xg) * ic, ya, yb);
+    const float yl = clamp_((b * xg) * ic, ya, yb);
+    const float aC = a * C;
+    xb = (-b * yr + __builtin_amdgcn_sqrtf(fmax_(aC - det * yr * yr, 0.0f))) * ia;
+    xa = (-b * yl - __builtin_amdgcn_sqrtf(fmax_(aC - det * yl * yl, 0.0f))) * ia;
+    return true;
+}
+
+// The strip columns c0 .. c1 whose box x-range meets [xa, xb] (relative to
+// the centre cx): bit j for column tx0 + j. A column's box is
+// [tx T + 0.5, min((tx + 1) T, W) - 0.5] in pixel centres.
+__device__ inline uint32_t columns_meeting(float xa, float xb, float cx, uint32_t T, uint32_t W, uint32_t tx0,
+                                          uint32_t c0, uint32_t c1) {
+    uint32_t bits = 0;
+    for (uint32_t tx = c0; tx <= c1; ++tx) {
+        const uint32_t e = (tx + 1u) * T < W ? (tx + 1u) * T : W;
+        const float lo = ((float)(tx * T) + 0.5f) - cx, hi = ((float)(e - 1u) + 0.5f) - cx;
+        if (hi >= xa && lo <= xb) {
+            bits |= 1u << (tx - tx0);
+        }
+    }
+    return bits;
+}
+
+// An entry's strip columns c0 .. c1 (tile row with band terms Y): the
+// columns surely touched, and those the float test must decide.
+struct Decided {
+    uint32_t sure, test;
+};
+
+__device__ inline Decided decide_columns(const BigFootprint& g, const BoxAxis& Y, uint32_t T, uint32_t W,
+                                         uint32_t tx0, uint32_t c0, uint32_t c1) {
+    Decided d{0u, 0u};
+    if (c1 < c0) {
+        return d;
+    }
+    const double detd = (double)g.a * (double)g.c - (double)g.b * (double)g.b;
+    if (!(g.a > 1e-9f && g.c > 1e-9f && detd > 0.0)) {
+        d.test = ((1u << (c1 - c0 + 1u)) - 1u) << (c0 - tx0);
+        return d;
+    }
+    const float det = (float)detd, inv_det = __builtin_amdgcn_rcpf(det);
+    const uint32_t ex = (c1 + 1u) * T < W ? (c1 + 1u) * T : W;
+    const float X = fmax_(fabsf(((float)(c0 * T) + 0.5f) - g.cx), fabsf(((float)(ex - 1u) + 0.5f) - g.cx));
+    const float Yb = fmax_(fabsf(Y.lo), fabsf(Y.hi));
+    const float slack = kSlack * (g.a * X * X + 2.0f * fabsf(g.b) * X * Yb + g.c * Yb * Yb);
+    const float sa = __builtin_amdgcn_sqrtf(g.a * inv_det), sc = __builtin_amdgcn_sqrtf(g.c * inv_det);
+    float xa = 0.0f, xb = 0.0f;
+    d.sure = band_extent(g.a, g.b, g.ia, g.ic, det, sa, sc, g.cutoff - slack, Y.lo, Y.hi, xa, xb)
+                 ? columns_meeting(xa, xb, g.cx, T, W, tx0, c0, c1)
+                 : 0u;
+    const uint32_t maybe = band_extent(g.a, g.b, g.ia, g.ic, det, sa, sc, g.cutoff + slack, Y.lo, Y.hi, xa, xb)
+                               ? columns_meeting(xa, xb, g.cx, T, W, tx0, c0, c1)
+                               : 0u;
+    d.test = maybe & ~d.sure;
+    return d;
 }
 
 // One strip entry for every (row, strip) a drawn footprint's rectangle
@@ -882,6 +977,96 @@ __global__ void __launch_bounds__(kEmitThreads) strip_emit_kernel(const DView* _
                 keys_out[at] = rk.y + row * rk.w + col;
                 ranks_out[at] = rk.x;
             }
+        }
+    }
+}
+
+// The strip entries with their tiles decided (frames of at most 65 536
+// strips): the emission above, but each entry's 16-bit mask of touched strip
+// tiles is computed here, where each rank's footprint is read once into LDS
+// (decide_columns, then the float test for the few columns it leaves), and
+// rides in the key's upper half: key = strip | mask << 16, value = the slot.
+// The strip sort orders the strip bits only, and the binning then reads the
+// keys and values in order and nothing else.
+__global__ void __launch_bounds__(kEmitThreads) strip_mask_emit_kernel(const DView* __restrict__ views, uint32_t n,
+                                                                       uint32_t T, const uint2* __restrict__ rects,
+                                                                       const uint32_t* __restrict__ sorted_slots,
+                                                                       const uint32_t* __restrict__ offsets,
+                                                                       const uint64_t* __restrict__ visible_dev,
+                                                                       uint32_t max_visible, uint32_t cap,
+                                                                       const float* __restrict__ records,
+                                                                       uint32_t* __restrict__ keys_out,
+                                                                       uint32_t* __restrict__ slots_out) {
+    __shared__ uint32_t s_end[kEmitThreads];
+    __shared__ float4 s_geo[kEmitThreads];   // rank: centre x, y, conic a, b
+    __shared__ float4 s_geo2[kEmitThreads];  // rank: conic c, cut power, 1 / a, 1 / c
+    __shared__ uint4 s_int[kEmitThreads];    // rank: slot, first | last tile column << 16, first row, first strip
+    __shared__ uint4 s_int2[kEmitThreads];   // rank: strips a row, strips between rows, view, first strip column
+    const uint32_t lane = threadIdx.x & 63u, wbase = threadIdx.x & ~63u;
+    const uint32_t visible = *visible_dev < max_visible ? (uint32_t)*visible_dev : max_visible;
+    const uint32_t k = blockIdx.x * kEmitThreads + threadIdx.x;
+    const uint32_t k0 = k - lane;
+    uint32_t count = 0;
+    if (k < visible) {
+        const uint32_t slot = sorted_slots[k];
+        const uint2 r = rects[slot];
+        const uint32_t x0 = r.x & 0xFFFFu, y0 = r.x >> 16, w = r.y & 0xFFFFu, h = r.y >> 16;
+        const uint32_t view = slot / n;
+        const DView& V = views[view];
+        const uint32_t sx0 = x0 / kStripTiles;
+        const uint32_t ns = (x0 + w - 1u) / kStripTiles - sx0 + 1u;
+        count = h * ns;
+        const float4* rec = reinterpret_cast<const float4*>(records + (size_t)slot * kRecordFloats);
+        const float4 screen = rec[0], conic = rec[1];
+        s_geo[threadIdx.x] = make_float4(screen.x, screen.y, conic.x, conic.y);
+        s_geo2[threadIdx.x] = make_float4(conic.z, screen.w, recip_of(conic.x), recip_of(conic.z));
+        s_int[threadIdx.x] = make_uint4(slot, x0 | ((x0 + w - 1u) << 16), y0, V.strip_base + y0 * V.strips_x + sx0);
+        s_int2[threadIdx.x] = make_uint4(ns, V.strips_x, view, sx0);
+    }
+    const uint32_t end = wave_inclusive_sum(count, lane);
+    s_end[threadIdx.x] = end;
+    const uint32_t total = lane_u(end, 63);
+    const uint32_t base = k0 < visible ? offsets[k0 / 64u] : 0u;
+    __syncthreads();
+    const uint32_t* ends = s_end + wbase;
+    for (uint32_t g0 = 0; g0 < total; g0 += 64) {
+        const uint32_t gi = g0 + lane;
+        if (gi >= total) {
+            continue;
+        }
+        const uint32_t j = run_of(ends, gi);
+        const uint32_t q = gi - (j != 0 ? ends[j - 1] : 0u);
+        const uint4 I = s_int[wbase + j], I2 = s_int2[wbase + j];
+        const float4 geo = s_geo[wbase + j], geo2 = s_geo2[wbase + j];
+        const uint32_t row = q / I2.x, sxi = q - row * I2.x;
+        const uint32_t tx0 = (I2.w + sxi) * kStripTiles;
+        const uint32_t x0 = I.y & 0xFFFFu, x1 = I.y >> 16;
+        const uint32_t c0 = x0 > tx0 ? x0 : tx0;
+        const uint32_t c1 = x1 < tx0 + kStripTiles - 1u ? x1 : tx0 + kStripTiles - 1u;
+        BigFootprint g{};
+        g.cx = geo.x;
+        g.cy = geo.y;
+        g.a = geo.z;
+        g.b = geo.w;
+        g.c = geo2.x;
+        g.cutoff = geo2.y;
+        g.ia = geo2.z;
+        g.ic = geo2.w;
+        const DView& V = views[I2.z];
+        const float b2 = 2.0f * g.b;
+        const BoxAxis Y = row_of(g, V, T, b2, I.z + row);
+        const Decided d = decide_columns(g, Y, T, V.width, tx0, c0, c1);
+        uint32_t mask = d.sure;
+        for (uint32_t t = d.test; t != 0; t &= t - 1u) {
+            const uint32_t bit = (uint32_t)__builtin_ctz(t);
+            if (box_hit(g, b2, column_of(g, V, T, b2, tx0 + bit), Y)) {
+                mask |= 1u << bit;
+            }
+        }
+        const uint64_t at = (uint64_t)base + gi;
+        if (at < cap) {
+            keys_out[at] = (I.w + row * I2.y + sxi) | (mask << 16);
+            slots_out[at] = I.x;
         }
     }
 }
@@ -986,6 +1171,14 @@ __device__ inline uint32_t strip_hits(const StripFootprint& sf, const DView& V, 
 constexpr uint32_t kSegThreads = 256;
 constexpr uint32_t kSegPer = kStripSeg / kSegThreads;
 
+// What a tile test of a strip entry reads: its footprint's column terms
+// and its row's terms.
+struct StripEntryLds {
+    float4 geo;   // centre x, conic a, conic b, conic c
+    float4 row;   // the row's box terms: lo, hi, q_lo, q_hi
+    float4 more;  // s_lo, s_hi, 1 / c, cut power
+};
+
 // Tests the segment's entries and counts each strip tile's hits:
 // table[16 base + j nseg + k] for tile j of segment k of a strip whose
 // first segment is base, so the table read in order is strip by strip,
@@ -999,11 +1192,55 @@ constexpr uint32_t kSegPer = kStripSeg / kSegThreads;
 // one test, folding a hit into its entry's mask with an LDS OR. Each entry's
 // footprint and row terms are in LDS beside the list. Every entry's slot,
 // then every footprint, is loaded before any is tested.
-struct StripEntryLds {
-    float4 geo;   // centre x, conic a, conic b, conic c
-    float4 row;   // the row's box terms: lo, hi, q_lo, q_hi
-    float4 more;  // s_lo, s_hi, 1 / c, cut power
-};
+
+// With the masks in the keys (strip_mask_emit_kernel), a segment only
+// reads them: each entry's mask is kept for the pairs, each tile's hits
+// counted into the table.
+__global__ void __launch_bounds__(kSegThreads) strip_mask_count_kernel(StripBins b) {
+    __shared__ uint32_t s_hits[kSegThreads / 64][kStripTiles];
+    const uint64_t total = *b.seg_total;
+    const uint32_t segs = total < b.seg_cap ? (uint32_t)total : b.seg_cap;
+    if (blockIdx.x >= segs) {
+        return;
+    }
+    const uint4 d0 = b.seg_desc[2u * blockIdx.x];
+    if (d0.x >= d0.y) {
+        return;
+    }
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    uint32_t count[kStripTiles];
+#pragma unroll
+    for (uint32_t t = 0; t < kStripTiles; ++t) {
+        count[t] = 0;
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < kSegPer; ++j) {
+        const uint32_t e = d0.x + j * kSegThreads + threadIdx.x;
+        const uint32_t mask = e < d0.y ? b.strip_keys[e] >> 16 : 0u;
+        if (e < d0.y) {
+            b.masks[e] = (uint16_t)mask;
+        }
+#pragma unroll
+        for (uint32_t t = 0; t < kStripTiles; ++t) {
+            count[t] += (uint32_t)__popcll(__ballot((mask >> t) & 1u));
+        }
+    }
+    if (lane == 0) {
+#pragma unroll
+        for (uint32_t t = 0; t < kStripTiles; ++t) {
+            s_hits[wave][t] = count[t];
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < kStripTiles) {
+        uint32_t sum = 0;
+#pragma unroll
+        for (uint32_t w = 0; w < kSegThreads / 64; ++w) {
+            sum += s_hits[w][threadIdx.x];
+        }
+        b.table[d0.z + threadIdx.x * d0.w] = sum;
+    }
+}
 
 __global__ void __launch_bounds__(kSegThreads) strip_count_kernel(StripBins b) {
     __shared__ StripEntryLds s_entry[kSegThreads];
@@ -1050,8 +1287,9 @@ __global__ void __launch_bounds__(kSegThreads) strip_count_kernel(StripBins b) {
     }
     uint16_t* items = s_items[wave];
     for (uint32_t j = 0; j < kSegPer; ++j) {
-        // This lane's entry: its columns in the strip and its terms in LDS.
-        uint32_t c0 = 0, cols = 0;
+        // This lane's entry: its columns in the strip, those decided and
+        // those to test, and its terms in LDS.
+        uint32_t sure = 0, test = 0;
         if (slot[j] != ~0u) {
             BigFootprint g{};
             g.cx = f[j].screen.x;
@@ -1062,19 +1300,23 @@ __global__ void __launch_bounds__(kSegThreads) strip_count_kernel(StripBins b) {
             g.c = f[j].conic.z;
             g.ia = recip_of(g.a);
             g.ic = recip_of(g.c);
-            c0 = f[j].x0 > tx0 ? f[j].x0 : tx0;
+            const uint32_t c0 = f[j].x0 > tx0 ? f[j].x0 : tx0;
             const uint32_t c1 = f[j].x1 < tx0 + kStripTiles - 1u ? f[j].x1 : tx0 + kStripTiles - 1u;
-            cols = c1 >= c0 ? c1 - c0 + 1u : 0u;
             const BoxAxis Y = row_of(g, V, T, 2.0f * g.b, ty);
             s_entry[threadIdx.x].geo = make_float4(g.cx, g.a, g.b, g.c);
             s_entry[threadIdx.x].row = make_float4(Y.lo, Y.hi, Y.q_lo, Y.q_hi);
             s_entry[threadIdx.x].more = make_float4(Y.s_lo, Y.s_hi, g.ic, g.cutoff);
+            // Sure hits and sure misses from the slice's extent; the rest tested.
+            const Decided d = decide_columns(g, Y, T, V.width, tx0, c0, c1);
+            sure = d.sure;
+            test = d.test;
         }
-        s_mask[threadIdx.x] = 0;
-        const uint32_t end = wave_inclusive_sum(cols, lane);
+        s_mask[threadIdx.x] = sure;
+        const uint32_t ntest = (uint32_t)__popc(test);
+        const uint32_t end = wave_inclusive_sum(ntest, lane);
         const uint32_t total = lane_u(end, 63);
-        for (uint32_t k = 0; k < cols; ++k) {
-            items[end - cols + k] = (uint16_t)(lane | ((c0 - tx0 + k) << 6));
+        for (uint32_t k = 0, t = test; t != 0; ++k, t &= t - 1u) {
+            items[end - ntest + k] = (uint16_t)(lane | ((uint32_t)__builtin_ctz(t) << 6));
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
         __builtin_amdgcn_wave_barrier();
@@ -1356,12 +1598,12 @@ hipError_t launch_fill_u32(uint32_t* p, uint32_t value, uint64_t n, hipStream_t 
 }
 
 hipError_t launch_ranges(const uint32_t* sorted_keys, const uint64_t* n_dev, uint32_t cap, uint32_t tiles,
-                         uint32_t* ranges, hipStream_t s) {
+                         uint32_t* ranges, hipStream_t s, uint32_t key_mask) {
     if (tiles == 0) {
         return hipSuccess;
     }
     hipLaunchKernelGGL(ranges_kernel, dim3(div_up(tiles, 256)), dim3(256), 0, s, sorted_keys, n_dev, cap, tiles,
-                       ranges);
+                       key_mask, ranges);
     return hipGetLastError();
 }
 
@@ -1405,9 +1647,27 @@ hipError_t launch_strip_count(const StripBins& b, hipStream_t s) {
     if (b.seg_cap == 0) {
         return hipSuccess;
     }
-    hipLaunchKernelGGL(strip_count_kernel, dim3(b.seg_cap), dim3(kSegThreads), 0, s, b);
+    if (b.strip_keys) {
+        hipLaunchKernelGGL(strip_mask_count_kernel, dim3(b.seg_cap), dim3(kSegThreads), 0, s, b);
+    } else {
+        hipLaunchKernelGGL(strip_count_kernel, dim3(b.seg_cap), dim3(kSegThreads), 0, s, b);
+    }
     return hipGetLastError();
 }
+
+hipError_t launch_strip_mask_emit(const DView* views_dev, uint32_t n, uint32_t tile, const uint2* rects,
+                                  const uint32_t* sorted_slots, const uint32_t* offsets, const uint64_t* visible_dev,
+                                  uint32_t max_visible, uint32_t cap, const float* records, uint32_t* strip_keys,
+                                  uint32_t* strip_slots, hipStream_t s) {
+    if (max_visible == 0 || cap == 0) {
+        return hipSuccess;
+    }
+    hipLaunchKernelGGL(strip_mask_emit_kernel, dim3(div_up(max_visible, kEmitThreads)), dim3(kEmitThreads), 0, s,
+                       views_dev, n, tile, rects, sorted_slots, offsets, visible_dev, max_visible, cap, records,
+                       strip_keys, strip_slots);
+    return hipGetLastError();
+}
+
 
 hipError_t launch_strip_pairs(const StripBins& b, hipStream_t s) {
     if (b.seg_cap == 0) {

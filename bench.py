@@ -46,7 +46,7 @@ def stage_bytes(n: int, visible: int, pairs: int, entries: int, strips: int, til
     and write whatever its implementation."""
     depth_passes = 3
     strip_passes = (max(strips - 1, 1).bit_length() + 7) // 8
-    segments = entries // 256 + strips
+    segments = entries // 1024 + strips
     return {
         # FP32 parameters read (236 B at SH degree 3, + 32 B of relight
         # attributes: albedo, normal, metallic, roughness); record, depth key,
@@ -58,15 +58,14 @@ def stage_bytes(n: int, visible: int, pairs: int, entries: int, strips: int, til
         "depth_sort": n * 8 + visible * 8 + visible * 4 + depth_passes * visible * 16,
         # sorted slots read, entry counts gathered
         "counts": visible * 8,
-        # sorted slot, rectangle and offset read a rank; (strip, slot) written
-        # an entry; the strip sort: a histogram read, then a read and a write
-        # of (strip, slot) a pass
-        "emit": visible * (4 + 8) + entries * 8 + entries * 4 + strip_passes * entries * 16,
-        # an entry's slot, its footprint's record half and rectangle read and
-        # its hit mask written (count); the table written, scanned (read and
-        # written) and read; the slot and mask read again and each pair's slot
-        # written at its place
-        "tile_sort": entries * (4 + 32 + 8 + 2) + segments * 16 * 4 * 4 + entries * (4 + 2) + pairs * 4,
+        # sorted slot, rectangle, the record's centre, conic and cut power
+        # read a rank; (strip | tile mask, slot) written an entry; the strip
+        # sort: a histogram read, then a read and a write of (key, slot) a pass
+        "emit": visible * (4 + 8 + 32) + entries * 8 + entries * 4 + strip_passes * entries * 16,
+        # an entry's key read and its mask kept (count); the table written,
+        # scanned (read and written) and read; the mask and slot read again
+        # and each pair's slot written at its place
+        "tile_sort": entries * (4 + 2) + segments * 16 * 4 * 4 + entries * (2 + 4) + pairs * 4,
         "ranges": tiles * 8,
         # each pair's slot read, each drawn record once (the blend's re-reads
         # of a record hit the caches); RGBA, depth, final T, last index written
@@ -79,9 +78,9 @@ STAGE_KERNELS = {
     "project": "project_kernel, settle_kernel",
     "depth_sort": "the visible compaction scan, histogram_kernel, hist_reduce_kernel, onesweep_kernel x3",
     "counts": "the strip-entry scan in depth order",
-    "emit": "strip_emit_kernel, the strip sort (histogram, onesweep or reduce-scan-scatter passes), "
-            "ranges_kernel over the strips, the segment scan",
-    "tile_sort": "strip_count_kernel, the table scan, strip_pairs_kernel",
+    "emit": "strip_mask_emit_kernel (entries with their tiles decided), the strip sort (onesweep or "
+            "reduce-scan-scatter passes), ranges_kernel over the strips, the segment scan, seg_desc_kernel",
+    "tile_sort": "strip_mask_count_kernel, the table scan, strip_pairs_kernel",
     "ranges": "tile_ranges_kernel",
     "blend": "blend_kernel",
 }
