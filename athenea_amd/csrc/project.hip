@@ -23,44 +23,10 @@ constexpr float kC3_0 = -0.5900435899266435f, kC3_1 = 2.890611442640554f, kC3_2 
                 kC3_3 = 0.3731763325901154f, kC3_4 = -0.4570457994644658f, kC3_5 = 1.445305721320277f,
                 kC3_6 = -0.5900435899266435f;
 
-__device__ inline float fmin_(float a, float b) { return a < b ? a : b; }
-__device__ inline float fmax_(float a, float b) { return a > b ? a : b; }
-__device__ inline float clamp_(float x, float lo, float hi) { return fmin_(fmax_(x, lo), hi); }
-
 // frame.slang:325-327
 __device__ inline float cutoff_power(float alpha) {
     return fmin_(2.0f * athx_logf(fmax_(255.0f * alpha, 1.0001f)), 9.0f);
 }
-
-// frame.slang:354-371: the least power a * x^2 + 2 b x y + c y^2 takes
-// over the box [lox, hix] x [loy, hiy]: zero when the box holds the centre,
-// else the least over its four edges, each at the edge's clamped 1D
-// minimiser. The minimiser ((-b) x) / c is taken as ((-b) x) * (1 / c),
-// with the reciprocals (ia = 1 / a, ic = 1 / c) formed once a footprint:
-// the reference's divide has no defined rounding across its backends, and
-// this form is the one every test of a tile here and in the host
-// restatement computes, bit for bit.
-__device__ inline float min_power_over_box(float a, float b, float c, float ia, float ic, float lox, float loy,
-                                           float hix, float hiy) {
-    if (lox <= 0.0f && hix >= 0.0f && loy <= 0.0f && hiy >= 0.0f) {
-        return 0.0f;
-    }
-    float best = 3.4e38f;
-    for (int e = 0; e < 2; ++e) {
-        const float x = e == 0 ? lox : hix;
-        const float y = c > 1e-12f ? clamp_(((-b) * x) * ic, loy, hiy) : loy;
-        best = fmin_(best, a * x * x + 2.0f * b * x * y + c * y * y);
-    }
-    for (int e = 0; e < 2; ++e) {
-        const float y = e == 0 ? loy : hiy;
-        const float x = a > 1e-12f ? clamp_(((-b) * y) * ia, lox, hix) : lox;
-        best = fmin_(best, a * x * x + 2.0f * b * x * y + c * y * y);
-    }
-    return best;
-}
-
-// The reciprocals min_power_over_box takes, formed once a footprint.
-__device__ inline float recip_of(float v) { return v > 1e-12f ? 1.0f / v : 0.0f; }
 
 // frame.slang:374-379
 __device__ inline bool tile_touched(uint32_t W, uint32_t H, uint32_t T, float cx, float cy, float a, float b,
@@ -865,18 +831,41 @@ __device__ inline bool band_extent(float a, float b, float ia, float ic, float d
 
 // The strip columns c0 .. c1 whose box x-range meets [xa, xb] (relative to
 // the centre cx): bit j for column tx0 + j. A column's box is
-// [tx T + 0.5, min((tx + 1) T, W) - 0.5] in pixel centres.
+// [tx T + 0.5, min((tx + 1) T, W) - 0.5] in pixel centres, and both its
+// edges, rounded relative to cx, are nondecreasing in the column: the
+// columns meeting the extent run from the first whose right edge reaches xa
+// to the last whose left edge is at most xb. Each is found from its
+// estimate (within a column of it) by stepping while a neighbour still
+// qualifies.
 __device__ inline uint32_t columns_meeting(float xa, float xb, float cx, uint32_t T, uint32_t W, uint32_t tx0,
                                           uint32_t c0, uint32_t c1) {
-    uint32_t bits = 0;
-    for (uint32_t tx = c0; tx <= c1; ++tx) {
-        const uint32_t e = (tx + 1u) * T < W ? (tx + 1u) * T : W;
-        const float lo = ((float)(tx * T) + 0.5f) - cx, hi = ((float)(e - 1u) + 0.5f) - cx;
-        if (hi >= xa && lo <= xb) {
-            bits |= 1u << (tx - tx0);
-        }
+    const auto lo_of = [&](int tx) { return ((float)((uint32_t)tx * T) + 0.5f) - cx; };
+    const auto hi_of = [&](int tx) {
+        const uint32_t e = ((uint32_t)tx + 1u) * T < W ? ((uint32_t)tx + 1u) * T : W;
+        return ((float)(e - 1u) + 0.5f) - cx;
+    };
+    const int lo = (int)c0, hi = (int)c1;
+    const float inv_t = 1.0f / (float)T;
+    // First column whose right edge reaches xa (hi + 1 when none).
+    int first = (int)ceilf(clamp_((xa + cx + 0.5f) * inv_t - 1.0f, (float)lo, (float)(hi + 1)));
+    while (first > lo && hi_of(first - 1) >= xa) {
+        --first;
     }
-    return bits;
+    while (first <= hi && hi_of(first) < xa) {
+        ++first;
+    }
+    // Last column whose left edge is at most xb (lo - 1 when none).
+    int last = (int)floorf(clamp_((xb + cx - 0.5f) * inv_t, (float)(lo - 1), (float)hi));
+    while (last < hi && lo_of(last + 1) <= xb) {
+        ++last;
+    }
+    while (last >= lo && lo_of(last) > xb) {
+        --last;
+    }
+    if (first > last) {
+        return 0u;
+    }
+    return ((1u << (uint32_t)(last - first + 1)) - 1u) << ((uint32_t)first - tx0);
 }
 
 // An entry's strip columns c0 .. c1 (tile row with band terms Y): the

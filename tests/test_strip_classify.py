@@ -115,3 +115,4 @@ def test_without_slack_the_rule_fails_somewhere():
         hit, sure, maybe = classify(seed, 0.0)
         fails += int(np.sum(sure & ~hit) + np.sum(~maybe & hit))
     assert fails > 0
+
