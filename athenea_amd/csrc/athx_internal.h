@@ -306,15 +306,16 @@ hipError_t launch_sort(uint64_t* keys, uint32_t* values, const uint64_t* n_dev, 
                        uint32_t key_bits, const SortScratch& sc, hipStream_t s, bool* in_alt = nullptr);
 hipError_t launch_sort(uint32_t* keys, uint32_t* values, const uint64_t* n_dev, uint32_t cap, uint32_t lo_bit,
                        uint32_t key_bits, const SortScratch& sc, hipStream_t s, bool* in_alt = nullptr);
-// The visible slots in slot order with their depth keys; the count lands in
+// The visible slots in slot order with their depth keys (each key's top byte
+// holding the slot's strip-entry count, capped at 255); the count lands in
 // *visible_total (device).
 hipError_t launch_compact(const uint32_t* touched, const uint32_t* depth_keys, uint32_t slots, uint32_t* keys_out,
                           uint32_t* slots_out, uint64_t* visible_total, uint32_t* status_words, hipStream_t s);
 // Strip entries of the depth-sorted slots, scanned in that order:
 // offsets[k] is rank k's first entry; the total lands in *entry_total.
-hipError_t launch_entry_scan(const uint32_t* touched, const uint32_t* sorted_slots, const uint64_t* visible_dev,
-                             uint32_t max_visible, uint32_t* offsets, uint64_t* entry_total, uint32_t* status_words,
-                             hipStream_t s);
+hipError_t launch_entry_scan(const uint32_t* touched, const uint32_t* sorted_slots, const uint32_t* sorted_keys,
+                             const uint64_t* visible_dev, uint32_t max_visible, uint32_t* offsets,
+                             uint64_t* entry_total, uint32_t* status_words, hipStream_t s);
 // Tile counts of the depth-sorted slots, scanned: slot_offsets[slot] is its
 // first pair; the pair total lands in *pair_total (device).
 hipError_t launch_gather_scan(const uint32_t* touched, const uint32_t* sorted_slots, const uint64_t* visible_dev,

@@ -611,7 +611,9 @@ athx_status enqueue_front(athx_ctx* ctx, const ProjectArgs& pa, const FrameShape
     if (timed) {
         ATHX_HIP(ctx, hipEventRecord(ctx->ev[2], s), "event");
     }
-    ATHX_HIP(ctx, launch_entry_scan(as<uint32_t>(ctx->touched), sorted_slots_of(ctx, in_alt), totals + kTotalVisible,
+    ATHX_HIP(ctx, launch_entry_scan(as<uint32_t>(ctx->touched), sorted_slots_of(ctx, in_alt),
+                                    as<uint32_t>(in_alt ? ctx->depth_alt_keys : ctx->depth_keys_sorted),
+                                    totals + kTotalVisible,
                                     fs.slots, as<uint32_t>(ctx->offsets), totals + kTotalEntries,
                                     as<uint32_t>(ctx->scan_status), s),
              "entries");

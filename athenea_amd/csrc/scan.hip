@@ -122,15 +122,21 @@ struct GatherScan {
     __device__ void store(uint32_t k, uint32_t, uint64_t excl) const { slot_offsets[sorted_slots[k]] = (uint32_t)excl; }
 };
 
+// A rank's entry count rides in its sorted depth key's top byte (the
+// compaction puts it there, 255 meaning "255 or more: read the slot's").
 struct EntryScan {
     using Value = uint32_t;
     const uint32_t* touched;
     const uint32_t* sorted_slots;
+    const uint32_t* sorted_keys;
     uint32_t* offsets;  // [ranks / 64]
     const uint64_t* n_dev;
     uint32_t cap;
     __device__ uint32_t size() const { return *n_dev < cap ? (uint32_t)*n_dev : cap; }
-    __device__ uint32_t load(uint32_t k) const { return touched[sorted_slots[k]]; }
+    __device__ uint32_t load(uint32_t k) const {
+        const uint32_t c = sorted_keys[k] >> 24;
+        return c != 255u ? c : touched[sorted_slots[k]];
+    }
     __device__ void store(uint32_t k, uint32_t, uint64_t excl) const {
         if ((k & 63u) == 0) {
             offsets[k >> 6] = (uint32_t)excl;
@@ -271,14 +277,239 @@ __global__ void clear_status_kernel(Policy pol, uint64_t* __restrict__ status, u
     }
 }
 
+// Long scans (past kLongScan elements) go in three launches instead: each
+// partition's sum, one workgroup's scan of the sums, then each partition
+// scanned from its offset. With tens of thousands of partitions in flight
+// the look-back above waits on predecessors that have not yet started; here
+// nothing waits, and the data is read twice.
+constexpr uint32_t kLongScan = 1u << 24;
+constexpr uint32_t kSumThreads = 1024;
+
+template <typename Policy>
+__global__ void __launch_bounds__(kThreads) partition_sum_kernel(Policy pol, uint64_t* __restrict__ sums) {
+    __shared__ uint64_t s_wave[kThreads / 64];
+    const uint32_t n = pol.size();
+    const uint32_t part = blockIdx.x;
+    if ((uint64_t)part * kTile >= n && part != 0) {
+        return;
+    }
+    const uint64_t first = (uint64_t)part * kTile + (uint64_t)threadIdx.x * kItems;
+    uint64_t mine = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < kItems; ++k) {
+        mine += first + k < n ? (uint64_t)pol.load((uint32_t)(first + k)) : 0u;
+    }
+    mine = wave_sum(mine);
+    if ((threadIdx.x & 63u) == 0) {
+        s_wave[threadIdx.x >> 6] = mine;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t t = 0;
+#pragma unroll
+        for (uint32_t w = 0; w < kThreads / 64; ++w) {
+            t += s_wave[w];
+        }
+        sums[part] = t;
+    }
+}
+
+// The partitions' sums scanned in place by one workgroup (a run of them a
+// thread); the total lands in *total.
+template <typename Policy>
+__global__ void __launch_bounds__(kSumThreads) partition_offsets_kernel(Policy pol, uint64_t* __restrict__ sums,
+                                                                        uint64_t* __restrict__ total) {
+    __shared__ uint64_t s_wave[kSumThreads / 64];
+    const uint32_t n = pol.size();
+    const uint32_t parts = div_up(n ? n : 1, kTile);
+    const uint32_t per = div_up(parts, kSumThreads);
+    const uint32_t lo = threadIdx.x * per, hi = lo + per < parts ? lo + per : parts;
+    uint64_t mine = 0;
+    for (uint32_t p = lo; p < hi; ++p) {
+        mine += sums[p];
+    }
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint64_t incl = wave_inclusive_scan(mine, lane);
+    if (lane == 63) {
+        s_wave[wave] = incl;
+    }
+    __syncthreads();
+    uint64_t run = incl - mine;
+    uint64_t all = 0;
+    for (uint32_t w = 0; w < kSumThreads / 64; ++w) {
+        run += w < wave ? s_wave[w] : 0u;
+        all += s_wave[w];
+    }
+    for (uint32_t p = lo; p < hi; ++p) {
+        const uint64_t v = sums[p];
+        sums[p] = run;
+        run += v;
+    }
+    if (threadIdx.x == 0) {
+        *total = n ? all : 0u;
+    }
+}
+
+template <typename Policy>
+__global__ void __launch_bounds__(kThreads) partition_scan_kernel(Policy pol, const uint64_t* __restrict__ offsets) {
+    __shared__ uint64_t s_wave[kThreads / 64];
+    const uint32_t n = pol.size();
+    const uint32_t part = blockIdx.x;
+    if ((uint64_t)part * kTile >= n) {
+        return;
+    }
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint64_t first = (uint64_t)part * kTile + (uint64_t)threadIdx.x * kItems;
+    typename Policy::Value v[kItems];
+#pragma unroll
+    for (uint32_t k = 0; k < kItems; ++k) {
+        v[k] = first + k < n ? pol.load((uint32_t)(first + k)) : 0u;
+    }
+    uint64_t mine = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < kItems; ++k) {
+        mine += v[k];
+    }
+    const uint64_t incl = wave_inclusive_scan(mine, lane);
+    if (lane == 63) {
+        s_wave[wave] = incl;
+    }
+    __syncthreads();
+    uint64_t run = offsets[part] + (incl - mine);
+#pragma unroll
+    for (uint32_t w = 0; w < kThreads / 64; ++w) {
+        run += w < wave ? s_wave[w] : 0u;
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < kItems; ++k) {
+        if (first + k < n) {
+            pol.store((uint32_t)(first + k), v[k], run);
+        }
+        run += v[k];
+    }
+}
+
 template <typename Policy>
 hipError_t run_scan(const Policy& pol, uint32_t max_n, uint64_t* total, uint32_t* status_words, hipStream_t s) {
     uint64_t* status = reinterpret_cast<uint64_t*>(status_words);
     const uint32_t parts = div_up(max_n ? max_n : 1, kTile);
+    if (max_n > kLongScan) {
+        // The status words hold one u64 a partition (and one more): the sums.
+        hipLaunchKernelGGL(partition_sum_kernel<Policy>, dim3(parts), dim3(kThreads), 0, s, pol, status);
+        hipLaunchKernelGGL(partition_offsets_kernel<Policy>, dim3(1), dim3(kSumThreads), 0, s, pol, status, total);
+        hipLaunchKernelGGL(partition_scan_kernel<Policy>, dim3(parts), dim3(kThreads), 0, s, pol, status);
+        return hipGetLastError();
+    }
     const uint32_t clear_blocks = div_up(parts + 1, 256) < 1024 ? div_up(parts + 1, 256) : 1024;
     hipLaunchKernelGGL(clear_status_kernel<Policy>, dim3(clear_blocks), dim3(256), 0, s, pol, status, total);
     hipLaunchKernelGGL(scan_kernel<Policy>, dim3(parts), dim3(kThreads), 0, s, pol, total, status);
     return hipGetLastError();
+}
+
+// The visible slots in slot order (splat_compact.slang:16-26), for any
+// number of slots, in three launches that neither wait nor scatter: each
+// partition of kTile slots counts its visible ones, one workgroup scans the
+// counts, and each partition writes its visible slots from its offset. A
+// partition is read in kItems rows of 256 consecutive slots, a slot a lane,
+// so every load is coalesced; a row's visible slots are ranked by a ballot,
+// so their (key, slot) stores are consecutive too. The key written is the
+// slot's depth key with its strip-entry count (capped at 255) in the top
+// byte, which the 24-bit depth sort carries along for the entry scan.
+__global__ void __launch_bounds__(kThreads) compact_count_kernel(const uint32_t* __restrict__ touched, uint32_t n,
+                                                                 uint64_t* __restrict__ sums) {
+    __shared__ uint32_t s_wave[kThreads / 64];
+    const uint64_t first = (uint64_t)blockIdx.x * kTile;
+    uint32_t count = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < kItems; ++k) {
+        const uint64_t i = first + (uint64_t)k * kThreads + threadIdx.x;
+        count += (uint32_t)__popcll(__ballot(i < n && touched[i] != 0));
+    }
+    if ((threadIdx.x & 63u) == 0) {
+        s_wave[threadIdx.x >> 6] = count;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t t = 0;
+#pragma unroll
+        for (uint32_t w = 0; w < kThreads / 64; ++w) {
+            t += s_wave[w];
+        }
+        sums[blockIdx.x] = t;
+    }
+}
+
+__global__ void __launch_bounds__(kSumThreads) compact_offsets_kernel(uint64_t* __restrict__ sums, uint32_t parts,
+                                                                      uint64_t* __restrict__ total) {
+    __shared__ uint64_t s_wave[kSumThreads / 64];
+    const uint32_t per = div_up(parts, kSumThreads);
+    const uint32_t lo = threadIdx.x * per, hi = lo + per < parts ? lo + per : parts;
+    uint64_t mine = 0;
+    for (uint32_t p = lo; p < hi; ++p) {
+        mine += sums[p];
+    }
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint64_t incl = wave_inclusive_scan(mine, lane);
+    if (lane == 63) {
+        s_wave[wave] = incl;
+    }
+    __syncthreads();
+    uint64_t run = incl - mine, all = 0;
+    for (uint32_t w = 0; w < kSumThreads / 64; ++w) {
+        run += w < wave ? s_wave[w] : 0u;
+        all += s_wave[w];
+    }
+    for (uint32_t p = lo; p < hi; ++p) {
+        const uint64_t v = sums[p];
+        sums[p] = run;
+        run += v;
+    }
+    if (threadIdx.x == 0) {
+        *total = all;
+    }
+}
+
+__global__ void __launch_bounds__(kThreads) compact_write_kernel(const uint32_t* __restrict__ touched,
+                                                                 const uint32_t* __restrict__ depth_keys, uint32_t n,
+                                                                 const uint64_t* __restrict__ offsets,
+                                                                 uint32_t* __restrict__ keys_out,
+                                                                 uint32_t* __restrict__ slots_out) {
+    __shared__ uint32_t s_count[kItems][kThreads / 64];
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint64_t lt_mask = (1ull << lane) - 1ull;
+    const uint64_t first = (uint64_t)blockIdx.x * kTile;
+    uint32_t entries[kItems];
+    uint64_t bal[kItems];
+#pragma unroll
+    for (uint32_t k = 0; k < kItems; ++k) {
+        const uint64_t i = first + (uint64_t)k * kThreads + threadIdx.x;
+        entries[k] = i < n ? touched[i] : 0u;
+        bal[k] = __ballot(entries[k] != 0);
+        if (lane == 0) {
+            s_count[k][wave] = (uint32_t)__popcll(bal[k]);
+        }
+    }
+    __syncthreads();
+    // Exclusive prefix of (row, wavefront) counts, in slot order.
+    uint64_t run = offsets[blockIdx.x];
+#pragma unroll
+    for (uint32_t k = 0; k < kItems; ++k) {
+        uint32_t before = 0, row = 0;
+#pragma unroll
+        for (uint32_t w = 0; w < kThreads / 64; ++w) {
+            const uint32_t c = s_count[k][w];
+            before += w < wave ? c : 0u;
+            row += c;
+        }
+        if (entries[k] != 0) {
+            const uint64_t i = first + (uint64_t)k * kThreads + threadIdx.x;
+            const uint64_t at = run + before + (uint64_t)__popcll(bal[k] & lt_mask);
+            const uint32_t count = entries[k] < 255u ? entries[k] : 255u;
+            keys_out[at] = (depth_keys[i] & 0xFFFFFFu) | (count << 24);
+            slots_out[at] = (uint32_t)i;
+        }
+        run += row;
+    }
 }
 
 }  // namespace
@@ -293,19 +524,24 @@ hipError_t launch_scan(const uint32_t* in, uint32_t* out, uint64_t* total, uint3
 
 hipError_t launch_compact(const uint32_t* touched, const uint32_t* depth_keys, uint32_t slots, uint32_t* keys_out,
                           uint32_t* slots_out, uint64_t* visible_total, uint32_t* status_words, hipStream_t s) {
-    return run_scan(CompactScan{touched, depth_keys, keys_out, slots_out, slots}, slots, visible_total, status_words,
-                    s);
+    uint64_t* sums = reinterpret_cast<uint64_t*>(status_words);
+    const uint32_t parts = div_up(slots ? slots : 1, kTile);
+    hipLaunchKernelGGL(compact_count_kernel, dim3(parts), dim3(kThreads), 0, s, touched, slots, sums);
+    hipLaunchKernelGGL(compact_offsets_kernel, dim3(1), dim3(kSumThreads), 0, s, sums, parts, visible_total);
+    hipLaunchKernelGGL(compact_write_kernel, dim3(parts), dim3(kThreads), 0, s, touched, depth_keys, slots, sums,
+                       keys_out, slots_out);
+    return hipGetLastError();
 }
 
 hipError_t launch_big_list(const Footprints& fp, uint32_t slots, uint32_t* status_words, hipStream_t s) {
     return run_scan(BigScan{fp.rects, fp.big_list, fp.big_mask_base, slots}, slots, fp.big_total, status_words, s);
 }
 
-hipError_t launch_entry_scan(const uint32_t* touched, const uint32_t* sorted_slots, const uint64_t* visible_dev,
-                             uint32_t max_visible, uint32_t* offsets, uint64_t* entry_total, uint32_t* status_words,
-                             hipStream_t s) {
-    return run_scan(EntryScan{touched, sorted_slots, offsets, visible_dev, max_visible}, max_visible, entry_total,
-                    status_words, s);
+hipError_t launch_entry_scan(const uint32_t* touched, const uint32_t* sorted_slots, const uint32_t* sorted_keys,
+                             const uint64_t* visible_dev, uint32_t max_visible, uint32_t* offsets,
+                             uint64_t* entry_total, uint32_t* status_words, hipStream_t s) {
+    return run_scan(EntryScan{touched, sorted_slots, sorted_keys, offsets, visible_dev, max_visible}, max_visible,
+                    entry_total, status_words, s);
 }
 
 hipError_t launch_seg_scan(const uint32_t* strip_ranges, uint32_t strips, uint32_t* segbase, uint64_t* seg_total,

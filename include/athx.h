@@ -289,11 +289,13 @@ ATHX_API athx_status athx_render(athx_ctx* ctx, const athx_view* views, uint32_t
 
 /*
  * athx_render without its wait: the pair buffers are sized for
- * pair_capacity pairs (0: 1.5x the largest frame this context has seen) and
- * every count after the scan is read on the device, so nothing waits for the
- * host. A frame with more pairs than the capacity draws incompletely (the
- * reference's emit drops pairs past its capacity the same way,
- * shaders/athenea/splat/splat_emit.slang:43-46): athx_frame_pairs says so.
+ * pair_capacity pairs (0: 1.25x the largest frame this context has seen),
+ * the binning's strip entries for as many as the largest frame seen needed
+ * (else pair_capacity), and every count is read on the device, so nothing
+ * waits for the host. A frame with more pairs (or entries) than that room
+ * draws incompletely (the reference's emit drops pairs past its capacity the
+ * same way, shaders/athenea/splat/splat_emit.slang:43-46): athx_frame_pairs
+ * says so.
  */
 ATHX_API athx_status athx_render_async(athx_ctx* ctx, const athx_view* views, uint32_t nviews,
                                        const athx_gaussians* gaussians, const athx_settings* settings,
@@ -301,7 +303,9 @@ ATHX_API athx_status athx_render_async(athx_ctx* ctx, const athx_view* views, ui
 
 /* The last frame's pair total and its pair capacity; waits for an async
  * frame. pairs > capacity: that frame was incomplete, draw it again with
- * a capacity of at least pairs. */
+ * a capacity of at least pairs (a frame whose strip entries outgrew their
+ * room also reports more pairs than its capacity; the next frame is given
+ * room for them). */
 ATHX_API athx_status athx_frame_pairs(athx_ctx* ctx, uint64_t* pairs, uint64_t* capacity);
 
 /*
@@ -354,13 +358,15 @@ ATHX_API athx_status athx_latest_counters(athx_ctx* ctx, athx_counters* out);
  *                16, colour channel > 0 = 32 << c, sRGB power branch = 256 << c),
  *                written for visible slots only
  *   TOUCHED      uint32[slots]: tiles the footprint touches, 0 when culled
+ *                (the frame bins its pairs without counting them a slot: the
+ *                counts are taken on the stream when this part is asked for)
  *   DEPTH_KEYS   uint32[slots]: asuint(z) >> 7, or 0xFF000000 | cull reason
  *   OFFSETS      uint32[slots]: a visible slot's first pair -- the exclusive
  *                scan of TOUCHED taken in depth order (visible slots stably
- *                sorted by depth key); undefined for culled slots
+ *                sorted by depth key); 0 for culled slots; taken on request
  *   PAIR_KEYS    uint64[pairs]: global tile << 24 | depth key, sorted (the
- *                frame stores each pair's tile; the key is rebuilt from it
- *                and the depth key of the pair's slot, on the stream)
+ *                frame keeps each tile's range and each pair's slot; the key
+ *                is rebuilt from them and the slot's depth key, on the stream)
  *   PAIR_VALUES  uint32[pairs]: the slot of each sorted pair
  *   RANGES       uint32[tiles][2]: a tile's [start, end) in the sorted pairs
  */

@@ -168,9 +168,10 @@ def test_rendering_twice_gives_the_same_bits(raster):
     assert torch.equal(first, b[0].colour)
 
 
-@pytest.mark.parametrize("n", [1, 7, 4095, 4096, 4097, 100003, 1 << 22])
+@pytest.mark.parametrize("n", [1, 7, 4095, 4096, 4097, 100003, 1 << 22, (1 << 24) + 4099])
 def test_scan(raster, n):
-    """test_algo.cpp:72-109's sizes."""
+    """test_algo.cpp:72-109's sizes, and one past 2^24 elements, where the scan
+    runs as partition sums, their scan and the partitions scanned from them."""
     say("scan", n)
     rng = np.random.default_rng(n)
     x = rng.integers(0, 1000, n, dtype=np.uint32)
@@ -218,3 +219,14 @@ def test_empty_cloud_draws_the_background(raster):
     assert stats.pairs == 0
     col = t[0].colour.cpu().numpy()
     assert np.allclose(col, [0.25, 0.5, 0.75, 1.0])
+
+
+def test_many_views_take_the_long_scans(raster):
+    """60 views of a 300k-Gaussian cloud: 18M (view, Gaussian) slots, past the
+    2^24 where the compaction and the strip-entry scan run as partition sums,
+    their scan, and the partitions scanned from them. Bit-exact with the
+    oracle, as every frame."""
+    say("60 views x 300k")
+    cloud = synthetic.walk_cloud(300000)
+    views = [camera.view_for(c, 64, 48) for c in synthetic.walk_cameras(60)]
+    check_frame(*render_both(raster, cloud, views, RenderSettings(), f64=False), f64=False)
