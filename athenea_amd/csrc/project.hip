@@ -31,10 +31,10 @@ __device__ inline float cutoff_power(float alpha) {
 // frame.slang:374-379
 __device__ inline bool tile_touched(uint32_t W, uint32_t H, uint32_t T, float cx, float cy, float a, float b,
                                     float c, float ia, float ic, float cutoff, uint32_t tx, uint32_t ty) {
-    const float fx = (float)(tx * T) + 0.5f;
-    const float fy = (float)(ty * T) + 0.5f;
-    const uint32_t ex = (tx + 1) * T < W ? (tx + 1) * T : W;
-    const uint32_t ey = (ty + 1) * T < H ? (ty + 1) * T : H;
+    const float fx = (float)__umul24(tx, T) + 0.5f;
+    const float fy = (float)__umul24(ty, T) + 0.5f;
+    const uint32_t ex = __umul24(tx + 1, T) < W ? __umul24(tx + 1, T) : W;
+    const uint32_t ey = __umul24(ty + 1, T) < H ? __umul24(ty + 1, T) : H;
     const float lx = (float)(ex - 1) + 0.5f;
     const float ly = (float)(ey - 1) + 0.5f;
     return min_power_over_box(a, b, c, ia, ic, fx - cx, fy - cy, lx - cx, ly - cy) <= cutoff;
@@ -483,8 +483,8 @@ struct BoxAxis {
 __device__ inline BoxAxis box_axis(uint32_t t, uint32_t T, uint32_t extent, float centre, float inv_across, float b,
                                    float along, float b2) {
     BoxAxis r;
-    const uint32_t e = (t + 1) * T < extent ? (t + 1) * T : extent;
-    r.lo = ((float)(t * T) + 0.5f) - centre;
+    const uint32_t e = __umul24(t + 1, T) < extent ? __umul24(t + 1, T) : extent;
+    r.lo = ((float)__umul24(t, T) + 0.5f) - centre;
     r.hi = ((float)(e - 1) + 0.5f) - centre;
     // The other coordinate's minimiser on each edge: ((-b) * edge) * (1 / across).
     r.q_lo = ((-b) * r.lo) * inv_across;
@@ -839,9 +839,9 @@ __device__ inline bool band_extent(float a, float b, float ia, float ic, float d
 // qualifies.
 __device__ inline uint32_t columns_meeting(float xa, float xb, float cx, uint32_t T, uint32_t W, uint32_t tx0,
                                           uint32_t c0, uint32_t c1) {
-    const auto lo_of = [&](int tx) { return ((float)((uint32_t)tx * T) + 0.5f) - cx; };
+    const auto lo_of = [&](int tx) { return ((float)__umul24((uint32_t)tx, T) + 0.5f) - cx; };
     const auto hi_of = [&](int tx) {
-        const uint32_t e = ((uint32_t)tx + 1u) * T < W ? ((uint32_t)tx + 1u) * T : W;
+        const uint32_t e = __umul24((uint32_t)tx + 1u, T) < W ? __umul24((uint32_t)tx + 1u, T) : W;
         return ((float)(e - 1u) + 0.5f) - cx;
     };
     const int lo = (int)c0, hi = (int)c1;
@@ -886,8 +886,8 @@ __device__ inline Decided decide_columns(const BigFootprint& g, const BoxAxis& Y
         return d;
     }
     const float det = (float)detd, inv_det = __builtin_amdgcn_rcpf(det);
-    const uint32_t ex = (c1 + 1u) * T < W ? (c1 + 1u) * T : W;
-    const float X = fmax_(fabsf(((float)(c0 * T) + 0.5f) - g.cx), fabsf(((float)(ex - 1u) + 0.5f) - g.cx));
+    const uint32_t ex = __umul24(c1 + 1u, T) < W ? __umul24(c1 + 1u, T) : W;
+    const float X = fmax_(fabsf(((float)__umul24(c0, T) + 0.5f) - g.cx), fabsf(((float)(ex - 1u) + 0.5f) - g.cx));
     const float Yb = fmax_(fabsf(Y.lo), fabsf(Y.hi));
     const float slack = kSlack * (g.a * X * X + 2.0f * fabsf(g.b) * X * Yb + g.c * Yb * Yb);
     const float sa = __builtin_amdgcn_sqrtf(g.a * inv_det), sc = __builtin_amdgcn_sqrtf(g.c * inv_det);
@@ -1027,7 +1027,11 @@ __global__ void __launch_bounds__(kEmitThreads) strip_mask_emit_kernel(const DVi
         const uint32_t q = gi - (j != 0 ? ends[j - 1] : 0u);
         const uint4 I = s_int[wbase + j], I2 = s_int2[wbase + j];
         const float4 geo = s_geo[wbase + j], geo2 = s_geo2[wbase + j];
-        const uint32_t row = q / I2.x, sxi = q - row * I2.x;
+        // q / ns by a float estimate made exact (q < 2^24, ns <= 4096).
+        uint32_t row = (uint32_t)((float)q * __builtin_amdgcn_rcpf((float)I2.x));
+        row = __umul24(row, I2.x) > q ? row - 1u : row;
+        row = __umul24(row + 1u, I2.x) <= q ? row + 1u : row;
+        const uint32_t sxi = q - __umul24(row, I2.x);
         const uint32_t tx0 = (I2.w + sxi) * kStripTiles;
         const uint32_t x0 = I.y & 0xFFFFu, x1 = I.y >> 16;
         const uint32_t c0 = x0 > tx0 ? x0 : tx0;
@@ -1054,7 +1058,7 @@ __global__ void __launch_bounds__(kEmitThreads) strip_mask_emit_kernel(const DVi
         }
         const uint64_t at = (uint64_t)base + gi;
         if (at < cap) {
-            keys_out[at] = (I.w + row * I2.y + sxi) | (mask << 16);
+            keys_out[at] = (I.w + __umul24(row, I2.y) + sxi) | (mask << 16);
             slots_out[at] = I.x;
         }
     }
