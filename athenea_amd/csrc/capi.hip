@@ -178,6 +178,9 @@ athx_status check_view(athx_ctx* ctx, const athx_view& v, uint32_t k) {
     if (v.width == 0 || v.height == 0) {
         return fail(ctx, ATHX_INVALID_ARGUMENT, "view " + std::to_string(k) + ": zero size");
     }
+    if (v.width > 65535u || v.height > 65535u) {
+        return fail(ctx, ATHX_INVALID_ARGUMENT, "view " + std::to_string(k) + ": width and height must be at most 65535");
+    }
     if (!(v.focal_x > 0.0f) || !(v.focal_y > 0.0f)) {
         return fail(ctx, ATHX_INVALID_ARGUMENT, "view " + std::to_string(k) + ": focal length must be positive");
     }

@@ -571,6 +571,20 @@ __device__ inline uint32_t wave_inclusive_sum(uint32_t v, uint32_t lane) {
     return v;
 }
 
+// Inclusive max over lanes 0 .. L of a wavefront, by DPP: within each row
+// of 16 lanes shifts right by 1, 2, 4 and 8 (lanes shifted in from outside
+// the row read 0), then lane 15 of rows 0 and 2 into rows 1 and 3, then
+// lane 31 into rows 2 and 3. Every lane must be active.
+__device__ inline uint32_t wave_inclusive_max(uint32_t v) {
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xF, 0xF, false));  // row_shr 1
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xF, 0xF, false));  // row_shr 2
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xF, 0xF, false));  // row_shr 4
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xF, 0xF, false));  // row_shr 8
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xA, 0xF, false));  // row_bcast 15
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xC, 0xF, false));  // row_bcast 31
+    return v;
+}
+
 __global__ void __launch_bounds__(kBigThreads) count_big_kernel(const DView* __restrict__ views, uint32_t n,
                                                                 uint32_t T, const float* __restrict__ records,
                                                                 Footprints fp, uint32_t* __restrict__ touched,
@@ -848,17 +862,21 @@ __device__ inline uint32_t columns_meeting(float xa, float xb, float cx, uint32_
     const float inv_t = 1.0f / (float)T;
     // First column whose right edge reaches xa (hi + 1 when none).
     int first = (int)ceilf(clamp_((xa + cx + 0.5f) * inv_t - 1.0f, (float)lo, (float)(hi + 1)));
+#pragma clang loop unroll(disable) interleave(disable) vectorize(disable)
     while (first > lo && hi_of(first - 1) >= xa) {
         --first;
     }
+#pragma clang loop unroll(disable) interleave(disable) vectorize(disable)
     while (first <= hi && hi_of(first) < xa) {
         ++first;
     }
     // Last column whose left edge is at most xb (lo - 1 when none).
     int last = (int)floorf(clamp_((xb + cx - 0.5f) * inv_t, (float)(lo - 1), (float)hi));
+#pragma clang loop unroll(disable) interleave(disable) vectorize(disable)
     while (last < hi && lo_of(last + 1) <= xb) {
         ++last;
     }
+#pragma clang loop unroll(disable) interleave(disable) vectorize(disable)
     while (last >= lo && lo_of(last) > xb) {
         --last;
     }
@@ -874,28 +892,47 @@ struct Decided {
     uint32_t sure, test;
 };
 
-__device__ inline Decided decide_columns(const BigFootprint& g, const BoxAxis& Y, uint32_t T, uint32_t W,
-                                         uint32_t tx0, uint32_t c0, uint32_t c1) {
+// A footprint's slice terms: its conic's determinant a c - b^2 (taken in
+// FP64, where a thin ellipse's does not cancel, and rounded once),
+// sqrt(a / det) and sqrt(c / det); det = 0 marks a conic the extents do
+// not handle (not positive definite, or a determinant outside the normal
+// float range), whose columns all go to the float test.
+struct SliceTerms {
+    float det, sa, sc;
+};
+
+__device__ inline SliceTerms slice_terms(float a, float b, float c) {
+    SliceTerms t{0.0f, 0.0f, 0.0f};
+    const double detd = (double)a * (double)c - (double)b * (double)b;
+    const float det = (float)detd;
+    if (a > 1e-9f && c > 1e-9f && det >= 0x1p-126f && det <= 0x1p126f) {
+        const float inv_det = __builtin_amdgcn_rcpf(det);
+        t.det = det;
+        t.sa = __builtin_amdgcn_sqrtf(a * inv_det);
+        t.sc = __builtin_amdgcn_sqrtf(c * inv_det);
+    }
+    return t;
+}
+
+__device__ inline Decided decide_columns(const BigFootprint& g, const SliceTerms& S, const BoxAxis& Y, uint32_t T,
+                                         uint32_t W, uint32_t tx0, uint32_t c0, uint32_t c1) {
     Decided d{0u, 0u};
     if (c1 < c0) {
         return d;
     }
-    const double detd = (double)g.a * (double)g.c - (double)g.b * (double)g.b;
-    if (!(g.a > 1e-9f && g.c > 1e-9f && detd > 0.0)) {
+    if (S.det == 0.0f) {
         d.test = ((1u << (c1 - c0 + 1u)) - 1u) << (c0 - tx0);
         return d;
     }
-    const float det = (float)detd, inv_det = __builtin_amdgcn_rcpf(det);
     const uint32_t ex = __umul24(c1 + 1u, T) < W ? __umul24(c1 + 1u, T) : W;
     const float X = fmax_(fabsf(((float)__umul24(c0, T) + 0.5f) - g.cx), fabsf(((float)(ex - 1u) + 0.5f) - g.cx));
     const float Yb = fmax_(fabsf(Y.lo), fabsf(Y.hi));
     const float slack = kSlack * (g.a * X * X + 2.0f * fabsf(g.b) * X * Yb + g.c * Yb * Yb);
-    const float sa = __builtin_amdgcn_sqrtf(g.a * inv_det), sc = __builtin_amdgcn_sqrtf(g.c * inv_det);
     float xa = 0.0f, xb = 0.0f;
-    d.sure = band_extent(g.a, g.b, g.ia, g.ic, det, sa, sc, g.cutoff - slack, Y.lo, Y.hi, xa, xb)
+    d.sure = band_extent(g.a, g.b, g.ia, g.ic, S.det, S.sa, S.sc, g.cutoff - slack, Y.lo, Y.hi, xa, xb)
                  ? columns_meeting(xa, xb, g.cx, T, W, tx0, c0, c1)
                  : 0u;
-    const uint32_t maybe = band_extent(g.a, g.b, g.ia, g.ic, det, sa, sc, g.cutoff + slack, Y.lo, Y.hi, xa, xb)
+    const uint32_t maybe = band_extent(g.a, g.b, g.ia, g.ic, S.det, S.sa, S.sc, g.cutoff + slack, Y.lo, Y.hi, xa, xb)
                                ? columns_meeting(xa, xb, g.cx, T, W, tx0, c0, c1)
                                : 0u;
     d.test = maybe & ~d.sure;
@@ -991,6 +1028,8 @@ __global__ void __launch_bounds__(kEmitThreads) strip_mask_emit_kernel(const DVi
     __shared__ float4 s_geo2[kEmitThreads];  // rank: conic c, cut power, 1 / a, 1 / c
     __shared__ uint4 s_int[kEmitThreads];    // rank: slot, first | last tile column << 16, first row, first strip
     __shared__ uint4 s_int2[kEmitThreads];   // rank: strips a row, strips between rows, view, first strip column
+    __shared__ float4 s_slice[kEmitThreads]; // rank: slice terms det, sa, sc; view width | height << 16
+    __shared__ uint32_t s_own[kEmitThreads]; // step position: step << 8 | (rank + 1) of the run starting there
     const uint32_t lane = threadIdx.x & 63u, wbase = threadIdx.x & ~63u;
     const uint32_t visible = *visible_dev < max_visible ? (uint32_t)*visible_dev : max_visible;
     const uint32_t k = blockIdx.x * kEmitThreads + threadIdx.x;
@@ -1011,22 +1050,41 @@ __global__ void __launch_bounds__(kEmitThreads) strip_mask_emit_kernel(const DVi
         s_geo2[threadIdx.x] = make_float4(conic.z, screen.w, recip_of(conic.x), recip_of(conic.z));
         s_int[threadIdx.x] = make_uint4(slot, x0 | ((x0 + w - 1u) << 16), y0, V.strip_base + y0 * V.strips_x + sx0);
         s_int2[threadIdx.x] = make_uint4(ns, V.strips_x, view, sx0);
+        const SliceTerms S = slice_terms(conic.x, conic.y, conic.z);
+        s_slice[threadIdx.x] = make_float4(S.det, S.sa, S.sc, __builtin_bit_cast(float, V.width | (V.height << 16)));
     }
     const uint32_t end = wave_inclusive_sum(count, lane);
     s_end[threadIdx.x] = end;
+    s_own[threadIdx.x] = ~0u;
     const uint32_t total = lane_u(end, 63);
     const uint32_t base = k0 < visible ? offsets[k0 / 64u] : 0u;
+    const uint32_t start = end - count;
     __syncthreads();
     const uint32_t* ends = s_end + wbase;
-    for (uint32_t g0 = 0; g0 < total; g0 += 64) {
+    // Each step's entries find their ranks without a search: the ranks whose
+    // runs start in the step mark their first position (tagged with the
+    // step), and a running max over the positions carries each mark on to
+    // the run's end; a step that starts inside a run continues the last
+    // step's final rank.
+    uint32_t carried = 0;
+    for (uint32_t g0 = 0, step = 0; g0 < total; g0 += 64, ++step) {
+        if (count != 0 && start >= g0 && start < g0 + 64u) {
+            s_own[wbase + (start - g0)] = (step << 8) | (lane + 1u);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        const uint32_t mark = s_own[threadIdx.x];
+        const uint32_t run = wave_inclusive_max(mark >> 8 == step ? mark & 0xFFu : 0u);
+        const uint32_t j = (run > carried + 1u ? run : carried + 1u) - 1u;
+        carried = lane_u(j, 63);
         const uint32_t gi = g0 + lane;
         if (gi >= total) {
             continue;
         }
-        const uint32_t j = run_of(ends, gi);
         const uint32_t q = gi - (j != 0 ? ends[j - 1] : 0u);
         const uint4 I = s_int[wbase + j], I2 = s_int2[wbase + j];
-        const float4 geo = s_geo[wbase + j], geo2 = s_geo2[wbase + j];
+        const float4 geo = s_geo[wbase + j], geo2 = s_geo2[wbase + j], sl = s_slice[wbase + j];
         // q / ns by a float estimate made exact (q < 2^24, ns <= 4096).
         uint32_t row = (uint32_t)((float)q * __builtin_amdgcn_rcpf((float)I2.x));
         row = __umul24(row, I2.x) > q ? row - 1u : row;
@@ -1045,14 +1103,14 @@ __global__ void __launch_bounds__(kEmitThreads) strip_mask_emit_kernel(const DVi
         g.cutoff = geo2.y;
         g.ia = geo2.z;
         g.ic = geo2.w;
-        const DView& V = views[I2.z];
+        const uint32_t wh = __builtin_bit_cast(uint32_t, sl.w), width = wh & 0xFFFFu, height = wh >> 16;
         const float b2 = 2.0f * g.b;
-        const BoxAxis Y = row_of(g, V, T, b2, I.z + row);
-        const Decided d = decide_columns(g, Y, T, V.width, tx0, c0, c1);
+        const BoxAxis Y = box_axis(I.z + row, T, height, g.cy, g.ia, g.b, g.c, b2);
+        const Decided d = decide_columns(g, SliceTerms{sl.x, sl.y, sl.z}, Y, T, width, tx0, c0, c1);
         uint32_t mask = d.sure;
         for (uint32_t t = d.test; t != 0; t &= t - 1u) {
             const uint32_t bit = (uint32_t)__builtin_ctz(t);
-            if (box_hit(g, b2, column_of(g, V, T, b2, tx0 + bit), Y)) {
+            if (box_hit(g, b2, box_axis(tx0 + bit, T, width, g.cx, g.ic, g.b, g.a, b2), Y)) {
                 mask |= 1u << bit;
             }
         }
@@ -1300,7 +1358,7 @@ __global__ void __launch_bounds__(kSegThreads) strip_count_kernel(StripBins b) {
             s_entry[threadIdx.x].row = make_float4(Y.lo, Y.hi, Y.q_lo, Y.q_hi);
             s_entry[threadIdx.x].more = make_float4(Y.s_lo, Y.s_hi, g.ic, g.cutoff);
             // Sure hits and sure misses from the slice's extent; the rest tested.
-            const Decided d = decide_columns(g, Y, T, V.width, tx0, c0, c1);
+            const Decided d = decide_columns(g, slice_terms(g.a, g.b, g.c), Y, T, V.width, tx0, c0, c1);
             sure = d.sure;
             test = d.test;
         }

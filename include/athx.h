@@ -94,7 +94,7 @@ typedef struct athx_ctx athx_ctx;
  * corner and pixel centres sit at +0.5 (frame.slang:22-30).
  */
 typedef struct athx_view {
-    uint32_t width, height;
+    uint32_t width, height;   /* 1 .. 65535 pixels each */
     float focal_x, focal_y;   /* pixels per unit of x/z (or per unit, orthographic) */
     float centre_x, centre_y; /* the principal point, pixels */
     float near_z, far_z;
