@@ -145,7 +145,8 @@ __device__ inline uint32_t xcd_band(uint32_t b, uint32_t n) {
 // scan over the slots (launch_big_list), so no kernel takes a place with an
 // atomic on one shared address -- those serialise in the memory system.
 struct Footprints {
-    uint2* rects;             // [slots] tile rect: x0 | y0 << 16, w | h << 16 (0, 0 when culled before)
+    uint2* rects;             // [slots] tile rect: x0 | y0 << 16, w | h << 16 (0, 0 when culled before),
+                              // recomputed from the records for the exact counts
     uint64_t* masks;          // [slots] a small footprint's hits, bit k = tile k of its rect, row-major
     uint32_t* big_list;       // [slots] slots whose rect holds over kBigRect tiles, in slot order
     uint32_t* big_mask_base;  // [slots] a big footprint's first hit word (its first step)
@@ -182,13 +183,13 @@ constexpr uint32_t kStripSeg = 1024;  // entries a segment: a workgroup of 256 l
 constexpr uint32_t kUnsettled = 1u << 31;
 
 // Launchers (each returns the hipError_t of its launches).
-// The projection: records, depth keys, tile rectangles and each slot's
+// The projection: records, depth keys and each slot's
 // strip entries (0: not drawn, with the cull in its depth key), then the
 // footprints whose centre tile is not touched settled by a wavefront each
 // walking their rectangle until a tile is (splat_project.slang:764-776's
 // zero count is the cull no-tile).
 hipError_t launch_project(const ProjectArgs& a, const DView* views_dev, uint32_t nviews, float* records,
-                          uint32_t* touched, uint32_t* depth_keys, uint2* rects, hipStream_t s);
+                          uint32_t* touched, uint32_t* depth_keys, hipStream_t s);
 // The exact tile count of every slot (splat_project.slang:764-776), for the
 // readers of the frame (copy_frame, the counters) -- not on the frame's
 // path: small rectangles spread over the lanes of a wavefront, big ones
@@ -200,7 +201,7 @@ hipError_t launch_exact_counts(const DView* views_dev, uint32_t nviews, uint32_t
 // rank), rank k's from offsets[k / 64] on (every 64th rank's first entry),
 // entries at or past cap not written; and each rank's footprint, 32 bytes
 // (fp[2k], fp[2k + 1]) the binning reads.
-hipError_t launch_strip_emit(const DView* views_dev, uint32_t n, const uint2* rects, const uint32_t* sorted_slots,
+hipError_t launch_strip_emit(const DView* views_dev, uint32_t n, uint32_t tile, const uint32_t* sorted_slots,
                              const uint32_t* offsets, const uint64_t* visible_dev, uint32_t max_visible, uint32_t cap,
                              const float* records, uint4* fp, uint32_t* strip_keys, uint32_t* strip_ranks,
                              hipStream_t s);
@@ -213,7 +214,6 @@ struct StripBins {
     uint32_t tile;
     uint32_t strips;
     const float* records;
-    const uint2* rects;
     uint32_t* strip_slots;         // [entries] strip-sorted entries' depth ranks, made slots by the count
                                    // (slots already when strip_keys is set)
     const uint32_t* strip_keys;    // [entries] sorted keys carrying the masks (strip_mask_emit), or null
@@ -232,10 +232,10 @@ struct StripBins {
 // The strip entries with their tiles decided (frames of at most 65 536
 // strips): key = strip | 16-bit mask of its touched tiles << 16, value = the
 // slot; the binning then reads only these.
-hipError_t launch_strip_mask_emit(const DView* views_dev, uint32_t n, uint32_t tile, const uint2* rects,
-                                  const uint32_t* sorted_slots, const uint32_t* offsets, const uint64_t* visible_dev,
-                                  uint32_t max_visible, uint32_t cap, const float* records, uint32_t* strip_keys,
-                                  uint32_t* strip_slots, hipStream_t s);
+hipError_t launch_strip_mask_emit(const DView* views_dev, uint32_t n, uint32_t tile, const uint32_t* sorted_slots,
+                                  const uint32_t* offsets, const uint64_t* visible_dev, uint32_t max_visible,
+                                  uint32_t cap, const float* records, uint32_t* strip_keys, uint32_t* strip_slots,
+                                  hipStream_t s);
 // Each strip's segments, scanned: segbase[s] and the total in *seg_total.
 hipError_t launch_seg_scan(const uint32_t* strip_ranges, uint32_t strips, uint32_t* segbase, uint64_t* seg_total,
                            uint32_t* status_words, hipStream_t s);

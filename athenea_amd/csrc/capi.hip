@@ -486,7 +486,6 @@ athx_status reserve_front(athx_ctx* ctx, const FrameShape& fs) {
     ATHX_HIP(ctx, reserve(ctx, ctx->depth_alt_slots, slots * 4 + 16), "depth sort slots");
     ATHX_HIP(ctx, reserve(ctx, ctx->depth_sort_meta, sort_meta_words(fs.slots, kDepthPasses) * 4 + 16),
              "depth sort metadata");
-    ATHX_HIP(ctx, reserve(ctx, ctx->fp_rects, slots * 8 + 16), "footprint rects");
     ATHX_HIP(ctx, reserve(ctx, ctx->strip_ranges, (size_t)fs.strips * 8 + 16), "strip ranges");
     ATHX_HIP(ctx, reserve(ctx, ctx->segbase, (size_t)fs.strips * 4 + 16), "strip segments");
     ATHX_HIP(ctx, reserve(ctx, ctx->rank_fp, slots * 32 + 32), "ranked footprints");
@@ -521,6 +520,7 @@ athx_status reserve_exact(athx_ctx* ctx, uint32_t slots) {
     ATHX_HIP(ctx, reserve(ctx, ctx->exact, (size_t)slots * 4 + 16), "exact counts");
     ATHX_HIP(ctx, reserve(ctx, ctx->exact_offsets, (size_t)slots * 4 + 16), "exact offsets");
     ATHX_HIP(ctx, reserve(ctx, ctx->exact_status, scan_status_words(slots) * 4 + 16), "exact scan status");
+    ATHX_HIP(ctx, reserve(ctx, ctx->fp_rects, (size_t)slots * 8 + 16), "footprint rects");
     ATHX_HIP(ctx, reserve(ctx, ctx->big_list, (size_t)slots * 4 + 16), "big footprints");
     ATHX_HIP(ctx, reserve(ctx, ctx->fp_masks, (size_t)slots * 8 + 16), "footprint masks");
     ATHX_HIP(ctx, reserve(ctx, ctx->fp_big_base, (size_t)slots * 4 + 16), "big footprint bases");
@@ -566,7 +566,6 @@ StripBins bins_of(athx_ctx* ctx, const FrameShape& fs, uint32_t n, uint32_t tile
     b.tile = tile;
     b.strips = fs.strips;
     b.records = as<float>(ctx->records);
-    b.rects = as<uint2>(ctx->fp_rects);
     b.strip_slots = as<uint32_t>(strips_in_alt ? ctx->strip_alt_slots : ctx->strip_slots);
     b.fp = as<uint4>(ctx->rank_fp);
     b.strip_keys = masks_in_keys(fs) ? as<uint32_t>(strips_in_alt ? ctx->strip_alt_keys : ctx->strip_keys) : nullptr;
@@ -594,8 +593,7 @@ athx_status enqueue_front(athx_ctx* ctx, const ProjectArgs& pa, const FrameShape
         ATHX_HIP(ctx, hipEventRecord(ctx->ev[0], s), "event");
     }
     ATHX_HIP(ctx, launch_project(pa, as<DView>(ctx->views), fs.nviews, as<float>(ctx->records),
-                                 as<uint32_t>(ctx->touched), as<uint32_t>(ctx->depth_keys), as<uint2>(ctx->fp_rects),
-                                 s),
+                                 as<uint32_t>(ctx->touched), as<uint32_t>(ctx->depth_keys), s),
              "project");
     if (timed) {
         ATHX_HIP(ctx, hipEventRecord(ctx->ev[1], s), "event");
@@ -635,13 +633,13 @@ athx_status enqueue_bins(athx_ctx* ctx, const FrameShape& fs, uint32_t n, uint32
     uint64_t* totals = as<uint64_t>(ctx->totals);
     const bool masked = masks_in_keys(fs);
     if (masked) {
-        ATHX_HIP(ctx, launch_strip_mask_emit(as<DView>(ctx->views), n, tile, as<uint2>(ctx->fp_rects),
+        ATHX_HIP(ctx, launch_strip_mask_emit(as<DView>(ctx->views), n, tile,
                                              sorted_slots_of(ctx, ctx->last_sorted_in_alt), as<uint32_t>(ctx->offsets),
                                              totals + kTotalVisible, fs.slots, fs.entry_cap, as<float>(ctx->records),
                                              as<uint32_t>(ctx->strip_keys), as<uint32_t>(ctx->strip_slots), s),
                  "strip entries");
     } else {
-        ATHX_HIP(ctx, launch_strip_emit(as<DView>(ctx->views), n, as<uint2>(ctx->fp_rects),
+        ATHX_HIP(ctx, launch_strip_emit(as<DView>(ctx->views), n, tile,
                                         sorted_slots_of(ctx, ctx->last_sorted_in_alt), as<uint32_t>(ctx->offsets),
                                         totals + kTotalVisible, fs.slots, fs.entry_cap, as<float>(ctx->records),
                                         as<uint4>(ctx->rank_fp), as<uint32_t>(ctx->strip_keys),

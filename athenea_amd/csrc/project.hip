@@ -137,10 +137,9 @@ __device__ inline void cull(uint32_t slot, uint32_t reason, uint32_t* touched, u
     keys[slot] = kCulledKey | reason;
 }
 
-// A slot culled before its tiles are counted: no rectangle to count over.
-__device__ inline void cull_early(uint32_t slot, uint32_t reason, uint32_t* touched, uint32_t* keys, uint2* rects) {
+// A slot culled before its tiles are counted.
+__device__ inline void cull_early(uint32_t slot, uint32_t reason, uint32_t* touched, uint32_t* keys) {
     cull(slot, reason, touched, keys);
-    rects[slot] = make_uint2(0u, 0u);
 }
 
 // packing.slang:59-76 (decodeQuaternion): three 10-bit components of the
@@ -176,7 +175,7 @@ constexpr uint32_t kInSoa32 = 0, kInSoa32Vec = 1, kInSoa16 = 2, kInSoa16Vec = 3,
 template <uint32_t kIn, bool kRelit>
 __global__ void __launch_bounds__(256) project_kernel(ProjectArgs a, const DView* __restrict__ views, uint32_t nviews,
                                                       float* __restrict__ records, uint32_t* __restrict__ touched,
-                                                      uint32_t* __restrict__ keys, uint2* __restrict__ rects) {
+                                                      uint32_t* __restrict__ keys) {
     constexpr bool kHalf = kIn == kInSoa16 || kIn == kInSoa16Vec;
     constexpr bool kVec = kIn == kInSoa32Vec || kIn == kInSoa16Vec;
     constexpr bool kPacked = kIn == kInPacked;
@@ -242,7 +241,7 @@ __global__ void __launch_bounds__(256) project_kernel(ProjectArgs a, const DView
         const float ty = m[4] * px + m[5] * py + m[6] * pz + m[7];
         const float tz = m[8] * px + m[9] * py + m[10] * pz + m[11];
         if (!(tz > V.near_z) || tz > V.far_z) {
-            cull_early(slot, ATHX_CULL_DEPTH, touched, keys, rects);
+            cull_early(slot, ATHX_CULL_DEPTH, touched, keys);
             continue;
         }
         float jx0, jx2, jy1, jy2, cx, cy;
@@ -294,7 +293,7 @@ __global__ void __launch_bounds__(256) project_kernel(ProjectArgs a, const DView
         cc = cc + 0.3f;
         const float det = ca * cc - cb * cb;
         if (!(det > 1e-12f)) {
-            cull_early(slot, ATHX_CULL_DEGENERATE, touched, keys, rects);
+            cull_early(slot, ATHX_CULL_DEGENERATE, touched, keys);
             continue;
         }
         const float filtered = sqrtf(fmax_(det_before, 0.0f) / det);
@@ -303,7 +302,7 @@ __global__ void __launch_bounds__(256) project_kernel(ProjectArgs a, const DView
         const float alpha = spread >= 1.0f / 255.0f ? spread + (1.0f - paid) * (1.0f / 255.0f) : spread;
         flags |= spread >= 1.0f / 255.0f ? kFlagPaidBack : 0u;
         if (!(alpha >= 1.0f / 255.0f)) {
-            cull_early(slot, ATHX_CULL_FAINT, touched, keys, rects);
+            cull_early(slot, ATHX_CULL_FAINT, touched, keys);
             continue;
         }
         const float bdet = ca * cc - cb * cb;
@@ -311,11 +310,10 @@ __global__ void __launch_bounds__(256) project_kernel(ProjectArgs a, const DView
         const float cutoff = cutoff_power(alpha);
         uint32_t x0, y0, x1, y1;
         if (!tile_rect(V.width, V.height, a.tile, cx, cy, ka, kb, kc, cutoff, x0, y0, x1, y1)) {
-            cull_early(slot, ATHX_CULL_OFFSCREEN, touched, keys, rects);
+            cull_early(slot, ATHX_CULL_OFFSCREEN, touched, keys);
             continue;
         }
-        const uint32_t w = x1 - x0 + 1, h = y1 - y0 + 1;
-        rects[slot] = make_uint2(x0 | (y0 << 16), w | (h << 16));
+        const uint32_t h = y1 - y0 + 1;
         // The reference counts the touched tiles here and culls a zero count
         // (splat_project.slang:764-776); what the frame needs is only whether
         // the count is zero, and the tile under the (clamped) centre nearly
@@ -662,6 +660,29 @@ __device__ inline uint32_t run_of(const uint32_t* ends, uint32_t g) {
     return lo;
 }
 
+// The exact counts' rectangles: each slot's tile rectangle, recomputed
+// from its record as the projection computed it (x0 | y0 << 16, w | h <<
+// 16), and (0, 0) for a slot culled before its tiles are counted.
+__global__ void __launch_bounds__(256) rect_kernel(const DView* __restrict__ views, uint32_t n, uint32_t slots,
+                                                   uint32_t T, const float* __restrict__ records,
+                                                   const uint32_t* __restrict__ keys, uint2* __restrict__ rects) {
+    const uint32_t slot = blockIdx.x * 256u + threadIdx.x;
+    if (slot >= slots) {
+        return;
+    }
+    const uint32_t key = keys[slot];
+    uint2 r = make_uint2(0u, 0u);
+    if (key < kCulledKey || (key & ~kCulledKey) == ATHX_CULL_NO_TILE) {
+        const DView& V = views[slot / n];
+        const float4* rec = reinterpret_cast<const float4*>(records + (size_t)slot * kRecordFloats);
+        const float4 screen = rec[0], conic = rec[1];
+        uint32_t x0, y0, x1, y1;
+        tile_rect(V.width, V.height, T, screen.x, screen.y, conic.x, conic.y, conic.z, screen.w, x0, y0, x1, y1);
+        r = make_uint2(x0 | (y0 << 16), (x1 - x0 + 1u) | ((y1 - y0 + 1u) << 16));
+    }
+    rects[slot] = r;
+}
+
 __global__ void __launch_bounds__(kSmallThreads) count_small_kernel(const DView* __restrict__ views, uint32_t n,
                                                                     uint32_t slots, uint32_t T,
                                                                     const float* __restrict__ records, Footprints fp,
@@ -953,7 +974,7 @@ __device__ inline Decided decide_columns(const BigFootprint& g, const SliceTerms
 constexpr uint32_t kEmitThreads = 256;
 
 __global__ void __launch_bounds__(kEmitThreads) strip_emit_kernel(const DView* __restrict__ views, uint32_t n,
-                                                                  const uint2* __restrict__ rects,
+                                                                  uint32_t T,
                                                                   const uint32_t* __restrict__ sorted_slots,
                                                                   const uint32_t* __restrict__ offsets,
                                                                   const uint64_t* __restrict__ visible_dev,
@@ -971,15 +992,16 @@ __global__ void __launch_bounds__(kEmitThreads) strip_emit_kernel(const DView* _
     uint32_t count = 0;
     if (k < visible) {
         const uint32_t slot = sorted_slots[k];
-        const uint2 r = rects[slot];
-        const uint32_t x0 = r.x & 0xFFFFu, y0 = r.x >> 16, w = r.y & 0xFFFFu, h = r.y >> 16;
         const DView& V = views[slot / n];
-        const uint32_t sx0 = x0 / kStripTiles;
-        const uint32_t cols = (x0 + w - 1u) / kStripTiles - sx0 + 1u;
-        count = h * cols;
-        s_rank[threadIdx.x] = make_uint4(k, V.strip_base + y0 * V.strips_x + sx0, cols, V.strips_x);
         const float4* rec = reinterpret_cast<const float4*>(records + (size_t)slot * kRecordFloats);
         const float4 screen = rec[0], conic = rec[1];
+        uint32_t x0, y0, x1, y1;
+        tile_rect(V.width, V.height, T, screen.x, screen.y, conic.x, conic.y, conic.z, screen.w, x0, y0, x1, y1);
+        const uint32_t w = x1 - x0 + 1u, h = y1 - y0 + 1u;
+        const uint32_t sx0 = x0 / kStripTiles;
+        const uint32_t cols = x1 / kStripTiles - sx0 + 1u;
+        count = h * cols;
+        s_rank[threadIdx.x] = make_uint4(k, V.strip_base + y0 * V.strips_x + sx0, cols, V.strips_x);
         fp_out[2u * k] = make_uint4(__builtin_bit_cast(uint32_t, screen.x), __builtin_bit_cast(uint32_t, screen.y),
                                     __builtin_bit_cast(uint32_t, conic.x), __builtin_bit_cast(uint32_t, conic.y));
         fp_out[2u * k + 1u] = make_uint4(__builtin_bit_cast(uint32_t, conic.z), __builtin_bit_cast(uint32_t, screen.w),
@@ -1015,7 +1037,7 @@ __global__ void __launch_bounds__(kEmitThreads) strip_emit_kernel(const DView* _
 // The strip sort orders the strip bits only, and the binning then reads the
 // keys and values in order and nothing else.
 __global__ void __launch_bounds__(kEmitThreads) strip_mask_emit_kernel(const DView* __restrict__ views, uint32_t n,
-                                                                       uint32_t T, const uint2* __restrict__ rects,
+                                                                       uint32_t T,
                                                                        const uint32_t* __restrict__ sorted_slots,
                                                                        const uint32_t* __restrict__ offsets,
                                                                        const uint64_t* __restrict__ visible_dev,
@@ -1037,15 +1059,16 @@ __global__ void __launch_bounds__(kEmitThreads) strip_mask_emit_kernel(const DVi
     uint32_t count = 0;
     if (k < visible) {
         const uint32_t slot = sorted_slots[k];
-        const uint2 r = rects[slot];
-        const uint32_t x0 = r.x & 0xFFFFu, y0 = r.x >> 16, w = r.y & 0xFFFFu, h = r.y >> 16;
         const uint32_t view = slot / n;
         const DView& V = views[view];
-        const uint32_t sx0 = x0 / kStripTiles;
-        const uint32_t ns = (x0 + w - 1u) / kStripTiles - sx0 + 1u;
-        count = h * ns;
         const float4* rec = reinterpret_cast<const float4*>(records + (size_t)slot * kRecordFloats);
         const float4 screen = rec[0], conic = rec[1];
+        uint32_t x0, y0, x1, y1;
+        tile_rect(V.width, V.height, T, screen.x, screen.y, conic.x, conic.y, conic.z, screen.w, x0, y0, x1, y1);
+        const uint32_t w = x1 - x0 + 1u, h = y1 - y0 + 1u;
+        const uint32_t sx0 = x0 / kStripTiles;
+        const uint32_t ns = x1 / kStripTiles - sx0 + 1u;
+        count = h * ns;
         s_geo[threadIdx.x] = make_float4(screen.x, screen.y, conic.x, conic.y);
         s_geo2[threadIdx.x] = make_float4(conic.z, screen.w, recip_of(conic.x), recip_of(conic.z));
         s_int[threadIdx.x] = make_uint4(slot, x0 | ((x0 + w - 1u) << 16), y0, V.strip_base + y0 * V.strips_x + sx0);
@@ -1592,7 +1615,7 @@ __global__ void __launch_bounds__(256) counters_kernel(const uint32_t* __restric
 }  // namespace
 
 hipError_t launch_project(const ProjectArgs& a, const DView* views_dev, uint32_t nviews, float* records,
-                          uint32_t* touched, uint32_t* depth_keys, uint2* rects, hipStream_t s) {
+                          uint32_t* touched, uint32_t* depth_keys, hipStream_t s) {
     if (a.n == 0 || nviews == 0) {
         return hipSuccess;
     }
@@ -1601,7 +1624,7 @@ hipError_t launch_project(const ProjectArgs& a, const DView* views_dev, uint32_t
     // a channel and the array is 16-byte aligned (torch's allocations are).
     const void* shp = a.sh_half ? static_cast<const void*>(a.sh_half) : static_cast<const void*>(a.sh);
     const bool vec = !a.packed && a.sh_coeffs == 16 && (reinterpret_cast<uintptr_t>(shp) & 15u) == 0;
-    using Kernel = void (*)(ProjectArgs, const DView*, uint32_t, float*, uint32_t*, uint32_t*, uint2*);
+    using Kernel = void (*)(ProjectArgs, const DView*, uint32_t, float*, uint32_t*, uint32_t*);
     const Kernel kernels[5][2] = {{project_kernel<kInSoa32, false>, project_kernel<kInSoa32, true>},
                                   {project_kernel<kInSoa32Vec, false>, project_kernel<kInSoa32Vec, true>},
                                   {project_kernel<kInSoa16, false>, project_kernel<kInSoa16, true>},
@@ -1609,7 +1632,7 @@ hipError_t launch_project(const ProjectArgs& a, const DView* views_dev, uint32_t
                                   {project_kernel<kInPacked, false>, project_kernel<kInPacked, true>}};
     const uint32_t in = a.packed ? kInPacked : (a.sh_half ? 2u : 0u) + (vec ? 1u : 0u);
     const Kernel kernel = kernels[in][a.relit ? 1 : 0];
-    hipLaunchKernelGGL(kernel, grid, dim3(256), 0, s, a, views_dev, nviews, records, touched, depth_keys, rects);
+    hipLaunchKernelGGL(kernel, grid, dim3(256), 0, s, a, views_dev, nviews, records, touched, depth_keys);
     const uint32_t slots = a.n * nviews;
     hipLaunchKernelGGL(settle_kernel, dim3(div_up(slots, 256)), dim3(256), 0, s, views_dev, a.n, a.tile, slots,
                        records, touched, depth_keys);
@@ -1626,6 +1649,8 @@ hipError_t launch_exact_counts(const DView* views_dev, uint32_t nviews, uint32_t
     if (hipError_t e = launch_fill_u32(counts, 0u, slots, s); e != hipSuccess) {
         return e;
     }
+    hipLaunchKernelGGL(rect_kernel, dim3(div_up(slots, 256)), dim3(256), 0, s, views_dev, n, slots, tile, records,
+                       depth_keys, fp.rects);
     hipLaunchKernelGGL(count_small_kernel, dim3(div_up(slots, kSmallThreads)), dim3(kSmallThreads), 0, s, views_dev,
                        n, slots, tile, records, fp, counts, depth_keys);
     if (hipError_t e = launch_big_list(fp, slots, status_words, s); e != hipSuccess) {
@@ -1671,7 +1696,7 @@ hipError_t launch_counters(const uint32_t* depth_keys, const uint32_t* touched, 
     return hipGetLastError();
 }
 
-hipError_t launch_strip_emit(const DView* views_dev, uint32_t n, const uint2* rects, const uint32_t* sorted_slots,
+hipError_t launch_strip_emit(const DView* views_dev, uint32_t n, uint32_t tile, const uint32_t* sorted_slots,
                              const uint32_t* offsets, const uint64_t* visible_dev, uint32_t max_visible, uint32_t cap,
                              const float* records, uint4* fp, uint32_t* strip_keys, uint32_t* strip_ranks,
                              hipStream_t s) {
@@ -1679,7 +1704,7 @@ hipError_t launch_strip_emit(const DView* views_dev, uint32_t n, const uint2* re
         return hipSuccess;
     }
     hipLaunchKernelGGL(strip_emit_kernel, dim3(div_up(max_visible, kEmitThreads)), dim3(kEmitThreads), 0, s,
-                       views_dev, n, rects, sorted_slots, offsets, visible_dev, max_visible, cap, records, fp,
+                       views_dev, n, tile, sorted_slots, offsets, visible_dev, max_visible, cap, records, fp,
                        strip_keys, strip_ranks);
     return hipGetLastError();
 }
@@ -1706,15 +1731,15 @@ hipError_t launch_strip_count(const StripBins& b, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_strip_mask_emit(const DView* views_dev, uint32_t n, uint32_t tile, const uint2* rects,
-                                  const uint32_t* sorted_slots, const uint32_t* offsets, const uint64_t* visible_dev,
-                                  uint32_t max_visible, uint32_t cap, const float* records, uint32_t* strip_keys,
-                                  uint32_t* strip_slots, hipStream_t s) {
+hipError_t launch_strip_mask_emit(const DView* views_dev, uint32_t n, uint32_t tile, const uint32_t* sorted_slots,
+                                  const uint32_t* offsets, const uint64_t* visible_dev, uint32_t max_visible,
+                                  uint32_t cap, const float* records, uint32_t* strip_keys, uint32_t* strip_slots,
+                                  hipStream_t s) {
     if (max_visible == 0 || cap == 0) {
         return hipSuccess;
     }
     hipLaunchKernelGGL(strip_mask_emit_kernel, dim3(div_up(max_visible, kEmitThreads)), dim3(kEmitThreads), 0, s,
-                       views_dev, n, tile, rects, sorted_slots, offsets, visible_dev, max_visible, cap, records,
+                       views_dev, n, tile, sorted_slots, offsets, visible_dev, max_visible, cap, records,
                        strip_keys, strip_slots);
     return hipGetLastError();
 }
