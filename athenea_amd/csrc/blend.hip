@@ -53,6 +53,14 @@ __device__ inline void stage(LdsBatch& s, uint32_t k, const float* records, uint
     s.colour[k] = make_float4(col.x, col.y, col.z, conic.y);
 }
 
+// e^(-power / 2) for a drawn pair: power <= the cut power <= 9, and above
+// a tiny negative rounding floor for a positive definite conic, so the
+// argument lies in [-4.5, 0+] where athx_expf_core equals athx_expf; the
+// median only bounds it for inputs no record holds.
+__device__ inline float blend_exp(float power) {
+    return athx_expf_core(__builtin_amdgcn_fmed3f(-0.5f * power, -80.0f, 80.0f));
+}
+
 template <uint32_t TILE>
 __global__ void __launch_bounds__(TILE * TILE) blend_kernel(const DView* __restrict__ views,
                                                             const uint32_t* __restrict__ tile_bases, uint32_t nviews,
@@ -93,7 +101,7 @@ __global__ void __launch_bounds__(TILE * TILE) blend_kernel(const DView* __restr
             if (power > q.z) {
                 continue;
             }
-            const float alpha = fminf(0.99f, q.w * athx_expf(-0.5f * power));
+            const float alpha = fminf(0.99f, q.w * blend_exp(power));
             if (alpha < 1.0f / 255.0f) {
                 continue;
             }
@@ -282,7 +290,7 @@ __global__ void __launch_bounds__(TILE * TILE) blend_backward_kernel(
                 const float dx = pxc - q.x, dy = pyc - q.y;
                 const float power = k.x * dx * dx + k.y * dx * dy + k.z * dy * dy;
                 if (power <= q.z) {
-                    const float e = athx_expf(-0.5f * power);
+                    const float e = blend_exp(power);
                     const float raw = q.w * e;
                     const float alpha = fminf(0.99f, raw);
                     if (alpha >= 1.0f / 255.0f) {

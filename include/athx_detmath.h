@@ -62,15 +62,8 @@ ATHX_DM_FN float athx_dm_float(uint32_t u) {
 #endif
 }
 
-/* e^x for x in [-87, 88]; 0 below, +inf above. */
-ATHX_DM_FN float athx_expf(float x) {
-    if (!(x > -87.0f)) {
-        return x == x ? 0.0f : x; /* NaN passes through */
-    }
-    if (x > 88.0f) {
-        return athx_dm_float(0x7F800000u);
-    }
-    const float k = rintf(x * 1.44269504088896341f);
+/* e^r for x = k ln2 + r: the reduction and the polynomial. */
+ATHX_DM_FN float athx_dm_exp_reduced(float x, float k) {
     /* ln2 = C1 + C2; C1 has 9 significant bits so k*C1 is exact for |k| <= 128. */
     float r = fmaf(k, -0.693359375f, x);
     r = fmaf(k, 2.12194440e-4f, r);
@@ -82,7 +75,19 @@ ATHX_DM_FN float athx_expf(float x) {
     p = fmaf(p, r, 5.0000001201e-1f);
     const float r2 = r * r;
     p = fmaf(p, r2, r);
-    p = p + 1.0f;
+    return p + 1.0f;
+}
+
+/* e^x for x in [-87, 88]; 0 below, +inf above. */
+ATHX_DM_FN float athx_expf(float x) {
+    if (!(x > -87.0f)) {
+        return x == x ? 0.0f : x; /* NaN passes through */
+    }
+    if (x > 88.0f) {
+        return athx_dm_float(0x7F800000u);
+    }
+    const float k = rintf(x * 1.44269504088896341f);
+    const float p = athx_dm_exp_reduced(x, k);
     /* 2^k, k in [-126, 127]: exact, and the product is exact unless it is subnormal. */
     const int ki = (int)k;
     if (ki < -125) {
@@ -90,6 +95,13 @@ ATHX_DM_FN float athx_expf(float x) {
         return (p * athx_dm_float((uint32_t)(ki + 127 + 2) << 23)) * 0.25f;
     }
     return p * athx_dm_float((uint32_t)(ki + 127) << 23);
+}
+
+/* athx_expf without its range checks, for callers whose x is in [-80, 80]:
+ * there 2^k is a normal number and the result is athx_expf's. */
+ATHX_DM_FN float athx_expf_core(float x) {
+    const float k = rintf(x * 1.44269504088896341f);
+    return athx_dm_exp_reduced(x, k) * athx_dm_float((uint32_t)((int)k + 127) << 23);
 }
 
 /* ln x for positive normal x; -inf at 0, NaN below 0, x at +inf. */
