@@ -42,6 +42,10 @@ struct DView {
     float m[12];
     float eye[3];
     float bg[4];
+    // The projection's Jacobian clamp, 1.3x the frustum in screen units:
+    // (-0.15 W - cx) / fx, (1.15 W - cx) / fx, then the same for y with H,
+    // cy, fy (the host's float operations are the kernels' own).
+    float lim[4];
     float* colour;
     float* depth;
     float* final_t;
@@ -183,11 +187,11 @@ constexpr uint32_t kStripSeg = 1024;  // entries a segment: a workgroup of 256 l
 constexpr uint32_t kUnsettled = 1u << 31;
 
 // Launchers (each returns the hipError_t of its launches).
-// The projection: records, depth keys and each slot's
-// strip entries (0: not drawn, with the cull in its depth key), then the
-// footprints whose centre tile is not touched settled by a wavefront each
-// walking their rectangle until a tile is (splat_project.slang:764-776's
-// zero count is the cull no-tile).
+// The projection: records, depth keys and each slot's strip entries (0:
+// not drawn, with the cull in its depth key); a footprint whose centre tile
+// is not touched is left unsettled (kUnsettled) for launch_settle_count, a
+// wavefront each walking its rectangle until a tile is
+// (splat_project.slang:764-776's zero count is the cull no-tile).
 hipError_t launch_project(const ProjectArgs& a, const DView* views_dev, uint32_t nviews, float* records,
                           uint32_t* touched, uint32_t* depth_keys, hipStream_t s);
 // The exact tile count of every slot (splat_project.slang:764-776), for the
@@ -306,11 +310,20 @@ hipError_t launch_sort(uint64_t* keys, uint32_t* values, const uint64_t* n_dev, 
                        uint32_t key_bits, const SortScratch& sc, hipStream_t s, bool* in_alt = nullptr);
 hipError_t launch_sort(uint32_t* keys, uint32_t* values, const uint64_t* n_dev, uint32_t cap, uint32_t lo_bit,
                        uint32_t key_bits, const SortScratch& sc, hipStream_t s, bool* in_alt = nullptr);
+// The compaction's block of slots: its first step counts each block's
+// drawn slots into a u64 a block at the head of its status words.
+constexpr uint32_t kCompactTile = 4096;
+// The unsettled footprints settled (launch_project leaves them), and each
+// block's drawn slots counted into sums (the compaction's status words).
+hipError_t launch_settle_count(const DView* views_dev, uint32_t n, uint32_t tile, uint32_t slots, const float* records,
+                               uint32_t* touched, uint32_t* depth_keys, uint64_t* sums, hipStream_t s);
 // The visible slots in slot order with their depth keys (each key's top byte
 // holding the slot's strip-entry count, capped at 255); the count lands in
-// *visible_total (device).
+// *visible_total (device). counted: the blocks' counts are already in the
+// status words (launch_settle_count).
 hipError_t launch_compact(const uint32_t* touched, const uint32_t* depth_keys, uint32_t slots, uint32_t* keys_out,
-                          uint32_t* slots_out, uint64_t* visible_total, uint32_t* status_words, hipStream_t s);
+                          uint32_t* slots_out, uint64_t* visible_total, uint32_t* status_words, hipStream_t s,
+                          bool counted = false);
 // Strip entries of the depth-sorted slots, scanned in that order:
 // offsets[k] is rank k's first entry; the total lands in *entry_total.
 hipError_t launch_entry_scan(const uint32_t* touched, const uint32_t* sorted_slots, const uint32_t* sorted_keys,

@@ -337,6 +337,13 @@ athx_status stage_views(athx_ctx* ctx, const athx_view* views, uint32_t nviews, 
         d.near_z = v.near_z;
         d.far_z = v.far_z;
         d.orthographic = v.orthographic ? 1u : 0u;
+        {
+            const float W = (float)v.width, H = (float)v.height;
+            d.lim[0] = (-0.15f * W - v.centre_x) / v.focal_x;
+            d.lim[1] = (1.15f * W - v.centre_x) / v.focal_x;
+            d.lim[2] = (-0.15f * H - v.centre_y) / v.focal_y;
+            d.lim[3] = (1.15f * H - v.centre_y) / v.focal_y;
+        }
         d.tile_base = (uint32_t)total_tiles;
         d.strips_x = div_up(d.tiles_x, kStripTiles);
         d.strip_base = (uint32_t)total_strips;
@@ -598,9 +605,13 @@ athx_status enqueue_front(athx_ctx* ctx, const ProjectArgs& pa, const FrameShape
     if (timed) {
         ATHX_HIP(ctx, hipEventRecord(ctx->ev[1], s), "event");
     }
+    ATHX_HIP(ctx, launch_settle_count(as<DView>(ctx->views), pa.n, pa.tile, fs.slots, as<float>(ctx->records),
+                                      as<uint32_t>(ctx->touched), as<uint32_t>(ctx->depth_keys),
+                                      reinterpret_cast<uint64_t*>(ctx->scan_status.ptr), s),
+             "settle");
     ATHX_HIP(ctx, launch_compact(as<uint32_t>(ctx->touched), as<uint32_t>(ctx->depth_keys), fs.slots,
                                  as<uint32_t>(ctx->depth_keys_sorted), as<uint32_t>(ctx->depth_slots),
-                                 totals + kTotalVisible, as<uint32_t>(ctx->scan_status), s),
+                                 totals + kTotalVisible, as<uint32_t>(ctx->scan_status), s, true),
              "compact");
     const SortScratch dsc = sort_scratch(ctx->depth_sort_meta, ctx->depth_alt_keys.ptr,
                                          as<uint32_t>(ctx->depth_alt_slots), fs.slots, kDepthPasses);

@@ -256,11 +256,7 @@ __global__ void __launch_bounds__(256) project_kernel(ProjectArgs a, const DView
             cx = V.focal_x * tx * invz + V.centre_x;
             cy = V.focal_y * ty * invz + V.centre_y;
             // The Jacobian at a point clamped to 1.3x the frustum (splat_project.slang:543-557).
-            const float W = (float)V.width, H = (float)V.height;
-            const float limlox = (-0.15f * W - V.centre_x) / V.focal_x;
-            const float limhix = (1.15f * W - V.centre_x) / V.focal_x;
-            const float limloy = (-0.15f * H - V.centre_y) / V.focal_y;
-            const float limhiy = (1.15f * H - V.centre_y) / V.focal_y;
+            const float limlox = V.lim[0], limhix = V.lim[1], limloy = V.lim[2], limhiy = V.lim[3];
             const float ux = tx * invz, uy = ty * invz;
             flags |= (ux < limlox ? kFlagClampXLo : 0u) | (ux > limhix ? kFlagClampXHi : 0u) |
                      (uy < limloy ? kFlagClampYLo : 0u) | (uy > limhiy ? kFlagClampYHi : 0u);
@@ -788,35 +784,68 @@ __global__ void __launch_bounds__(256) ranges_kernel(const uint32_t* __restrict_
 // not touched): a wavefront takes each in turn and tests its rectangle's
 // tiles 64 a step, row-major, until one is touched. None touched is the
 // reference's zero count: culled, no tile (splat_project.slang:764-776).
-__global__ void __launch_bounds__(256) settle_kernel(const DView* __restrict__ views, uint32_t n, uint32_t T,
-                                                     uint32_t slots, const float* __restrict__ records,
-                                                     uint32_t* __restrict__ touched, uint32_t* __restrict__ keys) {
+// The same pass counts the drawn slots of each block of kCompactTile slots
+// into sums[block], the compaction's first step.
+// A workgroup a block of slots, 1024 lanes, so that a small frame still
+// spreads its unsettled footprints over many wavefronts.
+constexpr uint32_t kSettleThreads = 1024;
+
+__global__ void __launch_bounds__(kSettleThreads) settle_count_kernel(const DView* __restrict__ views, uint32_t n, uint32_t T,
+                                                           uint32_t slots, const float* __restrict__ records,
+                                                           uint32_t* __restrict__ touched, uint32_t* __restrict__ keys,
+                                                           uint64_t* __restrict__ sums) {
+    __shared__ uint32_t s_wave[kSettleThreads / 64];
     const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t word = i < slots ? touched[i] : 0u;
-    uint64_t todo = __ballot((word & kUnsettled) != 0u);
-    const uint32_t first = i - lane;
-    while (todo != 0) {
-        const uint32_t j = (uint32_t)__builtin_ctzll(todo);
-        todo &= todo - 1;
-        const uint32_t slot = first + j;
-        const BigFootprint g = big_footprint(views, n, T, records, slot);
-        const DView& V = views[g.view];
-        const uint32_t area = g.w * g.h;
-        bool any = false;
-        for (uint32_t k0 = 0; k0 < area && !any; k0 += 64) {
-            const uint32_t k = k0 + lane;
-            const uint32_t r = k / g.w;
-            const bool hit = k < area && tile_touched(V.width, V.height, T, g.cx, g.cy, g.a, g.b, g.c, g.ia, g.ic,
-                                                      g.cutoff, g.x0 + (k - r * g.w), g.y0 + r);
-            any = __ballot(hit) != 0;
-        }
-        if (lane == j) {
-            touched[slot] = any ? (word & ~kUnsettled) : 0u;
-            if (!any) {
-                keys[slot] = kCulledKey | ATHX_CULL_NO_TILE;
+    const uint64_t first_block = (uint64_t)blockIdx.x * kCompactTile;
+    constexpr uint32_t kPer = kCompactTile / kSettleThreads;
+    uint32_t words[kPer];
+#pragma unroll
+    for (uint32_t item = 0; item < kPer; ++item) {
+        const uint64_t i = first_block + (uint64_t)item * kSettleThreads + threadIdx.x;
+        words[item] = i < slots ? touched[i] : 0u;
+    }
+    uint32_t count = 0;
+#pragma unroll
+    for (uint32_t item = 0; item < kPer; ++item) {
+        const uint64_t i = first_block + (uint64_t)item * kSettleThreads + threadIdx.x;
+        uint32_t word = words[item];
+        uint64_t todo = __ballot((word & kUnsettled) != 0u);
+        const uint32_t first = (uint32_t)(i - lane);
+        while (todo != 0) {
+            const uint32_t j = (uint32_t)__builtin_ctzll(todo);
+            todo &= todo - 1;
+            const uint32_t slot = first + j;
+            const BigFootprint g = big_footprint(views, n, T, records, slot);
+            const DView& V = views[g.view];
+            const uint32_t area = g.w * g.h;
+            bool any = false;
+            for (uint32_t k0 = 0; k0 < area && !any; k0 += 64) {
+                const uint32_t k = k0 + lane;
+                const uint32_t r = k / g.w;
+                const bool hit = k < area && tile_touched(V.width, V.height, T, g.cx, g.cy, g.a, g.b, g.c, g.ia,
+                                                          g.ic, g.cutoff, g.x0 + (k - r * g.w), g.y0 + r);
+                any = __ballot(hit) != 0;
+            }
+            if (lane == j) {
+                word = any ? (word & ~kUnsettled) : 0u;
+                touched[slot] = word;
+                if (!any) {
+                    keys[slot] = kCulledKey | ATHX_CULL_NO_TILE;
+                }
             }
         }
+        count += (uint32_t)__popcll(__ballot(word != 0u));
+    }
+    if (lane == 0) {
+        s_wave[threadIdx.x >> 6] = count;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t t = 0;
+        for (uint32_t w = 0; w < kSettleThreads / 64; ++w) {
+            t += s_wave[w];
+        }
+        sums[blockIdx.x] = t;
     }
 }
 
@@ -1633,9 +1662,16 @@ hipError_t launch_project(const ProjectArgs& a, const DView* views_dev, uint32_t
     const uint32_t in = a.packed ? kInPacked : (a.sh_half ? 2u : 0u) + (vec ? 1u : 0u);
     const Kernel kernel = kernels[in][a.relit ? 1 : 0];
     hipLaunchKernelGGL(kernel, grid, dim3(256), 0, s, a, views_dev, nviews, records, touched, depth_keys);
-    const uint32_t slots = a.n * nviews;
-    hipLaunchKernelGGL(settle_kernel, dim3(div_up(slots, 256)), dim3(256), 0, s, views_dev, a.n, a.tile, slots,
-                       records, touched, depth_keys);
+    return hipGetLastError();
+}
+
+hipError_t launch_settle_count(const DView* views_dev, uint32_t n, uint32_t tile, uint32_t slots, const float* records,
+                               uint32_t* touched, uint32_t* depth_keys, uint64_t* sums, hipStream_t s) {
+    if (slots == 0) {
+        return hipSuccess;
+    }
+    hipLaunchKernelGGL(settle_count_kernel, dim3(div_up(slots, kCompactTile)), dim3(kSettleThreads), 0, s,
+                       views_dev, n, tile, slots, records, touched, depth_keys, sums);
     return hipGetLastError();
 }
 

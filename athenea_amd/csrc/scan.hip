@@ -164,6 +164,28 @@ struct TableScan {
     __device__ void store(uint32_t i, uint32_t, uint64_t excl) const { table[i] = (uint32_t)excl; }
 };
 
+// A partition's elements, kItems consecutive ones a thread (the scan's
+// order), read with coalesced loads: row k of kThreads consecutive elements
+// a load, through LDS (rows of kItems + 1 words, so neither side's accesses
+// collide in a bank).
+static_assert(kItems == 16, "the transpose indexes rows of 16");
+template <typename Policy>
+__device__ inline void load_partition(const Policy& pol, uint64_t part_first, uint32_t n,
+                                      typename Policy::Value (&v)[kItems], typename Policy::Value* s_x) {
+    const uint32_t tid = threadIdx.x;
+#pragma unroll
+    for (uint32_t k = 0; k < kItems; ++k) {
+        const uint32_t r = k * kThreads + tid;
+        const uint64_t e = part_first + r;
+        s_x[(r >> 4) * (kItems + 1) + (r & 15u)] = e < n ? pol.load((uint32_t)e) : 0u;
+    }
+    __syncthreads();
+#pragma unroll
+    for (uint32_t k = 0; k < kItems; ++k) {
+        v[k] = s_x[tid * (kItems + 1) + k];
+    }
+}
+
 // status[0] is the ticket counter; status[1 + p] partition p's word. The grid
 // may hold more partitions than the scan's length needs (a length read on the
 // device): tickets past it leave at once, and nobody waits on them.
@@ -188,6 +210,8 @@ __global__ void __launch_bounds__(kThreads) scan_kernel(Policy pol, uint64_t* __
     }
     const uint64_t first = (uint64_t)part * kTile + (uint64_t)tid * kItems;
 
+    // Each lane's run read directly: at the sizes this kernel takes, a
+    // partition's lines stay in cache between a lane's loads.
     typename Policy::Value v[kItems];
 #pragma unroll
     for (uint32_t k = 0; k < kItems; ++k) {
@@ -293,11 +317,13 @@ __global__ void __launch_bounds__(kThreads) partition_sum_kernel(Policy pol, uin
     if ((uint64_t)part * kTile >= n && part != 0) {
         return;
     }
-    const uint64_t first = (uint64_t)part * kTile + (uint64_t)threadIdx.x * kItems;
+    // The sum in any order: rows of kThreads consecutive elements a load.
+    const uint64_t first = (uint64_t)part * kTile + threadIdx.x;
     uint64_t mine = 0;
 #pragma unroll
     for (uint32_t k = 0; k < kItems; ++k) {
-        mine += first + k < n ? (uint64_t)pol.load((uint32_t)(first + k)) : 0u;
+        const uint64_t e = first + (uint64_t)k * kThreads;
+        mine += e < n ? (uint64_t)pol.load((uint32_t)e) : 0u;
     }
     mine = wave_sum(mine);
     if ((threadIdx.x & 63u) == 0) {
@@ -353,6 +379,7 @@ __global__ void __launch_bounds__(kSumThreads) partition_offsets_kernel(Policy p
 template <typename Policy>
 __global__ void __launch_bounds__(kThreads) partition_scan_kernel(Policy pol, const uint64_t* __restrict__ offsets) {
     __shared__ uint64_t s_wave[kThreads / 64];
+    __shared__ typename Policy::Value s_x[kThreads * (kItems + 1)];
     const uint32_t n = pol.size();
     const uint32_t part = blockIdx.x;
     if ((uint64_t)part * kTile >= n) {
@@ -361,10 +388,7 @@ __global__ void __launch_bounds__(kThreads) partition_scan_kernel(Policy pol, co
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     const uint64_t first = (uint64_t)part * kTile + (uint64_t)threadIdx.x * kItems;
     typename Policy::Value v[kItems];
-#pragma unroll
-    for (uint32_t k = 0; k < kItems; ++k) {
-        v[k] = first + k < n ? pol.load((uint32_t)(first + k)) : 0u;
-    }
+    load_partition(pol, (uint64_t)part * kTile, n, v, s_x);
     uint64_t mine = 0;
 #pragma unroll
     for (uint32_t k = 0; k < kItems; ++k) {
@@ -523,10 +547,14 @@ hipError_t launch_scan(const uint32_t* in, uint32_t* out, uint64_t* total, uint3
 }
 
 hipError_t launch_compact(const uint32_t* touched, const uint32_t* depth_keys, uint32_t slots, uint32_t* keys_out,
-                          uint32_t* slots_out, uint64_t* visible_total, uint32_t* status_words, hipStream_t s) {
+                          uint32_t* slots_out, uint64_t* visible_total, uint32_t* status_words, hipStream_t s,
+                          bool counted) {
+    static_assert(kTile == kCompactTile, "the settle pass counts the compaction's blocks");
     uint64_t* sums = reinterpret_cast<uint64_t*>(status_words);
     const uint32_t parts = div_up(slots ? slots : 1, kTile);
-    hipLaunchKernelGGL(compact_count_kernel, dim3(parts), dim3(kThreads), 0, s, touched, slots, sums);
+    if (!counted || slots == 0) {
+        hipLaunchKernelGGL(compact_count_kernel, dim3(parts), dim3(kThreads), 0, s, touched, slots, sums);
+    }
     hipLaunchKernelGGL(compact_offsets_kernel, dim3(1), dim3(kSumThreads), 0, s, sums, parts, visible_total);
     hipLaunchKernelGGL(compact_write_kernel, dim3(parts), dim3(kThreads), 0, s, touched, depth_keys, slots, sums,
                        keys_out, slots_out);
