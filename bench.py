@@ -49,19 +49,21 @@ def stage_bytes(n: int, visible: int, pairs: int, entries: int, strips: int, til
     segments = entries // 1024 + strips
     return {
         # FP32 parameters read (236 B at SH degree 3, + 32 B of relight
-        # attributes: albedo, normal, metallic, roughness); record, depth key,
-        # strip-entry count and tile rectangle written
-        "project": n * (236 + (32 if relit else 0) + 48 + 4 + 4 + 8),
-        # compaction (counts and keys read, visible (key, slot) written), then
+        # attributes: albedo, normal, metallic, roughness); record, depth key
+        # and strip-entry count written
+        "project": n * (236 + (32 if relit else 0) + 48 + 4 + 4),
+        # settling and compaction (counts and keys read, visible (key, slot)
+        # written), then
         # a stable 24-bit sort of the visible u32 keys: a histogram read and a
         # read and write of (key, slot) a pass
         "depth_sort": n * 8 + visible * 8 + visible * 4 + depth_passes * visible * 16,
         # sorted slots read, entry counts gathered
         "counts": visible * 8,
-        # sorted slot, rectangle, the record's centre, conic and cut power
-        # read a rank; (strip | tile mask, slot) written an entry; the strip
-        # sort: a histogram read, then a read and a write of (key, slot) a pass
-        "emit": visible * (4 + 8 + 32) + entries * 8 + entries * 4 + strip_passes * entries * 16,
+        # sorted slot and the record's centre, conic and cut power read a rank
+        # (the tile rectangle is recomputed from them); (strip | tile mask,
+        # slot) written an entry; the strip sort: a histogram read, then a
+        # read and a write of (key, slot) a pass
+        "emit": visible * (4 + 32) + entries * 8 + entries * 4 + strip_passes * entries * 16,
         # an entry's key read and its mask kept (count); the table written,
         # scanned (read and written) and read; the mask and slot read again
         # and each pair's slot written at its place
@@ -75,8 +77,9 @@ def stage_bytes(n: int, visible: int, pairs: int, entries: int, strips: int, til
 
 # The kernels each timed stage launches, in order.
 STAGE_KERNELS = {
-    "project": "project_kernel, settle_kernel",
-    "depth_sort": "the visible compaction scan, histogram_kernel, hist_reduce_kernel, onesweep_kernel x3",
+    "project": "project_kernel",
+    "depth_sort": "settle_count_kernel (unsettled footprints settled, each block's drawn slots counted), the "
+                  "compaction, histogram_kernel, hist_reduce_kernel, onesweep_kernel x3",
     "counts": "the strip-entry scan in depth order",
     "emit": "strip_mask_emit_kernel (entries with their tiles decided), the strip sort (onesweep or "
             "reduce-scan-scatter passes), ranges_kernel over the strips, the segment scan, seg_desc_kernel",
