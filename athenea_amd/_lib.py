@@ -176,6 +176,40 @@ FRAME_PARTS = {"records": 0, "touched": 1, "depth_keys": 2, "offsets": 3, "pair_
 
 
 # Every symbol include/athx.h declares, with its signature.
+PLY_MAX_PROPERTIES = 256
+# athx_ply_type: the property types a vertex may hold.
+PLY_TYPES = {"f32": 0, "f64": 1, "u8": 2, "i8": 3, "u16": 4, "i16": 5, "u32": 6, "i32": 7}
+
+
+class PlyLayout(ctypes.Structure):
+    _fields_ = [
+        ("data_offset", ctypes.c_uint64),
+        ("count", ctypes.c_uint64),
+        ("stride", ctypes.c_uint32),
+        ("properties", ctypes.c_uint32),
+        ("offset", ctypes.c_uint32 * PLY_MAX_PROPERTIES),
+        ("type", ctypes.c_uint32 * PLY_MAX_PROPERTIES),
+        ("x", ctypes.c_uint32),
+        ("y", ctypes.c_uint32),
+        ("z", ctypes.c_uint32),
+        ("opacity", ctypes.c_uint32),
+        ("scale", ctypes.c_uint32 * 3),
+        ("rot", ctypes.c_uint32 * 4),
+        ("dc", ctypes.c_uint32 * 3),
+        ("rest_base", ctypes.c_uint32),
+        ("rest_per_colour", ctypes.c_uint32),
+    ]
+
+
+class IngestOut(ctypes.Structure):
+    _fields_ = [
+        ("positions", ctypes.c_void_p),
+        ("shape", ctypes.c_void_p),
+        ("sh", ctypes.c_void_p),
+        ("origin", ctypes.c_void_p),
+    ]
+
+
 SIGNATURES = {
     "athx_version": (ctypes.c_char_p, []),
     "athx_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]),
@@ -258,6 +292,15 @@ SIGNATURES = {
     "athx_sort_pairs": (
         ctypes.c_int,
         [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p],
+    ),
+    "athx_ply_parse": (
+        ctypes.c_int,
+        [ctypes.c_char_p, ctypes.c_uint64, ctypes.POINTER(PlyLayout), ctypes.c_char_p, ctypes.c_uint64],
+    ),
+    "athx_ingest_ply": (
+        ctypes.c_int,
+        [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(PlyLayout), ctypes.c_uint32, ctypes.c_uint32,
+         ctypes.POINTER(IngestOut), ctypes.POINTER(ctypes.c_uint32)],
     ),
 }
 

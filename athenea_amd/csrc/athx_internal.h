@@ -339,4 +339,24 @@ hipError_t launch_gather_scan(const uint32_t* touched, const uint32_t* sorted_sl
 // centre x, y, conic a, b, c, alpha, colour r, g, b.
 constexpr uint32_t kGrad2d = 9;
 
+// A PLY vertex's fields the ingest reads, each as its byte offset in the
+// vertex | its athx_ply_type << 24: x y z, opacity, scale 0..2, rot 0..3
+// (w x y z), f_dc 0..2, then from kIngestBase the kept rest coefficients
+// in the order they are packed (rgb a basis).
+constexpr uint32_t kIngestBase = 14;
+constexpr uint32_t kIngestCodes = kIngestBase + 45;
+struct IngestFields {
+    uint32_t count;       // vertices
+    uint32_t stride;      // bytes a vertex
+    uint32_t rest;        // rest coefficients kept a splat (3 a basis)
+    uint32_t sh_words;    // words a splat in the output's sh
+    uint32_t keep_faint;  // keep every vertex with a finite position
+    uint32_t code[kIngestCodes];
+};
+// Each vertex's keep flag (1 or 0).
+hipError_t launch_ingest_flags(const uint8_t* vertices, const IngestFields& f, uint32_t* keep, hipStream_t s);
+// The kept vertices decoded and packed at dest[i] (the flags' exclusive scan).
+hipError_t launch_ingest_pack(const uint8_t* vertices, const IngestFields& f, const uint32_t* keep,
+                              const uint32_t* dest, const athx_ingest_out& out, hipStream_t s);
+
 }  // namespace athx

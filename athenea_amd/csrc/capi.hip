@@ -18,6 +18,8 @@
 #include <cstring>
 #include <exception>
 #include <new>
+#include <sstream>
+#include <string_view>
 #include <string>
 #include <vector>
 
@@ -74,6 +76,8 @@ struct athx_ctx {
     // For the frame's readers: exact tile counts and their scan in depth order.
     DeviceBuffer exact, exact_offsets, exact_status;
     DeviceBuffer api_sort_meta, api_alt_keys, api_alt_vals, api_scan_status, api_count;
+    // The PLY ingest: each vertex's keep flag, their scan, the kept total.
+    DeviceBuffer ingest_keep, ingest_dest, ingest_total;
     uint64_t* host_totals = nullptr;  // pinned: the totals words (kTotal*)
     DView* host_views = nullptr;      // pinned staging for the views
     uint32_t* host_tile_bases = nullptr;  // [nviews + 1] tile bases, then [nviews + 1] strip bases
@@ -110,7 +114,7 @@ std::vector<DeviceBuffer*> all_buffers(athx_ctx* ctx) {
             &ctx->strip_keys, &ctx->strip_slots, &ctx->strip_alt_keys, &ctx->strip_alt_slots, &ctx->strip_sort_meta,
             &ctx->strip_ranges, &ctx->segbase, &ctx->seg_map, &ctx->rank_fp, &ctx->masks, &ctx->seg_table, &ctx->exact, &ctx->exact_offsets,
             &ctx->exact_status, &ctx->api_sort_meta, &ctx->api_alt_keys, &ctx->api_alt_vals, &ctx->api_scan_status,
-            &ctx->api_count};
+            &ctx->api_count, &ctx->ingest_keep, &ctx->ingest_dest, &ctx->ingest_total};
 }
 
 athx_status fail(athx_ctx* ctx, athx_status code, const std::string& msg) {
@@ -1291,6 +1295,337 @@ ATHX_API athx_status athx_sort_pairs(athx_ctx* ctx, uint64_t* keys, uint32_t* va
     ATHX_HIP(ctx, launch_fill_u32(as<uint32_t>(ctx->api_count), n, 1, s), "sort count");
     ATHX_HIP(ctx, launch_fill_u32(as<uint32_t>(ctx->api_count) + 1, 0u, 1, s), "sort count");
     ATHX_HIP(ctx, launch_sort(keys, values, as<uint64_t>(ctx->api_count), n, 0, key_bits, sc, s), "sort");
+    return ATHX_OK;
+}
+
+namespace {
+
+// io::parsePlyHeader's tokens (modules/io/src/PlyHeader.cpp:10-21): a type's
+// bytes and its athx_ply_type (-1: not one the ingest reads).
+int ply_type_of(const std::string& t, uint32_t& bytes) {
+    struct Entry {
+        const char* name;
+        uint32_t bytes;
+        int type;
+    };
+    static const Entry kTypes[] = {
+        {"float", 4, ATHX_PLY_F32},  {"float32", 4, ATHX_PLY_F32}, {"double", 8, ATHX_PLY_F64},
+        {"float64", 8, ATHX_PLY_F64}, {"uchar", 1, ATHX_PLY_U8},   {"uint8", 1, ATHX_PLY_U8},
+        {"char", 1, ATHX_PLY_I8},    {"int8", 1, ATHX_PLY_I8},     {"ushort", 2, ATHX_PLY_U16},
+        {"uint16", 2, ATHX_PLY_U16}, {"short", 2, ATHX_PLY_I16},   {"int16", 2, ATHX_PLY_I16},
+        {"uint", 4, ATHX_PLY_U32},   {"uint32", 4, ATHX_PLY_U32},  {"int", 4, ATHX_PLY_I32},
+        {"int32", 4, ATHX_PLY_I32}};
+    for (const Entry& e : kTypes) {
+        if (t == e.name) {
+            bytes = e.bytes;
+            return e.type;
+        }
+    }
+    bytes = 4;  // the reference's size of an unknown type
+    return -1;
+}
+
+struct PlyProperty {
+    std::string name, type;
+    uint32_t offset = 0, bytes = 0;
+    int code = -1;
+};
+
+struct PlyElement {
+    std::string name;
+    uint64_t count = 0;
+    uint64_t stride = 0;
+    bool has_list = false;
+    std::vector<PlyProperty> props;
+};
+
+athx_status ply_fail(athx_status code, const std::string& msg, char* message, uint64_t message_bytes) {
+    if (message && message_bytes > 0) {
+        const size_t n = std::min<size_t>(msg.size(), message_bytes - 1);
+        std::memcpy(message, msg.data(), n);
+        message[n] = '\0';
+    }
+    return code;
+}
+
+}  // namespace
+
+ATHX_API athx_status athx_ply_parse(const char* header, uint64_t bytes, athx_ply_layout* out, char* message,
+                                    uint64_t message_bytes) {
+    if (message && message_bytes > 0) {
+        message[0] = '\0';
+    }
+    if (!header || !out) {
+        return ply_fail(ATHX_INVALID_ARGUMENT, "null header or layout", message, message_bytes);
+    }
+    const std::string_view text(header, bytes);
+    size_t at = 0;
+    std::string line;
+    const auto next_line = [&]() {
+        if (at >= text.size()) {
+            return false;
+        }
+        size_t end = text.find('\n', at);
+        if (end == std::string_view::npos) {
+            end = text.size();
+        }
+        line.assign(text.substr(at, end - at));
+        at = end + 1;
+        if (!line.empty() && line.back() == '\r') {
+            line.pop_back();
+        }
+        return true;
+    };
+    if (!next_line() || line.rfind("ply", 0) != 0) {
+        return ply_fail(ATHX_INVALID_ARGUMENT, "not a PLY file", message, message_bytes);
+    }
+    std::string format;
+    std::vector<PlyElement> elements;
+    bool ended = false;
+    while (next_line()) {
+        std::istringstream words(line);
+        std::string word;
+        words >> word;
+        if (word == "format") {
+            words >> format;
+        } else if (word == "element") {
+            PlyElement e;
+            words >> e.name >> e.count;
+            elements.push_back(std::move(e));
+        } else if (word == "property") {
+            if (elements.empty()) {
+                return ply_fail(ATHX_INVALID_ARGUMENT, "a PLY property before any element", message,
+                                message_bytes);
+            }
+            PlyElement& e = elements.back();
+            std::string type;
+            words >> type;
+            PlyProperty p;
+            if (type == "list") {
+                std::string count_type, item_type;
+                words >> count_type >> item_type >> p.name;
+                p.type = "list";
+                e.has_list = true;
+                e.props.push_back(std::move(p));
+                continue;
+            }
+            words >> p.name;
+            p.type = type;
+            p.offset = (uint32_t)e.stride;
+            p.code = ply_type_of(type, p.bytes);
+            e.stride += p.bytes;
+            e.props.push_back(std::move(p));
+        } else if (word == "end_header") {
+            ended = true;
+            break;
+        }
+    }
+    if (!ended) {
+        return ply_fail(ATHX_INVALID_ARGUMENT, "PLY header has no end_header", message, message_bytes);
+    }
+    if (format.empty()) {
+        return ply_fail(ATHX_INVALID_ARGUMENT, "PLY header has no format line", message, message_bytes);
+    }
+    if (format != "binary_little_endian") {
+        return ply_fail(ATHX_UNSUPPORTED, "PLY format '" + format + "'; splat clouds are binary_little_endian",
+                        message, message_bytes);
+    }
+    // The vertex element, and its offset past the elements before it
+    // (PlyHeader::binaryOffsetOf, SplatReaders.cpp:60-85).
+    const PlyElement* vertex = nullptr;
+    uint64_t offset = at;
+    for (const PlyElement& e : elements) {
+        if (e.name == "vertex") {
+            vertex = &e;
+            break;
+        }
+        if (e.has_list) {
+            return ply_fail(ATHX_UNSUPPORTED, "a list element ('" + e.name + "') before 'vertex' makes its offset "
+                            "unknowable", message, message_bytes);
+        }
+        offset += e.stride * e.count;
+    }
+    if (!vertex || vertex->count == 0) {
+        return ply_fail(ATHX_INVALID_ARGUMENT, "no vertices", message, message_bytes);
+    }
+    if (vertex->has_list) {
+        return ply_fail(ATHX_UNSUPPORTED, "a list in the vertex element", message, message_bytes);
+    }
+    if (vertex->count > 0xFFFFFFFFull) {
+        return ply_fail(ATHX_UNSUPPORTED, "more than 2^32 splats", message, message_bytes);
+    }
+    if (vertex->props.size() > ATHX_PLY_MAX_PROPERTIES) {
+        return ply_fail(ATHX_UNSUPPORTED, "more than " + std::to_string(ATHX_PLY_MAX_PROPERTIES) +
+                        " vertex properties", message, message_bytes);
+    }
+    const auto index_of = [&](const std::string& name) -> int {
+        for (size_t k = 0; k < vertex->props.size(); ++k) {
+            if (vertex->props[k].name == name) {
+                return (int)k;
+            }
+        }
+        return -1;
+    };
+    static const char* kRequired[] = {"x",       "y",       "z",       "opacity", "scale_0", "scale_1", "scale_2",
+                                      "rot_0",   "rot_1",   "rot_2",   "rot_3",   "f_dc_0",  "f_dc_1",  "f_dc_2"};
+    for (const char* name : kRequired) {
+        if (index_of(name) < 0) {
+            return ply_fail(ATHX_INVALID_ARGUMENT,
+                            std::string("no '") + name + "' -- this does not look like a trained splat cloud",
+                            message, message_bytes);
+        }
+    }
+    athx_ply_layout L;
+    std::memset(&L, 0, sizeof L);
+    L.data_offset = offset;
+    L.count = vertex->count;
+    L.stride = (uint32_t)vertex->stride;
+    L.properties = (uint32_t)vertex->props.size();
+    for (size_t k = 0; k < vertex->props.size(); ++k) {
+        const PlyProperty& p = vertex->props[k];
+        if (p.code < 0) {
+            return ply_fail(ATHX_UNSUPPORTED, "property '" + p.name + "' of type '" + p.type + "'", message,
+                            message_bytes);
+        }
+        L.offset[k] = p.offset;
+        L.type[k] = (uint32_t)p.code;
+    }
+    L.x = (uint32_t)index_of("x");
+    L.y = (uint32_t)index_of("y");
+    L.z = (uint32_t)index_of("z");
+    L.opacity = (uint32_t)index_of("opacity");
+    for (uint32_t k = 0; k < 3; ++k) {
+        L.scale[k] = (uint32_t)index_of("scale_" + std::to_string(k));
+        L.dc[k] = (uint32_t)index_of("f_dc_" + std::to_string(k));
+    }
+    for (uint32_t k = 0; k < 4; ++k) {
+        L.rot[k] = (uint32_t)index_of("rot_" + std::to_string(k));
+    }
+    // The harmonics: counted, a whole degree, contiguous, red's first
+    // (SplatReaders.cpp:142-180).
+    uint32_t rest = 0;
+    for (const PlyProperty& p : vertex->props) {
+        if (p.name.rfind("f_rest_", 0) == 0) {
+            ++rest;
+        }
+    }
+    uint32_t per_colour = 0;
+    for (uint32_t basis : {15u, 8u, 3u}) {
+        if (rest >= basis * 3) {
+            per_colour = basis;
+            break;
+        }
+    }
+    if (per_colour > 0) {
+        const int base = index_of("f_rest_0");
+        const uint32_t file_per_colour = rest / 3;
+        for (uint32_t k = 0; k < per_colour * 3; ++k) {
+            const uint32_t colour = k / per_colour, basis = k % per_colour;
+            const uint32_t want = colour * file_per_colour + basis;
+            if (base < 0 || index_of("f_rest_" + std::to_string(want)) != base + (int)want) {
+                return ply_fail(ATHX_UNSUPPORTED, "harmonics are not contiguous in the vertex element", message,
+                                message_bytes);
+            }
+        }
+        if (file_per_colour != per_colour) {
+            return ply_fail(ATHX_UNSUPPORTED, std::to_string(rest) + " harmonic coefficients is not a whole degree",
+                            message, message_bytes);
+        }
+        L.rest_base = (uint32_t)base;
+        L.rest_per_colour = per_colour;
+    }
+    *out = L;
+    return ATHX_OK;
+}
+
+ATHX_API athx_status athx_ingest_ply(athx_ctx* ctx, const void* vertices, const athx_ply_layout* layout,
+                                     uint32_t max_degree, uint32_t keep_faint, const athx_ingest_out* out,
+                                     uint32_t* kept) {
+    if (!ctx) {
+        return ATHX_INVALID_ARGUMENT;
+    }
+    ctx->error.clear();
+    if (!layout || !out || !kept) {
+        return fail(ctx, ATHX_INVALID_ARGUMENT, "null layout, output or kept");
+    }
+    *kept = 0;
+    const athx_ply_layout& L = *layout;
+    if (L.count > 0xFFFFFFFFull || L.properties > ATHX_PLY_MAX_PROPERTIES || L.stride == 0) {
+        return fail(ctx, ATHX_INVALID_ARGUMENT, "layout out of range");
+    }
+    if (L.count == 0) {
+        return ATHX_OK;
+    }
+    if (!vertices || !out->positions || !out->shape) {
+        return fail(ctx, ATHX_INVALID_ARGUMENT, "null vertices, positions or shape");
+    }
+    IngestFields f{};
+    f.count = (uint32_t)L.count;
+    f.stride = L.stride;
+    f.keep_faint = keep_faint ? 1u : 0u;
+    const auto code_of = [&](uint32_t index, uint32_t& code) {
+        if (index >= L.properties || L.type[index] > ATHX_PLY_I32) {
+            return false;
+        }
+        const uint32_t bytes = L.type[index] == ATHX_PLY_F64 ? 8u
+                               : (L.type[index] == ATHX_PLY_U8 || L.type[index] == ATHX_PLY_I8) ? 1u
+                               : (L.type[index] == ATHX_PLY_U16 || L.type[index] == ATHX_PLY_I16) ? 2u
+                                                                                                   : 4u;
+        if ((uint64_t)L.offset[index] + bytes > L.stride || L.offset[index] > 0xFFFFFFu) {
+            return false;
+        }
+        code = L.offset[index] | (L.type[index] << 24);
+        return true;
+    };
+    const uint32_t fields[kIngestBase] = {L.x,        L.y,        L.z,        L.opacity, L.scale[0],
+                                          L.scale[1], L.scale[2], L.rot[0],   L.rot[1],  L.rot[2],
+                                          L.rot[3],   L.dc[0],    L.dc[1],    L.dc[2]};
+    for (uint32_t k = 0; k < kIngestBase; ++k) {
+        if (!code_of(fields[k], f.code[k])) {
+            return fail(ctx, ATHX_INVALID_ARGUMENT, "a decoded field outside the vertex");
+        }
+    }
+    // Rest coefficients kept a colour: the file's, capped by the degree
+    // (0, 3, 8, 15 for degrees 0..3); packed rgb a basis.
+    static const uint32_t kBasisOfDegree[4] = {0, 3, 8, 15};
+    const uint32_t cap = kBasisOfDegree[max_degree < 3 ? max_degree : 3];
+    if (L.rest_per_colour != 0 && L.rest_per_colour != 3 && L.rest_per_colour != 8 && L.rest_per_colour != 15) {
+        return fail(ctx, ATHX_INVALID_ARGUMENT, "rest_per_colour must be 0, 3, 8 or 15");
+    }
+    const uint32_t keep = L.rest_per_colour < cap ? L.rest_per_colour : cap;
+    f.rest = keep * 3;
+    f.sh_words = (f.rest + 1) / 2;
+    for (uint32_t basis = 0; basis < keep; ++basis) {
+        for (uint32_t colour = 0; colour < 3; ++colour) {
+            if (!code_of(L.rest_base + colour * L.rest_per_colour + basis, f.code[kIngestBase + basis * 3 + colour])) {
+                return fail(ctx, ATHX_INVALID_ARGUMENT, "a rest coefficient outside the vertex");
+            }
+        }
+    }
+    if (f.rest != 0 && !out->sh) {
+        return fail(ctx, ATHX_INVALID_ARGUMENT, "sh is null with rest coefficients kept");
+    }
+    athx_ingest_out o = *out;
+    if (f.rest == 0) {
+        o.sh = nullptr;
+    }
+    ATHX_HIP(ctx, hipSetDevice(ctx->device), "set device");
+    ATHX_HIP(ctx, reserve(ctx, ctx->ingest_keep, (size_t)f.count * 4 + 16), "ingest flags");
+    ATHX_HIP(ctx, reserve(ctx, ctx->ingest_dest, (size_t)f.count * 4 + 16), "ingest places");
+    ATHX_HIP(ctx, reserve(ctx, ctx->ingest_total, 16), "ingest total");
+    ATHX_HIP(ctx, reserve(ctx, ctx->api_scan_status, scan_status_words(f.count) * 4), "scan status");
+    const hipStream_t s = nullptr;
+    const uint8_t* bytes = static_cast<const uint8_t*>(vertices);
+    ATHX_HIP(ctx, launch_ingest_flags(bytes, f, as<uint32_t>(ctx->ingest_keep), s), "ingest flags");
+    ATHX_HIP(ctx, launch_scan(as<uint32_t>(ctx->ingest_keep), as<uint32_t>(ctx->ingest_dest),
+                              as<uint64_t>(ctx->ingest_total), f.count, as<uint32_t>(ctx->api_scan_status), s),
+             "ingest scan");
+    ATHX_HIP(ctx, launch_ingest_pack(bytes, f, as<uint32_t>(ctx->ingest_keep), as<uint32_t>(ctx->ingest_dest), o, s),
+             "ingest pack");
+    uint64_t total = 0;
+    ATHX_HIP(ctx, hipMemcpyAsync(&total, ctx->ingest_total.ptr, sizeof total, hipMemcpyDeviceToHost, s), "kept");
+    ATHX_HIP(ctx, hipStreamSynchronize(s), "ingest wait");
+    *kept = (uint32_t)total;
     return ATHX_OK;
 }
 

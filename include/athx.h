@@ -410,6 +410,81 @@ ATHX_API athx_status athx_exclusive_scan(athx_ctx* ctx, const uint32_t* in, uint
 ATHX_API athx_status athx_sort_pairs(athx_ctx* ctx, uint64_t* keys, uint32_t* values, uint32_t n,
                                      uint32_t key_bits, void* stream);
 
+/*
+ * Ingest: a trained splat cloud's PLY into the packed layout above.
+ *
+ * A binary_little_endian PLY's vertex element as its header lays it out
+ * (io::parsePlyHeader, modules/io/src/PlyHeader.cpp:59-128, and the checks of
+ * io::readSplatPly, modules/io/src/SplatReaders.cpp:48-195): the vertex
+ * block's place in the file, a vertex's bytes, and each property's byte
+ * offset and type. The fields the decode reads are property indices.
+ */
+#define ATHX_PLY_MAX_PROPERTIES 256
+typedef enum athx_ply_type {
+    ATHX_PLY_F32 = 0,
+    ATHX_PLY_F64 = 1,
+    ATHX_PLY_U8 = 2,
+    ATHX_PLY_I8 = 3,
+    ATHX_PLY_U16 = 4,
+    ATHX_PLY_I16 = 5,
+    ATHX_PLY_U32 = 6,
+    ATHX_PLY_I32 = 7
+} athx_ply_type;
+
+typedef struct athx_ply_layout {
+    uint64_t data_offset;  /* the vertex block's first byte in the file */
+    uint64_t count;        /* vertices */
+    uint32_t stride;       /* bytes a vertex */
+    uint32_t properties;   /* properties a vertex */
+    uint32_t offset[ATHX_PLY_MAX_PROPERTIES]; /* byte offset of each in the vertex */
+    uint32_t type[ATHX_PLY_MAX_PROPERTIES];   /* athx_ply_type of each */
+    /* x y z, opacity (a logit), scale_0..2 (logs), rot_0..3 (w x y z), f_dc_0..2 */
+    uint32_t x, y, z, opacity, scale[3], rot[4], dc[3];
+    /* f_rest_*: the first one, the file's count a colour (0, 3, 8 or 15;
+       red's all, then green's, then blue's) */
+    uint32_t rest_base, rest_per_colour;
+} athx_ply_layout;
+
+/*
+ * Reads the header at the start of a PLY file (bytes of it, up to the whole
+ * file; the reference reads the first 64 KiB). Errors as the reference's:
+ * INVALID_ARGUMENT for not a PLY, no end_header, no format, a property before
+ * any element, no vertices or a missing required property; UNSUPPORTED for a
+ * format other than binary_little_endian, a list in or before the vertex
+ * element, a property type outside athx_ply_type, more than 2^32 vertices or
+ * ATHX_PLY_MAX_PROPERTIES properties, or harmonics that are not a whole
+ * degree or not contiguous. message (may be NULL) receives the reason.
+ * Host only: no device is touched.
+ */
+ATHX_API athx_status athx_ply_parse(const char* header, uint64_t bytes, athx_ply_layout* out, char* message,
+                                    uint64_t message_bytes);
+
+/* Where athx_ingest_ply writes the kept splats (device memory, the caller's). */
+typedef struct athx_ingest_out {
+    float* positions;  /* [count][4]: x y z, linear opacity clamped to [0, 5] */
+    uint32_t* shape;   /* [count][4]: quaternion word; f16 log-scales; f16 base colour */
+    uint32_t* sh;      /* [count][sh_words] f16 rest coefficients, rgb a basis; NULL when none kept */
+    uint32_t* origin;  /* [count] each kept splat's vertex index, or NULL */
+} athx_ingest_out;
+
+/*
+ * The vertex block of a PLY (device memory: layout->count * layout->stride
+ * bytes, read as bytes at any alignment) validated, decoded and packed on
+ * the device (scene::CloudLoader::upload, GpuClouds.h:349-516, over
+ * packed_records.slang, splat_validate.slang and splat_decode.slang): each
+ * property widened to a float as the file types it; a vertex kept when its
+ * position is finite, at most one log-scale is a flat -inf (below -30 in the
+ * file) and the others are finite and under 60 in magnitude, and its
+ * opacity (sigmoid of the logit) is at least 1/255, or, with keep_faint,
+ * whenever its position is finite. Kept vertices keep file order. Rest
+ * coefficients kept: min(the file's, max_degree's 0/3/8/15) a colour, whose
+ * words a splat are (kept * 3 + 1) / 2. *kept (host) receives the count;
+ * the call waits for it.
+ */
+ATHX_API athx_status athx_ingest_ply(athx_ctx* ctx, const void* vertices, const athx_ply_layout* layout,
+                                     uint32_t max_degree, uint32_t keep_faint, const athx_ingest_out* out,
+                                     uint32_t* kept);
+
 #ifdef __cplusplus
 }
 #endif

@@ -24,6 +24,10 @@
  *                    stable tile sort (TileRasterizer.cpp:1111-1176)
  *   orc_fd_grads     FP64 central finite differences of an L2 image loss
  *                    with every discrete decision frozen
+ *   orc_ingest       a PLY vertex block widened, validated, decoded and
+ *                    packed: packed_records.slang:1-110, splat_validate.slang,
+ *                    splat_decode.slang:1-173 (PLY's encodings),
+ *                    splat_encoding.slang, packing.slang:33-57,319-328
  *
  * Parity of each function is pinned by tests/test_oracle.py (closed forms,
  * invariants, brute force and the reference's own test assertions).
@@ -1529,3 +1533,159 @@ ORC_EXPORT void orc_fd_grads(const orc_view* v, const float* rec, const uint32_t
     free(tile_of_pair);
     free(r64);
 }
+
+
+/* ---- orc_ingest: a PLY vertex block into the packed layout ---- */
+
+/* One property of one vertex as a float: the little-endian bytes of its
+ * type, a double rounded to the nearest float (packed_records.slang). */
+static float ingest_widen(const uint8_t* rec, uint32_t offset, uint32_t type) {
+    const uint8_t* b = rec + offset;
+    uint32_t w = 0;
+    switch (type) {
+    case 2: return (float)b[0];
+    case 3: return (float)(int8_t)b[0];
+    case 4: return (float)(uint16_t)(b[0] | (b[1] << 8));
+    case 5: return (float)(int16_t)(uint16_t)(b[0] | (b[1] << 8));
+    case 1: {
+        uint64_t u = 0;
+        for (int k = 7; k >= 0; --k) {
+            u = (u << 8) | b[k];
+        }
+        double d;
+        memcpy(&d, &u, sizeof d);
+        return (float)d;
+    }
+    default:
+        w = (uint32_t)b[0] | ((uint32_t)b[1] << 8) | ((uint32_t)b[2] << 16) | ((uint32_t)b[3] << 24);
+        if (type == 0) {
+            float f;
+            memcpy(&f, &w, sizeof f);
+            return f;
+        }
+        return type == 6 ? (float)w : (float)(int32_t)w;
+    }
+}
+
+/* IEEE binary16 bits of f, round to nearest even (f32tof16), after
+ * packing.slang's halfSafe clamp to +-65000 (a NaN clamps to -65000, as
+ * max(NaN, lo) is lo). */
+static uint32_t ingest_half(float v) {
+    float c = v > -65000.0f ? v : -65000.0f;
+    c = c < 65000.0f ? c : 65000.0f;
+    uint32_t x;
+    memcpy(&x, &c, sizeof x);
+    const uint32_t sign = (x >> 16) & 0x8000u, ax = x & 0x7FFFFFFFu;
+    if (ax < 0x38800000u) {
+        /* below 2^-14: a subnormal half, round(|c| / 2^-24) */
+        const float scaled = fabsf(c) * 16777216.0f;
+        return sign | (uint32_t)rintf(scaled);
+    }
+    uint32_t h = ((((ax >> 23) - 127u + 15u) & 0x1Fu) << 10) | ((ax & 0x7FFFFFu) >> 13);
+    const uint32_t rest = ax & 0x1FFFu;
+    if (rest > 0x1000u || (rest == 0x1000u && (h & 1u))) {
+        h += 1u;
+    }
+    return sign | h;
+}
+
+static int ingest_finite(float v) { return isfinite(v) ? 1 : 0; }
+
+/* The smallest-three word of a unit-normalised (x, y, z, w): the largest
+ * magnitude (first of equals) dropped and made non-negative, the others
+ * mapped from [-sqrt(1/2), sqrt(1/2)] to 10 bits, rounded half up. */
+static uint32_t ingest_quaternion(float x, float y, float z, float w) {
+    const float len = sqrtf(((x * x + y * y) + z * z) + w * w);
+    float q[4];
+    q[0] = x / len;
+    q[1] = y / len;
+    q[2] = z / len;
+    q[3] = w / len;
+    uint32_t largest = 0;
+    for (uint32_t k = 1; k < 4; ++k) {
+        if (fabsf(q[k]) > fabsf(q[largest])) {
+            largest = k;
+        }
+    }
+    if (q[largest] < 0.0f) {
+        for (int k = 0; k < 4; ++k) {
+            q[k] = -q[k];
+        }
+    }
+    uint32_t word = largest, shift = 22;
+    for (uint32_t k = 0; k < 4; ++k) {
+        if (k == largest) {
+            continue;
+        }
+        float unit = q[k] / 0.70710678f * 0.5f + 0.5f;
+        unit = unit < 0.0f ? 0.0f : (unit > 1.0f ? 1.0f : unit);
+        word |= (uint32_t)floorf(unit * 1023.0f + 0.5f) << shift;
+        shift -= 10;
+    }
+    return word;
+}
+
+/*
+ * count vertices of `stride` bytes; fields[k] = (byte offset, type) of x y z,
+ * opacity, scale_0..2, rot_0..3 (w x y z), f_dc_0..2 (k < 14), then `rest`
+ * rest coefficients in packing order (rgb a basis). Writes the kept vertices
+ * (file order) and returns their count.
+ */
+ORC_EXPORT uint32_t orc_ingest(const uint8_t* vertices, uint32_t count, uint32_t stride, const uint32_t* offsets,
+                               const uint32_t* types, uint32_t rest, uint32_t keep_faint, float* positions,
+                               uint32_t* shape, uint32_t* sh, uint32_t* origin) {
+    const uint32_t words = (rest + 1) / 2;
+    uint32_t kept = 0;
+    for (uint32_t i = 0; i < count; ++i) {
+        const uint8_t* rec = vertices + (size_t)i * stride;
+        float v[14];
+        for (int k = 0; k < 14; ++k) {
+            v[k] = ingest_widen(rec, offsets[k], types[k]);
+        }
+        const float opacity = athx_sigmoidf(v[3]);
+        float ls[3];
+        int flats = 0, scales_ok = 1;
+        for (int k = 0; k < 3; ++k) {
+            ls[k] = v[4 + k] < -30.0f ? -INFINITY : v[4 + k];
+            const int flat = ls[k] <= -1024.0f;
+            flats += flat;
+            const float kept_ls = flat ? 0.0f : ls[k];
+            scales_ok = scales_ok && ingest_finite(kept_ls) && fabsf(kept_ls) < 60.0f;
+        }
+        const int finite = ingest_finite(v[0]) && ingest_finite(v[1]) && ingest_finite(v[2]);
+        const int keep = (finite && flats <= 1 && scales_ok && opacity >= 1.0f / 255.0f) || (keep_faint && finite);
+        if (!keep) {
+            continue;
+        }
+        const uint32_t j = kept++;
+        if (origin) {
+            origin[j] = i;
+        }
+        float a = opacity < 0.0f ? 0.0f : opacity;
+        a = a > 5.0f ? 5.0f : a;
+        positions[4 * j + 0] = v[0];
+        positions[4 * j + 1] = v[1];
+        positions[4 * j + 2] = v[2];
+        positions[4 * j + 3] = a;
+        /* rot_0 is w; a quaternion of no length is the identity */
+        const float qw = v[7], qx = v[8], qy = v[9], qz = v[10];
+        const float qlen = sqrtf(((qx * qx + qy * qy) + qz * qz) + qw * qw);
+        const uint32_t rotation = qlen > 1e-8f ? ingest_quaternion(qx, qy, qz, qw)
+                                               : ingest_quaternion(0.0f, 0.0f, 0.0f, 1.0f);
+        const float sh0 = 0.28209479177387814f;
+        const float base[3] = {0.5f + sh0 * v[11], 0.5f + sh0 * v[12], 0.5f + sh0 * v[13]};
+        shape[4 * j + 0] = rotation;
+        shape[4 * j + 1] = ingest_half(ls[0]) | (ingest_half(ls[1]) << 16);
+        shape[4 * j + 2] = ingest_half(ls[2]) | (ingest_half(base[0]) << 16);
+        shape[4 * j + 3] = ingest_half(base[1]) | (ingest_half(base[2]) << 16);
+        for (uint32_t k = 0; k < rest; k += 2) {
+            const uint32_t lo = ingest_half(ingest_widen(rec, offsets[14 + k], types[14 + k]));
+            const uint32_t hi = k + 1 < rest ? ingest_half(ingest_widen(rec, offsets[15 + k], types[15 + k])) : 0u;
+            sh[(size_t)j * words + k / 2] = lo | (hi << 16);
+        }
+    }
+    return kept;
+}
+
+ORC_EXPORT uint32_t orc_half_bits(float v) { return ingest_half(v); }
+ORC_EXPORT uint32_t orc_encode_quaternion(float x, float y, float z, float w) { return ingest_quaternion(x, y, z, w); }

@@ -177,6 +177,13 @@ def load():
         lib.orc_expf.restype = ctypes.c_float
         lib.orc_logf.argtypes = [ctypes.c_float]
         lib.orc_logf.restype = ctypes.c_float
+        lib.orc_ingest.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, _u32p, _u32p, ctypes.c_uint32,
+                                   ctypes.c_uint32, _f32p, _u32p, ctypes.c_void_p, _u32p]
+        lib.orc_ingest.restype = ctypes.c_uint32
+        lib.orc_half_bits.argtypes = [ctypes.c_float]
+        lib.orc_half_bits.restype = ctypes.c_uint32
+        lib.orc_encode_quaternion.argtypes = [ctypes.c_float] * 4
+        lib.orc_encode_quaternion.restype = ctypes.c_uint32
         _lib = lib
     return _lib
 
@@ -510,3 +517,33 @@ def decode_quaternion(word: int) -> np.ndarray:
     out = (ctypes.c_float * 4)()
     load().orc_decode_quaternion(word, out)
     return np.array(out[:], np.float32)
+
+
+def ingest(vertices: np.ndarray, count: int, stride: int, fields: list[tuple[int, int]], keep_faint: bool = False):
+    """orc_ingest: `count` vertices of `stride` bytes (a uint8 array); fields =
+    (byte offset, athx_ply_type) of x y z, opacity, scale_0..2, rot_0..3 (w x y
+    z), f_dc_0..2, then the kept rest coefficients rgb a basis. Returns the
+    kept vertices' positions [k, 4] f32, shape [k, 4] u32, sh [k, words] u32
+    and origin [k] u32."""
+    vertices = np.ascontiguousarray(vertices, np.uint8)
+    assert vertices.size >= count * stride and len(fields) >= 14
+    rest = len(fields) - 14
+    words = (rest + 1) // 2
+    offsets = np.array([f[0] for f in fields], np.uint32)
+    types = np.array([f[1] for f in fields], np.uint32)
+    positions = np.zeros((max(count, 1), 4), np.float32)
+    shape = np.zeros((max(count, 1), 4), np.uint32)
+    sh = np.zeros((max(count, 1), max(words, 1)), np.uint32)
+    origin = np.zeros(max(count, 1), np.uint32)
+    kept = load().orc_ingest(vertices.ctypes.data, count, stride, _c(offsets, np.uint32), _c(types, np.uint32), rest,
+                             1 if keep_faint else 0, _c(positions, np.float32), _c(shape, np.uint32),
+                             sh.ctypes.data, _c(origin, np.uint32))
+    return positions[:kept], shape[:kept], sh[:kept, :words], origin[:kept]
+
+
+def half_bits(v: float) -> int:
+    return int(load().orc_half_bits(v))
+
+
+def encode_quaternion(x: float, y: float, z: float, w: float) -> int:
+    return int(load().orc_encode_quaternion(x, y, z, w))

@@ -57,6 +57,8 @@ STRUCTS = {
     "athx_counters": _lib.Counters,
     "athx_grads": _lib.Grads,
     "athx_frame_info": _lib.FrameInfo,
+    "athx_ply_layout": _lib.PlyLayout,
+    "athx_ingest_out": _lib.IngestOut,
 }
 
 
