@@ -210,6 +210,16 @@ class IngestOut(ctypes.Structure):
     ]
 
 
+class ImageDifference(ctypes.Structure):
+    _fields_ = [("pixels", ctypes.c_uint64), ("p99", ctypes.c_uint32), ("max", ctypes.c_uint32),
+                ("over2", ctypes.c_uint64)]
+
+
+class HdrDifference(ctypes.Structure):
+    _fields_ = [("pixels", ctypes.c_uint64), ("rel_mse", ctypes.c_double), ("p99_relative", ctypes.c_double),
+                ("max_relative", ctypes.c_double)]
+
+
 SIGNATURES = {
     "athx_version": (ctypes.c_char_p, []),
     "athx_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]),
@@ -296,6 +306,21 @@ SIGNATURES = {
     "athx_ply_parse": (
         ctypes.c_int,
         [ctypes.c_char_p, ctypes.c_uint64, ctypes.POINTER(PlyLayout), ctypes.c_char_p, ctypes.c_uint64],
+    ),
+    "athx_reference_render": (
+        ctypes.c_int,
+        [ctypes.c_void_p, ctypes.POINTER(View), ctypes.POINTER(Gaussians), ctypes.POINTER(Settings),
+         ctypes.POINTER(Targets), ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint32)],
+    ),
+    "athx_compare_images": (
+        ctypes.c_int,
+        [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32,
+         ctypes.POINTER(ImageDifference)],
+    ),
+    "athx_compare_hdr": (
+        ctypes.c_int,
+        [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32,
+         ctypes.POINTER(HdrDifference)],
     ),
     "athx_ingest_ply": (
         ctypes.c_int,

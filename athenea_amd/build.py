@@ -24,7 +24,8 @@ INCLUDE = HERE.parent / "include"
 LIB = HERE / "libathx.so"
 BUILD = HERE / "_build"
 
-SOURCES = ["project.hip", "scan.hip", "sort.hip", "blend.hip", "project_backward.hip", "ingest.hip", "capi.hip"]
+SOURCES = ["project.hip", "scan.hip", "sort.hip", "blend.hip", "project_backward.hip", "ingest.hip", "reference.hip",
+           "capi.hip"]
 
 HIPFLAGS = [
     "-O3",

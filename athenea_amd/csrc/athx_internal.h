@@ -353,6 +353,22 @@ struct IngestFields {
     uint32_t keep_faint;  // keep every vertex with a finite position
     uint32_t code[kIngestCodes];
 };
+// The ground truth (reference.hip): the reference's own projection into
+// records of the rasterizer's layout, keys (the view depth's bits, all ones
+// when culled) and values (the index) for a stable sort; then every drawn
+// splat blended at every pixel in that order (overflow: pixels with more
+// than max_hits contributors); and the image comparisons.
+hipError_t launch_ref_project(const float* positions, const uint32_t* shape, const uint32_t* sh, uint32_t n,
+                              uint32_t rest_per_colour, uint32_t sh_limit, uint32_t antialias, uint32_t linear,
+                              const DView* view, float* records, uint32_t* keys, uint32_t* values, hipStream_t s);
+hipError_t launch_ref_blend(const DView* view, uint32_t width, uint32_t height, const float* records,
+                            const uint32_t* sorted_keys, const uint32_t* sorted, uint32_t n, uint32_t max_hits,
+                            uint32_t depth_mode, float depth_threshold, uint32_t* overflow, hipStream_t s);
+hipError_t launch_compare_codes(const float* a, const float* b, uint32_t width, uint32_t height, uint32_t* rows,
+                                uint32_t* bins, hipStream_t s);
+hipError_t launch_compare_hdr(const float* a, const float* b, uint32_t width, uint32_t height, uint32_t* rows,
+                              float* row_error, uint32_t* bins, float* total, hipStream_t s);
+
 // Each vertex's keep flag (1 or 0).
 hipError_t launch_ingest_flags(const uint8_t* vertices, const IngestFields& f, uint32_t* keep, hipStream_t s);
 // The kept vertices decoded and packed at dest[i] (the flags' exclusive scan).

@@ -17,6 +17,7 @@
 #include <cstdio>
 #include <cstring>
 #include <exception>
+#include <cmath>
 #include <new>
 #include <sstream>
 #include <string_view>
@@ -78,6 +79,9 @@ struct athx_ctx {
     DeviceBuffer api_sort_meta, api_alt_keys, api_alt_vals, api_scan_status, api_count;
     // The PLY ingest: each vertex's keep flag, their scan, the kept total.
     DeviceBuffer ingest_keep, ingest_dest, ingest_total;
+    // The ground truth and the image comparisons.
+    DeviceBuffer ref_records, ref_keys, ref_values, ref_alt_keys, ref_alt_values, ref_sort_meta, ref_count;
+    DeviceBuffer cmp_rows, cmp_row_error, cmp_bins;
     uint64_t* host_totals = nullptr;  // pinned: the totals words (kTotal*)
     DView* host_views = nullptr;      // pinned staging for the views
     uint32_t* host_tile_bases = nullptr;  // [nviews + 1] tile bases, then [nviews + 1] strip bases
@@ -114,7 +118,9 @@ std::vector<DeviceBuffer*> all_buffers(athx_ctx* ctx) {
             &ctx->strip_keys, &ctx->strip_slots, &ctx->strip_alt_keys, &ctx->strip_alt_slots, &ctx->strip_sort_meta,
             &ctx->strip_ranges, &ctx->segbase, &ctx->seg_map, &ctx->rank_fp, &ctx->masks, &ctx->seg_table, &ctx->exact, &ctx->exact_offsets,
             &ctx->exact_status, &ctx->api_sort_meta, &ctx->api_alt_keys, &ctx->api_alt_vals, &ctx->api_scan_status,
-            &ctx->api_count, &ctx->ingest_keep, &ctx->ingest_dest, &ctx->ingest_total};
+            &ctx->api_count, &ctx->ingest_keep, &ctx->ingest_dest, &ctx->ingest_total, &ctx->ref_records,
+            &ctx->ref_keys, &ctx->ref_values, &ctx->ref_alt_keys, &ctx->ref_alt_values, &ctx->ref_sort_meta,
+            &ctx->ref_count, &ctx->cmp_rows, &ctx->cmp_row_error, &ctx->cmp_bins};
 }
 
 athx_status fail(athx_ctx* ctx, athx_status code, const std::string& msg) {
@@ -1849,6 +1855,179 @@ ATHX_API athx_status athx_graph_pairs(athx_graph* g, uint64_t* pairs, uint64_t* 
         return ATHX_INVALID_ARGUMENT;
     }
     return athx_frame_pairs(g->ctx, pairs, capacity);
+}
+
+}  // extern "C"
+
+namespace athx {
+namespace {
+
+// The percentile the reference reads off a 256-bin histogram
+// (ReferenceRenderer.cpp:292-312, 404-422): the first bin whose running
+// count reaches pixels - pixels / 100.
+uint32_t p99_bin(const uint32_t bins[256], uint64_t pixels) {
+    const uint64_t at = pixels - pixels / 100;
+    uint64_t running = 0;
+    for (uint32_t k = 0; k < 256; ++k) {
+        running += bins[k];
+        if (running >= at) {
+            return k;
+        }
+    }
+    return 255;
+}
+
+athx_status compare_setup(athx_ctx* ctx, const float* a, const float* b, uint32_t width, uint32_t height,
+                          const void* out) {
+    if (!a || !b || !out) {
+        return fail(ctx, ATHX_INVALID_ARGUMENT, "null image or result");
+    }
+    if (width == 0 || height == 0) {
+        return fail(ctx, ATHX_INVALID_ARGUMENT, "an empty image");
+    }
+    ATHX_HIP(ctx, hipSetDevice(ctx->device), "set device");
+    ATHX_HIP(ctx, reserve(ctx, ctx->cmp_rows, (size_t)height * 256 * 4), "comparison rows");
+    ATHX_HIP(ctx, reserve(ctx, ctx->cmp_row_error, (size_t)height * 4 + 16), "comparison row errors");
+    ATHX_HIP(ctx, reserve(ctx, ctx->cmp_bins, 256 * 4 + 16), "comparison bins");
+    return ATHX_OK;
+}
+
+}  // namespace
+}  // namespace athx
+
+using namespace athx;
+
+extern "C" {
+
+ATHX_API athx_status athx_compare_images(athx_ctx* ctx, const float* a, const float* b, uint32_t width,
+                                         uint32_t height, athx_image_difference* out) {
+    if (!ctx) {
+        return ATHX_INVALID_ARGUMENT;
+    }
+    ctx->error.clear();
+    if (athx_status st = compare_setup(ctx, a, b, width, height, out); st != ATHX_OK) {
+        return st;
+    }
+    ATHX_HIP(ctx, launch_compare_codes(a, b, width, height, as<uint32_t>(ctx->cmp_rows), as<uint32_t>(ctx->cmp_bins),
+                                       nullptr),
+             "compare");
+    uint32_t bins[256];
+    ATHX_HIP(ctx, hipMemcpy(bins, ctx->cmp_bins.ptr, sizeof bins, hipMemcpyDeviceToHost), "read bins");
+    athx_image_difference d{};
+    for (uint32_t k = 0; k < 256; ++k) {
+        d.pixels += bins[k];
+        if (bins[k] > 0) {
+            d.max = k;
+        }
+        if (k > 2) {
+            d.over2 += bins[k];
+        }
+    }
+    d.p99 = p99_bin(bins, d.pixels);
+    *out = d;
+    return ATHX_OK;
+}
+
+ATHX_API athx_status athx_compare_hdr(athx_ctx* ctx, const float* a, const float* b, uint32_t width, uint32_t height,
+                                      athx_hdr_difference* out) {
+    if (!ctx) {
+        return ATHX_INVALID_ARGUMENT;
+    }
+    ctx->error.clear();
+    if (athx_status st = compare_setup(ctx, a, b, width, height, out); st != ATHX_OK) {
+        return st;
+    }
+    float* row_error = as<float>(ctx->cmp_row_error);
+    ATHX_HIP(ctx, launch_compare_hdr(a, b, width, height, as<uint32_t>(ctx->cmp_rows), row_error,
+                                     as<uint32_t>(ctx->cmp_bins), row_error + height, nullptr),
+             "compare hdr");
+    uint32_t bins[256];
+    float error = 0.0f;
+    ATHX_HIP(ctx, hipMemcpy(bins, ctx->cmp_bins.ptr, sizeof bins, hipMemcpyDeviceToHost), "read bins");
+    ATHX_HIP(ctx, hipMemcpy(&error, row_error + height, sizeof error, hipMemcpyDeviceToHost), "read error");
+    // Bin k > 0 holds relative differences below 2^(k / 8 - 16).
+    const auto upper = [](uint32_t bin) { return bin == 0 ? 0.0 : std::pow(2.0, bin / 8.0 - 16.0); };
+    athx_hdr_difference d{};
+    for (uint32_t k = 0; k < 256; ++k) {
+        d.pixels += bins[k];
+        if (bins[k] > 0) {
+            d.max_relative = upper(k);
+        }
+    }
+    d.p99_relative = upper(p99_bin(bins, d.pixels));
+    d.rel_mse = d.pixels > 0 ? (double)error / (double)d.pixels : 0.0;
+    *out = d;
+    return ATHX_OK;
+}
+
+ATHX_API athx_status athx_reference_render(athx_ctx* ctx, const athx_view* view, const athx_gaussians* g,
+                                           const athx_settings* settings, const athx_targets* targets,
+                                           float* records, uint32_t* overflow) {
+    if (!ctx) {
+        return ATHX_INVALID_ARGUMENT;
+    }
+    ctx->error.clear();
+    if (!view || !g || !targets || !overflow) {
+        return fail(ctx, ATHX_INVALID_ARGUMENT, "null view, gaussians, targets or overflow");
+    }
+    if (athx_status st = check_settings(ctx, settings); st != ATHX_OK) {
+        return st;
+    }
+    const athx_packed_splats* pk = g->packed;
+    if (!pk || !pk->positions || !pk->shape) {
+        return fail(ctx, ATHX_UNSUPPORTED, "the reference renderer reads the packed layout (gaussians->packed)");
+    }
+    if (pk->rest_per_colour != 0 && pk->rest_per_colour != 3 && pk->rest_per_colour != 8 &&
+        pk->rest_per_colour != 15) {
+        return fail(ctx, ATHX_INVALID_ARGUMENT, "rest_per_colour must be 0, 3, 8 or 15");
+    }
+    if (pk->rest_per_colour != 0 && !pk->sh) {
+        return fail(ctx, ATHX_INVALID_ARGUMENT, "null sh with rest coefficients");
+    }
+    *overflow = 0;
+    ATHX_HIP(ctx, hipSetDevice(ctx->device), "set device");
+    uint64_t tiles = 0;
+    if (athx_status st = stage_views(ctx, view, 1, settings, targets, nullptr, tiles); st != ATHX_OK) {
+        return st;
+    }
+    const hipStream_t s = nullptr;
+    if (athx_status st = upload_views(ctx, 1, s); st != ATHX_OK) {
+        return st;
+    }
+    const uint32_t n = g->count;
+    const size_t cap = n ? n : 1;
+    ATHX_HIP(ctx, reserve(ctx, ctx->ref_records, cap * kRecordFloats * 4), "reference records");
+    ATHX_HIP(ctx, reserve(ctx, ctx->ref_keys, cap * 4 + 16), "reference keys");
+    ATHX_HIP(ctx, reserve(ctx, ctx->ref_values, cap * 4 + 16), "reference values");
+    ATHX_HIP(ctx, reserve(ctx, ctx->ref_alt_keys, cap * 4 + 16), "reference keys");
+    ATHX_HIP(ctx, reserve(ctx, ctx->ref_alt_values, cap * 4 + 16), "reference values");
+    ATHX_HIP(ctx, reserve(ctx, ctx->ref_sort_meta, sort_meta_words((uint32_t)cap, 4) * 4 + 16), "reference sort");
+    ATHX_HIP(ctx, reserve(ctx, ctx->ref_count, 64), "reference counts");
+    float* rec = records ? records : as<float>(ctx->ref_records);
+    uint64_t* count = as<uint64_t>(ctx->ref_count);
+    // The sort's count (a u64: n, 0) and the overflow counter, written on the stream.
+    ATHX_HIP(ctx, launch_fill_u32(reinterpret_cast<uint32_t*>(count), n, 1, s), "count");
+    ATHX_HIP(ctx, launch_fill_u32(reinterpret_cast<uint32_t*>(count) + 1, 0u, 3, s), "count and overflow");
+    ATHX_HIP(ctx, launch_ref_project(pk->positions, pk->shape, pk->sh, n, pk->rest_per_colour, settings->max_sh_degree,
+                                     settings->antialias, settings->linear, as<DView>(ctx->views), rec,
+                                     as<uint32_t>(ctx->ref_keys), as<uint32_t>(ctx->ref_values), s),
+             "reference projection");
+    // The drawn splats in (depth, index) order: a stable sort of the
+    // depth's 32 bits (culled keys, all ones, last).
+    const SortScratch sc = sort_scratch(ctx->ref_sort_meta, ctx->ref_alt_keys.ptr, as<uint32_t>(ctx->ref_alt_values),
+                                        (uint32_t)cap, 4);
+    ATHX_HIP(ctx, launch_sort(as<uint32_t>(ctx->ref_keys), as<uint32_t>(ctx->ref_values), count, (uint32_t)cap, 0, 32,
+                              sc, s, nullptr),
+             "reference sort");
+    uint32_t* over = reinterpret_cast<uint32_t*>(count + 1);
+    ATHX_HIP(ctx, launch_ref_blend(as<DView>(ctx->views), view->width, view->height, rec, as<uint32_t>(ctx->ref_keys),
+                                   as<uint32_t>(ctx->ref_values), n, 1024, settings->depth_mode,
+                                   settings->depth_threshold, over, s),
+             "reference blend");
+    uint32_t host_over = 0;
+    ATHX_HIP(ctx, hipMemcpy(&host_over, over, sizeof host_over, hipMemcpyDeviceToHost), "read overflow");
+    *overflow = host_over;
+    return ATHX_OK;
 }
 
 }  // extern "C"

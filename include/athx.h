@@ -34,6 +34,13 @@
  *       gpu::RadixSort::sort (modules/gpu/include/athenea/gpu/algo/
  *       RadixSort.h:63-64) for 64-bit keys: the reference's keysLo/keysHi
  *       halves are one uint64 here.
+ *   athx_ply_parse / athx_ingest_ply
+ *       io::parsePlyHeader + io::readSplatPly (modules/io/src/PlyHeader.cpp,
+ *       SplatReaders.cpp:48-195) and scene::CloudLoader::upload for a PLY
+ *       (modules/scene/include/athenea/scene/GpuClouds.h:349-516).
+ *   athx_reference_render / athx_compare_images / athx_compare_hdr
+ *       render::ReferenceRenderer::render, compareImages, compareHdr
+ *       (modules/render/include/athenea/render/ReferenceRenderer.h:20-66).
  *
  * Threading: a context is used by one host thread at a time. Every call is
  * asynchronous on the stream it is given, except where a comment says it
@@ -484,6 +491,53 @@ typedef struct athx_ingest_out {
 ATHX_API athx_status athx_ingest_ply(athx_ctx* ctx, const void* vertices, const athx_ply_layout* layout,
                                      uint32_t max_degree, uint32_t keep_faint, const athx_ingest_out* out,
                                      uint32_t* kept);
+
+/*
+ * The ground truth (render::ReferenceRenderer::render, ReferenceRenderer.h:
+ * 20-37; reference/reference_project.slang, reference_blend.slang): one view
+ * of a packed cloud (g->packed; count splats) projected by the reference's
+ * own plain projection and blended per pixel over every drawn splat in
+ * (depth, index) order -- no tiles, no pairs, no ranges. Writes targets'
+ * colour and depth (settings: antialias, linear, max_sh_degree, depth_mode,
+ * depth_threshold, background from the view); records (device, may be NULL)
+ * receives each splat's projection in the frame's record layout (12 floats:
+ * centre xy, view z (-1: culled), cut power, conic a b c, alpha, rgb, 0).
+ * *overflow (host) receives the pixels more than 1024 splats reach, which the
+ * reference counts because its per-pixel list holds 1024: a comparison
+ * means something only at zero. Slow by design: pixels x splats tests.
+ * Waits for the count.
+ */
+ATHX_API athx_status athx_reference_render(athx_ctx* ctx, const athx_view* view, const athx_gaussians* gaussians,
+                                           const athx_settings* settings, const athx_targets* targets,
+                                           float* records, uint32_t* overflow);
+
+/* Two renders compared as 8-bit sRGB code values (ReferenceRenderer.h:52-58):
+   each pixel's largest channel difference, alpha included. */
+typedef struct athx_image_difference {
+    uint64_t pixels;
+    uint32_t p99;    /* the smallest difference at least 99% of the pixels are within */
+    uint32_t max;
+    uint64_t over2;  /* pixels differing by more than 2 code values */
+} athx_image_difference;
+
+/* Two linear images compared as radiance (ReferenceRenderer.h:60-66). */
+typedef struct athx_hdr_difference {
+    uint64_t pixels;
+    double rel_mse;       /* mean over pixels of the channel mean of (a - b)^2 / (b^2 + 1e-2) */
+    double p99_relative;  /* 99th percentile of max_c |a - b| / max(|b|, 1e-3), to an eighth of a binade */
+    double max_relative;
+} athx_hdr_difference;
+
+/*
+ * a, b: [height][width] float4 linear premultiplied images (device). The
+ * histograms are built on the device, a row of pixels a thread as the
+ * reference's image_compare.slang and image_compare_hdr.slang do; the
+ * percentiles are read off them on the host. Wait for the result.
+ */
+ATHX_API athx_status athx_compare_images(athx_ctx* ctx, const float* a, const float* b, uint32_t width,
+                                         uint32_t height, athx_image_difference* out);
+ATHX_API athx_status athx_compare_hdr(athx_ctx* ctx, const float* a, const float* b, uint32_t width, uint32_t height,
+                                      athx_hdr_difference* out);
 
 #ifdef __cplusplus
 }

@@ -24,6 +24,12 @@
  *                    stable tile sort (TileRasterizer.cpp:1111-1176)
  *   orc_fd_grads     FP64 central finite differences of an L2 image loss
  *                    with every discrete decision frozen
+ *   orc_ref_project  reference/reference_project.slang:1-107, the ground
+ *                    truth's own projection of the packed layout (its
+ *                    blend is orc_reference)
+ *   orc_compare_codes / orc_compare_hdr
+ *                    reference/image_compare.slang, image_compare_hdr.slang:
+ *                    the per-row histograms and relMSE rows, summed
  *   orc_ingest       a PLY vertex block widened, validated, decoded and
  *                    packed: packed_records.slang:1-110, splat_validate.slang,
  *                    splat_decode.slang:1-173 (PLY's encodings),
@@ -1689,3 +1695,238 @@ ORC_EXPORT uint32_t orc_ingest(const uint8_t* vertices, uint32_t count, uint32_t
 
 ORC_EXPORT uint32_t orc_half_bits(float v) { return ingest_half(v); }
 ORC_EXPORT uint32_t orc_encode_quaternion(float x, float y, float z, float w) { return ingest_quaternion(x, y, z, w); }
+
+
+/* ---- the ground truth's projection and the image comparisons ---- */
+
+static float ref_half(uint32_t h) { return half_to_float(h & 0xFFFFu); }
+
+/* reference_project.slang over the packed layout, every product written out
+ * in the order it is summed: rec[12] a splat (the frame's record layout), z
+ * = -1 when culled. rest: coefficients a colour (0, 3, 8, 15). */
+ORC_EXPORT void orc_ref_project(const orc_view* v, const float* positions, const uint32_t* shape,
+                                const uint32_t* sh, uint32_t n, uint32_t rest, float* records) {
+    const uint32_t words = (rest * 3 + 1) / 2;
+    for (uint32_t i = 0; i < n; ++i) {
+        float* rec = records + (size_t)i * REC;
+        memset(rec, 0, REC * sizeof(float));
+        rec[2] = -1.0f;
+        const float* p = positions + (size_t)i * 4;
+        const uint32_t* w4 = shape + (size_t)i * 4;
+        const float s[3] = {athx_expf(ref_half(w4[1])), athx_expf(ref_half(w4[1] >> 16)), athx_expf(ref_half(w4[2]))};
+        const float* m = v->m;
+        const float vx = ((m[0] * p[0] + m[1] * p[1]) + m[2] * p[2]) + m[3];
+        const float vy = ((m[4] * p[0] + m[5] * p[1]) + m[6] * p[2]) + m[7];
+        const float vz = ((m[8] * p[0] + m[9] * p[1]) + m[10] * p[2]) + m[11];
+        if (!(vz > v->near_z) || vz > v->far_z) {
+            continue;
+        }
+        const uint32_t largest = w4[0] & 3u;
+        float small[3];
+        small[0] = ((float)((w4[0] >> 22) & 1023u) / 1023.0f * 2.0f - 1.0f) * 0.70710678f;
+        small[1] = ((float)((w4[0] >> 12) & 1023u) / 1023.0f * 2.0f - 1.0f) * 0.70710678f;
+        small[2] = ((float)((w4[0] >> 2) & 1023u) / 1023.0f * 2.0f - 1.0f) * 0.70710678f;
+        const float rest_q = sqrtf(fmaxf(0.0f, 1.0f - ((small[0] * small[0] + small[1] * small[1]) + small[2] * small[2])));
+        float q[4];
+        for (uint32_t k = 0, next = 0; k < 4; ++k) {
+            q[k] = k == largest ? rest_q : small[next < 3 ? next++ : 2];
+        }
+        const float x = q[0], y = q[1], z = q[2], w = q[3];
+        const float R[3][3] = {{1.0f - 2.0f * (y * y + z * z), 2.0f * (x * y - w * z), 2.0f * (x * z + w * y)},
+                               {2.0f * (x * y + w * z), 1.0f - 2.0f * (x * x + z * z), 2.0f * (y * z - w * x)},
+                               {2.0f * (x * z - w * y), 2.0f * (y * z + w * x), 1.0f - 2.0f * (x * x + y * y)}};
+        float M[3][3], C3[3][3], Wm[3][3], T1[3][3], CV[3][3];
+        for (int r = 0; r < 3; ++r) {
+            for (int c = 0; c < 3; ++c) {
+                M[r][c] = R[r][c] * s[c];
+                Wm[r][c] = m[4 * r + c];
+            }
+        }
+        for (int r = 0; r < 3; ++r) {
+            for (int c = 0; c < 3; ++c) {
+                C3[r][c] = (M[r][0] * M[c][0] + M[r][1] * M[c][1]) + M[r][2] * M[c][2];
+            }
+        }
+        for (int r = 0; r < 3; ++r) {
+            for (int c = 0; c < 3; ++c) {
+                T1[r][c] = (Wm[r][0] * C3[0][c] + Wm[r][1] * C3[1][c]) + Wm[r][2] * C3[2][c];
+            }
+        }
+        for (int r = 0; r < 3; ++r) {
+            for (int c = 0; c < 3; ++c) {
+                CV[r][c] = (T1[r][0] * Wm[c][0] + T1[r][1] * Wm[c][1]) + T1[r][2] * Wm[c][2];
+            }
+        }
+        float cx, cy, J[2][3] = {{0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}};
+        if (v->orthographic) {
+            cx = v->focal_x * vx + v->centre_x;
+            cy = v->focal_y * vy + v->centre_y;
+            J[0][0] = v->focal_x;
+            J[1][1] = v->focal_y;
+        } else {
+            cx = v->focal_x * vx / vz + v->centre_x;
+            cy = v->focal_y * vy / vz + v->centre_y;
+            const float lox = (-0.15f * (float)v->width - v->centre_x) / v->focal_x;
+            const float hix = (1.15f * (float)v->width - v->centre_x) / v->focal_x;
+            const float loy = (-0.15f * (float)v->height - v->centre_y) / v->focal_y;
+            const float hiy = (1.15f * (float)v->height - v->centre_y) / v->focal_y;
+            const float tx = fminf(fmaxf(vx / vz, lox), hix) * vz;
+            const float ty = fminf(fmaxf(vy / vz, loy), hiy) * vz;
+            J[0][0] = v->focal_x / vz;
+            J[0][2] = -v->focal_x * tx / (vz * vz);
+            J[1][1] = v->focal_y / vz;
+            J[1][2] = -v->focal_y * ty / (vz * vz);
+        }
+        float X[3][2];
+        for (int k = 0; k < 3; ++k) {
+            for (int c = 0; c < 2; ++c) {
+                X[k][c] = (CV[k][0] * J[c][0] + CV[k][1] * J[c][1]) + CV[k][2] * J[c][2];
+            }
+        }
+        const float c00 = (J[0][0] * X[0][0] + J[0][1] * X[1][0]) + J[0][2] * X[2][0];
+        const float c01 = (J[0][0] * X[0][1] + J[0][1] * X[1][1]) + J[0][2] * X[2][1];
+        const float c11 = (J[1][0] * X[0][1] + J[1][1] * X[1][1]) + J[1][2] * X[2][1];
+        const float before = c00 * c11 - c01 * c01;
+        const float ca = c00 + 0.3f, cb = c01, cc = c11 + 0.3f;
+        const float det = ca * cc - cb * cb;
+        if (!(det > 1e-12f)) {
+            continue;
+        }
+        const float filtered = sqrtf(fmaxf(before, 0.0f) / det);
+        const float paid = v->antialias ? filtered : 1.0f;
+        const float spread = paid * p[3];
+        const float alpha = spread >= 1.0f / 255.0f ? spread + (1.0f - paid) * (1.0f / 255.0f) : spread;
+        if (!(alpha >= 1.0f / 255.0f)) {
+            continue;
+        }
+        const float cutoff = fminf(2.0f * athx_logf(fmaxf(255.0f * alpha, 1.0001f)), 9.0f);
+        const float dx = p[0] - v->eye[0], dy = p[1] - v->eye[1], dz = p[2] - v->eye[2];
+        const float len = sqrtf((dx * dx + dy * dy) + dz * dz);
+        const float ux = dx / len, uy = dy / len, uz = dz / len;
+        const float xx = ux * ux, yy = uy * uy, zz = uz * uz, xy = ux * uy, yz = uy * uz, xz = ux * uz;
+        const float c1 = 0.4886025119029199f;
+        float Y[16];
+        Y[0] = 0.0f;
+        Y[1] = -c1 * uy;
+        Y[2] = c1 * uz;
+        Y[3] = -c1 * ux;
+        Y[4] = 1.0925484305920792f * xy;
+        Y[5] = -1.0925484305920792f * yz;
+        Y[6] = 0.31539156525252005f * (2.0f * zz - xx - yy);
+        Y[7] = -1.0925484305920792f * xz;
+        Y[8] = 0.5462742152960396f * (xx - yy);
+        Y[9] = -0.5900435899266435f * uy * (3.0f * xx - yy);
+        Y[10] = 2.890611442640554f * xy * uz;
+        Y[11] = -0.4570457994644658f * uy * (4.0f * zz - xx - yy);
+        Y[12] = 0.3731763325901154f * uz * (2.0f * zz - 3.0f * xx - 3.0f * yy);
+        Y[13] = -0.4570457994644658f * ux * (4.0f * zz - xx - yy);
+        Y[14] = 1.445305721320277f * uz * (xx - yy);
+        Y[15] = -0.5900435899266435f * ux * (xx - 3.0f * yy);
+        const uint32_t rest_degree = rest >= 15 ? 3u : rest >= 8 ? 2u : rest >= 3 ? 1u : 0u;
+        const uint32_t degree = rest_degree < v->sh_limit ? rest_degree : v->sh_limit;
+        const uint32_t terms = (degree + 1) * (degree + 1);
+        const float base[3] = {ref_half(w4[2] >> 16), ref_half(w4[3]), ref_half(w4[3] >> 16)};
+        float col[3];
+        for (uint32_t ch = 0; ch < 3; ++ch) {
+            float val = base[ch];
+            for (uint32_t k = 1; k < terms; ++k) {
+                const uint32_t h = (k - 1) * 3 + ch;
+                const uint32_t word = sh[(size_t)i * words + (h >> 1)];
+                val = val + Y[k] * ref_half((h & 1u) ? word >> 16 : word);
+            }
+            col[ch] = v->linear ? fmaxf(val, 0.0f) : val;
+        }
+        if (!v->linear) {
+            for (uint32_t ch = 0; ch < 3; ++ch) {
+                const float c = fmaxf(col[ch], 0.0f);
+                col[ch] = c <= 0.04045f ? c / 12.92f : athx_powf_pos((c + 0.055f) / 1.055f, 2.4f);
+            }
+        }
+        rec[0] = cx;
+        rec[1] = cy;
+        rec[2] = vz;
+        rec[3] = cutoff;
+        rec[4] = cc / det;
+        rec[5] = -cb / det;
+        rec[6] = ca / det;
+        rec[7] = alpha;
+        rec[8] = col[0];
+        rec[9] = col[1];
+        rec[10] = col[2];
+    }
+}
+
+static uint32_t ref_code(float linear) {
+    const float c = fminf(fmaxf(linear, 0.0f), 1.0f);
+    const float e = c <= 0.0031308f ? c * 12.92f : 1.055f * athx_powf_pos(c, 1.0f / 2.4f) - 0.055f;
+    return (uint32_t)rintf(e * 255.0f);
+}
+
+/* 256 bins of the per-pixel worst code difference (rgb and alpha). */
+ORC_EXPORT void orc_compare_codes(const float* a, const float* b, uint32_t width, uint32_t height, uint32_t* bins) {
+    memset(bins, 0, 256 * sizeof(uint32_t));
+    for (uint64_t p = 0; p < (uint64_t)width * height; ++p) {
+        const float* pa = a + p * 4;
+        const float* pb = b + p * 4;
+        uint32_t worst = 0;
+        for (int c = 0; c < 3; ++c) {
+            const int d = (int)ref_code(pa[c]) - (int)ref_code(pb[c]);
+            const uint32_t ad = (uint32_t)(d < 0 ? -d : d);
+            worst = ad > worst ? ad : worst;
+        }
+        const int da = (int)rintf(fminf(fmaxf(pa[3], 0.0f), 1.0f) * 255.0f) -
+                       (int)rintf(fminf(fmaxf(pb[3], 0.0f), 1.0f) * 255.0f);
+        const uint32_t ada = (uint32_t)(da < 0 ? -da : da);
+        worst = ada > worst ? ada : worst;
+        bins[worst] += 1;
+    }
+}
+
+static uint32_t ref_hdr_bin(float v) {
+    if (!(v > 0.0f)) {
+        return 0u;
+    }
+    if (!(v < 3.0e38f)) {
+        return 255u;
+    }
+    int e = 0;
+    const float m = frexpf(v, &e) * 2.0f;
+    static const float steps[7] = {1.0905077f, 1.1892071f, 1.2968396f, 1.4142135f, 1.5422108f, 1.6817929f,
+                                   1.8340081f};
+    int frac = 0;
+    for (int j = 0; j < 7; ++j) {
+        frac += m >= steps[j] ? 1 : 0;
+    }
+    const int bin = 8 * ((e - 1) + 16) + frac + 1;
+    return (uint32_t)(bin < 1 ? 1 : bin > 255 ? 255 : bin);
+}
+
+/* 256 bins of the per-pixel worst relative difference and relMSE's sum:
+ * Kahan along each row, the rows' sums added with Kahan in row order. */
+ORC_EXPORT float orc_compare_hdr(const float* a, const float* b, uint32_t width, uint32_t height, uint32_t* bins) {
+    memset(bins, 0, 256 * sizeof(uint32_t));
+    float total = 0.0f, total_carry = 0.0f;
+    for (uint32_t y = 0; y < height; ++y) {
+        float sum = 0.0f, carry = 0.0f;
+        for (uint32_t x = 0; x < width; ++x) {
+            const float* pa = a + ((uint64_t)y * width + x) * 4;
+            const float* pb = b + ((uint64_t)y * width + x) * 4;
+            float worst = 0.0f, squared = 0.0f;
+            for (int c = 0; c < 4; ++c) {
+                const float d = fabsf(pa[c] - pb[c]);
+                worst = fmaxf(worst, d / fmaxf(fabsf(pb[c]), 1e-3f));
+                squared = squared + d * d / (pb[c] * pb[c] + 1e-2f);
+            }
+            bins[ref_hdr_bin(worst)] += 1;
+            const float term = squared * 0.25f;
+            const float t = sum + term;
+            carry = carry + (fabsf(sum) >= fabsf(term) ? (sum - t) + term : (term - t) + sum);
+            sum = t;
+        }
+        const float row = sum + carry;
+        const float t = total + row;
+        total_carry = total_carry + (fabsf(total) >= fabsf(row) ? (total - t) + row : (row - t) + total);
+        total = t;
+    }
+    return total + total_carry;
+}

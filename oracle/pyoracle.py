@@ -180,6 +180,11 @@ def load():
         lib.orc_ingest.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, _u32p, _u32p, ctypes.c_uint32,
                                    ctypes.c_uint32, _f32p, _u32p, ctypes.c_void_p, _u32p]
         lib.orc_ingest.restype = ctypes.c_uint32
+        lib.orc_ref_project.argtypes = [vp, _f32p, _u32p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32,
+                                        _f32p]
+        lib.orc_compare_codes.argtypes = [_f32p, _f32p, ctypes.c_uint32, ctypes.c_uint32, _u32p]
+        lib.orc_compare_hdr.argtypes = [_f32p, _f32p, ctypes.c_uint32, ctypes.c_uint32, _u32p]
+        lib.orc_compare_hdr.restype = ctypes.c_float
         lib.orc_half_bits.argtypes = [ctypes.c_float]
         lib.orc_half_bits.restype = ctypes.c_uint32
         lib.orc_encode_quaternion.argtypes = [ctypes.c_float] * 4
@@ -547,3 +552,28 @@ def half_bits(v: float) -> int:
 
 def encode_quaternion(x: float, y: float, z: float, w: float) -> int:
     return int(load().orc_encode_quaternion(x, y, z, w))
+
+
+def ref_project(v: OrcView, packed) -> np.ndarray:
+    """orc_ref_project: the ground truth's own projection of a packed cloud
+    (host arrays), records [n, 12] (z = -1: culled)."""
+    n = packed.positions.shape[0]
+    rec = np.zeros((max(n, 1), 12), np.float32)
+    sh = np.ascontiguousarray(packed.sh, np.uint32) if packed.sh is not None else None
+    load().orc_ref_project(ctypes.byref(v), _c(packed.positions, np.float32), _c(packed.shape, np.uint32),
+                           sh.ctypes.data if sh is not None else None, n, packed.rest_per_colour, _c(rec, np.float32))
+    return rec[:n]
+
+
+def compare_codes(a: np.ndarray, b: np.ndarray) -> np.ndarray:
+    h, w = a.shape[:2]
+    bins = np.zeros(256, np.uint32)
+    load().orc_compare_codes(_c(a, np.float32), _c(b, np.float32), w, h, _c(bins, np.uint32))
+    return bins
+
+
+def compare_hdr(a: np.ndarray, b: np.ndarray) -> tuple[np.ndarray, float]:
+    h, w = a.shape[:2]
+    bins = np.zeros(256, np.uint32)
+    total = load().orc_compare_hdr(_c(a, np.float32), _c(b, np.float32), w, h, _c(bins, np.uint32))
+    return bins, float(np.float32(total))

@@ -59,6 +59,8 @@ STRUCTS = {
     "athx_frame_info": _lib.FrameInfo,
     "athx_ply_layout": _lib.PlyLayout,
     "athx_ingest_out": _lib.IngestOut,
+    "athx_image_difference": _lib.ImageDifference,
+    "athx_hdr_difference": _lib.HdrDifference,
 }
 
 
