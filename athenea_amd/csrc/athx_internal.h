@@ -369,6 +369,13 @@ hipError_t launch_compare_codes(const float* a, const float* b, uint32_t width, 
 hipError_t launch_compare_hdr(const float* a, const float* b, uint32_t width, uint32_t height, uint32_t* rows,
                               float* row_error, uint32_t* bins, float* total, hipStream_t s);
 
+// A sky prepared on the device (sky.hip): texels of the dome for this base
+// side (sides and offsets of the eight levels when not null), and the
+// launch that writes the texel words and the 16 float4 coefficients.
+uint32_t sky_texels(uint32_t base_side, uint32_t* sides, uint32_t* offsets);
+hipError_t launch_prepare_sky(const float* equirect, uint32_t width, uint32_t height, uint32_t base_side,
+                              uint32_t* texels, float* sh, hipStream_t s);
+
 // Each vertex's keep flag (1 or 0).
 hipError_t launch_ingest_flags(const uint8_t* vertices, const IngestFields& f, uint32_t* keep, hipStream_t s);
 // The kept vertices decoded and packed at dest[i] (the flags' exclusive scan).

@@ -1960,6 +1960,29 @@ ATHX_API athx_status athx_compare_hdr(athx_ctx* ctx, const float* a, const float
     return ATHX_OK;
 }
 
+ATHX_API uint32_t athx_sky_texels(uint32_t base_side) { return sky_texels(base_side, nullptr, nullptr); }
+
+ATHX_API athx_status athx_prepare_sky(athx_ctx* ctx, const float* equirect, uint32_t width, uint32_t height,
+                                      uint32_t base_side, uint32_t* texels, float* sh) {
+    if (!ctx) {
+        return ATHX_INVALID_ARGUMENT;
+    }
+    ctx->error.clear();
+    if (!equirect || !texels || !sh) {
+        return fail(ctx, ATHX_INVALID_ARGUMENT, "null equirect, texels or sh");
+    }
+    if (width == 0 || height == 0 || (uint64_t)width * height > (1u << 26)) {
+        return fail(ctx, ATHX_INVALID_ARGUMENT, "equirect size out of range");
+    }
+    if (base_side == 0 || base_side > 8192) {
+        return fail(ctx, ATHX_INVALID_ARGUMENT, "base_side out of range");
+    }
+    ATHX_HIP(ctx, hipSetDevice(ctx->device), "set device");
+    ATHX_HIP(ctx, launch_prepare_sky(equirect, width, height, base_side, texels, sh, nullptr), "prepare sky");
+    ATHX_HIP(ctx, hipStreamSynchronize(nullptr), "prepare sky wait");
+    return ATHX_OK;
+}
+
 ATHX_API athx_status athx_reference_render(athx_ctx* ctx, const athx_view* view, const athx_gaussians* g,
                                            const athx_settings* settings, const athx_targets* targets,
                                            float* records, uint32_t* overflow) {

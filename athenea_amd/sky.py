@@ -5,10 +5,12 @@ radiance coefficients in real spherical harmonics, and an optional sun.
 
 The reference prepares this on the GPU (technique::Environment,
 modules/technique/include/athenea/technique/Environment.h, with
-env_project / env_prefilter / env_sun .slang); here the preparation is a
-direct numpy quadrature over the equirect's texels, which the relit colour
-only reads -- the GPU path and the oracle are fed the same buffers, so
-their agreement does not depend on how the sky was prepared.
+env_project / env_prefilter / env_sun .slang); so does
+prepare_sky_device (athx_prepare_sky: FP64 sums over the equirect's
+texels, one rounding to f16 a texel). prepare_sky is the same quadrature in
+numpy on the host: what the device preparation is checked against, and the
+way to make a sky without a device. The relit colour only reads the
+buffers, so the GPU frame and the oracle are fed the same ones.
 
 Conventions: +y is up; an equirect texel (i, j) of a W x H map looks along
 theta = pi (j + 0.5) / H from +y and phi = 2 pi (i + 0.5) / W from +x
@@ -261,3 +263,40 @@ def relight_attributes(n: int, sky: PreparedSky, seed: int = 0x5EED0001, with_al
     normals = (normals / np.linalg.norm(normals, axis=1, keepdims=True)).astype(np.float32)
     return RelightAttributes(metallic, roughness, sky, normals if with_normals else None,
                              albedo if with_albedo else None)
+
+
+def prepare_sky_device(equirect, base_side: int = COARSE_SIDE, sun: tuple | None = None, raster=None,
+                       device="cuda:0") -> DeviceSky:
+    """prepare_sky on the device (athx_prepare_sky): an equirect [H, W, 3]
+    (numpy or a device tensor) into a DeviceSky; `sun` as prepare_sky's."""
+    import ctypes
+
+    import torch
+
+    from . import _lib
+    from .raster import TileRasterizer
+
+    img = torch.as_tensor(np.asarray(equirect, np.float32) if not torch.is_tensor(equirect) else equirect,
+                          dtype=torch.float32).to(device).contiguous()
+    if img.dim() != 3 or img.shape[2] != 3:
+        raise ValueError("the equirect must be [height, width, 3]")
+    lib = _lib.load()
+    count = int(lib.athx_sky_texels(int(base_side)))
+    texels = torch.empty((count * 2,), dtype=torch.int32, device=img.device)
+    sh = torch.zeros((COEFFICIENTS, 4), dtype=torch.float32, device=img.device)
+    own = raster is None
+    raster = raster or TileRasterizer(img.device.index or 0)
+    try:
+        raster._check(lib.athx_prepare_sky(raster._ctx, img.data_ptr(), int(img.shape[1]), int(img.shape[0]),
+                                           int(base_side), texels.data_ptr(), sh.data_ptr()))
+    finally:
+        if own:
+            raster.close()
+    sun_words = np.zeros((2, 4), np.float32)
+    if sun is not None:
+        direction, size, irradiance = sun
+        dvec = np.asarray(direction, np.float64)
+        sun_words[0, :3] = dvec / np.linalg.norm(dvec)
+        sun_words[0, 3] = size
+        sun_words[1, :3] = irradiance
+    return DeviceSky(texels, sh, torch.from_numpy(sun_words).to(img.device), int(base_side))

@@ -511,6 +511,17 @@ ATHX_API athx_status athx_reference_render(athx_ctx* ctx, const athx_view* view,
                                            const athx_settings* settings, const athx_targets* targets,
                                            float* records, uint32_t* overflow);
 
+/*
+ * A sky prepared on the device (technique::Environment, modules/technique/
+ * include/athenea/technique/Environment.h:1-155), into athx_sky's layout:
+ * equirect, [height][width][3] linear radiance (device, FP32); texels,
+ * 2 words each of athx_sky_texels(base_side) texels; sh, 16 float4. The sun
+ * is the caller's to fill (athx_sky.sun). Waits for the result.
+ */
+ATHX_API uint32_t athx_sky_texels(uint32_t base_side);
+ATHX_API athx_status athx_prepare_sky(athx_ctx* ctx, const float* equirect, uint32_t width, uint32_t height,
+                                      uint32_t base_side, uint32_t* texels, float* sh);
+
 /* Two renders compared as 8-bit sRGB code values (ReferenceRenderer.h:52-58):
    each pixel's largest channel difference, alpha included. */
 typedef struct athx_image_difference {

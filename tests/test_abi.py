@@ -111,3 +111,11 @@ def test_calls_fail_cleanly_without_a_device():
     assert lib.athx_render(None, None, 0, None, None, None, None, None) == 3  # InvalidArgument
     assert lib.athx_last_error(None) == b"no context"
     lib.athx_destroy(None)
+
+
+def test_sky_texel_count_is_the_layouts():
+    """athx_sky_texels (host only) is environment.slang:66-92's dome size."""
+    from athenea_amd import sky
+    lib = _lib.load()
+    for base in (16, 100, 256, 300, 512, 1024):
+        assert lib.athx_sky_texels(base) == sky.texels_per_dome(base)
