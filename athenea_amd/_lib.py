@@ -307,6 +307,11 @@ SIGNATURES = {
         ctypes.c_int,
         [ctypes.c_char_p, ctypes.c_uint64, ctypes.POINTER(PlyLayout), ctypes.c_char_p, ctypes.c_uint64],
     ),
+    "athx_contribution": (
+        ctypes.c_int,
+        [ctypes.c_void_p, ctypes.POINTER(View), ctypes.c_uint32, ctypes.POINTER(Gaussians), ctypes.POINTER(Settings),
+         ctypes.POINTER(Targets), ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
+    ),
     "athx_sky_texels": (ctypes.c_uint32, [ctypes.c_uint32]),
     "athx_prepare_sky": (
         ctypes.c_int,

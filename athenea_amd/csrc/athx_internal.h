@@ -277,6 +277,14 @@ hipError_t launch_blend(const DView* views_dev, const uint32_t* view_of_tile_bas
                         uint32_t total_tiles, uint32_t tile, uint32_t batch, const uint32_t* ranges,
                         const uint32_t* pair_slots, const float* records, uint32_t depth_mode,
                         float depth_threshold, hipStream_t s);
+// What each splat gave the last frame's walk (the frame's ranges, pair
+// slots and records): mode 1 the blend weight in 1/4096, one word a slot;
+// mode 4 the colour gradient against each view's dl_dcolour in 1/256 (three
+// signed words) and the weight in 1/256, four words a slot. Adds into out.
+hipError_t launch_contribution(const DView* views_dev, const uint32_t* tile_bases, uint32_t nviews,
+                               uint32_t total_tiles, uint32_t tile, uint32_t batch, const uint32_t* ranges,
+                               const uint32_t* pair_slots, const float* records, uint32_t mode, uint32_t* out,
+                               hipStream_t s);
 hipError_t launch_blend_backward(const DView* views_dev, const uint32_t* tile_bases, uint32_t nviews,
                                  uint32_t total_tiles, uint32_t tile, uint32_t batch, const uint32_t* ranges,
                                  const uint32_t* pair_slots, const float* records, float* grad2d, hipStream_t s);

@@ -146,6 +146,94 @@ __global__ void __launch_bounds__(TILE * TILE) blend_kernel(const DView* __restr
     }
 }
 
+// What each splat gave the last frame (splat_blend.slang:36-48, 364-398):
+// the same walk as blend_kernel -- the same pairs, records, arithmetic and
+// early stop -- counting at each pair's slot, with integer atomics (so in
+// any order the same sum): mode 1 the blend weight T alpha in 1/4096 of a
+// pixel, rounded; mode 4 NeuralFit's colour gradient, the weight times
+// dL/dC (the view's dl_dcolour) in 1/256, each channel rounded down or up by
+// color.slang's hash01 of (pixel, channel, slot) so that steps below one are
+// kept on average, and the weight itself in 1/256 as a fourth word.
+__device__ inline float hash01(uint32_t v) {
+    const uint32_t state = v * 747796405u + 2891336453u;
+    const uint32_t word = ((state >> ((state >> 28u) + 4u)) ^ state) * 277803737u;
+    return (float)((word >> 22u) ^ word) / 4294967295.0f;
+}
+
+template <uint32_t TILE, uint32_t MODE>
+__global__ void __launch_bounds__(TILE * TILE) contribution_kernel(const DView* __restrict__ views,
+                                                                   const uint32_t* __restrict__ tile_bases,
+                                                                   uint32_t nviews, uint32_t batch,
+                                                                   const uint32_t* __restrict__ ranges,
+                                                                   const uint32_t* __restrict__ pair_slots,
+                                                                   const float* __restrict__ records,
+                                                                   uint32_t* __restrict__ contribution) {
+    constexpr uint32_t kThreads = TILE * TILE;
+    __shared__ LdsBatch s;
+    __shared__ uint32_t s_slot[kMaxBatch];
+    const uint32_t gtile = blockIdx.x;
+    const uint32_t v = view_of_tile(tile_bases, nviews, gtile);
+    const DView& V = views[v];
+    const uint32_t local = gtile - V.tile_base;
+    const uint32_t tx = local % V.tiles_x, ty = local / V.tiles_x;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t px = tx * TILE + tid % TILE, py = ty * TILE + tid / TILE;
+    const bool inside = px < V.width && py < V.height;
+    const uint32_t start = ranges[2 * gtile], end = ranges[2 * gtile + 1];
+    const float pxc = (float)px + 0.5f, pyc = (float)py + 0.5f;
+    const uint32_t at = inside ? py * V.width + px : 0u;
+    float4 dl = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (MODE == 4 && inside) {
+        dl = reinterpret_cast<const float4*>(V.dl_dcolour)[at];
+    }
+    float T = 1.0f;
+    bool done = !inside;
+    for (uint32_t first = start; first < end; first += batch) {
+        if (!__syncthreads_or(!done)) {
+            break;
+        }
+        const uint32_t count = end - first < batch ? end - first : batch;
+        for (uint32_t k = tid; k < count; k += kThreads) {
+            const uint32_t slot = pair_slots[first + k];
+            stage(s, k, records, slot);
+            s_slot[k] = slot;
+        }
+        __syncthreads();
+        for (uint32_t j = 0; j < count && !done; ++j) {
+            const float4 q = s.xy_cut_op[j];
+            const float4 k = s.conic[j];
+            const float dx = pxc - q.x, dy = pyc - q.y;
+            const float power = k.x * dx * dx + k.y * dx * dy + k.z * dy * dy;
+            if (power > q.z) {
+                continue;
+            }
+            const float alpha = fminf(0.99f, q.w * blend_exp(power));
+            if (alpha < 1.0f / 255.0f) {
+                continue;
+            }
+            const float w = T * alpha;
+            const uint32_t slot = s_slot[j];
+            if (MODE == 1) {
+                atomicAdd(&contribution[slot], (uint32_t)(w * 4096.0f + 0.5f));
+            } else {
+                const float g[3] = {w * dl.x * 256.0f, w * dl.y * 256.0f, w * dl.z * 256.0f};
+                for (uint32_t c = 0; c < 3; ++c) {
+                    const int step = (int)floorf(g[c] + hash01((at * 3u + c) ^ (slot * 2654435761u)));
+                    if (step != 0) {
+                        atomicAdd(&contribution[(size_t)slot * 4 + c], (uint32_t)step);
+                    }
+                }
+                atomicAdd(&contribution[(size_t)slot * 4 + 3], (uint32_t)(w * 256.0f + 0.5f));
+            }
+            T = T * (1.0f - alpha);
+            if (T < 1e-4f) {
+                done = true;
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // The sum over a wavefront's 64 lanes, every lane active: DPP moves within
 // each row of 16 lanes (quad swaps, then rotations by 4 and 8) leave each row's
 // sum in all of its lanes, and four lane reads add the rows. All VALU, no LDS.
@@ -384,6 +472,45 @@ hipError_t launch_blend(const DView* views_dev, const uint32_t* tile_bases, uint
     case 32:
         return blend_for<32>(views_dev, tile_bases, nviews, total_tiles, batch, ranges, pair_slots, records,
                              depth_mode, depth_threshold, s);
+    default:
+        return hipErrorInvalidValue;
+    }
+}
+
+template <uint32_t TILE>
+static hipError_t contribution_for(const DView* views, const uint32_t* tile_bases, uint32_t nviews, uint32_t tiles,
+                            uint32_t batch, const uint32_t* ranges, const uint32_t* slots, const float* records,
+                            uint32_t mode, uint32_t* out, hipStream_t s) {
+    if (mode == 1) {
+        hipLaunchKernelGGL((contribution_kernel<TILE, 1>), dim3(tiles), dim3(TILE * TILE), 0, s, views, tile_bases,
+                           nviews, batch, ranges, slots, records, out);
+    } else {
+        hipLaunchKernelGGL((contribution_kernel<TILE, 4>), dim3(tiles), dim3(TILE * TILE), 0, s, views, tile_bases,
+                           nviews, batch, ranges, slots, records, out);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_contribution(const DView* views_dev, const uint32_t* tile_bases, uint32_t nviews,
+                               uint32_t total_tiles, uint32_t tile, uint32_t batch, const uint32_t* ranges,
+                               const uint32_t* pair_slots, const float* records, uint32_t mode, uint32_t* out,
+                               hipStream_t s) {
+    if (total_tiles == 0) {
+        return hipSuccess;
+    }
+    if (batch == 0 || batch > kMaxBatch || (mode != 1 && mode != 4)) {
+        return hipErrorInvalidValue;
+    }
+    switch (tile) {
+    case 8:
+        return contribution_for<8>(views_dev, tile_bases, nviews, total_tiles, batch, ranges, pair_slots, records,
+                                   mode, out, s);
+    case 16:
+        return contribution_for<16>(views_dev, tile_bases, nviews, total_tiles, batch, ranges, pair_slots, records,
+                                    mode, out, s);
+    case 32:
+        return contribution_for<32>(views_dev, tile_bases, nviews, total_tiles, batch, ranges, pair_slots, records,
+                                    mode, out, s);
     default:
         return hipErrorInvalidValue;
     }

@@ -1141,6 +1141,56 @@ ATHX_API athx_status athx_render_backward(athx_ctx* ctx, const athx_view* views,
     return fail(ctx, ATHX_INTERNAL_ERROR, e.what());
 }
 
+ATHX_API athx_status athx_contribution(athx_ctx* ctx, const athx_view* views, uint32_t nviews,
+                                      const athx_gaussians* gaussians, const athx_settings* settings,
+                                      const athx_targets* targets, uint32_t mode, const float* const* dl_dcolour,
+                                      uint32_t* out, void* stream) {
+    if (!ctx) {
+        return ATHX_INVALID_ARGUMENT;
+    }
+    ctx->error.clear();
+    if (mode != 1 && mode != 4) {
+        return fail(ctx, ATHX_UNSUPPORTED, "contribution modes 1 (weight) and 4 (colour gradient) are built");
+    }
+    if (!views || nviews == 0 || !targets || !out || (mode == 4 && !dl_dcolour)) {
+        return fail(ctx, ATHX_INVALID_ARGUMENT, "no views, targets or output (or no dL/dcolour for mode 4)");
+    }
+    if (athx_status st = check_settings(ctx, settings); st != ATHX_OK) {
+        return st;
+    }
+    if (!gaussians || !ctx->have_frame || ctx->last_n != gaussians->count || ctx->last_nviews != nviews ||
+        ctx->last_tile != settings->tile) {
+        return fail(ctx, ATHX_INVALID_ARGUMENT, "contributions must follow a render of the same frame");
+    }
+    if (mode == 4) {
+        for (uint32_t k = 0; k < nviews; ++k) {
+            if (!dl_dcolour[k]) {
+                return fail(ctx, ATHX_INVALID_ARGUMENT, "view " + std::to_string(k) + ": no dL/dcolour");
+            }
+        }
+    }
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    ATHX_HIP(ctx, hipSetDevice(ctx->device), "set device");
+    uint64_t tiles64 = 0;
+    if (athx_status st = stage_views(ctx, views, nviews, settings, targets, mode == 4 ? dl_dcolour : nullptr, tiles64);
+        st != ATHX_OK) {
+        return st;
+    }
+    if (tiles64 != ctx->last_tiles) {
+        return fail(ctx, ATHX_INVALID_ARGUMENT, "the views differ from the rendered frame's");
+    }
+    if (athx_status st = upload_views(ctx, nviews, s); st != ATHX_OK) {
+        return st;
+    }
+    const uint64_t words = (uint64_t)gaussians->count * nviews * (mode == 4 ? 4u : 1u);
+    ATHX_HIP(ctx, launch_fill_u32(out, 0u, words, s), "clear contributions");
+    ATHX_HIP(ctx, launch_contribution(as<DView>(ctx->views), as<uint32_t>(ctx->tile_bases), nviews, ctx->last_tiles,
+                                      settings->tile, settings->batch, as<uint32_t>(ctx->ranges),
+                                      as<uint32_t>(ctx->pair_vals), as<float>(ctx->records), mode, out, s),
+             "contribution");
+    return ATHX_OK;
+}
+
 ATHX_API athx_status athx_latest_counters(athx_ctx* ctx, athx_counters* out) {
     if (!ctx || !out) {
         return ATHX_INVALID_ARGUMENT;

@@ -448,6 +448,32 @@ class TileRasterizer:
         self._check(self._lib.athx_frame_pairs(self._ctx, ctypes.byref(pairs), ctypes.byref(cap)))
         return int(pairs.value), int(cap.value)
 
+    def contribution(self, views: list, cloud, settings: RenderSettings | None, targets: list[RenderTargets],
+                     mode: int = 1, dl_dcolour: list[torch.Tensor] | None = None) -> torch.Tensor:
+        """athx_contribution for the frame the last ``render`` drew: mode 1 an
+        int32 [slots] of blend weights in 1/4096, mode 4 an int32 [slots, 4]
+        of the colour gradient against ``dl_dcolour`` and the weight, in
+        1/256 (slot = view * count + gaussian)."""
+        settings = settings or RenderSettings()
+        dev = _device_of(cloud)
+        c_views, c_targets = _views_and_targets(views, targets)
+        nv = len(views)
+        slots = cloud.count * nv
+        out = torch.empty((slots, 4) if mode == 4 else (slots,), dtype=torch.int32, device=dev)
+        dl = None
+        keep = []
+        if dl_dcolour is not None:
+            dl = (ctypes.c_void_p * nv)()
+            for k, t in enumerate(dl_dcolour):
+                t = t.contiguous()
+                keep.append(t)
+                dl[k] = t.data_ptr()
+        g = _gaussians_c(cloud)
+        s = settings.to_c(cloud.linear)
+        self._check(self._lib.athx_contribution(self._ctx, c_views, nv, ctypes.byref(g), ctypes.byref(s), c_targets,
+                                                int(mode), dl, out.data_ptr(), _stream_handle(dev)))
+        return out
+
     def render_backward(
         self,
         views: list,

@@ -24,6 +24,8 @@
  *       no counterpart in the reference (its only gradient is the blend's
  *       contributionMode 4, shaders/athenea/splat/splat_blend.slang:381-398):
  *       the adjoint of athx_render's colour with respect to every parameter.
+ *   athx_contribution
+ *       RenderSettings::contributionMode 1 and 4 (splat_blend.slang:364-398).
  *   athx_latest_counters
  *       TileRasterizer::latestCounters (TileRasterizer.h:481) and
  *       SplatCounters (TileRasterizer.h:370-401).
@@ -349,6 +351,25 @@ ATHX_API athx_status athx_render_backward(athx_ctx* ctx, const athx_view* views,
                                           const athx_gaussians* gaussians, const athx_settings* settings,
                                           const athx_targets* targets, const float* const* dl_dcolour,
                                           const athx_grads* grads, void* stream);
+
+/*
+ * What each splat gave the last frame (splat_blend.slang:36-48, 364-398:
+ * RenderSettings::contributionMode), taken by walking that frame's pairs
+ * again with the same arithmetic and early stop. out (device) is cleared
+ * and then holds, at each slot (view * count + gaussian):
+ *   mode 1: one word, the sum over its pixels of round(T alpha * 4096);
+ *   mode 4: four words (NeuralFit's colour gradient, NeuralFit.cpp:226-228):
+ *           for each of r, g, b the sum of floor(T alpha dL/dC * 256 + h),
+ *           h = hash01((pixel * 3 + channel) ^ (slot * 2654435761)) of
+ *           color.slang:43-47 (signed, two's complement), then the sum of
+ *           round(T alpha * 256). dl_dcolour[k]: view k's [height][width][4].
+ * Modes 2 and 3 (removal error, lifted masks) are not built. The views,
+ * gaussians, settings and targets are the last render's. Queued.
+ */
+ATHX_API athx_status athx_contribution(athx_ctx* ctx, const athx_view* views, uint32_t nviews,
+                                       const athx_gaussians* gaussians, const athx_settings* settings,
+                                       const athx_targets* targets, uint32_t mode, const float* const* dl_dcolour,
+                                       uint32_t* out, void* stream);
 
 /* The counters of the newest frame rendered with count_splats (waits for it). */
 ATHX_API athx_status athx_latest_counters(athx_ctx* ctx, athx_counters* out);

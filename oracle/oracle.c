@@ -30,6 +30,8 @@
  *   orc_compare_codes / orc_compare_hdr
  *                    reference/image_compare.slang, image_compare_hdr.slang:
  *                    the per-row histograms and relMSE rows, summed
+ *   orc_contribution splat_blend.slang:364-398's contribution modes 1 and 4
+ *                    over the same per-pixel walk as orc_blend
  *   orc_ingest       a PLY vertex block widened, validated, decoded and
  *                    packed: packed_records.slang:1-110, splat_validate.slang,
  *                    splat_decode.slang:1-173 (PLY's encodings),
@@ -1929,4 +1931,51 @@ ORC_EXPORT float orc_compare_hdr(const float* a, const float* b, uint32_t width,
         total = t;
     }
     return total + total_carry;
+}
+
+
+/* ---- contributions: what each slot gave the frame (modes 1 and 4) ---- */
+
+static float contribution_hash01(uint32_t v) {
+    const uint32_t state = v * 747796405u + 2891336453u;
+    const uint32_t word = ((state >> ((state >> 28u) + 4u)) ^ state) * 277803737u;
+    return (float)((word >> 22u) ^ word) / 4294967295.0f;
+}
+
+/* The walk of blend_pixel_f32 for every pixel of one view, adding at each
+ * contributing pair's slot: mode 1 round(w 4096); mode 4 floor(w dL/dC
+ * 256 + hash) a channel and round(w 256). dl: [h][w][4] (mode 4). */
+ORC_EXPORT void orc_contribution(const orc_view* v, const float* rec, const uint32_t* ranges, const uint32_t* vals,
+                                 uint32_t tile_offset, uint32_t mode, const float* dl, uint32_t* out) {
+    const uint32_t W = v->width, H = v->height, T = v->tile;
+    const uint32_t tiles_x = (W + T - 1) / T;
+    for (uint64_t p = 0; p < (uint64_t)W * H; ++p) {
+        const uint32_t px = (uint32_t)(p % W), py = (uint32_t)(p / W);
+        const uint32_t tile = tile_offset + (py / T) * tiles_x + px / T;
+        const float pxc = (float)px + 0.5f, pyc = (float)py + 0.5f;
+        float Tr = 1.0f;
+        for (uint32_t j = ranges[tile * 2]; j < ranges[tile * 2 + 1]; ++j) {
+            const uint32_t slot = vals[j];
+            const float* r = rec + (size_t)slot * REC;
+            const float dx = pxc - r[0], dy = pyc - r[1];
+            const float power = r[4] * dx * dx + 2.0f * r[5] * dx * dy + r[6] * dy * dy;
+            if (power > r[3]) continue;
+            const float alpha = minf_(0.99f, r[7] * athx_expf(-0.5f * power));
+            if (alpha < 1.0f / 255.0f) continue;
+            const float w = Tr * alpha;
+            if (mode == 1) {
+                out[slot] += (uint32_t)(w * 4096.0f + 0.5f);
+            } else {
+                const uint32_t at = (uint32_t)p;
+                for (uint32_t c = 0; c < 3; ++c) {
+                    const float g = w * dl[p * 4 + c] * 256.0f;
+                    const int step = (int)floorf(g + contribution_hash01((at * 3u + c) ^ (slot * 2654435761u)));
+                    out[(size_t)slot * 4 + c] += (uint32_t)step;
+                }
+                out[(size_t)slot * 4 + 3] += (uint32_t)(w * 256.0f + 0.5f);
+            }
+            Tr = Tr * (1.0f - alpha);
+            if (Tr < 1e-4f) break;
+        }
+    }
 }

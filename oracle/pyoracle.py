@@ -185,6 +185,8 @@ def load():
         lib.orc_compare_codes.argtypes = [_f32p, _f32p, ctypes.c_uint32, ctypes.c_uint32, _u32p]
         lib.orc_compare_hdr.argtypes = [_f32p, _f32p, ctypes.c_uint32, ctypes.c_uint32, _u32p]
         lib.orc_compare_hdr.restype = ctypes.c_float
+        lib.orc_contribution.argtypes = [vp, _f32p, _u32p, _u32p, ctypes.c_uint32, ctypes.c_uint32,
+                                         ctypes.c_void_p, _u32p]
         lib.orc_half_bits.argtypes = [ctypes.c_float]
         lib.orc_half_bits.restype = ctypes.c_uint32
         lib.orc_encode_quaternion.argtypes = [ctypes.c_float] * 4
@@ -577,3 +579,14 @@ def compare_hdr(a: np.ndarray, b: np.ndarray) -> tuple[np.ndarray, float]:
     bins = np.zeros(256, np.uint32)
     total = load().orc_compare_hdr(_c(a, np.float32), _c(b, np.float32), w, h, _c(bins, np.uint32))
     return bins, float(np.float32(total))
+
+
+def contribution(v: OrcView, frame: dict, slots: int, mode: int = 1, dl: np.ndarray | None = None) -> np.ndarray:
+    """orc_contribution over one view's frame (render's dict): uint32
+    [slots] (mode 1) or [slots, 4] (mode 4)."""
+    out = np.zeros(slots * (4 if mode == 4 else 1), np.uint32)
+    d = np.ascontiguousarray(dl, np.float32) if dl is not None else None
+    load().orc_contribution(ctypes.byref(v), _c(frame["records"], np.float32), _c(frame["ranges"], np.uint32),
+                            _c(frame["pair_values"], np.uint32), 0, mode, d.ctypes.data if d is not None else None,
+                            out)
+    return out.reshape(slots, 4) if mode == 4 else out
