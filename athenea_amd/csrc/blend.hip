@@ -182,11 +182,16 @@ __global__ void __launch_bounds__(TILE * TILE) contribution_kernel(const DView* 
     const uint32_t start = ranges[2 * gtile], end = ranges[2 * gtile + 1];
     const float pxc = (float)px + 0.5f, pyc = (float)py + 0.5f;
     const uint32_t at = inside ? py * V.width + px : 0u;
+    // The per-pixel input: mode 2 the pixel drawn whole, mode 3 the mask,
+    // mode 4 dL/dC.
     float4 dl = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    if (MODE == 4 && inside) {
+    if ((MODE == 2 || MODE == 4) && inside) {
         dl = reinterpret_cast<const float4*>(V.dl_dcolour)[at];
+    } else if (MODE == 3 && inside) {
+        dl.x = fminf(fmaxf(V.dl_dcolour[at], 0.0f), 1.0f);
     }
     float T = 1.0f;
+    float r = 0.0f, g = 0.0f, b = 0.0f;
     bool done = !inside;
     for (uint32_t first = start; first < end; first += batch) {
         if (!__syncthreads_or(!done)) {
@@ -215,6 +220,34 @@ __global__ void __launch_bounds__(TILE * TILE) contribution_kernel(const DView* 
             const uint32_t slot = s_slot[j];
             if (MODE == 1) {
                 atomicAdd(&contribution[slot], (uint32_t)(w * 4096.0f + 0.5f));
+            } else if (MODE == 2) {
+                // The pixel without this splat: what is behind it is
+                // (C - P) / T' with P the walk once it is in, T' the
+                // transmittance after it; the change is w (c - behind).
+                const float4 c = s.colour[j];
+                const float after = T * (1.0f - alpha);
+                const float keep = fmaxf(after, 1.0e-4f);
+                const float er = w * (c.x - (dl.x - (r + w * c.x)) / keep);
+                const float eg = w * (c.y - (dl.y - (g + w * c.y)) / keep);
+                const float eb = w * (c.z - (dl.z - (b + w * c.z)) / keep);
+                const float e2 = fminf(er * er + eg * eg + eb * eb, 1.0f);
+                const uint32_t add = (uint32_t)(e2 * 16777216.0f + 0.5f);
+                const uint32_t before = atomicAdd(&contribution[(size_t)slot * 2], add);
+                if (before + add < before) {
+                    atomicAdd(&contribution[(size_t)slot * 2 + 1], 1u);
+                }
+                r = r + w * c.x;
+                g = g + w * c.y;
+                b = b + w * c.z;
+            } else if (MODE == 3) {
+                const uint32_t seen = (uint32_t)(w * 1024.0f + 0.5f);
+                if (seen != 0u) {
+                    atomicAdd(&contribution[(size_t)slot * 2 + 1], seen);
+                    const uint32_t within = (uint32_t)(w * dl.x * 1024.0f + 0.5f);
+                    if (within != 0u) {
+                        atomicAdd(&contribution[(size_t)slot * 2], within);
+                    }
+                }
             } else {
                 const float g[3] = {w * dl.x * 256.0f, w * dl.y * 256.0f, w * dl.z * 256.0f};
                 for (uint32_t c = 0; c < 3; ++c) {
@@ -484,6 +517,12 @@ static hipError_t contribution_for(const DView* views, const uint32_t* tile_base
     if (mode == 1) {
         hipLaunchKernelGGL((contribution_kernel<TILE, 1>), dim3(tiles), dim3(TILE * TILE), 0, s, views, tile_bases,
                            nviews, batch, ranges, slots, records, out);
+    } else if (mode == 2) {
+        hipLaunchKernelGGL((contribution_kernel<TILE, 2>), dim3(tiles), dim3(TILE * TILE), 0, s, views, tile_bases,
+                           nviews, batch, ranges, slots, records, out);
+    } else if (mode == 3) {
+        hipLaunchKernelGGL((contribution_kernel<TILE, 3>), dim3(tiles), dim3(TILE * TILE), 0, s, views, tile_bases,
+                           nviews, batch, ranges, slots, records, out);
     } else {
         hipLaunchKernelGGL((contribution_kernel<TILE, 4>), dim3(tiles), dim3(TILE * TILE), 0, s, views, tile_bases,
                            nviews, batch, ranges, slots, records, out);
@@ -498,7 +537,7 @@ hipError_t launch_contribution(const DView* views_dev, const uint32_t* tile_base
     if (total_tiles == 0) {
         return hipSuccess;
     }
-    if (batch == 0 || batch > kMaxBatch || (mode != 1 && mode != 4)) {
+    if (batch == 0 || batch > kMaxBatch || mode < 1 || mode > 4) {
         return hipErrorInvalidValue;
     }
     switch (tile) {

@@ -1149,11 +1149,11 @@ ATHX_API athx_status athx_contribution(athx_ctx* ctx, const athx_view* views, ui
         return ATHX_INVALID_ARGUMENT;
     }
     ctx->error.clear();
-    if (mode != 1 && mode != 4) {
-        return fail(ctx, ATHX_UNSUPPORTED, "contribution modes 1 (weight) and 4 (colour gradient) are built");
+    if (mode < 1 || mode > 4) {
+        return fail(ctx, ATHX_INVALID_ARGUMENT, "contribution modes are 1 to 4");
     }
-    if (!views || nviews == 0 || !targets || !out || (mode == 4 && !dl_dcolour)) {
-        return fail(ctx, ATHX_INVALID_ARGUMENT, "no views, targets or output (or no dL/dcolour for mode 4)");
+    if (!views || nviews == 0 || !targets || !out || (mode != 1 && !dl_dcolour)) {
+        return fail(ctx, ATHX_INVALID_ARGUMENT, "no views, targets or output (or no per-pixel input for modes 2-4)");
     }
     if (athx_status st = check_settings(ctx, settings); st != ATHX_OK) {
         return st;
@@ -1162,17 +1162,17 @@ ATHX_API athx_status athx_contribution(athx_ctx* ctx, const athx_view* views, ui
         ctx->last_tile != settings->tile) {
         return fail(ctx, ATHX_INVALID_ARGUMENT, "contributions must follow a render of the same frame");
     }
-    if (mode == 4) {
+    if (mode != 1) {
         for (uint32_t k = 0; k < nviews; ++k) {
             if (!dl_dcolour[k]) {
-                return fail(ctx, ATHX_INVALID_ARGUMENT, "view " + std::to_string(k) + ": no dL/dcolour");
+                return fail(ctx, ATHX_INVALID_ARGUMENT, "view " + std::to_string(k) + ": no per-pixel input");
             }
         }
     }
     hipStream_t s = static_cast<hipStream_t>(stream);
     ATHX_HIP(ctx, hipSetDevice(ctx->device), "set device");
     uint64_t tiles64 = 0;
-    if (athx_status st = stage_views(ctx, views, nviews, settings, targets, mode == 4 ? dl_dcolour : nullptr, tiles64);
+    if (athx_status st = stage_views(ctx, views, nviews, settings, targets, mode != 1 ? dl_dcolour : nullptr, tiles64);
         st != ATHX_OK) {
         return st;
     }
@@ -1182,7 +1182,7 @@ ATHX_API athx_status athx_contribution(athx_ctx* ctx, const athx_view* views, ui
     if (athx_status st = upload_views(ctx, nviews, s); st != ATHX_OK) {
         return st;
     }
-    const uint64_t words = (uint64_t)gaussians->count * nviews * (mode == 4 ? 4u : 1u);
+    const uint64_t words = (uint64_t)gaussians->count * nviews * (mode == 4 ? 4u : mode == 1 ? 1u : 2u);
     ATHX_HIP(ctx, launch_fill_u32(out, 0u, words, s), "clear contributions");
     ATHX_HIP(ctx, launch_contribution(as<DView>(ctx->views), as<uint32_t>(ctx->tile_bases), nviews, ctx->last_tiles,
                                       settings->tile, settings->batch, as<uint32_t>(ctx->ranges),

@@ -450,16 +450,23 @@ class TileRasterizer:
 
     def contribution(self, views: list, cloud, settings: RenderSettings | None, targets: list[RenderTargets],
                      mode: int = 1, dl_dcolour: list[torch.Tensor] | None = None) -> torch.Tensor:
-        """athx_contribution for the frame the last ``render`` drew: mode 1 an
-        int32 [slots] of blend weights in 1/4096, mode 4 an int32 [slots, 4]
-        of the colour gradient against ``dl_dcolour`` and the weight, in
-        1/256 (slot = view * count + gaussian)."""
+        """athx_contribution for the frame the last ``render`` drew (slot =
+        view * count + gaussian): mode 1 an int32 [slots] of blend weights in
+        1/4096; mode 2 an int32 [slots, 2], the (low, high) words of the
+        squared removal error in 2^-24 against the drawn image given as
+        ``dl_dcolour``; mode 3 an int32 [slots, 2], the weight times the mask
+        given as ``dl_dcolour`` ([height, width]) and the weight, in 1/1024;
+        mode 4 an int32 [slots, 4] of the colour gradient against
+        ``dl_dcolour`` and the weight, in 1/256."""
         settings = settings or RenderSettings()
         dev = _device_of(cloud)
         c_views, c_targets = _views_and_targets(views, targets)
         nv = len(views)
         slots = cloud.count * nv
-        out = torch.empty((slots, 4) if mode == 4 else (slots,), dtype=torch.int32, device=dev)
+        words = {1: 1, 2: 2, 3: 2, 4: 4}.get(int(mode))
+        if words is None:
+            raise ValueError("contribution modes are 1 to 4")
+        out = torch.empty((slots, words) if words > 1 else (slots,), dtype=torch.int32, device=dev)
         dl = None
         keep = []
         if dl_dcolour is not None:

@@ -25,7 +25,7 @@
  *       contributionMode 4, shaders/athenea/splat/splat_blend.slang:381-398):
  *       the adjoint of athx_render's colour with respect to every parameter.
  *   athx_contribution
- *       RenderSettings::contributionMode 1 and 4 (splat_blend.slang:364-398).
+ *       RenderSettings::contributionMode 1 to 4 (splat_blend.slang:364-408).
  *   athx_latest_counters
  *       TileRasterizer::latestCounters (TileRasterizer.h:481) and
  *       SplatCounters (TileRasterizer.h:370-401).
@@ -362,9 +362,17 @@ ATHX_API athx_status athx_render_backward(athx_ctx* ctx, const athx_view* views,
  *           for each of r, g, b the sum of floor(T alpha dL/dC * 256 + h),
  *           h = hash01((pixel * 3 + channel) ^ (slot * 2654435761)) of
  *           color.slang:43-47 (signed, two's complement), then the sum of
- *           round(T alpha * 256). dl_dcolour[k]: view k's [height][width][4].
- * Modes 2 and 3 (removal error, lifted masks) are not built. The views,
- * gaussians, settings and targets are the last render's. Queued.
+ *           round(T alpha * 256). dl_dcolour[k]: view k's [height][width][4];
+ *   mode 2: two words, low then high, of the 64-bit sum of round(min(|e|^2,
+ *           1) * 2^24), e = T alpha (c - (C - P) / max(T', 1e-4)) the change
+ *           removing the splat makes (GaussianPOP's score): C the pixel drawn
+ *           whole, dl_dcolour[k] [height][width][4]; P the walk once the
+ *           splat is in; T' the transmittance after it; |e|^2 summed r, g, b
+ *           in that order;
+ *   mode 3: two words (splat_lift.slang): the sum of round(T alpha m * 1024)
+ *           with m the pixel's mask clamped to [0, 1], then the sum of
+ *           round(T alpha * 1024); dl_dcolour[k] the mask, [height][width].
+ * The views, gaussians, settings and targets are the last render's. Queued.
  */
 ATHX_API athx_status athx_contribution(athx_ctx* ctx, const athx_view* views, uint32_t nviews,
                                        const athx_gaussians* gaussians, const athx_settings* settings,

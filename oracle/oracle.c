@@ -30,7 +30,7 @@
  *   orc_compare_codes / orc_compare_hdr
  *                    reference/image_compare.slang, image_compare_hdr.slang:
  *                    the per-row histograms and relMSE rows, summed
- *   orc_contribution splat_blend.slang:364-398's contribution modes 1 and 4
+ *   orc_contribution splat_blend.slang:364-408's contribution modes 1 to 4
  *                    over the same per-pixel walk as orc_blend
  *   orc_ingest       a PLY vertex block widened, validated, decoded and
  *                    packed: packed_records.slang:1-110, splat_validate.slang,
@@ -1934,7 +1934,7 @@ ORC_EXPORT float orc_compare_hdr(const float* a, const float* b, uint32_t width,
 }
 
 
-/* ---- contributions: what each slot gave the frame (modes 1 and 4) ---- */
+/* ---- contributions: what each slot gave the frame (modes 1 to 4) ---- */
 
 static float contribution_hash01(uint32_t v) {
     const uint32_t state = v * 747796405u + 2891336453u;
@@ -1943,8 +1943,11 @@ static float contribution_hash01(uint32_t v) {
 }
 
 /* The walk of blend_pixel_f32 for every pixel of one view, adding at each
- * contributing pair's slot: mode 1 round(w 4096); mode 4 floor(w dL/dC
- * 256 + hash) a channel and round(w 256). dl: [h][w][4] (mode 4). */
+ * contributing pair's slot: mode 1 round(w 4096); mode 2 the removal
+ * error round(min(|e|^2, 1) 2^24) into a 64-bit (low, high) pair, dl the
+ * drawn image [h][w][4]; mode 3 round(w m 1024) and round(w 1024), dl the
+ * mask [h][w]; mode 4 floor(w dL/dC 256 + hash) a channel and round(w 256),
+ * dl: [h][w][4]. */
 ORC_EXPORT void orc_contribution(const orc_view* v, const float* rec, const uint32_t* ranges, const uint32_t* vals,
                                  uint32_t tile_offset, uint32_t mode, const float* dl, uint32_t* out) {
     const uint32_t W = v->width, H = v->height, T = v->tile;
@@ -1954,6 +1957,7 @@ ORC_EXPORT void orc_contribution(const orc_view* v, const float* rec, const uint
         const uint32_t tile = tile_offset + (py / T) * tiles_x + px / T;
         const float pxc = (float)px + 0.5f, pyc = (float)py + 0.5f;
         float Tr = 1.0f;
+        float run[3] = {0.0f, 0.0f, 0.0f};
         for (uint32_t j = ranges[tile * 2]; j < ranges[tile * 2 + 1]; ++j) {
             const uint32_t slot = vals[j];
             const float* r = rec + (size_t)slot * REC;
@@ -1965,6 +1969,29 @@ ORC_EXPORT void orc_contribution(const orc_view* v, const float* rec, const uint
             const float w = Tr * alpha;
             if (mode == 1) {
                 out[slot] += (uint32_t)(w * 4096.0f + 0.5f);
+            } else if (mode == 2) {
+                /* splat_blend.slang:366-379: the change removing it makes */
+                const float keep = maxf_(Tr * (1.0f - alpha), 1.0e-4f);
+                float e[3];
+                for (uint32_t c = 0; c < 3; ++c) {
+                    e[c] = w * (r[8 + c] - (dl[p * 4 + c] - (run[c] + w * r[8 + c])) / keep);
+                }
+                const float e2 = minf_(e[0] * e[0] + e[1] * e[1] + e[2] * e[2], 1.0f);
+                const uint32_t add = (uint32_t)(e2 * 16777216.0f + 0.5f);
+                const uint32_t before = out[(size_t)slot * 2];
+                out[(size_t)slot * 2] = before + add;
+                out[(size_t)slot * 2 + 1] += (before + add < before) ? 1u : 0u;
+                for (uint32_t c = 0; c < 3; ++c) {
+                    run[c] = run[c] + w * r[8 + c];
+                }
+            } else if (mode == 3) {
+                /* splat_blend.slang:399-408 */
+                const uint32_t seen = (uint32_t)(w * 1024.0f + 0.5f);
+                if (seen != 0u) {
+                    out[(size_t)slot * 2 + 1] += seen;
+                    const float m = minf_(maxf_(dl[p], 0.0f), 1.0f);
+                    out[(size_t)slot * 2] += (uint32_t)(w * m * 1024.0f + 0.5f);
+                }
             } else {
                 const uint32_t at = (uint32_t)p;
                 for (uint32_t c = 0; c < 3; ++c) {

@@ -583,10 +583,11 @@ def compare_hdr(a: np.ndarray, b: np.ndarray) -> tuple[np.ndarray, float]:
 
 def contribution(v: OrcView, frame: dict, slots: int, mode: int = 1, dl: np.ndarray | None = None) -> np.ndarray:
     """orc_contribution over one view's frame (render's dict): uint32
-    [slots] (mode 1) or [slots, 4] (mode 4)."""
-    out = np.zeros(slots * (4 if mode == 4 else 1), np.uint32)
+    [slots] (mode 1), [slots, 2] (modes 2 and 3) or [slots, 4] (mode 4)."""
+    words = {1: 1, 2: 2, 3: 2, 4: 4}[mode]
+    out = np.zeros(slots * words, np.uint32)
     d = np.ascontiguousarray(dl, np.float32) if dl is not None else None
     load().orc_contribution(ctypes.byref(v), _c(frame["records"], np.float32), _c(frame["ranges"], np.uint32),
                             _c(frame["pair_values"], np.uint32), 0, mode, d.ctypes.data if d is not None else None,
                             out)
-    return out.reshape(slots, 4) if mode == 4 else out
+    return out.reshape(slots, words) if words > 1 else out

@@ -1,4 +1,4 @@
-"""athx_contribution (RenderSettings::contributionMode 1 and 4) on the GPU
+"""athx_contribution (RenderSettings::contributionMode 1 to 4) on the GPU
 against orc_contribution over the same frame: every slot's words equal
 (integer atomics, so the sums do not depend on their order)."""
 from __future__ import annotations
@@ -36,6 +36,10 @@ def test_contributions_match_the_oracle(raster, tile, batch):
     dl = rng.normal(0, 2.0, (120, 160, 4)).astype(np.float32)
     w1 = raster.contribution([view], cloud, settings, targets, mode=1)
     w4 = raster.contribution([view], cloud, settings, targets, mode=4, dl_dcolour=[torch.from_numpy(dl).to("cuda:0")])
+    mask = rng.uniform(-0.2, 1.2, (120, 160)).astype(np.float32)
+    w3 = raster.contribution([view], cloud, settings, targets, mode=3, dl_dcolour=[torch.from_numpy(mask).to("cuda:0")])
+    drawn = targets[0].colour.contiguous()
+    w2 = raster.contribution([view], cloud, settings, targets, mode=2, dl_dcolour=[drawn])
     torch.cuda.synchronize()
     ov = po.orc_view(view, tile=tile)
     frame = po.render([ov], cloud_np)
@@ -44,3 +48,8 @@ def test_contributions_match_the_oracle(raster, tile, batch):
     assert np.array_equal(w1.cpu().numpy().view(np.uint32), want1)
     assert np.array_equal(w4.cpu().numpy().view(np.uint32), want4)
     assert want1.sum() > 0 and (want4[:, :3].view(np.int32) != 0).any()
+    want3 = po.contribution(ov, frame, 20000, mode=3, dl=mask)
+    want2 = po.contribution(ov, frame, 20000, mode=2, dl=drawn.cpu().numpy())
+    assert np.array_equal(w3.cpu().numpy().view(np.uint32), want3)
+    assert np.array_equal(w2.cpu().numpy().view(np.uint32), want2)
+    assert (want3[:, 0] < want3[:, 1]).any() and want2[:, 0].any()
