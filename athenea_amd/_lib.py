@@ -14,7 +14,8 @@ from pathlib import Path
 # ATHX_LIBRARY names another build of the same library (to compare builds).
 LIB_PATH = Path(os.environ.get("ATHX_LIBRARY") or Path(__file__).resolve().parent / "libathx.so")
 
-# athx_status: athenea's ErrorCode (modules/core/include/athenea/core/Error.h:14-26), 0 = OK.
+# athx_status: 0 = OK, else athenea's ErrorCode + 1 (ErrorCode numbers Unknown 0 ... InternalError 10,
+# modules/core/include/athenea/core/Error.h:14-26); athx_error_code() gives the ErrorCode value.
 STATUS_NAMES = {
     0: "Ok",
     1: "Unknown",
@@ -136,6 +137,7 @@ class FrameStats(ctypes.Structure):
         ("ranges_ms", ctypes.c_double),
         ("blend_ms", ctypes.c_double),
         ("total_ms", ctypes.c_double),
+        ("frame_serial", ctypes.c_uint64),
     ]
 
 
@@ -249,11 +251,14 @@ SIGNATURES = {
             ctypes.POINTER(Gaussians),
             ctypes.POINTER(Settings),
             ctypes.POINTER(Targets),
+            ctypes.c_uint64,
             ctypes.POINTER(ctypes.c_void_p),
             ctypes.POINTER(Grads),
             ctypes.c_void_p,
         ],
     ),
+    "athx_frame_serial": (ctypes.c_uint64, [ctypes.c_void_p]),
+    "athx_error_code": (ctypes.c_int, [ctypes.c_int]),
     "athx_render_async": (
         ctypes.c_int,
         [
@@ -310,7 +315,8 @@ SIGNATURES = {
     "athx_contribution": (
         ctypes.c_int,
         [ctypes.c_void_p, ctypes.POINTER(View), ctypes.c_uint32, ctypes.POINTER(Gaussians), ctypes.POINTER(Settings),
-         ctypes.POINTER(Targets), ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
+         ctypes.POINTER(Targets), ctypes.c_uint64, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p,
+         ctypes.c_void_p],
     ),
     "athx_sky_texels": (ctypes.c_uint32, [ctypes.c_uint32]),
     "athx_prepare_sky": (
@@ -345,11 +351,13 @@ _lib: ctypes.CDLL | None = None
 
 
 class AthxError(RuntimeError):
-    """A failed call of the C ABI: ``code`` is the athenea ErrorCode name."""
+    """A failed call of the C ABI: ``code`` is the athenea ErrorCode name,
+    ``error_code`` its ErrorCode value (status - 1)."""
 
     def __init__(self, status: int, message: str):
         self.status = status
         self.code = STATUS_NAMES.get(status, f"status {status}")
+        self.error_code = status - 1 if 1 <= status <= 11 else -1
         super().__init__(f"{self.code}: {message}")
 
 

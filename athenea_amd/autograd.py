@@ -44,7 +44,14 @@ class _Rasterize(torch.autograd.Function):
             "opacity_logits": torch.zeros_like(cloud.opacity_logits),
             "sh": torch.zeros(cloud.sh.shape, dtype=torch.float32, device=cloud.sh.device),
         }
-        ctx.raster.render_backward(ctx.views, cloud, ctx.settings, ctx.targets, dl, grads)
+        targets = ctx.targets
+        if ctx.raster.frame_serial() != targets[0].frame_serial:
+            # A later render on this rasterizer replaced the frame's pairs and
+            # records (two forwards before one backward). The frame is drawn
+            # again from the saved parameters -- the same bits, since the
+            # render is deterministic -- and its adjoint taken from that.
+            targets, _ = ctx.raster.render(ctx.views, cloud, ctx.settings, keep_backward=True)
+        ctx.raster.render_backward(ctx.views, cloud, ctx.settings, targets, dl, grads)
         g_sh = grads["sh"].to(ctx.sh_dtype)
         return (grads["means"], grads["log_scales"], grads["quats"], grads["opacity_logits"], g_sh,
                 None, None, None, None)
@@ -62,8 +69,9 @@ def rasterize(
     linear: bool = True,
 ) -> tuple[list[torch.Tensor], list[torch.Tensor]]:
     """Renders every view; returns (colours [h, w, 4], depths [h, w]), both
-    per view. Colours are differentiable with respect to every parameter. The
-    backward pass must run before ``raster`` renders another frame."""
+    per view. Colours are differentiable with respect to every parameter.
+    Several calls may share one ``raster`` before one backward pass: a call
+    whose frame a later render replaced draws it again in its backward."""
     settings = settings or RenderSettings()
     outs = _Rasterize.apply(means, log_scales, quats, opacity_logits, sh, raster, views, settings, linear)
     n = len(views)

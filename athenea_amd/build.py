@@ -25,7 +25,7 @@ LIB = HERE / "libathx.so"
 BUILD = HERE / "_build"
 
 SOURCES = ["project.hip", "scan.hip", "sort.hip", "blend.hip", "project_backward.hip", "ingest.hip", "reference.hip",
-           "sky.hip", "capi.hip"]
+           "sky.hip", "ply_header.hip", "capi.hip"]
 
 HIPFLAGS = [
     "-O3",

@@ -102,31 +102,27 @@ __device__ inline float clamp_(float x, float lo, float hi) { return fmin_(fmax_
 // frame.slang:354-371: the least power a * x^2 + 2 b x y + c y^2 takes
 // over the box [lox, hix] x [loy, hiy]: zero when the box holds the centre,
 // else the least over its four edges, each at the edge's clamped 1D
-// minimiser. The minimiser ((-b) x) / c is taken as ((-b) x) * (1 / c),
-// with the reciprocals (ia = 1 / a, ic = 1 / c) formed once a footprint:
-// the reference's divide has no defined rounding across its backends, and
-// this form is the one every test of a tile here and in the host
-// restatement computes, bit for bit.
-__device__ inline float min_power_over_box(float a, float b, float c, float ia, float ic, float lox, float loy,
-                                           float hix, float hiy) {
+// minimiser ((-b) x) / c, an IEEE divide as the reference writes it
+// (-b * x / c; the library is built with correctly rounded division).
+__device__ inline float min_power_over_box(float a, float b, float c, float lox, float loy, float hix, float hiy) {
     if (lox <= 0.0f && hix >= 0.0f && loy <= 0.0f && hiy >= 0.0f) {
         return 0.0f;
     }
     float best = 3.4e38f;
     for (int e = 0; e < 2; ++e) {
         const float x = e == 0 ? lox : hix;
-        const float y = c > 1e-12f ? clamp_(((-b) * x) * ic, loy, hiy) : loy;
+        const float y = c > 1e-12f ? clamp_(((-b) * x) / c, loy, hiy) : loy;
         best = fmin_(best, a * x * x + 2.0f * b * x * y + c * y * y);
     }
     for (int e = 0; e < 2; ++e) {
         const float y = e == 0 ? loy : hiy;
-        const float x = a > 1e-12f ? clamp_(((-b) * y) * ia, lox, hix) : lox;
+        const float x = a > 1e-12f ? clamp_(((-b) * y) / a, lox, hix) : lox;
         best = fmin_(best, a * x * x + 2.0f * b * x * y + c * y * y);
     }
     return best;
 }
 
-// The reciprocals min_power_over_box takes, formed once a footprint.
+// The reciprocals the strip classifier's slice extents take, formed once a footprint.
 __device__ inline float recip_of(float v) { return v > 1e-12f ? 1.0f / v : 0.0f; }
 
 // MI355X deals a launch's workgroups to its 8 XCDs round robin (workgroup b

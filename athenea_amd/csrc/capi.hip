@@ -98,6 +98,10 @@ struct athx_ctx {
     hipEvent_t async_done = nullptr;
     bool async_pending = false;
     bool have_frame = false;
+    // Numbers the frames rendered on this context (1, 2, ...): the adjoint
+    // and the contributions name the frame they follow, so a later render
+    // that replaced its buffers is refused rather than walked.
+    uint64_t serial = 0;
     bool counters_valid = false;
     athx_counters counters{};
 
@@ -764,6 +768,21 @@ void finish_frame(athx_ctx* ctx, const athx_gaussians* g, const athx_settings* s
     ctx->last_key_bits = fs.key_bits;
     ctx->last_true_pairs = pairs_known;
     ctx->have_frame = true;
+    ctx->serial += 1;
+}
+
+// The frame a backward pass or a contribution walk names must be the one
+// whose buffers the context still holds.
+athx_status check_serial(athx_ctx* ctx, uint64_t frame_serial) {
+    if (!ctx->have_frame) {
+        return fail(ctx, ATHX_INVALID_ARGUMENT, "no frame has been rendered on this context");
+    }
+    if (frame_serial != ctx->serial) {
+        return fail(ctx, ATHX_INVALID_ARGUMENT,
+                    "frame " + std::to_string(frame_serial) + " was replaced by frame " + std::to_string(ctx->serial) +
+                        ": its pairs and records are gone; render it again before walking it");
+    }
+    return ATHX_OK;
 }
 
 
@@ -780,7 +799,11 @@ athx_status read_totals(athx_ctx* ctx, hipStream_t s) {
 
 extern "C" {
 
-ATHX_API const char* athx_version(void) { return "athx 0.1 (gfx950)"; }
+ATHX_API const char* athx_version(void) { return "athx 0.2 (gfx950)"; }
+
+ATHX_API int athx_error_code(athx_status status) {
+    return status >= ATHX_UNKNOWN && status <= ATHX_INTERNAL_ERROR ? (int)status - 1 : -1;
+}
 
 ATHX_API athx_status athx_create(int device, athx_ctx** out) {
     if (!out) {
@@ -852,7 +875,7 @@ ATHX_API void athx_destroy(athx_ctx* ctx) {
 
 ATHX_API const char* athx_last_error(const athx_ctx* ctx) { return ctx ? ctx->error.c_str() : "no context"; }
 
-ATHX_API athx_status athx_release_scratch(athx_ctx* ctx) {
+ATHX_API athx_status athx_release_scratch(athx_ctx* ctx)  try {
     if (!ctx) {
         return ATHX_INVALID_ARGUMENT;
     }
@@ -871,6 +894,10 @@ ATHX_API athx_status athx_release_scratch(athx_ctx* ctx) {
     ctx->entry_last = 0;
     ctx->have_frame = false;
     return ATHX_OK;
+} catch (const std::bad_alloc&) {
+    return fail(ctx, ATHX_OUT_OF_MEMORY, "host allocation failed");
+} catch (const std::exception& e) {
+    return fail(ctx, ATHX_INTERNAL_ERROR, e.what());
 }
 
 ATHX_API uint64_t athx_scratch_bytes(const athx_ctx* ctx) {
@@ -985,6 +1012,7 @@ ATHX_API athx_status athx_render(athx_ctx* ctx, const athx_view* views, uint32_t
         stats->splats = gaussians->count;
         stats->visible = (uint32_t)host[kTotalVisible];
         stats->pairs = pairs64;
+        stats->frame_serial = ctx->serial;
         if (timed) {
             ATHX_HIP(ctx, hipEventSynchronize(ctx->ev[7]), "wait for frame");
             stats->project_ms = elapsed_ms(ctx->ev[0], ctx->ev[1]);
@@ -1063,7 +1091,7 @@ ATHX_API athx_status athx_render_async(athx_ctx* ctx, const athx_view* views, ui
     return fail(ctx, ATHX_INTERNAL_ERROR, e.what());
 }
 
-ATHX_API athx_status athx_frame_pairs(athx_ctx* ctx, uint64_t* pairs, uint64_t* capacity) {
+ATHX_API athx_status athx_frame_pairs(athx_ctx* ctx, uint64_t* pairs, uint64_t* capacity)  try {
     if (!ctx || !pairs) {
         return ATHX_INVALID_ARGUMENT;
     }
@@ -1085,12 +1113,16 @@ ATHX_API athx_status athx_frame_pairs(athx_ctx* ctx, uint64_t* pairs, uint64_t* 
         *capacity = ctx->last_capacity;
     }
     return ATHX_OK;
+} catch (const std::bad_alloc&) {
+    return fail(ctx, ATHX_OUT_OF_MEMORY, "host allocation failed");
+} catch (const std::exception& e) {
+    return fail(ctx, ATHX_INTERNAL_ERROR, e.what());
 }
 
 ATHX_API athx_status athx_render_backward(athx_ctx* ctx, const athx_view* views, uint32_t nviews,
                                           const athx_gaussians* gaussians, const athx_settings* settings,
-                                          const athx_targets* targets, const float* const* dl_dcolour,
-                                          const athx_grads* grads, void* stream) try {
+                                          const athx_targets* targets, uint64_t frame_serial,
+                                          const float* const* dl_dcolour, const athx_grads* grads, void* stream) try {
     if (!ctx) {
         return ATHX_INVALID_ARGUMENT;
     }
@@ -1104,8 +1136,10 @@ ATHX_API athx_status athx_render_backward(athx_ctx* ctx, const athx_view* views,
     if (athx_status st = check_gaussians(ctx, gaussians); st != ATHX_OK) {
         return st;
     }
-    if (!ctx->have_frame || ctx->last_n != gaussians->count || ctx->last_nviews != nviews ||
-        ctx->last_tile != settings->tile) {
+    if (athx_status st = check_serial(ctx, frame_serial); st != ATHX_OK) {
+        return st;
+    }
+    if (ctx->last_n != gaussians->count || ctx->last_nviews != nviews || ctx->last_tile != settings->tile) {
         return fail(ctx, ATHX_INVALID_ARGUMENT, "the backward pass must follow a render of the same frame");
     }
     if (gaussians->relight) {
@@ -1143,8 +1177,8 @@ ATHX_API athx_status athx_render_backward(athx_ctx* ctx, const athx_view* views,
 
 ATHX_API athx_status athx_contribution(athx_ctx* ctx, const athx_view* views, uint32_t nviews,
                                       const athx_gaussians* gaussians, const athx_settings* settings,
-                                      const athx_targets* targets, uint32_t mode, const float* const* dl_dcolour,
-                                      uint32_t* out, void* stream) {
+                                      const athx_targets* targets, uint64_t frame_serial, uint32_t mode,
+                                      const float* const* dl_dcolour, uint32_t* out, void* stream) try {
     if (!ctx) {
         return ATHX_INVALID_ARGUMENT;
     }
@@ -1158,7 +1192,10 @@ ATHX_API athx_status athx_contribution(athx_ctx* ctx, const athx_view* views, ui
     if (athx_status st = check_settings(ctx, settings); st != ATHX_OK) {
         return st;
     }
-    if (!gaussians || !ctx->have_frame || ctx->last_n != gaussians->count || ctx->last_nviews != nviews ||
+    if (athx_status st = check_serial(ctx, frame_serial); st != ATHX_OK) {
+        return st;
+    }
+    if (!gaussians || ctx->last_n != gaussians->count || ctx->last_nviews != nviews ||
         ctx->last_tile != settings->tile) {
         return fail(ctx, ATHX_INVALID_ARGUMENT, "contributions must follow a render of the same frame");
     }
@@ -1189,9 +1226,15 @@ ATHX_API athx_status athx_contribution(athx_ctx* ctx, const athx_view* views, ui
                                       as<uint32_t>(ctx->pair_vals), as<float>(ctx->records), mode, out, s),
              "contribution");
     return ATHX_OK;
+} catch (const std::bad_alloc&) {
+    return fail(ctx, ATHX_OUT_OF_MEMORY, "host allocation failed");
+} catch (const std::exception& e) {
+    return fail(ctx, ATHX_INTERNAL_ERROR, e.what());
 }
 
-ATHX_API athx_status athx_latest_counters(athx_ctx* ctx, athx_counters* out) {
+ATHX_API uint64_t athx_frame_serial(const athx_ctx* ctx) { return ctx && ctx->have_frame ? ctx->serial : 0; }
+
+ATHX_API athx_status athx_latest_counters(athx_ctx* ctx, athx_counters* out)  try {
     if (!ctx || !out) {
         return ATHX_INVALID_ARGUMENT;
     }
@@ -1211,9 +1254,13 @@ ATHX_API athx_status athx_latest_counters(athx_ctx* ctx, athx_counters* out) {
         out->culled[r] = words[8 + r];
     }
     return ATHX_OK;
+} catch (const std::bad_alloc&) {
+    return fail(ctx, ATHX_OUT_OF_MEMORY, "host allocation failed");
+} catch (const std::exception& e) {
+    return fail(ctx, ATHX_INTERNAL_ERROR, e.what());
 }
 
-ATHX_API athx_status athx_frame_info_get(athx_ctx* ctx, athx_frame_info* out) {
+ATHX_API athx_status athx_frame_info_get(athx_ctx* ctx, athx_frame_info* out)  try {
     if (!ctx || !out) {
         return ATHX_INVALID_ARGUMENT;
     }
@@ -1228,9 +1275,13 @@ ATHX_API athx_status athx_frame_info_get(athx_ctx* ctx, athx_frame_info* out) {
     out->strips = ctx->last_strips;
     out->entries = ctx->host_totals[kTotalEntries];
     return ATHX_OK;
+} catch (const std::bad_alloc&) {
+    return fail(ctx, ATHX_OUT_OF_MEMORY, "host allocation failed");
+} catch (const std::exception& e) {
+    return fail(ctx, ATHX_INTERNAL_ERROR, e.what());
 }
 
-ATHX_API athx_status athx_copy_frame(athx_ctx* ctx, athx_frame_part part, void* dst, uint64_t bytes, void* stream) {
+ATHX_API athx_status athx_copy_frame(athx_ctx* ctx, athx_frame_part part, void* dst, uint64_t bytes, void* stream)  try {
     if (!ctx) {
         return ATHX_INVALID_ARGUMENT;
     }
@@ -1298,10 +1349,14 @@ ATHX_API athx_status athx_copy_frame(athx_ctx* ctx, athx_frame_part part, void* 
     }
     ATHX_HIP(ctx, hipMemcpyAsync(dst, src, size, hipMemcpyDeviceToDevice, s), "copy frame part");
     return ATHX_OK;
+} catch (const std::bad_alloc&) {
+    return fail(ctx, ATHX_OUT_OF_MEMORY, "host allocation failed");
+} catch (const std::exception& e) {
+    return fail(ctx, ATHX_INTERNAL_ERROR, e.what());
 }
 
 ATHX_API athx_status athx_exclusive_scan(athx_ctx* ctx, const uint32_t* in, uint32_t* out, uint64_t* total,
-                                         uint32_t n, void* stream) {
+                                         uint32_t n, void* stream)  try {
     if (!ctx) {
         return ATHX_INVALID_ARGUMENT;
     }
@@ -1317,10 +1372,14 @@ ATHX_API athx_status athx_exclusive_scan(athx_ctx* ctx, const uint32_t* in, uint
     ATHX_HIP(ctx, launch_scan(in, out, total, n, as<uint32_t>(ctx->api_scan_status), static_cast<hipStream_t>(stream)),
              "scan");
     return ATHX_OK;
+} catch (const std::bad_alloc&) {
+    return fail(ctx, ATHX_OUT_OF_MEMORY, "host allocation failed");
+} catch (const std::exception& e) {
+    return fail(ctx, ATHX_INTERNAL_ERROR, e.what());
 }
 
 ATHX_API athx_status athx_sort_pairs(athx_ctx* ctx, uint64_t* keys, uint32_t* values, uint32_t n, uint32_t key_bits,
-                                     void* stream) {
+                                     void* stream)  try {
     if (!ctx) {
         return ATHX_INVALID_ARGUMENT;
     }
@@ -1352,251 +1411,15 @@ ATHX_API athx_status athx_sort_pairs(athx_ctx* ctx, uint64_t* keys, uint32_t* va
     ATHX_HIP(ctx, launch_fill_u32(as<uint32_t>(ctx->api_count) + 1, 0u, 1, s), "sort count");
     ATHX_HIP(ctx, launch_sort(keys, values, as<uint64_t>(ctx->api_count), n, 0, key_bits, sc, s), "sort");
     return ATHX_OK;
-}
-
-namespace {
-
-// io::parsePlyHeader's tokens (modules/io/src/PlyHeader.cpp:10-21): a type's
-// bytes and its athx_ply_type (-1: not one the ingest reads).
-int ply_type_of(const std::string& t, uint32_t& bytes) {
-    struct Entry {
-        const char* name;
-        uint32_t bytes;
-        int type;
-    };
-    static const Entry kTypes[] = {
-        {"float", 4, ATHX_PLY_F32},  {"float32", 4, ATHX_PLY_F32}, {"double", 8, ATHX_PLY_F64},
-        {"float64", 8, ATHX_PLY_F64}, {"uchar", 1, ATHX_PLY_U8},   {"uint8", 1, ATHX_PLY_U8},
-        {"char", 1, ATHX_PLY_I8},    {"int8", 1, ATHX_PLY_I8},     {"ushort", 2, ATHX_PLY_U16},
-        {"uint16", 2, ATHX_PLY_U16}, {"short", 2, ATHX_PLY_I16},   {"int16", 2, ATHX_PLY_I16},
-        {"uint", 4, ATHX_PLY_U32},   {"uint32", 4, ATHX_PLY_U32},  {"int", 4, ATHX_PLY_I32},
-        {"int32", 4, ATHX_PLY_I32}};
-    for (const Entry& e : kTypes) {
-        if (t == e.name) {
-            bytes = e.bytes;
-            return e.type;
-        }
-    }
-    bytes = 4;  // the reference's size of an unknown type
-    return -1;
-}
-
-struct PlyProperty {
-    std::string name, type;
-    uint32_t offset = 0, bytes = 0;
-    int code = -1;
-};
-
-struct PlyElement {
-    std::string name;
-    uint64_t count = 0;
-    uint64_t stride = 0;
-    bool has_list = false;
-    std::vector<PlyProperty> props;
-};
-
-athx_status ply_fail(athx_status code, const std::string& msg, char* message, uint64_t message_bytes) {
-    if (message && message_bytes > 0) {
-        const size_t n = std::min<size_t>(msg.size(), message_bytes - 1);
-        std::memcpy(message, msg.data(), n);
-        message[n] = '\0';
-    }
-    return code;
-}
-
-}  // namespace
-
-ATHX_API athx_status athx_ply_parse(const char* header, uint64_t bytes, athx_ply_layout* out, char* message,
-                                    uint64_t message_bytes) {
-    if (message && message_bytes > 0) {
-        message[0] = '\0';
-    }
-    if (!header || !out) {
-        return ply_fail(ATHX_INVALID_ARGUMENT, "null header or layout", message, message_bytes);
-    }
-    const std::string_view text(header, bytes);
-    size_t at = 0;
-    std::string line;
-    const auto next_line = [&]() {
-        if (at >= text.size()) {
-            return false;
-        }
-        size_t end = text.find('\n', at);
-        if (end == std::string_view::npos) {
-            end = text.size();
-        }
-        line.assign(text.substr(at, end - at));
-        at = end + 1;
-        if (!line.empty() && line.back() == '\r') {
-            line.pop_back();
-        }
-        return true;
-    };
-    if (!next_line() || line.rfind("ply", 0) != 0) {
-        return ply_fail(ATHX_INVALID_ARGUMENT, "not a PLY file", message, message_bytes);
-    }
-    std::string format;
-    std::vector<PlyElement> elements;
-    bool ended = false;
-    while (next_line()) {
-        std::istringstream words(line);
-        std::string word;
-        words >> word;
-        if (word == "format") {
-            words >> format;
-        } else if (word == "element") {
-            PlyElement e;
-            words >> e.name >> e.count;
-            elements.push_back(std::move(e));
-        } else if (word == "property") {
-            if (elements.empty()) {
-                return ply_fail(ATHX_INVALID_ARGUMENT, "a PLY property before any element", message,
-                                message_bytes);
-            }
-            PlyElement& e = elements.back();
-            std::string type;
-            words >> type;
-            PlyProperty p;
-            if (type == "list") {
-                std::string count_type, item_type;
-                words >> count_type >> item_type >> p.name;
-                p.type = "list";
-                e.has_list = true;
-                e.props.push_back(std::move(p));
-                continue;
-            }
-            words >> p.name;
-            p.type = type;
-            p.offset = (uint32_t)e.stride;
-            p.code = ply_type_of(type, p.bytes);
-            e.stride += p.bytes;
-            e.props.push_back(std::move(p));
-        } else if (word == "end_header") {
-            ended = true;
-            break;
-        }
-    }
-    if (!ended) {
-        return ply_fail(ATHX_INVALID_ARGUMENT, "PLY header has no end_header", message, message_bytes);
-    }
-    if (format.empty()) {
-        return ply_fail(ATHX_INVALID_ARGUMENT, "PLY header has no format line", message, message_bytes);
-    }
-    if (format != "binary_little_endian") {
-        return ply_fail(ATHX_UNSUPPORTED, "PLY format '" + format + "'; splat clouds are binary_little_endian",
-                        message, message_bytes);
-    }
-    // The vertex element, and its offset past the elements before it
-    // (PlyHeader::binaryOffsetOf, SplatReaders.cpp:60-85).
-    const PlyElement* vertex = nullptr;
-    uint64_t offset = at;
-    for (const PlyElement& e : elements) {
-        if (e.name == "vertex") {
-            vertex = &e;
-            break;
-        }
-        if (e.has_list) {
-            return ply_fail(ATHX_UNSUPPORTED, "a list element ('" + e.name + "') before 'vertex' makes its offset "
-                            "unknowable", message, message_bytes);
-        }
-        offset += e.stride * e.count;
-    }
-    if (!vertex || vertex->count == 0) {
-        return ply_fail(ATHX_INVALID_ARGUMENT, "no vertices", message, message_bytes);
-    }
-    if (vertex->has_list) {
-        return ply_fail(ATHX_UNSUPPORTED, "a list in the vertex element", message, message_bytes);
-    }
-    if (vertex->count > 0xFFFFFFFFull) {
-        return ply_fail(ATHX_UNSUPPORTED, "more than 2^32 splats", message, message_bytes);
-    }
-    if (vertex->props.size() > ATHX_PLY_MAX_PROPERTIES) {
-        return ply_fail(ATHX_UNSUPPORTED, "more than " + std::to_string(ATHX_PLY_MAX_PROPERTIES) +
-                        " vertex properties", message, message_bytes);
-    }
-    const auto index_of = [&](const std::string& name) -> int {
-        for (size_t k = 0; k < vertex->props.size(); ++k) {
-            if (vertex->props[k].name == name) {
-                return (int)k;
-            }
-        }
-        return -1;
-    };
-    static const char* kRequired[] = {"x",       "y",       "z",       "opacity", "scale_0", "scale_1", "scale_2",
-                                      "rot_0",   "rot_1",   "rot_2",   "rot_3",   "f_dc_0",  "f_dc_1",  "f_dc_2"};
-    for (const char* name : kRequired) {
-        if (index_of(name) < 0) {
-            return ply_fail(ATHX_INVALID_ARGUMENT,
-                            std::string("no '") + name + "' -- this does not look like a trained splat cloud",
-                            message, message_bytes);
-        }
-    }
-    athx_ply_layout L;
-    std::memset(&L, 0, sizeof L);
-    L.data_offset = offset;
-    L.count = vertex->count;
-    L.stride = (uint32_t)vertex->stride;
-    L.properties = (uint32_t)vertex->props.size();
-    for (size_t k = 0; k < vertex->props.size(); ++k) {
-        const PlyProperty& p = vertex->props[k];
-        if (p.code < 0) {
-            return ply_fail(ATHX_UNSUPPORTED, "property '" + p.name + "' of type '" + p.type + "'", message,
-                            message_bytes);
-        }
-        L.offset[k] = p.offset;
-        L.type[k] = (uint32_t)p.code;
-    }
-    L.x = (uint32_t)index_of("x");
-    L.y = (uint32_t)index_of("y");
-    L.z = (uint32_t)index_of("z");
-    L.opacity = (uint32_t)index_of("opacity");
-    for (uint32_t k = 0; k < 3; ++k) {
-        L.scale[k] = (uint32_t)index_of("scale_" + std::to_string(k));
-        L.dc[k] = (uint32_t)index_of("f_dc_" + std::to_string(k));
-    }
-    for (uint32_t k = 0; k < 4; ++k) {
-        L.rot[k] = (uint32_t)index_of("rot_" + std::to_string(k));
-    }
-    // The harmonics: counted, a whole degree, contiguous, red's first
-    // (SplatReaders.cpp:142-180).
-    uint32_t rest = 0;
-    for (const PlyProperty& p : vertex->props) {
-        if (p.name.rfind("f_rest_", 0) == 0) {
-            ++rest;
-        }
-    }
-    uint32_t per_colour = 0;
-    for (uint32_t basis : {15u, 8u, 3u}) {
-        if (rest >= basis * 3) {
-            per_colour = basis;
-            break;
-        }
-    }
-    if (per_colour > 0) {
-        const int base = index_of("f_rest_0");
-        const uint32_t file_per_colour = rest / 3;
-        for (uint32_t k = 0; k < per_colour * 3; ++k) {
-            const uint32_t colour = k / per_colour, basis = k % per_colour;
-            const uint32_t want = colour * file_per_colour + basis;
-            if (base < 0 || index_of("f_rest_" + std::to_string(want)) != base + (int)want) {
-                return ply_fail(ATHX_UNSUPPORTED, "harmonics are not contiguous in the vertex element", message,
-                                message_bytes);
-            }
-        }
-        if (file_per_colour != per_colour) {
-            return ply_fail(ATHX_UNSUPPORTED, std::to_string(rest) + " harmonic coefficients is not a whole degree",
-                            message, message_bytes);
-        }
-        L.rest_base = (uint32_t)base;
-        L.rest_per_colour = per_colour;
-    }
-    *out = L;
-    return ATHX_OK;
+} catch (const std::bad_alloc&) {
+    return fail(ctx, ATHX_OUT_OF_MEMORY, "host allocation failed");
+} catch (const std::exception& e) {
+    return fail(ctx, ATHX_INTERNAL_ERROR, e.what());
 }
 
 ATHX_API athx_status athx_ingest_ply(athx_ctx* ctx, const void* vertices, const athx_ply_layout* layout,
                                      uint32_t max_degree, uint32_t keep_faint, const athx_ingest_out* out,
-                                     uint32_t* kept) {
+                                     uint32_t* kept) try {
     if (!ctx) {
         return ATHX_INVALID_ARGUMENT;
     }
@@ -1683,6 +1506,10 @@ ATHX_API athx_status athx_ingest_ply(athx_ctx* ctx, const void* vertices, const 
     ATHX_HIP(ctx, hipStreamSynchronize(s), "ingest wait");
     *kept = (uint32_t)total;
     return ATHX_OK;
+} catch (const std::bad_alloc&) {
+    return fail(ctx, ATHX_OUT_OF_MEMORY, "host allocation failed");
+} catch (const std::exception& e) {
+    return fail(ctx, ATHX_INTERNAL_ERROR, e.what());
 }
 
 
@@ -1874,7 +1701,7 @@ ATHX_API athx_status athx_graph_create(int device, const athx_view* views, uint3
     return ATHX_OUT_OF_MEMORY;
 }
 
-ATHX_API athx_status athx_graph_launch(athx_graph* g, void* stream) {
+ATHX_API athx_status athx_graph_launch(athx_graph* g, void* stream)  try {
     if (!g || !g->forward_exec) {
         return ATHX_INVALID_ARGUMENT;
     }
@@ -1886,9 +1713,13 @@ ATHX_API athx_status athx_graph_launch(athx_graph* g, void* stream) {
     ctx->async_pending = true;
     ctx->have_frame = true;
     return ATHX_OK;
+} catch (const std::bad_alloc&) {
+    return fail(g ? g->ctx : nullptr, ATHX_OUT_OF_MEMORY, "host allocation failed");
+} catch (const std::exception& e) {
+    return fail(g ? g->ctx : nullptr, ATHX_INTERNAL_ERROR, e.what());
 }
 
-ATHX_API athx_status athx_graph_launch_backward(athx_graph* g, void* stream) {
+ATHX_API athx_status athx_graph_launch_backward(athx_graph* g, void* stream)  try {
     if (!g || !g->backward_exec) {
         return ATHX_INVALID_ARGUMENT;
     }
@@ -1898,13 +1729,21 @@ ATHX_API athx_status athx_graph_launch_backward(athx_graph* g, void* stream) {
     ATHX_HIP(g->ctx, hipSetDevice(g->ctx->device), "set device");
     ATHX_HIP(g->ctx, hipGraphLaunch(g->backward_exec, static_cast<hipStream_t>(stream)), "launch the adjoint");
     return ATHX_OK;
+} catch (const std::bad_alloc&) {
+    return fail(g ? g->ctx : nullptr, ATHX_OUT_OF_MEMORY, "host allocation failed");
+} catch (const std::exception& e) {
+    return fail(g ? g->ctx : nullptr, ATHX_INTERNAL_ERROR, e.what());
 }
 
-ATHX_API athx_status athx_graph_pairs(athx_graph* g, uint64_t* pairs, uint64_t* capacity) {
+ATHX_API athx_status athx_graph_pairs(athx_graph* g, uint64_t* pairs, uint64_t* capacity)  try {
     if (!g) {
         return ATHX_INVALID_ARGUMENT;
     }
     return athx_frame_pairs(g->ctx, pairs, capacity);
+} catch (const std::bad_alloc&) {
+    return fail(g ? g->ctx : nullptr, ATHX_OUT_OF_MEMORY, "host allocation failed");
+} catch (const std::exception& e) {
+    return fail(g ? g->ctx : nullptr, ATHX_INTERNAL_ERROR, e.what());
 }
 
 }  // extern "C"
@@ -1950,7 +1789,7 @@ using namespace athx;
 extern "C" {
 
 ATHX_API athx_status athx_compare_images(athx_ctx* ctx, const float* a, const float* b, uint32_t width,
-                                         uint32_t height, athx_image_difference* out) {
+                                         uint32_t height, athx_image_difference* out)  try {
     if (!ctx) {
         return ATHX_INVALID_ARGUMENT;
     }
@@ -1976,10 +1815,14 @@ ATHX_API athx_status athx_compare_images(athx_ctx* ctx, const float* a, const fl
     d.p99 = p99_bin(bins, d.pixels);
     *out = d;
     return ATHX_OK;
+} catch (const std::bad_alloc&) {
+    return fail(ctx, ATHX_OUT_OF_MEMORY, "host allocation failed");
+} catch (const std::exception& e) {
+    return fail(ctx, ATHX_INTERNAL_ERROR, e.what());
 }
 
 ATHX_API athx_status athx_compare_hdr(athx_ctx* ctx, const float* a, const float* b, uint32_t width, uint32_t height,
-                                      athx_hdr_difference* out) {
+                                      athx_hdr_difference* out)  try {
     if (!ctx) {
         return ATHX_INVALID_ARGUMENT;
     }
@@ -2008,12 +1851,16 @@ ATHX_API athx_status athx_compare_hdr(athx_ctx* ctx, const float* a, const float
     d.rel_mse = d.pixels > 0 ? (double)error / (double)d.pixels : 0.0;
     *out = d;
     return ATHX_OK;
+} catch (const std::bad_alloc&) {
+    return fail(ctx, ATHX_OUT_OF_MEMORY, "host allocation failed");
+} catch (const std::exception& e) {
+    return fail(ctx, ATHX_INTERNAL_ERROR, e.what());
 }
 
 ATHX_API uint32_t athx_sky_texels(uint32_t base_side) { return sky_texels(base_side, nullptr, nullptr); }
 
 ATHX_API athx_status athx_prepare_sky(athx_ctx* ctx, const float* equirect, uint32_t width, uint32_t height,
-                                      uint32_t base_side, uint32_t* texels, float* sh) {
+                                      uint32_t base_side, uint32_t* texels, float* sh)  try {
     if (!ctx) {
         return ATHX_INVALID_ARGUMENT;
     }
@@ -2031,11 +1878,15 @@ ATHX_API athx_status athx_prepare_sky(athx_ctx* ctx, const float* equirect, uint
     ATHX_HIP(ctx, launch_prepare_sky(equirect, width, height, base_side, texels, sh, nullptr), "prepare sky");
     ATHX_HIP(ctx, hipStreamSynchronize(nullptr), "prepare sky wait");
     return ATHX_OK;
+} catch (const std::bad_alloc&) {
+    return fail(ctx, ATHX_OUT_OF_MEMORY, "host allocation failed");
+} catch (const std::exception& e) {
+    return fail(ctx, ATHX_INTERNAL_ERROR, e.what());
 }
 
 ATHX_API athx_status athx_reference_render(athx_ctx* ctx, const athx_view* view, const athx_gaussians* g,
                                            const athx_settings* settings, const athx_targets* targets,
-                                           float* records, uint32_t* overflow) {
+                                           float* records, uint32_t* overflow)  try {
     if (!ctx) {
         return ATHX_INVALID_ARGUMENT;
     }
@@ -2101,6 +1952,10 @@ ATHX_API athx_status athx_reference_render(athx_ctx* ctx, const athx_view* view,
     ATHX_HIP(ctx, hipMemcpy(&host_over, over, sizeof host_over, hipMemcpyDeviceToHost), "read overflow");
     *overflow = host_over;
     return ATHX_OK;
+} catch (const std::bad_alloc&) {
+    return fail(ctx, ATHX_OUT_OF_MEMORY, "host allocation failed");
+} catch (const std::exception& e) {
+    return fail(ctx, ATHX_INTERNAL_ERROR, e.what());
 }
 
 }  // extern "C"

@@ -30,14 +30,14 @@ __device__ inline float cutoff_power(float alpha) {
 
 // frame.slang:374-379
 __device__ inline bool tile_touched(uint32_t W, uint32_t H, uint32_t T, float cx, float cy, float a, float b,
-                                    float c, float ia, float ic, float cutoff, uint32_t tx, uint32_t ty) {
+                                    float c, float cutoff, uint32_t tx, uint32_t ty) {
     const float fx = (float)__umul24(tx, T) + 0.5f;
     const float fy = (float)__umul24(ty, T) + 0.5f;
     const uint32_t ex = __umul24(tx + 1, T) < W ? __umul24(tx + 1, T) : W;
     const uint32_t ey = __umul24(ty + 1, T) < H ? __umul24(ty + 1, T) : H;
     const float lx = (float)(ex - 1) + 0.5f;
     const float ly = (float)(ey - 1) + 0.5f;
-    return min_power_over_box(a, b, c, ia, ic, fx - cx, fy - cy, lx - cx, ly - cy) <= cutoff;
+    return min_power_over_box(a, b, c, fx - cx, fy - cy, lx - cx, ly - cy) <= cutoff;
 }
 
 // frame.slang:383-402: the footprint's tile rectangle, inclusive; false when
@@ -319,8 +319,7 @@ __global__ void __launch_bounds__(256) project_kernel(ProjectArgs a, const DView
         // rectangle.
         const uint32_t ctx = min(max((uint32_t)clamp_(cx, 0.0f, (float)(V.width - 1u)) / a.tile, x0), x1);
         const uint32_t cty = min(max((uint32_t)clamp_(cy, 0.0f, (float)(V.height - 1u)) / a.tile, y0), y1);
-        const bool settled = tile_touched(V.width, V.height, a.tile, cx, cy, ka, kb, kc, recip_of(ka), recip_of(kc),
-                                          cutoff, ctx, cty);
+        const bool settled = tile_touched(V.width, V.height, a.tile, cx, cy, ka, kb, kc, cutoff, ctx, cty);
         const uint32_t entries = h * (x1 / kStripTiles - x0 / kStripTiles + 1u);
         // Colour from this eye, in the cloud's space (splat_project.slang:781-785).
         const float dx = px - V.eye[0], dy = py - V.eye[1], dz = pz - V.eye[2];
@@ -474,15 +473,16 @@ struct BoxAxis {
     float lo, hi, q_lo, q_hi, s_lo, s_hi, t_lo, t_hi;
 };
 
-__device__ inline BoxAxis box_axis(uint32_t t, uint32_t T, uint32_t extent, float centre, float inv_across, float b,
+__device__ inline BoxAxis box_axis(uint32_t t, uint32_t T, uint32_t extent, float centre, float across, float b,
                                    float along, float b2) {
     BoxAxis r;
     const uint32_t e = __umul24(t + 1, T) < extent ? __umul24(t + 1, T) : extent;
     r.lo = ((float)__umul24(t, T) + 0.5f) - centre;
     r.hi = ((float)(e - 1) + 0.5f) - centre;
-    // The other coordinate's minimiser on each edge: ((-b) * edge) * (1 / across).
-    r.q_lo = ((-b) * r.lo) * inv_across;
-    r.q_hi = ((-b) * r.hi) * inv_across;
+    // The other coordinate's minimiser on each edge: ((-b) * edge) / across
+    // (read only when across > 1e-12).
+    r.q_lo = ((-b) * r.lo) / across;
+    r.q_hi = ((-b) * r.hi) / across;
     r.s_lo = along * r.lo * r.lo;
     r.s_hi = along * r.hi * r.hi;
     r.t_lo = b2 * r.lo;
@@ -518,10 +518,10 @@ __device__ inline bool box_hit(const BigFootprint& g, float b2, const BoxAxis& X
 }
 
 __device__ inline BoxAxis column_of(const BigFootprint& g, const DView& V, uint32_t T, float b2, uint32_t tx) {
-    return box_axis(tx, T, V.width, g.cx, g.ic, g.b, g.a, b2);
+    return box_axis(tx, T, V.width, g.cx, g.c, g.b, g.a, b2);
 }
 __device__ inline BoxAxis row_of(const BigFootprint& g, const DView& V, uint32_t T, float b2, uint32_t ty) {
-    return box_axis(ty, T, V.height, g.cy, g.ia, g.b, g.c, b2);
+    return box_axis(ty, T, V.height, g.cy, g.a, g.b, g.c, b2);
 }
 
 // Walks footprint g's steps, calling step(s, hit, tile) with this lane's tile
@@ -687,7 +687,6 @@ __global__ void __launch_bounds__(kSmallThreads) count_small_kernel(const DView*
     __shared__ uint32_t s_end[kSmallThreads];
     __shared__ float4 s_geo[kSmallThreads];     // centre x, y, cut power, conic a
     __shared__ float4 s_shape[kSmallThreads];   // conic b, c, x0 | y0 << 16, w | view << 16 (bits)
-    __shared__ float2 s_inv[kSmallThreads];     // 1 / conic a, 1 / conic c
     __shared__ uint64_t s_mask[kSmallThreads];
     const uint32_t lane = threadIdx.x & 63u, wbase = threadIdx.x & ~63u;
     const uint32_t slot = blockIdx.x * kSmallThreads + threadIdx.x;
@@ -700,7 +699,6 @@ __global__ void __launch_bounds__(kSmallThreads) count_small_kernel(const DView*
         s_geo[threadIdx.x] = make_float4(r0.x, r0.y, r0.w, r1.x);
         s_shape[threadIdx.x] = make_float4(r1.y, r1.z, __builtin_bit_cast(float, rect.x),
                                            __builtin_bit_cast(float, w | (slot / n) << 16));
-        s_inv[threadIdx.x] = make_float2(recip_of(r1.x), recip_of(r1.z));
     }
     const uint32_t end = wave_inclusive_sum(mine, lane);
     s_end[threadIdx.x] = end;
@@ -716,13 +714,12 @@ __global__ void __launch_bounds__(kSmallThreads) count_small_kernel(const DView*
         bool hit = false;
         if (on) {
             const float4 geo = s_geo[wbase + j], shape = s_shape[wbase + j];
-            const float2 inv = s_inv[wbase + j];
             const uint32_t corner = __builtin_bit_cast(uint32_t, shape.z);
             const uint32_t wv = __builtin_bit_cast(uint32_t, shape.w);
             const uint32_t wj = wv & 0xFFFFu;
             const DView& V = views[wv >> 16];
             const uint32_t q = small_row(k, wj);
-            hit = tile_touched(V.width, V.height, T, geo.x, geo.y, geo.w, shape.x, shape.y, inv.x, inv.y, geo.z,
+            hit = tile_touched(V.width, V.height, T, geo.x, geo.y, geo.w, shape.x, shape.y, geo.z,
                                (corner & 0xFFFFu) + (k - q * wj), (corner >> 16) + q);
         }
         // The first lane of each footprint's run in this step folds the
@@ -822,8 +819,8 @@ __global__ void __launch_bounds__(kSettleThreads) settle_count_kernel(const DVie
             for (uint32_t k0 = 0; k0 < area && !any; k0 += 64) {
                 const uint32_t k = k0 + lane;
                 const uint32_t r = k / g.w;
-                const bool hit = k < area && tile_touched(V.width, V.height, T, g.cx, g.cy, g.a, g.b, g.c, g.ia,
-                                                          g.ic, g.cutoff, g.x0 + (k - r * g.w), g.y0 + r);
+                const bool hit = k < area && tile_touched(V.width, V.height, T, g.cx, g.cy, g.a, g.b, g.c,
+                                                          g.cutoff, g.x0 + (k - r * g.w), g.y0 + r);
                 any = __ballot(hit) != 0;
             }
             if (lane == j) {
@@ -1157,12 +1154,12 @@ __global__ void __launch_bounds__(kEmitThreads) strip_mask_emit_kernel(const DVi
         g.ic = geo2.w;
         const uint32_t wh = __builtin_bit_cast(uint32_t, sl.w), width = wh & 0xFFFFu, height = wh >> 16;
         const float b2 = 2.0f * g.b;
-        const BoxAxis Y = box_axis(I.z + row, T, height, g.cy, g.ia, g.b, g.c, b2);
+        const BoxAxis Y = box_axis(I.z + row, T, height, g.cy, g.a, g.b, g.c, b2);
         const Decided d = decide_columns(g, SliceTerms{sl.x, sl.y, sl.z}, Y, T, width, tx0, c0, c1);
         uint32_t mask = d.sure;
         for (uint32_t t = d.test; t != 0; t &= t - 1u) {
             const uint32_t bit = (uint32_t)__builtin_ctz(t);
-            if (box_hit(g, b2, box_axis(tx0 + bit, T, width, g.cx, g.ic, g.b, g.a, b2), Y)) {
+            if (box_hit(g, b2, box_axis(tx0 + bit, T, width, g.cx, g.c, g.b, g.a, b2), Y)) {
                 mask |= 1u << bit;
             }
         }

@@ -124,6 +124,10 @@ class RenderTargets:
     depth: torch.Tensor  # [h, w]
     final_t: torch.Tensor | None = None  # [h, w]
     last: torch.Tensor | None = None  # [h, w] int32
+    # The number of the frame that last wrote these targets on its rasterizer
+    # (athx_frame_serial): the backward pass and the contributions walk that
+    # frame and are refused once a later render has replaced it.
+    frame_serial: int = 0
 
     @staticmethod
     def allocate(width: int, height: int, device, keep_backward: bool) -> "RenderTargets":
@@ -147,6 +151,7 @@ class FrameStats:
     ranges_ms: float = 0.0
     blend_ms: float = 0.0
     total_ms: float = 0.0
+    frame_serial: int = 0
 
 
 CULL_NAMES = ["unprojected", "edit", "depth", "degenerate", "faint", "offscreen", "no_tile"]
@@ -283,6 +288,14 @@ def _views_and_targets(views: list, targets: list[RenderTargets]):
     return c_views, c_targets
 
 
+def _serial_of(targets: list[RenderTargets]) -> int:
+    """The one frame every target was written by."""
+    serials = {t.frame_serial for t in targets}
+    if len(serials) != 1:
+        raise ValueError("the targets were written by different frames")
+    return serials.pop()
+
+
 def _grads_c(grads: dict[str, torch.Tensor]) -> _lib.Grads:
     cg = _lib.Grads()
     for name in ("means", "log_scales", "quats", "opacity_logits", "sh"):
@@ -416,7 +429,9 @@ class TileRasterizer:
         )
         out = FrameStats(stats.splats, stats.visible, stats.pairs, stats.project_ms, stats.depth_sort_ms,
                          stats.counts_ms, stats.emit_ms, stats.tile_sort_ms, stats.ranges_ms, stats.blend_ms,
-                         stats.total_ms)
+                         stats.total_ms, int(stats.frame_serial))
+        for t in targets:
+            t.frame_serial = int(stats.frame_serial)
         return targets, out
 
     def render_async(
@@ -440,7 +455,14 @@ class TileRasterizer:
         s = settings.to_c(cloud.linear)
         self._check(self._lib.athx_render_async(self._ctx, c_views, len(views), ctypes.byref(g), ctypes.byref(s),
                                                 c_targets, int(pair_capacity), _stream_handle(dev)))
+        serial = self.frame_serial()
+        for t in targets:
+            t.frame_serial = serial
         return targets
+
+    def frame_serial(self) -> int:
+        """athx_frame_serial: the number of the last frame rendered here (0: none)."""
+        return int(self._lib.athx_frame_serial(self._ctx))
 
     def frame_pairs(self) -> tuple[int, int]:
         """(pair total, pair capacity) of the last frame; waits for it."""
@@ -466,19 +488,30 @@ class TileRasterizer:
         words = {1: 1, 2: 2, 3: 2, 4: 4}.get(int(mode))
         if words is None:
             raise ValueError("contribution modes are 1 to 4")
-        out = torch.empty((slots, words) if words > 1 else (slots,), dtype=torch.int32, device=dev)
         dl = None
         keep = []
-        if dl_dcolour is not None:
+        if mode != 1:
+            if dl_dcolour is None or len(dl_dcolour) != nv:
+                raise ValueError(f"contribution mode {mode} needs one per-pixel input per view")
             dl = (ctypes.c_void_p * nv)()
             for k, t in enumerate(dl_dcolour):
-                t = t.contiguous()
+                want = (views[k].height, views[k].width) + ((4,) if mode in (2, 4) else ())
+                if not isinstance(t, torch.Tensor) or tuple(t.shape) != want:
+                    got = list(t.shape) if isinstance(t, torch.Tensor) else type(t).__name__
+                    raise ValueError(f"view {k}: mode {mode} reads a {list(want)} tensor, got {got}")
+                if not t.is_floating_point():
+                    raise ValueError(f"view {k}: the per-pixel input must be floating point")
+                if t.device != dev:
+                    raise ValueError(f"view {k}: the per-pixel input must be on {dev}, not {t.device}")
+                t = t.to(torch.float32).contiguous()
                 keep.append(t)
                 dl[k] = t.data_ptr()
+        out = torch.empty((slots, words) if words > 1 else (slots,), dtype=torch.int32, device=dev)
         g = _gaussians_c(cloud)
         s = settings.to_c(cloud.linear)
         self._check(self._lib.athx_contribution(self._ctx, c_views, nv, ctypes.byref(g), ctypes.byref(s), c_targets,
-                                                int(mode), dl, out.data_ptr(), _stream_handle(dev)))
+                                                _serial_of(targets), int(mode), dl, out.data_ptr(),
+                                                _stream_handle(dev)))
         return out
 
     def render_backward(
@@ -517,8 +550,9 @@ class TileRasterizer:
         g = _gaussians_c(cloud)
         s = settings.to_c(cloud.linear)
         self._check(
-            self._lib.athx_render_backward(self._ctx, c_views, nv, ctypes.byref(g), ctypes.byref(s), c_targets, dl,
-                                           ctypes.byref(cg), _stream_handle(_device_of(cloud)))
+            self._lib.athx_render_backward(self._ctx, c_views, nv, ctypes.byref(g), ctypes.byref(s), c_targets,
+                                           _serial_of(targets), dl, ctypes.byref(cg),
+                                           _stream_handle(_device_of(cloud)))
         )
         # ``keep`` may hold temporaries the queued kernels read; PyTorch's
         # allocator is stream-ordered, so releasing them here is safe.
@@ -584,3 +618,4 @@ class TileRasterizer:
             raise ValueError("keys must be int64 and values int32, of one length")
         self._check(self._lib.athx_sort_pairs(self._ctx, _ptr(keys), _ptr(values), keys.numel(), key_bits,
                                               _stream_handle(keys.device)))
+

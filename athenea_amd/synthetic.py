@@ -28,6 +28,7 @@ import numpy as np
 from .camera import Camera, orbit_camera
 
 SH_COEFFS = 16
+SH_CHUNK = 1 << 20
 
 
 @dataclass
@@ -74,9 +75,14 @@ def make_cloud(
     q /= np.linalg.norm(q, axis=1, keepdims=True)
     quats = q.astype(np.float32)
     logits = rng.normal(logit_mean, logit_sd, size=n).astype(np.float32)
-    sh = rng.normal(0.0, sh_sd, size=(n, SH_COEFFS, 3))
-    sh[:, 0, :] += 0.5
-    sh = sh.astype(np.float16 if half_sh else np.float32)
+    # The harmonics in chunks of rows: the same draws in the same order as
+    # one call, without an [n, 16, 3] float64 array (7.7 GB at 20M).
+    sh = np.empty((n, SH_COEFFS, 3), np.float16 if half_sh else np.float32)
+    for lo in range(0, n, SH_CHUNK):
+        hi = min(lo + SH_CHUNK, n)
+        block = rng.normal(0.0, sh_sd, size=(hi - lo, SH_COEFFS, 3))
+        block[:, 0, :] += 0.5
+        sh[lo:hi] = block
     return Cloud(means, log_scales, quats, logits, sh)
 
 

@@ -3,27 +3,44 @@
 A step is one forward frame of the 1080p workload (1.4M Gaussians, SH degree
 3, FP32, 1920x1080, 16x16 tiles, each Gaussian with relight attributes --
 albedo, roughness, metallic, a stored normal -- under a prepared 64x32
-log-normal HDR sky): projection with the relit colour, the depth sort, the binning of
-(tile, Gaussian) pairs through 16-tile strips, tile ranges and the blend, with the Gaussians and the sky
-already resident in HBM. Prints ONE JSON line:
+log-normal HDR sky): projection with the relit colour, the depth sort, the
+binning of (tile, Gaussian) pairs through 16-tile strips, tile ranges and the
+blend, with the Gaussians and the sky already resident in HBM. Prints ONE
+JSON line:
 
   value         forward ms/frame (lower is better)
+  parity        the timed frame against the oracle's frame of the same
+                inputs (the reference's two-sort route, rendered by the
+                cpu_baseline leg): pairs, pair values, ranges, image, final
+                transmittance, walk lengths -- "bit-exact" or what differs
   fwd_sh_ms     the same frame coloured by the harmonics instead
   fwd_bwd_ms    forward + adjoint backward ms/frame of the SH-coloured frame
                 (loss L2 against a target rendered from jittered means and SH)
-  batched       the 60-view walk (2.36M Gaussians, 960x540 views) in one
-                launch: views/s, and the per-view time of the same views
-                rendered one launch each
+  batched       the 60-view walk (2.36M Gaussians, 960x540 views): views/s
+                as one captured graph, with the tile side and blend batch the
+                autotuner picks, against the same views rendered one launch
+                each
+  mixed         20M Gaussians with FP16 SH, 8 views from 640x360 to 3840x2160
+                in one frame: ms/frame, views/s, pairs, strip entries
+  sort          athx_sort_pairs on 10M pairs with 32-bit keys (the
+                reference's own sort benchmark size): keys/s
   stages_ms     per-stage times from HIP events on the rasterizer's stream
-  roofline      the dominant stage's algorithmic bytes / its time vs 8 TB/s
+  roofline      the dominant kernel (blend_kernel): its VALU issue rate
+                against MI355X's one wave instruction a CU a clock, from the
+                committed rocprofv3 counters over its live event time; its
+                HBM traffic beside it
+  roofline_hbm  the projection's compulsory bytes over its event time
+                against 8 TB/s (the largest HBM-bound kernel)
   cpu_baseline  the oracle (the FP32 C restatement of the reference's
-                algorithm) on this host's cores, one frame of the same workload
+                algorithm, its two-sort route) on this host's cores, one frame
+                of the same workload, per stage
 """
 from __future__ import annotations
 
 import argparse
 import json
 import os
+import platform
 import statistics
 import sys
 import time
@@ -33,73 +50,48 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK = 8.0e12
+# MI355X: 256 CUs at 2.4 GHz; a CU's four SIMDs issue one wave64 VALU
+# instruction a clock between them (each takes four clocks for 64 lanes).
+CUS = 256
+CLOCK_HZ = 2.4e9
+VALU_PEAK = CUS * CLOCK_HZ  # wave instructions a second
 METRIC = "ms/frame fwd & fwd+bwd, 1.4M splats@1080p; batched views/s; % HBM peak"
+STAGES = ("project", "depth_sort", "counts", "emit", "tile_sort", "ranges", "blend")
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def stage_bytes(n: int, visible: int, pairs: int, entries: int, strips: int, tiles: int, pixels: int,
-                relit: bool = False) -> dict:
-    """Compulsory HBM bytes per stage (one view): what each stage must read
-    and write whatever its implementation."""
-    depth_passes = 3
-    strip_passes = (max(strips - 1, 1).bit_length() + 7) // 8
-    segments = entries // 1024 + strips
+def stage_bytes(n: int, visible: int, pairs: int, entries: int, tiles: int, pixels: int, relit: bool = False) -> dict:
+    """Compulsory HBM bytes per stage of one frame -- what each stage must
+    read and write whatever its implementation (the per-unit figures of the
+    rasterizer's algorithmic byte model: parameters and records once, (key,
+    value) once in and out of each sort's passes)."""
     return {
         # FP32 parameters read (236 B at SH degree 3, + 32 B of relight
-        # attributes: albedo, normal, metallic, roughness); record, depth key
-        # and strip-entry count written
+        # attributes); the 48 B record, depth key and entry count written
         "project": n * (236 + (32 if relit else 0) + 48 + 4 + 4),
-        # settling and compaction (counts and keys read, visible (key, slot)
-        # written), then
-        # a stable 24-bit sort of the visible u32 keys: a histogram read and a
-        # read and write of (key, slot) a pass
-        "depth_sort": n * 8 + visible * 8 + visible * 4 + depth_passes * visible * 16,
-        # sorted slots read, entry counts gathered
+        # depth keys read; visible (key, slot) compacted; a stable 24-bit sort
+        # of them, 3 passes of a read and a write of 8 B
+        "depth_sort": n * 4 + visible * 8 + 3 * visible * 16,
         "counts": visible * 8,
-        # sorted slot and the record's centre, conic and cut power read a rank
-        # (the tile rectangle is recomputed from them); (strip | tile mask,
-        # slot) written an entry; the strip sort: a histogram read, then a
-        # read and a write of (key, slot) a pass
-        "emit": visible * (4 + 32) + entries * 8 + entries * 4 + strip_passes * entries * 16,
-        # an entry's key read and its mask kept (count); the table written,
-        # scanned (read and written) and read; the mask and slot read again
-        # and each pair's slot written at its place
-        "tile_sort": entries * (4 + 2) + segments * 16 * 4 * 4 + entries * (2 + 4) + pairs * 4,
+        # the sorted slot and the record's 32 B of geometry read a drawn
+        # footprint; (strip | tile mask, slot) written an entry
+        "emit": visible * (4 + 32) + entries * 8,
+        # the entries read once; each pair's slot written
+        "tile_sort": entries * 8 + pairs * 4,
         "ranges": tiles * 8,
-        # each pair's slot read, each drawn record once (the blend's re-reads
-        # of a record hit the caches); RGBA, depth, final T, last index written
-        "blend": pairs * 4 + visible * 48 + pixels * 28,
+        # each pair's slot and record read, the targets written
+        "blend": pairs * (4 + 48) + pixels * 28,
     }
 
 
-# The kernels each timed stage launches, in order.
-STAGE_KERNELS = {
-    "project": "project_kernel",
-    "depth_sort": "settle_count_kernel (unsettled footprints settled, each block's drawn slots counted), the "
-                  "compaction, histogram_kernel, hist_reduce_kernel, onesweep_kernel x3",
-    "counts": "the strip-entry scan in depth order",
-    "emit": "strip_mask_emit_kernel (entries with their tiles decided), the strip sort (onesweep or "
-            "reduce-scan-scatter passes), ranges_kernel over the strips, the segment scan, seg_desc_kernel",
-    "tile_sort": "strip_mask_count_kernel, the table scan, strip_pairs_kernel",
-    "ranges": "tile_ranges_kernel",
-    "blend": "blend_kernel",
-}
-
-
-def measured_traffic(stage: str):
-    """HBM bytes a frame's stage moves, from the committed rocprofv3 --pmc
-    summary (FETCH_SIZE and WRITE_SIZE summed over the stage's kernels of
-    one frame, each scaled by what the counter reads for a copy of known
-    size): profiles/traffic_1080p_relit.json. None when absent."""
-    path = ROOT / "profiles" / "traffic_1080p_relit.json"
-    if not path.exists():
-        return None
-    stages = json.loads(path.read_text()).get("stages", {})
-    entry = stages.get(stage)
-    return int(entry["fetch_bytes"] + entry["write_bytes"]) if entry else None
+def committed_counters() -> dict:
+    """profiles/counters_1080p_relit.json: rocprofv3 --pmc values a launch of
+    the 1080p relit frame's kernels (VALU instructions, HBM bytes), or {}."""
+    path = ROOT / "profiles" / "counters_1080p_relit.json"
+    return json.loads(path.read_text()) if path.exists() else {}
 
 
 def copy_probe() -> None:
@@ -114,26 +106,151 @@ def copy_probe() -> None:
     torch.cuda.synchronize()
 
 
-def cpu_baseline(cloud_np, view, tile: int, relight) -> dict:
+def cpu_model() -> str:
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def cpu_baseline(cloud_np, view, tile: int, relight) -> tuple[dict, dict]:
+    """The oracle's reference-order frame, 1 warm-up and the median of 5 (the
+    reference's own protocol, CmdRender.cpp:379-437), per stage; returns the
+    baseline and the last frame (for the parity check)."""
     from oracle import pyoracle as po
 
     po.load()
     threads = po.threads()
     ov = po.orc_view(view, tile=tile)
-    times = []
-    po.render_ref_order(ov, cloud_np, relight=relight)  # warm-up
-    for _ in range(3):
+    frame = po.render_ref_order(ov, cloud_np, relight=relight)  # warm-up
+    totals, per_stage = [], []
+    for _ in range(5):
         t0 = time.perf_counter()
-        po.render_ref_order(ov, cloud_np, relight=relight)
-        times.append((time.perf_counter() - t0) * 1e3)
-    return {
-        "value": round(statistics.median(times), 2),
+        frame = po.render_ref_order(ov, cloud_np, relight=relight)
+        totals.append((time.perf_counter() - t0) * 1e3)
+        per_stage.append(frame["ms"])
+    names = ("project", "depth_sort", "counts", "emit", "tile_sort", "ranges", "blend")
+    stages = {k: round(statistics.median(float(ms[i]) for ms in per_stage), 2) for i, k in enumerate(names)}
+    out = {
+        "value": round(statistics.median(totals), 2),
         "unit": "ms/frame",
         "cores": threads,
         "kind": "port",
-        "sample": "one full forward frame of the same relit 1.4M-Gaussian 1920x1080 workload (project with the "
-                  "relit colour, 24-bit depth sort, emit, stable tile sort, ranges, blend), median of 3 after one "
-                  f"warm-up, OpenMP over {threads} host threads",
+        "cpu": cpu_model(),
+        "host_cpus": os.cpu_count(),
+        "stages_ms": stages,
+        "sample": "one full forward frame of the same relit 1.4M-Gaussian 1920x1080 workload through the oracle's "
+                  "reference-order route (project with the relit colour, 24-bit stable depth sort, emit, stable "
+                  "tile sort, ranges, blend), median of 5 after one warm-up; OpenMP over the threads in `cores` "
+                  "for projection, emission and blend, the two radix sorts single-threaded",
+    }
+    return out, frame
+
+
+def compare_with_oracle(gpu: dict, frame: dict) -> dict:
+    """The GPU frame (host copies) against the oracle's: every part equal
+    bit for bit, or the parts that are not."""
+    import numpy as np
+
+    checks = {
+        "pairs": gpu["pairs"] == frame["pairs"],
+        "pair_values": np.array_equal(gpu["pair_values"], frame["pair_values"]),
+        "ranges": np.array_equal(gpu["ranges"], frame["ranges"]),
+        "image": np.array_equal(gpu["colour"], frame["colour"]),
+        "depth": np.array_equal(gpu["depth"], frame["depth"]),
+        "final_t": np.array_equal(gpu["final_t"], frame["final_t"]),
+        "last": np.array_equal(gpu["last"], frame["last"]),
+    }
+    bad = [k for k, ok in checks.items() if not ok]
+    out = {"status": "bit-exact" if not bad else "differs", "checked": list(checks)}
+    if bad:
+        out["differs"] = bad
+        out["max_abs_image"] = float(np.abs(gpu["colour"].astype(np.float64) - frame["colour"]).max())
+    return out
+
+
+def timed_frames(raster, views, cloud, settings, steps, warmup):
+    import torch
+
+    targets = None
+    for _ in range(warmup):
+        targets, stats = raster.render(views, cloud, settings, targets)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        targets, stats = raster.render(views, cloud, settings, targets)
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) * 1e3 / steps, stats, targets
+
+
+def sort_leg(raster, n: int = 10_000_000, reps: int = 10) -> dict:
+    """RadixSort::sort's benchmark size (tests/gpu/test_algo.cpp:381-407): 10M
+    pairs with 32-bit keys, stable, timed with events on the stream."""
+    import torch
+
+    g = torch.Generator(device="cuda:0").manual_seed(7)
+    keys0 = torch.randint(0, 1 << 32, (n,), dtype=torch.int64, device="cuda:0", generator=g)
+    vals0 = torch.arange(n, dtype=torch.int32, device="cuda:0")
+    keys, vals = keys0.clone(), vals0.clone()
+    raster.sort_pairs(keys, vals, 32)  # warm-up, sizes the scratch
+    torch.cuda.synchronize()
+    ok = bool(torch.all(keys[1:] >= keys[:-1]).item())
+    times = []
+    stream = torch.cuda.current_stream()
+    for _ in range(reps):
+        keys.copy_(keys0)
+        vals.copy_(vals0)
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        raster.sort_pairs(keys, vals, 32)
+        b.record(stream)
+        b.synchronize()
+        times.append(a.elapsed_time(b))
+    ms = statistics.median(times)
+    return {"pairs": n, "key_bits": 32, "ms": round(ms, 4), "keys_per_s": round(n / (ms * 1e-3)),
+            "sorted": ok, "note": "64-bit key words holding 32-bit keys: 4 passes of 8 bits"}
+
+
+def mixed_leg(raster, steps: int) -> dict:
+    import numpy as np
+    import torch
+
+    from athenea_amd import RenderSettings, camera, synthetic
+    from athenea_amd.raster import GaussianCloud
+
+    log("mixed: generating 20M Gaussians (FP16 SH)")
+    cloud_np = synthetic.mixed_cloud()
+    cloud = GaussianCloud.from_numpy(cloud_np, "cuda:0")
+    del cloud_np
+    cams = synthetic.walk_cameras(60)[:8]
+    views = [camera.view_for(c, w, h) for c, (w, h) in zip(cams, synthetic.MIXED_SIZES)]
+    settings = RenderSettings()
+    reps = max(2, steps // 4)
+    ms, stats, targets = timed_frames(raster, views, cloud, settings, reps, 1)
+    info = raster.frame_info()
+    staged = RenderSettings(time_stages=True)
+    targets, s = raster.render(views, cloud, staged, targets)
+    finite = all(bool(torch.isfinite(t.colour).all().item()) for t in targets)
+    del targets, cloud
+    torch.cuda.empty_cache()
+    raster.release_scratch()
+    pixels = int(sum(w * h for w, h in synthetic.MIXED_SIZES))
+    return {
+        "workload": "mixed_8: 20M Gaussians, FP16 SH, 8 views (3840x2160, 1920x1080, 1280x720, 640x360, two each) "
+                    "on the walk, one frame",
+        "ms_per_frame": round(ms, 3),
+        "views_per_s": round(8 / (ms * 1e-3), 1),
+        "pixels": pixels,
+        "pairs": int(stats.pairs),
+        "visible": int(stats.visible),
+        "strip_entries": int(info["entries"]),
+        "depth_sort_keys_per_s": round(s.visible / (s.depth_sort_ms * 1e-3)) if s.depth_sort_ms > 0 else None,
+        "stages_ms": {k: round(getattr(s, f"{k}_ms"), 3) for k in STAGES},
+        "images_finite": finite,
+        "parity": "tests/test_gpu_configs.py checks this shape bit-exact against the oracle at 2M Gaussians",
     }
 
 
@@ -144,9 +261,12 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--tile", type=int, default=16)
     ap.add_argument("--batch", type=int, default=256)
-    ap.add_argument("--no-extras", action="store_true", help="forward only: no backward, batch or CPU legs")
+    ap.add_argument("--no-extras", action="store_true", help="forward only: no backward, batch, mixed, sort or CPU legs")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the 60-view walk")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the 20M mixed-resolution frame")
+    ap.add_argument("--no-sort", action="store_true", help="skip the 10M-pair sort")
+    ap.add_argument("--no-autotune", action="store_true", help="the walk at --tile/--batch instead of tuned")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run one 256 MiB device-to-device copy, the known-size kernel that scales the "
                          "FETCH_SIZE / WRITE_SIZE counters of a rocprofv3 --pmc run of this command")
@@ -175,47 +295,75 @@ def main() -> int:
     settings = RenderSettings(tile=args.tile, batch=args.batch)
     raster = TileRasterizer(0)
 
-    def time_forward(c, steps, warmup):
-        targets = None
-        for _ in range(warmup):
-            targets, stats = raster.render([view], c, settings, targets)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(steps):
-            targets, stats = raster.render([view], c, settings, targets)
-        torch.cuda.synchronize()
-        return (time.perf_counter() - t0) * 1e3 / steps, stats, targets
-
     log(f"timing {args.steps} SH-coloured forward frames")
-    fwd_sh_ms, _, _ = time_forward(cloud_sh, args.steps, args.warmup)
+    fwd_sh_ms, _, _ = timed_frames(raster, [view], cloud_sh, settings, args.steps, args.warmup)
     log(f"timing {args.steps} relit forward frames")
-    fwd_ms, stats, targets = time_forward(cloud, args.steps, args.warmup)
+    fwd_ms, stats, targets = timed_frames(raster, [view], cloud, settings, args.steps, args.warmup)
     pairs = int(stats.pairs)
     info = raster.frame_info()
 
-    # Per-stage times from HIP events recorded on the rasterizer's stream.
+    # Per-stage times from HIP events recorded on the rasterizer's stream:
+    # "blend" is blend_kernel's launch alone.
     staged = RenderSettings(tile=args.tile, batch=args.batch, time_stages=True, count_splats=True)
-    stage_samples = {k: [] for k in ("project", "depth_sort", "counts", "emit", "tile_sort", "ranges", "blend")}
+    stage_samples = {k: [] for k in STAGES}
     for _ in range(max(3, min(args.steps, 10))):
         targets, s = raster.render([view], cloud, staged, targets)
         for k in stage_samples:
             stage_samples[k].append(getattr(s, f"{k}_ms"))
-    stages = {k: statistics.mean(v) for k, v in stage_samples.items()}
+    stages = {k: statistics.median(v) for k, v in stage_samples.items()}
     counters = raster.latest_counters()
+    # The frame kept for the parity check (host copies of the last staged frame).
+    bufs = raster.frame_buffers()
+    gpu_frame = {
+        "pairs": int(s.pairs),
+        "pair_values": bufs["pair_values"].cpu().numpy().view(np.uint32),
+        "ranges": bufs["ranges"].cpu().numpy().view(np.uint32).reshape(-1),
+        "colour": targets[0].colour.cpu().numpy(),
+        "depth": targets[0].depth.cpu().numpy(),
+    }
+    del bufs
+    # final_t and last: one more frame with them kept.
+    kept, _ = raster.render([view], cloud, settings, keep_backward=True)
+    gpu_frame["final_t"] = kept[0].final_t.cpu().numpy()
+    gpu_frame["last"] = kept[0].last.cpu().numpy().view(np.uint32)
+    walked = int(kept[0].last.to(torch.int64).sum().item())  # (pixel, pair) evaluations of the blend
+    del kept
+    torch.cuda.synchronize()
+
     tiles = info["tiles"]
-    sbytes = stage_bytes(cloud.count, counters.visible, pairs, info["entries"], info["strips"], tiles, W * H,
-                         relit=True)
-    dominant = max(stages, key=stages.get)
-    achieved = sbytes[dominant] / (stages[dominant] * 1e-3)
+    sbytes = stage_bytes(cloud.count, counters.visible, pairs, info["entries"], tiles, W * H, relit=True)
+    prof = committed_counters().get("kernels", {})
+    blend_prof = prof.get("blend_kernel", {})
+    blend_s = stages["blend"] * 1e-3
+    valu = blend_prof.get("SQ_INSTS_VALU")
     roofline = {
-        "kernel": f"{dominant} ({STAGE_KERNELS[dominant]})",
+        "kernel": "blend_kernel",
+        "bound": "valu",
+        "achieved": round(valu / blend_s / 1e9, 1) if valu else None,
+        "peak": round(VALU_PEAK / 1e9, 1),
+        "unit": "G VALU wave-instructions/s",
+        "frac": round(valu / blend_s / VALU_PEAK, 4) if valu else None,
+        "traffic": blend_prof.get("hbm_bytes"),
+        "hbm_frac": round(blend_prof["hbm_bytes"] / blend_s / HBM_PEAK, 4) if blend_prof.get("hbm_bytes") else None,
+        "ms": round(stages["blend"], 4),
+        "evaluations": walked,
+        "evaluations_per_s": round(walked / blend_s),
+        "valu_per_evaluation": round(valu * 64 / walked, 2) if valu else None,
+        "counters": "profiles/counters_1080p_relit.json (rocprofv3 --pmc, a launch of this frame)",
+    }
+    proj_s = stages["project"] * 1e-3
+    proj_prof = prof.get("project_kernel", {})
+    roofline_hbm = {
+        "kernel": "project_kernel",
         "bound": "hbm",
-        "achieved": round(achieved / 1e9, 1),
+        "achieved": round(sbytes["project"] / proj_s / 1e9, 1),
         "peak": HBM_PEAK / 1e9,
         "unit": "GB/s",
-        "frac": round(achieved / HBM_PEAK, 4),
-        "traffic": measured_traffic(dominant),
-        "algorithmic_bytes": sbytes[dominant],
+        "frac": round(sbytes["project"] / proj_s / HBM_PEAK, 4),
+        "traffic": proj_prof.get("hbm_bytes"),
+        "algorithmic_bytes": sbytes["project"],
+        "valu_frac": round(proj_prof["SQ_INSTS_VALU"] / proj_s / VALU_PEAK, 4) if proj_prof.get("SQ_INSTS_VALU")
+        else None,
     }
     result = {
         "metric": METRIC,
@@ -244,8 +392,11 @@ def main() -> int:
         "strip_entries": info["entries"],
         "visible": counters.visible,
         "stages_ms": {k: round(v, 4) for k, v in stages.items()},
+        "stage_hbm_frac": {k: round(sbytes[k] / (stages[k] * 1e-3) / HBM_PEAK, 4) for k in STAGES if stages[k] > 0},
+        "depth_sort_keys_per_s": round(counters.visible / (stages["depth_sort"] * 1e-3)),
         "pairs_binned_per_s": round(pairs / ((stages["emit"] + stages["tile_sort"]) * 1e-3)),
         "roofline": roofline,
+        "roofline_hbm": roofline_hbm,
     }
 
     if args.copy_probe:
@@ -283,25 +434,23 @@ def main() -> int:
         del params
 
     if not args.no_extras and not args.no_batch:
+        from athenea_amd.raster import FrameGraph
+
         log("batched 60-view walk")
         walk_np = synthetic.walk_cloud()
         walk = GaussianCloud.from_numpy(walk_np, dev)
         views = [camera.view_for(c, 960, 540) for c in synthetic.walk_cameras(60)]
-        bt = None
-        for _ in range(2):
-            bt, bstats = raster.render(views, walk, settings, bt)
-        torch.cuda.synchronize()
         reps = max(2, args.steps // 5)
-        t0 = time.perf_counter()
-        for _ in range(reps):
-            bt, bstats = raster.render(views, walk, settings, bt)
-        torch.cuda.synchronize()
-        batch_ms = (time.perf_counter() - t0) * 1e3 / reps
-        del bt
-        # The same batch as one captured graph: no host wait, one launch.
-        from athenea_amd.raster import FrameGraph
+        tuned = settings
+        table = None
+        if not args.no_autotune:
+            from athenea_amd.autotune import autotune
 
-        graph = FrameGraph(views, walk, settings)
+            log("autotuning the walk's tile side and blend batch")
+            tuned, table = autotune(raster, views, walk, settings, reps=2)
+        batch_ms, bstats, _ = timed_frames(raster, views, walk, tuned, reps, 1)
+        # The same batch as one captured graph: no host wait, one launch.
+        graph = FrameGraph(views, walk, tuned)
         graph.launch()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -312,6 +461,9 @@ def main() -> int:
         gpairs, gcap = graph.pairs()
         graph.close()
         del graph
+        raster.release_scratch()
+        # The plain single-view rasterizer: each view its own render, at the
+        # reference's own tile and batch.
         single = [None] * len(views)
         for k, v in enumerate(views):
             single[k], _ = raster.render([v], walk, settings, single[k])
@@ -322,6 +474,7 @@ def main() -> int:
                 single[k], _ = raster.render([v], walk, settings, single[k])
         torch.cuda.synchronize()
         single_ms = (time.perf_counter() - t0) * 1e3 / reps
+        del single
         result["batched"] = {
             "workload": "walk_60: 2.36M Gaussians, 60 views at 960x540 on a circular walk",
             "views_per_s": round(60 / (graph_ms * 1e-3), 1),
@@ -329,14 +482,29 @@ def main() -> int:
             "ms_per_view_batched_eager": round(batch_ms / 60, 4),
             "ms_per_view_single": round(single_ms / 60, 4),
             "speedup_graph_vs_single": round(single_ms / graph_ms, 3),
+            "tile": tuned.tile,
+            "batch": tuned.batch,
+            "autotune": table,
             "pairs": int(bstats.pairs),
             "graph_pairs_within_capacity": bool(gpairs <= gcap),
         }
+        del walk
+        torch.cuda.empty_cache()
         raster.release_scratch()
 
+    if not args.no_extras and not args.no_sort:
+        log("sort: 10M pairs, 32-bit keys")
+        result["sort"] = sort_leg(raster)
+        raster.release_scratch()
+
+    if not args.no_extras and not args.no_mixed:
+        result["mixed"] = mixed_leg(raster, args.steps)
+
     if not args.no_extras and not args.no_cpu:
-        log("CPU baseline (the oracle on the host cores)")
-        result["cpu_baseline"] = cpu_baseline(cloud_np, view, args.tile, attrs)
+        log("CPU baseline (the oracle on the host cores) and the parity check")
+        base, frame = cpu_baseline(cloud_np, view, args.tile, attrs)
+        result["cpu_baseline"] = base
+        result["parity"] = compare_with_oracle(gpu_frame, frame)
 
     print(json.dumps(result), flush=True)
     return 0

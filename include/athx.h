@@ -2,8 +2,9 @@
  * athx.h -- the C ABI of the MI355X Gaussian-splat tile rasterizer.
  *
  * Plain C: device pointers, sizes and a hipStream_t passed as void*. No
- * exceptions cross it; every call returns an athx_status whose values are
- * athenea's ErrorCode (modules/core/include/athenea/core/Error.h:14-26), and
+ * exceptions cross it; every call returns an athx_status: 0 for success, else
+ * athenea's ErrorCode (modules/core/include/athenea/core/Error.h:14-26) plus
+ * one, in the enum's order (athx_error_code() gives the ErrorCode value), and
  * athx_last_error() returns the message of the last failure on a context.
  *
  * What each entry point replaces in the reference:
@@ -64,7 +65,10 @@ extern "C" {
 #define ATHX_API __attribute__((visibility("default")))
 #endif
 
-/* athenea's ErrorCode, value for value (Error.h:14-26), plus 0 for success. */
+/* 0 for success, else athenea's ErrorCode + 1 (Error.h:14-26 numbers
+ * Unknown 0, NotFound 1, ... InternalError 10; a Result<T> holds success
+ * apart from the code, which a C status cannot). athx_error_code() maps a
+ * status back to the ErrorCode value. */
 typedef enum athx_status {
     ATHX_OK = 0,
     ATHX_UNKNOWN = 1,
@@ -92,6 +96,11 @@ enum {
     ATHX_CULL_NO_TILE = 6,
     ATHX_CULL_REASONS = 7
 };
+
+/* The athenea ErrorCode value of a failed status (ATHX_UNKNOWN -> 0 =
+ * ErrorCode::Unknown, ... ATHX_INTERNAL_ERROR -> 10), or -1 for ATHX_OK and
+ * for a value outside the enum. */
+ATHX_API int athx_error_code(athx_status status);
 
 typedef struct athx_ctx athx_ctx;
 
@@ -248,6 +257,7 @@ typedef struct athx_frame_stats {
     uint32_t visible;
     uint64_t pairs;
     double project_ms, depth_sort_ms, counts_ms, emit_ms, tile_sort_ms, ranges_ms, blend_ms, total_ms;
+    uint64_t frame_serial; /* this frame's number on the context (athx_frame_serial) */
 } athx_frame_stats;
 
 /* SplatCounters (TileRasterizer.h:370-401) for the whole frame. */
@@ -342,15 +352,25 @@ ATHX_API athx_status athx_graph_pairs(athx_graph* graph, uint64_t* pairs, uint64
 ATHX_API void athx_graph_destroy(athx_graph* graph);
 
 /*
- * The adjoint of the last athx_render on ctx (same views, Gaussians and
- * settings; targets[k].final_t and .last must have been written by it):
- * dL/dcolour is [height][width][4] per view (premultiplied RGBA, FP32), and
- * gradients are accumulated into grads. Queued, not waited for.
+ * The number of the last frame rendered on ctx (athx_render, athx_render_async;
+ * 1 for the first), 0 before any or after athx_release_scratch. A context
+ * holds one frame's pairs and records: athx_render_backward and
+ * athx_contribution take the number of the frame they follow and refuse any
+ * other (INVALID_ARGUMENT), so a frame replaced by a later render is never
+ * walked with the later frame's pairs.
+ */
+ATHX_API uint64_t athx_frame_serial(const athx_ctx* ctx);
+
+/*
+ * The adjoint of frame frame_serial, the last athx_render on ctx (same views,
+ * Gaussians and settings; targets[k].final_t and .last must have been written
+ * by it): dL/dcolour is [height][width][4] per view (premultiplied RGBA,
+ * FP32), and gradients are accumulated into grads. Queued, not waited for.
  */
 ATHX_API athx_status athx_render_backward(athx_ctx* ctx, const athx_view* views, uint32_t nviews,
                                           const athx_gaussians* gaussians, const athx_settings* settings,
-                                          const athx_targets* targets, const float* const* dl_dcolour,
-                                          const athx_grads* grads, void* stream);
+                                          const athx_targets* targets, uint64_t frame_serial,
+                                          const float* const* dl_dcolour, const athx_grads* grads, void* stream);
 
 /*
  * What each splat gave the last frame (splat_blend.slang:36-48, 364-398:
@@ -372,12 +392,13 @@ ATHX_API athx_status athx_render_backward(athx_ctx* ctx, const athx_view* views,
  *   mode 3: two words (splat_lift.slang): the sum of round(T alpha m * 1024)
  *           with m the pixel's mask clamped to [0, 1], then the sum of
  *           round(T alpha * 1024); dl_dcolour[k] the mask, [height][width].
- * The views, gaussians, settings and targets are the last render's. Queued.
+ * The views, gaussians, settings and targets are the last render's, frame
+ * frame_serial (athx_frame_serial). Queued.
  */
 ATHX_API athx_status athx_contribution(athx_ctx* ctx, const athx_view* views, uint32_t nviews,
                                        const athx_gaussians* gaussians, const athx_settings* settings,
-                                       const athx_targets* targets, uint32_t mode, const float* const* dl_dcolour,
-                                       uint32_t* out, void* stream);
+                                       const athx_targets* targets, uint64_t frame_serial, uint32_t mode,
+                                       const float* const* dl_dcolour, uint32_t* out, void* stream);
 
 /* The counters of the newest frame rendered with count_splats (waits for it). */
 ATHX_API athx_status athx_latest_counters(athx_ctx* ctx, athx_counters* out);

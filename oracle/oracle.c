@@ -128,23 +128,21 @@ static float paid_opacity(float alpha, float paid, uint32_t* flags) {
     return spread;
 }
 
-/* frame.slang:354-371, the edge minimiser ((-b) x) / c taken as
- * ((-b) x) * (1 / c): the reference's divide has no defined rounding across
- * its backends, and this is the form the kernels compute too. */
+/* frame.slang:354-371: the least of a x^2 + 2 b x y + c y^2 over a box, each
+ * edge at its clamped 1D minimiser -b * x / c (that is ((-b) x) / c). */
 static float min_power_over_box(float a, float b, float c, float lox, float loy, float hix, float hiy) {
     if (lox <= 0.0f && hix >= 0.0f && loy <= 0.0f && hiy >= 0.0f) {
         return 0.0f;
     }
-    const float ia = a > 1e-12f ? 1.0f / a : 0.0f, ic = c > 1e-12f ? 1.0f / c : 0.0f;
     float best = 3.4e38f;
     for (int e = 0; e < 2; ++e) {
         const float x = e == 0 ? lox : hix;
-        const float y = c > 1e-12f ? clampf_(((-b) * x) * ic, loy, hiy) : loy;
+        const float y = c > 1e-12f ? clampf_(((-b) * x) / c, loy, hiy) : loy;
         best = minf_(best, a * x * x + 2.0f * b * x * y + c * y * y);
     }
     for (int e = 0; e < 2; ++e) {
         const float y = e == 0 ? loy : hiy;
-        const float x = a > 1e-12f ? clampf_(((-b) * y) * ia, lox, hix) : lox;
+        const float x = a > 1e-12f ? clampf_(((-b) * y) / a, lox, hix) : lox;
         best = minf_(best, a * x * x + 2.0f * b * x * y + c * y * y);
     }
     return best;

@@ -398,7 +398,8 @@ def render_ref_order(v: OrcView, cloud, pair_cap: int | None = None, relight=Non
                                  ctypes.byref(rl) if rl is not None else None)
     del keep
     return {"pairs": int(P), "pair_values": pv[:P], "ranges": ranges, "colour": col.reshape(H, W, 4),
-            "depth": dep.reshape(H, W), "ms": ms, "records": rec, "touched": touched}
+            "depth": dep.reshape(H, W), "final_t": ft.reshape(H, W), "last": last.reshape(H, W), "ms": ms,
+            "records": rec, "touched": touched}
 
 
 def reference_brute_force(v: OrcView, records: np.ndarray, touched: np.ndarray, max_hits: int = 1024):

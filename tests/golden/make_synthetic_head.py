@@ -12,7 +12,7 @@ from athenea_amd import synthetic  # noqa: E402
 
 out = {}
 for name, make in (("plain_512", synthetic.plain_512_cloud), ("plain_1080p", synthetic.plain_1080p_cloud),
-                   ("walk", synthetic.walk_cloud)):
+                   ("walk", synthetic.walk_cloud), ("mixed", synthetic.mixed_cloud)):
     c = make(16)
     for field in ("means", "log_scales", "quats", "opacity_logits", "sh"):
         out[f"{name}_{field}"] = getattr(c, field)

@@ -89,15 +89,28 @@ def test_ctypes_layouts_match_the_header():
             assert seen[f"{cname}.{field}"] == getattr(py, field).offset, (cname, field)
 
 
-def test_status_codes_are_athenea_error_codes():
-    """include/athx.h's athx_status is modules/core/include/athenea/core/
-    Error.h:14-26's ErrorCode, value for value, with 0 for success."""
+# modules/core/include/athenea/core/Error.h:14-26: enum class ErrorCode, in
+# declaration order, so ErrorCode::Unknown is 0 and InternalError 10.
+ERROR_CODE_ORDER = ["Unknown", "NotFound", "InvalidArgument", "Unsupported", "IoFailure", "DeviceFailure",
+                    "ShaderFailure", "PluginFailure", "OutOfMemory", "Cancelled", "InternalError"]
+
+
+def test_status_codes_map_to_athenea_error_codes():
+    """athx_status is 0 for success and ErrorCode + 1 otherwise, in the
+    reference enum's order; athx_error_code() gives back the ErrorCode value."""
     text = HEADER.read_text()
     values = dict((name, int(v)) for name, v in re.findall(r"ATHX_(\w+) = (\d+)", text) if not name.startswith("CULL"))
-    expected = ["OK", "UNKNOWN", "NOT_FOUND", "INVALID_ARGUMENT", "UNSUPPORTED", "IO_FAILURE", "DEVICE_FAILURE",
-                "SHADER_FAILURE", "PLUGIN_FAILURE", "OUT_OF_MEMORY", "CANCELLED", "INTERNAL_ERROR"]
-    assert [values[e] for e in expected] == list(range(12))
-    assert [_lib.STATUS_NAMES[i] for i in range(12)][3] == "InvalidArgument"
+    header_names = ["UNKNOWN", "NOT_FOUND", "INVALID_ARGUMENT", "UNSUPPORTED", "IO_FAILURE", "DEVICE_FAILURE",
+                    "SHADER_FAILURE", "PLUGIN_FAILURE", "OUT_OF_MEMORY", "CANCELLED", "INTERNAL_ERROR"]
+    assert values["OK"] == 0
+    lib = _lib.load()
+    for code, (hname, rname) in enumerate(zip(header_names, ERROR_CODE_ORDER)):
+        status = values[hname]
+        assert status == code + 1, hname
+        assert lib.athx_error_code(status) == code, hname
+        assert _lib.STATUS_NAMES[status] == rname
+        assert _lib.AthxError(status, "").error_code == code
+    assert lib.athx_error_code(0) == -1 and lib.athx_error_code(12) == -1 and lib.athx_error_code(-3) == -1
 
 
 def test_calls_fail_cleanly_without_a_device():
@@ -119,3 +132,40 @@ def test_sky_texel_count_is_the_layouts():
     lib = _lib.load()
     for base in (16, 100, 256, 300, 512, 1024):
         assert lib.athx_sky_texels(base) == sky.texels_per_dome(base)
+
+
+def test_contribution_refuses_wrong_per_pixel_inputs():
+    """TileRasterizer.contribution checks each per-pixel input before the
+    library reads it: mode 2 and 4 read [h, w, 4], mode 3 [h, w], floating
+    point, on the cloud's device. No device is needed: every refusal comes
+    before any call into the library."""
+    torch = pytest.importorskip("torch")
+    from athenea_amd import camera, synthetic
+    from athenea_amd.raster import GaussianCloud, RenderTargets, TileRasterizer
+
+    raster = object.__new__(TileRasterizer)  # no context: nothing may reach the library
+    raster._lib = None
+    raster._ctx = None
+    view = camera.view_for(synthetic.front_camera(), 8, 6)
+    cloud = GaussianCloud(torch.zeros(2, 3), torch.zeros(2, 3), torch.zeros(2, 4), torch.zeros(2),
+                          torch.zeros(2, 1, 3))
+    targets = [RenderTargets.allocate(8, 6, "cpu", True)]
+    bad = {
+        4: [torch.zeros(6, 8), torch.zeros(6, 8, 4, dtype=torch.int32), torch.zeros(6, 8, 3), None],
+        2: [torch.zeros(6, 8)],
+        3: [torch.zeros(6, 8, 4), torch.zeros(8, 6)],
+    }
+    for mode, inputs in bad.items():
+        for t in inputs:
+            with pytest.raises(ValueError):
+                raster.contribution([view], cloud, None, targets, mode=mode, dl_dcolour=[t])
+    with pytest.raises(ValueError):
+        raster.contribution([view], cloud, None, targets, mode=4, dl_dcolour=None)
+    with pytest.raises(ValueError):
+        raster.contribution([view], cloud, None, targets, mode=5)
+    # The right shape on another device than the cloud's.
+    meta = GaussianCloud(*(t.to("meta") for t in (cloud.means, cloud.log_scales, cloud.quats,
+                                                   cloud.opacity_logits, cloud.sh)))
+    meta_targets = [RenderTargets(8, 6, torch.zeros(6, 8, 4, device="meta"), torch.zeros(6, 8, device="meta"))]
+    with pytest.raises(ValueError, match="must be on"):
+        raster.contribution([view], meta, None, meta_targets, mode=4, dl_dcolour=[torch.zeros(6, 8, 4)])

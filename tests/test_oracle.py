@@ -327,7 +327,150 @@ def test_multi_view_frame_is_each_view_alone():
 def test_generator_is_pinned_by_the_golden_fixture():
     fixture = np.load(po.HERE.parent / "tests" / "golden" / "synthetic_head.npz")
     for name, make in (("plain_512", synthetic.plain_512_cloud), ("plain_1080p", synthetic.plain_1080p_cloud),
-                       ("walk", synthetic.walk_cloud)):
+                       ("walk", synthetic.walk_cloud), ("mixed", synthetic.mixed_cloud)):
         c = make(16)
         for field in ("means", "log_scales", "quats", "opacity_logits", "sh"):
             assert np.array_equal(getattr(c, field), fixture[f"{name}_{field}"]), (name, field)
+
+
+# --- the tile test against brute force ------------------------------------------------------------
+
+def _box_min_f64(a, b, c, lox, loy, hix, hiy):
+    """Least of q = a x^2 + 2 b x y + c y^2 over a box, in FP64: zero when the
+    box holds the centre; else, q being convex, the least over the four
+    edges, each a 1D quadratic minimised where its derivative vanishes."""
+    if lox <= 0 <= hix and loy <= 0 <= hiy:
+        return 0.0
+    q = lambda x, y: a * x * x + 2 * b * x * y + c * y * y  # noqa: E731
+    best = np.inf
+    for x in (lox, hix):
+        y = min(max(-b * x / c, loy), hiy) if c > 0 else loy
+        best = min(best, q(x, y))
+    for y in (loy, hiy):
+        x = min(max(-b * y / a, lox), hix) if a > 0 else lox
+        best = min(best, q(x, y))
+    return best
+
+
+def test_box_minimum_matches_dense_sampling():
+    """The FP64 box minimum used below, against a dense grid over the box
+    (the grid's least is an upper bound within one grid step of it)."""
+    rng = np.random.default_rng(9)
+    for _ in range(200):
+        s1, s2 = np.exp(rng.uniform(-1, 4, 2))
+        th = rng.uniform(0, np.pi)
+        ct, st = np.cos(th), np.sin(th)
+        cov = np.array([[ct * ct * s1 * s1 + st * st * s2 * s2, ct * st * (s1 * s1 - s2 * s2)],
+                        [ct * st * (s1 * s1 - s2 * s2), st * st * s1 * s1 + ct * ct * s2 * s2]])
+        a, b, c = np.linalg.inv(cov)[0, 0], np.linalg.inv(cov)[0, 1], np.linalg.inv(cov)[1, 1]
+        lox, loy = rng.uniform(-60, 40, 2)
+        hix, hiy = lox + 15, loy + 15
+        xs, ys = np.meshgrid(np.linspace(lox, hix, 301), np.linspace(loy, hiy, 301))
+        grid = (a * xs * xs + 2 * b * xs * ys + c * ys * ys).min()
+        m = _box_min_f64(a, b, c, lox, loy, hix, hiy)
+        assert m <= grid + 1e-12 * max(1.0, grid)
+        assert grid - m <= 1e-3 * max(1.0, grid)
+
+
+@pytest.mark.parametrize("tile", [8, 16])
+def test_tile_membership_by_brute_force(tile):
+    """The tiles each footprint is paired with (orc_project's counts and
+    orc_emit_pairs' pairs), against brute force in FP64: every tile holding a
+    pixel centre whose power is within the cut is paired, and every paired
+    tile's box of pixel centres comes within the cut (frame.slang:354-379),
+    both up to a relative 1e-5 for the FP32 rounding at the edge."""
+    cloud = random_cloud(600, 21, min_size=0.01, max_size=0.5)
+    cam = Camera.looking_at((0.3, -0.2, 5.0), (0, 0, 0))
+    v = camera.view_for(cam, 200, 150)
+    ov = po.orc_view(v, tile=tile)
+    frame = po.render([ov], cloud)
+    rec, touched = frame["records"], frame["touched"]
+    tx_n, ty_n = po.tiles_of(ov)
+    paired = [set() for _ in range(rec.shape[0])]
+    ranges = frame["ranges"].reshape(-1, 2)
+    for t in range(tx_n * ty_n):
+        for p in range(ranges[t, 0], ranges[t, 1]):
+            paired[frame["pair_values"][p]].add(t)
+    px = np.arange(ov.width) + 0.5
+    py = np.arange(ov.height) + 0.5
+    checked = 0
+    for s in np.nonzero(touched)[0]:
+        cx, cy, _, cut, a, b, c = (float(x) for x in rec[s, :7])
+        assert len(paired[s]) == touched[s]
+        dx, dy = px - cx, py - cy
+        power = a * dx[None, :] ** 2 + 2 * b * dx[None, :] * dy[:, None] + c * dy[:, None] ** 2
+        inside = power <= cut * (1 - 1e-5)
+        for t in range(tx_n * ty_n):
+            tx, ty = t % tx_n, t // tx_n
+            x0, y0 = tx * tile, ty * tile
+            x1, y1 = min(x0 + tile, ov.width), min(y0 + tile, ov.height)
+            if inside[y0:y1, x0:x1].any():
+                assert t in paired[s], (s, t)
+            if t in paired[s]:
+                m = _box_min_f64(a, b, c, x0 + 0.5 - cx, y0 + 0.5 - cy, x1 - 0.5 - cx, y1 - 0.5 - cy)
+                assert m <= cut * (1 + 1e-5) + 1e-6, (s, t, m, cut)
+                checked += 1
+    assert checked > 1000
+
+
+# --- the image comparisons against numpy ----------------------------------------------------------
+
+def _codes_f64(x):
+    c = np.clip(x.astype(np.float64), 0.0, 1.0)
+    e = np.where(c <= 0.0031308, c * 12.92, 1.055 * c ** (1 / 2.4) - 0.055) * 255.0
+    return e
+
+
+def test_compare_codes_against_numpy():
+    """image_compare.slang's histogram of the per-pixel largest code
+    difference (rgb through the sRGB curve, alpha linear), restated in numpy
+    FP64; pixels whose code sits within 1e-3 of a rounding boundary are
+    redrawn so FP32 and FP64 cannot round them apart."""
+    rng = np.random.default_rng(4)
+    h, w = 37, 53
+    a = rng.uniform(-0.1, 1.1, (h, w, 4)).astype(np.float32)
+    b = np.clip(a + rng.normal(0, 0.01, a.shape), -0.2, 1.2).astype(np.float32)
+    for img in (a, b):
+        for _ in range(20):
+            ea = _codes_f64(img)
+            ea[..., 3] = np.clip(img[..., 3].astype(np.float64), 0, 1) * 255.0
+            near = np.abs(ea - np.floor(ea) - 0.5) < 1e-3
+            if not near.any():
+                break
+            img[near] = rng.uniform(0, 1, int(near.sum())).astype(np.float32)
+    ca, cb = _codes_f64(a), _codes_f64(b)
+    worst = np.abs(np.rint(ca[..., :3]) - np.rint(cb[..., :3])).max(axis=-1)
+    alpha = np.abs(np.rint(np.clip(a[..., 3], 0, 1).astype(np.float64) * 255) -
+                   np.rint(np.clip(b[..., 3], 0, 1).astype(np.float64) * 255))
+    worst = np.maximum(worst, alpha).astype(np.int64)
+    want = np.bincount(worst.ravel(), minlength=256)
+    assert np.array_equal(po.compare_codes(a, b), want)
+    assert want[1:].sum() > 100  # the noise moves codes
+
+
+def test_compare_hdr_against_numpy():
+    """image_compare_hdr.slang: the per-pixel worst relative difference binned
+    at floor((log2 + 16) * 8) + 1 (0 for none), and relMSE's sum
+    (channel mean of (a - b)^2 / (b^2 + 1e-2)), against numpy FP64."""
+    rng = np.random.default_rng(6)
+    h, w = 29, 41
+    b = np.exp(rng.normal(0, 2, (h, w, 4))).astype(np.float32)
+    a = (b * np.exp(rng.normal(0, 0.3, b.shape))).astype(np.float32)
+    a[0, :5] = b[0, :5]  # identical pixels land in bin 0
+    for _ in range(20):
+        d = np.abs(a.astype(np.float64) - b)
+        worst = (d / np.maximum(np.abs(b.astype(np.float64)), 1e-3)).max(axis=-1)
+        pos = (np.log2(np.where(worst > 0, worst, 1.0)) + 16) * 8
+        near = (worst > 0) & (np.abs(pos - np.round(pos)) < 1e-3)
+        if not near.any():
+            break
+        a[near] = (b[near] * np.float32(1.37)).astype(np.float32)
+    d = np.abs(a.astype(np.float64) - b)
+    worst = (d / np.maximum(np.abs(b.astype(np.float64)), 1e-3)).max(axis=-1)
+    bins = np.where(worst > 0, np.clip(np.floor((np.log2(np.where(worst > 0, worst, 1.0)) + 16) * 8) + 1, 1, 255), 0)
+    want = np.bincount(bins.astype(np.int64).ravel(), minlength=256)
+    got, total = po.compare_hdr(a, b)
+    assert np.array_equal(got, want)
+    rel = (d * d / (b.astype(np.float64) ** 2 + 1e-2)).mean(axis=-1).sum()
+    assert abs(total - rel) <= 1e-5 * rel
+    assert want[0] == 5

@@ -25,14 +25,17 @@ def recip(v):
     return np.where(v > f32(1e-12), f32(1.0) / v, f32(0.0)).astype(f32)
 
 
-def min_power(a, b, c, ia, ic, lox, loy, hix, hiy):
+def min_power(a, b, c, lox, loy, hix, hiy):
+    """frame.slang:354-371 in float32: each edge's minimiser ((-b) x) / c."""
     inside = (lox <= 0) & (hix >= 0) & (loy <= 0) & (hiy >= 0)
     best = np.full(a.shape, f32(3.4e38), f32)
+    safe_a = np.where(a > f32(1e-12), a, f32(1.0)).astype(f32)
+    safe_c = np.where(c > f32(1e-12), c, f32(1.0)).astype(f32)
     for x in (lox, hix):
-        y = np.where(c > f32(1e-12), np.minimum(np.maximum(((-b) * x) * ic, loy), hiy), loy).astype(f32)
+        y = np.where(c > f32(1e-12), np.minimum(np.maximum(((-b) * x) / safe_c, loy), hiy), loy).astype(f32)
         best = np.minimum(best, (a * x * x + f32(2.0) * b * x * y + c * y * y).astype(f32))
     for y in (loy, hiy):
-        x = np.where(a > f32(1e-12), np.minimum(np.maximum(((-b) * y) * ia, lox), hix), lox).astype(f32)
+        x = np.where(a > f32(1e-12), np.minimum(np.maximum(((-b) * y) / safe_a, lox), hix), lox).astype(f32)
         best = np.minimum(best, (a * x * x + f32(2.0) * b * x * y + c * y * y).astype(f32))
     return np.where(inside, f32(0.0), best)
 
@@ -81,7 +84,7 @@ def classify(seed, k_slack, n=300000, W=1920, H=1080, T=16):
     hiy = ((ey - 1).astype(f32) + f32(0.5)) - cy
     lox = ((tx * T).astype(f32) + f32(0.5)) - cx
     hix = ((ex - 1).astype(f32) + f32(0.5)) - cx
-    hit = min_power(a, b, c, recip(a), recip(c), lox, loy, hix, hiy) <= cut
+    hit = min_power(a, b, c, lox, loy, hix, hiy) <= cut
     # The kernel's rule, in float32 but for the determinant.
     det = (a.astype(np.float64) * c.astype(np.float64) - b.astype(np.float64) ** 2)
     usable = (a > f32(1e-9)) & (c > f32(1e-9)) & (det > 0)
