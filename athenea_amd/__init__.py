@@ -14,7 +14,9 @@ from .raster import (  # noqa: F401
     RenderSettings,
     RenderTargets,
     SplatCounters,
+    SplatInstance,
     TileRasterizer,
+    UnderLayer,
 )
 
 __all__ = [
@@ -29,5 +31,7 @@ __all__ = [
     "RenderSettings",
     "RenderTargets",
     "SplatCounters",
+    "SplatInstance",
     "TileRasterizer",
+    "UnderLayer",
 ]

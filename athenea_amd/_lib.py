@@ -98,6 +98,15 @@ class Gaussians(ctypes.Structure):
         ("sh_half", ctypes.c_void_p),
         ("relight", ctypes.POINTER(Relight)),
         ("packed", ctypes.POINTER(PackedSplats)),
+        ("crypto_ids", ctypes.c_void_p),
+    ]
+
+
+class Instance(ctypes.Structure):
+    _fields_ = [
+        ("gaussians", ctypes.POINTER(Gaussians)),
+        ("object_to_view", ctypes.c_void_p),
+        ("eye", ctypes.c_void_p),
     ]
 
 
@@ -112,7 +121,14 @@ class Settings(ctypes.Structure):
         ("depth_threshold", ctypes.c_float),
         ("time_stages", ctypes.c_uint32),
         ("count_splats", ctypes.c_uint32),
+        ("counters_tag", ctypes.c_uint64),
     ]
+
+
+TRACE_SLOTS = 4
+TRACE_WORDS = TRACE_SLOTS * 2 + TRACE_SLOTS // 2 + 1
+TRACE_NONE = 0xFFFFFFFF
+TRACE_EMPTY = 0xFFFFFFFE
 
 
 class Targets(ctypes.Structure):
@@ -121,6 +137,11 @@ class Targets(ctypes.Structure):
         ("depth", ctypes.c_void_p),
         ("final_t", ctypes.c_void_p),
         ("last", ctypes.c_void_p),
+        ("crypto", ctypes.c_void_p),
+        ("trace", ctypes.c_void_p),
+        ("under_colour", ctypes.c_void_p),
+        ("under_depth", ctypes.c_void_p),
+        ("under_crypto", ctypes.c_void_p),
     ]
 
 
@@ -141,13 +162,20 @@ class FrameStats(ctypes.Structure):
     ]
 
 
+MAX_COUNTED_CLOUDS = 64
+
+
 class Counters(ctypes.Structure):
     _fields_ = [
+        ("tag", ctypes.c_uint64),
         ("slots", ctypes.c_uint32),
         ("visible", ctypes.c_uint32),
         ("pairs", ctypes.c_uint64),
         ("max_tiles", ctypes.c_uint32),
         ("culled", ctypes.c_uint32 * CULL_REASONS),
+        ("clouds", ctypes.c_uint32),
+        ("cloud_visible", ctypes.c_uint32 * MAX_COUNTED_CLOUDS),
+        ("cloud_pairs", ctypes.c_uint32 * MAX_COUNTED_CLOUDS),
     ]
 
 
@@ -158,6 +186,7 @@ class Grads(ctypes.Structure):
         ("quats", ctypes.c_void_p),
         ("opacity_logits", ctypes.c_void_p),
         ("sh", ctypes.c_void_p),
+        ("overwrite", ctypes.c_uint32),
     ]
 
 
@@ -323,6 +352,17 @@ SIGNATURES = {
         ctypes.c_int,
         [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p,
          ctypes.c_void_p],
+    ),
+    "athx_render_instances": (
+        ctypes.c_int,
+        [ctypes.c_void_p, ctypes.POINTER(View), ctypes.c_uint32, ctypes.POINTER(Instance), ctypes.c_uint32,
+         ctypes.POINTER(Settings), ctypes.POINTER(Targets), ctypes.c_void_p, ctypes.POINTER(FrameStats),
+         ctypes.POINTER(ctypes.c_uint32)],
+    ),
+    "athx_reference_render_instances": (
+        ctypes.c_int,
+        [ctypes.c_void_p, ctypes.POINTER(View), ctypes.POINTER(Instance), ctypes.c_uint32, ctypes.POINTER(Settings),
+         ctypes.POINTER(Targets), ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint32)],
     ),
     "athx_reference_render": (
         ctypes.c_int,

@@ -37,12 +37,13 @@ class _Rasterize(torch.autograd.Function):
         for k in range(n):
             g = grad_outputs[k]
             dl.append(torch.zeros_like(ctx.targets[k].colour) if g is None else g)
+        # Written whole by the adjoint (athx_grads.overwrite): no clearing pass.
         grads = {
-            "means": torch.zeros_like(cloud.means),
-            "log_scales": torch.zeros_like(cloud.log_scales),
-            "quats": torch.zeros_like(cloud.quats),
-            "opacity_logits": torch.zeros_like(cloud.opacity_logits),
-            "sh": torch.zeros(cloud.sh.shape, dtype=torch.float32, device=cloud.sh.device),
+            "means": torch.empty_like(cloud.means),
+            "log_scales": torch.empty_like(cloud.log_scales),
+            "quats": torch.empty_like(cloud.quats),
+            "opacity_logits": torch.empty_like(cloud.opacity_logits),
+            "sh": torch.empty(cloud.sh.shape, dtype=torch.float32, device=cloud.sh.device),
         }
         targets = ctx.targets
         if ctx.raster.frame_serial() != targets[0].frame_serial:
@@ -51,7 +52,7 @@ class _Rasterize(torch.autograd.Function):
             # again from the saved parameters -- the same bits, since the
             # render is deterministic -- and its adjoint taken from that.
             targets, _ = ctx.raster.render(ctx.views, cloud, ctx.settings, keep_backward=True)
-        ctx.raster.render_backward(ctx.views, cloud, ctx.settings, targets, dl, grads)
+        ctx.raster.render_backward(ctx.views, cloud, ctx.settings, targets, dl, grads, overwrite=True)
         g_sh = grads["sh"].to(ctx.sh_dtype)
         return (grads["means"], grads["log_scales"], grads["quats"], grads["opacity_logits"], g_sh,
                 None, None, None, None)

@@ -21,6 +21,7 @@ import time
 
 import torch
 
+from ._lib import AthxError
 from .raster import GaussianCloud, RenderSettings, TileRasterizer
 
 TILES = (8, 16, 32)
@@ -37,7 +38,10 @@ def _shape_key(views: list, cloud: GaussianCloud, base: RenderSettings) -> tuple
 def autotune(raster: TileRasterizer, views: list, cloud: GaussianCloud, base: RenderSettings | None = None,
              tiles=TILES, batches=BATCHES, reps: int = 3, use_cache: bool = True
              ) -> tuple[RenderSettings, list[dict]]:
-    """Returns (the fastest settings, every candidate's median ms)."""
+    """Returns (the fastest settings, every candidate's median ms). A tile
+    side whose frame the rasterizer refuses as too large (more pairs than a
+    frame holds: small tiles multiply them) is listed with its reason and
+    skipped."""
     base = base or RenderSettings()
     key = _shape_key(views, cloud, base)
     if use_cache and key in _cache:
@@ -48,7 +52,13 @@ def autotune(raster: TileRasterizer, views: list, cloud: GaussianCloud, base: Re
         targets = None
         for batch in batches:
             s = dataclasses.replace(base, tile=tile, batch=batch, time_stages=False, count_splats=False)
-            targets, _ = raster.render(views, cloud, s, targets)  # warm-up, and sizes the scratch
+            try:
+                targets, _ = raster.render(views, cloud, s, targets)  # warm-up, and sizes the scratch
+            except AthxError as e:
+                if e.code != "Unsupported":
+                    raise
+                table.append({"tile": tile, "batch": batch, "ms": None, "refused": str(e)})
+                break
             torch.cuda.synchronize()
             times = []
             for _ in range(reps):

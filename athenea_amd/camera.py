@@ -33,6 +33,7 @@ class Lens:
     far_z: float = 10000.0
     window_translate: tuple[float, float] = (0.0, 0.0)
     window_scale: tuple[float, float] = (1.0, 1.0)
+    window_roll: float = 0.0  # degrees, counter-clockwise (Camera.h:33)
 
 
 @dataclass
@@ -84,10 +85,23 @@ class Projection:
     eye_world: np.ndarray
 
 
+def rotation_z(degrees: float) -> np.ndarray:
+    """A counter-clockwise turn about z by ``degrees``, 4x4."""
+    t = math.radians(degrees)
+    c, s = math.cos(t), math.sin(t)
+    m = np.eye(4)
+    m[0, 0], m[0, 1] = c, -s
+    m[1, 0], m[1, 1] = s, c
+    return m
+
+
 def projection_for(camera: Camera, width: int, height: int) -> Projection:
-    """Camera.cpp:32-64."""
+    """Camera.cpp:32-64; the window roll turns the picture about the view's
+    own z after the flip (Camera.cpp:35-38)."""
     flip = np.diag([1.0, 1.0, -1.0, 1.0])
     world_to_view = flip @ np.linalg.inv(camera.camera_to_world)
+    if abs(camera.lens.window_roll) > 1e-12:
+        world_to_view = rotation_z(camera.lens.window_roll) @ world_to_view
     w, h = float(width), float(height)
     focal = w * max(camera.lens.focal, 1e-9) / max(camera.lens.haperture, 1e-9)
     return Projection(
@@ -131,7 +145,26 @@ def make_view(
         v.eye[k] = float(eye_obj[k])
     for k in range(4):
         v.background[k] = float(background[k])
+    # The doubles the view was made from: a multi-instance frame folds each
+    # instance's object-to-world into them (instance_transform).
+    v.projection = projection
     return v
+
+
+def instance_transform(view, object_to_world=None) -> tuple[np.ndarray, np.ndarray]:
+    """(object-to-view rows [12], eye [3]) as float32 for one instance under a
+    view made by make_view / view_for: Projection::worldToView times the
+    instance's objectToWorld and the eye taken into the instance's space, in
+    double, then cast (TileRasterizer.cpp:598-600)."""
+    projection = getattr(view, "projection", None)
+    if projection is None:
+        raise ValueError("a multi-instance frame needs views made by camera.make_view / view_for")
+    o2w = np.eye(4) if object_to_world is None else np.asarray(object_to_world, np.float64)
+    if o2w.shape == (3, 4):
+        o2w = np.vstack([o2w, [0.0, 0.0, 0.0, 1.0]])
+    o2v = projection.world_to_view @ o2w
+    eye_obj = np.linalg.inv(o2w) @ np.append(projection.eye_world, 1.0)
+    return o2v[:3, :4].reshape(-1).astype(np.float32), eye_obj[:3].astype(np.float32)
 
 
 def view_for(camera: Camera, width: int, height: int, object_to_world=None, background=(0.0, 0.0, 0.0, 0.0)):

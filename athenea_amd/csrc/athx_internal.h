@@ -51,6 +51,12 @@ struct DView {
     float* final_t;
     uint32_t* last;
     const float* dl_dcolour;
+    // The walk's variants (athx_targets): null when off.
+    float* crypto;
+    uint32_t* trace;
+    const float* under_colour;
+    const float* under_depth;
+    const uint32_t* under_crypto;
 };
 
 // What the relit colour reads besides the Gaussian's own geometry
@@ -78,6 +84,12 @@ struct ProjectArgs {
     const float* sh;
     const uint16_t* sh_half;
     uint32_t n;
+    // Where this cloud's Gaussians sit among the frame's slots: Gaussian i of
+    // view v is slot v * slot_stride + slot_base + i. One cloud: stride n,
+    // base 0; several instances (athx_render_instances) share each view's
+    // slot range, each from its own base (TileRasterizer.cpp:896-898).
+    uint32_t slot_stride;
+    uint32_t slot_base;
     uint32_t sh_coeffs;
     uint32_t sh_degree;  // min(degree the coefficients hold, the frame's limit)
     uint32_t linear;
@@ -91,6 +103,10 @@ struct ProjectArgs {
     const uint32_t* shape;      // 4 words a Gaussian
     const uint32_t* packed_sh;  // packed_words a Gaussian: rest coefficients as halves
     uint32_t packed_words;
+    // A matte frame: each drawn slot's Cryptomatte id (crypto_ids[i], or 0
+    // without them) into slot_ids.
+    const uint32_t* crypto_ids;
+    uint32_t* slot_ids;
 };
 
 __host__ __device__ inline uint32_t div_up(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
@@ -235,7 +251,7 @@ struct StripBins {
 hipError_t launch_strip_mask_emit(const DView* views_dev, uint32_t n, uint32_t tile, const uint32_t* sorted_slots,
                                   const uint32_t* offsets, const uint64_t* visible_dev, uint32_t max_visible,
                                   uint32_t cap, const float* records, uint32_t* strip_keys, uint32_t* strip_slots,
-                                  hipStream_t s);
+                                  uint32_t* rank_tiles, hipStream_t s);
 // Each strip's segments, scanned: segbase[s] and the total in *seg_total.
 hipError_t launch_seg_scan(const uint32_t* strip_ranges, uint32_t strips, uint32_t* segbase, uint64_t* seg_total,
                            uint32_t* status_words, hipStream_t s);
@@ -271,8 +287,8 @@ hipError_t launch_pair_keys(const uint32_t* ranges, uint32_t tiles, const uint32
                             const uint32_t* depth_keys, uint64_t* keys, hipStream_t s);
 hipError_t launch_blend(const DView* views_dev, const uint32_t* view_of_tile_base, uint32_t nviews,
                         uint32_t total_tiles, uint32_t tile, uint32_t batch, const uint32_t* ranges,
-                        const uint32_t* pair_slots, const float* records, uint32_t depth_mode,
-                        float depth_threshold, hipStream_t s);
+                        const uint32_t* pair_slots, const float* records, uint32_t depth_mode, float depth_threshold,
+                        hipStream_t s, bool variants = false, const uint32_t* slot_ids = nullptr);
 // What each splat gave the last frame's walk (the frame's ranges, pair
 // slots and records): mode 1 the blend weight in 1/4096, one word a slot;
 // mode 4 the colour gradient against each view's dl_dcolour in 1/256 (three
@@ -287,8 +303,26 @@ hipError_t launch_blend_backward(const DView* views_dev, const uint32_t* tile_ba
 hipError_t launch_project_backward(const ProjectArgs& a, const DView* views_dev, uint32_t nviews,
                                    const uint32_t* touched, const float* records, const float* grad2d,
                                    const athx_grads& g, hipStream_t s);
-hipError_t launch_counters(const uint32_t* depth_keys, const uint32_t* touched, uint64_t slots,
-                           uint32_t* words, hipStream_t s);
+// SplatCounters words (TileRasterizer.h:370-401; splat_frame_counters.slang):
+// [0] slots (the host's), [1] visible, [2] pairs low, [3] pairs high, [4]
+// most tiles of one footprint, [8 + r] culls of reason r, and for cloud j <
+// ATHX_MAX_COUNTED_CLOUDS [16 + 2j] its drawn footprints, [17 + 2j] their pairs.
+constexpr uint32_t kCounterWords = 16 + 2 * ATHX_MAX_COUNTED_CLOUDS;
+
+// What the counters read: every slot's depth key (a culled one holds its
+// reason), the depth-ordered drawn slots, and each drawn footprint's tiles
+// -- by rank (rank_tiles) or by slot (slot_tiles); the instances' bases in a
+// view's slots, base[clouds] the last counted one's end.
+struct CounterArgs {
+    const uint32_t* depth_keys;
+    const uint32_t* sorted_slots;
+    const uint32_t* rank_tiles;
+    const uint32_t* slot_tiles;
+    const uint64_t* visible_dev;
+    uint32_t slots, count, clouds;
+    uint32_t base[ATHX_MAX_COUNTED_CLOUDS + 1];
+};
+hipError_t launch_frame_counters(const CounterArgs& a, uint32_t* words, hipStream_t s);
 
 // Onesweep radix sort of (uint32 or uint64 key, uint32 value) pairs.
 struct SortScratch {
@@ -364,7 +398,8 @@ struct IngestFields {
 // than max_hits contributors); and the image comparisons.
 hipError_t launch_ref_project(const float* positions, const uint32_t* shape, const uint32_t* sh, uint32_t n,
                               uint32_t rest_per_colour, uint32_t sh_limit, uint32_t antialias, uint32_t linear,
-                              const DView* view, float* records, uint32_t* keys, uint32_t* values, hipStream_t s);
+                              const DView* view, float* records, uint32_t* keys, uint32_t* values, hipStream_t s,
+                              uint32_t base = 0);
 hipError_t launch_ref_blend(const DView* view, uint32_t width, uint32_t height, const float* records,
                             const uint32_t* sorted_keys, const uint32_t* sorted, uint32_t n, uint32_t max_hits,
                             uint32_t depth_mode, float depth_threshold, uint32_t* overflow, hipStream_t s);

@@ -146,6 +146,287 @@ __global__ void __launch_bounds__(TILE * TILE) blend_kernel(const DView* __restr
     }
 }
 
+// The walk's variants (splat_blend.slang:236-578 with composite, matte and
+// traced; athx_targets): the same walk as blend_kernel, and
+//   under    an opaque layer under the splats (the view's under_colour and
+//            under_depth): a splat at or behind its depth ends the pixel's
+//            walk, and the layer takes the light left, as the background
+//            would, its depth weighing in the mean (:287-300, 330-333,
+//            512-566);
+//   matte    the six heaviest Cryptomatte ids of the pixel with their
+//            coverage, the layer's own id taking what the walk left
+//            (:88-143, 421-423, 526-531);
+//   traced   the four heaviest slots, their depths and weights, and what the
+//            list left out and what closed the walk (splat_trace.slang).
+// Both lists live in registers, every access to them unrolled, so they cost
+// no memory; the flags are the frame's, uniform across the launch.
+constexpr uint32_t kCryptoSlots = 6;
+
+__device__ inline void crypto_add(uint32_t (&ids)[kCryptoSlots], float (&wts)[kCryptoSlots], uint32_t id,
+                                  float weight) {
+    if (weight <= 0.0f || id == 0u) {
+        return;
+    }
+    uint32_t lightest = 0;
+    float least = wts[0];
+    bool merged = false;
+#pragma unroll
+    for (uint32_t k = 0; k < kCryptoSlots; ++k) {
+        // The first rank naming this id takes the weight (splat_blend.slang:101-110).
+        const bool same = !merged && ids[k] == id && wts[k] > 0.0f;
+        if (same) {
+            wts[k] = wts[k] + weight;
+            merged = true;
+        }
+        if (!merged && wts[k] < least) {
+            least = wts[k];
+            lightest = k;
+        }
+    }
+    if (!merged && weight > least) {
+#pragma unroll
+        for (uint32_t k = 0; k < kCryptoSlots; ++k) {
+            if (k == lightest) {
+                ids[k] = id;
+                wts[k] = weight;
+            }
+        }
+    }
+}
+
+// Ranks by descending weight, ties in list order (an insertion sort's).
+template <uint32_t N>
+__device__ inline void order_desc(const float (&w)[N], uint32_t (&order)[N]) {
+#pragma unroll
+    for (uint32_t k = 0; k < N; ++k) {
+        uint32_t rank = 0;
+#pragma unroll
+        for (uint32_t m = 0; m < N; ++m) {
+            rank += (w[m] > w[k] || (w[m] == w[k] && m < k)) ? 1u : 0u;
+        }
+#pragma unroll
+        for (uint32_t r = 0; r < N; ++r) {
+            if (r == rank) {
+                order[r] = k;
+            }
+        }
+    }
+}
+
+template <uint32_t N, typename T>
+__device__ inline T pick(const T (&a)[N], uint32_t k) {
+    T v = a[0];
+#pragma unroll
+    for (uint32_t m = 1; m < N; ++m) {
+        v = m == k ? a[m] : v;
+    }
+    return v;
+}
+
+__device__ inline uint32_t half_of(float v) { return (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)v); }
+
+template <uint32_t TILE>
+__global__ void __launch_bounds__(TILE * TILE) blend_ex_kernel(const DView* __restrict__ views,
+                                                               const uint32_t* __restrict__ tile_bases,
+                                                               uint32_t nviews, uint32_t batch,
+                                                               const uint32_t* __restrict__ ranges,
+                                                               const uint32_t* __restrict__ pair_slots,
+                                                               const float* __restrict__ records,
+                                                               const uint32_t* __restrict__ slot_ids,
+                                                               uint32_t depth_mode, float depth_threshold) {
+    constexpr uint32_t kThreads = TILE * TILE;
+    __shared__ LdsBatch s;
+    __shared__ uint32_t s_slot[kMaxBatch];
+    __shared__ uint32_t s_id[kMaxBatch];
+    const uint32_t gtile = xcd_band(blockIdx.x, gridDim.x);
+    const uint32_t v = view_of_tile(tile_bases, nviews, gtile);
+    const DView& V = views[v];
+    const uint32_t local = gtile - V.tile_base;
+    const uint32_t tx = local % V.tiles_x, ty = local / V.tiles_x;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t px = tx * TILE + tid % TILE, py = ty * TILE + tid / TILE;
+    const bool inside = px < V.width && py < V.height;
+    const uint32_t start = ranges[2 * gtile], end = ranges[2 * gtile + 1];
+    const float pxc = (float)px + 0.5f, pyc = (float)py + 0.5f;
+    const uint32_t pixels = V.width * V.height;
+    const size_t pix = inside ? (size_t)py * V.width + px : 0;
+    const bool matte = V.crypto != nullptr, traced = V.trace != nullptr;
+    const bool under = inside && V.under_colour != nullptr && V.under_depth != nullptr;
+    const float under_z = under ? V.under_depth[pix] : 0.0f;
+    const bool under_hit = under && under_z > 0.0f;
+
+    float r = 0.0f, g = 0.0f, b = 0.0f, T = 1.0f, dsum = 0.0f, dat = -1.0f;
+    uint32_t last = 0;
+    uint32_t ids[kCryptoSlots];
+    float cw[kCryptoSlots];
+#pragma unroll
+    for (uint32_t k = 0; k < kCryptoSlots; ++k) {
+        ids[k] = 0u;
+        cw[k] = 0.0f;
+    }
+    uint32_t tslot[ATHX_TRACE_SLOTS];
+    float tw[ATHX_TRACE_SLOTS];
+#pragma unroll
+    for (uint32_t k = 0; k < ATHX_TRACE_SLOTS; ++k) {
+        tslot[k] = ATHX_TRACE_NONE;
+        tw[k] = 0.0f;
+    }
+    float ttotal = 0.0f, tfloor = 0.0f;
+    uint32_t tlightest = 0;
+    bool done = !inside;
+    for (uint32_t first = start; first < end; first += batch) {
+        if (!__syncthreads_or(!done)) {
+            break;
+        }
+        const uint32_t count = end - first < batch ? end - first : batch;
+        for (uint32_t k = tid; k < count; k += kThreads) {
+            const uint32_t slot = pair_slots[first + k];
+            stage(s, k, records, slot);
+            s_slot[k] = slot;
+            if (matte) {
+                s_id[k] = slot_ids[slot];
+            }
+        }
+        __syncthreads();
+        for (uint32_t j = 0; j < count && !done; ++j) {
+            const float4 q = s.xy_cut_op[j];
+            const float4 k = s.conic[j];
+            if (under_hit && k.w >= under_z) {
+                done = true;
+                break;
+            }
+            const float dx = pxc - q.x, dy = pyc - q.y;
+            const float power = k.x * dx * dx + k.y * dx * dy + k.z * dy * dy;
+            if (power > q.z) {
+                continue;
+            }
+            const float alpha = fminf(0.99f, q.w * blend_exp(power));
+            if (alpha < 1.0f / 255.0f) {
+                continue;
+            }
+            const float4 c = s.colour[j];
+            const float w = T * alpha;
+            if (matte) {
+                crypto_add(ids, cw, s_id[j], w);
+            }
+            if (traced) {
+                ttotal = ttotal + w;
+                if (w > tfloor) {
+                    // TraceList::enter (splat_trace.slang:81-90)
+                    const uint32_t slot = s_slot[j];
+#pragma unroll
+                    for (uint32_t m = 0; m < ATHX_TRACE_SLOTS; ++m) {
+                        if (m == tlightest) {
+                            tslot[m] = slot;
+                            tw[m] = w;
+                        }
+                    }
+                    tlightest = 0;
+                    float least = tw[0];
+#pragma unroll
+                    for (uint32_t m = 1; m < ATHX_TRACE_SLOTS; ++m) {
+                        if (tw[m] < least) {
+                            least = tw[m];
+                            tlightest = m;
+                        }
+                    }
+                    tfloor = least;
+                }
+            }
+            r = r + w * c.x;
+            g = g + w * c.y;
+            b = b + w * c.z;
+            dsum = dsum + w * k.w;
+            T = T * (1.0f - alpha);
+            if (dat < 0.0f && 1.0f - T >= depth_threshold) {
+                dat = k.w;
+            }
+            last = first - start + j + 1;
+            if (T < 1e-4f) {
+                done = true;
+            }
+        }
+        __syncthreads();
+    }
+    if (!inside) {
+        return;
+    }
+    float4 behind = make_float4(V.bg[0], V.bg[1], V.bg[2], V.bg[3]);
+    float opaque_z = 0.0f;
+    uint32_t opaque_id = 0u;
+    if (under_hit) {
+        behind = reinterpret_cast<const float4*>(V.under_colour)[pix];
+        opaque_z = under_z;
+        opaque_id = matte && V.under_crypto ? V.under_crypto[pix] : 0u;
+    }
+    if (matte) {
+        crypto_add(ids, cw, opaque_id, under_hit ? T : 0.0f);
+        uint32_t order[kCryptoSlots];
+        order_desc<kCryptoSlots>(cw, order);
+#pragma unroll
+        for (uint32_t layer = 0; layer < kCryptoSlots / 2; ++layer) {
+            const uint32_t ka = order[layer * 2], kb = order[layer * 2 + 1];
+            const float wa = pick(cw, ka), wb = pick(cw, kb);
+            const uint32_t ia = pick(ids, ka), ib = pick(ids, kb);
+            reinterpret_cast<float4*>(V.crypto)[(size_t)layer * pixels + pix] =
+                make_float4(wa > 0.0f ? __builtin_bit_cast(float, ia) : 0.0f, fmaxf(wa, 0.0f),
+                            wb > 0.0f ? __builtin_bit_cast(float, ib) : 0.0f, fmaxf(wb, 0.0f));
+        }
+    }
+    if (traced) {
+        const float under_w = under_hit ? T : 0.0f;
+        if (!(ttotal > 0.0f) && !(under_w > 0.0f)) {
+            V.trace[pix] = ATHX_TRACE_EMPTY;
+        } else {
+            uint32_t order[ATHX_TRACE_SLOTS];
+            order_desc<ATHX_TRACE_SLOTS>(tw, order);
+            float held = 0.0f;
+            float sorted_w[ATHX_TRACE_SLOTS];
+#pragma unroll
+            for (uint32_t k = 0; k < ATHX_TRACE_SLOTS; ++k) {
+                const float wk = pick(tw, order[k]);
+                const uint32_t sk = pick(tslot, order[k]);
+                const bool some = wk > 0.0f;
+                sorted_w[k] = wk;
+                V.trace[(size_t)k * pixels + pix] = some ? sk : ATHX_TRACE_NONE;
+                V.trace[(size_t)(ATHX_TRACE_SLOTS + k) * pixels + pix] =
+                    some ? __builtin_bit_cast(uint32_t, records[(size_t)sk * kRecordFloats + 2]) : 0u;
+                held = held + wk;
+            }
+#pragma unroll
+            for (uint32_t pr = 0; pr < ATHX_TRACE_SLOTS / 2; ++pr) {
+                V.trace[(size_t)(ATHX_TRACE_SLOTS * 2 + pr) * pixels + pix] =
+                    half_of(sorted_w[pr * 2]) | (half_of(sorted_w[pr * 2 + 1]) << 16);
+            }
+            V.trace[(size_t)(ATHX_TRACE_WORDS - 1) * pixels + pix] =
+                half_of(fmaxf(ttotal - held, 0.0f)) | (half_of(fmaxf(under_w, 0.0f)) << 16);
+        }
+    }
+    const float coverage = 1.0f - T;
+    if (under_hit) {
+        dsum = dsum + T * opaque_z;
+        if (dat < 0.0f) {
+            dat = opaque_z;
+        }
+    }
+    const float total = under_hit ? 1.0f : coverage;
+    float4 out;
+    out.x = r + T * behind.x;
+    out.y = g + T * behind.y;
+    out.z = b + T * behind.z;
+    out.w = coverage + T * behind.w;
+    reinterpret_cast<float4*>(V.colour)[pix] = out;
+    if (V.depth) {
+        V.depth[pix] = depth_mode == 0 ? (total > 1e-6f ? dsum / total : 0.0f) : fmaxf(dat, 0.0f);
+    }
+    if (V.final_t) {
+        V.final_t[pix] = T;
+    }
+    if (V.last) {
+        V.last[pix] = last;
+    }
+}
+
 // What each splat gave the last frame (splat_blend.slang:36-48, 364-398):
 // the same walk as blend_kernel -- the same pairs, records, arithmetic and
 // early stop -- counting at each pair's slot, with integer atomics (so in
@@ -488,12 +769,25 @@ hipError_t blend_backward_for(const DView* views, const uint32_t* tile_bases, ui
 
 hipError_t launch_blend(const DView* views_dev, const uint32_t* tile_bases, uint32_t nviews, uint32_t total_tiles,
                         uint32_t tile, uint32_t batch, const uint32_t* ranges, const uint32_t* pair_slots,
-                        const float* records, uint32_t depth_mode, float depth_threshold, hipStream_t s) {
+                        const float* records, uint32_t depth_mode, float depth_threshold, hipStream_t s,
+                        bool variants, const uint32_t* slot_ids) {
     if (total_tiles == 0) {
         return hipSuccess;
     }
     if (batch == 0 || batch > kMaxBatch) {
         return hipErrorInvalidValue;
+    }
+    if (variants) {
+        using Kernel = void (*)(const DView*, const uint32_t*, uint32_t, uint32_t, const uint32_t*, const uint32_t*,
+                                const float*, const uint32_t*, uint32_t, float);
+        const Kernel k = tile == 8 ? blend_ex_kernel<8> : tile == 16 ? blend_ex_kernel<16> : tile == 32 ? blend_ex_kernel<32>
+                                                                                                          : nullptr;
+        if (!k) {
+            return hipErrorInvalidValue;
+        }
+        hipLaunchKernelGGL(k, dim3(total_tiles), dim3(tile * tile), 0, s, views_dev, tile_bases, nviews, batch, ranges,
+                           pair_slots, records, slot_ids, depth_mode, depth_threshold);
+        return hipGetLastError();
     }
     switch (tile) {
     case 8:

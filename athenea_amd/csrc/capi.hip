@@ -13,6 +13,7 @@
 // Scratch grows to 1.5x what a frame needs and, after athx_release_scratch,
 // to exactly what it needs (TileRasterizer.cpp:30-36, 178-191).
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cstdio>
 #include <cstring>
@@ -66,7 +67,7 @@ struct athx_ctx {
     bool exact_growth = false;
 
     DeviceBuffer views, tile_bases, records, touched, depth_keys, offsets, scan_status, totals;
-    DeviceBuffer pair_vals, ranges, grad2d, counter_words, big_list;
+    DeviceBuffer pair_vals, ranges, grad2d, counter_words, big_list, rank_tiles, slot_ids;
     DeviceBuffer depth_keys_sorted, depth_slots, depth_alt_keys, depth_alt_slots, depth_sort_meta;
     DeviceBuffer fp_rects, fp_masks, fp_big_base, fp_big_masks;
     // The binning: strip entries (keys, slots, the sort's alternates and
@@ -90,6 +91,8 @@ struct athx_ctx {
     // What the last render left for the adjoint and the frame's readers.
     uint32_t last_n = 0, last_nviews = 0, last_tiles = 0, last_tile = 0, last_key_bits = 0, last_capacity = 0;
     uint32_t last_strips = 0, last_entry_cap = 0;
+    uint32_t last_instances = 0;  // the clouds the last frame projected (athx_render_instances)
+    bool last_under = false;      // the last frame's walks ended at an under layer
     bool last_sorted_in_alt = false;  // the depth-sorted slots are in depth_alt_slots
     uint64_t last_true_pairs = 0;  // the last frame's pair total (an async frame's once athx_frame_pairs ran)
     uint64_t pair_hint = 0;        // the pair capacity a frame takes before it knows: 1.25x the largest seen
@@ -102,8 +105,18 @@ struct athx_ctx {
     // and the contributions name the frame they follow, so a later render
     // that replaced its buffers is refused rather than walked.
     uint64_t serial = 0;
-    bool counters_valid = false;
-    athx_counters counters{};
+    // The counted frames' words on their way to the host: a ring of three
+    // pinned copies, each made on its frame's stream after the counts and
+    // marked by an event (TileRasterizer's AsyncReadback of 3); seq 0 is an
+    // empty slot, and the newest finished one is what latestCounters reads.
+    struct CounterCopy {
+        uint32_t* host = nullptr;  // kCounterWords, pinned
+        hipEvent_t done = nullptr;
+        uint64_t seq = 0, tag = 0;
+        uint32_t slots = 0, clouds = 0;
+    };
+    CounterCopy counter_ring[3];
+    uint64_t counter_seq = 0;
 
     hipEvent_t ev[8] = {};
     // Recorded after each upload from the pinned staging views: the staging
@@ -117,6 +130,7 @@ namespace {
 std::vector<DeviceBuffer*> all_buffers(athx_ctx* ctx) {
     return {&ctx->views, &ctx->tile_bases, &ctx->records, &ctx->touched, &ctx->depth_keys, &ctx->offsets,
             &ctx->scan_status, &ctx->totals, &ctx->pair_vals, &ctx->ranges, &ctx->grad2d, &ctx->counter_words,
+            &ctx->rank_tiles, &ctx->slot_ids,
             &ctx->big_list, &ctx->depth_keys_sorted, &ctx->depth_slots, &ctx->depth_alt_keys, &ctx->depth_alt_slots,
             &ctx->depth_sort_meta, &ctx->fp_rects, &ctx->fp_masks, &ctx->fp_big_base, &ctx->fp_big_masks,
             &ctx->strip_keys, &ctx->strip_slots, &ctx->strip_alt_keys, &ctx->strip_alt_slots, &ctx->strip_sort_meta,
@@ -269,6 +283,10 @@ ProjectArgs project_args(const athx_gaussians* g, const athx_settings* s) {
     a.sh = g->sh;
     a.sh_half = g->sh_half;
     a.n = g->count;
+    a.slot_stride = g->count;
+    a.slot_base = 0;
+    a.crypto_ids = g->crypto_ids;
+    a.slot_ids = nullptr;
     a.sh_coeffs = g->sh_coeffs;
     if (const athx_packed_splats* p = g->packed) {
         a.packed = 1;
@@ -302,15 +320,51 @@ ProjectArgs project_args(const athx_gaussians* g, const athx_settings* s) {
     return a;
 }
 
+// One cloud of a frame and where it stands in each view (athx_instance).
+struct FrameInstance {
+    const athx_gaussians* g = nullptr;
+    const float* object_to_view = nullptr;  // [nviews][12], or null: the view's own
+    const float* eye = nullptr;             // [nviews][3], or null
+};
+
+// The projection's arguments of each instance: its own cloud, its slots
+// from the counts of the instances before it.
+std::vector<ProjectArgs> instance_args(const FrameInstance* inst, uint32_t ninst, const athx_settings* s,
+                                       uint32_t count, uint32_t* slot_bases = nullptr) {
+    std::vector<ProjectArgs> out;
+    out.reserve(ninst);
+    uint32_t base = 0;
+    for (uint32_t j = 0; j < ninst; ++j) {
+        ProjectArgs a = project_args(inst[j].g, s);
+        a.slot_stride = count;
+        a.slot_base = base;
+        if (slot_bases) {
+            slot_bases[j] = base;
+        }
+        base += inst[j].g->count;
+        out.push_back(a);
+    }
+    return out;
+}
+
 // Fills the pinned staging views and tile bases; returns the frame's tiles.
+// With several instances, the staging holds nviews views an instance, instance
+// by instance, each with that instance's object-to-view rows and eye; the
+// kernels after the projection read the first nviews (the screens, tiles and
+// targets, which every instance shares).
 athx_status stage_views(athx_ctx* ctx, const athx_view* views, uint32_t nviews, const athx_settings* s,
                         const athx_targets* targets, const float* const* dl, uint64_t& total_tiles,
-                        uint64_t* strips_out = nullptr) {
+                        uint64_t* strips_out = nullptr, const FrameInstance* inst = nullptr, uint32_t ninst = 1) {
     if (ctx->upload_pending) {
         ATHX_HIP(ctx, hipEventSynchronize(ctx->upload_done), "wait for the last upload");
         ctx->upload_pending = false;
     }
-    if (nviews > ctx->host_view_capacity) {
+    const uint64_t staged64 = (uint64_t)nviews * (ninst ? ninst : 1);
+    if (staged64 > (uint64_t{1} << 24)) {
+        return fail(ctx, ATHX_UNSUPPORTED, "more than 2^24 (view, instance) pairs in one frame");
+    }
+    const uint32_t staged = (uint32_t)staged64;
+    if (staged > ctx->host_view_capacity) {
         if (ctx->host_views) {
             (void)hipHostFree(ctx->host_views);
             ctx->host_views = nullptr;
@@ -320,12 +374,12 @@ athx_status stage_views(athx_ctx* ctx, const athx_view* views, uint32_t nviews, 
             ctx->host_tile_bases = nullptr;
         }
         ctx->host_view_capacity = 0;
-        ATHX_HIP(ctx, hipHostMalloc(reinterpret_cast<void**>(&ctx->host_views), sizeof(DView) * nviews),
+        ATHX_HIP(ctx, hipHostMalloc(reinterpret_cast<void**>(&ctx->host_views), sizeof(DView) * staged),
                  "pinned views");
         ATHX_HIP(ctx, hipHostMalloc(reinterpret_cast<void**>(&ctx->host_tile_bases),
-                                    sizeof(uint32_t) * 2 * (nviews + 1)),
+                                    sizeof(uint32_t) * 2 * (staged + 1)),
                  "pinned tile bases");
-        ctx->host_view_capacity = nviews;
+        ctx->host_view_capacity = staged;
     }
     total_tiles = 0;
     uint64_t total_strips = 0;
@@ -371,6 +425,24 @@ athx_status stage_views(athx_ctx* ctx, const athx_view* views, uint32_t nviews, 
         d.final_t = targets[k].final_t;
         d.last = targets[k].last;
         d.dl_dcolour = dl ? dl[k] : nullptr;
+        d.crypto = targets[k].crypto;
+        d.trace = targets[k].trace;
+        d.under_colour = targets[k].under_colour;
+        d.under_depth = targets[k].under_depth;
+        d.under_crypto = targets[k].under_crypto;
+        // The walk's variants are the frame's: every view has each or none.
+        const auto on = [&](uint32_t j) {
+            return std::array<bool, 3>{targets[j].crypto != nullptr, targets[j].trace != nullptr,
+                                       targets[j].under_colour != nullptr};
+        };
+        if (on(k) != on(0)) {
+            return fail(ctx, ATHX_INVALID_ARGUMENT,
+                        "targets[" + std::to_string(k) + "]: crypto, trace and the under layer are each the "
+                        "frame's -- given for every view or for none");
+        }
+        if ((targets[k].under_colour != nullptr) != (targets[k].under_depth != nullptr)) {
+            return fail(ctx, ATHX_INVALID_ARGUMENT, "the under layer needs both its colour and its depth");
+        }
         ctx->host_tile_bases[k] = (uint32_t)total_tiles;
         total_tiles += (uint64_t)d.tiles_x * d.tiles_y;
         if (total_tiles >= (uint64_t{1} << 32) / 2) {
@@ -382,13 +454,32 @@ athx_status stage_views(athx_ctx* ctx, const athx_view* views, uint32_t nviews, 
     if (strips_out) {
         *strips_out = total_strips;
     }
+    // Each instance's copy of the views, its own transform in place of the
+    // view's (the last first, so the first block is copied before it is
+    // changed).
+    for (uint32_t j = ninst; inst && j-- > 0;) {
+        for (uint32_t k = 0; k < nviews; ++k) {
+            DView& d = ctx->host_views[(size_t)j * nviews + k];
+            if (j != 0) {
+                d = ctx->host_views[k];
+            }
+            if (inst[j].object_to_view) {
+                std::memcpy(d.m, inst[j].object_to_view + (size_t)k * 12, sizeof d.m);
+            }
+            if (inst[j].eye) {
+                std::memcpy(d.eye, inst[j].eye + (size_t)k * 3, sizeof d.eye);
+            }
+        }
+    }
     return ATHX_OK;
 }
 
-athx_status upload_views(athx_ctx* ctx, uint32_t nviews, hipStream_t s) {
-    ATHX_HIP(ctx, reserve(ctx, ctx->views, sizeof(DView) * nviews), "views");
+// Uploads the staged views (nviews of them an instance) and tile bases.
+athx_status upload_views(athx_ctx* ctx, uint32_t nviews, hipStream_t s, uint32_t ninst = 1) {
+    const size_t staged = (size_t)nviews * ninst;
+    ATHX_HIP(ctx, reserve(ctx, ctx->views, sizeof(DView) * staged), "views");
     ATHX_HIP(ctx, reserve(ctx, ctx->tile_bases, sizeof(uint32_t) * 2 * (nviews + 1)), "tile bases");
-    ATHX_HIP(ctx, hipMemcpyAsync(ctx->views.ptr, ctx->host_views, sizeof(DView) * nviews, hipMemcpyHostToDevice, s),
+    ATHX_HIP(ctx, hipMemcpyAsync(ctx->views.ptr, ctx->host_views, sizeof(DView) * staged, hipMemcpyHostToDevice, s),
              "upload views");
     ATHX_HIP(ctx, hipMemcpyAsync(ctx->tile_bases.ptr, ctx->host_tile_bases, sizeof(uint32_t) * 2 * (nviews + 1),
                                  hipMemcpyHostToDevice, s),
@@ -441,7 +532,9 @@ SortScratch sort_scratch(const DeviceBuffer& meta, void* alt_keys, uint32_t* alt
 // The sizes a frame runs at.
 struct FrameShape {
     uint32_t nviews = 0;
-    uint32_t slots = 0;        // views * Gaussians
+    uint32_t instances = 1;    // clouds projected into each view
+    uint32_t count = 0;        // Gaussians a view: every instance's, back to back
+    uint32_t slots = 0;        // views * count
     uint32_t tiles = 0;        // over all views
     uint32_t strips = 0;       // over all views
     uint32_t key_bits = 0;     // 24 depth bits + the tile bits (the reference's pair keys)
@@ -450,6 +543,9 @@ struct FrameShape {
     uint32_t entry_cap = 0;    // strip entries the binning holds
     uint32_t seg_cap = 0;      // segments: entry_cap / kStripSeg + a partial one a strip
     uint32_t capacity = 0;     // pairs the pair buffer holds
+    std::vector<uint32_t> bases;  // each instance's first Gaussian in a view's slots, then count
+    bool matte = false, traced = false, under = false;  // the walk's variants (athx_targets)
+    bool variants() const { return matte || traced || under; }
 };
 
 void set_entry_cap(FrameShape& fs, uint32_t entries) {
@@ -458,29 +554,49 @@ void set_entry_cap(FrameShape& fs, uint32_t entries) {
 }
 
 // Validates the call and stages the views (pinned), filling the shape.
-athx_status begin_frame(athx_ctx* ctx, const athx_view* views, uint32_t nviews, const athx_gaussians* gaussians,
-                        const athx_settings* settings, const athx_targets* targets, const float* const* dl,
-                        FrameShape& fs) {
+athx_status begin_frame(athx_ctx* ctx, const athx_view* views, uint32_t nviews, const FrameInstance* inst,
+                        uint32_t ninst, const athx_settings* settings, const athx_targets* targets,
+                        const float* const* dl, FrameShape& fs) {
     if (!views || nviews == 0 || !targets) {
         return fail(ctx, ATHX_INVALID_ARGUMENT, "no views or targets");
+    }
+    if (!inst || ninst == 0) {
+        return fail(ctx, ATHX_INVALID_ARGUMENT, "no instances");
     }
     if (athx_status st = check_settings(ctx, settings); st != ATHX_OK) {
         return st;
     }
-    if (athx_status st = check_gaussians(ctx, gaussians); st != ATHX_OK) {
-        return st;
+    uint64_t count64 = 0;
+    for (uint32_t j = 0; j < ninst; ++j) {
+        if (athx_status st = check_gaussians(ctx, inst[j].g); st != ATHX_OK) {
+            if (ninst > 1) {
+                ctx->error = "instance " + std::to_string(j) + ": " + ctx->error;
+            }
+            return st;
+        }
+        count64 += inst[j].g->count;
     }
-    const uint64_t slots64 = (uint64_t)gaussians->count * nviews;
+    const uint64_t slots64 = count64 * nviews;
     if (slots64 >= (uint64_t{1} << 32)) {
         // TileRasterizer.cpp:323-325: slots are 32-bit.
         return fail(ctx, ATHX_UNSUPPORTED, "more than 2^32 (view, Gaussian) slots in one frame");
     }
     ATHX_HIP(ctx, hipSetDevice(ctx->device), "set device");
     uint64_t tiles64 = 0, strips64 = 0;
-    if (athx_status st = stage_views(ctx, views, nviews, settings, targets, dl, tiles64, &strips64); st != ATHX_OK) {
+    if (athx_status st = stage_views(ctx, views, nviews, settings, targets, dl, tiles64, &strips64, inst, ninst);
+        st != ATHX_OK) {
         return st;
     }
     fs.nviews = nviews;
+    fs.instances = ninst;
+    fs.count = (uint32_t)count64;
+    fs.matte = targets[0].crypto != nullptr;
+    fs.traced = targets[0].trace != nullptr;
+    fs.under = targets[0].under_colour != nullptr;
+    fs.bases.assign(1, 0u);
+    for (uint32_t j = 0; j < ninst; ++j) {
+        fs.bases.push_back(fs.bases.back() + inst[j].g->count);
+    }
     fs.slots = (uint32_t)slots64;
     fs.tiles = (uint32_t)tiles64;
     fs.strips = (uint32_t)strips64;
@@ -500,7 +616,7 @@ athx_status reserve_front(athx_ctx* ctx, const FrameShape& fs) {
              "scan status");
     ATHX_HIP(ctx, reserve(ctx, ctx->totals, 8 * kTotalWords), "totals");
     ATHX_HIP(ctx, reserve(ctx, ctx->ranges, (size_t)fs.tiles * 8 + 16), "ranges");
-    ATHX_HIP(ctx, reserve(ctx, ctx->counter_words, 64), "counters");
+    ATHX_HIP(ctx, reserve(ctx, ctx->counter_words, kCounterWords * 4), "counters");
     ATHX_HIP(ctx, reserve(ctx, ctx->depth_keys_sorted, slots * 4 + 16), "depth sort keys");
     ATHX_HIP(ctx, reserve(ctx, ctx->depth_slots, slots * 4 + 16), "depth sort slots");
     ATHX_HIP(ctx, reserve(ctx, ctx->depth_alt_keys, slots * 4 + 16), "depth sort keys");
@@ -510,6 +626,9 @@ athx_status reserve_front(athx_ctx* ctx, const FrameShape& fs) {
     ATHX_HIP(ctx, reserve(ctx, ctx->strip_ranges, (size_t)fs.strips * 8 + 16), "strip ranges");
     ATHX_HIP(ctx, reserve(ctx, ctx->segbase, (size_t)fs.strips * 4 + 16), "strip segments");
     ATHX_HIP(ctx, reserve(ctx, ctx->rank_fp, slots * 32 + 32), "ranked footprints");
+    if (fs.matte) {
+        ATHX_HIP(ctx, reserve(ctx, ctx->slot_ids, slots * 4 + 16), "Cryptomatte ids");
+    }
     return ATHX_OK;
 }
 
@@ -608,18 +727,26 @@ StripBins bins_of(athx_ctx* ctx, const FrameShape& fs, uint32_t n, uint32_t tile
 // depth keys, and their strip entries scanned in that order. The visible
 // total lands in totals[kTotalVisible], the entry total in
 // totals[kTotalEntries], both on the device.
-athx_status enqueue_front(athx_ctx* ctx, const ProjectArgs& pa, const FrameShape& fs, hipStream_t s, bool timed) {
+athx_status enqueue_front(athx_ctx* ctx, const std::vector<ProjectArgs>& pas, const FrameShape& fs, hipStream_t s,
+                          bool timed) {
     uint64_t* totals = as<uint64_t>(ctx->totals);
     if (timed) {
         ATHX_HIP(ctx, hipEventRecord(ctx->ev[0], s), "event");
     }
-    ATHX_HIP(ctx, launch_project(pa, as<DView>(ctx->views), fs.nviews, as<float>(ctx->records),
-                                 as<uint32_t>(ctx->touched), as<uint32_t>(ctx->depth_keys), s),
-             "project");
+    // A projection launch an instance (TileRasterizer.cpp:896-898), each with
+    // its own copy of the views.
+    for (size_t j = 0; j < pas.size(); ++j) {
+        ProjectArgs pa = pas[j];
+        pa.slot_ids = fs.matte ? as<uint32_t>(ctx->slot_ids) : nullptr;
+        ATHX_HIP(ctx, launch_project(pa, as<DView>(ctx->views) + j * fs.nviews, fs.nviews, as<float>(ctx->records),
+                                     as<uint32_t>(ctx->touched), as<uint32_t>(ctx->depth_keys), s),
+                 "project");
+    }
     if (timed) {
         ATHX_HIP(ctx, hipEventRecord(ctx->ev[1], s), "event");
     }
-    ATHX_HIP(ctx, launch_settle_count(as<DView>(ctx->views), pa.n, pa.tile, fs.slots, as<float>(ctx->records),
+    const uint32_t tile = pas.empty() ? 16u : pas[0].tile;
+    ATHX_HIP(ctx, launch_settle_count(as<DView>(ctx->views), fs.count, tile, fs.slots, as<float>(ctx->records),
                                       as<uint32_t>(ctx->touched), as<uint32_t>(ctx->depth_keys),
                                       reinterpret_cast<uint64_t*>(ctx->scan_status.ptr), s),
              "settle");
@@ -654,14 +781,21 @@ athx_status enqueue_front(athx_ctx* ctx, const ProjectArgs& pa, const FrameShape
 // total lands in totals[kTotalPairs]. Every count is read on the device
 // and bounded by fs's capacities, so the launches can be captured.
 athx_status enqueue_bins(athx_ctx* ctx, const FrameShape& fs, uint32_t n, uint32_t tile, hipStream_t s, bool timed,
-                         bool& strips_in_alt) {
+                         bool& strips_in_alt, bool count = false) {
     uint64_t* totals = as<uint64_t>(ctx->totals);
     const bool masked = masks_in_keys(fs);
     if (masked) {
+        // Counting frames keep each rank's tiles (its entries' mask bits) for the counters.
+        uint32_t* rank_tiles = nullptr;
+        if (count) {
+            ATHX_HIP(ctx, reserve(ctx, ctx->rank_tiles, (size_t)fs.slots * 4 + 16), "rank tiles");
+            rank_tiles = as<uint32_t>(ctx->rank_tiles);
+        }
         ATHX_HIP(ctx, launch_strip_mask_emit(as<DView>(ctx->views), n, tile,
                                              sorted_slots_of(ctx, ctx->last_sorted_in_alt), as<uint32_t>(ctx->offsets),
                                              totals + kTotalVisible, fs.slots, fs.entry_cap, as<float>(ctx->records),
-                                             as<uint32_t>(ctx->strip_keys), as<uint32_t>(ctx->strip_slots), s),
+                                             as<uint32_t>(ctx->strip_keys), as<uint32_t>(ctx->strip_slots), rank_tiles,
+                                             s),
                  "strip entries");
     } else {
         ATHX_HIP(ctx, launch_strip_emit(as<DView>(ctx->views), n, tile,
@@ -699,9 +833,9 @@ athx_status enqueue_bins(athx_ctx* ctx, const FrameShape& fs, uint32_t n, uint32
 }
 
 // Ranges, the pairs at their places, the blend and the counters.
-athx_status enqueue_draw(athx_ctx* ctx, const ProjectArgs& pa, const FrameShape& fs, const athx_settings* settings,
-                         hipStream_t s, bool timed, bool strips_in_alt) {
-    const StripBins b = bins_of(ctx, fs, pa.n, settings->tile, strips_in_alt);
+athx_status enqueue_draw(athx_ctx* ctx, const FrameShape& fs, const athx_settings* settings, hipStream_t s,
+                         bool timed, bool strips_in_alt, bool capturing = false) {
+    const StripBins b = bins_of(ctx, fs, fs.count, settings->tile, strips_in_alt);
     ATHX_HIP(ctx, launch_strip_pairs(b, s), "pairs");
     if (timed) {
         ATHX_HIP(ctx, hipEventRecord(ctx->ev[5], s), "event");
@@ -713,22 +847,50 @@ athx_status enqueue_draw(athx_ctx* ctx, const ProjectArgs& pa, const FrameShape&
     }
     ATHX_HIP(ctx, launch_blend(as<DView>(ctx->views), as<uint32_t>(ctx->tile_bases), fs.nviews, fs.tiles,
                                settings->tile, settings->batch, as<uint32_t>(ctx->ranges), as<uint32_t>(ctx->pair_vals),
-                               as<float>(ctx->records), settings->depth_mode, settings->depth_threshold, s),
+                               as<float>(ctx->records), settings->depth_mode, settings->depth_threshold, s,
+                               fs.variants(), fs.matte ? as<uint32_t>(ctx->slot_ids) : nullptr),
              "blend");
     if (timed) {
         ATHX_HIP(ctx, hipEventRecord(ctx->ev[7], s), "event");
     }
     if (settings->count_splats) {
-        if (athx_status st = reserve_exact(ctx, fs.slots); st != ATHX_OK) {
-            return st;
+        // SplatCounters from what the frame already holds: each slot's cull
+        // reason in its depth key, and each drawn footprint's tiles -- kept
+        // by the emission a rank when it decided them, else counted exactly
+        // a slot (frames past 65 536 strips).
+        CounterArgs ca{};
+        ca.depth_keys = as<uint32_t>(ctx->depth_keys);
+        ca.sorted_slots = sorted_slots_of(ctx, ctx->last_sorted_in_alt);
+        ca.visible_dev = as<uint64_t>(ctx->totals) + kTotalVisible;
+        ca.slots = fs.slots;
+        ca.count = fs.count;
+        ca.clouds = std::min<uint32_t>(fs.instances, ATHX_MAX_COUNTED_CLOUDS);
+        for (uint32_t j = 0; j <= ca.clouds; ++j) {
+            ca.base[j] = fs.bases[j];
         }
-        if (athx_status st = enqueue_exact(ctx, pa.n, fs.nviews, settings->tile, s); st != ATHX_OK) {
-            return st;
+        if (masks_in_keys(fs)) {
+            ca.rank_tiles = as<uint32_t>(ctx->rank_tiles);
+        } else {
+            if (athx_status st = reserve_exact(ctx, fs.slots); st != ATHX_OK) {
+                return st;
+            }
+            if (athx_status st = enqueue_exact(ctx, fs.count, fs.nviews, settings->tile, s); st != ATHX_OK) {
+                return st;
+            }
+            ca.slot_tiles = as<uint32_t>(ctx->exact);
         }
-        ATHX_HIP(ctx, launch_fill_u32(as<uint32_t>(ctx->counter_words), 0u, 16, s), "clear counters");
-        ATHX_HIP(ctx, launch_counters(as<uint32_t>(ctx->depth_keys), as<uint32_t>(ctx->exact), fs.slots,
-                                      as<uint32_t>(ctx->counter_words), s),
-                 "counters");
+        ATHX_HIP(ctx, launch_fill_u32(as<uint32_t>(ctx->counter_words), 0u, kCounterWords, s), "clear counters");
+        ATHX_HIP(ctx, launch_frame_counters(ca, as<uint32_t>(ctx->counter_words), s), "counters");
+        if (!capturing) {
+            athx_ctx::CounterCopy& c = ctx->counter_ring[ctx->counter_seq % 3];
+            ATHX_HIP(ctx, hipMemcpyAsync(c.host, ctx->counter_words.ptr, kCounterWords * 4, hipMemcpyDeviceToHost, s),
+                     "read counters back");
+            ATHX_HIP(ctx, hipEventRecord(c.done, s), "record counters");
+            c.seq = ++ctx->counter_seq;
+            c.tag = settings->counters_tag;
+            c.slots = fs.slots;
+            c.clouds = ca.clouds;
+        }
     }
     return ATHX_OK;
 }
@@ -752,13 +914,10 @@ athx_status enqueue_backward(athx_ctx* ctx, const ProjectArgs& pa, uint32_t nvie
 uint64_t grown(uint64_t seen) { return std::max<uint64_t>(seen + seen / 4, 1024); }
 
 // What the adjoint and the readers of the frame's buffers need to know.
-void finish_frame(athx_ctx* ctx, const athx_gaussians* g, const athx_settings* settings, const FrameShape& fs,
-                  uint64_t pairs_known) {
-    if (settings->count_splats) {
-        ctx->counters_valid = true;
-        ctx->counters.slots = fs.slots;
-    }
-    ctx->last_n = g->count;
+void finish_frame(athx_ctx* ctx, const athx_settings* settings, const FrameShape& fs, uint64_t pairs_known) {
+    ctx->last_n = fs.count;
+    ctx->last_instances = fs.instances;
+    ctx->last_under = fs.under;
     ctx->last_nviews = fs.nviews;
     ctx->last_tiles = fs.tiles;
     ctx->last_strips = fs.strips;
@@ -837,6 +996,19 @@ ATHX_API athx_status athx_create(int device, athx_ctx** out) {
         athx_destroy(ctx);
         return ATHX_DEVICE_FAILURE;
     }
+    for (athx_ctx::CounterCopy& c : ctx->counter_ring) {
+        if (hipHostMalloc(reinterpret_cast<void**>(&c.host), kCounterWords * 4) != hipSuccess ||
+            hipEventCreateWithFlags(&c.done, hipEventDisableTiming) != hipSuccess) {
+            athx_destroy(ctx);
+            return ATHX_DEVICE_FAILURE;
+        }
+    }
+    try {
+        ctx->error.reserve(512);
+    } catch (const std::bad_alloc&) {
+        athx_destroy(ctx);
+        return ATHX_OUT_OF_MEMORY;
+    }
     *out = ctx;
     return ATHX_OK;
 }
@@ -852,6 +1024,14 @@ ATHX_API void athx_destroy(athx_ctx* ctx) {
     }
     if (ctx->host_totals) {
         (void)hipHostFree(ctx->host_totals);
+    }
+    for (athx_ctx::CounterCopy& c : ctx->counter_ring) {
+        if (c.host) {
+            (void)hipHostFree(c.host);
+        }
+        if (c.done) {
+            (void)hipEventDestroy(c.done);
+        }
     }
     if (ctx->host_views) {
         (void)hipHostFree(ctx->host_views);
@@ -911,27 +1091,29 @@ ATHX_API uint64_t athx_scratch_bytes(const athx_ctx* ctx) {
     return sum;
 }
 
-ATHX_API athx_status athx_render(athx_ctx* ctx, const athx_view* views, uint32_t nviews,
-                                 const athx_gaussians* gaussians, const athx_settings* settings,
-                                 const athx_targets* targets, void* stream, athx_frame_stats* stats) try {
-    if (!ctx) {
-        return ATHX_INVALID_ARGUMENT;
-    }
-    ctx->error.clear();
+}  // extern "C"
+
+namespace {
+
+// athx_render and athx_render_instances: one frame of every instance in
+// every view.
+athx_status render_frame(athx_ctx* ctx, const athx_view* views, uint32_t nviews, const FrameInstance* inst,
+                         uint32_t ninst, const athx_settings* settings, const athx_targets* targets, void* stream,
+                         athx_frame_stats* stats, uint32_t* slot_bases) {
     FrameShape fs{};
-    if (athx_status st = begin_frame(ctx, views, nviews, gaussians, settings, targets, nullptr, fs); st != ATHX_OK) {
+    if (athx_status st = begin_frame(ctx, views, nviews, inst, ninst, settings, targets, nullptr, fs); st != ATHX_OK) {
         return st;
     }
     hipStream_t s = static_cast<hipStream_t>(stream);
     const bool timed = settings->time_stages != 0 && ctx->ev[7] != nullptr;
     const auto wall0 = std::chrono::steady_clock::now();
-    if (athx_status st = upload_views(ctx, nviews, s); st != ATHX_OK) {
+    if (athx_status st = upload_views(ctx, nviews, s, ninst); st != ATHX_OK) {
         return st;
     }
     if (athx_status st = reserve_front(ctx, fs); st != ATHX_OK) {
         return st;
     }
-    const ProjectArgs pa = project_args(gaussians, settings);
+    const std::vector<ProjectArgs> pas = instance_args(inst, ninst, settings, fs.count, slot_bases);
     const uint64_t* host = ctx->host_totals;
     // With the sizes of earlier frames known, the whole frame is queued at
     // once and the one wait is at its end, where the totals say whether it
@@ -949,16 +1131,16 @@ ATHX_API athx_status athx_render(athx_ctx* ctx, const athx_view* views, uint32_t
             return st;
         }
     }
-    if (athx_status st = enqueue_front(ctx, pa, fs, s, timed); st != ATHX_OK) {
+    if (athx_status st = enqueue_front(ctx, pas, fs, s, timed); st != ATHX_OK) {
         return st;
     }
     bool strips_in_alt = false;
     bool fitted = ahead;
     if (ahead) {
-        if (athx_status st = enqueue_bins(ctx, fs, pa.n, settings->tile, s, timed, strips_in_alt); st != ATHX_OK) {
+        if (athx_status st = enqueue_bins(ctx, fs, fs.count, settings->tile, s, timed, strips_in_alt, settings->count_splats != 0); st != ATHX_OK) {
             return st;
         }
-        if (athx_status st = enqueue_draw(ctx, pa, fs, settings, s, timed, strips_in_alt); st != ATHX_OK) {
+        if (athx_status st = enqueue_draw(ctx, fs, settings, s, timed, strips_in_alt); st != ATHX_OK) {
             return st;
         }
         if (athx_status st = read_totals(ctx, s); st != ATHX_OK) {
@@ -979,7 +1161,7 @@ ATHX_API athx_status athx_render(athx_ctx* ctx, const athx_view* views, uint32_t
             if (athx_status st = reserve_bins(ctx, fs); st != ATHX_OK) {
                 return st;
             }
-            if (athx_status st = enqueue_bins(ctx, fs, pa.n, settings->tile, s, timed, strips_in_alt); st != ATHX_OK) {
+            if (athx_status st = enqueue_bins(ctx, fs, fs.count, settings->tile, s, timed, strips_in_alt, settings->count_splats != 0); st != ATHX_OK) {
                 return st;
             }
             if (athx_status st = read_totals(ctx, s); st != ATHX_OK) {
@@ -994,22 +1176,27 @@ ATHX_API athx_status athx_render(athx_ctx* ctx, const athx_view* views, uint32_t
         if (athx_status st = reserve_pairs(ctx, fs); st != ATHX_OK) {
             return st;
         }
-        if (athx_status st = enqueue_draw(ctx, pa, fs, settings, s, timed, strips_in_alt); st != ATHX_OK) {
+        if (athx_status st = enqueue_draw(ctx, fs, settings, s, timed, strips_in_alt); st != ATHX_OK) {
             return st;
         }
         if (timed) {
             ATHX_HIP(ctx, hipStreamSynchronize(s), "wait for frame");
         }
     }
+    if (settings->count_splats) {
+        // A counting frame waits for itself (TileRasterizer.cpp:1269-1282), so
+        // its counters are the newest finished when render returns.
+        ATHX_HIP(ctx, hipStreamSynchronize(s), "wait for the counted frame");
+    }
     const uint64_t pairs64 = host[kTotalPairs];
     ctx->entry_last = std::min<uint64_t>(grown(host[kTotalEntries]), kMaxPairs - 1);
     ctx->pair_last = std::min<uint64_t>(grown(pairs64), kMaxPairs - 1);
     ctx->entry_hint = std::max(ctx->entry_hint, ctx->entry_last);
     ctx->pair_hint = std::max(ctx->pair_hint, ctx->pair_last);
-    finish_frame(ctx, gaussians, settings, fs, pairs64);
+    finish_frame(ctx, settings, fs, pairs64);
     if (stats) {
         std::memset(stats, 0, sizeof *stats);
-        stats->splats = gaussians->count;
+        stats->splats = fs.count;
         stats->visible = (uint32_t)host[kTotalVisible];
         stats->pairs = pairs64;
         stats->frame_serial = ctx->serial;
@@ -1026,6 +1213,46 @@ ATHX_API athx_status athx_render(athx_ctx* ctx, const athx_view* views, uint32_t
         }
     }
     return ATHX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+ATHX_API athx_status athx_render(athx_ctx* ctx, const athx_view* views, uint32_t nviews,
+                                 const athx_gaussians* gaussians, const athx_settings* settings,
+                                 const athx_targets* targets, void* stream, athx_frame_stats* stats) try {
+    if (!ctx) {
+        return ATHX_INVALID_ARGUMENT;
+    }
+    ctx->error.clear();
+    FrameInstance one;
+    one.g = gaussians;
+    return render_frame(ctx, views, nviews, &one, 1, settings, targets, stream, stats, nullptr);
+} catch (const std::bad_alloc&) {
+    return fail(ctx, ATHX_OUT_OF_MEMORY, "host allocation failed");
+} catch (const std::exception& e) {
+    return fail(ctx, ATHX_INTERNAL_ERROR, e.what());
+}
+
+ATHX_API athx_status athx_render_instances(athx_ctx* ctx, const athx_view* views, uint32_t nviews,
+                                           const athx_instance* instances, uint32_t ninstances,
+                                           const athx_settings* settings, const athx_targets* targets, void* stream,
+                                           athx_frame_stats* stats, uint32_t* slot_bases) try {
+    if (!ctx) {
+        return ATHX_INVALID_ARGUMENT;
+    }
+    ctx->error.clear();
+    if (!instances || ninstances == 0) {
+        return fail(ctx, ATHX_INVALID_ARGUMENT, "no instances");
+    }
+    std::vector<FrameInstance> inst(ninstances);
+    for (uint32_t j = 0; j < ninstances; ++j) {
+        inst[j].g = instances[j].gaussians;
+        inst[j].object_to_view = instances[j].object_to_view;
+        inst[j].eye = instances[j].eye;
+    }
+    return render_frame(ctx, views, nviews, inst.data(), ninstances, settings, targets, stream, stats, slot_bases);
 } catch (const std::bad_alloc&) {
     return fail(ctx, ATHX_OUT_OF_MEMORY, "host allocation failed");
 } catch (const std::exception& e) {
@@ -1040,7 +1267,9 @@ ATHX_API athx_status athx_render_async(athx_ctx* ctx, const athx_view* views, ui
     }
     ctx->error.clear();
     FrameShape fs{};
-    if (athx_status st = begin_frame(ctx, views, nviews, gaussians, settings, targets, nullptr, fs); st != ATHX_OK) {
+    FrameInstance one;
+    one.g = gaussians;
+    if (athx_status st = begin_frame(ctx, views, nviews, &one, 1, settings, targets, nullptr, fs); st != ATHX_OK) {
         return st;
     }
     const uint64_t cap = pair_capacity ? pair_capacity : ctx->pair_hint;
@@ -1068,22 +1297,22 @@ ATHX_API athx_status athx_render_async(athx_ctx* ctx, const athx_view* views, ui
     if (athx_status st = reserve_pairs(ctx, fs); st != ATHX_OK) {
         return st;
     }
-    const ProjectArgs pa = project_args(gaussians, settings);
-    if (athx_status st = enqueue_front(ctx, pa, fs, s, false); st != ATHX_OK) {
+    const std::vector<ProjectArgs> pas = instance_args(&one, 1, settings, fs.count);
+    if (athx_status st = enqueue_front(ctx, pas, fs, s, false); st != ATHX_OK) {
         return st;
     }
     bool strips_in_alt = false;
-    if (athx_status st = enqueue_bins(ctx, fs, pa.n, settings->tile, s, false, strips_in_alt); st != ATHX_OK) {
+    if (athx_status st = enqueue_bins(ctx, fs, fs.count, settings->tile, s, false, strips_in_alt, settings->count_splats != 0); st != ATHX_OK) {
         return st;
     }
-    if (athx_status st = enqueue_draw(ctx, pa, fs, settings, s, false, strips_in_alt); st != ATHX_OK) {
+    if (athx_status st = enqueue_draw(ctx, fs, settings, s, false, strips_in_alt); st != ATHX_OK) {
         return st;
     }
     ATHX_HIP(ctx, hipMemcpyAsync(ctx->host_totals, ctx->totals.ptr, 8 * kTotalWords, hipMemcpyDeviceToHost, s),
              "read totals");
     ATHX_HIP(ctx, hipEventRecord(ctx->async_done, s), "record frame");
     ctx->async_pending = true;
-    finish_frame(ctx, gaussians, settings, fs, cap);
+    finish_frame(ctx, settings, fs, cap);
     return ATHX_OK;
 } catch (const std::bad_alloc&) {
     return fail(ctx, ATHX_OUT_OF_MEMORY, "host allocation failed");
@@ -1138,6 +1367,12 @@ ATHX_API athx_status athx_render_backward(athx_ctx* ctx, const athx_view* views,
     }
     if (athx_status st = check_serial(ctx, frame_serial); st != ATHX_OK) {
         return st;
+    }
+    if (ctx->last_instances != 1) {
+        return fail(ctx, ATHX_UNSUPPORTED, "the adjoint follows a one-cloud frame, not a multi-instance one");
+    }
+    if (ctx->last_under) {
+        return fail(ctx, ATHX_UNSUPPORTED, "the adjoint of a frame over an under layer is not built");
     }
     if (ctx->last_n != gaussians->count || ctx->last_nviews != nviews || ctx->last_tile != settings->tile) {
         return fail(ctx, ATHX_INVALID_ARGUMENT, "the backward pass must follow a render of the same frame");
@@ -1195,9 +1430,12 @@ ATHX_API athx_status athx_contribution(athx_ctx* ctx, const athx_view* views, ui
     if (athx_status st = check_serial(ctx, frame_serial); st != ATHX_OK) {
         return st;
     }
-    if (!gaussians || ctx->last_n != gaussians->count || ctx->last_nviews != nviews ||
+    if ((gaussians && ctx->last_n != gaussians->count) || ctx->last_nviews != nviews ||
         ctx->last_tile != settings->tile) {
         return fail(ctx, ATHX_INVALID_ARGUMENT, "contributions must follow a render of the same frame");
+    }
+    if (ctx->last_under) {
+        return fail(ctx, ATHX_UNSUPPORTED, "contributions of a frame over an under layer are not built");
     }
     if (mode != 1) {
         for (uint32_t k = 0; k < nviews; ++k) {
@@ -1219,7 +1457,7 @@ ATHX_API athx_status athx_contribution(athx_ctx* ctx, const athx_view* views, ui
     if (athx_status st = upload_views(ctx, nviews, s); st != ATHX_OK) {
         return st;
     }
-    const uint64_t words = (uint64_t)gaussians->count * nviews * (mode == 4 ? 4u : mode == 1 ? 1u : 2u);
+    const uint64_t words = (uint64_t)ctx->last_n * nviews * (mode == 4 ? 4u : mode == 1 ? 1u : 2u);
     ATHX_HIP(ctx, launch_fill_u32(out, 0u, words, s), "clear contributions");
     ATHX_HIP(ctx, launch_contribution(as<DView>(ctx->views), as<uint32_t>(ctx->tile_bases), nviews, ctx->last_tiles,
                                       settings->tile, settings->batch, as<uint32_t>(ctx->ranges),
@@ -1238,20 +1476,36 @@ ATHX_API athx_status athx_latest_counters(athx_ctx* ctx, athx_counters* out)  tr
     if (!ctx || !out) {
         return ATHX_INVALID_ARGUMENT;
     }
-    if (!ctx->counters_valid) {
-        return fail(ctx, ATHX_NOT_FOUND, "no frame was rendered with count_splats");
+    ctx->error.clear();
+    const athx_ctx::CounterCopy* newest = nullptr;
+    for (const athx_ctx::CounterCopy& c : ctx->counter_ring) {
+        if (c.seq == 0 || (newest && c.seq < newest->seq)) {
+            continue;
+        }
+        const hipError_t q = hipEventQuery(c.done);
+        if (q == hipSuccess) {
+            newest = &c;
+        } else if (q != hipErrorNotReady) {
+            return hip_fail(ctx, q, "counters readback");
+        }
     }
-    uint32_t words[16] = {};
-    ATHX_HIP(ctx, hipSetDevice(ctx->device), "set device");
-    ATHX_HIP(ctx, hipDeviceSynchronize(), "wait for counters");
-    ATHX_HIP(ctx, hipMemcpy(words, ctx->counter_words.ptr, sizeof words, hipMemcpyDeviceToHost), "read counters");
+    if (!newest) {
+        return fail(ctx, ATHX_NOT_FOUND, "no frame counted with count_splats has finished");
+    }
+    const uint32_t* w = newest->host;
     std::memset(out, 0, sizeof *out);
-    out->slots = ctx->counters.slots;
-    out->visible = words[1];
-    out->pairs = (uint64_t)words[2] | ((uint64_t)words[3] << 32);
-    out->max_tiles = words[4];
+    out->tag = newest->tag;
+    out->slots = newest->slots;
+    out->visible = w[1];
+    out->pairs = (uint64_t)w[2] | ((uint64_t)w[3] << 32);
+    out->max_tiles = w[4];
     for (uint32_t r = 0; r < ATHX_CULL_REASONS; ++r) {
-        out->culled[r] = words[8 + r];
+        out->culled[r] = w[8 + r];
+    }
+    out->clouds = newest->clouds;
+    for (uint32_t j = 0; j < newest->clouds; ++j) {
+        out->cloud_visible[j] = w[16 + 2 * j];
+        out->cloud_pairs[j] = w[17 + 2 * j];
     }
     return ATHX_OK;
 } catch (const std::bad_alloc&) {
@@ -1561,13 +1815,13 @@ athx_status graph_fail(athx_graph* g, athx_status st, char* error, uint32_t erro
 athx_status capture_forward(athx_graph* g, const ProjectArgs& pa, const FrameShape& fs, const athx_settings* settings) {
     athx_ctx* ctx = g->ctx;
     ATHX_HIP(ctx, hipStreamBeginCapture(g->capture, hipStreamCaptureModeThreadLocal), "begin capture");
-    athx_status st = enqueue_front(ctx, pa, fs, g->capture, false);
+    athx_status st = enqueue_front(ctx, std::vector<ProjectArgs>{pa}, fs, g->capture, false);
     bool strips_in_alt = false;
     if (st == ATHX_OK) {
-        st = enqueue_bins(ctx, fs, pa.n, settings->tile, g->capture, false, strips_in_alt);
+        st = enqueue_bins(ctx, fs, fs.count, settings->tile, g->capture, false, strips_in_alt, settings->count_splats != 0);
     }
     if (st == ATHX_OK) {
-        st = enqueue_draw(ctx, pa, fs, settings, g->capture, false, strips_in_alt);
+        st = enqueue_draw(ctx, fs, settings, g->capture, false, strips_in_alt, true);
     }
     hipError_t copy = hipSuccess;
     if (st == ATHX_OK) {
@@ -1642,7 +1896,9 @@ ATHX_API athx_status athx_graph_create(int device, const athx_view* views, uint3
         return graph_fail(g, fail(ctx, ATHX_UNSUPPORTED, "a pair capacity of 2^30 or more"), error, error_bytes);
     }
     FrameShape fs{};
-    if (athx_status st = begin_frame(ctx, views, nviews, gaussians, settings, targets,
+    FrameInstance one;
+    one.g = gaussians;
+    if (athx_status st = begin_frame(ctx, views, nviews, &one, 1, settings, targets,
                                      with_backward ? dl_dcolour : nullptr, fs);
         st != ATHX_OK) {
         return graph_fail(g, st, error, error_bytes);
@@ -1661,6 +1917,9 @@ ATHX_API athx_status athx_graph_create(int device, const athx_view* views, uint3
     if (settings && settings->count_splats) {
         if (athx_status st = reserve_exact(ctx, fs.slots); st != ATHX_OK) {
             return graph_fail(g, st, error, error_bytes);
+        }
+        if (hipError_t e = reserve(ctx, ctx->rank_tiles, (size_t)fs.slots * 4 + 16); e != hipSuccess) {
+            return graph_fail(g, hip_fail(ctx, e, "rank tiles"), error, error_bytes);
         }
     }
     if (with_backward) {
@@ -1685,7 +1944,7 @@ ATHX_API athx_status athx_graph_create(int device, const athx_view* views, uint3
         return graph_fail(g, fail(ctx, ATHX_DEVICE_FAILURE, "capture stream"), error, error_bytes);
     }
     const ProjectArgs pa = project_args(gaussians, settings);
-    finish_frame(ctx, gaussians, settings, fs, 0);
+    finish_frame(ctx, settings, fs, 0);
     if (athx_status st = capture_forward(g, pa, fs, settings); st != ATHX_OK) {
         return graph_fail(g, st, error, error_bytes);
     }
@@ -1868,11 +2127,13 @@ ATHX_API athx_status athx_prepare_sky(athx_ctx* ctx, const float* equirect, uint
     if (!equirect || !texels || !sh) {
         return fail(ctx, ATHX_INVALID_ARGUMENT, "null equirect, texels or sh");
     }
-    if (width == 0 || height == 0 || (uint64_t)width * height > (1u << 26)) {
-        return fail(ctx, ATHX_INVALID_ARGUMENT, "equirect size out of range");
+    // The rough levels integrate every source texel for every output texel:
+    // these bounds keep one launch under about 10^11 lobe evaluations.
+    if (width == 0 || height == 0 || (uint64_t)width * height > (1u << 18)) {
+        return fail(ctx, ATHX_INVALID_ARGUMENT, "equirect size out of range (at most 2^18 texels)");
     }
-    if (base_side == 0 || base_side > 8192) {
-        return fail(ctx, ATHX_INVALID_ARGUMENT, "base_side out of range");
+    if (base_side == 0 || base_side > 1024) {
+        return fail(ctx, ATHX_INVALID_ARGUMENT, "base_side out of range (1 to 1024)");
     }
     ATHX_HIP(ctx, hipSetDevice(ctx->device), "set device");
     ATHX_HIP(ctx, launch_prepare_sky(equirect, width, height, base_side, texels, sh, nullptr), "prepare sky");
@@ -1884,41 +2145,51 @@ ATHX_API athx_status athx_prepare_sky(athx_ctx* ctx, const float* equirect, uint
     return fail(ctx, ATHX_INTERNAL_ERROR, e.what());
 }
 
-ATHX_API athx_status athx_reference_render(athx_ctx* ctx, const athx_view* view, const athx_gaussians* g,
-                                           const athx_settings* settings, const athx_targets* targets,
-                                           float* records, uint32_t* overflow)  try {
-    if (!ctx) {
-        return ATHX_INVALID_ARGUMENT;
-    }
-    ctx->error.clear();
-    if (!view || !g || !targets || !overflow) {
+}  // extern "C"
+
+namespace {
+
+// athx_reference_render(_instances): the ground truth of one view.
+athx_status reference_frame(athx_ctx* ctx, const athx_view* view, const FrameInstance* inst, uint32_t ninst,
+                            const athx_settings* settings, const athx_targets* targets, float* records,
+                            uint32_t* overflow) {
+    if (!view || !inst || ninst == 0 || !targets || !overflow) {
         return fail(ctx, ATHX_INVALID_ARGUMENT, "null view, gaussians, targets or overflow");
     }
     if (athx_status st = check_settings(ctx, settings); st != ATHX_OK) {
         return st;
     }
-    const athx_packed_splats* pk = g->packed;
-    if (!pk || !pk->positions || !pk->shape) {
-        return fail(ctx, ATHX_UNSUPPORTED, "the reference renderer reads the packed layout (gaussians->packed)");
+    uint64_t total = 0;
+    for (uint32_t j = 0; j < ninst; ++j) {
+        const athx_gaussians* g = inst[j].g;
+        const athx_packed_splats* pk = g ? g->packed : nullptr;
+        if (!pk || !pk->positions || !pk->shape) {
+            return fail(ctx, ATHX_UNSUPPORTED, "the reference renderer reads the packed layout (gaussians->packed)");
+        }
+        if (pk->rest_per_colour != 0 && pk->rest_per_colour != 3 && pk->rest_per_colour != 8 &&
+            pk->rest_per_colour != 15) {
+            return fail(ctx, ATHX_INVALID_ARGUMENT, "rest_per_colour must be 0, 3, 8 or 15");
+        }
+        if (pk->rest_per_colour != 0 && !pk->sh) {
+            return fail(ctx, ATHX_INVALID_ARGUMENT, "null sh with rest coefficients");
+        }
+        total += g->count;
     }
-    if (pk->rest_per_colour != 0 && pk->rest_per_colour != 3 && pk->rest_per_colour != 8 &&
-        pk->rest_per_colour != 15) {
-        return fail(ctx, ATHX_INVALID_ARGUMENT, "rest_per_colour must be 0, 3, 8 or 15");
-    }
-    if (pk->rest_per_colour != 0 && !pk->sh) {
-        return fail(ctx, ATHX_INVALID_ARGUMENT, "null sh with rest coefficients");
+    if (total >= (uint64_t{1} << 31)) {
+        return fail(ctx, ATHX_UNSUPPORTED, "more than 2^31 splats for the ground truth");
     }
     *overflow = 0;
     ATHX_HIP(ctx, hipSetDevice(ctx->device), "set device");
     uint64_t tiles = 0;
-    if (athx_status st = stage_views(ctx, view, 1, settings, targets, nullptr, tiles); st != ATHX_OK) {
+    if (athx_status st = stage_views(ctx, view, 1, settings, targets, nullptr, tiles, nullptr, inst, ninst);
+        st != ATHX_OK) {
         return st;
     }
     const hipStream_t s = nullptr;
-    if (athx_status st = upload_views(ctx, 1, s); st != ATHX_OK) {
+    if (athx_status st = upload_views(ctx, 1, s, ninst); st != ATHX_OK) {
         return st;
     }
-    const uint32_t n = g->count;
+    const uint32_t n = (uint32_t)total;
     const size_t cap = n ? n : 1;
     ATHX_HIP(ctx, reserve(ctx, ctx->ref_records, cap * kRecordFloats * 4), "reference records");
     ATHX_HIP(ctx, reserve(ctx, ctx->ref_keys, cap * 4 + 16), "reference keys");
@@ -1932,10 +2203,16 @@ ATHX_API athx_status athx_reference_render(athx_ctx* ctx, const athx_view* view,
     // The sort's count (a u64: n, 0) and the overflow counter, written on the stream.
     ATHX_HIP(ctx, launch_fill_u32(reinterpret_cast<uint32_t*>(count), n, 1, s), "count");
     ATHX_HIP(ctx, launch_fill_u32(reinterpret_cast<uint32_t*>(count) + 1, 0u, 3, s), "count and overflow");
-    ATHX_HIP(ctx, launch_ref_project(pk->positions, pk->shape, pk->sh, n, pk->rest_per_colour, settings->max_sh_degree,
-                                     settings->antialias, settings->linear, as<DView>(ctx->views), rec,
-                                     as<uint32_t>(ctx->ref_keys), as<uint32_t>(ctx->ref_values), s),
-             "reference projection");
+    uint32_t base = 0;
+    for (uint32_t j = 0; j < ninst; ++j) {
+        const athx_packed_splats* pk = inst[j].g->packed;
+        ATHX_HIP(ctx, launch_ref_project(pk->positions, pk->shape, pk->sh, inst[j].g->count, pk->rest_per_colour,
+                                         settings->max_sh_degree, settings->antialias, settings->linear,
+                                         as<DView>(ctx->views) + j, rec, as<uint32_t>(ctx->ref_keys),
+                                         as<uint32_t>(ctx->ref_values), s, base),
+                 "reference projection");
+        base += inst[j].g->count;
+    }
     // The drawn splats in (depth, index) order: a stable sort of the
     // depth's 32 bits (culled keys, all ones, last).
     const SortScratch sc = sort_scratch(ctx->ref_sort_meta, ctx->ref_alt_keys.ptr, as<uint32_t>(ctx->ref_alt_values),
@@ -1952,6 +2229,46 @@ ATHX_API athx_status athx_reference_render(athx_ctx* ctx, const athx_view* view,
     ATHX_HIP(ctx, hipMemcpy(&host_over, over, sizeof host_over, hipMemcpyDeviceToHost), "read overflow");
     *overflow = host_over;
     return ATHX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+ATHX_API athx_status athx_reference_render(athx_ctx* ctx, const athx_view* view, const athx_gaussians* g,
+                                           const athx_settings* settings, const athx_targets* targets,
+                                           float* records, uint32_t* overflow)  try {
+    if (!ctx) {
+        return ATHX_INVALID_ARGUMENT;
+    }
+    ctx->error.clear();
+    FrameInstance one;
+    one.g = g;
+    return reference_frame(ctx, view, g ? &one : nullptr, 1, settings, targets, records, overflow);
+} catch (const std::bad_alloc&) {
+    return fail(ctx, ATHX_OUT_OF_MEMORY, "host allocation failed");
+} catch (const std::exception& e) {
+    return fail(ctx, ATHX_INTERNAL_ERROR, e.what());
+}
+
+ATHX_API athx_status athx_reference_render_instances(athx_ctx* ctx, const athx_view* view,
+                                                     const athx_instance* instances, uint32_t ninstances,
+                                                     const athx_settings* settings, const athx_targets* targets,
+                                                     float* records, uint32_t* overflow)  try {
+    if (!ctx) {
+        return ATHX_INVALID_ARGUMENT;
+    }
+    ctx->error.clear();
+    if (!instances || ninstances == 0) {
+        return fail(ctx, ATHX_INVALID_ARGUMENT, "no instances");
+    }
+    std::vector<FrameInstance> inst(ninstances);
+    for (uint32_t j = 0; j < ninstances; ++j) {
+        inst[j].g = instances[j].gaussians;
+        inst[j].object_to_view = instances[j].object_to_view;
+        inst[j].eye = instances[j].eye;
+    }
+    return reference_frame(ctx, view, inst.data(), ninstances, settings, targets, records, overflow);
 } catch (const std::bad_alloc&) {
     return fail(ctx, ATHX_OUT_OF_MEMORY, "host allocation failed");
 } catch (const std::exception& e) {

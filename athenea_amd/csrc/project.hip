@@ -235,7 +235,7 @@ __global__ void __launch_bounds__(256) project_kernel(ProjectArgs a, const DView
 
     for (uint32_t v = 0; v < nviews; ++v) {
         const DView& V = views[v];
-        const uint32_t slot = v * a.n + i;
+        const uint32_t slot = v * a.slot_stride + a.slot_base + i;
         const float* m = V.m;
         const float tx = m[0] * px + m[1] * py + m[2] * pz + m[3];
         const float ty = m[4] * px + m[5] * py + m[6] * pz + m[7];
@@ -372,6 +372,10 @@ __global__ void __launch_bounds__(256) project_kernel(ProjectArgs a, const DView
         rec[2] = make_float4(col[0], col[1], col[2], __builtin_bit_cast(float, flags));
         keys[slot] = __builtin_bit_cast(uint32_t, tz) >> 7;
         touched[slot] = entries | (settled ? 0u : kUnsettled);
+        if (a.slot_ids) {
+            // splat_project.slang:1116-1118
+            a.slot_ids[slot] = a.crypto_ids ? a.crypto_ids[i] : 0u;
+        }
     }
 }
 
@@ -1070,8 +1074,10 @@ __global__ void __launch_bounds__(kEmitThreads) strip_mask_emit_kernel(const DVi
                                                                        uint32_t max_visible, uint32_t cap,
                                                                        const float* __restrict__ records,
                                                                        uint32_t* __restrict__ keys_out,
-                                                                       uint32_t* __restrict__ slots_out) {
+                                                                       uint32_t* __restrict__ slots_out,
+                                                                       uint32_t* __restrict__ rank_tiles) {
     __shared__ uint32_t s_end[kEmitThreads];
+    __shared__ uint32_t s_tiles[kEmitThreads]; // rank: tiles its entries' masks hold (counting frames)
     __shared__ float4 s_geo[kEmitThreads];   // rank: centre x, y, conic a, b
     __shared__ float4 s_geo2[kEmitThreads];  // rank: conic c, cut power, 1 / a, 1 / c
     __shared__ uint4 s_int[kEmitThreads];    // rank: slot, first | last tile column << 16, first row, first strip
@@ -1105,6 +1111,7 @@ __global__ void __launch_bounds__(kEmitThreads) strip_mask_emit_kernel(const DVi
     const uint32_t end = wave_inclusive_sum(count, lane);
     s_end[threadIdx.x] = end;
     s_own[threadIdx.x] = ~0u;
+    s_tiles[threadIdx.x] = 0;
     const uint32_t total = lane_u(end, 63);
     const uint32_t base = k0 < visible ? offsets[k0 / 64u] : 0u;
     const uint32_t start = end - count;
@@ -1168,6 +1175,16 @@ __global__ void __launch_bounds__(kEmitThreads) strip_mask_emit_kernel(const DVi
             keys_out[at] = (I.w + __umul24(row, I2.y) + sxi) | (mask << 16);
             slots_out[at] = I.x;
         }
+        if (rank_tiles) {
+            atomicAdd(&s_tiles[wbase + j], (uint32_t)__builtin_popcount(mask));
+        }
+    }
+    if (rank_tiles && k < visible) {
+        // The wavefront's own ranks: its LDS adds are done when it gets here.
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        rank_tiles[k] = s_tiles[threadIdx.x];
     }
 }
 
@@ -1599,42 +1616,64 @@ __global__ void __launch_bounds__(256) pair_keys_kernel(const uint32_t* __restri
     }
 }
 
-// SplatCounters words (splat_frame_counters.slang): [0] slots, [1] visible,
-// [2] pairs (low), [3] pairs (high), [4] most tiles of one slot, [8 + r] culls
-// of reason r.
-__global__ void __launch_bounds__(256) counters_kernel(const uint32_t* __restrict__ depth_keys,
-                                                       const uint32_t* __restrict__ touched, uint64_t slots,
-                                                       uint32_t* __restrict__ words) {
-    __shared__ uint32_t local[16];
-    if (threadIdx.x < 16) {
-        local[threadIdx.x] = 0;
+// SplatCounters (splat_frame_counters.slang; words as kCounterWords says):
+// each slot's cull reason from its depth key, each drawn footprint's tiles
+// from what the frame kept of them, and per cloud its drawn footprints and
+// pairs. A workgroup tallies in LDS and adds each nonzero word once.
+__global__ void __launch_bounds__(256) frame_counters_kernel(CounterArgs a, uint32_t* __restrict__ words) {
+    __shared__ uint32_t local[kCounterWords];
+    for (uint32_t i = threadIdx.x; i < kCounterWords; i += 256) {
+        local[i] = 0;
     }
     __syncthreads();
-    uint32_t vis = 0, most = 0;
-    uint64_t pairs = 0;
-    for (uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; s < slots; s += (uint64_t)gridDim.x * blockDim.x) {
-        const uint32_t t = touched[s];
-        if (t != 0) {
-            vis += 1;
-            pairs += t;
-            most = t > most ? t : most;
-        } else {
-            const uint32_t k = depth_keys[s];
-            const uint32_t reason = (k & 0xFF000000u) == kCulledKey ? (k & 0xFFu) : 0u;
-            atomicAdd(&local[8 + (reason < ATHX_CULL_REASONS ? reason : 0u)], 1u);
+    const uint64_t vis_total = *a.visible_dev;
+    const uint32_t visible = vis_total < a.slots ? (uint32_t)vis_total : a.slots;
+    const uint32_t stride = gridDim.x * 256u;
+    for (uint32_t s = blockIdx.x * 256u + threadIdx.x; s < a.slots; s += stride) {
+        const uint32_t k = a.depth_keys[s];
+        if ((k & 0xFF000000u) == kCulledKey) {
+            const uint32_t reason = (k & 0xFFu) < ATHX_CULL_REASONS - 1 ? (k & 0xFFu) : ATHX_CULL_REASONS - 1;
+            atomicAdd(&local[8 + reason], 1u);
         }
     }
+    uint32_t vis = 0, pairs = 0, most = 0;
+    for (uint32_t r = blockIdx.x * 256u + threadIdx.x; r < visible; r += stride) {
+        const uint32_t slot = a.sorted_slots[r];
+        const uint32_t tiles = a.rank_tiles ? a.rank_tiles[r] : a.slot_tiles[slot];
+        vis += 1;
+        pairs += tiles;
+        most = tiles > most ? tiles : most;
+        if (a.clouds != 0) {
+            const uint32_t g = slot % a.count;
+            if (g < a.base[a.clouds]) {
+                uint32_t lo = 0, hi = a.clouds;  // base[lo] <= g < base[hi]
+                while (hi - lo > 1u) {
+                    const uint32_t mid = (lo + hi) >> 1;
+                    if (a.base[mid] <= g) {
+                        lo = mid;
+                    } else {
+                        hi = mid;
+                    }
+                }
+                atomicAdd(&local[16 + 2 * lo], 1u);
+                atomicAdd(&local[17 + 2 * lo], tiles);
+            }
+        }
+    }
+    // Pairs stay below 2^30 a frame, so one word holds a workgroup's sum.
     atomicAdd(&local[1], vis);
-    atomicAdd(&local[2], (uint32_t)pairs);
-    atomicAdd(&local[3], (uint32_t)(pairs >> 32));
+    atomicAdd(&local[2], pairs);
     atomicMax(&local[4], most);
     __syncthreads();
-    if (threadIdx.x < 16 && threadIdx.x != 0 && threadIdx.x != 4) {
-        if (local[threadIdx.x] != 0) {
-            atomicAdd(&words[threadIdx.x], local[threadIdx.x]);
+    for (uint32_t i = threadIdx.x; i < kCounterWords; i += 256) {
+        if (i == 0 || local[i] == 0) {
+            continue;
         }
-    } else if (threadIdx.x == 4) {
-        atomicMax(&words[4], local[4]);
+        if (i == 4) {
+            atomicMax(&words[4], local[4]);
+        } else {
+            atomicAdd(&words[i], local[i]);
+        }
     }
 }
 
@@ -1716,16 +1755,13 @@ hipError_t launch_ranges(const uint32_t* sorted_keys, const uint64_t* n_dev, uin
     return hipGetLastError();
 }
 
-hipError_t launch_counters(const uint32_t* depth_keys, const uint32_t* touched, uint64_t slots, uint32_t* words,
-                           hipStream_t s) {
-    if (slots == 0) {
+hipError_t launch_frame_counters(const CounterArgs& a, uint32_t* words, hipStream_t s) {
+    if (a.slots == 0) {
         return hipSuccess;
     }
-    uint64_t blocks = (slots + 255) / 256;
-    if (blocks > 1024) {
-        blocks = 1024;
-    }
-    hipLaunchKernelGGL(counters_kernel, dim3((uint32_t)blocks), dim3(256), 0, s, depth_keys, touched, slots, words);
+    // Few workgroups, each over many slots: every one adds its words once.
+    const uint32_t blocks = div_up(a.slots, 256u * 32u) < 512u ? div_up(a.slots, 256u * 32u) : 512u;
+    hipLaunchKernelGGL(frame_counters_kernel, dim3(blocks), dim3(256), 0, s, a, words);
     return hipGetLastError();
 }
 
@@ -1767,13 +1803,13 @@ hipError_t launch_strip_count(const StripBins& b, hipStream_t s) {
 hipError_t launch_strip_mask_emit(const DView* views_dev, uint32_t n, uint32_t tile, const uint32_t* sorted_slots,
                                   const uint32_t* offsets, const uint64_t* visible_dev, uint32_t max_visible,
                                   uint32_t cap, const float* records, uint32_t* strip_keys, uint32_t* strip_slots,
-                                  hipStream_t s) {
+                                  uint32_t* rank_tiles, hipStream_t s) {
     if (max_visible == 0 || cap == 0) {
         return hipSuccess;
     }
     hipLaunchKernelGGL(strip_mask_emit_kernel, dim3(div_up(max_visible, kEmitThreads)), dim3(kEmitThreads), 0, s,
                        views_dev, n, tile, sorted_slots, offsets, visible_dev, max_visible, cap, records,
-                       strip_keys, strip_slots);
+                       strip_keys, strip_slots, rank_tiles);
     return hipGetLastError();
 }
 

@@ -102,9 +102,20 @@ __global__ void __launch_bounds__(256) project_backward_kernel(ProjectArgs a, co
     }
     bool any = false;
     for (uint32_t v = 0; v < nviews; ++v) {
-        any = any || touched[(size_t)v * a.n + i] != 0;
+        any = any || touched[(size_t)v * a.slot_stride + a.slot_base + i] != 0;
     }
     if (!any) {
+        if (out.overwrite) {
+            // Not drawn in any view: its geometry gradients are zero.
+            for (int c = 0; c < 3; ++c) {
+                if (out.means) out.means[i * 3 + c] = 0.0f;
+                if (out.log_scales) out.log_scales[i * 3 + c] = 0.0f;
+            }
+            for (int c = 0; c < 4; ++c) {
+                if (out.quats) out.quats[i * 4 + c] = 0.0f;
+            }
+            if (out.opacity_logits) out.opacity_logits[i] = 0.0f;
+        }
         return;
     }
     const double px = a.means[i * 3 + 0], py = a.means[i * 3 + 1], pz = a.means[i * 3 + 2];
@@ -134,7 +145,7 @@ __global__ void __launch_bounds__(256) project_backward_kernel(ProjectArgs a, co
     double g_M[3][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
     double g_opacity = 0.0;
     for (uint32_t v = 0; v < nviews; ++v) {
-        const size_t slot = (size_t)v * a.n + i;
+        const size_t slot = (size_t)v * a.slot_stride + a.slot_base + i;
         if (touched[slot] == 0) {
             continue;
         }
@@ -260,15 +271,23 @@ __global__ void __launch_bounds__(256) project_backward_kernel(ProjectArgs a, co
 
     }
 
+    // Accumulated (+=), or written (=) when the caller asked for overwrite: the
+    // old value is not read then (it may be any bits, NaN included).
+    const auto put = [&](float* p, float g) { *p = out.overwrite ? g : *p + g; };
     if (out.means) {
         for (int c = 0; c < 3; ++c) {
-            out.means[i * 3 + c] += (float)g_p[c];
+            put(&out.means[i * 3 + c], (float)g_p[c]);
         }
     }
     if (out.log_scales) {
         for (int c = 0; c < 3; ++c) {
             const double g_s = g_M[0][c] * R[0][c] + g_M[1][c] * R[1][c] + g_M[2][c] * R[2][c];
-            out.log_scales[i * 3 + c] += (float)(g_s * s[c]);
+            put(&out.log_scales[i * 3 + c], (float)(g_s * s[c]));
+        }
+    }
+    if (out.quats && !(qn > 0.0) && out.overwrite) {
+        for (int c = 0; c < 4; ++c) {
+            out.quats[i * 4 + c] = 0.0f;
         }
     }
     if (out.quats && qn > 0.0) {
@@ -286,13 +305,13 @@ __global__ void __launch_bounds__(256) project_backward_kernel(ProjectArgs a, co
                           4.0 * qz * (gR[0][0] + gR[1][1]);
         const double gw = 2.0 * (qz * (gR[1][0] - gR[0][1]) + qy * (gR[0][2] - gR[2][0]) + qx * (gR[2][1] - gR[1][2]));
         const double d = qx * gx + qy * gy + qz * gz + qw * gw;
-        out.quats[i * 4 + 0] += (float)((gx - qx * d) * qinv);
-        out.quats[i * 4 + 1] += (float)((gy - qy * d) * qinv);
-        out.quats[i * 4 + 2] += (float)((gz - qz * d) * qinv);
-        out.quats[i * 4 + 3] += (float)((gw - qw * d) * qinv);
+        put(&out.quats[i * 4 + 0], (float)((gx - qx * d) * qinv));
+        put(&out.quats[i * 4 + 1], (float)((gy - qy * d) * qinv));
+        put(&out.quats[i * 4 + 2], (float)((gz - qz * d) * qinv));
+        put(&out.quats[i * 4 + 3], (float)((gw - qw * d) * qinv));
     }
     if (out.opacity_logits) {
-        out.opacity_logits[i] += (float)(g_opacity * opacity * (1.0 - opacity));
+        put(&out.opacity_logits[i], (float)(g_opacity * opacity * (1.0 - opacity)));
     }
 }
 
@@ -314,14 +333,21 @@ __global__ void __launch_bounds__(256) colour_backward_kernel(ProjectArgs a, con
     }
     bool any = false;
     for (uint32_t v = 0; v < nviews; ++v) {
-        any = any || touched[(size_t)v * a.n + i] != 0;
-    }
-    if (!any) {
-        return;
+        any = any || touched[(size_t)v * a.slot_stride + a.slot_base + i] != 0;
     }
     const uint32_t coeffs = a.sh_coeffs;
     const uint32_t nk = (a.sh_degree + 1) * (a.sh_degree + 1);
     const size_t row = (size_t)i * coeffs * 3;
+    if (!any) {
+        if (out.overwrite && out.sh) {
+            // Not drawn in any view: its coefficients' gradients are zero
+            // (its mean's were written by the geometry's adjoint).
+            for (uint32_t k = 0; k < coeffs * 3; ++k) {
+                out.sh[row + k] = 0.0f;
+            }
+        }
+        return;
+    }
     // Coefficient k of channel ch at c[k * 3 + ch]; zero past the row.
     float c[48];
     const bool whole = coeffs == 16 &&
@@ -364,7 +390,7 @@ __global__ void __launch_bounds__(256) colour_backward_kernel(ProjectArgs a, con
     const float px = a.means[i * 3 + 0], py = a.means[i * 3 + 1], pz = a.means[i * 3 + 2];
     float g_p[3] = {0.0f, 0.0f, 0.0f};
     for (uint32_t v = 0; v < nviews; ++v) {
-        const size_t slot = (size_t)v * a.n + i;
+        const size_t slot = (size_t)v * a.slot_stride + a.slot_base + i;
         if (touched[slot] == 0) {
             continue;
         }
@@ -419,7 +445,7 @@ __global__ void __launch_bounds__(256) colour_backward_kernel(ProjectArgs a, con
             float4* p = reinterpret_cast<float4*>(out.sh + row);
 #pragma unroll
             for (uint32_t q = 0; q < 12; ++q) {
-                float4 w = p[q];
+                float4 w = out.overwrite ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : p[q];
                 w.x += g_sh[q * 4 + 0];
                 w.y += g_sh[q * 4 + 1];
                 w.z += g_sh[q * 4 + 2];
@@ -430,7 +456,7 @@ __global__ void __launch_bounds__(256) colour_backward_kernel(ProjectArgs a, con
 #pragma unroll
             for (uint32_t k = 0; k < 48; ++k) {
                 if (k < coeffs * 3) {
-                    out.sh[row + k] += g_sh[k];
+                    out.sh[row + k] = (out.overwrite ? 0.0f : out.sh[row + k]) + g_sh[k];
                 }
             }
         }

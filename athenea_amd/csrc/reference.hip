@@ -62,6 +62,7 @@ struct RefArgs {
     const uint32_t* shape;   // [n][4]
     const uint32_t* sh;      // [n][words] or null
     uint32_t n, rest_per_colour, words, sh_limit, antialias, linear;
+    uint32_t base;  // this cloud's first index among the frame's splats (instances back to back)
 };
 
 __global__ void __launch_bounds__(kRefThreads) ref_project_kernel(RefArgs a, const DView* __restrict__ view,
@@ -73,9 +74,10 @@ __global__ void __launch_bounds__(kRefThreads) ref_project_kernel(RefArgs a, con
         return;
     }
     const DView& V = *view;
-    float* rec = records + (size_t)i * kRecordFloats;
-    keys[i] = 0xFFFFFFFFu;  // culled: after every drawn splat
-    values[i] = i;
+    const uint32_t o = a.base + i;  // the splat's index in the frame
+    float* rec = records + (size_t)o * kRecordFloats;
+    keys[o] = 0xFFFFFFFFu;  // culled: after every drawn splat
+    values[o] = o;
     float4* rec4 = reinterpret_cast<float4*>(rec);
     rec4[0] = make_float4(0.0f, 0.0f, -1.0f, 0.0f);  // a culled record: zeros, z = -1
     rec4[1] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -209,7 +211,7 @@ __global__ void __launch_bounds__(kRefThreads) ref_project_kernel(RefArgs a, con
     rec[9] = col[1];
     rec[10] = col[2];
     rec[11] = 0.0f;
-    keys[i] = __builtin_bit_cast(uint32_t, vz);  // a positive float: its bits order as it does
+    keys[o] = __builtin_bit_cast(uint32_t, vz);  // a positive float: its bits order as it does
 }
 
 struct RefStage {
@@ -421,11 +423,13 @@ __global__ void float_sum_kernel(const float* __restrict__ values, uint32_t coun
 
 hipError_t launch_ref_project(const float* positions, const uint32_t* shape, const uint32_t* sh, uint32_t n,
                               uint32_t rest_per_colour, uint32_t sh_limit, uint32_t antialias, uint32_t linear,
-                              const DView* view, float* records, uint32_t* keys, uint32_t* values, hipStream_t s) {
+                              const DView* view, float* records, uint32_t* keys, uint32_t* values, hipStream_t s,
+                              uint32_t base) {
     if (n == 0) {
         return hipSuccess;
     }
-    RefArgs a{positions, shape, sh, n, rest_per_colour, (rest_per_colour * 3 + 1) / 2, sh_limit, antialias, linear};
+    RefArgs a{positions, shape, sh, n, rest_per_colour, (rest_per_colour * 3 + 1) / 2, sh_limit, antialias, linear,
+              base};
     hipLaunchKernelGGL(ref_project_kernel, dim3(div_up(n, kRefThreads)), dim3(kRefThreads), 0, s, a, view, records,
                        keys, values);
     return hipGetLastError();

@@ -3,7 +3,8 @@
 // env_project / env_prefilter .slang), into the layout athx_sky reads
 // (shaders/athenea/light/environment.slang:50-110): eight octahedral levels
 // -- level 0 the sky itself, level k the sky convolved with a GGX lobe of
-// roughness (k/7)^2 taking n = v = r -- and the sky's 16 radiance
+// roughness (k/7)^2, alpha max(roughness^2, 1e-3), taking n = v = r -- and
+// the sky's 16 radiance
 // coefficients in real spherical harmonics.
 //
 // The source is an equirect of linear radiance (texel (i, j) looks along
@@ -106,7 +107,9 @@ __global__ void __launch_bounds__(kSkyThreads) sky_texels_kernel(const float* __
         oct_dir(((double)col + 0.5) / side, ((double)row + 0.5) / side, n);
     }
     const double rough = (double)level / (double)(kSkyLevels - 1);
-    const double alpha = fmax(rough * rough * rough * rough, 1e-4), a2 = alpha * alpha;
+    // roughness (k/7)^2 (Environment.cpp:286-287), alpha = max(roughness^2,
+    // 1e-3) (env_prefilter.slang:139).
+    const double alpha = fmax(rough * rough * rough * rough, 1e-3), a2 = alpha * alpha;
     double acc[3] = {0.0, 0.0, 0.0}, wsum = 0.0;
     // Levels past 0 integrate the whole source; every lane of the workgroup
     // stages it, so the loop is the same for all.

@@ -17,6 +17,7 @@ from __future__ import annotations
 import ctypes
 from dataclasses import dataclass, field
 
+import numpy as np
 import torch
 
 from . import _lib
@@ -32,6 +33,9 @@ class RenderSettings:
     depth_threshold: float = 0.5
     time_stages: bool = False
     count_splats: bool = False
+    counters_tag: int = 0  # RenderSettings::countersTag, returned with the frame's counters
+    cryptomatte: bool = False  # RenderSettings::cryptomatte: allocated targets get the matte planes
+    trace: bool = False  # RenderSettings::trace: allocated targets get the trace planes
 
     def to_c(self, linear: bool) -> _lib.Settings:
         s = _lib.Settings()
@@ -44,6 +48,7 @@ class RenderSettings:
         s.depth_threshold = self.depth_threshold
         s.time_stages = 1 if self.time_stages else 0
         s.count_splats = 1 if self.count_splats else 0
+        s.counters_tag = int(self.counters_tag)
         return s
 
 
@@ -78,6 +83,7 @@ class GaussianCloud:
     sh: torch.Tensor  # [n, k, 3], float32 or float16
     linear: bool = True
     relight: Relight | None = None
+    crypto_ids: torch.Tensor | None = None  # [n] int32 Cryptomatte ids (GpuSplats::crypto), or None
 
     @property
     def count(self) -> int:
@@ -108,10 +114,21 @@ class PackedGaussians:
     rest_per_colour: int
     linear: bool = True
     relight: Relight | None = None
+    crypto_ids: torch.Tensor | None = None  # [n] int32 Cryptomatte ids, or None
 
     @property
     def count(self) -> int:
         return int(self.positions.shape[0])
+
+
+@dataclass
+class SplatInstance:
+    """SplatInstance (TileRasterizer.h:85-172): a cloud and where it stands,
+    objectToWorld as a 4x4 (or 3x4) in double; None is the identity. A relit
+    cloud's lighting is taken to the world by the same transform."""
+
+    cloud: object  # GaussianCloud or PackedGaussians
+    object_to_world: object = None
 
 
 @dataclass
@@ -128,14 +145,33 @@ class RenderTargets:
     # (athx_frame_serial): the backward pass and the contributions walk that
     # frame and are refused once a later render has replaced it.
     frame_serial: int = 0
+    # RenderTargets::crypto: three planes of (id bits, coverage, id bits,
+    # coverage) a pixel, descending; RenderTargets::trace: TRACE_WORDS planes
+    # of words a pixel (splat_trace.slang). None: not kept.
+    crypto: torch.Tensor | None = None  # [3, h, w, 4] float32
+    trace: torch.Tensor | None = None  # [TRACE_WORDS, h, w] int32
 
     @staticmethod
-    def allocate(width: int, height: int, device, keep_backward: bool) -> "RenderTargets":
+    def allocate(width: int, height: int, device, keep_backward: bool, crypto: bool = False,
+                 trace: bool = False) -> "RenderTargets":
         colour = torch.empty((height, width, 4), dtype=torch.float32, device=device)
         depth = torch.empty((height, width), dtype=torch.float32, device=device)
         ft = torch.empty((height, width), dtype=torch.float32, device=device) if keep_backward else None
         last = torch.empty((height, width), dtype=torch.int32, device=device) if keep_backward else None
-        return RenderTargets(width, height, colour, depth, ft, last)
+        cr = torch.empty((3, height, width, 4), dtype=torch.float32, device=device) if crypto else None
+        tr = torch.empty((_lib.TRACE_WORDS, height, width), dtype=torch.int32, device=device) if trace else None
+        return RenderTargets(width, height, colour, depth, ft, last, 0, cr, tr)
+
+
+@dataclass
+class UnderLayer:
+    """What is under the splats (the reference's `under` RenderTargets): an
+    opaque layer's linear premultiplied colour, its view depth (0 where it has
+    nothing) and, for the matte, its Cryptomatte ids."""
+
+    colour: torch.Tensor  # [h, w, 4] float32
+    depth: torch.Tensor  # [h, w] float32
+    crypto_ids: torch.Tensor | None = None  # [h, w] int32
 
 
 @dataclass
@@ -152,18 +188,33 @@ class FrameStats:
     blend_ms: float = 0.0
     total_ms: float = 0.0
     frame_serial: int = 0
+    # Where each instance's Gaussians start among a view's slots
+    # (FrameStats::slotBases); [0] for a one-cloud frame.
+    slot_bases: list = field(default_factory=lambda: [0])
 
 
 CULL_NAMES = ["unprojected", "edit", "depth", "degenerate", "faint", "offscreen", "no_tile"]
 
 
 @dataclass
+class CloudCounters:
+    """SplatCounters::clouds[j]: one instance's drawn footprints and pairs."""
+
+    visible: int = 0
+    pairs: int = 0
+
+
+@dataclass
 class SplatCounters:
+    """SplatCounters (TileRasterizer.h:370-401)."""
+
     slots: int = 0
     visible: int = 0
     pairs: int = 0
     max_tiles: int = 0
     culled: dict = field(default_factory=dict)
+    tag: int = 0
+    clouds: list = field(default_factory=list)
 
 
 def _ptr(t: torch.Tensor | None) -> int | None:
@@ -226,7 +277,9 @@ def _check_packed(cloud: PackedGaussians) -> None:
         raise ValueError(f"packed sh must be 32-bit words [n, {words}]")
 
 
-def _gaussians_c(cloud) -> _lib.Gaussians:
+def _gaussians_c(cloud, object_to_world=None) -> _lib.Gaussians:
+    """athx_gaussians of a cloud; ``object_to_world`` (rows of a 3x4) replaces
+    a relit cloud's own."""
     g = _lib.Gaussians()
     g.count = cloud.count
     if isinstance(cloud, PackedGaussians):
@@ -252,6 +305,11 @@ def _gaussians_c(cloud) -> _lib.Gaussians:
         else:
             g.sh = _ptr(cloud.sh)
             g.sh_half = None
+    ids = getattr(cloud, "crypto_ids", None)
+    if ids is not None:
+        if tuple(ids.shape) != (cloud.count,) or ids.dtype != torch.int32 or not ids.is_cuda:
+            raise ValueError("crypto_ids must be an int32 device tensor [count]")
+        g.crypto_ids = _ptr(ids)
     r = cloud.relight
     if r is not None:
         c = _lib.Relight()
@@ -261,8 +319,9 @@ def _gaussians_c(cloud) -> _lib.Gaussians:
         c.roughness = _ptr(r.roughness)
         c.normals_packed = _ptr(r.normals_packed)
         c.pbr = _ptr(r.pbr)
+        rows = r.object_to_world if object_to_world is None else object_to_world
         for k in range(12):
-            c.object_to_world[k] = float(r.object_to_world[k])
+            c.object_to_world[k] = float(rows[k])
         c.sky.texels = _ptr(r.sky.texels)
         c.sky.sh = _ptr(r.sky.sh)
         c.sky.sun = _ptr(r.sky.sun)
@@ -272,19 +331,40 @@ def _gaussians_c(cloud) -> _lib.Gaussians:
     return g
 
 
-def _views_and_targets(views: list, targets: list[RenderTargets]):
+def _views_and_targets(views: list, targets: list[RenderTargets], under: list[UnderLayer] | None = None):
     if len(targets) != len(views):
         raise ValueError("one target per view")
+    if under is not None and len(under) != len(views):
+        raise ValueError("one under layer per view")
     nv = len(views)
     c_views = (_lib.View * nv)(*views)
     c_targets = (_lib.Targets * nv)()
     for k, t in enumerate(targets):
-        if t.colour.shape != (views[k].height, views[k].width, 4):
+        h, w = views[k].height, views[k].width
+        if t.colour.shape != (h, w, 4):
             raise ValueError(f"target {k} does not match its view's size")
         c_targets[k].colour = _ptr(t.colour)
         c_targets[k].depth = _ptr(t.depth)
         c_targets[k].final_t = _ptr(t.final_t)
         c_targets[k].last = _ptr(t.last)
+        if t.crypto is not None:
+            if tuple(t.crypto.shape) != (3, h, w, 4) or t.crypto.dtype != torch.float32:
+                raise ValueError(f"target {k}: crypto must be float32 [3, {h}, {w}, 4]")
+            c_targets[k].crypto = _ptr(t.crypto)
+        if t.trace is not None:
+            if tuple(t.trace.shape) != (_lib.TRACE_WORDS, h, w) or t.trace.dtype != torch.int32:
+                raise ValueError(f"target {k}: trace must be int32 [{_lib.TRACE_WORDS}, {h}, {w}]")
+            c_targets[k].trace = _ptr(t.trace)
+        if under is not None:
+            u = under[k]
+            if tuple(u.colour.shape) != (h, w, 4) or u.colour.dtype != torch.float32 or \
+                    tuple(u.depth.shape) != (h, w) or u.depth.dtype != torch.float32:
+                raise ValueError(f"under layer {k}: colour float32 [{h}, {w}, 4] and depth float32 [{h}, {w}]")
+            if u.crypto_ids is not None and (tuple(u.crypto_ids.shape) != (h, w) or u.crypto_ids.dtype != torch.int32):
+                raise ValueError(f"under layer {k}: crypto_ids int32 [{h}, {w}]")
+            c_targets[k].under_colour = _ptr(u.colour)
+            c_targets[k].under_depth = _ptr(u.depth)
+            c_targets[k].under_crypto = _ptr(u.crypto_ids)
     return c_views, c_targets
 
 
@@ -401,25 +481,18 @@ class TileRasterizer:
         settings: RenderSettings | None = None,
         targets: list[RenderTargets] | None = None,
         keep_backward: bool = False,
+        under: list[UnderLayer] | None = None,
     ) -> tuple[list[RenderTargets], FrameStats]:
         """Draws ``cloud`` once per view (a list of athx View structs); all
-        views go through one projection, one sort and one blend launch."""
+        views go through one projection, one sort and one blend launch.
+        ``under``: an opaque layer under the splats, one a view."""
         settings = settings or RenderSettings()
         dev = _device_of(cloud)
         if targets is None:
-            targets = [RenderTargets.allocate(v.width, v.height, dev, keep_backward) for v in views]
-        if len(targets) != len(views):
-            raise ValueError("one target per view")
+            targets = [RenderTargets.allocate(v.width, v.height, dev, keep_backward, settings.cryptomatte,
+                                              settings.trace) for v in views]
         nv = len(views)
-        c_views = (_lib.View * nv)(*views)
-        c_targets = (_lib.Targets * nv)()
-        for k, t in enumerate(targets):
-            if t.colour.shape != (views[k].height, views[k].width, 4):
-                raise ValueError(f"target {k} does not match its view's size")
-            c_targets[k].colour = _ptr(t.colour)
-            c_targets[k].depth = _ptr(t.depth)
-            c_targets[k].final_t = _ptr(t.final_t)
-            c_targets[k].last = _ptr(t.last)
+        c_views, c_targets = _views_and_targets(views, targets, under)
         g = _gaussians_c(cloud)
         s = settings.to_c(cloud.linear)
         stats = _lib.FrameStats()
@@ -430,6 +503,68 @@ class TileRasterizer:
         out = FrameStats(stats.splats, stats.visible, stats.pairs, stats.project_ms, stats.depth_sort_ms,
                          stats.counts_ms, stats.emit_ms, stats.tile_sort_ms, stats.ranges_ms, stats.blend_ms,
                          stats.total_ms, int(stats.frame_serial))
+        for t in targets:
+            t.frame_serial = int(stats.frame_serial)
+        return targets, out
+
+    def render_instances(
+        self,
+        views: list,
+        instances: list[SplatInstance],
+        settings: RenderSettings | None = None,
+        targets: list[RenderTargets] | None = None,
+        keep_backward: bool = False,
+        under: list[UnderLayer] | None = None,
+    ) -> tuple[list[RenderTargets], FrameStats]:
+        """TileRasterizer::render over a span of SplatInstances
+        (athx_render_instances): each cloud projected with its own
+        object-to-world, all of them sorted and blended together. ``views``
+        come from camera.make_view / view_for (their own object-to-world is
+        replaced by each instance's). A view's slots are the instances'
+        Gaussians back to back; ``stats.slot_bases`` gives where each starts."""
+        from . import camera as _camera
+
+        settings = settings or RenderSettings()
+        if not instances:
+            raise ValueError("no instances")
+        dev = _device_of(instances[0].cloud)
+        if any(_device_of(i.cloud) != dev for i in instances):
+            raise ValueError("every instance's cloud must be on one device")
+        linear = {bool(i.cloud.linear) for i in instances}
+        if len(linear) != 1:
+            raise ValueError("the instances' clouds must all be linear or all sRGB")
+        if targets is None:
+            targets = [RenderTargets.allocate(v.width, v.height, dev, keep_backward, settings.cryptomatte,
+                                              settings.trace) for v in views]
+        c_views, c_targets = _views_and_targets(views, targets, under)
+        nv, ni = len(views), len(instances)
+        keep = []
+        c_inst = (_lib.Instance * ni)()
+        for j, inst in enumerate(instances):
+            rows, eyes = [], []
+            for v in views:
+                r, e = _camera.instance_transform(v, inst.object_to_world)
+                rows.append(r)
+                eyes.append(e)
+            rows = np.ascontiguousarray(np.stack(rows), np.float32)
+            eyes = np.ascontiguousarray(np.stack(eyes), np.float32)
+            o2w = None
+            if inst.object_to_world is not None:
+                o2w = np.asarray(inst.object_to_world, np.float64)[:3, :4].reshape(-1)
+            g = _gaussians_c(inst.cloud, o2w)
+            keep += [rows, eyes, g]
+            c_inst[j].gaussians = ctypes.pointer(g)
+            c_inst[j].object_to_view = rows.ctypes.data
+            c_inst[j].eye = eyes.ctypes.data
+        s = settings.to_c(linear.pop())
+        stats = _lib.FrameStats()
+        bases = (ctypes.c_uint32 * ni)()
+        self._check(self._lib.athx_render_instances(self._ctx, c_views, nv, c_inst, ni, ctypes.byref(s), c_targets,
+                                                    _stream_handle(dev), ctypes.byref(stats), bases))
+        del keep
+        out = FrameStats(stats.splats, stats.visible, stats.pairs, stats.project_ms, stats.depth_sort_ms,
+                         stats.counts_ms, stats.emit_ms, stats.tile_sort_ms, stats.ranges_ms, stats.blend_ms,
+                         stats.total_ms, int(stats.frame_serial), [int(b) for b in bases])
         for t in targets:
             t.frame_serial = int(stats.frame_serial)
         return targets, out
@@ -473,7 +608,8 @@ class TileRasterizer:
     def contribution(self, views: list, cloud, settings: RenderSettings | None, targets: list[RenderTargets],
                      mode: int = 1, dl_dcolour: list[torch.Tensor] | None = None) -> torch.Tensor:
         """athx_contribution for the frame the last ``render`` drew (slot =
-        view * count + gaussian): mode 1 an int32 [slots] of blend weights in
+        view * count + gaussian; ``cloud`` may be the list of SplatInstances a
+        ``render_instances`` frame drew, whose count is theirs): mode 1 an int32 [slots] of blend weights in
         1/4096; mode 2 an int32 [slots, 2], the (low, high) words of the
         squared removal error in 2^-24 against the drawn image given as
         ``dl_dcolour``; mode 3 an int32 [slots, 2], the weight times the mask
@@ -481,10 +617,19 @@ class TileRasterizer:
         mode 4 an int32 [slots, 4] of the colour gradient against
         ``dl_dcolour`` and the weight, in 1/256."""
         settings = settings or RenderSettings()
-        dev = _device_of(cloud)
+        instances = cloud if isinstance(cloud, (list, tuple)) else None
+        if instances is not None:
+            # A multi-instance frame: its slots are every instance's.
+            dev = _device_of(instances[0].cloud)
+            count = sum(i.cloud.count for i in instances)
+            linear = bool(instances[0].cloud.linear)
+        else:
+            dev = _device_of(cloud)
+            count = cloud.count
+            linear = bool(cloud.linear)
         c_views, c_targets = _views_and_targets(views, targets)
         nv = len(views)
-        slots = cloud.count * nv
+        slots = count * nv
         words = {1: 1, 2: 2, 3: 2, 4: 4}.get(int(mode))
         if words is None:
             raise ValueError("contribution modes are 1 to 4")
@@ -507,9 +652,9 @@ class TileRasterizer:
                 keep.append(t)
                 dl[k] = t.data_ptr()
         out = torch.empty((slots, words) if words > 1 else (slots,), dtype=torch.int32, device=dev)
-        g = _gaussians_c(cloud)
-        s = settings.to_c(cloud.linear)
-        self._check(self._lib.athx_contribution(self._ctx, c_views, nv, ctypes.byref(g), ctypes.byref(s), c_targets,
+        g = ctypes.byref(_gaussians_c(cloud)) if instances is None else None
+        s = settings.to_c(linear)
+        self._check(self._lib.athx_contribution(self._ctx, c_views, nv, g, ctypes.byref(s), c_targets,
                                                 _serial_of(targets), int(mode), dl, out.data_ptr(),
                                                 _stream_handle(dev)))
         return out
@@ -522,20 +667,18 @@ class TileRasterizer:
         targets: list[RenderTargets],
         dl_dcolour: list[torch.Tensor],
         grads: dict[str, torch.Tensor],
+        overwrite: bool = False,
     ) -> None:
         """Accumulates dL/d(parameter) into ``grads`` (keys: means, log_scales,
-        quats, opacity_logits, sh) for the frame the last ``render`` drew."""
+        quats, opacity_logits, sh) for the frame the last ``render`` drew; with
+        ``overwrite`` writes them instead (every element, so ``grads`` may be
+        uninitialised)."""
         settings = settings or RenderSettings()
         nv = len(views)
-        c_views = (_lib.View * nv)(*views)
-        c_targets = (_lib.Targets * nv)()
+        c_views, c_targets = _views_and_targets(views, targets)
         keep = []
         dl = (ctypes.c_void_p * nv)()
         for k, t in enumerate(targets):
-            c_targets[k].colour = _ptr(t.colour)
-            c_targets[k].depth = _ptr(t.depth)
-            c_targets[k].final_t = _ptr(t.final_t)
-            c_targets[k].last = _ptr(t.last)
             d = dl_dcolour[k].to(torch.float32).contiguous()
             if d.shape != t.colour.shape:
                 raise ValueError("dL/dcolour must match the colour target")
@@ -547,6 +690,7 @@ class TileRasterizer:
             if t is not None and (t.dtype != torch.float32 or not t.is_contiguous()):
                 raise ValueError(f"gradient {name} must be a contiguous float32 tensor")
             setattr(cg, name, _ptr(t))
+        cg.overwrite = 1 if overwrite else 0
         g = _gaussians_c(cloud)
         s = settings.to_c(cloud.linear)
         self._check(
@@ -558,11 +702,17 @@ class TileRasterizer:
         # allocator is stream-ordered, so releasing them here is safe.
         del keep
 
-    def latest_counters(self) -> SplatCounters:
+    def latest_counters(self) -> SplatCounters | None:
+        """TileRasterizer::latestCounters: the newest counted frame's counters
+        that have reached the host, without waiting; None before any has."""
         c = _lib.Counters()
-        self._check(self._lib.athx_latest_counters(self._ctx, ctypes.byref(c)))
+        status = self._lib.athx_latest_counters(self._ctx, ctypes.byref(c))
+        if status == 2:  # NotFound: nothing counted has finished
+            return None
+        self._check(status)
         return SplatCounters(c.slots, c.visible, c.pairs, c.max_tiles,
-                             {CULL_NAMES[r]: int(c.culled[r]) for r in range(_lib.CULL_REASONS)})
+                             {CULL_NAMES[r]: int(c.culled[r]) for r in range(_lib.CULL_REASONS)}, int(c.tag),
+                             [CloudCounters(int(c.cloud_visible[j]), int(c.cloud_pairs[j])) for j in range(c.clouds)])
 
     def frame_info(self) -> dict:
         info = _lib.FrameInfo()

@@ -49,6 +49,40 @@ def reference_render(raster: TileRasterizer, view, cloud: PackedGaussians, setti
     return targets, (records[:cloud.count] if records is not None else None), int(over.value)
 
 
+def reference_render_instances(raster: TileRasterizer, view, instances: list, settings: RenderSettings | None = None,
+                               want_records: bool = False) -> tuple[RenderTargets, torch.Tensor | None, int]:
+    """The ground truth of several packed clouds, each with its own
+    object-to-world (raster.SplatInstance), under one view made by
+    camera.view_for: every drawn splat of all of them at every pixel in
+    (depth, index) order, indices the instances' back to back."""
+    from . import camera as _camera
+
+    settings = settings or RenderSettings()
+    if not instances or any(not isinstance(i.cloud, PackedGaussians) for i in instances):
+        raise ValueError("the ground truth reads packed clouds")
+    dev = instances[0].cloud.positions.device
+    targets = RenderTargets.allocate(view.width, view.height, dev, False)
+    c_views, c_targets = _views_and_targets([view], [targets])
+    total = sum(i.cloud.count for i in instances)
+    keep = []
+    c_inst = (_lib.Instance * len(instances))()
+    for j, inst in enumerate(instances):
+        rows, eye = _camera.instance_transform(view, inst.object_to_world)
+        g = _gaussians_c(inst.cloud)
+        keep += [rows, eye, g]
+        c_inst[j].gaussians = ctypes.pointer(g)
+        c_inst[j].object_to_view = rows.ctypes.data
+        c_inst[j].eye = eye.ctypes.data
+    s = settings.to_c(instances[0].cloud.linear)
+    records = torch.empty((max(total, 1), 12), dtype=torch.float32, device=dev) if want_records else None
+    over = ctypes.c_uint32(0)
+    raster._check(raster._lib.athx_reference_render_instances(raster._ctx, c_views, c_inst, len(instances),
+                                                              ctypes.byref(s), c_targets, _ptr(records),
+                                                              ctypes.byref(over)))
+    del keep
+    return targets, (records[:total] if records is not None else None), int(over.value)
+
+
 def _images(a: torch.Tensor, b: torch.Tensor) -> tuple[int, int]:
     if a.shape != b.shape or a.dim() != 3 or a.shape[2] != 4 or a.dtype != torch.float32 or b.dtype != torch.float32:
         raise ValueError("two float32 [height, width, 4] images of one size")

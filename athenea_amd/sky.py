@@ -120,7 +120,7 @@ def _ggx_prefilter(dirs: np.ndarray, src_d: np.ndarray, src_l: np.ndarray, src_w
     """The sky convolved with a GGX lobe of this roughness about each
     direction, taking n = v = r (the usual split-sum assumption):
     sum L(l) D(n.h) (n.l) dw / sum D(n.h) (n.l) dw over n.l > 0."""
-    alpha = max(roughness * roughness, 1e-4)
+    alpha = max(roughness * roughness, 1e-3)  # env_prefilter.slang:139
     a2 = alpha * alpha
     out = np.zeros((dirs.shape[0], 3))
     for s in range(0, dirs.shape[0], 2048):

@@ -50,11 +50,12 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK = 8.0e12
-# MI355X: 256 CUs at 2.4 GHz; a CU's four SIMDs issue one wave64 VALU
-# instruction a clock between them (each takes four clocks for 64 lanes).
-CUS = 256
-CLOCK_HZ = 2.4e9
-VALU_PEAK = CUS * CLOCK_HZ  # wave instructions a second
+# MI355X's published FP32 vector peak, 157.3 TFLOP/s, in wave64 instructions
+# a second: an FMA instruction does 64 lanes x 2 FLOPs. The blend's rate is
+# every VALU instruction it issues (SQ_INSTS_VALU, whole device, a dispatch)
+# over its event-timed duration, against that.
+FP32_PEAK_FLOPS = 157.3e12
+VALU_PEAK = FP32_PEAK_FLOPS / 128.0  # wave instructions a second
 METRIC = "ms/frame fwd & fwd+bwd, 1.4M splats@1080p; batched views/s; % HBM peak"
 STAGES = ("project", "depth_sort", "counts", "emit", "tile_sort", "ranges", "blend")
 
@@ -349,10 +350,11 @@ def main() -> int:
         "evaluations": walked,
         "evaluations_per_s": round(walked / blend_s),
         "valu_per_evaluation": round(valu * 64 / walked, 2) if valu else None,
+        "peak_basis": "157.3 TFLOP/s FP32 / 128 FLOP a wave64 FMA instruction",
         "counters": "profiles/counters_1080p_relit.json (rocprofv3 --pmc, a launch of this frame)",
     }
     proj_s = stages["project"] * 1e-3
-    proj_prof = prof.get("project_kernel", {})
+    proj_prof = prof.get("project_kernel relit", {})
     roofline_hbm = {
         "kernel": "project_kernel",
         "bound": "hbm",
@@ -402,7 +404,7 @@ def main() -> int:
     if args.copy_probe:
         copy_probe()
 
-    if not args.no_extras:
+    def fwd_bwd_leg():
         from athenea_amd.autograd import rasterize
 
         log("forward + backward")
@@ -433,7 +435,7 @@ def main() -> int:
         result["fwd_bwd_over_fwd"] = round(result["fwd_bwd_ms"] / fwd_sh_ms, 3)
         del params
 
-    if not args.no_extras and not args.no_batch:
+    def batch_leg():
         from athenea_amd.raster import FrameGraph
 
         log("batched 60-view walk")
@@ -448,6 +450,7 @@ def main() -> int:
 
             log("autotuning the walk's tile side and blend batch")
             tuned, table = autotune(raster, views, walk, settings, reps=2)
+            raster.release_scratch()
         batch_ms, bstats, _ = timed_frames(raster, views, walk, tuned, reps, 1)
         # The same batch as one captured graph: no host wait, one launch.
         graph = FrameGraph(views, walk, tuned)
@@ -492,19 +495,35 @@ def main() -> int:
         torch.cuda.empty_cache()
         raster.release_scratch()
 
-    if not args.no_extras and not args.no_sort:
+    def sort_leg_run():
         log("sort: 10M pairs, 32-bit keys")
         result["sort"] = sort_leg(raster)
         raster.release_scratch()
 
-    if not args.no_extras and not args.no_mixed:
+    def mixed_leg_run():
         result["mixed"] = mixed_leg(raster, args.steps)
 
-    if not args.no_extras and not args.no_cpu:
+    def cpu_leg():
         log("CPU baseline (the oracle on the host cores) and the parity check")
         base, frame = cpu_baseline(cloud_np, view, args.tile, attrs)
         result["cpu_baseline"] = base
         result["parity"] = compare_with_oracle(gpu_frame, frame)
+
+    legs = [("fwd_bwd_ms", fwd_bwd_leg, False), ("batched", batch_leg, args.no_batch),
+            ("sort", sort_leg_run, args.no_sort), ("mixed", mixed_leg_run, args.no_mixed),
+            ("cpu_baseline", cpu_leg, args.no_cpu)]
+    for name, fn, skip in legs:
+        if args.no_extras or skip:
+            continue
+        try:
+            fn()
+        except Exception as e:  # a leg that fails is reported in the line; the headline stands
+            import traceback
+
+            traceback.print_exc()
+            result.setdefault("leg_errors", {})[name] = f"{type(e).__name__}: {e}"
+            torch.cuda.synchronize()
+            raster.release_scratch()
 
     print(json.dumps(result), flush=True)
     return 0

@@ -212,6 +212,8 @@ typedef struct athx_gaussians {
     const athx_relight* relight; /* NULL: the SH colour; else the relit colour (forward only) */
     const athx_packed_splats* packed; /* non-NULL: the packed layout is read instead of the arrays
                                          above, which may then be NULL (forward only) */
+    const uint32_t* crypto_ids;  /* [count] each Gaussian's Cryptomatte id (GpuSplats::crypto), or NULL:
+                                    a cloud without ancestry, whose splats name nothing (id 0) */
 } athx_gaussians;
 
 /*
@@ -228,6 +230,7 @@ typedef struct athx_settings {
     float depth_threshold;
     uint32_t time_stages;  /* 1: wait after each stage and fill athx_frame_stats' times */
     uint32_t count_splats; /* 1: count cull reasons for athx_latest_counters */
+    uint64_t counters_tag; /* RenderSettings::countersTag: returned with this frame's counters */
 } athx_settings;
 
 /*
@@ -238,11 +241,37 @@ typedef struct athx_settings {
  * are what athx_render_backward replays; either may be NULL when no backward
  * pass follows.
  */
+/*
+ * The walk's variants (splat_blend.slang:236-578), each on when its pointer
+ * is given, for every view of a frame alike:
+ *   under_colour / under_depth   an opaque layer under the splats (the
+ *       reference's `under` RenderTargets; linear premultiplied float4 and
+ *       view z, 0 where it has nothing): a splat at or behind it ends the
+ *       pixel's walk, and the layer takes the light that is left
+ *       (TileRasterizer.h:465-477, splat_blend.slang:287-300, 512-566)
+ *   crypto   the Cryptomatte (RenderTargets::crypto, TileRasterizer.h:427-
+ *       431): the six heaviest ids of a pixel (the Gaussians' crypto_ids, the
+ *       layer's under_crypto) as three planes of float4 (id bits, coverage,
+ *       id bits, coverage), descending
+ *   trace    RenderTargets::trace (splat_trace.slang): ATHX_TRACE_WORDS planes
+ *       of words a pixel -- the four heaviest slots, their view depths, their
+ *       weights as halves, (rest, under) as halves; a pixel nothing reached
+ *       holds ATHX_TRACE_EMPTY in its first plane alone
+ */
+#define ATHX_TRACE_SLOTS 4
+#define ATHX_TRACE_WORDS (ATHX_TRACE_SLOTS * 2 + ATHX_TRACE_SLOTS / 2 + 1)
+#define ATHX_TRACE_NONE 0xFFFFFFFFu
+#define ATHX_TRACE_EMPTY 0xFFFFFFFEu
 typedef struct athx_targets {
     float* colour;    /* [height][width][4] */
     float* depth;     /* [height][width] or NULL */
     float* final_t;   /* [height][width] or NULL */
     uint32_t* last;   /* [height][width] or NULL */
+    float* crypto;    /* [3][height][width][4] or NULL */
+    uint32_t* trace;  /* [ATHX_TRACE_WORDS][height][width] or NULL */
+    const float* under_colour;     /* [height][width][4] or NULL */
+    const float* under_depth;      /* [height][width], with under_colour */
+    const uint32_t* under_crypto;  /* [height][width] or NULL: the layer's ids, for the matte */
 } athx_targets;
 
 /* FrameStats (TileRasterizer.h:408-419), summed over the views. The stage
@@ -260,20 +289,31 @@ typedef struct athx_frame_stats {
     uint64_t frame_serial; /* this frame's number on the context (athx_frame_serial) */
 } athx_frame_stats;
 
-/* SplatCounters (TileRasterizer.h:370-401) for the whole frame. */
+/* SplatCounters (TileRasterizer.h:370-401) for the whole frame, summed over
+ * its views: slots, the drawn (visible) ones, their (tile, Gaussian) pairs,
+ * the most tiles one footprint touched, the culled ones by reason, and for
+ * each of the first ATHX_MAX_COUNTED_CLOUDS instances (in the order given)
+ * its drawn footprints and their pairs (SplatCounters::clouds). */
+#define ATHX_MAX_COUNTED_CLOUDS 64
 typedef struct athx_counters {
+    uint64_t tag;      /* the frame's athx_settings.counters_tag */
     uint32_t slots;
     uint32_t visible;
     uint64_t pairs;
     uint32_t max_tiles;
     uint32_t culled[ATHX_CULL_REASONS];
+    uint32_t clouds;   /* rows below: min(instances, ATHX_MAX_COUNTED_CLOUDS) */
+    uint32_t cloud_visible[ATHX_MAX_COUNTED_CLOUDS];
+    uint32_t cloud_pairs[ATHX_MAX_COUNTED_CLOUDS];
 } athx_counters;
 
 /*
  * Gradients of a scalar loss with respect to every Gaussian parameter, in the
  * layouts of athx_gaussians (caller-owned device memory, FP32). They are
- * ACCUMULATED into (+=), so the caller zeroes them first. sh is
- * [count][sh_coeffs][3]. Any pointer may be NULL to skip that parameter.
+ * ACCUMULATED into (+=), so the caller zeroes them first -- or, with
+ * overwrite = 1, WRITTEN (=), every element, so the caller need not clear
+ * them. sh is [count][sh_coeffs][3]. Any pointer may be NULL to skip that
+ * parameter.
  */
 typedef struct athx_grads {
     float* means;
@@ -281,6 +321,7 @@ typedef struct athx_grads {
     float* quats;
     float* opacity_logits;
     float* sh;
+    uint32_t overwrite;
 } athx_grads;
 
 ATHX_API const char* athx_version(void);
@@ -305,6 +346,39 @@ ATHX_API uint64_t athx_scratch_bytes(const athx_ctx* ctx);
 ATHX_API athx_status athx_render(athx_ctx* ctx, const athx_view* views, uint32_t nviews,
                                  const athx_gaussians* gaussians, const athx_settings* settings,
                                  const athx_targets* targets, void* stream, athx_frame_stats* stats);
+
+/*
+ * One cloud of a multi-instance frame (render::SplatInstance, TileRasterizer.h:
+ * 85-172: a cloud and its objectToWorld). object_to_view and eye are host
+ * memory, one entry a view: Projection::worldToView * objectToWorld, and the
+ * view's eye taken into the cloud's own space by the inverse of objectToWorld,
+ * computed in double and cast to float, as TileRasterizer.cpp:598-600 does for
+ * each instance's dispatch. NULL takes the view's own object_to_view / eye
+ * (an instance standing where the view's matrix already puts it). A relit
+ * instance's relight->object_to_world is its objectToWorld.
+ */
+typedef struct athx_instance {
+    const athx_gaussians* gaussians;
+    const float* object_to_view; /* [nviews][12] rows of a 3x4, or NULL */
+    const float* eye;            /* [nviews][3], or NULL */
+} athx_instance;
+
+/*
+ * TileRasterizer::render over a span of SplatInstances (TileRasterizer.h:
+ * 465-477): every instance's Gaussians are projected with its own transform
+ * and the frame sorts and blends them together. Each view's slots are the
+ * instances' Gaussians back to back in the order given: Gaussian i of
+ * instance j in view k is slot k * S + slot_bases[j] + i, S the instances'
+ * total, slot_bases[j] the counts of the instances before j
+ * (FrameStats::slotBases, TileRasterizer.h:408-419). slot_bases (host, may be
+ * NULL) receives them. Waits as athx_render does; stats->splats is S. The
+ * frame's records serve athx_contribution (gaussians NULL) and the frame
+ * readers; athx_render_backward needs a one-instance frame.
+ */
+ATHX_API athx_status athx_render_instances(athx_ctx* ctx, const athx_view* views, uint32_t nviews,
+                                           const athx_instance* instances, uint32_t ninstances,
+                                           const athx_settings* settings, const athx_targets* targets, void* stream,
+                                           athx_frame_stats* stats, uint32_t* slot_bases);
 
 /*
  * athx_render without its wait: the pair buffers are sized for
@@ -393,14 +467,21 @@ ATHX_API athx_status athx_render_backward(athx_ctx* ctx, const athx_view* views,
  *           with m the pixel's mask clamped to [0, 1], then the sum of
  *           round(T alpha * 1024); dl_dcolour[k] the mask, [height][width].
  * The views, gaussians, settings and targets are the last render's, frame
- * frame_serial (athx_frame_serial). Queued.
+ * frame_serial (athx_frame_serial); gaussians may be NULL (a multi-instance
+ * frame's slots are all its instances'), else its count must be the frame's
+ * Gaussians a view. Queued.
  */
 ATHX_API athx_status athx_contribution(athx_ctx* ctx, const athx_view* views, uint32_t nviews,
                                        const athx_gaussians* gaussians, const athx_settings* settings,
                                        const athx_targets* targets, uint64_t frame_serial, uint32_t mode,
                                        const float* const* dl_dcolour, uint32_t* out, void* stream);
 
-/* The counters of the newest frame rendered with count_splats (waits for it). */
+/* TileRasterizer::latestCounters (TileRasterizer.h:479-481): the counters of
+ * the newest frame rendered with count_splats whose counts have reached the
+ * host -- a ring of three readbacks, each copied on the frame's stream after
+ * its counts; never waits. NOT_FOUND when no counted frame has finished
+ * (athx_render waits for its frame, so after it returns its counts are the
+ * newest). A frame without count_splats leaves the last counts as they were. */
 ATHX_API athx_status athx_latest_counters(athx_ctx* ctx, athx_counters* out);
 
 /*
@@ -560,6 +641,15 @@ ATHX_API athx_status athx_ingest_ply(athx_ctx* ctx, const void* vertices, const 
 ATHX_API athx_status athx_reference_render(athx_ctx* ctx, const athx_view* view, const athx_gaussians* gaussians,
                                            const athx_settings* settings, const athx_targets* targets,
                                            float* records, uint32_t* overflow);
+/* The ground truth of several instances (ReferenceRenderer::render over a span
+ * of SplatInstances, ReferenceRenderer.h:20-37): each packed cloud projected
+ * with its own transform (athx_instance, one view), indices back to back in
+ * the order given, every drawn splat of all of them walked per pixel in
+ * (depth, index) order. records as above, for all instances in turn. */
+ATHX_API athx_status athx_reference_render_instances(athx_ctx* ctx, const athx_view* view,
+                                                     const athx_instance* instances, uint32_t ninstances,
+                                                     const athx_settings* settings, const athx_targets* targets,
+                                                     float* records, uint32_t* overflow);
 
 /*
  * A sky prepared on the device (technique::Environment, modules/technique/

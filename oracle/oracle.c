@@ -1016,6 +1016,214 @@ ORC_EXPORT void orc_blend(const orc_view* v, const float* rec, const uint32_t* r
 }
 
 /* ---------------------------------------------------------------------------
+ * The walk's variants (splat_blend.slang:236-578 with composite, matte and
+ * traced): an opaque layer under the splats that ends the walk where a splat
+ * stands at or behind it and takes what light is left (:287-300, 330-333,
+ * 512-566); the Cryptomatte list of the six heaviest ids (:88-143, 421-423);
+ * the trace of the four heaviest slots with what the list left out and what
+ * closed the walk (splat_trace.slang). Plain splats only: no discs, catchers,
+ * media or slopes.
+ */
+#define ORC_CRYPTO_SLOTS 6
+#define ORC_TRACE_SLOTS 4
+#define ORC_TRACE_WORDS (ORC_TRACE_SLOTS * 2 + ORC_TRACE_SLOTS / 2 + 1)
+#define ORC_TRACE_NONE 0xFFFFFFFFu
+#define ORC_TRACE_EMPTY 0xFFFFFFFEu
+
+typedef struct {
+    uint32_t ids[ORC_CRYPTO_SLOTS];
+    float weights[ORC_CRYPTO_SLOTS];
+} crypto_list;
+
+/* splat_blend.slang:93-116 */
+static void crypto_add(crypto_list* l, uint32_t id, float weight) {
+    if (weight <= 0.0f || id == 0) return;
+    uint32_t lightest = 0;
+    for (uint32_t k = 0; k < ORC_CRYPTO_SLOTS; ++k) {
+        if (l->ids[k] == id && l->weights[k] > 0.0f) {
+            l->weights[k] = l->weights[k] + weight;
+            return;
+        }
+        if (l->weights[k] < l->weights[lightest]) lightest = k;
+    }
+    if (weight > l->weights[lightest]) {
+        l->ids[lightest] = id;
+        l->weights[lightest] = weight;
+    }
+}
+
+/* Stable insertion order by descending weight (splat_blend.slang:119-131,
+ * splat_trace.slang:102-113). */
+static void order_desc(const float* w, uint32_t n, uint32_t* order) {
+    for (uint32_t k = 0; k < n; ++k) order[k] = k;
+    for (uint32_t i = 1; i < n; ++i) {
+        const uint32_t pick = order[i];
+        uint32_t j = i;
+        while (j > 0 && w[order[j - 1]] < w[pick]) {
+            order[j] = order[j - 1];
+            --j;
+        }
+        order[j] = pick;
+    }
+}
+
+static float bits_float(uint32_t u) {
+    float f;
+    memcpy(&f, &u, sizeof f);
+    return f;
+}
+
+static uint32_t float_bits(float f) {
+    uint32_t u;
+    memcpy(&u, &f, sizeof u);
+    return u;
+}
+
+static uint32_t ingest_half(float v);
+
+static void blend_pixel_ex(const orc_view* v, const float* rec, const uint32_t* vals, uint32_t start, uint32_t end,
+                           uint32_t px, uint32_t py, const float* under_colour, const float* under_depth,
+                           const uint32_t* slot_ids, const uint32_t* under_crypto, int matte, int traced,
+                           float out[4], float* depth, float* final_t, uint32_t* last, float* crypto,
+                           uint32_t* trace) {
+    const uint32_t W = v->width, H = v->height, pixels = W * H, at = py * W + px;
+    const float pxc = (float)px + 0.5f, pyc = (float)py + 0.5f;
+    const int under = under_colour != NULL && under_depth != NULL;
+    const float under_z = under ? under_depth[at] : 0.0f;
+    const int under_hit = under && under_z > 0.0f;
+    float rgb0 = 0.0f, rgb1 = 0.0f, rgb2 = 0.0f;
+    float T = 1.0f, depth_sum = 0.0f, depth_at = -1.0f;
+    uint32_t stop = 0;
+    crypto_list list;
+    memset(&list, 0, sizeof list);
+    uint32_t tslots[ORC_TRACE_SLOTS];
+    float tweights[ORC_TRACE_SLOTS];
+    for (uint32_t k = 0; k < ORC_TRACE_SLOTS; ++k) {
+        tslots[k] = ORC_TRACE_NONE;
+        tweights[k] = 0.0f;
+    }
+    float ttotal = 0.0f, tfloor = 0.0f;
+    uint32_t tlightest = 0;
+    for (uint32_t j = start; j < end; ++j) {
+        const uint32_t slot = vals[j];
+        const float* r = rec + (size_t)slot * REC;
+        if (under_hit && r[2] >= under_z) break; /* the layer under the splats ends the walk */
+        const float dx = pxc - r[0], dy = pyc - r[1];
+        const float power = r[4] * dx * dx + 2.0f * r[5] * dx * dy + r[6] * dy * dy;
+        if (power > r[3]) continue;
+        const float alpha = minf_(0.99f, r[7] * athx_expf(-0.5f * power));
+        if (alpha < 1.0f / 255.0f) continue;
+        const float w = T * alpha;
+        if (matte) crypto_add(&list, slot_ids ? slot_ids[slot] : 0u, w);
+        if (traced) {
+            ttotal = ttotal + w;
+            if (w > tfloor) {
+                /* TraceList::enter (splat_trace.slang:81-90) */
+                tslots[tlightest] = slot;
+                tweights[tlightest] = w;
+                tlightest = 0;
+                for (uint32_t k = 1; k < ORC_TRACE_SLOTS; ++k) {
+                    if (tweights[k] < tweights[tlightest]) tlightest = k;
+                }
+                tfloor = tweights[tlightest];
+            }
+        }
+        rgb0 = rgb0 + w * r[8];
+        rgb1 = rgb1 + w * r[9];
+        rgb2 = rgb2 + w * r[10];
+        depth_sum = depth_sum + w * r[2];
+        T = T * (1.0f - alpha);
+        if (depth_at < 0.0f && 1.0f - T >= v->depth_threshold) depth_at = r[2];
+        stop = j - start + 1;
+        if (T < 1e-4f) break;
+    }
+    int opaque = 0;
+    float behind[4] = {v->bg[0], v->bg[1], v->bg[2], v->bg[3]};
+    float opaque_z = 0.0f;
+    uint32_t opaque_id = 0;
+    if (under_hit) {
+        opaque = 1;
+        for (int c = 0; c < 4; ++c) behind[c] = under_colour[(size_t)at * 4 + c];
+        opaque_z = under_z;
+        if (matte && under_crypto) opaque_id = under_crypto[at];
+    }
+    if (matte) {
+        crypto_add(&list, opaque_id, opaque ? T : 0.0f);
+        uint32_t order[ORC_CRYPTO_SLOTS];
+        order_desc(list.weights, ORC_CRYPTO_SLOTS, order);
+        for (uint32_t layer = 0; layer < ORC_CRYPTO_SLOTS / 2; ++layer) {
+            const uint32_t a = order[layer * 2], b = order[layer * 2 + 1];
+            float* o = crypto + ((size_t)layer * pixels + at) * 4;
+            o[0] = list.weights[a] > 0.0f ? bits_float(list.ids[a]) : 0.0f;
+            o[1] = maxf_(list.weights[a], 0.0f);
+            o[2] = list.weights[b] > 0.0f ? bits_float(list.ids[b]) : 0.0f;
+            o[3] = maxf_(list.weights[b], 0.0f);
+        }
+    }
+    if (traced) {
+        const float under_w = opaque ? T : 0.0f;
+        if (!(ttotal > 0.0f) && !(under_w > 0.0f)) {
+            trace[at] = ORC_TRACE_EMPTY;
+        } else {
+            uint32_t order[ORC_TRACE_SLOTS];
+            order_desc(tweights, ORC_TRACE_SLOTS, order);
+            float held = 0.0f;
+            for (uint32_t k = 0; k < ORC_TRACE_SLOTS; ++k) {
+                const uint32_t from = order[k];
+                const int some = tweights[from] > 0.0f;
+                trace[(size_t)k * pixels + at] = some ? tslots[from] : ORC_TRACE_NONE;
+                trace[(size_t)(ORC_TRACE_SLOTS + k) * pixels + at] =
+                    some ? float_bits(rec[(size_t)tslots[from] * REC + 2]) : 0u;
+                held = held + tweights[from];
+            }
+            for (uint32_t pr = 0; pr < ORC_TRACE_SLOTS / 2; ++pr) {
+                trace[(size_t)(ORC_TRACE_SLOTS * 2 + pr) * pixels + at] =
+                    ingest_half(tweights[order[pr * 2]]) | (ingest_half(tweights[order[pr * 2 + 1]]) << 16);
+            }
+            trace[(size_t)(ORC_TRACE_WORDS - 1) * pixels + at] =
+                ingest_half(maxf_(ttotal - held, 0.0f)) | (ingest_half(maxf_(under_w, 0.0f)) << 16);
+        }
+    }
+    const float coverage = 1.0f - T;
+    if (opaque) {
+        depth_sum = depth_sum + T * opaque_z;
+        if (depth_at < 0.0f) depth_at = opaque_z;
+    }
+    const float total = opaque ? 1.0f : coverage;
+    out[0] = rgb0 + T * behind[0];
+    out[1] = rgb1 + T * behind[1];
+    out[2] = rgb2 + T * behind[2];
+    out[3] = coverage + T * behind[3];
+    if (v->depth_mode == 0) {
+        *depth = total > 1e-6f ? depth_sum / total : 0.0f;
+    } else {
+        *depth = maxf_(depth_at, 0.0f);
+    }
+    *final_t = T;
+    *last = stop;
+}
+
+/* orc_blend with the under layer (under_colour [H][W][4] and under_depth
+ * [H][W], or NULL), the Cryptomatte (slot_ids a slot, under_crypto a pixel
+ * or NULL; crypto [3][H][W][4]) and the trace ([11][H][W] words), each off
+ * when its output is NULL. */
+ORC_EXPORT void orc_blend_ex(const orc_view* v, const float* rec, const uint32_t* ranges, const uint32_t* vals,
+                             uint32_t tile_offset, const float* under_colour, const float* under_depth,
+                             const uint32_t* slot_ids, const uint32_t* under_crypto, float* colour, float* depth,
+                             float* final_t, uint32_t* last, float* crypto, uint32_t* trace) {
+    const uint32_t W = v->width, H = v->height, T = v->tile;
+    const uint32_t tiles_x = (W + T - 1) / T;
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t p = 0; p < (int64_t)W * H; ++p) {
+        const uint32_t px = (uint32_t)(p % W), py = (uint32_t)(p / W);
+        const uint32_t tile = tile_offset + (py / T) * tiles_x + px / T;
+        blend_pixel_ex(v, rec, vals, ranges[tile * 2], ranges[tile * 2 + 1], px, py, under_colour, under_depth,
+                       slot_ids, under_crypto, crypto != NULL, trace != NULL, colour + p * 4, depth + p, final_t + p,
+                       last + p, crypto, trace);
+    }
+}
+
+/* ---------------------------------------------------------------------------
  * FP64: the same projection with the FP32 path's branches replayed.
  * rec64 layout is REC doubles a slot; only slots the FP32 path kept are written.
  */

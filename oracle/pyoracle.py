@@ -162,6 +162,10 @@ def load():
         lib.orc_ranges.restype = None
         lib.orc_blend.argtypes = [vp, _f32p, _u32p, _u32p, ctypes.c_uint32, _f32p, _f32p, _f32p, _u32p]
         lib.orc_blend.restype = None
+        lib.orc_blend_ex.argtypes = [vp, _f32p, _u32p, _u32p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p,
+                                     ctypes.c_void_p, ctypes.c_void_p, _f32p, _f32p, _f32p, _u32p, ctypes.c_void_p,
+                                     ctypes.c_void_p]
+        lib.orc_blend_ex.restype = None
         lib.orc_blend_f64.argtypes = [vp, _f32p, _u32p, _u32p, _u32p, _u32p, ctypes.c_uint32, _f64p, _f64p, _f64p,
                                       _f64p, _f64p, ctypes.c_uint32, ctypes.c_uint32, _f64p, _f64p]
         lib.orc_blend_f64.restype = None
@@ -338,6 +342,108 @@ def render(views: list[OrcView], cloud, with_f64: bool = False, relight=None, f6
         out["last"].append(last.reshape(H, W))
     if with_f64:
         out.update(render_f64(views, f64_cloud if f64_cloud is not None else cloud, out))
+    return out
+
+
+def render_instances(views: list[list[OrcView]], clouds: list, relights: list | None = None) -> dict:
+    """A frame of several instances (TileRasterizer::render over a span of
+    SplatInstances): views[j][k] is instance j's view k -- the same screen
+    with that instance's object-to-view rows and eye. Each instance is
+    projected on its own, its records placed after the instances before it
+    (slot = k * total + base_j + i, bases the running count,
+    TileRasterizer.cpp:896-898); the depth order, pairs, ranges and blend are
+    one frame's over all of them, as ``render`` takes them."""
+    lib = load()
+    counts = [c.positions.shape[0] if hasattr(c, "positions") else c.means.shape[0] for c in clouds]
+    bases = [int(b) for b in np.concatenate([[0], np.cumsum(counts)[:-1]])] if counts else []
+    n = int(sum(counts))
+    nv = len(views[0])
+    relights = relights or [None] * len(clouds)
+    rec = np.zeros((nv * n, 12), np.float32)
+    touched = np.zeros(nv * n, np.uint32)
+    keys = np.zeros(nv * n, np.uint32)
+    tile_bases = []
+    total_tiles = 0
+    for k in range(nv):
+        for j, cloud in enumerate(clouds):
+            r, t, kk = project(views[j][k], cloud, relight=relights[j])
+            sl = slice(k * n + bases[j], k * n + bases[j] + counts[j])
+            rec[sl] = r
+            touched[sl] = t
+            keys[sl] = kk
+        tile_bases.append(total_tiles)
+        tx, ty = tiles_of(views[0][k])
+        total_tiles += tx * ty
+    visible = np.nonzero(touched)[0]
+    order = visible[np.argsort(keys[visible] & np.uint32(0xFFFFFF), kind="stable")]
+    sorted_offsets, pairs = exclusive_scan(touched[order])
+    offsets = np.zeros(nv * n, np.uint32)
+    offsets[order] = sorted_offsets
+    pk = np.zeros(pairs, np.uint64)
+    pv = np.zeros(pairs, np.uint32)
+    for k in range(nv):
+        sl = slice(k * n, (k + 1) * n)
+        lib.orc_emit_pairs(ctypes.byref(views[0][k]), _c(rec[sl], np.float32), _c(touched[sl], np.uint32),
+                           _c(keys[sl], np.uint32), _c(offsets[sl], np.uint32), n, tile_bases[k], k * n, pk, pv)
+    key_bits = 24 + bits_for(total_tiles)
+    lib.orc_sort_pairs(pk, pv, pairs, key_bits)
+    ranges = np.zeros(total_tiles * 2, np.uint32)
+    lib.orc_ranges(pk, pairs, total_tiles, ranges)
+    out = {"records": rec, "touched": touched, "depth_keys": keys, "offsets": offsets, "pairs": pairs,
+           "pair_keys": pk, "pair_values": pv, "ranges": ranges, "tile_bases": tile_bases, "key_bits": key_bits,
+           "slot_bases": bases, "colour": [], "depth": [], "final_t": [], "last": []}
+    for k in range(nv):
+        v = views[0][k]
+        H, W = v.height, v.width
+        col = np.zeros(H * W * 4, np.float32)
+        dep = np.zeros(H * W, np.float32)
+        ft = np.zeros(H * W, np.float32)
+        last = np.zeros(H * W, np.uint32)
+        lib.orc_blend(ctypes.byref(v), rec, ranges, pv, tile_bases[k], col, dep, ft, last)
+        out["colour"].append(col.reshape(H, W, 4))
+        out["depth"].append(dep.reshape(H, W))
+        out["final_t"].append(ft.reshape(H, W))
+        out["last"].append(last.reshape(H, W))
+    return out
+
+
+TRACE_WORDS = 11
+
+
+def blend_ex(views: list[OrcView], frame: dict, under: list | None = None, slot_ids: np.ndarray | None = None,
+             matte: bool = False, traced: bool = False) -> dict:
+    """orc_blend_ex over a frame ``render`` (or ``render_instances``) made:
+    the walk's variants -- ``under`` a list of (colour [H, W, 4], depth [H,
+    W], crypto ids [H, W] or None) a view, ``slot_ids`` each slot's
+    Cryptomatte id, and the matte and trace planes when asked for."""
+    lib = load()
+    out = {"colour": [], "depth": [], "final_t": [], "last": [], "crypto": [], "trace": []}
+    ids = _c(slot_ids, np.uint32) if slot_ids is not None else None
+    for k, v in enumerate(views):
+        H, W = v.height, v.width
+        col = np.zeros(H * W * 4, np.float32)
+        dep = np.zeros(H * W, np.float32)
+        ft = np.zeros(H * W, np.float32)
+        last = np.zeros(H * W, np.uint32)
+        cr = np.zeros(3 * H * W * 4, np.float32) if matte else None
+        tr = np.full(TRACE_WORDS * H * W, 0xDEADBEEF, np.uint32) if traced else None
+        uc = ud = ux = None
+        if under is not None:
+            uc = _c(under[k][0], np.float32)
+            ud = _c(under[k][1], np.float32)
+            ux = _c(under[k][2], np.uint32) if under[k][2] is not None else None
+        lib.orc_blend_ex(ctypes.byref(v), _c(frame["records"], np.float32), _c(frame["ranges"], np.uint32),
+                         _c(frame["pair_values"], np.uint32), frame["tile_bases"][k],
+                         uc.ctypes.data if uc is not None else None, ud.ctypes.data if ud is not None else None,
+                         ids.ctypes.data if ids is not None else None, ux.ctypes.data if ux is not None else None,
+                         col, dep, ft, last, cr.ctypes.data if cr is not None else None,
+                         tr.ctypes.data if tr is not None else None)
+        out["colour"].append(col.reshape(H, W, 4))
+        out["depth"].append(dep.reshape(H, W))
+        out["final_t"].append(ft.reshape(H, W))
+        out["last"].append(last.reshape(H, W))
+        out["crypto"].append(cr.reshape(3, H, W, 4) if cr is not None else None)
+        out["trace"].append(tr.reshape(TRACE_WORDS, H, W) if tr is not None else None)
     return out
 
 

@@ -48,6 +48,7 @@ def test_library_exports_nothing_else():
 STRUCTS = {
     "athx_view": _lib.View,
     "athx_gaussians": _lib.Gaussians,
+    "athx_instance": _lib.Instance,
     "athx_sky": _lib.Sky,
     "athx_relight": _lib.Relight,
     "athx_packed_splats": _lib.PackedSplats,

@@ -163,3 +163,28 @@ def test_two_forwards_before_one_backward():
     with pytest.raises(_lib.AthxError, match="replaced"):
         raster.contribution([views[0]], gc, RenderSettings(), first, mode=1)
     raster.close()
+
+
+def test_overwritten_gradients_equal_accumulated_ones():
+    """athx_grads.overwrite: gradients written into tensors full of NaN equal,
+    bit for bit, the same adjoint added into zeros; Gaussians no view draws
+    get zeros written."""
+    print("backward overwrite", flush=True)
+    cloud = synthetic.plain_1080p_cloud(3000)
+    views = [camera.view_for(synthetic.front_camera(), 80, 64)]
+    raster = TileRasterizer(0)
+    gc = GaussianCloud.from_numpy(cloud, "cuda:0")
+    targets, _ = raster.render(views, gc, keep_backward=True)
+    dl = [torch.randn(targets[0].colour.shape, device="cuda", generator=torch.Generator("cuda").manual_seed(4))]
+    names = ("means", "log_scales", "quats", "opacity_logits", "sh")
+    added = {k: torch.zeros_like(getattr(gc, k)) for k in names}
+    written = {k: torch.full_like(getattr(gc, k), float("nan")) for k in names}
+    raster.render_backward(views, gc, RenderSettings(), targets, dl, added)
+    raster.render_backward(views, gc, RenderSettings(), targets, dl, written, overwrite=True)
+    torch.cuda.synchronize()
+    for k in names:
+        assert torch.equal(added[k], written[k]), k
+    drawn = raster.frame_buffers()["touched"].cpu().numpy() > 0
+    assert (~drawn).any() and drawn.any()
+    assert float(written["sh"][torch.from_numpy(~drawn).cuda()].abs().max()) == 0.0
+    raster.close()

@@ -15,7 +15,7 @@ import torch
 from athenea_amd import RenderSettings, TileRasterizer, camera, synthetic
 from athenea_amd._lib import AthxError
 from athenea_amd.camera import Camera
-from athenea_amd.raster import GaussianCloud
+from athenea_amd.raster import GaussianCloud, SplatInstance
 from oracle import pyoracle as po
 from tests.splat_fixtures import counters_cloud, random_cloud
 from tests.test_oracle import sort_patterns
@@ -142,20 +142,54 @@ def test_half_precision_sh(raster):
     assert torch.equal(t16[0].colour, t32[0].colour)
 
 
-def test_counters_match_the_reference_test(raster):
-    """test_splat_render.cpp:1831-1917."""
+def test_counters_match_the_reference_test():
+    """test_splat_render.cpp:1831-1917: two clouds as two instances, tags 41
+    and 42, nothing before a counted frame, a quiet frame leaving the last."""
     say("counters")
-    cam = Camera()
-    cam.lens.focal = cam.lens.haperture
-    view = camera.view_for(cam, 320, 240)
-    cloud = GaussianCloud.from_numpy(counters_cloud(), "cuda:0", linear=False)
-    _, stats = raster.render([view], cloud, RenderSettings(count_splats=True))
+    raster = TileRasterizer(0)
+    try:
+        cam = Camera()
+        cam.lens.focal = cam.lens.haperture
+        view = camera.view_for(cam, 320, 240)
+        both = counters_cloud()
+        first = GaussianCloud.from_numpy(both.subset(22), "cuda:0", linear=False)
+        rest = synthetic.Cloud(*(np.ascontiguousarray(getattr(both, k)[22:]) for k in
+                                 ("means", "log_scales", "quats", "opacity_logits", "sh")))
+        second = GaussianCloud.from_numpy(rest, "cuda:0", linear=False)
+        instances = [SplatInstance(first), SplatInstance(second)]
+        assert raster.latest_counters() is None
+        for tag in (41, 42):
+            _, stats = raster.render_instances([view], instances, RenderSettings(count_splats=True, counters_tag=tag))
+            c = raster.latest_counters()
+            assert c is not None and c.tag == tag
+            assert c.slots == 28 and c.visible == 16 and c.visible == stats.visible
+            assert c.culled["depth"] == 4 and c.culled["faint"] == 3 and c.culled["offscreen"] == 5
+            assert c.culled["edit"] == 0 and c.culled["unprojected"] == 0
+            assert sum(c.culled.values()) + c.visible == c.slots
+            assert c.pairs == stats.pairs and c.max_tiles >= 1
+            assert len(c.clouds) == 2 and c.clouds[0].visible == 10 and c.clouds[1].visible == 6
+            assert c.clouds[0].pairs + c.clouds[1].pairs == c.pairs
+        raster.render_instances([view], instances, RenderSettings(count_splats=False))
+        still = raster.latest_counters()
+        assert still is not None and still.tag == 42
+    finally:
+        raster.close()
+
+
+def test_counters_agree_with_the_exact_tile_counts(raster):
+    """The counters' pairs and most tiles, taken from the masks the emission
+    decided, against the frame's exact per-slot tile counts."""
+    say("counters vs exact counts")
+    cloud = GaussianCloud.from_numpy(synthetic.plain_1080p_cloud(200000), "cuda:0")
+    views = [camera.view_for(synthetic.front_camera(), 640, 360), camera.view_for(synthetic.walk_cameras(60)[5], 320, 200)]
+    _, stats = raster.render(views, cloud, RenderSettings(count_splats=True, counters_tag=7))
     c = raster.latest_counters()
-    assert c.slots == 28 and c.visible == 16
-    assert c.culled["depth"] == 4 and c.culled["faint"] == 3 and c.culled["offscreen"] == 5
-    assert c.culled["edit"] == 0 and c.culled["unprojected"] == 0
-    assert sum(c.culled.values()) + c.visible == c.slots
-    assert c.pairs == stats.pairs and c.max_tiles >= 1
+    touched = u32(raster.frame_buffers()["touched"]).astype(np.int64)
+    assert c.tag == 7 and c.slots == 2 * 200000
+    assert c.visible == stats.visible == int((touched > 0).sum())
+    assert c.pairs == stats.pairs == int(touched.sum())
+    assert c.max_tiles == int(touched.max()) > 16
+    assert len(c.clouds) == 1 and c.clouds[0].visible == c.visible and c.clouds[0].pairs == c.pairs
 
 
 def test_rendering_twice_gives_the_same_bits(raster):
