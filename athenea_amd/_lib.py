@@ -229,6 +229,10 @@ class PlyLayout(ctypes.Structure):
         ("dc", ctypes.c_uint32 * 3),
         ("rest_base", ctypes.c_uint32),
         ("rest_per_colour", ctypes.c_uint32),
+        ("opacity_mode", ctypes.c_uint32),
+        ("scale_mode", ctypes.c_uint32),
+        ("colour_mode", ctypes.c_uint32),
+        ("rotation_mode", ctypes.c_uint32),
     ]
 
 
@@ -352,6 +356,10 @@ SIGNATURES = {
         ctypes.c_int,
         [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p,
          ctypes.c_void_p],
+    ),
+    "athx_splat_layout": (
+        ctypes.c_int,
+        [ctypes.c_uint64, ctypes.POINTER(PlyLayout), ctypes.c_char_p, ctypes.c_uint64],
     ),
     "athx_render_instances": (
         ctypes.c_int,

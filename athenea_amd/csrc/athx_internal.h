@@ -389,6 +389,8 @@ struct IngestFields {
     uint32_t rest;        // rest coefficients kept a splat (3 a basis)
     uint32_t sh_words;    // words a splat in the output's sh
     uint32_t keep_faint;  // keep every vertex with a finite position
+    // athx_ply_layout's encodings of opacity, scale, colour and rotation.
+    uint32_t opacity_mode, scale_mode, colour_mode, rotation_mode;
     uint32_t code[kIngestCodes];
 };
 // The ground truth (reference.hip): the reference's own projection into

@@ -61,6 +61,78 @@ __device__ inline float blend_exp(float power) {
     return athx_expf_core(__builtin_amdgcn_fmed3f(-0.5f * power, -80.0f, 80.0f));
 }
 
+// A WAVEFRONT'S BAND. A tile's pairs are every Gaussian whose ellipse meets
+// the tile; a wavefront's 64 pixels are a band of it (TILE / 64 ... 64 / TILE
+// rows), and a small Gaussian often misses the band altogether, so that no
+// lane of the wavefront could take it. Each wavefront tests its band against
+// 64 staged Gaussians at once, a lane each, and walks only those that meet
+// it: the walk and every result are the same, pair for pair, since a
+// Gaussian skipped is one whose power exceeds the cut at every pixel centre
+// of the band. The test is conservative: the least power over the band's box
+// of pixel centres (zero when the box holds the centre, else the least over
+// its edges, each at its clamped stationary point) must exceed the cut by
+// kBandSlack * M, M = a X^2 + 2 |b| X Y + c Y^2 with X, Y the box's largest
+// distances from the centre -- far above the rounding of both this float
+// evaluation and the per-pixel one (a few units of 2^-24 M).
+constexpr float kBandSlack = 256.0f * 0x1p-24f;
+
+struct Band {
+    float x0, x1, y0, y1;  // the band's first and last pixel centres
+};
+
+template <uint32_t TILE>
+__device__ inline Band wave_band(const DView& V, uint32_t tx, uint32_t ty, uint32_t wave) {
+    constexpr uint32_t kRows = 64 / TILE;
+    const uint32_t c0 = tx * TILE, c1 = min(c0 + TILE, V.width) - 1u;
+    const uint32_t r0 = ty * TILE + wave * kRows;
+    const uint32_t r1 = min(r0 + kRows, V.height);  // r0 >= height: the band holds no pixel
+    Band b;
+    b.x0 = (float)c0 + 0.5f;
+    b.x1 = (float)c1 + 0.5f;
+    b.y0 = (float)r0 + 0.5f;
+    b.y1 = (float)(r1 > r0 ? r1 - 1u : r0) + 0.5f;
+    return b;
+}
+
+__device__ inline bool band_meets(const Band& B, float cx, float cy, float a, float b, float c, float cut) {
+    const float lox = B.x0 - cx, hix = B.x1 - cx, loy = B.y0 - cy, hiy = B.y1 - cy;
+    if (lox <= 0.0f && hix >= 0.0f && loy <= 0.0f && hiy >= 0.0f) {
+        return true;
+    }
+    if (!(a > 1e-12f) || !(c > 1e-12f)) {
+        return true;
+    }
+    const float ia = __builtin_amdgcn_rcpf(a), ic = __builtin_amdgcn_rcpf(c);
+    const auto at = [&](float x, float y) { return (a * x * x + 2.0f * b * x * y) + c * y * y; };
+    const float ya = clamp_(-b * lox * ic, loy, hiy), yb = clamp_(-b * hix * ic, loy, hiy);
+    const float xa = clamp_(-b * loy * ia, lox, hix), xb = clamp_(-b * hiy * ia, lox, hix);
+    const float least = fmin_(fmin_(at(lox, ya), at(hix, yb)), fmin_(at(xa, loy), at(xb, hiy)));
+    const float X = fmax_(fabsf(lox), fabsf(hix)), Y = fmax_(fabsf(loy), fabsf(hiy));
+    const float M = (a * X * X + 2.0f * fabsf(b) * X * Y) + c * Y * Y;
+    // Skipped only on a definite answer: a NaN anywhere keeps the Gaussian.
+    return !(least > cut + kBandSlack * M);
+}
+
+// The staged Gaussians first .. first + 63 of the batch (count staged) that
+// meet the band, as a mask of lanes: bit i for Gaussian first + i.
+template <uint32_t TILE>
+__device__ inline uint64_t band_mask(const LdsBatch& s, const Band& B, uint32_t first, uint32_t count, uint32_t lane) {
+    if constexpr (TILE == 8) {
+        // A wavefront is the whole tile, which every pair meets.
+        const uint32_t n = count - first < 64u ? count - first : 64u;
+        return n == 64u ? ~0ull : ((1ull << n) - 1ull);
+    } else {
+        const uint32_t j = first + lane;
+        bool meets = false;
+        if (j < count) {
+            const float4 q = s.xy_cut_op[j];
+            const float4 k = s.conic[j];
+            meets = band_meets(B, q.x, q.y, k.x, s.colour[j].w, k.z, q.z);
+        }
+        return __ballot(meets);
+    }
+}
+
 template <uint32_t TILE>
 __global__ void __launch_bounds__(TILE * TILE) blend_kernel(const DView* __restrict__ views,
                                                             const uint32_t* __restrict__ tile_bases, uint32_t nviews,
@@ -84,6 +156,8 @@ __global__ void __launch_bounds__(TILE * TILE) blend_kernel(const DView* __restr
     float r = 0.0f, g = 0.0f, b = 0.0f, T = 1.0f, dsum = 0.0f, dat = -1.0f;
     uint32_t last = 0;
     bool done = !inside;
+    const uint32_t lane = tid & 63u;
+    const Band band = wave_band<TILE>(V, tx, ty, tid >> 6);
     for (uint32_t first = start; first < end; first += batch) {
         if (!__syncthreads_or(!done)) {
             break;
@@ -93,7 +167,9 @@ __global__ void __launch_bounds__(TILE * TILE) blend_kernel(const DView* __restr
             stage(s, k, records, pair_slots[first + k]);
         }
         __syncthreads();
-        for (uint32_t j = 0; j < count && !done; ++j) {
+        for (uint32_t c0 = 0; c0 < count && __ballot(!done) != 0; c0 += 64) {
+          for (uint64_t mm = band_mask<TILE>(s, band, c0, count, lane); mm != 0 && !done; mm &= mm - 1ull) {
+            const uint32_t j = c0 + (uint32_t)__builtin_ctzll(mm);
             const float4 q = s.xy_cut_op[j];
             const float4 k = s.conic[j];
             const float dx = pxc - q.x, dy = pyc - q.y;
@@ -119,6 +195,7 @@ __global__ void __launch_bounds__(TILE * TILE) blend_kernel(const DView* __restr
             if (T < 1e-4f) {
                 done = true;
             }
+          }
         }
         // Pairs a pixel skipped after its last contribution are read again by
         // nobody: `last` is the walk's own extent.
@@ -274,6 +351,8 @@ __global__ void __launch_bounds__(TILE * TILE) blend_ex_kernel(const DView* __re
     float ttotal = 0.0f, tfloor = 0.0f;
     uint32_t tlightest = 0;
     bool done = !inside;
+    const uint32_t lane = tid & 63u;
+    const Band band = wave_band<TILE>(V, tx, ty, tid >> 6);
     for (uint32_t first = start; first < end; first += batch) {
         if (!__syncthreads_or(!done)) {
             break;
@@ -288,7 +367,12 @@ __global__ void __launch_bounds__(TILE * TILE) blend_ex_kernel(const DView* __re
             }
         }
         __syncthreads();
-        for (uint32_t j = 0; j < count && !done; ++j) {
+        // A pair the band skips holds nothing for these pixels; the under
+        // layer's stop comes at the next pair walked, or none is left to stop:
+        // pairs run front to back, so any later one lies behind the layer too.
+        for (uint32_t c0 = 0; c0 < count && __ballot(!done) != 0; c0 += 64) {
+          for (uint64_t mm = band_mask<TILE>(s, band, c0, count, lane); mm != 0 && !done; mm &= mm - 1ull) {
+            const uint32_t j = c0 + (uint32_t)__builtin_ctzll(mm);
             const float4 q = s.xy_cut_op[j];
             const float4 k = s.conic[j];
             if (under_hit && k.w >= under_z) {
@@ -345,6 +429,7 @@ __global__ void __launch_bounds__(TILE * TILE) blend_ex_kernel(const DView* __re
             if (T < 1e-4f) {
                 done = true;
             }
+          }
         }
         __syncthreads();
     }
@@ -474,6 +559,8 @@ __global__ void __launch_bounds__(TILE * TILE) contribution_kernel(const DView* 
     float T = 1.0f;
     float r = 0.0f, g = 0.0f, b = 0.0f;
     bool done = !inside;
+    const uint32_t lane = tid & 63u;
+    const Band band = wave_band<TILE>(V, tx, ty, tid >> 6);
     for (uint32_t first = start; first < end; first += batch) {
         if (!__syncthreads_or(!done)) {
             break;
@@ -485,7 +572,9 @@ __global__ void __launch_bounds__(TILE * TILE) contribution_kernel(const DView* 
             s_slot[k] = slot;
         }
         __syncthreads();
-        for (uint32_t j = 0; j < count && !done; ++j) {
+        for (uint32_t c0 = 0; c0 < count && __ballot(!done) != 0; c0 += 64) {
+          for (uint64_t mm = band_mask<TILE>(s, band, c0, count, lane); mm != 0 && !done; mm &= mm - 1ull) {
+            const uint32_t j = c0 + (uint32_t)__builtin_ctzll(mm);
             const float4 q = s.xy_cut_op[j];
             const float4 k = s.conic[j];
             const float dx = pxc - q.x, dy = pyc - q.y;
@@ -543,6 +632,7 @@ __global__ void __launch_bounds__(TILE * TILE) contribution_kernel(const DView* 
             if (T < 1e-4f) {
                 done = true;
             }
+          }
         }
         __syncthreads();
     }
@@ -667,6 +757,7 @@ __global__ void __launch_bounds__(TILE * TILE) blend_backward_kernel(
     const uint32_t max_last = s_max_last;
     // A wavefront walks only the pairs below its own pixels' furthest.
     const uint32_t wave_last = s_wave_last[tid >> 6];
+    const Band band = wave_band<TILE>(V, tx, ty, tid >> 6);
     for (uint32_t hi = max_last; hi > 0;) {
         const uint32_t lo = hi > batch ? hi - batch : 0;
         const uint32_t count = hi - lo;
@@ -680,8 +771,14 @@ __global__ void __launch_bounds__(TILE * TILE) blend_backward_kernel(
         }
         __syncthreads();
         const uint32_t top = wave_last > lo ? (wave_last - lo < count ? wave_last - lo : count) : 0u;
-        for (uint32_t jj = top; jj > 0; --jj) {
-            const uint32_t j = jj - 1;
+        // Back to front over the pairs below `top` that meet the band, 64 at a
+        // time; the rest hold no pixel of it and add nothing.
+        for (uint32_t c1 = top; c1 > 0; c1 = c1 > 64u ? c1 - 64u : 0u) {
+          const uint32_t c0 = c1 > 64u ? c1 - 64u : 0u;
+          for (uint64_t mm = band_mask<TILE>(s, band, c0, c1, lane); mm != 0;) {
+            const uint32_t bit = 63u - (uint32_t)__builtin_clzll(mm);
+            mm &= ~(1ull << bit);
+            const uint32_t j = c0 + bit;
             const uint32_t idx = lo + j;
             bool contrib = false;
             float g_cx = 0.f, g_cy = 0.f, g_ka = 0.f, g_kb = 0.f, g_kc = 0.f, g_al = 0.f, g_r = 0.f, g_g = 0.f,
@@ -734,6 +831,7 @@ __global__ void __launch_bounds__(TILE * TILE) blend_backward_kernel(
                     atomicAdd(&s_grad[j * kGrad2d + 8], sum_b);
                 }
             }
+          }
         }
         __syncthreads();
         for (uint32_t k = tid; k < count * kGrad2d; k += kThreads) {

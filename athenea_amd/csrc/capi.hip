@@ -1696,6 +1696,13 @@ ATHX_API athx_status athx_ingest_ply(athx_ctx* ctx, const void* vertices, const 
     f.count = (uint32_t)L.count;
     f.stride = L.stride;
     f.keep_faint = keep_faint ? 1u : 0u;
+    if (L.opacity_mode > 2 || L.scale_mode > 2 || L.colour_mode > 4 || L.rotation_mode > 1) {
+        return fail(ctx, ATHX_UNSUPPORTED, "an encoding this ingest does not decode");
+    }
+    f.opacity_mode = L.opacity_mode;
+    f.scale_mode = L.scale_mode;
+    f.colour_mode = L.colour_mode;
+    f.rotation_mode = L.rotation_mode;
     const auto code_of = [&](uint32_t index, uint32_t& code) {
         if (index >= L.properties || L.type[index] > ATHX_PLY_I32) {
             return false;

@@ -2,9 +2,10 @@
 // each property widened from the file's bytes (packed_records.slang:1-110),
 // each vertex validated (splat_validate.slang:1-31) and the kept ones decoded
 // and packed in file order (splat_decode.slang:1-173, splat_encoding.slang,
-// common/packing.slang:33-57,319-328) -- the 3DGS encodings a PLY carries:
-// opacity a logit, log-scales (below -30 a flat axis), float rotation w x y
-// z, the SH DC coefficient, rest coefficients red's, green's, blue's.
+// common/packing.slang:33-57,319-328) -- by default the 3DGS encodings a PLY
+// carries (opacity a logit, log-scales, below -30 a flat axis, float rotation
+// w x y z, the SH DC coefficient, rest coefficients red's, green's, blue's),
+// or as the layout's modes say (a .splat's bytes and linear scales).
 //
 // Two passes over the block, a vertex a lane: flags, their exclusive scan
 // (each kept vertex's place), then the pack. The block is read as bytes at
@@ -48,9 +49,42 @@ __device__ inline float widen(const uint8_t* __restrict__ rec, uint32_t code) {
 
 __device__ inline bool finite_(float v) { return (__builtin_bit_cast(uint32_t, v) & 0x7F800000u) != 0x7F800000u; }
 
-// A file's log-scale: below -30 a flat axis, -inf (splat_encoding.slang's
-// decodeLogScale, PLY mode).
-__device__ inline float log_scale_of(float v) { return v < -30.0f ? -__builtin_inff() : v; }
+// splat_encoding.slang's decodeOpacity, decodeLogScale, decodeColour and
+// decodeRotation, by the layout's modes (RawSplats.h:85-101).
+__device__ inline float opacity_of(float v, uint32_t mode) {
+    return mode == 0 ? athx_sigmoidf(v) : mode == 2 ? v / 255.0f : v;
+}
+
+// A flat axis: -inf, or a hard surfel's p as packing.slang's surfelHardnessLog keeps it.
+__device__ inline float log_scale_of(float v, uint32_t mode) {
+    if (mode == 2) {
+        return v / 16.0f - 10.0f;
+    }
+    if (mode == 0) {
+        return v < -30.0f ? -__builtin_inff() : v;
+    }
+    if (v < 0.0f) {
+        const float p = -v;
+        return p <= 2.001f ? -65000.0f : -(1024.0f + 64.0f * (fminf(p, 32.0f) - 2.0f));
+    }
+    return v == 0.0f ? -__builtin_inff() : athx_logf(fmaxf(v, 1e-30f));
+}
+
+__device__ inline float colour_of(float v, uint32_t mode) {
+    constexpr float kSh0 = 0.28209479177387814f;
+    if (mode == 0) {
+        return 0.5f + kSh0 * v;
+    }
+    if (mode == 2) {
+        return v / 255.0f;
+    }
+    if (mode == 3) {
+        return 0.5f + kSh0 * ((v / 255.0f - 0.5f) / 0.15f);
+    }
+    return v;
+}
+
+__device__ inline float rotation_of(float v, uint32_t mode) { return mode == 1 ? (v - 128.0f) / 128.0f : v; }
 
 // At most one flat axis; the others finite and under 60 in magnitude
 // (splat_encoding.slang's scalesValid).
@@ -74,10 +108,10 @@ __device__ inline bool keep_vertex(const uint8_t* rec, const IngestFields& f, fl
     px = widen(rec, f.code[0]);
     py = widen(rec, f.code[1]);
     pz = widen(rec, f.code[2]);
-    alpha = athx_sigmoidf(widen(rec, f.code[3]));
-    ls[0] = log_scale_of(widen(rec, f.code[4]));
-    ls[1] = log_scale_of(widen(rec, f.code[5]));
-    ls[2] = log_scale_of(widen(rec, f.code[6]));
+    alpha = opacity_of(widen(rec, f.code[3]), f.opacity_mode);
+    ls[0] = log_scale_of(widen(rec, f.code[4]), f.scale_mode);
+    ls[1] = log_scale_of(widen(rec, f.code[5]), f.scale_mode);
+    ls[2] = log_scale_of(widen(rec, f.code[6]), f.scale_mode);
     const bool finite = finite_(px) && finite_(py) && finite_(pz);
     return (finite && scales_valid(ls[0], ls[1], ls[2]) && alpha >= 1.0f / 255.0f) || (f.keep_faint != 0 && finite);
 }
@@ -140,14 +174,15 @@ __global__ void __launch_bounds__(kIngestThreads) ingest_pack_kernel(const uint8
         out.origin[j] = i;
     }
     // Rotation: the file's rot_0 is w. A quaternion of no length is the identity.
-    const float qw = widen(rec, f.code[7]), qx = widen(rec, f.code[8]), qy = widen(rec, f.code[9]),
-                qz = widen(rec, f.code[10]);
+    const float qw = rotation_of(widen(rec, f.code[7]), f.rotation_mode),
+                qx = rotation_of(widen(rec, f.code[8]), f.rotation_mode),
+                qy = rotation_of(widen(rec, f.code[9]), f.rotation_mode),
+                qz = rotation_of(widen(rec, f.code[10]), f.rotation_mode);
     const bool some = sqrtf(((qx * qx + qy * qy) + qz * qz) + qw * qw) > 1e-8f;
     const uint32_t rotation = some ? quaternion_word(qx, qy, qz, qw) : quaternion_word(0.0f, 0.0f, 0.0f, 1.0f);
-    constexpr float kSh0 = 0.28209479177387814f;
-    const float r = 0.5f + kSh0 * widen(rec, f.code[11]);
-    const float g = 0.5f + kSh0 * widen(rec, f.code[12]);
-    const float b = 0.5f + kSh0 * widen(rec, f.code[13]);
+    const float r = colour_of(widen(rec, f.code[11]), f.colour_mode);
+    const float g = colour_of(widen(rec, f.code[12]), f.colour_mode);
+    const float b = colour_of(widen(rec, f.code[13]), f.colour_mode);
     reinterpret_cast<uint4*>(out.shape)[j] =
         make_uint4(rotation, half_bits(ls[0]) | (half_bits(ls[1]) << 16), half_bits(ls[2]) | (half_bits(r) << 16),
                    half_bits(g) | (half_bits(b) << 16));

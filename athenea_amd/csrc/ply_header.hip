@@ -292,11 +292,68 @@ extern "C" ATHX_API athx_status athx_ply_parse(const char* header, uint64_t byte
     if (r.status != ATHX_OK) {
         return report(r, message, message_bytes);
     }
-    athx_ply_layout L;
+    athx_ply_layout L{};
     r = vertex_layout(h, L);
     if (r.status != ATHX_OK) {
         return report(r, message, message_bytes);
     }
+    *out = L;
+    return ATHX_OK;
+} catch (const std::bad_alloc&) {
+    return report({ATHX_OUT_OF_MEMORY, "host allocation failed"}, message, message_bytes);
+} catch (const std::exception& e) {
+    return report({ATHX_INTERNAL_ERROR, e.what()}, message, message_bytes);
+}
+
+extern "C" ATHX_API athx_status athx_splat_layout(uint64_t bytes, athx_ply_layout* out, char* message,
+                                                  uint64_t message_bytes) try {
+    if (message && message_bytes > 0) {
+        message[0] = '\0';
+    }
+    if (!out) {
+        return report({ATHX_INVALID_ARGUMENT, "null layout"}, message, message_bytes);
+    }
+    // SplatReaders.cpp:208-213: whole 32-byte records, at least one.
+    constexpr uint64_t kRecord = 32;
+    if (bytes == 0 || bytes % kRecord != 0) {
+        return report({ATHX_INVALID_ARGUMENT,
+                       std::to_string(bytes) + " bytes is not a whole number of 32-byte .splat records"},
+                      message, message_bytes);
+    }
+    if (bytes / kRecord > 0xFFFFFFFFull) {
+        return report({ATHX_UNSUPPORTED, "more than 2^32 records"}, message, message_bytes);
+    }
+    athx_ply_layout L{};
+    L.data_offset = 0;
+    L.count = bytes / kRecord;
+    L.stride = (uint32_t)kRecord;
+    // pos f32 x 3 | scale f32 x 3 | rgba u8 x 4 | rot u8 x 4 (w x y z): six
+    // floats, then eight bytes (SplatReaders.cpp:215-235).
+    L.properties = 14;
+    for (uint32_t k = 0; k < 14; ++k) {
+        L.offset[k] = k < 6 ? 4 * k : 24 + (k - 6);
+        L.type[k] = k < 6 ? ATHX_PLY_F32 : ATHX_PLY_U8;
+    }
+    L.x = 0;
+    L.y = 1;
+    L.z = 2;
+    L.scale[0] = 3;
+    L.scale[1] = 4;
+    L.scale[2] = 5;
+    L.dc[0] = 6;
+    L.dc[1] = 7;
+    L.dc[2] = 8;
+    L.opacity = 9;
+    L.rot[0] = 10;
+    L.rot[1] = 11;
+    L.rot[2] = 12;
+    L.rot[3] = 13;
+    L.rest_base = 0;
+    L.rest_per_colour = 0;
+    L.opacity_mode = 2;   // a byte
+    L.scale_mode = 1;     // linear
+    L.colour_mode = 2;    // a byte
+    L.rotation_mode = 1;  // bytes
     *out = L;
     return ATHX_OK;
 } catch (const std::bad_alloc&) {

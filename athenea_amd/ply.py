@@ -1,7 +1,7 @@
-"""A trained splat cloud's PLY, read into the packed device layout the
-rasterizer draws (io::readSplatPly, modules/io/src/SplatReaders.cpp:48-195,
-then scene::CloudLoader::upload, modules/scene/include/athenea/scene/
-GpuClouds.h:349-516).
+"""A trained splat cloud's PLY or .splat file, read into the packed device
+layout the rasterizer draws (io::readSplatPly and io::readDotSplat,
+modules/io/src/SplatReaders.cpp:48-240, then scene::CloudLoader::upload,
+modules/scene/include/athenea/scene/GpuClouds.h:349-516).
 
 The header is parsed by the library (athx_ply_parse: the reference's header
 grammar and checks), the vertex block is copied to the device as the file
@@ -107,3 +107,39 @@ def read_ply(path, device="cuda:0", max_degree: int = 3, keep_faint: bool = Fals
             block = np.frombuffer(data[start:start + n * stride], dtype=np.uint8)  # a copy: the map closes
     vertices = torch.from_numpy(block.copy()).to(device)
     return ingest(vertices, layout, max_degree, keep_faint, raster)
+
+
+def read_dot_splat(path, device="cuda:0", keep_faint: bool = False,
+                   raster: TileRasterizer | None = None) -> PlyCloud:
+    """A .splat file (32-byte records: position and linear scale floats,
+    rgba and rotation bytes) into the packed layout on `device`: its layout
+    from the file's size (athx_splat_layout), the records copied to the
+    device as they are and decoded there (athx_ingest_ply)."""
+    path = Path(path)
+    lib = _lib.load()
+    layout = _lib.PlyLayout()
+    message = ctypes.create_string_buffer(512)
+    size = path.stat().st_size
+    status = lib.athx_splat_layout(size, ctypes.byref(layout), message, len(message))
+    if status != 0:
+        raise _lib.AthxError(status, f"'{path}': {message.value.decode(errors='replace')}")
+    block = np.fromfile(path, dtype=np.uint8)
+    vertices = torch.from_numpy(block).to(device)
+    return ingest(vertices, layout, 0, keep_faint, raster)
+
+
+def read_splats(path, device="cuda:0", max_degree: int = 3, keep_faint: bool = False,
+                raster: TileRasterizer | None = None) -> PlyCloud:
+    """io::readSplats by extension (SplatReaders.cpp:28-46): .ply or .splat."""
+    ext = Path(path).suffix.lower()
+    if ext == ".splat":
+        return read_dot_splat(path, device, keep_faint, raster)
+    if ext == ".ply":
+        return read_ply(path, device, max_degree, keep_faint, raster)
+    if ext == ".spz":
+        # SPZ is a zstd/gzip stream decoded by Niantic's spz library, which the
+        # reference links when it finds it (SplatReaders.cpp:275-279).
+        raise _lib.AthxError(4, f"'{path}': this build reads no .spz")
+    if ext == ".sog" or Path(path).name == "meta.json":
+        raise _lib.AthxError(4, f"'{path}': SOG (WebP planes and a palette) is not read by this build")
+    raise _lib.AthxError(4, f"'{path}': no splat reader for '{ext}'")

@@ -18,9 +18,24 @@
  *   athx_render
  *       TileRasterizer::render(const Projection&, span<const SplatInstance>,
  *       const RenderSettings&, RenderTargets&) (TileRasterizer.h:465-477;
- *       TileRasterizer.cpp:296-1295), for plain (not relit, no under layer,
- *       no points, no matte/trace) clouds, generalised to several views in
- *       one call: views[k] is a camera's Projection and writes targets[k].
+ *       TileRasterizer.cpp:296-1295) for one instance at the identity:
+ *       plain or relit clouds, an under layer, Cryptomatte and trace targets
+ *       (no point sprites), generalised to several views in one call:
+ *       views[k] is a camera's Projection and writes targets[k].
+ *   athx_render_instances
+ *       the same render over a span of SplatInstance (TileRasterizer.h:
+ *       44-60, 465-477): several placed clouds, or one cloud placed several
+ *       times, in one frame (TileRasterizer.cpp:896-898's slot numbering).
+ *   athx_render_async / athx_frame_pairs
+ *       the render's asynchronous pair readback (TileRasterizer.cpp:
+ *       1240-1290): no host wait, the pair count read a frame later.
+ *   athx_graph_create / athx_graph_launch / athx_graph_launch_backward /
+ *   athx_graph_pairs / athx_graph_destroy
+ *       no counterpart (the reference records a command buffer per frame):
+ *       one frame's launches captured in a HIP graph and replayed.
+ *   athx_frame_serial / athx_frame_info_get / athx_copy_frame
+ *       the frame's buffers as RenderTargets' debug views (TileRasterizer.h:
+ *       403-452): pairs, ranges and records of the last frame, for tests.
  *   athx_render_backward
  *       no counterpart in the reference (its only gradient is the blend's
  *       contributionMode 4, shaders/athenea/splat/splat_blend.slang:381-398):
@@ -37,13 +52,19 @@
  *       gpu::RadixSort::sort (modules/gpu/include/athenea/gpu/algo/
  *       RadixSort.h:63-64) for 64-bit keys: the reference's keysLo/keysHi
  *       halves are one uint64 here.
- *   athx_ply_parse / athx_ingest_ply
+ *   athx_ply_parse / athx_splat_layout / athx_ingest_ply
  *       io::parsePlyHeader + io::readSplatPly (modules/io/src/PlyHeader.cpp,
- *       SplatReaders.cpp:48-195) and scene::CloudLoader::upload for a PLY
- *       (modules/scene/include/athenea/scene/GpuClouds.h:349-516).
- *   athx_reference_render / athx_compare_images / athx_compare_hdr
+ *       SplatReaders.cpp:48-195), the .splat record reader (SplatReaders.cpp:
+ *       203-241) and scene::CloudLoader::upload for either (modules/scene/
+ *       include/athenea/scene/GpuClouds.h:349-516).
+ *   athx_reference_render / athx_reference_render_instances /
+ *   athx_compare_images / athx_compare_hdr
  *       render::ReferenceRenderer::render, compareImages, compareHdr
  *       (modules/render/include/athenea/render/ReferenceRenderer.h:20-66).
+ *   athx_sky_texels / athx_prepare_sky
+ *       technique::Environment (modules/technique/include/athenea/technique/
+ *       Environment.h:1-155) for the relit path: the sky's prefiltered
+ *       octahedral levels and SH as the blend reads them.
  *
  * Threading: a context is used by one host thread at a time. Every call is
  * asynchronous on the stream it is given, except where a comment says it
@@ -581,6 +602,15 @@ typedef struct athx_ply_layout {
     /* f_rest_*: the first one, the file's count a colour (0, 3, 8 or 15;
        red's all, then green's, then blue's) */
     uint32_t rest_base, rest_per_colour;
+    /* How the fields are encoded (io::SplatEncoding, modules/io/include/
+       athenea/io/RawSplats.h:85-101); all 0 for a 3DGS PLY:
+         opacity   0 a logit, 1 linear, 2 a byte (v / 255)
+         scale     0 a log (below -30 a flat axis), 1 linear (0 a flat axis,
+                   negative a hard surfel's p), 2 SPZ's byte (v / 16 - 10)
+         colour    0 the SH DC coefficient, 1 linear, 2 a byte (v / 255),
+                   3 SPZ's byte, 4 light
+         rotation  0 floats, 1 bytes ((v - 128) / 128), w x y z either way */
+    uint32_t opacity_mode, scale_mode, colour_mode, rotation_mode;
 } athx_ply_layout;
 
 /*
@@ -597,7 +627,19 @@ typedef struct athx_ply_layout {
 ATHX_API athx_status athx_ply_parse(const char* header, uint64_t bytes, athx_ply_layout* out, char* message,
                                     uint64_t message_bytes);
 
-/* Where athx_ingest_ply writes the kept splats (device memory, the caller's). */
+/*
+ * io::readDotSplat (modules/io/src/SplatReaders.cpp:203-240): the layout of a
+ * .splat file of `bytes` bytes, 32-byte records of position f32 x 3, linear
+ * scale f32 x 3, rgba bytes (alpha the opacity), rotation bytes w x y z --
+ * the vertex block athx_ingest_ply reads, from the file's first byte.
+ * INVALID_ARGUMENT when the size is not a whole number of records (or 0).
+ * Host only.
+ */
+ATHX_API athx_status athx_splat_layout(uint64_t bytes, athx_ply_layout* out, char* message, uint64_t message_bytes);
+
+/* Where athx_ingest_ply writes the kept splats (device memory, the caller's);
+ * it reads a PLY's vertex block or a .splat file's records alike, as the
+ * layout's encodings say. */
 typedef struct athx_ingest_out {
     float* positions;  /* [count][4]: x y z, linear opacity clamped to [0, 5] */
     uint32_t* shape;   /* [count][4]: quaternion word; f16 log-scales; f16 base colour */

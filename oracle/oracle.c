@@ -1845,9 +1845,38 @@ static uint32_t ingest_quaternion(float x, float y, float z, float w) {
  * rest coefficients in packing order (rgb a basis). Writes the kept vertices
  * (file order) and returns their count.
  */
-ORC_EXPORT uint32_t orc_ingest(const uint8_t* vertices, uint32_t count, uint32_t stride, const uint32_t* offsets,
-                               const uint32_t* types, uint32_t rest, uint32_t keep_faint, float* positions,
-                               uint32_t* shape, uint32_t* sh, uint32_t* origin) {
+/* splat_encoding.slang's decodeOpacity / decodeLogScale / decodeColour /
+ * decodeRotation by mode (io::SplatEncoding, RawSplats.h:85-101); a hard
+ * surfel's p is kept as packing.slang:393-395's surfelHardnessLog. */
+static float decode_opacity(float v, uint32_t mode) {
+    return mode == 0 ? athx_sigmoidf(v) : mode == 2 ? v / 255.0f : v;
+}
+
+static float decode_log_scale(float v, uint32_t mode) {
+    if (mode == 2) return v / 16.0f - 10.0f;
+    if (mode == 0) return v < -30.0f ? -INFINITY : v;
+    if (v < 0.0f) {
+        const float p = -v;
+        return p <= 2.001f ? -65000.0f : -(1024.0f + 64.0f * ((p < 32.0f ? p : 32.0f) - 2.0f));
+    }
+    return v == 0.0f ? -INFINITY : athx_logf(v > 1e-30f ? v : 1e-30f);
+}
+
+static float decode_colour(float v, uint32_t mode) {
+    const float sh0 = 0.28209479177387814f;
+    if (mode == 0) return 0.5f + sh0 * v;
+    if (mode == 2) return v / 255.0f;
+    if (mode == 3) return 0.5f + sh0 * ((v / 255.0f - 0.5f) / 0.15f);
+    return v;
+}
+
+static float decode_rotation(float v, uint32_t mode) { return mode == 1 ? (v - 128.0f) / 128.0f : v; }
+
+/* orc_ingest with the layout's encodings: modes = opacity, scale, colour,
+ * rotation (athx_ply_layout's). */
+ORC_EXPORT uint32_t orc_ingest_modes(const uint8_t* vertices, uint32_t count, uint32_t stride, const uint32_t* offsets,
+                                     const uint32_t* types, uint32_t rest, uint32_t keep_faint, const uint32_t* modes,
+                                     float* positions, uint32_t* shape, uint32_t* sh, uint32_t* origin) {
     const uint32_t words = (rest + 1) / 2;
     uint32_t kept = 0;
     for (uint32_t i = 0; i < count; ++i) {
@@ -1856,11 +1885,11 @@ ORC_EXPORT uint32_t orc_ingest(const uint8_t* vertices, uint32_t count, uint32_t
         for (int k = 0; k < 14; ++k) {
             v[k] = ingest_widen(rec, offsets[k], types[k]);
         }
-        const float opacity = athx_sigmoidf(v[3]);
+        const float opacity = decode_opacity(v[3], modes[0]);
         float ls[3];
         int flats = 0, scales_ok = 1;
         for (int k = 0; k < 3; ++k) {
-            ls[k] = v[4 + k] < -30.0f ? -INFINITY : v[4 + k];
+            ls[k] = decode_log_scale(v[4 + k], modes[1]);
             const int flat = ls[k] <= -1024.0f;
             flats += flat;
             const float kept_ls = flat ? 0.0f : ls[k];
@@ -1882,12 +1911,13 @@ ORC_EXPORT uint32_t orc_ingest(const uint8_t* vertices, uint32_t count, uint32_t
         positions[4 * j + 2] = v[2];
         positions[4 * j + 3] = a;
         /* rot_0 is w; a quaternion of no length is the identity */
-        const float qw = v[7], qx = v[8], qy = v[9], qz = v[10];
+        const float qw = decode_rotation(v[7], modes[3]), qx = decode_rotation(v[8], modes[3]),
+                    qy = decode_rotation(v[9], modes[3]), qz = decode_rotation(v[10], modes[3]);
         const float qlen = sqrtf(((qx * qx + qy * qy) + qz * qz) + qw * qw);
         const uint32_t rotation = qlen > 1e-8f ? ingest_quaternion(qx, qy, qz, qw)
                                                : ingest_quaternion(0.0f, 0.0f, 0.0f, 1.0f);
-        const float sh0 = 0.28209479177387814f;
-        const float base[3] = {0.5f + sh0 * v[11], 0.5f + sh0 * v[12], 0.5f + sh0 * v[13]};
+        const float base[3] = {decode_colour(v[11], modes[2]), decode_colour(v[12], modes[2]),
+                               decode_colour(v[13], modes[2])};
         shape[4 * j + 0] = rotation;
         shape[4 * j + 1] = ingest_half(ls[0]) | (ingest_half(ls[1]) << 16);
         shape[4 * j + 2] = ingest_half(ls[2]) | (ingest_half(base[0]) << 16);
@@ -1899,6 +1929,14 @@ ORC_EXPORT uint32_t orc_ingest(const uint8_t* vertices, uint32_t count, uint32_t
         }
     }
     return kept;
+}
+
+ORC_EXPORT uint32_t orc_ingest(const uint8_t* vertices, uint32_t count, uint32_t stride, const uint32_t* offsets,
+                               const uint32_t* types, uint32_t rest, uint32_t keep_faint, float* positions,
+                               uint32_t* shape, uint32_t* sh, uint32_t* origin) {
+    static const uint32_t kPly[4] = {0, 0, 0, 0};
+    return orc_ingest_modes(vertices, count, stride, offsets, types, rest, keep_faint, kPly, positions, shape, sh,
+                            origin);
 }
 
 ORC_EXPORT uint32_t orc_half_bits(float v) { return ingest_half(v); }

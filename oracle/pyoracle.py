@@ -184,6 +184,10 @@ def load():
         lib.orc_ingest.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, _u32p, _u32p, ctypes.c_uint32,
                                    ctypes.c_uint32, _f32p, _u32p, ctypes.c_void_p, _u32p]
         lib.orc_ingest.restype = ctypes.c_uint32
+        lib.orc_ingest_modes.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, _u32p, _u32p,
+                                         ctypes.c_uint32, ctypes.c_uint32, _u32p, _f32p, _u32p, ctypes.c_void_p,
+                                         _u32p]
+        lib.orc_ingest_modes.restype = ctypes.c_uint32
         lib.orc_ref_project.argtypes = [vp, _f32p, _u32p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32,
                                         _f32p]
         lib.orc_compare_codes.argtypes = [_f32p, _f32p, ctypes.c_uint32, ctypes.c_uint32, _u32p]
@@ -633,7 +637,8 @@ def decode_quaternion(word: int) -> np.ndarray:
     return np.array(out[:], np.float32)
 
 
-def ingest(vertices: np.ndarray, count: int, stride: int, fields: list[tuple[int, int]], keep_faint: bool = False):
+def ingest(vertices: np.ndarray, count: int, stride: int, fields: list[tuple[int, int]], keep_faint: bool = False,
+           modes: tuple = (0, 0, 0, 0)):
     """orc_ingest: `count` vertices of `stride` bytes (a uint8 array); fields =
     (byte offset, athx_ply_type) of x y z, opacity, scale_0..2, rot_0..3 (w x y
     z), f_dc_0..2, then the kept rest coefficients rgb a basis. Returns the
@@ -649,9 +654,10 @@ def ingest(vertices: np.ndarray, count: int, stride: int, fields: list[tuple[int
     shape = np.zeros((max(count, 1), 4), np.uint32)
     sh = np.zeros((max(count, 1), max(words, 1)), np.uint32)
     origin = np.zeros(max(count, 1), np.uint32)
-    kept = load().orc_ingest(vertices.ctypes.data, count, stride, _c(offsets, np.uint32), _c(types, np.uint32), rest,
-                             1 if keep_faint else 0, _c(positions, np.float32), _c(shape, np.uint32),
-                             sh.ctypes.data, _c(origin, np.uint32))
+    kept = load().orc_ingest_modes(vertices.ctypes.data, count, stride, _c(offsets, np.uint32),
+                                   _c(types, np.uint32), rest, 1 if keep_faint else 0,
+                                   _c(np.asarray(modes, np.uint32), np.uint32), _c(positions, np.float32),
+                                   _c(shape, np.uint32), sh.ctypes.data, _c(origin, np.uint32))
     return positions[:kept], shape[:kept], sh[:kept, :words], origin[:kept]
 
 

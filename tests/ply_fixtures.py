@@ -130,3 +130,31 @@ def random_cloud(count: int, seed: int, degree: int = 3, types: dict[str, str] |
             info = np.iinfo(np.dtype(dt))
             cols[n] = np.clip(np.nan_to_num(cols[n] * 10, nan=0, posinf=0, neginf=0), info.min, info.max)
     return props, header(props, count) + block(props, cols, count), cols
+
+
+def dot_splat_record(pos=(0.5, -0.5, 2.0), scale=(0.25, 0.5, 1.0), rgba=(255, 0, 51, 255),
+                     rot=(255, 128, 128, 128)) -> bytes:
+    """One 32-byte .splat record (SplatReaders.cpp:215): position and linear
+    scale as floats, rgba and rotation (w x y z) as bytes; the defaults are
+    tests/scene/test_loading.cpp:168-199's record."""
+    return struct.pack("<3f3f4B4B", *pos, *scale, *rgba, *rot)
+
+
+def dot_splat_file(n: int, seed: int) -> bytes:
+    """n seeded .splat records: sizes from tiny to large, every alpha byte
+    (0 is too faint to keep), zero and negative scales (a flat axis, a hard
+    surfel's p), and rotations of every byte."""
+    rng = np.random.default_rng(seed)
+    out = bytearray()
+    for i in range(n):
+        scale = list(np.exp(rng.normal(-3.0, 1.0, 3)).astype(np.float32))
+        if i % 17 == 0:
+            scale[i % 3] = 0.0
+        if i % 23 == 0:
+            scale[(i + 1) % 3] = -float(rng.uniform(1.0, 40.0))
+        rgba = [int(v) for v in rng.integers(0, 256, 4)]
+        rot = [int(v) for v in rng.integers(0, 256, 4)]
+        if i % 29 == 0:
+            rot = [128, 128, 128, 128]  # no length: the identity
+        out += dot_splat_record(tuple(rng.uniform(-2, 2, 3)), tuple(scale), tuple(rgba), tuple(rot))
+    return bytes(out)

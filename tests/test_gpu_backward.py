@@ -166,9 +166,11 @@ def test_two_forwards_before_one_backward():
 
 
 def test_overwritten_gradients_equal_accumulated_ones():
-    """athx_grads.overwrite: gradients written into tensors full of NaN equal,
-    bit for bit, the same adjoint added into zeros; Gaussians no view draws
-    get zeros written."""
+    """athx_grads.overwrite: gradients written into tensors full of NaN equal
+    the same adjoint added into zeros, and Gaussians no view draws get zeros
+    written. Equal to the float atomics' rounding: the blend's adjoint sums
+    each Gaussian's screen-space gradient over pixels and tiles with float
+    atomic adds, whose order (and so last bits) differs between two runs."""
     print("backward overwrite", flush=True)
     cloud = synthetic.plain_1080p_cloud(3000)
     views = [camera.view_for(synthetic.front_camera(), 80, 64)]
@@ -183,8 +185,13 @@ def test_overwritten_gradients_equal_accumulated_ones():
     raster.render_backward(views, gc, RenderSettings(), targets, dl, written, overwrite=True)
     torch.cuda.synchronize()
     for k in names:
-        assert torch.equal(added[k], written[k]), k
+        assert bool(torch.isfinite(written[k]).all()), k
+        scale = float(added[k].abs().max())
+        assert scale > 0.0, k
+        assert float((written[k] - added[k]).abs().max()) <= 1e-4 * scale, k
     drawn = raster.frame_buffers()["touched"].cpu().numpy() > 0
     assert (~drawn).any() and drawn.any()
-    assert float(written["sh"][torch.from_numpy(~drawn).cuda()].abs().max()) == 0.0
+    undrawn = torch.from_numpy(~drawn).cuda()
+    for k in names:
+        assert float(written[k][undrawn].abs().max()) == 0.0, k
     raster.close()

@@ -117,3 +117,31 @@ def test_ingested_cloud_renders_bit_exact(raster, name):
                                s.sh.cpu().numpy().view(np.uint32) if s.sh is not None else None, s.rest_per_colour)
     view = camera.view_for(synthetic.front_camera(), 256, 192)
     check_frame(*render_packed(raster, host, [view]))
+
+
+@pytest.mark.parametrize("keep_faint", [False, True])
+def test_dot_splat_files_are_bit_exact(raster, tmp_path, keep_faint):
+    """.splat records (athx_splat_layout + athx_ingest_ply with the byte and
+    linear encodings) against the oracle, through a file on disk: the
+    reference's decode record first, then seeded records with faint, flat,
+    hard-surfel and zero-length ones."""
+    from tests.test_ply_oracle import ingest_splat_bytes
+
+    say(".splat ingest, keep_faint", keep_faint)
+    data = F.dot_splat_record() + F.dot_splat_file(50_001, 3)
+    path = tmp_path / "cloud.splat"
+    path.write_bytes(data)
+    cloud = plymod.read_splats(path, raster=raster, keep_faint=keep_faint)
+    torch.cuda.synchronize()
+    op, os_, _, oo = ingest_splat_bytes(data, keep_faint)
+    s = cloud.splats
+    assert cloud.vertices == 50_002 and s.count == len(op) and s.rest_per_colour == 0
+    assert np.array_equal(cloud.origin.cpu().numpy().view(np.uint32), oo)
+    assert np.array_equal(s.positions.cpu().numpy().view(np.uint32), op.view(np.uint32))
+    assert np.array_equal(s.shape.cpu().numpy().view(np.uint32), os_)
+    assert keep_faint or len(op) < 50_002
+    with pytest.raises(plymod._lib.AthxError) as e:
+        bad = tmp_path / "bad.splat"
+        bad.write_bytes(data[:-5])
+        plymod.read_splats(bad, raster=raster)
+    assert e.value.code == "InvalidArgument"

@@ -205,3 +205,72 @@ def test_quaternion_codes_round_trip_within_their_step():
         assert (w & 3) == (o & 3)
         for shift in (22, 12, 2):
             assert abs(int((w >> shift) & 1023) - int((o >> shift) & 1023)) <= 1
+
+
+# ---- .splat files ----
+
+def splat_layout(size: int):
+    layout = _lib.PlyLayout()
+    message = __import__("ctypes").create_string_buffer(256)
+    status = _lib.load().athx_splat_layout(size, __import__("ctypes").byref(layout), message, 256)
+    return status, layout, message.value.decode()
+
+
+def ingest_splat_bytes(data: bytes, keep_faint: bool = False):
+    status, layout, _ = splat_layout(len(data))
+    assert status == 0
+    fields = [(layout.offset[k], layout.type[k]) for k in
+              [layout.x, layout.y, layout.z, layout.opacity, *layout.scale, *layout.rot, *layout.dc]]
+    modes = (layout.opacity_mode, layout.scale_mode, layout.colour_mode, layout.rotation_mode)
+    return pyoracle.ingest(np.frombuffer(data, np.uint8), layout.count, layout.stride, fields, keep_faint, modes)
+
+
+def test_splat_layout_is_the_references_record():
+    status, layout, _ = splat_layout(96)
+    assert status == 0 and layout.count == 3 and layout.stride == 32 and layout.data_offset == 0
+    assert [layout.offset[k] for k in range(14)] == [0, 4, 8, 12, 16, 20, 24, 25, 26, 27, 28, 29, 30, 31]
+    assert [layout.type[k] for k in range(14)] == [0] * 6 + [2] * 8
+    assert (layout.x, layout.y, layout.z, list(layout.scale), list(layout.dc), layout.opacity, list(layout.rot)) == \
+        (0, 1, 2, [3, 4, 5], [6, 7, 8], 9, [10, 11, 12, 13])
+    assert (layout.opacity_mode, layout.scale_mode, layout.colour_mode, layout.rotation_mode) == (2, 1, 2, 1)
+    assert layout.rest_per_colour == 0
+
+
+@pytest.mark.parametrize("size", [0, 31, 33, 100])
+def test_splat_sizes_that_are_not_whole_records_are_refused(size):
+    status, _, message = splat_layout(size)
+    assert status == 3 and "32-byte .splat records" in message
+
+
+def test_oracle_meets_the_references_splat_decode_assertions():
+    """test_loading.cpp:168-199: byte opacity 255 -> 1, scales 0.25 .. 1,
+    w from byte 255 (0.992, normalised) -> 1, colour bytes 255 and 51 -> 1
+    and 0.2."""
+    positions, shape, sh, origin = ingest_splat_bytes(F.dot_splat_record())
+    assert len(positions) == 1 and list(origin) == [0]
+    v = unpack(positions, shape, sh, 0)
+    assert v[0:3] == [0.5, -0.5, 2.0]
+    assert v[3] == pytest.approx(1.0, abs=1e-6)
+    assert v[4] == pytest.approx(0.25, rel=2e-3) and v[6] == pytest.approx(1.0, rel=2e-3)
+    assert v[10] == pytest.approx(1.0, abs=1e-2)
+    assert v[11] == pytest.approx(1.0, abs=1e-3)
+    assert v[12] == pytest.approx(0.0, abs=1e-3)
+    assert v[13] == pytest.approx(0.2, abs=1e-3)
+
+
+def test_splat_decodes_flat_and_hard_axes_and_drops_faint_records():
+    data = b"".join([F.dot_splat_record(rgba=(10, 20, 30, 0)),            # alpha 0: too faint
+                     F.dot_splat_record(scale=(0.0, 0.5, 1.0)),            # a flat axis
+                     F.dot_splat_record(scale=(0.2, -8.0, 1.0)),           # a hard surfel's p = 8
+                     F.dot_splat_record(scale=(0.0, 0.0, 1.0)),            # two flat axes: invalid
+                     F.dot_splat_record(rot=(128, 128, 128, 128))])        # no length: the identity
+    positions, shape, sh, origin = ingest_splat_bytes(data)
+    assert list(origin) == [1, 2, 4]
+    half = lambda h: float(np.array([h], np.uint16).view(np.float16)[0])
+    # a flat axis: -inf, clamped into the half range as packing.slang's halfSafe does (exp of it is 0)
+    assert half(shape[0][1] & 0xFFFF) == -64992.0
+    # -(1024 + 64 (8 - 2)) as a half
+    assert half(shape[1][1] >> 16) == -(1024.0 + 64.0 * 6.0)
+    assert unpack(positions, shape, sh, 2)[7:11] == pytest.approx([0.0, 0.0, 0.0, 1.0], abs=1e-3)
+    _, _, _, origin_all = ingest_splat_bytes(data, keep_faint=True)
+    assert list(origin_all) == [0, 1, 2, 3, 4]
