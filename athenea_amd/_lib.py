@@ -357,6 +357,10 @@ SIGNATURES = {
         [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p,
          ctypes.c_void_p],
     ),
+    "athx_exclusive_scan_noalloc": (
+        ctypes.c_int,
+        [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p],
+    ),
     "athx_splat_layout": (
         ctypes.c_int,
         [ctypes.c_uint64, ctypes.POINTER(PlyLayout), ctypes.c_char_p, ctypes.c_uint64],

@@ -50,6 +50,16 @@ constexpr uint32_t kTotalExact = 5;  // the exact counts' total (copy_frame's of
 constexpr uint32_t kTotalWords = 6;
 constexpr uint64_t kMaxPairs = uint64_t{1} << 30;  // pairs and entries are 32-bit, with room
 
+// The scratch of athx_exclusive_scan_noalloc: the single-pass scan's
+// look-back status words for up to kNoAllocScanMax elements (two words a
+// partition of 4096 and two more), held by each device the library runs on
+// until the process ends. Host code never names it directly: athx_create
+// looks up its device address (hipGetSymbolAddress) and keeps it in the
+// context.
+constexpr uint32_t kNoAllocScanMax = 1u << 24;
+constexpr uint32_t kNoAllocScanWords = 2u * (1u + kNoAllocScanMax / 4096u);
+static __device__ __align__(256) uint32_t scratch[kNoAllocScanWords];
+
 uint32_t bits_for(uint32_t values) {
     // modules/render/src/FrameParams.h:18-24
     uint32_t bits = 1;
@@ -117,6 +127,9 @@ struct athx_ctx {
     };
     CounterCopy counter_ring[3];
     uint64_t counter_seq = 0;
+
+    // Device address of this device's `scratch` (athx_exclusive_scan_noalloc).
+    uint32_t* noalloc_scratch = nullptr;
 
     hipEvent_t ev[8] = {};
     // Recorded after each upload from the pinned staging views: the staging
@@ -1003,6 +1016,14 @@ ATHX_API athx_status athx_create(int device, athx_ctx** out) {
             return ATHX_DEVICE_FAILURE;
         }
     }
+    // This device's copy of the no-allocation scan's scratch, with the device
+    // current; and room for error messages, so a failing call of the
+    // no-allocation entry point does not grow the string.
+    if (hipGetSymbolAddress(reinterpret_cast<void**>(&ctx->noalloc_scratch), HIP_SYMBOL(scratch)) != hipSuccess ||
+        !ctx->noalloc_scratch) {
+        athx_destroy(ctx);
+        return ATHX_DEVICE_FAILURE;
+    }
     try {
         ctx->error.reserve(512);
     } catch (const std::bad_alloc&) {
@@ -1630,6 +1651,37 @@ ATHX_API athx_status athx_exclusive_scan(athx_ctx* ctx, const uint32_t* in, uint
     return fail(ctx, ATHX_OUT_OF_MEMORY, "host allocation failed");
 } catch (const std::exception& e) {
     return fail(ctx, ATHX_INTERNAL_ERROR, e.what());
+}
+
+ATHX_API athx_status athx_exclusive_scan_noalloc(athx_ctx* ctx, const uint32_t* in, uint32_t* out, uint64_t* total,
+                                                 uint32_t n, void* stream) {
+    if (!ctx) {
+        return ATHX_INVALID_ARGUMENT;
+    }
+    ctx->error.clear();
+    if ((n != 0 && (!in || !out)) || !total) {
+        ctx->error.assign("null buffer");
+        return ATHX_INVALID_ARGUMENT;
+    }
+    if (in == out) {
+        ctx->error.assign("input and output must be different buffers (PrefixSum.h:41-42)");
+        return ATHX_INVALID_ARGUMENT;
+    }
+    if (n > kNoAllocScanMax || scan_status_words(n) > kNoAllocScanWords) {
+        ctx->error.assign("athx_exclusive_scan_noalloc scans at most 2^24 elements (its static scratch)");
+        return ATHX_INVALID_ARGUMENT;
+    }
+    if (hipSetDevice(ctx->device) != hipSuccess) {
+        ctx->error.assign("set device");
+        return ATHX_DEVICE_FAILURE;
+    }
+    // The status words are cleared by a fill kernel inside launch_scan before
+    // the scan reads them, so whatever the last call left does not matter.
+    if (launch_scan(in, out, total, n, ctx->noalloc_scratch, static_cast<hipStream_t>(stream)) != hipSuccess) {
+        ctx->error.assign("scan launch failed");
+        return ATHX_DEVICE_FAILURE;
+    }
+    return ATHX_OK;
 }
 
 ATHX_API athx_status athx_sort_pairs(athx_ctx* ctx, uint64_t* keys, uint32_t* values, uint32_t n, uint32_t key_bits,

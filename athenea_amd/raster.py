@@ -761,6 +761,18 @@ class TileRasterizer:
                                                   _stream_handle(values.device)))
         return out, total
 
+    def exclusive_scan_noalloc(self, values: torch.Tensor, out: torch.Tensor, total: torch.Tensor) -> None:
+        """athx_exclusive_scan_noalloc: exclusive_scan into caller-given ``out``
+        (int32, values' length) and ``total`` (one int64), with nothing
+        allocated -- the library's static scratch; at most 2^24 values."""
+        for name, t, dtype in (("values", values, torch.int32), ("out", out, torch.int32), ("total", total, torch.int64)):
+            if t.dtype != dtype or not t.is_cuda or not t.is_contiguous():
+                raise ValueError(f"{name} must be a contiguous {dtype} device tensor")
+        if out.numel() != values.numel() or total.numel() < 1:
+            raise ValueError("out must match values, total hold one int64")
+        self._check(self._lib.athx_exclusive_scan_noalloc(self._ctx, _ptr(values), _ptr(out), _ptr(total),
+                                                          values.numel(), _stream_handle(values.device)))
+
     def sort_pairs(self, keys: torch.Tensor, values: torch.Tensor, key_bits: int) -> None:
         """RadixSort::sort in place: a stable sort by the low key_bits bits of
         int64 keys (uint64 bits), values (int32) carried along."""

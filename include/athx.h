@@ -562,6 +562,20 @@ ATHX_API athx_status athx_exclusive_scan(athx_ctx* ctx, const uint32_t* in, uint
                                          uint32_t n, void* stream);
 
 /*
+ * athx_exclusive_scan that allocates nothing. No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). Its look-back scratch is a static device array of the library
+ * (32 KiB on each GPU it runs on, held until the process ends), so n is at
+ * most ATHX_NOALLOC_SCAN_MAX (INVALID_ARGUMENT beyond). Calls that overlap
+ * on one GPU (two streams, or two contexts of one device) share that array
+ * and race: queue them on one stream. Same results as athx_exclusive_scan.
+ */
+#define ATHX_NOALLOC_SCAN_MAX (1u << 24)
+ATHX_API athx_status athx_exclusive_scan_noalloc(athx_ctx* ctx, const uint32_t* in, uint32_t* out, uint64_t* total,
+                                                 uint32_t n, void* stream);
+
+/*
  * Stable LSD radix sort of n (key, value) pairs by the low key_bits bits of
  * the key (1..64), onesweep: one histogram pass, then one pass per 8-bit
  * digit. The result is back in keys/values. Queued.
