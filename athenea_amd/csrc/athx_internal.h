@@ -348,6 +348,12 @@ hipError_t launch_sort(uint64_t* keys, uint32_t* values, const uint64_t* n_dev, 
                        uint32_t key_bits, const SortScratch& sc, hipStream_t s, bool* in_alt = nullptr);
 hipError_t launch_sort(uint32_t* keys, uint32_t* values, const uint64_t* n_dev, uint32_t cap, uint32_t lo_bit,
                        uint32_t key_bits, const SortScratch& sc, hipStream_t s, bool* in_alt = nullptr);
+// One more stable pass over a sorted (key, slot) array whose digit is the
+// slot's view, slot / per_view (2 <= nviews <= 256): the pairs end grouped by
+// view, each view's in the order they had. Reads the alternate buffers when
+// in_alt says the array is there, writes the other pair, and flips in_alt.
+hipError_t launch_view_pass(uint32_t* keys, uint32_t* values, const uint64_t* n_dev, uint32_t cap, uint32_t per_view,
+                            uint32_t nviews, const SortScratch& sc, hipStream_t s, bool& in_alt);
 // The compaction's block of slots: its first step counts each block's
 // drawn slots into a u64 a block at the head of its status words.
 constexpr uint32_t kCompactTile = 4096;

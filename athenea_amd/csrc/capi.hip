@@ -103,7 +103,11 @@ struct athx_ctx {
     uint32_t last_strips = 0, last_entry_cap = 0;
     uint32_t last_instances = 0;  // the clouds the last frame projected (athx_render_instances)
     bool last_under = false;      // the last frame's walks ended at an under layer
-    bool last_sorted_in_alt = false;  // the depth-sorted slots are in depth_alt_slots
+    bool last_sorted_in_alt = false;  // the ranked slots are in depth_alt_slots
+    // Where the frame's whole depth order stands: with several views the
+    // ranks are grouped by view (launch_view_pass) into the other buffers,
+    // and the exact offsets are scanned in the depth order itself.
+    bool depth_order_in_alt = false;
     uint64_t last_true_pairs = 0;  // the last frame's pair total (an async frame's once athx_frame_pairs ran)
     uint64_t pair_hint = 0;        // the pair capacity a frame takes before it knows: 1.25x the largest seen
     uint64_t entry_hint = 0;       // the same for strip entries
@@ -773,6 +777,17 @@ athx_status enqueue_front(athx_ctx* ctx, const std::vector<ProjectArgs>& pas, co
     ATHX_HIP(ctx, launch_sort(as<uint32_t>(ctx->depth_keys_sorted), as<uint32_t>(ctx->depth_slots),
                               totals + kTotalVisible, fs.slots, 0, kDepthBits, dsc, s, &in_alt),
              "depth sort");
+    // Several views: one more pass groups the ranks by view, each view's in
+    // depth order. A strip is one view's, so its entries -- and the pairs --
+    // keep the same order, but the emission then reads one view's records at
+    // a time, which the caches hold, where the frame's depth order reads them
+    // scattered over every view's.
+    ctx->depth_order_in_alt = in_alt;
+    if (fs.nviews >= 2 && fs.nviews <= 256 && fs.count >= 2) {
+        ATHX_HIP(ctx, launch_view_pass(as<uint32_t>(ctx->depth_keys_sorted), as<uint32_t>(ctx->depth_slots),
+                                       totals + kTotalVisible, fs.slots, fs.count, fs.nviews, dsc, s, in_alt),
+                 "view pass");
+    }
     ctx->last_sorted_in_alt = in_alt;
     if (timed) {
         ATHX_HIP(ctx, hipEventRecord(ctx->ev[2], s), "event");
@@ -1615,7 +1630,7 @@ ATHX_API athx_status athx_copy_frame(athx_ctx* ctx, athx_frame_part part, void* 
         if (part == ATHX_PART_OFFSETS) {
             uint64_t* totals = as<uint64_t>(ctx->totals);
             ATHX_HIP(ctx, launch_fill_u32(as<uint32_t>(ctx->exact_offsets), 0u, slots, s), "clear offsets");
-            ATHX_HIP(ctx, launch_gather_scan(as<uint32_t>(ctx->exact), sorted_slots_of(ctx, ctx->last_sorted_in_alt),
+            ATHX_HIP(ctx, launch_gather_scan(as<uint32_t>(ctx->exact), sorted_slots_of(ctx, ctx->depth_order_in_alt),
                                              totals + kTotalVisible, (uint32_t)slots, as<uint32_t>(ctx->exact_offsets),
                                              totals + kTotalExact, as<uint32_t>(ctx->exact_status), s),
                      "exact offsets");

@@ -308,10 +308,51 @@ __global__ void __launch_bounds__(kThreads) onesweep_kernel(const K* __restrict_
 // does and scatters from those slots. One more read of the keys a pass, but
 // no partition waits on another: on MI355X the look-back's coherent loads
 // and the ticket atomics cost more than that read.
-template <typename K>
-__global__ void __launch_bounds__(kThreads) upsweep_kernel(const K* __restrict__ kin,
-                                                           const uint64_t* __restrict__ n_dev, uint32_t cap,
-                                                           uint32_t shift, uint32_t width, uint32_t parts_cap,
+// Where a reduce-scan-scatter pass takes its digit from: bits of the key,
+// or (the view pass) the view a slot value belongs to, slot / per_view.
+struct KeyDigit {
+    static constexpr bool kFromValue = false;
+    uint32_t shift, mask;
+    template <typename K>
+    __device__ uint32_t operator()(K key, uint32_t) const {
+        return (uint32_t)(key >> shift) & mask;
+    }
+};
+
+struct ViewDigit {
+    static constexpr bool kFromValue = true;
+    uint32_t per_view;  // >= 2
+    uint32_t magic;     // floor(2^32 / per_view)
+    template <typename K>
+    __device__ uint32_t operator()(K, uint32_t slot) const {
+        // slot * magic / 2^32 lies within one below slot / per_view.
+        uint32_t q = (uint32_t)(((uint64_t)slot * magic) >> 32);
+        q += (uint64_t)(q + 1u) * per_view <= slot ? 1u : 0u;
+        return q;
+    }
+};
+
+// One add a digit a wavefront when every live lane holds the same digit
+// (the partitions of a nearly sorted pass), else one a lane.
+__device__ inline void hist_add(uint32_t* s_hist, uint32_t d, bool ok) {
+    const uint64_t live = __ballot(ok);
+    if (live == 0) {
+        return;
+    }
+    const uint32_t first = __builtin_amdgcn_readfirstlane(d);
+    if (__ballot(ok && d == first) == live) {
+        if ((uint32_t)__builtin_ctzll(live) == (threadIdx.x & 63u)) {
+            atomicAdd(&s_hist[first], (uint32_t)__popcll(live));
+        }
+    } else if (ok) {
+        atomicAdd(&s_hist[d], 1u);
+    }
+}
+
+template <typename K, typename D>
+__global__ void __launch_bounds__(kThreads) upsweep_kernel(const K* __restrict__ kin, const uint32_t* __restrict__ vin,
+                                                           const uint64_t* __restrict__ n_dev, uint32_t cap, D digit,
+                                                           uint32_t width, uint32_t parts_cap,
                                                            uint32_t* __restrict__ counts) {
     __shared__ uint32_t s_hist[kRadix];
     const uint32_t tid = threadIdx.x;
@@ -322,13 +363,15 @@ __global__ void __launch_bounds__(kThreads) upsweep_kernel(const K* __restrict__
     __syncthreads();
     const uint32_t part_first = part * kPart;
     if (part_first < n) {
-        const uint32_t dmask = radix - 1u;
 #pragma unroll
         for (uint32_t j = 0; j < kItems; ++j) {
             const uint32_t at = part_first + j * kThreads + tid;
-            if (at < n) {
-                atomicAdd(&s_hist[(uint32_t)(kin[at] >> shift) & dmask], 1u);
+            const bool ok = at < n;
+            uint32_t d = 0;
+            if (ok) {
+                d = D::kFromValue ? digit(K{}, vin[at]) : digit(kin[at], 0u);
             }
+            hist_add(s_hist, d, ok);
         }
     }
     __syncthreads();
@@ -337,12 +380,12 @@ __global__ void __launch_bounds__(kThreads) upsweep_kernel(const K* __restrict__
     }
 }
 
-template <typename K>
+template <typename K, typename D>
 __global__ void __launch_bounds__(kThreads) downsweep_kernel(const K* __restrict__ kin,
                                                              const uint32_t* __restrict__ vin,
                                                              K* __restrict__ kout, uint32_t* __restrict__ vout,
                                                              const uint64_t* __restrict__ n_dev, uint32_t cap,
-                                                             uint32_t shift, uint32_t width, uint32_t parts_cap,
+                                                             D digit, uint32_t width, uint32_t parts_cap,
                                                              const uint32_t* __restrict__ offsets) {
     __shared__ K s_keys[kPart];
     __shared__ uint32_t s_vals[kPart];
@@ -382,7 +425,7 @@ __global__ void __launch_bounds__(kThreads) downsweep_kernel(const K* __restrict
     for (uint32_t j = 0; j < kItems; ++j) {
         const uint32_t at = wave_first + j * 64 + lane;
         const bool ok = at < n;
-        const uint32_t d = (uint32_t)(key[j] >> shift) & dmask;
+        const uint32_t d = digit(key[j], val[j]);
         const uint64_t m = match_digit(d, width, ok);
         const uint32_t before = ok ? my_hist[d] : 0u;
         __builtin_amdgcn_wave_barrier();
@@ -410,7 +453,7 @@ __global__ void __launch_bounds__(kThreads) downsweep_kernel(const K* __restrict
     for (uint32_t j = 0; j < kItems; ++j) {
         const uint32_t at = wave_first + j * 64 + lane;
         if (at < n) {
-            const uint32_t dj = (uint32_t)(key[j] >> shift) & dmask;
+            const uint32_t dj = digit(key[j], val[j]);
             const uint32_t pos = s_block_start[dj] + s_wave_hist[wave * kRadix + dj] + rank[j];
             s_keys[pos] = key[j];
             s_vals[pos] = val[j];
@@ -420,10 +463,11 @@ __global__ void __launch_bounds__(kThreads) downsweep_kernel(const K* __restrict
     const uint32_t valid = n - part_first < kPart ? n - part_first : kPart;
     for (uint32_t i = tid; i < valid; i += kThreads) {
         const K k = s_keys[i];
-        const uint32_t dk = (uint32_t)(k >> shift) & dmask;
+        const uint32_t v = s_vals[i];
+        const uint32_t dk = digit(k, v);
         const uint32_t dst = s_digit_base[dk] + (i - s_block_start[dk]);
         kout[dst] = k;
-        vout[dst] = s_vals[i];
+        vout[dst] = v;
     }
 }
 
@@ -482,15 +526,16 @@ hipError_t sort_impl(K* keys, uint32_t* values, const uint64_t* n_dev, uint32_t 
         for (uint32_t p = 0; p < passes; ++p) {
             uint32_t shift, width;
             pass_digit(p, lo_bit, key_bits - lo_bit, passes, shift, width);
-            hipLaunchKernelGGL(upsweep_kernel<K>, dim3(parts), dim3(kThreads), 0, s, kin, n_dev, cap, shift, width,
-                               parts, sc.counts);
+            const KeyDigit digit{shift, (1u << width) - 1u};
+            hipLaunchKernelGGL((upsweep_kernel<K, KeyDigit>), dim3(parts), dim3(kThreads), 0, s, kin, vin, n_dev, cap,
+                               digit, width, parts, sc.counts);
             if (hipError_t e = launch_scan(sc.counts, sc.offsets, sc.scan_total, (1u << width) * parts,
                                            sc.scan_status, s);
                 e != hipSuccess) {
                 return e;
             }
-            hipLaunchKernelGGL(downsweep_kernel<K>, dim3(parts), dim3(kThreads), 0, s, kin, vin, kout, vout, n_dev,
-                               cap, shift, width, parts, sc.offsets);
+            hipLaunchKernelGGL((downsweep_kernel<K, KeyDigit>), dim3(parts), dim3(kThreads), 0, s, kin, vin, kout,
+                               vout, n_dev, cap, digit, width, parts, sc.offsets);
             K* tk = kin; kin = kout; kout = tk;
             uint32_t* tv = vin; vin = vout; vout = tv;
         }
@@ -555,6 +600,36 @@ hipError_t launch_sort(uint64_t* keys, uint32_t* values, const uint64_t* n_dev, 
 hipError_t launch_sort(uint32_t* keys, uint32_t* values, const uint64_t* n_dev, uint32_t cap, uint32_t lo_bit,
                        uint32_t key_bits, const SortScratch& sc, hipStream_t s, bool* in_alt) {
     return sort_impl(keys, values, n_dev, cap, lo_bit, key_bits, sc, s, in_alt);
+}
+
+hipError_t launch_view_pass(uint32_t* keys, uint32_t* values, const uint64_t* n_dev, uint32_t cap, uint32_t per_view,
+                            uint32_t nviews, const SortScratch& sc, hipStream_t s, bool& in_alt) {
+    if (cap < 2 || nviews < 2) {
+        return hipSuccess;
+    }
+    if (nviews > kRadix || per_view < 2 || cap > kValueMask) {
+        return hipErrorInvalidValue;
+    }
+    uint32_t width = 1;
+    while ((1u << width) < nviews) {
+        ++width;
+    }
+    const uint32_t parts = sort_partitions(cap);
+    uint32_t* kin = in_alt ? static_cast<uint32_t*>(sc.keys_alt) : keys;
+    uint32_t* vin = in_alt ? sc.values_alt : values;
+    uint32_t* kout = in_alt ? keys : static_cast<uint32_t*>(sc.keys_alt);
+    uint32_t* vout = in_alt ? values : sc.values_alt;
+    const ViewDigit digit{per_view, (uint32_t)((1ull << 32) / per_view)};
+    hipLaunchKernelGGL((upsweep_kernel<uint32_t, ViewDigit>), dim3(parts), dim3(kThreads), 0, s, kin, vin, n_dev, cap,
+                       digit, width, parts, sc.counts);
+    if (hipError_t e = launch_scan(sc.counts, sc.offsets, sc.scan_total, (1u << width) * parts, sc.scan_status, s);
+        e != hipSuccess) {
+        return e;
+    }
+    hipLaunchKernelGGL((downsweep_kernel<uint32_t, ViewDigit>), dim3(parts), dim3(kThreads), 0, s, kin, vin, kout, vout,
+                       n_dev, cap, digit, width, parts, sc.offsets);
+    in_alt = !in_alt;
+    return hipGetLastError();
 }
 
 }  // namespace athx

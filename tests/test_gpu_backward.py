@@ -148,8 +148,12 @@ def test_two_forwards_before_one_backward():
     c1, _ = rasterize(*separate, [views[1]], raster)
     (c1[0][..., :3] * 0.5).sum().backward()
     torch.cuda.synchronize()
+    # Equal to the float atomics' rounding: the adjoint sums each Gaussian's
+    # screen-space gradient with float atomic adds in no fixed order, about
+    # 1e-6 of the terms' magnitudes (which a cancelling sum can leave larger
+    # than the result) between two runs.
     for a, b in zip(joint, separate):
-        assert torch.allclose(a.grad, b.grad, rtol=1e-5, atol=1e-7 * float(b.grad.abs().max())), \
+        assert torch.allclose(a.grad, b.grad, rtol=1e-5, atol=1e-5 * float(b.grad.abs().max())), \
             float((a.grad - b.grad).abs().max())
     # The C ABI itself: a backward naming a replaced frame is refused.
     gc = GaussianCloud.from_numpy(cloud, "cuda:0")
