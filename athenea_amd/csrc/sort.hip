@@ -37,7 +37,8 @@ __device__ inline void store_word(uint32_t* p, uint32_t v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Exclusive scan of one uint32 a thread over the 256 threads of the block.
+// Exclusive scan of one uint32 a thread over the kW wavefronts of the block.
+template <uint32_t kW = kWaves>
 __device__ inline uint32_t block_exclusive_scan(uint32_t x, uint32_t* s_tmp, uint32_t& total) {
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     uint32_t incl = x;
@@ -54,7 +55,7 @@ __device__ inline uint32_t block_exclusive_scan(uint32_t x, uint32_t* s_tmp, uin
     __syncthreads();
     uint32_t before = 0, all = 0;
 #pragma unroll
-    for (uint32_t w = 0; w < kWaves; ++w) {
+    for (uint32_t w = 0; w < kW; ++w) {
         const uint32_t t = s_tmp[w];
         before += w < wave ? t : 0u;
         all += t;
@@ -349,23 +350,27 @@ __device__ inline void hist_add(uint32_t* s_hist, uint32_t d, bool ok) {
     }
 }
 
-template <typename K, typename D>
-__global__ void __launch_bounds__(kThreads) upsweep_kernel(const K* __restrict__ kin, const uint32_t* __restrict__ vin,
-                                                           const uint64_t* __restrict__ n_dev, uint32_t cap, D digit,
-                                                           uint32_t width, uint32_t parts_cap,
-                                                           uint32_t* __restrict__ counts) {
+template <typename K, typename D, uint32_t kT>
+__global__ void __launch_bounds__(kT) upsweep_kernel(const K* __restrict__ kin, const uint32_t* __restrict__ vin,
+                                                     const uint64_t* __restrict__ n_dev, uint32_t cap, D digit,
+                                                     uint32_t width, uint32_t parts_cap,
+                                                     uint32_t* __restrict__ counts) {
+    static_assert(kT >= kRadix && kT % 64 == 0, "a thread a digit");
+    constexpr uint32_t kP = kT * kItems;
     __shared__ uint32_t s_hist[kRadix];
     const uint32_t tid = threadIdx.x;
     const uint32_t part = blockIdx.x;
     const uint32_t n = count_of(n_dev, cap);
     const uint32_t radix = 1u << width;
-    s_hist[tid] = 0;
+    if (tid < kRadix) {
+        s_hist[tid] = 0;
+    }
     __syncthreads();
-    const uint32_t part_first = part * kPart;
+    const uint32_t part_first = part * kP;
     if (part_first < n) {
 #pragma unroll
         for (uint32_t j = 0; j < kItems; ++j) {
-            const uint32_t at = part_first + j * kThreads + tid;
+            const uint32_t at = part_first + j * kT + tid;
             const bool ok = at < n;
             uint32_t d = 0;
             if (ok) {
@@ -380,33 +385,35 @@ __global__ void __launch_bounds__(kThreads) upsweep_kernel(const K* __restrict__
     }
 }
 
-template <typename K, typename D>
-__global__ void __launch_bounds__(kThreads) downsweep_kernel(const K* __restrict__ kin,
-                                                             const uint32_t* __restrict__ vin,
-                                                             K* __restrict__ kout, uint32_t* __restrict__ vout,
-                                                             const uint64_t* __restrict__ n_dev, uint32_t cap,
-                                                             D digit, uint32_t width, uint32_t parts_cap,
-                                                             const uint32_t* __restrict__ offsets) {
-    __shared__ K s_keys[kPart];
-    __shared__ uint32_t s_vals[kPart];
-    __shared__ uint32_t s_wave_hist[kWaves * kRadix];
+template <typename K, typename D, uint32_t kT>
+__global__ void __launch_bounds__(kT) downsweep_kernel(const K* __restrict__ kin, const uint32_t* __restrict__ vin,
+                                                       K* __restrict__ kout, uint32_t* __restrict__ vout,
+                                                       const uint64_t* __restrict__ n_dev, uint32_t cap, D digit,
+                                                       uint32_t width, uint32_t parts_cap,
+                                                       const uint32_t* __restrict__ offsets) {
+    static_assert(kT >= kRadix && kT % 64 == 0, "a thread a digit");
+    constexpr uint32_t kW = kT / 64;
+    constexpr uint32_t kP = kT * kItems;
+    __shared__ K s_keys[kP];
+    __shared__ uint32_t s_vals[kP];
+    __shared__ uint32_t s_wave_hist[kW * kRadix];
     __shared__ uint32_t s_block_start[kRadix];
     __shared__ uint32_t s_digit_base[kRadix];
-    __shared__ uint32_t s_tmp[kWaves];
+    __shared__ uint32_t s_tmp[kW];
 
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = tid & 63u;
     const uint32_t wave = tid >> 6;
-    for (uint32_t i = tid; i < kWaves * kRadix; i += kThreads) {
+    for (uint32_t i = tid; i < kW * kRadix; i += kT) {
         s_wave_hist[i] = 0;
     }
     __syncthreads();
     const uint32_t n = count_of(n_dev, cap);
     const uint32_t part = blockIdx.x;
-    if (part >= div_up(n, kPart)) {
+    if (part >= div_up(n, kP)) {
         return;
     }
-    const uint32_t part_first = part * kPart;
+    const uint32_t part_first = part * kP;
     const uint32_t wave_first = part_first + wave * (64 * kItems);
     K key[kItems];
     uint32_t val[kItems];
@@ -437,17 +444,23 @@ __global__ void __launch_bounds__(kThreads) downsweep_kernel(const K* __restrict
         __builtin_amdgcn_wave_barrier();
     }
     __syncthreads();
+    // A thread a digit (threads past the radix scan zeros).
     const uint32_t d = tid;
     uint32_t count = 0;
+    if (d < kRadix) {
 #pragma unroll
-    for (uint32_t w = 0; w < kWaves; ++w) {
-        const uint32_t c = s_wave_hist[w * kRadix + d];
-        s_wave_hist[w * kRadix + d] = count;
-        count += c;
+        for (uint32_t w = 0; w < kW; ++w) {
+            const uint32_t c = s_wave_hist[w * kRadix + d];
+            s_wave_hist[w * kRadix + d] = count;
+            count += c;
+        }
     }
     uint32_t unused;
-    s_block_start[d] = block_exclusive_scan(count, s_tmp, unused);
-    s_digit_base[d] = d <= dmask ? offsets[(size_t)d * parts_cap + part] : 0u;
+    const uint32_t start = block_exclusive_scan<kW>(count, s_tmp, unused);
+    if (d < kRadix) {
+        s_block_start[d] = start;
+        s_digit_base[d] = d <= dmask ? offsets[(size_t)d * parts_cap + part] : 0u;
+    }
     __syncthreads();
 #pragma unroll
     for (uint32_t j = 0; j < kItems; ++j) {
@@ -460,8 +473,8 @@ __global__ void __launch_bounds__(kThreads) downsweep_kernel(const K* __restrict
         }
     }
     __syncthreads();
-    const uint32_t valid = n - part_first < kPart ? n - part_first : kPart;
-    for (uint32_t i = tid; i < valid; i += kThreads) {
+    const uint32_t valid = n - part_first < kP ? n - part_first : kP;
+    for (uint32_t i = tid; i < valid; i += kT) {
         const K k = s_keys[i];
         const uint32_t v = s_vals[i];
         const uint32_t dk = digit(k, v);
@@ -503,6 +516,15 @@ __global__ void copy_back_kernel(K* __restrict__ keys, uint32_t* __restrict__ va
     }
 }
 
+// The reduce-scan-scatter passes rank 8192 32-bit keys a workgroup of 512
+// (74 KB of LDS, two workgroups a CU): twice the onesweep partition, so each
+// digit's run out of a partition is twice as long and the digit-count scan
+// half as long. 64-bit keys keep 4096 (their keys alone take 64 KB).
+template <typename K>
+constexpr uint32_t rs_threads() {
+    return sizeof(K) == 4 ? 512u : 256u;
+}
+
 template <typename K>
 hipError_t sort_impl(K* keys, uint32_t* values, const uint64_t* n_dev, uint32_t cap, uint32_t lo_bit,
                      uint32_t key_bits, const SortScratch& sc, hipStream_t s, bool* in_alt) {
@@ -519,6 +541,8 @@ hipError_t sort_impl(K* keys, uint32_t* values, const uint64_t* n_dev, uint32_t 
     const uint32_t parts = sort_partitions(cap);
     const uint32_t spread = parts < 2048 ? parts : 2048;
     if (cap >= kReduceScanMin) {
+        constexpr uint32_t kT = rs_threads<K>();
+        const uint32_t rparts = div_up(cap, kT * kItems);
         K* kin = keys;
         uint32_t* vin = values;
         K* kout = static_cast<K*>(sc.keys_alt);
@@ -527,15 +551,15 @@ hipError_t sort_impl(K* keys, uint32_t* values, const uint64_t* n_dev, uint32_t 
             uint32_t shift, width;
             pass_digit(p, lo_bit, key_bits - lo_bit, passes, shift, width);
             const KeyDigit digit{shift, (1u << width) - 1u};
-            hipLaunchKernelGGL((upsweep_kernel<K, KeyDigit>), dim3(parts), dim3(kThreads), 0, s, kin, vin, n_dev, cap,
-                               digit, width, parts, sc.counts);
-            if (hipError_t e = launch_scan(sc.counts, sc.offsets, sc.scan_total, (1u << width) * parts,
+            hipLaunchKernelGGL((upsweep_kernel<K, KeyDigit, kT>), dim3(rparts), dim3(kT), 0, s, kin, vin, n_dev, cap,
+                               digit, width, rparts, sc.counts);
+            if (hipError_t e = launch_scan(sc.counts, sc.offsets, sc.scan_total, (1u << width) * rparts,
                                            sc.scan_status, s);
                 e != hipSuccess) {
                 return e;
             }
-            hipLaunchKernelGGL((downsweep_kernel<K, KeyDigit>), dim3(parts), dim3(kThreads), 0, s, kin, vin, kout,
-                               vout, n_dev, cap, digit, width, parts, sc.offsets);
+            hipLaunchKernelGGL((downsweep_kernel<K, KeyDigit, kT>), dim3(rparts), dim3(kT), 0, s, kin, vin, kout,
+                               vout, n_dev, cap, digit, width, rparts, sc.offsets);
             K* tk = kin; kin = kout; kout = tk;
             uint32_t* tv = vin; vin = vout; vout = tv;
         }
@@ -614,19 +638,20 @@ hipError_t launch_view_pass(uint32_t* keys, uint32_t* values, const uint64_t* n_
     while ((1u << width) < nviews) {
         ++width;
     }
-    const uint32_t parts = sort_partitions(cap);
+    constexpr uint32_t kT = rs_threads<uint32_t>();
+    const uint32_t parts = div_up(cap, kT * kItems);
     uint32_t* kin = in_alt ? static_cast<uint32_t*>(sc.keys_alt) : keys;
     uint32_t* vin = in_alt ? sc.values_alt : values;
     uint32_t* kout = in_alt ? keys : static_cast<uint32_t*>(sc.keys_alt);
     uint32_t* vout = in_alt ? values : sc.values_alt;
     const ViewDigit digit{per_view, (uint32_t)((1ull << 32) / per_view)};
-    hipLaunchKernelGGL((upsweep_kernel<uint32_t, ViewDigit>), dim3(parts), dim3(kThreads), 0, s, kin, vin, n_dev, cap,
+    hipLaunchKernelGGL((upsweep_kernel<uint32_t, ViewDigit, kT>), dim3(parts), dim3(kT), 0, s, kin, vin, n_dev, cap,
                        digit, width, parts, sc.counts);
     if (hipError_t e = launch_scan(sc.counts, sc.offsets, sc.scan_total, (1u << width) * parts, sc.scan_status, s);
         e != hipSuccess) {
         return e;
     }
-    hipLaunchKernelGGL((downsweep_kernel<uint32_t, ViewDigit>), dim3(parts), dim3(kThreads), 0, s, kin, vin, kout, vout,
+    hipLaunchKernelGGL((downsweep_kernel<uint32_t, ViewDigit, kT>), dim3(parts), dim3(kT), 0, s, kin, vin, kout, vout,
                        n_dev, cap, digit, width, parts, sc.offsets);
     in_alt = !in_alt;
     return hipGetLastError();
