@@ -1330,10 +1330,18 @@ __global__ void __launch_bounds__(kSegThreads) strip_mask_count_kernel(StripBins
     for (uint32_t t = 0; t < kStripTiles; ++t) {
         count[t] = 0;
     }
+    // The keys read together (a lane past the segment reads its last key and
+    // counts nothing), then counted.
+    uint32_t key[kSegPer];
 #pragma unroll
     for (uint32_t j = 0; j < kSegPer; ++j) {
         const uint32_t e = d0.x + j * kSegThreads + threadIdx.x;
-        const uint32_t mask = e < d0.y ? b.strip_keys[e] >> 16 : 0u;
+        key[j] = b.strip_keys[e < d0.y ? e : d0.y - 1u];
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < kSegPer; ++j) {
+        const uint32_t e = d0.x + j * kSegThreads + threadIdx.x;
+        const uint32_t mask = e < d0.y ? key[j] >> 16 : 0u;
         if (e < d0.y) {
             b.masks[e] = (uint16_t)mask;
         }

@@ -27,6 +27,8 @@
 //                device: the pair total and every (tile, segment)'s start.
 #include "athx_internal.h"
 
+#include <type_traits>
+
 namespace athx {
 namespace {
 
@@ -132,17 +134,64 @@ struct EntryScan {
     uint32_t* offsets;  // [ranks / 64]
     const uint64_t* n_dev;
     uint32_t cap;
+    // Read in two steps (every key first, then the rare 255s' slots), so a
+    // partition's key loads are all in flight at once.
+    static constexpr bool kTwoStep = true;
     __device__ uint32_t size() const { return *n_dev < cap ? (uint32_t)*n_dev : cap; }
-    __device__ uint32_t load(uint32_t k) const {
-        const uint32_t c = sorted_keys[k] >> 24;
+    __device__ uint32_t raw(uint32_t k) const { return sorted_keys[k]; }
+    __device__ uint32_t resolve(uint32_t k, uint32_t key) const {
+        const uint32_t c = key >> 24;
         return c != 255u ? c : touched[sorted_slots[k]];
     }
+    __device__ uint32_t load(uint32_t k) const { return resolve(k, raw(k)); }
     __device__ void store(uint32_t k, uint32_t, uint64_t excl) const {
         if ((k & 63u) == 0) {
             offsets[k >> 6] = (uint32_t)excl;
         }
     }
 };
+
+// Policies whose load splits into a raw read and a resolve (kTwoStep).
+template <typename P, typename = void>
+struct TwoStep {
+    static constexpr bool value = false;
+};
+template <typename P>
+struct TwoStep<P, std::void_t<decltype(P::kTwoStep)>> {
+    static constexpr bool value = P::kTwoStep;
+};
+
+// Elements at(0) .. at(kItems - 1) (0 past n): a two-step policy's raw reads
+// are issued together, at in-bounds indices, before any is resolved.
+template <typename Policy, typename At>
+__device__ inline void load_items(const Policy& pol, uint32_t n, At at, typename Policy::Value (&v)[kItems]) {
+    if constexpr (TwoStep<Policy>::value) {
+        if (n == 0) {
+#pragma unroll
+            for (uint32_t k = 0; k < kItems; ++k) {
+                v[k] = 0;
+            }
+            return;
+        }
+        uint32_t r[kItems];
+#pragma unroll
+        for (uint32_t k = 0; k < kItems; ++k) {
+            const uint64_t e = at(k);
+            r[k] = pol.raw(e < n ? (uint32_t)e : n - 1u);
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < kItems; ++k) {
+            const uint64_t e = at(k);
+            v[k] = e < n ? pol.resolve((uint32_t)e, r[k]) : 0u;
+        }
+    } else {
+#pragma unroll
+        for (uint32_t k = 0; k < kItems; ++k) {
+            const uint64_t e = at(k);
+            v[k] = e < n ? pol.load((uint32_t)e) : 0u;
+        }
+    }
+}
 
 struct SegScan {
     using Value = uint32_t;
@@ -173,11 +222,12 @@ template <typename Policy>
 __device__ inline void load_partition(const Policy& pol, uint64_t part_first, uint32_t n,
                                       typename Policy::Value (&v)[kItems], typename Policy::Value* s_x) {
     const uint32_t tid = threadIdx.x;
+    typename Policy::Value x[kItems];
+    load_items(pol, n, [&](uint32_t k) { return part_first + k * kThreads + tid; }, x);
 #pragma unroll
     for (uint32_t k = 0; k < kItems; ++k) {
         const uint32_t r = k * kThreads + tid;
-        const uint64_t e = part_first + r;
-        s_x[(r >> 4) * (kItems + 1) + (r & 15u)] = e < n ? pol.load((uint32_t)e) : 0u;
+        s_x[(r >> 4) * (kItems + 1) + (r & 15u)] = x[k];
     }
     __syncthreads();
 #pragma unroll
@@ -213,10 +263,7 @@ __global__ void __launch_bounds__(kThreads) scan_kernel(Policy pol, uint64_t* __
     // Each lane's run read directly: at the sizes this kernel takes, a
     // partition's lines stay in cache between a lane's loads.
     typename Policy::Value v[kItems];
-#pragma unroll
-    for (uint32_t k = 0; k < kItems; ++k) {
-        v[k] = first + k < n ? pol.load((uint32_t)(first + k)) : 0u;
-    }
+    load_items(pol, n, [&](uint32_t k) { return first + k; }, v);
     uint64_t mine = 0;
 #pragma unroll
     for (uint32_t k = 0; k < kItems; ++k) {
@@ -319,11 +366,12 @@ __global__ void __launch_bounds__(kThreads) partition_sum_kernel(Policy pol, uin
     }
     // The sum in any order: rows of kThreads consecutive elements a load.
     const uint64_t first = (uint64_t)part * kTile + threadIdx.x;
+    typename Policy::Value x[kItems];
+    load_items(pol, n, [&](uint32_t k) { return first + (uint64_t)k * kThreads; }, x);
     uint64_t mine = 0;
 #pragma unroll
     for (uint32_t k = 0; k < kItems; ++k) {
-        const uint64_t e = first + (uint64_t)k * kThreads;
-        mine += e < n ? (uint64_t)pol.load((uint32_t)e) : 0u;
+        mine += (uint64_t)x[k];
     }
     mine = wave_sum(mine);
     if ((threadIdx.x & 63u) == 0) {

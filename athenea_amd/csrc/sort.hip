@@ -368,15 +368,24 @@ __global__ void __launch_bounds__(kT) upsweep_kernel(const K* __restrict__ kin, 
     __syncthreads();
     const uint32_t part_first = part * kP;
     if (part_first < n) {
+        // Every load issued before the first add, so the adds wait on them
+        // once: a lane past the end reads the last key (in bounds, unbranched)
+        // and adds nothing.
+        K key[kItems];
+        uint32_t val[kItems];
 #pragma unroll
         for (uint32_t j = 0; j < kItems; ++j) {
-            const uint32_t at = part_first + j * kT + tid;
-            const bool ok = at < n;
-            uint32_t d = 0;
-            if (ok) {
-                d = D::kFromValue ? digit(K{}, vin[at]) : digit(kin[at], 0u);
+            const uint32_t at = min(part_first + j * kT + tid, n - 1u);
+            if constexpr (D::kFromValue) {
+                val[j] = vin[at];
+            } else {
+                key[j] = kin[at];
             }
-            hist_add(s_hist, d, ok);
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < kItems; ++j) {
+            const bool ok = part_first + j * kT + tid < n;
+            hist_add(s_hist, D::kFromValue ? digit(K{}, val[j]) : digit(key[j], 0u), ok);
         }
     }
     __syncthreads();
