@@ -53,6 +53,22 @@ __device__ inline void stage(LdsBatch& s, uint32_t k, const float* records, uint
     s.colour[k] = make_float4(col.x, col.y, col.z, conic.y);
 }
 
+// A pair's record words as stage() reads them, and their LDS form.
+struct Staged {
+    float4 screen, conic, col;
+};
+
+__device__ inline Staged fetch(const float* records, uint32_t slot) {
+    const float4* rec = reinterpret_cast<const float4*>(records + (size_t)slot * kRecordFloats);
+    return Staged{rec[0], rec[1], rec[2]};
+}
+
+__device__ inline void put(LdsBatch& s, uint32_t k, const Staged& r) {
+    s.xy_cut_op[k] = make_float4(r.screen.x, r.screen.y, r.screen.w, r.conic.w);
+    s.conic[k] = make_float4(r.conic.x, 2.0f * r.conic.y, r.conic.z, r.screen.z);
+    s.colour[k] = make_float4(r.col.x, r.col.y, r.col.z, r.conic.y);
+}
+
 // e^(-power / 2) for a drawn pair: power <= the cut power <= 9, and above
 // a tiny negative rounding floor for a positive definite conic, so the
 // argument lies in [-4.5, 0+] where athx_expf_core equals athx_expf; the
@@ -133,7 +149,7 @@ __device__ inline uint64_t band_mask(const LdsBatch& s, const Band& B, uint32_t 
     }
 }
 
-template <uint32_t TILE>
+template <uint32_t TILE, bool kPiped>
 __global__ void __launch_bounds__(TILE * TILE) blend_kernel(const DView* __restrict__ views,
                                                             const uint32_t* __restrict__ tile_bases, uint32_t nviews,
                                                             uint32_t batch, const uint32_t* __restrict__ ranges,
@@ -158,15 +174,39 @@ __global__ void __launch_bounds__(TILE * TILE) blend_kernel(const DView* __restr
     bool done = !inside;
     const uint32_t lane = tid & 63u;
     const Band band = wave_band<TILE>(V, tx, ty, tid >> 6);
+    // With a batch of at most a record a thread (kPiped), the staging is
+    // pipelined:
+    // while a batch is walked, each thread already reads its record of the
+    // next batch (its slot read during the batch before) and its slot of the
+    // batch after, so a batch's staging waits on no memory.
+    // The reads are unconditional (a thread with no pair reads the tile's
+    // last one again, and its record is never staged), so no branch holds
+    // them back.
+    Staged ahead{};
+    uint32_t slot_ahead = 0;
+    if (kPiped && start < end) {
+        ahead = fetch(records, pair_slots[min(start + tid, end - 1u)]);
+        slot_ahead = pair_slots[min(start + batch + tid, end - 1u)];
+    }
     for (uint32_t first = start; first < end; first += batch) {
         if (!__syncthreads_or(!done)) {
             break;
         }
         const uint32_t count = end - first < batch ? end - first : batch;
-        for (uint32_t k = tid; k < count; k += kThreads) {
-            stage(s, k, records, pair_slots[first + k]);
+        if constexpr (kPiped) {
+            if (tid < count) {
+                put(s, tid, ahead);
+            }
+        } else {
+            for (uint32_t k = tid; k < count; k += kThreads) {
+                stage(s, k, records, pair_slots[first + k]);
+            }
         }
         __syncthreads();
+        if constexpr (kPiped) {
+            ahead = fetch(records, slot_ahead);
+            slot_ahead = pair_slots[min(first + 2u * batch + tid, end - 1u)];
+        }
         for (uint32_t c0 = 0; c0 < count && __ballot(!done) != 0; c0 += 64) {
           for (uint64_t mm = band_mask<TILE>(s, band, c0, count, lane); mm != 0 && !done; mm &= mm - 1ull) {
             const uint32_t j = c0 + (uint32_t)__builtin_ctzll(mm);
@@ -698,7 +738,7 @@ __device__ inline float wave_sum8(const float v[8], uint32_t lane) {
     return x;
 }
 
-template <uint32_t TILE>
+template <uint32_t TILE, bool kPiped>
 __global__ void __launch_bounds__(TILE * TILE) blend_backward_kernel(
     const DView* __restrict__ views, const uint32_t* __restrict__ tile_bases, uint32_t nviews, uint32_t batch,
     const uint32_t* __restrict__ ranges, const uint32_t* __restrict__ pair_slots, const float* __restrict__ records,
@@ -758,18 +798,51 @@ __global__ void __launch_bounds__(TILE * TILE) blend_backward_kernel(
     // A wavefront walks only the pairs below its own pixels' furthest.
     const uint32_t wave_last = s_wave_last[tid >> 6];
     const Band band = wave_band<TILE>(V, tx, ty, tid >> 6);
+    // kPiped: the staging runs a batch ahead of the walk, as in the forward
+    // blend -- while a batch is walked each thread reads its record of the
+    // next (lower) batch and its slot of the one after. The reads are
+    // unconditional, at in-bounds pairs of the tile (the tile has max_last
+    // pairs or more).
+    const auto pair_of = [&](uint32_t hi_b) {
+        const uint32_t lo_b = hi_b > batch ? hi_b - batch : 0u;
+        const uint32_t k = lo_b + tid;
+        return start + (k < hi_b ? k : (hi_b != 0u ? hi_b - 1u : 0u));
+    };
+    uint32_t slot_now = 0, slot_ahead = 0;
+    Staged rec_now{};
+    if (kPiped && max_last > 0) {
+        slot_now = pair_slots[pair_of(max_last)];
+        rec_now = fetch(records, slot_now);
+        const uint32_t hi1 = max_last > batch ? max_last - batch : 0u;
+        slot_ahead = pair_slots[pair_of(hi1)];
+    }
     for (uint32_t hi = max_last; hi > 0;) {
         const uint32_t lo = hi > batch ? hi - batch : 0;
         const uint32_t count = hi - lo;
-        for (uint32_t k = tid; k < count; k += kThreads) {
-            const uint32_t slot = pair_slots[start + lo + k];
-            s_slot[k] = slot;
-            stage(s, k, records, slot);
+        if constexpr (kPiped) {
+            if (tid < count) {
+                s_slot[tid] = slot_now;
+                put(s, tid, rec_now);
+            }
+        } else {
+            for (uint32_t k = tid; k < count; k += kThreads) {
+                const uint32_t slot = pair_slots[start + lo + k];
+                s_slot[k] = slot;
+                stage(s, k, records, slot);
+            }
         }
         for (uint32_t k = tid; k < count * kGrad2d; k += kThreads) {
             s_grad[k] = 0.0f;
         }
         __syncthreads();
+        if constexpr (kPiped) {
+            // The next batch is [lo - batch, lo); the one after it ends where
+            // that one starts.
+            slot_now = slot_ahead;
+            rec_now = fetch(records, slot_ahead);
+            const uint32_t lo1 = lo > batch ? lo - batch : 0u;
+            slot_ahead = pair_slots[pair_of(lo1)];
+        }
         const uint32_t top = wave_last > lo ? (wave_last - lo < count ? wave_last - lo : count) : 0u;
         // Back to front over the pairs below `top` that meet the band, 64 at a
         // time; the rest hold no pixel of it and add nothing.
@@ -849,8 +922,13 @@ template <uint32_t TILE>
 hipError_t blend_for(const DView* views, const uint32_t* tile_bases, uint32_t nviews, uint32_t tiles, uint32_t batch,
                      const uint32_t* ranges, const uint32_t* slots, const float* records, uint32_t depth_mode,
                      float depth_threshold, hipStream_t s) {
-    hipLaunchKernelGGL(blend_kernel<TILE>, dim3(tiles), dim3(TILE * TILE), 0, s, views, tile_bases, nviews, batch,
-                       ranges, slots, records, depth_mode, depth_threshold);
+    if (batch <= TILE * TILE) {
+        hipLaunchKernelGGL((blend_kernel<TILE, true>), dim3(tiles), dim3(TILE * TILE), 0, s, views, tile_bases, nviews,
+                           batch, ranges, slots, records, depth_mode, depth_threshold);
+    } else {
+        hipLaunchKernelGGL((blend_kernel<TILE, false>), dim3(tiles), dim3(TILE * TILE), 0, s, views, tile_bases,
+                           nviews, batch, ranges, slots, records, depth_mode, depth_threshold);
+    }
     return hipGetLastError();
 }
 
@@ -858,8 +936,13 @@ template <uint32_t TILE>
 hipError_t blend_backward_for(const DView* views, const uint32_t* tile_bases, uint32_t nviews, uint32_t tiles,
                               uint32_t batch, const uint32_t* ranges, const uint32_t* slots, const float* records,
                               float* grad2d, hipStream_t s) {
-    hipLaunchKernelGGL(blend_backward_kernel<TILE>, dim3(tiles), dim3(TILE * TILE), 0, s, views, tile_bases, nviews,
-                       batch, ranges, slots, records, grad2d);
+    if (batch <= TILE * TILE) {
+        hipLaunchKernelGGL((blend_backward_kernel<TILE, true>), dim3(tiles), dim3(TILE * TILE), 0, s, views,
+                           tile_bases, nviews, batch, ranges, slots, records, grad2d);
+    } else {
+        hipLaunchKernelGGL((blend_backward_kernel<TILE, false>), dim3(tiles), dim3(TILE * TILE), 0, s, views,
+                           tile_bases, nviews, batch, ranges, slots, records, grad2d);
+    }
     return hipGetLastError();
 }
 

@@ -350,6 +350,17 @@ __device__ inline void hist_add(uint32_t* s_hist, uint32_t d, bool ok) {
     }
 }
 
+// One add a distinct digit of a wavefront (the lanes holding it matched by
+// ballots, the lowest adding their count): for passes of few digits, where
+// lanes adding one at a time would meet on the same words.
+__device__ inline void hist_add_matched(uint32_t* s_hist, uint32_t d, bool ok, uint32_t width) {
+    const uint64_t m = match_digit(d, width, ok);
+    const uint64_t below = (1ull << (threadIdx.x & 63u)) - 1ull;
+    if (ok && (m & below) == 0) {
+        atomicAdd(&s_hist[d], (uint32_t)__popcll(m));
+    }
+}
+
 template <typename K, typename D, uint32_t kT>
 __global__ void __launch_bounds__(kT) upsweep_kernel(const K* __restrict__ kin, const uint32_t* __restrict__ vin,
                                                      const uint64_t* __restrict__ n_dev, uint32_t cap, D digit,
@@ -385,7 +396,11 @@ __global__ void __launch_bounds__(kT) upsweep_kernel(const K* __restrict__ kin, 
 #pragma unroll
         for (uint32_t j = 0; j < kItems; ++j) {
             const bool ok = part_first + j * kT + tid < n;
-            hist_add(s_hist, D::kFromValue ? digit(K{}, val[j]) : digit(key[j], 0u), ok);
+            if constexpr (D::kFromValue) {
+                hist_add_matched(s_hist, digit(K{}, val[j]), ok, width);
+            } else {
+                hist_add(s_hist, digit(key[j], 0u), ok);
+            }
         }
     }
     __syncthreads();
