@@ -110,6 +110,7 @@ struct ProjectArgs {
 };
 
 __host__ __device__ inline uint32_t div_up(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
+__host__ __device__ inline uint64_t div_up64(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
 
 __device__ inline float fmin_(float a, float b) { return a < b ? a : b; }
 __device__ inline float fmax_(float a, float b) { return a > b ? a : b; }
@@ -251,7 +252,9 @@ struct StripBins {
 hipError_t launch_strip_mask_emit(const DView* views_dev, uint32_t n, uint32_t tile, const uint32_t* sorted_slots,
                                   const uint32_t* offsets, const uint64_t* visible_dev, uint32_t max_visible,
                                   uint32_t cap, const float* records, uint32_t* strip_keys, uint32_t* strip_slots,
-                                  uint32_t* rank_tiles, hipStream_t s);
+                                  uint32_t* rank_tiles, hipStream_t s,
+                                  const uint32_t* view_first = nullptr, uint32_t view_stride = 0,
+                                  uint32_t* view_entry = nullptr);
 // Each strip's segments, scanned: segbase[s] and the total in *seg_total.
 hipError_t launch_seg_scan(const uint32_t* strip_ranges, uint32_t strips, uint32_t* segbase, uint64_t* seg_total,
                            uint32_t* status_words, hipStream_t s);
@@ -348,6 +351,48 @@ hipError_t launch_sort(uint64_t* keys, uint32_t* values, const uint64_t* n_dev, 
                        uint32_t key_bits, const SortScratch& sc, hipStream_t s, bool* in_alt = nullptr);
 hipError_t launch_sort(uint32_t* keys, uint32_t* values, const uint64_t* n_dev, uint32_t cap, uint32_t lo_bit,
                        uint32_t key_bits, const SortScratch& sc, hipStream_t s, bool* in_alt = nullptr);
+// A multi-view frame's strip entries come out of the emission view after
+// view (the ranks are grouped by view), so a view's entries are one run of
+// the entry array: a segment. The segmented strip sort cuts each run into
+// partitions of its own (a partition holds one view's entries) and orders
+// each run by the view's own strip number (key & 0xFFFF minus seg_base) -- a
+// pass or two where the frame's whole strip number would take more. Filled
+// on the device by launch_strip_segments.
+constexpr uint32_t kSegmentPart = 8192;
+struct SegmentArgs {
+    const uint32_t* part_seg;    // partition -> view
+    const uint32_t* seg_start;   // a view's first entry
+    const uint32_t* seg_len;     // its entries (those the capacity holds)
+    const uint32_t* seg_part;    // its first partition
+    const uint32_t* seg_base;    // its first strip
+    const uint32_t* parts;       // partitions in use
+};
+// Sorts the segments' (strip key, slot) pairs stably by the low `bits` bits
+// of the key's strip relative to its view's first: view after view, each by
+// strip, in the input's order within a strip; in the alternate buffers when
+// in_alt ends true. parts_cap bounds the partitions (div_up(cap, part) + a
+// partial one a view).
+hipError_t launch_segment_sort(uint32_t* keys, uint32_t* values, const SegmentArgs& g, uint32_t parts_cap,
+                               uint32_t bits, const SortScratch& sc, hipStream_t s, bool& in_alt);
+constexpr uint32_t kMaxSegmentViews = 256;
+struct SegmentBuild {
+    uint32_t nviews;
+    const uint32_t* view_first;  // the view pass's scanned counts: view v's first rank at [v * view_stride]
+    uint32_t view_stride;
+    const uint32_t* view_entry;  // a view's first entry, written by the emission (views with ranks)
+    const uint64_t* visible_dev;
+    uint32_t max_visible;
+    const uint64_t* entry_total_dev;
+    uint32_t cap;        // entries the strip arrays hold
+    uint32_t parts_cap;
+    uint32_t* part_seg;
+    uint32_t* seg_start;
+    uint32_t* seg_len;
+    uint32_t* seg_part;
+    uint32_t* parts;
+    uint64_t* binned_total;
+};
+hipError_t launch_strip_segments(const SegmentBuild& g, hipStream_t s);
 // One more stable pass over a sorted (key, slot) array whose digit is the
 // slot's view, slot / per_view (2 <= nviews <= 256): the pairs end grouped by
 // view, each view's in the order they had. Reads the alternate buffers when

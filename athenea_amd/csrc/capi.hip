@@ -47,7 +47,8 @@ constexpr uint32_t kDepthPasses = 3;  // 24-bit depth keys (splat_project.slang:
 // segments, the exact counts' sum.
 constexpr uint32_t kTotalPairs = 0, kTotalVisible = 1, kTotalBig = 2, kTotalEntries = 3, kTotalSegs = 4;
 constexpr uint32_t kTotalExact = 5;  // the exact counts' total (copy_frame's offsets)
-constexpr uint32_t kTotalWords = 6;
+constexpr uint32_t kTotalBinned = 6;  // the entries a segmented frame's strip sort holds
+constexpr uint32_t kTotalWords = 7;
 constexpr uint64_t kMaxPairs = uint64_t{1} << 30;  // pairs and entries are 32-bit, with room
 
 // The scratch of athx_exclusive_scan_noalloc: the single-pass scan's
@@ -79,6 +80,7 @@ struct athx_ctx {
     DeviceBuffer views, tile_bases, records, touched, depth_keys, offsets, scan_status, totals;
     DeviceBuffer pair_vals, ranges, grad2d, counter_words, big_list, rank_tiles, slot_ids;
     DeviceBuffer depth_keys_sorted, depth_slots, depth_alt_keys, depth_alt_slots, depth_sort_meta;
+    DeviceBuffer segments;  // a multi-view frame's strip segments (SegmentBuild's outputs)
     DeviceBuffer fp_rects, fp_masks, fp_big_base, fp_big_masks;
     // The binning: strip entries (keys, slots, the sort's alternates and
     // metadata), each strip's range and first segment, the entries' hit
@@ -151,6 +153,7 @@ std::vector<DeviceBuffer*> all_buffers(athx_ctx* ctx) {
             &ctx->big_list, &ctx->depth_keys_sorted, &ctx->depth_slots, &ctx->depth_alt_keys, &ctx->depth_alt_slots,
             &ctx->depth_sort_meta, &ctx->fp_rects, &ctx->fp_masks, &ctx->fp_big_base, &ctx->fp_big_masks,
             &ctx->strip_keys, &ctx->strip_slots, &ctx->strip_alt_keys, &ctx->strip_alt_slots, &ctx->strip_sort_meta,
+            &ctx->segments,
             &ctx->strip_ranges, &ctx->segbase, &ctx->seg_map, &ctx->rank_fp, &ctx->masks, &ctx->seg_table, &ctx->exact, &ctx->exact_offsets,
             &ctx->exact_status, &ctx->api_sort_meta, &ctx->api_alt_keys, &ctx->api_alt_vals, &ctx->api_scan_status,
             &ctx->api_count, &ctx->ingest_keep, &ctx->ingest_dest, &ctx->ingest_total, &ctx->ref_records,
@@ -559,14 +562,29 @@ struct FrameShape {
     uint32_t strip_passes = 0;
     uint32_t entry_cap = 0;    // strip entries the binning holds
     uint32_t seg_cap = 0;      // segments: entry_cap / kStripSeg + a partial one a strip
+    // A frame of several views whose ranks are grouped by view (the view
+    // pass), so that each view's strip entries are one run: its strip sort
+    // orders each run by the view's own strip number (launch_segment_sort),
+    // view_strip_bits bits; the sort's counts are sized for entry_pad more
+    // entries (a partial partition a view).
+    bool segmented = false;
+    uint32_t view_strip_bits = 0;
+    uint32_t entry_pad = 0;
     uint32_t capacity = 0;     // pairs the pair buffer holds
     std::vector<uint32_t> bases;  // each instance's first Gaussian in a view's slots, then count
     bool matte = false, traced = false, under = false;  // the walk's variants (athx_targets)
     bool variants() const { return matte || traced || under; }
 };
 
+bool masks_in_keys(const FrameShape& fs) { return fs.strip_bits <= 16; }
+// The segmented strip sort's partitions: the entries' and a partial one a view.
+uint32_t segment_parts_cap(const FrameShape& fs) { return div_up(fs.entry_cap, kSegmentPart) + fs.nviews; }
+// Several views: the depth sort's ranks are grouped by view (launch_view_pass).
+bool grouped_by_view(const FrameShape& fs) { return fs.nviews >= 2 && fs.nviews <= 256 && fs.count >= 2; }
+
 void set_entry_cap(FrameShape& fs, uint32_t entries) {
     fs.entry_cap = entries;
+    fs.entry_pad = fs.segmented ? fs.nviews * kSegmentPart : 0u;
     fs.seg_cap = div_up(entries, kStripSeg) + fs.strips;
 }
 
@@ -620,6 +638,13 @@ athx_status begin_frame(athx_ctx* ctx, const athx_view* views, uint32_t nviews, 
     fs.key_bits = kDepthBits + bits_for(fs.tiles);
     fs.strip_bits = bits_for(fs.strips);
     fs.strip_passes = div_up(fs.strip_bits, 8);
+    uint32_t view_strips = 1;
+    const uint32_t* strip_bases = ctx->host_tile_bases + nviews + 1;
+    for (uint32_t k = 0; k < nviews; ++k) {
+        view_strips = std::max(view_strips, strip_bases[k + 1] - strip_bases[k]);
+    }
+    fs.view_strip_bits = bits_for(view_strips);
+    fs.segmented = grouped_by_view(fs) && masks_in_keys(fs) && nviews <= kMaxSegmentViews;
     return ATHX_OK;
 }
 
@@ -656,8 +681,13 @@ athx_status reserve_bins(athx_ctx* ctx, const FrameShape& fs) {
     ATHX_HIP(ctx, reserve(ctx, ctx->strip_slots, cap * 4 + 16), "strip slots");
     ATHX_HIP(ctx, reserve(ctx, ctx->strip_alt_keys, cap * 4 + 16), "strip sort keys");
     ATHX_HIP(ctx, reserve(ctx, ctx->strip_alt_slots, cap * 4 + 16), "strip sort slots");
-    ATHX_HIP(ctx, reserve(ctx, ctx->strip_sort_meta, sort_meta_words(fs.entry_cap, fs.strip_passes) * 4 + 16),
+    const uint32_t passes = fs.segmented ? div_up(fs.view_strip_bits, 8) : fs.strip_passes;
+    ATHX_HIP(ctx, reserve(ctx, ctx->strip_sort_meta, sort_meta_words(fs.entry_cap + fs.entry_pad, passes) * 4 + 16),
              "strip sort metadata");
+    if (fs.segmented) {
+        ATHX_HIP(ctx, reserve(ctx, ctx->segments, ((size_t)segment_parts_cap(fs) + 4u * fs.nviews + 4) * 4),
+                 "strip segments");
+    }
     ATHX_HIP(ctx, reserve(ctx, ctx->masks, cap * 2 + 16), "entry masks");
     ATHX_HIP(ctx, reserve(ctx, ctx->seg_map, (size_t)fs.seg_cap * 32 + 32), "segment descriptors");
     const size_t table = (size_t)kStripTiles * fs.seg_cap;
@@ -712,8 +742,6 @@ const uint32_t* sorted_slots_of(const athx_ctx* ctx, bool in_alt) {
 
 // Frames of at most 65 536 strips decide their entries' tiles in the
 // emission and carry the masks in the keys' upper half.
-bool masks_in_keys(const FrameShape& fs) { return fs.strip_bits <= 16; }
-
 StripBins bins_of(athx_ctx* ctx, const FrameShape& fs, uint32_t n, uint32_t tile, bool strips_in_alt) {
     StripBins b{};
     b.views = as<DView>(ctx->views);
@@ -783,7 +811,7 @@ athx_status enqueue_front(athx_ctx* ctx, const std::vector<ProjectArgs>& pas, co
     // a time, which the caches hold, where the frame's depth order reads them
     // scattered over every view's.
     ctx->depth_order_in_alt = in_alt;
-    if (fs.nviews >= 2 && fs.nviews <= 256 && fs.count >= 2) {
+    if (grouped_by_view(fs)) {
         ATHX_HIP(ctx, launch_view_pass(as<uint32_t>(ctx->depth_keys_sorted), as<uint32_t>(ctx->depth_slots),
                                        totals + kTotalVisible, fs.slots, fs.count, fs.nviews, dsc, s, in_alt),
                  "view pass");
@@ -804,6 +832,57 @@ athx_status enqueue_front(athx_ctx* ctx, const std::vector<ProjectArgs>& pas, co
     return ATHX_OK;
 }
 
+// A segmented frame's segment arrays in ctx->segments: part_seg (a word a
+// partition), then seg_start, seg_len, seg_part and view_entry (a word a
+// view), and the partitions in use.
+SegmentArgs segments_of(athx_ctx* ctx, const FrameShape& fs) {
+    uint32_t* w = as<uint32_t>(ctx->segments);
+    SegmentArgs a{};
+    a.part_seg = w;
+    a.seg_start = w + segment_parts_cap(fs);
+    a.seg_len = a.seg_start + fs.nviews;
+    a.seg_part = a.seg_len + fs.nviews;
+    a.parts = a.seg_part + 2 * fs.nviews;  // after view_entry
+    a.seg_base = as<uint32_t>(ctx->tile_bases) + fs.nviews + 1;  // each view's first strip
+    return a;
+}
+
+uint32_t* view_entry_of(athx_ctx* ctx, const FrameShape& fs) {
+    return const_cast<uint32_t*>(segments_of(ctx, fs).seg_part) + fs.nviews;
+}
+
+// The view pass's scanned counts: view v's first rank at [v * stride].
+const uint32_t* view_first_of(athx_ctx* ctx, const FrameShape& fs) {
+    return sort_scratch(ctx->depth_sort_meta, ctx->depth_alt_keys.ptr, as<uint32_t>(ctx->depth_alt_slots), fs.slots,
+                        kDepthPasses)
+        .offsets;
+}
+
+// The views' runs of strip entries (launch_strip_segments), from the view
+// pass's counts and the first entries the emission recorded.
+athx_status enqueue_segments(athx_ctx* ctx, const FrameShape& fs, hipStream_t s) {
+    uint64_t* totals = as<uint64_t>(ctx->totals);
+    const SegmentArgs a = segments_of(ctx, fs);
+    SegmentBuild g{};
+    g.nviews = fs.nviews;
+    g.view_first = view_first_of(ctx, fs);
+    g.view_stride = div_up(fs.slots, kSegmentPart);
+    g.view_entry = view_entry_of(ctx, fs);
+    g.visible_dev = totals + kTotalVisible;
+    g.max_visible = fs.slots;
+    g.entry_total_dev = totals + kTotalEntries;
+    g.cap = fs.entry_cap;
+    g.parts_cap = segment_parts_cap(fs);
+    g.part_seg = const_cast<uint32_t*>(a.part_seg);
+    g.seg_start = const_cast<uint32_t*>(a.seg_start);
+    g.seg_len = const_cast<uint32_t*>(a.seg_len);
+    g.seg_part = const_cast<uint32_t*>(a.seg_part);
+    g.parts = const_cast<uint32_t*>(a.parts);
+    g.binned_total = totals + kTotalBinned;
+    ATHX_HIP(ctx, launch_strip_segments(g, s), "strip segments");
+    return ATHX_OK;
+}
+
 // The strip entries, emitted in depth order, sorted on the strip, their
 // segments scanned, tested and counted, and the counts scanned: the pair
 // total lands in totals[kTotalPairs]. Every count is read on the device
@@ -819,12 +898,20 @@ athx_status enqueue_bins(athx_ctx* ctx, const FrameShape& fs, uint32_t n, uint32
             ATHX_HIP(ctx, reserve(ctx, ctx->rank_tiles, (size_t)fs.slots * 4 + 16), "rank tiles");
             rank_tiles = as<uint32_t>(ctx->rank_tiles);
         }
+        // A segmented frame's emission records each view's first entry.
         ATHX_HIP(ctx, launch_strip_mask_emit(as<DView>(ctx->views), n, tile,
                                              sorted_slots_of(ctx, ctx->last_sorted_in_alt), as<uint32_t>(ctx->offsets),
                                              totals + kTotalVisible, fs.slots, fs.entry_cap, as<float>(ctx->records),
                                              as<uint32_t>(ctx->strip_keys), as<uint32_t>(ctx->strip_slots), rank_tiles,
-                                             s),
+                                             s, fs.segmented ? view_first_of(ctx, fs) : nullptr,
+                                             div_up(fs.slots, kSegmentPart),
+                                             fs.segmented ? view_entry_of(ctx, fs) : nullptr),
                  "strip entries");
+        if (fs.segmented) {
+            if (athx_status st = enqueue_segments(ctx, fs, s); st != ATHX_OK) {
+                return st;
+            }
+        }
     } else {
         ATHX_HIP(ctx, launch_strip_emit(as<DView>(ctx->views), n, tile,
                                         sorted_slots_of(ctx, ctx->last_sorted_in_alt), as<uint32_t>(ctx->offsets),
@@ -833,15 +920,28 @@ athx_status enqueue_bins(athx_ctx* ctx, const FrameShape& fs, uint32_t n, uint32
                                         as<uint32_t>(ctx->strip_slots), s),
                  "strip entries");
     }
+    const bool segmented = masked && fs.segmented;
+    // The segmented sort's counts are sized for a partial partition a view
+    // more (entry_pad entries).
     const SortScratch sc = sort_scratch(ctx->strip_sort_meta, ctx->strip_alt_keys.ptr,
-                                        as<uint32_t>(ctx->strip_alt_slots), fs.entry_cap, fs.strip_passes);
+                                        as<uint32_t>(ctx->strip_alt_slots), fs.entry_cap + fs.entry_pad,
+                                        segmented ? div_up(fs.view_strip_bits, 8) : fs.strip_passes);
     strips_in_alt = false;
-    ATHX_HIP(ctx, launch_sort(as<uint32_t>(ctx->strip_keys), as<uint32_t>(ctx->strip_slots), totals + kTotalEntries,
-                              fs.entry_cap, 0, fs.strip_bits, sc, s, &strips_in_alt),
-             "strip sort");
+    if (segmented) {
+        ATHX_HIP(ctx, launch_segment_sort(as<uint32_t>(ctx->strip_keys), as<uint32_t>(ctx->strip_slots),
+                                          segments_of(ctx, fs), segment_parts_cap(fs), fs.view_strip_bits, sc, s,
+                                          strips_in_alt),
+                 "strip sort");
+    } else {
+        ATHX_HIP(ctx, launch_sort(as<uint32_t>(ctx->strip_keys), as<uint32_t>(ctx->strip_slots),
+                                  totals + kTotalEntries, fs.entry_cap, 0, fs.strip_bits, sc, s, &strips_in_alt),
+                 "strip sort");
+    }
+    // The sorted entries are dense from here on; a segmented frame's count is
+    // what its segments held.
     const uint32_t* keys = as<uint32_t>(strips_in_alt ? ctx->strip_alt_keys : ctx->strip_keys);
-    ATHX_HIP(ctx, launch_ranges(keys, totals + kTotalEntries, fs.entry_cap, fs.strips, as<uint32_t>(ctx->strip_ranges),
-                                s, masked ? 0xFFFFu : ~0u),
+    ATHX_HIP(ctx, launch_ranges(keys, totals + (segmented ? kTotalBinned : kTotalEntries), fs.entry_cap, fs.strips,
+                                as<uint32_t>(ctx->strip_ranges), s, masked ? 0xFFFFu : ~0u),
              "strip ranges");
     ATHX_HIP(ctx, launch_seg_scan(as<uint32_t>(ctx->strip_ranges), fs.strips, as<uint32_t>(ctx->segbase),
                                   totals + kTotalSegs, as<uint32_t>(ctx->scan_status), s),

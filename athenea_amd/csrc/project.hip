@@ -1075,7 +1075,10 @@ __global__ void __launch_bounds__(kEmitThreads) strip_mask_emit_kernel(const DVi
                                                                        const float* __restrict__ records,
                                                                        uint32_t* __restrict__ keys_out,
                                                                        uint32_t* __restrict__ slots_out,
-                                                                       uint32_t* __restrict__ rank_tiles) {
+                                                                       uint32_t* __restrict__ rank_tiles,
+                                                                       const uint32_t* __restrict__ view_first,
+                                                                       uint32_t view_stride,
+                                                                       uint32_t* __restrict__ view_entry) {
     __shared__ uint32_t s_end[kEmitThreads];
     __shared__ uint32_t s_tiles[kEmitThreads]; // rank: tiles its entries' masks hold (counting frames)
     __shared__ float4 s_geo[kEmitThreads];   // rank: centre x, y, conic a, b
@@ -1115,6 +1118,14 @@ __global__ void __launch_bounds__(kEmitThreads) strip_mask_emit_kernel(const DVi
     const uint32_t total = lane_u(end, 63);
     const uint32_t base = k0 < visible ? offsets[k0 / 64u] : 0u;
     const uint32_t start = end - count;
+    if (view_entry && k < visible) {
+        // The first rank of its view records the view's first entry (the
+        // segmented strip sort's run of that view).
+        const uint32_t view = s_int2[threadIdx.x].z;
+        if (view_first[(size_t)view * view_stride] == k) {
+            view_entry[view] = base + start;
+        }
+    }
     __syncthreads();
     const uint32_t* ends = s_end + wbase;
     // Each step's entries find their ranks without a search: the ranks whose
@@ -1811,13 +1822,82 @@ hipError_t launch_strip_count(const StripBins& b, hipStream_t s) {
 hipError_t launch_strip_mask_emit(const DView* views_dev, uint32_t n, uint32_t tile, const uint32_t* sorted_slots,
                                   const uint32_t* offsets, const uint64_t* visible_dev, uint32_t max_visible,
                                   uint32_t cap, const float* records, uint32_t* strip_keys, uint32_t* strip_slots,
-                                  uint32_t* rank_tiles, hipStream_t s) {
+                                  uint32_t* rank_tiles, hipStream_t s, const uint32_t* view_first,
+                                  uint32_t view_stride, uint32_t* view_entry) {
     if (max_visible == 0 || cap == 0) {
         return hipSuccess;
     }
     hipLaunchKernelGGL(strip_mask_emit_kernel, dim3(div_up(max_visible, kEmitThreads)), dim3(kEmitThreads), 0, s,
                        views_dev, n, tile, sorted_slots, offsets, visible_dev, max_visible, cap, records,
-                       strip_keys, strip_slots, rank_tiles);
+                       strip_keys, strip_slots, rank_tiles, view_first, view_stride, view_entry);
+    return hipGetLastError();
+}
+
+// The strip entries' segments (one a view) of a frame whose ranks are
+// grouped by view: view v's run starts at the entry its first rank recorded
+// in the emission (a view with no visible rank has an empty run where the
+// next starts), cut where the capacity ends (the frame is then binned again
+// at its exact size); its partitions follow the earlier views', a partial one
+// closing each run. Every workgroup lays the runs out in LDS and maps its 256
+// partitions to their views; the first also writes the runs and the totals.
+__global__ void __launch_bounds__(256) strip_segments_kernel(SegmentBuild g) {
+    __shared__ uint32_t s_start[kMaxSegmentViews + 1];
+    __shared__ uint32_t s_part[kMaxSegmentViews + 1];
+    const uint32_t nv = g.nviews;
+    if (threadIdx.x == 0) {
+        const uint32_t visible = *g.visible_dev < g.max_visible ? (uint32_t)*g.visible_dev : g.max_visible;
+        const uint64_t etotal = *g.entry_total_dev;
+        uint32_t next = etotal < g.cap ? (uint32_t)etotal : g.cap;
+        s_start[nv] = next;
+        uint32_t next_rank = visible;
+        for (uint32_t v = nv; v-- > 0;) {
+            const uint32_t R = g.view_first[(size_t)v * g.view_stride];
+            if (R < next_rank) {
+                const uint32_t e = g.view_entry[v];
+                next = e < next ? e : next;  // within the capacity, never past the next run
+            }
+            next_rank = R < next_rank ? R : next_rank;
+            s_start[v] = next;
+        }
+        uint32_t parts = 0;
+        for (uint32_t v = 0; v < nv; ++v) {
+            s_part[v] = parts;
+            parts += div_up(s_start[v + 1] - s_start[v], kSegmentPart);
+        }
+        s_part[nv] = parts;
+        if (blockIdx.x == 0) {
+            for (uint32_t v = 0; v < nv; ++v) {
+                g.seg_start[v] = s_start[v];
+                g.seg_len[v] = s_start[v + 1] - s_start[v];
+                g.seg_part[v] = s_part[v];
+            }
+            *g.parts = parts < g.parts_cap ? parts : g.parts_cap;
+            *g.binned_total = s_start[nv] - s_start[0];
+        }
+    }
+    __syncthreads();
+    const uint32_t p = blockIdx.x * 256u + threadIdx.x;
+    if (p < s_part[nv] && p < g.parts_cap) {
+        // The last view whose partitions start at or before p (views with no
+        // entries start where the next does).
+        uint32_t lo = 0, hi = nv;
+        while (hi - lo > 1u) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (s_part[mid] <= p) {
+                lo = mid;
+            } else {
+                hi = mid;
+            }
+        }
+        g.part_seg[p] = lo;
+    }
+}
+
+hipError_t launch_strip_segments(const SegmentBuild& g, hipStream_t s) {
+    if (g.nviews == 0 || g.nviews > kMaxSegmentViews || g.parts_cap == 0) {
+        return hipErrorInvalidValue;
+    }
+    hipLaunchKernelGGL(strip_segments_kernel, dim3(div_up(g.parts_cap, 256)), dim3(256), 0, s, g);
     return hipGetLastError();
 }
 

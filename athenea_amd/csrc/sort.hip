@@ -310,13 +310,24 @@ __global__ void __launch_bounds__(kThreads) onesweep_kernel(const K* __restrict_
 // no partition waits on another: on MI355X the look-back's coherent loads
 // and the ticket atomics cost more than that read.
 // Where a reduce-scan-scatter pass takes its digit from: bits of the key,
-// or (the view pass) the view a slot value belongs to, slot / per_view.
+// bits of the key's strip relative to its segment's first (the segmented
+// strip sort), or (the view pass) the view a slot value belongs to,
+// slot / per_view. `base` is the partition's segment's.
 struct KeyDigit {
     static constexpr bool kFromValue = false;
     uint32_t shift, mask;
     template <typename K>
-    __device__ uint32_t operator()(K key, uint32_t) const {
+    __device__ uint32_t operator()(K key, uint32_t, uint32_t) const {
         return (uint32_t)(key >> shift) & mask;
+    }
+};
+
+struct StripDigit {
+    static constexpr bool kFromValue = false;
+    uint32_t shift, mask;
+    template <typename K>
+    __device__ uint32_t operator()(K key, uint32_t, uint32_t base) const {
+        return ((((uint32_t)key & 0xFFFFu) - base) >> shift) & mask;
     }
 };
 
@@ -325,11 +336,64 @@ struct ViewDigit {
     uint32_t per_view;  // >= 2
     uint32_t magic;     // floor(2^32 / per_view)
     template <typename K>
-    __device__ uint32_t operator()(K, uint32_t slot) const {
+    __device__ uint32_t operator()(K, uint32_t slot, uint32_t) const {
         // slot * magic / 2^32 lies within one below slot / per_view.
         uint32_t q = (uint32_t)(((uint64_t)slot * magic) >> 32);
         q += (uint64_t)(q + 1u) * per_view <= slot ? 1u : 0u;
         return q;
+    }
+};
+
+// A partition of a reduce-scan-scatter pass: its elements [first, end) (end
+// past first + kP means a whole partition), where its digit counts go
+// (cell + digit * stride), and its digit base.
+struct Part {
+    uint32_t first, end, cell, stride, base;
+};
+
+// The first n elements (a count in device memory, bounded by cap), cut into
+// parts_cap partitions whose counts lie digit-major: every partition has its
+// cells, those past n count nothing.
+struct PlainLayout {
+    const uint64_t* n_dev;
+    uint32_t cap, parts_cap;
+    __device__ bool part(uint32_t p, uint32_t kP, Part& r) const {
+        r.first = p * kP;
+        r.end = count_of(n_dev, cap);
+        r.cell = p;
+        r.stride = parts_cap;
+        r.base = 0;
+        return true;
+    }
+};
+
+// Segments (a multi-view frame's views): runs of the array, each cut into
+// partitions of its own, so that a partition holds one segment's elements --
+// segment s is [seg_start[s], seg_start[s] + seg_len[s]), its partitions
+// numbered from seg_part[s]. A segment's counts lie together, digit-major
+// within it, so one exclusive scan over every cell gives each (segment,
+// digit, partition) its place in the output -- the elements of the earlier
+// segments, then of the smaller digits, then of the earlier partitions.
+struct SegLayout {
+    const uint32_t* part_seg;  // partition -> segment
+    const uint32_t* seg_start;
+    const uint32_t* seg_len;
+    const uint32_t* seg_part;
+    const uint32_t* seg_base;  // subtracted from a strip key's low half
+    const uint32_t* parts_dev;  // partitions in use
+    uint32_t radix;
+    __device__ bool part(uint32_t p, uint32_t kP, Part& r) const {
+        if (p >= *parts_dev) {
+            return false;
+        }
+        const uint32_t sg = part_seg[p];
+        const uint32_t q = p - seg_part[sg];
+        r.first = seg_start[sg] + q * kP;
+        r.end = seg_start[sg] + seg_len[sg];
+        r.cell = radix * seg_part[sg] + q;
+        r.stride = div_up(seg_len[sg], kP);
+        r.base = seg_base[sg];
+        return true;
     }
 };
 
@@ -361,24 +425,24 @@ __device__ inline void hist_add_matched(uint32_t* s_hist, uint32_t d, bool ok, u
     }
 }
 
-template <typename K, typename D, uint32_t kT>
+template <typename K, typename D, uint32_t kT, typename L>
 __global__ void __launch_bounds__(kT) upsweep_kernel(const K* __restrict__ kin, const uint32_t* __restrict__ vin,
-                                                     const uint64_t* __restrict__ n_dev, uint32_t cap, D digit,
-                                                     uint32_t width, uint32_t parts_cap,
+                                                     L layout, D digit, uint32_t width,
                                                      uint32_t* __restrict__ counts) {
     static_assert(kT >= kRadix && kT % 64 == 0, "a thread a digit");
     constexpr uint32_t kP = kT * kItems;
     __shared__ uint32_t s_hist[kRadix];
     const uint32_t tid = threadIdx.x;
-    const uint32_t part = blockIdx.x;
-    const uint32_t n = count_of(n_dev, cap);
     const uint32_t radix = 1u << width;
+    Part pt;
+    if (!layout.part(blockIdx.x, kP, pt)) {
+        return;
+    }
     if (tid < kRadix) {
         s_hist[tid] = 0;
     }
     __syncthreads();
-    const uint32_t part_first = part * kP;
-    if (part_first < n) {
+    if (pt.first < pt.end) {
         // Every load issued before the first add, so the adds wait on them
         // once: a lane past the end reads the last key (in bounds, unbranched)
         // and adds nothing.
@@ -386,7 +450,7 @@ __global__ void __launch_bounds__(kT) upsweep_kernel(const K* __restrict__ kin, 
         uint32_t val[kItems];
 #pragma unroll
         for (uint32_t j = 0; j < kItems; ++j) {
-            const uint32_t at = min(part_first + j * kT + tid, n - 1u);
+            const uint32_t at = min(pt.first + j * kT + tid, pt.end - 1u);
             if constexpr (D::kFromValue) {
                 val[j] = vin[at];
             } else {
@@ -395,25 +459,25 @@ __global__ void __launch_bounds__(kT) upsweep_kernel(const K* __restrict__ kin, 
         }
 #pragma unroll
         for (uint32_t j = 0; j < kItems; ++j) {
-            const bool ok = part_first + j * kT + tid < n;
+            const uint32_t at = pt.first + j * kT + tid;
+            const bool ok = at < pt.end && at - pt.first < kP;
             if constexpr (D::kFromValue) {
-                hist_add_matched(s_hist, digit(K{}, val[j]), ok, width);
+                hist_add_matched(s_hist, digit(K{}, val[j], pt.base), ok, width);
             } else {
-                hist_add(s_hist, digit(key[j], 0u), ok);
+                hist_add(s_hist, digit(key[j], 0u, pt.base), ok);
             }
         }
     }
     __syncthreads();
     if (tid < radix) {
-        counts[(size_t)tid * parts_cap + part] = s_hist[tid];
+        counts[(size_t)pt.cell + (size_t)tid * pt.stride] = s_hist[tid];
     }
 }
 
-template <typename K, typename D, uint32_t kT>
+template <typename K, typename D, uint32_t kT, typename L>
 __global__ void __launch_bounds__(kT) downsweep_kernel(const K* __restrict__ kin, const uint32_t* __restrict__ vin,
-                                                       K* __restrict__ kout, uint32_t* __restrict__ vout,
-                                                       const uint64_t* __restrict__ n_dev, uint32_t cap, D digit,
-                                                       uint32_t width, uint32_t parts_cap,
+                                                       K* __restrict__ kout, uint32_t* __restrict__ vout, L layout,
+                                                       D digit, uint32_t width,
                                                        const uint32_t* __restrict__ offsets) {
     static_assert(kT >= kRadix && kT % 64 == 0, "a thread a digit");
     constexpr uint32_t kW = kT / 64;
@@ -428,35 +492,33 @@ __global__ void __launch_bounds__(kT) downsweep_kernel(const K* __restrict__ kin
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = tid & 63u;
     const uint32_t wave = tid >> 6;
+    Part pt;
+    if (!layout.part(blockIdx.x, kP, pt) || pt.first >= pt.end) {
+        return;
+    }
     for (uint32_t i = tid; i < kW * kRadix; i += kT) {
         s_wave_hist[i] = 0;
     }
     __syncthreads();
-    const uint32_t n = count_of(n_dev, cap);
-    const uint32_t part = blockIdx.x;
-    if (part >= div_up(n, kP)) {
-        return;
-    }
-    const uint32_t part_first = part * kP;
-    const uint32_t wave_first = part_first + wave * (64 * kItems);
+    const uint32_t valid = pt.end - pt.first < kP ? pt.end - pt.first : kP;
+    const uint32_t wave_first = wave * (64 * kItems);  // relative to the partition
     K key[kItems];
     uint32_t val[kItems];
     uint32_t rank[kItems];
 #pragma unroll
     for (uint32_t j = 0; j < kItems; ++j) {
         const uint32_t at = wave_first + j * 64 + lane;
-        const bool ok = at < n;
-        key[j] = ok ? kin[at] : 0;
-        val[j] = ok ? vin[at] : 0;
+        const bool ok = at < valid;
+        key[j] = ok ? kin[pt.first + at] : 0;
+        val[j] = ok ? vin[pt.first + at] : 0;
     }
     const uint64_t lt_mask = (1ull << lane) - 1ull;
     const uint32_t dmask = (1u << width) - 1u;
     uint32_t* my_hist = s_wave_hist + wave * kRadix;
 #pragma unroll
     for (uint32_t j = 0; j < kItems; ++j) {
-        const uint32_t at = wave_first + j * 64 + lane;
-        const bool ok = at < n;
-        const uint32_t d = digit(key[j], val[j]);
+        const bool ok = wave_first + j * 64 + lane < valid;
+        const uint32_t d = digit(key[j], val[j], pt.base);
         const uint64_t m = match_digit(d, width, ok);
         const uint32_t before = ok ? my_hist[d] : 0u;
         __builtin_amdgcn_wave_barrier();
@@ -483,25 +545,23 @@ __global__ void __launch_bounds__(kT) downsweep_kernel(const K* __restrict__ kin
     const uint32_t start = block_exclusive_scan<kW>(count, s_tmp, unused);
     if (d < kRadix) {
         s_block_start[d] = start;
-        s_digit_base[d] = d <= dmask ? offsets[(size_t)d * parts_cap + part] : 0u;
+        s_digit_base[d] = d <= dmask ? offsets[(size_t)pt.cell + (size_t)d * pt.stride] : 0u;
     }
     __syncthreads();
 #pragma unroll
     for (uint32_t j = 0; j < kItems; ++j) {
-        const uint32_t at = wave_first + j * 64 + lane;
-        if (at < n) {
-            const uint32_t dj = digit(key[j], val[j]);
+        if (wave_first + j * 64 + lane < valid) {
+            const uint32_t dj = digit(key[j], val[j], pt.base);
             const uint32_t pos = s_block_start[dj] + s_wave_hist[wave * kRadix + dj] + rank[j];
             s_keys[pos] = key[j];
             s_vals[pos] = val[j];
         }
     }
     __syncthreads();
-    const uint32_t valid = n - part_first < kP ? n - part_first : kP;
     for (uint32_t i = tid; i < valid; i += kT) {
         const K k = s_keys[i];
         const uint32_t v = s_vals[i];
-        const uint32_t dk = digit(k, v);
+        const uint32_t dk = digit(k, v, pt.base);
         const uint32_t dst = s_digit_base[dk] + (i - s_block_start[dk]);
         kout[dst] = k;
         vout[dst] = v;
@@ -575,15 +635,16 @@ hipError_t sort_impl(K* keys, uint32_t* values, const uint64_t* n_dev, uint32_t 
             uint32_t shift, width;
             pass_digit(p, lo_bit, key_bits - lo_bit, passes, shift, width);
             const KeyDigit digit{shift, (1u << width) - 1u};
-            hipLaunchKernelGGL((upsweep_kernel<K, KeyDigit, kT>), dim3(rparts), dim3(kT), 0, s, kin, vin, n_dev, cap,
-                               digit, width, rparts, sc.counts);
+            const PlainLayout layout{n_dev, cap, rparts};
+            hipLaunchKernelGGL((upsweep_kernel<K, KeyDigit, kT, PlainLayout>), dim3(rparts), dim3(kT), 0, s, kin, vin,
+                               layout, digit, width, sc.counts);
             if (hipError_t e = launch_scan(sc.counts, sc.offsets, sc.scan_total, (1u << width) * rparts,
                                            sc.scan_status, s);
                 e != hipSuccess) {
                 return e;
             }
-            hipLaunchKernelGGL((downsweep_kernel<K, KeyDigit, kT>), dim3(rparts), dim3(kT), 0, s, kin, vin, kout,
-                               vout, n_dev, cap, digit, width, rparts, sc.offsets);
+            hipLaunchKernelGGL((downsweep_kernel<K, KeyDigit, kT, PlainLayout>), dim3(rparts), dim3(kT), 0, s, kin,
+                               vin, kout, vout, layout, digit, width, sc.offsets);
             K* tk = kin; kin = kout; kout = tk;
             uint32_t* tv = vin; vin = vout; vout = tv;
         }
@@ -650,6 +711,41 @@ hipError_t launch_sort(uint32_t* keys, uint32_t* values, const uint64_t* n_dev, 
     return sort_impl(keys, values, n_dev, cap, lo_bit, key_bits, sc, s, in_alt);
 }
 
+hipError_t launch_segment_sort(uint32_t* keys, uint32_t* values, const SegmentArgs& g, uint32_t parts_cap,
+                               uint32_t bits, const SortScratch& sc, hipStream_t s, bool& in_alt) {
+    in_alt = false;
+    if (parts_cap == 0 || bits == 0) {
+        return hipSuccess;
+    }
+    if (bits > 16) {
+        return hipErrorInvalidValue;
+    }
+    constexpr uint32_t kT = rs_threads<uint32_t>();
+    static_assert(kT * kItems == kSegmentPart, "a segment's partition is the pass's");
+    const uint32_t parts = parts_cap;
+    const uint32_t passes = div_up(bits, 8);
+    for (uint32_t p = 0; p < passes; ++p) {
+        uint32_t shift, width;
+        pass_digit(p, 0, bits, passes, shift, width);
+        uint32_t* kin = in_alt ? static_cast<uint32_t*>(sc.keys_alt) : keys;
+        uint32_t* vin = in_alt ? sc.values_alt : values;
+        uint32_t* kout = in_alt ? keys : static_cast<uint32_t*>(sc.keys_alt);
+        uint32_t* vout = in_alt ? values : sc.values_alt;
+        const StripDigit digit{shift, (1u << width) - 1u};
+        const SegLayout layout{g.part_seg, g.seg_start, g.seg_len, g.seg_part, g.seg_base, g.parts, 1u << width};
+        hipLaunchKernelGGL((upsweep_kernel<uint32_t, StripDigit, kT, SegLayout>), dim3(parts), dim3(kT), 0, s, kin,
+                           vin, layout, digit, width, sc.counts);
+        if (hipError_t e = launch_scan(sc.counts, sc.offsets, sc.scan_total, (1u << width) * parts, sc.scan_status, s);
+            e != hipSuccess) {
+            return e;
+        }
+        hipLaunchKernelGGL((downsweep_kernel<uint32_t, StripDigit, kT, SegLayout>), dim3(parts), dim3(kT), 0, s, kin,
+                           vin, kout, vout, layout, digit, width, sc.offsets);
+        in_alt = !in_alt;
+    }
+    return hipGetLastError();
+}
+
 hipError_t launch_view_pass(uint32_t* keys, uint32_t* values, const uint64_t* n_dev, uint32_t cap, uint32_t per_view,
                             uint32_t nviews, const SortScratch& sc, hipStream_t s, bool& in_alt) {
     if (cap < 2 || nviews < 2) {
@@ -669,14 +765,15 @@ hipError_t launch_view_pass(uint32_t* keys, uint32_t* values, const uint64_t* n_
     uint32_t* kout = in_alt ? keys : static_cast<uint32_t*>(sc.keys_alt);
     uint32_t* vout = in_alt ? values : sc.values_alt;
     const ViewDigit digit{per_view, (uint32_t)((1ull << 32) / per_view)};
-    hipLaunchKernelGGL((upsweep_kernel<uint32_t, ViewDigit, kT>), dim3(parts), dim3(kT), 0, s, kin, vin, n_dev, cap,
-                       digit, width, parts, sc.counts);
+    const PlainLayout layout{n_dev, cap, parts};
+    hipLaunchKernelGGL((upsweep_kernel<uint32_t, ViewDigit, kT, PlainLayout>), dim3(parts), dim3(kT), 0, s, kin, vin,
+                       layout, digit, width, sc.counts);
     if (hipError_t e = launch_scan(sc.counts, sc.offsets, sc.scan_total, (1u << width) * parts, sc.scan_status, s);
         e != hipSuccess) {
         return e;
     }
-    hipLaunchKernelGGL((downsweep_kernel<uint32_t, ViewDigit, kT>), dim3(parts), dim3(kT), 0, s, kin, vin, kout, vout,
-                       n_dev, cap, digit, width, parts, sc.offsets);
+    hipLaunchKernelGGL((downsweep_kernel<uint32_t, ViewDigit, kT, PlainLayout>), dim3(parts), dim3(kT), 0, s, kin,
+                       vin, kout, vout, layout, digit, width, sc.offsets);
     in_alt = !in_alt;
     return hipGetLastError();
 }
