@@ -356,6 +356,44 @@ __global__ void clear_status_kernel(Policy pol, uint64_t* __restrict__ status, u
 constexpr uint32_t kLongScan = 1u << 24;
 constexpr uint32_t kSumThreads = 1024;
 
+// A thread's run sums[lo, hi) of a single-workgroup scan, read eight loads
+// at a time (one at a time, each waits on the last): its sum, and then each
+// element replaced by run + the elements before it.
+constexpr uint32_t kRunGroup = 8;
+
+__device__ inline uint64_t run_sum(const uint64_t* sums, uint32_t lo, uint32_t hi) {
+    uint64_t mine = 0;
+    for (uint32_t p = lo; p < hi; p += kRunGroup) {
+        uint64_t x[kRunGroup];
+#pragma unroll
+        for (uint32_t u = 0; u < kRunGroup; ++u) {
+            x[u] = p + u < hi ? sums[p + u] : 0u;
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < kRunGroup; ++u) {
+            mine += x[u];
+        }
+    }
+    return mine;
+}
+
+__device__ inline void run_exclusive(uint64_t* sums, uint32_t lo, uint32_t hi, uint64_t run) {
+    for (uint32_t p = lo; p < hi; p += kRunGroup) {
+        uint64_t x[kRunGroup];
+#pragma unroll
+        for (uint32_t u = 0; u < kRunGroup; ++u) {
+            x[u] = p + u < hi ? sums[p + u] : 0u;
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < kRunGroup; ++u) {
+            if (p + u < hi) {
+                sums[p + u] = run;
+            }
+            run += x[u];
+        }
+    }
+}
+
 template <typename Policy>
 __global__ void __launch_bounds__(kThreads) partition_sum_kernel(Policy pol, uint64_t* __restrict__ sums) {
     __shared__ uint64_t s_wave[kThreads / 64];
@@ -398,10 +436,7 @@ __global__ void __launch_bounds__(kSumThreads) partition_offsets_kernel(Policy p
     const uint32_t parts = div_up(n ? n : 1, kTile);
     const uint32_t per = div_up(parts, kSumThreads);
     const uint32_t lo = threadIdx.x * per, hi = lo + per < parts ? lo + per : parts;
-    uint64_t mine = 0;
-    for (uint32_t p = lo; p < hi; ++p) {
-        mine += sums[p];
-    }
+    uint64_t mine = run_sum(sums, lo, hi);
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     const uint64_t incl = wave_inclusive_scan(mine, lane);
     if (lane == 63) {
@@ -414,11 +449,7 @@ __global__ void __launch_bounds__(kSumThreads) partition_offsets_kernel(Policy p
         run += w < wave ? s_wave[w] : 0u;
         all += s_wave[w];
     }
-    for (uint32_t p = lo; p < hi; ++p) {
-        const uint64_t v = sums[p];
-        sums[p] = run;
-        run += v;
-    }
+    run_exclusive(sums, lo, hi, run);
     if (threadIdx.x == 0) {
         *total = n ? all : 0u;
     }
@@ -516,10 +547,7 @@ __global__ void __launch_bounds__(kSumThreads) compact_offsets_kernel(uint64_t* 
     __shared__ uint64_t s_wave[kSumThreads / 64];
     const uint32_t per = div_up(parts, kSumThreads);
     const uint32_t lo = threadIdx.x * per, hi = lo + per < parts ? lo + per : parts;
-    uint64_t mine = 0;
-    for (uint32_t p = lo; p < hi; ++p) {
-        mine += sums[p];
-    }
+    uint64_t mine = run_sum(sums, lo, hi);
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     const uint64_t incl = wave_inclusive_scan(mine, lane);
     if (lane == 63) {
@@ -531,11 +559,7 @@ __global__ void __launch_bounds__(kSumThreads) compact_offsets_kernel(uint64_t* 
         run += w < wave ? s_wave[w] : 0u;
         all += s_wave[w];
     }
-    for (uint32_t p = lo; p < hi; ++p) {
-        const uint64_t v = sums[p];
-        sums[p] = run;
-        run += v;
-    }
+    run_exclusive(sums, lo, hi, run);
     if (threadIdx.x == 0) {
         *total = all;
     }
