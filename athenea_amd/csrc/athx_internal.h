@@ -372,14 +372,20 @@ struct SegmentArgs {
 // strip, in the input's order within a strip; in the alternate buffers when
 // in_alt ends true. parts_cap bounds the partitions (div_up(cap, part) + a
 // partial one a view).
+// strips false: the digits are the key's own low `bits` bits (the depth
+// sort's segments, the frame's views' visible slots), seg_base unused.
 hipError_t launch_segment_sort(uint32_t* keys, uint32_t* values, const SegmentArgs& g, uint32_t parts_cap,
-                               uint32_t bits, const SortScratch& sc, hipStream_t s, bool& in_alt);
+                               uint32_t bits, const SortScratch& sc, hipStream_t s, bool& in_alt,
+                               bool strips = true);
 constexpr uint32_t kMaxSegmentViews = 256;
 struct SegmentBuild {
     uint32_t nviews;
-    const uint32_t* view_first;  // the view pass's scanned counts: view v's first rank at [v * view_stride]
+    // view_first null: view_entry holds every view's start. Else view v's
+    // first rank is view_first[v * view_stride], and view_entry holds the
+    // first entry of each view with ranks (written by the emission).
+    const uint32_t* view_first;
     uint32_t view_stride;
-    const uint32_t* view_entry;  // a view's first entry, written by the emission (views with ranks)
+    const uint32_t* view_entry;
     const uint64_t* visible_dev;
     uint32_t max_visible;
     const uint64_t* entry_total_dev;
@@ -393,12 +399,6 @@ struct SegmentBuild {
     uint64_t* binned_total;
 };
 hipError_t launch_strip_segments(const SegmentBuild& g, hipStream_t s);
-// One more stable pass over a sorted (key, slot) array whose digit is the
-// slot's view, slot / per_view (2 <= nviews <= 256): the pairs end grouped by
-// view, each view's in the order they had. Reads the alternate buffers when
-// in_alt says the array is there, writes the other pair, and flips in_alt.
-hipError_t launch_view_pass(uint32_t* keys, uint32_t* values, const uint64_t* n_dev, uint32_t cap, uint32_t per_view,
-                            uint32_t nviews, const SortScratch& sc, hipStream_t s, bool& in_alt);
 // The compaction's block of slots: its first step counts each block's
 // drawn slots into a u64 a block at the head of its status words.
 constexpr uint32_t kCompactTile = 4096;
@@ -409,10 +409,13 @@ hipError_t launch_settle_count(const DView* views_dev, uint32_t n, uint32_t tile
 // The visible slots in slot order with their depth keys (each key's top byte
 // holding the slot's strip-entry count, capped at 255); the count lands in
 // *visible_total (device). counted: the blocks' counts are already in the
-// status words (launch_settle_count).
+// status words (launch_settle_count). With view_start (per_view >= 2: slot =
+// view * per_view + Gaussian), view v's first compacted position lands in
+// view_start[v].
 hipError_t launch_compact(const uint32_t* touched, const uint32_t* depth_keys, uint32_t slots, uint32_t* keys_out,
                           uint32_t* slots_out, uint64_t* visible_total, uint32_t* status_words, hipStream_t s,
-                          bool counted = false);
+                          bool counted = false, uint32_t per_view = 0,
+                          uint32_t* view_start = nullptr);
 // Strip entries of the depth-sorted slots, scanned in that order:
 // offsets[k] is rank k's first entry; the total lands in *entry_total.
 hipError_t launch_entry_scan(const uint32_t* touched, const uint32_t* sorted_slots, const uint32_t* sorted_keys,

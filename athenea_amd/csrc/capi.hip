@@ -48,7 +48,9 @@ constexpr uint32_t kDepthPasses = 3;  // 24-bit depth keys (splat_project.slang:
 constexpr uint32_t kTotalPairs = 0, kTotalVisible = 1, kTotalBig = 2, kTotalEntries = 3, kTotalSegs = 4;
 constexpr uint32_t kTotalExact = 5;  // the exact counts' total (copy_frame's offsets)
 constexpr uint32_t kTotalBinned = 6;  // the entries a segmented frame's strip sort holds
-constexpr uint32_t kTotalWords = 7;
+constexpr uint32_t kTotalOrder = 7;   // the visible slots of the whole depth order (copy_frame's offsets)
+constexpr uint32_t kTotalDepthBinned = 8;  // the visible slots a grouped frame's depth segments hold
+constexpr uint32_t kTotalWords = 9;
 constexpr uint64_t kMaxPairs = uint64_t{1} << 30;  // pairs and entries are 32-bit, with room
 
 // The scratch of athx_exclusive_scan_noalloc: the single-pass scan's
@@ -81,6 +83,8 @@ struct athx_ctx {
     DeviceBuffer pair_vals, ranges, grad2d, counter_words, big_list, rank_tiles, slot_ids;
     DeviceBuffer depth_keys_sorted, depth_slots, depth_alt_keys, depth_alt_slots, depth_sort_meta;
     DeviceBuffer segments;  // a multi-view frame's strip segments (SegmentBuild's outputs)
+    DeviceBuffer depth_segments;  // and its depth sort's, with each view's first compacted position
+    DeviceBuffer order_keys, order_slots, order_alt_keys, order_alt_slots, order_meta;  // the whole depth order
     DeviceBuffer fp_rects, fp_masks, fp_big_base, fp_big_masks;
     // The binning: strip entries (keys, slots, the sort's alternates and
     // metadata), each strip's range and first segment, the entries' hit
@@ -106,10 +110,10 @@ struct athx_ctx {
     uint32_t last_instances = 0;  // the clouds the last frame projected (athx_render_instances)
     bool last_under = false;      // the last frame's walks ended at an under layer
     bool last_sorted_in_alt = false;  // the ranked slots are in depth_alt_slots
-    // Where the frame's whole depth order stands: with several views the
-    // ranks are grouped by view (launch_view_pass) into the other buffers,
-    // and the exact offsets are scanned in the depth order itself.
-    bool depth_order_in_alt = false;
+    // The last frame sorted each view's slots apart (its ranks grouped by
+    // view): the exact offsets, scanned in the frame's whole depth order,
+    // then sort that order on request (order_* buffers).
+    bool last_grouped = false;
     uint64_t last_true_pairs = 0;  // the last frame's pair total (an async frame's once athx_frame_pairs ran)
     uint64_t pair_hint = 0;        // the pair capacity a frame takes before it knows: 1.25x the largest seen
     uint64_t entry_hint = 0;       // the same for strip entries
@@ -153,7 +157,8 @@ std::vector<DeviceBuffer*> all_buffers(athx_ctx* ctx) {
             &ctx->big_list, &ctx->depth_keys_sorted, &ctx->depth_slots, &ctx->depth_alt_keys, &ctx->depth_alt_slots,
             &ctx->depth_sort_meta, &ctx->fp_rects, &ctx->fp_masks, &ctx->fp_big_base, &ctx->fp_big_masks,
             &ctx->strip_keys, &ctx->strip_slots, &ctx->strip_alt_keys, &ctx->strip_alt_slots, &ctx->strip_sort_meta,
-            &ctx->segments,
+            &ctx->segments, &ctx->depth_segments, &ctx->order_keys, &ctx->order_slots, &ctx->order_alt_keys,
+            &ctx->order_alt_slots, &ctx->order_meta,
             &ctx->strip_ranges, &ctx->segbase, &ctx->seg_map, &ctx->rank_fp, &ctx->masks, &ctx->seg_table, &ctx->exact, &ctx->exact_offsets,
             &ctx->exact_status, &ctx->api_sort_meta, &ctx->api_alt_keys, &ctx->api_alt_vals, &ctx->api_scan_status,
             &ctx->api_count, &ctx->ingest_keep, &ctx->ingest_dest, &ctx->ingest_total, &ctx->ref_records,
@@ -579,7 +584,11 @@ struct FrameShape {
 bool masks_in_keys(const FrameShape& fs) { return fs.strip_bits <= 16; }
 // The segmented strip sort's partitions: the entries' and a partial one a view.
 uint32_t segment_parts_cap(const FrameShape& fs) { return div_up(fs.entry_cap, kSegmentPart) + fs.nviews; }
-// Several views: the depth sort's ranks are grouped by view (launch_view_pass).
+// The depth sort's, over the views' visible slots.
+uint32_t depth_parts_cap(const FrameShape& fs) { return div_up(fs.slots, kSegmentPart) + fs.nviews; }
+// Several views: the depth sort orders each view's visible slots apart
+// (launch_segment_sort over the views' runs), so the ranks come out grouped
+// by view, each view's in depth order.
 bool grouped_by_view(const FrameShape& fs) { return fs.nviews >= 2 && fs.nviews <= 256 && fs.count >= 2; }
 
 void set_entry_cap(FrameShape& fs, uint32_t entries) {
@@ -663,8 +672,14 @@ athx_status reserve_front(athx_ctx* ctx, const FrameShape& fs) {
     ATHX_HIP(ctx, reserve(ctx, ctx->depth_slots, slots * 4 + 16), "depth sort slots");
     ATHX_HIP(ctx, reserve(ctx, ctx->depth_alt_keys, slots * 4 + 16), "depth sort keys");
     ATHX_HIP(ctx, reserve(ctx, ctx->depth_alt_slots, slots * 4 + 16), "depth sort slots");
-    ATHX_HIP(ctx, reserve(ctx, ctx->depth_sort_meta, sort_meta_words(fs.slots, kDepthPasses) * 4 + 16),
+    const bool grouped = grouped_by_view(fs);
+    ATHX_HIP(ctx, reserve(ctx, ctx->depth_sort_meta,
+                          sort_meta_words(fs.slots + (grouped ? fs.nviews * kSegmentPart : 0u), kDepthPasses) * 4 + 16),
              "depth sort metadata");
+    if (grouped) {
+        ATHX_HIP(ctx, reserve(ctx, ctx->depth_segments, ((size_t)depth_parts_cap(fs) + 5u * fs.nviews + 8) * 4),
+                 "depth segments");
+    }
     ATHX_HIP(ctx, reserve(ctx, ctx->strip_ranges, (size_t)fs.strips * 8 + 16), "strip ranges");
     ATHX_HIP(ctx, reserve(ctx, ctx->segbase, (size_t)fs.strips * 4 + 16), "strip segments");
     ATHX_HIP(ctx, reserve(ctx, ctx->rank_fp, slots * 32 + 32), "ranked footprints");
@@ -767,6 +782,26 @@ StripBins bins_of(athx_ctx* ctx, const FrameShape& fs, uint32_t n, uint32_t tile
     return b;
 }
 
+// A grouped frame's depth segments in ctx->depth_segments: part_seg (a word
+// a partition), then seg_start, seg_len, seg_part (a word a view), each
+// view's first compacted position (a word a view and the total), and the
+// partitions in use.
+SegmentArgs depth_segments_of(athx_ctx* ctx, const FrameShape& fs) {
+    uint32_t* w = as<uint32_t>(ctx->depth_segments);
+    SegmentArgs a{};
+    a.part_seg = w;
+    a.seg_start = w + depth_parts_cap(fs);
+    a.seg_len = a.seg_start + fs.nviews;
+    a.seg_part = a.seg_len + fs.nviews;
+    a.parts = a.seg_part + fs.nviews + (fs.nviews + 1);  // after the view starts
+    a.seg_base = nullptr;
+    return a;
+}
+
+uint32_t* view_start_of(athx_ctx* ctx, const FrameShape& fs) {
+    return const_cast<uint32_t*>(depth_segments_of(ctx, fs).seg_part) + fs.nviews;
+}
+
 // Projection, then the reference's depth order (TileRasterizer.cpp:1111-1137):
 // the visible slots compacted in slot order, stably sorted by their 24-bit
 // depth keys, and their strip entries scanned in that order. The visible
@@ -795,28 +830,49 @@ athx_status enqueue_front(athx_ctx* ctx, const std::vector<ProjectArgs>& pas, co
                                       as<uint32_t>(ctx->touched), as<uint32_t>(ctx->depth_keys),
                                       reinterpret_cast<uint64_t*>(ctx->scan_status.ptr), s),
              "settle");
+    // Several views: each view's visible slots are sorted apart, so the
+    // ranks come out grouped by view, each view's in depth order. A strip is
+    // one view's, so its entries -- and the pairs -- keep the reference's
+    // order, while the emission reads one view's records at a time (which the
+    // caches hold, where the frame's whole depth order would read them
+    // scattered over every view's) and a view's entries come out as one run.
+    const bool grouped = grouped_by_view(fs);
+    const SegmentArgs dseg = grouped ? depth_segments_of(ctx, fs) : SegmentArgs{};
     ATHX_HIP(ctx, launch_compact(as<uint32_t>(ctx->touched), as<uint32_t>(ctx->depth_keys), fs.slots,
                                  as<uint32_t>(ctx->depth_keys_sorted), as<uint32_t>(ctx->depth_slots),
-                                 totals + kTotalVisible, as<uint32_t>(ctx->scan_status), s, true),
+                                 totals + kTotalVisible, as<uint32_t>(ctx->scan_status), s, true,
+                                 grouped ? fs.count : 0u, grouped ? view_start_of(ctx, fs) : nullptr),
              "compact");
-    const SortScratch dsc = sort_scratch(ctx->depth_sort_meta, ctx->depth_alt_keys.ptr,
-                                         as<uint32_t>(ctx->depth_alt_slots), fs.slots, kDepthPasses);
+    const SortScratch dsc =
+        sort_scratch(ctx->depth_sort_meta, ctx->depth_alt_keys.ptr, as<uint32_t>(ctx->depth_alt_slots),
+                     fs.slots + (grouped ? fs.nviews * kSegmentPart : 0u), kDepthPasses);
     bool in_alt = false;  // three passes end in the alternate buffers: read them there
-    ATHX_HIP(ctx, launch_sort(as<uint32_t>(ctx->depth_keys_sorted), as<uint32_t>(ctx->depth_slots),
-                              totals + kTotalVisible, fs.slots, 0, kDepthBits, dsc, s, &in_alt),
-             "depth sort");
-    // Several views: one more pass groups the ranks by view, each view's in
-    // depth order. A strip is one view's, so its entries -- and the pairs --
-    // keep the same order, but the emission then reads one view's records at
-    // a time, which the caches hold, where the frame's depth order reads them
-    // scattered over every view's.
-    ctx->depth_order_in_alt = in_alt;
-    if (grouped_by_view(fs)) {
-        ATHX_HIP(ctx, launch_view_pass(as<uint32_t>(ctx->depth_keys_sorted), as<uint32_t>(ctx->depth_slots),
-                                       totals + kTotalVisible, fs.slots, fs.count, fs.nviews, dsc, s, in_alt),
-                 "view pass");
+    if (grouped) {
+        SegmentBuild g{};
+        g.nviews = fs.nviews;
+        g.view_entry = view_start_of(ctx, fs);  // every view's start, from the compaction
+        g.visible_dev = totals + kTotalVisible;
+        g.max_visible = fs.slots;
+        g.entry_total_dev = totals + kTotalVisible;
+        g.cap = fs.slots;
+        g.parts_cap = depth_parts_cap(fs);
+        g.part_seg = const_cast<uint32_t*>(dseg.part_seg);
+        g.seg_start = const_cast<uint32_t*>(dseg.seg_start);
+        g.seg_len = const_cast<uint32_t*>(dseg.seg_len);
+        g.seg_part = const_cast<uint32_t*>(dseg.seg_part);
+        g.parts = const_cast<uint32_t*>(dseg.parts);
+        g.binned_total = totals + kTotalDepthBinned;
+        ATHX_HIP(ctx, launch_strip_segments(g, s), "depth segments");
+        ATHX_HIP(ctx, launch_segment_sort(as<uint32_t>(ctx->depth_keys_sorted), as<uint32_t>(ctx->depth_slots), dseg,
+                                          depth_parts_cap(fs), kDepthBits, dsc, s, in_alt, false),
+                 "depth sort");
+    } else {
+        ATHX_HIP(ctx, launch_sort(as<uint32_t>(ctx->depth_keys_sorted), as<uint32_t>(ctx->depth_slots),
+                                  totals + kTotalVisible, fs.slots, 0, kDepthBits, dsc, s, &in_alt),
+                 "depth sort");
     }
     ctx->last_sorted_in_alt = in_alt;
+    ctx->last_grouped = grouped;
     if (timed) {
         ATHX_HIP(ctx, hipEventRecord(ctx->ev[2], s), "event");
     }
@@ -851,22 +907,19 @@ uint32_t* view_entry_of(athx_ctx* ctx, const FrameShape& fs) {
     return const_cast<uint32_t*>(segments_of(ctx, fs).seg_part) + fs.nviews;
 }
 
-// The view pass's scanned counts: view v's first rank at [v * stride].
-const uint32_t* view_first_of(athx_ctx* ctx, const FrameShape& fs) {
-    return sort_scratch(ctx->depth_sort_meta, ctx->depth_alt_keys.ptr, as<uint32_t>(ctx->depth_alt_slots), fs.slots,
-                        kDepthPasses)
-        .offsets;
-}
+// Each view's first rank: its first compacted position (the depth sort keeps
+// the views' runs where they were).
+const uint32_t* view_first_of(athx_ctx* ctx, const FrameShape& fs) { return view_start_of(ctx, fs); }
 
-// The views' runs of strip entries (launch_strip_segments), from the view
-// pass's counts and the first entries the emission recorded.
+// The views' runs of strip entries (launch_strip_segments), from the views'
+// first ranks and the first entries the emission recorded.
 athx_status enqueue_segments(athx_ctx* ctx, const FrameShape& fs, hipStream_t s) {
     uint64_t* totals = as<uint64_t>(ctx->totals);
     const SegmentArgs a = segments_of(ctx, fs);
     SegmentBuild g{};
     g.nviews = fs.nviews;
     g.view_first = view_first_of(ctx, fs);
-    g.view_stride = div_up(fs.slots, kSegmentPart);
+    g.view_stride = 1;
     g.view_entry = view_entry_of(ctx, fs);
     g.visible_dev = totals + kTotalVisible;
     g.max_visible = fs.slots;
@@ -903,8 +956,7 @@ athx_status enqueue_bins(athx_ctx* ctx, const FrameShape& fs, uint32_t n, uint32
                                              sorted_slots_of(ctx, ctx->last_sorted_in_alt), as<uint32_t>(ctx->offsets),
                                              totals + kTotalVisible, fs.slots, fs.entry_cap, as<float>(ctx->records),
                                              as<uint32_t>(ctx->strip_keys), as<uint32_t>(ctx->strip_slots), rank_tiles,
-                                             s, fs.segmented ? view_first_of(ctx, fs) : nullptr,
-                                             div_up(fs.slots, kSegmentPart),
+                                             s, fs.segmented ? view_first_of(ctx, fs) : nullptr, 1u,
                                              fs.segmented ? view_entry_of(ctx, fs) : nullptr),
                  "strip entries");
         if (fs.segmented) {
@@ -1730,9 +1782,35 @@ ATHX_API athx_status athx_copy_frame(athx_ctx* ctx, athx_frame_part part, void* 
         if (part == ATHX_PART_OFFSETS) {
             uint64_t* totals = as<uint64_t>(ctx->totals);
             ATHX_HIP(ctx, launch_fill_u32(as<uint32_t>(ctx->exact_offsets), 0u, slots, s), "clear offsets");
-            ATHX_HIP(ctx, launch_gather_scan(as<uint32_t>(ctx->exact), sorted_slots_of(ctx, ctx->depth_order_in_alt),
-                                             totals + kTotalVisible, (uint32_t)slots, as<uint32_t>(ctx->exact_offsets),
-                                             totals + kTotalExact, as<uint32_t>(ctx->exact_status), s),
+            const uint32_t* order = sorted_slots_of(ctx, ctx->last_sorted_in_alt);
+            const uint64_t* order_total = totals + kTotalVisible;
+            if (ctx->last_grouped) {
+                // The frame sorted each view apart: the whole depth order
+                // (TileRasterizer.cpp:1111-1123) is sorted here, from the
+                // slots' depth keys.
+                const uint32_t n32 = (uint32_t)slots;
+                ATHX_HIP(ctx, reserve(ctx, ctx->order_keys, slots * 4 + 16), "depth order");
+                ATHX_HIP(ctx, reserve(ctx, ctx->order_slots, slots * 4 + 16), "depth order");
+                ATHX_HIP(ctx, reserve(ctx, ctx->order_alt_keys, slots * 4 + 16), "depth order");
+                ATHX_HIP(ctx, reserve(ctx, ctx->order_alt_slots, slots * 4 + 16), "depth order");
+                ATHX_HIP(ctx, reserve(ctx, ctx->order_meta, sort_meta_words(n32, kDepthPasses) * 4 + 16),
+                         "depth order");
+                ATHX_HIP(ctx, launch_compact(as<uint32_t>(ctx->touched), as<uint32_t>(ctx->depth_keys), n32,
+                                             as<uint32_t>(ctx->order_keys), as<uint32_t>(ctx->order_slots),
+                                             totals + kTotalOrder, as<uint32_t>(ctx->exact_status), s),
+                         "depth order compaction");
+                const SortScratch osc = sort_scratch(ctx->order_meta, ctx->order_alt_keys.ptr,
+                                                     as<uint32_t>(ctx->order_alt_slots), n32, kDepthPasses);
+                bool oalt = false;
+                ATHX_HIP(ctx, launch_sort(as<uint32_t>(ctx->order_keys), as<uint32_t>(ctx->order_slots),
+                                          totals + kTotalOrder, n32, 0, kDepthBits, osc, s, &oalt),
+                         "depth order sort");
+                order = as<uint32_t>(oalt ? ctx->order_alt_slots : ctx->order_slots);
+                order_total = totals + kTotalOrder;
+            }
+            ATHX_HIP(ctx, launch_gather_scan(as<uint32_t>(ctx->exact), order, order_total, (uint32_t)slots,
+                                             as<uint32_t>(ctx->exact_offsets), totals + kTotalExact,
+                                             as<uint32_t>(ctx->exact_status), s),
                      "exact offsets");
             src = ctx->exact_offsets.ptr;
         }

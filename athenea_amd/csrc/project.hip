@@ -1833,12 +1833,13 @@ hipError_t launch_strip_mask_emit(const DView* views_dev, uint32_t n, uint32_t t
     return hipGetLastError();
 }
 
-// The strip entries' segments (one a view) of a frame whose ranks are
-// grouped by view: view v's run starts at the entry its first rank recorded
-// in the emission (a view with no visible rank has an empty run where the
-// next starts), cut where the capacity ends (the frame is then binned again
-// at its exact size); its partitions follow the earlier views', a partial one
-// closing each run. Every workgroup lays the runs out in LDS and maps its 256
+// The segments (one a view) of a multi-view frame's depth sort or strip
+// sort: view v's run starts at its given start (the depth sort's: where the
+// compaction placed the view's first visible slot) or at the entry its first
+// rank recorded in the emission (the strip sort's; a view with no visible
+// rank has an empty run where the next starts), cut where the capacity ends
+// (the frame is then binned again at its exact size); its partitions follow
+// the earlier views', a partial one closing each run. Every workgroup lays the runs out in LDS and maps its 256
 // partitions to their views; the first also writes the runs and the totals.
 __global__ void __launch_bounds__(256) strip_segments_kernel(SegmentBuild g) {
     __shared__ uint32_t s_start[kMaxSegmentViews + 1];
@@ -1851,8 +1852,11 @@ __global__ void __launch_bounds__(256) strip_segments_kernel(SegmentBuild g) {
         s_start[nv] = next;
         uint32_t next_rank = visible;
         for (uint32_t v = nv; v-- > 0;) {
-            const uint32_t R = g.view_first[(size_t)v * g.view_stride];
-            if (R < next_rank) {
+            // A view with ranks starts where its first rank's entries do;
+            // one without, where the next view does (view_first null: every
+            // view's start is given).
+            const uint32_t R = g.view_first ? g.view_first[(size_t)v * g.view_stride] : 0u;
+            if (!g.view_first || R < next_rank) {
                 const uint32_t e = g.view_entry[v];
                 next = e < next ? e : next;  // within the capacity, never past the next run
             }

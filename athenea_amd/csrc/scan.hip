@@ -569,7 +569,9 @@ __global__ void __launch_bounds__(kThreads) compact_write_kernel(const uint32_t*
                                                                  const uint32_t* __restrict__ depth_keys, uint32_t n,
                                                                  const uint64_t* __restrict__ offsets,
                                                                  uint32_t* __restrict__ keys_out,
-                                                                 uint32_t* __restrict__ slots_out) {
+                                                                 uint32_t* __restrict__ slots_out,
+                                                                 uint32_t per_view, uint32_t magic,
+                                                                 uint32_t* __restrict__ view_start) {
     __shared__ uint32_t s_count[kItems][kThreads / 64];
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     const uint64_t lt_mask = (1ull << lane) - 1ull;
@@ -588,6 +590,8 @@ __global__ void __launch_bounds__(kThreads) compact_write_kernel(const uint32_t*
     __syncthreads();
     // Exclusive prefix of (row, wavefront) counts, in slot order.
     uint64_t run = offsets[blockIdx.x];
+    // The first view starting at or after the block's first slot.
+    const uint32_t boundary = per_view >= kTile ? (uint32_t)((first + per_view - 1u) / per_view) : 0u;
 #pragma unroll
     for (uint32_t k = 0; k < kItems; ++k) {
         uint32_t before = 0, row = 0;
@@ -597,8 +601,27 @@ __global__ void __launch_bounds__(kThreads) compact_write_kernel(const uint32_t*
             before += w < wave ? c : 0u;
             row += c;
         }
+        const uint64_t i = first + (uint64_t)k * kThreads + threadIdx.x;
+        if (view_start && i < n) {
+            // A view's first slot (slot = view * per_view + Gaussian) records
+            // where the view's visible slots start: the visible ones before it.
+            // Views of a block's size or more have at most one first slot in
+            // a block, found once; smaller ones divide each slot.
+            bool starts;
+            uint32_t q;
+            if (per_view >= kTile) {
+                q = boundary;
+                starts = (uint64_t)boundary * per_view == i;
+            } else {
+                q = (uint32_t)(((uint64_t)(uint32_t)i * magic) >> 32);
+                q += (uint64_t)(q + 1u) * per_view <= i ? 1u : 0u;
+                starts = (uint64_t)q * per_view == i;
+            }
+            if (starts) {
+                view_start[q] = (uint32_t)(run + before + (uint64_t)__popcll(bal[k] & lt_mask));
+            }
+        }
         if (entries[k] != 0) {
-            const uint64_t i = first + (uint64_t)k * kThreads + threadIdx.x;
             const uint64_t at = run + before + (uint64_t)__popcll(bal[k] & lt_mask);
             const uint32_t count = entries[k] < 255u ? entries[k] : 255u;
             keys_out[at] = (depth_keys[i] & 0xFFFFFFu) | (count << 24);
@@ -620,7 +643,7 @@ hipError_t launch_scan(const uint32_t* in, uint32_t* out, uint64_t* total, uint3
 
 hipError_t launch_compact(const uint32_t* touched, const uint32_t* depth_keys, uint32_t slots, uint32_t* keys_out,
                           uint32_t* slots_out, uint64_t* visible_total, uint32_t* status_words, hipStream_t s,
-                          bool counted) {
+                          bool counted, uint32_t per_view, uint32_t* view_start) {
     static_assert(kTile == kCompactTile, "the settle pass counts the compaction's blocks");
     uint64_t* sums = reinterpret_cast<uint64_t*>(status_words);
     const uint32_t parts = div_up(slots ? slots : 1, kTile);
@@ -628,8 +651,9 @@ hipError_t launch_compact(const uint32_t* touched, const uint32_t* depth_keys, u
         hipLaunchKernelGGL(compact_count_kernel, dim3(parts), dim3(kThreads), 0, s, touched, slots, sums);
     }
     hipLaunchKernelGGL(compact_offsets_kernel, dim3(1), dim3(kSumThreads), 0, s, sums, parts, visible_total);
+    const uint32_t magic = per_view >= 2 ? (uint32_t)((1ull << 32) / per_view) : 0u;
     hipLaunchKernelGGL(compact_write_kernel, dim3(parts), dim3(kThreads), 0, s, touched, depth_keys, slots, sums,
-                       keys_out, slots_out);
+                       keys_out, slots_out, per_view, magic, per_view >= 2 ? view_start : nullptr);
     return hipGetLastError();
 }
 

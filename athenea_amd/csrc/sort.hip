@@ -309,12 +309,10 @@ __global__ void __launch_bounds__(kThreads) onesweep_kernel(const K* __restrict_
 // does and scatters from those slots. One more read of the keys a pass, but
 // no partition waits on another: on MI355X the look-back's coherent loads
 // and the ticket atomics cost more than that read.
-// Where a reduce-scan-scatter pass takes its digit from: bits of the key,
+// Where a reduce-scan-scatter pass takes its digit from: bits of the key, or
 // bits of the key's strip relative to its segment's first (the segmented
-// strip sort), or (the view pass) the view a slot value belongs to,
-// slot / per_view. `base` is the partition's segment's.
+// strip sort). `base` is the partition's segment's.
 struct KeyDigit {
-    static constexpr bool kFromValue = false;
     uint32_t shift, mask;
     template <typename K>
     __device__ uint32_t operator()(K key, uint32_t, uint32_t) const {
@@ -323,24 +321,10 @@ struct KeyDigit {
 };
 
 struct StripDigit {
-    static constexpr bool kFromValue = false;
     uint32_t shift, mask;
     template <typename K>
     __device__ uint32_t operator()(K key, uint32_t, uint32_t base) const {
         return ((((uint32_t)key & 0xFFFFu) - base) >> shift) & mask;
-    }
-};
-
-struct ViewDigit {
-    static constexpr bool kFromValue = true;
-    uint32_t per_view;  // >= 2
-    uint32_t magic;     // floor(2^32 / per_view)
-    template <typename K>
-    __device__ uint32_t operator()(K, uint32_t slot, uint32_t) const {
-        // slot * magic / 2^32 lies within one below slot / per_view.
-        uint32_t q = (uint32_t)(((uint64_t)slot * magic) >> 32);
-        q += (uint64_t)(q + 1u) * per_view <= slot ? 1u : 0u;
-        return q;
     }
 };
 
@@ -392,7 +376,7 @@ struct SegLayout {
         r.end = seg_start[sg] + seg_len[sg];
         r.cell = radix * seg_part[sg] + q;
         r.stride = div_up(seg_len[sg], kP);
-        r.base = seg_base[sg];
+        r.base = seg_base ? seg_base[sg] : 0u;
         return true;
     }
 };
@@ -411,17 +395,6 @@ __device__ inline void hist_add(uint32_t* s_hist, uint32_t d, bool ok) {
         }
     } else if (ok) {
         atomicAdd(&s_hist[d], 1u);
-    }
-}
-
-// One add a distinct digit of a wavefront (the lanes holding it matched by
-// ballots, the lowest adding their count): for passes of few digits, where
-// lanes adding one at a time would meet on the same words.
-__device__ inline void hist_add_matched(uint32_t* s_hist, uint32_t d, bool ok, uint32_t width) {
-    const uint64_t m = match_digit(d, width, ok);
-    const uint64_t below = (1ull << (threadIdx.x & 63u)) - 1ull;
-    if (ok && (m & below) == 0) {
-        atomicAdd(&s_hist[d], (uint32_t)__popcll(m));
     }
 }
 
@@ -447,25 +420,14 @@ __global__ void __launch_bounds__(kT) upsweep_kernel(const K* __restrict__ kin, 
         // once: a lane past the end reads the last key (in bounds, unbranched)
         // and adds nothing.
         K key[kItems];
-        uint32_t val[kItems];
 #pragma unroll
         for (uint32_t j = 0; j < kItems; ++j) {
-            const uint32_t at = min(pt.first + j * kT + tid, pt.end - 1u);
-            if constexpr (D::kFromValue) {
-                val[j] = vin[at];
-            } else {
-                key[j] = kin[at];
-            }
+            key[j] = kin[min(pt.first + j * kT + tid, pt.end - 1u)];
         }
 #pragma unroll
         for (uint32_t j = 0; j < kItems; ++j) {
             const uint32_t at = pt.first + j * kT + tid;
-            const bool ok = at < pt.end && at - pt.first < kP;
-            if constexpr (D::kFromValue) {
-                hist_add_matched(s_hist, digit(K{}, val[j], pt.base), ok, width);
-            } else {
-                hist_add(s_hist, digit(key[j], 0u, pt.base), ok);
-            }
+            hist_add(s_hist, digit(key[j], 0u, pt.base), at < pt.end && at - pt.first < kP);
         }
     }
     __syncthreads();
@@ -711,18 +673,10 @@ hipError_t launch_sort(uint32_t* keys, uint32_t* values, const uint64_t* n_dev, 
     return sort_impl(keys, values, n_dev, cap, lo_bit, key_bits, sc, s, in_alt);
 }
 
-hipError_t launch_segment_sort(uint32_t* keys, uint32_t* values, const SegmentArgs& g, uint32_t parts_cap,
-                               uint32_t bits, const SortScratch& sc, hipStream_t s, bool& in_alt) {
-    in_alt = false;
-    if (parts_cap == 0 || bits == 0) {
-        return hipSuccess;
-    }
-    if (bits > 16) {
-        return hipErrorInvalidValue;
-    }
+template <typename D>
+hipError_t segment_passes(uint32_t* keys, uint32_t* values, const SegmentArgs& g, uint32_t parts, uint32_t bits,
+                          const SortScratch& sc, hipStream_t s, bool& in_alt) {
     constexpr uint32_t kT = rs_threads<uint32_t>();
-    static_assert(kT * kItems == kSegmentPart, "a segment's partition is the pass's");
-    const uint32_t parts = parts_cap;
     const uint32_t passes = div_up(bits, 8);
     for (uint32_t p = 0; p < passes; ++p) {
         uint32_t shift, width;
@@ -731,51 +685,34 @@ hipError_t launch_segment_sort(uint32_t* keys, uint32_t* values, const SegmentAr
         uint32_t* vin = in_alt ? sc.values_alt : values;
         uint32_t* kout = in_alt ? keys : static_cast<uint32_t*>(sc.keys_alt);
         uint32_t* vout = in_alt ? values : sc.values_alt;
-        const StripDigit digit{shift, (1u << width) - 1u};
+        const D digit{shift, (1u << width) - 1u};
         const SegLayout layout{g.part_seg, g.seg_start, g.seg_len, g.seg_part, g.seg_base, g.parts, 1u << width};
-        hipLaunchKernelGGL((upsweep_kernel<uint32_t, StripDigit, kT, SegLayout>), dim3(parts), dim3(kT), 0, s, kin,
-                           vin, layout, digit, width, sc.counts);
+        hipLaunchKernelGGL((upsweep_kernel<uint32_t, D, kT, SegLayout>), dim3(parts), dim3(kT), 0, s, kin, vin, layout,
+                           digit, width, sc.counts);
         if (hipError_t e = launch_scan(sc.counts, sc.offsets, sc.scan_total, (1u << width) * parts, sc.scan_status, s);
             e != hipSuccess) {
             return e;
         }
-        hipLaunchKernelGGL((downsweep_kernel<uint32_t, StripDigit, kT, SegLayout>), dim3(parts), dim3(kT), 0, s, kin,
-                           vin, kout, vout, layout, digit, width, sc.offsets);
+        hipLaunchKernelGGL((downsweep_kernel<uint32_t, D, kT, SegLayout>), dim3(parts), dim3(kT), 0, s, kin, vin,
+                           kout, vout, layout, digit, width, sc.offsets);
         in_alt = !in_alt;
     }
     return hipGetLastError();
 }
 
-hipError_t launch_view_pass(uint32_t* keys, uint32_t* values, const uint64_t* n_dev, uint32_t cap, uint32_t per_view,
-                            uint32_t nviews, const SortScratch& sc, hipStream_t s, bool& in_alt) {
-    if (cap < 2 || nviews < 2) {
+hipError_t launch_segment_sort(uint32_t* keys, uint32_t* values, const SegmentArgs& g, uint32_t parts_cap,
+                               uint32_t bits, const SortScratch& sc, hipStream_t s, bool& in_alt, bool strips) {
+    in_alt = false;
+    if (parts_cap == 0 || bits == 0) {
         return hipSuccess;
     }
-    if (nviews > kRadix || per_view < 2 || cap > kValueMask) {
+    if (bits > (strips ? 16u : 32u)) {
         return hipErrorInvalidValue;
     }
-    uint32_t width = 1;
-    while ((1u << width) < nviews) {
-        ++width;
-    }
-    constexpr uint32_t kT = rs_threads<uint32_t>();
-    const uint32_t parts = div_up(cap, kT * kItems);
-    uint32_t* kin = in_alt ? static_cast<uint32_t*>(sc.keys_alt) : keys;
-    uint32_t* vin = in_alt ? sc.values_alt : values;
-    uint32_t* kout = in_alt ? keys : static_cast<uint32_t*>(sc.keys_alt);
-    uint32_t* vout = in_alt ? values : sc.values_alt;
-    const ViewDigit digit{per_view, (uint32_t)((1ull << 32) / per_view)};
-    const PlainLayout layout{n_dev, cap, parts};
-    hipLaunchKernelGGL((upsweep_kernel<uint32_t, ViewDigit, kT, PlainLayout>), dim3(parts), dim3(kT), 0, s, kin, vin,
-                       layout, digit, width, sc.counts);
-    if (hipError_t e = launch_scan(sc.counts, sc.offsets, sc.scan_total, (1u << width) * parts, sc.scan_status, s);
-        e != hipSuccess) {
-        return e;
-    }
-    hipLaunchKernelGGL((downsweep_kernel<uint32_t, ViewDigit, kT, PlainLayout>), dim3(parts), dim3(kT), 0, s, kin,
-                       vin, kout, vout, layout, digit, width, sc.offsets);
-    in_alt = !in_alt;
-    return hipGetLastError();
+    static_assert(rs_threads<uint32_t>() * kItems == kSegmentPart, "a segment's partition is the pass's");
+    return strips ? segment_passes<StripDigit>(keys, values, g, parts_cap, bits, sc, s, in_alt)
+                  : segment_passes<KeyDigit>(keys, values, g, parts_cap, bits, sc, s, in_alt);
 }
+
 
 }  // namespace athx
