@@ -158,7 +158,11 @@ __global__ void __launch_bounds__(TILE * TILE) blend_kernel(const DView* __restr
                                                             float depth_threshold) {
     constexpr uint32_t kThreads = TILE * TILE;
     __shared__ LdsBatch s;
-    const uint32_t gtile = xcd_band(blockIdx.x, gridDim.x);
+    // Tiles in dispatch order, not in bands an XCD: every XCD then walks the
+    // same view at once, so the records gathered at any time are one view's
+    // (a working set the 256 MB infinity cache holds, where eight views' do
+    // not), and the views' unequal loads even out over the XCDs.
+    const uint32_t gtile = blockIdx.x;
     const uint32_t v = view_of_tile(tile_bases, nviews, gtile);
     const DView& V = views[v];
     const uint32_t local = gtile - V.tile_base;
@@ -355,7 +359,7 @@ __global__ void __launch_bounds__(TILE * TILE) blend_ex_kernel(const DView* __re
     __shared__ LdsBatch s;
     __shared__ uint32_t s_slot[kMaxBatch];
     __shared__ uint32_t s_id[kMaxBatch];
-    const uint32_t gtile = xcd_band(blockIdx.x, gridDim.x);
+    const uint32_t gtile = blockIdx.x;
     const uint32_t v = view_of_tile(tile_bases, nviews, gtile);
     const DView& V = views[v];
     const uint32_t local = gtile - V.tile_base;

@@ -341,6 +341,7 @@ struct Part {
 struct PlainLayout {
     const uint64_t* n_dev;
     uint32_t cap, parts_cap;
+    __device__ uint32_t used(uint32_t kP) const { return div_up(count_of(n_dev, cap), kP); }
     __device__ bool part(uint32_t p, uint32_t kP, Part& r) const {
         r.first = p * kP;
         r.end = count_of(n_dev, cap);
@@ -366,6 +367,7 @@ struct SegLayout {
     const uint32_t* seg_base;  // subtracted from a strip key's low half
     const uint32_t* parts_dev;  // partitions in use
     uint32_t radix;
+    __device__ uint32_t used(uint32_t) const { return *parts_dev; }
     __device__ bool part(uint32_t p, uint32_t kP, Part& r) const {
         if (p >= *parts_dev) {
             return false;
@@ -396,6 +398,16 @@ __device__ inline void hist_add(uint32_t* s_hist, uint32_t d, bool ok) {
     } else if (ok) {
         atomicAdd(&s_hist[d], 1u);
     }
+}
+
+// The partition a workgroup takes: the partitions in use are dealt to the
+// XCDs in contiguous bands (workgroup b runs on XCD b % 8), so neighbouring
+// partitions -- whose runs of one digit meet in the output -- share an L2;
+// a workgroup past them keeps its own number (and finds no keys).
+template <typename L>
+__device__ inline uint32_t banded_part(const L& layout, uint32_t kP) {
+    const uint32_t used = layout.used(kP);
+    return blockIdx.x < used ? xcd_band(blockIdx.x, used) : blockIdx.x;
 }
 
 template <typename K, typename D, uint32_t kT, typename L>
@@ -455,7 +467,7 @@ __global__ void __launch_bounds__(kT) downsweep_kernel(const K* __restrict__ kin
     const uint32_t lane = tid & 63u;
     const uint32_t wave = tid >> 6;
     Part pt;
-    if (!layout.part(blockIdx.x, kP, pt) || pt.first >= pt.end) {
+    if (!layout.part(banded_part(layout, kP), kP, pt) || pt.first >= pt.end) {
         return;
     }
     for (uint32_t i = tid; i < kW * kRadix; i += kT) {
