@@ -787,10 +787,13 @@ __global__ void __launch_bounds__(256) ranges_kernel(const uint32_t* __restrict_
 // reference's zero count: culled, no tile (splat_project.slang:764-776).
 // The same pass counts the drawn slots of each block of kCompactTile slots
 // into sums[block], the compaction's first step.
-// A workgroup a block of slots, 1024 lanes, so that a small frame still
-// spreads its unsettled footprints over many wavefronts.
-constexpr uint32_t kSettleThreads = 1024;
+// A workgroup a block of slots: 1024 lanes in a small frame, so that it
+// still spreads its unsettled footprints over many wavefronts; 256 in a
+// large one (kSettleWide slots and up), whose workgroups are then fewer
+// wavefronts to launch for the same slots.
+constexpr uint32_t kSettleWide = 1u << 24;
 
+template <uint32_t kSettleThreads>
 __global__ void __launch_bounds__(kSettleThreads) settle_count_kernel(const DView* __restrict__ views, uint32_t n, uint32_t T,
                                                            uint32_t slots, const float* __restrict__ records,
                                                            uint32_t* __restrict__ touched, uint32_t* __restrict__ keys,
@@ -1725,8 +1728,13 @@ hipError_t launch_settle_count(const DView* views_dev, uint32_t n, uint32_t tile
     if (slots == 0) {
         return hipSuccess;
     }
-    hipLaunchKernelGGL(settle_count_kernel, dim3(div_up(slots, kCompactTile)), dim3(kSettleThreads), 0, s,
-                       views_dev, n, tile, slots, records, touched, depth_keys, sums);
+    if (slots >= kSettleWide) {
+        hipLaunchKernelGGL(settle_count_kernel<256>, dim3(div_up(slots, kCompactTile)), dim3(256), 0, s, views_dev, n,
+                           tile, slots, records, touched, depth_keys, sums);
+    } else {
+        hipLaunchKernelGGL(settle_count_kernel<1024>, dim3(div_up(slots, kCompactTile)), dim3(1024), 0, s, views_dev,
+                           n, tile, slots, records, touched, depth_keys, sums);
+    }
     return hipGetLastError();
 }
 

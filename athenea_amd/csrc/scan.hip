@@ -576,12 +576,18 @@ __global__ void __launch_bounds__(kThreads) compact_write_kernel(const uint32_t*
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     const uint64_t lt_mask = (1ull << lane) - 1ull;
     const uint64_t first = (uint64_t)blockIdx.x * kTile;
-    uint32_t entries[kItems];
+    // Every slot's entry count and depth key read at once, before any is
+    // waited on (the keys of slots that are not drawn are read and dropped).
+    uint32_t entries[kItems], dkey[kItems];
     uint64_t bal[kItems];
 #pragma unroll
     for (uint32_t k = 0; k < kItems; ++k) {
         const uint64_t i = first + (uint64_t)k * kThreads + threadIdx.x;
         entries[k] = i < n ? touched[i] : 0u;
+        dkey[k] = i < n ? depth_keys[i] : 0u;
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < kItems; ++k) {
         bal[k] = __ballot(entries[k] != 0);
         if (lane == 0) {
             s_count[k][wave] = (uint32_t)__popcll(bal[k]);
@@ -624,7 +630,7 @@ __global__ void __launch_bounds__(kThreads) compact_write_kernel(const uint32_t*
         if (entries[k] != 0) {
             const uint64_t at = run + before + (uint64_t)__popcll(bal[k] & lt_mask);
             const uint32_t count = entries[k] < 255u ? entries[k] : 255u;
-            keys_out[at] = (depth_keys[i] & 0xFFFFFFu) | (count << 24);
+            keys_out[at] = (dkey[k] & 0xFFFFFFu) | (count << 24);
             slots_out[at] = (uint32_t)i;
         }
         run += row;
