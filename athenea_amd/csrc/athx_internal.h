@@ -248,13 +248,12 @@ struct StripBins {
 };
 // The strip entries with their tiles decided (frames of at most 65 536
 // strips): key = strip | 16-bit mask of its touched tiles << 16, value = the
-// slot; the binning then reads only these.
+// slot; the binning then reads only these. view_entry, given when the ranks
+// are grouped by view, gets each view's first entry (from its first rank).
 hipError_t launch_strip_mask_emit(const DView* views_dev, uint32_t n, uint32_t tile, const uint32_t* sorted_slots,
                                   const uint32_t* offsets, const uint64_t* visible_dev, uint32_t max_visible,
                                   uint32_t cap, const float* records, uint32_t* strip_keys, uint32_t* strip_slots,
-                                  uint32_t* rank_tiles, hipStream_t s,
-                                  const uint32_t* view_first = nullptr, uint32_t view_stride = 0,
-                                  uint32_t* view_entry = nullptr);
+                                  uint32_t* rank_tiles, hipStream_t s, uint32_t* view_entry = nullptr);
 // Each strip's segments, scanned: segbase[s] and the total in *seg_total.
 hipError_t launch_seg_scan(const uint32_t* strip_ranges, uint32_t strips, uint32_t* segbase, uint64_t* seg_total,
                            uint32_t* status_words, hipStream_t s);

@@ -956,8 +956,7 @@ athx_status enqueue_bins(athx_ctx* ctx, const FrameShape& fs, uint32_t n, uint32
                                              sorted_slots_of(ctx, ctx->last_sorted_in_alt), as<uint32_t>(ctx->offsets),
                                              totals + kTotalVisible, fs.slots, fs.entry_cap, as<float>(ctx->records),
                                              as<uint32_t>(ctx->strip_keys), as<uint32_t>(ctx->strip_slots), rank_tiles,
-                                             s, fs.segmented ? view_first_of(ctx, fs) : nullptr, 1u,
-                                             fs.segmented ? view_entry_of(ctx, fs) : nullptr),
+                                             s, fs.segmented ? view_entry_of(ctx, fs) : nullptr),
                  "strip entries");
         if (fs.segmented) {
             if (athx_status st = enqueue_segments(ctx, fs, s); st != ATHX_OK) {

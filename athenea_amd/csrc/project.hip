@@ -1079,25 +1079,32 @@ __global__ void __launch_bounds__(kEmitThreads) strip_mask_emit_kernel(const DVi
                                                                        uint32_t* __restrict__ keys_out,
                                                                        uint32_t* __restrict__ slots_out,
                                                                        uint32_t* __restrict__ rank_tiles,
-                                                                       const uint32_t* __restrict__ view_first,
-                                                                       uint32_t view_stride,
                                                                        uint32_t* __restrict__ view_entry) {
     __shared__ uint32_t s_end[kEmitThreads];
     __shared__ uint32_t s_tiles[kEmitThreads]; // rank: tiles its entries' masks hold (counting frames)
     __shared__ float4 s_geo[kEmitThreads];   // rank: centre x, y, conic a, b
     __shared__ float4 s_geo2[kEmitThreads];  // rank: conic c, cut power, 1 / a, 1 / c
-    __shared__ uint4 s_int[kEmitThreads];    // rank: slot, first | last tile column << 16, first row, first strip
-    __shared__ uint4 s_int2[kEmitThreads];   // rank: strips a row, strips between rows, view, first strip column
+    // (Each rank's strips, first strip column and the view's strips a row
+    // are formed again from these where an entry needs them: the smaller a
+    // rank's LDS, the more workgroups a CU holds to cover the gathers' wait.)
+    __shared__ uint4 s_int[kEmitThreads];    // rank: slot, first | last tile column << 16,
+                                             // first row | strips a row << 16, first strip
     __shared__ float4 s_slice[kEmitThreads]; // rank: slice terms det, sa, sc; view width | height << 16
     __shared__ uint32_t s_own[kEmitThreads]; // step position: step << 8 | (rank + 1) of the run starting there
     const uint32_t lane = threadIdx.x & 63u, wbase = threadIdx.x & ~63u;
     const uint32_t visible = *visible_dev < max_visible ? (uint32_t)*visible_dev : max_visible;
     const uint32_t k = blockIdx.x * kEmitThreads + threadIdx.x;
     const uint32_t k0 = k - lane;
-    uint32_t count = 0;
+    uint32_t count = 0, view = 0;
+    bool opens = false;  // the first rank of its view
     if (k < visible) {
         const uint32_t slot = sorted_slots[k];
-        const uint32_t view = slot / n;
+        // With view_entry given the ranks are grouped by view, so a rank
+        // opens its view where the rank before it is another view's: read
+        // beside the rank's own slot, not after it.
+        const uint32_t before = view_entry && k != 0 ? sorted_slots[k - 1u] : 0u;
+        view = slot / n;
+        opens = k == 0 || before / n != view;
         const DView& V = views[view];
         const float4* rec = reinterpret_cast<const float4*>(records + (size_t)slot * kRecordFloats);
         const float4 screen = rec[0], conic = rec[1];
@@ -1109,8 +1116,8 @@ __global__ void __launch_bounds__(kEmitThreads) strip_mask_emit_kernel(const DVi
         count = h * ns;
         s_geo[threadIdx.x] = make_float4(screen.x, screen.y, conic.x, conic.y);
         s_geo2[threadIdx.x] = make_float4(conic.z, screen.w, recip_of(conic.x), recip_of(conic.z));
-        s_int[threadIdx.x] = make_uint4(slot, x0 | ((x0 + w - 1u) << 16), y0, V.strip_base + y0 * V.strips_x + sx0);
-        s_int2[threadIdx.x] = make_uint4(ns, V.strips_x, view, sx0);
+        s_int[threadIdx.x] = make_uint4(slot, x0 | ((x0 + w - 1u) << 16), y0 | (V.strips_x << 16),
+                                        V.strip_base + y0 * V.strips_x + sx0);
         const SliceTerms S = slice_terms(conic.x, conic.y, conic.z);
         s_slice[threadIdx.x] = make_float4(S.det, S.sa, S.sc, __builtin_bit_cast(float, V.width | (V.height << 16)));
     }
@@ -1124,8 +1131,7 @@ __global__ void __launch_bounds__(kEmitThreads) strip_mask_emit_kernel(const DVi
     if (view_entry && k < visible) {
         // The first rank of its view records the view's first entry (the
         // segmented strip sort's run of that view).
-        const uint32_t view = s_int2[threadIdx.x].z;
-        if (view_first[(size_t)view * view_stride] == k) {
+        if (opens) {
             view_entry[view] = base + start;
         }
     }
@@ -1153,15 +1159,17 @@ __global__ void __launch_bounds__(kEmitThreads) strip_mask_emit_kernel(const DVi
             continue;
         }
         const uint32_t q = gi - (j != 0 ? ends[j - 1] : 0u);
-        const uint4 I = s_int[wbase + j], I2 = s_int2[wbase + j];
+        const uint4 I = s_int[wbase + j];
         const float4 geo = s_geo[wbase + j], geo2 = s_geo2[wbase + j], sl = s_slice[wbase + j];
-        // q / ns by a float estimate made exact (q < 2^24, ns <= 4096).
-        uint32_t row = (uint32_t)((float)q * __builtin_amdgcn_rcpf((float)I2.x));
-        row = __umul24(row, I2.x) > q ? row - 1u : row;
-        row = __umul24(row + 1u, I2.x) <= q ? row + 1u : row;
-        const uint32_t sxi = q - __umul24(row, I2.x);
-        const uint32_t tx0 = (I2.w + sxi) * kStripTiles;
         const uint32_t x0 = I.y & 0xFFFFu, x1 = I.y >> 16;
+        const uint32_t sx0 = x0 / kStripTiles, ns = x1 / kStripTiles - sx0 + 1u;
+        const uint32_t y0 = I.z & 0xFFFFu, strips_x = I.z >> 16;
+        // q / ns by a float estimate made exact (q < 2^24, ns <= 4096).
+        uint32_t row = (uint32_t)((float)q * __builtin_amdgcn_rcpf((float)ns));
+        row = __umul24(row, ns) > q ? row - 1u : row;
+        row = __umul24(row + 1u, ns) <= q ? row + 1u : row;
+        const uint32_t sxi = q - __umul24(row, ns);
+        const uint32_t tx0 = (sx0 + sxi) * kStripTiles;
         const uint32_t c0 = x0 > tx0 ? x0 : tx0;
         const uint32_t c1 = x1 < tx0 + kStripTiles - 1u ? x1 : tx0 + kStripTiles - 1u;
         BigFootprint g{};
@@ -1175,7 +1183,7 @@ __global__ void __launch_bounds__(kEmitThreads) strip_mask_emit_kernel(const DVi
         g.ic = geo2.w;
         const uint32_t wh = __builtin_bit_cast(uint32_t, sl.w), width = wh & 0xFFFFu, height = wh >> 16;
         const float b2 = 2.0f * g.b;
-        const BoxAxis Y = box_axis(I.z + row, T, height, g.cy, g.a, g.b, g.c, b2);
+        const BoxAxis Y = box_axis(y0 + row, T, height, g.cy, g.a, g.b, g.c, b2);
         const Decided d = decide_columns(g, SliceTerms{sl.x, sl.y, sl.z}, Y, T, width, tx0, c0, c1);
         uint32_t mask = d.sure;
         for (uint32_t t = d.test; t != 0; t &= t - 1u) {
@@ -1186,7 +1194,7 @@ __global__ void __launch_bounds__(kEmitThreads) strip_mask_emit_kernel(const DVi
         }
         const uint64_t at = (uint64_t)base + gi;
         if (at < cap) {
-            keys_out[at] = (I.w + __umul24(row, I2.y) + sxi) | (mask << 16);
+            keys_out[at] = (I.w + __umul24(row, strips_x) + sxi) | (mask << 16);
             slots_out[at] = I.x;
         }
         if (rank_tiles) {
@@ -1830,14 +1838,13 @@ hipError_t launch_strip_count(const StripBins& b, hipStream_t s) {
 hipError_t launch_strip_mask_emit(const DView* views_dev, uint32_t n, uint32_t tile, const uint32_t* sorted_slots,
                                   const uint32_t* offsets, const uint64_t* visible_dev, uint32_t max_visible,
                                   uint32_t cap, const float* records, uint32_t* strip_keys, uint32_t* strip_slots,
-                                  uint32_t* rank_tiles, hipStream_t s, const uint32_t* view_first,
-                                  uint32_t view_stride, uint32_t* view_entry) {
+                                  uint32_t* rank_tiles, hipStream_t s, uint32_t* view_entry) {
     if (max_visible == 0 || cap == 0) {
         return hipSuccess;
     }
     hipLaunchKernelGGL(strip_mask_emit_kernel, dim3(div_up(max_visible, kEmitThreads)), dim3(kEmitThreads), 0, s,
                        views_dev, n, tile, sorted_slots, offsets, visible_dev, max_visible, cap, records,
-                       strip_keys, strip_slots, rank_tiles, view_first, view_stride, view_entry);
+                       strip_keys, strip_slots, rank_tiles, view_entry);
     return hipGetLastError();
 }
 
