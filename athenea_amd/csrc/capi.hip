@@ -53,9 +53,9 @@ constexpr uint32_t kTotalDepthBinned = 8;  // the visible slots a grouped frame'
 constexpr uint32_t kTotalWords = 9;
 constexpr uint64_t kMaxPairs = uint64_t{1} << 30;  // pairs and entries are 32-bit, with room
 
-// The scratch of athx_exclusive_scan_noalloc: the single-pass scan's
-// look-back status words for up to kNoAllocScanMax elements (two words a
-// partition of 4096 and two more), held by each device the library runs on
+// The scratch of athx_exclusive_scan_noalloc: the scan's look-back status
+// words, or a long scan's partition sums, for up to kNoAllocScanMax elements
+// (two words a partition of 4096 and two more), held by each device the library runs on
 // until the process ends. Host code never names it directly: athx_create
 // looks up its device address (hipGetSymbolAddress) and keeps it in the
 // context.
@@ -1867,8 +1867,9 @@ ATHX_API athx_status athx_exclusive_scan_noalloc(athx_ctx* ctx, const uint32_t* 
         ctx->error.assign("set device");
         return ATHX_DEVICE_FAILURE;
     }
-    // The status words are cleared by a fill kernel inside launch_scan before
-    // the scan reads them, so whatever the last call left does not matter.
+    // The status words are cleared by a fill kernel inside launch_scan (or,
+    // for a long scan, written whole as partition sums) before the scan reads
+    // them, so whatever the last call left does not matter.
     if (launch_scan(in, out, total, n, ctx->noalloc_scratch, static_cast<hipStream_t>(stream)) != hipSuccess) {
         ctx->error.assign("scan launch failed");
         return ATHX_DEVICE_FAILURE;

@@ -348,12 +348,13 @@ __global__ void clear_status_kernel(Policy pol, uint64_t* __restrict__ status, u
     }
 }
 
-// Long scans (past kLongScan elements) go in three launches instead: each
-// partition's sum, one workgroup's scan of the sums, then each partition
-// scanned from its offset. With tens of thousands of partitions in flight
-// the look-back above waits on predecessors that have not yet started; here
-// nothing waits, and the data is read twice.
-constexpr uint32_t kLongScan = 1u << 24;
+// Long scans (past kLongScan elements, 256 partitions) go in three launches
+// instead: each partition's sum, one workgroup's scan of the sums, then each
+// partition scanned from its offset. Past a few hundred partitions in flight
+// the look-back above waits on a chain of predecessors (measured on MI355X:
+// a 4 M-element scan 53 us that way, about 25 us this way); here nothing
+// waits, and the data is read twice.
+constexpr uint32_t kLongScan = 1u << 20;
 constexpr uint32_t kSumThreads = 1024;
 
 // A thread's run sums[lo, hi) of a single-workgroup scan, read eight loads
