@@ -53,10 +53,12 @@ __device__ inline bool tile_rect(uint32_t W, uint32_t H, uint32_t T, float cx, f
         x0 = 1; y0 = 1; x1 = 0; y1 = 0;
         return false;
     }
-    x0 = (uint32_t)fmax_(lx, 0.0f) / T;
-    y0 = (uint32_t)fmax_(ly, 0.0f) / T;
-    x1 = (uint32_t)fmin_(hx, (float)W - 1.0f) / T;
-    y1 = (uint32_t)fmin_(hy, (float)H - 1.0f) / T;
+    // T is 8, 16 or 32: the quotient is a shift.
+    const uint32_t shift = (uint32_t)__builtin_ctz(T);
+    x0 = (uint32_t)fmax_(lx, 0.0f) >> shift;
+    y0 = (uint32_t)fmax_(ly, 0.0f) >> shift;
+    x1 = (uint32_t)fmin_(hx, (float)W - 1.0f) >> shift;
+    y1 = (uint32_t)fmin_(hy, (float)H - 1.0f) >> shift;
     return x0 <= x1 && y0 <= y1;
 }
 
@@ -317,8 +319,9 @@ __global__ void __launch_bounds__(256) project_kernel(ProjectArgs a, const DView
         // box of pixel centres. The rest are settled by settle_kernel. The
         // slot's word is its strip entries: one a (row, strip) of its
         // rectangle.
-        const uint32_t ctx = min(max((uint32_t)clamp_(cx, 0.0f, (float)(V.width - 1u)) / a.tile, x0), x1);
-        const uint32_t cty = min(max((uint32_t)clamp_(cy, 0.0f, (float)(V.height - 1u)) / a.tile, y0), y1);
+        const uint32_t tshift = (uint32_t)__builtin_ctz(a.tile);
+        const uint32_t ctx = min(max((uint32_t)clamp_(cx, 0.0f, (float)(V.width - 1u)) >> tshift, x0), x1);
+        const uint32_t cty = min(max((uint32_t)clamp_(cy, 0.0f, (float)(V.height - 1u)) >> tshift, y0), y1);
         const bool settled = tile_touched(V.width, V.height, a.tile, cx, cy, ka, kb, kc, cutoff, ctx, cty);
         const uint32_t entries = h * (x1 / kStripTiles - x0 / kStripTiles + 1u);
         // Colour from this eye, in the cloud's space (splat_project.slang:781-785).
