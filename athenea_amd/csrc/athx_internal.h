@@ -139,8 +139,10 @@ __device__ inline float min_power_over_box(float a, float b, float c, float lox,
     return best;
 }
 
-// The reciprocals the strip classifier's slice extents take, formed once a footprint.
-__device__ inline float recip_of(float v) { return v > 1e-12f ? 1.0f / v : 0.0f; }
+// The reciprocals the strip classifier's slice extents take, formed once a
+// footprint with the hardware reciprocal (within an ulp of 1 / v): they only
+// place the extents, whose margin is hundreds of ulps, never a tile test.
+__device__ inline float recip_of(float v) { return v > 1e-12f ? __builtin_amdgcn_rcpf(v) : 0.0f; }
 
 // MI355X deals a launch's workgroups to its 8 XCDs round robin (workgroup b
 // to XCD b % 8), and each XCD has its own L2. Work item xcd_band(b, n) gives

@@ -21,8 +21,19 @@ f32 = np.float32
 EPS = 2.0 ** -24
 
 
-def recip(v):
-    return np.where(v > f32(1e-12), f32(1.0) / v, f32(0.0)).astype(f32)
+def ulp_jitter(r, rng):
+    """r moved by -1, 0 or +1 unit in the last place, element by element: the
+    kernels take these reciprocals with the hardware's (within an ulp of the
+    correctly rounded one), and the rule must hold for any of them."""
+    k = rng.integers(-1, 2, size=r.shape)
+    down = np.nextafter(r, f32(-np.inf)).astype(f32)
+    up = np.nextafter(r, f32(np.inf)).astype(f32)
+    return np.where(k < 0, down, np.where(k > 0, up, r)).astype(f32)
+
+
+def recip(v, rng):
+    r = np.where(v > f32(1e-12), f32(1.0) / np.where(v > f32(1e-12), v, f32(1)), f32(0.0)).astype(f32)
+    return np.where(v > f32(1e-12), ulp_jitter(r, rng), f32(0.0)).astype(f32)
 
 
 def min_power(a, b, c, lox, loy, hix, hiy):
@@ -89,12 +100,13 @@ def classify(seed, k_slack, n=300000, W=1920, H=1080, T=16):
     det = (a.astype(np.float64) * c.astype(np.float64) - b.astype(np.float64) ** 2)
     usable = (a > f32(1e-9)) & (c > f32(1e-9)) & (det > 0)
     detf = det.astype(f32)
-    inv_det = np.where(usable, f32(1.0) / np.where(usable, detf, f32(1)), f32(0)).astype(f32)
+    inv_det = np.where(usable, ulp_jitter((f32(1.0) / np.where(usable, detf, f32(1))).astype(f32), rng),
+                       f32(0)).astype(f32)
     X = np.maximum(np.abs(lox), np.abs(hix))
     Y = np.maximum(np.abs(loy), np.abs(hiy))
     M = (a * X * X + f32(2.0) * np.abs(b) * X * Y + c * Y * Y).astype(f32)
     slack = (f32(k_slack * EPS) * M).astype(f32)
-    ia, ic = recip(a), recip(c)
+    ia, ic = recip(a, rng), recip(c, rng)
     ok_lo, xa_lo, xb_lo = band_extent(a, b, c, ia, ic, detf, inv_det, (cut - slack).astype(f32), loy, hiy)
     ok_hi, xa_hi, xb_hi = band_extent(a, b, c, ia, ic, detf, inv_det, (cut + slack).astype(f32), loy, hiy)
     sure = usable & ok_lo & (hix >= xa_lo) & (lox <= xb_lo)
