@@ -469,6 +469,18 @@ hipError_t launch_compare_hdr(const float* a, const float* b, uint32_t width, ui
 // side (sides and offsets of the eight levels when not null), and the
 // launch that writes the texel words and the 16 float4 coefficients.
 uint32_t sky_texels(uint32_t base_side, uint32_t* sides, uint32_t* offsets);
+// The sun's extraction (sky.hip): a level down, the histogram and peak, the
+// sun's sums, and the residual equirect. rows: (sin theta, cos theta) a row;
+// cols: (cos phi, sin phi) a column; row_weight: sin theta in 2^-24 units.
+hipError_t launch_sky_half(const float* src, uint32_t w, uint32_t h, float* dst, uint32_t dw, uint32_t dh,
+                           hipStream_t s);
+hipError_t launch_sky_sun_scan(const float* img, uint32_t w, uint32_t h, const uint32_t* row_weight,
+                               unsigned long long* hist, unsigned long long* peak, hipStream_t s);
+hipError_t launch_sky_sun_sum(const float* img, uint32_t w, uint32_t h, const float* rows, const float* cols,
+                              float area, float threshold, const float axis[3], float cos_widest, double* sums,
+                              hipStream_t s);
+hipError_t launch_sky_residual(const float* img, uint32_t w, uint32_t h, const float* rows, const float* cols,
+                               float threshold, const float axis[3], float cos_widest, float* out, hipStream_t s);
 hipError_t launch_prepare_sky(const float* equirect, uint32_t width, uint32_t height, uint32_t base_side,
                               uint32_t* texels, float* sh, hipStream_t s);
 

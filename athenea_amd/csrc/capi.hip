@@ -18,6 +18,7 @@
 #include <cstdio>
 #include <cstring>
 #include <exception>
+#include <limits>
 #include <cmath>
 #include <new>
 #include <sstream>
@@ -81,6 +82,7 @@ struct athx_ctx {
 
     DeviceBuffer views, tile_bases, records, touched, depth_keys, offsets, scan_status, totals;
     DeviceBuffer pair_vals, ranges, grad2d, counter_words, big_list, rank_tiles, slot_ids;
+    DeviceBuffer sky_levels, sky_tables, sky_words;  // athx_sky_sun / athx_sky_residual
     DeviceBuffer depth_keys_sorted, depth_slots, depth_alt_keys, depth_alt_slots, depth_sort_meta;
     DeviceBuffer segments;  // a multi-view frame's strip segments (SegmentBuild's outputs)
     DeviceBuffer depth_segments;  // and its depth sort's, with each view's first compacted position
@@ -153,7 +155,7 @@ namespace {
 std::vector<DeviceBuffer*> all_buffers(athx_ctx* ctx) {
     return {&ctx->views, &ctx->tile_bases, &ctx->records, &ctx->touched, &ctx->depth_keys, &ctx->offsets,
             &ctx->scan_status, &ctx->totals, &ctx->pair_vals, &ctx->ranges, &ctx->grad2d, &ctx->counter_words,
-            &ctx->rank_tiles, &ctx->slot_ids,
+            &ctx->rank_tiles, &ctx->slot_ids, &ctx->sky_levels, &ctx->sky_tables, &ctx->sky_words,
             &ctx->big_list, &ctx->depth_keys_sorted, &ctx->depth_slots, &ctx->depth_alt_keys, &ctx->depth_alt_slots,
             &ctx->depth_sort_meta, &ctx->fp_rects, &ctx->fp_masks, &ctx->fp_big_base, &ctx->fp_big_masks,
             &ctx->strip_keys, &ctx->strip_slots, &ctx->strip_alt_keys, &ctx->strip_alt_slots, &ctx->strip_sort_meta,
@@ -2390,6 +2392,200 @@ ATHX_API athx_status athx_prepare_sky(athx_ctx* ctx, const float* equirect, uint
     ATHX_HIP(ctx, hipSetDevice(ctx->device), "set device");
     ATHX_HIP(ctx, launch_prepare_sky(equirect, width, height, base_side, texels, sh, nullptr), "prepare sky");
     ATHX_HIP(ctx, hipStreamSynchronize(nullptr), "prepare sky wait");
+    return ATHX_OK;
+} catch (const std::bad_alloc&) {
+    return fail(ctx, ATHX_OUT_OF_MEMORY, "host allocation failed");
+} catch (const std::exception& e) {
+    return fail(ctx, ATHX_INTERNAL_ERROR, e.what());
+}
+
+}  // extern "C"
+
+namespace {
+
+// env_sun.slang's constants: a texel is the sun at kSunOverMedian times the
+// sky's median and kSunOfPeak of its peak, within kSunWidest of the peak; a sky
+// whose peak is not kSunPeakOverMedian times its median has none.
+constexpr float kSunOverMedian = 16.0f, kSunOfPeak = 0.02f, kSunPeakOverMedian = 32.0f;
+constexpr double kSunWidest = 0.175;
+constexpr uint32_t kSunLevelWidth = 256;  // the level searched: at most this wide (env_sun.slang:18-20)
+
+// A level's direction tables, computed once in double: rows (sin theta,
+// cos theta), theta = pi (j + 0.5) / h from +y; columns (cos phi, sin phi),
+// phi = 2 pi (i + 0.5) / w from +x towards +z; and each row's sin theta in
+// 2^-24 units.
+void sky_tables(uint32_t w, uint32_t h, std::vector<float>& rows, std::vector<float>& cols,
+                std::vector<uint32_t>* weights) {
+    const double pi = 3.14159265358979323846;
+    rows.resize(2 * (size_t)h);
+    cols.resize(2 * (size_t)w);
+    if (weights) {
+        weights->resize(h);
+    }
+    for (uint32_t j = 0; j < h; ++j) {
+        const double theta = pi * ((double)j + 0.5) / (double)h;
+        rows[2 * j] = (float)std::sin(theta);
+        rows[2 * j + 1] = (float)std::cos(theta);
+        if (weights) {
+            (*weights)[j] = (uint32_t)std::llround(std::max(std::sin(theta), 0.0) * 16777216.0);
+        }
+    }
+    for (uint32_t i = 0; i < w; ++i) {
+        const double phi = 2.0 * pi * ((double)i + 0.5) / (double)w;
+        cols[2 * i] = (float)std::cos(phi);
+        cols[2 * i + 1] = (float)std::sin(phi);
+    }
+}
+
+athx_status check_sky_input(athx_ctx* ctx, const float* equirect, uint32_t width, uint32_t height) {
+    if (!equirect) {
+        return fail(ctx, ATHX_INVALID_ARGUMENT, "null equirect");
+    }
+    if (width == 0 || height == 0 || (uint64_t)width * height > (1u << 24)) {
+        return fail(ctx, ATHX_INVALID_ARGUMENT, "equirect size out of range (at most 2^24 texels)");
+    }
+    return ATHX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+ATHX_API athx_status athx_sky_sun(athx_ctx* ctx, const float* equirect, uint32_t width, uint32_t height,
+                                  float* sun)  try {
+    if (!ctx) {
+        return ATHX_INVALID_ARGUMENT;
+    }
+    ctx->error.clear();
+    if (athx_status st = check_sky_input(ctx, equirect, width, height); st != ATHX_OK) {
+        return st;
+    }
+    if (!sun) {
+        return fail(ctx, ATHX_INVALID_ARGUMENT, "null sun");
+    }
+    ATHX_HIP(ctx, hipSetDevice(ctx->device), "set device");
+    hipStream_t s = nullptr;
+    // The level searched and summed: halved until at most kSunLevelWidth wide.
+    uint32_t w = width, h = height, levels = 0;
+    size_t level_floats = 0;
+    while (w > kSunLevelWidth) {
+        w = std::max(w >> 1, 1u);
+        h = std::max(h >> 1, 1u);
+        level_floats += 3 * (size_t)w * h;
+        ++levels;
+    }
+    ATHX_HIP(ctx, reserve(ctx, ctx->sky_levels, level_floats * 4 + 16), "sky levels");
+    const float* img = equirect;
+    {
+        uint32_t sw = width, sh = height;
+        float* dst = as<float>(ctx->sky_levels);
+        for (uint32_t k = 0; k < levels; ++k) {
+            const uint32_t dw = std::max(sw >> 1, 1u), dh = std::max(sh >> 1, 1u);
+            ATHX_HIP(ctx, launch_sky_half(img, sw, sh, dst, dw, dh, s), "sky level");
+            img = dst;
+            dst += 3 * (size_t)dw * dh;
+            sw = dw;
+            sh = dh;
+        }
+    }
+    std::vector<float> rows, cols;
+    std::vector<uint32_t> weights;
+    sky_tables(w, h, rows, cols, &weights);
+    const size_t table_bytes = (rows.size() + cols.size()) * 4 + weights.size() * 4;
+    ATHX_HIP(ctx, reserve(ctx, ctx->sky_tables, table_bytes + 16), "sky tables");
+    float* d_rows = as<float>(ctx->sky_tables);
+    float* d_cols = d_rows + rows.size();
+    uint32_t* d_weights = reinterpret_cast<uint32_t*>(d_cols + cols.size());
+    ATHX_HIP(ctx, hipMemcpy(d_rows, rows.data(), rows.size() * 4, hipMemcpyHostToDevice), "sky tables");
+    ATHX_HIP(ctx, hipMemcpy(d_cols, cols.data(), cols.size() * 4, hipMemcpyHostToDevice), "sky tables");
+    ATHX_HIP(ctx, hipMemcpy(d_weights, weights.data(), weights.size() * 4, hipMemcpyHostToDevice), "sky tables");
+    // Words: 80 histogram bins, the peak, then the sun's 8 sums (doubles).
+    ATHX_HIP(ctx, reserve(ctx, ctx->sky_words, (80 + 1 + 8) * 8 + 16), "sky words");
+    unsigned long long* d_hist = as<unsigned long long>(ctx->sky_words);
+    unsigned long long* d_peak = d_hist + 80;
+    double* d_sums = reinterpret_cast<double*>(d_peak + 1);
+    ATHX_HIP(ctx, launch_fill_u32(reinterpret_cast<uint32_t*>(d_hist), 0u, 2 * (80 + 1 + 8), s), "sky words clear");
+    ATHX_HIP(ctx, launch_sky_sun_scan(img, w, h, d_weights, d_hist, d_peak, s), "sky scan");
+    std::vector<unsigned long long> hist(81);
+    ATHX_HIP(ctx, hipMemcpy(hist.data(), d_hist, 81 * 8, hipMemcpyDeviceToHost), "sky scan read");
+    // The median: the first half stop whose cumulative weight reaches half.
+    unsigned long long total = 0;
+    for (uint32_t b = 0; b < 80; ++b) {
+        total += hist[b];
+    }
+    float median = 0.0f;
+    unsigned long long below = 0;
+    for (uint32_t b = 0; b < 80; ++b) {
+        below += hist[b];
+        if (2 * below >= total && total > 0) {
+            median = (float)std::exp2((double)b * 0.5 - 20.0 + 0.25);
+            break;
+        }
+    }
+    const unsigned long long peak_word = hist[80];
+    const float peak = peak_word ? __builtin_bit_cast(float, (uint32_t)(peak_word >> 32)) : -1.0f;
+    const uint32_t at = peak_word ? 0xFFFFFFFFu - (uint32_t)(peak_word & 0xFFFFFFFFull) : 0u;
+    const uint32_t py = at / w, px = at - py * w;
+    const float axis[3] = {rows[2 * py] * cols[2 * px], rows[2 * py + 1], rows[2 * py] * cols[2 * px + 1]};
+    const float threshold = std::max(kSunOverMedian * median, kSunOfPeak * peak);
+    const bool found = peak_word != 0 && peak > kSunPeakOverMedian * std::max(median, 1.0e-9f);
+    double sums[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    const float cos_widest = (float)std::cos(kSunWidest);
+    if (found) {
+        const double pi = 3.14159265358979323846;
+        const float area = (float)(2.0 * pi * pi / ((double)w * (double)h));
+        ATHX_HIP(ctx, launch_sky_sun_sum(img, w, h, d_rows, d_cols, area, threshold, axis, cos_widest, d_sums, s),
+                 "sky sums");
+        ATHX_HIP(ctx, hipMemcpy(sums, d_sums, 8 * 8, hipMemcpyDeviceToHost), "sky sums read");
+    }
+    // env_sun.slang:236-261: the disc's cone from its solid angle, and a region
+    // spread past twice a disc's mean square angle left in the sky.
+    const double pi = 3.14159265358979323846;
+    const double solid = sums[7], weight = sums[6];
+    const double clen = std::sqrt(sums[3] * sums[3] + sums[4] * sums[4] + sums[5] * sums[5]);
+    const double disc = solid > 0.0 ? std::acos(std::min(std::max(1.0 - solid / (2.0 * pi), 0.0), 1.0)) : 0.0;
+    const double spread = weight > 0.0 ? 2.0 * std::max(1.0 - clen / weight, 0.0) : 0.0;
+    const bool compact = spread <= disc * disc + 1.0e-6;
+    const bool is_sun = found && solid > 0.0 && clen > 0.0 && compact;
+    for (int c = 0; c < 3; ++c) {
+        sun[c] = is_sun ? (float)((c == 0 ? sums[3] : c == 1 ? sums[4] : sums[5]) / clen) : axis[c];
+        sun[4 + c] = (float)sums[c];
+    }
+    sun[3] = is_sun ? std::max((float)disc, 1.0e-4f) : 0.0f;
+    sun[7] = threshold;
+    return ATHX_OK;
+} catch (const std::bad_alloc&) {
+    return fail(ctx, ATHX_OUT_OF_MEMORY, "host allocation failed");
+} catch (const std::exception& e) {
+    return fail(ctx, ATHX_INTERNAL_ERROR, e.what());
+}
+
+ATHX_API athx_status athx_sky_residual(athx_ctx* ctx, const float* equirect, uint32_t width, uint32_t height,
+                                       const float* sun, float* residual)  try {
+    if (!ctx) {
+        return ATHX_INVALID_ARGUMENT;
+    }
+    ctx->error.clear();
+    if (athx_status st = check_sky_input(ctx, equirect, width, height); st != ATHX_OK) {
+        return st;
+    }
+    if (!sun || !residual) {
+        return fail(ctx, ATHX_INVALID_ARGUMENT, "null sun or residual");
+    }
+    ATHX_HIP(ctx, hipSetDevice(ctx->device), "set device");
+    std::vector<float> rows, cols;
+    sky_tables(width, height, rows, cols, nullptr);
+    ATHX_HIP(ctx, reserve(ctx, ctx->sky_tables, (rows.size() + cols.size()) * 4 + 16), "sky tables");
+    float* d_rows = as<float>(ctx->sky_tables);
+    float* d_cols = d_rows + rows.size();
+    ATHX_HIP(ctx, hipMemcpy(d_rows, rows.data(), rows.size() * 4, hipMemcpyHostToDevice), "sky tables");
+    ATHX_HIP(ctx, hipMemcpy(d_cols, cols.data(), cols.size() * 4, hipMemcpyHostToDevice), "sky tables");
+    // No sun (a half angle of 0): nothing is cut.
+    const float threshold = sun[3] > 0.0f ? sun[7] : std::numeric_limits<float>::infinity();
+    ATHX_HIP(ctx, launch_sky_residual(equirect, width, height, d_rows, d_cols, threshold, sun,
+                                      (float)std::cos(kSunWidest), residual, nullptr),
+             "sky residual");
+    ATHX_HIP(ctx, hipStreamSynchronize(nullptr), "sky residual wait");
     return ATHX_OK;
 } catch (const std::bad_alloc&) {
     return fail(ctx, ATHX_OUT_OF_MEMORY, "host allocation failed");

@@ -224,6 +224,144 @@ __global__ void __launch_bounds__(kSkyThreads) sky_sh_kernel(const float* __rest
     }
 }
 
+// THE SUN, TAKEN OUT OF THE SKY (env_sun.slang:154-262, env_residual.slang:
+// 55-78): the brightest compact source of an equirect, found on a box-filtered
+// level at most 256 texels wide, and the equirect with its texels set to
+// nothing. The texel directions and the rows' solid-angle weights come in as
+// tables the host computes once in double (sin and cos nowhere on the device),
+// so the host's restatement decides every texel the same way.
+
+// One level down: each texel the mean of its 2 x 2 block, read clamped at an
+// odd edge.
+__global__ void __launch_bounds__(kSkyThreads) sky_half_kernel(const float* __restrict__ src, uint32_t w, uint32_t h,
+                                                               float* __restrict__ dst, uint32_t dw, uint32_t dh) {
+    const uint32_t t = blockIdx.x * kSkyThreads + threadIdx.x;
+    if (t >= dw * dh) {
+        return;
+    }
+    const uint32_t y = t / dw, x = t - y * dw;
+    const uint32_t x0 = min(2u * x, w - 1u), x1 = min(2u * x + 1u, w - 1u);
+    const uint32_t y0 = min(2u * y, h - 1u), y1 = min(2u * y + 1u, h - 1u);
+    for (int c = 0; c < 3; ++c) {
+        const float a = src[((size_t)y0 * w + x0) * 3 + c], b = src[((size_t)y0 * w + x1) * 3 + c];
+        const float e = src[((size_t)y1 * w + x0) * 3 + c], f = src[((size_t)y1 * w + x1) * 3 + c];
+        dst[(size_t)t * 3 + c] = 0.25f * ((a + b) + (e + f));
+    }
+}
+
+// env_sun.slang:154-156's grey, as luminance.
+__device__ inline float sky_lum(const float* rgb) { return rgb[0] * 0.2126f + rgb[1] * 0.7152f + rgb[2] * 0.0722f; }
+
+// The log-luminance histogram (80 half stops from 2^-20, each texel weighed by
+// its row's fixed-point sin theta, so the sums are exact in any order) and the
+// brightest texel: peak = max of lum bits << 32 | ~index (the first of equal
+// ones, as a scan in raster order keeps).
+__global__ void __launch_bounds__(kSkyThreads) sky_sun_scan_kernel(const float* __restrict__ img, uint32_t w,
+                                                                   uint32_t h, const uint32_t* __restrict__ row_weight,
+                                                                   unsigned long long* __restrict__ hist,
+                                                                   unsigned long long* __restrict__ peak) {
+    __shared__ unsigned long long s_hist[80];
+    if (threadIdx.x < 80) {
+        s_hist[threadIdx.x] = 0;
+    }
+    __syncthreads();
+    const uint32_t t = blockIdx.x * kSkyThreads + threadIdx.x;
+    unsigned long long best = 0;
+    if (t < w * h) {
+        const uint32_t y = t / w;
+        const float lum = sky_lum(img + (size_t)t * 3);
+        const float l2 = athx_logf(fmaxf(lum, 1.0e-9f)) * 1.44269504088896341f;
+        const float fb = fminf(fmaxf((l2 + 20.0f) * 2.0f, 0.0f), 79.0f);
+        if (row_weight[y] != 0u) {
+            atomicAdd(&s_hist[(uint32_t)fb], (unsigned long long)row_weight[y]);
+            best = ((unsigned long long)__float_as_uint(fmaxf(lum, 0.0f)) << 32) | (0xFFFFFFFFull - t);
+        }
+    }
+    for (uint32_t d = 32; d >= 1; d >>= 1) {
+        const unsigned long long o = __shfl_xor(best, d, 64);
+        best = o > best ? o : best;
+    }
+    if ((threadIdx.x & 63u) == 0 && best != 0) {
+        atomicMax(peak, best);
+    }
+    __syncthreads();
+    if (threadIdx.x < 80 && s_hist[threadIdx.x] != 0) {
+        atomicAdd(&hist[threadIdx.x], s_hist[threadIdx.x]);
+    }
+}
+
+// The sun's texels summed (one workgroup, a lane taking every 256th texel,
+// then a fixed tree; FP64): its irradiance, its light-weighted direction, the
+// light weight and the solid angle. dir = (sin theta cos phi, cos theta,
+// sin theta sin phi) from the float tables; omega = sin theta * area.
+__global__ void __launch_bounds__(kSkyThreads) sky_sun_sum_kernel(const float* __restrict__ img, uint32_t w,
+                                                                  uint32_t h, const float* __restrict__ rows,
+                                                                  const float* __restrict__ cols, float area,
+                                                                  float threshold, float ax, float ay, float az,
+                                                                  float cos_widest, double* __restrict__ sums) {
+    __shared__ double s_sum[kSkyThreads][8];
+    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (uint32_t t = threadIdx.x; t < w * h; t += kSkyThreads) {
+        const uint32_t y = t / w, x = t - y * w;
+        const float st = rows[2u * y], ct = rows[2u * y + 1u];
+        if (!(st > 0.0f)) {
+            continue;
+        }
+        const float dx = st * cols[2u * x], dy = ct, dz = st * cols[2u * x + 1u];
+        const float* rgb = img + (size_t)t * 3;
+        const float lum = sky_lum(rgb);
+        if (!(lum >= threshold) || !(((dx * ax + dy * ay) + dz * az) >= cos_widest)) {
+            continue;
+        }
+        const float omega = st * area;
+        const float lw = lum * omega;
+        acc[0] += (double)(rgb[0] * omega);
+        acc[1] += (double)(rgb[1] * omega);
+        acc[2] += (double)(rgb[2] * omega);
+        acc[3] += (double)(dx * lw);
+        acc[4] += (double)(dy * lw);
+        acc[5] += (double)(dz * lw);
+        acc[6] += (double)lw;
+        acc[7] += (double)omega;
+    }
+    for (int q = 0; q < 8; ++q) {
+        s_sum[threadIdx.x][q] = acc[q];
+    }
+    __syncthreads();
+    for (uint32_t half = kSkyThreads / 2; half > 0; half >>= 1) {
+        if (threadIdx.x < half) {
+            for (int q = 0; q < 8; ++q) {
+                s_sum[threadIdx.x][q] += s_sum[threadIdx.x + half][q];
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 8) {
+        sums[threadIdx.x] = s_sum[0][threadIdx.x];
+    }
+}
+
+// env_residual.slang:55-78: the equirect with the sun's texels (as bright as
+// the threshold, within the widest cone of the sun's axis) set to nothing.
+__global__ void __launch_bounds__(kSkyThreads) sky_residual_kernel(const float* __restrict__ img, uint32_t w,
+                                                                   uint32_t h, const float* __restrict__ rows,
+                                                                   const float* __restrict__ cols, float threshold,
+                                                                   float ax, float ay, float az, float cos_widest,
+                                                                   float* __restrict__ out) {
+    const uint32_t t = blockIdx.x * kSkyThreads + threadIdx.x;
+    if (t >= w * h) {
+        return;
+    }
+    const uint32_t y = t / w, x = t - y * w;
+    const float st = rows[2u * y], ct = rows[2u * y + 1u];
+    const float dx = st * cols[2u * x], dy = ct, dz = st * cols[2u * x + 1u];
+    const float* rgb = img + (size_t)t * 3;
+    const bool sunlit = sky_lum(rgb) >= threshold && ((dx * ax + dy * ay) + dz * az) >= cos_widest;
+    for (int c = 0; c < 3; ++c) {
+        out[(size_t)t * 3 + c] = sunlit ? 0.0f : rgb[c];
+    }
+}
+
 }  // namespace
 
 // environment.slang:66-92: level k's side is max(max(base, 256) >> k, 16),
@@ -249,6 +387,35 @@ hipError_t launch_prepare_sky(const float* equirect, uint32_t width, uint32_t he
     hipLaunchKernelGGL(sky_texels_kernel, dim3(div_up(L.total, kSkyThreads)), dim3(kSkyThreads), 0, s, equirect, width,
                        height, L, texels);
     hipLaunchKernelGGL(sky_sh_kernel, dim3(1), dim3(kSkyThreads), 0, s, equirect, width, height, sh);
+    return hipGetLastError();
+}
+
+hipError_t launch_sky_half(const float* src, uint32_t w, uint32_t h, float* dst, uint32_t dw, uint32_t dh,
+                           hipStream_t s) {
+    hipLaunchKernelGGL(sky_half_kernel, dim3(div_up(dw * dh, kSkyThreads)), dim3(kSkyThreads), 0, s, src, w, h, dst,
+                       dw, dh);
+    return hipGetLastError();
+}
+
+hipError_t launch_sky_sun_scan(const float* img, uint32_t w, uint32_t h, const uint32_t* row_weight,
+                               unsigned long long* hist, unsigned long long* peak, hipStream_t s) {
+    hipLaunchKernelGGL(sky_sun_scan_kernel, dim3(div_up(w * h, kSkyThreads)), dim3(kSkyThreads), 0, s, img, w, h,
+                       row_weight, hist, peak);
+    return hipGetLastError();
+}
+
+hipError_t launch_sky_sun_sum(const float* img, uint32_t w, uint32_t h, const float* rows, const float* cols,
+                              float area, float threshold, const float axis[3], float cos_widest, double* sums,
+                              hipStream_t s) {
+    hipLaunchKernelGGL(sky_sun_sum_kernel, dim3(1), dim3(kSkyThreads), 0, s, img, w, h, rows, cols, area, threshold,
+                       axis[0], axis[1], axis[2], cos_widest, sums);
+    return hipGetLastError();
+}
+
+hipError_t launch_sky_residual(const float* img, uint32_t w, uint32_t h, const float* rows, const float* cols,
+                               float threshold, const float axis[3], float cos_widest, float* out, hipStream_t s) {
+    hipLaunchKernelGGL(sky_residual_kernel, dim3(div_up(w * h, kSkyThreads)), dim3(kSkyThreads), 0, s, img, w, h,
+                       rows, cols, threshold, axis[0], axis[1], axis[2], cos_widest, out);
     return hipGetLastError();
 }
 

@@ -265,10 +265,51 @@ def relight_attributes(n: int, sky: PreparedSky, seed: int = 0x5EED0001, with_al
                              albedo if with_albedo else None)
 
 
-def prepare_sky_device(equirect, base_side: int = COARSE_SIDE, sun: tuple | None = None, raster=None,
+def _device_equirect(equirect, device):
+    import torch
+
+    img = torch.as_tensor(np.asarray(equirect, np.float32) if not torch.is_tensor(equirect) else equirect,
+                          dtype=torch.float32).to(device).contiguous()
+    if img.dim() != 3 or img.shape[2] != 3:
+        raise ValueError("the equirect must be [height, width, 3]")
+    return img
+
+
+def sun_of(equirect, raster, device="cuda:0") -> np.ndarray:
+    """The sun found in an equirect [H, W, 3] (athx_sky_sun; Environment::sun,
+    env_sun.slang): float32 [2, 4] -- (direction, cone half angle, 0 for no
+    sun), (irradiance rgb, the luminance it was cut at)."""
+    from . import _lib
+
+    img = _device_equirect(equirect, device)
+    out = np.zeros(8, np.float32)
+    raster._check(_lib.load().athx_sky_sun(raster._ctx, img.data_ptr(), int(img.shape[1]), int(img.shape[0]),
+                                           out.ctypes.data))
+    return out.reshape(2, 4)
+
+
+def residual_of(equirect, sun: np.ndarray, raster, device="cuda:0"):
+    """The equirect without its sun's texels (athx_sky_residual; env_residual
+    .slang), a device tensor [H, W, 3]."""
+    import torch
+
+    from . import _lib
+
+    img = _device_equirect(equirect, device)
+    words = np.ascontiguousarray(sun, np.float32).reshape(8)
+    out = torch.empty_like(img)
+    raster._check(_lib.load().athx_sky_residual(raster._ctx, img.data_ptr(), int(img.shape[1]), int(img.shape[0]),
+                                                words.ctypes.data, out.data_ptr()))
+    return out
+
+
+def prepare_sky_device(equirect, base_side: int = COARSE_SIDE, sun: tuple | str | None = None, raster=None,
                        device="cuda:0") -> DeviceSky:
     """prepare_sky on the device (athx_prepare_sky): an equirect [H, W, 3]
-    (numpy or a device tensor) into a DeviceSky; `sun` as prepare_sky's."""
+    (numpy or a device tensor) into a DeviceSky; `sun` as prepare_sky's, or
+    "extract": the sun found in the sky (athx_sky_sun) and the levels and
+    harmonics prepared from the sky without it (athx_sky_residual), as
+    Environment::build does."""
     import ctypes
 
     import torch
@@ -276,24 +317,29 @@ def prepare_sky_device(equirect, base_side: int = COARSE_SIDE, sun: tuple | None
     from . import _lib
     from .raster import TileRasterizer
 
-    img = torch.as_tensor(np.asarray(equirect, np.float32) if not torch.is_tensor(equirect) else equirect,
-                          dtype=torch.float32).to(device).contiguous()
-    if img.dim() != 3 or img.shape[2] != 3:
-        raise ValueError("the equirect must be [height, width, 3]")
+    img = _device_equirect(equirect, device)
     lib = _lib.load()
     count = int(lib.athx_sky_texels(int(base_side)))
     texels = torch.empty((count * 2,), dtype=torch.int32, device=img.device)
     sh = torch.zeros((COEFFICIENTS, 4), dtype=torch.float32, device=img.device)
     own = raster is None
     raster = raster or TileRasterizer(img.device.index or 0)
+    extracted = None
     try:
+        if isinstance(sun, str):
+            if sun != "extract":
+                raise ValueError('sun must be None, (direction, size, irradiance) or "extract"')
+            extracted = sun_of(img, raster, device=img.device)
+            img = residual_of(img, extracted, raster, device=img.device)
         raster._check(lib.athx_prepare_sky(raster._ctx, img.data_ptr(), int(img.shape[1]), int(img.shape[0]),
                                            int(base_side), texels.data_ptr(), sh.data_ptr()))
     finally:
         if own:
             raster.close()
     sun_words = np.zeros((2, 4), np.float32)
-    if sun is not None:
+    if extracted is not None:
+        sun_words = extracted.copy()
+    elif sun is not None:
         direction, size, irradiance = sun
         dvec = np.asarray(direction, np.float64)
         sun_words[0, :3] = dvec / np.linalg.norm(dvec)

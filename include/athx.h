@@ -65,6 +65,9 @@
  *       technique::Environment (modules/technique/include/athenea/technique/
  *       Environment.h:1-155) for the relit path: the sky's prefiltered
  *       octahedral levels and SH as the blend reads them.
+ *   athx_sky_sun / athx_sky_residual
+ *       Environment::sun (Environment.h:72-84; env_sun.slang) and
+ *       env_residual.slang: the sun found in a sky, and the sky without it.
  *
  * Threading: a context is used by one host thread at a time. Every call is
  * asynchronous on the stream it is given, except where a comment says it
@@ -717,6 +720,32 @@ ATHX_API athx_status athx_reference_render_instances(athx_ctx* ctx, const athx_v
 ATHX_API uint32_t athx_sky_texels(uint32_t base_side);
 ATHX_API athx_status athx_prepare_sky(athx_ctx* ctx, const float* equirect, uint32_t width, uint32_t height,
                                       uint32_t base_side, uint32_t* texels, float* sh);
+
+/*
+ * The sun taken out of a sky (Environment::sun, Environment.h:72-84;
+ * technique/env_sun.slang:154-262): equirect as athx_prepare_sky's (device,
+ * at most 2^24 texels). The sky is box-filtered 2 x 2 a level at a time until
+ * at most 256 texels wide; there a texel is the sun when it is at least 16
+ * times the sky's median luminance (the median of a sin-theta-weighted
+ * histogram of log2 luminance in half stops from 2^-20) and 0.02 of the
+ * brightest texel's, within 0.175 rad of the brightest; a sky whose brightest
+ * texel is not 32 times its median has none, and a region whose light spreads
+ * past twice a disc's mean square angle is left in the sky. sun (host, 8
+ * floats) receives athx_sky.sun's words: (direction, the half angle of the
+ * cone of the sun's solid angle, 0 for no sun), (the sun's irradiance rgb,
+ * the luminance it was cut at). Waits for the result.
+ */
+ATHX_API athx_status athx_sky_sun(athx_ctx* ctx, const float* equirect, uint32_t width, uint32_t height,
+                                  float* sun);
+/*
+ * The sky without its sun (technique/env_residual.slang:55-78), what the
+ * relit frames' prepared sky is made from when the sun is lit on its own:
+ * equirect's texels at least as bright as sun[7] within 0.175 rad of sun's
+ * direction set to 0 (none when sun[3] is 0). sun (host) as athx_sky_sun wrote
+ * it; residual (device) the equirect's size. Waits for the result.
+ */
+ATHX_API athx_status athx_sky_residual(athx_ctx* ctx, const float* equirect, uint32_t width, uint32_t height,
+                                       const float* sun, float* residual);
 
 /* Two renders compared as 8-bit sRGB code values (ReferenceRenderer.h:52-58):
    each pixel's largest channel difference, alpha included. */

@@ -2250,3 +2250,164 @@ ORC_EXPORT void orc_contribution(const orc_view* v, const float* rec, const uint
         }
     }
 }
+
+/* ------------------------------------------------------------------------
+ * THE SUN TAKEN OUT OF A SKY (technique/env_sun.slang:154-262) and the sky
+ * without it (technique/env_residual.slang:55-78), for an equirect of linear
+ * radiance [h][w][3]: texel (i, j) looks along theta = pi (j + 0.5) / h from
+ * +y, phi = 2 pi (i + 0.5) / w from +x towards +z. The search and the sums
+ * run on the level halved (2 x 2 means, clamped at an odd edge) until at most
+ * 256 wide, as env_sun reads its projection level. The reference's float
+ * sums become a histogram of integer weights (sin theta in 2^-24 units: exact
+ * in any order) and FP64 sums of the float per-texel terms; its sin, cos and
+ * acos are taken in double, the directions in float from those.
+ * ------------------------------------------------------------------------ */
+static float sky_lum_(const float* rgb) { return rgb[0] * 0.2126f + rgb[1] * 0.7152f + rgb[2] * 0.0722f; }
+
+static void sky_tables_(uint32_t w, uint32_t h, float* rows, float* cols, uint32_t* weights) {
+    const double pi = 3.14159265358979323846;
+    for (uint32_t j = 0; j < h; ++j) {
+        const double theta = pi * ((double)j + 0.5) / (double)h;
+        rows[2 * j] = (float)sin(theta);
+        rows[2 * j + 1] = (float)cos(theta);
+        if (weights) weights[j] = (uint32_t)llround(fmax(sin(theta), 0.0) * 16777216.0);
+    }
+    for (uint32_t i = 0; i < w; ++i) {
+        const double phi = 2.0 * pi * ((double)i + 0.5) / (double)w;
+        cols[2 * i] = (float)cos(phi);
+        cols[2 * i + 1] = (float)sin(phi);
+    }
+}
+
+/* env_sun.slang:154-261 into sun[8]: (direction, cone half angle or 0),
+ * (irradiance rgb, the luminance cut at). Returns 0, or -1 on no memory. */
+int orc_sky_sun(const float* img, uint32_t width, uint32_t height, float* sun) {
+    const double pi = 3.14159265358979323846;
+    uint32_t w = width, h = height;
+    float* level = (float*)malloc((size_t)w * h * 3 * sizeof(float));
+    if (!level) return -1;
+    memcpy(level, img, (size_t)w * h * 3 * sizeof(float));
+    while (w > 256u) {
+        const uint32_t dw = w >> 1 > 0 ? w >> 1 : 1u, dh = h >> 1 > 0 ? h >> 1 : 1u;
+        float* next = (float*)malloc((size_t)dw * dh * 3 * sizeof(float));
+        if (!next) { free(level); return -1; }
+        for (uint32_t y = 0; y < dh; ++y) {
+            for (uint32_t x = 0; x < dw; ++x) {
+                const uint32_t x0 = 2 * x < w - 1 ? 2 * x : w - 1, x1 = 2 * x + 1 < w - 1 ? 2 * x + 1 : w - 1;
+                const uint32_t y0 = 2 * y < h - 1 ? 2 * y : h - 1, y1 = 2 * y + 1 < h - 1 ? 2 * y + 1 : h - 1;
+                for (int c = 0; c < 3; ++c) {
+                    const float a = level[((size_t)y0 * w + x0) * 3 + c], b = level[((size_t)y0 * w + x1) * 3 + c];
+                    const float e = level[((size_t)y1 * w + x0) * 3 + c], f = level[((size_t)y1 * w + x1) * 3 + c];
+                    next[((size_t)y * dw + x) * 3 + c] = 0.25f * ((a + b) + (e + f));
+                }
+            }
+        }
+        free(level);
+        level = next;
+        w = dw;
+        h = dh;
+    }
+    float* rows = (float*)malloc(2 * (size_t)h * sizeof(float));
+    float* cols = (float*)malloc(2 * (size_t)w * sizeof(float));
+    uint32_t* weights = (uint32_t*)malloc((size_t)h * sizeof(uint32_t));
+    if (!rows || !cols || !weights) { free(level); free(rows); free(cols); free(weights); return -1; }
+    sky_tables_(w, h, rows, cols, weights);
+    /* sunRegionOf (env_sun.slang:165-204): the histogram, its median, the peak. */
+    uint64_t hist[80];
+    memset(hist, 0, sizeof(hist));
+    uint64_t total = 0;
+    float peak = -1.0f;
+    uint32_t peak_at = 0;
+    for (uint32_t y = 0; y < h; ++y) {
+        if (weights[y] == 0u) continue;
+        for (uint32_t x = 0; x < w; ++x) {
+            const float lum = sky_lum_(level + ((size_t)y * w + x) * 3);
+            const float l2 = athx_logf(maxf_(lum, 1.0e-9f)) * 1.44269504088896341f;
+            const float fb = minf_(maxf_((l2 + 20.0f) * 2.0f, 0.0f), 79.0f);
+            hist[(uint32_t)fb] += weights[y];
+            total += weights[y];
+            if (maxf_(lum, 0.0f) > peak) {
+                peak = maxf_(lum, 0.0f);
+                peak_at = y * w + x;
+            }
+        }
+    }
+    float median = 0.0f;
+    uint64_t below = 0;
+    for (uint32_t b = 0; b < 80; ++b) {
+        below += hist[b];
+        if (2 * below >= total && total > 0) {
+            median = (float)exp2((double)b * 0.5 - 20.0 + 0.25);
+            break;
+        }
+    }
+    const uint32_t py = peak_at / w, px = peak_at - py * w;
+    const float axis[3] = {rows[2 * py] * cols[2 * px], rows[2 * py + 1], rows[2 * py] * cols[2 * px + 1]};
+    const float threshold = maxf_(16.0f * median, 0.02f * peak);
+    const int found = peak >= 0.0f && peak > 32.0f * maxf_(median, 1.0e-9f);
+    const float cos_widest = (float)cos(0.175);
+    double sums[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (found) {
+        const float area = (float)(2.0 * pi * pi / ((double)w * (double)h));
+        for (uint32_t y = 0; y < h; ++y) {
+            const float st = rows[2 * y], ct = rows[2 * y + 1];
+            if (!(st > 0.0f)) continue;
+            for (uint32_t x = 0; x < w; ++x) {
+                const float dx = st * cols[2 * x], dy = ct, dz = st * cols[2 * x + 1];
+                const float* rgb = level + ((size_t)y * w + x) * 3;
+                const float lum = sky_lum_(rgb);
+                if (!(lum >= threshold) || !(((dx * axis[0] + dy * axis[1]) + dz * axis[2]) >= cos_widest)) continue;
+                const float omega = st * area, lw = lum * omega;
+                sums[0] += (double)(rgb[0] * omega);
+                sums[1] += (double)(rgb[1] * omega);
+                sums[2] += (double)(rgb[2] * omega);
+                sums[3] += (double)(dx * lw);
+                sums[4] += (double)(dy * lw);
+                sums[5] += (double)(dz * lw);
+                sums[6] += (double)lw;
+                sums[7] += (double)omega;
+            }
+        }
+    }
+    /* env_sun.slang:236-261 */
+    const double solid = sums[7], weight = sums[6];
+    const double clen = sqrt(sums[3] * sums[3] + sums[4] * sums[4] + sums[5] * sums[5]);
+    const double disc = solid > 0.0 ? acos(fmin(fmax(1.0 - solid / (2.0 * pi), 0.0), 1.0)) : 0.0;
+    const double spread = weight > 0.0 ? 2.0 * fmax(1.0 - clen / weight, 0.0) : 0.0;
+    const int compact = spread <= disc * disc + 1.0e-6;
+    const int is_sun = found && solid > 0.0 && clen > 0.0 && compact;
+    for (int c = 0; c < 3; ++c) {
+        sun[c] = is_sun ? (float)(sums[3 + c] / clen) : axis[c];
+        sun[4 + c] = (float)sums[c];
+    }
+    sun[3] = is_sun ? maxf_((float)disc, 1.0e-4f) : 0.0f;
+    sun[7] = threshold;
+    free(level);
+    free(rows);
+    free(cols);
+    free(weights);
+    return 0;
+}
+
+/* env_residual.slang:55-78: the sun's texels (at least sun[7] bright, within
+ * 0.175 rad of its direction) set to 0; none when sun[3] is 0. */
+int orc_sky_residual(const float* img, uint32_t w, uint32_t h, const float* sun, float* out) {
+    float* rows = (float*)malloc(2 * (size_t)h * sizeof(float));
+    float* cols = (float*)malloc(2 * (size_t)w * sizeof(float));
+    if (!rows || !cols) { free(rows); free(cols); return -1; }
+    sky_tables_(w, h, rows, cols, NULL);
+    const float threshold = sun[3] > 0.0f ? sun[7] : INFINITY;
+    const float cos_widest = (float)cos(0.175);
+    for (uint32_t y = 0; y < h; ++y) {
+        for (uint32_t x = 0; x < w; ++x) {
+            const float st = rows[2 * y], ct = rows[2 * y + 1];
+            const float dx = st * cols[2 * x], dy = ct, dz = st * cols[2 * x + 1];
+            const float* rgb = img + ((size_t)y * w + x) * 3;
+            const int sunlit = sky_lum_(rgb) >= threshold && ((dx * sun[0] + dy * sun[1]) + dz * sun[2]) >= cos_widest;
+            for (int c = 0; c < 3; ++c) out[((size_t)y * w + x) * 3 + c] = sunlit ? 0.0f : rgb[c];
+        }
+    }
+    free(rows);
+    free(cols);
+    return 0;
+}

@@ -133,6 +133,10 @@ def load():
         lib.orc_env_radiance.restype = None
         lib.orc_env_irradiance.argtypes = [rp, f3, f3]
         lib.orc_env_irradiance.restype = None
+        lib.orc_sky_sun.argtypes = [_f32p, ctypes.c_uint32, ctypes.c_uint32, _f32p]
+        lib.orc_sky_sun.restype = ctypes.c_int
+        lib.orc_sky_residual.argtypes = [_f32p, ctypes.c_uint32, ctypes.c_uint32, _f32p, _f32p]
+        lib.orc_sky_residual.restype = ctypes.c_int
         lib.orc_env_level_offset.argtypes = [ctypes.c_uint32, ctypes.c_uint32]
         lib.orc_env_level_offset.restype = ctypes.c_uint32
         lib.orc_half_to_float.argtypes = [ctypes.c_uint32]
@@ -593,6 +597,27 @@ def env_irradiance(relight, normal) -> np.ndarray:
     out = _f3((0, 0, 0))
     load().orc_env_irradiance(ctypes.byref(rl), _f3(normal), out)
     return np.array(out[:], np.float32)
+
+
+def sky_sun(equirect: np.ndarray) -> np.ndarray:
+    """orc_sky_sun (env_sun.slang:154-261): the sun words [2, 4] of an
+    equirect [H, W, 3] of linear radiance."""
+    img = np.ascontiguousarray(equirect, np.float32)
+    out = np.zeros(8, np.float32)
+    if load().orc_sky_sun(_c(img, np.float32), img.shape[1], img.shape[0], out) != 0:
+        raise MemoryError("orc_sky_sun")
+    return out.reshape(2, 4)
+
+
+def sky_residual(equirect: np.ndarray, sun: np.ndarray) -> np.ndarray:
+    """orc_sky_residual (env_residual.slang:55-78): the equirect with the sun's
+    texels set to 0."""
+    img = np.ascontiguousarray(equirect, np.float32)
+    out = np.zeros_like(img)
+    words = np.ascontiguousarray(sun, np.float32).reshape(8)
+    if load().orc_sky_residual(_c(img, np.float32), img.shape[1], img.shape[0], words, out) != 0:
+        raise MemoryError("orc_sky_residual")
+    return out
 
 
 def env_level_offset(level: int, base_side: int) -> int:
