@@ -243,6 +243,7 @@ struct StripBins {
     const uint64_t* seg_total;     // segments (device)
     uint32_t seg_cap;              // segments the table holds
     uint16_t* masks;               // [entries] tiles of its strip an entry touches
+    uint32_t masks_sorted;         // the strip sort wrote masks (strip_keys unused by the count)
     uint32_t* table;               // [16 * seg_cap] hit counts, then their exclusive scan
     const uint64_t* pair_total;    // pairs (device; the table's total)
     uint32_t pair_cap;
@@ -375,9 +376,16 @@ struct SegmentArgs {
 // partial one a view).
 // strips false: the digits are the key's own low `bits` bits (the depth
 // sort's segments, the frame's views' visible slots), seg_base unused.
+// mask_out (strips only): the last pass writes each key's upper half (its
+// tile mask) there instead of the key itself.
 hipError_t launch_segment_sort(uint32_t* keys, uint32_t* values, const SegmentArgs& g, uint32_t parts_cap,
                                uint32_t bits, const SortScratch& sc, hipStream_t s, bool& in_alt,
-                               bool strips = true);
+                               bool strips = true, uint16_t* mask_out = nullptr);
+// A one-pass segmented strip sort's strip ranges ([strips][2], (0, 0) when
+// empty) from its scanned counts (radix digits a partition).
+hipError_t launch_strip_ranges_seg(const uint32_t* offsets, const SegmentArgs& g, uint32_t nviews,
+                                   const uint32_t* strip_bases, uint32_t strips, uint32_t radix,
+                                   const uint64_t* total_dev, uint32_t* ranges, hipStream_t s);
 constexpr uint32_t kMaxSegmentViews = 256;
 struct SegmentBuild {
     uint32_t nviews;

@@ -757,6 +757,11 @@ const uint32_t* sorted_slots_of(const athx_ctx* ctx, bool in_alt) {
     return static_cast<const uint32_t*>(in_alt ? ctx->depth_alt_slots.ptr : ctx->depth_slots.ptr);
 }
 
+// A segmented frame whose views sort their strips in one pass keeps only the
+// masks of the sorted keys (the sort writes them), its strip ranges read off
+// the sort's counts.
+bool sorts_masks(const FrameShape& fs) { return fs.segmented && masks_in_keys(fs) && fs.view_strip_bits <= 8; }
+
 // Frames of at most 65 536 strips decide their entries' tiles in the
 // emission and carry the masks in the keys' upper half.
 StripBins bins_of(athx_ctx* ctx, const FrameShape& fs, uint32_t n, uint32_t tile, bool strips_in_alt) {
@@ -770,7 +775,10 @@ StripBins bins_of(athx_ctx* ctx, const FrameShape& fs, uint32_t n, uint32_t tile
     b.records = as<float>(ctx->records);
     b.strip_slots = as<uint32_t>(strips_in_alt ? ctx->strip_alt_slots : ctx->strip_slots);
     b.fp = as<uint4>(ctx->rank_fp);
-    b.strip_keys = masks_in_keys(fs) ? as<uint32_t>(strips_in_alt ? ctx->strip_alt_keys : ctx->strip_keys) : nullptr;
+    b.masks_sorted = sorts_masks(fs) ? 1u : 0u;
+    b.strip_keys = masks_in_keys(fs) && !b.masks_sorted
+                       ? as<uint32_t>(strips_in_alt ? ctx->strip_alt_keys : ctx->strip_keys)
+                       : nullptr;
     b.strip_ranges = as<uint32_t>(ctx->strip_ranges);
     b.segbase = as<uint32_t>(ctx->segbase);
     b.seg_desc = as<uint4>(ctx->seg_map);
@@ -980,10 +988,11 @@ athx_status enqueue_bins(athx_ctx* ctx, const FrameShape& fs, uint32_t n, uint32
                                         as<uint32_t>(ctx->strip_alt_slots), fs.entry_cap + fs.entry_pad,
                                         segmented ? div_up(fs.view_strip_bits, 8) : fs.strip_passes);
     strips_in_alt = false;
+    const bool mask_sort = sorts_masks(fs);
     if (segmented) {
         ATHX_HIP(ctx, launch_segment_sort(as<uint32_t>(ctx->strip_keys), as<uint32_t>(ctx->strip_slots),
                                           segments_of(ctx, fs), segment_parts_cap(fs), fs.view_strip_bits, sc, s,
-                                          strips_in_alt),
+                                          strips_in_alt, true, mask_sort ? as<uint16_t>(ctx->masks) : nullptr),
                  "strip sort");
     } else {
         ATHX_HIP(ctx, launch_sort(as<uint32_t>(ctx->strip_keys), as<uint32_t>(ctx->strip_slots),
@@ -993,9 +1002,17 @@ athx_status enqueue_bins(athx_ctx* ctx, const FrameShape& fs, uint32_t n, uint32
     // The sorted entries are dense from here on; a segmented frame's count is
     // what its segments held.
     const uint32_t* keys = as<uint32_t>(strips_in_alt ? ctx->strip_alt_keys : ctx->strip_keys);
-    ATHX_HIP(ctx, launch_ranges(keys, totals + (segmented ? kTotalBinned : kTotalEntries), fs.entry_cap, fs.strips,
-                                as<uint32_t>(ctx->strip_ranges), s, masked ? 0xFFFFu : ~0u),
-             "strip ranges");
+    if (mask_sort) {
+        ATHX_HIP(ctx, launch_strip_ranges_seg(sc.offsets, segments_of(ctx, fs), fs.nviews,
+                                              as<uint32_t>(ctx->tile_bases) + fs.nviews + 1, fs.strips,
+                                              1u << fs.view_strip_bits, totals + kTotalBinned,
+                                              as<uint32_t>(ctx->strip_ranges), s),
+                 "strip ranges");
+    } else {
+        ATHX_HIP(ctx, launch_ranges(keys, totals + (segmented ? kTotalBinned : kTotalEntries), fs.entry_cap,
+                                    fs.strips, as<uint32_t>(ctx->strip_ranges), s, masked ? 0xFFFFu : ~0u),
+                 "strip ranges");
+    }
     ATHX_HIP(ctx, launch_seg_scan(as<uint32_t>(ctx->strip_ranges), fs.strips, as<uint32_t>(ctx->segbase),
                                   totals + kTotalSegs, as<uint32_t>(ctx->scan_status), s),
              "segments");

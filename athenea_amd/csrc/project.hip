@@ -1338,6 +1338,7 @@ struct StripEntryLds {
 // With the masks in the keys (strip_mask_emit_kernel), a segment only
 // reads them: each entry's mask is kept for the pairs, each tile's hits
 // counted into the table.
+template <bool kSorted>
 __global__ void __launch_bounds__(kSegThreads) strip_mask_count_kernel(StripBins b) {
     __shared__ uint32_t s_hits[kSegThreads / 64][kStripTiles];
     const uint64_t total = *b.seg_total;
@@ -1357,17 +1358,19 @@ __global__ void __launch_bounds__(kSegThreads) strip_mask_count_kernel(StripBins
     }
     // The keys read together (a lane past the segment reads its last key and
     // counts nothing), then counted.
+    // (kSorted: the strip sort wrote the masks themselves.)
     uint32_t key[kSegPer];
 #pragma unroll
     for (uint32_t j = 0; j < kSegPer; ++j) {
         const uint32_t e = d0.x + j * kSegThreads + threadIdx.x;
-        key[j] = b.strip_keys[e < d0.y ? e : d0.y - 1u];
+        const uint32_t ec = e < d0.y ? e : d0.y - 1u;
+        key[j] = kSorted ? (uint32_t)b.masks[ec] << 16 : b.strip_keys[ec];
     }
 #pragma unroll
     for (uint32_t j = 0; j < kSegPer; ++j) {
         const uint32_t e = d0.x + j * kSegThreads + threadIdx.x;
         const uint32_t mask = e < d0.y ? key[j] >> 16 : 0u;
-        if (e < d0.y) {
+        if (!kSorted && e < d0.y) {
             b.masks[e] = (uint16_t)mask;
         }
 #pragma unroll
@@ -1830,8 +1833,10 @@ hipError_t launch_strip_count(const StripBins& b, hipStream_t s) {
     if (b.seg_cap == 0) {
         return hipSuccess;
     }
-    if (b.strip_keys) {
-        hipLaunchKernelGGL(strip_mask_count_kernel, dim3(b.seg_cap), dim3(kSegThreads), 0, s, b);
+    if (b.masks_sorted) {
+        hipLaunchKernelGGL(strip_mask_count_kernel<true>, dim3(b.seg_cap), dim3(kSegThreads), 0, s, b);
+    } else if (b.strip_keys) {
+        hipLaunchKernelGGL(strip_mask_count_kernel<false>, dim3(b.seg_cap), dim3(kSegThreads), 0, s, b);
     } else {
         hipLaunchKernelGGL(strip_count_kernel, dim3(b.seg_cap), dim3(kSegThreads), 0, s, b);
     }

@@ -452,7 +452,8 @@ template <typename K, typename D, uint32_t kT, typename L>
 __global__ void __launch_bounds__(kT) downsweep_kernel(const K* __restrict__ kin, const uint32_t* __restrict__ vin,
                                                        K* __restrict__ kout, uint32_t* __restrict__ vout, L layout,
                                                        D digit, uint32_t width,
-                                                       const uint32_t* __restrict__ offsets) {
+                                                       const uint32_t* __restrict__ offsets,
+                                                       uint16_t* __restrict__ mout) {
     static_assert(kT >= kRadix && kT % 64 == 0, "a thread a digit");
     constexpr uint32_t kW = kT / 64;
     constexpr uint32_t kP = kT * kItems;
@@ -537,7 +538,13 @@ __global__ void __launch_bounds__(kT) downsweep_kernel(const K* __restrict__ kin
         const uint32_t v = s_vals[i];
         const uint32_t dk = digit(k, v, pt.base);
         const uint32_t dst = s_digit_base[dk] + (i - s_block_start[dk]);
-        kout[dst] = k;
+        // mout: a strip sort's last pass keeps only each key's upper half
+        // (its tile mask); the strips' runs come from the counts instead.
+        if (mout) {
+            mout[dst] = (uint16_t)((uint32_t)k >> 16);
+        } else {
+            kout[dst] = k;
+        }
         vout[dst] = v;
     }
 }
@@ -618,7 +625,7 @@ hipError_t sort_impl(K* keys, uint32_t* values, const uint64_t* n_dev, uint32_t 
                 return e;
             }
             hipLaunchKernelGGL((downsweep_kernel<K, KeyDigit, kT, PlainLayout>), dim3(rparts), dim3(kT), 0, s, kin,
-                               vin, kout, vout, layout, digit, width, sc.offsets);
+                               vin, kout, vout, layout, digit, width, sc.offsets, nullptr);
             K* tk = kin; kin = kout; kout = tk;
             uint32_t* tv = vin; vin = vout; vout = tv;
         }
@@ -687,7 +694,7 @@ hipError_t launch_sort(uint32_t* keys, uint32_t* values, const uint64_t* n_dev, 
 
 template <typename D>
 hipError_t segment_passes(uint32_t* keys, uint32_t* values, const SegmentArgs& g, uint32_t parts, uint32_t bits,
-                          const SortScratch& sc, hipStream_t s, bool& in_alt) {
+                          const SortScratch& sc, hipStream_t s, bool& in_alt, uint16_t* mask_out) {
     constexpr uint32_t kT = rs_threads<uint32_t>();
     const uint32_t passes = div_up(bits, 8);
     for (uint32_t p = 0; p < passes; ++p) {
@@ -706,14 +713,15 @@ hipError_t segment_passes(uint32_t* keys, uint32_t* values, const SegmentArgs& g
             return e;
         }
         hipLaunchKernelGGL((downsweep_kernel<uint32_t, D, kT, SegLayout>), dim3(parts), dim3(kT), 0, s, kin, vin,
-                           kout, vout, layout, digit, width, sc.offsets);
+                           kout, vout, layout, digit, width, sc.offsets, p + 1 == passes ? mask_out : nullptr);
         in_alt = !in_alt;
     }
     return hipGetLastError();
 }
 
 hipError_t launch_segment_sort(uint32_t* keys, uint32_t* values, const SegmentArgs& g, uint32_t parts_cap,
-                               uint32_t bits, const SortScratch& sc, hipStream_t s, bool& in_alt, bool strips) {
+                               uint32_t bits, const SortScratch& sc, hipStream_t s, bool& in_alt, bool strips,
+                               uint16_t* mask_out) {
     in_alt = false;
     if (parts_cap == 0 || bits == 0) {
         return hipSuccess;
@@ -722,8 +730,58 @@ hipError_t launch_segment_sort(uint32_t* keys, uint32_t* values, const SegmentAr
         return hipErrorInvalidValue;
     }
     static_assert(rs_threads<uint32_t>() * kItems == kSegmentPart, "a segment's partition is the pass's");
-    return strips ? segment_passes<StripDigit>(keys, values, g, parts_cap, bits, sc, s, in_alt)
-                  : segment_passes<KeyDigit>(keys, values, g, parts_cap, bits, sc, s, in_alt);
+    return strips ? segment_passes<StripDigit>(keys, values, g, parts_cap, bits, sc, s, in_alt, mask_out)
+                  : segment_passes<KeyDigit>(keys, values, g, parts_cap, bits, sc, s, in_alt, nullptr);
+}
+
+namespace {
+
+// Strip s of view v is digit d = s - (v's first strip) of v's run: its entries
+// start at the scanned count of (v, d, v's first partition) and end where the
+// next digit's (or the next view's) start, the total past the last cell.
+__global__ void __launch_bounds__(256) strip_ranges_seg_kernel(const uint32_t* __restrict__ offsets, SegmentArgs g,
+                                                               uint32_t nviews, const uint32_t* __restrict__ strip_bases,
+                                                               uint32_t strips, uint32_t radix,
+                                                               const uint64_t* __restrict__ total_dev,
+                                                               uint32_t* __restrict__ ranges) {
+    const uint32_t st = blockIdx.x * blockDim.x + threadIdx.x;
+    if (st >= strips) {
+        return;
+    }
+    uint32_t lo = 0, hi = nviews;
+    while (hi - lo > 1u) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (strip_bases[mid] <= st) {
+            lo = mid;
+        } else {
+            hi = mid;
+        }
+    }
+    const uint32_t v = lo, d = st - strip_bases[v];
+    const uint32_t cells = radix * *g.parts;
+    const uint32_t stride = div_up(g.seg_len[v], kSegmentPart);
+    const uint32_t total = (uint32_t)*total_dev;
+    uint32_t start = total, end = total;
+    if (d < radix) {
+        const uint32_t cell = radix * g.seg_part[v] + d * stride;
+        start = cell < cells ? offsets[cell] : total;
+        end = cell + stride < cells ? offsets[cell + stride] : total;
+    }
+    ranges[2u * st] = start < end ? start : 0u;
+    ranges[2u * st + 1u] = start < end ? end : 0u;
+}
+
+}  // namespace
+
+hipError_t launch_strip_ranges_seg(const uint32_t* offsets, const SegmentArgs& g, uint32_t nviews,
+                                   const uint32_t* strip_bases, uint32_t strips, uint32_t radix,
+                                   const uint64_t* total_dev, uint32_t* ranges, hipStream_t s) {
+    if (strips == 0) {
+        return hipSuccess;
+    }
+    hipLaunchKernelGGL(strip_ranges_seg_kernel, dim3(div_up(strips, 256)), dim3(256), 0, s, offsets, g, nviews,
+                       strip_bases, strips, radix, total_dev, ranges);
+    return hipGetLastError();
 }
 
 
