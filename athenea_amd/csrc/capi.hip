@@ -55,13 +55,14 @@ constexpr uint32_t kTotalWords = 9;
 constexpr uint64_t kMaxPairs = uint64_t{1} << 30;  // pairs and entries are 32-bit, with room
 
 // The scratch of athx_exclusive_scan_noalloc: the scan's look-back status
-// words, or a long scan's partition sums, for up to kNoAllocScanMax elements
-// (two words a partition of 4096 and two more), held by each device the library runs on
+// words, or a long scan's partition sums and their runs' tops, for up to
+// kNoAllocScanMax elements (two words a partition of 4096, two a run of 1024
+// partitions and four more: scan_status_words), held by each device the library runs on
 // until the process ends. Host code never names it directly: athx_create
 // looks up its device address (hipGetSymbolAddress) and keeps it in the
 // context.
 constexpr uint32_t kNoAllocScanMax = 1u << 24;
-constexpr uint32_t kNoAllocScanWords = 2u * (1u + kNoAllocScanMax / 4096u);
+constexpr uint32_t kNoAllocScanWords = 2u * (2u + kNoAllocScanMax / 4096u + kNoAllocScanMax / 4096u / 1024u);
 static __device__ __align__(256) uint32_t scratch[kNoAllocScanWords];
 
 uint32_t bits_for(uint32_t values) {

@@ -348,52 +348,14 @@ __global__ void clear_status_kernel(Policy pol, uint64_t* __restrict__ status, u
     }
 }
 
-// Long scans (past kLongScan elements, 256 partitions) go in three launches
-// instead: each partition's sum, one workgroup's scan of the sums, then each
+// Long scans (past kLongScan elements, 256 partitions) go in four launches
+// instead: each partition's sum, the sums scanned in two levels, then each
 // partition scanned from its offset. Past a few hundred partitions in flight
 // the look-back above waits on a chain of predecessors (measured on MI355X:
 // a 4 M-element scan 53 us that way, about 25 us this way); here nothing
 // waits, and the data is read twice.
 constexpr uint32_t kLongScan = 1u << 20;
 constexpr uint32_t kSumThreads = 1024;
-
-// A thread's run sums[lo, hi) of a single-workgroup scan, read eight loads
-// at a time (one at a time, each waits on the last): its sum, and then each
-// element replaced by run + the elements before it.
-constexpr uint32_t kRunGroup = 8;
-
-__device__ inline uint64_t run_sum(const uint64_t* sums, uint32_t lo, uint32_t hi) {
-    uint64_t mine = 0;
-    for (uint32_t p = lo; p < hi; p += kRunGroup) {
-        uint64_t x[kRunGroup];
-#pragma unroll
-        for (uint32_t u = 0; u < kRunGroup; ++u) {
-            x[u] = p + u < hi ? sums[p + u] : 0u;
-        }
-#pragma unroll
-        for (uint32_t u = 0; u < kRunGroup; ++u) {
-            mine += x[u];
-        }
-    }
-    return mine;
-}
-
-__device__ inline void run_exclusive(uint64_t* sums, uint32_t lo, uint32_t hi, uint64_t run) {
-    for (uint32_t p = lo; p < hi; p += kRunGroup) {
-        uint64_t x[kRunGroup];
-#pragma unroll
-        for (uint32_t u = 0; u < kRunGroup; ++u) {
-            x[u] = p + u < hi ? sums[p + u] : 0u;
-        }
-#pragma unroll
-        for (uint32_t u = 0; u < kRunGroup; ++u) {
-            if (p + u < hi) {
-                sums[p + u] = run;
-            }
-            run += x[u];
-        }
-    }
-}
 
 template <typename Policy>
 __global__ void __launch_bounds__(kThreads) partition_sum_kernel(Policy pol, uint64_t* __restrict__ sums) {
@@ -427,37 +389,81 @@ __global__ void __launch_bounds__(kThreads) partition_sum_kernel(Policy pol, uin
     }
 }
 
-// The partitions' sums scanned in place by one workgroup (a run of them a
-// thread); the total lands in *total.
+// The partitions' sums scanned in two levels, so that no single workgroup
+// walks them all (one did, its loads a lane apart each a line of their own,
+// at tens of microseconds past 30 000 partitions): each run of kSumThreads
+// sums scanned in place by a workgroup, its total kept in tops[run]; then
+// one workgroup scans the tops (at most kSumThreads of them), the grand total
+// to *total. A sum's offset is then sums[p] + tops[p / kSumThreads].
 template <typename Policy>
-__global__ void __launch_bounds__(kSumThreads) partition_offsets_kernel(Policy pol, uint64_t* __restrict__ sums,
-                                                                        uint64_t* __restrict__ total) {
+__global__ void __launch_bounds__(kSumThreads) run_scan_kernel(Policy pol, uint64_t* __restrict__ sums,
+                                                               uint64_t* __restrict__ tops) {
     __shared__ uint64_t s_wave[kSumThreads / 64];
     const uint32_t n = pol.size();
     const uint32_t parts = div_up(n ? n : 1, kTile);
-    const uint32_t per = div_up(parts, kSumThreads);
-    const uint32_t lo = threadIdx.x * per, hi = lo + per < parts ? lo + per : parts;
-    uint64_t mine = run_sum(sums, lo, hi);
+    if (blockIdx.x * kSumThreads >= parts) {
+        return;
+    }
+    const uint32_t i = blockIdx.x * kSumThreads + threadIdx.x;
+    const uint64_t x = i < parts ? sums[i] : 0u;
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-    const uint64_t incl = wave_inclusive_scan(mine, lane);
+    const uint64_t incl = wave_inclusive_scan(x, lane);
     if (lane == 63) {
         s_wave[wave] = incl;
     }
     __syncthreads();
-    uint64_t run = incl - mine;
-    uint64_t all = 0;
+    uint64_t before = 0, all = 0;
+#pragma unroll
     for (uint32_t w = 0; w < kSumThreads / 64; ++w) {
-        run += w < wave ? s_wave[w] : 0u;
-        all += s_wave[w];
+        const uint64_t t = s_wave[w];
+        before += w < wave ? t : 0u;
+        all += t;
     }
-    run_exclusive(sums, lo, hi, run);
+    if (i < parts) {
+        sums[i] = before + incl - x;
+    }
+    if (threadIdx.x == 0) {
+        tops[blockIdx.x] = all;
+    }
+}
+
+template <typename Policy>
+__global__ void __launch_bounds__(kSumThreads) tops_scan_kernel(Policy pol, uint64_t* __restrict__ tops,
+                                                                uint64_t* __restrict__ total) {
+    __shared__ uint64_t s_wave[kSumThreads / 64];
+    const uint32_t n = pol.size();
+    const uint32_t runs = div_up(div_up(n ? n : 1, kTile), kSumThreads);
+    const uint64_t x = threadIdx.x < runs ? tops[threadIdx.x] : 0u;
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint64_t incl = wave_inclusive_scan(x, lane);
+    if (lane == 63) {
+        s_wave[wave] = incl;
+    }
+    __syncthreads();
+    uint64_t before = 0, all = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < kSumThreads / 64; ++w) {
+        const uint64_t t = s_wave[w];
+        before += w < wave ? t : 0u;
+        all += t;
+    }
+    if (threadIdx.x < runs) {
+        tops[threadIdx.x] = before + incl - x;
+    }
     if (threadIdx.x == 0) {
         *total = n ? all : 0u;
     }
 }
 
+// A count as a policy: what the compaction's two-level scan needs of one.
+struct CountOnly {
+    uint32_t n;
+    __device__ uint32_t size() const { return n; }
+};
+
 template <typename Policy>
-__global__ void __launch_bounds__(kThreads) partition_scan_kernel(Policy pol, const uint64_t* __restrict__ offsets) {
+__global__ void __launch_bounds__(kThreads) partition_scan_kernel(Policy pol, const uint64_t* __restrict__ offsets,
+                                                                  const uint64_t* __restrict__ tops) {
     __shared__ uint64_t s_wave[kThreads / 64];
     __shared__ typename Policy::Value s_x[kThreads * (kItems + 1)];
     const uint32_t n = pol.size();
@@ -479,7 +485,7 @@ __global__ void __launch_bounds__(kThreads) partition_scan_kernel(Policy pol, co
         s_wave[wave] = incl;
     }
     __syncthreads();
-    uint64_t run = offsets[part] + (incl - mine);
+    uint64_t run = offsets[part] + tops[part / kSumThreads] + (incl - mine);
 #pragma unroll
     for (uint32_t w = 0; w < kThreads / 64; ++w) {
         run += w < wave ? s_wave[w] : 0u;
@@ -498,10 +504,14 @@ hipError_t run_scan(const Policy& pol, uint32_t max_n, uint64_t* total, uint32_t
     uint64_t* status = reinterpret_cast<uint64_t*>(status_words);
     const uint32_t parts = div_up(max_n ? max_n : 1, kTile);
     if (max_n > kLongScan) {
-        // The status words hold one u64 a partition (and one more): the sums.
+        // The status words hold one u64 a partition, then one a run of
+        // kSumThreads partitions (scan_status_words).
+        uint64_t* tops = status + parts;
         hipLaunchKernelGGL(partition_sum_kernel<Policy>, dim3(parts), dim3(kThreads), 0, s, pol, status);
-        hipLaunchKernelGGL(partition_offsets_kernel<Policy>, dim3(1), dim3(kSumThreads), 0, s, pol, status, total);
-        hipLaunchKernelGGL(partition_scan_kernel<Policy>, dim3(parts), dim3(kThreads), 0, s, pol, status);
+        hipLaunchKernelGGL(run_scan_kernel<Policy>, dim3(div_up(parts, kSumThreads)), dim3(kSumThreads), 0, s, pol,
+                           status, tops);
+        hipLaunchKernelGGL(tops_scan_kernel<Policy>, dim3(1), dim3(kSumThreads), 0, s, pol, tops, total);
+        hipLaunchKernelGGL(partition_scan_kernel<Policy>, dim3(parts), dim3(kThreads), 0, s, pol, status, tops);
         return hipGetLastError();
     }
     const uint32_t clear_blocks = div_up(parts + 1, 256) < 1024 ? div_up(parts + 1, 256) : 1024;
@@ -543,32 +553,10 @@ __global__ void __launch_bounds__(kThreads) compact_count_kernel(const uint32_t*
     }
 }
 
-__global__ void __launch_bounds__(kSumThreads) compact_offsets_kernel(uint64_t* __restrict__ sums, uint32_t parts,
-                                                                      uint64_t* __restrict__ total) {
-    __shared__ uint64_t s_wave[kSumThreads / 64];
-    const uint32_t per = div_up(parts, kSumThreads);
-    const uint32_t lo = threadIdx.x * per, hi = lo + per < parts ? lo + per : parts;
-    uint64_t mine = run_sum(sums, lo, hi);
-    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-    const uint64_t incl = wave_inclusive_scan(mine, lane);
-    if (lane == 63) {
-        s_wave[wave] = incl;
-    }
-    __syncthreads();
-    uint64_t run = incl - mine, all = 0;
-    for (uint32_t w = 0; w < kSumThreads / 64; ++w) {
-        run += w < wave ? s_wave[w] : 0u;
-        all += s_wave[w];
-    }
-    run_exclusive(sums, lo, hi, run);
-    if (threadIdx.x == 0) {
-        *total = all;
-    }
-}
-
 __global__ void __launch_bounds__(kThreads) compact_write_kernel(const uint32_t* __restrict__ touched,
                                                                  const uint32_t* __restrict__ depth_keys, uint32_t n,
                                                                  const uint64_t* __restrict__ offsets,
+                                                                 const uint64_t* __restrict__ tops,
                                                                  uint32_t* __restrict__ keys_out,
                                                                  uint32_t* __restrict__ slots_out,
                                                                  uint32_t per_view, uint32_t magic,
@@ -596,7 +584,7 @@ __global__ void __launch_bounds__(kThreads) compact_write_kernel(const uint32_t*
     }
     __syncthreads();
     // Exclusive prefix of (row, wavefront) counts, in slot order.
-    uint64_t run = offsets[blockIdx.x];
+    uint64_t run = offsets[blockIdx.x] + tops[blockIdx.x / kSumThreads];
     // The first view starting at or after the block's first slot.
     const uint32_t boundary = per_view >= kTile ? (uint32_t)((first + per_view - 1u) / per_view) : 0u;
 #pragma unroll
@@ -640,8 +628,13 @@ __global__ void __launch_bounds__(kThreads) compact_write_kernel(const uint32_t*
 
 }  // namespace
 
-// u32 words of status a scan of up to n elements needs (u64 words, counted in u32).
-size_t scan_status_words(uint32_t n) { return 2 * (1 + (size_t)div_up(n ? n : 1, kTile)); }
+// u32 words of status a scan of up to n elements needs: the look-back's u64
+// a partition and one more, or the long route's u64 a partition and one a
+// run of kSumThreads of them (and one more).
+size_t scan_status_words(uint32_t n) {
+    const size_t parts = div_up(n ? n : 1, kTile);
+    return 2 * (2 + parts + div_up64(parts, kSumThreads));
+}
 
 hipError_t launch_scan(const uint32_t* in, uint32_t* out, uint64_t* total, uint32_t n, uint32_t* status_words,
                        hipStream_t s) {
@@ -657,10 +650,14 @@ hipError_t launch_compact(const uint32_t* touched, const uint32_t* depth_keys, u
     if (!counted || slots == 0) {
         hipLaunchKernelGGL(compact_count_kernel, dim3(parts), dim3(kThreads), 0, s, touched, slots, sums);
     }
-    hipLaunchKernelGGL(compact_offsets_kernel, dim3(1), dim3(kSumThreads), 0, s, sums, parts, visible_total);
+    uint64_t* tops = sums + parts;
+    const CountOnly count{slots};
+    hipLaunchKernelGGL(run_scan_kernel<CountOnly>, dim3(div_up(parts, kSumThreads)), dim3(kSumThreads), 0, s, count,
+                       sums, tops);
+    hipLaunchKernelGGL(tops_scan_kernel<CountOnly>, dim3(1), dim3(kSumThreads), 0, s, count, tops, visible_total);
     const uint32_t magic = per_view >= 2 ? (uint32_t)((1ull << 32) / per_view) : 0u;
     hipLaunchKernelGGL(compact_write_kernel, dim3(parts), dim3(kThreads), 0, s, touched, depth_keys, slots, sums,
-                       keys_out, slots_out, per_view, magic, per_view >= 2 ? view_start : nullptr);
+                       tops, keys_out, slots_out, per_view, magic, per_view >= 2 ? view_start : nullptr);
     return hipGetLastError();
 }
 
