@@ -586,6 +586,18 @@ __device__ inline uint32_t wave_inclusive_max(uint32_t v) {
     return v;
 }
 
+// Inclusive sum over lanes 0 .. L, by DPP as wave_inclusive_max. Every lane
+// must be active.
+__device__ inline uint32_t wave_inclusive_sum_dpp(uint32_t v) {
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xF, 0xF, false);  // row_shr 1
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xF, 0xF, false);  // row_shr 2
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xF, 0xF, false);  // row_shr 4
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xF, 0xF, false);  // row_shr 8
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xA, 0xF, false);  // row_bcast 15
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xC, 0xF, false);  // row_bcast 31
+    return v;
+}
+
 __global__ void __launch_bounds__(kBigThreads) count_big_kernel(const DView* __restrict__ views, uint32_t n,
                                                                 uint32_t T, const float* __restrict__ records,
                                                                 Footprints fp, uint32_t* __restrict__ touched,
@@ -1124,7 +1136,7 @@ __global__ void __launch_bounds__(kEmitThreads) strip_mask_emit_kernel(const DVi
         const SliceTerms S = slice_terms(conic.x, conic.y, conic.z);
         s_slice[threadIdx.x] = make_float4(S.det, S.sa, S.sc, __builtin_bit_cast(float, V.width | (V.height << 16)));
     }
-    const uint32_t end = wave_inclusive_sum(count, lane);
+    const uint32_t end = wave_inclusive_sum_dpp(count);
     s_end[threadIdx.x] = end;
     s_own[threadIdx.x] = ~0u;
     s_tiles[threadIdx.x] = 0;
