@@ -1879,7 +1879,15 @@ hipError_t launch_strip_mask_emit(const DView* views_dev, uint32_t n, uint32_t t
 __global__ void __launch_bounds__(256) strip_segments_kernel(SegmentBuild g) {
     __shared__ uint32_t s_start[kMaxSegmentViews + 1];
     __shared__ uint32_t s_part[kMaxSegmentViews + 1];
+    __shared__ uint32_t s_first[kMaxSegmentViews], s_entry[kMaxSegmentViews];
     const uint32_t nv = g.nviews;
+    // Every view's first rank and first entry read at once (a lane a view),
+    // then laid out by one lane from LDS.
+    if (threadIdx.x < nv) {
+        s_first[threadIdx.x] = g.view_first ? g.view_first[(size_t)threadIdx.x * g.view_stride] : 0u;
+        s_entry[threadIdx.x] = g.view_entry[threadIdx.x];
+    }
+    __syncthreads();
     if (threadIdx.x == 0) {
         const uint32_t visible = *g.visible_dev < g.max_visible ? (uint32_t)*g.visible_dev : g.max_visible;
         const uint64_t etotal = *g.entry_total_dev;
@@ -1890,9 +1898,9 @@ __global__ void __launch_bounds__(256) strip_segments_kernel(SegmentBuild g) {
             // A view with ranks starts where its first rank's entries do;
             // one without, where the next view does (view_first null: every
             // view's start is given).
-            const uint32_t R = g.view_first ? g.view_first[(size_t)v * g.view_stride] : 0u;
+            const uint32_t R = s_first[v];
             if (!g.view_first || R < next_rank) {
-                const uint32_t e = g.view_entry[v];
+                const uint32_t e = s_entry[v];
                 next = e < next ? e : next;  // within the capacity, never past the next run
             }
             next_rank = R < next_rank ? R : next_rank;
