@@ -569,7 +569,7 @@ ATHX_API athx_status athx_exclusive_scan(athx_ctx* ctx, const uint32_t* in, uint
  * reference: it is for callers that cannot allocate device memory (code
  * under HIP graph capture, tight serving loops, harnesses that forbid
  * hipMalloc). Its look-back scratch is a static device array of the library
- * (32 KiB on each GPU it runs on, held until the process ends), so n is at
+ * (32 816 bytes on each GPU it runs on, held until the process ends), so n is at
  * most ATHX_NOALLOC_SCAN_MAX (INVALID_ARGUMENT beyond). Calls that overlap
  * on one GPU (two streams, or two contexts of one device) share that array
  * and race: queue them on one stream. Same results as athx_exclusive_scan.
